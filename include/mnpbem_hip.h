@@ -1,0 +1,162 @@
+/*
+ * mnpbem_hip.h - C ABI of the MI355X (gfx950) retarded-BEM library
+ * (libmnpbem_hip.so).
+ *
+ * Drop-in boundary for MNPBEM's full-Maxwell (retarded) BEM hot path.  The
+ * Matlab classes keep their calling conventions; a MEX adapter (or the
+ * Python host layer in mnpbem_amd/) converts Matlab arrays and calls these
+ * functions.  Each entry point names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns MNP_OK (0) or an MNP_ERR_* code; the message of
+ *     the last failure on the calling thread is mnp_last_error().  Nothing
+ *     here calls exit().
+ *   - Arrays are column-major (Matlab/LAPACK layout).  Geometry arrays of n
+ *     faces are n x 3 (x column, then y, then z).
+ *   - Complex arrays are interleaved (re, im) pairs: complex128 for
+ *     precision MNP_FP64, complex64 for MNP_FP32.
+ *   - Pointers documented as "device" address GPU memory (HBM) of the current
+ *     device; "host" pointers address CPU memory.  Work is queued on the
+ *     stream set with mnp_set_stream (default: the null stream).
+ *   - Face indices are 0-based (MEX adapters subtract 1 from Matlab faces).
+ */
+#ifndef MNPBEM_HIP_H
+#define MNPBEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  double re, im;
+} mnp_complex;
+
+enum {
+  MNP_OK = 0,
+  MNP_ERR_ARG = 1,       /* invalid argument (sizes, NULL pointers, options)   */
+  MNP_ERR_HIP = 2,       /* HIP runtime failure                                */
+  MNP_ERR_SINGULAR = 3,  /* exactly singular pivot in an LU factorisation      */
+  MNP_ERR_NOCONV = 4,    /* iterative refinement did not reach its tolerance   */
+  MNP_ERR_INTERNAL = 5,
+  MNP_ERR_STATE = 6      /* call out of order (e.g. solve before initmat)      */
+};
+
+enum { MNP_FP64 = 0, MNP_FP32 = 1 };
+enum { MNP_KEY_G = 0, MNP_KEY_F = 1, MNP_KEY_H1 = 2, MNP_KEY_H2 = 3 };
+
+/* Solver variants of mnp_bemret_create. */
+enum {
+  MNP_SOLVER_FP64 = 0,  /* complex FP64 chain: the plain drop-in of bemret      */
+  MNP_SOLVER_MIXED = 1  /* complex FP32 chain as preconditioner + FP64 refinement
+                           with the exact operator (bemretiter, precond 'full') */
+};
+
+/* ------------------------------------------------------------- library */
+const char* mnp_last_error(void);
+int mnp_version(void);
+/* Selects the HIP device used by subsequent calls. */
+int mnp_set_device(int device);
+/* Stream for all queued work (a hipStream_t passed as void*; NULL = null stream). */
+int mnp_set_stream(void* stream);
+void* mnp_get_stream(void);
+int mnp_synchronize(void);
+
+/* ------------------------------------------------ Green function (greenret)
+ * Replaces compgreenret(p, p, op) for one closed particle of flat triangles:
+ *   Greenfun/@compgreenret/private/init.m:4-31 -> @greenret/private/init.m
+ *   (refinematrix, self/near-field Taylor tables, deriv 'cart') and
+ *   @compgreenret/private/initclosed.m (closed-surface sum rule).
+ * Everything, including the refinement tables, is computed on the GPU.
+ */
+typedef struct mnp_greenret_s* mnp_greenret_t;
+
+typedef struct {
+  int order;          /* order of the exp(ikr) expansion (bemoptions: 5)      */
+  double rel_cutoff;  /* RelCutoff (bemoptions: 3)                            */
+  double abs_cutoff;  /* AbsCutoff (0)                                        */
+  int npol_rho;       /* polar self-integration rule, radial points - 1 (7)   */
+  int npol_phi;       /* polar self-integration rule, angular points - 1 (5)  */
+  int closed;         /* 1: closed surface, apply the sum rule (comparticle   */
+                      /*    closed argument)                                  */
+} mnp_green_options;
+
+/* Fills op with bemoptions('sim','ret') defaults (Misc/bemoptions.m:18-24,
+ * Misc/integration/@quadface/private/init.m:8-15). */
+void mnp_green_options_default(mnp_green_options* op);
+
+/* verts: host nverts x 3 (nm); faces: host nfaces x 3, 0-based vertex indices,
+ * normal (v1-v2)x(v3-v2) pointing out of the particle (norm_flat.m:36-38). */
+int mnp_greenret_create(const double* verts, int64_t nverts, const int64_t* faces,
+                        int64_t nfaces, const mnp_green_options* op, mnp_greenret_t* out);
+int mnp_greenret_destroy(mnp_greenret_t g);
+/* Number of faces and of refined (self + near-field) matrix elements. */
+int mnp_greenret_size(mnp_greenret_t g, int64_t* nfaces, int64_t* nrefined);
+/* Copies face geometry to host arrays (each may be NULL): pos, nvec, tvec1,
+ * tvec2 are n x 3, area n (Particles/@particle/private/norm_flat.m). */
+int mnp_greenret_geometry(mnp_greenret_t g, double* pos, double* nvec, double* tvec1,
+                          double* tvec2, double* area);
+/* Copies the refinement tables to host arrays (each may be NULL):
+ * rows/cols (nrefined, column-major order as Matlab find(ir)), gtab
+ * nrefined x (order+1), ftab nrefined x 3 x (order+1)  (greenret.g, .f). */
+int mnp_greenret_refined(mnp_greenret_t g, int64_t* rows, int64_t* cols, double* gtab,
+                         double* ftab);
+/* g{i,j}.G/.F/.H1/.H2(k) as one dense n x n matrix (device out, leading
+ * dimension ld >= n) at wavenumber k (Greenfun/@greenret/private/eval1.m). */
+int mnp_greenret_eval(mnp_greenret_t g, mnp_complex k, int key, int precision, void* out,
+                      int64_t ld);
+
+/* -------------------------------------------------- dense solver (bemret)
+ * Replaces bemsolver(p, op) -> bemret (BEM/@bemret): initmat builds the
+ * chain for one wavelength (private/initmat.m:5-72), solve applies it
+ * (mldivide.m:11-49).  With MNP_SOLVER_MIXED the chain is built in complex
+ * FP32 and used as preconditioner of the exact operator
+ * (BEM/@bemretiter/private/afun.m, mfun.m) with FP64 residuals until the
+ * relative residual is below refine_tol.
+ */
+typedef struct mnp_bemret_s* mnp_bemret_t;
+
+int mnp_bemret_create(mnp_greenret_t g, int solver, mnp_bemret_t* out);
+int mnp_bemret_destroy(mnp_bemret_t bem);
+/* eps1: dielectric function inside the particle, eps2 outside, at enei (nm). */
+int mnp_bemret_initmat(mnp_bemret_t bem, double enei, mnp_complex eps1, mnp_complex eps2);
+/* Excitation potentials (device, complex128; NULL means zero): phi* are
+ * n x npol, a* are n x 3 x npol.  Outputs (device, complex128): sig1, sig2
+ * n x npol, h1, h2 n x 3 x npol (compstruct fields of mldivide.m:48-49). */
+int mnp_bemret_solve(mnp_bemret_t bem, int64_t npol, const void* phi1, const void* phi1p,
+                     const void* a1, const void* a1p, const void* phi2, const void* phi2p,
+                     const void* a2, const void* a2p, void* sig1, void* sig2, void* h1,
+                     void* h2);
+/* Refinement statistics of the last solve (MIXED): sweeps and final relative residual. */
+int mnp_bemret_stats(mnp_bemret_t bem, int* iterations, double* relres);
+/* Refinement settings for MNP_SOLVER_MIXED (defaults: 1e-12, 10). */
+int mnp_bemret_set_refinement(mnp_bemret_t bem, double tol, int maxit);
+
+/* --------------------------------------------- plane wave and observables
+ * Simulation/retarded/@planewaveret/potential.m:47-88 for a particle whose
+ * outside is the embedding medium: a2 and a2p (device out, complex128,
+ * n x 3 x npol).  pol, dir: host npol x 3, pol perpendicular to dir. */
+int mnp_planewave_potential(mnp_greenret_t g, double enei, mnp_complex eps_medium,
+                            int64_t npol, const double* pol, const double* dir, void* a2,
+                            void* a2p);
+/* Far fields of sig2/h2 in directions dir (device ndir x 3 doubles):
+ * e, h device out ndir x 3 x npol complex128 (@spectrumret/farfield.m). */
+int mnp_farfield(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t ndir,
+                 const void* dir, int64_t npol, const void* sig2, const void* h2, void* e,
+                 void* h);
+/* Extinction (optical theorem), scattering (Poynting flux over the surface
+ * with normals pinfty_nvec, device npinf x 3, and areas pinfty_area, device
+ * npinf) and absorption cross sections, host out (npol each), FP64
+ * reductions (@planewaveret/extinction.m, scattering.m, absorption.m). */
+int mnp_planewave_cross_sections(mnp_greenret_t g, double enei, mnp_complex eps_medium,
+                                 int64_t npol, const double* pol, const double* dir,
+                                 int64_t npinf, const void* pinfty_nvec,
+                                 const void* pinfty_area, const void* sig2, const void* h2,
+                                 double* ext, double* sca, double* abs_);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MNPBEM_HIP_H */

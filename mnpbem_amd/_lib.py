@@ -1,0 +1,97 @@
+"""ctypes binding of libmnpbem_hip.so (declarations in include/mnpbem_hip.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmnpbem_hip.so")
+
+MNP_OK = 0
+MNP_FP64, MNP_FP32 = 0, 1
+MNP_KEY = {"G": 0, "F": 1, "H1": 2, "H2": 3}
+MNP_SOLVER_FP64, MNP_SOLVER_MIXED = 0, 1
+
+
+class MnpComplex(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+    @classmethod
+    def of(cls, z):
+        z = complex(z)
+        return cls(z.real, z.imag)
+
+
+class GreenOptions(C.Structure):
+    _fields_ = [("order", C.c_int), ("rel_cutoff", C.c_double), ("abs_cutoff", C.c_double),
+                ("npol_rho", C.c_int), ("npol_phi", C.c_int), ("closed", C.c_int)]
+
+
+P, I64, D, INT = C.c_void_p, C.c_int64, C.c_double, C.c_int
+PD = C.POINTER(C.c_double)
+PI64 = C.POINTER(C.c_int64)
+
+# name -> (restype, argtypes); the list is the exported C ABI
+PROTOTYPES = {
+    "mnp_last_error": (C.c_char_p, []),
+    "mnp_version": (INT, []),
+    "mnp_set_device": (INT, [INT]),
+    "mnp_set_stream": (INT, [P]),
+    "mnp_get_stream": (P, []),
+    "mnp_synchronize": (INT, []),
+    "mnp_green_options_default": (None, [C.POINTER(GreenOptions)]),
+    "mnp_greenret_create": (INT, [P, I64, P, I64, C.POINTER(GreenOptions), C.POINTER(P)]),
+    "mnp_greenret_destroy": (INT, [P]),
+    "mnp_greenret_size": (INT, [P, PI64, PI64]),
+    "mnp_greenret_geometry": (INT, [P, P, P, P, P, P]),
+    "mnp_greenret_refined": (INT, [P, P, P, P, P]),
+    "mnp_greenret_eval": (INT, [P, MnpComplex, INT, INT, P, I64]),
+    "mnp_bemret_create": (INT, [P, INT, C.POINTER(P)]),
+    "mnp_bemret_destroy": (INT, [P]),
+    "mnp_bemret_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
+    "mnp_bemret_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mnp_bemret_stats": (INT, [P, C.POINTER(INT), PD]),
+    "mnp_bemret_set_refinement": (INT, [P, D, INT]),
+    "mnp_planewave_potential": (INT, [P, D, MnpComplex, I64, P, P, P, P]),
+    "mnp_farfield": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P]),
+    "mnp_planewave_cross_sections": (INT, [P, D, MnpComplex, I64, P, P, I64, P, P, P, P,
+                                           P, P, P]),
+}
+
+
+class MnpError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if libmnpbem_hip.so is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MnpError(f"{LIB_PATH} not built: run `python -m mnpbem_amd.build` "
+                           "(there is no CPU fallback)")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(h, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def check(status, what=""):
+    if status != MNP_OK:
+        msg = lib().mnp_last_error().decode(errors="replace")
+        raise MnpError(f"{what} failed with status {status}: {msg}")
+
+
+def call(name, *args):
+    fn = getattr(lib(), name)
+    check(fn(*args), name)

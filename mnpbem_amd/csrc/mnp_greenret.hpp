@@ -1,0 +1,74 @@
+// Device-resident retarded Green function of one closed particle.
+#pragma once
+
+#include "mnp_common.hpp"
+
+namespace mnp {
+
+constexpr int MNP_MAX_ORDER = 8;
+constexpr int MNP_MAX_POLAR = 3 * 16 * 16;
+
+hipStream_t stream();
+
+template <class T>
+T* dev_alloc(int64_t count) {
+  T* p = nullptr;
+  if (count <= 0) count = 1;
+  MNP_HIP(hipMalloc((void**)&p, sizeof(T) * (size_t)count));
+  return p;
+}
+
+// Face geometry, structure of arrays, column-major n x 3 (x, y, z blocks).
+struct Geo {
+  double* pos = nullptr;    // centroids
+  double* nvec = nullptr;   // unit outer normals
+  double* t1 = nullptr;     // unit tangent (v1 - v2)/|v1 - v2|
+  double* t2 = nullptr;     // nvec x t1
+  double* vtx = nullptr;    // vertices, (a*3 + c)*n + i for vertex a, coordinate c
+  double* area = nullptr;   // 0.5 |(v1-v2) x (v3-v2)|
+  double* qarea = nullptr;  // 0.5 |(v2-v1) x (v3-v1)| (quadrature weights)
+  double* rad = nullptr;    // max centroid-vertex distance
+};
+
+// Refined (row, col) pairs in column-major order.
+struct Pairs {
+  int32_t* row = nullptr;
+  int32_t* col = nullptr;
+  int8_t* kind = nullptr;   // 2 = self (coincident centroids), 1 = near field
+  int64_t* diag = nullptr;  // index of pair (i, i), or -1
+};
+
+// Assembly output: an FP64 matrix and/or an FP32 mirror of the same values.
+struct OutPair {
+  double2* d = nullptr;
+  float2* f = nullptr;
+  __host__ __device__ bool any() const { return d || f; }
+  __device__ void store(int64_t idx, double2 v) const {
+    if (d) d[idx] = v;
+    if (f) f[idx] = make_float2((float)v.x, (float)v.y);
+  }
+};
+
+struct GreenRet {
+  int64_t n = 0;
+  int64_t nnz = 0;
+  int order = 5;
+  int npolq = 0;
+  mnp_green_options op;
+  Geo geo;
+  Pairs pairs;
+  int64_t* off = nullptr;  // column offsets into the pair list (n + 1)
+  double* gtab = nullptr;  // nnz x (order + 1)
+  double* ftab = nullptr;  // nnz x 3 x (order + 1)
+
+  GreenRet(const double* verts, int64_t nv, const int64_t* faces, int64_t nf,
+           const mnp_green_options& op);
+  ~GreenRet();
+  GreenRet(const GreenRet&) = delete;
+  GreenRet& operator=(const GreenRet&) = delete;
+
+  // G(k) into g and F(k) + sgn2pi * I into h (either may be empty).
+  void assemble(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t ld) const;
+};
+
+}  // namespace mnp
