@@ -133,6 +133,29 @@ int mnp_bemret_stats(mnp_bemret_t bem, int* iterations, double* relres);
 /* Refinement settings for MNP_SOLVER_MIXED (defaults: 1e-12, 10). */
 int mnp_bemret_set_refinement(mnp_bemret_t bem, double tol, int maxit);
 
+/* ------------------------------------------------ dense building blocks
+ * Replace the LAPACK/BLAS calls behind Matlab's inv and mtimes in
+ * BEM/@bemret/private/initmat.m:27-72 (zgetrf/zgetri/zgemm) and in
+ * mex/hlib/basemat.cpp:105-120,224-240.  Device arrays, column-major,
+ * complex128 (MNP_FP64) or complex64 (MNP_FP32).
+ */
+/* C = alpha A B + beta C, A m x k, B k x n. */
+int mnp_gemm(int precision, int64_t m, int64_t n, int64_t k, mnp_complex alpha, const void* A,
+             int64_t lda, const void* B, int64_t ldb, mnp_complex beta, void* C, int64_t ldc);
+/* In-place LU with partial pivoting of an m x n matrix (m >= n); ipiv
+ * (device int32, n) holds 0-based row interchanges.  *info (host) is 0 or
+ * the 1-based column of the first exactly zero pivot. */
+int mnp_getrf(int precision, int64_t m, int64_t n, void* A, int64_t lda, int32_t* ipiv, int* info);
+/* B <- A^{-1} B (B n x nrhs) from mnp_getrf factors. */
+int mnp_getrs(int precision, int64_t n, int64_t nrhs, const void* LU, int64_t lda,
+              const int32_t* ipiv, void* B, int64_t ldb);
+/* B <- B A^{-1} (B m x n) from mnp_getrf factors. */
+int mnp_getrs_right(int precision, int64_t m, int64_t n, const void* LU, int64_t lda,
+                    const int32_t* ipiv, void* B, int64_t ldb);
+/* X = A^{-1} (n x n) from mnp_getrf factors. */
+int mnp_getri(int precision, int64_t n, const void* LU, int64_t lda, const int32_t* ipiv, void* X,
+              int64_t ldx);
+
 /* --------------------------------------------- plane wave and observables
  * Simulation/retarded/@planewaveret/potential.m:47-88 for a particle whose
  * outside is the embedding medium: a2 and a2p (device out, complex128,

@@ -54,6 +54,11 @@ PROTOTYPES = {
     "mnp_bemret_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mnp_bemret_stats": (INT, [P, C.POINTER(INT), PD]),
     "mnp_bemret_set_refinement": (INT, [P, D, INT]),
+    "mnp_gemm": (INT, [INT, I64, I64, I64, MnpComplex, P, I64, P, I64, MnpComplex, P, I64]),
+    "mnp_getrf": (INT, [INT, I64, I64, P, I64, P, C.POINTER(INT)]),
+    "mnp_getrs": (INT, [INT, I64, I64, P, I64, P, P, I64]),
+    "mnp_getrs_right": (INT, [INT, I64, I64, P, I64, P, P, I64]),
+    "mnp_getri": (INT, [INT, I64, P, I64, P, P, I64]),
     "mnp_planewave_potential": (INT, [P, D, MnpComplex, I64, P, P, P, P]),
     "mnp_farfield": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P]),
     "mnp_planewave_cross_sections": (INT, [P, D, MnpComplex, I64, P, P, I64, P, P, P, P,

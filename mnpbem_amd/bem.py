@@ -1,0 +1,290 @@
+"""Host-side mirror of MNPBEM's retarded BEM operator API.
+
+The names and argument meanings follow MNPBEM:
+
+    op   = bemoptions(sim='ret')                       # Misc/bemoptions.m
+    p    = comparticle([epsconst(1), epsdrude(...)], [(verts, faces)], [2, 1], 1, op)
+    bem  = bemsolver(p, op)                            # Base/bemsolver.m -> bemret
+    exc  = planewave([[1, 0, 0]], [[0, 0, 1]], op)     # Base/planewave.m -> planewaveret
+    sig  = bem.solve(exc(p, enei))                     # sig = bem \\ exc(p, enei)
+    ext, sca, ab = exc.ext(sig), exc.sca(sig), exc.abs(sig)
+
+Everything numerical runs in libmnpbem_hip.so on the GPU; this module only
+keeps handles, allocates device buffers through torch, and converts the
+Matlab-style options.  Supported here: one closed particle of flat triangles
+embedded in one medium (inout = [inside, outside]).
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import device as dev
+from .greenfun import CompGreenRet
+
+EV2NM = 1.0 / 8.0655477e-4  # Misc/units.m:2
+
+
+# ---------------------------------------------------------------- materials
+class epsconst:
+    """Constant dielectric function (Material/@epsconst)."""
+
+    def __init__(self, eps):
+        self.eps = complex(eps)
+
+    def __call__(self, enei):
+        return self.eps, 2 * np.pi / enei * np.sqrt(self.eps)
+
+
+class epsdrude:
+    """Drude dielectric function eps0 - wp^2 / (w (w + i gammad)), w = eV2nm / enei
+    (Material/@epsdrude/subsref.m:16-28); parameters in eV."""
+
+    def __init__(self, eps0, wp, gammad):
+        self.eps0, self.wp, self.gammad = float(eps0), float(wp), float(gammad)
+
+    def __call__(self, enei):
+        w = EV2NM / enei
+        eps = self.eps0 - self.wp ** 2 / (w * (w + 1j * self.gammad))
+        return eps, 2 * np.pi / enei * np.sqrt(eps)
+
+
+class epsfun:
+    """User function of the wavelength in nm (Material/epsfun.m)."""
+
+    def __init__(self, fun, key="nm"):
+        self.fun, self.key = fun, key
+
+    def __call__(self, enei):
+        eps = complex(self.fun(enei if self.key == "nm" else EV2NM / enei))
+        return eps, 2 * np.pi / enei * np.sqrt(eps)
+
+
+# ------------------------------------------------------------------ options
+def bemoptions(**kw):
+    """bemoptions('sim','ret', ...) defaults (Misc/bemoptions.m:18-24)."""
+    op = dict(sim="ret", waitbar=0, RelCutoff=3.0, order=5, interp="flat")
+    op.update(kw)
+    return op
+
+
+# --------------------------------------------------------------- particles
+class comparticle:
+    """comparticle(eps, {p}, inout, closed, op) for one particle.
+
+    ``eps`` is the table of dielectric functions, ``inout`` = [inside, outside]
+    1-based indices into it (Particles/@comparticle/comparticle.m:11-28).
+    """
+
+    def __init__(self, eps, parts, inout, closed=1, op=None):
+        if len(parts) != 1:
+            raise NotImplementedError("one particle per comparticle is supported")
+        op = bemoptions() if op is None else op
+        if op.get("interp", "flat") != "flat":
+            raise NotImplementedError("only flat boundary elements are supported")
+        self.eps = list(eps)
+        self.verts, self.faces = (np.asarray(a) for a in parts[0])
+        self.inout = np.atleast_2d(np.asarray(inout, dtype=int))
+        self.closed = bool(closed)
+        self.op = op
+        self._green = None
+
+    @property
+    def n(self):
+        return len(self.faces)
+
+    def eps1(self, enei):
+        return self.eps[self.inout[0, 0] - 1](enei)[0]
+
+    def eps2(self, enei):
+        return self.eps[self.inout[0, 1] - 1](enei)[0]
+
+    @property
+    def green(self):
+        """The particle's compgreenret(p, p, op), built once per mesh."""
+        if self._green is None:
+            op = self.op
+            self._green = CompGreenRet(
+                self.verts, self.faces, order=op.get("order", 5),
+                rel_cutoff=op.get("RelCutoff", 0.0), abs_cutoff=op.get("AbsCutoff", 0.0),
+                npol=tuple(np.broadcast_to(op.get("npol", (7, 5)), (2,))), closed=self.closed)
+        return self._green
+
+    def geometry(self):
+        return self.green.geometry()
+
+
+# ---------------------------------------------------------- result struct
+class compstruct(dict):
+    """Fields of a BEM quantity on the particle surface (Particles/@compstruct).
+
+    Field values are device tensors: scalars n x npol, vectors n x 3 x npol.
+    """
+
+    def __init__(self, p, enei, **fields):
+        super().__init__(**fields)
+        self.p, self.enei = p, enei
+
+    def __getattr__(self, key):
+        try:
+            return self[key]
+        except KeyError as e:
+            raise AttributeError(key) from e
+
+
+# ------------------------------------------------------------------ solvers
+class bemret:
+    """Dense retarded BEM solver (BEM/@bemret), FP64 or FP32 + FP64 refinement.
+
+    ``precision='fp64'`` is the drop-in of bemret; ``precision='mixed'``
+    builds the chain in complex FP32 and refines with the exact operator in
+    FP64 (the structure of bemretiter with precond 'full').
+    """
+
+    def __init__(self, p, op=None, precision=None):
+        op = p.op if op is None else op
+        self.p = p
+        self.precision = precision or op.get("precision", "fp64")
+        solver = _lib.MNP_SOLVER_FP64 if self.precision == "fp64" else _lib.MNP_SOLVER_MIXED
+        h = C.c_void_p()
+        _lib.call("mnp_bemret_create", p.green.handle, solver, C.byref(h))
+        self.handle = h
+        if "refine_tol" in op or "refine_maxit" in op:
+            _lib.call("mnp_bemret_set_refinement", h, float(op.get("refine_tol", 1e-12)),
+                      int(op.get("refine_maxit", 10)))
+        self.enei = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib.lib().mnp_bemret_destroy(h)
+            self.handle = None
+
+    def __call__(self, enei):
+        """bem(enei): initialise the matrices for one wavelength (initmat.m)."""
+        if self.enei != enei:
+            e1, e2 = self.p.eps1(enei), self.p.eps2(enei)
+            _lib.call("mnp_bemret_initmat", self.handle, float(enei), _lib.MnpComplex.of(e1),
+                      _lib.MnpComplex.of(e2))
+            self.enei = enei
+        return self
+
+    def solve(self, exc):
+        """sig = bem \\ exc  (BEM/@bemret/mldivide.m)."""
+        self(exc.enei)
+        n = self.p.n
+        npol = next(v.shape[-1] for v in exc.values() if v is not None)
+        args = []
+        for key in ("phi1", "phi1p", "a1", "a1p", "phi2", "phi2p", "a2", "a2p"):
+            v = exc.get(key)
+            args.append(None if v is None else dev.ptr(dev.base(v)))
+        sig1 = dev.colmajor((n, npol))
+        sig2 = dev.colmajor((n, npol))
+        h1 = dev.colmajor((n, 3, npol))
+        h2 = dev.colmajor((n, 3, npol))
+        _lib.call("mnp_bemret_solve", self.handle, npol, *args,
+                  *[dev.ptr(dev.base(t)) for t in (sig1, sig2, h1, h2)])
+        return compstruct(self.p, exc.enei, sig1=sig1, sig2=sig2, h1=h1, h2=h2)
+
+    mldivide = solve
+    __truediv__ = None
+
+    def stats(self):
+        """(refinement sweeps, final relative residual) of the last solve."""
+        it, rr = C.c_int(), C.c_double()
+        _lib.call("mnp_bemret_stats", self.handle, C.byref(it), C.byref(rr))
+        return it.value, rr.value
+
+
+def bemsolver(p, op=None, **kw):
+    """Base/bemsolver.m: picks the retarded dense solver for op.sim == 'ret'."""
+    op = dict(p.op if op is None else op)
+    op.update(kw)
+    if op.get("sim", "ret") != "ret":
+        raise NotImplementedError("only the retarded solver is implemented")
+    return bemret(p, op)
+
+
+# ------------------------------------------------------------ excitations
+class planewaveret:
+    """Plane-wave excitation (Simulation/retarded/@planewaveret).
+
+    ``pol`` and ``dir`` are npol x 3; ``medium`` is the 1-based index of the
+    embedding medium.  Scattering is integrated over trisphere(256, 2).
+    """
+
+    def __init__(self, pol, dir, op=None, medium=1, pinfty=None):
+        self.pol = np.atleast_2d(np.asarray(pol, dtype=np.float64))
+        self.dir = np.atleast_2d(np.asarray(dir, dtype=np.float64))
+        if np.any(np.sum(self.pol * self.dir, axis=1) != 0):
+            raise ValueError("polarisation must be perpendicular to the propagation direction")
+        self.medium = medium
+        if pinfty is None:
+            from . import shapes
+            v, f = shapes.trisphere256(2.0)
+            nvec, area = _flat_normals(v, f)
+        else:
+            nvec, area = pinfty
+        dev.ensure()
+        self.pinf_nvec = dev.to_device_colmajor(nvec, torch.float64)
+        self.pinf_area = torch.as_tensor(np.asarray(area, np.float64), device="cuda")
+
+    def _eps_med(self, p, enei):
+        return p.eps[self.medium - 1](enei)[0]
+
+    def __call__(self, p, enei):
+        """exc(p, enei): potentials a2, a2p on the faces (potential.m)."""
+        if p.inout[0, 1] != self.medium:
+            raise NotImplementedError("the particle's outside must be the embedding medium")
+        n, npol = p.n, len(self.pol)
+        a2 = dev.colmajor((n, 3, npol))
+        a2p = dev.colmajor((n, 3, npol))
+        pol = np.asfortranarray(self.pol)
+        dr = np.asfortranarray(self.dir)
+        _lib.call("mnp_planewave_potential", p.green.handle, float(enei),
+                  _lib.MnpComplex.of(self._eps_med(p, enei)), npol,
+                  pol.ctypes.data_as(C.c_void_p), dr.ctypes.data_as(C.c_void_p),
+                  dev.ptr(dev.base(a2)), dev.ptr(dev.base(a2p)))
+        return compstruct(p, enei, a2=a2, a2p=a2p)
+
+    potential = __call__
+
+    def cross_sections(self, sig):
+        """(ext, sca, abs) per polarisation, FP64."""
+        p, enei = sig.p, sig.enei
+        npol = len(self.pol)
+        out = [np.zeros(npol) for _ in range(3)]
+        pol = np.asfortranarray(self.pol)
+        dr = np.asfortranarray(self.dir)
+        _lib.call("mnp_planewave_cross_sections", p.green.handle, float(enei),
+                  _lib.MnpComplex.of(self._eps_med(p, enei)), npol,
+                  pol.ctypes.data_as(C.c_void_p), dr.ctypes.data_as(C.c_void_p),
+                  self.pinf_area.numel(), dev.ptr(dev.base(self.pinf_nvec)),
+                  dev.ptr(self.pinf_area), dev.ptr(dev.base(sig.sig2)),
+                  dev.ptr(dev.base(sig.h2)),
+                  *[o.ctypes.data_as(C.c_void_p) for o in out])
+        return tuple(out)
+
+    def ext(self, sig):
+        return self.cross_sections(sig)[0]
+
+    def sca(self, sig):
+        return self.cross_sections(sig)[1]
+
+    def abs(self, sig):
+        return self.cross_sections(sig)[2]
+
+    extinction, scattering, absorption = ext, sca, abs
+
+
+def planewave(pol, dir, op=None, **kw):
+    """Base/planewave.m for op.sim == 'ret'."""
+    return planewaveret(pol, dir, op, **kw)
+
+
+def _flat_normals(verts, faces):
+    v1, v2, v3 = verts[faces[:, 0]], verts[faces[:, 1]], verts[faces[:, 2]]
+    nv = np.cross(v1 - v2, v3 - v2)
+    nn = np.sqrt(np.sum(nv * nv, axis=1))
+    return nv / nn[:, None], 0.5 * nn

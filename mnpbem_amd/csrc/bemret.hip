@@ -1,0 +1,730 @@
+// Dense retarded BEM solver (MNPBEM BEM/@bemret) on gfx950.
+//
+// initmat (BEM/@bemret/private/initmat.m:5-72) builds, per wavelength,
+//   G1 = G(k1), G2 = G(k2), H1 = F(k1) + 2 pi, H2 = F(k2) - 2 pi,
+//   Sigma1 = H1 G1^{-1}, Sigma2 = H2 G2^{-1}, Deltai = (Sigma1 - Sigma2)^{-1},
+//   Sigma = eps1 Sigma1 - eps2 Sigma2 + k^2 (eps1 - eps2)^2 (Deltai o n n^T).
+// Instead of the reference's explicit inverses of G1, G2 and Sigma, the
+// chain keeps their LU factors (G1^{-1} and G2^{-1} are only ever applied)
+// and forms Sigma1, Sigma2 by right triangular solves; only Deltai, whose
+// entries enter Sigma, is inverted explicitly.
+//
+// MNP_SOLVER_FP64 runs the chain and mldivide (mldivide.m:11-49) in complex
+// FP64.  MNP_SOLVER_MIXED runs the chain in complex FP32 and uses it as the
+// preconditioner mfun of the exact operator afun (BEM/@bemretiter/private/
+// afun.m:6-35, mfun.m:10-52) applied with the FP64 matrices:
+//   x <- x + M^{-1} (b - A x)  until |b - A x| / |b| < tol.
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "mnp_linalg.hpp"
+
+namespace mnp {
+
+// ---------------------------------------------------------- chain state
+template <class T>
+struct Chain {
+  int64_t n = 0;
+  cplx<T>*G1 = nullptr, *G2 = nullptr, *S1 = nullptr, *S2 = nullptr, *D = nullptr, *Di = nullptr;
+  int32_t *piv = nullptr, *sig1 = nullptr, *sig2 = nullptr, *sigD = nullptr, *sigS = nullptr;
+  double2 eps1{}, eps2{};
+  double k0 = 0;
+  Workspace ws;
+
+  explicit Chain(int64_t n_) : n(n_) {
+    const int64_t nn = n * n;
+    G1 = dev_alloc<cplx<T>>(nn);
+    G2 = dev_alloc<cplx<T>>(nn);
+    S1 = dev_alloc<cplx<T>>(nn);
+    S2 = dev_alloc<cplx<T>>(nn);
+    D = dev_alloc<cplx<T>>(nn);
+    Di = dev_alloc<cplx<T>>(nn);
+    piv = dev_alloc<int32_t>(n);
+    sig1 = dev_alloc<int32_t>(n);
+    sig2 = dev_alloc<int32_t>(n);
+    sigD = dev_alloc<int32_t>(n);
+    sigS = dev_alloc<int32_t>(n);
+  }
+  ~Chain() {
+    void* ps[] = {G1, G2, S1, S2, D, Di, piv, sig1, sig2, sigD, sigS};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
+  MatView<T> v(cplx<T>* p) const { return MatView<T>{p, n}; }
+};
+
+// Sigma = e1 S1 - e2 S2 + c (Di o n n^T), written over S2.
+template <class T>
+__global__ void k_sigma(int64_t n, const cplx<T>* S1, cplx<T>* S2, const cplx<T>* Di,
+                        const double* nvec, double2 e1, double2 e2, double2 c) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (i >= n || j >= n) return;
+  const int64_t idx = j * n + i;
+  const double nn = nvec[i] * nvec[j] + nvec[n + i] * nvec[n + j] + nvec[2 * n + i] * nvec[2 * n + j];
+  const cplx<T> s1 = S1[idx], s2 = S2[idx], di = Di[idx];
+  const double2 a = cmul<double>(e1, mk<double>(s1.x, s1.y));
+  const double2 b = cmul<double>(e2, mk<double>(s2.x, s2.y));
+  const double2 d = cmul<double>(c, mk<double>(di.x * nn, di.y * nn));
+  S2[idx] = mk<T>((T)(a.x - b.x + d.x), (T)(a.y - b.y + d.y));
+}
+template <class T>
+__global__ void k_sub(int64_t n, const cplx<T>* A, const cplx<T>* B, cplx<T>* C) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (i >= n || j >= n) return;
+  const int64_t idx = j * n + i;
+  C[idx] = csub<T>(A[idx], B[idx]);
+}
+static dim3 gridnn(int64_t m, int64_t n) {
+  return dim3((unsigned)cdiv_up(m, 256), (unsigned)std::min<int64_t>(n, 65535),
+              (unsigned)cdiv_up(n, 65535));
+}
+
+// Factorises the chain.  On entry G1, G2, S1 (= H1), S2 (= H2) hold the
+// assembled matrices.
+template <class T>
+static void chain_build(Chain<T>& c, const GreenRet& g) {
+  const int64_t n = c.n;
+  auto lu = [&](cplx<T>* A, int32_t* sigma, const char* what) {
+    const int info = getrf<T>(n, n, c.v(A), c.piv, c.ws);
+    MNP_CHECK(info == 0, MNP_ERR_SINGULAR,
+              std::string("bemret: singular ") + what + " (zero pivot " + std::to_string(info) + ")");
+    perm_from_pivots(n, c.piv, sigma, c.ws);
+  };
+  lu(c.G1, c.sig1, "G1");
+  getrs_right<T>(n, n, c.v(c.G1), c.sig1, c.v(c.S1), c.ws);  // Sigma1 = H1 G1^{-1}
+  lu(c.G2, c.sig2, "G2");
+  getrs_right<T>(n, n, c.v(c.G2), c.sig2, c.v(c.S2), c.ws);  // Sigma2 = H2 G2^{-1}
+  hipLaunchKernelGGL(k_sub<T>, gridnn(n, n), dim3(256), 0, stream(), n, c.S1, c.S2, c.D);
+  MNP_HIP(hipGetLastError());
+  lu(c.D, c.sigD, "Sigma1 - Sigma2");
+  getri<T>(n, c.v(c.D), c.sigD, c.v(c.Di), c.ws);  // Deltai
+  const double2 L = csub<double>(c.eps1, c.eps2);
+  const double2 cc = cscale<double>(cmul<double>(L, L), c.k0 * c.k0);
+  hipLaunchKernelGGL(k_sigma<T>, gridnn(n, n), dim3(256), 0, stream(), n, c.S1, c.S2, c.Di,
+                     g.geo.nvec, c.eps1, c.eps2, cc);
+  MNP_HIP(hipGetLastError());
+  lu(c.S2, c.sigS, "Sigma");
+}
+
+// ------------------------------------------------------------- mldivide
+// Vectors are n x (8 p) blocks.  b = [phi (p) | a (3p) | De (p) | alpha (3p)],
+// x = [sig1 (p) | h1 (3p) | sig2 (p) | h2 (3p)]; vector columns are ordered
+// component-fastest (c + 3 q), as Matlab's n x 3 x npol arrays.
+
+// alpha <- alpha - Y_a + i k L1 n (x) phi ;  De <- De - L1 Y_phi + i k L1 n . a
+template <class T>
+__global__ void k_mld1(int64_t n, int64_t p, const cplx<T>* b, const cplx<T>* Y, cplx<T>* alpha,
+                       cplx<T>* De, const double* nvec, double k, double2 L1) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  const cplx<T>* phi = b;
+  const cplx<T>* a = b + n * p;
+  const cplx<T>* De0 = b + 4 * n * p;
+  const cplx<T>* al0 = b + 5 * n * p;
+  const double2 ikL1 = cmul<double>(mk<double>(0, k), L1);
+  const cplx<T> ph = phi[q * n + i];
+  const double2 ikL1phi = cmul<double>(ikL1, mk<double>(ph.x, ph.y));
+  double2 na = mk<double>(0, 0);
+  for (int c = 0; c < 3; ++c) {
+    const int64_t col = c + 3 * q;
+    const double nc = nvec[c * n + i];
+    const cplx<T> ya = Y[col * n + i], a0 = al0[col * n + i], ac = a[col * n + i];
+    alpha[col * n + i] = mk<T>((T)(a0.x - ya.x + nc * ikL1phi.x), (T)(a0.y - ya.y + nc * ikL1phi.y));
+    na.x += nc * ac.x;
+    na.y += nc * ac.y;
+  }
+  const cplx<T> yp = Y[(3 * p + q) * n + i], d0 = De0[q * n + i];
+  const double2 l1y = cmul<double>(L1, mk<double>(yp.x, yp.y));
+  const double2 t = cmul<double>(ikL1, na);
+  De[q * n + i] = mk<T>((T)(d0.x - l1y.x + t.x), (T)(d0.y - l1y.y + t.y));
+}
+// rhs = De + i k (L1 - L2) n . t
+template <class T>
+__global__ void k_mld2(int64_t n, int64_t p, const cplx<T>* De, const cplx<T>* t, cplx<T>* rhs,
+                       const double* nvec, double2 ikdL) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  double2 s = mk<double>(0, 0);
+  for (int c = 0; c < 3; ++c) {
+    const cplx<T> v = t[(c + 3 * q) * n + i];
+    s.x += nvec[c * n + i] * v.x;
+    s.y += nvec[c * n + i] * v.y;
+  }
+  const double2 u = cmul<double>(ikdL, s);
+  const cplx<T> d = De[q * n + i];
+  rhs[q * n + i] = mk<T>((T)(d.x + u.x), (T)(d.y + u.y));
+}
+// u = i k (L1 - L2) n (x) sig2 + alpha
+template <class T>
+__global__ void k_mld3(int64_t n, int64_t p, const cplx<T>* sig2, const cplx<T>* alpha, cplx<T>* u,
+                       const double* nvec, double2 ikdL) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  const cplx<T> s = sig2[q * n + i];
+  const double2 w = cmul<double>(ikdL, mk<double>(s.x, s.y));
+  for (int c = 0; c < 3; ++c) {
+    const int64_t col = c + 3 * q;
+    const cplx<T> a = alpha[col * n + i];
+    const double nc = nvec[c * n + i];
+    u[col * n + i] = mk<T>((T)(a.x + nc * w.x), (T)(a.y + nc * w.y));
+  }
+}
+// V1 = [sig2 + phi | h2 + a] ; V2 = [sig2 | h2]   (n x 4p each)
+template <class T>
+__global__ void k_mld4(int64_t n, int64_t p, const cplx<T>* b, const cplx<T>* sig2,
+                       const cplx<T>* h2, cplx<T>* V1, cplx<T>* V2) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t col = blockIdx.y;  // 0 .. 4p
+  if (i >= n) return;
+  const cplx<T> src = col < p ? sig2[col * n + i] : h2[(col - p) * n + i];
+  const cplx<T> add = b[col * n + i];  // phi then a
+  V1[col * n + i] = cadd<T>(src, add);
+  V2[col * n + i] = src;
+}
+
+// x = M^{-1} b for the chain c (precision T), b and x n x 8p in T.
+template <class T>
+static void chain_apply(Chain<T>& c, const GreenRet& g, int64_t p, const cplx<T>* b, cplx<T>* x,
+                        cplx<T>* tmp) {
+  const int64_t n = c.n;
+  const double k = c.k0;
+  const double2 L1 = c.eps1, L2 = c.eps2;
+  const double2 ikdL = cmul<double>(mk<double>(0, k), csub<double>(L1, L2));
+  // tmp layout: Y (4p) | alpha (3p) | De (p) | t (3p) | rhs/sig2 (p) | u (3p) | h2 (3p)
+  cplx<T>* Y = tmp;
+  cplx<T>* alpha = Y + 4 * p * n;
+  cplx<T>* De = alpha + 3 * p * n;
+  cplx<T>* t = De + p * n;
+  cplx<T>* s2 = t + 3 * p * n;
+  cplx<T>* u = s2 + p * n;
+  cplx<T>* h2 = u + 3 * p * n;
+  const double2 one = {1, 0}, zero = {0, 0};
+  // Y = Sigma1 [a | phi]: a occupies b columns p..4p, phi columns 0..p
+  gemm<T>(n, 3 * p, n, one, c.v(c.S1), MatView<T>{const_cast<cplx<T>*>(b) + n * p, n}, zero,
+          MatView<T>{Y, n});
+  gemm<T>(n, p, n, one, c.v(c.S1), MatView<T>{const_cast<cplx<T>*>(b), n}, zero,
+          MatView<T>{Y + 3 * p * n, n});
+  const dim3 gp((unsigned)cdiv_up(n, 256), (unsigned)p);
+  hipLaunchKernelGGL(k_mld1<T>, gp, dim3(256), 0, stream(), n, p, b, Y, alpha, De, g.geo.nvec, k, L1);
+  gemm<T>(n, 3 * p, n, one, c.v(c.Di), MatView<T>{alpha, n}, zero, MatView<T>{t, n});
+  hipLaunchKernelGGL(k_mld2<T>, gp, dim3(256), 0, stream(), n, p, De, t, s2, g.geo.nvec, ikdL);
+  getrs<T>(n, p, c.v(c.S2), c.sigS, MatView<T>{s2, n}, c.ws);  // sig2 = Sigma^{-1} rhs
+  hipLaunchKernelGGL(k_mld3<T>, gp, dim3(256), 0, stream(), n, p, s2, alpha, u, g.geo.nvec, ikdL);
+  gemm<T>(n, 3 * p, n, one, c.v(c.Di), MatView<T>{u, n}, zero, MatView<T>{h2, n});
+  // V1 -> x[0:4p] (sig1 | h1), V2 -> x[4p:8p] (sig2 | h2)
+  hipLaunchKernelGGL(k_mld4<T>, dim3((unsigned)cdiv_up(n, 256), (unsigned)(4 * p)), dim3(256), 0,
+                     stream(), n, p, b, s2, h2, x, x + 4 * p * n);
+  MNP_HIP(hipGetLastError());
+  getrs<T>(n, 4 * p, c.v(c.G1), c.sig1, MatView<T>{x, n}, c.ws);
+  getrs<T>(n, 4 * p, c.v(c.G2), c.sig2, MatView<T>{x + 4 * p * n, n}, c.ws);
+}
+
+// b = [phi | a | De | alpha] from the excitation potentials (excitation.m:21-29).
+__global__ void k_excitation(int64_t n, int64_t p, const double2* phi1, const double2* phi1p,
+                             const double2* a1, const double2* a1p, const double2* phi2,
+                             const double2* phi2p, const double2* a2, const double2* a2p,
+                             double2* b, const double* nvec, double k, double2 e1, double2 e2) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  const double2 z = mk<double>(0, 0);
+  auto ld = [&](const double2* v, int64_t col) { return v ? v[col * n + i] : z; };
+  const double2 p1 = ld(phi1, q), p2 = ld(phi2, q);
+  const double2 ik = mk<double>(0, k);
+  b[q * n + i] = csub<double>(p2, p1);
+  double2 na1 = z, na2 = z;
+  const double2 e2p2 = cmul<double>(e2, p2), e1p1 = cmul<double>(e1, p1);
+  for (int c = 0; c < 3; ++c) {
+    const int64_t col = c + 3 * q;
+    const double nc = nvec[c * n + i];
+    const double2 va1 = ld(a1, col), va2 = ld(a2, col);
+    b[(p + col) * n + i] = csub<double>(va2, va1);
+    const double2 w = csub<double>(cscale<double>(e2p2, nc), cscale<double>(e1p1, nc));
+    b[(5 * p + col) * n + i] = csub<double>(csub<double>(ld(a2p, col), ld(a1p, col)), cmul<double>(ik, w));
+    na1 = cadd<double>(na1, cscale<double>(va1, nc));
+    na2 = cadd<double>(na2, cscale<double>(va2, nc));
+  }
+  const double2 t = csub<double>(cmul<double>(e2, na2), cmul<double>(e1, na1));
+  b[(4 * p + q) * n + i] = csub<double>(
+      csub<double>(cmul<double>(e2, ld(phi2p, q)), cmul<double>(e1, ld(phi1p, q))), cmul<double>(ik, t));
+}
+
+// r = b - A x with the exact operator (afun.m): GX1 = G1 [sig1|h1],
+// GX2 = G2 [sig2|h2], HX1 = H1 [sig1|h1], HX2 = H2 [sig2|h2]  (each n x 4p).
+__global__ void k_residual(int64_t n, int64_t p, const double2* b, const double2* GX1,
+                           const double2* GX2, const double2* HX1, const double2* HX2, double2* r,
+                           const double* nvec, double k, double2 e1, double2 e2) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  const double2 ik = mk<double>(0, k);
+  const double2 Gs1 = GX1[q * n + i], Gs2 = GX2[q * n + i];
+  const double2 Hs1 = HX1[q * n + i], Hs2 = HX2[q * n + i];
+  // phi
+  r[q * n + i] = csub<double>(b[q * n + i], csub<double>(Gs1, Gs2));
+  const double2 w = csub<double>(cmul<double>(e1, Gs1), cmul<double>(e2, Gs2));
+  double2 nGh = mk<double>(0, 0);
+  for (int c = 0; c < 3; ++c) {
+    const int64_t col = c + 3 * q;
+    const double nc = nvec[c * n + i];
+    const double2 Gh1 = GX1[(p + col) * n + i], Gh2 = GX2[(p + col) * n + i];
+    const double2 Hh1 = HX1[(p + col) * n + i], Hh2 = HX2[(p + col) * n + i];
+    r[(p + col) * n + i] = csub<double>(b[(p + col) * n + i], csub<double>(Gh1, Gh2));
+    const double2 al = csub<double>(csub<double>(Hh1, Hh2), cmul<double>(ik, cscale<double>(w, nc)));
+    r[(5 * p + col) * n + i] = csub<double>(b[(5 * p + col) * n + i], al);
+    nGh = cadd<double>(nGh, cscale<double>(csub<double>(cmul<double>(e1, Gh1), cmul<double>(e2, Gh2)), nc));
+  }
+  const double2 De = csub<double>(csub<double>(cmul<double>(e1, Hs1), cmul<double>(e2, Hs2)),
+                                  cmul<double>(ik, nGh));
+  r[(4 * p + q) * n + i] = csub<double>(b[(4 * p + q) * n + i], De);
+}
+
+__global__ void k_norm2(int64_t len, const double2* v, double* out) {
+  __shared__ double red[256];
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += v[i].x * v[i].x + v[i].y * v[i].y;
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+__global__ void k_axpy(int64_t len, double2* x, const double2* d) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < len) x[i] = cadd<double>(x[i], d[i]);
+}
+__global__ void k_unpack(int64_t n, int64_t p, const double2* x, double2* sig1, double2* h1,
+                         double2* sig2, double2* h2) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t col = blockIdx.y;  // 0 .. 8p
+  if (i >= n) return;
+  const double2 v = x[col * n + i];
+  if (col < p) {
+    if (sig1) sig1[col * n + i] = v;
+  } else if (col < 4 * p) {
+    if (h1) h1[(col - p) * n + i] = v;
+  } else if (col < 5 * p) {
+    if (sig2) sig2[(col - 4 * p) * n + i] = v;
+  } else {
+    if (h2) h2[(col - 5 * p) * n + i] = v;
+  }
+}
+
+struct BemRet {
+  const GreenRet* g;
+  int solver;
+  int64_t n;
+  bool ready = false;
+  double enei = 0;
+  double2 eps1{}, eps2{};
+  std::unique_ptr<Chain<double>> c64;
+  std::unique_ptr<Chain<float>> c32;
+  double2 *G1 = nullptr, *G2 = nullptr, *H1 = nullptr, *H2 = nullptr;  // FP64 operator (mixed)
+  Workspace vws;  // vectors
+  double tol = 1e-12;
+  int maxit = 10;
+  int last_iter = 0;
+  double last_relres = 0;
+
+  BemRet(const GreenRet* g_, int solver_) : g(g_), solver(solver_), n(g_->n) {
+    if (solver == MNP_SOLVER_FP64) {
+      c64.reset(new Chain<double>(n));
+    } else {
+      c32.reset(new Chain<float>(n));
+      G1 = dev_alloc<double2>(n * n);
+      G2 = dev_alloc<double2>(n * n);
+      H1 = dev_alloc<double2>(n * n);
+      H2 = dev_alloc<double2>(n * n);
+    }
+  }
+  ~BemRet() {
+    void* ps[] = {G1, G2, H1, H2};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
+
+  void initmat(double enei_, double2 e1, double2 e2) {
+    enei = enei_;
+    eps1 = e1;
+    eps2 = e2;
+    const double k0 = 2 * M_PI / enei;
+    // k = 2 pi / enei sqrt(eps), principal root (Material/@epsdrude/subsref.m:24)
+    auto ksq = [&](double2 e) {
+      const double r = std::hypot(e.x, e.y);
+      double sr = std::sqrt(0.5 * (r + e.x)), si = std::sqrt(0.5 * (r - e.x));
+      if (e.y < 0) si = -si;
+      return mk<double>(k0 * sr, k0 * si);
+    };
+    const double2 k1 = ksq(e1), k2 = ksq(e2);
+    if (solver == MNP_SOLVER_FP64) {
+      Chain<double>& c = *c64;
+      c.eps1 = e1;
+      c.eps2 = e2;
+      c.k0 = k0;
+      g->assemble(k1, 2 * M_PI, OutPair{c.G1, nullptr}, OutPair{c.S1, nullptr}, n);
+      g->assemble(k2, -2 * M_PI, OutPair{c.G2, nullptr}, OutPair{c.S2, nullptr}, n);
+      chain_build<double>(c, *g);
+    } else {
+      Chain<float>& c = *c32;
+      c.eps1 = e1;
+      c.eps2 = e2;
+      c.k0 = k0;
+      g->assemble(k1, 2 * M_PI, OutPair{G1, c.G1}, OutPair{H1, c.S1}, n);
+      g->assemble(k2, -2 * M_PI, OutPair{G2, c.G2}, OutPair{H2, c.S2}, n);
+      chain_build<float>(c, *g);
+    }
+    ready = true;
+  }
+
+  double norm2(const double2* v, int64_t len) {
+    const int nb = 512;
+    double* part = (double*)vws.get(5, sizeof(double) * nb);
+    hipLaunchKernelGGL(k_norm2, dim3(nb), dim3(256), 0, stream(), len, v, part);
+    std::vector<double> h(nb);
+    MNP_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipStreamSynchronize(stream()));
+    double s = 0;
+    for (double v2 : h) s += v2;
+    return std::sqrt(s);
+  }
+
+  void solve(int64_t p, const double2* const in[8], double2* sig1, double2* sig2, double2* h1,
+             double2* h2) {
+    MNP_CHECK(ready, MNP_ERR_STATE, "bemret_solve: initmat not called");
+    MNP_CHECK(p > 0 && p <= 64, MNP_ERR_ARG, "bemret_solve: npol must be in 1..64");
+    const int64_t len = 8 * p * n;
+    const double k0 = 2 * M_PI / enei;
+    double2* b = (double2*)vws.get(0, sizeof(double2) * len);
+    double2* x = (double2*)vws.get(1, sizeof(double2) * len);
+    hipLaunchKernelGGL(k_excitation, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0,
+                       stream(), n, p, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], b,
+                       g->geo.nvec, k0, eps1, eps2);
+    MNP_HIP(hipGetLastError());
+    const size_t tmp_elems = (size_t)(18 * p * n);
+    last_iter = 0;
+    last_relres = 0;
+    if (solver == MNP_SOLVER_FP64) {
+      double2* tmp = (double2*)vws.get(2, sizeof(double2) * tmp_elems);
+      chain_apply<double>(*c64, *g, p, b, x, tmp);
+    } else {
+      Chain<float>& c = *c32;
+      float2* b32 = (float2*)vws.get(2, sizeof(float2) * len);
+      float2* x32 = (float2*)vws.get(3, sizeof(float2) * len);
+      float2* tmp = (float2*)vws.get(4, sizeof(float2) * tmp_elems);
+      // residual buffers: 4 products (n x 4p) + r (n x 8p), FP64
+      double2* work = (double2*)res_ws.get(0, sizeof(double2) * (size_t)(24 * p * n));
+      double2 *GX1 = work, *GX2 = work + 4 * p * n, *HX1 = work + 8 * p * n, *HX2 = work + 12 * p * n,
+              *rr = work + 16 * p * n;
+      auto precond = [&](const double2* src, double2* dst) {
+        convert_mat<double, float>(n, 8 * p, MatView<double>{const_cast<double2*>(src), n},
+                                   MatView<float>{b32, n});
+        chain_apply<float>(c, *g, p, b32, x32, tmp);
+        convert_mat<float, double>(n, 8 * p, MatView<float>{x32, n}, MatView<double>{dst, n});
+      };
+      precond(b, x);
+      const double bn = norm2(b, len);
+      const double2 one = {1, 0}, zero = {0, 0};
+      for (int it = 0;; ++it) {
+        gemm<double>(n, 4 * p, n, one, MatView<double>{G1, n}, MatView<double>{x, n}, zero,
+                     MatView<double>{GX1, n});
+        gemm<double>(n, 4 * p, n, one, MatView<double>{G2, n}, MatView<double>{x + 4 * p * n, n}, zero,
+                     MatView<double>{GX2, n});
+        gemm<double>(n, 4 * p, n, one, MatView<double>{H1, n}, MatView<double>{x, n}, zero,
+                     MatView<double>{HX1, n});
+        gemm<double>(n, 4 * p, n, one, MatView<double>{H2, n}, MatView<double>{x + 4 * p * n, n}, zero,
+                     MatView<double>{HX2, n});
+        hipLaunchKernelGGL(k_residual, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0,
+                           stream(), n, p, b, GX1, GX2, HX1, HX2, rr, g->geo.nvec, k0, eps1, eps2);
+        MNP_HIP(hipGetLastError());
+        const double rn = norm2(rr, len);
+        last_relres = bn > 0 ? rn / bn : 0;
+        last_iter = it;
+        if (!(last_relres > tol) || it >= maxit) break;
+        precond(rr, GX1);  // correction into GX1 (n x 8p fits in GX1..GX2)
+        hipLaunchKernelGGL(k_axpy, dim3((unsigned)cdiv_up(len, 256)), dim3(256), 0, stream(), len, x,
+                           GX1);
+        MNP_HIP(hipGetLastError());
+      }
+      MNP_CHECK(!(last_relres > tol), MNP_ERR_NOCONV,
+                "bemret_solve: refinement stalled at relative residual " + std::to_string(last_relres));
+    }
+    hipLaunchKernelGGL(k_unpack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0,
+                       stream(), n, p, x, sig1, h1, sig2, h2);
+    MNP_HIP(hipGetLastError());
+  }
+  Workspace res_ws;
+};
+
+// ---------------------------------------------------- excitation / output
+// a2 = e^{i k pos.dir}/(i k0) pol ; a2p = (i k n.dir) a2  (planewaveret/potential.m:72-76)
+__global__ void k_planewave(int64_t n, int64_t p, const double* pos, const double* nvec,
+                            const double* pol, const double* dir, double2 k, double k0, double2* a,
+                            double2* ap) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  double pd = 0, nd = 0;
+  for (int c = 0; c < 3; ++c) {
+    pd += pos[c * n + i] * dir[c * p + q];
+    nd += nvec[c * n + i] * dir[c * p + q];
+  }
+  const double2 ph0 = cexp_ikd(k, pd);  // exp(i k pd) with pd possibly negative
+  const double2 ph = cdiv<double>(ph0, mk<double>(0, k0));
+  const double2 ikn = cmul<double>(mk<double>(0, nd), k);
+  const double2 php = cmul<double>(ikn, ph);
+  for (int c = 0; c < 3; ++c) {
+    const double pc = pol[c * p + q];
+    a[(c + 3 * q) * n + i] = cscale<double>(ph, pc);
+    ap[(c + 3 * q) * n + i] = cscale<double>(php, pc);
+  }
+}
+
+// Far field (spectrumret/farfield.m:59-84) of the outside charges/currents,
+// one workgroup per direction:
+//   e = i k0 sum_j ph_j h2_j - i k dir sum_j ph_j sig2_j,
+//   h = i k dir x sum_j ph_j h2_j,   ph_j = e^{-i k dir.pos_j} area_j.
+constexpr int FF_MAXP = 8;
+__global__ void __launch_bounds__(256) k_farfield(int64_t n, int64_t p, int64_t ndir, const double* dir,
+                                                  const double* pos, const double* area,
+                                                  const double2* sig2, const double2* h2, double2 k,
+                                                  double k0, double2* e, double2* h) {
+  const int64_t d = blockIdx.x;
+  const double dx = dir[d], dy = dir[ndir + d], dz = dir[2 * ndir + d];
+  double2 acc[FF_MAXP][4];
+  for (int q = 0; q < FF_MAXP; ++q)
+    for (int c = 0; c < 4; ++c) acc[q][c] = mk<double>(0, 0);
+  const double2 mk_ = mk<double>(-k.x, -k.y);
+  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+    const double pd = dx * pos[j] + dy * pos[n + j] + dz * pos[2 * n + j];
+    const double2 ph = cscale<double>(cexp_ikd(mk_, pd), area[j]);
+    for (int q = 0; q < p && q < FF_MAXP; ++q) {
+      for (int c = 0; c < 3; ++c) acc[q][c] = cadd<double>(acc[q][c], cmul<double>(ph, h2[(c + 3 * q) * n + j]));
+      acc[q][3] = cadd<double>(acc[q][3], cmul<double>(ph, sig2[q * n + j]));
+    }
+  }
+  __shared__ double2 red[256];
+  for (int q = 0; q < p && q < FF_MAXP; ++q)
+    for (int c = 0; c < 4; ++c) {
+      red[threadIdx.x] = acc[q][c];
+      __syncthreads();
+      for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = cadd<double>(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) acc[q][c] = red[0];
+      __syncthreads();
+    }
+  if (threadIdx.x == 0) {
+    const double2 ik0 = mk<double>(0, k0), ik = mk<double>(-k.y, k.x);
+    const double dv[3] = {dx, dy, dz};
+    for (int q = 0; q < p && q < FF_MAXP; ++q) {
+      const double2 s = acc[q][3];
+      for (int c = 0; c < 3; ++c) {
+        const double2 ev = csub<double>(cmul<double>(ik0, acc[q][c]), cscale<double>(cmul<double>(ik, s), dv[c]));
+        e[d + ndir * (c + 3 * q)] = ev;
+      }
+      // dir x A
+      const double2 A0 = acc[q][0], A1 = acc[q][1], A2 = acc[q][2];
+      const double2 cx = csub<double>(cscale<double>(A2, dy), cscale<double>(A1, dz));
+      const double2 cy = csub<double>(cscale<double>(A0, dz), cscale<double>(A2, dx));
+      const double2 cz = csub<double>(cscale<double>(A1, dx), cscale<double>(A0, dy));
+      h[d + ndir * (0 + 3 * q)] = cmul<double>(ik, cx);
+      h[d + ndir * (1 + 3 * q)] = cmul<double>(ik, cy);
+      h[d + ndir * (2 + 3 * q)] = cmul<double>(ik, cz);
+    }
+  }
+}
+
+static double2 medium_k(double enei, mnp_complex em) {
+  const double k0 = 2 * M_PI / enei;
+  const double r = std::hypot(em.re, em.im);
+  double sr = std::sqrt(0.5 * (r + em.re)), si = std::sqrt(0.5 * (r - em.re));
+  if (em.im < 0) si = -si;
+  return mk<double>(k0 * sr, k0 * si);
+}
+
+static void farfield(const GreenRet& g, double enei, mnp_complex em, int64_t ndir, const double* dir,
+                     int64_t p, const double2* sig2, const double2* h2, double2* e, double2* h) {
+  MNP_CHECK(p >= 1 && p <= FF_MAXP, MNP_ERR_ARG, "farfield: npol must be in 1..8");
+  const double2 k = medium_k(enei, em);
+  hipLaunchKernelGGL(k_farfield, dim3((unsigned)ndir), dim3(256), 0, stream(), g.n, p, ndir, dir,
+                     g.geo.pos, g.geo.area, sig2, h2, k, 2 * M_PI / enei, e, h);
+  MNP_HIP(hipGetLastError());
+}
+
+}  // namespace mnp
+
+// ============================================================== C ABI
+using namespace mnp;
+
+struct mnp_bemret_s {
+  BemRet* b;
+};
+
+extern "C" {
+
+int mnp_bemret_create(mnp_greenret_t g, int solver, mnp_bemret_t* out) {
+  return guarded([&] {
+    MNP_CHECK(g && out, MNP_ERR_ARG, "bemret_create: NULL argument");
+    MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED, MNP_ERR_ARG,
+              "bemret_create: unknown solver");
+    auto* h = new mnp_bemret_s{nullptr};
+    try {
+      h->b = new BemRet(g->g, solver);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int mnp_bemret_destroy(mnp_bemret_t bem) {
+  return guarded([&] {
+    if (!bem) return;
+    delete bem->b;
+    delete bem;
+  });
+}
+
+int mnp_bemret_initmat(mnp_bemret_t bem, double enei, mnp_complex eps1, mnp_complex eps2) {
+  return guarded([&] {
+    MNP_CHECK(bem, MNP_ERR_ARG, "bemret_initmat: NULL handle");
+    MNP_CHECK(enei > 0, MNP_ERR_ARG, "bemret_initmat: wavelength must be positive");
+    bem->b->initmat(enei, to_d2(eps1), to_d2(eps2));
+  });
+}
+
+int mnp_bemret_solve(mnp_bemret_t bem, int64_t npol, const void* phi1, const void* phi1p,
+                     const void* a1, const void* a1p, const void* phi2, const void* phi2p,
+                     const void* a2, const void* a2p, void* sig1, void* sig2, void* h1, void* h2) {
+  return guarded([&] {
+    MNP_CHECK(bem, MNP_ERR_ARG, "bemret_solve: NULL handle");
+    const double2* in[8] = {(const double2*)phi1, (const double2*)phi1p, (const double2*)a1,
+                            (const double2*)a1p,  (const double2*)phi2,  (const double2*)phi2p,
+                            (const double2*)a2,   (const double2*)a2p};
+    bem->b->solve(npol, in, (double2*)sig1, (double2*)sig2, (double2*)h1, (double2*)h2);
+  });
+}
+
+int mnp_bemret_stats(mnp_bemret_t bem, int* iterations, double* relres) {
+  return guarded([&] {
+    MNP_CHECK(bem, MNP_ERR_ARG, "bemret_stats: NULL handle");
+    if (iterations) *iterations = bem->b->last_iter;
+    if (relres) *relres = bem->b->last_relres;
+  });
+}
+
+int mnp_bemret_set_refinement(mnp_bemret_t bem, double tol, int maxit) {
+  return guarded([&] {
+    MNP_CHECK(bem && tol > 0 && maxit >= 0, MNP_ERR_ARG, "bemret_set_refinement: bad argument");
+    bem->b->tol = tol;
+    bem->b->maxit = maxit;
+  });
+}
+
+int mnp_planewave_potential(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t npol,
+                            const double* pol, const double* dir, void* a2, void* a2p) {
+  return guarded([&] {
+    MNP_CHECK(g && pol && dir && a2 && a2p, MNP_ERR_ARG, "planewave_potential: NULL argument");
+    MNP_CHECK(npol >= 1 && npol <= 64, MNP_ERR_ARG, "planewave_potential: npol out of range");
+    for (int64_t q = 0; q < npol; ++q) {
+      double s = 0;
+      for (int c = 0; c < 3; ++c) s += pol[c * npol + q] * dir[c * npol + q];
+      MNP_CHECK(s == 0, MNP_ERR_ARG, "planewave_potential: polarisation not perpendicular to direction");
+    }
+    const GreenRet& G = *g->g;
+    double* d = dev_alloc<double>(6 * npol);
+    MNP_HIP(hipMemcpyAsync(d, pol, sizeof(double) * 3 * npol, hipMemcpyHostToDevice, stream()));
+    MNP_HIP(hipMemcpyAsync(d + 3 * npol, dir, sizeof(double) * 3 * npol, hipMemcpyHostToDevice, stream()));
+    hipLaunchKernelGGL(k_planewave, dim3((unsigned)cdiv_up(G.n, 256), (unsigned)npol), dim3(256), 0,
+                       stream(), G.n, npol, G.geo.pos, G.geo.nvec, d, d + 3 * npol,
+                       medium_k(enei, eps_medium), 2 * M_PI / enei, (double2*)a2, (double2*)a2p);
+    MNP_HIP(hipGetLastError());
+    MNP_HIP(hipStreamSynchronize(stream()));
+    MNP_HIP(hipFree(d));
+  });
+}
+
+int mnp_farfield(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t ndir, const void* dir,
+                 int64_t npol, const void* sig2, const void* h2, void* e, void* h) {
+  return guarded([&] {
+    MNP_CHECK(g && dir && sig2 && h2 && e && h, MNP_ERR_ARG, "farfield: NULL argument");
+    farfield(*g->g, enei, eps_medium, ndir, (const double*)dir, npol, (const double2*)sig2,
+             (const double2*)h2, (double2*)e, (double2*)h);
+  });
+}
+
+int mnp_planewave_cross_sections(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t npol,
+                                 const double* pol, const double* dir, int64_t npinf,
+                                 const void* pinfty_nvec, const void* pinfty_area, const void* sig2,
+                                 const void* h2, double* ext, double* sca, double* abs_) {
+  return guarded([&] {
+    MNP_CHECK(g && pol && dir && pinfty_nvec && pinfty_area && sig2 && h2, MNP_ERR_ARG,
+              "cross_sections: NULL argument");
+    MNP_CHECK(npol >= 1 && npol <= FF_MAXP, MNP_ERR_ARG, "cross_sections: npol must be in 1..8");
+    const GreenRet& G = *g->g;
+    const int64_t nd = npol + npinf;
+    // directions: the npol propagation directions, then the pinfty normals
+    double* ddir = dev_alloc<double>(3 * nd);
+    for (int c = 0; c < 3; ++c) {
+      MNP_HIP(hipMemcpyAsync(ddir + c * nd, dir + c * npol, sizeof(double) * npol, hipMemcpyHostToDevice,
+                             stream()));
+      MNP_HIP(hipMemcpyAsync(ddir + c * nd + npol, (const double*)pinfty_nvec + c * npinf,
+                             sizeof(double) * npinf, hipMemcpyDeviceToDevice, stream()));
+    }
+    double2* ef = dev_alloc<double2>(nd * 3 * npol);
+    double2* hf = dev_alloc<double2>(nd * 3 * npol);
+    farfield(G, enei, eps_medium, nd, ddir, npol, (const double2*)sig2, (const double2*)h2, ef, hf);
+    std::vector<double2> he(nd * 3 * npol), hh(nd * 3 * npol);
+    std::vector<double> pn(3 * npinf), pa(npinf);
+    MNP_HIP(hipMemcpyAsync(he.data(), ef, sizeof(double2) * he.size(), hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipMemcpyAsync(hh.data(), hf, sizeof(double2) * hh.size(), hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipMemcpyAsync(pn.data(), pinfty_nvec, sizeof(double) * 3 * npinf, hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipMemcpyAsync(pa.data(), pinfty_area, sizeof(double) * npinf, hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipStreamSynchronize(stream()));
+    MNP_HIP(hipFree(ddir));
+    MNP_HIP(hipFree(ef));
+    MNP_HIP(hipFree(hf));
+    const double2 k = medium_k(enei, eps_medium);
+    const double nb = std::sqrt(std::hypot(eps_medium.re, eps_medium.im));  // |sqrt(eps)| (lossless medium)
+    for (int64_t q = 0; q < npol; ++q) {
+      // extinction: 4 pi / k Im(pol . e(dir_q))  (planewaveret/extinction.m:12-16)
+      double2 pe = mk<double>(0, 0);
+      for (int c = 0; c < 3; ++c) pe = cadd<double>(pe, cscale<double>(he[q + nd * (c + 3 * q)], pol[c * npol + q]));
+      const double e_val = 4 * M_PI / k.x * pe.y;
+      // scattering: sum area 0.5 Re(n . (e x conj h)) / (0.5 nb)  (scattering.m:20-29)
+      double s = 0;
+      for (int64_t d = 0; d < npinf; ++d) {
+        const int64_t dd = npol + d;
+        double2 E[3], H[3];
+        for (int c = 0; c < 3; ++c) {
+          E[c] = he[dd + nd * (c + 3 * q)];
+          H[c] = hh[dd + nd * (c + 3 * q)];
+          H[c].y = -H[c].y;
+        }
+        const double2 x = csub<double>(cmul<double>(E[1], H[2]), cmul<double>(E[2], H[1]));
+        const double2 y = csub<double>(cmul<double>(E[2], H[0]), cmul<double>(E[0], H[2]));
+        const double2 z = csub<double>(cmul<double>(E[0], H[1]), cmul<double>(E[1], H[0]));
+        const double ds = 0.5 * (pn[d] * x.x + pn[npinf + d] * y.x + pn[2 * npinf + d] * z.x);
+        s += pa[d] * ds;
+      }
+      const double s_val = s / (0.5 * nb);
+      if (ext) ext[q] = e_val;
+      if (sca) sca[q] = s_val;
+      if (abs_) abs_[q] = e_val - s_val;
+    }
+  });
+}
+
+}  // extern "C"

@@ -10,7 +10,6 @@
 // a coalesced column-major streaming kernel for the O(n^2) assembly.
 #include <algorithm>
 #include <cmath>
-#include <mutex>
 #include <vector>
 
 #include "mnp_greenret.hpp"
@@ -512,9 +511,6 @@ void GreenRet::assemble(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t 
   }
 }
 
-static std::once_flag g_tables_once;
-static int g_npol_points = 0;
-
 GreenRet::GreenRet(const double* verts, int64_t nv, const int64_t* faces, int64_t nf,
                    const mnp_green_options& op_)
     : n(nf), op(op_) {
@@ -610,10 +606,6 @@ GreenRet::~GreenRet() {
 
 // ============================================================== C ABI
 using namespace mnp;
-
-struct mnp_greenret_s {
-  GreenRet* g;
-};
 
 extern "C" {
 

@@ -1,0 +1,783 @@
+// Dense complex linear algebra for the bemret chain on gfx950.
+//
+// - k_cgemm: complex GEMM on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32),
+//   64x64 output tile per 256-thread workgroup, 2x2 waves of 32x32, LDS-staged
+//   16-deep k slices with register prefetch of the next slice.
+// - k_getf2_coop: cooperative (grid-synchronised) LU of a tall panel with
+//   partial pivoting; the panel stays in LDS across the grid and each column
+//   costs one grid barrier (candidate rows are published with their values).
+// - recursive LU / triangular solves: all O(n^3) work lands in k_cgemm.
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "mnp_linalg.hpp"
+
+namespace cg = cooperative_groups;
+
+namespace mnp {
+
+// ------------------------------------------------------------ MFMA traits
+template <class T> struct MF;
+template <> struct MF<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct MF<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <class T> __device__ __forceinline__ cplx<T> czero() { return mk<T>(T(0), T(0)); }
+
+// ------------------------------------------------------------------ GEMM
+constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 1;
+
+// C = alpha A B + beta C.  Operand layout of the 16x16x4 MFMA: lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16]; it holds the four
+// results D[4 (l / 16) + r][l % 16], r = 0..3.
+template <class T>
+__global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, cplx<T> alpha,
+                                               const cplx<T>* __restrict__ A, int64_t lda,
+                                               const cplx<T>* __restrict__ B, int64_t ldb,
+                                               cplx<T> beta, int beta_zero, cplx<T>* C,
+                                               int64_t ldc) {
+  using acc_t = typename MF<T>::acc_t;
+  __shared__ T sAr[GBK][GBM + GPAD], sAi[GBK][GBM + GPAD];
+  __shared__ T sBr[GBK][GBN + GPAD], sBi[GBK][GBN + GPAD];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid & 1, wn = wid >> 1;
+  const int64_t m0 = (int64_t)blockIdx.x * GBM, n0 = (int64_t)blockIdx.y * GBN;
+
+  acc_t cr[2][2], ci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      cr[a][b] = acc_t{0, 0, 0, 0};
+      ci[a][b] = acc_t{0, 0, 0, 0};
+    }
+  cplx<T> ra[4], rb[4];
+  auto loadA = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e, m = idx & 63, k = idx >> 6;
+      const int64_t gm = m0 + m, gk = k0 + k;
+      ra[e] = (gm < M && gk < K) ? A[gk * lda + gm] : czero<T>();
+    }
+  };
+  auto loadB = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e, k = idx & 15, n = idx >> 4;
+      const int64_t gk = k0 + k, gn = n0 + n;
+      rb[e] = (gk < K && gn < N) ? B[gn * ldb + gk] : czero<T>();
+    }
+  };
+  loadA(0);
+  loadB(0);
+  for (int64_t k0 = 0; k0 < K; k0 += GBK) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      sAr[idx >> 6][idx & 63] = ra[e].x;
+      sAi[idx >> 6][idx & 63] = ra[e].y;
+      sBr[idx & 15][idx >> 4] = rb[e].x;
+      sBi[idx & 15][idx >> 4] = rb[e].y;
+    }
+    __syncthreads();
+    if (k0 + GBK < K) {
+      loadA(k0 + GBK);
+      loadB(k0 + GBK);
+    }
+#pragma unroll
+    for (int kk = 0; kk < GBK / 4; ++kk) {
+      const int kr = kk * 4 + (lane >> 4), cl = lane & 15;
+      T ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        ar[a] = sAr[kr][wm * 32 + a * 16 + cl];
+        ai[a] = sAi[kr][wm * 32 + a * 16 + cl];
+        nai[a] = -ai[a];
+        br[a] = sBr[kr][wn * 32 + a * 16 + cl];
+        bi[a] = sBi[kr][wn * 32 + a * 16 + cl];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          cr[a][b] = MF<T>::mma(ar[a], br[b], cr[a][b]);
+          cr[a][b] = MF<T>::mma(nai[a], bi[b], cr[a][b]);
+          ci[a][b] = MF<T>::mma(ar[a], bi[b], ci[a][b]);
+          ci[a][b] = MF<T>::mma(ai[a], br[b], ci[a][b]);
+        }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int64_t col = n0 + wn * 32 + b * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm * 32 + a * 16 + 4 * (lane >> 4) + r;
+        if (row < M && col < N) {
+          const cplx<T> v = mk<T>(cr[a][b][r], ci[a][b][r]);
+          cplx<T> o = cmul<T>(alpha, v);
+          cplx<T>* pc = C + col * ldc + row;
+          if (!beta_zero) o = cadd<T>(o, cmul<T>(beta, *pc));
+          *pc = o;
+        }
+      }
+    }
+}
+
+template <class T>
+void gemm(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, MatView<T> B, double2 beta,
+          MatView<T> C) {
+  if (m <= 0 || n <= 0) return;
+  const bool bz = beta.x == 0 && beta.y == 0;
+  const cplx<T> al = mk<T>((T)alpha.x, (T)alpha.y), be = mk<T>((T)beta.x, (T)beta.y);
+  if (k <= 0) {
+    MNP_CHECK(bz, MNP_ERR_INTERNAL, "gemm: k == 0 with beta != 0 unsupported");
+  }
+  const int64_t gx = cdiv_up(m, GBM);
+  int64_t ny = cdiv_up(n, GBN);
+  for (int64_t y0 = 0; y0 < ny; y0 += 65535) {
+    const int64_t gy = std::min<int64_t>(65535, ny - y0);
+    const int64_t noff = y0 * GBN;
+    hipLaunchKernelGGL(k_cgemm<T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream(), m,
+                       n - noff, std::max<int64_t>(k, 0), al, A.p, A.ld, B.p + noff * B.ld, B.ld, be,
+                       (int)bz, C.p + noff * C.ld, C.ld);
+  }
+  MNP_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------- cooperative panel
+// LU with partial pivoting (pivot = first row of largest |Re|+|Im|, the
+// izamax rule of LAPACK getf2) of an m x nb panel.  Workgroup b keeps rows
+// [b*chunk, (b+1)*chunk) in LDS for the whole factorisation.
+template <class T>
+__global__ void __launch_bounds__(256) k_getf2_coop(int64_t m, int nb, cplx<T>* A, int64_t lda,
+                                                    int32_t* ipiv, int* info, int info_off,
+                                                    int64_t chunk, T* cval, int32_t* cidx,
+                                                    cplx<T>* crow, cplx<T>* rowj) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* P = (cplx<T>*)smem_raw;  // chunk x nb, row-major
+  cplx<T>* urow = P + chunk * nb;   // pivot row
+  __shared__ T rv[256];
+  __shared__ int64_t ri[256];
+  __shared__ int rw[256];
+  cg::grid_group grid = cg::this_grid();
+  const int tid = threadIdx.x;
+  const int64_t b = blockIdx.x, nblk = gridDim.x;
+  const int64_t r0 = b * chunk, r1 = min(m, r0 + chunk), nr = r1 > r0 ? r1 - r0 : 0;
+  for (int64_t idx = tid; idx < nr * nb; idx += 256) {
+    const int64_t r = idx % nr, c = idx / nr;
+    P[r * nb + c] = A[c * lda + r0 + r];
+  }
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    const int par = j & 1;
+    // local candidate
+    T best = T(-1);
+    int64_t bi = -1;
+    for (int64_t r = tid; r < nr; r += 256) {
+      if (r0 + r >= j) {
+        const T v = cabs1<T>(P[r * nb + j]);
+        if (v > best) {
+          best = v;
+          bi = r0 + r;
+        }
+      }
+    }
+    rv[tid] = best;
+    ri[tid] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) {
+        const T v2 = rv[tid + s];
+        const int64_t i2 = ri[tid + s];
+        if (i2 >= 0 && (v2 > rv[tid] || (v2 == rv[tid] && (ri[tid] < 0 || i2 < ri[tid])))) {
+          rv[tid] = v2;
+          ri[tid] = i2;
+        }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      cval[par * nblk + b] = rv[0];
+      cidx[par * nblk + b] = (int32_t)ri[0];
+    }
+    if (ri[0] >= 0)
+      for (int c = tid; c < nb; c += 256) crow[(par * nblk + b) * nb + c] = P[(ri[0] - r0) * nb + c];
+    if (j >= r0 && j < r1)
+      for (int c = tid; c < nb; c += 256) rowj[par * nb + c] = P[(j - r0) * nb + c];
+    __syncthreads();
+    grid.sync();
+    // global pivot: every workgroup reduces the candidates redundantly
+    best = T(-1);
+    bi = -1;
+    int bw = -1;
+    for (int64_t w = tid; w < nblk; w += 256) {
+      const T v = cval[par * nblk + w];
+      const int64_t id = cidx[par * nblk + w];
+      if (id >= 0 && (v > best || (v == best && (bi < 0 || id < bi)))) {
+        best = v;
+        bi = id;
+        bw = (int)w;
+      }
+    }
+    rv[tid] = best;
+    ri[tid] = bi;
+    rw[tid] = bw;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) {
+        const T v2 = rv[tid + s];
+        const int64_t i2 = ri[tid + s];
+        if (i2 >= 0 && (v2 > rv[tid] || (v2 == rv[tid] && (ri[tid] < 0 || i2 < ri[tid])))) {
+          rv[tid] = v2;
+          ri[tid] = i2;
+          rw[tid] = rw[tid + s];
+        }
+      }
+      __syncthreads();
+    }
+    const int64_t p = ri[0];
+    const int w = rw[0];
+    for (int c = tid; c < nb; c += 256) urow[c] = crow[(par * nblk + w) * nb + c];
+    __syncthreads();
+    if (j >= r0 && j < r1)
+      for (int c = tid; c < nb; c += 256) P[(j - r0) * nb + c] = urow[c];
+    if (p != j && p >= r0 && p < r1)
+      for (int c = tid; c < nb; c += 256) P[(p - r0) * nb + c] = rowj[par * nb + c];
+    if (b == 0 && tid == 0) {
+      ipiv[j] = (int32_t)p;
+      if (urow[j].x == 0 && urow[j].y == 0 && *info == 0) *info = info_off + j + 1;
+    }
+    __syncthreads();
+    const cplx<T> piv = urow[j];
+    if (piv.x != 0 || piv.y != 0) {
+      for (int64_t r = tid; r < nr; r += 256)
+        if (r0 + r > j) P[r * nb + j] = cdiv<T>(P[r * nb + j], piv);
+      __syncthreads();
+      const int nc = nb - j - 1;
+      for (int64_t idx = tid; idx < nr * nc; idx += 256) {
+        const int64_t r = idx % nr;
+        const int c = j + 1 + (int)(idx / nr);
+        if (r0 + r > j) P[r * nb + c] = csub<T>(P[r * nb + c], cmul<T>(P[r * nb + j], urow[c]));
+      }
+    }
+    __syncthreads();
+  }
+  for (int64_t idx = tid; idx < nr * nb; idx += 256) {
+    const int64_t r = idx % nr, c = idx / nr;
+    A[c * lda + r0 + r] = P[r * nb + c];
+  }
+}
+
+struct DevInfo {
+  int ncu = 0;
+  int lds_max = 0;
+};
+static DevInfo dev_info() {
+  int d;
+  MNP_HIP(hipGetDevice(&d));
+  DevInfo di;
+  MNP_HIP(hipDeviceGetAttribute(&di.ncu, hipDeviceAttributeMultiprocessorCount, d));
+  MNP_HIP(hipDeviceGetAttribute(&di.lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, d));
+  return di;
+}
+
+template <class T>
+static void getf2_leaf(int64_t m, int nb, MatView<T> A, int32_t* ipiv, int* info, int info_off,
+                       Workspace& ws) {
+  static DevInfo di = dev_info();
+  const size_t esz = sizeof(cplx<T>);
+  const int64_t lds_budget = 64 * 1024;
+  int64_t rows_max = (lds_budget - (int64_t)(nb * esz)) / (int64_t)(nb * esz);
+  MNP_CHECK(rows_max >= 1, MNP_ERR_INTERNAL, "getf2: panel too wide");
+  int64_t P = std::max<int64_t>(cdiv_up(m, rows_max), std::min<int64_t>(di.ncu, cdiv_up(m, 32)));
+  int64_t chunk = cdiv_up(m, P);
+  P = cdiv_up(m, chunk);
+  const size_t lds = (size_t)(chunk * nb + nb) * esz;
+  int occ = 0;
+  MNP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_getf2_coop<T>, 256, lds));
+  MNP_CHECK((int64_t)occ * di.ncu >= P, MNP_ERR_INTERNAL,
+            "getf2: panel of " + std::to_string(m) + " rows does not fit co-resident workgroups");
+  // scratch: cval[2P], cidx[2P], crow[2P*nb], rowj[2*nb]
+  const size_t off_idx = 2 * P * sizeof(T);
+  const size_t off_row = ((off_idx + 2 * P * sizeof(int32_t)) + 15) / 16 * 16;
+  const size_t off_rj = off_row + 2 * P * nb * esz;
+  char* base = (char*)ws.get(4, off_rj + 2 * nb * esz);
+  T* cval = (T*)base;
+  int32_t* cidx = (int32_t*)(base + off_idx);
+  cplx<T>* crow = (cplx<T>*)(base + off_row);
+  cplx<T>* rowj = (cplx<T>*)(base + off_rj);
+  cplx<T>* Ap = A.p;
+  int64_t lda = A.ld;
+  void* args[] = {&m, &nb, &Ap, &lda, &ipiv, &info, &info_off, &chunk, &cval, &cidx, &crow, &rowj};
+  MNP_HIP(hipLaunchCooperativeKernel((const void*)k_getf2_coop<T>, dim3((unsigned)P), dim3(256), args,
+                                     lds, stream()));
+}
+
+// ------------------------------------------------------------ permutations
+// Applies the swaps s <-> ipiv[s] (s in [k0, k1), rows in [rlo, rhi)) to an
+// identity permutation held in LDS by one thread, then lists the rows whose
+// source changed: dst[t] = r, src[t] = sigma(r).  Optionally writes sigma.
+__global__ void __launch_bounds__(256) k_perm(int64_t k0, int64_t k1, int64_t rlo, int64_t rhi,
+                                              const int32_t* __restrict__ ipiv, int32_t* dst,
+                                              int32_t* src, int32_t* count, int32_t* sigma) {
+  extern __shared__ int32_t perm[];
+  const int64_t len = rhi - rlo;
+  for (int64_t r = threadIdx.x; r < len; r += blockDim.x) perm[r] = (int32_t)(rlo + r);
+  if (threadIdx.x == 0) *count = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int64_t s = k0; s < k1; ++s) {
+      const int64_t p = ipiv[s];
+      const int32_t t = perm[s - rlo];
+      perm[s - rlo] = perm[p - rlo];
+      perm[p - rlo] = t;
+    }
+  }
+  __syncthreads();
+  for (int64_t r = threadIdx.x; r < len; r += blockDim.x) {
+    const int32_t v = perm[r];
+    if (sigma) sigma[r] = v;
+    if (v != rlo + r) {
+      const int t = atomicAdd(count, 1);
+      dst[t] = (int32_t)(rlo + r);
+      src[t] = v;
+    }
+  }
+}
+
+template <class T>
+__global__ void k_rows_gather(int64_t maxcnt, const int32_t* count, const int32_t* src,
+                              const cplx<T>* A, int64_t lda, int64_t col0, cplx<T>* tmp) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t c = blockIdx.y;
+  if (t >= *count) return;
+  tmp[c * maxcnt + t] = A[(col0 + c) * lda + src[t]];
+}
+template <class T>
+__global__ void k_rows_scatter(int64_t maxcnt, const int32_t* count, const int32_t* dst, cplx<T>* A,
+                               int64_t lda, int64_t col0, const cplx<T>* tmp) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t c = blockIdx.y;
+  if (t >= *count) return;
+  A[(col0 + c) * lda + dst[t]] = tmp[c * maxcnt + t];
+}
+
+static void launch_perm(int64_t k0, int64_t k1, int64_t rlo, int64_t rhi, const int32_t* ipiv,
+                        int32_t* dst, int32_t* src, int32_t* count, int32_t* sigma) {
+  const size_t lds = sizeof(int32_t) * (size_t)(rhi - rlo);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)k_perm, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    attr = true;
+  }
+  MNP_CHECK(lds <= 160 * 1024, MNP_ERR_ARG, "permutation longer than 40960 rows");
+  hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), lds, stream(), k0, k1, rlo, rhi, ipiv, dst, src,
+                     count, sigma);
+  MNP_HIP(hipGetLastError());
+}
+
+// Row interchanges s <-> ipiv[s], s in [k0, k1), on columns [c0, c1) of A,
+// all touched rows within [k0, m).
+template <class T>
+static void laswp(MatView<T> A, int64_t m, int64_t c0, int64_t c1, int64_t k0, int64_t k1,
+                  const int32_t* ipiv, Workspace& ws) {
+  if (c1 <= c0 || k1 <= k0) return;
+  const int64_t maxcnt = std::min<int64_t>(2 * (k1 - k0), m - k0);
+  int32_t* ib = (int32_t*)ws.get(0, sizeof(int32_t) * (size_t)(2 * maxcnt + 4));
+  int32_t* dst = ib;
+  int32_t* src = ib + maxcnt;
+  int32_t* cnt = ib + 2 * maxcnt;
+  launch_perm(k0, k1, k0, m, ipiv, dst, src, cnt, nullptr);
+  const int64_t ncols = c1 - c0;
+  const size_t budget = (size_t)256 << 20;
+  int64_t cchunk = std::max<int64_t>(1, (int64_t)(budget / (sizeof(cplx<T>) * (size_t)maxcnt)));
+  cchunk = std::min<int64_t>({cchunk, ncols, 65535});
+  cplx<T>* tmp = (cplx<T>*)ws.get(1, sizeof(cplx<T>) * (size_t)(maxcnt * cchunk));
+  for (int64_t c = c0; c < c1; c += cchunk) {
+    const int64_t nc = std::min(cchunk, c1 - c);
+    dim3 grid((unsigned)cdiv_up(maxcnt, 256), (unsigned)nc);
+    hipLaunchKernelGGL(k_rows_gather<T>, grid, dim3(256), 0, stream(), maxcnt, cnt, src, A.p, A.ld, c,
+                       tmp);
+    hipLaunchKernelGGL(k_rows_scatter<T>, grid, dim3(256), 0, stream(), maxcnt, cnt, dst, A.p, A.ld, c,
+                       tmp);
+  }
+  MNP_HIP(hipGetLastError());
+}
+
+void perm_from_pivots(int64_t n, const int32_t* ipiv, int32_t* sigma, Workspace& ws) {
+  int32_t* ib = (int32_t*)ws.get(0, sizeof(int32_t) * (size_t)(2 * n + 4));
+  launch_perm(0, n, 0, n, ipiv, ib, ib + n, ib + 2 * n, sigma);
+}
+
+__global__ void k_add_offset(int64_t n, int32_t* ipiv, int32_t off) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) ipiv[i] += off;
+}
+
+// ----------------------------------------------------- small helpers
+template <class T>
+__global__ void k_copy(int64_t m, int64_t n, const cplx<T>* A, int64_t lda, cplx<T>* B, int64_t ldb) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (i < m && j < n) B[j * ldb + i] = A[j * lda + i];
+}
+template <class Ts, class Td>
+__global__ void k_convert(int64_t m, int64_t n, const cplx<Ts>* A, int64_t lda, cplx<Td>* B,
+                          int64_t ldb) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (i < m && j < n) {
+    const cplx<Ts> v = A[j * lda + i];
+    B[j * ldb + i] = mk<Td>((Td)v.x, (Td)v.y);
+  }
+}
+static dim3 grid2(int64_t m, int64_t n) {
+  return dim3((unsigned)cdiv_up(m, 256), (unsigned)std::min<int64_t>(n, 65535),
+              (unsigned)cdiv_up(n, 65535));
+}
+template <class T>
+void copy_mat(int64_t m, int64_t n, MatView<T> A, MatView<T> B) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(k_copy<T>, grid2(m, n), dim3(256), 0, stream(), m, n, A.p, A.ld, B.p, B.ld);
+  MNP_HIP(hipGetLastError());
+}
+template <class Ts, class Td>
+void convert_mat(int64_t m, int64_t n, MatView<Ts> A, MatView<Td> B) {
+  if (m <= 0 || n <= 0) return;
+  hipLaunchKernelGGL((k_convert<Ts, Td>), grid2(m, n), dim3(256), 0, stream(), m, n, A.p, A.ld, B.p,
+                     B.ld);
+  MNP_HIP(hipGetLastError());
+}
+
+// Inverse of the nb x nb unit-lower (upper == 0) or upper (upper == 1)
+// triangle of LU (nb <= 64), written densely to X.  Thread j owns column j.
+template <class T>
+__global__ void __launch_bounds__(64) k_trtri(int nb, int upper, const cplx<T>* LU, int64_t ld,
+                                              cplx<T>* X, int64_t ldx) {
+  __shared__ cplx<T> sL[64][65];
+  __shared__ cplx<T> sX[64][65];
+  const int j = threadIdx.x;
+  for (int c = 0; c < nb; ++c)
+    if (j < nb) sL[j][c] = LU[c * ld + j];
+  __syncthreads();
+  if (j < nb) {
+    if (!upper) {
+      for (int i = 0; i < nb; ++i) {
+        cplx<T> s = i == j ? mk<T>(1, 0) : czero<T>();
+        if (i > j)
+          for (int k = j; k < i; ++k) s = csub<T>(s, cmul<T>(sL[i][k], sX[k][j]));
+        sX[i][j] = i < j ? czero<T>() : s;
+      }
+    } else {
+      for (int i = nb - 1; i >= 0; --i) {
+        if (i > j) {
+          sX[i][j] = czero<T>();
+          continue;
+        }
+        cplx<T> s = i == j ? mk<T>(1, 0) : czero<T>();
+        for (int k = i + 1; k <= j; ++k) s = csub<T>(s, cmul<T>(sL[i][k], sX[k][j]));
+        sX[i][j] = cdiv<T>(s, sL[i][i]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = 0; c < nb; ++c)
+    if (j < nb) X[c * ldx + j] = sX[j][c];
+}
+
+constexpr int TRSM_LEAF = 64;
+constexpr int LU_LEAF = 32;
+
+template <class T>
+static cplx<T>* trtri_block(int nb, int upper, MatView<T> LU, Workspace& ws) {
+  cplx<T>* X = (cplx<T>*)ws.get(3, sizeof(cplx<T>) * 64 * 64);
+  hipLaunchKernelGGL(k_trtri<T>, dim3(1), dim3(64), 0, stream(), nb, upper, LU.p, LU.ld, X, (int64_t)64);
+  MNP_HIP(hipGetLastError());
+  return X;
+}
+
+static const double2 ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
+
+static int64_t split(int64_t n, int64_t leaf) {
+  int64_t n1 = (n / 2) / leaf * leaf;
+  return n1 < leaf ? leaf : n1;
+}
+
+// B <- L^{-1} B (L unit lower n x n)
+template <class T>
+void trsm_llu(int64_t n, int64_t nrhs, MatView<T> LU, MatView<T> B, Workspace& ws) {
+  if (n <= 0 || nrhs <= 0) return;
+  if (n <= TRSM_LEAF) {
+    cplx<T>* X = trtri_block<T>((int)n, 0, LU, ws);
+    cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(n * nrhs));
+    copy_mat<T>(n, nrhs, B, MatView<T>{W, n});
+    gemm<T>(n, nrhs, n, ONE, MatView<T>{X, 64}, MatView<T>{W, n}, ZERO, B);
+    return;
+  }
+  const int64_t n1 = split(n, TRSM_LEAF), n2 = n - n1;
+  trsm_llu<T>(n1, nrhs, LU, B, ws);
+  gemm<T>(n2, nrhs, n1, MONE, MatView<T>{LU.at(n1, 0), LU.ld}, B, ONE, MatView<T>{B.at(n1, 0), B.ld});
+  trsm_llu<T>(n2, nrhs, MatView<T>{LU.at(n1, n1), LU.ld}, MatView<T>{B.at(n1, 0), B.ld}, ws);
+}
+
+// B <- U^{-1} B (U upper n x n)
+template <class T>
+void trsm_lun(int64_t n, int64_t nrhs, MatView<T> LU, MatView<T> B, Workspace& ws) {
+  if (n <= 0 || nrhs <= 0) return;
+  if (n <= TRSM_LEAF) {
+    cplx<T>* X = trtri_block<T>((int)n, 1, LU, ws);
+    cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(n * nrhs));
+    copy_mat<T>(n, nrhs, B, MatView<T>{W, n});
+    gemm<T>(n, nrhs, n, ONE, MatView<T>{X, 64}, MatView<T>{W, n}, ZERO, B);
+    return;
+  }
+  const int64_t n1 = split(n, TRSM_LEAF), n2 = n - n1;
+  trsm_lun<T>(n2, nrhs, MatView<T>{LU.at(n1, n1), LU.ld}, MatView<T>{B.at(n1, 0), B.ld}, ws);
+  gemm<T>(n1, nrhs, n2, MONE, MatView<T>{LU.at(0, n1), LU.ld}, MatView<T>{B.at(n1, 0), B.ld}, ONE, B);
+  trsm_lun<T>(n1, nrhs, LU, B, ws);
+}
+
+// B <- B U^{-1} (B m x n)
+template <class T>
+void trsm_run(int64_t m, int64_t n, MatView<T> LU, MatView<T> B, Workspace& ws) {
+  if (n <= 0 || m <= 0) return;
+  if (n <= TRSM_LEAF) {
+    cplx<T>* X = trtri_block<T>((int)n, 1, LU, ws);
+    cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(m * n));
+    copy_mat<T>(m, n, B, MatView<T>{W, m});
+    gemm<T>(m, n, n, ONE, MatView<T>{W, m}, MatView<T>{X, 64}, ZERO, B);
+    return;
+  }
+  const int64_t n1 = split(n, TRSM_LEAF), n2 = n - n1;
+  trsm_run<T>(m, n1, LU, B, ws);
+  gemm<T>(m, n2, n1, MONE, B, MatView<T>{LU.at(0, n1), LU.ld}, ONE, MatView<T>{B.at(0, n1), B.ld});
+  trsm_run<T>(m, n2, MatView<T>{LU.at(n1, n1), LU.ld}, MatView<T>{B.at(0, n1), B.ld}, ws);
+}
+
+// B <- B L^{-1} (B m x n, L unit lower)
+template <class T>
+void trsm_rlu(int64_t m, int64_t n, MatView<T> LU, MatView<T> B, Workspace& ws) {
+  if (n <= 0 || m <= 0) return;
+  if (n <= TRSM_LEAF) {
+    cplx<T>* X = trtri_block<T>((int)n, 0, LU, ws);
+    cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(m * n));
+    copy_mat<T>(m, n, B, MatView<T>{W, m});
+    gemm<T>(m, n, n, ONE, MatView<T>{W, m}, MatView<T>{X, 64}, ZERO, B);
+    return;
+  }
+  const int64_t n1 = split(n, TRSM_LEAF), n2 = n - n1;
+  trsm_rlu<T>(m, n2, MatView<T>{LU.at(n1, n1), LU.ld}, MatView<T>{B.at(0, n1), B.ld}, ws);
+  gemm<T>(m, n1, n2, MONE, MatView<T>{B.at(0, n1), B.ld}, MatView<T>{LU.at(n1, 0), LU.ld}, ONE, B);
+  trsm_rlu<T>(m, n1, LU, B, ws);
+}
+
+// ------------------------------------------------------------ recursive LU
+template <class T>
+static void getrf_rec(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, int* info, int info_off,
+                      Workspace& ws) {
+  if (n <= LU_LEAF) {
+    getf2_leaf<T>(m, (int)n, A, ipiv, info, info_off, ws);
+    return;
+  }
+  const int64_t n1 = split(n, LU_LEAF), n2 = n - n1;
+  getrf_rec<T>(m, n1, A, ipiv, info, info_off, ws);
+  laswp<T>(A, m, n1, n, 0, n1, ipiv, ws);
+  trsm_llu<T>(n1, n2, A, MatView<T>{A.at(0, n1), A.ld}, ws);
+  gemm<T>(m - n1, n2, n1, MONE, MatView<T>{A.at(n1, 0), A.ld}, MatView<T>{A.at(0, n1), A.ld}, ONE,
+          MatView<T>{A.at(n1, n1), A.ld});
+  getrf_rec<T>(m - n1, n2, MatView<T>{A.at(n1, n1), A.ld}, ipiv + n1, info, info_off + (int)n1, ws);
+  hipLaunchKernelGGL(k_add_offset, dim3((unsigned)cdiv_up(n2, 256)), dim3(256), 0, stream(), n2,
+                     ipiv + n1, (int32_t)n1);
+  laswp<T>(A, m, 0, n1, n1, n, ipiv, ws);
+}
+
+template <class T>
+int getrf(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, Workspace& ws) {
+  MNP_CHECK(m >= n && n > 0, MNP_ERR_ARG, "getrf: need m >= n > 0");
+  int* info = (int*)ws.get(5, sizeof(int));
+  MNP_HIP(hipMemsetAsync(info, 0, sizeof(int), stream()));
+  getrf_rec<T>(m, n, A, ipiv, info, 0, ws);
+  int h = 0;
+  MNP_HIP(hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  MNP_HIP(hipStreamSynchronize(stream()));
+  return h;
+}
+
+template <class T>
+__global__ void k_gather_rows(int64_t n, int64_t ncols, const int32_t* sigma, const cplx<T>* A,
+                              int64_t lda, cplx<T>* B, int64_t ldb) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (i < n && j < ncols) B[j * ldb + i] = A[j * lda + sigma[i]];
+}
+template <class T>
+void gather_rows(int64_t n, int64_t ncols, const int32_t* sigma, MatView<T> A, MatView<T> B) {
+  if (n <= 0 || ncols <= 0) return;
+  hipLaunchKernelGGL(k_gather_rows<T>, grid2(n, ncols), dim3(256), 0, stream(), n, ncols, sigma, A.p,
+                     A.ld, B.p, B.ld);
+  MNP_HIP(hipGetLastError());
+}
+
+// X[:, sigma[r]] = W[:, r]
+template <class T>
+__global__ void k_scatter_cols(int64_t m, int64_t n, const int32_t* sigma, const cplx<T>* W, int64_t ldw,
+                               cplx<T>* X, int64_t ldx) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t r = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (i < m && r < n) X[(int64_t)sigma[r] * ldx + i] = W[r * ldw + i];
+}
+
+template <class T>
+void getrs(int64_t n, int64_t nrhs, MatView<T> LU, const int32_t* sigma, MatView<T> B, Workspace& ws) {
+  cplx<T>* W = (cplx<T>*)ws.get(1, sizeof(cplx<T>) * (size_t)(n * nrhs));
+  MatView<T> Wv{W, n};
+  gather_rows<T>(n, nrhs, sigma, B, Wv);
+  trsm_llu<T>(n, nrhs, LU, Wv, ws);
+  trsm_lun<T>(n, nrhs, LU, Wv, ws);
+  copy_mat<T>(n, nrhs, Wv, B);
+}
+
+template <class T>
+void getrs_right(int64_t m, int64_t n, MatView<T> LU, const int32_t* sigma, MatView<T> B,
+                 Workspace& ws) {
+  trsm_run<T>(m, n, LU, B, ws);
+  trsm_rlu<T>(m, n, LU, B, ws);
+  cplx<T>* W = (cplx<T>*)ws.get(1, sizeof(cplx<T>) * (size_t)(m * n));
+  copy_mat<T>(m, n, B, MatView<T>{W, m});
+  hipLaunchKernelGGL(k_scatter_cols<T>, grid2(m, n), dim3(256), 0, stream(), m, n, sigma, W, m, B.p,
+                     B.ld);
+  MNP_HIP(hipGetLastError());
+}
+
+template <class T>
+__global__ void k_perm_identity(int64_t n, const int32_t* sigma, cplx<T>* X, int64_t ldx) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (i < n && j < n) X[j * ldx + i] = sigma[i] == j ? mk<T>(1, 0) : czero<T>();
+}
+
+template <class T>
+void getri(int64_t n, MatView<T> LU, const int32_t* sigma, MatView<T> X, Workspace& ws) {
+  hipLaunchKernelGGL(k_perm_identity<T>, grid2(n, n), dim3(256), 0, stream(), n, sigma, X.p, X.ld);
+  MNP_HIP(hipGetLastError());
+  trsm_llu<T>(n, n, LU, X, ws);
+  trsm_lun<T>(n, n, LU, X, ws);
+}
+
+// ---------------------------------------------------- instantiations
+#define MNP_INST(T)                                                                              \
+  template void gemm<T>(int64_t, int64_t, int64_t, double2, MatView<T>, MatView<T>, double2,     \
+                        MatView<T>);                                                             \
+  template int getrf<T>(int64_t, int64_t, MatView<T>, int32_t*, Workspace&);                     \
+  template void gather_rows<T>(int64_t, int64_t, const int32_t*, MatView<T>, MatView<T>);        \
+  template void trsm_llu<T>(int64_t, int64_t, MatView<T>, MatView<T>, Workspace&);               \
+  template void trsm_lun<T>(int64_t, int64_t, MatView<T>, MatView<T>, Workspace&);               \
+  template void trsm_run<T>(int64_t, int64_t, MatView<T>, MatView<T>, Workspace&);               \
+  template void trsm_rlu<T>(int64_t, int64_t, MatView<T>, MatView<T>, Workspace&);               \
+  template void getrs<T>(int64_t, int64_t, MatView<T>, const int32_t*, MatView<T>, Workspace&);  \
+  template void getrs_right<T>(int64_t, int64_t, MatView<T>, const int32_t*, MatView<T>,         \
+                               Workspace&);                                                      \
+  template void getri<T>(int64_t, MatView<T>, const int32_t*, MatView<T>, Workspace&);           \
+  template void copy_mat<T>(int64_t, int64_t, MatView<T>, MatView<T>);
+MNP_INST(double)
+MNP_INST(float)
+template void convert_mat<double, float>(int64_t, int64_t, MatView<double>, MatView<float>);
+template void convert_mat<float, double>(int64_t, int64_t, MatView<float>, MatView<double>);
+
+}  // namespace mnp
+
+// ============================================================== C ABI
+using namespace mnp;
+
+namespace {
+Workspace& abi_ws() {
+  static Workspace ws;
+  return ws;
+}
+template <class F>
+int by_precision(int precision, F&& f) {
+  return guarded([&] {
+    MNP_CHECK(precision == MNP_FP64 || precision == MNP_FP32, MNP_ERR_ARG, "unknown precision");
+    if (precision == MNP_FP64)
+      f(double());
+    else
+      f(float());
+  });
+}
+}  // namespace
+
+extern "C" {
+
+int mnp_gemm(int precision, int64_t m, int64_t n, int64_t k, mnp_complex alpha, const void* A,
+             int64_t lda, const void* B, int64_t ldb, mnp_complex beta, void* C, int64_t ldc) {
+  return by_precision(precision, [&](auto t) {
+    using T = decltype(t);
+    MNP_CHECK(m >= 0 && n >= 0 && k >= 0 && lda >= std::max<int64_t>(1, m) &&
+                  ldb >= std::max<int64_t>(1, k) && ldc >= std::max<int64_t>(1, m),
+              MNP_ERR_ARG, "gemm: bad sizes");
+    gemm<T>(m, n, k, to_d2(alpha), MatView<T>{(cplx<T>*)A, lda}, MatView<T>{(cplx<T>*)B, ldb},
+            to_d2(beta), MatView<T>{(cplx<T>*)C, ldc});
+  });
+}
+
+int mnp_getrf(int precision, int64_t m, int64_t n, void* A, int64_t lda, int32_t* ipiv, int* info) {
+  return by_precision(precision, [&](auto t) {
+    using T = decltype(t);
+    MNP_CHECK(A && ipiv && m >= n && n > 0 && lda >= m, MNP_ERR_ARG, "getrf: bad arguments");
+    const int r = getrf<T>(m, n, MatView<T>{(cplx<T>*)A, lda}, ipiv, abi_ws());
+    if (info) *info = r;
+  });
+}
+
+int mnp_getrs(int precision, int64_t n, int64_t nrhs, const void* LU, int64_t lda, const int32_t* ipiv,
+              void* B, int64_t ldb) {
+  return by_precision(precision, [&](auto t) {
+    using T = decltype(t);
+    MNP_CHECK(LU && ipiv && B && n > 0 && nrhs >= 0 && lda >= n && ldb >= n, MNP_ERR_ARG,
+              "getrs: bad arguments");
+    int32_t* sigma = (int32_t*)abi_ws().get(5, sizeof(int32_t) * n);
+    perm_from_pivots(n, ipiv, sigma, abi_ws());
+    getrs<T>(n, nrhs, MatView<T>{(cplx<T>*)LU, lda}, sigma, MatView<T>{(cplx<T>*)B, ldb}, abi_ws());
+  });
+}
+
+int mnp_getrs_right(int precision, int64_t m, int64_t n, const void* LU, int64_t lda,
+                    const int32_t* ipiv, void* B, int64_t ldb) {
+  return by_precision(precision, [&](auto t) {
+    using T = decltype(t);
+    MNP_CHECK(LU && ipiv && B && n > 0 && m >= 0 && lda >= n && ldb >= m, MNP_ERR_ARG,
+              "getrs_right: bad arguments");
+    int32_t* sigma = (int32_t*)abi_ws().get(5, sizeof(int32_t) * n);
+    perm_from_pivots(n, ipiv, sigma, abi_ws());
+    getrs_right<T>(m, n, MatView<T>{(cplx<T>*)LU, lda}, sigma, MatView<T>{(cplx<T>*)B, ldb}, abi_ws());
+  });
+}
+
+int mnp_getri(int precision, int64_t n, const void* LU, int64_t lda, const int32_t* ipiv, void* X,
+              int64_t ldx) {
+  return by_precision(precision, [&](auto t) {
+    using T = decltype(t);
+    MNP_CHECK(LU && ipiv && X && n > 0 && lda >= n && ldx >= n, MNP_ERR_ARG, "getri: bad arguments");
+    int32_t* sigma = (int32_t*)abi_ws().get(5, sizeof(int32_t) * n);
+    perm_from_pivots(n, ipiv, sigma, abi_ws());
+    getri<T>(n, MatView<T>{(cplx<T>*)LU, lda}, sigma, MatView<T>{(cplx<T>*)X, ldx}, abi_ws());
+  });
+}
+
+}  // extern "C"

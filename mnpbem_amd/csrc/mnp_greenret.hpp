@@ -72,3 +72,8 @@ struct GreenRet {
 };
 
 }  // namespace mnp
+
+// Opaque handle behind mnp_greenret_t.
+struct mnp_greenret_s {
+  mnp::GreenRet* g;
+};

@@ -1,0 +1,109 @@
+"""GPU parity of the dense retarded solver (FP64 drop-in and FP32 + FP64
+refinement) and of the plane-wave observables against the oracle and Mie."""
+import numpy as np
+import pytest
+
+from mnpbem_amd import shapes
+from oracle import mnpbem_oracle as O
+from oracle.mie import mie_cross_sections
+
+pytestmark = pytest.mark.gpu
+
+DRUDE = (10.0, 9.065, 0.0708)
+
+
+def _setup(verts, faces, eps_out, precision):
+    from mnpbem_amd import bem as B
+    op = B.bemoptions(sim="ret")
+    epstab = [B.epsconst(eps_out), B.epsdrude(*DRUDE)]
+    p = B.comparticle(epstab, [(verts, faces)], [2, 1], 1, op)
+    return p, B.bemsolver(p, op, precision=precision)
+
+
+def _oracle(verts, faces, enei, eps_out, pol, dirs, pinfty):
+    g = O.GreenRet(verts, faces)
+    e1 = O.eps_drude(enei, *DRUDE)
+    bem = O.BemRet(g).initmat(enei, e1, eps_out)
+    a2, a2p = O.planewave_potential(g.pos, g.nvec, pol, dirs, enei, eps_out)
+    sig = bem.solve({"a2": a2, "a2p": a2p})
+    ext = O.extinction(g.pos, g.area, sig, pol, dirs, enei, eps_out)
+    sca = O.scattering(g.pos, g.area, sig, enei, eps_out, *pinfty)
+    return sig, ext, sca, (a2, a2p)
+
+
+CASES = {
+    "sphere_l2": (lambda: shapes.icosphere(2, 10.0), 1.0, 520.0,
+                  [[1.0, 0, 0]], [[0, 0, 1.0]]),
+    "ellipsoid_l2": (lambda: shapes.ellipsoid(2, (10.0, 10.0, 40.0)), 1.77, 700.0,
+                     [[1.0, 0, 0], [0, 0, 1.0]], [[0, 0, 1.0], [1.0, 0, 0]]),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solver_matches_oracle(case, precision, pinfty):
+    from mnpbem_amd import bem as B, device as dev
+    mesh, eps_out, enei, pol, dirs = CASES[case]
+    print(f"[bemret parity] {case} {precision}", flush=True)
+    v, f = mesh()
+    pol, dirs = np.array(pol), np.array(dirs)
+    sig_ref, ext_ref, sca_ref, (a2_ref, a2p_ref) = _oracle(v, f, enei, eps_out, pol, dirs, pinfty)
+    p, bem = _setup(v, f, eps_out, precision)
+    exc = B.planewave(pol, dirs)
+    pot = exc(p, enei)
+    assert np.allclose(dev.to_host(pot.a2), a2_ref, rtol=1e-13, atol=1e-15)
+    assert np.allclose(dev.to_host(pot.a2p), a2p_ref, rtol=1e-13, atol=1e-15)
+    sig = bem.solve(pot)
+    ext, sca, ab = exc.cross_sections(sig)
+    if precision == "fp64":
+        for key in ("sig1", "sig2", "h1", "h2"):
+            got, want = dev.to_host(sig[key]), sig_ref[key]
+            assert np.max(np.abs(got - want)) < 1e-9 * np.abs(want).max(), key
+        tol = 1e-10
+    else:
+        it, rr = bem.stats()
+        print(f"  refinement sweeps {it}, relres {rr:.2e}", flush=True)
+        assert rr < 1e-12 and it <= 4
+        tol = 1e-8
+    assert np.allclose(ext, ext_ref, rtol=tol)
+    assert np.allclose(sca, sca_ref, rtol=tol)
+    assert np.allclose(ab, ext_ref - sca_ref, rtol=1e3 * tol)
+
+
+def test_config1_sphere_against_mie():
+    """Config-1 sphere (1,280 faces, 20 nm): BEM spectrum vs Mie within the
+    discretisation error, FP64 and mixed agree to 1e-8."""
+    from mnpbem_amd import bem as B
+    v, f = shapes.icosphere(3, 10.0)
+    p64, bem64 = _setup(v, f, 1.0, "fp64")
+    _, bemmx = _setup(v, f, 1.0, "mixed")
+    exc = B.planewave([[1.0, 0, 0]], [[0, 0, 1.0]])
+    enei = np.linspace(400, 800, 5)
+    drude = B.epsdrude(*DRUDE)
+    for lam in enei:
+        print(f"[config1] enei {lam}", flush=True)
+        e64 = exc.cross_sections(bem64.solve(exc(p64, lam)))
+        emx = exc.cross_sections(bemmx.solve(exc(p64, lam)))
+        me, ms, _ = mie_cross_sections(lam, drude(lam)[0], 1.0, 20.0)
+        assert np.allclose(emx[0], e64[0], rtol=1e-8) and np.allclose(emx[1], e64[1], rtol=1e-8)
+        if lam < 700:
+            assert abs(e64[0][0] / me[0] - 1) < 0.03
+            assert abs(e64[1][0] / ms[0] - 1) < 0.06
+
+
+def test_solve_before_initmat_and_bad_polarisation():
+    from mnpbem_amd import _lib, bem as B
+    v, f = shapes.icosphere(1, 5.0)
+    p, bem = _setup(v, f, 1.0, "fp64")
+    with pytest.raises(ValueError):
+        B.planewave([[1.0, 0, 0]], [[1.0, 0, 0]])
+    import ctypes as C
+    import torch
+    n = p.n
+    z = torch.zeros((1, 3, n), dtype=torch.complex128, device="cuda")
+    s = torch.zeros((1, n), dtype=torch.complex128, device="cuda")
+    st = _lib.lib().mnp_bemret_solve(bem.handle, 1, None, None, None, None, None, None,
+                                     C.c_void_p(z.data_ptr()), C.c_void_p(z.data_ptr()),
+                                     C.c_void_p(s.data_ptr()), C.c_void_p(s.data_ptr()),
+                                     C.c_void_p(z.data_ptr()), C.c_void_p(z.data_ptr()))
+    assert st == 6  # MNP_ERR_STATE
