@@ -63,6 +63,28 @@ int mnp_set_stream(void* stream);
 void* mnp_get_stream(void);
 int mnp_synchronize(void);
 
+/* Kernel timers, the counterpart of the tic/toc timer map of
+ * mex/hlib/hoptions.h:65-72.  When enabled, every launch of the listed
+ * kernel families is bracketed by HIP events on the library stream;
+ * mnp_timers_read synchronises and sums the elapsed times.  work is the
+ * algorithmic work of the family: real flops (gemm: 8 m n k per complex
+ * product) or bytes written (assembly). */
+enum {
+  MNP_TIMER_GEMM_FP64 = 0, /* complex GEMM, complex128 operands                 */
+  MNP_TIMER_GEMM_FP32 = 1, /* complex GEMM, complex64 operands                  */
+  MNP_TIMER_PANEL = 2,     /* cooperative LU panel                              */
+  MNP_TIMER_ASSEMBLE = 3,  /* dense G/H assembly                                */
+  MNP_TIMER_COUNT = 4
+};
+typedef struct {
+  int64_t launches[MNP_TIMER_COUNT];
+  double ms[MNP_TIMER_COUNT];
+  double work[MNP_TIMER_COUNT];
+} mnp_timers;
+/* on != 0 enables and resets the timers; 0 disables them. */
+int mnp_timers_enable(int on);
+int mnp_timers_read(mnp_timers* out);
+
 /* ------------------------------------------------ Green function (greenret)
  * Replaces compgreenret(p, p, op) for one closed particle of flat triangles:
  *   Greenfun/@compgreenret/private/init.m:4-31 -> @greenret/private/init.m

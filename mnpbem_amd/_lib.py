@@ -29,6 +29,10 @@ class GreenOptions(C.Structure):
                 ("npol_rho", C.c_int), ("npol_phi", C.c_int), ("closed", C.c_int)]
 
 
+class Timers(C.Structure):
+    _fields_ = [("launches", C.c_int64 * 4), ("ms", C.c_double * 4), ("work", C.c_double * 4)]
+
+
 P, I64, D, INT = C.c_void_p, C.c_int64, C.c_double, C.c_int
 PD = C.POINTER(C.c_double)
 PI64 = C.POINTER(C.c_int64)
@@ -41,6 +45,8 @@ PROTOTYPES = {
     "mnp_set_stream": (INT, [P]),
     "mnp_get_stream": (P, []),
     "mnp_synchronize": (INT, []),
+    "mnp_timers_enable": (INT, [INT]),
+    "mnp_timers_read": (INT, [C.POINTER(Timers)]),
     "mnp_green_options_default": (None, [C.POINTER(GreenOptions)]),
     "mnp_greenret_create": (INT, [P, I64, P, I64, C.POINTER(GreenOptions), C.POINTER(P)]),
     "mnp_greenret_destroy": (INT, [P]),

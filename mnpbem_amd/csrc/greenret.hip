@@ -22,6 +22,53 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 static hipStream_t g_stream = nullptr;
 hipStream_t stream() { return g_stream; }
 
+// ------------------------------------------------------------------ timers
+namespace {
+struct TimerRec {
+  int kind;
+  double work;
+  hipEvent_t a, b;
+};
+bool g_timers_on = false;
+std::vector<TimerRec> g_timer_recs;
+mnp_timers g_timer_acc{};
+void timers_flush() {
+  if (g_timer_recs.empty()) return;
+  MNP_HIP(hipStreamSynchronize(g_stream));
+  for (auto& r : g_timer_recs) {
+    float ms = 0;
+    MNP_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    g_timer_acc.launches[r.kind] += 1;
+    g_timer_acc.ms[r.kind] += ms;
+    g_timer_acc.work[r.kind] += r.work;
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  g_timer_recs.clear();
+}
+}  // namespace
+
+TimerScope::TimerScope(int kind_, double work_) : kind(kind_), work(work_) {
+  if (!g_timers_on) return;
+  hipEvent_t a;
+  MNP_HIP(hipEventCreate(&a));
+  MNP_HIP(hipEventRecord(a, g_stream));
+  ev0 = a;
+}
+TimerScope::~TimerScope() {
+  if (!ev0) return;
+  hipEvent_t b;
+  if (hipEventCreate(&b) != hipSuccess) return;
+  (void)hipEventRecord(b, g_stream);
+  g_timer_recs.push_back(TimerRec{kind, work, (hipEvent_t)ev0, b});
+  if (g_timer_recs.size() > 4096) {
+    try {
+      timers_flush();
+    } catch (...) {
+    }
+  }
+}
+
 // ------------------------------------------------------------- quadrature
 // Quadrature tables live in constant memory; the 28-point rule is the
 // Lyness-Jespersen rule 18 (Misc/integration/@quadface/private/
@@ -112,6 +159,7 @@ static int polar_rule(int nr, int np, double* x, double* y, double* w) {
 // 0.5|(v2-v1)x(v3-v1)| (quad_flat.m:21-24).
 __global__ void k_geometry(int64_t n, const double* __restrict__ verts, int64_t nv,
                            const int64_t* __restrict__ faces, Geo geo) {
+#pragma clang fp contract(off)
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   double v[3][3];
@@ -256,6 +304,7 @@ __global__ void __launch_bounds__(256) k_refine_fill(int64_t n, Geo geo, double 
 // (greenret/private/init.m:77).
 __global__ void k_self_rmax(int64_t nnz, Geo geo, int64_t n, Pairs pr, int npol,
                             unsigned long long* rmax_bits) {
+#pragma clang fp contract(off)
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   double rm = 0;
   if (p < nnz && pr.kind[p] == 2) {
@@ -288,6 +337,11 @@ __global__ void k_self_rmax(int64_t nnz, Geo geo, int64_t n, Pairs pr, int npol,
 __global__ void __launch_bounds__(256) k_tables(int64_t nnz, Geo geo, int64_t n, Pairs pr,
                                                 int order, int npol, const unsigned long long* rmax_bits,
                                                 double* __restrict__ gt, double* __restrict__ ft) {
+  // Without FMA contraction the quadrature points and distances round as in
+  // the reference's vectorised expressions (x v1 + y v2 + (1-x-y) v3 and
+  // |pos - pt|); for the innermost polar ring, 5e-7 face radii from the
+  // centroid, that subtraction cancels about nine digits.
+#pragma clang fp contract(off)
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= nnz) return;
   const int64_t i = pr.row[p], j = pr.col[p];
@@ -502,6 +556,9 @@ __global__ void __launch_bounds__(256) k_assemble_refined(int64_t nnz, int64_t n
 void GreenRet::assemble(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t ld) const {
   if (!g.any() && !h.any()) return;
   dim3 grid((unsigned)cdiv_up(n, 256), (unsigned)n);
+  const double bytes = (double)n * n *
+                       ((g.d ? 16 : 0) + (g.f ? 8 : 0) + (h.d ? 16 : 0) + (h.f ? 8 : 0));
+  TimerScope ts(MNP_TIMER_ASSEMBLE, bytes);
   hipLaunchKernelGGL(k_assemble<OutPair>, grid, dim3(256), 0, stream(), n, geo, k, sgn2pi, g, h, ld);
   MNP_HIP(hipGetLastError());
   if (nnz) {
@@ -619,6 +676,20 @@ int mnp_set_stream(void* s) {
   return guarded([&] { mnp::g_stream = (hipStream_t)s; });
 }
 void* mnp_get_stream(void) { return (void*)mnp::g_stream; }
+int mnp_timers_enable(int on) {
+  return guarded([&] {
+    mnp::timers_flush();
+    mnp::g_timers_on = on != 0;
+    mnp::g_timer_acc = mnp_timers{};
+  });
+}
+int mnp_timers_read(mnp_timers* out) {
+  return guarded([&] {
+    MNP_CHECK(out, MNP_ERR_ARG, "timers_read: NULL argument");
+    mnp::timers_flush();
+    *out = mnp::g_timer_acc;
+  });
+}
 int mnp_synchronize(void) {
   return guarded([&] { MNP_HIP(hipStreamSynchronize(mnp::g_stream)); });
 }

@@ -20,113 +20,143 @@ namespace mnp {
 
 // ------------------------------------------------------------ MFMA traits
 template <class T> struct MF;
+// Both 16x16x4 variants take A[l % 16][l / 16] and B[l / 16][l % 16] from
+// lane l.  Their accumulators differ: result r of lane l is
+// D[l / 16 + 4 r][l % 16] for f64 and D[4 (l / 16) + r][l % 16] for f32.
 template <> struct MF<double> {
   typedef double acc_t __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <> struct MF<float> {
   typedef float acc_t __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 template <class T> __device__ __forceinline__ cplx<T> czero() { return mk<T>(T(0), T(0)); }
 
 // ------------------------------------------------------------------ GEMM
-constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 1;
+// C = alpha A B + beta C on MFMA (operand layouts: MF above).  A complex
+// product is four real MFMA products (re re, -im im -> re; re im, im re -> im).
+//
+// Tile: 2 x 2 waves, each wave WM x WN blocks of 16 x 16, so the workgroup
+// computes (32 WM) x (32 WN); k advances in slices of 16.  The LDS holds two
+// slices: while the MFMAs consume one, the next slice (prefetched into
+// registers during the previous step) is stored into the other, so each
+// step needs a single barrier.
+constexpr int GBK = 16;
 
-// C = alpha A B + beta C.  Operand layout of the 16x16x4 MFMA: lane l
-// supplies A[l % 16][l / 16] and B[l / 16][l % 16]; it holds the four
-// results D[4 (l / 16) + r][l % 16], r = 0..3.
-template <class T>
+template <class T, int WM, int WN>
 __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, cplx<T> alpha,
                                                const cplx<T>* __restrict__ A, int64_t lda,
                                                const cplx<T>* __restrict__ B, int64_t ldb,
                                                cplx<T> beta, int beta_zero, cplx<T>* C,
                                                int64_t ldc) {
   using acc_t = typename MF<T>::acc_t;
-  __shared__ T sAr[GBK][GBM + GPAD], sAi[GBK][GBM + GPAD];
-  __shared__ T sBr[GBK][GBN + GPAD], sBi[GBK][GBN + GPAD];
+  constexpr int BM = 32 * WM, BN = 32 * WN, PAD = 1;
+  constexpr int LA = BM * GBK / 256, LB = BN * GBK / 256;  // elements per thread
+  __shared__ T sAr[2][GBK][BM + PAD], sAi[2][GBK][BM + PAD];
+  __shared__ T sBr[2][GBK][BN + PAD], sBi[2][GBK][BN + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wm = wid & 1, wn = wid >> 1;
-  const int64_t m0 = (int64_t)blockIdx.x * GBM, n0 = (int64_t)blockIdx.y * GBN;
+  const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
 
-  acc_t cr[2][2], ci[2][2];
+  acc_t cr[WM][WN], ci[WM][WN];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < WM; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
+    for (int b = 0; b < WN; ++b) {
       cr[a][b] = acc_t{0, 0, 0, 0};
       ci[a][b] = acc_t{0, 0, 0, 0};
     }
-  cplx<T> ra[4], rb[4];
+  cplx<T> ra[LA], rb[LB];
   auto loadA = [&](int64_t k0) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int idx = tid + 256 * e, m = idx & 63, k = idx >> 6;
+    for (int e = 0; e < LA; ++e) {
+      const int idx = tid + 256 * e, m = idx % BM, k = idx / BM;
       const int64_t gm = m0 + m, gk = k0 + k;
       ra[e] = (gm < M && gk < K) ? A[gk * lda + gm] : czero<T>();
     }
   };
   auto loadB = [&](int64_t k0) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int idx = tid + 256 * e, k = idx & 15, n = idx >> 4;
+    for (int e = 0; e < LB; ++e) {
+      const int idx = tid + 256 * e, k = idx % GBK, n = idx / GBK;
       const int64_t gk = k0 + k, gn = n0 + n;
       rb[e] = (gk < K && gn < N) ? B[gn * ldb + gk] : czero<T>();
     }
   };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < LA; ++e) {
+      const int idx = tid + 256 * e;
+      sAr[buf][idx / BM][idx % BM] = ra[e].x;
+      sAi[buf][idx / BM][idx % BM] = ra[e].y;
+    }
+#pragma unroll
+    for (int e = 0; e < LB; ++e) {
+      const int idx = tid + 256 * e;
+      sBr[buf][idx % GBK][idx / GBK] = rb[e].x;
+      sBi[buf][idx % GBK][idx / GBK] = rb[e].y;
+    }
+  };
   loadA(0);
   loadB(0);
+  store(0);
+  int buf = 0;
   for (int64_t k0 = 0; k0 < K; k0 += GBK) {
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int idx = tid + 256 * e;
-      sAr[idx >> 6][idx & 63] = ra[e].x;
-      sAi[idx >> 6][idx & 63] = ra[e].y;
-      sBr[idx & 15][idx >> 4] = rb[e].x;
-      sBi[idx & 15][idx >> 4] = rb[e].y;
-    }
-    __syncthreads();
-    if (k0 + GBK < K) {
+    __syncthreads();  // slice `buf` is stored; slice `buf ^ 1` is no longer read
+    const bool more = k0 + GBK < K;
+    if (more) {
       loadA(k0 + GBK);
       loadB(k0 + GBK);
     }
 #pragma unroll
     for (int kk = 0; kk < GBK / 4; ++kk) {
       const int kr = kk * 4 + (lane >> 4), cl = lane & 15;
-      T ar[2], ai[2], nai[2], br[2], bi[2];
+      T ar[WM], ai[WM], nai[WM], br[WN], bi[WN];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        ar[a] = sAr[kr][wm * 32 + a * 16 + cl];
-        ai[a] = sAi[kr][wm * 32 + a * 16 + cl];
+      for (int a = 0; a < WM; ++a) {
+        ar[a] = sAr[buf][kr][wm * 16 * WM + a * 16 + cl];
+        ai[a] = sAi[buf][kr][wm * 16 * WM + a * 16 + cl];
         nai[a] = -ai[a];
-        br[a] = sBr[kr][wn * 32 + a * 16 + cl];
-        bi[a] = sBi[kr][wn * 32 + a * 16 + cl];
       }
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < WN; ++b) {
+        br[b] = sBr[buf][kr][wn * 16 * WN + b * 16 + cl];
+        bi[b] = sBi[buf][kr][wn * 16 * WN + b * 16 + cl];
+      }
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+      for (int a = 0; a < WM; ++a)
+#pragma unroll
+        for (int b = 0; b < WN; ++b) {
           cr[a][b] = MF<T>::mma(ar[a], br[b], cr[a][b]);
-          cr[a][b] = MF<T>::mma(nai[a], bi[b], cr[a][b]);
           ci[a][b] = MF<T>::mma(ar[a], bi[b], ci[a][b]);
+        }
+#pragma unroll
+      for (int a = 0; a < WM; ++a)
+#pragma unroll
+        for (int b = 0; b < WN; ++b) {
+          cr[a][b] = MF<T>::mma(nai[a], bi[b], cr[a][b]);
           ci[a][b] = MF<T>::mma(ai[a], br[b], ci[a][b]);
         }
     }
+    if (more) store(buf ^ 1);
+    buf ^= 1;
   }
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < WM; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int64_t col = n0 + wn * 32 + b * 16 + (lane & 15);
+    for (int b = 0; b < WN; ++b) {
+      const int64_t col = n0 + wn * 16 * WN + b * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t row = m0 + wm * 32 + a * 16 + 4 * (lane >> 4) + r;
+        const int64_t row = m0 + wm * 16 * WM + a * 16 + MF<T>::row(lane, r);
         if (row < M && col < N) {
           const cplx<T> v = mk<T>(cr[a][b][r], ci[a][b][r]);
           cplx<T> o = cmul<T>(alpha, v);
@@ -138,6 +168,25 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
     }
 }
 
+template <class T, int WM, int WN>
+static void gemm_launch(int64_t m, int64_t n, int64_t k, cplx<T> al, MatView<T> A, MatView<T> B,
+                        cplx<T> be, bool bz, MatView<T> C) {
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  const int64_t gx = cdiv_up(m, BM);
+  const int64_t ny = cdiv_up(n, BN);
+  for (int64_t y0 = 0; y0 < ny; y0 += 65535) {
+    const int64_t gy = std::min<int64_t>(65535, ny - y0);
+    const int64_t noff = y0 * BN;
+    const int64_t ncols = std::min<int64_t>(n - noff, gy * BN);
+    TimerScope ts(sizeof(T) == 8 ? MNP_TIMER_GEMM_FP64 : MNP_TIMER_GEMM_FP32,
+                  8.0 * (double)m * (double)ncols * (double)k);
+    hipLaunchKernelGGL((k_cgemm<T, WM, WN>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream(), m,
+                       n - noff, k, al, A.p, A.ld, B.p + noff * B.ld, B.ld, be, (int)bz,
+                       C.p + noff * C.ld, C.ld);
+  }
+  MNP_HIP(hipGetLastError());
+}
+
 template <class T>
 void gemm(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, MatView<T> B, double2 beta,
           MatView<T> C) {
@@ -146,17 +195,17 @@ void gemm(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, MatView<
   const cplx<T> al = mk<T>((T)alpha.x, (T)alpha.y), be = mk<T>((T)beta.x, (T)beta.y);
   if (k <= 0) {
     MNP_CHECK(bz, MNP_ERR_INTERNAL, "gemm: k == 0 with beta != 0 unsupported");
+    k = 0;
   }
-  const int64_t gx = cdiv_up(m, GBM);
-  int64_t ny = cdiv_up(n, GBN);
-  for (int64_t y0 = 0; y0 < ny; y0 += 65535) {
-    const int64_t gy = std::min<int64_t>(65535, ny - y0);
-    const int64_t noff = y0 * GBN;
-    hipLaunchKernelGGL(k_cgemm<T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream(), m,
-                       n - noff, std::max<int64_t>(k, 0), al, A.p, A.ld, B.p + noff * B.ld, B.ld, be,
-                       (int)bz, C.p + noff * C.ld, C.ld);
-  }
-  MNP_HIP(hipGetLastError());
+  // Large products use the 128 x 128 tile; thin ones keep enough workgroups.
+  const int64_t big_tiles = cdiv_up(m, 128) * cdiv_up(n, 128);
+  if (m >= 128 && n >= 128 && k >= 64 && big_tiles >= 256) {
+    if constexpr (sizeof(T) == 8)
+      gemm_launch<T, 4, 2>(m, n, k, al, A, B, be, bz, C);  // 128 x 64: accumulators fit without spills
+    else
+      gemm_launch<T, 4, 4>(m, n, k, al, A, B, be, bz, C);
+  } else
+    gemm_launch<T, 2, 2>(m, n, k, al, A, B, be, bz, C);
 }
 
 // ------------------------------------------------------- cooperative panel
@@ -324,6 +373,7 @@ static void getf2_leaf(int64_t m, int nb, MatView<T> A, int32_t* ipiv, int* info
   cplx<T>* Ap = A.p;
   int64_t lda = A.ld;
   void* args[] = {&m, &nb, &Ap, &lda, &ipiv, &info, &info_off, &chunk, &cval, &cidx, &crow, &rowj};
+  TimerScope ts(MNP_TIMER_PANEL, 0.0);
   MNP_HIP(hipLaunchCooperativeKernel((const void*)k_getf2_coop<T>, dim3((unsigned)P), dim3(256), args,
                                      lds, stream()));
 }

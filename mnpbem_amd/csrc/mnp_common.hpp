@@ -92,6 +92,16 @@ __device__ inline double2 cexp_ikd(double2 k, double d) {
 
 inline int64_t cdiv_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Brackets one kernel launch with HIP events when the timers are enabled
+// (mnp_timers_enable); `work` is its algorithmic flops or bytes.
+struct TimerScope {
+  TimerScope(int kind, double work);
+  ~TimerScope();
+  int kind;
+  double work;
+  void* ev0 = nullptr;
+};
+
 // Converts a user-facing complex scalar into the device type.
 inline double2 to_d2(mnp_complex z) { return mk<double>(z.re, z.im); }
 
