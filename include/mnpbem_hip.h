@@ -168,6 +168,12 @@ int mnp_gemm(int precision, int64_t m, int64_t n, int64_t k, mnp_complex alpha, 
  * (device int32, n) holds 0-based row interchanges.  *info (host) is 0 or
  * the 1-based column of the first exactly zero pivot. */
 int mnp_getrf(int precision, int64_t m, int64_t n, void* A, int64_t lda, int32_t* ipiv, int* info);
+/* LU without row exchanges (the bemret chain's first attempt).  *would_swap
+ * (host) is set to 1 when partial pivoting would have exchanged rows, i.e.
+ * when the factors differ from mnp_getrf's; otherwise they are the same
+ * factors with ipiv = 0, 1, ..., n-1. */
+int mnp_getrf_nopiv(int precision, int64_t m, int64_t n, void* A, int64_t lda, int* info,
+                    int* would_swap);
 /* B <- A^{-1} B (B n x nrhs) from mnp_getrf factors. */
 int mnp_getrs(int precision, int64_t n, int64_t nrhs, const void* LU, int64_t lda,
               const int32_t* ipiv, void* B, int64_t ldb);

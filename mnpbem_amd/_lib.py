@@ -62,6 +62,7 @@ PROTOTYPES = {
     "mnp_bemret_set_refinement": (INT, [P, D, INT]),
     "mnp_gemm": (INT, [INT, I64, I64, I64, MnpComplex, P, I64, P, I64, MnpComplex, P, I64]),
     "mnp_getrf": (INT, [INT, I64, I64, P, I64, P, C.POINTER(INT)]),
+    "mnp_getrf_nopiv": (INT, [INT, I64, I64, P, I64, C.POINTER(INT), C.POINTER(INT)]),
     "mnp_getrs": (INT, [INT, I64, I64, P, I64, P, P, I64]),
     "mnp_getrs_right": (INT, [INT, I64, I64, P, I64, P, P, I64]),
     "mnp_getri": (INT, [INT, I64, P, I64, P, P, I64]),

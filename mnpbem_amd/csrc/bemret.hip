@@ -26,13 +26,18 @@ namespace mnp {
 template <class T>
 struct Chain {
   int64_t n = 0;
+  bool exact_pivoting = true;  // FP64: factors must equal partial pivoting's
   cplx<T>*G1 = nullptr, *G2 = nullptr, *S1 = nullptr, *S2 = nullptr, *D = nullptr, *Di = nullptr;
-  int32_t *piv = nullptr, *sig1 = nullptr, *sig2 = nullptr, *sigD = nullptr, *sigS = nullptr;
+  cplx<T>* backup = nullptr;   // copy of a matrix while it is factorised without pivoting
+  cplx<T>* inv = nullptr;      // diagonal-block inverses: 4 factors x (L, U)
+  int32_t *piv = nullptr, *sig[4] = {nullptr, nullptr, nullptr, nullptr};
+  LU<T> f[4];                  // G1, G2, Sigma1 - Sigma2, Sigma
+  int fallbacks = 0;           // factorisations redone with partial pivoting
   double2 eps1{}, eps2{};
   double k0 = 0;
   Workspace ws;
 
-  explicit Chain(int64_t n_) : n(n_) {
+  explicit Chain(int64_t n_, bool exact) : n(n_), exact_pivoting(exact) {
     const int64_t nn = n * n;
     G1 = dev_alloc<cplx<T>>(nn);
     G2 = dev_alloc<cplx<T>>(nn);
@@ -40,18 +45,41 @@ struct Chain {
     S2 = dev_alloc<cplx<T>>(nn);
     D = dev_alloc<cplx<T>>(nn);
     Di = dev_alloc<cplx<T>>(nn);
+    backup = dev_alloc<cplx<T>>(nn);
+    inv = dev_alloc<cplx<T>>(8 * inv_blocks(n));
     piv = dev_alloc<int32_t>(n);
-    sig1 = dev_alloc<int32_t>(n);
-    sig2 = dev_alloc<int32_t>(n);
-    sigD = dev_alloc<int32_t>(n);
-    sigS = dev_alloc<int32_t>(n);
+    for (auto& q : sig) q = dev_alloc<int32_t>(n);
   }
   ~Chain() {
-    void* ps[] = {G1, G2, S1, S2, D, Di, piv, sig1, sig2, sigD, sigS};
+    void* ps[] = {G1, G2, S1, S2, D, Di, backup, inv, piv, sig[0], sig[1], sig[2], sig[3]};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
   MatView<T> v(cplx<T>* p) const { return MatView<T>{p, n}; }
+
+  // Factorises A in place as factor slot `s`.  First without row exchanges
+  // (no grid-wide pivot search); when partial pivoting would have exchanged
+  // rows and exact pivoting is required, or on a zero pivot, the matrix is
+  // restored and factorised again with partial pivoting.
+  const LU<T>& factor(int s, cplx<T>* A, const char* what) {
+    cplx<T>* Linv = inv + (2 * s) * inv_blocks(n);
+    cplx<T>* Uinv = Linv + inv_blocks(n);
+    copy_mat<T>(n, n, v(A), v(backup));
+    int swap = 0;
+    int info = getrf<T>(n, n, v(A), piv, Linv, Uinv, PIVOT_NONE, &swap, ws);
+    const int32_t* sigma = nullptr;
+    if (info != 0 || (swap && exact_pivoting)) {
+      ++fallbacks;
+      copy_mat<T>(n, n, v(backup), v(A));
+      info = getrf<T>(n, n, v(A), piv, Linv, Uinv, PIVOT_PARTIAL, nullptr, ws);
+      perm_from_pivots(n, piv, sig[s], ws);
+      sigma = sig[s];
+    }
+    MNP_CHECK(info == 0, MNP_ERR_SINGULAR,
+              std::string("bemret: singular ") + what + " (zero pivot " + std::to_string(info) + ")");
+    f[s] = LU<T>{v(A), Linv, Uinv, sigma};
+    return f[s];
+  }
 };
 
 // Sigma = e1 S1 - e2 S2 + c (Di o n n^T), written over S2.
@@ -87,26 +115,18 @@ static dim3 gridnn(int64_t m, int64_t n) {
 template <class T>
 static void chain_build(Chain<T>& c, const GreenRet& g) {
   const int64_t n = c.n;
-  auto lu = [&](cplx<T>* A, int32_t* sigma, const char* what) {
-    const int info = getrf<T>(n, n, c.v(A), c.piv, c.ws);
-    MNP_CHECK(info == 0, MNP_ERR_SINGULAR,
-              std::string("bemret: singular ") + what + " (zero pivot " + std::to_string(info) + ")");
-    perm_from_pivots(n, c.piv, sigma, c.ws);
-  };
-  lu(c.G1, c.sig1, "G1");
-  getrs_right<T>(n, n, c.v(c.G1), c.sig1, c.v(c.S1), c.ws);  // Sigma1 = H1 G1^{-1}
-  lu(c.G2, c.sig2, "G2");
-  getrs_right<T>(n, n, c.v(c.G2), c.sig2, c.v(c.S2), c.ws);  // Sigma2 = H2 G2^{-1}
+  c.fallbacks = 0;
+  getrs_right<T>(n, n, c.factor(0, c.G1, "G1"), c.v(c.S1), c.ws);  // Sigma1 = H1 G1^{-1}
+  getrs_right<T>(n, n, c.factor(1, c.G2, "G2"), c.v(c.S2), c.ws);  // Sigma2 = H2 G2^{-1}
   hipLaunchKernelGGL(k_sub<T>, gridnn(n, n), dim3(256), 0, stream(), n, c.S1, c.S2, c.D);
   MNP_HIP(hipGetLastError());
-  lu(c.D, c.sigD, "Sigma1 - Sigma2");
-  getri<T>(n, c.v(c.D), c.sigD, c.v(c.Di), c.ws);  // Deltai
+  getri<T>(n, c.factor(2, c.D, "Sigma1 - Sigma2"), c.v(c.Di), c.ws);  // Deltai
   const double2 L = csub<double>(c.eps1, c.eps2);
   const double2 cc = cscale<double>(cmul<double>(L, L), c.k0 * c.k0);
   hipLaunchKernelGGL(k_sigma<T>, gridnn(n, n), dim3(256), 0, stream(), n, c.S1, c.S2, c.Di,
                      g.geo.nvec, c.eps1, c.eps2, cc);
   MNP_HIP(hipGetLastError());
-  lu(c.S2, c.sigS, "Sigma");
+  c.factor(3, c.S2, "Sigma");
 }
 
 // ------------------------------------------------------------- mldivide
@@ -214,15 +234,15 @@ static void chain_apply(Chain<T>& c, const GreenRet& g, int64_t p, const cplx<T>
   hipLaunchKernelGGL(k_mld1<T>, gp, dim3(256), 0, stream(), n, p, b, Y, alpha, De, g.geo.nvec, k, L1);
   gemm<T>(n, 3 * p, n, one, c.v(c.Di), MatView<T>{alpha, n}, zero, MatView<T>{t, n});
   hipLaunchKernelGGL(k_mld2<T>, gp, dim3(256), 0, stream(), n, p, De, t, s2, g.geo.nvec, ikdL);
-  getrs<T>(n, p, c.v(c.S2), c.sigS, MatView<T>{s2, n}, c.ws);  // sig2 = Sigma^{-1} rhs
+  getrs<T>(n, p, c.f[3], MatView<T>{s2, n}, c.ws);  // sig2 = Sigma^{-1} rhs
   hipLaunchKernelGGL(k_mld3<T>, gp, dim3(256), 0, stream(), n, p, s2, alpha, u, g.geo.nvec, ikdL);
   gemm<T>(n, 3 * p, n, one, c.v(c.Di), MatView<T>{u, n}, zero, MatView<T>{h2, n});
   // V1 -> x[0:4p] (sig1 | h1), V2 -> x[4p:8p] (sig2 | h2)
   hipLaunchKernelGGL(k_mld4<T>, dim3((unsigned)cdiv_up(n, 256), (unsigned)(4 * p)), dim3(256), 0,
                      stream(), n, p, b, s2, h2, x, x + 4 * p * n);
   MNP_HIP(hipGetLastError());
-  getrs<T>(n, 4 * p, c.v(c.G1), c.sig1, MatView<T>{x, n}, c.ws);
-  getrs<T>(n, 4 * p, c.v(c.G2), c.sig2, MatView<T>{x + 4 * p * n, n}, c.ws);
+  getrs<T>(n, 4 * p, c.f[0], MatView<T>{x, n}, c.ws);
+  getrs<T>(n, 4 * p, c.f[1], MatView<T>{x + 4 * p * n, n}, c.ws);
 }
 
 // b = [phi | a | De | alpha] from the excitation potentials (excitation.m:21-29).
@@ -338,9 +358,9 @@ struct BemRet {
 
   BemRet(const GreenRet* g_, int solver_) : g(g_), solver(solver_), n(g_->n) {
     if (solver == MNP_SOLVER_FP64) {
-      c64.reset(new Chain<double>(n));
+      c64.reset(new Chain<double>(n, true));
     } else {
-      c32.reset(new Chain<float>(n));
+      c32.reset(new Chain<float>(n, false));
       G1 = dev_alloc<double2>(n * n);
       G2 = dev_alloc<double2>(n * n);
       H1 = dev_alloc<double2>(n * n);

@@ -53,8 +53,8 @@ constexpr int GBK = 16;
 
 template <class T, int WM, int WN>
 __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, cplx<T> alpha,
-                                               const cplx<T>* __restrict__ A, int64_t lda,
-                                               const cplx<T>* __restrict__ B, int64_t ldb,
+                                               const cplx<T>* A, int64_t lda,
+                                               const cplx<T>* B, int64_t ldb,
                                                cplx<T> beta, int beta_zero, cplx<T>* C,
                                                int64_t ldc) {
   using acc_t = typename MF<T>::acc_t;
@@ -515,51 +515,63 @@ void convert_mat(int64_t m, int64_t n, MatView<Ts> A, MatView<Td> B) {
   MNP_HIP(hipGetLastError());
 }
 
-// Inverse of the nb x nb unit-lower (upper == 0) or upper (upper == 1)
-// triangle of LU (nb <= 64), written densely to X.  Thread j owns column j.
+// ------------------------------------------- diagonal-block inverses
+// The triangular solves multiply by the inverses of the 32 x 32 diagonal
+// blocks of L (unit lower) and U.  Block b covers rows/columns
+// [32 b, 32 b + 32) and is stored column-major (ld 32) at inv + 1024 b;
+// a partial last block is padded with the identity.
+constexpr int TB = 32;
+constexpr int64_t TB2 = TB * TB;
+
+// One workgroup per block: threads 0..31 build column j of L^{-1},
+// threads 32..63 column j of U^{-1}.
 template <class T>
-__global__ void __launch_bounds__(64) k_trtri(int nb, int upper, const cplx<T>* LU, int64_t ld,
-                                              cplx<T>* X, int64_t ldx) {
-  __shared__ cplx<T> sL[64][65];
-  __shared__ cplx<T> sX[64][65];
-  const int j = threadIdx.x;
-  for (int c = 0; c < nb; ++c)
-    if (j < nb) sL[j][c] = LU[c * ld + j];
+__global__ void __launch_bounds__(64) k_trtri_blocks(int64_t n, const cplx<T>* LU, int64_t ld,
+                                                     cplx<T>* Linv, cplx<T>* Uinv) {
+  __shared__ cplx<T> sA[TB][TB + 1];
+  __shared__ cplx<T> sL[TB][TB + 1];
+  __shared__ cplx<T> sU[TB][TB + 1];
+  const int64_t b = blockIdx.x;
+  const int nb = (int)min((int64_t)TB, n - b * TB);
+  const cplx<T>* A = LU + (b * TB) * ld + b * TB;
+  for (int idx = threadIdx.x; idx < TB * TB; idx += 64) {
+    const int r = idx % TB, c = idx / TB;
+    sA[r][c] = (r < nb && c < nb) ? A[c * ld + r] : (r == c ? mk<T>(1, 0) : czero<T>());
+  }
   __syncthreads();
-  if (j < nb) {
-    if (!upper) {
-      for (int i = 0; i < nb; ++i) {
-        cplx<T> s = i == j ? mk<T>(1, 0) : czero<T>();
-        if (i > j)
-          for (int k = j; k < i; ++k) s = csub<T>(s, cmul<T>(sL[i][k], sX[k][j]));
-        sX[i][j] = i < j ? czero<T>() : s;
+  const int j = threadIdx.x & 31;
+  if (threadIdx.x < 32) {
+    for (int i = 0; i < TB; ++i) {
+      cplx<T> s = i == j ? mk<T>(1, 0) : czero<T>();
+      if (i > j && i < nb)
+        for (int k = j; k < i; ++k) s = csub<T>(s, cmul<T>(sA[i][k], sL[k][j]));
+      sL[i][j] = i < j ? czero<T>() : s;
+    }
+  } else {
+    for (int i = TB - 1; i >= 0; --i) {
+      if (i > j) {
+        sU[i][j] = czero<T>();
+        continue;
       }
-    } else {
-      for (int i = nb - 1; i >= 0; --i) {
-        if (i > j) {
-          sX[i][j] = czero<T>();
-          continue;
-        }
-        cplx<T> s = i == j ? mk<T>(1, 0) : czero<T>();
-        for (int k = i + 1; k <= j; ++k) s = csub<T>(s, cmul<T>(sL[i][k], sX[k][j]));
-        sX[i][j] = cdiv<T>(s, sL[i][i]);
-      }
+      cplx<T> s = i == j ? mk<T>(1, 0) : czero<T>();
+      for (int k = i + 1; k <= j; ++k) s = csub<T>(s, cmul<T>(sA[i][k], sU[k][j]));
+      sU[i][j] = cdiv<T>(s, sA[i][i]);
     }
   }
   __syncthreads();
-  for (int c = 0; c < nb; ++c)
-    if (j < nb) X[c * ldx + j] = sX[j][c];
+  for (int idx = threadIdx.x; idx < TB * TB; idx += 64) {
+    const int r = idx % TB, c = idx / TB;
+    if (Linv) Linv[b * TB2 + idx] = sL[r][c];
+    if (Uinv) Uinv[b * TB2 + idx] = sU[r][c];
+  }
 }
 
-constexpr int TRSM_LEAF = 64;
-constexpr int LU_LEAF = 32;
-
 template <class T>
-static cplx<T>* trtri_block(int nb, int upper, MatView<T> LU, Workspace& ws) {
-  cplx<T>* X = (cplx<T>*)ws.get(3, sizeof(cplx<T>) * 64 * 64);
-  hipLaunchKernelGGL(k_trtri<T>, dim3(1), dim3(64), 0, stream(), nb, upper, LU.p, LU.ld, X, (int64_t)64);
+void block_inverses(int64_t n, MatView<T> LU, cplx<T>* Linv, cplx<T>* Uinv) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_trtri_blocks<T>, dim3((unsigned)cdiv_up(n, TB)), dim3(64), 0, stream(), n, LU.p,
+                     LU.ld, Linv, Uinv);
   MNP_HIP(hipGetLastError());
-  return X;
 }
 
 static const double2 ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
@@ -569,104 +581,204 @@ static int64_t split(int64_t n, int64_t leaf) {
   return n1 < leaf ? leaf : n1;
 }
 
-// B <- L^{-1} B (L unit lower n x n)
+// Recursive triangular solves.  Every split point is a multiple of 32 from
+// the factor's origin, so each leaf is one diagonal block and is applied as
+// an in-place product with its inverse (the GEMM tile covers the whole
+// 32-row or 32-column leaf, so each workgroup reads its operand block before
+// it overwrites it).
+// B <- L^{-1} B
 template <class T>
-void trsm_llu(int64_t n, int64_t nrhs, MatView<T> LU, MatView<T> B, Workspace& ws) {
+void trsm_llu(int64_t n, int64_t nrhs, MatView<T> LU, const cplx<T>* Linv, MatView<T> B) {
   if (n <= 0 || nrhs <= 0) return;
-  if (n <= TRSM_LEAF) {
-    cplx<T>* X = trtri_block<T>((int)n, 0, LU, ws);
-    cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(n * nrhs));
-    copy_mat<T>(n, nrhs, B, MatView<T>{W, n});
-    gemm<T>(n, nrhs, n, ONE, MatView<T>{X, 64}, MatView<T>{W, n}, ZERO, B);
+  if (n <= TB) {
+    gemm<T>(n, nrhs, n, ONE, MatView<T>{const_cast<cplx<T>*>(Linv), TB}, B, ZERO, B);
     return;
   }
-  const int64_t n1 = split(n, TRSM_LEAF), n2 = n - n1;
-  trsm_llu<T>(n1, nrhs, LU, B, ws);
+  const int64_t n1 = split(n, TB), n2 = n - n1;
+  trsm_llu<T>(n1, nrhs, LU, Linv, B);
   gemm<T>(n2, nrhs, n1, MONE, MatView<T>{LU.at(n1, 0), LU.ld}, B, ONE, MatView<T>{B.at(n1, 0), B.ld});
-  trsm_llu<T>(n2, nrhs, MatView<T>{LU.at(n1, n1), LU.ld}, MatView<T>{B.at(n1, 0), B.ld}, ws);
+  trsm_llu<T>(n2, nrhs, MatView<T>{LU.at(n1, n1), LU.ld}, Linv + (n1 / TB) * TB2,
+              MatView<T>{B.at(n1, 0), B.ld});
 }
-
-// B <- U^{-1} B (U upper n x n)
+// B <- U^{-1} B
 template <class T>
-void trsm_lun(int64_t n, int64_t nrhs, MatView<T> LU, MatView<T> B, Workspace& ws) {
+void trsm_lun(int64_t n, int64_t nrhs, MatView<T> LU, const cplx<T>* Uinv, MatView<T> B) {
   if (n <= 0 || nrhs <= 0) return;
-  if (n <= TRSM_LEAF) {
-    cplx<T>* X = trtri_block<T>((int)n, 1, LU, ws);
-    cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(n * nrhs));
-    copy_mat<T>(n, nrhs, B, MatView<T>{W, n});
-    gemm<T>(n, nrhs, n, ONE, MatView<T>{X, 64}, MatView<T>{W, n}, ZERO, B);
+  if (n <= TB) {
+    gemm<T>(n, nrhs, n, ONE, MatView<T>{const_cast<cplx<T>*>(Uinv), TB}, B, ZERO, B);
     return;
   }
-  const int64_t n1 = split(n, TRSM_LEAF), n2 = n - n1;
-  trsm_lun<T>(n2, nrhs, MatView<T>{LU.at(n1, n1), LU.ld}, MatView<T>{B.at(n1, 0), B.ld}, ws);
+  const int64_t n1 = split(n, TB), n2 = n - n1;
+  trsm_lun<T>(n2, nrhs, MatView<T>{LU.at(n1, n1), LU.ld}, Uinv + (n1 / TB) * TB2,
+              MatView<T>{B.at(n1, 0), B.ld});
   gemm<T>(n1, nrhs, n2, MONE, MatView<T>{LU.at(0, n1), LU.ld}, MatView<T>{B.at(n1, 0), B.ld}, ONE, B);
-  trsm_lun<T>(n1, nrhs, LU, B, ws);
+  trsm_lun<T>(n1, nrhs, LU, Uinv, B);
 }
-
-// B <- B U^{-1} (B m x n)
+// B <- B U^{-1}   (B m x n)
 template <class T>
-void trsm_run(int64_t m, int64_t n, MatView<T> LU, MatView<T> B, Workspace& ws) {
+void trsm_run(int64_t m, int64_t n, MatView<T> LU, const cplx<T>* Uinv, MatView<T> B) {
   if (n <= 0 || m <= 0) return;
-  if (n <= TRSM_LEAF) {
-    cplx<T>* X = trtri_block<T>((int)n, 1, LU, ws);
-    cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(m * n));
-    copy_mat<T>(m, n, B, MatView<T>{W, m});
-    gemm<T>(m, n, n, ONE, MatView<T>{W, m}, MatView<T>{X, 64}, ZERO, B);
+  if (n <= TB) {
+    gemm<T>(m, n, n, ONE, B, MatView<T>{const_cast<cplx<T>*>(Uinv), TB}, ZERO, B);
     return;
   }
-  const int64_t n1 = split(n, TRSM_LEAF), n2 = n - n1;
-  trsm_run<T>(m, n1, LU, B, ws);
+  const int64_t n1 = split(n, TB), n2 = n - n1;
+  trsm_run<T>(m, n1, LU, Uinv, B);
   gemm<T>(m, n2, n1, MONE, B, MatView<T>{LU.at(0, n1), LU.ld}, ONE, MatView<T>{B.at(0, n1), B.ld});
-  trsm_run<T>(m, n2, MatView<T>{LU.at(n1, n1), LU.ld}, MatView<T>{B.at(0, n1), B.ld}, ws);
+  trsm_run<T>(m, n2, MatView<T>{LU.at(n1, n1), LU.ld}, Uinv + (n1 / TB) * TB2,
+              MatView<T>{B.at(0, n1), B.ld});
 }
-
-// B <- B L^{-1} (B m x n, L unit lower)
+// B <- B L^{-1}   (B m x n, L unit lower)
 template <class T>
-void trsm_rlu(int64_t m, int64_t n, MatView<T> LU, MatView<T> B, Workspace& ws) {
+void trsm_rlu(int64_t m, int64_t n, MatView<T> LU, const cplx<T>* Linv, MatView<T> B) {
   if (n <= 0 || m <= 0) return;
-  if (n <= TRSM_LEAF) {
-    cplx<T>* X = trtri_block<T>((int)n, 0, LU, ws);
-    cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(m * n));
-    copy_mat<T>(m, n, B, MatView<T>{W, m});
-    gemm<T>(m, n, n, ONE, MatView<T>{W, m}, MatView<T>{X, 64}, ZERO, B);
+  if (n <= TB) {
+    gemm<T>(m, n, n, ONE, B, MatView<T>{const_cast<cplx<T>*>(Linv), TB}, ZERO, B);
     return;
   }
-  const int64_t n1 = split(n, TRSM_LEAF), n2 = n - n1;
-  trsm_rlu<T>(m, n2, MatView<T>{LU.at(n1, n1), LU.ld}, MatView<T>{B.at(0, n1), B.ld}, ws);
+  const int64_t n1 = split(n, TB), n2 = n - n1;
+  trsm_rlu<T>(m, n2, MatView<T>{LU.at(n1, n1), LU.ld}, Linv + (n1 / TB) * TB2,
+              MatView<T>{B.at(0, n1), B.ld});
   gemm<T>(m, n1, n2, MONE, MatView<T>{B.at(0, n1), B.ld}, MatView<T>{LU.at(n1, 0), LU.ld}, ONE, B);
-  trsm_rlu<T>(m, n1, LU, B, ws);
+  trsm_rlu<T>(m, n1, LU, Linv, B);
+}
+
+// ------------------------------------------------ unpivoted panel leaf
+// LU without pivoting of the nb x nb diagonal block (nb <= 32) by one
+// workgroup, which also writes the block's L^{-1} and U^{-1}.
+template <class T>
+__global__ void __launch_bounds__(256) k_lu_nopiv_diag(int nb, cplx<T>* A, int64_t lda, int* info,
+                                                       int info_off, cplx<T>* Linv, cplx<T>* Uinv) {
+  __shared__ cplx<T> P[TB][TB + 1];
+  __shared__ cplx<T> sL[TB][TB + 1];
+  __shared__ cplx<T> sU[TB][TB + 1];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < TB * TB; idx += 256) {
+    const int r = idx % TB, c = idx / TB;
+    P[r][c] = (r < nb && c < nb) ? A[c * lda + r] : (r == c ? mk<T>(1, 0) : czero<T>());
+  }
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    const cplx<T> piv = P[j][j];
+    const bool ok = piv.x != 0 || piv.y != 0;
+    if (!ok && tid == 0 && *info == 0) *info = info_off + j + 1;
+    if (ok && tid > j && tid < nb) P[tid][j] = cdiv<T>(P[tid][j], piv);
+    __syncthreads();
+    if (ok)
+      for (int idx = tid; idx < TB * TB; idx += 256) {
+        const int r = idx % TB, c = idx / TB;
+        if (r > j && c > j && r < nb && c < nb) P[r][c] = csub<T>(P[r][c], cmul<T>(P[r][j], P[j][c]));
+      }
+    __syncthreads();
+  }
+  const int j = tid & 31;
+  if (tid < 32) {
+    for (int i = 0; i < TB; ++i) {
+      cplx<T> s = i == j ? mk<T>(1, 0) : czero<T>();
+      if (i > j && i < nb)
+        for (int k = j; k < i; ++k) s = csub<T>(s, cmul<T>(P[i][k], sL[k][j]));
+      sL[i][j] = i < j ? czero<T>() : s;
+    }
+  } else if (tid < 64) {
+    for (int i = TB - 1; i >= 0; --i) {
+      if (i > j) {
+        sU[i][j] = czero<T>();
+        continue;
+      }
+      cplx<T> s = i == j ? mk<T>(1, 0) : czero<T>();
+      for (int k = i + 1; k <= j; ++k) s = csub<T>(s, cmul<T>(P[i][k], sU[k][j]));
+      sU[i][j] = cdiv<T>(s, P[i][i]);
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < TB * TB; idx += 256) {
+    const int r = idx % TB, c = idx / TB;
+    if (r < nb && c < nb) A[c * lda + r] = P[r][c];
+    Linv[idx] = sL[r][c];
+    Uinv[idx] = sU[r][c];
+  }
+}
+
+// Partial pivoting would keep the diagonal of column j iff no entry below
+// it has a larger |Re|+|Im| at step j; that entry is l_ij u_jj (ties go to
+// the diagonal, the lower index).  Sets *flag when some column would swap.
+template <class T>
+__global__ void k_verify_nopiv(int64_t m, int nb, const cplx<T>* A, int64_t lda, int* flag) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int jmax = (int)min((int64_t)nb, i);
+  bool bad = false;
+  for (int j = 0; j < jmax; ++j) {
+    const cplx<T> u = A[j * lda + j];
+    const cplx<T> v = cmul<T>(A[j * lda + i], u);
+    bad |= cabs1<T>(v) > cabs1<T>(u);
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
+__global__ void k_iota(int64_t n, int32_t* ipiv, int32_t off) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) ipiv[i] = (int32_t)(off + i);
 }
 
 // ------------------------------------------------------------ recursive LU
+struct LUState {
+  int mode;
+  int* info;  // device
+  int* flag;  // device: partial pivoting would have swapped (mode PIVOT_NONE)
+};
+
 template <class T>
-static void getrf_rec(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, int* info, int info_off,
-                      Workspace& ws) {
-  if (n <= LU_LEAF) {
-    getf2_leaf<T>(m, (int)n, A, ipiv, info, info_off, ws);
+static void getrf_rec(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, int info_off,
+                      cplx<T>* Linv, cplx<T>* Uinv, const LUState& st, Workspace& ws) {
+  if (n <= TB) {
+    if (st.mode == PIVOT_PARTIAL) {
+      getf2_leaf<T>(m, (int)n, A, ipiv, st.info, info_off, ws);
+      block_inverses<T>(n, A, Linv, Uinv);
+    } else {
+      {
+        TimerScope ts(MNP_TIMER_PANEL, 0.0);
+        hipLaunchKernelGGL(k_lu_nopiv_diag<T>, dim3(1), dim3(256), 0, stream(), (int)n, A.p, A.ld,
+                           st.info, info_off, Linv, Uinv);
+      }
+      if (m > n) {
+        MatView<T> A21{A.at(n, 0), A.ld};
+        gemm<T>(m - n, n, n, ONE, A21, MatView<T>{Uinv, TB}, ZERO, A21);  // L21 = A21 U11^{-1}
+      }
+      hipLaunchKernelGGL(k_verify_nopiv<T>, dim3((unsigned)cdiv_up(m, 256)), dim3(256), 0, stream(), m,
+                         (int)n, A.p, A.ld, st.flag);
+      hipLaunchKernelGGL(k_iota, dim3((unsigned)cdiv_up(n, 256)), dim3(256), 0, stream(), n, ipiv, 0);
+      MNP_HIP(hipGetLastError());
+    }
     return;
   }
-  const int64_t n1 = split(n, LU_LEAF), n2 = n - n1;
-  getrf_rec<T>(m, n1, A, ipiv, info, info_off, ws);
-  laswp<T>(A, m, n1, n, 0, n1, ipiv, ws);
-  trsm_llu<T>(n1, n2, A, MatView<T>{A.at(0, n1), A.ld}, ws);
+  const int64_t n1 = split(n, TB), n2 = n - n1;
+  const bool piv = st.mode == PIVOT_PARTIAL;
+  getrf_rec<T>(m, n1, A, ipiv, info_off, Linv, Uinv, st, ws);
+  if (piv) laswp<T>(A, m, n1, n, 0, n1, ipiv, ws);
+  trsm_llu<T>(n1, n2, A, Linv, MatView<T>{A.at(0, n1), A.ld});
   gemm<T>(m - n1, n2, n1, MONE, MatView<T>{A.at(n1, 0), A.ld}, MatView<T>{A.at(0, n1), A.ld}, ONE,
           MatView<T>{A.at(n1, n1), A.ld});
-  getrf_rec<T>(m - n1, n2, MatView<T>{A.at(n1, n1), A.ld}, ipiv + n1, info, info_off + (int)n1, ws);
+  getrf_rec<T>(m - n1, n2, MatView<T>{A.at(n1, n1), A.ld}, ipiv + n1, info_off + (int)n1,
+               Linv + (n1 / TB) * TB2, Uinv + (n1 / TB) * TB2, st, ws);
   hipLaunchKernelGGL(k_add_offset, dim3((unsigned)cdiv_up(n2, 256)), dim3(256), 0, stream(), n2,
                      ipiv + n1, (int32_t)n1);
-  laswp<T>(A, m, 0, n1, n1, n, ipiv, ws);
+  if (piv) laswp<T>(A, m, 0, n1, n1, n, ipiv, ws);
 }
 
 template <class T>
-int getrf(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, Workspace& ws) {
+int getrf(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, cplx<T>* Linv, cplx<T>* Uinv, int mode,
+          int* would_swap, Workspace& ws) {
   MNP_CHECK(m >= n && n > 0, MNP_ERR_ARG, "getrf: need m >= n > 0");
-  int* info = (int*)ws.get(5, sizeof(int));
-  MNP_HIP(hipMemsetAsync(info, 0, sizeof(int), stream()));
-  getrf_rec<T>(m, n, A, ipiv, info, 0, ws);
-  int h = 0;
-  MNP_HIP(hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  int* d = (int*)ws.get(5, 2 * sizeof(int));
+  MNP_HIP(hipMemsetAsync(d, 0, 2 * sizeof(int), stream()));
+  LUState st{mode, d, d + 1};
+  getrf_rec<T>(m, n, A, ipiv, 0, Linv, Uinv, st, ws);
+  int h[2] = {0, 0};
+  MNP_HIP(hipMemcpyAsync(h, d, 2 * sizeof(int), hipMemcpyDeviceToHost, stream()));
   MNP_HIP(hipStreamSynchronize(stream()));
-  return h;
+  if (would_swap) *would_swap = h[1];
+  return h[0];
 }
 
 template <class T>
@@ -694,56 +806,56 @@ __global__ void k_scatter_cols(int64_t m, int64_t n, const int32_t* sigma, const
 }
 
 template <class T>
-void getrs(int64_t n, int64_t nrhs, MatView<T> LU, const int32_t* sigma, MatView<T> B, Workspace& ws) {
-  cplx<T>* W = (cplx<T>*)ws.get(1, sizeof(cplx<T>) * (size_t)(n * nrhs));
-  MatView<T> Wv{W, n};
-  gather_rows<T>(n, nrhs, sigma, B, Wv);
-  trsm_llu<T>(n, nrhs, LU, Wv, ws);
-  trsm_lun<T>(n, nrhs, LU, Wv, ws);
-  copy_mat<T>(n, nrhs, Wv, B);
+void getrs(int64_t n, int64_t nrhs, const LU<T>& f, MatView<T> B, Workspace& ws) {
+  if (f.sigma) {
+    cplx<T>* W = (cplx<T>*)ws.get(1, sizeof(cplx<T>) * (size_t)(n * nrhs));
+    MatView<T> Wv{W, n};
+    gather_rows<T>(n, nrhs, f.sigma, B, Wv);
+    copy_mat<T>(n, nrhs, Wv, B);
+  }
+  trsm_llu<T>(n, nrhs, f.A, f.Linv, B);
+  trsm_lun<T>(n, nrhs, f.A, f.Uinv, B);
 }
 
 template <class T>
-void getrs_right(int64_t m, int64_t n, MatView<T> LU, const int32_t* sigma, MatView<T> B,
-                 Workspace& ws) {
-  trsm_run<T>(m, n, LU, B, ws);
-  trsm_rlu<T>(m, n, LU, B, ws);
-  cplx<T>* W = (cplx<T>*)ws.get(1, sizeof(cplx<T>) * (size_t)(m * n));
-  copy_mat<T>(m, n, B, MatView<T>{W, m});
-  hipLaunchKernelGGL(k_scatter_cols<T>, grid2(m, n), dim3(256), 0, stream(), m, n, sigma, W, m, B.p,
-                     B.ld);
-  MNP_HIP(hipGetLastError());
+void getrs_right(int64_t m, int64_t n, const LU<T>& f, MatView<T> B, Workspace& ws) {
+  trsm_run<T>(m, n, f.A, f.Uinv, B);
+  trsm_rlu<T>(m, n, f.A, f.Linv, B);
+  if (f.sigma) {
+    cplx<T>* W = (cplx<T>*)ws.get(1, sizeof(cplx<T>) * (size_t)(m * n));
+    copy_mat<T>(m, n, B, MatView<T>{W, m});
+    hipLaunchKernelGGL(k_scatter_cols<T>, grid2(m, n), dim3(256), 0, stream(), m, n, f.sigma, W, m, B.p,
+                       B.ld);
+    MNP_HIP(hipGetLastError());
+  }
 }
 
 template <class T>
 __global__ void k_perm_identity(int64_t n, const int32_t* sigma, cplx<T>* X, int64_t ldx) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
-  if (i < n && j < n) X[j * ldx + i] = sigma[i] == j ? mk<T>(1, 0) : czero<T>();
+  if (i < n && j < n) X[j * ldx + i] = (sigma ? sigma[i] : i) == j ? mk<T>(1, 0) : czero<T>();
 }
 
 template <class T>
-void getri(int64_t n, MatView<T> LU, const int32_t* sigma, MatView<T> X, Workspace& ws) {
-  hipLaunchKernelGGL(k_perm_identity<T>, grid2(n, n), dim3(256), 0, stream(), n, sigma, X.p, X.ld);
+void getri(int64_t n, const LU<T>& f, MatView<T> X, Workspace& ws) {
+  hipLaunchKernelGGL(k_perm_identity<T>, grid2(n, n), dim3(256), 0, stream(), n, f.sigma, X.p, X.ld);
   MNP_HIP(hipGetLastError());
-  trsm_llu<T>(n, n, LU, X, ws);
-  trsm_lun<T>(n, n, LU, X, ws);
+  trsm_llu<T>(n, n, f.A, f.Linv, X);
+  trsm_lun<T>(n, n, f.A, f.Uinv, X);
 }
 
 // ---------------------------------------------------- instantiations
 #define MNP_INST(T)                                                                              \
   template void gemm<T>(int64_t, int64_t, int64_t, double2, MatView<T>, MatView<T>, double2,     \
                         MatView<T>);                                                             \
-  template int getrf<T>(int64_t, int64_t, MatView<T>, int32_t*, Workspace&);                     \
+  template int getrf<T>(int64_t, int64_t, MatView<T>, int32_t*, cplx<T>*, cplx<T>*, int, int*,   \
+                        Workspace&);                                                             \
+  template void block_inverses<T>(int64_t, MatView<T>, cplx<T>*, cplx<T>*);                      \
   template void gather_rows<T>(int64_t, int64_t, const int32_t*, MatView<T>, MatView<T>);        \
-  template void trsm_llu<T>(int64_t, int64_t, MatView<T>, MatView<T>, Workspace&);               \
-  template void trsm_lun<T>(int64_t, int64_t, MatView<T>, MatView<T>, Workspace&);               \
-  template void trsm_run<T>(int64_t, int64_t, MatView<T>, MatView<T>, Workspace&);               \
-  template void trsm_rlu<T>(int64_t, int64_t, MatView<T>, MatView<T>, Workspace&);               \
-  template void getrs<T>(int64_t, int64_t, MatView<T>, const int32_t*, MatView<T>, Workspace&);  \
-  template void getrs_right<T>(int64_t, int64_t, MatView<T>, const int32_t*, MatView<T>,         \
-                               Workspace&);                                                      \
-  template void getri<T>(int64_t, MatView<T>, const int32_t*, MatView<T>, Workspace&);           \
+  template void getrs<T>(int64_t, int64_t, const LU<T>&, MatView<T>, Workspace&);                \
+  template void getrs_right<T>(int64_t, int64_t, const LU<T>&, MatView<T>, Workspace&);          \
+  template void getri<T>(int64_t, const LU<T>&, MatView<T>, Workspace&);                         \
   template void copy_mat<T>(int64_t, int64_t, MatView<T>, MatView<T>);
 MNP_INST(double)
 MNP_INST(float)
@@ -760,6 +872,22 @@ Workspace& abi_ws() {
   static Workspace ws;
   return ws;
 }
+Workspace& abi_inv() {
+  static Workspace ws;
+  return ws;
+}
+// Factors handed over through the ABI: permutation and block inverses are
+// rebuilt from (LU, ipiv).
+template <class T>
+static LU<T> abi_factors(int64_t n, const void* A, int64_t lda, const int32_t* ipiv) {
+  int32_t* sigma = (int32_t*)abi_inv().get(1, sizeof(int32_t) * n);
+  perm_from_pivots(n, ipiv, sigma, abi_ws());
+  cplx<T>* inv = (cplx<T>*)abi_inv().get(0, 2 * sizeof(cplx<T>) * inv_blocks(n));
+  MatView<T> f{(cplx<T>*)A, lda};
+  block_inverses<T>(n, f, inv, inv + inv_blocks(n));
+  return LU<T>{f, inv, inv + inv_blocks(n), sigma};
+}
+
 template <class F>
 int by_precision(int precision, F&& f) {
   return guarded([&] {
@@ -790,43 +918,54 @@ int mnp_getrf(int precision, int64_t m, int64_t n, void* A, int64_t lda, int32_t
   return by_precision(precision, [&](auto t) {
     using T = decltype(t);
     MNP_CHECK(A && ipiv && m >= n && n > 0 && lda >= m, MNP_ERR_ARG, "getrf: bad arguments");
-    const int r = getrf<T>(m, n, MatView<T>{(cplx<T>*)A, lda}, ipiv, abi_ws());
+    cplx<T>* inv = (cplx<T>*)abi_inv().get(0, 2 * sizeof(cplx<T>) * inv_blocks(n));
+    const int r = getrf<T>(m, n, MatView<T>{(cplx<T>*)A, lda}, ipiv, inv, inv + inv_blocks(n),
+                           PIVOT_PARTIAL, nullptr, abi_ws());
     if (info) *info = r;
   });
 }
 
-int mnp_getrs(int precision, int64_t n, int64_t nrhs, const void* LU, int64_t lda, const int32_t* ipiv,
+int mnp_getrf_nopiv(int precision, int64_t m, int64_t n, void* A, int64_t lda, int* info,
+                    int* would_swap) {
+  return by_precision(precision, [&](auto t) {
+    using T = decltype(t);
+    MNP_CHECK(A && m >= n && n > 0 && lda >= m, MNP_ERR_ARG, "getrf_nopiv: bad arguments");
+    cplx<T>* inv = (cplx<T>*)abi_inv().get(0, 2 * sizeof(cplx<T>) * inv_blocks(n));
+    int32_t* ipiv = (int32_t*)abi_inv().get(1, sizeof(int32_t) * n);
+    int sw = 0;
+    const int r = getrf<T>(m, n, MatView<T>{(cplx<T>*)A, lda}, ipiv, inv, inv + inv_blocks(n),
+                           PIVOT_NONE, &sw, abi_ws());
+    if (info) *info = r;
+    if (would_swap) *would_swap = sw;
+  });
+}
+
+int mnp_getrs(int precision, int64_t n, int64_t nrhs, const void* LUp, int64_t lda, const int32_t* ipiv,
               void* B, int64_t ldb) {
   return by_precision(precision, [&](auto t) {
     using T = decltype(t);
-    MNP_CHECK(LU && ipiv && B && n > 0 && nrhs >= 0 && lda >= n && ldb >= n, MNP_ERR_ARG,
+    MNP_CHECK(LUp && ipiv && B && n > 0 && nrhs >= 0 && lda >= n && ldb >= n, MNP_ERR_ARG,
               "getrs: bad arguments");
-    int32_t* sigma = (int32_t*)abi_ws().get(5, sizeof(int32_t) * n);
-    perm_from_pivots(n, ipiv, sigma, abi_ws());
-    getrs<T>(n, nrhs, MatView<T>{(cplx<T>*)LU, lda}, sigma, MatView<T>{(cplx<T>*)B, ldb}, abi_ws());
+    getrs<T>(n, nrhs, abi_factors<T>(n, LUp, lda, ipiv), MatView<T>{(cplx<T>*)B, ldb}, abi_ws());
   });
 }
 
-int mnp_getrs_right(int precision, int64_t m, int64_t n, const void* LU, int64_t lda,
+int mnp_getrs_right(int precision, int64_t m, int64_t n, const void* LUp, int64_t lda,
                     const int32_t* ipiv, void* B, int64_t ldb) {
   return by_precision(precision, [&](auto t) {
     using T = decltype(t);
-    MNP_CHECK(LU && ipiv && B && n > 0 && m >= 0 && lda >= n && ldb >= m, MNP_ERR_ARG,
+    MNP_CHECK(LUp && ipiv && B && n > 0 && m >= 0 && lda >= n && ldb >= m, MNP_ERR_ARG,
               "getrs_right: bad arguments");
-    int32_t* sigma = (int32_t*)abi_ws().get(5, sizeof(int32_t) * n);
-    perm_from_pivots(n, ipiv, sigma, abi_ws());
-    getrs_right<T>(m, n, MatView<T>{(cplx<T>*)LU, lda}, sigma, MatView<T>{(cplx<T>*)B, ldb}, abi_ws());
+    getrs_right<T>(m, n, abi_factors<T>(n, LUp, lda, ipiv), MatView<T>{(cplx<T>*)B, ldb}, abi_ws());
   });
 }
 
-int mnp_getri(int precision, int64_t n, const void* LU, int64_t lda, const int32_t* ipiv, void* X,
+int mnp_getri(int precision, int64_t n, const void* LUp, int64_t lda, const int32_t* ipiv, void* X,
               int64_t ldx) {
   return by_precision(precision, [&](auto t) {
     using T = decltype(t);
-    MNP_CHECK(LU && ipiv && X && n > 0 && lda >= n && ldx >= n, MNP_ERR_ARG, "getri: bad arguments");
-    int32_t* sigma = (int32_t*)abi_ws().get(5, sizeof(int32_t) * n);
-    perm_from_pivots(n, ipiv, sigma, abi_ws());
-    getri<T>(n, MatView<T>{(cplx<T>*)LU, lda}, sigma, MatView<T>{(cplx<T>*)X, ldx}, abi_ws());
+    MNP_CHECK(LUp && ipiv && X && n > 0 && lda >= n && ldx >= n, MNP_ERR_ARG, "getri: bad arguments");
+    getri<T>(n, abi_factors<T>(n, LUp, lda, ipiv), MatView<T>{(cplx<T>*)X, ldx}, abi_ws());
   });
 }
 

@@ -37,49 +37,55 @@ template <class T>
 void gemm(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, MatView<T> B,
           double2 beta, MatView<T> C);
 
-// In-place LU with partial pivoting of an m x n matrix (m >= n), LAPACK
-// convention: swaps applied to rows produce L U; ipiv[j] = absolute row
-// (0-based) swapped with row j.  Returns the first zero pivot + 1, or 0.
-template <class T>
-int getrf(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, Workspace& ws);
+enum { PIVOT_PARTIAL = 0, PIVOT_NONE = 1 };
 
-// sigma (device, n) with (S A)[r] = A[sigma[r]] for the swaps ipiv[0..n).
+// In-place LU of an m x n matrix (m >= n) in LAPACK convention: the swaps
+// ipiv[j] (0-based row exchanged with row j) applied to the rows give L U.
+//   PIVOT_PARTIAL: partial pivoting on the first largest |Re|+|Im| (LAPACK getf2).
+//   PIVOT_NONE:    no row exchanges (ipiv[j] = j); *would_swap is set when
+//                  partial pivoting would have exchanged a row, i.e. when the
+//                  factors differ from PIVOT_PARTIAL's.
+// Also writes the inverses of the 32 x 32 diagonal blocks of L and U
+// (ceil(n/32) blocks of 1024 entries each).  Returns the first zero pivot
+// + 1, or 0.
+template <class T>
+int getrf(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, cplx<T>* Linv, cplx<T>* Uinv,
+          int mode, int* would_swap, Workspace& ws);
+// Diagonal-block inverses of existing factors.
+template <class T>
+void block_inverses(int64_t n, MatView<T> LU, cplx<T>* Linv, cplx<T>* Uinv);
+
+// Factors of an n x n matrix.  sigma (device, n) gives (S A)[r] = A[sigma[r]]
+// for the pivoting permutation S, or is NULL when no rows were exchanged.
+template <class T>
+struct LU {
+  MatView<T> A;
+  cplx<T>* Linv;
+  cplx<T>* Uinv;
+  const int32_t* sigma;
+};
+
+// sigma from the swaps ipiv[0..n).
 void perm_from_pivots(int64_t n, const int32_t* ipiv, int32_t* sigma, Workspace& ws);
 
 // Row gather B[r, :] = A[sigma[r], :] for r < n over ncols columns (A != B).
 template <class T>
 void gather_rows(int64_t n, int64_t ncols, const int32_t* sigma, MatView<T> A, MatView<T> B);
 
-// Triangular solves with the factors of getrf (n x n), in place on B:
-//   lower unit from the left  : B <- L^{-1} B        (B n x nrhs)
-//   upper from the left       : B <- U^{-1} B
-//   upper from the right      : B <- B U^{-1}        (B m x n)
-//   lower unit from the right : B <- B L^{-1}
+// B <- A^{-1} B (B n x nrhs); B <- B A^{-1} (B m x n); X = A^{-1}.
 template <class T>
-void trsm_llu(int64_t n, int64_t nrhs, MatView<T> LU, MatView<T> B, Workspace& ws);
+void getrs(int64_t n, int64_t nrhs, const LU<T>& f, MatView<T> B, Workspace& ws);
 template <class T>
-void trsm_lun(int64_t n, int64_t nrhs, MatView<T> LU, MatView<T> B, Workspace& ws);
+void getrs_right(int64_t m, int64_t n, const LU<T>& f, MatView<T> B, Workspace& ws);
 template <class T>
-void trsm_run(int64_t m, int64_t n, MatView<T> LU, MatView<T> B, Workspace& ws);
-template <class T>
-void trsm_rlu(int64_t m, int64_t n, MatView<T> LU, MatView<T> B, Workspace& ws);
-
-// B <- A^{-1} B using getrf factors and sigma (B n x nrhs).
-template <class T>
-void getrs(int64_t n, int64_t nrhs, MatView<T> LU, const int32_t* sigma, MatView<T> B,
-           Workspace& ws);
-// X <- B A^{-1} (B m x n, in place).
-template <class T>
-void getrs_right(int64_t m, int64_t n, MatView<T> LU, const int32_t* sigma, MatView<T> B,
-                 Workspace& ws);
-// X = A^{-1} (n x n) from getrf factors and sigma.
-template <class T>
-void getri(int64_t n, MatView<T> LU, const int32_t* sigma, MatView<T> X, Workspace& ws);
+void getri(int64_t n, const LU<T>& f, MatView<T> X, Workspace& ws);
 
 // Elementwise helpers.
 template <class T>
 void copy_mat(int64_t m, int64_t n, MatView<T> A, MatView<T> B);  // B = A
 template <class Ts, class Td>
 void convert_mat(int64_t m, int64_t n, MatView<Ts> A, MatView<Td> B);
+
+inline int64_t inv_blocks(int64_t n) { return (n + 31) / 32 * 1024; }  // entries per inverse array
 
 }  // namespace mnp

@@ -119,6 +119,35 @@ def test_solves_and_inverse(dtype):
     assert np.max(np.abs(dev.to_host(dX) - Ai)) < tol * np.abs(Ai).max()
 
 
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", [45, 300, 700])
+def test_getrf_nopiv(dtype, n):
+    """Unpivoted LU: equals LAPACK's factors when partial pivoting would not
+    exchange rows, and reports it when it would."""
+    import scipy.linalg as sl
+    import torch
+    from mnpbem_amd import _lib, device as dev
+    dev.ensure()
+    print(f"[getrf_nopiv] {dtype.__name__} n={n}", flush=True)
+    rng = np.random.default_rng(n + 1)
+    tdt = torch.complex128 if dtype == np.complex128 else torch.complex64
+    A = _rand(rng, n, n) + (2.5 * n) * np.eye(n)  # diagonally dominant: no swaps
+    lu_ref, piv_ref = sl.lu_factor(A)
+    assert np.array_equal(piv_ref, np.arange(n))
+    dA = _dev(A, tdt)
+    info, sw = C.c_int(-1), C.c_int(-1)
+    _lib.call("mnp_getrf_nopiv", _prec(dtype), n, n, dev.ptr(dev.base(dA)), n, C.byref(info),
+              C.byref(sw))
+    assert info.value == 0 and sw.value == 0
+    tol = 1e-12 if dtype == np.complex128 else 2e-5
+    assert np.allclose(dev.to_host(dA), lu_ref, rtol=0, atol=tol * np.abs(lu_ref).max())
+    B = _rand(rng, n, n)  # a general matrix needs exchanges
+    dB = _dev(B, tdt)
+    _lib.call("mnp_getrf_nopiv", _prec(dtype), n, n, dev.ptr(dev.base(dB)), n, C.byref(info),
+              C.byref(sw))
+    assert sw.value == 1
+
+
 def test_singular_pivot_reported():
     from mnpbem_amd import device as dev
     dev.ensure()
