@@ -51,7 +51,9 @@ template <class T> __device__ __forceinline__ cplx<T> czero() { return mk<T>(T(0
 // step needs a single barrier.
 constexpr int GBK = 16;
 
-template <class T, int WM, int WN>
+// THREE_M selects the 3-multiplication product: P1 = Ar Br, P2 = Ai Bi,
+// P3 = (Ar + Ai)(Br + Bi), Cr = P1 - P2, Ci = P3 - P1 - P2 (3 MFMA per 4).
+template <class T, int WM, int WN, bool THREE_M>
 __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, cplx<T> alpha,
                                                const cplx<T>* A, int64_t lda,
                                                const cplx<T>* B, int64_t ldb,
@@ -64,15 +66,32 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
   __shared__ T sBr[2][GBK][BN + PAD], sBi[2][GBK][BN + PAD];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int wm = wid & 1, wn = wid >> 1;
-  const int64_t m0 = (int64_t)blockIdx.x * BM, n0 = (int64_t)blockIdx.y * BN;
+  // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+  // L2).  Renumber them so that each XCD gets a contiguous range of tiles,
+  // and walk that range in groups of 8 tile rows so neighbouring workgroups
+  // share A row panels and B column panels in the XCD's L2.
+  int64_t tx, ty;
+  {
+    const int64_t gx = gridDim.x, gy = gridDim.y, tot = gx * gy;
+    const int64_t L = blockIdx.x + gx * (int64_t)blockIdx.y;
+    const int64_t per = tot / 8, rem = tot % 8, xcd = L % 8, idx = L / 8;
+    const int64_t nl = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    constexpr int64_t G = 8;
+    const int64_t group = nl / (G * gy), first = group * G;
+    const int64_t gsz = min(G, gx - first), r = nl % (G * gy);
+    tx = first + r % gsz;
+    ty = r / gsz;
+  }
+  const int64_t m0 = tx * BM, n0 = ty * BN;
 
-  acc_t cr[WM][WN], ci[WM][WN];
+  acc_t cr[WM][WN], ci[WM][WN], c3[THREE_M ? WM : 1][THREE_M ? WN : 1];
 #pragma unroll
   for (int a = 0; a < WM; ++a)
 #pragma unroll
     for (int b = 0; b < WN; ++b) {
       cr[a][b] = acc_t{0, 0, 0, 0};
       ci[a][b] = acc_t{0, 0, 0, 0};
+      if constexpr (THREE_M) c3[a][b] = acc_t{0, 0, 0, 0};
     }
   cplx<T> ra[LA], rb[LB];
   auto loadA = [&](int64_t k0) {
@@ -131,20 +150,32 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
         br[b] = sBr[buf][kr][wn * 16 * WN + b * 16 + cl];
         bi[b] = sBi[buf][kr][wn * 16 * WN + b * 16 + cl];
       }
+      if constexpr (THREE_M) {
+        // cr accumulates P1, ci accumulates P2, c3 accumulates P3
 #pragma unroll
-      for (int a = 0; a < WM; ++a)
+        for (int a = 0; a < WM; ++a)
 #pragma unroll
-        for (int b = 0; b < WN; ++b) {
-          cr[a][b] = MF<T>::mma(ar[a], br[b], cr[a][b]);
-          ci[a][b] = MF<T>::mma(ar[a], bi[b], ci[a][b]);
-        }
+          for (int b = 0; b < WN; ++b) {
+            cr[a][b] = MF<T>::mma(ar[a], br[b], cr[a][b]);
+            ci[a][b] = MF<T>::mma(ai[a], bi[b], ci[a][b]);
+            c3[a][b] = MF<T>::mma(ar[a] + ai[a], br[b] + bi[b], c3[a][b]);
+          }
+      } else {
 #pragma unroll
-      for (int a = 0; a < WM; ++a)
+        for (int a = 0; a < WM; ++a)
 #pragma unroll
-        for (int b = 0; b < WN; ++b) {
-          cr[a][b] = MF<T>::mma(nai[a], bi[b], cr[a][b]);
-          ci[a][b] = MF<T>::mma(ai[a], br[b], ci[a][b]);
-        }
+          for (int b = 0; b < WN; ++b) {
+            cr[a][b] = MF<T>::mma(ar[a], br[b], cr[a][b]);
+            ci[a][b] = MF<T>::mma(ar[a], bi[b], ci[a][b]);
+          }
+#pragma unroll
+        for (int a = 0; a < WM; ++a)
+#pragma unroll
+          for (int b = 0; b < WN; ++b) {
+            cr[a][b] = MF<T>::mma(nai[a], bi[b], cr[a][b]);
+            ci[a][b] = MF<T>::mma(ai[a], br[b], ci[a][b]);
+          }
+      }
     }
     if (more) store(buf ^ 1);
     buf ^= 1;
@@ -158,7 +189,11 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
       for (int r = 0; r < 4; ++r) {
         const int64_t row = m0 + wm * 16 * WM + a * 16 + MF<T>::row(lane, r);
         if (row < M && col < N) {
-          const cplx<T> v = mk<T>(cr[a][b][r], ci[a][b][r]);
+          cplx<T> v;
+          if constexpr (THREE_M)
+            v = mk<T>(cr[a][b][r] - ci[a][b][r], c3[a][b][r] - cr[a][b][r] - ci[a][b][r]);
+          else
+            v = mk<T>(cr[a][b][r], ci[a][b][r]);
           cplx<T> o = cmul<T>(alpha, v);
           cplx<T>* pc = C + col * ldc + row;
           if (!beta_zero) o = cadd<T>(o, cmul<T>(beta, *pc));
@@ -168,7 +203,7 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
     }
 }
 
-template <class T, int WM, int WN>
+template <class T, int WM, int WN, bool THREE_M = false>
 static void gemm_launch(int64_t m, int64_t n, int64_t k, cplx<T> al, MatView<T> A, MatView<T> B,
                         cplx<T> be, bool bz, MatView<T> C) {
   constexpr int BM = 32 * WM, BN = 32 * WN;
@@ -178,9 +213,10 @@ static void gemm_launch(int64_t m, int64_t n, int64_t k, cplx<T> al, MatView<T> 
     const int64_t gy = std::min<int64_t>(65535, ny - y0);
     const int64_t noff = y0 * BN;
     const int64_t ncols = std::min<int64_t>(n - noff, gy * BN);
+    // work = the complex product's 8 m n k real flops, whichever algorithm runs it
     TimerScope ts(sizeof(T) == 8 ? MNP_TIMER_GEMM_FP64 : MNP_TIMER_GEMM_FP32,
                   8.0 * (double)m * (double)ncols * (double)k);
-    hipLaunchKernelGGL((k_cgemm<T, WM, WN>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream(), m,
+    hipLaunchKernelGGL((k_cgemm<T, WM, WN, THREE_M>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream(), m,
                        n - noff, k, al, A.p, A.ld, B.p + noff * B.ld, B.ld, be, (int)bz,
                        C.p + noff * C.ld, C.ld);
   }
@@ -203,7 +239,7 @@ void gemm(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, MatView<
     if constexpr (sizeof(T) == 8)
       gemm_launch<T, 4, 2>(m, n, k, al, A, B, be, bz, C);  // 128 x 64: accumulators fit without spills
     else
-      gemm_launch<T, 4, 4>(m, n, k, al, A, B, be, bz, C);
+      gemm_launch<T, 4, 3, true>(m, n, k, al, A, B, be, bz, C);  // 3M on the FP32 chain
   } else
     gemm_launch<T, 2, 2>(m, n, k, al, A, B, be, bz, C);
 }
