@@ -184,6 +184,64 @@ int mnp_getrs_right(int precision, int64_t m, int64_t n, const void* LU, int64_t
 int mnp_getri(int precision, int64_t n, const void* LU, int64_t lda, const int32_t* ipiv, void* X,
               int64_t ldx);
 
+/* ------------------------------------------------------ H-matrices (ACA)
+ * Replaces hmatrix(tree) filled by aca.compgreenret (Greenfun/+aca/
+ * @compgreenret/eval.m:16-62): full leaf blocks element-wise with
+ * refinement (@hmatrix/fillval.m, @greenret/private/eval2.m), admissible
+ * blocks by partially pivoted ACA of the collocation kernel (MEX
+ * hmatgreenret, mex/hlib/aca.cpp:158-218, mex/acagreen/acagreen.cpp:79-149),
+ * and the H-matrix times matrix product (MEX hmatmul1, mex/hlib/hmatrix.h:
+ * 107-116).  The cluster tree and block tree are built by the caller
+ * (@clustertree, admissibility.m), as in the reference.
+ */
+typedef struct mnp_hmat_s* mnp_hmat_t;
+/* perm[c] = face index of cluster index c (n entries); full and low are
+ * 4 x nblocks (column-major) [row_begin, row_end, col_begin, col_end) in
+ * cluster indices; the blocks must tile the n x n matrix and the full blocks
+ * must include the diagonal leaf blocks.  htol and kmax are the ACA
+ * tolerance and maximum rank (hoptions; kmax <= 128). */
+int mnp_hmat_create(mnp_greenret_t g, const int64_t* perm, int64_t nfull, const int64_t* full,
+                    int64_t nlow, const int64_t* low, double htol, int kmax, mnp_hmat_t* out);
+int mnp_hmat_destroy(mnp_hmat_t h);
+/* Fills G, F, H1 (F + 2 pi) or H2 (F - 2 pi) at wavenumber k (key MNP_KEY_*). */
+int mnp_hmat_fill(mnp_hmat_t h, mnp_complex k, int key);
+/* y = H x (device, complex128, face order, n x nrhs), computed in complex FP32. */
+int mnp_hmat_matvec(mnp_hmat_t h, int64_t nrhs, const void* x, void* y);
+/* Stored entries of full and low-rank blocks, largest rank and compression
+ * (stored entries / n^2) of the last fill (hmatrix/compression.m). */
+int mnp_hmat_stats(mnp_hmat_t h, int64_t* full_entries, int64_t* lowrank_entries, int* max_rank,
+                   double* compression);
+/* Ranks of the low-rank blocks (host, nlow entries) of the last fill. */
+int mnp_hmat_ranks(mnp_hmat_t h, int32_t* ranks);
+
+/* ------------------------------------------ iterative solver (bemretiter)
+ * Replaces bemsolver(p, op) with op.iter set -> bemretiter (BEM/@bemretiter):
+ * initmat fills G1, G2, H1, H2 as H-matrices (private/initmat.m:5-31) and
+ * the preconditioner; solve runs GMRES without restart on the exact
+ * operator afun (private/afun.m:6-35) with left preconditioning
+ * (bemiter/solve.m:33-36, private/mfun.m).  The preconditioner is the
+ * bemret chain applied exactly on the diagonal leaf blocks (leaf clusters
+ * of at most 32 faces), in place of the reference's rank-4 H-matrix chain
+ * (private/initprecond.m).  Block arguments as for mnp_hmat_create. */
+typedef struct mnp_bemretiter_s* mnp_bemretiter_t;
+int mnp_bemretiter_create(mnp_greenret_t g, const int64_t* perm, int64_t nfull, const int64_t* full,
+                          int64_t nlow, const int64_t* low, double htol, int kmax,
+                          mnp_bemretiter_t* out);
+int mnp_bemretiter_destroy(mnp_bemretiter_t it);
+/* GMRES tolerance and iteration limit (bemiter.options: 1e-6, 100); precond 0 disables it. */
+int mnp_bemretiter_set_options(mnp_bemretiter_t it, double tol, int maxit, int precond);
+int mnp_bemretiter_initmat(mnp_bemretiter_t it, double enei, mnp_complex eps1, mnp_complex eps2);
+/* Same arguments as mnp_bemret_solve (npol <= 8). */
+int mnp_bemretiter_solve(mnp_bemretiter_t it, int64_t npol, const void* phi1, const void* phi1p,
+                         const void* a1, const void* a1p, const void* phi2, const void* phi2p,
+                         const void* a2, const void* a2p, void* sig1, void* sig2, void* h1,
+                         void* h2);
+/* Last solve: flag (0 converged, 1 not), preconditioned relative residual,
+ * iterations, true relative residual |b - A x| / |b|; compression of G1
+ * (bemiter/setiter.m, hinfo.m). */
+int mnp_bemretiter_stats(mnp_bemretiter_t it, int* flag, double* relres, int* iter,
+                         double* true_relres, double* compression);
+
 /* --------------------------------------------- plane wave and observables
  * Simulation/retarded/@planewaveret/potential.m:47-88 for a particle whose
  * outside is the embedding medium: a2 and a2p (device out, complex128,

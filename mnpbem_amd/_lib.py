@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmnpbem_hip.so")
 
 MNP_OK = 0
+MNP_ERR_ARG, MNP_ERR_HIP, MNP_ERR_SINGULAR, MNP_ERR_NOCONV, MNP_ERR_INTERNAL, MNP_ERR_STATE = 1, 2, 3, 4, 5, 6
 MNP_FP64, MNP_FP32 = 0, 1
 MNP_KEY = {"G": 0, "F": 1, "H1": 2, "H2": 3}
 MNP_SOLVER_FP64, MNP_SOLVER_MIXED = 0, 1
@@ -66,6 +67,18 @@ PROTOTYPES = {
     "mnp_getrs": (INT, [INT, I64, I64, P, I64, P, P, I64]),
     "mnp_getrs_right": (INT, [INT, I64, I64, P, I64, P, P, I64]),
     "mnp_getri": (INT, [INT, I64, P, I64, P, P, I64]),
+    "mnp_hmat_create": (INT, [P, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
+    "mnp_hmat_destroy": (INT, [P]),
+    "mnp_hmat_fill": (INT, [P, MnpComplex, INT]),
+    "mnp_hmat_matvec": (INT, [P, I64, P, P]),
+    "mnp_hmat_stats": (INT, [P, PI64, PI64, C.POINTER(INT), PD]),
+    "mnp_hmat_ranks": (INT, [P, P]),
+    "mnp_bemretiter_create": (INT, [P, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
+    "mnp_bemretiter_destroy": (INT, [P]),
+    "mnp_bemretiter_set_options": (INT, [P, D, INT, INT]),
+    "mnp_bemretiter_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
+    "mnp_bemretiter_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mnp_bemretiter_stats": (INT, [P, C.POINTER(INT), PD, C.POINTER(INT), PD, PD]),
     "mnp_planewave_potential": (INT, [P, D, MnpComplex, I64, P, P, P, P]),
     "mnp_farfield": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P]),
     "mnp_planewave_cross_sections": (INT, [P, D, MnpComplex, I64, P, P, I64, P, P, P, P,

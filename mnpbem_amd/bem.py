@@ -197,12 +197,92 @@ class bemret:
         return it.value, rr.value
 
 
+def bemiter_options(**kw):
+    """bemiter.options defaults (BEM/@bemiter/options.m:18-44)."""
+    op = dict(solver="gmres", tol=1e-6, maxit=100, restart=None, precond="hmat", output=0,
+              cleaf=200, htol=1e-6, kmax=[4, 100], fadmiss=None)
+    op.update(kw)
+    return op
+
+
+class bemretiter:
+    """Iterative retarded BEM solver with H-matrices (BEM/@bemretiter).
+
+    ``op['iter']`` holds bemiter options (cleaf, htol, kmax, tol, maxit,
+    precond).  GMRES runs on the GPU with the exact operator (four
+    H-matvecs) and a block preconditioner; see mnp_bemretiter_* in
+    include/mnpbem_hip.h.
+    """
+
+    def __init__(self, p, op):
+        from .hmatrix import admissibility, block_ranges, clustertree
+        it = bemiter_options(**op.get("iter", {}))
+        if it["solver"] != "gmres":
+            raise NotImplementedError("only GMRES is implemented")
+        self.p, self.iter = p, it
+        geo = p.geometry()
+        self.tree = clustertree(geo["pos"], it["cleaf"])
+        full, low = admissibility(self.tree, it["fadmiss"])
+        fr = np.ascontiguousarray(block_ranges(self.tree, full).T)
+        lr = np.ascontiguousarray(block_ranges(self.tree, low).T)
+        perm = np.ascontiguousarray(self.tree.perm, dtype=np.int64)
+        kmax = int(np.max(np.atleast_1d(it["kmax"])))
+        htol = float(np.min(np.atleast_1d(it["htol"])))
+        h = C.c_void_p()
+        _lib.call("mnp_bemretiter_create", p.green.handle, perm.ctypes.data_as(C.c_void_p), len(fr),
+                  fr.ctypes.data_as(C.c_void_p), len(lr), lr.ctypes.data_as(C.c_void_p), htol, kmax,
+                  C.byref(h))
+        self.handle = h
+        _lib.call("mnp_bemretiter_set_options", h, float(it["tol"]), int(it["maxit"]),
+                  int(it["precond"] not in (None, "none", False)))
+        self.enei = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib.lib().mnp_bemretiter_destroy(h)
+            self.handle = None
+
+    def __call__(self, enei):
+        if self.enei != enei:
+            e1, e2 = self.p.eps1(enei), self.p.eps2(enei)
+            _lib.call("mnp_bemretiter_initmat", self.handle, float(enei), _lib.MnpComplex.of(e1),
+                      _lib.MnpComplex.of(e2))
+            self.enei = enei
+        return self
+
+    def solve(self, exc):
+        """[sig, bem] = solve(bem, exc)  (BEM/@bemretiter/solve.m)."""
+        self(exc.enei)
+        n = self.p.n
+        npol = next(v.shape[-1] for v in exc.values() if v is not None)
+        args = [None if exc.get(k) is None else dev.ptr(dev.base(exc.get(k)))
+                for k in ("phi1", "phi1p", "a1", "a1p", "phi2", "phi2p", "a2", "a2p")]
+        out = dict(sig1=dev.colmajor((n, npol)), sig2=dev.colmajor((n, npol)),
+                   h1=dev.colmajor((n, 3, npol)), h2=dev.colmajor((n, 3, npol)))
+        _lib.call("mnp_bemretiter_solve", self.handle, npol, *args,
+                  *[dev.ptr(dev.base(out[k])) for k in ("sig1", "sig2", "h1", "h2")])
+        return compstruct(self.p, exc.enei, **out)
+
+    mldivide = solve
+
+    def stats(self):
+        """dict(flag, relres, iter, true_relres, compression) of the last solve."""
+        f, rr, it, tr, cp = C.c_int(), C.c_double(), C.c_int(), C.c_double(), C.c_double()
+        _lib.call("mnp_bemretiter_stats", self.handle, C.byref(f), C.byref(rr), C.byref(it),
+                  C.byref(tr), C.byref(cp))
+        return dict(flag=f.value, relres=rr.value, iter=it.value, true_relres=tr.value,
+                    compression=cp.value)
+
+
 def bemsolver(p, op=None, **kw):
-    """Base/bemsolver.m: picks the retarded dense solver for op.sim == 'ret'."""
+    """Base/bemsolver.m: bemret for op.sim == 'ret', bemretiter when op.iter is set."""
     op = dict(p.op if op is None else op)
     op.update(kw)
     if op.get("sim", "ret") != "ret":
         raise NotImplementedError("only the retarded solver is implemented")
+    if op.get("iter"):
+        return bemretiter(p, op)
     return bemret(p, op)
 
 

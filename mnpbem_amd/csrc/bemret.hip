@@ -277,17 +277,20 @@ __global__ void k_excitation(int64_t n, int64_t p, const double2* phi1, const do
 
 // r = b - A x with the exact operator (afun.m): GX1 = G1 [sig1|h1],
 // GX2 = G2 [sig2|h2], HX1 = H1 [sig1|h1], HX2 = H2 [sig2|h2]  (each n x 4p).
-__global__ void k_residual(int64_t n, int64_t p, const double2* b, const double2* GX1,
+// With b == NULL the kernel writes -A x instead.
+__global__ void k_residual(int64_t n, int64_t p, const double2* b_, const double2* GX1,
                            const double2* GX2, const double2* HX1, const double2* HX2, double2* r,
                            const double* nvec, double k, double2 e1, double2 e2) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t q = blockIdx.y;
   if (i >= n) return;
+  const double2 zero = mk<double>(0, 0);
+  auto b = [&](int64_t idx) { return b_ ? b_[idx] : zero; };
   const double2 ik = mk<double>(0, k);
   const double2 Gs1 = GX1[q * n + i], Gs2 = GX2[q * n + i];
   const double2 Hs1 = HX1[q * n + i], Hs2 = HX2[q * n + i];
   // phi
-  r[q * n + i] = csub<double>(b[q * n + i], csub<double>(Gs1, Gs2));
+  r[q * n + i] = csub<double>(b(q * n + i), csub<double>(Gs1, Gs2));
   const double2 w = csub<double>(cmul<double>(e1, Gs1), cmul<double>(e2, Gs2));
   double2 nGh = mk<double>(0, 0);
   for (int c = 0; c < 3; ++c) {
@@ -295,14 +298,14 @@ __global__ void k_residual(int64_t n, int64_t p, const double2* b, const double2
     const double nc = nvec[c * n + i];
     const double2 Gh1 = GX1[(p + col) * n + i], Gh2 = GX2[(p + col) * n + i];
     const double2 Hh1 = HX1[(p + col) * n + i], Hh2 = HX2[(p + col) * n + i];
-    r[(p + col) * n + i] = csub<double>(b[(p + col) * n + i], csub<double>(Gh1, Gh2));
+    r[(p + col) * n + i] = csub<double>(b((p + col) * n + i), csub<double>(Gh1, Gh2));
     const double2 al = csub<double>(csub<double>(Hh1, Hh2), cmul<double>(ik, cscale<double>(w, nc)));
-    r[(5 * p + col) * n + i] = csub<double>(b[(5 * p + col) * n + i], al);
+    r[(5 * p + col) * n + i] = csub<double>(b((5 * p + col) * n + i), al);
     nGh = cadd<double>(nGh, cscale<double>(csub<double>(cmul<double>(e1, Gh1), cmul<double>(e2, Gh2)), nc));
   }
   const double2 De = csub<double>(csub<double>(cmul<double>(e1, Hs1), cmul<double>(e2, Hs2)),
                                   cmul<double>(ik, nGh));
-  r[(4 * p + q) * n + i] = csub<double>(b[(4 * p + q) * n + i], De);
+  r[(4 * p + q) * n + i] = csub<double>(b((4 * p + q) * n + i), De);
 }
 
 __global__ void k_norm2(int64_t len, const double2* v, double* out) {
@@ -338,6 +341,26 @@ __global__ void k_unpack(int64_t n, int64_t p, const double2* x, double2* sig1, 
   } else {
     if (h2) h2[(col - 5 * p) * n + i] = v;
   }
+}
+
+void launch_excitation(int64_t n, int64_t p, const double2* const in[8], double2* b, const double* nvec,
+                       double k, double2 e1, double2 e2) {
+  hipLaunchKernelGGL(k_excitation, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0, stream(), n, p,
+                     in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], b, nvec, k, e1, e2);
+  MNP_HIP(hipGetLastError());
+}
+void launch_residual(int64_t n, int64_t p, const double2* b, const double2* GX1, const double2* GX2,
+                     const double2* HX1, const double2* HX2, double2* r, const double* nvec, double k,
+                     double2 e1, double2 e2) {
+  hipLaunchKernelGGL(k_residual, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0, stream(), n, p, b,
+                     GX1, GX2, HX1, HX2, r, nvec, k, e1, e2);
+  MNP_HIP(hipGetLastError());
+}
+void launch_unpack(int64_t n, int64_t p, const double2* x, double2* sig1, double2* h1, double2* sig2,
+                   double2* h2) {
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n, p, x,
+                     sig1, h1, sig2, h2);
+  MNP_HIP(hipGetLastError());
 }
 
 struct BemRet {

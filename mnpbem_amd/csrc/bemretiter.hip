@@ -1,0 +1,628 @@
+// Iterative retarded BEM solver with H-matrices (MNPBEM BEM/@bemretiter).
+//
+// initmat (bemretiter/private/initmat.m:5-31): G1 = G(k1), G2 = G(k2),
+// H1 = F(k1) + 2 pi, H2 = F(k2) - 2 pi as H-matrices (full blocks with
+// refinement + ACA blocks, hmatrix.hip), plus the preconditioner.
+// solve (bemretiter/solve.m, bemiter/solve.m): GMRES without restart on
+// the exact operator afun (private/afun.m:6-35, four H-matvecs) with left
+// preconditioning by mfun (private/mfun.m:10-52).
+//
+// Preconditioner: the reference compresses the whole bemret chain into
+// H-matrices of rank 4 (private/initprecond.m, H-LU and H-inverse).  Here
+// the same chain is applied exactly on each diagonal leaf block (block
+// Jacobi): G1, G2, Sigma1, Deltai, Sigma of every leaf cluster are formed
+// and factorised in LDS by one workgroup.  The preconditioner changes the
+// iteration count only; the converged solution solves the same equations.
+#include <complex>
+#include <memory>
+#include <vector>
+
+#include "mnp_hmatrix.hpp"
+
+namespace mnp {
+
+void launch_excitation(int64_t n, int64_t p, const double2* const in[8], double2* b, const double* nvec,
+                       double k, double2 e1, double2 e2);
+void launch_residual(int64_t n, int64_t p, const double2* b, const double2* GX1, const double2* GX2,
+                     const double2* HX1, const double2* HX2, double2* r, const double* nvec, double k,
+                     double2 e1, double2 e2);
+void launch_unpack(int64_t n, int64_t p, const double2* x, double2* sig1, double2* h1, double2* sig2,
+                   double2* h2);
+
+constexpr int PB = 32;  // largest leaf cluster handled by the block preconditioner
+using cf = float2;
+
+// ------------------------------------------------ small dense LDS algebra
+// All matrices are PB x PB row-major in LDS, nb <= PB active rows/cols.
+__device__ void lds_lu(cf (*M)[PB + 1], int nb) {  // in place, no pivoting
+  for (int j = 0; j < nb; ++j) {
+    const cf piv = M[j][j];
+    for (int i = j + 1 + (int)threadIdx.x; i < nb; i += blockDim.x) M[i][j] = cdiv<float>(M[i][j], piv);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
+      const int i = e / nb, c = e % nb;
+      if (i > j && c > j) M[i][c] = csub<float>(M[i][c], cmul<float>(M[i][j], M[j][c]));
+    }
+    __syncthreads();
+  }
+}
+// X <- X (L U)^{-1}  (rows of X independent; X nb x nb)
+__device__ void lds_rsolve(cf (*X)[PB + 1], cf (*LU)[PB + 1], int nb) {
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    for (int j = 0; j < nb; ++j) {  // X U = Y
+      cf s = X[i][j];
+      for (int k = 0; k < j; ++k) s = csub<float>(s, cmul<float>(X[i][k], LU[k][j]));
+      X[i][j] = cdiv<float>(s, LU[j][j]);
+    }
+    for (int j = nb - 1; j >= 0; --j) {  // Z L = X
+      cf s = X[i][j];
+      for (int k = j + 1; k < nb; ++k) s = csub<float>(s, cmul<float>(X[i][k], LU[k][j]));
+      X[i][j] = s;
+    }
+  }
+  __syncthreads();
+}
+// columns of V (nb x ncol, column c at V[.][c]) <- (L U)^{-1} V
+__device__ void lds_lsolve(cf (*V)[2 * PB + 1], int ncol, cf (*LU)[PB + 1], int nb) {
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+    for (int i = 0; i < nb; ++i) {
+      cf s = V[i][c];
+      for (int k = 0; k < i; ++k) s = csub<float>(s, cmul<float>(LU[i][k], V[k][c]));
+      V[i][c] = s;
+    }
+    for (int i = nb - 1; i >= 0; --i) {
+      cf s = V[i][c];
+      for (int k = i + 1; k < nb; ++k) s = csub<float>(s, cmul<float>(LU[i][k], V[k][c]));
+      V[i][c] = cdiv<float>(s, LU[i][i]);
+    }
+  }
+  __syncthreads();
+}
+
+// Per leaf: LU(G1), LU(G2), Sigma1, Deltai, LU(Sigma) (5 x PB x PB, FP32),
+// the bemret chain (initmat.m) on the diagonal block.
+__global__ void __launch_bounds__(256) k_bjprec_build(int64_t nleaf, const int64_t* lrow, const int32_t* diag,
+                                                      const int64_t* voff, const float2* G1v, const float2* G2v,
+                                                      const float2* H1v, const float2* H2v, const int32_t* perm,
+                                                      const double* nvec, int64_t n, double2 e1, double2 e2,
+                                                      double2 cc, float2* out) {
+  __shared__ cf A[PB][PB + 1], B[PB][PB + 1], S1[PB][PB + 1], S2[PB][PB + 1], D[PB][PB + 1], X[PB][PB + 1];
+  const int64_t l = blockIdx.x;
+  const int r0 = (int)lrow[2 * l], nb = (int)(lrow[2 * l + 1] - lrow[2 * l]);
+  const int64_t vo = voff[diag[l]];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < PB * PB; e += 256) {
+    const int i = e % PB, j = e / PB;
+    const bool in = i < nb && j < nb;
+    const cf id = i == j ? make_float2(1.f, 0.f) : make_float2(0.f, 0.f);
+    A[i][j] = in ? G1v[vo + (int64_t)j * nb + i] : id;
+    B[i][j] = in ? G2v[vo + (int64_t)j * nb + i] : id;
+    S1[i][j] = in ? H1v[vo + (int64_t)j * nb + i] : id;
+    S2[i][j] = in ? H2v[vo + (int64_t)j * nb + i] : id;
+  }
+  __syncthreads();
+  lds_lu(A, nb);
+  lds_lu(B, nb);
+  lds_rsolve(S1, A, nb);  // Sigma1 = H1 G1^{-1}
+  lds_rsolve(S2, B, nb);  // Sigma2 = H2 G2^{-1}
+  for (int e = tid; e < PB * PB; e += 256) {
+    const int i = e % PB, j = e / PB;
+    D[i][j] = csub<float>(S1[i][j], S2[i][j]);
+    X[i][j] = i == j ? make_float2(1.f, 0.f) : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  lds_lu(D, nb);
+  // Deltai = D^{-1}: X = I, rows of X^T ... solve D Y = I column by column
+  for (int c = tid; c < nb; c += 256) {
+    for (int i = 0; i < nb; ++i) {
+      cf s = X[i][c];
+      for (int k = 0; k < i; ++k) s = csub<float>(s, cmul<float>(D[i][k], X[k][c]));
+      X[i][c] = s;
+    }
+    for (int i = nb - 1; i >= 0; --i) {
+      cf s = X[i][c];
+      for (int k = i + 1; k < nb; ++k) s = csub<float>(s, cmul<float>(D[i][k], X[k][c]));
+      X[i][c] = cdiv<float>(s, D[i][i]);
+    }
+  }
+  __syncthreads();
+  // Sigma = e1 Sigma1 - e2 Sigma2 + cc (Deltai o n n^T), into S2
+  for (int e = tid; e < nb * nb; e += 256) {
+    const int i = e % nb, j = e / nb;
+    const int64_t fi = perm[r0 + i], fj = perm[r0 + j];
+    const double nn = nvec[fi] * nvec[fj] + nvec[n + fi] * nvec[n + fj] + nvec[2 * n + fi] * nvec[2 * n + fj];
+    const double2 a = cmul<double>(e1, mk<double>(S1[i][j].x, S1[i][j].y));
+    const double2 b = cmul<double>(e2, mk<double>(S2[i][j].x, S2[i][j].y));
+    const double2 d = cmul<double>(cc, mk<double>(X[i][j].x * nn, X[i][j].y * nn));
+    S2[i][j] = make_float2((float)(a.x - b.x + d.x), (float)(a.y - b.y + d.y));
+  }
+  __syncthreads();
+  lds_lu(S2, nb);
+  float2* o = out + l * 5 * PB * PB;
+  for (int e = tid; e < PB * PB; e += 256) {
+    const int i = e / PB, j = e % PB;
+    o[e] = A[i][j];
+    o[PB * PB + e] = B[i][j];
+    o[2 * PB * PB + e] = S1[i][j];
+    o[3 * PB * PB + e] = X[i][j];
+    o[4 * PB * PB + e] = S2[i][j];
+  }
+}
+
+// x = M^{-1} b on one leaf's rows (mfun.m:27-52 with the block matrices).
+// b, x: n x 8p FP64 in face order, [phi | a | De | alpha] -> [sig1 | h1 | sig2 | h2].
+__global__ void __launch_bounds__(256) k_bjprec_apply(const int64_t* lrow, const int32_t* perm,
+                                                      const float2* fac, const double* nvec, int64_t n, int p,
+                                                      double k, double2 L1, double2 L2, const double2* b,
+                                                      double2* x) {
+  __shared__ cf M[PB][PB + 1];
+  __shared__ cf V[PB][2 * PB + 1];   // up to 8p <= 64 vector columns (p <= 8)
+  __shared__ cf W[PB][2 * PB + 1];
+  const int64_t l = blockIdx.x;
+  const int r0 = (int)lrow[2 * l], nb = (int)(lrow[2 * l + 1] - lrow[2 * l]);
+  const float2* F = fac + l * 5 * PB * PB;
+  const int tid = threadIdx.x, nc = 8 * p;
+  auto loadM = [&](int which) {
+    __syncthreads();
+    for (int e = tid; e < PB * PB; e += 256) M[e / PB][e % PB] = F[which * PB * PB + e];
+    __syncthreads();
+  };
+  // V = local b (rows of this leaf)
+  for (int e = tid; e < nb * nc; e += 256) {
+    const int i = e % nb, c = e / nb;
+    const double2 v = b[(int64_t)c * n + perm[r0 + i]];
+    V[i][c] = make_float2((float)v.x, (float)v.y);
+  }
+  const double2 ik = mk<double>(0, k);
+  const double2 ikL1 = cmul<double>(ik, L1), ikdL = cmul<double>(ik, csub<double>(L1, L2));
+  auto nv = [&](int i, int c) { return (float)nvec[(int64_t)c * n + perm[r0 + i]]; };
+  auto mul = [&](const cf z, double2 s) {
+    const double2 r = cmul<double>(mk<double>(z.x, z.y), s);
+    return make_float2((float)r.x, (float)r.y);
+  };
+  // W[:, 0:4p] = Sigma1 [a | phi]  (a = V cols p..4p, phi = V cols 0..p)
+  loadM(2);
+  for (int e = tid; e < nb * 4 * p; e += 256) {
+    const int i = e % nb, c = e / nb;
+    const int src = c < 3 * p ? p + c : c - 3 * p;
+    cf s = make_float2(0.f, 0.f);
+    for (int kk = 0; kk < nb; ++kk) s = cadd<float>(s, cmul<float>(M[i][kk], V[kk][src]));
+    W[i][c] = s;
+  }
+  __syncthreads();
+  // alpha' = alpha - Ya + ik L1 n phi (V cols 5p..8p), De' = De - L1 Yphi + ik L1 n.a (V col 4p+q)
+  for (int e = tid; e < nb * p; e += 256) {
+    const int i = e % nb, q = e / nb;
+    const cf phi = V[i][q];
+    const cf w = mul(phi, ikL1);
+    cf na = make_float2(0.f, 0.f);
+    for (int c = 0; c < 3; ++c) {
+      const int col = c + 3 * q;
+      const float n_c = nv(i, c);
+      V[i][5 * p + col] = cadd<float>(csub<float>(V[i][5 * p + col], W[i][col]), make_float2(n_c * w.x, n_c * w.y));
+      na = cadd<float>(na, make_float2(n_c * V[i][p + col].x, n_c * V[i][p + col].y));
+    }
+    V[i][4 * p + q] = cadd<float>(csub<float>(V[i][4 * p + q], mul(W[i][3 * p + q], L1)), mul(na, ikL1));
+  }
+  // t = Deltai alpha' -> W[:, 0:3p]
+  loadM(3);
+  for (int e = tid; e < nb * 3 * p; e += 256) {
+    const int i = e % nb, c = e / nb;
+    cf s = make_float2(0.f, 0.f);
+    for (int kk = 0; kk < nb; ++kk) s = cadd<float>(s, cmul<float>(M[i][kk], V[kk][5 * p + c]));
+    W[i][c] = s;
+  }
+  __syncthreads();
+  // rhs = De' + ik (L1-L2) n.t -> W[:, 3p+q]
+  for (int e = tid; e < nb * p; e += 256) {
+    const int i = e % nb, q = e / nb;
+    cf s = make_float2(0.f, 0.f);
+    for (int c = 0; c < 3; ++c) {
+      const float n_c = nv(i, c);
+      s = cadd<float>(s, make_float2(n_c * W[i][c + 3 * q].x, n_c * W[i][c + 3 * q].y));
+    }
+    W[i][3 * p + q] = cadd<float>(V[i][4 * p + q], mul(s, ikdL));
+  }
+  loadM(4);
+  {  // sig2 = Sigma^{-1} rhs, in W[:, 3p .. 4p]
+    for (int c = tid; c < p; c += 256) {
+      const int col = 3 * p + c;
+      for (int i = 0; i < nb; ++i) {
+        cf s = W[i][col];
+        for (int kk = 0; kk < i; ++kk) s = csub<float>(s, cmul<float>(M[i][kk], W[kk][col]));
+        W[i][col] = s;
+      }
+      for (int i = nb - 1; i >= 0; --i) {
+        cf s = W[i][col];
+        for (int kk = i + 1; kk < nb; ++kk) s = csub<float>(s, cmul<float>(M[i][kk], W[kk][col]));
+        W[i][col] = cdiv<float>(s, M[i][i]);
+      }
+    }
+  }
+  __syncthreads();
+  // u = ik (L1-L2) n sig2 + alpha' -> W[:, 4p .. 7p]
+  for (int e = tid; e < nb * p; e += 256) {
+    const int i = e % nb, q = e / nb;
+    const cf w = mul(W[i][3 * p + q], ikdL);
+    for (int c = 0; c < 3; ++c) {
+      const float n_c = nv(i, c);
+      W[i][4 * p + c + 3 * q] = cadd<float>(V[i][5 * p + c + 3 * q], make_float2(n_c * w.x, n_c * w.y));
+    }
+  }
+  loadM(3);
+  // h2 = Deltai u -> W[:, 0:3p]  (t no longer needed)
+  for (int e = tid; e < nb * 3 * p; e += 256) {
+    const int i = e % nb, c = e / nb;
+    cf s = make_float2(0.f, 0.f);
+    for (int kk = 0; kk < nb; ++kk) s = cadd<float>(s, cmul<float>(M[i][kk], W[kk][4 * p + c]));
+    W[i][c] = s;
+  }
+  __syncthreads();
+  // V1 = [sig2 + phi | h2 + a] -> V cols 0..4p ; V2 = [sig2 | h2] -> V cols 4p..8p
+  for (int e = tid; e < nb * 4 * p; e += 256) {
+    const int i = e % nb, c = e / nb;
+    const cf src = c < p ? W[i][3 * p + c] : W[i][c - p];
+    V[i][c] = cadd<float>(src, V[i][c]);
+    V[i][4 * p + c] = src;
+  }
+  __syncthreads();
+  // G1^{-1} on V cols 0..4p, G2^{-1} on V cols 4p..8p
+  loadM(0);
+  for (int c = tid; c < 4 * p; c += 256) {
+    for (int i = 0; i < nb; ++i) {
+      cf s = V[i][c];
+      for (int kk = 0; kk < i; ++kk) s = csub<float>(s, cmul<float>(M[i][kk], V[kk][c]));
+      V[i][c] = s;
+    }
+    for (int i = nb - 1; i >= 0; --i) {
+      cf s = V[i][c];
+      for (int kk = i + 1; kk < nb; ++kk) s = csub<float>(s, cmul<float>(M[i][kk], V[kk][c]));
+      V[i][c] = cdiv<float>(s, M[i][i]);
+    }
+  }
+  loadM(1);
+  for (int c = 4 * p + tid; c < 8 * p; c += 256) {
+    for (int i = 0; i < nb; ++i) {
+      cf s = V[i][c];
+      for (int kk = 0; kk < i; ++kk) s = csub<float>(s, cmul<float>(M[i][kk], V[kk][c]));
+      V[i][c] = s;
+    }
+    for (int i = nb - 1; i >= 0; --i) {
+      cf s = V[i][c];
+      for (int kk = i + 1; kk < nb; ++kk) s = csub<float>(s, cmul<float>(M[i][kk], V[kk][c]));
+      V[i][c] = cdiv<float>(s, M[i][i]);
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < nb * nc; e += 256) {
+    const int i = e % nb, c = e / nb;
+    x[(int64_t)c * n + perm[r0 + i]] = mk<double>(V[i][c].x, V[i][c].y);
+  }
+}
+
+// ------------------------------------------------------- GMRES kernels
+// part[c + nchunk * j] = sum over chunk c of conj(V_j) w
+__global__ void __launch_bounds__(256) k_mdot(int64_t len, const double2* V, int64_t ldv, const double2* w,
+                                              double2* part) {
+  const int64_t c = blockIdx.x, j = blockIdx.y, nchunk = gridDim.x;
+  const double2* v = V + j * ldv;
+  double sr = 0, si = 0;
+  for (int64_t i = c * blockDim.x + threadIdx.x; i < len; i += nchunk * blockDim.x) {
+    const double2 a = v[i], b = w[i];
+    sr += a.x * b.x + a.y * b.y;
+    si += a.x * b.y - a.y * b.x;
+  }
+  __shared__ double rr[256], ri[256];
+  rr[threadIdx.x] = sr;
+  ri[threadIdx.x] = si;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      rr[threadIdx.x] += rr[threadIdx.x + s];
+      ri[threadIdx.x] += ri[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[c + nchunk * j] = mk<double>(rr[0], ri[0]);
+}
+// w <- w - sum_j V_j h_j  (h on the device, nj vectors)
+__global__ void k_msub(int64_t len, const double2* V, int64_t ldv, const double2* h, int nj, double2* w) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double2 s = w[i];
+  for (int j = 0; j < nj; ++j) s = csub<double>(s, cmul<double>(V[j * ldv + i], h[j]));
+  w[i] = s;
+}
+// x = sum_j V_j y_j
+__global__ void k_mcomb(int64_t len, const double2* V, int64_t ldv, const double2* y, int nj, double2* x) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double2 s = mk<double>(0, 0);
+  for (int j = 0; j < nj; ++j) s = cadd<double>(s, cmul<double>(V[j * ldv + i], y[j]));
+  x[i] = s;
+}
+__global__ void k_scale2(int64_t len, double2* v, double a) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < len) v[i] = cscale<double>(v[i], a);
+}
+
+// ------------------------------------------------------------ solver
+struct BemRetIter {
+  const GreenRet* g;
+  int64_t n;
+  std::unique_ptr<HMat> G1, G2, H1, H2;
+  float2* prec = nullptr;  // nleaf x 5 x PB x PB
+  bool use_prec = true;
+  double tol = 1e-6;
+  int maxit = 100;
+  double enei = 0, k0 = 0;
+  double2 eps1{}, eps2{};
+  bool ready = false;
+  int last_iter = 0, last_flag = 0;
+  double last_relres = 0, last_true_relres = 0;
+  Workspace ws, hws;
+  int64_t* d_lrow = nullptr;
+
+  BemRetIter(const GreenRet* g_, const int64_t* perm, int64_t nfull, const int64_t* full, int64_t nlow,
+             const int64_t* low, double htol, int kmax)
+      : g(g_), n(g_->n) {
+    G1.reset(new HMat(g, perm, nfull, full, nlow, low, htol, kmax));
+    G2.reset(new HMat(g, perm, nfull, full, nlow, low, htol, kmax));
+    H1.reset(new HMat(g, perm, nfull, full, nlow, low, htol, kmax));
+    H2.reset(new HMat(g, perm, nfull, full, nlow, low, htol, kmax));
+    for (int64_t l = 0; l < G1->nleaf; ++l) {
+      MNP_CHECK(G1->diag_h[l] >= 0, MNP_ERR_ARG, "bemretiter: a leaf cluster has no diagonal full block");
+      MNP_CHECK(G1->lrow_h[2 * l + 1] - G1->lrow_h[2 * l] <= PB, MNP_ERR_ARG,
+                "bemretiter: the block preconditioner needs leaf clusters of at most 32 faces (cleaf <= 32)");
+    }
+    prec = dev_alloc<float2>(G1->nleaf * 5 * PB * PB);
+    int32_t* diag = dev_alloc<int32_t>(G1->nleaf);
+    MNP_HIP(hipMemcpy(diag, G1->diag_h.data(), sizeof(int32_t) * G1->nleaf, hipMemcpyHostToDevice));
+    d_diag = diag;
+  }
+  ~BemRetIter() {
+    if (prec) (void)hipFree(prec);
+    if (d_diag) (void)hipFree(d_diag);
+  }
+  int32_t* d_diag = nullptr;
+
+  void initmat(double enei_, double2 e1, double2 e2) {
+    enei = enei_;
+    eps1 = e1;
+    eps2 = e2;
+    k0 = 2 * M_PI / enei;
+    auto ksq = [&](double2 e) {
+      const double r = std::hypot(e.x, e.y);
+      double sr = std::sqrt(0.5 * (r + e.x)), si = std::sqrt(0.5 * (r - e.x));
+      if (e.y < 0) si = -si;
+      return mk<double>(k0 * sr, k0 * si);
+    };
+    const double2 k1 = ksq(e1), k2 = ksq(e2);
+    G1->fill(k1, MNP_KEY_G);
+    H1->fill(k1, MNP_KEY_H1);
+    G2->fill(k2, MNP_KEY_G);
+    H2->fill(k2, MNP_KEY_H2);
+    const double2 L = csub<double>(e1, e2);
+    const double2 cc = cscale<double>(cmul<double>(L, L), k0 * k0);
+    hipLaunchKernelGGL(k_bjprec_build, dim3((unsigned)G1->nleaf), dim3(256), 0, stream(), G1->nleaf, G1->d_lrow,
+                       d_diag, G1->d_voff, G1->fvals, G2->fvals, H1->fvals, H2->fvals, G1->perm, g->geo.nvec, n,
+                       e1, e2, cc, prec);
+    MNP_HIP(hipGetLastError());
+    ready = true;
+  }
+
+  // y = A x (afun.m), x, y n x 8p
+  void afun(int64_t p, const double2* x, double2* y, double2* prod) {
+    double2 *GX1 = prod, *GX2 = prod + 4 * p * n, *HX1 = prod + 8 * p * n, *HX2 = prod + 12 * p * n;
+    G1->matvec(4 * p, x, GX1, hws);
+    G2->matvec(4 * p, x + 4 * p * n, GX2, hws);
+    H1->matvec(4 * p, x, HX1, hws);
+    H2->matvec(4 * p, x + 4 * p * n, HX2, hws);
+    launch_residual(n, p, nullptr, GX1, GX2, HX1, HX2, y, g->geo.nvec, k0, eps1, eps2);  // y = -A x
+    const int64_t len = 8 * p * n;
+    hipLaunchKernelGGL(k_scale2, dim3((unsigned)cdiv_up(len, 256)), dim3(256), 0, stream(), len, y, -1.0);
+  }
+  void mfun(int64_t p, const double2* b, double2* x) {
+    if (!use_prec) {
+      MNP_HIP(hipMemcpyAsync(x, b, sizeof(double2) * 8 * p * n, hipMemcpyDeviceToDevice, stream()));
+      return;
+    }
+    hipLaunchKernelGGL(k_bjprec_apply, dim3((unsigned)G1->nleaf), dim3(256), 0, stream(), G1->d_lrow, G1->perm,
+                       prec, g->geo.nvec, n, (int)p, k0, eps1, eps2, b, x);
+    MNP_HIP(hipGetLastError());
+  }
+
+  std::vector<std::complex<double>> mdot(int64_t len, const double2* V, int nj, const double2* w) {
+    const int nchunk = 128;
+    double2* part = (double2*)ws.get(3, sizeof(double2) * nchunk * (size_t)nj);
+    hipLaunchKernelGGL(k_mdot, dim3(nchunk, nj), dim3(256), 0, stream(), len, V, len, w, part);
+    std::vector<double2> h(nchunk * (size_t)nj);
+    MNP_HIP(hipMemcpyAsync(h.data(), part, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipStreamSynchronize(stream()));
+    std::vector<std::complex<double>> r(nj);
+    for (int j = 0; j < nj; ++j)
+      for (int c = 0; c < nchunk; ++c) r[j] += std::complex<double>(h[c + (size_t)nchunk * j].x, h[c + (size_t)nchunk * j].y);
+    return r;
+  }
+
+  void solve(int64_t p, const double2* const in[8], double2* sig1, double2* sig2, double2* h1, double2* h2) {
+    MNP_CHECK(ready, MNP_ERR_STATE, "bemretiter_solve: initmat not called");
+    MNP_CHECK(p >= 1 && p <= 8, MNP_ERR_ARG, "bemretiter_solve: npol must be in 1..8");
+    const int64_t len = 8 * p * n;
+    const int m = std::max(1, std::min<int>(maxit, (int)len));
+    double2* b = (double2*)ws.get(0, sizeof(double2) * len);
+    double2* V = (double2*)ws.get(1, sizeof(double2) * len * (m + 1));
+    double2* Ax = (double2*)ws.get(2, sizeof(double2) * len);
+    double2* prod = (double2*)ws.get(6, sizeof(double2) * 16 * p * n);
+    double2* w = (double2*)ws.get(4, sizeof(double2) * len);
+    double2* hbuf = (double2*)ws.get(5, sizeof(double2) * (m + 2));
+    launch_excitation(n, p, in, b, g->geo.nvec, k0, eps1, eps2);
+    auto norm = [&](const double2* v) { return std::sqrt(std::max(0.0, mdot(len, v, 1, v)[0].real())); };
+    const unsigned gl = (unsigned)cdiv_up(len, 256);
+    // r0 = M^{-1} b (x0 = 0)
+    mfun(p, b, V);
+    const double beta = norm(V);
+    using C = std::complex<double>;
+    std::vector<std::vector<C>> Hm(m + 1, std::vector<C>(m, 0.0));
+    std::vector<C> cs(m), sn(m), gv(m + 1, 0.0);
+    gv[0] = beta;
+    int it = 0;
+    double res = beta;
+    last_flag = 0;
+    if (beta > 0) {
+      hipLaunchKernelGGL(k_scale2, dim3(gl), dim3(256), 0, stream(), len, V, 1.0 / beta);
+      for (it = 0; it < m; ++it) {
+        double2* vj = V + (int64_t)it * len;
+        afun(p, vj, Ax, prod);
+        mfun(p, Ax, w);
+        std::vector<C> h(it + 2, 0.0);
+        for (int pass = 0; pass < 2; ++pass) {  // classical Gram-Schmidt, twice
+          std::vector<C> d = mdot(len, V, it + 1, w);
+          std::vector<double2> dh(it + 1);
+          for (int j = 0; j <= it; ++j) {
+            h[j] += d[j];
+            dh[j] = mk<double>(d[j].real(), d[j].imag());
+          }
+          MNP_HIP(hipMemcpyAsync(hbuf, dh.data(), sizeof(double2) * (it + 1), hipMemcpyHostToDevice, stream()));
+          hipLaunchKernelGGL(k_msub, dim3(gl), dim3(256), 0, stream(), len, V, len, hbuf, it + 1, w);
+          MNP_HIP(hipGetLastError());
+        }
+        const double hn = norm(w);
+        h[it + 1] = hn;
+        // Givens rotations on column it of the Hessenberg matrix
+        for (int j = 0; j < it; ++j) {
+          const C t = std::conj(cs[j]) * h[j] + std::conj(sn[j]) * h[j + 1];
+          h[j + 1] = -sn[j] * h[j] + cs[j] * h[j + 1];
+          h[j] = t;
+        }
+        const double a = std::abs(h[it]), bb = std::abs(h[it + 1]);
+        const double rr = std::hypot(a, bb);
+        cs[it] = rr > 0 ? h[it] / rr : C(1.0);
+        sn[it] = rr > 0 ? h[it + 1] / rr : C(0.0);
+        h[it] = rr;
+        h[it + 1] = 0.0;
+        gv[it + 1] = -sn[it] * gv[it];
+        gv[it] = std::conj(cs[it]) * gv[it];
+        for (int j = 0; j <= it; ++j) Hm[j][it] = h[j];
+        res = std::abs(gv[it + 1]);
+        if (hn > 0 && it + 1 < m) {
+          MNP_HIP(hipMemcpyAsync(V + (int64_t)(it + 1) * len, w, sizeof(double2) * len, hipMemcpyDeviceToDevice,
+                                 stream()));
+          hipLaunchKernelGGL(k_scale2, dim3(gl), dim3(256), 0, stream(), len, V + (int64_t)(it + 1) * len, 1.0 / hn);
+        }
+        if (res <= tol * beta || hn == 0) {
+          ++it;
+          break;
+        }
+      }
+      if (it > m) it = m;
+      // y = R^{-1} g
+      std::vector<C> y(it, 0.0);
+      for (int i = it - 1; i >= 0; --i) {
+        C s = gv[i];
+        for (int j = i + 1; j < it; ++j) s -= Hm[i][j] * y[j];
+        y[i] = s / Hm[i][i];
+      }
+      std::vector<double2> dy(it);
+      for (int j = 0; j < it; ++j) dy[j] = mk<double>(y[j].real(), y[j].imag());
+      MNP_HIP(hipMemcpyAsync(hbuf, dy.data(), sizeof(double2) * it, hipMemcpyHostToDevice, stream()));
+      hipLaunchKernelGGL(k_mcomb, dim3(gl), dim3(256), 0, stream(), len, V, len, hbuf, it, w);
+      MNP_HIP(hipGetLastError());
+    } else {
+      MNP_HIP(hipMemsetAsync(w, 0, sizeof(double2) * len, stream()));
+    }
+    last_iter = it;
+    last_relres = beta > 0 ? res / beta : 0;
+    if (last_relres > tol) last_flag = 1;
+    // true relative residual |b - A x| / |b|
+    afun(p, w, Ax, prod);
+    double2* r = V;  // reuse the first basis vector
+    MNP_HIP(hipMemcpyAsync(r, b, sizeof(double2) * len, hipMemcpyDeviceToDevice, stream()));
+    {
+      std::vector<double2> one{mk<double>(1, 0)};
+      MNP_HIP(hipMemcpyAsync(hbuf, one.data(), sizeof(double2), hipMemcpyHostToDevice, stream()));
+      hipLaunchKernelGGL(k_msub, dim3(gl), dim3(256), 0, stream(), len, Ax, len, hbuf, 1, r);
+    }
+    const double bn = norm(b);
+    last_true_relres = bn > 0 ? norm(r) / bn : 0;
+    launch_unpack(n, p, w, sig1, h1, sig2, h2);
+  }
+};
+
+}  // namespace mnp
+
+// ============================================================== C ABI
+using namespace mnp;
+
+struct mnp_bemretiter_s {
+  BemRetIter* b;
+};
+
+extern "C" {
+
+int mnp_bemretiter_create(mnp_greenret_t g, const int64_t* perm, int64_t nfull, const int64_t* full,
+                          int64_t nlow, const int64_t* low, double htol, int kmax, mnp_bemretiter_t* out) {
+  return guarded([&] {
+    MNP_CHECK(g && perm && out, MNP_ERR_ARG, "bemretiter_create: NULL argument");
+    auto* h = new mnp_bemretiter_s{nullptr};
+    try {
+      h->b = new BemRetIter(g->g, perm, nfull, full, nlow, low, htol, kmax);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int mnp_bemretiter_destroy(mnp_bemretiter_t it) {
+  return guarded([&] {
+    if (!it) return;
+    delete it->b;
+    delete it;
+  });
+}
+
+int mnp_bemretiter_set_options(mnp_bemretiter_t it, double tol, int maxit, int precond) {
+  return guarded([&] {
+    MNP_CHECK(it && tol > 0 && maxit >= 1, MNP_ERR_ARG, "bemretiter_set_options: bad argument");
+    it->b->tol = tol;
+    it->b->maxit = maxit;
+    it->b->use_prec = precond != 0;
+  });
+}
+
+int mnp_bemretiter_initmat(mnp_bemretiter_t it, double enei, mnp_complex eps1, mnp_complex eps2) {
+  return guarded([&] {
+    MNP_CHECK(it && enei > 0, MNP_ERR_ARG, "bemretiter_initmat: bad argument");
+    it->b->initmat(enei, to_d2(eps1), to_d2(eps2));
+  });
+}
+
+int mnp_bemretiter_solve(mnp_bemretiter_t it, int64_t npol, const void* phi1, const void* phi1p,
+                         const void* a1, const void* a1p, const void* phi2, const void* phi2p,
+                         const void* a2, const void* a2p, void* sig1, void* sig2, void* h1, void* h2) {
+  return guarded([&] {
+    MNP_CHECK(it, MNP_ERR_ARG, "bemretiter_solve: NULL handle");
+    const double2* in[8] = {(const double2*)phi1, (const double2*)phi1p, (const double2*)a1,
+                            (const double2*)a1p,  (const double2*)phi2,  (const double2*)phi2p,
+                            (const double2*)a2,   (const double2*)a2p};
+    it->b->solve(npol, in, (double2*)sig1, (double2*)sig2, (double2*)h1, (double2*)h2);
+  });
+}
+
+int mnp_bemretiter_stats(mnp_bemretiter_t it, int* flag, double* relres, int* iter, double* true_relres,
+                         double* compression) {
+  return guarded([&] {
+    MNP_CHECK(it, MNP_ERR_ARG, "bemretiter_stats: NULL handle");
+    const BemRetIter& B = *it->b;
+    if (flag) *flag = B.last_flag;
+    if (relres) *relres = B.last_relres;
+    if (iter) *iter = B.last_iter;
+    if (true_relres) *true_relres = B.last_true_relres;
+    if (compression)
+      *compression = B.G1->filled ? (double)(B.G1->nfullvals + B.G1->nlowvals) / ((double)B.n * B.n) : 0.0;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,611 @@
+// H-matrices of the retarded Green function on gfx950 (MNPBEM Greenfun/+aca,
+// Greenfun/hmatrices, mex/hmatgreenret.cpp, mex/hlib).
+//
+// The block structure (cluster order, full leaf blocks, admissible blocks)
+// comes from the host, as in the reference where Matlab builds the tree and
+// hands it to the MEX kernels.  On the GPU:
+//  - k_hfull: full blocks element-wise, with the refined near-field/self
+//    entries (greenret eval2) and +-2 pi on coincident centroids (H1/H2);
+//  - k_aca:   one workgroup per admissible block runs the partially pivoted
+//    ACA of mex/hlib/aca.cpp:158-218 on the collocation kernel of
+//    mex/acagreen/acagreen.cpp:79-127, in FP64, rank <= min(kmax, m, n);
+//  - k_hmv_low / k_hmv_rows: y = H x in complex FP32: t_b = R_b^T x per
+//    low-rank block, then one workgroup per leaf row cluster sums its full
+//    blocks and L_b t_b of every low-rank block covering its rows
+//    (deterministic, no atomics).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mnp_hmatrix.hpp"
+
+namespace mnp {
+
+// ------------------------------------------------------------ kernels
+// Collocation kernel between cluster indices i (row) and j (column).
+__device__ __forceinline__ double2 colloc(const double* cpos, const double* cnvec, const double* carea,
+                                          int64_t n, int64_t i, int64_t j, double2 k, int isF) {
+  const double x = cpos[i] - cpos[j], y = cpos[n + i] - cpos[n + j], z = cpos[2 * n + i] - cpos[2 * n + j];
+  const double d = sqrt(x * x + y * y + z * z);
+  const double2 fac = cexp_ikd(k, d);
+  const double a = carea[j];
+  if (!isF) return mk<double>(fac.x / d * a, fac.y / d * a);
+  const double in = x * cnvec[i] + y * cnvec[n + i] + z * cnvec[2 * n + i];
+  const double2 t = mk<double>(-k.y - 1.0 / d, k.x);  // i k - 1/d
+  const double2 v = cmul<double>(t, fac);
+  const double s = in / (d * d) * a;
+  return mk<double>(v.x * s, v.y * s);
+}
+
+// Full blocks: entry (i, j) of block b -> faces fi = perm[r0+i], fj = perm[c0+j].
+template <class TS>
+__global__ void __launch_bounds__(256) k_hfull(int64_t nb, const int32_t* blk, const int64_t* voff,
+                                               const int32_t* perm, int64_t n, Geo geo, Pairs pr,
+                                               const int64_t* off, int64_t nnz, int order,
+                                               const double* gt, const double* ft, double2 k,
+                                               int key, cplx<TS>* vals) {
+  const int64_t b = blockIdx.x;
+  const int r0 = blk[4 * b], r1 = blk[4 * b + 1], c0 = blk[4 * b + 2], c1 = blk[4 * b + 3];
+  const int m = r1 - r0, w = c1 - c0;
+  cplx<TS>* out = vals + voff[b];
+  const double2 ik = mk<double>(-k.y, k.x);
+  for (int64_t e = threadIdx.x; e < (int64_t)m * w; e += blockDim.x) {
+    const int i = (int)(e % m), j = (int)(e / m);
+    const int64_t fi = perm[r0 + i], fj = perm[c0 + j];
+    const double dx = geo.pos[fi] - geo.pos[fj], dy = geo.pos[n + fi] - geo.pos[n + fj],
+                 dz = geo.pos[2 * n + fi] - geo.pos[2 * n + fj];
+    const double d = fmax(sqrt(dx * dx + dy * dy + dz * dz), 2.220446049250313e-16);
+    const double2 ph = cexp_ikd(k, d);
+    // refined pair?  rows of column fj are sorted in [off[fj], off[fj+1])
+    int64_t lo = off[fj], hi = off[fj + 1] - 1, p = -1;
+    while (lo <= hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      const int32_t rr = pr.row[mid];
+      if (rr == fi) {
+        p = mid;
+        break;
+      }
+      if (rr < fi)
+        lo = mid + 1;
+      else
+        hi = mid - 1;
+    }
+    double2 v;
+    if (p >= 0) {
+      double2 pw = mk<double>(1, 0), s = mk<double>(0, 0);
+      for (int o = 0; o <= order; ++o) {
+        double c;
+        if (key == MNP_KEY_G)
+          c = gt[o * nnz + p];
+        else
+          c = geo.nvec[fi] * ft[(o * 3 + 0) * nnz + p] + geo.nvec[n + fi] * ft[(o * 3 + 1) * nnz + p] +
+              geo.nvec[2 * n + fi] * ft[(o * 3 + 2) * nnz + p];
+        s = cadd<double>(s, cscale<double>(pw, c));
+        pw = cmul<double>(pw, ik);
+      }
+      v = cmul<double>(s, ph);
+    } else if (key == MNP_KEY_G) {
+      const double g = (1.0 / d) * geo.area[fj];
+      v = mk<double>(g * ph.x, g * ph.y);
+    } else {
+      const double id2 = 1.0 / (d * d);
+      const double2 fk = mk<double>((-k.y - 1.0 / d) * id2, k.x * id2);
+      const double in = dx * geo.nvec[fi] + dy * geo.nvec[n + fi] + dz * geo.nvec[2 * n + fi];
+      v = cmul<double>(mk<double>(fk.x * in * geo.area[fj], fk.y * in * geo.area[fj]), ph);
+    }
+    if (fi == fj) v.x += key == MNP_KEY_H1 ? 2 * M_PI : key == MNP_KEY_H2 ? -2 * M_PI : 0.0;
+    out[(int64_t)j * m + i] = mk<TS>((TS)v.x, (TS)v.y);
+  }
+}
+
+// Block reductions over 256 threads (4 waves).
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s);
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+// argmax of val with the first (lowest) index on ties; idx < 0 means none.
+__device__ __forceinline__ void block_argmax(double& val, int64_t& idx, double* rv, int64_t* ri) {
+  for (int s = 32; s > 0; s >>= 1) {
+    const double v2 = __shfl_xor(val, s);
+    const int64_t i2 = __shfl_xor(idx, s);
+    if (i2 >= 0 && (v2 > val || (v2 == val && (idx < 0 || i2 < idx)))) {
+      val = v2;
+      idx = i2;
+    }
+  }
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    rv[w] = val;
+    ri[w] = idx;
+  }
+  __syncthreads();
+  val = rv[0];
+  idx = ri[0];
+  for (int q = 1; q < 4; ++q)
+    if (ri[q] >= 0 && (rv[q] > val || (rv[q] == val && (idx < 0 || ri[q] < idx)))) {
+      val = rv[q];
+      idx = ri[q];
+    }
+}
+
+// ACA of admissible block b (rows [r0, r1), cols [c0, c1) in cluster order).
+// Scratch: A (m x kb) and B (n x kb) at aoff[b]; rank[b] on exit.
+__global__ void __launch_bounds__(256) k_aca(int64_t first, const int32_t* blk, const int64_t* soff,
+                                             const double* cpos, const double* cnvec, const double* carea,
+                                             int64_t n, double2 wav, int isF, double tol, int kmax,
+                                             double2* scratch, int32_t* rank, uint32_t* usedbits) {
+  const int64_t b = first + blockIdx.x;
+  const int r0 = blk[4 * b], r1 = blk[4 * b + 1], c0 = blk[4 * b + 2], c1 = blk[4 * b + 3];
+  const int m = r1 - r0, w = c1 - c0;
+  const int kb = min(kmax, min(m, w));
+  double2* A = scratch + soff[b];
+  double2* B = A + (int64_t)m * kb;
+  uint32_t* used = usedbits + (soff[b] >> 5);  // (m + n) kb >= m bits, disjoint per block
+  __shared__ double red[4];
+  __shared__ double rv[4];
+  __shared__ int64_t ri[4];
+  __shared__ double2 coef[128];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < (m + 31) / 32; i += 256) used[i] = 0;
+  __syncthreads();
+  int r = 0, k = 0;
+  double nsum = 0;
+  for (k = 0; k < kb; ++k) {
+    // row r -> B(:, k)
+    double s2 = 0;
+    for (int c = tid; c < w; c += 256) {
+      const double2 v = colloc(cpos, cnvec, carea, n, r0 + r, c0 + c, wav, isF);
+      B[(int64_t)k * w + c] = v;
+      s2 += v.x * v.x + v.y * v.y;
+    }
+    for (int i = tid; i < m; i += 256) A[(int64_t)k * m + i] = mk<double>(0, 0);
+    const double rown = sqrt(block_sum(s2, red));
+    if (rown < 1e-10) break;
+    if (k) {
+      for (int l = tid; l < k; l += 256) coef[l] = A[(int64_t)l * m + r];
+      __syncthreads();
+      for (int c = tid; c < w; c += 256) {
+        double2 v = B[(int64_t)k * w + c];
+        for (int l = 0; l < k; ++l) v = csub<double>(v, cmul<double>(B[(int64_t)l * w + c], coef[l]));
+        B[(int64_t)k * w + c] = v;
+      }
+    }
+    // pivot column: first largest |Re| + |Im|
+    double bv = -1;
+    int64_t bi = -1;
+    for (int c = tid; c < w; c += 256) {
+      const double a = cabs1<double>(B[(int64_t)k * w + c]);
+      if (a > bv) {
+        bv = a;
+        bi = c;
+      }
+    }
+    __syncthreads();
+    block_argmax(bv, bi, rv, ri);
+    const int cp = (int)bi;
+    const double2 scale = cdiv<double>(mk<double>(1, 0), B[(int64_t)k * w + cp]);
+    __syncthreads();
+    double nb2 = 0;
+    for (int c = tid; c < w; c += 256) {
+      const double2 v = cmul<double>(B[(int64_t)k * w + c], scale);
+      B[(int64_t)k * w + c] = v;
+      nb2 += v.x * v.x + v.y * v.y;
+    }
+    __syncthreads();
+    for (int l = tid; l < k; l += 256) coef[l] = B[(int64_t)l * w + cp];
+    __syncthreads();
+    // column cp -> A(:, k)
+    double na2 = 0;
+    for (int i = tid; i < m; i += 256) {
+      double2 v = colloc(cpos, cnvec, carea, n, r0 + i, c0 + cp, wav, isF);
+      for (int l = 0; l < k; ++l) v = csub<double>(v, cmul<double>(A[(int64_t)l * m + i], coef[l]));
+      A[(int64_t)k * m + i] = v;
+      na2 += v.x * v.x + v.y * v.y;
+    }
+    if (tid == 0) used[r >> 5] |= 1u << (r & 31);
+    __syncthreads();
+    // next pivot row: first largest |A(i,k)| over unused rows (izamax of the
+    // work vector, whose used rows are zero)
+    double av = -1;
+    int64_t ai = -1;
+    for (int i = tid; i < m; i += 256) {
+      const bool u = used[i >> 5] & (1u << (i & 31));
+      const double a = u ? 0.0 : cabs1<double>(A[(int64_t)k * m + i]);
+      if (a > av) {
+        av = a;
+        ai = i;
+      }
+    }
+    block_argmax(av, ai, rv, ri);
+    r = (int)ai;
+    const double nk = sqrt(block_sum(na2, red)) * sqrt(block_sum(nb2, red));
+    if (nk < tol * nsum || k + 1 >= m) break;
+    nsum = sqrt(nsum * nsum + nk * nk);
+  }
+  if (tid == 0) rank[b] = min(k + 1, kb);
+}
+
+// Compacts the ACA scratch of block b into L (m x r), R (n x r) (FP32 or FP64).
+template <class TS>
+__global__ void k_aca_store(int64_t first, const int32_t* blk, const int64_t* soff, const double2* scratch,
+                            const int32_t* rank, int kmax, const int64_t* loff, cplx<TS>* pool) {
+  const int64_t b = first + blockIdx.x;
+  const int m = blk[4 * b + 1] - blk[4 * b], w = blk[4 * b + 3] - blk[4 * b + 2];
+  const int kb = min(kmax, min(m, w));
+  const int r = rank[b];
+  const double2* A = scratch + soff[b];
+  const double2* B = A + (int64_t)m * kb;
+  cplx<TS>* L = pool + loff[b];
+  cplx<TS>* R = L + (int64_t)m * r;
+  for (int64_t e = threadIdx.x; e < (int64_t)m * r; e += blockDim.x) L[e] = mk<TS>((TS)A[e].x, (TS)A[e].y);
+  for (int64_t e = threadIdx.x; e < (int64_t)w * r; e += blockDim.x) R[e] = mk<TS>((TS)B[e].x, (TS)B[e].y);
+}
+
+// x (face order, FP64, n x nrhs) -> xc (cluster order, FP32)
+__global__ void k_to_cluster(int64_t n, int64_t nrhs, const int32_t* perm, const double2* x, float2* xc) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i < n) {
+    const double2 v = x[q * n + perm[i]];
+    xc[q * n + i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+__global__ void k_from_cluster(int64_t n, int64_t nrhs, const int32_t* perm, const float2* yc, double2* y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i < n) {
+    const float2 v = yc[q * n + i];
+    y[q * n + perm[i]] = mk<double>(v.x, v.y);
+  }
+}
+
+// t_b = R_b^T x(cols)  (r x nrhs) per low-rank block; one wave per rank index.
+__global__ void __launch_bounds__(256) k_hmv_low(const int32_t* blk, const int32_t* rank, const int64_t* loff,
+                                                 const int64_t* toff, const float2* pool, int64_t n,
+                                                 int nrhs, const float2* xc, float2* t) {
+  const int64_t b = blockIdx.x;
+  const int r0 = blk[4 * b], r1 = blk[4 * b + 1], c0 = blk[4 * b + 2], c1 = blk[4 * b + 3];
+  const int m = r1 - r0, w = c1 - c0, r = rank[b];
+  const float2* R = pool + loff[b] + (int64_t)m * r;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int l = wv; l < r; l += 4) {
+    float2 acc[MNP_HMV_MAXRHS];
+    for (int q = 0; q < nrhs; ++q) acc[q] = make_float2(0.f, 0.f);
+    for (int j = lane; j < w; j += 64) {
+      const float2 rv = R[(int64_t)l * w + j];
+      for (int q = 0; q < nrhs; ++q) acc[q] = cadd<float>(acc[q], cmul<float>(rv, xc[(int64_t)q * n + c0 + j]));
+    }
+    for (int q = 0; q < nrhs; ++q) {
+      float2 v = acc[q];
+      for (int s = 32; s > 0; s >>= 1) {
+        v.x += __shfl_xor(v.x, s);
+        v.y += __shfl_xor(v.y, s);
+      }
+      if (lane == 0) t[toff[b] + (int64_t)q * r + l] = v;
+    }
+  }
+}
+
+// One workgroup per leaf row cluster: y(rows) = sum of full blocks B x and
+// of L_b t_b over the low-rank blocks that cover these rows.
+__global__ void __launch_bounds__(256) k_hmv_rows(const int64_t* lrow, const int64_t* loffs,
+                                                  const int32_t* lentry, const int32_t* fblk,
+                                                  const int64_t* voff, const float2* fvals,
+                                                  const int32_t* lblk, const int32_t* rank,
+                                                  const int64_t* loff, const int64_t* toff,
+                                                  const float2* pool, const float2* t, int64_t n, int nrhs,
+                                                  const float2* xc, float2* yc) {
+  const int64_t leaf = blockIdx.x;
+  const int64_t row0 = lrow[2 * leaf], row1 = lrow[2 * leaf + 1];
+  const int mrows = (int)(row1 - row0);
+  for (int e = threadIdx.x; e < mrows * nrhs; e += blockDim.x) {
+    const int i = e % mrows, q = e / mrows;
+    const int64_t row = row0 + i;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int64_t s = loffs[leaf]; s < loffs[leaf + 1]; ++s) {
+      const int32_t ent = lentry[s];
+      if (ent >= 0) {  // full block
+        const int32_t* bk = fblk + 4 * ent;
+        const int m = bk[1] - bk[0], c0 = bk[2], w = bk[3] - bk[2], ii = (int)(row - bk[0]);
+        const float2* V = fvals + voff[ent];
+        const float2* x = xc + (int64_t)q * n + c0;
+        for (int j = 0; j < w; ++j) acc = cadd<float>(acc, cmul<float>(V[(int64_t)j * m + ii], x[j]));
+      } else {  // low-rank block -ent-1
+        const int64_t lb = -(int64_t)ent - 1;
+        const int32_t* bk = lblk + 4 * lb;
+        const int m = bk[1] - bk[0], ii = (int)(row - bk[0]), r = rank[lb];
+        const float2* L = pool + loff[lb];
+        const float2* tt = t + toff[lb] + (int64_t)q * r;
+        for (int l = 0; l < r; ++l) acc = cadd<float>(acc, cmul<float>(L[(int64_t)l * m + ii], tt[l]));
+      }
+    }
+    yc[(int64_t)q * n + row] = acc;
+  }
+}
+
+// ---------------------------------------------------------------- host
+HMat::HMat(const GreenRet* g_, const int64_t* perm_h, int64_t nfull_, const int64_t* full_h, int64_t nlow_,
+           const int64_t* low_h, double htol_, int kmax_)
+    : g(g_), n(g_->n), nfull(nfull_), nlow(nlow_), htol(htol_), kmax(kmax_) {
+  MNP_CHECK(kmax >= 1 && kmax <= 128, MNP_ERR_ARG, "hmat: kmax must be in 1..128");
+  std::vector<int32_t> perm32(n), seen(n, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    MNP_CHECK(perm_h[i] >= 0 && perm_h[i] < n && !seen[perm_h[i]], MNP_ERR_ARG, "hmat: perm is not a permutation");
+    seen[perm_h[i]] = 1;
+    perm32[i] = (int32_t)perm_h[i];
+  }
+  // blocks (column-major 4 x nb on input) -> int32 rows of 4; coverage check
+  auto pack = [&](int64_t nb, const int64_t* src, std::vector<int32_t>& dst) {
+    dst.resize(4 * nb);
+    for (int64_t b = 0; b < nb; ++b)
+      for (int c = 0; c < 4; ++c) {
+        const int64_t v = src[4 * b + c];
+        MNP_CHECK(v >= 0 && v <= n, MNP_ERR_ARG, "hmat: block range out of bounds");
+        dst[4 * b + c] = (int32_t)v;
+      }
+    for (int64_t b = 0; b < nb; ++b)
+      MNP_CHECK(dst[4 * b] < dst[4 * b + 1] && dst[4 * b + 2] < dst[4 * b + 3], MNP_ERR_ARG, "hmat: empty block");
+  };
+  pack(nfull, full_h, fblk_h);
+  pack(nlow, low_h, lblk_h);
+  double area = 0;
+  for (int64_t b = 0; b < nfull; ++b) area += (double)(fblk_h[4 * b + 1] - fblk_h[4 * b]) * (fblk_h[4 * b + 3] - fblk_h[4 * b + 2]);
+  for (int64_t b = 0; b < nlow; ++b) area += (double)(lblk_h[4 * b + 1] - lblk_h[4 * b]) * (lblk_h[4 * b + 3] - lblk_h[4 * b + 2]);
+  MNP_CHECK(area == (double)n * (double)n, MNP_ERR_ARG, "hmat: blocks do not tile the matrix");
+  // full-block value offsets
+  std::vector<int64_t> voff(nfull + 1, 0);
+  for (int64_t b = 0; b < nfull; ++b)
+    voff[b + 1] = voff[b] + (int64_t)(fblk_h[4 * b + 1] - fblk_h[4 * b]) * (fblk_h[4 * b + 3] - fblk_h[4 * b + 2]);
+  nfullvals = voff[nfull];
+  // leaf row clusters = row ranges of full blocks (leaf x leaf); every row lies in one
+  std::vector<std::pair<int32_t, int32_t>> leaves;
+  for (int64_t b = 0; b < nfull; ++b) leaves.emplace_back(fblk_h[4 * b], fblk_h[4 * b + 1]);
+  std::sort(leaves.begin(), leaves.end());
+  leaves.erase(std::unique(leaves.begin(), leaves.end()), leaves.end());
+  nleaf = (int64_t)leaves.size();
+  std::vector<int64_t> lrow(2 * nleaf);
+  std::vector<int32_t> rowleaf(n, -1);
+  for (int64_t l = 0; l < nleaf; ++l) {
+    lrow[2 * l] = leaves[l].first;
+    lrow[2 * l + 1] = leaves[l].second;
+    for (int32_t i = leaves[l].first; i < leaves[l].second; ++i) {
+      MNP_CHECK(rowleaf[i] < 0, MNP_ERR_ARG, "hmat: overlapping leaf row clusters");
+      rowleaf[i] = (int32_t)l;
+    }
+  }
+  for (int64_t i = 0; i < n; ++i) MNP_CHECK(rowleaf[i] >= 0, MNP_ERR_ARG, "hmat: row not in a leaf cluster");
+  // per-leaf entry lists (full block ids >= 0, low-rank -(id+1)); diagonal blocks
+  std::vector<std::vector<int32_t>> lists(nleaf);
+  diag_h.assign(nleaf, -1);
+  for (int64_t b = 0; b < nfull; ++b) {
+    const int32_t l = rowleaf[fblk_h[4 * b]];
+    lists[l].push_back((int32_t)b);
+    if (fblk_h[4 * b] == fblk_h[4 * b + 2] && fblk_h[4 * b + 1] == fblk_h[4 * b + 3]) diag_h[l] = (int32_t)b;
+  }
+  for (int64_t b = 0; b < nlow; ++b) {
+    const int32_t la = rowleaf[lblk_h[4 * b]], lb = rowleaf[lblk_h[4 * b + 1] - 1];
+    for (int32_t l = la; l <= lb; ++l) lists[l].push_back((int32_t)(-b - 1));
+  }
+  std::vector<int64_t> loffs(nleaf + 1, 0);
+  std::vector<int32_t> lentry;
+  for (int64_t l = 0; l < nleaf; ++l) {
+    lentry.insert(lentry.end(), lists[l].begin(), lists[l].end());
+    loffs[l + 1] = (int64_t)lentry.size();
+  }
+  lrow_h = lrow;
+  // device copies
+  auto up = [&](auto& vec, auto*& dptr) {
+    using E = typename std::remove_reference<decltype(vec)>::type::value_type;
+    dptr = dev_alloc<E>((int64_t)vec.size());
+    if (!vec.empty()) MNP_HIP(hipMemcpy(dptr, vec.data(), sizeof(E) * vec.size(), hipMemcpyHostToDevice));
+  };
+  up(perm32, perm);
+  up(fblk_h, fblk);
+  up(lblk_h, lblk);
+  up(voff, d_voff);
+  up(lrow, d_lrow);
+  up(loffs, d_loffs);
+  up(lentry, d_lentry);
+  // cluster-ordered geometry for ACA
+  std::vector<double> pos(3 * n), nv(3 * n), ar(n), cp(3 * n), cn(3 * n), ca(n);
+  MNP_HIP(hipMemcpy(pos.data(), g->geo.pos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  MNP_HIP(hipMemcpy(nv.data(), g->geo.nvec, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  MNP_HIP(hipMemcpy(ar.data(), g->geo.area, sizeof(double) * n, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n; ++i) {
+    for (int c = 0; c < 3; ++c) {
+      cp[c * n + i] = pos[c * n + perm32[i]];
+      cn[c * n + i] = nv[c * n + perm32[i]];
+    }
+    ca[i] = ar[perm32[i]];
+  }
+  up(cp, cpos);
+  up(cn, cnvec);
+  up(ca, carea);
+  rank = dev_alloc<int32_t>(nlow);
+  d_loff = dev_alloc<int64_t>(nlow + 1);
+  d_toff = dev_alloc<int64_t>(nlow + 1);
+}
+
+HMat::~HMat() {
+  void* ps[] = {perm, fblk, lblk, d_voff, d_lrow, d_loffs, d_lentry, cpos, cnvec, carea, rank,
+                d_loff, d_toff, fvals, pool};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+}
+
+void HMat::fill(double2 k, int key_) {
+  key = key_;
+  hipStream_t s = stream();
+  // full blocks
+  if (!fvals) fvals = dev_alloc<float2>(nfullvals);
+  if (nfull) {
+    const GreenRet& G = *g;
+    TimerScope ts(MNP_TIMER_ASSEMBLE, 8.0 * (double)nfullvals);
+    hipLaunchKernelGGL(k_hfull<float>, dim3((unsigned)nfull), dim3(256), 0, s, nfull, fblk, d_voff, perm, n,
+                       G.geo, G.pairs, G.off, G.nnz, G.order, G.gtab, G.ftab, k, key, fvals);
+    MNP_HIP(hipGetLastError());
+  }
+  // ACA in batches whose FP64 scratch stays below a budget
+  const int isF = key == MNP_KEY_G ? 0 : 1;
+  std::vector<int64_t> soff_h(nlow + 1, 0);
+  for (int64_t b = 0; b < nlow; ++b) {
+    const int64_t m = lblk_h[4 * b + 1] - lblk_h[4 * b], w = lblk_h[4 * b + 3] - lblk_h[4 * b + 2];
+    const int64_t kb = std::min<int64_t>(kmax, std::min(m, w));
+    soff_h[b + 1] = soff_h[b] + ((m + w) * kb + 31) / 32 * 32;
+  }
+  std::vector<int32_t> rank_h(nlow, 0);
+  std::vector<int64_t> loff_h(nlow + 1, 0), toff_h(nlow + 1, 0);
+  const int64_t budget = (int64_t)(1) << 27;  // double2 entries (2 GiB)
+  int64_t* d_soff = dev_alloc<int64_t>(nlow + 1);
+  // batch boundaries
+  std::vector<int64_t> cuts{0};
+  for (int64_t b = 0; b < nlow; ++b)
+    if (soff_h[b + 1] - soff_h[cuts.back()] > budget && b > cuts.back()) cuts.push_back(b);
+  cuts.push_back(nlow);
+  int64_t maxbatch = 0;
+  for (size_t c = 0; c + 1 < cuts.size(); ++c) maxbatch = std::max(maxbatch, soff_h[cuts[c + 1]] - soff_h[cuts[c]]);
+  double2* scratch = dev_alloc<double2>(maxbatch);
+  uint32_t* used = dev_alloc<uint32_t>(maxbatch / 32 + 1);
+  std::vector<int64_t> rel(nlow + 1);
+  for (size_t c = 0; c + 1 < cuts.size(); ++c) {
+    const int64_t b0 = cuts[c], b1 = cuts[c + 1];
+    for (int64_t b = b0; b <= b1; ++b) rel[b] = soff_h[b] - soff_h[b0];
+    MNP_HIP(hipMemcpy(d_soff + b0, rel.data() + b0, sizeof(int64_t) * (b1 - b0 + 1), hipMemcpyHostToDevice));
+    {
+      TimerScope ts(MNP_TIMER_ASSEMBLE, 0.0);
+      hipLaunchKernelGGL(k_aca, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, b0, lblk, d_soff, cpos, cnvec, carea, n,
+                         k, isF, htol, kmax, scratch, rank, used);
+      MNP_HIP(hipGetLastError());
+    }
+    MNP_HIP(hipMemcpyAsync(rank_h.data() + b0, rank + b0, sizeof(int32_t) * (b1 - b0), hipMemcpyDeviceToHost, s));
+    MNP_HIP(hipStreamSynchronize(s));
+    for (int64_t b = b0; b < b1; ++b) {
+      const int64_t m = lblk_h[4 * b + 1] - lblk_h[4 * b], w = lblk_h[4 * b + 3] - lblk_h[4 * b + 2];
+      loff_h[b + 1] = loff_h[b] + (m + w) * rank_h[b];
+      toff_h[b + 1] = toff_h[b] + rank_h[b] * (int64_t)MNP_HMV_MAXRHS;
+    }
+    if (loff_h[b1] > poolcap) {  // grow the pool, keeping earlier batches
+      const int64_t cap = std::max<int64_t>(loff_h[b1] + loff_h[b1] / 2, 1 << 20);
+      float2* np = dev_alloc<float2>(cap);
+      if (pool && loff_h[b0]) MNP_HIP(hipMemcpyAsync(np, pool, sizeof(float2) * loff_h[b0], hipMemcpyDeviceToDevice, s));
+      MNP_HIP(hipStreamSynchronize(s));
+      if (pool) MNP_HIP(hipFree(pool));
+      pool = np;
+      poolcap = cap;
+    }
+    MNP_HIP(hipMemcpy(d_loff + b0, loff_h.data() + b0, sizeof(int64_t) * (b1 - b0 + 1), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_aca_store<float>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, b0, lblk, d_soff, scratch,
+                       rank, kmax, d_loff, pool);
+    MNP_HIP(hipGetLastError());
+    MNP_HIP(hipStreamSynchronize(s));
+  }
+  MNP_HIP(hipMemcpy(d_toff, toff_h.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice));
+  MNP_HIP(hipFree(scratch));
+  MNP_HIP(hipFree(used));
+  MNP_HIP(hipFree(d_soff));
+  nlowvals = loff_h[nlow];
+  maxrank = 0;
+  for (int32_t r : rank_h) maxrank = std::max(maxrank, (int)r);
+  rank_host = rank_h;
+  tlen = toff_h[nlow];
+  filled = true;
+}
+
+void HMat::matvec(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const {
+  MNP_CHECK(filled, MNP_ERR_STATE, "hmat: matvec before fill");
+  MNP_CHECK(nrhs >= 1, MNP_ERR_ARG, "hmat: nrhs < 1");
+  hipStream_t s = stream();
+  for (int64_t q0 = 0; q0 < nrhs; q0 += MNP_HMV_MAXRHS) {
+    const int nq = (int)std::min<int64_t>(MNP_HMV_MAXRHS, nrhs - q0);
+    float2* xc = (float2*)ws.get(0, sizeof(float2) * n * nq);
+    float2* yc = (float2*)ws.get(1, sizeof(float2) * n * nq);
+    float2* t = (float2*)ws.get(2, sizeof(float2) * std::max<int64_t>(tlen, 1));
+    const dim3 gv((unsigned)cdiv_up(n, 256), (unsigned)nq);
+    hipLaunchKernelGGL(k_to_cluster, gv, dim3(256), 0, s, n, nq, perm, x + q0 * n, xc);
+    if (nlow)
+      hipLaunchKernelGGL(k_hmv_low, dim3((unsigned)nlow), dim3(256), 0, s, lblk, rank, d_loff, d_toff, pool, n, nq,
+                         xc, t);
+    hipLaunchKernelGGL(k_hmv_rows, dim3((unsigned)nleaf), dim3(256), 0, s, d_lrow, d_loffs, d_lentry, fblk, d_voff,
+                       fvals, lblk, rank, d_loff, d_toff, pool, t, n, nq, xc, yc);
+    hipLaunchKernelGGL(k_from_cluster, gv, dim3(256), 0, s, n, nq, perm, yc, y + q0 * n);
+    MNP_HIP(hipGetLastError());
+  }
+}
+
+}  // namespace mnp
+
+// ============================================================== C ABI
+using namespace mnp;
+
+struct mnp_hmat_s {
+  HMat* h;
+  Workspace ws;
+};
+
+extern "C" {
+
+int mnp_hmat_create(mnp_greenret_t g, const int64_t* perm, int64_t nfull, const int64_t* full, int64_t nlow,
+                    const int64_t* low, double htol, int kmax, mnp_hmat_t* out) {
+  return guarded([&] {
+    MNP_CHECK(g && perm && out && (full || !nfull) && (low || !nlow), MNP_ERR_ARG, "hmat_create: NULL argument");
+    MNP_CHECK(htol > 0, MNP_ERR_ARG, "hmat_create: htol must be positive");
+    auto* h = new mnp_hmat_s{nullptr, {}};
+    try {
+      h->h = new HMat(g->g, perm, nfull, full, nlow, low, htol, kmax);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int mnp_hmat_destroy(mnp_hmat_t h) {
+  return guarded([&] {
+    if (!h) return;
+    delete h->h;
+    delete h;
+  });
+}
+
+int mnp_hmat_fill(mnp_hmat_t h, mnp_complex k, int key) {
+  return guarded([&] {
+    MNP_CHECK(h, MNP_ERR_ARG, "hmat_fill: NULL handle");
+    MNP_CHECK(key >= MNP_KEY_G && key <= MNP_KEY_H2, MNP_ERR_ARG, "hmat_fill: bad key");
+    h->h->fill(to_d2(k), key);
+  });
+}
+
+int mnp_hmat_matvec(mnp_hmat_t h, int64_t nrhs, const void* x, void* y) {
+  return guarded([&] {
+    MNP_CHECK(h && x && y, MNP_ERR_ARG, "hmat_matvec: NULL argument");
+    h->h->matvec(nrhs, (const double2*)x, (double2*)y, h->ws);
+  });
+}
+
+int mnp_hmat_stats(mnp_hmat_t h, int64_t* full_entries, int64_t* lowrank_entries, int* max_rank,
+                   double* compression) {
+  return guarded([&] {
+    MNP_CHECK(h, MNP_ERR_ARG, "hmat_stats: NULL handle");
+    const HMat& H = *h->h;
+    if (full_entries) *full_entries = H.nfullvals;
+    if (lowrank_entries) *lowrank_entries = H.nlowvals;
+    if (max_rank) *max_rank = H.maxrank;
+    if (compression) *compression = (double)(H.nfullvals + H.nlowvals) / ((double)H.n * (double)H.n);
+  });
+}
+
+int mnp_hmat_ranks(mnp_hmat_t h, int32_t* ranks) {
+  return guarded([&] {
+    MNP_CHECK(h && ranks, MNP_ERR_ARG, "hmat_ranks: NULL argument");
+    MNP_CHECK(h->h->filled, MNP_ERR_STATE, "hmat_ranks: not filled");
+    std::copy(h->h->rank_host.begin(), h->h->rank_host.end(), ranks);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,44 @@
+// H-matrix of the retarded Green function (full leaf blocks + ACA blocks).
+#pragma once
+
+#include <vector>
+
+#include "mnp_linalg.hpp"
+
+namespace mnp {
+
+constexpr int MNP_HMV_MAXRHS = 8;  // right-hand sides per H-matvec pass
+
+struct HMat {
+  const GreenRet* g;
+  int64_t n, nfull, nlow;
+  double htol;
+  int kmax;
+  int key = -1;
+  bool filled = false;
+  // block structure: [row_begin, row_end, col_begin, col_end) in cluster order
+  std::vector<int32_t> fblk_h, lblk_h, diag_h;
+  std::vector<int64_t> lrow_h;
+  std::vector<int32_t> rank_host;
+  int64_t nleaf = 0, nfullvals = 0, nlowvals = 0, poolcap = 0, tlen = 0;
+  int maxrank = 0;
+  int32_t *perm = nullptr, *fblk = nullptr, *lblk = nullptr, *rank = nullptr, *d_lentry = nullptr;
+  int64_t *d_voff = nullptr, *d_lrow = nullptr, *d_loffs = nullptr, *d_loff = nullptr, *d_toff = nullptr;
+  double *cpos = nullptr, *cnvec = nullptr, *carea = nullptr;
+  float2* fvals = nullptr;  // full blocks, column-major per block, complex FP32
+  float2* pool = nullptr;   // low-rank factors L (m x r) then R (n x r) per block, complex FP32
+
+  HMat(const GreenRet* g, const int64_t* perm, int64_t nfull, const int64_t* full, int64_t nlow,
+       const int64_t* low, double htol, int kmax);
+  ~HMat();
+  HMat(const HMat&) = delete;
+  HMat& operator=(const HMat&) = delete;
+  // Fills the operator `key` (MNP_KEY_*) at wavenumber k.
+  void fill(double2 k, int key);
+  // y = H x, x and y complex FP64 in face order (n x nrhs).
+  void matvec(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const;
+};
+
+}  // namespace mnp
+
+

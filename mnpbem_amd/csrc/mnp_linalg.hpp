@@ -7,9 +7,11 @@ namespace mnp {
 
 // Grow-only device scratch owned by a solver handle.
 struct Workspace {
-  void* buf[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  size_t cap[6] = {0, 0, 0, 0, 0, 0};
+  static constexpr int NSLOT = 8;
+  void* buf[NSLOT] = {};
+  size_t cap[NSLOT] = {};
   void* get(int slot, size_t bytes) {
+    MNP_CHECK(slot >= 0 && slot < NSLOT, MNP_ERR_INTERNAL, "workspace slot out of range");
     if (bytes > cap[slot]) {
       if (buf[slot]) MNP_HIP(hipFree(buf[slot]));
       buf[slot] = nullptr;
