@@ -73,8 +73,10 @@ enum {
   MNP_TIMER_GEMM_FP64 = 0, /* complex GEMM, complex128 operands                 */
   MNP_TIMER_GEMM_FP32 = 1, /* complex GEMM, complex64 operands                  */
   MNP_TIMER_PANEL = 2,     /* cooperative LU panel                              */
-  MNP_TIMER_ASSEMBLE = 3,  /* dense G/H assembly                                */
-  MNP_TIMER_COUNT = 4
+  MNP_TIMER_ASSEMBLE = 3,  /* dense G/H assembly and H-matrix full blocks        */
+  MNP_TIMER_HMATVEC = 4,   /* H-matrix times matrix (bytes of stored blocks)     */
+  MNP_TIMER_ACA = 5,       /* ACA of admissible blocks                           */
+  MNP_TIMER_COUNT = 6
 };
 typedef struct {
   int64_t launches[MNP_TIMER_COUNT];
@@ -195,17 +197,20 @@ int mnp_getri(int precision, int64_t n, const void* LU, int64_t lda, const int32
  * (@clustertree, admissibility.m), as in the reference.
  */
 typedef struct mnp_hmat_s* mnp_hmat_t;
-/* perm[c] = face index of cluster index c (n entries); full and low are
+/* precision (MNP_FP64 or MNP_FP32) is the storage of the full blocks and
+ * low-rank factors; ACA runs in FP64 either way and products accumulate in
+ * FP64.  perm[c] = face index of cluster index c (n entries); full and low are
  * 4 x nblocks (column-major) [row_begin, row_end, col_begin, col_end) in
  * cluster indices; the blocks must tile the n x n matrix and the full blocks
  * must include the diagonal leaf blocks.  htol and kmax are the ACA
  * tolerance and maximum rank (hoptions; kmax <= 128). */
-int mnp_hmat_create(mnp_greenret_t g, const int64_t* perm, int64_t nfull, const int64_t* full,
-                    int64_t nlow, const int64_t* low, double htol, int kmax, mnp_hmat_t* out);
+int mnp_hmat_create(mnp_greenret_t g, int precision, const int64_t* perm, int64_t nfull,
+                    const int64_t* full, int64_t nlow, const int64_t* low, double htol, int kmax,
+                    mnp_hmat_t* out);
 int mnp_hmat_destroy(mnp_hmat_t h);
 /* Fills G, F, H1 (F + 2 pi) or H2 (F - 2 pi) at wavenumber k (key MNP_KEY_*). */
 int mnp_hmat_fill(mnp_hmat_t h, mnp_complex k, int key);
-/* y = H x (device, complex128, face order, n x nrhs), computed in complex FP32. */
+/* y = H x (device, complex128, face order, n x nrhs). */
 int mnp_hmat_matvec(mnp_hmat_t h, int64_t nrhs, const void* x, void* y);
 /* Stored entries of full and low-rank blocks, largest rank and compression
  * (stored entries / n^2) of the last fill (hmatrix/compression.m). */
@@ -218,15 +223,17 @@ int mnp_hmat_ranks(mnp_hmat_t h, int32_t* ranks);
  * Replaces bemsolver(p, op) with op.iter set -> bemretiter (BEM/@bemretiter):
  * initmat fills G1, G2, H1, H2 as H-matrices (private/initmat.m:5-31) and
  * the preconditioner; solve runs GMRES without restart on the exact
- * operator afun (private/afun.m:6-35) with left preconditioning
- * (bemiter/solve.m:33-36, private/mfun.m).  The preconditioner is the
- * bemret chain applied exactly on the diagonal leaf blocks (leaf clusters
- * of at most 32 faces), in place of the reference's rank-4 H-matrix chain
- * (private/initprecond.m).  Block arguments as for mnp_hmat_create. */
+ * operator afun (private/afun.m:6-35), preconditioned by mfun
+ * (bemiter/solve.m:33-36, private/mfun.m) from the right in the flexible
+ * form, and stops when |b - A x| <= tol |b|.  The preconditioner is the
+ * bemret chain applied exactly on diagonal blocks of at most 32 faces
+ * (consecutive cluster indices inside each leaf cluster), in place of the
+ * reference's rank-4 H-matrix chain (private/initprecond.m).  Precision and
+ * block arguments as for mnp_hmat_create. */
 typedef struct mnp_bemretiter_s* mnp_bemretiter_t;
-int mnp_bemretiter_create(mnp_greenret_t g, const int64_t* perm, int64_t nfull, const int64_t* full,
-                          int64_t nlow, const int64_t* low, double htol, int kmax,
-                          mnp_bemretiter_t* out);
+int mnp_bemretiter_create(mnp_greenret_t g, int precision, const int64_t* perm, int64_t nfull,
+                          const int64_t* full, int64_t nlow, const int64_t* low, double htol,
+                          int kmax, mnp_bemretiter_t* out);
 int mnp_bemretiter_destroy(mnp_bemretiter_t it);
 /* GMRES tolerance and iteration limit (bemiter.options: 1e-6, 100); precond 0 disables it. */
 int mnp_bemretiter_set_options(mnp_bemretiter_t it, double tol, int maxit, int precond);
@@ -236,9 +243,9 @@ int mnp_bemretiter_solve(mnp_bemretiter_t it, int64_t npol, const void* phi1, co
                          const void* a1, const void* a1p, const void* phi2, const void* phi2p,
                          const void* a2, const void* a2p, void* sig1, void* sig2, void* h1,
                          void* h2);
-/* Last solve: flag (0 converged, 1 not), preconditioned relative residual,
- * iterations, true relative residual |b - A x| / |b|; compression of G1
- * (bemiter/setiter.m, hinfo.m). */
+/* Last solve: flag (0 converged, 1 not), GMRES relative residual estimate,
+ * iterations, relative residual |b - A x| / |b| recomputed from x;
+ * compression of G1 (bemiter/setiter.m, hinfo.m). */
 int mnp_bemretiter_stats(mnp_bemretiter_t it, int* flag, double* relres, int* iter,
                          double* true_relres, double* compression);
 

@@ -31,7 +31,7 @@ class GreenOptions(C.Structure):
 
 
 class Timers(C.Structure):
-    _fields_ = [("launches", C.c_int64 * 4), ("ms", C.c_double * 4), ("work", C.c_double * 4)]
+    _fields_ = [("launches", C.c_int64 * 6), ("ms", C.c_double * 6), ("work", C.c_double * 6)]
 
 
 P, I64, D, INT = C.c_void_p, C.c_int64, C.c_double, C.c_int
@@ -67,13 +67,13 @@ PROTOTYPES = {
     "mnp_getrs": (INT, [INT, I64, I64, P, I64, P, P, I64]),
     "mnp_getrs_right": (INT, [INT, I64, I64, P, I64, P, P, I64]),
     "mnp_getri": (INT, [INT, I64, P, I64, P, P, I64]),
-    "mnp_hmat_create": (INT, [P, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
+    "mnp_hmat_create": (INT, [P, INT, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
     "mnp_hmat_destroy": (INT, [P]),
     "mnp_hmat_fill": (INT, [P, MnpComplex, INT]),
     "mnp_hmat_matvec": (INT, [P, I64, P, P]),
     "mnp_hmat_stats": (INT, [P, PI64, PI64, C.POINTER(INT), PD]),
     "mnp_hmat_ranks": (INT, [P, P]),
-    "mnp_bemretiter_create": (INT, [P, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
+    "mnp_bemretiter_create": (INT, [P, INT, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
     "mnp_bemretiter_destroy": (INT, [P]),
     "mnp_bemretiter_set_options": (INT, [P, D, INT, INT]),
     "mnp_bemretiter_initmat": (INT, [P, D, MnpComplex, MnpComplex]),

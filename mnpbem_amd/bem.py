@@ -157,7 +157,7 @@ class bemret:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib._lib is not None:
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
             _lib.lib().mnp_bemret_destroy(h)
             self.handle = None
 
@@ -200,7 +200,7 @@ class bemret:
 def bemiter_options(**kw):
     """bemiter.options defaults (BEM/@bemiter/options.m:18-44)."""
     op = dict(solver="gmres", tol=1e-6, maxit=100, restart=None, precond="hmat", output=0,
-              cleaf=200, htol=1e-6, kmax=[4, 100], fadmiss=None)
+              cleaf=200, htol=1e-6, kmax=[4, 100], fadmiss=None, hprecision="fp64")
     op.update(kw)
     return op
 
@@ -209,7 +209,8 @@ class bemretiter:
     """Iterative retarded BEM solver with H-matrices (BEM/@bemretiter).
 
     ``op['iter']`` holds bemiter options (cleaf, htol, kmax, tol, maxit,
-    precond).  GMRES runs on the GPU with the exact operator (four
+    precond) and ``hprecision`` ('fp64', the reference's H-matrices, or
+    'fp32' storage).  GMRES runs on the GPU with the exact operator (four
     H-matvecs) and a block preconditioner; see mnp_bemretiter_* in
     include/mnpbem_hip.h.
     """
@@ -229,7 +230,8 @@ class bemretiter:
         kmax = int(np.max(np.atleast_1d(it["kmax"])))
         htol = float(np.min(np.atleast_1d(it["htol"])))
         h = C.c_void_p()
-        _lib.call("mnp_bemretiter_create", p.green.handle, perm.ctypes.data_as(C.c_void_p), len(fr),
+        prec = {"fp64": _lib.MNP_FP64, "fp32": _lib.MNP_FP32}[it["hprecision"]]
+        _lib.call("mnp_bemretiter_create", p.green.handle, prec, perm.ctypes.data_as(C.c_void_p), len(fr),
                   fr.ctypes.data_as(C.c_void_p), len(lr), lr.ctypes.data_as(C.c_void_p), htol, kmax,
                   C.byref(h))
         self.handle = h
@@ -239,7 +241,7 @@ class bemretiter:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib._lib is not None:
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
             _lib.lib().mnp_bemretiter_destroy(h)
             self.handle = None
 

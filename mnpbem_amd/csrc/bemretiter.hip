@@ -4,14 +4,16 @@
 // H1 = F(k1) + 2 pi, H2 = F(k2) - 2 pi as H-matrices (full blocks with
 // refinement + ACA blocks, hmatrix.hip), plus the preconditioner.
 // solve (bemretiter/solve.m, bemiter/solve.m): GMRES without restart on
-// the exact operator afun (private/afun.m:6-35, four H-matvecs) with left
-// preconditioning by mfun (private/mfun.m:10-52).
+// the exact operator afun (private/afun.m:6-35, four H-matvecs),
+// preconditioned by mfun (private/mfun.m:10-52) from the right in the
+// flexible form, so that the minimised residual is |b - A x|.
 //
 // Preconditioner: the reference compresses the whole bemret chain into
 // H-matrices of rank 4 (private/initprecond.m, H-LU and H-inverse).  Here
-// the same chain is applied exactly on each diagonal leaf block (block
-// Jacobi): G1, G2, Sigma1, Deltai, Sigma of every leaf cluster are formed
-// and factorised in LDS by one workgroup.  The preconditioner changes the
+// the same chain is applied exactly on diagonal blocks of at most 32
+// consecutive cluster indices inside each leaf cluster (block Jacobi): G1,
+// G2, Sigma1, Deltai, Sigma of every block are formed and factorised in LDS
+// by one workgroup, in complex FP32.  The preconditioner changes the
 // iteration count only; the converged solution solves the same equations.
 #include <complex>
 #include <memory>
@@ -79,26 +81,32 @@ __device__ void lds_lsolve(cf (*V)[2 * PB + 1], int ncol, cf (*LU)[PB + 1], int 
   __syncthreads();
 }
 
-// Per leaf: LU(G1), LU(G2), Sigma1, Deltai, LU(Sigma) (5 x PB x PB, FP32),
-// the bemret chain (initmat.m) on the diagonal block.
-__global__ void __launch_bounds__(256) k_bjprec_build(int64_t nleaf, const int64_t* lrow, const int32_t* diag,
-                                                      const int64_t* voff, const float2* G1v, const float2* G2v,
-                                                      const float2* H1v, const float2* H2v, const int32_t* perm,
+// Per preconditioner block: LU(G1), LU(G2), Sigma1, Deltai, LU(Sigma)
+// (5 x PB x PB, FP32), the bemret chain (initmat.m) on a diagonal block.
+// pb[5 b ..] = {first cluster index, size nb <= PB, offset in its leaf,
+// leaf size m, value offset of the leaf's diagonal full block}.
+template <class TS>
+__global__ void __launch_bounds__(256) k_bjprec_build(const int64_t* pb, const cplx<TS>* G1v, const cplx<TS>* G2v,
+                                                      const cplx<TS>* H1v, const cplx<TS>* H2v, const int32_t* perm,
                                                       const double* nvec, int64_t n, double2 e1, double2 e2,
                                                       double2 cc, float2* out) {
   __shared__ cf A[PB][PB + 1], B[PB][PB + 1], S1[PB][PB + 1], S2[PB][PB + 1], D[PB][PB + 1], X[PB][PB + 1];
   const int64_t l = blockIdx.x;
-  const int r0 = (int)lrow[2 * l], nb = (int)(lrow[2 * l + 1] - lrow[2 * l]);
-  const int64_t vo = voff[diag[l]];
+  const int r0 = (int)pb[5 * l], nb = (int)pb[5 * l + 1], off = (int)pb[5 * l + 2];
+  const int64_t m = pb[5 * l + 3], vo = pb[5 * l + 4];
   const int tid = threadIdx.x;
+  auto ld = [&](const cplx<TS>* v, int i, int j) {
+    const cplx<TS> z = v[vo + (int64_t)(off + j) * m + off + i];
+    return make_float2((float)z.x, (float)z.y);
+  };
   for (int e = tid; e < PB * PB; e += 256) {
     const int i = e % PB, j = e / PB;
     const bool in = i < nb && j < nb;
     const cf id = i == j ? make_float2(1.f, 0.f) : make_float2(0.f, 0.f);
-    A[i][j] = in ? G1v[vo + (int64_t)j * nb + i] : id;
-    B[i][j] = in ? G2v[vo + (int64_t)j * nb + i] : id;
-    S1[i][j] = in ? H1v[vo + (int64_t)j * nb + i] : id;
-    S2[i][j] = in ? H2v[vo + (int64_t)j * nb + i] : id;
+    A[i][j] = in ? ld(G1v, i, j) : id;
+    B[i][j] = in ? ld(G2v, i, j) : id;
+    S1[i][j] = in ? ld(H1v, i, j) : id;
+    S2[i][j] = in ? ld(H2v, i, j) : id;
   }
   __syncthreads();
   lds_lu(A, nb);
@@ -149,9 +157,10 @@ __global__ void __launch_bounds__(256) k_bjprec_build(int64_t nleaf, const int64
   }
 }
 
-// x = M^{-1} b on one leaf's rows (mfun.m:27-52 with the block matrices).
-// b, x: n x 8p FP64 in face order, [phi | a | De | alpha] -> [sig1 | h1 | sig2 | h2].
-__global__ void __launch_bounds__(256) k_bjprec_apply(const int64_t* lrow, const int32_t* perm,
+// x = M^{-1} b on one preconditioner block's rows (mfun.m:27-52 with the
+// block matrices).  b, x: n x 8p FP64 in face order,
+// [phi | a | De | alpha] -> [sig1 | h1 | sig2 | h2].
+__global__ void __launch_bounds__(256) k_bjprec_apply(const int64_t* pb, const int32_t* perm,
                                                       const float2* fac, const double* nvec, int64_t n, int p,
                                                       double k, double2 L1, double2 L2, const double2* b,
                                                       double2* x) {
@@ -159,7 +168,7 @@ __global__ void __launch_bounds__(256) k_bjprec_apply(const int64_t* lrow, const
   __shared__ cf V[PB][2 * PB + 1];   // up to 8p <= 64 vector columns (p <= 8)
   __shared__ cf W[PB][2 * PB + 1];
   const int64_t l = blockIdx.x;
-  const int r0 = (int)lrow[2 * l], nb = (int)(lrow[2 * l + 1] - lrow[2 * l]);
+  const int r0 = (int)pb[5 * l], nb = (int)pb[5 * l + 1];
   const float2* F = fac + l * 5 * PB * PB;
   const int tid = threadIdx.x, nc = 8 * p;
   auto loadM = [&](int which) {
@@ -351,7 +360,10 @@ struct BemRetIter {
   const GreenRet* g;
   int64_t n;
   std::unique_ptr<HMat> G1, G2, H1, H2;
-  float2* prec = nullptr;  // nleaf x 5 x PB x PB
+  float2* prec = nullptr;  // npb x 5 x PB x PB
+  int64_t* d_pb = nullptr;  // npb x 5 block descriptors (k_bjprec_build)
+  int64_t npb = 0;
+  int precision;
   bool use_prec = true;
   double tol = 1e-6;
   int maxit = 100;
@@ -361,30 +373,39 @@ struct BemRetIter {
   int last_iter = 0, last_flag = 0;
   double last_relres = 0, last_true_relres = 0;
   Workspace ws, hws;
-  int64_t* d_lrow = nullptr;
 
-  BemRetIter(const GreenRet* g_, const int64_t* perm, int64_t nfull, const int64_t* full, int64_t nlow,
-             const int64_t* low, double htol, int kmax)
-      : g(g_), n(g_->n) {
-    G1.reset(new HMat(g, perm, nfull, full, nlow, low, htol, kmax));
-    G2.reset(new HMat(g, perm, nfull, full, nlow, low, htol, kmax));
-    H1.reset(new HMat(g, perm, nfull, full, nlow, low, htol, kmax));
-    H2.reset(new HMat(g, perm, nfull, full, nlow, low, htol, kmax));
+  BemRetIter(const GreenRet* g_, int prec_, const int64_t* perm, int64_t nfull, const int64_t* full,
+             int64_t nlow, const int64_t* low, double htol, int kmax)
+      : g(g_), n(g_->n), precision(prec_) {
+    G1.reset(new HMat(g, prec_, perm, nfull, full, nlow, low, htol, kmax));
+    G2.reset(new HMat(g, prec_, perm, nfull, full, nlow, low, htol, kmax));
+    H1.reset(new HMat(g, prec_, perm, nfull, full, nlow, low, htol, kmax));
+    H2.reset(new HMat(g, prec_, perm, nfull, full, nlow, low, htol, kmax));
+    // preconditioner blocks: each leaf's diagonal block cut into near-equal
+    // runs of at most PB consecutive cluster indices
+    std::vector<int64_t> pb;
+    std::vector<int64_t> voff(G1->nfull + 1);
+    for (int64_t b = 0; b < G1->nfull; ++b)
+      voff[b + 1] = voff[b] + (int64_t)(G1->fblk_h[4 * b + 1] - G1->fblk_h[4 * b]) *
+                                  (G1->fblk_h[4 * b + 3] - G1->fblk_h[4 * b + 2]);
     for (int64_t l = 0; l < G1->nleaf; ++l) {
       MNP_CHECK(G1->diag_h[l] >= 0, MNP_ERR_ARG, "bemretiter: a leaf cluster has no diagonal full block");
-      MNP_CHECK(G1->lrow_h[2 * l + 1] - G1->lrow_h[2 * l] <= PB, MNP_ERR_ARG,
-                "bemretiter: the block preconditioner needs leaf clusters of at most 32 faces (cleaf <= 32)");
+      const int64_t a = G1->lrow_h[2 * l], m = G1->lrow_h[2 * l + 1] - a;
+      const int64_t nch = (m + PB - 1) / PB;
+      for (int64_t c = 0; c < nch; ++c) {
+        const int64_t o0 = c * m / nch, o1 = (c + 1) * m / nch;
+        pb.insert(pb.end(), {a + o0, o1 - o0, o0, m, voff[G1->diag_h[l]]});
+      }
     }
-    prec = dev_alloc<float2>(G1->nleaf * 5 * PB * PB);
-    int32_t* diag = dev_alloc<int32_t>(G1->nleaf);
-    MNP_HIP(hipMemcpy(diag, G1->diag_h.data(), sizeof(int32_t) * G1->nleaf, hipMemcpyHostToDevice));
-    d_diag = diag;
+    npb = (int64_t)pb.size() / 5;
+    d_pb = dev_alloc<int64_t>((int64_t)pb.size());
+    MNP_HIP(hipMemcpy(d_pb, pb.data(), sizeof(int64_t) * pb.size(), hipMemcpyHostToDevice));
+    prec = dev_alloc<float2>(npb * 5 * PB * PB);
   }
   ~BemRetIter() {
     if (prec) (void)hipFree(prec);
-    if (d_diag) (void)hipFree(d_diag);
+    if (d_pb) (void)hipFree(d_pb);
   }
-  int32_t* d_diag = nullptr;
 
   void initmat(double enei_, double2 e1, double2 e2) {
     enei = enei_;
@@ -404,9 +425,14 @@ struct BemRetIter {
     H2->fill(k2, MNP_KEY_H2);
     const double2 L = csub<double>(e1, e2);
     const double2 cc = cscale<double>(cmul<double>(L, L), k0 * k0);
-    hipLaunchKernelGGL(k_bjprec_build, dim3((unsigned)G1->nleaf), dim3(256), 0, stream(), G1->nleaf, G1->d_lrow,
-                       d_diag, G1->d_voff, G1->fvals, G2->fvals, H1->fvals, H2->fvals, G1->perm, g->geo.nvec, n,
-                       e1, e2, cc, prec);
+    if (precision == MNP_FP64)
+      hipLaunchKernelGGL(k_bjprec_build<double>, dim3((unsigned)npb), dim3(256), 0, stream(), d_pb,
+                         (const double2*)G1->fvals, (const double2*)G2->fvals, (const double2*)H1->fvals,
+                         (const double2*)H2->fvals, G1->perm, g->geo.nvec, n, e1, e2, cc, prec);
+    else
+      hipLaunchKernelGGL(k_bjprec_build<float>, dim3((unsigned)npb), dim3(256), 0, stream(), d_pb,
+                         (const float2*)G1->fvals, (const float2*)G2->fvals, (const float2*)H1->fvals,
+                         (const float2*)H2->fvals, G1->perm, g->geo.nvec, n, e1, e2, cc, prec);
     MNP_HIP(hipGetLastError());
     ready = true;
   }
@@ -427,8 +453,8 @@ struct BemRetIter {
       MNP_HIP(hipMemcpyAsync(x, b, sizeof(double2) * 8 * p * n, hipMemcpyDeviceToDevice, stream()));
       return;
     }
-    hipLaunchKernelGGL(k_bjprec_apply, dim3((unsigned)G1->nleaf), dim3(256), 0, stream(), G1->d_lrow, G1->perm,
-                       prec, g->geo.nvec, n, (int)p, k0, eps1, eps2, b, x);
+    hipLaunchKernelGGL(k_bjprec_apply, dim3((unsigned)npb), dim3(256), 0, stream(), d_pb, G1->perm, prec,
+                       g->geo.nvec, n, (int)p, k0, eps1, eps2, b, x);
     MNP_HIP(hipGetLastError());
   }
 
@@ -445,6 +471,10 @@ struct BemRetIter {
     return r;
   }
 
+  // GMRES without restart, right-preconditioned in the flexible form
+  // (z_j = M^{-1} v_j kept, x = Z y): the residual it minimises is the true
+  // |b - A x|, also when M^{-1} is applied inexactly (complex FP32), and
+  // convergence is |b - A x| <= tol |b| as gmres reports relres.
   void solve(int64_t p, const double2* const in[8], double2* sig1, double2* sig2, double2* h1, double2* h2) {
     MNP_CHECK(ready, MNP_ERR_STATE, "bemretiter_solve: initmat not called");
     MNP_CHECK(p >= 1 && p <= 8, MNP_ERR_ARG, "bemretiter_solve: npol must be in 1..8");
@@ -452,6 +482,7 @@ struct BemRetIter {
     const int m = std::max(1, std::min<int>(maxit, (int)len));
     double2* b = (double2*)ws.get(0, sizeof(double2) * len);
     double2* V = (double2*)ws.get(1, sizeof(double2) * len * (m + 1));
+    double2* Z = (double2*)ws.get(7, sizeof(double2) * len * m);
     double2* Ax = (double2*)ws.get(2, sizeof(double2) * len);
     double2* prod = (double2*)ws.get(6, sizeof(double2) * 16 * p * n);
     double2* w = (double2*)ws.get(4, sizeof(double2) * len);
@@ -459,9 +490,7 @@ struct BemRetIter {
     launch_excitation(n, p, in, b, g->geo.nvec, k0, eps1, eps2);
     auto norm = [&](const double2* v) { return std::sqrt(std::max(0.0, mdot(len, v, 1, v)[0].real())); };
     const unsigned gl = (unsigned)cdiv_up(len, 256);
-    // r0 = M^{-1} b (x0 = 0)
-    mfun(p, b, V);
-    const double beta = norm(V);
+    const double beta = norm(b);  // r0 = b (x0 = 0)
     using C = std::complex<double>;
     std::vector<std::vector<C>> Hm(m + 1, std::vector<C>(m, 0.0));
     std::vector<C> cs(m), sn(m), gv(m + 1, 0.0);
@@ -470,11 +499,12 @@ struct BemRetIter {
     double res = beta;
     last_flag = 0;
     if (beta > 0) {
+      MNP_HIP(hipMemcpyAsync(V, b, sizeof(double2) * len, hipMemcpyDeviceToDevice, stream()));
       hipLaunchKernelGGL(k_scale2, dim3(gl), dim3(256), 0, stream(), len, V, 1.0 / beta);
       for (it = 0; it < m; ++it) {
-        double2* vj = V + (int64_t)it * len;
-        afun(p, vj, Ax, prod);
-        mfun(p, Ax, w);
+        double2* zj = Z + (int64_t)it * len;
+        mfun(p, V + (int64_t)it * len, zj);
+        afun(p, zj, w, prod);
         std::vector<C> h(it + 2, 0.0);
         for (int pass = 0; pass < 2; ++pass) {  // classical Gram-Schmidt, twice
           std::vector<C> d = mdot(len, V, it + 1, w);
@@ -505,35 +535,39 @@ struct BemRetIter {
         gv[it] = std::conj(cs[it]) * gv[it];
         for (int j = 0; j <= it; ++j) Hm[j][it] = h[j];
         res = std::abs(gv[it + 1]);
-        if (hn > 0 && it + 1 < m) {
+        if (res <= tol * beta || hn == 0 || rr == 0) {
+          ++it;
+          break;
+        }
+        if (it + 1 < m) {
           MNP_HIP(hipMemcpyAsync(V + (int64_t)(it + 1) * len, w, sizeof(double2) * len, hipMemcpyDeviceToDevice,
                                  stream()));
           hipLaunchKernelGGL(k_scale2, dim3(gl), dim3(256), 0, stream(), len, V + (int64_t)(it + 1) * len, 1.0 / hn);
         }
-        if (res <= tol * beta || hn == 0) {
-          ++it;
-          break;
-        }
       }
       if (it > m) it = m;
-      // y = R^{-1} g
-      std::vector<C> y(it, 0.0);
-      for (int i = it - 1; i >= 0; --i) {
+      // y = R^{-1} g (columns with a zero diagonal, from a breakdown, are dropped)
+      int kk = it;
+      while (kk > 0 && Hm[kk - 1][kk - 1] == C(0.0)) --kk;
+      std::vector<C> y(kk, 0.0);
+      for (int i = kk - 1; i >= 0; --i) {
         C s = gv[i];
-        for (int j = i + 1; j < it; ++j) s -= Hm[i][j] * y[j];
+        for (int j = i + 1; j < kk; ++j) s -= Hm[i][j] * y[j];
         y[i] = s / Hm[i][i];
       }
-      std::vector<double2> dy(it);
-      for (int j = 0; j < it; ++j) dy[j] = mk<double>(y[j].real(), y[j].imag());
-      MNP_HIP(hipMemcpyAsync(hbuf, dy.data(), sizeof(double2) * it, hipMemcpyHostToDevice, stream()));
-      hipLaunchKernelGGL(k_mcomb, dim3(gl), dim3(256), 0, stream(), len, V, len, hbuf, it, w);
+      std::vector<double2> dy(std::max(kk, 1));
+      for (int j = 0; j < kk; ++j) dy[j] = mk<double>(y[j].real(), y[j].imag());
+      if (kk) {
+        MNP_HIP(hipMemcpyAsync(hbuf, dy.data(), sizeof(double2) * kk, hipMemcpyHostToDevice, stream()));
+        hipLaunchKernelGGL(k_mcomb, dim3(gl), dim3(256), 0, stream(), len, Z, len, hbuf, kk, w);
+      } else {
+        MNP_HIP(hipMemsetAsync(w, 0, sizeof(double2) * len, stream()));
+      }
       MNP_HIP(hipGetLastError());
     } else {
       MNP_HIP(hipMemsetAsync(w, 0, sizeof(double2) * len, stream()));
     }
     last_iter = it;
-    last_relres = beta > 0 ? res / beta : 0;
-    if (last_relres > tol) last_flag = 1;
     // true relative residual |b - A x| / |b|
     afun(p, w, Ax, prod);
     double2* r = V;  // reuse the first basis vector
@@ -543,8 +577,9 @@ struct BemRetIter {
       MNP_HIP(hipMemcpyAsync(hbuf, one.data(), sizeof(double2), hipMemcpyHostToDevice, stream()));
       hipLaunchKernelGGL(k_msub, dim3(gl), dim3(256), 0, stream(), len, Ax, len, hbuf, 1, r);
     }
-    const double bn = norm(b);
-    last_true_relres = bn > 0 ? norm(r) / bn : 0;
+    last_relres = beta > 0 ? res / beta : 0;
+    last_true_relres = beta > 0 ? norm(r) / beta : 0;
+    last_flag = last_relres > tol ? 1 : 0;
     launch_unpack(n, p, w, sig1, h1, sig2, h2);
   }
 };
@@ -560,13 +595,14 @@ struct mnp_bemretiter_s {
 
 extern "C" {
 
-int mnp_bemretiter_create(mnp_greenret_t g, const int64_t* perm, int64_t nfull, const int64_t* full,
-                          int64_t nlow, const int64_t* low, double htol, int kmax, mnp_bemretiter_t* out) {
+int mnp_bemretiter_create(mnp_greenret_t g, int precision, const int64_t* perm, int64_t nfull,
+                          const int64_t* full, int64_t nlow, const int64_t* low, double htol, int kmax,
+                          mnp_bemretiter_t* out) {
   return guarded([&] {
     MNP_CHECK(g && perm && out, MNP_ERR_ARG, "bemretiter_create: NULL argument");
     auto* h = new mnp_bemretiter_s{nullptr};
     try {
-      h->b = new BemRetIter(g->g, perm, nfull, full, nlow, low, htol, kmax);
+      h->b = new BemRetIter(g->g, precision, perm, nfull, full, nlow, low, htol, kmax);
     } catch (...) {
       delete h;
       throw;

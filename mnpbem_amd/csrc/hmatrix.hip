@@ -246,42 +246,54 @@ __global__ void k_aca_store(int64_t first, const int32_t* blk, const int64_t* so
   for (int64_t e = threadIdx.x; e < (int64_t)w * r; e += blockDim.x) R[e] = mk<TS>((TS)B[e].x, (TS)B[e].y);
 }
 
-// x (face order, FP64, n x nrhs) -> xc (cluster order, FP32)
-__global__ void k_to_cluster(int64_t n, int64_t nrhs, const int32_t* perm, const double2* x, float2* xc) {
+// x (face order, FP64, n x nrhs) -> xc (cluster order, storage precision TS)
+template <class TS>
+__global__ void k_to_cluster(int64_t n, const int32_t* perm, const double2* x, cplx<TS>* xc) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t q = blockIdx.y;
   if (i < n) {
     const double2 v = x[q * n + perm[i]];
-    xc[q * n + i] = make_float2((float)v.x, (float)v.y);
+    xc[q * n + i] = mk<TS>((TS)v.x, (TS)v.y);
   }
 }
-__global__ void k_from_cluster(int64_t n, int64_t nrhs, const int32_t* perm, const float2* yc, double2* y) {
+__global__ void k_from_cluster(int64_t n, const int32_t* perm, const double2* yc, double2* y) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t q = blockIdx.y;
-  if (i < n) {
-    const float2 v = yc[q * n + i];
-    y[q * n + perm[i]] = mk<double>(v.x, v.y);
-  }
+  if (i < n) y[q * n + perm[i]] = yc[q * n + i];
 }
 
-// t_b = R_b^T x(cols)  (r x nrhs) per low-rank block; one wave per rank index.
+// acc += a * b with a, b in TS and acc in FP64
+template <class TS>
+__device__ __forceinline__ void cfma(double2& acc, const cplx<TS> a, const cplx<TS> b) {
+  const double ar = a.x, ai = a.y, br = b.x, bi = b.y;
+  acc.x += ar * br - ai * bi;
+  acc.y += ar * bi + ai * br;
+}
+
+// t_b = R_b^T x(cols)  (r x nrhs, FP64) per low-rank block; one wave per rank index.
+template <class TS>
 __global__ void __launch_bounds__(256) k_hmv_low(const int32_t* blk, const int32_t* rank, const int64_t* loff,
-                                                 const int64_t* toff, const float2* pool, int64_t n,
-                                                 int nrhs, const float2* xc, float2* t) {
+                                                 const int64_t* toff, const cplx<TS>* pool, int64_t n,
+                                                 int nrhs, const cplx<TS>* xc, double2* t) {
   const int64_t b = blockIdx.x;
   const int r0 = blk[4 * b], r1 = blk[4 * b + 1], c0 = blk[4 * b + 2], c1 = blk[4 * b + 3];
   const int m = r1 - r0, w = c1 - c0, r = rank[b];
-  const float2* R = pool + loff[b] + (int64_t)m * r;
+  const cplx<TS>* R = pool + loff[b] + (int64_t)m * r;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int l = wv; l < r; l += 4) {
-    float2 acc[MNP_HMV_MAXRHS];
-    for (int q = 0; q < nrhs; ++q) acc[q] = make_float2(0.f, 0.f);
+    double2 acc[MNP_HMV_MAXRHS];
+#pragma unroll
+    for (int q = 0; q < MNP_HMV_MAXRHS; ++q) acc[q] = mk<double>(0, 0);
     for (int j = lane; j < w; j += 64) {
-      const float2 rv = R[(int64_t)l * w + j];
-      for (int q = 0; q < nrhs; ++q) acc[q] = cadd<float>(acc[q], cmul<float>(rv, xc[(int64_t)q * n + c0 + j]));
+      const cplx<TS> rv = R[(int64_t)l * w + j];
+#pragma unroll
+      for (int q = 0; q < MNP_HMV_MAXRHS; ++q)
+        if (q < nrhs) cfma<TS>(acc[q], rv, xc[(int64_t)q * n + c0 + j]);
     }
-    for (int q = 0; q < nrhs; ++q) {
-      float2 v = acc[q];
+#pragma unroll
+    for (int q = 0; q < MNP_HMV_MAXRHS; ++q) {
+      if (q >= nrhs) break;
+      double2 v = acc[q];
       for (int s = 32; s > 0; s >>= 1) {
         v.x += __shfl_xor(v.x, s);
         v.y += __shfl_xor(v.y, s);
@@ -293,35 +305,41 @@ __global__ void __launch_bounds__(256) k_hmv_low(const int32_t* blk, const int32
 
 // One workgroup per leaf row cluster: y(rows) = sum of full blocks B x and
 // of L_b t_b over the low-rank blocks that cover these rows.
+template <class TS>
 __global__ void __launch_bounds__(256) k_hmv_rows(const int64_t* lrow, const int64_t* loffs,
                                                   const int32_t* lentry, const int32_t* fblk,
-                                                  const int64_t* voff, const float2* fvals,
+                                                  const int64_t* voff, const cplx<TS>* fvals,
                                                   const int32_t* lblk, const int32_t* rank,
                                                   const int64_t* loff, const int64_t* toff,
-                                                  const float2* pool, const float2* t, int64_t n, int nrhs,
-                                                  const float2* xc, float2* yc) {
+                                                  const cplx<TS>* pool, const double2* t, int64_t n, int nrhs,
+                                                  const cplx<TS>* xc, double2* yc) {
   const int64_t leaf = blockIdx.x;
   const int64_t row0 = lrow[2 * leaf], row1 = lrow[2 * leaf + 1];
   const int mrows = (int)(row1 - row0);
   for (int e = threadIdx.x; e < mrows * nrhs; e += blockDim.x) {
     const int i = e % mrows, q = e / mrows;
     const int64_t row = row0 + i;
-    float2 acc = make_float2(0.f, 0.f);
+    double2 acc = mk<double>(0, 0);
     for (int64_t s = loffs[leaf]; s < loffs[leaf + 1]; ++s) {
       const int32_t ent = lentry[s];
       if (ent >= 0) {  // full block
         const int32_t* bk = fblk + 4 * ent;
         const int m = bk[1] - bk[0], c0 = bk[2], w = bk[3] - bk[2], ii = (int)(row - bk[0]);
-        const float2* V = fvals + voff[ent];
-        const float2* x = xc + (int64_t)q * n + c0;
-        for (int j = 0; j < w; ++j) acc = cadd<float>(acc, cmul<float>(V[(int64_t)j * m + ii], x[j]));
+        const cplx<TS>* V = fvals + voff[ent];
+        const cplx<TS>* x = xc + (int64_t)q * n + c0;
+        for (int j = 0; j < w; ++j) cfma<TS>(acc, V[(int64_t)j * m + ii], x[j]);
       } else {  // low-rank block -ent-1
         const int64_t lb = -(int64_t)ent - 1;
         const int32_t* bk = lblk + 4 * lb;
         const int m = bk[1] - bk[0], ii = (int)(row - bk[0]), r = rank[lb];
-        const float2* L = pool + loff[lb];
-        const float2* tt = t + toff[lb] + (int64_t)q * r;
-        for (int l = 0; l < r; ++l) acc = cadd<float>(acc, cmul<float>(L[(int64_t)l * m + ii], tt[l]));
+        const cplx<TS>* L = pool + loff[lb];
+        const double2* tt = t + toff[lb] + (int64_t)q * r;
+        for (int l = 0; l < r; ++l) {
+          const cplx<TS> a = L[(int64_t)l * m + ii];
+          const double2 b = tt[l];
+          acc.x += (double)a.x * b.x - (double)a.y * b.y;
+          acc.y += (double)a.x * b.y + (double)a.y * b.x;
+        }
       }
     }
     yc[(int64_t)q * n + row] = acc;
@@ -329,9 +347,10 @@ __global__ void __launch_bounds__(256) k_hmv_rows(const int64_t* lrow, const int
 }
 
 // ---------------------------------------------------------------- host
-HMat::HMat(const GreenRet* g_, const int64_t* perm_h, int64_t nfull_, const int64_t* full_h, int64_t nlow_,
-           const int64_t* low_h, double htol_, int kmax_)
-    : g(g_), n(g_->n), nfull(nfull_), nlow(nlow_), htol(htol_), kmax(kmax_) {
+HMat::HMat(const GreenRet* g_, int prec_, const int64_t* perm_h, int64_t nfull_, const int64_t* full_h,
+           int64_t nlow_, const int64_t* low_h, double htol_, int kmax_)
+    : g(g_), n(g_->n), nfull(nfull_), nlow(nlow_), htol(htol_), kmax(kmax_), prec(prec_) {
+  MNP_CHECK(prec == MNP_FP64 || prec == MNP_FP32, MNP_ERR_ARG, "hmat: precision must be MNP_FP64 or MNP_FP32");
   MNP_CHECK(kmax >= 1 && kmax <= 128, MNP_ERR_ARG, "hmat: kmax must be in 1..128");
   std::vector<int32_t> perm32(n), seen(n, 0);
   for (int64_t i = 0; i < n; ++i) {
@@ -439,15 +458,24 @@ HMat::~HMat() {
 }
 
 void HMat::fill(double2 k, int key_) {
+  if (prec == MNP_FP64)
+    fill_impl<double>(k, key_);
+  else
+    fill_impl<float>(k, key_);
+}
+
+template <class TS>
+void HMat::fill_impl(double2 k, int key_) {
   key = key_;
   hipStream_t s = stream();
+  const size_t es = sizeof(cplx<TS>);
   // full blocks
-  if (!fvals) fvals = dev_alloc<float2>(nfullvals);
+  if (!fvals) fvals = dev_alloc<cplx<TS>>(nfullvals);
   if (nfull) {
     const GreenRet& G = *g;
-    TimerScope ts(MNP_TIMER_ASSEMBLE, 8.0 * (double)nfullvals);
-    hipLaunchKernelGGL(k_hfull<float>, dim3((unsigned)nfull), dim3(256), 0, s, nfull, fblk, d_voff, perm, n,
-                       G.geo, G.pairs, G.off, G.nnz, G.order, G.gtab, G.ftab, k, key, fvals);
+    TimerScope ts(MNP_TIMER_ASSEMBLE, (double)es * (double)nfullvals);
+    hipLaunchKernelGGL(k_hfull<TS>, dim3((unsigned)nfull), dim3(256), 0, s, nfull, fblk, d_voff, perm, n, G.geo,
+                       G.pairs, G.off, G.nnz, G.order, G.gtab, G.ftab, k, key, (cplx<TS>*)fvals);
     MNP_HIP(hipGetLastError());
   }
   // ACA in batches whose FP64 scratch stays below a budget
@@ -460,54 +488,57 @@ void HMat::fill(double2 k, int key_) {
   }
   std::vector<int32_t> rank_h(nlow, 0);
   std::vector<int64_t> loff_h(nlow + 1, 0), toff_h(nlow + 1, 0);
-  const int64_t budget = (int64_t)(1) << 27;  // double2 entries (2 GiB)
-  int64_t* d_soff = dev_alloc<int64_t>(nlow + 1);
-  // batch boundaries
-  std::vector<int64_t> cuts{0};
-  for (int64_t b = 0; b < nlow; ++b)
-    if (soff_h[b + 1] - soff_h[cuts.back()] > budget && b > cuts.back()) cuts.push_back(b);
-  cuts.push_back(nlow);
-  int64_t maxbatch = 0;
-  for (size_t c = 0; c + 1 < cuts.size(); ++c) maxbatch = std::max(maxbatch, soff_h[cuts[c + 1]] - soff_h[cuts[c]]);
-  double2* scratch = dev_alloc<double2>(maxbatch);
-  uint32_t* used = dev_alloc<uint32_t>(maxbatch / 32 + 1);
-  std::vector<int64_t> rel(nlow + 1);
-  for (size_t c = 0; c + 1 < cuts.size(); ++c) {
-    const int64_t b0 = cuts[c], b1 = cuts[c + 1];
-    for (int64_t b = b0; b <= b1; ++b) rel[b] = soff_h[b] - soff_h[b0];
-    MNP_HIP(hipMemcpy(d_soff + b0, rel.data() + b0, sizeof(int64_t) * (b1 - b0 + 1), hipMemcpyHostToDevice));
-    {
-      TimerScope ts(MNP_TIMER_ASSEMBLE, 0.0);
-      hipLaunchKernelGGL(k_aca, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, b0, lblk, d_soff, cpos, cnvec, carea, n,
-                         k, isF, htol, kmax, scratch, rank, used);
-      MNP_HIP(hipGetLastError());
-    }
-    MNP_HIP(hipMemcpyAsync(rank_h.data() + b0, rank + b0, sizeof(int32_t) * (b1 - b0), hipMemcpyDeviceToHost, s));
-    MNP_HIP(hipStreamSynchronize(s));
-    for (int64_t b = b0; b < b1; ++b) {
-      const int64_t m = lblk_h[4 * b + 1] - lblk_h[4 * b], w = lblk_h[4 * b + 3] - lblk_h[4 * b + 2];
-      loff_h[b + 1] = loff_h[b] + (m + w) * rank_h[b];
-      toff_h[b + 1] = toff_h[b] + rank_h[b] * (int64_t)MNP_HMV_MAXRHS;
-    }
-    if (loff_h[b1] > poolcap) {  // grow the pool, keeping earlier batches
-      const int64_t cap = std::max<int64_t>(loff_h[b1] + loff_h[b1] / 2, 1 << 20);
-      float2* np = dev_alloc<float2>(cap);
-      if (pool && loff_h[b0]) MNP_HIP(hipMemcpyAsync(np, pool, sizeof(float2) * loff_h[b0], hipMemcpyDeviceToDevice, s));
+  if (nlow) {
+    const int64_t budget = (int64_t)(1) << 27;  // double2 entries (2 GiB)
+    int64_t* d_soff = dev_alloc<int64_t>(nlow + 1);
+    // batch boundaries
+    std::vector<int64_t> cuts{0};
+    for (int64_t b = 0; b < nlow; ++b)
+      if (soff_h[b + 1] - soff_h[cuts.back()] > budget && b > cuts.back()) cuts.push_back(b);
+    cuts.push_back(nlow);
+    int64_t maxbatch = 0;
+    for (size_t c = 0; c + 1 < cuts.size(); ++c)
+      maxbatch = std::max(maxbatch, soff_h[cuts[c + 1]] - soff_h[cuts[c]]);
+    double2* scratch = dev_alloc<double2>(maxbatch);
+    uint32_t* used = dev_alloc<uint32_t>(maxbatch / 32 + 1);
+    std::vector<int64_t> rel(nlow + 1);
+    for (size_t c = 0; c + 1 < cuts.size(); ++c) {
+      const int64_t b0 = cuts[c], b1 = cuts[c + 1];
+      for (int64_t b = b0; b <= b1; ++b) rel[b] = soff_h[b] - soff_h[b0];
+      MNP_HIP(hipMemcpy(d_soff + b0, rel.data() + b0, sizeof(int64_t) * (b1 - b0 + 1), hipMemcpyHostToDevice));
+      {
+        TimerScope ts(MNP_TIMER_ACA, 0.0);
+        hipLaunchKernelGGL(k_aca, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, b0, lblk, d_soff, cpos, cnvec, carea,
+                           n, k, isF, htol, kmax, scratch, rank, used);
+        MNP_HIP(hipGetLastError());
+      }
+      MNP_HIP(hipMemcpyAsync(rank_h.data() + b0, rank + b0, sizeof(int32_t) * (b1 - b0), hipMemcpyDeviceToHost, s));
       MNP_HIP(hipStreamSynchronize(s));
-      if (pool) MNP_HIP(hipFree(pool));
-      pool = np;
-      poolcap = cap;
+      for (int64_t b = b0; b < b1; ++b) {
+        const int64_t m = lblk_h[4 * b + 1] - lblk_h[4 * b], w = lblk_h[4 * b + 3] - lblk_h[4 * b + 2];
+        loff_h[b + 1] = loff_h[b] + (m + w) * rank_h[b];
+        toff_h[b + 1] = toff_h[b] + rank_h[b] * (int64_t)MNP_HMV_MAXRHS;
+      }
+      if (loff_h[b1] > poolcap) {  // grow the pool, keeping earlier batches
+        const int64_t cap = std::max<int64_t>(loff_h[b1] + loff_h[b1] / 2, 1 << 20);
+        void* np = dev_alloc<cplx<TS>>(cap);
+        if (pool && loff_h[b0]) MNP_HIP(hipMemcpyAsync(np, pool, es * loff_h[b0], hipMemcpyDeviceToDevice, s));
+        MNP_HIP(hipStreamSynchronize(s));
+        if (pool) MNP_HIP(hipFree(pool));
+        pool = np;
+        poolcap = cap;
+      }
+      MNP_HIP(hipMemcpy(d_loff + b0, loff_h.data() + b0, sizeof(int64_t) * (b1 - b0 + 1), hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_aca_store<TS>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, b0, lblk, d_soff, scratch,
+                         rank, kmax, d_loff, (cplx<TS>*)pool);
+      MNP_HIP(hipGetLastError());
+      MNP_HIP(hipStreamSynchronize(s));
     }
-    MNP_HIP(hipMemcpy(d_loff + b0, loff_h.data() + b0, sizeof(int64_t) * (b1 - b0 + 1), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_aca_store<float>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, b0, lblk, d_soff, scratch,
-                       rank, kmax, d_loff, pool);
-    MNP_HIP(hipGetLastError());
-    MNP_HIP(hipStreamSynchronize(s));
+    MNP_HIP(hipFree(scratch));
+    MNP_HIP(hipFree(used));
+    MNP_HIP(hipFree(d_soff));
   }
   MNP_HIP(hipMemcpy(d_toff, toff_h.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice));
-  MNP_HIP(hipFree(scratch));
-  MNP_HIP(hipFree(used));
-  MNP_HIP(hipFree(d_soff));
   nlowvals = loff_h[nlow];
   maxrank = 0;
   for (int32_t r : rank_h) maxrank = std::max(maxrank, (int)r);
@@ -517,22 +548,33 @@ void HMat::fill(double2 k, int key_) {
 }
 
 void HMat::matvec(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const {
+  if (prec == MNP_FP64)
+    matvec_impl<double>(nrhs, x, y, ws);
+  else
+    matvec_impl<float>(nrhs, x, y, ws);
+}
+
+template <class TS>
+void HMat::matvec_impl(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const {
   MNP_CHECK(filled, MNP_ERR_STATE, "hmat: matvec before fill");
   MNP_CHECK(nrhs >= 1, MNP_ERR_ARG, "hmat: nrhs < 1");
   hipStream_t s = stream();
+  TimerScope ts(MNP_TIMER_HMATVEC, (double)sizeof(cplx<TS>) * (double)(nfullvals + nlowvals) *
+                                       (double)((nrhs + MNP_HMV_MAXRHS - 1) / MNP_HMV_MAXRHS));
   for (int64_t q0 = 0; q0 < nrhs; q0 += MNP_HMV_MAXRHS) {
     const int nq = (int)std::min<int64_t>(MNP_HMV_MAXRHS, nrhs - q0);
-    float2* xc = (float2*)ws.get(0, sizeof(float2) * n * nq);
-    float2* yc = (float2*)ws.get(1, sizeof(float2) * n * nq);
-    float2* t = (float2*)ws.get(2, sizeof(float2) * std::max<int64_t>(tlen, 1));
+    cplx<TS>* xc = (cplx<TS>*)ws.get(0, sizeof(cplx<TS>) * n * nq);
+    double2* yc = (double2*)ws.get(1, sizeof(double2) * n * nq);
+    double2* t = (double2*)ws.get(2, sizeof(double2) * std::max<int64_t>(tlen, 1));
     const dim3 gv((unsigned)cdiv_up(n, 256), (unsigned)nq);
-    hipLaunchKernelGGL(k_to_cluster, gv, dim3(256), 0, s, n, nq, perm, x + q0 * n, xc);
+    hipLaunchKernelGGL(k_to_cluster<TS>, gv, dim3(256), 0, s, n, perm, x + q0 * n, xc);
     if (nlow)
-      hipLaunchKernelGGL(k_hmv_low, dim3((unsigned)nlow), dim3(256), 0, s, lblk, rank, d_loff, d_toff, pool, n, nq,
-                         xc, t);
-    hipLaunchKernelGGL(k_hmv_rows, dim3((unsigned)nleaf), dim3(256), 0, s, d_lrow, d_loffs, d_lentry, fblk, d_voff,
-                       fvals, lblk, rank, d_loff, d_toff, pool, t, n, nq, xc, yc);
-    hipLaunchKernelGGL(k_from_cluster, gv, dim3(256), 0, s, n, nq, perm, yc, y + q0 * n);
+      hipLaunchKernelGGL(k_hmv_low<TS>, dim3((unsigned)nlow), dim3(256), 0, s, lblk, rank, d_loff, d_toff,
+                         (const cplx<TS>*)pool, n, nq, xc, t);
+    hipLaunchKernelGGL(k_hmv_rows<TS>, dim3((unsigned)nleaf), dim3(256), 0, s, d_lrow, d_loffs, d_lentry, fblk,
+                       d_voff, (const cplx<TS>*)fvals, lblk, rank, d_loff, d_toff, (const cplx<TS>*)pool, t, n, nq,
+                       xc, yc);
+    hipLaunchKernelGGL(k_from_cluster, gv, dim3(256), 0, s, n, perm, yc, y + q0 * n);
     MNP_HIP(hipGetLastError());
   }
 }
@@ -549,14 +591,14 @@ struct mnp_hmat_s {
 
 extern "C" {
 
-int mnp_hmat_create(mnp_greenret_t g, const int64_t* perm, int64_t nfull, const int64_t* full, int64_t nlow,
-                    const int64_t* low, double htol, int kmax, mnp_hmat_t* out) {
+int mnp_hmat_create(mnp_greenret_t g, int precision, const int64_t* perm, int64_t nfull, const int64_t* full,
+                    int64_t nlow, const int64_t* low, double htol, int kmax, mnp_hmat_t* out) {
   return guarded([&] {
     MNP_CHECK(g && perm && out && (full || !nfull) && (low || !nlow), MNP_ERR_ARG, "hmat_create: NULL argument");
     MNP_CHECK(htol > 0, MNP_ERR_ARG, "hmat_create: htol must be positive");
     auto* h = new mnp_hmat_s{nullptr, {}};
     try {
-      h->h = new HMat(g->g, perm, nfull, full, nlow, low, htol, kmax);
+      h->h = new HMat(g->g, precision, perm, nfull, full, nlow, low, htol, kmax);
     } catch (...) {
       delete h;
       throw;

@@ -14,6 +14,7 @@ struct HMat {
   int64_t n, nfull, nlow;
   double htol;
   int kmax;
+  int prec;  // MNP_FP64 or MNP_FP32 storage of blocks and factors
   int key = -1;
   bool filled = false;
   // block structure: [row_begin, row_end, col_begin, col_end) in cluster order
@@ -25,10 +26,10 @@ struct HMat {
   int32_t *perm = nullptr, *fblk = nullptr, *lblk = nullptr, *rank = nullptr, *d_lentry = nullptr;
   int64_t *d_voff = nullptr, *d_lrow = nullptr, *d_loffs = nullptr, *d_loff = nullptr, *d_toff = nullptr;
   double *cpos = nullptr, *cnvec = nullptr, *carea = nullptr;
-  float2* fvals = nullptr;  // full blocks, column-major per block, complex FP32
-  float2* pool = nullptr;   // low-rank factors L (m x r) then R (n x r) per block, complex FP32
+  void* fvals = nullptr;  // full blocks, column-major per block, complex (prec)
+  void* pool = nullptr;   // low-rank factors L (m x r) then R (n x r) per block, complex (prec)
 
-  HMat(const GreenRet* g, const int64_t* perm, int64_t nfull, const int64_t* full, int64_t nlow,
+  HMat(const GreenRet* g, int prec, const int64_t* perm, int64_t nfull, const int64_t* full, int64_t nlow,
        const int64_t* low, double htol, int kmax);
   ~HMat();
   HMat(const HMat&) = delete;
@@ -37,6 +38,12 @@ struct HMat {
   void fill(double2 k, int key);
   // y = H x, x and y complex FP64 in face order (n x nrhs).
   void matvec(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const;
+
+ private:
+  template <class TS>
+  void fill_impl(double2 k, int key);
+  template <class TS>
+  void matvec_impl(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const;
 };
 
 }  // namespace mnp

@@ -109,12 +109,14 @@ def block_ranges(tree, blocks):
 class hmatrix:
     """H-matrix of one Green-function operator on the GPU (G, F, H1 or H2).
 
-    ``hmatrix(p, tree, htol=1e-6, kmax=100)`` builds the block tree; ``fill``
-    evaluates full blocks with refinement and compresses admissible blocks
-    by ACA (Greenfun/+aca/@compgreenret/eval.m); ``matvec`` multiplies.
+    ``hmatrix(green, tree, htol=1e-6, kmax=100)`` builds the block tree;
+    ``fill`` evaluates full blocks with refinement and compresses admissible
+    blocks by ACA (Greenfun/+aca/@compgreenret/eval.m); ``matvec``
+    multiplies.  ``precision`` ('fp64' or 'fp32') is the storage of blocks
+    and factors.
     """
 
-    def __init__(self, green, tree, htol=1e-6, kmax=100, fadmiss=None):
+    def __init__(self, green, tree, htol=1e-6, kmax=100, fadmiss=None, precision="fp64"):
         import ctypes as C
         from . import _lib
         self.green, self.tree = green, tree
@@ -123,13 +125,17 @@ class hmatrix:
         self.low = np.ascontiguousarray(block_ranges(tree, low).T)
         perm = np.ascontiguousarray(tree.perm, dtype=np.int64)
         h = C.c_void_p()
-        _lib.call("mnp_hmat_create", green.handle, perm.ctypes.data_as(C.c_void_p), len(self.full),
+        prec = {"fp64": _lib.MNP_FP64, "fp32": _lib.MNP_FP32}[precision]
+        _lib.call("mnp_hmat_create", green.handle, prec, perm.ctypes.data_as(C.c_void_p), len(self.full),
                   self.full.ctypes.data_as(C.c_void_p), len(self.low),
                   self.low.ctypes.data_as(C.c_void_p), float(htol), int(kmax), C.byref(h))
         self.handle = h
 
     def __del__(self):
-        from . import _lib
+        try:
+            from . import _lib
+        except ImportError:  # interpreter shutdown
+            return
         h = getattr(self, "handle", None)
         if h is not None and h.value and _lib._lib is not None:
             _lib.lib().mnp_hmat_destroy(h)

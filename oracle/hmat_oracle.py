@@ -120,3 +120,24 @@ def aca(getrow, getcol, m, n, tol=1e-6, kmax=100, acatol=1e-10):
         k += 1
     rank = min(k + 1, kmax)
     return A[:, :rank], B[:, :rank]
+
+
+def hmat_dense(green, tree, blocks_low, k, key, htol=1e-6, kmax=100):
+    """The H-matrix of operator ``key`` as a dense (n, n) array in face order.
+
+    Full blocks hold the refined dense values (green.eval, as @hmatrix/fillval.m
+    fills leaf blocks through greenret eval2); each admissible block
+    (row node, col node) in ``blocks_low`` holds the ACA approximant L R^T of
+    the collocation kernel (aca.cpp:158-218, no refinement, no 2 pi term).
+    """
+    dense = green.eval(k, (key,))[key].copy()
+    perm = tree["perm"] if isinstance(tree, dict) else tree.perm
+    cbeg = tree["cbeg"] if isinstance(tree, dict) else tree.cbeg
+    cend = tree["cend"] if isinstance(tree, dict) else tree.cend
+    kk = "G" if key == "G" else "F"
+    for i1, i2 in blocks_low:
+        rows, cols = perm[cbeg[i1]:cend[i1]], perm[cbeg[i2]:cend[i2]]
+        M = kernel_block(green.pos, green.nvec, green.area, rows, cols, k, kk)
+        L, R = aca(lambda r: M[r], lambda c: M[:, c], len(rows), len(cols), tol=htol, kmax=kmax)
+        dense[np.ix_(rows, cols)] = L @ R.T
+    return dense

@@ -335,8 +335,12 @@ class BemRet:
     outside medium eps2).  BEM/@bemret/private/initmat.m, mldivide.m.
     """
 
-    def __init__(self, green):
+    def __init__(self, green, evalfun=None):
+        """``evalfun(k, keys)`` -> dict of dense matrices; defaults to green.eval.
+        Passing the H-matrix operators here gives the direct solution of the
+        system that bemretiter iterates on."""
         self.g = green
+        self.evalfun = evalfun or green.eval
 
     def initmat(self, enei, eps1, eps2):
         """BEM/@bemret/private/initmat.m:5-72 (scalar eps1, eps2; con{1,2}==0)."""
@@ -345,8 +349,8 @@ class BemRet:
         k = 2 * np.pi / enei
         k1 = 2 * np.pi / enei * np.sqrt(complex(eps1))
         k2 = 2 * np.pi / enei * np.sqrt(complex(eps2))
-        m1 = g.eval(k1, ("G", "H1"))
-        m2 = g.eval(k2, ("G", "H2"))
+        m1 = self.evalfun(k1, ("G", "H1"))
+        m2 = self.evalfun(k2, ("G", "H2"))
         G1, H1 = m1["G"], m1["H1"]
         G2, H2 = m2["G"], m2["H2"]
         G1i = np.linalg.inv(G1)

@@ -91,3 +91,24 @@ def test_host_tree_equals_oracle():
         full, low = admissibility(t)
         rf, rl = H.block_tree(ref)
         assert sorted(full) == sorted(rf) and sorted(low) == sorted(rl)
+
+
+def test_hmat_dense_blocks():
+    """hmat_dense: refined values in full blocks, ACA of the collocation kernel
+    (no refinement, no 2 pi) in admissible blocks."""
+    v, f = shapes.ellipsoid(2, (10.0, 10.0, 25.0))
+    g = O.GreenRet(v, f)
+    t = H.cluster_tree(g.pos, cleaf=16)
+    full, low = H.block_tree(t, 2.5)
+    k = 0.1 + 0.02j
+    A = H.hmat_dense(g, t, low, k, "H1", htol=1e-8)
+    D = g.eval(k, ("H1",))["H1"]
+    perm, cb, ce = t["perm"], t["cbeg"], t["cend"]
+    for i1, i2 in full:
+        r, c = perm[cb[i1]:ce[i1]], perm[cb[i2]:ce[i2]]
+        assert np.array_equal(A[np.ix_(r, c)], D[np.ix_(r, c)])
+    assert low
+    for i1, i2 in low:
+        r, c = perm[cb[i1]:ce[i1]], perm[cb[i2]:ce[i2]]
+        M = H.kernel_block(g.pos, g.nvec, g.area, r, c, k, "F")
+        assert np.linalg.norm(A[np.ix_(r, c)] - M) <= 1e-6 * np.linalg.norm(M)
