@@ -271,6 +271,42 @@ int mnp_planewave_cross_sections(mnp_greenret_t g, double enei, mnp_complex eps_
                                  const void* pinfty_area, const void* sig2, const void* h2,
                                  double* ext, double* sca, double* abs_);
 
+/* ------------------------------------------------ dipoles and point fields
+ * Dipoles at points pts (host npt x 3, column-major) with moments dip (host
+ * npt x 3 x ndip); column q = ipt + npt * idip of the outputs, as the
+ * reference's (n, npt, ndip) arrays.  eps_medium is the permittivity of the
+ * medium holding the points.
+ *
+ * Replaces @dipoleret/potential.m:64-122 for the side of the particle that
+ * faces the dipoles: phi, phip device out n x npol, a, ap device out
+ * n x 3 x npol (complex128); pass them as phi1.. or phi2.. to the solver. */
+int mnp_dipole_potential(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t npt,
+                         const double* pts, int64_t ndip, const double* dip, void* phi, void* phip,
+                         void* a, void* ap);
+/* Fields at points from the surface charges sig (device n x ncol) and
+ * currents h (device n x 3 x ncol) of the particle side facing the points:
+ * e = i k0 G h - Gp sig, hfield = Gp x h, device out npt x 3 x ncol
+ * (@compgreenret/field.m:14-40 with compgreenret(pt, p)).  Point-face pairs
+ * within the refinement cutoff of the greenret options (refinematrix.m,
+ * the point taking the face radius) use the rule-18 Taylor form
+ * (greenret/private/init.m:96-177).  Points must not coincide with
+ * quadrature points of a face. */
+int mnp_field_points(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t npt,
+                     const double* pts, int64_t ncol, const void* sig, const void* h, void* e,
+                     void* hfield);
+/* Total and radiative decay rates in units of the free-space rate and the
+ * free-space rate nb 4/3 k0^3 (host out npt x ndip) for dipoles outside the
+ * particle, from sig2/h2 of the solve (device n x npol, n x 3 x npol):
+ * @dipoleret/decayrate.m:13-62 with the radiated power of the surface far
+ * field plus the dipole far field over the pinfty surface (device
+ * npinf x 3 normals, npinf areas; farfield.m:40-55, scattering.m).  The
+ * medium must be lossless (the reference warns otherwise); real parts are
+ * returned.  rad0 may be NULL. */
+int mnp_dipole_decayrate(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t npt,
+                         const double* pts, int64_t ndip, const double* dip, int64_t npinf,
+                         const void* pinfty_nvec, const void* pinfty_area, const void* sig2,
+                         const void* h2, double* tot, double* rad, double* rad0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,9 @@ PROTOTYPES = {
     "mnp_bemretiter_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
     "mnp_bemretiter_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mnp_bemretiter_stats": (INT, [P, C.POINTER(INT), PD, C.POINTER(INT), PD, PD]),
+    "mnp_dipole_potential": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P, P]),
+    "mnp_field_points": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P]),
+    "mnp_dipole_decayrate": (INT, [P, D, MnpComplex, I64, P, I64, P, I64, P, P, P, P, P, P, P]),
     "mnp_planewave_potential": (INT, [P, D, MnpComplex, I64, P, P, P, P]),
     "mnp_farfield": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P]),
     "mnp_planewave_cross_sections": (INT, [P, D, MnpComplex, I64, P, P, I64, P, P, P, P,

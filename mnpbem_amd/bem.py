@@ -370,3 +370,98 @@ def _flat_normals(verts, faces):
     nv = np.cross(v1 - v2, v3 - v2)
     nn = np.sqrt(np.sum(nv * nv, axis=1))
     return nv / nn[:, None], 0.5 * nn
+
+
+class dipoleret:
+    """Dipole excitation (Simulation/retarded/@dipoleret) for dipoles outside the particle.
+
+    ``pt`` (npt x 3) are the positions; ``dip`` is ndip x 3 (the same
+    orientations at every position, default eye(3)) or, with ``full=True``,
+    npt x 3 x ndip (init.m:11-46).  ``medium`` is the 1-based index of the
+    medium holding the dipoles (default: the particle's outside).
+    Columns of the potentials and solutions are ipt + npt * idip.
+    """
+
+    def __init__(self, pt, dip=None, op=None, medium=None, pinfty=None, full=False):
+        self.pt = np.atleast_2d(np.asarray(pt, dtype=np.float64))
+        npt = len(self.pt)
+        if dip is None:
+            dip = np.eye(3)
+        dip = np.asarray(dip, dtype=np.float64)
+        if full:
+            self.dip = dip.reshape(npt, 3, -1)
+        else:
+            d = np.atleast_2d(dip)
+            self.dip = np.repeat(d.T[None, :, :], npt, axis=0)
+        self.medium = medium
+        if pinfty is None:
+            from . import shapes
+            v, f = shapes.trisphere256(2.0)
+            nvec, area = _flat_normals(v, f)
+        else:
+            nvec, area = pinfty
+        dev.ensure()
+        self.pinf_nvec = dev.to_device_colmajor(nvec, torch.float64)
+        self.pinf_area = torch.as_tensor(np.asarray(area, np.float64), device="cuda")
+
+    @property
+    def npol(self):
+        return self.dip.shape[0] * self.dip.shape[2]
+
+    def _medium(self, p):
+        med = p.inout[0, 1] if self.medium is None else self.medium
+        if med != p.inout[0, 1]:
+            raise NotImplementedError("dipoles must lie in the particle's outside medium")
+        return med
+
+    def _eps_med(self, p, enei):
+        return p.eps[self._medium(p) - 1](enei)[0]
+
+    def _host(self):
+        pts = np.asfortranarray(self.pt)
+        dip = np.asfortranarray(self.dip)
+        return pts, dip
+
+    def __call__(self, p, enei):
+        """exc(p, enei): phi2, phi2p, a2, a2p on the faces (potential.m:64-122)."""
+        n, npol = p.n, self.npol
+        out = dict(phi2=dev.colmajor((n, npol)), phi2p=dev.colmajor((n, npol)),
+                   a2=dev.colmajor((n, 3, npol)), a2p=dev.colmajor((n, 3, npol)))
+        pts, dip = self._host()
+        _lib.call("mnp_dipole_potential", p.green.handle, float(enei),
+                  _lib.MnpComplex.of(self._eps_med(p, enei)), len(pts), pts.ctypes.data_as(C.c_void_p),
+                  dip.shape[2], dip.ctypes.data_as(C.c_void_p),
+                  *[dev.ptr(dev.base(out[k])) for k in ("phi2", "phi2p", "a2", "a2p")])
+        return compstruct(p, enei, **out)
+
+    potential = __call__
+
+    def field(self, sig):
+        """Induced e, h (npt x 3 x npol) at the dipole positions (compgreenret/field.m)."""
+        p, enei = sig.p, sig.enei
+        npt, npol = len(self.pt), sig.sig2.shape[1]
+        e = dev.colmajor((npt, 3, npol))
+        h = dev.colmajor((npt, 3, npol))
+        pts, _ = self._host()
+        _lib.call("mnp_field_points", p.green.handle, float(enei), _lib.MnpComplex.of(self._eps_med(p, enei)),
+                  npt, pts.ctypes.data_as(C.c_void_p), npol, dev.ptr(dev.base(sig.sig2)),
+                  dev.ptr(dev.base(sig.h2)), dev.ptr(dev.base(e)), dev.ptr(dev.base(h)))
+        return compstruct(p, enei, e=e, h=h)
+
+    def decayrate(self, sig):
+        """[tot, rad, rad0] = decayrate(exc, sig), each npt x ndip (decayrate.m:13-62)."""
+        p, enei = sig.p, sig.enei
+        npt, ndip = self.dip.shape[0], self.dip.shape[2]
+        out = [np.zeros((npt, ndip), order="F") for _ in range(3)]
+        pts, dip = self._host()
+        _lib.call("mnp_dipole_decayrate", p.green.handle, float(enei),
+                  _lib.MnpComplex.of(self._eps_med(p, enei)), npt, pts.ctypes.data_as(C.c_void_p), ndip,
+                  dip.ctypes.data_as(C.c_void_p), self.pinf_area.numel(), dev.ptr(dev.base(self.pinf_nvec)),
+                  dev.ptr(self.pinf_area), dev.ptr(dev.base(sig.sig2)), dev.ptr(dev.base(sig.h2)),
+                  *[o.ctypes.data_as(C.c_void_p) for o in out])
+        return tuple(out)
+
+
+def dipole(pt, dip=None, op=None, **kw):
+    """Base/dipole.m for op.sim == 'ret'."""
+    return dipoleret(pt, dip, op, **kw)

@@ -596,7 +596,7 @@ static double2 medium_k(double enei, mnp_complex em) {
   return mk<double>(k0 * sr, k0 * si);
 }
 
-static void farfield(const GreenRet& g, double enei, mnp_complex em, int64_t ndir, const double* dir,
+void farfield(const GreenRet& g, double enei, mnp_complex em, int64_t ndir, const double* dir,
                      int64_t p, const double2* sig2, const double2* h2, double2* e, double2* h) {
   MNP_CHECK(p >= 1 && p <= FF_MAXP, MNP_ERR_ARG, "farfield: npol must be in 1..8");
   const double2 k = medium_k(enei, em);

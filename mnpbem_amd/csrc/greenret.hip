@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "mnp_greenret.hpp"
+#include "mnp_nearfield.hpp"
 
 namespace mnp {
 
@@ -77,7 +78,7 @@ TimerScope::~TimerScope() {
 __constant__ double c_q18[3][28];
 __constant__ double c_qpol[3][MNP_MAX_POLAR];
 
-static void rule18(double* x, double* y, double* w) {
+void rule18(double* x, double* y, double* w) {
   const double a = 1.0 / 3.0, b = 0.9480217181434233, c = 0.02598914092828833,
                d = 0.8114249947041546, e = 0.09428750264792270, f = 0.01072644996557060,
                g = 0.4946367750172147, p = 0.5853132347709715, q = 0.2073433826145142,
@@ -395,30 +396,7 @@ __global__ void __launch_bounds__(256) k_tables(int64_t nnz, Geo geo, int64_t n,
       for (int c = 0; c < 3; ++c) f[c][o] = nu[c] * a + t1[c] * b + t2[c] * c2;
     }
   } else {
-    double pj[3];
-    for (int c = 0; c < 3; ++c) pj[c] = geo.pos[c * n + j];
-    const double r0 = sqrt((pj[0] - pi3[0]) * (pj[0] - pi3[0]) + (pj[1] - pi3[1]) * (pj[1] - pi3[1]) +
-                           (pj[2] - pi3[2]) * (pj[2] - pi3[2]));
-    const double qa = geo.qarea[j];
-    for (int q = 0; q < 28; ++q) {
-      const double x = c_q18[0][q], y = c_q18[1][q], z = 1 - x - y, w = c_q18[2][q] * qa;
-      double xv[3];
-      for (int c = 0; c < 3; ++c) xv[c] = pi3[c] - (x * vj[0][c] + y * vj[1][c] + z * vj[2][c]);
-      const double r = sqrt(xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2]);
-      const double dr = r - r0, ir = 1.0 / r, ir2 = ir * ir, ir3 = ir2 * ir;
-      double drp = 1.0;  // dr^o
-      for (int o = 0; o <= order; ++o) {
-        g[o] += drp * ir / fact[o] * w;
-        drp *= dr;
-      }
-      for (int c = 0; c < 3; ++c) f[c][0] -= xv[c] * (w * ir3);
-      double drm = 1.0;  // dr^(o-1)
-      for (int o = 1; o <= order; ++o) {
-        const double fac = -(drm * dr) * ir3 / fact[o] + drm * ir2 / fact[o - 1];
-        for (int c = 0; c < 3; ++c) f[c][o] += xv[c] * (fac * w);
-        drm *= dr;
-      }
-    }
+    near_taylor(pi3, geo, n, j, order, fact, &c_q18[0][0], g, f);
   }
   for (int o = 0; o <= order; ++o) {
     gt[o * nnz + p] = g[o];

@@ -1,0 +1,173 @@
+"""Pins for the dipole oracle: each check is fixed by the mathematics, not by
+retyping the oracle's formulas (Lorenz gauge, Jackson's dipole field,
+finite differences, free-space limits, a completeness identity, Mie)."""
+import numpy as np
+
+from mnpbem_amd import shapes
+from oracle import dipole_oracle as D
+from oracle import mnpbem_oracle as O
+
+ENEI, EPS = 600.0, 1.7 + 0.05j
+
+
+def _dipole_field_jackson(x, pt, p, enei, eps):
+    """Jackson (9.18) in Gaussian units inside a medium eps:
+    E = k^2 (n x p) x n e^{ikr}/r / eps + [3 n (n.p) - p](1/r^3 - i k/r^2) e^{ikr} / eps."""
+    k = 2 * np.pi / enei * np.sqrt(eps)
+    r_vec = x - pt
+    r = np.linalg.norm(r_vec, axis=1)
+    nn = r_vec / r[:, None]
+    e1 = np.cross(np.cross(nn, p), nn) * (k ** 2 * np.exp(1j * k * r) / r)[:, None]
+    e2 = (3 * nn * (nn @ p)[:, None] - p) * ((1 / r ** 3 - 1j * k / r ** 2) * np.exp(1j * k * r))[:, None]
+    return (e1 + e2) / eps
+
+
+def _pot(x, nvec, pt, p):
+    phi, phip, a, ap = D.dipole_potential(x, nvec, pt[None, :], p[None, :, None], ENEI, EPS)
+    return phi[:, 0], phip[:, 0], a[:, :, 0], ap[:, :, 0]
+
+
+def test_dipole_potentials_gauge_field_and_normal_derivatives():
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(20, 3)) * 8.0
+    x[np.linalg.norm(x, axis=1) < 3] += 5.0
+    nvec = rng.normal(size=(20, 3))
+    nvec /= np.linalg.norm(nvec, axis=1)[:, None]
+    pt, p = np.array([0.3, -0.2, 0.1]), np.array([0.2, 0.7, -0.4])
+    k0 = 2 * np.pi / ENEI
+    h = 1e-4
+    phi, phip, a, ap = _pot(x, nvec, pt, p)
+    grad_phi = np.zeros((20, 3), complex)
+    div_a = np.zeros(20, complex)
+    for c in range(3):
+        dx = np.zeros(3)
+        dx[c] = h
+        fp, _, ap_, _ = _pot(x + dx, nvec, pt, p)
+        fm, _, am_, _ = _pot(x - dx, nvec, pt, p)
+        grad_phi[:, c] = (fp - fm) / (2 * h)
+        div_a += (ap_[:, c] - am_[:, c]) / (2 * h)
+    # Lorenz gauge of MNPBEM's potentials: div A = i k0 eps phi
+    assert np.allclose(div_a, 1j * k0 * EPS * phi, rtol=1e-6, atol=1e-9 * np.abs(phi).max())
+    # E = i k0 A - grad phi is the dipole field
+    e = 1j * k0 * a - grad_phi
+    want = _dipole_field_jackson(x, pt, p, ENEI, EPS)
+    assert np.allclose(e, want, rtol=1e-6, atol=1e-8 * np.abs(want).max())
+    # phi' and A' are the derivatives along nvec
+    fp, _, ap_, _ = _pot(x + h * nvec, nvec, pt, p)
+    fm, _, am_, _ = _pot(x - h * nvec, nvec, pt, p)
+    assert np.allclose(phip, (fp - fm) / (2 * h), rtol=1e-6, atol=1e-9 * np.abs(phip).max())
+    assert np.allclose(ap, (ap_ - am_) / (2 * h), rtol=1e-6, atol=1e-9 * np.abs(ap).max())
+
+
+def test_point_green_refined_equals_face_quadrature():
+    """Forcing refinement on every face, the Taylor form equals the rule-18
+    integral of e^{ikr}/r (and of its gradient) over each face."""
+    v, f = shapes.icosphere(2, 10.0)
+    g = O.GreenRet(v, f)
+    pts = np.array([[0.0, 0.0, 11.0], [3.0, -4.0, 9.5]])
+    k = 0.02 + 0.003j
+    G, Gp = D.point_green(g, pts, k, relcutoff=1e12)
+    q, w = O.quad_flat(g.verts, g.faces, np.arange(g.n))
+    xv = pts[:, None, None, :] - q[None]
+    r = np.sqrt(np.sum(xv * xv, axis=3))
+    Gq = np.sum(w[None] * np.exp(1j * k * r) / r, axis=2)
+    fq = (1j * k - 1 / r) * np.exp(1j * k * r) / r ** 2 * w[None]
+    Gpq = np.transpose(np.sum(xv * fq[..., None], axis=2), (0, 2, 1))
+    assert np.allclose(G, Gq, rtol=1e-9)
+    assert np.allclose(Gp, Gpq, rtol=1e-8, atol=1e-12 * np.abs(Gpq).max())
+    # unrefined values are the collocation kernel
+    G0, Gp0 = D.point_green(g, pts, k, relcutoff=-1e12, abscutoff=-1e12)
+    x = pts[:, None, :] - g.pos[None]
+    d = np.linalg.norm(x, axis=2)
+    assert np.allclose(G0, np.exp(1j * k * d) / d * g.area[None])
+    # far from the surface the two agree to quadrature accuracy
+    assert np.allclose(G0[:, :5], Gq[:, :5], rtol=1e-2)
+
+
+def test_field_points_is_ik0A_minus_grad_phi_and_curl_A():
+    v, f = shapes.icosphere(1, 10.0)
+    g = O.GreenRet(v, f)
+    rng = np.random.default_rng(2)
+    sig = rng.normal(size=(g.n, 2)) + 1j * rng.normal(size=(g.n, 2))
+    h = rng.normal(size=(g.n, 3, 2)) + 1j * rng.normal(size=(g.n, 3, 2))
+    pt = np.array([[2.0, 25.0, -3.0]])
+    k, enei = 0.03 + 0.001j, 500.0
+    k0 = 2 * np.pi / enei
+    e, hf = D.field_points(g, pt, sig, h, k, enei)
+    step = 1e-4
+
+    def pots(p):
+        G, _ = D.point_green(g, p, k)
+        return G @ sig, np.einsum("pj,jcq->pcq", G, h)
+
+    phi0, A0 = pots(pt)
+    grad = np.zeros((1, 3, 2), complex)
+    dA = np.zeros((3, 3, 2), complex)  # dA[c] = d A / d x_c
+    for c in range(3):
+        dx = np.zeros(3)
+        dx[c] = step
+        pp, Ap = pots(pt + dx)
+        pm, Am = pots(pt - dx)
+        grad[0, c] = (pp - pm)[0] / (2 * step)
+        dA[c] = (Ap - Am)[0] / (2 * step)
+    assert np.allclose(e, 1j * k0 * A0 - grad, rtol=1e-6)
+    curl = np.stack([dA[1, 2] - dA[2, 1], dA[2, 0] - dA[0, 2], dA[0, 1] - dA[1, 0]])
+    assert np.allclose(hf[0], curl, rtol=1e-6)
+
+
+def test_free_dipole_rates_are_one(pinfty):
+    """Without a particle (sig = 0) tot = 1 and rad = 1 up to the faceting of
+    the integration sphere: the flat trisphere(256) has 0.988 of the unit
+    sphere's area, and the dipole pattern averages to that ratio."""
+    v, f = shapes.icosphere(1, 10.0)
+    g = O.GreenRet(v, f)
+    pn, pa = pinfty
+    pts = np.array([[0.0, 0.0, 15.0], [4.0, 1.0, -14.0]])
+    dip = np.stack([np.eye(3)[:, 0], np.array([0.6, 0.0, 0.8])], axis=1)[None].repeat(2, axis=0)
+    npol = 4
+    z2 = np.zeros((g.n, npol), complex)
+    z3 = np.zeros((g.n, 3, npol), complex)
+    zf = np.zeros((len(pn), 3, npol), complex)
+    tot, rad, rad0 = D.decayrate(g, pts, dip, 550.0, 1.0, z2, z3, zf, zf, pn, pa)
+    assert np.allclose(tot, 1.0)
+    assert np.allclose(rad, pa.sum() / (4 * np.pi), rtol=2e-3)
+    assert np.allclose(rad0, 4 / 3 * (2 * np.pi / 550.0) ** 3)
+
+
+def test_mie_decayrate_limits():
+    # eps_in == eps_out: no sphere; completeness gives rad = 1
+    tot, rad = D.mie_decayrate(600.0, 1.0, 1.0, 20.0, [15.0, 40.0])
+    assert np.allclose(tot, 1.0) and np.allclose(rad, 1.0, rtol=1e-10)
+    # far from a gold sphere both rates tend to one (the expansion about the
+    # sphere centre needs l beyond k z, so lmax is raised with z)
+    eps = O.eps_drude(600.0, 10.0, 9.065, 0.0708)
+    tot, rad = D.mie_decayrate(600.0, eps, 1.0, 20.0, [500.0])
+    assert np.allclose(tot, 1.0, atol=2e-4) and np.allclose(rad, 1.0, atol=2e-4)
+    tot, rad = D.mie_decayrate(600.0, eps, 1.0, 20.0, [5000.0], lmax=100)
+    assert np.allclose(tot, 1.0, atol=1e-5) and np.allclose(rad, 1.0, atol=1e-5)
+    # close to it the total rate is enhanced
+    tot, rad = D.mie_decayrate(600.0, eps, 1.0, 20.0, [15.0])
+    assert np.all(tot > 2.0)
+
+
+def test_bem_decayrate_converges_to_mie(pinfty):
+    """Dense oracle BEM with the dipole excitation on a sphere vs Mie."""
+    pn, pa = pinfty
+    enei, diameter = 520.0, 20.0
+    eps_in = O.eps_drude(enei, 10.0, 9.065, 0.0708)
+    z = 15.0
+    pts = np.array([[0.0, 0.0, z]])
+    dip = np.eye(3)[None][:, :, [0, 2]]                 # x and z orientations
+    tm, rm = D.mie_decayrate(enei, eps_in, 1.0, diameter, [z])
+    errs = []
+    for level in (2, 3):
+        v, f = shapes.icosphere(level, diameter / 2)
+        g = O.GreenRet(v, f)
+        phi2, phi2p, a2, a2p = D.dipole_potential(g.pos, g.nvec, pts, dip, enei, 1.0)
+        bem = O.BemRet(g).initmat(enei, eps_in, 1.0)
+        sig = bem.solve({"phi2": phi2, "phi2p": phi2p, "a2": a2, "a2p": a2p})
+        ef, hf, _ = O.farfield(g.pos, g.area, sig["sig2"], sig["h2"], pn, enei, 1.0)
+        tot, rad, _ = D.decayrate(g, pts, dip, enei, 1.0, sig["sig2"], sig["h2"], ef, hf, pn, pa)
+        errs.append((np.abs(tot[0] / tm[0] - 1).max(), np.abs(rad[0] / rm[0] - 1).max()))
+    assert errs[1][0] < 0.05 and errs[1][1] < 0.05, errs
+    assert errs[1][0] < errs[0][0]
