@@ -9,10 +9,12 @@
 //  - k_aca:   one workgroup per admissible block runs the partially pivoted
 //    ACA of mex/hlib/aca.cpp:158-218 on the collocation kernel of
 //    mex/acagreen/acagreen.cpp:79-127, in FP64, rank <= min(kmax, m, n);
-//  - k_hmv_low / k_hmv_rows: y = H x in complex FP32: t_b = R_b^T x per
-//    low-rank block, then one workgroup per leaf row cluster sums its full
-//    blocks and L_b t_b of every low-rank block covering its rows
-//    (deterministic, no atomics).
+//  - k_hmv_low / k_hmv_tile: y = H x with blocks and factors stored in FP64
+//    or FP32 and FP64 accumulation: t_b = R_b^T x per low-rank block (R
+//    stored transposed, read contiguously), then one workgroup per row tile
+//    (<= 32 rows of a leaf) sums its slices of the full blocks times x and
+//    the tile's contiguous L_b segment times t_b over the low-rank blocks
+//    covering its rows (deterministic, no atomics).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -230,119 +232,247 @@ __global__ void __launch_bounds__(256) k_aca(int64_t first, const int32_t* blk, 
   if (tid == 0) rank[b] = min(k + 1, kb);
 }
 
-// Compacts the ACA scratch of block b into L (m x r), R (n x r) (FP32 or FP64).
+// Compacts the ACA scratch of block b into its factors (FP32 or FP64):
+// L (m x r) as consecutive row-tile segments, each (rows of the tile) x r
+// column-major, so that the product kernel reads one contiguous segment per
+// tile; then R^T (r x n, column-major: the r entries of each column index
+// contiguous).  rowtile[c] = tile of cluster row c, tilebeg[t] = first row.
 template <class TS>
 __global__ void k_aca_store(int64_t first, const int32_t* blk, const int64_t* soff, const double2* scratch,
-                            const int32_t* rank, int kmax, const int64_t* loff, cplx<TS>* pool) {
+                            const int32_t* rank, int kmax, const int64_t* loff, const int32_t* rowtile,
+                            const int32_t* tilebeg, cplx<TS>* pool) {
   const int64_t b = first + blockIdx.x;
-  const int m = blk[4 * b + 1] - blk[4 * b], w = blk[4 * b + 3] - blk[4 * b + 2];
+  const int r0 = blk[4 * b];
+  const int m = blk[4 * b + 1] - r0, w = blk[4 * b + 3] - blk[4 * b + 2];
   const int kb = min(kmax, min(m, w));
   const int r = rank[b];
   const double2* A = scratch + soff[b];
   const double2* B = A + (int64_t)m * kb;
   cplx<TS>* L = pool + loff[b];
-  cplx<TS>* R = L + (int64_t)m * r;
-  for (int64_t e = threadIdx.x; e < (int64_t)m * r; e += blockDim.x) L[e] = mk<TS>((TS)A[e].x, (TS)A[e].y);
-  for (int64_t e = threadIdx.x; e < (int64_t)w * r; e += blockDim.x) R[e] = mk<TS>((TS)B[e].x, (TS)B[e].y);
+  cplx<TS>* RT = L + (int64_t)m * r;
+  for (int64_t e = threadIdx.x; e < (int64_t)m * r; e += blockDim.x) {
+    const int i = (int)(e % m), l = (int)(e / m);
+    const int row = r0 + i, t = rowtile[row], tb = tilebeg[t], ms = tilebeg[t + 1] - tb;
+    L[(int64_t)(tb - r0) * r + (int64_t)l * ms + (row - tb)] = mk<TS>((TS)A[e].x, (TS)A[e].y);
+  }
+  for (int64_t e = threadIdx.x; e < (int64_t)w * r; e += blockDim.x) {
+    const int j = (int)(e % w), l = (int)(e / w);
+    RT[(int64_t)j * r + l] = mk<TS>((TS)B[e].x, (TS)B[e].y);
+  }
 }
 
-// x (face order, FP64, n x nrhs) -> xc (cluster order, storage precision TS)
+// x (face order, FP64, n x nrhs) -> xc (cluster order, storage precision
+// TS, right-hand sides interleaved: xc[c * nrhs + q])
 template <class TS>
-__global__ void k_to_cluster(int64_t n, const int32_t* perm, const double2* x, cplx<TS>* xc) {
+__global__ void k_to_cluster(int64_t n, int nrhs, const int32_t* perm, const double2* x, cplx<TS>* xc) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t q = blockIdx.y;
   if (i < n) {
     const double2 v = x[q * n + perm[i]];
-    xc[q * n + i] = mk<TS>((TS)v.x, (TS)v.y);
+    xc[i * nrhs + q] = mk<TS>((TS)v.x, (TS)v.y);
   }
 }
-__global__ void k_from_cluster(int64_t n, const int32_t* perm, const double2* yc, double2* y) {
+__global__ void k_from_cluster(int64_t n, int nrhs, const int32_t* perm, const double2* yc, double2* y) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t q = blockIdx.y;
-  if (i < n) y[q * n + perm[i]] = yc[q * n + i];
+  if (i < n) y[q * n + perm[i]] = yc[i * nrhs + q];
 }
 
 // acc += a * b with a, b in TS and acc in FP64
-template <class TS>
-__device__ __forceinline__ void cfma(double2& acc, const cplx<TS> a, const cplx<TS> b) {
+template <class TS, class TB>
+__device__ __forceinline__ void cfma(double2& acc, const cplx<TS> a, const TB b) {
   const double ar = a.x, ai = a.y, br = b.x, bi = b.y;
   acc.x += ar * br - ai * bi;
   acc.y += ar * bi + ai * br;
 }
 
-// t_b = R_b^T x(cols)  (r x nrhs, FP64) per low-rank block; one wave per rank index.
+// acc[q] += a * row[q], q < nrhs (row: the interleaved right-hand sides of one index)
+template <class TS, class TB>
+__device__ __forceinline__ void row_fma(double2 (&acc)[MNP_HMV_MAXRHS], const cplx<TS> a, const TB* row, int nrhs) {
+#pragma unroll
+  for (int q = 0; q < MNP_HMV_MAXRHS; ++q)
+    if (q < nrhs) cfma<TS>(acc[q], a, row[q]);
+}
+
+constexpr int HMV_UNROLL = 8;  // independent loads in flight per lane
+constexpr int HMV_CHUNK = 256;  // columns of R^T per wave in the first phase
+constexpr int HMV_KC = 64;      // right-hand-side rows staged in LDS per wave
+
+// Stages rows [c0, c0 + kc) of the interleaved right-hand sides B (row j at
+// B[j * nrhs], FP64 or FP32) into this wave's LDS buffer as FP64, with
+// coalesced loads; LDS operations of one wave complete in order, so the
+// wave's later reads see these writes.
+template <class TB>
+__device__ __forceinline__ void stage_rows(double2* bs, const TB* B, int c0, int kc, int nrhs, int lane) {
+  const TB* src = B + (int64_t)c0 * nrhs;
+  for (int e = lane; e < kc * nrhs; e += 64) {
+    const TB v = src[e];
+    bs[e] = mk<double>(v.x, v.y);
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+// acc[q] += a * bs[q], q < nrhs, bs in LDS (broadcast reads)
 template <class TS>
-__global__ void __launch_bounds__(256) k_hmv_low(const int32_t* blk, const int32_t* rank, const int64_t* loff,
-                                                 const int64_t* toff, const cplx<TS>* pool, int64_t n,
-                                                 int nrhs, const cplx<TS>* xc, double2* t) {
-  const int64_t b = blockIdx.x;
-  const int r0 = blk[4 * b], r1 = blk[4 * b + 1], c0 = blk[4 * b + 2], c1 = blk[4 * b + 3];
-  const int m = r1 - r0, w = c1 - c0, r = rank[b];
-  const cplx<TS>* R = pool + loff[b] + (int64_t)m * r;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int l = wv; l < r; l += 4) {
+__device__ __forceinline__ void lds_fma(double2 (&acc)[MNP_HMV_MAXRHS], const cplx<TS> a, const double2* bs,
+                                        int nrhs) {
+  const double ar = a.x, ai = a.y;
+#pragma unroll
+  for (int q = 0; q < MNP_HMV_MAXRHS; ++q)
+    if (q < nrhs) {
+      const double2 b = bs[q];
+      acc[q].x += ar * b.x - ai * b.y;
+      acc[q].y += ar * b.y + ai * b.x;
+    }
+}
+
+// First phase, t_b = R_b^T x(cols) (r x nrhs, FP64, t[l * nrhs + q]): one
+// wave per item = (block, <= HMV_CHUNK consecutive columns).  Lane =
+// (l, jj) = (lane % RP, lane / RP), RP = the rank rounded up to 16, 32 or
+// 64 (ranks above 64 take several passes), so each load instruction reads
+// 64 / RP consecutive columns of R^T, contiguous in memory; x is staged in
+// LDS HMV_KC columns at a time and the partial sums over jj are combined by
+// shuffles.  Items of blocks with one item write t_b, the others write
+// partial sums that k_hmv_low_sum adds in item order.
+// item[4 i ..] = {block, first column, end column, partial-sum slot or -1}.
+template <class TS>
+__global__ void __launch_bounds__(256) k_hmv_low(int64_t nitems, const int32_t* item, const int64_t* tpoff,
+                                                 const int32_t* blk, const int32_t* rank, const int64_t* loff,
+                                                 const int64_t* toff, const cplx<TS>* pool, int nrhs,
+                                                 const cplx<TS>* xc, double2* t, double2* tpart) {
+  __shared__ double2 bsh[4][HMV_KC * MNP_HMV_MAXRHS];
+  const int64_t it = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (it >= nitems) return;
+  const int lane = threadIdx.x & 63;
+  double2* bs = bsh[threadIdx.x >> 6];
+  const int b = item[4 * it], j0 = item[4 * it + 1], j1 = item[4 * it + 2], slot = item[4 * it + 3];
+  const int r0 = blk[4 * b], m = blk[4 * b + 1] - r0, c0 = blk[4 * b + 2];
+  const int r = rank[b];
+  const int RP = r <= 16 ? 16 : r <= 32 ? 32 : 64;
+  const int JS = 64 / RP;
+  const cplx<TS>* RT = pool + loff[b] + (int64_t)m * r;
+  const cplx<TS>* xb = xc + (int64_t)c0 * nrhs;
+  double2* dst = slot < 0 ? t + toff[b] : tpart + tpoff[it];
+  for (int lb = 0; lb < r; lb += 64) {
+    const int l = lb + lane % RP, jj = lane / RP;
     double2 acc[MNP_HMV_MAXRHS];
 #pragma unroll
     for (int q = 0; q < MNP_HMV_MAXRHS; ++q) acc[q] = mk<double>(0, 0);
-    for (int j = lane; j < w; j += 64) {
-      const cplx<TS> rv = R[(int64_t)l * w + j];
+    for (int k0 = j0; k0 < j1; k0 += HMV_KC) {
+      const int kc = min(HMV_KC, j1 - k0);
+      stage_rows(bs, xb, k0, kc, nrhs, lane);
+      if (l < r) {
+        int j = jj;
+        for (; j + (HMV_UNROLL - 1) * JS < kc; j += HMV_UNROLL * JS) {
+          cplx<TS> a[HMV_UNROLL];
+#pragma unroll
+          for (int u = 0; u < HMV_UNROLL; ++u) a[u] = RT[(int64_t)(k0 + j + u * JS) * r + l];
+#pragma unroll
+          for (int u = 0; u < HMV_UNROLL; ++u) lds_fma<TS>(acc, a[u], bs + (j + u * JS) * nrhs, nrhs);
+        }
+        for (; j < kc; j += JS) lds_fma<TS>(acc, RT[(int64_t)(k0 + j) * r + l], bs + j * nrhs, nrhs);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    for (int sh = RP; sh < 64; sh <<= 1)
+#pragma unroll
+      for (int q = 0; q < MNP_HMV_MAXRHS; ++q) {
+        acc[q].x += __shfl_xor(acc[q].x, sh);
+        acc[q].y += __shfl_xor(acc[q].y, sh);
+      }
+    if (jj == 0 && l < r)
 #pragma unroll
       for (int q = 0; q < MNP_HMV_MAXRHS; ++q)
-        if (q < nrhs) cfma<TS>(acc[q], rv, xc[(int64_t)q * n + c0 + j]);
-    }
-#pragma unroll
-    for (int q = 0; q < MNP_HMV_MAXRHS; ++q) {
-      if (q >= nrhs) break;
-      double2 v = acc[q];
-      for (int s = 32; s > 0; s >>= 1) {
-        v.x += __shfl_xor(v.x, s);
-        v.y += __shfl_xor(v.y, s);
-      }
-      if (lane == 0) t[toff[b] + (int64_t)q * r + l] = v;
-    }
+        if (q < nrhs) dst[(int64_t)l * nrhs + q] = acc[q];
   }
 }
 
-// One workgroup per leaf row cluster: y(rows) = sum of full blocks B x and
-// of L_b t_b over the low-rank blocks that cover these rows.
-template <class TS>
-__global__ void __launch_bounds__(256) k_hmv_rows(const int64_t* lrow, const int64_t* loffs,
-                                                  const int32_t* lentry, const int32_t* fblk,
-                                                  const int64_t* voff, const cplx<TS>* fvals,
-                                                  const int32_t* lblk, const int32_t* rank,
-                                                  const int64_t* loff, const int64_t* toff,
-                                                  const cplx<TS>* pool, const double2* t, int64_t n, int nrhs,
-                                                  const cplx<TS>* xc, double2* yc) {
-  const int64_t leaf = blockIdx.x;
-  const int64_t row0 = lrow[2 * leaf], row1 = lrow[2 * leaf + 1];
-  const int mrows = (int)(row1 - row0);
-  for (int e = threadIdx.x; e < mrows * nrhs; e += blockDim.x) {
-    const int i = e % mrows, q = e / mrows;
-    const int64_t row = row0 + i;
-    double2 acc = mk<double>(0, 0);
-    for (int64_t s = loffs[leaf]; s < loffs[leaf + 1]; ++s) {
-      const int32_t ent = lentry[s];
-      if (ent >= 0) {  // full block
-        const int32_t* bk = fblk + 4 * ent;
-        const int m = bk[1] - bk[0], c0 = bk[2], w = bk[3] - bk[2], ii = (int)(row - bk[0]);
-        const cplx<TS>* V = fvals + voff[ent];
-        const cplx<TS>* x = xc + (int64_t)q * n + c0;
-        for (int j = 0; j < w; ++j) cfma<TS>(acc, V[(int64_t)j * m + ii], x[j]);
-      } else {  // low-rank block -ent-1
-        const int64_t lb = -(int64_t)ent - 1;
-        const int32_t* bk = lblk + 4 * lb;
-        const int m = bk[1] - bk[0], ii = (int)(row - bk[0]), r = rank[lb];
-        const cplx<TS>* L = pool + loff[lb];
-        const double2* tt = t + toff[lb] + (int64_t)q * r;
-        for (int l = 0; l < r; ++l) {
-          const cplx<TS> a = L[(int64_t)l * m + ii];
-          const double2 b = tt[l];
-          acc.x += (double)a.x * b.x - (double)a.y * b.y;
-          acc.y += (double)a.x * b.y + (double)a.y * b.x;
-        }
+// t_b = sum of the partial sums of its items, in item order.
+// multi[3 i ..] = {block, first item, number of items}.
+__global__ void k_hmv_low_sum(int64_t nmulti, const int32_t* multi, const int64_t* tpoff, const int32_t* rank,
+                              const int64_t* toff, int nrhs, const double2* tpart, double2* t) {
+  const int64_t i = blockIdx.x;
+  if (i >= nmulti) return;
+  const int b = multi[3 * i], i0 = multi[3 * i + 1], ni = multi[3 * i + 2];
+  const int len = rank[b] * nrhs;
+  for (int e = threadIdx.x; e < len; e += blockDim.x) {
+    double2 v = tpart[tpoff[i0] + e];
+    for (int k = 1; k < ni; ++k) v = cadd<double>(v, tpart[tpoff[i0 + k] + e]);
+    t[toff[b] + e] = v;
+  }
+}
+
+// Second phase: one workgroup per row tile (<= 32 rows of one leaf) sums
+// the products of its descriptor list, one descriptor per full block (the
+// tile's rows of V times x(cols)) or low-rank block (the tile's contiguous
+// L segment times t_b): A at aoff (rows at stride lda, K columns); boff >= 0
+// is the first column of x, boff < 0 encodes toff as -boff - 1.  The four
+// waves take descriptors round-robin; each stages the descriptor's
+// right-hand-side rows in LDS (HMV_KC at a time) so that the matrix stream
+// is the only per-column global load; lane = (row, kk), kk splitting the
+// columns into even/odd, so one load instruction reads two 32-row columns,
+// HMV_UNROLL of them in flight.
+template <class TS, class TB>
+__device__ __forceinline__ void tile_product(double2 (&acc)[MNP_HMV_MAXRHS], double2* bs, const cplx<TS>* A,
+                                             int64_t lda, int K, const TB* B, int lane, int kk, bool active,
+                                             int nrhs) {
+  for (int k0 = 0; k0 < K; k0 += HMV_KC) {
+    const int kc = min(HMV_KC, K - k0);
+    stage_rows(bs, B, k0, kc, nrhs, lane);
+    if (active) {
+      const cplx<TS>* Ak = A + (int64_t)k0 * lda;
+      int j = kk;
+      for (; j + 2 * (HMV_UNROLL - 1) < kc; j += 2 * HMV_UNROLL) {
+        cplx<TS> a[HMV_UNROLL];
+#pragma unroll
+        for (int u = 0; u < HMV_UNROLL; ++u) a[u] = Ak[(int64_t)(j + 2 * u) * lda];
+#pragma unroll
+        for (int u = 0; u < HMV_UNROLL; ++u) lds_fma<TS>(acc, a[u], bs + (j + 2 * u) * nrhs, nrhs);
       }
+      for (; j < kc; j += 2) lds_fma<TS>(acc, Ak[(int64_t)j * lda], bs + j * nrhs, nrhs);
     }
-    yc[(int64_t)q * n + row] = acc;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+template <class TS>
+__global__ void __launch_bounds__(256) k_hmv_tile(const int32_t* tilebeg, const int32_t* tdbeg, const int64_t* taoff,
+                                                  const int64_t* tboff, const int32_t* tlda, const int32_t* tK,
+                                                  const cplx<TS>* fvals, const cplx<TS>* pool, const double2* t,
+                                                  int nrhs, const cplx<TS>* xc, double2* yc) {
+  __shared__ double2 red[4][32][MNP_HMV_MAXRHS];
+  __shared__ double2 bsh[4][HMV_KC * MNP_HMV_MAXRHS];
+  const int tile = blockIdx.x;
+  const int tb = tilebeg[tile], mt = tilebeg[tile + 1] - tb;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, row = lane & 31, kk = lane >> 5;
+  const bool active = row < mt;
+  double2* bs = bsh[wv];
+  double2 acc[MNP_HMV_MAXRHS];
+#pragma unroll
+  for (int q = 0; q < MNP_HMV_MAXRHS; ++q) acc[q] = mk<double>(0, 0);
+  const int s1 = tdbeg[tile + 1];
+  for (int s = tdbeg[tile] + wv; s < s1; s += 4) {
+    const int64_t aoff = taoff[s], boff = tboff[s];
+    const int lda = tlda[s], K = tK[s];
+    if (boff >= 0)
+      tile_product<TS>(acc, bs, fvals + aoff + row, lda, K, xc + boff * nrhs, lane, kk, active, nrhs);
+    else
+      tile_product<TS>(acc, bs, pool + aoff + row, lda, K, t + (-boff - 1), lane, kk, active, nrhs);
+  }
+#pragma unroll
+  for (int q = 0; q < MNP_HMV_MAXRHS; ++q) {
+    acc[q].x += __shfl_xor(acc[q].x, 32);
+    acc[q].y += __shfl_xor(acc[q].y, 32);
+  }
+  if (kk == 0)
+#pragma unroll
+    for (int q = 0; q < MNP_HMV_MAXRHS; ++q) red[wv][row][q] = acc[q];
+  __syncthreads();
+  for (int e = threadIdx.x; e < mt * nrhs; e += blockDim.x) {
+    const int i = e / nrhs, q = e % nrhs;
+    double2 v = red[0][i][q];
+    for (int w2 = 1; w2 < 4; ++w2) v = cadd<double>(v, red[w2][i][q]);
+    yc[(int64_t)(tb + i) * nrhs + q] = v;
   }
 }
 
@@ -417,6 +547,24 @@ HMat::HMat(const GreenRet* g_, int prec_, const int64_t* perm_h, int64_t nfull_,
     loffs[l + 1] = (int64_t)lentry.size();
   }
   lrow_h = lrow;
+  // row tiles: each leaf cut into near-equal runs of at most 32 rows
+  std::vector<int32_t> tilebeg{0}, tileleaf, rowtile(n);
+  for (int64_t l = 0; l < nleaf; ++l) {
+    const int64_t a = lrow[2 * l], m = lrow[2 * l + 1] - a, nt = (m + 31) / 32;
+    MNP_CHECK(a == tilebeg.back(), MNP_ERR_INTERNAL, "hmat: leaf clusters are not contiguous");
+    for (int64_t c = 0; c < nt; ++c) {
+      const int64_t t0 = a + c * m / nt, t1 = a + (c + 1) * m / nt;
+      for (int64_t i = t0; i < t1; ++i) rowtile[i] = (int32_t)tileleaf.size();
+      tilebeg.push_back((int32_t)t1);
+      tileleaf.push_back((int32_t)l);
+    }
+  }
+  ntile = (int64_t)tileleaf.size();
+  tilebeg_h = tilebeg;
+  tileleaf_h = tileleaf;
+  lentry_h = lentry;
+  loffs_h = loffs;
+  voff_h = voff;
   // device copies
   auto up = [&](auto& vec, auto*& dptr) {
     using E = typename std::remove_reference<decltype(vec)>::type::value_type;
@@ -428,6 +576,9 @@ HMat::HMat(const GreenRet* g_, int prec_, const int64_t* perm_h, int64_t nfull_,
   up(lblk_h, lblk);
   up(voff, d_voff);
   up(lrow, d_lrow);
+  up(tilebeg, d_tilebeg);
+  up(tileleaf, d_tileleaf);
+  up(rowtile, d_rowtile);
   up(loffs, d_loffs);
   up(lentry, d_lentry);
   // cluster-ordered geometry for ACA
@@ -451,7 +602,9 @@ HMat::HMat(const GreenRet* g_, int prec_, const int64_t* perm_h, int64_t nfull_,
 }
 
 HMat::~HMat() {
-  void* ps[] = {perm, fblk, lblk, d_voff, d_lrow, d_loffs, d_lentry, cpos, cnvec, carea, rank,
+  void* ps[] = {d_items, d_itpoff, d_multi, d_tdbeg, d_taoff, d_tboff, d_tlda, d_tK,
+                perm, fblk, lblk, d_voff, d_lrow, d_loffs, d_lentry, d_tilebeg, d_tileleaf, d_rowtile,
+                cpos, cnvec, carea, rank,
                 d_loff, d_toff, fvals, pool};
   for (void* p : ps)
     if (p) (void)hipFree(p);
@@ -530,7 +683,7 @@ void HMat::fill_impl(double2 k, int key_) {
       }
       MNP_HIP(hipMemcpy(d_loff + b0, loff_h.data() + b0, sizeof(int64_t) * (b1 - b0 + 1), hipMemcpyHostToDevice));
       hipLaunchKernelGGL(k_aca_store<TS>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, b0, lblk, d_soff, scratch,
-                         rank, kmax, d_loff, (cplx<TS>*)pool);
+                         rank, kmax, d_loff, d_rowtile, d_tilebeg, (cplx<TS>*)pool);
       MNP_HIP(hipGetLastError());
       MNP_HIP(hipStreamSynchronize(s));
     }
@@ -544,7 +697,72 @@ void HMat::fill_impl(double2 k, int key_) {
   for (int32_t r : rank_h) maxrank = std::max(maxrank, (int)r);
   rank_host = rank_h;
   tlen = toff_h[nlow];
+  build_plan(loff_h, toff_h);
   filled = true;
+}
+
+// Product plan for the ranks of the last fill: first-phase items and the
+// row tiles' descriptor lists (see k_hmv_low and k_hmv_tile).
+void HMat::build_plan(const std::vector<int64_t>& loff_h, const std::vector<int64_t>& toff_h) {
+  for (void** p : {(void**)&d_items, (void**)&d_itpoff, (void**)&d_multi, (void**)&d_tdbeg, (void**)&d_taoff,
+                   (void**)&d_tboff, (void**)&d_tlda, (void**)&d_tK})
+    if (*p) {
+      MNP_HIP(hipFree(*p));
+      *p = nullptr;
+    }
+  std::vector<int32_t> items, multi;
+  std::vector<int64_t> itpoff;
+  int64_t tp = 0;
+  for (int64_t b = 0; b < nlow; ++b) {
+    const int32_t c0 = lblk_h[4 * b + 2], w = lblk_h[4 * b + 3] - c0;
+    const int32_t ni = (w + HMV_CHUNK - 1) / HMV_CHUNK;
+    if (ni > 1) multi.insert(multi.end(), {(int32_t)b, (int32_t)(items.size() / 4), ni});
+    for (int32_t c = 0; c < ni; ++c) {
+      const int32_t j0 = c * w / ni, j1 = (c + 1) * w / ni;
+      items.insert(items.end(), {(int32_t)b, j0, j1, ni > 1 ? (int32_t)c : -1});
+      itpoff.push_back(ni > 1 ? tp : -1);
+      if (ni > 1) tp += (int64_t)rank_host[b] * MNP_HMV_MAXRHS;
+    }
+  }
+  nitems = (int64_t)items.size() / 4;
+  nmulti = (int64_t)multi.size() / 3;
+  tpartlen = tp;
+  std::vector<int32_t> tdbeg{0}, tlda, tK;
+  std::vector<int64_t> taoff, tboff;
+  for (int64_t t = 0; t < ntile; ++t) {
+    const int32_t tb = tilebeg_h[t], mt = tilebeg_h[t + 1] - tb, leaf = tileleaf_h[t];
+    for (int64_t s = loffs_h[leaf]; s < loffs_h[leaf + 1]; ++s) {
+      const int32_t ent = lentry_h[s];
+      if (ent >= 0) {
+        const int32_t* bk = &fblk_h[4 * ent];
+        taoff.push_back(voff_h[ent] + (tb - bk[0]));
+        tboff.push_back(bk[2]);
+        tlda.push_back(bk[1] - bk[0]);
+        tK.push_back(bk[3] - bk[2]);
+      } else {
+        const int64_t lb = -(int64_t)ent - 1;
+        const int32_t r = rank_host[lb];
+        taoff.push_back(loff_h[lb] + (int64_t)(tb - lblk_h[4 * lb]) * r);
+        tboff.push_back(-toff_h[lb] - 1);
+        tlda.push_back(mt);
+        tK.push_back(r);
+      }
+    }
+    tdbeg.push_back((int32_t)taoff.size());
+  }
+  auto up = [&](auto& vec, auto*& dptr) {
+    using E = typename std::remove_reference<decltype(vec)>::type::value_type;
+    dptr = dev_alloc<E>((int64_t)vec.size());
+    if (!vec.empty()) MNP_HIP(hipMemcpy(dptr, vec.data(), sizeof(E) * vec.size(), hipMemcpyHostToDevice));
+  };
+  up(items, d_items);
+  up(itpoff, d_itpoff);
+  up(multi, d_multi);
+  up(tdbeg, d_tdbeg);
+  up(taoff, d_taoff);
+  up(tboff, d_tboff);
+  up(tlda, d_tlda);
+  up(tK, d_tK);
 }
 
 void HMat::matvec(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const {
@@ -566,15 +784,18 @@ void HMat::matvec_impl(int64_t nrhs, const double2* x, double2* y, Workspace& ws
     cplx<TS>* xc = (cplx<TS>*)ws.get(0, sizeof(cplx<TS>) * n * nq);
     double2* yc = (double2*)ws.get(1, sizeof(double2) * n * nq);
     double2* t = (double2*)ws.get(2, sizeof(double2) * std::max<int64_t>(tlen, 1));
+    double2* tpart = (double2*)ws.get(3, sizeof(double2) * std::max<int64_t>(tpartlen, 1));
     const dim3 gv((unsigned)cdiv_up(n, 256), (unsigned)nq);
-    hipLaunchKernelGGL(k_to_cluster<TS>, gv, dim3(256), 0, s, n, perm, x + q0 * n, xc);
-    if (nlow)
-      hipLaunchKernelGGL(k_hmv_low<TS>, dim3((unsigned)nlow), dim3(256), 0, s, lblk, rank, d_loff, d_toff,
-                         (const cplx<TS>*)pool, n, nq, xc, t);
-    hipLaunchKernelGGL(k_hmv_rows<TS>, dim3((unsigned)nleaf), dim3(256), 0, s, d_lrow, d_loffs, d_lentry, fblk,
-                       d_voff, (const cplx<TS>*)fvals, lblk, rank, d_loff, d_toff, (const cplx<TS>*)pool, t, n, nq,
-                       xc, yc);
-    hipLaunchKernelGGL(k_from_cluster, gv, dim3(256), 0, s, n, perm, yc, y + q0 * n);
+    hipLaunchKernelGGL(k_to_cluster<TS>, gv, dim3(256), 0, s, n, nq, perm, x + q0 * n, xc);
+    if (nitems)
+      hipLaunchKernelGGL(k_hmv_low<TS>, dim3((unsigned)cdiv_up(nitems, 4)), dim3(256), 0, s, nitems, d_items,
+                         d_itpoff, lblk, rank, d_loff, d_toff, (const cplx<TS>*)pool, nq, xc, t, tpart);
+    if (nmulti)
+      hipLaunchKernelGGL(k_hmv_low_sum, dim3((unsigned)nmulti), dim3(128), 0, s, nmulti, d_multi, d_itpoff, rank,
+                         d_toff, nq, tpart, t);
+    hipLaunchKernelGGL(k_hmv_tile<TS>, dim3((unsigned)ntile), dim3(256), 0, s, d_tilebeg, d_tdbeg, d_taoff, d_tboff,
+                       d_tlda, d_tK, (const cplx<TS>*)fvals, (const cplx<TS>*)pool, t, nq, xc, yc);
+    hipLaunchKernelGGL(k_from_cluster, gv, dim3(256), 0, s, n, nq, perm, yc, y + q0 * n);
     MNP_HIP(hipGetLastError());
   }
 }

@@ -21,13 +21,20 @@ struct HMat {
   std::vector<int32_t> fblk_h, lblk_h, diag_h;
   std::vector<int64_t> lrow_h;
   std::vector<int32_t> rank_host;
-  int64_t nleaf = 0, nfullvals = 0, nlowvals = 0, poolcap = 0, tlen = 0;
+  int64_t nleaf = 0, ntile = 0, nfullvals = 0, nlowvals = 0, poolcap = 0, tlen = 0;
   int maxrank = 0;
   int32_t *perm = nullptr, *fblk = nullptr, *lblk = nullptr, *rank = nullptr, *d_lentry = nullptr;
+  int32_t *d_tilebeg = nullptr, *d_tileleaf = nullptr, *d_rowtile = nullptr;  // row tiles of <= 32 rows
   int64_t *d_voff = nullptr, *d_lrow = nullptr, *d_loffs = nullptr, *d_loff = nullptr, *d_toff = nullptr;
   double *cpos = nullptr, *cnvec = nullptr, *carea = nullptr;
+  // product plan (rebuilt by each fill): first-phase items, row-tile descriptors
+  std::vector<int32_t> lentry_h, tilebeg_h, tileleaf_h;
+  std::vector<int64_t> loffs_h, voff_h;
+  int64_t nitems = 0, nmulti = 0, tpartlen = 0;
+  int32_t *d_items = nullptr, *d_multi = nullptr, *d_tdbeg = nullptr, *d_tlda = nullptr, *d_tK = nullptr;
+  int64_t *d_itpoff = nullptr, *d_taoff = nullptr, *d_tboff = nullptr;
   void* fvals = nullptr;  // full blocks, column-major per block, complex (prec)
-  void* pool = nullptr;   // low-rank factors L (m x r) then R (n x r) per block, complex (prec)
+  void* pool = nullptr;   // per low-rank block: L (m x r, row-tile segments) then R^T (r x n), complex (prec)
 
   HMat(const GreenRet* g, int prec, const int64_t* perm, int64_t nfull, const int64_t* full, int64_t nlow,
        const int64_t* low, double htol, int kmax);
@@ -40,6 +47,7 @@ struct HMat {
   void matvec(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const;
 
  private:
+  void build_plan(const std::vector<int64_t>& loff_h, const std::vector<int64_t>& toff_h);
   template <class TS>
   void fill_impl(double2 k, int key);
   template <class TS>
