@@ -8,6 +8,10 @@ waves polarised x (propagating z) and z (propagating x), wavelengths from
 bemret chain, the solve for both polarisations and the FP64 extinction and
 scattering cross sections, with every input resident on the GPU.
 
+The JSON line also carries `hmatrix_dipole`: the H-matrix iterative path
+with dipole excitation on an 81,920-face sphere (16 wavelengths, decay
+rates), with the H-matrix product's achieved bandwidth.
+
 `value` is the mixed-precision path (complex FP32 chain on MFMA used as the
 preconditioner of the exact FP64 operator, refined to a relative residual of
 1e-12).  The FP64 drop-in is timed on `--fp64-steps` wavelengths for the
@@ -87,10 +91,13 @@ def run_gpu(level, steps, warmup, fp64_steps):
     del mixed
     torch.cuda.empty_cache()
 
-    fp64 = B.bemsolver(p, op, precision="fp64")
     lam64 = enei[warmup:warmup + fp64_steps]
-    res_64, t_64, tm64 = spectrum(fp64, lam64, True)
-    log(f"[bench] fp64 {len(lam64)} wavelengths in {t_64:.2f}s")
+    res_64, t_64, tm64 = [], 0.0, None
+    if fp64_steps:
+        fp64 = B.bemsolver(p, op, precision="fp64")
+        res_64, t_64, tm64 = spectrum(fp64, lam64, True)
+        log(f"[bench] fp64 {len(lam64)} wavelengths in {t_64:.2f}s")
+        del fp64
     errs = []
     for a, b in zip(res_mx, res_64):
         for q in range(2):
@@ -98,6 +105,66 @@ def run_gpu(level, steps, warmup, fp64_steps):
     max_err = float(np.max(errs)) if errs else None
     return dict(n=n, t_mx=t_mx, t_64=t_64, n64=len(lam64), timers=tm, timers64=tm64,
                 max_err=max_err, sweeps=sweeps, ext=[list(map(float, r[0])) for r in res_mx])
+
+
+def run_config4(nlam=16, seed=2024):
+    """H-matrix iterative path with dipole excitation: sphere of diameter 150 nm
+    (icosphere level 6, 81,920 faces), Drude gold in eps=1, dipole 5 nm above
+    the surface along a seeded random direction with a seeded random moment,
+    cleaf 32, eta 2.5, htol 1e-6, kmax [4, 100], GMRES tol 1e-6 with FP32
+    H-matrix storage, 16 wavelengths from 500 to 700 nm; total and radiative
+    decay rates."""
+    import torch
+
+    from mnpbem_amd import _lib, bem as B, shapes
+
+    v, f = shapes.icosphere(6, 75.0)
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    dip = rng.normal(size=3)
+    dip /= np.linalg.norm(dip)
+    op = B.bemoptions(sim="ret", iter=dict(cleaf=32, htol=1e-6, kmax=[4, 100], tol=1e-6, hprecision="fp32"))
+    p = B.comparticle([B.epsconst(1.0), B.epsdrude(*DRUDE)], [(v, f)], [2, 1], 1, op)
+    t0 = time.perf_counter()
+    p.green
+    it = B.bemsolver(p, op)
+    exc = B.dipoleret(80.0 * u[None], dip[None])
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    lams = np.linspace(500.0, 700.0, nlam)
+    _lib.call("mnp_timers_enable", 1)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    tot, rad, iters = [], [], []
+    for lam in lams:
+        sig = it.solve(exc(p, lam))
+        a, b, _ = exc.decayrate(sig)
+        tot.append(float(a[0, 0]))
+        rad.append(float(b[0, 0]))
+        iters.append(it.stats()["iter"])
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    tm = _lib.Timers()
+    _lib.call("mnp_timers_read", C.byref(tm))
+    _lib.call("mnp_timers_enable", 0)
+    hv = 4  # MNP_TIMER_HMATVEC
+    mv_ms = tm.ms[hv] / max(tm.launches[hv], 1)
+    mv_bytes = tm.work[hv] / max(tm.launches[hv], 1)
+    gbs = mv_bytes / (mv_ms * 1e-3) / 1e9 if mv_ms > 0 else 0.0
+    log(f"[bench] config 4: {nlam} wavelengths in {el:.2f}s (setup {setup:.2f}s), iterations {iters}")
+    return {
+        "workload": "sphere d=150 nm, icosphere level 6 (81920 faces), Drude gold in eps=1, dipole 5 nm "
+                    "from the surface (seeded), bemretiter cleaf 32, htol 1e-6, GMRES tol 1e-6, FP32 "
+                    "H-matrix storage, 16 wavelengths 500-700 nm, total and radiative decay rates",
+        "wavelengths_per_s": nlam / el, "ms_per_wavelength": 1e3 * el / nlam, "setup_s": setup,
+        "gmres_iterations": iters, "compression": it.stats()["compression"],
+        "tot": tot, "rad": rad,
+        "hmatvec": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0,
+                    "avg_launch_ms": mv_ms, "bytes_per_launch": mv_bytes,
+                    "launches": int(tm.launches[hv]), "share_of_time": tm.ms[hv] * 1e-3 / el},
+        "aca_share_of_time": tm.ms[5] * 1e-3 / el,
+    }
 
 
 def cpu_baseline(level=4):
@@ -137,11 +204,28 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-config4", action="store_true", help="skip the H-matrix dipole measurement")
+    ap.add_argument("--config4-only", action="store_true",
+                    help="only the H-matrix dipole measurement (for profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy three times (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; wavelengths shard across GPUs without "
               "communication, but multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        import torch
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: 3 copies of 1 GiB")
+        del src, dst
+    if a.config4_only:
+        print(json.dumps({"hmatrix_dipole": run_config4()}), flush=True)
+        return
     r = run_gpu(a.level, a.steps, a.warmup, a.fp64_steps)
     tm = r["timers"]
     gi = 1  # MNP_TIMER_GEMM_FP32
@@ -171,14 +255,15 @@ def main():
             "faces": n, "npol": 2, "precision": "complex FP32 chain + FP64 refinement",
         },
         "roofline": {
-            "bound": "mfma", "kernel": "k_cgemm<float> (complex GEMM, 4 real MFMA products)",
+            "bound": "mfma", "kernel": "k_cgemm<float,4,3,true> (3M complex GEMM on FP32 MFMA)",
             "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
             "frac": achieved / PEAK["fp32"], "traffic": None,
             "launches_per_step": tm.launches[gi] / a.steps,
             "avg_launch_ms": gemm_ms, "avg_flops_per_launch": gemm_flops,
+            "avg_algorithmic_bytes_per_launch": tm.bytes[gi] / max(tm.launches[gi], 1),
         },
-        "fp64_wavelengths_per_s": r["n64"] / r["t_64"] if r["t_64"] else None,
-        "speedup_vs_fp64": (a.steps / r["t_mx"]) / (r["n64"] / r["t_64"]) if r["t_64"] else None,
+        "fp64_wavelengths_per_s": r["n64"] / r["t_64"] if r["n64"] else None,
+        "speedup_vs_fp64": (a.steps / r["t_mx"]) / (r["n64"] / r["t_64"]) if r["n64"] else None,
         "max_rel_err_vs_fp64": r["max_err"],
         "refinement": {"sweeps": r["sweeps"][0], "relres": r["sweeps"][1]},
         "chain_effective_tflops": chain_flops / step_s / 1e12,
@@ -191,6 +276,8 @@ def main():
     if t64 is not None and t64.launches[0] and t64.ms[0] > 0:
         a64 = t64.work[0] / (t64.ms[0] * 1e-3) / 1e12
         out["fp64_gemm"] = {"achieved": a64, "peak": PEAK["fp64"], "frac": a64 / PEAK["fp64"]}
+    if not a.no_config4:
+        out["hmatrix_dipole"] = run_config4()
     if not a.no_cpu_baseline:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline()

@@ -81,7 +81,8 @@ enum {
 typedef struct {
   int64_t launches[MNP_TIMER_COUNT];
   double ms[MNP_TIMER_COUNT];
-  double work[MNP_TIMER_COUNT];
+  double work[MNP_TIMER_COUNT];   /* algorithmic flops (GEMM) or bytes (others)   */
+  double bytes[MNP_TIMER_COUNT];  /* algorithmic global-memory bytes              */
 } mnp_timers;
 /* on != 0 enables and resets the timers; 0 disables them. */
 int mnp_timers_enable(int on);

@@ -31,7 +31,8 @@ class GreenOptions(C.Structure):
 
 
 class Timers(C.Structure):
-    _fields_ = [("launches", C.c_int64 * 6), ("ms", C.c_double * 6), ("work", C.c_double * 6)]
+    _fields_ = [("launches", C.c_int64 * 6), ("ms", C.c_double * 6), ("work", C.c_double * 6),
+                ("bytes", C.c_double * 6)]
 
 
 P, I64, D, INT = C.c_void_p, C.c_int64, C.c_double, C.c_int

@@ -27,7 +27,7 @@ hipStream_t stream() { return g_stream; }
 namespace {
 struct TimerRec {
   int kind;
-  double work;
+  double work, bytes;
   hipEvent_t a, b;
 };
 bool g_timers_on = false;
@@ -42,6 +42,7 @@ void timers_flush() {
     g_timer_acc.launches[r.kind] += 1;
     g_timer_acc.ms[r.kind] += ms;
     g_timer_acc.work[r.kind] += r.work;
+    g_timer_acc.bytes[r.kind] += r.bytes;
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
@@ -49,7 +50,7 @@ void timers_flush() {
 }
 }  // namespace
 
-TimerScope::TimerScope(int kind_, double work_) : kind(kind_), work(work_) {
+TimerScope::TimerScope(int kind_, double work_, double bytes_) : kind(kind_), work(work_), bytes(bytes_) {
   if (!g_timers_on) return;
   hipEvent_t a;
   MNP_HIP(hipEventCreate(&a));
@@ -61,7 +62,7 @@ TimerScope::~TimerScope() {
   hipEvent_t b;
   if (hipEventCreate(&b) != hipSuccess) return;
   (void)hipEventRecord(b, g_stream);
-  g_timer_recs.push_back(TimerRec{kind, work, (hipEvent_t)ev0, b});
+  g_timer_recs.push_back(TimerRec{kind, work, bytes, (hipEvent_t)ev0, b});
   if (g_timer_recs.size() > 4096) {
     try {
       timers_flush();
@@ -536,7 +537,7 @@ void GreenRet::assemble(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t 
   dim3 grid((unsigned)cdiv_up(n, 256), (unsigned)n);
   const double bytes = (double)n * n *
                        ((g.d ? 16 : 0) + (g.f ? 8 : 0) + (h.d ? 16 : 0) + (h.f ? 8 : 0));
-  TimerScope ts(MNP_TIMER_ASSEMBLE, bytes);
+  TimerScope ts(MNP_TIMER_ASSEMBLE, bytes, bytes);
   hipLaunchKernelGGL(k_assemble<OutPair>, grid, dim3(256), 0, stream(), n, geo, k, sgn2pi, g, h, ld);
   MNP_HIP(hipGetLastError());
   if (nnz) {

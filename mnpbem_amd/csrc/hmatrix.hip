@@ -626,7 +626,7 @@ void HMat::fill_impl(double2 k, int key_) {
   if (!fvals) fvals = dev_alloc<cplx<TS>>(nfullvals);
   if (nfull) {
     const GreenRet& G = *g;
-    TimerScope ts(MNP_TIMER_ASSEMBLE, (double)es * (double)nfullvals);
+    TimerScope ts(MNP_TIMER_ASSEMBLE, (double)es * (double)nfullvals, (double)es * (double)nfullvals);
     hipLaunchKernelGGL(k_hfull<TS>, dim3((unsigned)nfull), dim3(256), 0, s, nfull, fblk, d_voff, perm, n, G.geo,
                        G.pairs, G.off, G.nnz, G.order, G.gtab, G.ftab, k, key, (cplx<TS>*)fvals);
     MNP_HIP(hipGetLastError());
@@ -777,8 +777,9 @@ void HMat::matvec_impl(int64_t nrhs, const double2* x, double2* y, Workspace& ws
   MNP_CHECK(filled, MNP_ERR_STATE, "hmat: matvec before fill");
   MNP_CHECK(nrhs >= 1, MNP_ERR_ARG, "hmat: nrhs < 1");
   hipStream_t s = stream();
-  TimerScope ts(MNP_TIMER_HMATVEC, (double)sizeof(cplx<TS>) * (double)(nfullvals + nlowvals) *
-                                       (double)((nrhs + MNP_HMV_MAXRHS - 1) / MNP_HMV_MAXRHS));
+  const double mvbytes = (double)sizeof(cplx<TS>) * (double)(nfullvals + nlowvals) *
+                         (double)((nrhs + MNP_HMV_MAXRHS - 1) / MNP_HMV_MAXRHS);
+  TimerScope ts(MNP_TIMER_HMATVEC, mvbytes, mvbytes);
   for (int64_t q0 = 0; q0 < nrhs; q0 += MNP_HMV_MAXRHS) {
     const int nq = (int)std::min<int64_t>(MNP_HMV_MAXRHS, nrhs - q0);
     cplx<TS>* xc = (cplx<TS>*)ws.get(0, sizeof(cplx<TS>) * n * nq);

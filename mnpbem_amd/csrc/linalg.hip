@@ -215,7 +215,9 @@ static void gemm_launch(int64_t m, int64_t n, int64_t k, cplx<T> al, MatView<T> 
     const int64_t ncols = std::min<int64_t>(n - noff, gy * BN);
     // work = the complex product's 8 m n k real flops, whichever algorithm runs it
     TimerScope ts(sizeof(T) == 8 ? MNP_TIMER_GEMM_FP64 : MNP_TIMER_GEMM_FP32,
-                  8.0 * (double)m * (double)ncols * (double)k);
+                  8.0 * (double)m * (double)ncols * (double)k,
+                  (double)sizeof(cplx<T>) * ((double)m * k + (double)k * ncols +
+                                             (double)m * ncols * (bz ? 1.0 : 2.0)));
     hipLaunchKernelGGL((k_cgemm<T, WM, WN, THREE_M>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream(), m,
                        n - noff, k, al, A.p, A.ld, B.p + noff * B.ld, B.ld, be, (int)bz,
                        C.p + noff * C.ld, C.ld);

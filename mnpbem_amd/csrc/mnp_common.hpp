@@ -93,12 +93,13 @@ __device__ inline double2 cexp_ikd(double2 k, double d) {
 inline int64_t cdiv_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Brackets one kernel launch with HIP events when the timers are enabled
-// (mnp_timers_enable); `work` is its algorithmic flops or bytes.
+// (mnp_timers_enable); `work` is its algorithmic flops or bytes and `bytes`
+// its algorithmic global-memory bytes (operands read once, results written once).
 struct TimerScope {
-  TimerScope(int kind, double work);
+  TimerScope(int kind, double work, double bytes = 0.0);
   ~TimerScope();
   int kind;
-  double work;
+  double work, bytes;
   void* ev0 = nullptr;
 };
 
