@@ -64,24 +64,29 @@ __device__ void lds_rsolve(cf (*X)[PB + 1], cf (*LU)[PB + 1], int nb) {
   }
   __syncthreads();
 }
-// columns of V (nb x ncol, column c at V[.][c]) <- (L U)^{-1} V
-__device__ void lds_lsolve(cf (*V)[2 * PB + 1], int ncol, cf (*LU)[PB + 1], int nb) {
-  for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+// Out <- (L U)^{-1}, LU from lds_lu: one thread per column of the identity.
+__device__ void lds_inverse(cf (*Out)[PB + 1], cf (*LU)[PB + 1], int nb) {
+  for (int e = threadIdx.x; e < PB * PB; e += blockDim.x) {
+    const int i = e % PB, j = e / PB;
+    Out[i][j] = i == j ? make_float2(1.f, 0.f) : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < nb; c += blockDim.x) {
     for (int i = 0; i < nb; ++i) {
-      cf s = V[i][c];
-      for (int k = 0; k < i; ++k) s = csub<float>(s, cmul<float>(LU[i][k], V[k][c]));
-      V[i][c] = s;
+      cf s = Out[i][c];
+      for (int k = 0; k < i; ++k) s = csub<float>(s, cmul<float>(LU[i][k], Out[k][c]));
+      Out[i][c] = s;
     }
     for (int i = nb - 1; i >= 0; --i) {
-      cf s = V[i][c];
-      for (int k = i + 1; k < nb; ++k) s = csub<float>(s, cmul<float>(LU[i][k], V[k][c]));
-      V[i][c] = cdiv<float>(s, LU[i][i]);
+      cf s = Out[i][c];
+      for (int k = i + 1; k < nb; ++k) s = csub<float>(s, cmul<float>(LU[i][k], Out[k][c]));
+      Out[i][c] = cdiv<float>(s, LU[i][i]);
     }
   }
   __syncthreads();
 }
 
-// Per preconditioner block: LU(G1), LU(G2), Sigma1, Deltai, LU(Sigma)
+// Per preconditioner block: G1^{-1}, G2^{-1}, Sigma1, Deltai, Sigma^{-1}
 // (5 x PB x PB, FP32), the bemret chain (initmat.m) on a diagonal block.
 // pb[5 b ..] = {first cluster index, size nb <= PB, offset in its leaf,
 // leaf size m, value offset of the leaf's diagonal full block}.
@@ -120,20 +125,7 @@ __global__ void __launch_bounds__(256) k_bjprec_build(const int64_t* pb, const c
   }
   __syncthreads();
   lds_lu(D, nb);
-  // Deltai = D^{-1}: X = I, rows of X^T ... solve D Y = I column by column
-  for (int c = tid; c < nb; c += 256) {
-    for (int i = 0; i < nb; ++i) {
-      cf s = X[i][c];
-      for (int k = 0; k < i; ++k) s = csub<float>(s, cmul<float>(D[i][k], X[k][c]));
-      X[i][c] = s;
-    }
-    for (int i = nb - 1; i >= 0; --i) {
-      cf s = X[i][c];
-      for (int k = i + 1; k < nb; ++k) s = csub<float>(s, cmul<float>(D[i][k], X[k][c]));
-      X[i][c] = cdiv<float>(s, D[i][i]);
-    }
-  }
-  __syncthreads();
+  lds_inverse(X, D, nb);  // Deltai
   // Sigma = e1 Sigma1 - e2 Sigma2 + cc (Deltai o n n^T), into S2
   for (int e = tid; e < nb * nb; e += 256) {
     const int i = e % nb, j = e / nb;
@@ -147,14 +139,18 @@ __global__ void __launch_bounds__(256) k_bjprec_build(const int64_t* pb, const c
   __syncthreads();
   lds_lu(S2, nb);
   float2* o = out + l * 5 * PB * PB;
-  for (int e = tid; e < PB * PB; e += 256) {
-    const int i = e / PB, j = e % PB;
-    o[e] = A[i][j];
-    o[PB * PB + e] = B[i][j];
-    o[2 * PB * PB + e] = S1[i][j];
-    o[3 * PB * PB + e] = X[i][j];
-    o[4 * PB * PB + e] = S2[i][j];
-  }
+  auto put = [&](int slot, cf (*M)[PB + 1]) {
+    for (int e = tid; e < PB * PB; e += 256) o[slot * PB * PB + e] = M[e / PB][e % PB];
+    __syncthreads();
+  };
+  lds_inverse(D, S2, nb);  // Sigma^{-1}
+  put(4, D);
+  put(2, S1);
+  put(3, X);
+  lds_inverse(S2, A, nb);  // G1^{-1}
+  put(0, S2);
+  lds_inverse(S2, B, nb);  // G2^{-1}
+  put(1, S2);
 }
 
 // x = M^{-1} b on one preconditioner block's rows (mfun.m:27-52 with the
@@ -233,26 +229,18 @@ __global__ void __launch_bounds__(256) k_bjprec_apply(const int64_t* pb, const i
     W[i][3 * p + q] = cadd<float>(V[i][4 * p + q], mul(s, ikdL));
   }
   loadM(4);
-  {  // sig2 = Sigma^{-1} rhs, in W[:, 3p .. 4p]
-    for (int c = tid; c < p; c += 256) {
-      const int col = 3 * p + c;
-      for (int i = 0; i < nb; ++i) {
-        cf s = W[i][col];
-        for (int kk = 0; kk < i; ++kk) s = csub<float>(s, cmul<float>(M[i][kk], W[kk][col]));
-        W[i][col] = s;
-      }
-      for (int i = nb - 1; i >= 0; --i) {
-        cf s = W[i][col];
-        for (int kk = i + 1; kk < nb; ++kk) s = csub<float>(s, cmul<float>(M[i][kk], W[kk][col]));
-        W[i][col] = cdiv<float>(s, M[i][i]);
-      }
-    }
+  // sig2 = Sigma^{-1} rhs -> W[:, 7p .. 8p]
+  for (int e = tid; e < nb * p; e += 256) {
+    const int i = e % nb, c = e / nb;
+    cf s = make_float2(0.f, 0.f);
+    for (int kk = 0; kk < nb; ++kk) s = cadd<float>(s, cmul<float>(M[i][kk], W[kk][3 * p + c]));
+    W[i][7 * p + c] = s;
   }
   __syncthreads();
   // u = ik (L1-L2) n sig2 + alpha' -> W[:, 4p .. 7p]
   for (int e = tid; e < nb * p; e += 256) {
     const int i = e % nb, q = e / nb;
-    const cf w = mul(W[i][3 * p + q], ikdL);
+    const cf w = mul(W[i][7 * p + q], ikdL);
     for (int c = 0; c < 3; ++c) {
       const float n_c = nv(i, c);
       W[i][4 * p + c + 3 * q] = cadd<float>(V[i][5 * p + c + 3 * q], make_float2(n_c * w.x, n_c * w.y));
@@ -270,42 +258,30 @@ __global__ void __launch_bounds__(256) k_bjprec_apply(const int64_t* pb, const i
   // V1 = [sig2 + phi | h2 + a] -> V cols 0..4p ; V2 = [sig2 | h2] -> V cols 4p..8p
   for (int e = tid; e < nb * 4 * p; e += 256) {
     const int i = e % nb, c = e / nb;
-    const cf src = c < p ? W[i][3 * p + c] : W[i][c - p];
+    const cf src = c < p ? W[i][7 * p + c] : W[i][c - p];
     V[i][c] = cadd<float>(src, V[i][c]);
     V[i][4 * p + c] = src;
   }
   __syncthreads();
-  // G1^{-1} on V cols 0..4p, G2^{-1} on V cols 4p..8p
+  // G1^{-1} on V cols 0..4p, G2^{-1} on V cols 4p..8p -> W
   loadM(0);
-  for (int c = tid; c < 4 * p; c += 256) {
-    for (int i = 0; i < nb; ++i) {
-      cf s = V[i][c];
-      for (int kk = 0; kk < i; ++kk) s = csub<float>(s, cmul<float>(M[i][kk], V[kk][c]));
-      V[i][c] = s;
-    }
-    for (int i = nb - 1; i >= 0; --i) {
-      cf s = V[i][c];
-      for (int kk = i + 1; kk < nb; ++kk) s = csub<float>(s, cmul<float>(M[i][kk], V[kk][c]));
-      V[i][c] = cdiv<float>(s, M[i][i]);
-    }
+  for (int e = tid; e < nb * 4 * p; e += 256) {
+    const int i = e % nb, c = e / nb;
+    cf s = make_float2(0.f, 0.f);
+    for (int kk = 0; kk < nb; ++kk) s = cadd<float>(s, cmul<float>(M[i][kk], V[kk][c]));
+    W[i][c] = s;
   }
   loadM(1);
-  for (int c = 4 * p + tid; c < 8 * p; c += 256) {
-    for (int i = 0; i < nb; ++i) {
-      cf s = V[i][c];
-      for (int kk = 0; kk < i; ++kk) s = csub<float>(s, cmul<float>(M[i][kk], V[kk][c]));
-      V[i][c] = s;
-    }
-    for (int i = nb - 1; i >= 0; --i) {
-      cf s = V[i][c];
-      for (int kk = i + 1; kk < nb; ++kk) s = csub<float>(s, cmul<float>(M[i][kk], V[kk][c]));
-      V[i][c] = cdiv<float>(s, M[i][i]);
-    }
+  for (int e = tid; e < nb * 4 * p; e += 256) {
+    const int i = e % nb, c = 4 * p + e / nb;
+    cf s = make_float2(0.f, 0.f);
+    for (int kk = 0; kk < nb; ++kk) s = cadd<float>(s, cmul<float>(M[i][kk], V[kk][c]));
+    W[i][c] = s;
   }
   __syncthreads();
   for (int e = tid; e < nb * nc; e += 256) {
     const int i = e % nb, c = e / nb;
-    x[(int64_t)c * n + perm[r0 + i]] = mk<double>(V[i][c].x, V[i][c].y);
+    x[(int64_t)c * n + perm[r0 + i]] = mk<double>(W[i][c].x, W[i][c].y);
   }
 }
 
