@@ -19,6 +19,10 @@ FLAGS = [
     f"--offload-arch={ARCH}",
     # no packed-FP32 instructions in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # keep MFMA sources and accumulators in ArchVGPRs: with the default
+    # heuristic the GEMM tiles shuttle accumulators between AGPRs and VGPRs
+    # around each MFMA (v_accvgpr_read/write), serialising the MFMA pipe
+    "-mllvm", "--amdgpu-mfma-vgpr-form",
     "-Wno-unused-command-line-argument",
 ]
 
