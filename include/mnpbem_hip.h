@@ -70,8 +70,8 @@ int mnp_synchronize(void);
  * algorithmic work of the family: real flops (gemm: 8 m n k per complex
  * product) or bytes written (assembly). */
 enum {
-  MNP_TIMER_GEMM_FP64 = 0, /* complex GEMM, complex128 operands                 */
-  MNP_TIMER_GEMM_FP32 = 1, /* complex GEMM, complex64 operands                  */
+  MNP_TIMER_GEMM_FP64 = 0, /* complex GEMM, complex128, large-tile launches      */
+  MNP_TIMER_GEMM_FP32 = 1, /* complex GEMM, complex64, large-tile launches       */
   MNP_TIMER_PANEL = 2,     /* cooperative LU panel                              */
   MNP_TIMER_ASSEMBLE = 3,  /* dense G/H assembly and H-matrix full blocks        */
   MNP_TIMER_HMATVEC = 4,   /* H-matrix times matrix (bytes of stored blocks)     */

@@ -32,7 +32,18 @@ struct TimerRec {
 };
 bool g_timers_on = false;
 std::vector<TimerRec> g_timer_recs;
+std::vector<hipEvent_t> g_event_pool;  // events are reused: creating one per launch costs host time
 mnp_timers g_timer_acc{};
+hipEvent_t event_get() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  MNP_HIP(hipEventCreate(&e));
+  return e;
+}
 void timers_flush() {
   if (g_timer_recs.empty()) return;
   MNP_HIP(hipStreamSynchronize(g_stream));
@@ -43,27 +54,30 @@ void timers_flush() {
     g_timer_acc.ms[r.kind] += ms;
     g_timer_acc.work[r.kind] += r.work;
     g_timer_acc.bytes[r.kind] += r.bytes;
-    (void)hipEventDestroy(r.a);
-    (void)hipEventDestroy(r.b);
+    g_event_pool.push_back(r.a);
+    g_event_pool.push_back(r.b);
   }
   g_timer_recs.clear();
 }
 }  // namespace
 
 TimerScope::TimerScope(int kind_, double work_, double bytes_) : kind(kind_), work(work_), bytes(bytes_) {
-  if (!g_timers_on) return;
-  hipEvent_t a;
-  MNP_HIP(hipEventCreate(&a));
+  if (!g_timers_on || kind < 0) return;
+  hipEvent_t a = event_get();
   MNP_HIP(hipEventRecord(a, g_stream));
   ev0 = a;
 }
 TimerScope::~TimerScope() {
   if (!ev0) return;
   hipEvent_t b;
-  if (hipEventCreate(&b) != hipSuccess) return;
+  try {
+    b = event_get();
+  } catch (...) {
+    return;
+  }
   (void)hipEventRecord(b, g_stream);
   g_timer_recs.push_back(TimerRec{kind, work, bytes, (hipEvent_t)ev0, b});
-  if (g_timer_recs.size() > 4096) {
+  if (g_timer_recs.size() > 16384) {
     try {
       timers_flush();
     } catch (...) {

@@ -213,8 +213,11 @@ static void gemm_launch(int64_t m, int64_t n, int64_t k, cplx<T> al, MatView<T> 
     const int64_t gy = std::min<int64_t>(65535, ny - y0);
     const int64_t noff = y0 * BN;
     const int64_t ncols = std::min<int64_t>(n - noff, gy * BN);
+    // Only the large-tile variant (the chain's dominant kernel) is timed:
+    // events around each of the recursion's tens of thousands of small
+    // launches would cost more host time than the launches themselves.
     // work = the complex product's 8 m n k real flops, whichever algorithm runs it
-    TimerScope ts(sizeof(T) == 8 ? MNP_TIMER_GEMM_FP64 : MNP_TIMER_GEMM_FP32,
+    TimerScope ts(WM == 4 ? (sizeof(T) == 8 ? MNP_TIMER_GEMM_FP64 : MNP_TIMER_GEMM_FP32) : -1,
                   8.0 * (double)m * (double)ncols * (double)k,
                   (double)sizeof(cplx<T>) * ((double)m * k + (double)k * ncols +
                                              (double)m * ncols * (bz ? 1.0 : 2.0)));
@@ -738,15 +741,19 @@ __global__ void __launch_bounds__(256) k_lu_nopiv_diag(int nb, cplx<T>* A, int64
 }
 
 // Partial pivoting would keep the diagonal of column j iff no entry below
-// it has a larger |Re|+|Im| at step j; that entry is l_ij u_jj (ties go to
-// the diagonal, the lower index).  Sets *flag when some column would swap.
+// it has a larger |Re|+|Im| at step j; in the unpivoted factors that entry
+// is l_ij u_jj (ties go to the diagonal, the lower index), so one pass over
+// the finished L and U decides it.  Thread (row i, chunk of 64 columns);
+// sets *flag when some column would swap.
+constexpr int VERIFY_COLS = 64;
 template <class T>
-__global__ void k_verify_nopiv(int64_t m, int nb, const cplx<T>* A, int64_t lda, int* flag) {
+__global__ void k_verify_nopiv(int64_t m, int64_t n, const cplx<T>* A, int64_t lda, int* flag) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j0 = (int64_t)blockIdx.y * VERIFY_COLS;
   if (i >= m) return;
-  const int jmax = (int)min((int64_t)nb, i);
+  const int64_t j1 = min(min(n, i), j0 + VERIFY_COLS);
   bool bad = false;
-  for (int j = 0; j < jmax; ++j) {
+  for (int64_t j = j0; j < j1; ++j) {
     const cplx<T> u = A[j * lda + j];
     const cplx<T> v = cmul<T>(A[j * lda + i], u);
     bad |= cabs1<T>(v) > cabs1<T>(u);
@@ -783,9 +790,6 @@ static void getrf_rec(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, int inf
         MatView<T> A21{A.at(n, 0), A.ld};
         gemm<T>(m - n, n, n, ONE, A21, MatView<T>{Uinv, TB}, ZERO, A21);  // L21 = A21 U11^{-1}
       }
-      hipLaunchKernelGGL(k_verify_nopiv<T>, dim3((unsigned)cdiv_up(m, 256)), dim3(256), 0, stream(), m,
-                         (int)n, A.p, A.ld, st.flag);
-      hipLaunchKernelGGL(k_iota, dim3((unsigned)cdiv_up(n, 256)), dim3(256), 0, stream(), n, ipiv, 0);
       MNP_HIP(hipGetLastError());
     }
     return;
@@ -799,9 +803,11 @@ static void getrf_rec(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, int inf
           MatView<T>{A.at(n1, n1), A.ld});
   getrf_rec<T>(m - n1, n2, MatView<T>{A.at(n1, n1), A.ld}, ipiv + n1, info_off + (int)n1,
                Linv + (n1 / TB) * TB2, Uinv + (n1 / TB) * TB2, st, ws);
-  hipLaunchKernelGGL(k_add_offset, dim3((unsigned)cdiv_up(n2, 256)), dim3(256), 0, stream(), n2,
-                     ipiv + n1, (int32_t)n1);
-  if (piv) laswp<T>(A, m, 0, n1, n1, n, ipiv, ws);
+  if (piv) {
+    hipLaunchKernelGGL(k_add_offset, dim3((unsigned)cdiv_up(n2, 256)), dim3(256), 0, stream(), n2,
+                       ipiv + n1, (int32_t)n1);
+    laswp<T>(A, m, 0, n1, n1, n, ipiv, ws);
+  }
 }
 
 template <class T>
@@ -812,6 +818,12 @@ int getrf(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, cplx<T>* Linv, cplx
   MNP_HIP(hipMemsetAsync(d, 0, 2 * sizeof(int), stream()));
   LUState st{mode, d, d + 1};
   getrf_rec<T>(m, n, A, ipiv, 0, Linv, Uinv, st, ws);
+  if (mode == PIVOT_NONE) {  // identity pivots; one pass for the pivoting check
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)cdiv_up(n, 256)), dim3(256), 0, stream(), n, ipiv, 0);
+    const dim3 grid((unsigned)cdiv_up(m, 256), (unsigned)cdiv_up(n, VERIFY_COLS));
+    hipLaunchKernelGGL(k_verify_nopiv<T>, grid, dim3(256), 0, stream(), m, n, A.p, A.ld, st.flag);
+    MNP_HIP(hipGetLastError());
+  }
   int h[2] = {0, 0};
   MNP_HIP(hipMemcpyAsync(h, d, 2 * sizeof(int), hipMemcpyDeviceToHost, stream()));
   MNP_HIP(hipStreamSynchronize(stream()));

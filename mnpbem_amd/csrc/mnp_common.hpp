@@ -94,7 +94,8 @@ inline int64_t cdiv_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Brackets one kernel launch with HIP events when the timers are enabled
 // (mnp_timers_enable); `work` is its algorithmic flops or bytes and `bytes`
-// its algorithmic global-memory bytes (operands read once, results written once).
+// its algorithmic global-memory bytes (operands read once, results written
+// once).  A negative kind records nothing.
 struct TimerScope {
   TimerScope(int kind, double work, double bytes = 0.0);
   ~TimerScope();
