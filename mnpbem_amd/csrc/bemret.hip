@@ -29,7 +29,8 @@ struct Chain {
   bool exact_pivoting = true;  // FP64: factors must equal partial pivoting's
   cplx<T>*G1 = nullptr, *G2 = nullptr, *S1 = nullptr, *S2 = nullptr, *D = nullptr, *Di = nullptr;
   cplx<T>* backup = nullptr;   // copy of a matrix while it is factorised without pivoting
-  cplx<T>* inv = nullptr;      // diagonal-block inverses: 4 factors x (L, U)
+  cplx<T>* inv = nullptr;      // 32-block inverses: 4 factors x (L, U)
+  cplx<T>* invnb = nullptr;    // 256-block inverses: 4 factors x (L, U)
   int32_t *piv = nullptr, *sig[4] = {nullptr, nullptr, nullptr, nullptr};
   LU<T> f[4];                  // G1, G2, Sigma1 - Sigma2, Sigma
   int fallbacks = 0;           // factorisations redone with partial pivoting
@@ -47,11 +48,12 @@ struct Chain {
     Di = dev_alloc<cplx<T>>(nn);
     backup = dev_alloc<cplx<T>>(nn);
     inv = dev_alloc<cplx<T>>(8 * inv_blocks(n));
+    invnb = dev_alloc<cplx<T>>(8 * inv_blocks_nb(n));
     piv = dev_alloc<int32_t>(n);
     for (auto& q : sig) q = dev_alloc<int32_t>(n);
   }
   ~Chain() {
-    void* ps[] = {G1, G2, S1, S2, D, Di, backup, inv, piv, sig[0], sig[1], sig[2], sig[3]};
+    void* ps[] = {G1, G2, S1, S2, D, Di, backup, inv, invnb, piv, sig[0], sig[1], sig[2], sig[3]};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -77,7 +79,10 @@ struct Chain {
     }
     MNP_CHECK(info == 0, MNP_ERR_SINGULAR,
               std::string("bemret: singular ") + what + " (zero pivot " + std::to_string(info) + ")");
-    f[s] = LU<T>{v(A), Linv, Uinv, sigma};
+    cplx<T>* LinvNB = invnb + (2 * s) * inv_blocks_nb(n);
+    cplx<T>* UinvNB = LinvNB + inv_blocks_nb(n);
+    block_inverses_nb<T>(n, v(A), Linv, Uinv, LinvNB, UinvNB, ws);
+    f[s] = LU<T>{v(A), Linv, Uinv, sigma, LinvNB, UinvNB};
     return f[s];
   }
 };

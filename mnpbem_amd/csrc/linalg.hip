@@ -58,8 +58,12 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
                                                const cplx<T>* A, int64_t lda,
                                                const cplx<T>* B, int64_t ldb,
                                                cplx<T> beta, int beta_zero, cplx<T>* C,
-                                               int64_t ldc) {
+                                               int64_t ldc, int64_t sa, int64_t sb, int64_t sc) {
   using acc_t = typename MF<T>::acc_t;
+  // batch index blockIdx.z: operands at constant element strides
+  A += blockIdx.z * sa;
+  B += blockIdx.z * sb;
+  C += blockIdx.z * sc;
   constexpr int BM = 32 * WM, BN = 32 * WN, PAD = 1;
   constexpr int LA = BM * GBK / 256, LB = BN * GBK / 256;  // elements per thread
   __shared__ T sAr[2][GBK][BM + PAD], sAi[2][GBK][BM + PAD];
@@ -205,7 +209,8 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
 
 template <class T, int WM, int WN, bool THREE_M = false>
 static void gemm_launch(int64_t m, int64_t n, int64_t k, cplx<T> al, MatView<T> A, MatView<T> B,
-                        cplx<T> be, bool bz, MatView<T> C) {
+                        cplx<T> be, bool bz, MatView<T> C, int64_t batch = 1, int64_t sa = 0,
+                        int64_t sb = 0, int64_t sc = 0) {
   constexpr int BM = 32 * WM, BN = 32 * WN;
   const int64_t gx = cdiv_up(m, BM);
   const int64_t ny = cdiv_up(n, BN);
@@ -218,12 +223,12 @@ static void gemm_launch(int64_t m, int64_t n, int64_t k, cplx<T> al, MatView<T> 
     // launches would cost more host time than the launches themselves.
     // work = the complex product's 8 m n k real flops, whichever algorithm runs it
     TimerScope ts(WM == 4 ? (sizeof(T) == 8 ? MNP_TIMER_GEMM_FP64 : MNP_TIMER_GEMM_FP32) : -1,
-                  8.0 * (double)m * (double)ncols * (double)k,
-                  (double)sizeof(cplx<T>) * ((double)m * k + (double)k * ncols +
-                                             (double)m * ncols * (bz ? 1.0 : 2.0)));
-    hipLaunchKernelGGL((k_cgemm<T, WM, WN, THREE_M>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream(), m,
-                       n - noff, k, al, A.p, A.ld, B.p + noff * B.ld, B.ld, be, (int)bz,
-                       C.p + noff * C.ld, C.ld);
+                  8.0 * (double)m * (double)ncols * (double)k * (double)batch,
+                  (double)sizeof(cplx<T>) * (double)batch *
+                      ((double)m * k + (double)k * ncols + (double)m * ncols * (bz ? 1.0 : 2.0)));
+    hipLaunchKernelGGL((k_cgemm<T, WM, WN, THREE_M>), dim3((unsigned)gx, (unsigned)gy, (unsigned)batch), dim3(256), 0,
+                       stream(), m, n - noff, k, al, A.p, A.ld, B.p + noff * B.ld, B.ld, be, (int)bz,
+                       C.p + noff * C.ld, C.ld, sa, sb, sc);
   }
   MNP_HIP(hipGetLastError());
 }
@@ -247,6 +252,17 @@ void gemm(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, MatView<
       gemm_launch<T, 4, 3, true>(m, n, k, al, A, B, be, bz, C);  // 3M on the FP32 chain
   } else
     gemm_launch<T, 2, 2>(m, n, k, al, A, B, be, bz, C);
+}
+
+// batch products C_z = alpha A_z B_z + beta C_z, X_z = X + z * sx (64 x 64 tiles)
+template <class T>
+void gemm_batched(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, int64_t sa, MatView<T> B,
+                  int64_t sb, double2 beta, MatView<T> C, int64_t sc, int64_t batch) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  MNP_CHECK(k > 0 && batch < 65536, MNP_ERR_INTERNAL, "gemm_batched: bad sizes");
+  const bool bz = beta.x == 0 && beta.y == 0;
+  gemm_launch<T, 2, 2>(m, n, k, mk<T>((T)alpha.x, (T)alpha.y), A, B, mk<T>((T)beta.x, (T)beta.y), bz, C,
+                       batch, sa, sb, sc);
 }
 
 // ------------------------------------------------------- cooperative panel
@@ -563,6 +579,7 @@ void convert_mat(int64_t m, int64_t n, MatView<Ts> A, MatView<Td> B) {
 // a partial last block is padded with the identity.
 constexpr int TB = 32;
 constexpr int64_t TB2 = TB * TB;
+constexpr int TBN = 256;  // diagonal blocks of the blocked triangular solves
 
 // One workgroup per block: threads 0..31 build column j of L^{-1},
 // threads 32..63 column j of U^{-1}.
@@ -616,6 +633,67 @@ void block_inverses(int64_t n, MatView<T> LU, cplx<T>* Linv, cplx<T>* Uinv) {
 }
 
 static const double2 ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
+
+// ---------------------------------------- 256-block inverses of L and U
+// The inverses of the 256 x 256 diagonal blocks of a factor are built from
+// its 32 x 32 ones by three doublings (32 -> 64 -> 128 -> 256), each a
+// batch over all blocks of that size:
+//   lower: [[A^-1, 0], [-B^-1 L21 A^-1, B^-1]],  upper: [[A^-1, -A^-1 U12 B^-1], [0, B^-1]].
+// k_inv_assemble copies the two diagonal quadrants and zeroes the empty
+// one; two batched GEMMs fill the coupling quadrant.
+template <class T>
+__global__ void k_inv_assemble(int s, const cplx<T>* in, cplx<T>* out, int lower) {
+  const int64_t i = blockIdx.y;  // block of size 2s
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t s2 = 2 * (int64_t)s;
+  if (e >= s2 * s2) return;
+  const int r = (int)(e % s2), c = (int)(e / s2);
+  cplx<T>* o = out + i * s2 * s2;
+  if (r < s && c < s)
+    o[e] = in[(2 * i) * s * s + (int64_t)c * s + r];
+  else if (r >= s && c >= s)
+    o[e] = in[(2 * i + 1) * s * s + (int64_t)(c - s) * s + (r - s)];
+  else if ((lower && r < s) || (!lower && r >= s))
+    o[e] = czero<T>();
+}
+
+template <class T>
+void block_inverses_nb(int64_t n, MatView<T> LU, const cplx<T>* Linv, const cplx<T>* Uinv, cplx<T>* LinvNB,
+                       cplx<T>* UinvNB, Workspace& ws) {
+  const int64_t nb = n / TBN;
+  if (nb == 0) return;
+  const int64_t full = nb * TBN;
+  // ping-pong buffers (full x TBN entries each) and the coupling temporaries
+  cplx<T>* buf[2] = {(cplx<T>*)ws.get(3, sizeof(cplx<T>) * (size_t)(full * TBN)),
+                     (cplx<T>*)ws.get(6, sizeof(cplx<T>) * (size_t)(full * TBN))};
+  cplx<T>* tmp = (cplx<T>*)ws.get(7, sizeof(cplx<T>) * (size_t)(full * TBN / 2));
+  for (int lower = 1; lower >= 0; --lower) {
+    const cplx<T>* in = lower ? Linv : Uinv;
+    for (int64_t sz = TB; sz < TBN; sz *= 2) {
+      const int64_t s2 = 2 * sz, nbl = full / s2, ss = sz * sz;
+      cplx<T>* out = (s2 == TBN) ? (lower ? LinvNB : UinvNB) : buf[(sz / TB) % 2];
+      hipLaunchKernelGGL(k_inv_assemble<T>, dim3((unsigned)cdiv_up(s2 * s2, 256), (unsigned)nbl), dim3(256), 0,
+                         stream(), (int)sz, in, out, lower);
+      MNP_HIP(hipGetLastError());
+      const int64_t sdiag = s2 * (LU.ld + 1);  // stride between consecutive 2s diagonal blocks of LU
+      if (lower) {
+        // tmp_i = L21_i A_i^-1 ;  X_ll = -B_i^-1 tmp_i
+        gemm_batched<T>(sz, sz, sz, ONE, MatView<T>{LU.at(sz, 0), LU.ld}, sdiag,
+                        MatView<T>{const_cast<cplx<T>*>(in), sz}, 2 * ss, ZERO, MatView<T>{tmp, sz}, ss, nbl);
+        gemm_batched<T>(sz, sz, sz, MONE, MatView<T>{const_cast<cplx<T>*>(in) + ss, sz}, 2 * ss,
+                        MatView<T>{tmp, sz}, ss, ZERO, MatView<T>{out + sz, s2}, s2 * s2, nbl);
+      } else {
+        // tmp_i = U12_i B_i^-1 ;  X_ur = -A_i^-1 tmp_i
+        gemm_batched<T>(sz, sz, sz, ONE, MatView<T>{LU.at(0, sz), LU.ld}, sdiag,
+                        MatView<T>{const_cast<cplx<T>*>(in) + ss, sz}, 2 * ss, ZERO, MatView<T>{tmp, sz}, ss,
+                        nbl);
+        gemm_batched<T>(sz, sz, sz, MONE, MatView<T>{const_cast<cplx<T>*>(in), sz}, 2 * ss,
+                        MatView<T>{tmp, sz}, ss, ZERO, MatView<T>{out + sz * s2, s2}, s2 * s2, nbl);
+      }
+      in = out;
+    }
+  }
+}
 
 static int64_t split(int64_t n, int64_t leaf) {
   int64_t n1 = (n / 2) / leaf * leaf;
@@ -682,6 +760,166 @@ void trsm_rlu(int64_t m, int64_t n, MatView<T> LU, const cplx<T>* Linv, MatView<
               MatView<T>{B.at(0, n1), B.ld});
   gemm<T>(m, n1, n2, MONE, MatView<T>{B.at(0, n1), B.ld}, MatView<T>{LU.at(n1, 0), LU.ld}, ONE, B);
   trsm_rlu<T>(m, n1, LU, Linv, B);
+}
+
+// Blocked triangular solves with the 256-block inverses: per block one
+// product with the block's inverse (into W, copied back) and one update of
+// the remaining rows or columns, so every product has an inner dimension of
+// 256.  A trailing partial block uses the recursive solves.
+// B <- L^{-1} B
+template <class T>
+void trsm_llu_nb(int64_t n, int64_t nrhs, const LU<T>& f, MatView<T> B, Workspace& ws) {
+  const int64_t nb = n / TBN;
+  cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(TBN * nrhs));
+  const MatView<T> Wv{W, TBN};
+  for (int64_t i = 0; i < nb; ++i) {
+    const int64_t r0 = i * TBN;
+    gemm<T>(TBN, nrhs, TBN, ONE, MatView<T>{f.LinvNB + i * TBN * TBN, TBN}, MatView<T>{B.at(r0, 0), B.ld}, ZERO,
+            Wv);
+    copy_mat<T>(TBN, nrhs, Wv, MatView<T>{B.at(r0, 0), B.ld});
+    if (r0 + TBN < n)
+      gemm<T>(n - r0 - TBN, nrhs, TBN, MONE, MatView<T>{f.A.at(r0 + TBN, r0), f.A.ld}, MatView<T>{B.at(r0, 0), B.ld},
+              ONE, MatView<T>{B.at(r0 + TBN, 0), B.ld});
+  }
+  const int64_t t0 = nb * TBN;
+  if (t0 < n)
+    trsm_llu<T>(n - t0, nrhs, MatView<T>{f.A.at(t0, t0), f.A.ld}, f.Linv + (t0 / TB) * TB2,
+                MatView<T>{B.at(t0, 0), B.ld});
+}
+// B <- U^{-1} B
+template <class T>
+void trsm_lun_nb(int64_t n, int64_t nrhs, const LU<T>& f, MatView<T> B, Workspace& ws) {
+  const int64_t nb = n / TBN, t0 = nb * TBN;
+  if (t0 < n) {
+    trsm_lun<T>(n - t0, nrhs, MatView<T>{f.A.at(t0, t0), f.A.ld}, f.Uinv + (t0 / TB) * TB2,
+                MatView<T>{B.at(t0, 0), B.ld});
+    if (t0 > 0)
+      gemm<T>(t0, nrhs, n - t0, MONE, MatView<T>{f.A.at(0, t0), f.A.ld}, MatView<T>{B.at(t0, 0), B.ld}, ONE, B);
+  }
+  cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(TBN * nrhs));
+  const MatView<T> Wv{W, TBN};
+  for (int64_t i = nb - 1; i >= 0; --i) {
+    const int64_t r0 = i * TBN;
+    gemm<T>(TBN, nrhs, TBN, ONE, MatView<T>{f.UinvNB + i * TBN * TBN, TBN}, MatView<T>{B.at(r0, 0), B.ld}, ZERO,
+            Wv);
+    copy_mat<T>(TBN, nrhs, Wv, MatView<T>{B.at(r0, 0), B.ld});
+    if (r0 > 0)
+      gemm<T>(r0, nrhs, TBN, MONE, MatView<T>{f.A.at(0, r0), f.A.ld}, MatView<T>{B.at(r0, 0), B.ld}, ONE, B);
+  }
+}
+// B <- B U^{-1}   (B m x n)
+template <class T>
+void trsm_run_nb(int64_t m, int64_t n, const LU<T>& f, MatView<T> B, Workspace& ws) {
+  const int64_t nb = n / TBN;
+  cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(m * TBN));
+  const MatView<T> Wv{W, m};
+  for (int64_t j = 0; j < nb; ++j) {
+    const int64_t c0 = j * TBN;
+    gemm<T>(m, TBN, TBN, ONE, MatView<T>{B.at(0, c0), B.ld}, MatView<T>{f.UinvNB + j * TBN * TBN, TBN}, ZERO, Wv);
+    copy_mat<T>(m, TBN, Wv, MatView<T>{B.at(0, c0), B.ld});
+    if (c0 + TBN < n)
+      gemm<T>(m, n - c0 - TBN, TBN, MONE, MatView<T>{B.at(0, c0), B.ld}, MatView<T>{f.A.at(c0, c0 + TBN), f.A.ld},
+              ONE, MatView<T>{B.at(0, c0 + TBN), B.ld});
+  }
+  const int64_t t0 = nb * TBN;
+  if (t0 < n)
+    trsm_run<T>(m, n - t0, MatView<T>{f.A.at(t0, t0), f.A.ld}, f.Uinv + (t0 / TB) * TB2,
+                MatView<T>{B.at(0, t0), B.ld});
+}
+// B <- B L^{-1}   (B m x n, L unit lower)
+template <class T>
+void trsm_rlu_nb(int64_t m, int64_t n, const LU<T>& f, MatView<T> B, Workspace& ws) {
+  const int64_t nb = n / TBN, t0 = nb * TBN;
+  if (t0 < n) {
+    trsm_rlu<T>(m, n - t0, MatView<T>{f.A.at(t0, t0), f.A.ld}, f.Linv + (t0 / TB) * TB2,
+                MatView<T>{B.at(0, t0), B.ld});
+    if (t0 > 0)
+      gemm<T>(m, t0, n - t0, MONE, MatView<T>{B.at(0, t0), B.ld}, MatView<T>{f.A.at(t0, 0), f.A.ld}, ONE, B);
+  }
+  cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(m * TBN));
+  const MatView<T> Wv{W, m};
+  for (int64_t j = nb - 1; j >= 0; --j) {
+    const int64_t c0 = j * TBN;
+    gemm<T>(m, TBN, TBN, ONE, MatView<T>{B.at(0, c0), B.ld}, MatView<T>{f.LinvNB + j * TBN * TBN, TBN}, ZERO, Wv);
+    copy_mat<T>(m, TBN, Wv, MatView<T>{B.at(0, c0), B.ld});
+    if (c0 > 0)
+      gemm<T>(m, c0, TBN, MONE, MatView<T>{B.at(0, c0), B.ld}, MatView<T>{f.A.at(c0, 0), f.A.ld}, ONE, B);
+  }
+}
+
+// Recursive solves whose leaves are the 256-row (or 256-column) diagonal
+// blocks, applied through their stored inverses (one product into W, copied
+// back); splits fall on multiples of 256 from the factor's origin, and a
+// trailing partial block uses the 32-leaf recursion.  For many right-hand
+// sides this keeps the large inner dimensions of the recursion and removes
+// its three lowest levels.
+template <class T>
+void trsm_llu_h(int64_t n, int64_t nrhs, MatView<T> LU, const cplx<T>* Linv, const cplx<T>* LinvNB, MatView<T> B,
+                Workspace& ws) {
+  if (n <= 0 || nrhs <= 0) return;
+  if (n < TBN) return trsm_llu<T>(n, nrhs, LU, Linv, B);
+  if (n == TBN) {
+    cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(TBN * nrhs));
+    gemm<T>(TBN, nrhs, TBN, ONE, MatView<T>{const_cast<cplx<T>*>(LinvNB), TBN}, B, ZERO, MatView<T>{W, TBN});
+    copy_mat<T>(TBN, nrhs, MatView<T>{W, TBN}, B);
+    return;
+  }
+  const int64_t n1 = std::max<int64_t>(TBN, (n / 2) / TBN * TBN), n2 = n - n1;
+  trsm_llu_h<T>(n1, nrhs, LU, Linv, LinvNB, B, ws);
+  gemm<T>(n2, nrhs, n1, MONE, MatView<T>{LU.at(n1, 0), LU.ld}, B, ONE, MatView<T>{B.at(n1, 0), B.ld});
+  trsm_llu_h<T>(n2, nrhs, MatView<T>{LU.at(n1, n1), LU.ld}, Linv + (n1 / TB) * TB2, LinvNB + (n1 / TBN) * TBN * TBN,
+                MatView<T>{B.at(n1, 0), B.ld}, ws);
+}
+template <class T>
+void trsm_lun_h(int64_t n, int64_t nrhs, MatView<T> LU, const cplx<T>* Uinv, const cplx<T>* UinvNB, MatView<T> B,
+                Workspace& ws) {
+  if (n <= 0 || nrhs <= 0) return;
+  if (n < TBN) return trsm_lun<T>(n, nrhs, LU, Uinv, B);
+  if (n == TBN) {
+    cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(TBN * nrhs));
+    gemm<T>(TBN, nrhs, TBN, ONE, MatView<T>{const_cast<cplx<T>*>(UinvNB), TBN}, B, ZERO, MatView<T>{W, TBN});
+    copy_mat<T>(TBN, nrhs, MatView<T>{W, TBN}, B);
+    return;
+  }
+  const int64_t n1 = std::max<int64_t>(TBN, (n / 2) / TBN * TBN), n2 = n - n1;
+  trsm_lun_h<T>(n2, nrhs, MatView<T>{LU.at(n1, n1), LU.ld}, Uinv + (n1 / TB) * TB2, UinvNB + (n1 / TBN) * TBN * TBN,
+                MatView<T>{B.at(n1, 0), B.ld}, ws);
+  gemm<T>(n1, nrhs, n2, MONE, MatView<T>{LU.at(0, n1), LU.ld}, MatView<T>{B.at(n1, 0), B.ld}, ONE, B);
+  trsm_lun_h<T>(n1, nrhs, LU, Uinv, UinvNB, B, ws);
+}
+template <class T>
+void trsm_run_h(int64_t m, int64_t n, MatView<T> LU, const cplx<T>* Uinv, const cplx<T>* UinvNB, MatView<T> B,
+                Workspace& ws) {
+  if (n <= 0 || m <= 0) return;
+  if (n < TBN) return trsm_run<T>(m, n, LU, Uinv, B);
+  if (n == TBN) {
+    cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(m * TBN));
+    gemm<T>(m, TBN, TBN, ONE, B, MatView<T>{const_cast<cplx<T>*>(UinvNB), TBN}, ZERO, MatView<T>{W, m});
+    copy_mat<T>(m, TBN, MatView<T>{W, m}, B);
+    return;
+  }
+  const int64_t n1 = std::max<int64_t>(TBN, (n / 2) / TBN * TBN), n2 = n - n1;
+  trsm_run_h<T>(m, n1, LU, Uinv, UinvNB, B, ws);
+  gemm<T>(m, n2, n1, MONE, B, MatView<T>{LU.at(0, n1), LU.ld}, ONE, MatView<T>{B.at(0, n1), B.ld});
+  trsm_run_h<T>(m, n2, MatView<T>{LU.at(n1, n1), LU.ld}, Uinv + (n1 / TB) * TB2, UinvNB + (n1 / TBN) * TBN * TBN,
+                MatView<T>{B.at(0, n1), B.ld}, ws);
+}
+template <class T>
+void trsm_rlu_h(int64_t m, int64_t n, MatView<T> LU, const cplx<T>* Linv, const cplx<T>* LinvNB, MatView<T> B,
+                Workspace& ws) {
+  if (n <= 0 || m <= 0) return;
+  if (n < TBN) return trsm_rlu<T>(m, n, LU, Linv, B);
+  if (n == TBN) {
+    cplx<T>* W = (cplx<T>*)ws.get(2, sizeof(cplx<T>) * (size_t)(m * TBN));
+    gemm<T>(m, TBN, TBN, ONE, B, MatView<T>{const_cast<cplx<T>*>(LinvNB), TBN}, ZERO, MatView<T>{W, m});
+    copy_mat<T>(m, TBN, MatView<T>{W, m}, B);
+    return;
+  }
+  const int64_t n1 = std::max<int64_t>(TBN, (n / 2) / TBN * TBN), n2 = n - n1;
+  trsm_rlu_h<T>(m, n2, MatView<T>{LU.at(n1, n1), LU.ld}, Linv + (n1 / TB) * TB2, LinvNB + (n1 / TBN) * TBN * TBN,
+                MatView<T>{B.at(0, n1), B.ld}, ws);
+  gemm<T>(m, n1, n2, MONE, MatView<T>{B.at(0, n1), B.ld}, MatView<T>{LU.at(n1, 0), LU.ld}, ONE, B);
+  trsm_rlu_h<T>(m, n1, LU, Linv, LinvNB, B, ws);
 }
 
 // ------------------------------------------------ unpivoted panel leaf
@@ -863,14 +1101,30 @@ void getrs(int64_t n, int64_t nrhs, const LU<T>& f, MatView<T> B, Workspace& ws)
     gather_rows<T>(n, nrhs, f.sigma, B, Wv);
     copy_mat<T>(n, nrhs, Wv, B);
   }
-  trsm_llu<T>(n, nrhs, f.A, f.Linv, B);
-  trsm_lun<T>(n, nrhs, f.A, f.Uinv, B);
+  if (f.LinvNB && n >= 2 * TBN && nrhs <= TBN) {  // few right-hand sides: blocked, 3 launches per block
+    trsm_llu_nb<T>(n, nrhs, f, B, ws);
+    trsm_lun_nb<T>(n, nrhs, f, B, ws);
+  } else if (f.LinvNB && n >= 2 * TBN) {
+    trsm_llu_h<T>(n, nrhs, f.A, f.Linv, f.LinvNB, B, ws);
+    trsm_lun_h<T>(n, nrhs, f.A, f.Uinv, f.UinvNB, B, ws);
+  } else {
+    trsm_llu<T>(n, nrhs, f.A, f.Linv, B);
+    trsm_lun<T>(n, nrhs, f.A, f.Uinv, B);
+  }
 }
 
 template <class T>
 void getrs_right(int64_t m, int64_t n, const LU<T>& f, MatView<T> B, Workspace& ws) {
-  trsm_run<T>(m, n, f.A, f.Uinv, B);
-  trsm_rlu<T>(m, n, f.A, f.Linv, B);
+  if (f.LinvNB && n >= 2 * TBN && m <= TBN) {
+    trsm_run_nb<T>(m, n, f, B, ws);
+    trsm_rlu_nb<T>(m, n, f, B, ws);
+  } else if (f.LinvNB && n >= 2 * TBN) {
+    trsm_run_h<T>(m, n, f.A, f.Uinv, f.UinvNB, B, ws);
+    trsm_rlu_h<T>(m, n, f.A, f.Linv, f.LinvNB, B, ws);
+  } else {
+    trsm_run<T>(m, n, f.A, f.Uinv, B);
+    trsm_rlu<T>(m, n, f.A, f.Linv, B);
+  }
   if (f.sigma) {
     cplx<T>* W = (cplx<T>*)ws.get(1, sizeof(cplx<T>) * (size_t)(m * n));
     copy_mat<T>(m, n, B, MatView<T>{W, m});
@@ -891,8 +1145,13 @@ template <class T>
 void getri(int64_t n, const LU<T>& f, MatView<T> X, Workspace& ws) {
   hipLaunchKernelGGL(k_perm_identity<T>, grid2(n, n), dim3(256), 0, stream(), n, f.sigma, X.p, X.ld);
   MNP_HIP(hipGetLastError());
-  trsm_llu<T>(n, n, f.A, f.Linv, X);
-  trsm_lun<T>(n, n, f.A, f.Uinv, X);
+  if (f.LinvNB && n >= 2 * TBN) {
+    trsm_llu_h<T>(n, n, f.A, f.Linv, f.LinvNB, X, ws);
+    trsm_lun_h<T>(n, n, f.A, f.Uinv, f.UinvNB, X, ws);
+  } else {
+    trsm_llu<T>(n, n, f.A, f.Linv, X);
+    trsm_lun<T>(n, n, f.A, f.Uinv, X);
+  }
 }
 
 // ---------------------------------------------------- instantiations
@@ -902,6 +1161,8 @@ void getri(int64_t n, const LU<T>& f, MatView<T> X, Workspace& ws) {
   template int getrf<T>(int64_t, int64_t, MatView<T>, int32_t*, cplx<T>*, cplx<T>*, int, int*,   \
                         Workspace&);                                                             \
   template void block_inverses<T>(int64_t, MatView<T>, cplx<T>*, cplx<T>*);                      \
+  template void block_inverses_nb<T>(int64_t, MatView<T>, const cplx<T>*, const cplx<T>*, cplx<T>*, \
+                                     cplx<T>*, Workspace&);                                       \
   template void gather_rows<T>(int64_t, int64_t, const int32_t*, MatView<T>, MatView<T>);        \
   template void getrs<T>(int64_t, int64_t, const LU<T>&, MatView<T>, Workspace&);                \
   template void getrs_right<T>(int64_t, int64_t, const LU<T>&, MatView<T>, Workspace&);          \

@@ -56,6 +56,12 @@ int getrf(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, cplx<T>* Linv, cplx
 // Diagonal-block inverses of existing factors.
 template <class T>
 void block_inverses(int64_t n, MatView<T> LU, cplx<T>* Linv, cplx<T>* Uinv);
+// Inverses of the full 256 x 256 diagonal blocks of L and U (n / 256 blocks,
+// 256 x 256 column-major each) from the 32 x 32 ones.
+template <class T>
+void block_inverses_nb(int64_t n, MatView<T> LU, const cplx<T>* Linv, const cplx<T>* Uinv, cplx<T>* LinvNB,
+                       cplx<T>* UinvNB, Workspace& ws);
+inline int64_t inv_blocks_nb(int64_t n) { return (n / 256) * 256 * 256; }  // entries per 256-block array
 
 // Factors of an n x n matrix.  sigma (device, n) gives (S A)[r] = A[sigma[r]]
 // for the pivoting permutation S, or is NULL when no rows were exchanged.
@@ -65,6 +71,8 @@ struct LU {
   cplx<T>* Linv;
   cplx<T>* Uinv;
   const int32_t* sigma;
+  cplx<T>* LinvNB = nullptr;  // inverses of the 256 x 256 diagonal blocks (block_inverses_nb), or null
+  cplx<T>* UinvNB = nullptr;
 };
 
 // sigma from the swaps ipiv[0..n).
