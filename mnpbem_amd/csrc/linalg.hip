@@ -950,26 +950,33 @@ __global__ void __launch_bounds__(256) k_lu_nopiv_diag(int nb, cplx<T>* A, int64
       }
     __syncthreads();
   }
-  const int j = tid & 31;
-  if (tid < 32) {
-    for (int i = 0; i < TB; ++i) {
-      cplx<T> s = i == j ? mk<T>(1, 0) : czero<T>();
-      if (i > j && i < nb)
-        for (int k = j; k < i; ++k) s = csub<T>(s, cmul<T>(P[i][k], sL[k][j]));
-      sL[i][j] = i < j ? czero<T>() : s;
-    }
-  } else if (tid < 64) {
-    for (int i = TB - 1; i >= 0; --i) {
-      if (i > j) {
-        sU[i][j] = czero<T>();
-        continue;
-      }
-      cplx<T> s = i == j ? mk<T>(1, 0) : czero<T>();
-      for (int k = i + 1; k <= j; ++k) s = csub<T>(s, cmul<T>(P[i][k], sU[k][j]));
-      sU[i][j] = cdiv<T>(s, P[i][i]);
-    }
+  // L^{-1} and U^{-1} by elimination on the identity, all 1024 entries per
+  // step: X <- L^{-1} I forward (row k is final at step k), Y <- U^{-1} I
+  // backward (row k scaled by 1/u_kk, then eliminated from the rows above).
+  // The padding rows/columns (>= nb) of P hold the identity.
+  for (int idx = tid; idx < TB * TB; idx += 256) {
+    const int r = idx % TB, c = idx / TB;
+    sL[r][c] = r == c ? mk<T>(1, 0) : czero<T>();
+    sU[r][c] = r == c ? mk<T>(1, 0) : czero<T>();
   }
   __syncthreads();
+  for (int k = 0; k < TB; ++k) {
+    for (int idx = tid; idx < TB * TB; idx += 256) {
+      const int r = idx % TB, c = idx / TB;
+      if (r > k && c <= k) sL[r][c] = csub<T>(sL[r][c], cmul<T>(P[r][k], sL[k][c]));
+    }
+    __syncthreads();
+  }
+  for (int k = TB - 1; k >= 0; --k) {
+    const cplx<T> inv = cdiv<T>(mk<T>(1, 0), P[k][k]);
+    if (tid < TB && tid >= k) sU[k][tid] = cmul<T>(sU[k][tid], inv);
+    __syncthreads();
+    for (int idx = tid; idx < TB * TB; idx += 256) {
+      const int r = idx % TB, c = idx / TB;
+      if (r < k && c >= k) sU[r][c] = csub<T>(sU[r][c], cmul<T>(P[r][k], sU[k][c]));
+    }
+    __syncthreads();
+  }
   for (int idx = tid; idx < TB * TB; idx += 256) {
     const int r = idx % TB, c = idx / TB;
     if (r < nb && c < nb) A[c * lda + r] = P[r][c];
