@@ -272,6 +272,34 @@ int mnp_planewave_cross_sections(mnp_greenret_t g, double enei, mnp_complex eps_
                                  const void* pinfty_area, const void* sig2, const void* h2,
                                  double* ext, double* sca, double* abs_);
 
+/* ---------------------------------------------------- quasistatic solver
+ * Replaces bemsolver(p, op) with op.sim = 'stat' -> bemstat (BEM/@bemstat):
+ * F is the static surface derivative with refinement and the closed-surface
+ * correction (Greenfun/@compgreenstat/init.m:24-55, @greenstat/private/
+ * init.m), equal to the retarded F at k = 0 and assembled once at create;
+ * initmat forms and factorises Lambda I + F with Lambda = 2 pi (eps1 + eps2)
+ * / (eps1 - eps2) (bemstat/subsref.m:31-40); solve returns
+ * sig = -(Lambda I + F)^{-1} phip (mldivide.m:13-15).  solver MNP_SOLVER_FP64
+ * or MNP_SOLVER_MIXED (complex FP32 factors, FP64 refinement to 1e-12). */
+typedef struct mnp_bemstat_s* mnp_bemstat_t;
+int mnp_bemstat_create(mnp_greenret_t g, int solver, mnp_bemstat_t* out);
+int mnp_bemstat_destroy(mnp_bemstat_t b);
+int mnp_bemstat_initmat(mnp_bemstat_t b, double enei, mnp_complex eps1, mnp_complex eps2);
+/* phip, sig: device complex128 n x npol. */
+int mnp_bemstat_solve(mnp_bemstat_t b, int64_t npol, const void* phip, void* sig);
+/* Refinement sweeps and final relative residual of the last mixed solve. */
+int mnp_bemstat_stats(mnp_bemstat_t b, int* sweeps, double* relres);
+/* phi' = -nvec . pol (Simulation/static/@planewavestat/potential.m:13):
+ * pol host npol x 3, phip device out n x npol complex128. */
+int mnp_planewavestat_potential(mnp_greenret_t g, int64_t npol, const double* pol, void* phip);
+/* Cross sections from the dipole moment p = sum area pos sig (FP64 device
+ * sums): sca = 8 pi / 3 k^4 |p|^2, abs = 4 pi k Im(pol . p), ext = sca + abs
+ * (@planewavestat/scattering.m:14-18, absorption.m:14-19, extinction.m);
+ * host out, npol each. */
+int mnp_planewavestat_cross_sections(mnp_greenret_t g, double enei, mnp_complex eps_medium,
+                                     int64_t npol, const double* pol, const void* sig, double* ext,
+                                     double* sca, double* abs_);
+
 /* ------------------------------------------------ dipoles and point fields
  * Dipoles at points pts (host npt x 3, column-major) with moments dip (host
  * npt x 3 x ndip); column q = ipt + npt * idip of the outputs, as the

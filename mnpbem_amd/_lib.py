@@ -80,6 +80,13 @@ PROTOTYPES = {
     "mnp_bemretiter_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
     "mnp_bemretiter_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mnp_bemretiter_stats": (INT, [P, C.POINTER(INT), PD, C.POINTER(INT), PD, PD]),
+    "mnp_bemstat_create": (INT, [P, INT, C.POINTER(P)]),
+    "mnp_bemstat_destroy": (INT, [P]),
+    "mnp_bemstat_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
+    "mnp_bemstat_solve": (INT, [P, I64, P, P]),
+    "mnp_bemstat_stats": (INT, [P, C.POINTER(INT), C.POINTER(D)]),
+    "mnp_planewavestat_potential": (INT, [P, I64, P, P]),
+    "mnp_planewavestat_cross_sections": (INT, [P, D, MnpComplex, I64, P, P, P, P, P]),
     "mnp_dipole_potential": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P, P]),
     "mnp_field_points": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P]),
     "mnp_dipole_decayrate": (INT, [P, D, MnpComplex, I64, P, I64, P, I64, P, P, P, P, P, P, P]),

@@ -277,12 +277,62 @@ class bemretiter:
                     compression=cp.value)
 
 
+class bemstat:
+    """Quasistatic BEM solver (BEM/@bemstat): sig = -(Lambda + F)^{-1} phi'.
+
+    ``precision='fp64'`` is the drop-in; ``'mixed'`` factorises in complex
+    FP32 and refines in FP64 against F x + Lambda x.
+    """
+
+    def __init__(self, p, op=None, precision=None):
+        op = p.op if op is None else op
+        self.p = p
+        self.precision = precision or op.get("precision", "fp64")
+        solver = _lib.MNP_SOLVER_FP64 if self.precision == "fp64" else _lib.MNP_SOLVER_MIXED
+        h = C.c_void_p()
+        _lib.call("mnp_bemstat_create", p.green.handle, solver, C.byref(h))
+        self.handle = h
+        self.enei = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib.lib().mnp_bemstat_destroy(h)
+            self.handle = None
+
+    def __call__(self, enei):
+        if self.enei != enei:
+            _lib.call("mnp_bemstat_initmat", self.handle, float(enei), _lib.MnpComplex.of(self.p.eps1(enei)),
+                      _lib.MnpComplex.of(self.p.eps2(enei)))
+            self.enei = enei
+        return self
+
+    def solve(self, exc):
+        """sig = bem \\ exc  (BEM/@bemstat/mldivide.m:13-15), exc with field phip."""
+        self(exc.enei)
+        n, npol = self.p.n, exc.phip.shape[-1]
+        sig = dev.colmajor((n, npol))
+        _lib.call("mnp_bemstat_solve", self.handle, npol, dev.ptr(dev.base(exc.phip)), dev.ptr(dev.base(sig)))
+        return compstruct(self.p, exc.enei, sig=sig)
+
+    mldivide = solve
+
+    def stats(self):
+        it, rr = C.c_int(), C.c_double()
+        _lib.call("mnp_bemstat_stats", self.handle, C.byref(it), C.byref(rr))
+        return it.value, rr.value
+
+
 def bemsolver(p, op=None, **kw):
-    """Base/bemsolver.m: bemret for op.sim == 'ret', bemretiter when op.iter is set."""
+    """Base/bemsolver.m: bemret for op.sim == 'ret' (bemretiter when op.iter is
+    set), bemstat for op.sim == 'stat'."""
     op = dict(p.op if op is None else op)
     op.update(kw)
-    if op.get("sim", "ret") != "ret":
-        raise NotImplementedError("only the retarded solver is implemented")
+    sim = op.get("sim", "ret")
+    if sim == "stat":
+        return bemstat(p, op)
+    if sim != "ret":
+        raise NotImplementedError(f"unknown simulation type {sim!r}")
     if op.get("iter"):
         return bemretiter(p, op)
     return bemret(p, op)
@@ -360,8 +410,50 @@ class planewaveret:
     extinction, scattering, absorption = ext, sca, abs
 
 
-def planewave(pol, dir, op=None, **kw):
-    """Base/planewave.m for op.sim == 'ret'."""
+class planewavestat:
+    """Quasistatic plane wave (Simulation/static/@planewavestat): phi' = -nvec . pol;
+    cross sections from the induced dipole moment."""
+
+    def __init__(self, pol, op=None, medium=1):
+        self.pol = np.atleast_2d(np.asarray(pol, dtype=np.float64))
+        self.medium = medium
+
+    def __call__(self, p, enei):
+        n, npol = p.n, len(self.pol)
+        phip = dev.colmajor((n, npol))
+        pol = np.asfortranarray(self.pol)
+        _lib.call("mnp_planewavestat_potential", p.green.handle, npol, pol.ctypes.data_as(C.c_void_p),
+                  dev.ptr(dev.base(phip)))
+        return compstruct(p, enei, phip=phip)
+
+    potential = __call__
+
+    def cross_sections(self, sig):
+        p, enei = sig.p, sig.enei
+        npol = len(self.pol)
+        out = [np.zeros(npol) for _ in range(3)]
+        pol = np.asfortranarray(self.pol)
+        _lib.call("mnp_planewavestat_cross_sections", p.green.handle, float(enei),
+                  _lib.MnpComplex.of(p.eps[self.medium - 1](enei)[0]), npol, pol.ctypes.data_as(C.c_void_p),
+                  dev.ptr(dev.base(sig.sig)), *[o.ctypes.data_as(C.c_void_p) for o in out])
+        return tuple(out)
+
+    def ext(self, sig):
+        return self.cross_sections(sig)[0]
+
+    def sca(self, sig):
+        return self.cross_sections(sig)[1]
+
+    def abs(self, sig):
+        return self.cross_sections(sig)[2]
+
+    extinction, scattering, absorption = ext, sca, abs
+
+
+def planewave(pol, dir=None, op=None, **kw):
+    """Base/planewave.m: planewaveret for op.sim == 'ret', planewavestat for 'stat'."""
+    if op is not None and op.get("sim", "ret") == "stat":
+        return planewavestat(pol, op, **kw)
     return planewaveret(pol, dir, op, **kw)
 
 

@@ -1,0 +1,345 @@
+// Quasistatic BEM solver and plane wave on gfx950 (MNPBEM BEM/@bemstat,
+// Simulation/static/@planewavestat).
+//
+// sig = -(Lambda I + F)^{-1} phi' with Lambda = 2 pi (eps1 + eps2)/(eps1 - eps2)
+// (bemstat/subsref.m:31-40, mldivide.m:13-15).  F is the static surface
+// derivative with refinement and the closed-surface correction
+// (compgreenstat/init.m:24-55), which is the retarded F at k = 0: it is
+// assembled once per mesh by GreenRet::assemble in FP64.  Per wavelength the
+// shifted matrix is factorised (LU without row exchanges first, partial
+// pivoting when the FP64 factors would differ or on a zero pivot), in FP64
+// (the drop-in) or in complex FP32 with FP64 iterative refinement against
+// F x + Lambda x.
+#include <complex>
+#include <memory>
+#include <vector>
+
+#include "mnp_linalg.hpp"
+
+namespace mnp {
+
+// M = F + lam I in precision T
+template <class T>
+__global__ void k_shift_diag(int64_t n, const double2* F, double2 lam, cplx<T>* M) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (i >= n || j >= n) return;
+  double2 v = F[j * n + i];
+  if (i == j) v = cadd<double>(v, lam);
+  M[j * n + i] = mk<T>((T)v.x, (T)v.y);
+}
+// y <- a * y (complex scalar), len entries
+__global__ void k_cscal(int64_t len, double2* y, double2 a) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < len) y[i] = cmul<double>(a, y[i]);
+}
+// r = -phip - lam x  (the F x part is added by a GEMM afterwards)
+__global__ void k_stat_res0(int64_t len, const double2* phip, const double2* x, double2 lam, double2* r) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  const double2 lx = cmul<double>(lam, x[i]);
+  r[i] = mk<double>(-phip[i].x - lx.x, -phip[i].y - lx.y);
+}
+__global__ void k_add(int64_t len, double2* x, const double2* d) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < len) x[i] = cadd<double>(x[i], d[i]);
+}
+// partial sums of |v|^2 over chunks
+__global__ void k_sumsq(int64_t len, const double2* v, double* part) {
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
+    s += v[i].x * v[i].x + v[i].y * v[i].y;
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// phi' = -nvec . pol  (planewavestat/potential.m:13); pol device npol x 3
+__global__ void k_pwstat(int64_t n, int64_t npol, const double* nvec, const double* pol, double2* phip) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  const double v =
+      nvec[i] * pol[q] + nvec[n + i] * pol[npol + q] + nvec[2 * n + i] * pol[2 * npol + q];
+  phip[q * n + i] = mk<double>(-v, 0.0);
+}
+// dipole moment partial sums: part[(q * nchunk + c) * 3 + d] = sum_i area_i pos_id sig_iq
+__global__ void k_dipmoment(int64_t n, const double* pos, const double* area, const double2* sig,
+                            double2* part) {
+  const int64_t c = blockIdx.x, q = blockIdx.y, nchunk = gridDim.x;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t i = c * blockDim.x + threadIdx.x; i < n; i += nchunk * blockDim.x) {
+    const double2 v = sig[q * n + i];
+    for (int d = 0; d < 3; ++d) {
+      const double w = area[i] * pos[d * n + i];
+      s[2 * d] += w * v.x;
+      s[2 * d + 1] += w * v.y;
+    }
+  }
+  __shared__ double red[6][256];
+  for (int t = 0; t < 6; ++t) red[t][threadIdx.x] = s[t];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int t = 0; t < 6; ++t) red[t][threadIdx.x] += red[t][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3)
+    part[(q * nchunk + c) * 3 + threadIdx.x] = mk<double>(red[2 * threadIdx.x][0], red[2 * threadIdx.x + 1][0]);
+}
+
+template <class T>
+struct StatFactor {
+  int64_t n;
+  bool exact;
+  cplx<T>*M = nullptr, *backup = nullptr, *inv = nullptr, *invnb = nullptr;
+  int32_t *piv = nullptr, *sig = nullptr;
+  LU<T> f{};
+  int fallbacks = 0;
+  Workspace ws;
+  StatFactor(int64_t n_, bool exact_) : n(n_), exact(exact_) {
+    M = dev_alloc<cplx<T>>(n * n);
+    backup = dev_alloc<cplx<T>>(n * n);
+    inv = dev_alloc<cplx<T>>(2 * inv_blocks(n));
+    invnb = dev_alloc<cplx<T>>(2 * inv_blocks_nb(n));
+    piv = dev_alloc<int32_t>(n);
+    sig = dev_alloc<int32_t>(n);
+  }
+  ~StatFactor() {
+    void* ps[] = {M, backup, inv, invnb, piv, sig};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
+  StatFactor(const StatFactor&) = delete;
+  StatFactor& operator=(const StatFactor&) = delete;
+  // M = F + lam I, factorised
+  void build(const double2* F, double2 lam) {
+    const MatView<T> Mv{M, n};
+    hipLaunchKernelGGL(k_shift_diag<T>, dim3((unsigned)cdiv_up(n, 256), (unsigned)std::min<int64_t>(n, 65535),
+                                             (unsigned)cdiv_up(n, 65535)),
+                       dim3(256), 0, stream(), n, F, lam, M);
+    MNP_HIP(hipGetLastError());
+    copy_mat<T>(n, n, Mv, MatView<T>{backup, n});
+    cplx<T>*Linv = inv, *Uinv = inv + inv_blocks(n);
+    int swap = 0;
+    int info = getrf<T>(n, n, Mv, piv, Linv, Uinv, PIVOT_NONE, &swap, ws);
+    const int32_t* sigma = nullptr;
+    if (info != 0 || (swap && exact)) {
+      ++fallbacks;
+      copy_mat<T>(n, n, MatView<T>{backup, n}, Mv);
+      info = getrf<T>(n, n, Mv, piv, Linv, Uinv, PIVOT_PARTIAL, nullptr, ws);
+      perm_from_pivots(n, piv, sig, ws);
+      sigma = sig;
+    }
+    MNP_CHECK(info == 0, MNP_ERR_SINGULAR, "bemstat: singular Lambda + F (zero pivot " + std::to_string(info) + ")");
+    cplx<T>*LinvNB = invnb, *UinvNB = invnb + inv_blocks_nb(n);
+    block_inverses_nb<T>(n, Mv, Linv, Uinv, LinvNB, UinvNB, ws);
+    f = LU<T>{Mv, Linv, Uinv, sigma, LinvNB, UinvNB};
+  }
+};
+
+struct BemStat {
+  const GreenRet* g;
+  int64_t n;
+  int solver;
+  double2* F = nullptr;  // static surface derivative, FP64, n x n
+  std::unique_ptr<StatFactor<double>> f64;
+  std::unique_ptr<StatFactor<float>> f32;
+  double2 lam{};
+  bool ready = false;
+  double refine_tol = 1e-12;
+  int refine_maxit = 10;
+  int last_sweeps = 0;
+  double last_relres = 0;
+  Workspace vws;
+
+  BemStat(const GreenRet* g_, int solver_) : g(g_), n(g_->n), solver(solver_) {
+    F = dev_alloc<double2>(n * n);
+    g->assemble(mk<double>(0, 0), 0.0, OutPair{}, OutPair{F, nullptr}, n);
+    if (solver == MNP_SOLVER_FP64)
+      f64.reset(new StatFactor<double>(n, true));
+    else
+      f32.reset(new StatFactor<float>(n, false));
+  }
+  ~BemStat() {
+    if (F) (void)hipFree(F);
+  }
+
+  void initmat(double2 e1, double2 e2) {
+    // Lambda = 2 pi (eps1 + eps2) / (eps1 - eps2)
+    lam = cscale<double>(cdiv<double>(cadd<double>(e1, e2), csub<double>(e1, e2)), 2 * M_PI);
+    if (f64)
+      f64->build(F, lam);
+    else
+      f32->build(F, lam);
+    ready = true;
+  }
+
+  double norm(const double2* v, int64_t len) {
+    const int nb = 256;
+    double* part = (double*)vws.get(5, sizeof(double) * nb);
+    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(256), 0, stream(), len, v, part);
+    std::vector<double> h(nb);
+    MNP_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipStreamSynchronize(stream()));
+    double s = 0;
+    for (double x : h) s += x;
+    return std::sqrt(s);
+  }
+
+  // sig = -(Lambda + F)^{-1} phip, n x p
+  void solve(int64_t p, const double2* phip, double2* sig) {
+    MNP_CHECK(ready, MNP_ERR_STATE, "bemstat_solve: initmat not called");
+    MNP_CHECK(p >= 1, MNP_ERR_ARG, "bemstat_solve: npol < 1");
+    const int64_t len = n * p;
+    const unsigned gl = (unsigned)cdiv_up(len, 256);
+    if (f64) {
+      copy_mat<double>(n, p, MatView<double>{const_cast<double2*>(phip), n}, MatView<double>{sig, n});
+      getrs<double>(n, p, f64->f, MatView<double>{sig, n}, f64->ws);
+      hipLaunchKernelGGL(k_cscal, dim3(gl), dim3(256), 0, stream(), len, sig, mk<double>(-1, 0));
+      MNP_HIP(hipGetLastError());
+      last_sweeps = 0;
+      last_relres = 0;
+      return;
+    }
+    // mixed: x0 = M32^{-1}(-phip), then x += M32^{-1}(-phip - (F + Lambda) x)
+    double2* r = (double2*)vws.get(0, sizeof(double2) * len);
+    float2* r32 = (float2*)vws.get(1, sizeof(float2) * len);
+    const double bn = norm(phip, len);
+    MNP_HIP(hipMemsetAsync(sig, 0, sizeof(double2) * len, stream()));
+    int it = 0;
+    double rel = 1.0;
+    for (; it <= refine_maxit; ++it) {
+      hipLaunchKernelGGL(k_stat_res0, dim3(gl), dim3(256), 0, stream(), len, phip, sig, lam, r);
+      gemm<double>(n, p, n, mk<double>(-1, 0), MatView<double>{F, n}, MatView<double>{sig, n}, mk<double>(1, 0),
+                   MatView<double>{r, n});
+      rel = bn > 0 ? norm(r, len) / bn : 0.0;
+      if (rel <= refine_tol) break;
+      if (it == refine_maxit) break;
+      convert_mat<double, float>(n, p, MatView<double>{r, n}, MatView<float>{r32, n});
+      getrs<float>(n, p, f32->f, MatView<float>{r32, n}, f32->ws);
+      convert_mat<float, double>(n, p, MatView<float>{r32, n}, MatView<double>{r, n});
+      hipLaunchKernelGGL(k_add, dim3(gl), dim3(256), 0, stream(), len, sig, r);
+      MNP_HIP(hipGetLastError());
+    }
+    last_sweeps = it;
+    last_relres = rel;
+    MNP_CHECK(rel <= refine_tol, MNP_ERR_NOCONV,
+              "bemstat_solve: refinement stopped at relative residual " + std::to_string(rel));
+  }
+};
+
+}  // namespace mnp
+
+// ============================================================== C ABI
+using namespace mnp;
+
+struct mnp_bemstat_s {
+  BemStat* b;
+};
+
+extern "C" {
+
+int mnp_bemstat_create(mnp_greenret_t g, int solver, mnp_bemstat_t* out) {
+  return guarded([&] {
+    MNP_CHECK(g && out, MNP_ERR_ARG, "bemstat_create: NULL argument");
+    MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED, MNP_ERR_ARG, "bemstat_create: unknown solver");
+    auto* h = new mnp_bemstat_s{nullptr};
+    try {
+      h->b = new BemStat(g->g, solver);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int mnp_bemstat_destroy(mnp_bemstat_t b) {
+  return guarded([&] {
+    if (!b) return;
+    delete b->b;
+    delete b;
+  });
+}
+
+int mnp_bemstat_initmat(mnp_bemstat_t b, double enei, mnp_complex eps1, mnp_complex eps2) {
+  return guarded([&] {
+    MNP_CHECK(b && enei > 0, MNP_ERR_ARG, "bemstat_initmat: bad argument");
+    MNP_CHECK(eps1.re != eps2.re || eps1.im != eps2.im, MNP_ERR_ARG, "bemstat_initmat: eps1 == eps2");
+    b->b->initmat(to_d2(eps1), to_d2(eps2));
+  });
+}
+
+int mnp_bemstat_solve(mnp_bemstat_t b, int64_t npol, const void* phip, void* sig) {
+  return guarded([&] {
+    MNP_CHECK(b && phip && sig, MNP_ERR_ARG, "bemstat_solve: NULL argument");
+    b->b->solve(npol, (const double2*)phip, (double2*)sig);
+  });
+}
+
+int mnp_bemstat_stats(mnp_bemstat_t b, int* sweeps, double* relres) {
+  return guarded([&] {
+    MNP_CHECK(b, MNP_ERR_ARG, "bemstat_stats: NULL handle");
+    if (sweeps) *sweeps = b->b->last_sweeps;
+    if (relres) *relres = b->b->last_relres;
+  });
+}
+
+int mnp_planewavestat_potential(mnp_greenret_t g, int64_t npol, const double* pol, void* phip) {
+  return guarded([&] {
+    MNP_CHECK(g && pol && phip && npol >= 1 && npol < 65536, MNP_ERR_ARG, "planewavestat_potential: bad argument");
+    const GreenRet& G = *g->g;
+    double* dpol = dev_alloc<double>(3 * npol);
+    MNP_HIP(hipMemcpyAsync(dpol, pol, sizeof(double) * 3 * npol, hipMemcpyHostToDevice, stream()));
+    hipLaunchKernelGGL(k_pwstat, dim3((unsigned)cdiv_up(G.n, 256), (unsigned)npol), dim3(256), 0, stream(), G.n,
+                       npol, G.geo.nvec, dpol, (double2*)phip);
+    MNP_HIP(hipGetLastError());
+    MNP_HIP(hipStreamSynchronize(stream()));
+    MNP_HIP(hipFree(dpol));
+  });
+}
+
+int mnp_planewavestat_cross_sections(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t npol,
+                                     const double* pol, const void* sig, double* ext, double* sca,
+                                     double* abs_) {
+  return guarded([&] {
+    MNP_CHECK(g && pol && sig && npol >= 1 && npol < 65536 && enei > 0, MNP_ERR_ARG,
+              "planewavestat_cross_sections: bad argument");
+    const GreenRet& G = *g->g;
+    const int nchunk = 64;
+    double2* part = dev_alloc<double2>(npol * nchunk * 3);
+    hipLaunchKernelGGL(k_dipmoment, dim3(nchunk, (unsigned)npol), dim3(256), 0, stream(), G.n, G.geo.pos,
+                       G.geo.area, (const double2*)sig, part);
+    MNP_HIP(hipGetLastError());
+    std::vector<double2> h(npol * nchunk * 3);
+    MNP_HIP(hipMemcpyAsync(h.data(), part, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipStreamSynchronize(stream()));
+    MNP_HIP(hipFree(part));
+    using C = std::complex<double>;
+    const C k = 2 * M_PI / enei * std::sqrt(C(eps_medium.re, eps_medium.im));
+    for (int64_t q = 0; q < npol; ++q) {
+      C dip[3] = {0, 0, 0};
+      for (int c = 0; c < nchunk; ++c)
+        for (int d = 0; d < 3; ++d) dip[d] += C(h[(q * nchunk + c) * 3 + d].x, h[(q * nchunk + c) * 3 + d].y);
+      double a2 = 0;
+      C pd = 0;
+      for (int d = 0; d < 3; ++d) {
+        a2 += std::norm(dip[d]);
+        pd += pol[d * npol + q] * dip[d];
+      }
+      const C s = 8 * M_PI / 3 * k * k * k * k * a2;  // @planewavestat/scattering.m:14-18
+      const C a = 4 * M_PI * k * pd.imag();          // absorption.m:14-19
+      if (sca) sca[q] = s.real();
+      if (abs_) abs_[q] = a.real();
+      if (ext) ext[q] = (s + a).real();              // extinction.m
+    }
+  });
+}
+
+}  // extern "C"

@@ -47,3 +47,17 @@ def mie_cross_sections(enei, eps_in, eps_out, diameter, lmax=20):
         ext[i] = 2 * np.pi / k ** 2 * np.sum((2 * l + 1) * np.real(a + b))
         sca[i] = 2 * np.pi / k ** 2 * np.sum((2 * l + 1) * (np.abs(a) ** 2 + np.abs(b) ** 2))
     return ext, sca, ext - sca
+
+
+def miestat_cross_sections(enei, eps_in, eps_out, diameter):
+    """Quasistatic sphere (Mie/@miestat/scattering.m:13-27, absorption.m,
+    extinction.m): alpha = (eps_z - 1)/(eps_z + 2) a^3, sca = 8 pi/3 k^4 |alpha|^2,
+    abs = 4 pi k Im(alpha)."""
+    enei = np.atleast_1d(np.asarray(enei, dtype=float))
+    eps_in = np.broadcast_to(np.asarray(eps_in, dtype=complex), enei.shape)
+    k = (2 * np.pi / enei * np.sqrt(complex(eps_out))).real
+    epsz = eps_in / eps_out
+    alpha = (epsz - 1) / (epsz + 2) * (diameter / 2) ** 3
+    sca = 8 * np.pi / 3 * k ** 4 * np.abs(alpha) ** 2
+    ab = 4 * np.pi * k * np.imag(alpha)
+    return sca + ab, sca, ab

@@ -495,3 +495,43 @@ def scattering(pos, area, sig, enei, eps_medium, pinfty_nvec, pinfty_area):
     sca = pinfty_area @ dsca
     nb = np.sqrt(complex(eps_medium))
     return np.real(sca / (0.5 * nb))
+
+
+# --------------------------------------------------------------------------
+# quasistatic solver (BEM/@bemstat) and plane wave (Simulation/static)
+# --------------------------------------------------------------------------
+class BemStat:
+    """Quasistatic BEM solver for one closed particle: sig = -(diag(Lambda) + F)^{-1} phi'
+    with Lambda = 2 pi (eps1 + eps2) / (eps1 - eps2) (BEM/@bemstat/subsref.m:31-40,
+    mldivide.m:13-15).  F is the static surface derivative with refinement and the
+    closed-surface correction (Greenfun/@compgreenstat/init.m:24-55,
+    @greenstat/private/init.m), which equals the retarded F at k = 0."""
+
+    def __init__(self, green):
+        self.g = green
+        self.F = green.eval(0.0, ("F",))["F"]
+
+    def initmat(self, enei, eps1, eps2):
+        lam = 2 * np.pi * (eps1 + eps2) / (eps1 - eps2)
+        self.enei = enei
+        self.mat = -np.linalg.inv(np.diag(np.full(self.g.n, lam)) + self.F)
+        return self
+
+    def solve(self, phip):
+        return self.mat @ phip
+
+
+def planewavestat_potential(nvec, pol):
+    """phi' = -nvec . pol (Simulation/static/@planewavestat/potential.m:13)."""
+    return -nvec @ np.atleast_2d(pol).T
+
+
+def stat_cross_sections(pos, area, sig, pol, enei, eps_medium):
+    """(ext, sca, abs) from the dipole moment p = sum area pos sig
+    (@planewavestat/scattering.m:14-18, absorption.m:14-19, extinction.m)."""
+    pol = np.atleast_2d(pol)
+    dip = (area[:, None] * pos).T @ sig                     # (3, npol)
+    k = 2 * np.pi / enei * np.sqrt(complex(eps_medium))
+    sca = 8 * np.pi / 3 * k ** 4 * np.sum(np.abs(dip) ** 2, axis=0)
+    ab = 4 * np.pi * k * np.imag(np.sum(pol.T * dip, axis=0))
+    return np.real(sca + ab), np.real(sca), np.real(ab)
