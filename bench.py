@@ -107,6 +107,37 @@ def run_gpu(level, steps, warmup, fp64_steps):
                 max_err=max_err, sweeps=sweeps, ext=[list(map(float, r[0])) for r in res_mx])
 
 
+def run_quasistatic(level=5, steps=3):
+    """Quasistatic solver (bemstat) on the same 20,480-face ellipsoid and
+    wavelengths as the headline: one complex LU of Lambda + F per wavelength,
+    mixed (FP32 factors + FP64 refinement) and FP64; extinction from the
+    induced dipole."""
+    import torch
+
+    from mnpbem_amd import bem as B, shapes
+
+    v, f = shapes.ellipsoid(level, (10.0, 10.0, 40.0))
+    op = B.bemoptions(sim="stat")
+    p = B.comparticle([B.epsconst(EPS_OUT), B.epsdrude(*DRUDE)], [(v, f)], [2, 1], 1, op)
+    exc = B.planewave(POL, op=op)
+    lams = np.linspace(500.0, 1000.0, 64)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        bem = B.bemsolver(p, op, precision=prec)
+        exc.ext(bem.solve(exc(p, lams[0])))  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ext = [exc.ext(bem.solve(exc(p, lam))) for lam in lams[1:1 + steps]]
+        torch.cuda.synchronize()
+        res[prec] = (steps / (time.perf_counter() - t), ext)
+        del bem
+    err = max(float(np.max(np.abs(a / b - 1))) for a, b in zip(res["mixed"][1], res["fp64"][1]))
+    log(f"[bench] quasistatic: mixed {res['mixed'][0]:.2f}/s, fp64 {res['fp64'][0]:.2f}/s")
+    return {"workload": f"quasistatic bemstat, same ellipsoid ({len(f)} faces) and wavelengths as the headline",
+            "wavelengths_per_s_mixed": res["mixed"][0], "wavelengths_per_s_fp64": res["fp64"][0],
+            "speedup": res["mixed"][0] / res["fp64"][0], "max_rel_err_vs_fp64": err}
+
+
 def run_config4(nlam=16, seed=2024):
     """H-matrix iterative path with dipole excitation: sphere of diameter 150 nm
     (icosphere level 6, 81,920 faces), Drude gold in eps=1, dipole 5 nm above
@@ -332,6 +363,7 @@ def main():
     if t64 is not None and t64.launches[0] and t64.ms[0] > 0:
         a64 = t64.work[0] / (t64.ms[0] * 1e-3) / 1e12
         out["fp64_gemm"] = {"achieved": a64, "peak": PEAK["fp64"], "frac": a64 / PEAK["fp64"]}
+    out["quasistatic"] = run_quasistatic(a.level)
     if not a.no_config4:
         out["hmatrix_dipole"] = run_config4()
         if not a.no_cpu_baseline:
