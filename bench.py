@@ -38,14 +38,14 @@ DRUDE = (10.0, 9.065, 0.0708)
 EPS_OUT = 1.77
 POL = [[1.0, 0, 0], [0, 0, 1.0]]
 DIR = [[0, 0, 1.0], [1.0, 0, 0]]
-PEAK = {"fp32": 157.3, "fp64": 78.6}  # dense MFMA TFLOP/s, MI355X
+PEAK = {"fp32": 157.3, "fp64": 78.6, "bf16": 2500.0}  # dense MFMA TFLOP/s, MI355X
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(level, steps, warmup, fp64_steps):
+def run_gpu(level, steps, warmup, fp64_steps, split_bf16=True):
     import torch
 
     from mnpbem_amd import _lib, bem as B, device as dev, shapes
@@ -91,6 +91,17 @@ def run_gpu(level, steps, warmup, fp64_steps):
     del mixed
     torch.cuda.empty_cache()
 
+    res_bf, t_bf, tm_bf, sweeps_bf = [], 0.0, None, None
+    if split_bf16:
+        sb = B.bemsolver(p, op, precision="mixed_bf16")
+        if warmup:
+            spectrum(sb, enei[:warmup], False)
+        res_bf, t_bf, tm_bf = spectrum(sb, enei[warmup:warmup + steps], True)
+        sweeps_bf = sb.stats()
+        log(f"[bench] mixed_bf16 {steps} wavelengths in {t_bf:.2f}s, refinement {sweeps_bf}")
+        del sb
+        torch.cuda.empty_cache()
+
     lam64 = enei[warmup:warmup + fp64_steps]
     res_64, t_64, tm64 = [], 0.0, None
     if fp64_steps:
@@ -103,8 +114,11 @@ def run_gpu(level, steps, warmup, fp64_steps):
         for q in range(2):
             errs.append(abs(a[q] - b[q]) / np.abs(b[q]))
     max_err = float(np.max(errs)) if errs else None
+    errs_bf = [abs(a[q] - b[q]) / np.abs(b[q]) for a, b in zip(res_bf, res_64) for q in range(2)]
     return dict(n=n, t_mx=t_mx, t_64=t_64, n64=len(lam64), timers=tm, timers64=tm64,
-                max_err=max_err, sweeps=sweeps, ext=[list(map(float, r[0])) for r in res_mx])
+                max_err=max_err, sweeps=sweeps, ext=[list(map(float, r[0])) for r in res_mx],
+                t_bf=t_bf, timers_bf=tm_bf, sweeps_bf=sweeps_bf,
+                max_err_bf=float(np.max(errs_bf)) if errs_bf else None)
 
 
 def run_quasistatic(level=5, steps=3):
@@ -292,6 +306,7 @@ def main():
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-config4", action="store_true", help="skip the H-matrix dipole measurement")
+    ap.add_argument("--no-split-bf16", action="store_true", help="skip the split-bf16 chain measurement")
     ap.add_argument("--config4-only", action="store_true",
                     help="only the H-matrix dipole measurement (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -313,7 +328,7 @@ def main():
     if a.config4_only:
         print(json.dumps({"hmatrix_dipole": run_config4()}), flush=True)
         return
-    r = run_gpu(a.level, a.steps, a.warmup, a.fp64_steps)
+    r = run_gpu(a.level, a.steps, a.warmup, a.fp64_steps, not a.no_split_bf16)
     tm = r["timers"]
     gi = 1  # MNP_TIMER_GEMM_FP32
     gemm_ms = tm.ms[gi] / max(tm.launches[gi], 1)
@@ -363,6 +378,30 @@ def main():
     if t64 is not None and t64.launches[0] and t64.ms[0] > 0:
         a64 = t64.work[0] / (t64.ms[0] * 1e-3) / 1e12
         out["fp64_gemm"] = {"achieved": a64, "peak": PEAK["fp64"], "frac": a64 / PEAK["fp64"]}
+    tb = r["timers_bf"]
+    if tb is not None:
+        si = 6  # MNP_TIMER_GEMM_SBF16
+        sb_ms = tb.ms[si] / max(tb.launches[si], 1)
+        sb_flops = tb.work[si] / max(tb.launches[si], 1)
+        sb_ach = sb_flops / (sb_ms * 1e-3) / 1e12 if sb_ms > 0 else 0.0
+        v_bf = a.steps / r["t_bf"]
+        out["split_bf16"] = {
+            "precision": "complex FP32 chain, large products from hi/lo bf16 splits on bf16 MFMA, "
+                         "+ FP64 refinement",
+            "wavelengths_per_s": v_bf, "ms_per_step": 1e3 * r["t_bf"] / a.steps,
+            "speedup_vs_fp64": v_bf / (r["n64"] / r["t_64"]) if r["n64"] else None,
+            "speedup_vs_fp32_mfma": v_bf / out["value"],
+            "max_rel_err_vs_fp64": r["max_err_bf"],
+            "refinement": {"sweeps": r["sweeps_bf"][0], "relres": r["sweeps_bf"][1]},
+            "roofline": {
+                "bound": "mfma", "kernel": "k_cgemm_sbf16<4,4,2> (4M complex GEMM, 3 bf16 products per real one, 128 x 128 tiles)",
+                "achieved": sb_ach, "peak": PEAK["bf16"], "unit": "TFLOP/s", "frac": sb_ach / PEAK["bf16"],
+                "executed_bf16_tflops": 3.0 * sb_ach, "executed_frac": 3.0 * sb_ach / PEAK["bf16"],
+                "launches_per_step": tb.launches[si] / a.steps, "avg_launch_ms": sb_ms,
+                "avg_flops_per_launch": sb_flops,
+            },
+            "gemm_share_of_step": tb.ms[si] * 1e-3 / r["t_bf"],
+        }
     out["quasistatic"] = run_quasistatic(a.level)
     if not a.no_config4:
         out["hmatrix_dipole"] = run_config4()

@@ -43,14 +43,24 @@ enum {
   MNP_ERR_STATE = 6      /* call out of order (e.g. solve before initmat)      */
 };
 
-enum { MNP_FP64 = 0, MNP_FP32 = 1 };
+enum {
+  MNP_FP64 = 0,
+  MNP_FP32 = 1,
+  /* mnp_gemm only: complex64 operands, products from hi/lo bf16 splits on
+   * bf16 MFMA with FP32 accumulation (the kernel of MNP_SOLVER_MIXED_BF16) */
+  MNP_FP32_SPLIT_BF16 = 2
+};
 enum { MNP_KEY_G = 0, MNP_KEY_F = 1, MNP_KEY_H1 = 2, MNP_KEY_H2 = 3 };
 
 /* Solver variants of mnp_bemret_create. */
 enum {
   MNP_SOLVER_FP64 = 0,  /* complex FP64 chain: the plain drop-in of bemret      */
-  MNP_SOLVER_MIXED = 1  /* complex FP32 chain as preconditioner + FP64 refinement
+  MNP_SOLVER_MIXED = 1, /* complex FP32 chain as preconditioner + FP64 refinement
                            with the exact operator (bemretiter, precond 'full') */
+  MNP_SOLVER_MIXED_BF16 = 2 /* as MNP_SOLVER_MIXED, with the chain's large
+                           complex64 products formed from hi/lo bf16 splits of
+                           the FP32 operands on bf16 MFMA (about 2^-16 relative
+                           per product); the refinement is unchanged */
 };
 
 /* ------------------------------------------------------------- library */
@@ -76,7 +86,8 @@ enum {
   MNP_TIMER_ASSEMBLE = 3,  /* dense G/H assembly and H-matrix full blocks        */
   MNP_TIMER_HMATVEC = 4,   /* H-matrix times matrix (bytes of stored blocks)     */
   MNP_TIMER_ACA = 5,       /* ACA of admissible blocks                           */
-  MNP_TIMER_COUNT = 6
+  MNP_TIMER_GEMM_SBF16 = 6, /* complex64 GEMM on split-bf16 MFMA, large tiles    */
+  MNP_TIMER_COUNT = 7
 };
 typedef struct {
   int64_t launches[MNP_TIMER_COUNT];

@@ -11,9 +11,9 @@ LIB_PATH = os.path.join(HERE, "libmnpbem_hip.so")
 
 MNP_OK = 0
 MNP_ERR_ARG, MNP_ERR_HIP, MNP_ERR_SINGULAR, MNP_ERR_NOCONV, MNP_ERR_INTERNAL, MNP_ERR_STATE = 1, 2, 3, 4, 5, 6
-MNP_FP64, MNP_FP32 = 0, 1
+MNP_FP64, MNP_FP32, MNP_FP32_SPLIT_BF16 = 0, 1, 2
 MNP_KEY = {"G": 0, "F": 1, "H1": 2, "H2": 3}
-MNP_SOLVER_FP64, MNP_SOLVER_MIXED = 0, 1
+MNP_SOLVER_FP64, MNP_SOLVER_MIXED, MNP_SOLVER_MIXED_BF16 = 0, 1, 2
 
 
 class MnpComplex(C.Structure):
@@ -31,8 +31,8 @@ class GreenOptions(C.Structure):
 
 
 class Timers(C.Structure):
-    _fields_ = [("launches", C.c_int64 * 6), ("ms", C.c_double * 6), ("work", C.c_double * 6),
-                ("bytes", C.c_double * 6)]
+    _fields_ = [("launches", C.c_int64 * 7), ("ms", C.c_double * 7), ("work", C.c_double * 7),
+                ("bytes", C.c_double * 7)]
 
 
 P, I64, D, INT = C.c_void_p, C.c_int64, C.c_double, C.c_int

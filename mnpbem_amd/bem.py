@@ -139,14 +139,21 @@ class bemret:
 
     ``precision='fp64'`` is the drop-in of bemret; ``precision='mixed'``
     builds the chain in complex FP32 and refines with the exact operator in
-    FP64 (the structure of bemretiter with precond 'full').
+    FP64 (the structure of bemretiter with precond 'full');
+    ``precision='mixed_bf16'`` forms the chain's large products from hi/lo
+    bf16 splits on bf16 MFMA, with the same refinement.
     """
+
+    _SOLVERS = {"fp64": _lib.MNP_SOLVER_FP64, "mixed": _lib.MNP_SOLVER_MIXED,
+                "mixed_bf16": _lib.MNP_SOLVER_MIXED_BF16}
 
     def __init__(self, p, op=None, precision=None):
         op = p.op if op is None else op
         self.p = p
         self.precision = precision or op.get("precision", "fp64")
-        solver = _lib.MNP_SOLVER_FP64 if self.precision == "fp64" else _lib.MNP_SOLVER_MIXED
+        if self.precision not in self._SOLVERS:
+            raise ValueError(f"bemret: unknown precision {self.precision!r}")
+        solver = self._SOLVERS[self.precision]
         h = C.c_void_p()
         _lib.call("mnp_bemret_create", p.green.handle, solver, C.byref(h))
         self.handle = h

@@ -14,6 +14,9 @@
 // preconditioner mfun of the exact operator afun (BEM/@bemretiter/private/
 // afun.m:6-35, mfun.m:10-52) applied with the FP64 matrices:
 //   x <- x + M^{-1} (b - A x)  until |b - A x| / |b| < tol.
+// MNP_SOLVER_MIXED_BF16 is MNP_SOLVER_MIXED with the chain's large complex64
+// products on the split-bf16 kernel (a less exact preconditioner; the
+// refinement against the FP64 operator sets the final accuracy).
 #include <cmath>
 #include <memory>
 #include <vector>
@@ -429,6 +432,7 @@ struct BemRet {
       c.k0 = k0;
       g->assemble(k1, 2 * M_PI, OutPair{G1, c.G1}, OutPair{H1, c.S1}, n);
       g->assemble(k2, -2 * M_PI, OutPair{G2, c.G2}, OutPair{H2, c.S2}, n);
+      SplitBf16Scope split(solver == MNP_SOLVER_MIXED_BF16);
       chain_build<float>(c, *g);
     }
     ready = true;
@@ -624,8 +628,8 @@ extern "C" {
 int mnp_bemret_create(mnp_greenret_t g, int solver, mnp_bemret_t* out) {
   return guarded([&] {
     MNP_CHECK(g && out, MNP_ERR_ARG, "bemret_create: NULL argument");
-    MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED, MNP_ERR_ARG,
-              "bemret_create: unknown solver");
+    MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED || solver == MNP_SOLVER_MIXED_BF16,
+              MNP_ERR_ARG, "bemret_create: unknown solver");
     auto* h = new mnp_bemret_s{nullptr};
     try {
       h->b = new BemRet(g->g, solver);

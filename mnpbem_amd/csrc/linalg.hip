@@ -10,6 +10,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "mnp_linalg.hpp"
@@ -97,40 +98,48 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
       ci[a][b] = acc_t{0, 0, 0, 0};
       if constexpr (THREE_M) c3[a][b] = acc_t{0, 0, 0, 0};
     }
+  // Loads read clamped (always valid) addresses without branches, so that
+  // they stay in flight during the MFMAs; entries outside the operands are
+  // zeroed when the slice is stored.  A predicated load would be compiled to
+  // a branch that waits for it at once.
   cplx<T> ra[LA], rb[LB];
   auto loadA = [&](int64_t k0) {
 #pragma unroll
     for (int e = 0; e < LA; ++e) {
       const int idx = tid + 256 * e, m = idx % BM, k = idx / BM;
-      const int64_t gm = m0 + m, gk = k0 + k;
-      ra[e] = (gm < M && gk < K) ? A[gk * lda + gm] : czero<T>();
+      const int64_t gm = min(m0 + m, M - 1), gk = min(k0 + k, K - 1);
+      ra[e] = A[gk * lda + gm];
     }
   };
   auto loadB = [&](int64_t k0) {
 #pragma unroll
     for (int e = 0; e < LB; ++e) {
       const int idx = tid + 256 * e, k = idx % GBK, n = idx / GBK;
-      const int64_t gk = k0 + k, gn = n0 + n;
-      rb[e] = (gk < K && gn < N) ? B[gn * ldb + gk] : czero<T>();
+      const int64_t gk = min(k0 + k, K - 1), gn = min(n0 + n, N - 1);
+      rb[e] = B[gn * ldb + gk];
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&](int buf, int64_t k0) {
 #pragma unroll
     for (int e = 0; e < LA; ++e) {
-      const int idx = tid + 256 * e;
-      sAr[buf][idx / BM][idx % BM] = ra[e].x;
-      sAi[buf][idx / BM][idx % BM] = ra[e].y;
+      const int idx = tid + 256 * e, m = idx % BM, k = idx / BM;
+      const cplx<T> v = (m0 + m < M && k0 + k < K) ? ra[e] : czero<T>();
+      sAr[buf][k][m] = v.x;
+      sAi[buf][k][m] = v.y;
     }
 #pragma unroll
     for (int e = 0; e < LB; ++e) {
-      const int idx = tid + 256 * e;
-      sBr[buf][idx % GBK][idx / GBK] = rb[e].x;
-      sBi[buf][idx % GBK][idx / GBK] = rb[e].y;
+      const int idx = tid + 256 * e, k = idx % GBK, n = idx / GBK;
+      const cplx<T> v = (k0 + k < K && n0 + n < N) ? rb[e] : czero<T>();
+      sBr[buf][k][n] = v.x;
+      sBi[buf][k][n] = v.y;
     }
   };
-  loadA(0);
-  loadB(0);
-  store(0);
+  if (K > 0) {
+    loadA(0);
+    loadB(0);
+    store(0, 0);
+  }
   int buf = 0;
   for (int64_t k0 = 0; k0 < K; k0 += GBK) {
     __syncthreads();  // slice `buf` is stored; slice `buf ^ 1` is no longer read
@@ -181,7 +190,7 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
           }
       }
     }
-    if (more) store(buf ^ 1);
+    if (more) store(buf ^ 1, k0 + GBK);
     buf ^= 1;
   }
 #pragma unroll
@@ -206,6 +215,318 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
       }
     }
 }
+
+// ------------------------------------------------ split-bf16 complex GEMM
+// C = alpha A B + beta C for complex64 operands on bf16 MFMA.  Each FP32
+// value x is held as a pair of bf16, hi = rn(x) and lo = rn(x - hi), so
+// x = hi + lo to about 2^-17 |x|, and each real product is formed as
+// Xh Yh + Xh Yl + Xl Yh (the dropped Xl Yl is below 2^-16 |X||Y|) by
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  The complex product takes
+// four real ones (Cr = Ar Br - Ai Bi, Ci = Ar Bi + Ai Br): 12 MFMAs per
+// 16 x 16 x 32 complex block, whose 196,608 bf16 flops stand for 65,536
+// complex-algorithmic ones.
+//
+// The operands are split once, before the product, by one HBM pass each
+// (k_split_a, k_split_b) into four planes (re hi, re lo, im hi, im lo).  A
+// plane of an operand with R rows (A: its rows; B: its columns) is an array
+// of 16-byte chunks [KS][Rp][4]: chunk (s, r, g) holds the 8 values
+// k = 32 s + 8 g .. 32 s + 8 g + 7 of row r, zero outside the operand; Rp is
+// R rounded up to the tile and KS = ceil(K / 32).  A 32-deep slice of 16
+// rows is thus 1 KB of contiguous memory.
+//
+// Operand layout of v_mfma_f32_16x16x32_bf16: lane l holds 8 consecutive-k
+// values A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16]; the result
+// of lane l is D[4 (l / 16) + r][l % 16], r = 0..3.  A and B use the same
+// k slots, so the sum does not depend on the hardware's order of k inside a
+// lane.  In LDS a row (or column) holds the four chunks of a slice; chunk g
+// of row m sits at slot g ^ ((m >> 2) & 3), so sixteen lanes reading one
+// k-group of sixteen consecutive rows touch all 64 banks once.
+//
+// The product streams slices from HBM straight into LDS with
+// global_load_lds_dwordx4 (lane l of a wave writes bytes 16 l .. 16 l + 15 of
+// the wave's LDS destination), NS stages deep: while the MFMAs consume one
+// stage, the loads of the next NS - 1 slices are in flight.  A wave load covers
+// 16 rows of one plane, its lane l fetching chunk g = (l % 4) ^ swizzle of
+// row l / 4, so that the swizzle above is applied by the fetch itself.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+// 16-byte chunk as a native vector (HIP's uint4 is a union-based struct,
+// which keeps per-thread arrays of it out of registers)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) {
+  const bf16x2_t h = {(__bf16)a, (__bf16)b};
+  hi = __builtin_bit_cast(uint32_t, h);
+  const float ah = __builtin_bit_cast(float, hi << 16), bh = __builtin_bit_cast(float, hi & 0xffff0000u);
+  const bf16x2_t l = {(__bf16)(a - ah), (__bf16)(b - bh)};
+  lo = __builtin_bit_cast(uint32_t, l);
+}
+
+// 8 complex values -> chunks re hi, re lo, im hi, im lo
+__device__ __forceinline__ void split8(const float2* v, u32x4* out) {
+  uint32_t h[4], l[4], ih[4], il[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    split_pair(v[2 * q].x, v[2 * q + 1].x, h[q], l[q]);
+    split_pair(v[2 * q].y, v[2 * q + 1].y, ih[q], il[q]);
+  }
+  out[0] = u32x4{h[0], h[1], h[2], h[3]};
+  out[1] = u32x4{l[0], l[1], l[2], l[3]};
+  out[2] = u32x4{ih[0], ih[1], ih[2], ih[3]};
+  out[3] = u32x4{il[0], il[1], il[2], il[3]};
+}
+
+__device__ __forceinline__ f32x4_t mfma_bf16(u32x4 a, u32x4 b, f32x4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                 __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+}
+
+// Planes of A (R x K, column-major): thread = (row r, slice blockIdx.y), 32
+// loads coalesced over rows.  Reads are clamped to the operand (always valid,
+// no branches) and zeroed outside it.
+__global__ void __launch_bounds__(256) k_split_a(int64_t R, int64_t K, const float2* A, int64_t lda,
+                                                 int64_t Rp, int64_t KS, u32x4* P) {
+  const int64_t r = blockIdx.x * (int64_t)256 + threadIdx.x, sl = blockIdx.y;
+  if (r >= Rp) return;
+  const float2* src = A + min(r, R - 1);
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    float2 v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = src[min(32 * sl + 8 * g + j, K - 1) * lda];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (r >= R || 32 * sl + 8 * g + j >= K) v[j] = make_float2(0.f, 0.f);
+    u32x4 c[4];
+    split8(v, c);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) P[((p * KS + sl) * Rp + r) * 4 + g] = c[p];
+  }
+}
+
+// Planes of B (K x R, column-major; its columns are the plane's rows):
+// thread = (column r, slice blockIdx.y), reading 32 consecutive k.
+__global__ void __launch_bounds__(256) k_split_b(int64_t K, int64_t R, const float2* B, int64_t ldb,
+                                                 int64_t Rp, int64_t KS, u32x4* P) {
+  const int64_t r = blockIdx.x * (int64_t)256 + threadIdx.x, sl = blockIdx.y;
+  if (r >= Rp) return;
+  const float2* src = B + min(r, R - 1) * ldb;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    float2 v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = src[min(32 * sl + 8 * g + j, K - 1)];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (r >= R || 32 * sl + 8 * g + j >= K) v[j] = make_float2(0.f, 0.f);
+    u32x4 c[4];
+    split8(v, c);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) P[((p * KS + sl) * Rp + r) * 4 + g] = c[p];
+  }
+}
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+template <int S> using stage_t = std::integral_constant<int, S>;
+
+// Each stage is its own __shared__ array and the main loop is unrolled over
+// the stages, so every LDS access names its stage at compile time: the
+// compiler then sees that the fragment reads of one stage do not alias the
+// loads landing in another and does not make them wait for those loads.
+template <int WM, int WN, int NS>
+__global__ void __launch_bounds__(256) k_cgemm_sbf16(int64_t M, int64_t N, int64_t KS, float2 alpha,
+                                                     const u32x4* PA, int64_t Mp, const u32x4* PB,
+                                                     int64_t Np, float2 beta, int beta_zero, float2* C,
+                                                     int64_t ldc) {
+  static_assert(NS == 2 || NS == 3, "two or three stages");
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  constexpr int LA = 4 * BM / 16, LB = 4 * BN / 16;  // wave loads per slice (4 planes x 16-row groups)
+  static_assert(LA % 4 == 0 && LB % 4 == 0, "wave loads must split evenly over the 4 waves");
+  // loads per wave and stage; waiting for the oldest stage leaves the loads
+  // of the NS - 2 younger ones outstanding (s_waitcnt: vmcnt in bits 3:0 and
+  // 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their maxima)
+  constexpr int PER = LA / 4 + LB / 4, OUT = (NS - 2) * PER;
+  static_assert(OUT < 64, "vmcnt holds at most 63 outstanding loads");
+  constexpr int WAIT_OLDEST = (OUT & 15) | ((OUT >> 4) << 14) | (7 << 4) | (15 << 8);
+  __shared__ u32x4 sA0[4][BM][4], sA1[4][BM][4], sA2[NS > 2 ? 4 : 1][BM][4];
+  __shared__ u32x4 sB0[4][BN][4], sB1[4][BN][4], sB2[NS > 2 ? 4 : 1][BN][4];
+  auto stA = [&](auto st) -> u32x4(&)[4][BM][4] {
+    if constexpr (decltype(st)::value == 0) return sA0;
+    else if constexpr (decltype(st)::value == 1) return sA1;
+    else return reinterpret_cast<u32x4(&)[4][BM][4]>(sA2);  // full size when NS == 3, the only use
+  };
+  auto stB = [&](auto st) -> u32x4(&)[4][BN][4] {
+    if constexpr (decltype(st)::value == 0) return sB0;
+    else if constexpr (decltype(st)::value == 1) return sB1;
+    else return reinterpret_cast<u32x4(&)[4][BN][4]>(sB2);
+  };
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid & 1, wn = wid >> 1;
+  int64_t tx, ty;
+  {  // XCD-aware tile order, as in k_cgemm
+    const int64_t gx = gridDim.x, gy = gridDim.y, tot = gx * gy;
+    const int64_t L = blockIdx.x + gx * (int64_t)blockIdx.y;
+    const int64_t per = tot / 8, rem = tot % 8, xcd = L % 8, idx = L / 8;
+    const int64_t nl = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    constexpr int64_t G = 8;
+    const int64_t group = nl / (G * gy), first = group * G;
+    const int64_t gsz = min(G, gx - first), r = nl % (G * gy);
+    tx = first + r % gsz;
+    ty = r / gsz;
+  }
+  const int64_t m0 = tx * BM, n0 = ty * BN;
+
+  f32x4_t cr[WM][WN], ci[WM][WN];
+#pragma unroll
+  for (int a = 0; a < WM; ++a)
+#pragma unroll
+    for (int b = 0; b < WN; ++b) {
+      cr[a][b] = f32x4_t{0, 0, 0, 0};
+      ci[a][b] = f32x4_t{0, 0, 0, 0};
+    }
+  // lane part of a wave load: row lane / 4 of the 16-row group, chunk
+  // (lane % 4) ^ swizzle; the swizzle depends on the row only through
+  // (row >> 2) & 3 = (lane >> 4) & 3 since groups start at multiples of 16
+  const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
+  auto issue = [&](auto st, int64_t sl) {
+    auto& dA = stA(st);
+    auto& dB = stB(st);
+#pragma unroll
+    for (int i = 0; i < LA / 4; ++i) {
+      const int w = wid + 4 * i, p = w / (BM / 16), rg = w % (BM / 16);
+      const u32x4* g = PA + ((p * KS + sl) * Mp + m0 + 16 * rg + lrow) * 4 + lchunk;
+      __builtin_amdgcn_global_load_lds((const void*)g, (lds_ptr_t)&dA[p][16 * rg][0], 16, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < LB / 4; ++i) {
+      const int w = wid + 4 * i, p = w / (BN / 16), rg = w % (BN / 16);
+      const u32x4* g = PB + ((p * KS + sl) * Np + n0 + 16 * rg + lrow) * 4 + lchunk;
+      __builtin_amdgcn_global_load_lds((const void*)g, (lds_ptr_t)&dB[p][16 * rg][0], 16, 0, 0);
+    }
+  };
+  // fragment slot of this lane: k-group lane / 16 of rows (columns) base + lane % 16
+  const int r16 = lane & 15, slot = (lane >> 4) ^ ((r16 >> 2) & 3);
+  const uint32_t sgn = 0x80008000u;
+  auto compute = [&](auto st) {
+    auto& xA = stA(st);
+    auto& xB = stB(st);
+    u32x4 brh[WN], brl[WN], bih[WN], bil[WN];
+#pragma unroll
+    for (int b = 0; b < WN; ++b) {
+      const int col = wn * 16 * WN + b * 16 + r16;
+      brh[b] = xB[0][col][slot];
+      brl[b] = xB[1][col][slot];
+      bih[b] = xB[2][col][slot];
+      bil[b] = xB[3][col][slot];
+    }
+#pragma unroll
+    for (int a = 0; a < WM; ++a) {
+      const int row = wm * 16 * WM + a * 16 + r16;
+      const u32x4 arh = xA[0][row][slot], arl = xA[1][row][slot];
+      const u32x4 aih = xA[2][row][slot], ail = xA[3][row][slot];
+      const u32x4 nih = aih ^ sgn, nil = ail ^ sgn;
+#pragma unroll
+      for (int b = 0; b < WN; ++b) {
+        cr[a][b] = mfma_bf16(arh, brh[b], cr[a][b]);
+        ci[a][b] = mfma_bf16(arh, bih[b], ci[a][b]);
+        cr[a][b] = mfma_bf16(arh, brl[b], cr[a][b]);
+        ci[a][b] = mfma_bf16(arh, bil[b], ci[a][b]);
+        cr[a][b] = mfma_bf16(arl, brh[b], cr[a][b]);
+        ci[a][b] = mfma_bf16(arl, bih[b], ci[a][b]);
+        cr[a][b] = mfma_bf16(nih, bih[b], cr[a][b]);
+        ci[a][b] = mfma_bf16(aih, brh[b], ci[a][b]);
+        cr[a][b] = mfma_bf16(nih, bil[b], cr[a][b]);
+        ci[a][b] = mfma_bf16(aih, brl[b], ci[a][b]);
+        cr[a][b] = mfma_bf16(nil, bih[b], cr[a][b]);
+        ci[a][b] = mfma_bf16(ail, brh[b], ci[a][b]);
+      }
+    }
+  };
+  // slice sl lives in stage sl % NS
+  auto step = [&](auto st, int64_t sl) {
+    constexpr int S = decltype(st)::value;
+    // this wave's loads of slice sl have landed (near the end, where fewer
+    // younger loads are outstanding, wait for all); after the barrier all
+    // waves' have, and no wave still reads the stage of slice sl - 1
+    if (sl + NS - 2 < KS)
+      __builtin_amdgcn_s_waitcnt(WAIT_OLDEST);
+    else
+      __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (sl + NS - 1 < KS) issue(stage_t<(S + NS - 1) % NS>{}, sl + NS - 1);
+    compute(st);
+  };
+  // prologue: slices 0 .. NS - 2 in flight
+  if (KS > 0) issue(stage_t<0>{}, 0);
+  if constexpr (NS == 3)
+    if (KS > 1) issue(stage_t<1>{}, 1);
+  for (int64_t sl = 0; sl < KS; sl += NS) {
+    step(stage_t<0>{}, sl);
+    if (sl + 1 < KS) step(stage_t<1>{}, sl + 1);
+    if constexpr (NS == 3)
+      if (sl + 2 < KS) step(stage_t<2>{}, sl + 2);
+  }
+#pragma unroll
+  for (int a = 0; a < WM; ++a)
+#pragma unroll
+    for (int b = 0; b < WN; ++b) {
+      const int64_t col = n0 + wn * 16 * WN + b * 16 + r16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm * 16 * WM + a * 16 + 4 * (lane >> 4) + r;
+        if (row < M && col < N) {
+          const float2 v = make_float2(cr[a][b][r], ci[a][b][r]);
+          float2 o = cmul<float>(alpha, v);
+          float2* pc = C + col * ldc + row;
+          if (!beta_zero) o = cadd<float>(o, cmul<float>(beta, *pc));
+          *pc = o;
+        }
+      }
+    }
+}
+
+// Split planes of the operands of the current product (reused by the next;
+// launches on one stream are ordered).
+static Workspace& split_ws() {
+  static Workspace ws;
+  return ws;
+}
+
+template <int WM, int WN, int NS>
+static void gemm_sbf16(int64_t m, int64_t n, int64_t k, float2 al, MatView<float> A, MatView<float> B,
+                       float2 be, bool bz, MatView<float> C) {
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  const int64_t Mp = cdiv_up(m, BM) * BM, Np = cdiv_up(n, BN) * BN, KS = cdiv_up(k, 32);
+  u32x4 *PA = nullptr, *PB = nullptr;
+  if (KS > 0) {
+    MNP_CHECK(KS <= 65535, MNP_ERR_ARG, "gemm: k too large for the split-bf16 product");
+    PA = (u32x4*)split_ws().get(0, sizeof(u32x4) * 16 * (size_t)(KS * Mp));
+    PB = (u32x4*)split_ws().get(1, sizeof(u32x4) * 16 * (size_t)(KS * Np));
+    hipLaunchKernelGGL(k_split_a, dim3((unsigned)cdiv_up(Mp, 256), (unsigned)KS), dim3(256), 0, stream(), m, k,
+                       A.p, A.ld, Mp, KS, PA);
+    hipLaunchKernelGGL(k_split_b, dim3((unsigned)cdiv_up(Np, 256), (unsigned)KS), dim3(256), 0, stream(), k, n,
+                       B.p, B.ld, Np, KS, PB);
+    MNP_HIP(hipGetLastError());
+  }
+  const int64_t gx = Mp / BM, ny = Np / BN;
+  for (int64_t y0 = 0; y0 < ny; y0 += 65535) {
+    const int64_t gy = std::min<int64_t>(65535, ny - y0);
+    const int64_t noff = y0 * BN;
+    const int64_t ncols = std::min<int64_t>(n - noff, gy * BN);
+    TimerScope ts(MNP_TIMER_GEMM_SBF16, 8.0 * (double)m * (double)ncols * (double)k,
+                  (double)sizeof(float2) * ((double)m * k + (double)k * ncols + (double)m * ncols * (bz ? 1.0 : 2.0)));
+    hipLaunchKernelGGL((k_cgemm_sbf16<WM, WN, NS>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream(), m,
+                       n - noff, KS, al, PA, Mp, PB ? PB + noff * 4 : PB, Np, be, (int)bz, C.p + noff * C.ld, C.ld);
+  }
+  MNP_HIP(hipGetLastError());
+}
+
+// Products of complex64 matrices on the split-bf16 kernel when set (the
+// MNP_SOLVER_MIXED_BF16 chain sets it around its factorisations).
+static int g_split_bf16 = 0;
+void set_split_bf16(int on) { g_split_bf16 = on; }
+int split_bf16() { return g_split_bf16; }
 
 template <class T, int WM, int WN, bool THREE_M = false>
 static void gemm_launch(int64_t m, int64_t n, int64_t k, cplx<T> al, MatView<T> A, MatView<T> B,
@@ -248,6 +569,8 @@ void gemm(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, MatView<
   if (m >= 128 && n >= 128 && k >= 64 && big_tiles >= 256) {
     if constexpr (sizeof(T) == 8)
       gemm_launch<T, 4, 2>(m, n, k, al, A, B, be, bz, C);  // 128 x 64: accumulators fit without spills
+    else if (g_split_bf16)
+      gemm_sbf16<4, 4, 2>(m, n, k, al, A, B, be, bz, C);
     else
       gemm_launch<T, 4, 3, true>(m, n, k, al, A, B, be, bz, C);  // 3M on the FP32 chain
   } else
@@ -1222,6 +1545,20 @@ extern "C" {
 
 int mnp_gemm(int precision, int64_t m, int64_t n, int64_t k, mnp_complex alpha, const void* A,
              int64_t lda, const void* B, int64_t ldb, mnp_complex beta, void* C, int64_t ldc) {
+  if (precision == MNP_FP32_SPLIT_BF16)
+    return guarded([&] {
+      MNP_CHECK(m >= 0 && n >= 0 && k >= 0 && lda >= std::max<int64_t>(1, m) &&
+                    ldb >= std::max<int64_t>(1, k) && ldc >= std::max<int64_t>(1, m),
+                MNP_ERR_ARG, "gemm: bad sizes");
+      MNP_CHECK(m == 0 || n == 0 || (A && B && C), MNP_ERR_ARG, "gemm: NULL operand");
+      if (m == 0 || n == 0) return;
+      const bool bz = beta.re == 0 && beta.im == 0;
+      MNP_CHECK(k > 0 || bz, MNP_ERR_ARG, "gemm: k == 0 with beta != 0 unsupported");
+      gemm_sbf16<4, 4, 2>(m, n, k, make_float2((float)alpha.re, (float)alpha.im),
+                                 MatView<float>{(float2*)A, lda}, MatView<float>{(float2*)B, ldb},
+                                 make_float2((float)beta.re, (float)beta.im), bz,
+                                 MatView<float>{(float2*)C, ldc});
+    });
   return by_precision(precision, [&](auto t) {
     using T = decltype(t);
     MNP_CHECK(m >= 0 && n >= 0 && k >= 0 && lda >= std::max<int64_t>(1, m) &&

@@ -39,6 +39,17 @@ template <class T>
 void gemm(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, MatView<T> B,
           double2 beta, MatView<T> C);
 
+// While set, large complex64 products (the 128 x 96 tile class) run on the
+// split-bf16 kernel: each FP32 operand as hi + lo bf16, three bf16 products
+// per real product, FP32 accumulation (about 2^-16 relative per product).
+void set_split_bf16(int on);
+int split_bf16();
+struct SplitBf16Scope {
+  int prev;
+  explicit SplitBf16Scope(int on) : prev(split_bf16()) { set_split_bf16(on); }
+  ~SplitBf16Scope() { set_split_bf16(prev); }
+};
+
 enum { PIVOT_PARTIAL = 0, PIVOT_NONE = 1 };
 
 // In-place LU of an m x n matrix (m >= n) in LAPACK convention: the swaps

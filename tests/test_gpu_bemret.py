@@ -40,7 +40,7 @@ CASES = {
 
 
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_bf16"])
 def test_solver_matches_oracle(case, precision, pinfty):
     from mnpbem_amd import bem as B, device as dev
     mesh, eps_out, enei, pol, dirs = CASES[case]
@@ -89,6 +89,34 @@ def test_config1_sphere_against_mie():
         if lam < 700:
             assert abs(e64[0][0] / me[0] - 1) < 0.03
             assert abs(e64[1][0] / ms[0] - 1) < 0.06
+
+
+def test_mixed_bf16_matches_fp64_5120_faces():
+    """Prolate ellipsoid, 5,120 faces: large enough for the chain's products to
+    run on the split-bf16 kernel (checked by its timer).  The refinement brings
+    the cross sections to the FP64 path's within 1e-8."""
+    import ctypes as C
+    from mnpbem_amd import _lib, bem as B
+    v, f = shapes.ellipsoid(4, (10.0, 10.0, 40.0))
+    pol, dirs = np.array([[1.0, 0, 0], [0, 0, 1.0]]), np.array([[0, 0, 1.0], [1.0, 0, 0]])
+    p, bem64 = _setup(v, f, 1.77, "fp64")
+    _, bembf = _setup(v, f, 1.77, "mixed_bf16")
+    exc = B.planewave(pol, dirs)
+    for lam in (550.0, 800.0):
+        print(f"[mixed_bf16] n={len(f)} enei {lam}", flush=True)
+        _lib.call("mnp_timers_enable", 1)
+        sbf = bembf.solve(exc(p, lam))
+        tm = _lib.Timers()
+        _lib.call("mnp_timers_read", C.byref(tm))
+        _lib.call("mnp_timers_enable", 0)
+        assert tm.launches[6] > 0  # MNP_TIMER_GEMM_SBF16
+        it, rr = bembf.stats()
+        print(f"  sweeps {it}, relres {rr:.2e}, split-bf16 launches {tm.launches[6]}", flush=True)
+        assert rr < 1e-12 and it <= 6
+        cb = exc.cross_sections(sbf)
+        c64 = exc.cross_sections(bem64.solve(exc(p, lam)))
+        for a, b in zip(cb[:2], c64[:2]):
+            assert np.allclose(a, b, rtol=1e-8)
 
 
 def test_solve_before_initmat_and_bad_polarisation():

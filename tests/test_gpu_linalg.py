@@ -52,6 +52,63 @@ def test_gemm(dtype, mnk):
     assert np.max(np.abs(got - A @ B)) <= tol * np.sqrt(k) * (np.abs(A @ B).max() + 1)
 
 
+def _bf16_rn(x):
+    """float32 -> nearest bf16 (ties to even), returned as float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def _split_bf16_product(A, B):
+    """The split-bf16 complex product in exact arithmetic: x = hi + lo with
+    hi = rn(x), lo = rn(x - hi); each real product X Y -> Xh Yh + Xh Yl + Xl Yh;
+    Cr = Ar Br - Ai Bi, Ci = Ar Bi + Ai Br."""
+    def split(x):
+        h = _bf16_rn(x)
+        return h.astype(np.float64), _bf16_rn(x - h).astype(np.float64)
+
+    def prod(X, Y):
+        (xh, xl), (yh, yl) = split(X), split(Y)
+        return xh @ yh + xh @ yl + xl @ yh
+
+    Ar, Ai = A.real.astype(np.float32), A.imag.astype(np.float32)
+    Br, Bi = B.real.astype(np.float32), B.imag.astype(np.float32)
+    return (prod(Ar, Br) - prod(Ai, Bi)) + 1j * (prod(Ar, Bi) + prod(Ai, Br))
+
+
+@pytest.mark.parametrize("mnk", [(64, 64, 16), (130, 77, 45), (1, 300, 257), (257, 3, 1000),
+                                 (2100, 2300, 700), (2049, 2177, 129)])
+def test_gemm_split_bf16(mnk):
+    """The split-bf16 kernel computes the documented scheme: it agrees with its
+    exact-arithmetic emulation to FP32 accumulation error (a dropped cross term
+    would be 2^-9 off), and the scheme is within 2^-15 of the exact product."""
+    import torch
+    from mnpbem_amd import _lib, device as dev
+    dev.ensure()
+    print(f"[gemm split-bf16] {mnk}", flush=True)
+    m, n, k = mnk
+    rng = np.random.default_rng(2)
+    A, B, C0 = (_rand(rng, m, k, np.complex64), _rand(rng, k, n, np.complex64), _rand(rng, m, n, np.complex64))
+    alpha, beta = 0.75 - 0.25j, -1.0 + 0.5j  # exact in FP32
+    dA, dB, dC = (_dev(x, torch.complex64) for x in (A, B, C0))
+    _lib.call("mnp_gemm", _lib.MNP_FP32_SPLIT_BF16, m, n, k, _lib.MnpComplex.of(alpha), dev.ptr(dev.base(dA)), m,
+              dev.ptr(dev.base(dB)), k, _lib.MnpComplex.of(beta), dev.ptr(dev.base(dC)), m)
+    got = dev.to_host(dC).astype(np.complex128)
+    emu = _split_bf16_product(A, B)
+    exact = A.astype(np.complex128) @ B.astype(np.complex128)
+    scale = np.sqrt(k) * 2.0  # rms size of one entry of A B (unit-variance complex normals)
+    want = alpha * emu + beta * C0.astype(np.complex128)
+    assert np.max(np.abs(got - want)) <= 2.0 ** -18 * scale * 4
+    assert np.max(np.abs(emu - exact)) <= 2.0 ** -15 * scale * 4
+    # beta = 0 ignores C
+    dC.fill_(float("nan"))
+    _lib.call("mnp_gemm", _lib.MNP_FP32_SPLIT_BF16, m, n, k, _lib.MnpComplex.of(1.0), dev.ptr(dev.base(dA)), m,
+              dev.ptr(dev.base(dB)), k, _lib.MnpComplex.of(0.0), dev.ptr(dev.base(dC)), m)
+    got = dev.to_host(dC).astype(np.complex128)
+    assert np.all(np.isfinite(got))
+    assert np.max(np.abs(got - emu)) <= 2.0 ** -18 * scale * 4
+
+
 def _lu_dev(A, dtype):
     import torch
     from mnpbem_amd import _lib, device as dev
