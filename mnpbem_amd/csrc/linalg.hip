@@ -227,7 +227,7 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
 // complex-algorithmic ones.
 //
 // The operands are split once, before the product, by one HBM pass each
-// (k_split_a, k_split_b) into four planes (re hi, re lo, im hi, im lo).  A
+// (k_split) into four planes (re hi, re lo, im hi, im lo).  A
 // plane of an operand with R rows (A: its rows; B: its columns) is an array
 // of 16-byte chunks [KS][Rp][4]: chunk (s, r, g) holds the 8 values
 // k = 32 s + 8 g .. 32 s + 8 g + 7 of row r, zero outside the operand; Rp is
@@ -282,49 +282,31 @@ __device__ __forceinline__ f32x4_t mfma_bf16(u32x4 a, u32x4 b, f32x4_t c) {
                                                  __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
 }
 
-// Planes of A (R x K, column-major): thread = (row r, slice blockIdx.y), 32
-// loads coalesced over rows.  Reads are clamped to the operand (always valid,
-// no branches) and zeroed outside it.
-__global__ void __launch_bounds__(256) k_split_a(int64_t R, int64_t K, const float2* A, int64_t lda,
-                                                 int64_t Rp, int64_t KS, u32x4* P) {
-  const int64_t r = blockIdx.x * (int64_t)256 + threadIdx.x, sl = blockIdx.y;
+// Planes of A (R x K, column-major) and of B (K x R, column-major; its
+// columns are the plane's rows).  A wave covers 16 rows of one 32-deep slice:
+// lane l takes row l / 4 and k-group l % 4, so that each of its four plane
+// stores writes 1 KB of contiguous memory.  Reads are clamped to the operand
+// (always valid, no branches) and zeroed outside it.
+template <bool B_OPERAND>
+__global__ void __launch_bounds__(256) k_split(int64_t R, int64_t K, const float2* X, int64_t ld, int64_t Rp,
+                                               int64_t KS, u32x4* P) {
+  const int64_t r = blockIdx.x * (int64_t)64 + (threadIdx.x >> 2), sl = blockIdx.y;
+  const int g = threadIdx.x & 3;
   if (r >= Rp) return;
-  const float2* src = A + min(r, R - 1);
+  const int64_t k0 = 32 * sl + 8 * g, rc = min(r, R - 1);
+  float2 v[8];
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    float2 v[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = src[min(32 * sl + 8 * g + j, K - 1) * lda];
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (r >= R || 32 * sl + 8 * g + j >= K) v[j] = make_float2(0.f, 0.f);
-    u32x4 c[4];
-    split8(v, c);
-#pragma unroll
-    for (int p = 0; p < 4; ++p) P[((p * KS + sl) * Rp + r) * 4 + g] = c[p];
+  for (int j = 0; j < 8; ++j) {
+    const int64_t k = min(k0 + j, K - 1);
+    v[j] = B_OPERAND ? X[rc * ld + k] : X[k * ld + rc];
   }
-}
-
-// Planes of B (K x R, column-major; its columns are the plane's rows):
-// thread = (column r, slice blockIdx.y), reading 32 consecutive k.
-__global__ void __launch_bounds__(256) k_split_b(int64_t K, int64_t R, const float2* B, int64_t ldb,
-                                                 int64_t Rp, int64_t KS, u32x4* P) {
-  const int64_t r = blockIdx.x * (int64_t)256 + threadIdx.x, sl = blockIdx.y;
-  if (r >= Rp) return;
-  const float2* src = B + min(r, R - 1) * ldb;
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    float2 v[8];
+  for (int j = 0; j < 8; ++j)
+    if (r >= R || k0 + j >= K) v[j] = make_float2(0.f, 0.f);
+  u32x4 c[4];
+  split8(v, c);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = src[min(32 * sl + 8 * g + j, K - 1)];
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (r >= R || 32 * sl + 8 * g + j >= K) v[j] = make_float2(0.f, 0.f);
-    u32x4 c[4];
-    split8(v, c);
-#pragma unroll
-    for (int p = 0; p < 4; ++p) P[((p * KS + sl) * Rp + r) * 4 + g] = c[p];
-  }
+  for (int p = 0; p < 4; ++p) P[((p * KS + sl) * Rp + r) * 4 + g] = c[p];
 }
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
@@ -503,10 +485,10 @@ static void gemm_sbf16(int64_t m, int64_t n, int64_t k, float2 al, MatView<float
     MNP_CHECK(KS <= 65535, MNP_ERR_ARG, "gemm: k too large for the split-bf16 product");
     PA = (u32x4*)split_ws().get(0, sizeof(u32x4) * 16 * (size_t)(KS * Mp));
     PB = (u32x4*)split_ws().get(1, sizeof(u32x4) * 16 * (size_t)(KS * Np));
-    hipLaunchKernelGGL(k_split_a, dim3((unsigned)cdiv_up(Mp, 256), (unsigned)KS), dim3(256), 0, stream(), m, k,
-                       A.p, A.ld, Mp, KS, PA);
-    hipLaunchKernelGGL(k_split_b, dim3((unsigned)cdiv_up(Np, 256), (unsigned)KS), dim3(256), 0, stream(), k, n,
-                       B.p, B.ld, Np, KS, PB);
+    hipLaunchKernelGGL(k_split<false>, dim3((unsigned)(Mp / 64), (unsigned)KS), dim3(256), 0, stream(), m, k, A.p,
+                       A.ld, Mp, KS, PA);
+    hipLaunchKernelGGL(k_split<true>, dim3((unsigned)(Np / 64), (unsigned)KS), dim3(256), 0, stream(), n, k, B.p,
+                       B.ld, Np, KS, PB);
     MNP_HIP(hipGetLastError());
   }
   const int64_t gx = Mp / BM, ny = Np / BN;
@@ -519,6 +501,113 @@ static void gemm_sbf16(int64_t m, int64_t n, int64_t k, float2 al, MatView<float
     hipLaunchKernelGGL((k_cgemm_sbf16<WM, WN, NS>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream(), m,
                        n - noff, KS, al, PA, Mp, PB ? PB + noff * 4 : PB, Np, be, (int)bz, C.p + noff * C.ld, C.ld);
   }
+  MNP_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------ tall-skinny products
+// C = alpha A B + beta C with few right-hand sides (n <= 16): the refinement
+// residual (the FP64 operator times n x 4 npol vectors), the chain's products
+// with excitation vectors and the updates of the triangular solves for few
+// right-hand sides.  Such a product streams A once and is bound by HBM; MFMA
+// tiles would use a quarter of their width or less, on too few workgroups.
+// One thread per row of A (coalesced column reads), 16-deep slices of B
+// broadcast from LDS, k split over blockIdx.y into partial sums that
+// k_skinny_sum adds in a fixed order, so results do not depend on timing.
+constexpr int SKT = 16;  // k per LDS slice
+constexpr int SNR = 16;  // most right-hand sides
+
+template <class T, int NR>
+__global__ void __launch_bounds__(256) k_skinny(int64_t m, int n, int64_t k, int64_t kc, const cplx<T>* A,
+                                                int64_t lda, const cplx<T>* B, int64_t ldb, cplx<T>* W,
+                                                cplx<T> alpha, cplx<T> beta, int beta_zero, cplx<T>* C,
+                                                int64_t ldc, int direct) {
+  __shared__ cplx<T> sb[SKT][NR];
+  const int64_t row = blockIdx.x * (int64_t)256 + threadIdx.x;
+  const int64_t kb = blockIdx.y * kc, ke = min(k, kb + kc);
+  const cplx<T>* a_row = A + min(row, m - 1);  // clamped: loads stay valid, unused rows are not stored
+  T ar[NR], ai[NR];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) ar[j] = ai[j] = T(0);
+  for (int64_t k0 = kb; k0 < ke; k0 += SKT) {
+    __syncthreads();  // the previous slice is no longer read
+    for (int e = threadIdx.x; e < SKT * NR; e += 256) {
+      const int kk = e % SKT, j = e / SKT;
+      sb[kk][j] = (k0 + kk < ke && j < n) ? B[j * ldb + k0 + kk] : czero<T>();
+    }
+    cplx<T> a[SKT];
+#pragma unroll
+    for (int kk = 0; kk < SKT; ++kk) a[kk] = a_row[min(k0 + kk, ke - 1) * lda];
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < SKT; ++kk) {
+      const cplx<T> x = k0 + kk < ke ? a[kk] : czero<T>();
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        const cplx<T> b = sb[kk][j];
+        ar[j] = fma(x.x, b.x, ar[j]);
+        ar[j] = fma(-x.y, b.y, ar[j]);
+        ai[j] = fma(x.x, b.y, ai[j]);
+        ai[j] = fma(x.y, b.x, ai[j]);
+      }
+    }
+  }
+  if (row >= m) return;
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    if (j >= n) break;
+    const cplx<T> v = mk<T>(ar[j], ai[j]);
+    if (direct) {
+      cplx<T>* pc = C + j * ldc + row;
+      cplx<T> o = cmul<T>(alpha, v);
+      if (!beta_zero) o = cadd<T>(o, cmul<T>(beta, *pc));
+      *pc = o;
+    } else {
+      W[(blockIdx.y * (int64_t)n + j) * m + row] = v;
+    }
+  }
+}
+
+// C = alpha (sum over the ks partial products W[s] (n x m each)) + beta C
+template <class T>
+__global__ void k_skinny_sum(int64_t m, int n, int64_t ks, const cplx<T>* W, cplx<T> alpha, cplx<T> beta,
+                             int beta_zero, cplx<T>* C, int64_t ldc) {
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (row >= m) return;
+  cplx<T> s = czero<T>();
+  for (int64_t q = 0; q < ks; ++q) s = cadd<T>(s, W[(q * n + j) * m + row]);
+  cplx<T>* pc = C + j * ldc + row;
+  cplx<T> o = cmul<T>(alpha, s);
+  if (!beta_zero) o = cadd<T>(o, cmul<T>(beta, *pc));
+  *pc = o;
+}
+
+template <class T>
+static Workspace& skinny_ws() {
+  static Workspace ws;
+  return ws;
+}
+
+template <class T>
+static void skinny(int64_t m, int64_t n, int64_t k, cplx<T> al, MatView<T> A, MatView<T> B, cplx<T> be, bool bz,
+                   MatView<T> C) {
+  // about 2,048 workgroups: row blocks x k chunks of at least 256
+  const int64_t mt = cdiv_up(m, 256);
+  int64_t ks = std::max<int64_t>(1, std::min<int64_t>(cdiv_up(2048, mt), k / 256));
+  const int64_t kc = cdiv_up(cdiv_up(k, ks), SKT) * SKT;
+  ks = cdiv_up(k, kc);
+  MNP_CHECK(ks <= 65535, MNP_ERR_INTERNAL, "skinny: too many k chunks");
+  cplx<T>* W = ks > 1 ? (cplx<T>*)skinny_ws<T>().get(0, sizeof(cplx<T>) * (size_t)(ks * n * m)) : nullptr;
+  const dim3 grid((unsigned)mt, (unsigned)ks);
+  if (n <= 8)
+    hipLaunchKernelGGL((k_skinny<T, 8>), grid, dim3(256), 0, stream(), m, (int)n, k, kc, A.p, A.ld, B.p, B.ld, W,
+                       al, be, (int)bz, C.p, C.ld, (int)(ks == 1));
+  else
+    hipLaunchKernelGGL((k_skinny<T, SNR>), grid, dim3(256), 0, stream(), m, (int)n, k, kc, A.p, A.ld, B.p, B.ld,
+                       W, al, be, (int)bz, C.p, C.ld, (int)(ks == 1));
+  if (ks > 1)
+    hipLaunchKernelGGL(k_skinny_sum<T>, dim3((unsigned)cdiv_up(m, 256), (unsigned)n), dim3(256), 0, stream(), m,
+                       (int)n, ks, W, al, be, (int)bz, C.p, C.ld);
   MNP_HIP(hipGetLastError());
 }
 
@@ -563,6 +652,11 @@ void gemm(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, MatView<
   if (k <= 0) {
     MNP_CHECK(bz, MNP_ERR_INTERNAL, "gemm: k == 0 with beta != 0 unsupported");
     k = 0;
+  }
+  // Few right-hand sides: a stream over A, bound by HBM.
+  if (n <= SNR && m >= 64 && k >= 256) {
+    skinny<T>(m, n, k, al, A, B, be, bz, C);
+    return;
   }
   // Large products use the 128 x 128 tile; thin ones keep enough workgroups.
   const int64_t big_tiles = cdiv_up(m, 128) * cdiv_up(n, 128);
