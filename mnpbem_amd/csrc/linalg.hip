@@ -239,8 +239,8 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
 // of lane l is D[4 (l / 16) + r][l % 16], r = 0..3.  A and B use the same
 // k slots, so the sum does not depend on the hardware's order of k inside a
 // lane.  In LDS a row (or column) holds the four chunks of a slice; chunk g
-// of row m sits at slot g ^ ((m >> 2) & 3), so sixteen lanes reading one
-// k-group of sixteen consecutive rows touch all 64 banks once.
+// of row m sits at slot g ^ lds_swz(m % 16), which keeps the fragment reads
+// free of bank conflicts.
 //
 // The product streams slices from HBM straight into LDS with
 // global_load_lds_dwordx4 (lane l of a wave writes bytes 16 l .. 16 l + 15 of
@@ -310,6 +310,13 @@ __global__ void __launch_bounds__(256) k_split(int64_t R, int64_t K, const float
 }
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// LDS slot of k-group g of row r: g ^ lds_swz(r % 16).  A fragment read
+// (ds_read_b128: lane l takes row l % 16, k-group l / 16) then hits distinct
+// banks both when 8 lanes are served per cycle from 32 banks (rows r, r + 2,
+// r + 4, r + 6 get distinct slots) and when 16 lanes are served from 64 banks
+// (rows r, r + 4, r + 8, r + 12 do).
+__device__ __forceinline__ int lds_swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 template <int S> using stage_t = std::integral_constant<int, S>;
 
 // Each stage is its own __shared__ array and the main loop is unrolled over
@@ -351,7 +358,7 @@ __global__ void __launch_bounds__(256) k_cgemm_sbf16(int64_t M, int64_t N, int64
     const int64_t L = blockIdx.x + gx * (int64_t)blockIdx.y;
     const int64_t per = tot / 8, rem = tot % 8, xcd = L % 8, idx = L / 8;
     const int64_t nl = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-    constexpr int64_t G = 8;
+    constexpr int64_t G = 4;
     const int64_t group = nl / (G * gy), first = group * G;
     const int64_t gsz = min(G, gx - first), r = nl % (G * gy);
     tx = first + r % gsz;
@@ -368,9 +375,8 @@ __global__ void __launch_bounds__(256) k_cgemm_sbf16(int64_t M, int64_t N, int64
       ci[a][b] = f32x4_t{0, 0, 0, 0};
     }
   // lane part of a wave load: row lane / 4 of the 16-row group, chunk
-  // (lane % 4) ^ swizzle; the swizzle depends on the row only through
-  // (row >> 2) & 3 = (lane >> 4) & 3 since groups start at multiples of 16
-  const int lrow = lane >> 2, lchunk = (lane & 3) ^ ((lane >> 4) & 3);
+  // (lane % 4) ^ swz(row) (groups start at multiples of 16)
+  const int lrow = lane >> 2, lchunk = (lane & 3) ^ lds_swz(lrow);
   auto issue = [&](auto st, int64_t sl) {
     auto& dA = stA(st);
     auto& dB = stB(st);
@@ -388,7 +394,7 @@ __global__ void __launch_bounds__(256) k_cgemm_sbf16(int64_t M, int64_t N, int64
     }
   };
   // fragment slot of this lane: k-group lane / 16 of rows (columns) base + lane % 16
-  const int r16 = lane & 15, slot = (lane >> 4) ^ ((r16 >> 2) & 3);
+  const int r16 = lane & 15, slot = (lane >> 4) ^ lds_swz(r16);
   const uint32_t sgn = 0x80008000u;
   auto compute = [&](auto st) {
     auto& xA = stA(st);
@@ -1340,10 +1346,113 @@ void trsm_rlu_h(int64_t m, int64_t n, MatView<T> LU, const cplx<T>* Linv, const 
 }
 
 // ------------------------------------------------ unpivoted panel leaf
-// LU without pivoting of the nb x nb diagonal block (nb <= 32) by one
-// workgroup, which also writes the block's L^{-1} and U^{-1}.
+// LU without pivoting of the nb x nb diagonal block (nb <= 32) of a complex64
+// factor, which also writes the block's L^{-1} and U^{-1}; the block is padded
+// to 32 x 32 with the identity, which the elimination leaves unchanged.  Three
+// waves:
+// - wave 0 factors the block in registers, lane r holding row r; the pivot
+//   row is broadcast with readlane at each (unrolled) step, so the
+//   elimination needs no barriers.  Multipliers are row entries times the
+//   reciprocal of the pivot, as LAPACK's getf2 scales them.
+// - wave 1 then forms L^{-1} and wave 2 U^{-1}, concurrently, each lane one
+//   row, by substitution with unit triangles: U = D U~ with U~ unit upper,
+//   U^{-1} = U~^{-1} D^{-1}.
+template <class T> __device__ __forceinline__ T bcast(T v, int l);
+template <> __device__ __forceinline__ float bcast<float>(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+template <> __device__ __forceinline__ double bcast<double>(double v, int l) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <class T> __device__ __forceinline__ cplx<T> bcastc(cplx<T> v, int l) {
+  return mk<T>(bcast<T>(v.x, l), bcast<T>(v.y, l));
+}
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i = B .. E - 1,
+// so that register arrays are indexed by constants.
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
 template <class T>
-__global__ void __launch_bounds__(256) k_lu_nopiv_diag(int nb, cplx<T>* A, int64_t lda, int* info,
+__global__ void __launch_bounds__(192) k_lu_nopiv_diag(int nb, cplx<T>* A, int64_t lda, int* info,
+                                                       int info_off, cplx<T>* Linv, cplx<T>* Uinv) {
+  __shared__ cplx<T> sP[TB][TB + 1];  // the block, row r at sP[r]; then its factors
+  __shared__ cplx<T> sX[TB][TB + 1];  // L^{-1}
+  __shared__ cplx<T> sY[TB][TB + 1];  // U^{-1}
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int r = lane & (TB - 1);  // lanes 32..63 repeat rows 0..31 and store nothing
+  const cplx<T> one = mk<T>(1, 0);
+  for (int idx = tid; idx < TB * TB; idx += 192) {
+    const int rr = idx % TB, c = idx / TB;
+    sP[rr][c] = (rr < nb && c < nb) ? A[c * lda + rr] : mk<T>(rr == c ? T(1) : T(0), T(0));
+  }
+  __syncthreads();
+  if (wid == 0) {
+    cplx<T> p[TB];
+    static_for<0, TB>([&](auto C) { p[C] = sP[r][C]; });
+    int first_zero = -1;
+    static_for<0, TB>([&](auto J) {
+      constexpr int j = J;
+      const cplx<T> piv = bcastc<T>(p[j], j);
+      // a zero pivot (uniform) is reported; its column is left as it is
+      const bool zp = piv.x == 0 && piv.y == 0;
+      if (zp && first_zero < 0) first_zero = j;
+      const cplx<T> inv = cdiv<T>(one, zp ? one : piv);
+      const T below = (r > j && !zp) ? T(1) : T(0);  // multiplier of this row (0 on and above)
+      const cplx<T> l = cscale<T>(cmul<T>(p[j], inv), below);
+      p[j] = (r > j && !zp) ? l : p[j];
+      static_for<j + 1, TB>([&](auto C) { p[C] = csub<T>(p[C], cmul<T>(l, bcastc<T>(p[C], j))); });
+    });
+    if (lane < TB) static_for<0, TB>([&](auto C) { sP[r][C] = p[C]; });
+    if (lane == 0 && first_zero >= 0 && first_zero < nb && *info == 0) *info = info_off + first_zero + 1;
+  }
+  __syncthreads();
+  if (wid == 1) {
+    // row r of L^{-1}: row k is final at step k and is eliminated from the rows below
+    cplx<T> x[TB];
+    static_for<0, TB>([&](auto C) { x[C] = mk<T>(C == r ? T(1) : T(0), T(0)); });
+    static_for<0, TB - 1>([&](auto K) {
+      constexpr int k = K;
+      const cplx<T> lrk = cscale<T>(sP[r][k], r > k ? T(1) : T(0));
+      static_for<0, k + 1>([&](auto C) { x[C] = csub<T>(x[C], cmul<T>(lrk, bcastc<T>(x[C], k))); });
+    });
+    if (lane < TB) static_for<0, TB>([&](auto C) { sX[r][C] = x[C]; });
+  } else if (wid == 2) {
+    // row r of U~^{-1} (U~ = D^{-1} U): row k is final at step k (descending)
+    const cplx<T> dinv = cdiv<T>(one, sP[r][r]);
+    cplx<T> y[TB];
+    static_for<0, TB>([&](auto C) { y[C] = mk<T>(C == r ? T(1) : T(0), T(0)); });
+    static_for<1, TB>([&](auto KK) {
+      constexpr int k = TB - KK;  // TB - 1 .. 1
+      const cplx<T> urk = cscale<T>(cmul<T>(sP[r][k], dinv), r < k ? T(1) : T(0));
+      static_for<k, TB>([&](auto C) { y[C] = csub<T>(y[C], cmul<T>(urk, bcastc<T>(y[C], k))); });
+    });
+    static_for<0, TB>([&](auto C) { y[C] = cmul<T>(y[C], bcastc<T>(dinv, C)); });  // columns times 1/u_cc
+    if (lane < TB) static_for<0, TB>([&](auto C) { sY[r][C] = y[C]; });
+  }
+  __syncthreads();
+  for (int idx = tid; idx < TB * TB; idx += 192) {
+    const int rr = idx % TB, c = idx / TB;
+    if (rr < nb && c < nb) A[c * lda + rr] = sP[rr][c];
+    Linv[idx] = sX[rr][c];
+    Uinv[idx] = sY[rr][c];
+  }
+}
+
+// The same for complex128, in LDS by one workgroup (the register form
+// above would need more than 256 VGPRs per lane for FP64 rows): 32 steps of
+// elimination over the 1,024 entries, then L^{-1} and U^{-1} by elimination
+// on the identity.
+template <class T>
+__global__ void __launch_bounds__(256) k_lu_nopiv_diag_lds(int nb, cplx<T>* A, int64_t lda, int* info,
                                                        int info_off, cplx<T>* Linv, cplx<T>* Uinv) {
   __shared__ cplx<T> P[TB][TB + 1];
   __shared__ cplx<T> sL[TB][TB + 1];
@@ -1445,8 +1554,12 @@ static void getrf_rec(int64_t m, int64_t n, MatView<T> A, int32_t* ipiv, int inf
     } else {
       {
         TimerScope ts(MNP_TIMER_PANEL, 0.0);
-        hipLaunchKernelGGL(k_lu_nopiv_diag<T>, dim3(1), dim3(256), 0, stream(), (int)n, A.p, A.ld,
-                           st.info, info_off, Linv, Uinv);
+        if constexpr (sizeof(T) == 4)
+          hipLaunchKernelGGL(k_lu_nopiv_diag<T>, dim3(1), dim3(192), 0, stream(), (int)n, A.p, A.ld, st.info,
+                             info_off, Linv, Uinv);
+        else
+          hipLaunchKernelGGL(k_lu_nopiv_diag_lds<T>, dim3(1), dim3(256), 0, stream(), (int)n, A.p, A.ld,
+                             st.info, info_off, Linv, Uinv);
       }
       if (m > n) {
         MatView<T> A21{A.at(n, 0), A.ld};
