@@ -581,6 +581,7 @@ __global__ void k_skinny_sum(int64_t m, int n, int64_t ks, const cplx<T>* W, cpl
   const int j = blockIdx.y;
   if (row >= m) return;
   cplx<T> s = czero<T>();
+#pragma unroll 8
   for (int64_t q = 0; q < ks; ++q) s = cadd<T>(s, W[(q * n + j) * m + row]);
   cplx<T>* pc = C + j * ldc + row;
   cplx<T> o = cmul<T>(alpha, s);
@@ -597,9 +598,10 @@ static Workspace& skinny_ws() {
 template <class T>
 static void skinny(int64_t m, int64_t n, int64_t k, cplx<T> al, MatView<T> A, MatView<T> B, cplx<T> be, bool bz,
                    MatView<T> C) {
-  // about 2,048 workgroups: row blocks x k chunks of at least 256
+  // about 4,096 workgroups: row blocks x k chunks of at least 64 (each
+  // workgroup's chain of dependent slices stays short when m is small)
   const int64_t mt = cdiv_up(m, 256);
-  int64_t ks = std::max<int64_t>(1, std::min<int64_t>(cdiv_up(2048, mt), k / 256));
+  int64_t ks = std::max<int64_t>(1, std::min<int64_t>(cdiv_up(4096, mt), k / 64));
   const int64_t kc = cdiv_up(cdiv_up(k, ks), SKT) * SKT;
   ks = cdiv_up(k, kc);
   MNP_CHECK(ks <= 65535, MNP_ERR_INTERNAL, "skinny: too many k chunks");
