@@ -307,6 +307,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-config4", action="store_true", help="skip the H-matrix dipole measurement")
     ap.add_argument("--no-split-bf16", action="store_true", help="skip the split-bf16 chain measurement")
+    ap.add_argument("--no-quasistatic", action="store_true", help="skip the quasistatic measurement")
     ap.add_argument("--config4-only", action="store_true",
                     help="only the H-matrix dipole measurement (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -402,7 +403,8 @@ def main():
             },
             "gemm_share_of_step": tb.ms[si] * 1e-3 / r["t_bf"],
         }
-    out["quasistatic"] = run_quasistatic(a.level)
+    if not a.no_quasistatic:
+        out["quasistatic"] = run_quasistatic(a.level)
     if not a.no_config4:
         out["hmatrix_dipole"] = run_config4()
         if not a.no_cpu_baseline:
