@@ -278,52 +278,79 @@ __global__ void k_from_cluster(int64_t n, int nrhs, const int32_t* perm, const d
   if (i < n) y[q * n + perm[i]] = yc[i * nrhs + q];
 }
 
-// acc += a * b with a, b in TS and acc in FP64
-template <class TS, class TB>
-__device__ __forceinline__ void cfma(double2& acc, const cplx<TS> a, const TB b) {
-  const double ar = a.x, ai = a.y, br = b.x, bi = b.y;
-  acc.x += ar * br - ai * bi;
-  acc.y += ar * bi + ai * br;
-}
-
-// acc[q] += a * row[q], q < nrhs (row: the interleaved right-hand sides of one index)
-template <class TS, class TB>
-__device__ __forceinline__ void row_fma(double2 (&acc)[MNP_HMV_MAXRHS], const cplx<TS> a, const TB* row, int nrhs) {
-#pragma unroll
-  for (int q = 0; q < MNP_HMV_MAXRHS; ++q)
-    if (q < nrhs) cfma<TS>(acc[q], a, row[q]);
-}
-
 constexpr int HMV_UNROLL = 8;  // independent loads in flight per lane
 constexpr int HMV_CHUNK = 256;  // columns of R^T per wave in the first phase
 constexpr int HMV_KC = 64;      // right-hand-side rows staged in LDS per wave
 
+// The product kernels are instantiated for NR = 4 or 8 right-hand sides (the
+// pass's nrhs <= NR): LDS and accumulators are sized by NR, and the columns
+// q >= nrhs are zero in LDS, so the inner loops carry no per-column branches.
+// Matrix and vector loads read clamped, always valid addresses and are
+// zeroed where they fall outside; a predicated load would be compiled to a
+// branch that waits for it at once.
+
 // Stages rows [c0, c0 + kc) of the interleaved right-hand sides B (row j at
-// B[j * nrhs], FP64 or FP32) into this wave's LDS buffer as FP64, with
-// coalesced loads; LDS operations of one wave complete in order, so the
-// wave's later reads see these writes.
-template <class TB>
+// B[j * nrhs], FP64 or FP32) into this wave's LDS buffer as FP64 rows of NR
+// (zero beyond nrhs), all loads in flight before the first store; LDS
+// operations of one wave complete in order, so the wave's later reads see
+// these writes.
+template <int NR, class TB>
 __device__ __forceinline__ void stage_rows(double2* bs, const TB* B, int c0, int kc, int nrhs, int lane) {
   const TB* src = B + (int64_t)c0 * nrhs;
-  for (int e = lane; e < kc * nrhs; e += 64) {
-    const TB v = src[e];
-    bs[e] = mk<double>(v.x, v.y);
+  constexpr int PER = HMV_KC * NR / 64;
+  double2 v[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = lane + 64 * i, row = e / NR, q = e % NR;
+    const TB x = src[min(row, kc - 1) * nrhs + min(q, nrhs - 1)];
+    v[i] = (row < kc && q < nrhs) ? mk<double>(x.x, x.y) : mk<double>(0, 0);
   }
+#pragma unroll
+  for (int i = 0; i < PER; ++i) bs[lane + 64 * i] = v[i];
   __builtin_amdgcn_wave_barrier();
 }
 
-// acc[q] += a * bs[q], q < nrhs, bs in LDS (broadcast reads)
-template <class TS>
-__device__ __forceinline__ void lds_fma(double2 (&acc)[MNP_HMV_MAXRHS], const cplx<TS> a, const double2* bs,
-                                        int nrhs) {
+// acc[q] += a * bs[q], q < NR, bs in LDS (broadcast reads)
+template <class TS, int NR>
+__device__ __forceinline__ void lds_fma(double2 (&acc)[NR], const cplx<TS> a, const double2* bs) {
   const double ar = a.x, ai = a.y;
 #pragma unroll
-  for (int q = 0; q < MNP_HMV_MAXRHS; ++q)
-    if (q < nrhs) {
-      const double2 b = bs[q];
-      acc[q].x += ar * b.x - ai * b.y;
-      acc[q].y += ar * b.y + ai * b.x;
+  for (int q = 0; q < NR; ++q) {
+    const double2 b = bs[q];
+    acc[q].x += ar * b.x - ai * b.y;
+    acc[q].y += ar * b.y + ai * b.x;
+  }
+}
+
+// acc += A(:, 0:K) * B(0:K, :) for this lane's row: A's columns at stride lda
+// (A already offset to the lane's row), B's rows staged HMV_KC at a time.
+// Lanes take the columns j = j0 + js * t; the first batch of A loads is
+// issued before the staging, so the two round trips overlap.
+template <class TS, class TB, int NR>
+__device__ __forceinline__ void col_product(double2 (&acc)[NR], double2* bs, const cplx<TS>* A, int64_t lda, int K,
+                                            const TB* B, int nrhs, int lane, int j0, int js) {
+  const cplx<TS> z = mk<TS>(TS(0), TS(0));
+  for (int k0 = 0; k0 < K; k0 += HMV_KC) {
+    const int kc = min(HMV_KC, K - k0);
+    const cplx<TS>* Ak = A + (int64_t)k0 * lda;
+    cplx<TS> a[HMV_UNROLL];
+#pragma unroll
+    for (int u = 0; u < HMV_UNROLL; ++u) a[u] = Ak[(int64_t)min(j0 + u * js, kc - 1) * lda];
+    stage_rows<NR>(bs, B, k0, kc, nrhs, lane);
+    const int step = HMV_UNROLL * js, nit = (kc + step - 1) / step;  // the same for every lane
+    for (int i = 0; i < nit; ++i) {
+      const int jb = j0 + i * step;
+      if (i > 0)
+#pragma unroll
+        for (int u = 0; u < HMV_UNROLL; ++u) a[u] = Ak[(int64_t)min(jb + u * js, kc - 1) * lda];
+#pragma unroll
+      for (int u = 0; u < HMV_UNROLL; ++u) {
+        const int j = jb + u * js;
+        lds_fma<TS, NR>(acc, j < kc ? a[u] : z, bs + min(j, kc - 1) * NR);
+      }
     }
+    __builtin_amdgcn_wave_barrier();
+  }
 }
 
 // First phase, t_b = R_b^T x(cols) (r x nrhs, FP64, t[l * nrhs + q]): one
@@ -335,12 +362,12 @@ __device__ __forceinline__ void lds_fma(double2 (&acc)[MNP_HMV_MAXRHS], const cp
 // shuffles.  Items of blocks with one item write t_b, the others write
 // partial sums that k_hmv_low_sum adds in item order.
 // item[4 i ..] = {block, first column, end column, partial-sum slot or -1}.
-template <class TS>
+template <class TS, int NR>
 __global__ void __launch_bounds__(256) k_hmv_low(int64_t nitems, const int32_t* item, const int64_t* tpoff,
                                                  const int32_t* blk, const int32_t* rank, const int64_t* loff,
                                                  const int64_t* toff, const cplx<TS>* pool, int nrhs,
                                                  const cplx<TS>* xc, double2* t, double2* tpart) {
-  __shared__ double2 bsh[4][HMV_KC * MNP_HMV_MAXRHS];
+  __shared__ double2 bsh[4][HMV_KC * NR];
   const int64_t it = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (it >= nitems) return;
   const int lane = threadIdx.x & 63;
@@ -351,38 +378,24 @@ __global__ void __launch_bounds__(256) k_hmv_low(int64_t nitems, const int32_t* 
   const int RP = r <= 16 ? 16 : r <= 32 ? 32 : 64;
   const int JS = 64 / RP;
   const cplx<TS>* RT = pool + loff[b] + (int64_t)m * r;
-  const cplx<TS>* xb = xc + (int64_t)c0 * nrhs;
   double2* dst = slot < 0 ? t + toff[b] : tpart + tpoff[it];
   for (int lb = 0; lb < r; lb += 64) {
     const int l = lb + lane % RP, jj = lane / RP;
-    double2 acc[MNP_HMV_MAXRHS];
+    double2 acc[NR];
 #pragma unroll
-    for (int q = 0; q < MNP_HMV_MAXRHS; ++q) acc[q] = mk<double>(0, 0);
-    for (int k0 = j0; k0 < j1; k0 += HMV_KC) {
-      const int kc = min(HMV_KC, j1 - k0);
-      stage_rows(bs, xb, k0, kc, nrhs, lane);
-      if (l < r) {
-        int j = jj;
-        for (; j + (HMV_UNROLL - 1) * JS < kc; j += HMV_UNROLL * JS) {
-          cplx<TS> a[HMV_UNROLL];
-#pragma unroll
-          for (int u = 0; u < HMV_UNROLL; ++u) a[u] = RT[(int64_t)(k0 + j + u * JS) * r + l];
-#pragma unroll
-          for (int u = 0; u < HMV_UNROLL; ++u) lds_fma<TS>(acc, a[u], bs + (j + u * JS) * nrhs, nrhs);
-        }
-        for (; j < kc; j += JS) lds_fma<TS>(acc, RT[(int64_t)(k0 + j) * r + l], bs + j * nrhs, nrhs);
-      }
-      __builtin_amdgcn_wave_barrier();
-    }
+    for (int q = 0; q < NR; ++q) acc[q] = mk<double>(0, 0);
+    // column j of the item is R^T row (j0 + j): entries at stride r, this lane's factor l
+    col_product<TS, cplx<TS>, NR>(acc, bs, RT + (int64_t)j0 * r + min(l, r - 1), r, j1 - j0,
+                                  xc + (int64_t)(c0 + j0) * nrhs, nrhs, lane, jj, JS);
     for (int sh = RP; sh < 64; sh <<= 1)
 #pragma unroll
-      for (int q = 0; q < MNP_HMV_MAXRHS; ++q) {
+      for (int q = 0; q < NR; ++q) {
         acc[q].x += __shfl_xor(acc[q].x, sh);
         acc[q].y += __shfl_xor(acc[q].y, sh);
       }
     if (jj == 0 && l < r)
 #pragma unroll
-      for (int q = 0; q < MNP_HMV_MAXRHS; ++q)
+      for (int q = 0; q < NR; ++q)
         if (q < nrhs) dst[(int64_t)l * nrhs + q] = acc[q];
   }
 }
@@ -407,66 +420,40 @@ __global__ void k_hmv_low_sum(int64_t nmulti, const int32_t* multi, const int64_
 // tile's rows of V times x(cols)) or low-rank block (the tile's contiguous
 // L segment times t_b): A at aoff (rows at stride lda, K columns); boff >= 0
 // is the first column of x, boff < 0 encodes toff as -boff - 1.  The four
-// waves take descriptors round-robin; each stages the descriptor's
-// right-hand-side rows in LDS (HMV_KC at a time) so that the matrix stream
-// is the only per-column global load; lane = (row, kk), kk splitting the
-// columns into even/odd, so one load instruction reads two 32-row columns,
-// HMV_UNROLL of them in flight.
-template <class TS, class TB>
-__device__ __forceinline__ void tile_product(double2 (&acc)[MNP_HMV_MAXRHS], double2* bs, const cplx<TS>* A,
-                                             int64_t lda, int K, const TB* B, int lane, int kk, bool active,
-                                             int nrhs) {
-  for (int k0 = 0; k0 < K; k0 += HMV_KC) {
-    const int kc = min(HMV_KC, K - k0);
-    stage_rows(bs, B, k0, kc, nrhs, lane);
-    if (active) {
-      const cplx<TS>* Ak = A + (int64_t)k0 * lda;
-      int j = kk;
-      for (; j + 2 * (HMV_UNROLL - 1) < kc; j += 2 * HMV_UNROLL) {
-        cplx<TS> a[HMV_UNROLL];
-#pragma unroll
-        for (int u = 0; u < HMV_UNROLL; ++u) a[u] = Ak[(int64_t)(j + 2 * u) * lda];
-#pragma unroll
-        for (int u = 0; u < HMV_UNROLL; ++u) lds_fma<TS>(acc, a[u], bs + (j + 2 * u) * nrhs, nrhs);
-      }
-      for (; j < kc; j += 2) lds_fma<TS>(acc, Ak[(int64_t)j * lda], bs + j * nrhs, nrhs);
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-}
-
-template <class TS>
+// waves take descriptors round-robin; lane = (row, kk), kk splitting the
+// columns into even/odd, so one load instruction reads two 32-row columns.
+template <class TS, int NR>
 __global__ void __launch_bounds__(256) k_hmv_tile(const int32_t* tilebeg, const int32_t* tdbeg, const int64_t* taoff,
                                                   const int64_t* tboff, const int32_t* tlda, const int32_t* tK,
                                                   const cplx<TS>* fvals, const cplx<TS>* pool, const double2* t,
                                                   int nrhs, const cplx<TS>* xc, double2* yc) {
-  __shared__ double2 red[4][32][MNP_HMV_MAXRHS];
-  __shared__ double2 bsh[4][HMV_KC * MNP_HMV_MAXRHS];
+  __shared__ double2 red[4][32][NR];
+  __shared__ double2 bsh[4][HMV_KC * NR];
   const int tile = blockIdx.x;
   const int tb = tilebeg[tile], mt = tilebeg[tile + 1] - tb;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, row = lane & 31, kk = lane >> 5;
-  const bool active = row < mt;
+  const int rowc = min(row, mt - 1);  // rows past the tile read valid memory and are not stored
   double2* bs = bsh[wv];
-  double2 acc[MNP_HMV_MAXRHS];
+  double2 acc[NR];
 #pragma unroll
-  for (int q = 0; q < MNP_HMV_MAXRHS; ++q) acc[q] = mk<double>(0, 0);
+  for (int q = 0; q < NR; ++q) acc[q] = mk<double>(0, 0);
   const int s1 = tdbeg[tile + 1];
   for (int s = tdbeg[tile] + wv; s < s1; s += 4) {
     const int64_t aoff = taoff[s], boff = tboff[s];
     const int lda = tlda[s], K = tK[s];
     if (boff >= 0)
-      tile_product<TS>(acc, bs, fvals + aoff + row, lda, K, xc + boff * nrhs, lane, kk, active, nrhs);
+      col_product<TS, cplx<TS>, NR>(acc, bs, fvals + aoff + rowc, lda, K, xc + boff * nrhs, nrhs, lane, kk, 2);
     else
-      tile_product<TS>(acc, bs, pool + aoff + row, lda, K, t + (-boff - 1), lane, kk, active, nrhs);
+      col_product<TS, double2, NR>(acc, bs, pool + aoff + rowc, lda, K, t + (-boff - 1), nrhs, lane, kk, 2);
   }
 #pragma unroll
-  for (int q = 0; q < MNP_HMV_MAXRHS; ++q) {
+  for (int q = 0; q < NR; ++q) {
     acc[q].x += __shfl_xor(acc[q].x, 32);
     acc[q].y += __shfl_xor(acc[q].y, 32);
   }
   if (kk == 0)
 #pragma unroll
-    for (int q = 0; q < MNP_HMV_MAXRHS; ++q) red[wv][row][q] = acc[q];
+    for (int q = 0; q < NR; ++q) red[wv][row][q] = acc[q];
   __syncthreads();
   for (int e = threadIdx.x; e < mt * nrhs; e += blockDim.x) {
     const int i = e / nrhs, q = e % nrhs;
@@ -788,14 +775,25 @@ void HMat::matvec_impl(int64_t nrhs, const double2* x, double2* y, Workspace& ws
     double2* tpart = (double2*)ws.get(3, sizeof(double2) * std::max<int64_t>(tpartlen, 1));
     const dim3 gv((unsigned)cdiv_up(n, 256), (unsigned)nq);
     hipLaunchKernelGGL(k_to_cluster<TS>, gv, dim3(256), 0, s, n, nq, perm, x + q0 * n, xc);
-    if (nitems)
-      hipLaunchKernelGGL(k_hmv_low<TS>, dim3((unsigned)cdiv_up(nitems, 4)), dim3(256), 0, s, nitems, d_items,
-                         d_itpoff, lblk, rank, d_loff, d_toff, (const cplx<TS>*)pool, nq, xc, t, tpart);
+    if (nitems) {
+      const dim3 g((unsigned)cdiv_up(nitems, 4));
+      if (nq <= 4)
+        hipLaunchKernelGGL((k_hmv_low<TS, 4>), g, dim3(256), 0, s, nitems, d_items, d_itpoff, lblk, rank, d_loff,
+                           d_toff, (const cplx<TS>*)pool, nq, xc, t, tpart);
+      else
+        hipLaunchKernelGGL((k_hmv_low<TS, MNP_HMV_MAXRHS>), g, dim3(256), 0, s, nitems, d_items, d_itpoff, lblk,
+                           rank, d_loff, d_toff, (const cplx<TS>*)pool, nq, xc, t, tpart);
+    }
     if (nmulti)
       hipLaunchKernelGGL(k_hmv_low_sum, dim3((unsigned)nmulti), dim3(128), 0, s, nmulti, d_multi, d_itpoff, rank,
                          d_toff, nq, tpart, t);
-    hipLaunchKernelGGL(k_hmv_tile<TS>, dim3((unsigned)ntile), dim3(256), 0, s, d_tilebeg, d_tdbeg, d_taoff, d_tboff,
-                       d_tlda, d_tK, (const cplx<TS>*)fvals, (const cplx<TS>*)pool, t, nq, xc, yc);
+    if (nq <= 4)
+      hipLaunchKernelGGL((k_hmv_tile<TS, 4>), dim3((unsigned)ntile), dim3(256), 0, s, d_tilebeg, d_tdbeg, d_taoff,
+                         d_tboff, d_tlda, d_tK, (const cplx<TS>*)fvals, (const cplx<TS>*)pool, t, nq, xc, yc);
+    else
+      hipLaunchKernelGGL((k_hmv_tile<TS, MNP_HMV_MAXRHS>), dim3((unsigned)ntile), dim3(256), 0, s, d_tilebeg,
+                         d_tdbeg, d_taoff, d_tboff, d_tlda, d_tK, (const cplx<TS>*)fvals, (const cplx<TS>*)pool, t,
+                         nq, xc, yc);
     hipLaunchKernelGGL(k_from_cluster, gv, dim3(256), 0, s, n, nq, perm, yc, y + q0 * n);
     MNP_HIP(hipGetLastError());
   }
