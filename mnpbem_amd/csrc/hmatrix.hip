@@ -604,6 +604,11 @@ void HMat::fill(double2 k, int key_) {
     fill_impl<float>(k, key_);
 }
 
+static Workspace& aca_ws() {
+  static Workspace ws;
+  return ws;
+}
+
 template <class TS>
 void HMat::fill_impl(double2 k, int key_) {
   key = key_;
@@ -629,7 +634,7 @@ void HMat::fill_impl(double2 k, int key_) {
   std::vector<int32_t> rank_h(nlow, 0);
   std::vector<int64_t> loff_h(nlow + 1, 0), toff_h(nlow + 1, 0);
   if (nlow) {
-    const int64_t budget = (int64_t)(1) << 27;  // double2 entries (2 GiB)
+    const int64_t budget = (int64_t)(1) << 29;  // double2 entries (8 GiB)
     int64_t* d_soff = dev_alloc<int64_t>(nlow + 1);
     // batch boundaries
     std::vector<int64_t> cuts{0};
@@ -639,8 +644,11 @@ void HMat::fill_impl(double2 k, int key_) {
     int64_t maxbatch = 0;
     for (size_t c = 0; c + 1 < cuts.size(); ++c)
       maxbatch = std::max(maxbatch, soff_h[cuts[c + 1]] - soff_h[cuts[c]]);
-    double2* scratch = dev_alloc<double2>(maxbatch);
-    uint32_t* used = dev_alloc<uint32_t>(maxbatch / 32 + 1);
+    // scratch shared by every H-matrix fill (grown on demand, kept until
+    // the library unloads): a fill per operator and wavelength would
+    // otherwise pay an allocation and a free of several GiB each
+    double2* scratch = (double2*)aca_ws().get(0, sizeof(double2) * (size_t)maxbatch);
+    uint32_t* used = (uint32_t*)aca_ws().get(1, sizeof(uint32_t) * (size_t)(maxbatch / 32 + 1));
     std::vector<int64_t> rel(nlow + 1);
     for (size_t c = 0; c + 1 < cuts.size(); ++c) {
       const int64_t b0 = cuts[c], b1 = cuts[c + 1];
@@ -674,8 +682,6 @@ void HMat::fill_impl(double2 k, int key_) {
       MNP_HIP(hipGetLastError());
       MNP_HIP(hipStreamSynchronize(s));
     }
-    MNP_HIP(hipFree(scratch));
-    MNP_HIP(hipFree(used));
     MNP_HIP(hipFree(d_soff));
   }
   MNP_HIP(hipMemcpy(d_toff, toff_h.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice));
