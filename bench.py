@@ -136,7 +136,7 @@ def run_quasistatic(level=5, steps=3):
     exc = B.planewave(POL, op=op)
     lams = np.linspace(500.0, 1000.0, 64)
     res = {}
-    for prec in ("mixed", "fp64"):
+    for prec in ("mixed", "mixed_bf16", "fp64"):
         bem = B.bemsolver(p, op, precision=prec)
         exc.ext(bem.solve(exc(p, lams[0])))  # warm-up
         torch.cuda.synchronize()
@@ -146,10 +146,14 @@ def run_quasistatic(level=5, steps=3):
         res[prec] = (steps / (time.perf_counter() - t), ext)
         del bem
     err = max(float(np.max(np.abs(a / b - 1))) for a, b in zip(res["mixed"][1], res["fp64"][1]))
-    log(f"[bench] quasistatic: mixed {res['mixed'][0]:.2f}/s, fp64 {res['fp64'][0]:.2f}/s")
+    err_bf = max(float(np.max(np.abs(a / b - 1))) for a, b in zip(res["mixed_bf16"][1], res["fp64"][1]))
+    log(f"[bench] quasistatic: mixed {res['mixed'][0]:.2f}/s, mixed_bf16 {res['mixed_bf16'][0]:.2f}/s, "
+        f"fp64 {res['fp64'][0]:.2f}/s")
     return {"workload": f"quasistatic bemstat, same ellipsoid ({len(f)} faces) and wavelengths as the headline",
-            "wavelengths_per_s_mixed": res["mixed"][0], "wavelengths_per_s_fp64": res["fp64"][0],
-            "speedup": res["mixed"][0] / res["fp64"][0], "max_rel_err_vs_fp64": err}
+            "wavelengths_per_s_mixed": res["mixed"][0], "wavelengths_per_s_mixed_bf16": res["mixed_bf16"][0],
+            "wavelengths_per_s_fp64": res["fp64"][0],
+            "speedup": res["mixed"][0] / res["fp64"][0], "speedup_mixed_bf16": res["mixed_bf16"][0] / res["fp64"][0],
+            "max_rel_err_vs_fp64": err, "max_rel_err_mixed_bf16_vs_fp64": err_bf}
 
 
 def run_config4(nlam=16, seed=2024):

@@ -290,8 +290,10 @@ int mnp_planewave_cross_sections(mnp_greenret_t g, double enei, mnp_complex eps_
  * init.m), equal to the retarded F at k = 0 and assembled once at create;
  * initmat forms and factorises Lambda I + F with Lambda = 2 pi (eps1 + eps2)
  * / (eps1 - eps2) (bemstat/subsref.m:31-40); solve returns
- * sig = -(Lambda I + F)^{-1} phip (mldivide.m:13-15).  solver MNP_SOLVER_FP64
- * or MNP_SOLVER_MIXED (complex FP32 factors, FP64 refinement to 1e-12). */
+ * sig = -(Lambda I + F)^{-1} phip (mldivide.m:13-15).  solver MNP_SOLVER_FP64,
+ * MNP_SOLVER_MIXED (complex FP32 factors, FP64 refinement to 1e-12) or
+ * MNP_SOLVER_MIXED_BF16 (the FP32 factorisation's large products on the
+ * split-bf16 kernel, the same refinement). */
 typedef struct mnp_bemstat_s* mnp_bemstat_t;
 int mnp_bemstat_create(mnp_greenret_t g, int solver, mnp_bemstat_t* out);
 int mnp_bemstat_destroy(mnp_bemstat_t b);

@@ -288,14 +288,17 @@ class bemstat:
     """Quasistatic BEM solver (BEM/@bemstat): sig = -(Lambda + F)^{-1} phi'.
 
     ``precision='fp64'`` is the drop-in; ``'mixed'`` factorises in complex
-    FP32 and refines in FP64 against F x + Lambda x.
+    FP32 and refines in FP64 against F x + Lambda x; ``'mixed_bf16'`` forms
+    the factorisation's large products from hi/lo bf16 splits.
     """
 
     def __init__(self, p, op=None, precision=None):
         op = p.op if op is None else op
         self.p = p
         self.precision = precision or op.get("precision", "fp64")
-        solver = _lib.MNP_SOLVER_FP64 if self.precision == "fp64" else _lib.MNP_SOLVER_MIXED
+        if self.precision not in bemret._SOLVERS:
+            raise ValueError(f"bemstat: unknown precision {self.precision!r}")
+        solver = bemret._SOLVERS[self.precision]
         h = C.c_void_p()
         _lib.call("mnp_bemstat_create", p.green.handle, solver, C.byref(h))
         self.handle = h

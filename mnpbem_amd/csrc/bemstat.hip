@@ -173,10 +173,12 @@ struct BemStat {
   void initmat(double2 e1, double2 e2) {
     // Lambda = 2 pi (eps1 + eps2) / (eps1 - eps2)
     lam = cscale<double>(cdiv<double>(cadd<double>(e1, e2), csub<double>(e1, e2)), 2 * M_PI);
-    if (f64)
+    if (f64) {
       f64->build(F, lam);
-    else
+    } else {
+      SplitBf16Scope split(solver == MNP_SOLVER_MIXED_BF16);
       f32->build(F, lam);
+    }
     ready = true;
   }
 
@@ -248,7 +250,8 @@ extern "C" {
 int mnp_bemstat_create(mnp_greenret_t g, int solver, mnp_bemstat_t* out) {
   return guarded([&] {
     MNP_CHECK(g && out, MNP_ERR_ARG, "bemstat_create: NULL argument");
-    MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED, MNP_ERR_ARG, "bemstat_create: unknown solver");
+    MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED || solver == MNP_SOLVER_MIXED_BF16, MNP_ERR_ARG,
+              "bemstat_create: unknown solver");
     auto* h = new mnp_bemstat_s{nullptr};
     try {
       h->b = new BemStat(g->g, solver);

@@ -62,3 +62,21 @@ def test_bemstat_sphere_vs_miestat():
         print(f"[bemstat vs miestat] level {level}: ext {ext[0]:.6g} vs {want[0][0]:.6g}, sweeps {bem.stats()}",
               flush=True)
     assert errs[1] < errs[0] < 0.05
+
+
+def test_bemstat_mixed_bf16_matches_fp64_5120_faces():
+    """Quasistatic ellipsoid, 5,120 faces: the split-bf16 factorisation with
+    FP64 refinement reproduces the FP64 solver's cross sections."""
+    B, p, op = _particle(shapes.ellipsoid(4, (10.0, 10.0, 40.0)), 1.77)
+    exc = B.planewave([[1.0, 0, 0], [0, 0, 1.0]], op=op)
+    b64 = B.bemsolver(p, op, precision="fp64")
+    bbf = B.bemsolver(p, op, precision="mixed_bf16")
+    for lam in (550.0, 800.0):
+        print(f"[bemstat mixed_bf16] enei {lam}", flush=True)
+        cb = exc.cross_sections(bbf.solve(exc(p, lam)))
+        c64 = exc.cross_sections(b64.solve(exc(p, lam)))
+        it, rr = bbf.stats()
+        print(f"  sweeps {it}, relres {rr:.2e}", flush=True)
+        assert rr < 1e-12 and it <= 6
+        for a, b in zip(cb, c64):
+            assert np.allclose(a, b, rtol=1e-10)
