@@ -156,6 +156,48 @@ def run_quasistatic(level=5, steps=3):
             "max_rel_err_vs_fp64": err, "max_rel_err_mixed_bf16_vs_fp64": err_bf}
 
 
+def run_multi_particle(per_particle=8):
+    """The seeded many-particle sweep (12 Drude particles, 320-5,120 faces, in
+    vacuum): per_particle of its 128 wavelengths (every 16th) per particle
+    through each solver; mesh set-up (geometry, refinement tables) is timed
+    apart from the solves."""
+    import torch
+
+    from mnpbem_amd import bem as B
+    from mnpbem_amd.workloads import multi_particle_sweep
+
+    parts, enei = multi_particle_sweep()
+    lams = enei[:: len(enei) // per_particle][:per_particle]
+    exc = B.planewave([[1.0, 0, 0]], [[0, 0, 1.0]])
+    out = {"workload": f"{len(parts)} seeded Drude ellipsoids (faces {sorted(len(p['faces']) for p in parts)}), "
+                       f"eps 1, pol x dir z, {per_particle} of 128 wavelengths 400-900 nm each",
+           "faces_total": int(sum(len(p["faces"]) for p in parts))}
+    res, setup = {}, 0.0
+    for prec in ("mixed", "mixed_bf16", "fp64"):
+        tot, ext = 0.0, []
+        for part in parts:
+            op = B.bemoptions(sim="ret")
+            t0 = time.perf_counter()
+            p = B.comparticle([B.epsconst(1.0), B.epsdrude(*part["drude"])], [(part["verts"], part["faces"])],
+                              [2, 1], 1, op)
+            p.green
+            bem = B.bemsolver(p, op, precision=prec)
+            torch.cuda.synchronize()
+            setup += time.perf_counter() - t0
+            t = time.perf_counter()
+            ext.append([exc.ext(bem.solve(exc(p, lam)))[0] for lam in lams])
+            torch.cuda.synchronize()
+            tot += time.perf_counter() - t
+            del bem, p
+        res[prec] = (len(parts) * len(lams) / tot, np.array(ext))
+        out[f"particle_wavelengths_per_s_{prec}"] = res[prec][0]
+        log(f"[bench] multi-particle {prec}: {res[prec][0]:.1f} particle-wavelengths/s")
+    for prec in ("mixed", "mixed_bf16"):
+        out[f"max_rel_err_{prec}_vs_fp64"] = float(np.max(np.abs(res[prec][1] / res["fp64"][1] - 1)))
+    out["setup_s_per_particle"] = setup / (3 * len(parts))
+    return out
+
+
 def run_config4(nlam=16, seed=2024):
     """H-matrix iterative path with dipole excitation: sphere of diameter 150 nm
     (icosphere level 6, 81,920 faces), Drude gold in eps=1, dipole 5 nm above
@@ -312,6 +354,7 @@ def main():
     ap.add_argument("--no-config4", action="store_true", help="skip the H-matrix dipole measurement")
     ap.add_argument("--no-split-bf16", action="store_true", help="skip the split-bf16 chain measurement")
     ap.add_argument("--no-quasistatic", action="store_true", help="skip the quasistatic measurement")
+    ap.add_argument("--no-multi-particle", action="store_true", help="skip the many-particle sweep measurement")
     ap.add_argument("--config4-only", action="store_true",
                     help="only the H-matrix dipole measurement (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -409,6 +452,8 @@ def main():
         }
     if not a.no_quasistatic:
         out["quasistatic"] = run_quasistatic(a.level)
+    if not a.no_multi_particle:
+        out["multi_particle"] = run_multi_particle()
     if not a.no_config4:
         out["hmatrix_dipole"] = run_config4()
         if not a.no_cpu_baseline:

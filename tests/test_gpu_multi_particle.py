@@ -1,0 +1,51 @@
+"""GPU parity over the seeded many-particle sweep (12 Drude particles of 320,
+1,280 and 5,120 faces in vacuum): particles up to 1,280 faces against the
+oracle, the 5,120-face ones FP64 against the mixed and split-bf16 solvers
+(the FP64 solver is pinned to the oracle by the other tests)."""
+import numpy as np
+import pytest
+
+from oracle import mnpbem_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+POL, DIR = np.array([[1.0, 0, 0]]), np.array([[0, 0, 1.0]])
+
+
+def _solver(B, part, precision):
+    op = B.bemoptions(sim="ret")
+    p = B.comparticle([B.epsconst(1.0), B.epsdrude(*part["drude"])], [(part["verts"], part["faces"])], [2, 1], 1, op)
+    return p, B.bemsolver(p, op, precision=precision)
+
+
+def test_multi_particle_sweep_parity(pinfty):
+    from mnpbem_amd import bem as B
+    from mnpbem_amd.workloads import multi_particle_sweep
+    parts, enei = multi_particle_sweep()
+    levels = sorted({p["level"] for p in parts})
+    assert len(parts) == 12 and levels == [2, 3, 4], levels
+    exc = B.planewave(POL, DIR)
+    for ip, part in enumerate(parts):
+        lams = enei[[0, 64]] if part["level"] <= 3 else enei[[32]]
+        print(f"[multi-particle] particle {ip} level {part['level']} axes "
+              f"{np.round(part['semi_axes'], 2)} drude {np.round(part['drude'], 3)}", flush=True)
+        p, b64 = _solver(B, part, "fp64")
+        _, bmx = _solver(B, part, "mixed")
+        _, bbf = _solver(B, part, "mixed_bf16")
+        for lam in lams:
+            c64 = exc.cross_sections(b64.solve(exc(p, lam)))
+            cmx = exc.cross_sections(bmx.solve(exc(p, lam)))
+            cbf = exc.cross_sections(bbf.solve(exc(p, lam)))
+            for a, b in zip(cmx[:2], c64[:2]):
+                assert np.allclose(a, b, rtol=1e-8)
+            for a, b in zip(cbf[:2], c64[:2]):
+                assert np.allclose(a, b, rtol=1e-8)
+            if part["level"] <= 3:
+                g = O.GreenRet(part["verts"], part["faces"])
+                eps_in = O.eps_drude(lam, *part["drude"])
+                bem = O.BemRet(g).initmat(lam, eps_in, 1.0)
+                a2, a2p = O.planewave_potential(g.pos, g.nvec, POL, DIR, lam, 1.0)
+                sig = bem.solve({"a2": a2, "a2p": a2p})
+                ext = O.extinction(g.pos, g.area, sig, POL, DIR, lam, 1.0)
+                sca = O.scattering(g.pos, g.area, sig, lam, 1.0, *pinfty)
+                assert np.allclose(c64[0], ext, rtol=1e-10) and np.allclose(c64[1], sca, rtol=1e-10)
