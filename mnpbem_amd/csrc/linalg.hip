@@ -1125,6 +1125,101 @@ static int64_t split(int64_t n, int64_t leaf) {
   return n1 < leaf ? leaf : n1;
 }
 
+// ----------------------------------------- fused diagonal-block solve
+// B <- L^{-1} B for one diagonal block of a unit lower factor (nb <= 256
+// rows; L at the block's origin, below the diagonal; Linv the inverses of its
+// 32 x 32 diagonal blocks), in one launch: workgroup = CB columns of B, held
+// in LDS for the whole solve.  For each 32-row block i in turn:
+//   x_i = Linv_i b_i,  then  b_j -= L_ji x_i for the rows below,
+// as 16 x 16 MFMA tiles (complex products as four real ones) spread over the
+// four waves, with a barrier after each stage.  This replaces the recursion's
+// 32-row leaves and bottom updates (about 15 dependent launches per 256 rows).
+// Rows past nb are zero in LDS and never stored; the padded part of the last
+// inverse block multiplies them only.
+template <class T, int CB>
+__global__ void __launch_bounds__(256) k_trsm_llu_blk(int nb, int64_t nrhs, const cplx<T>* L, int64_t ldl,
+                                                      const cplx<T>* Linv, cplx<T>* B, int64_t ldb) {
+  using acc_t = typename MF<T>::acc_t;
+  constexpr int NCT = CB / 16;  // column tiles
+  __shared__ T sr[TBN][CB + 1], si[TBN][CB + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int l16 = lane & 15, lk = lane >> 4;
+  const int64_t c0 = (int64_t)blockIdx.x * CB;
+  for (int idx = tid; idx < TBN * CB; idx += 256) {
+    const int r = idx % TBN, c = idx / TBN;
+    cplx<T> v = czero<T>();
+    if (r < nb && c0 + c < nrhs) v = B[(c0 + c) * ldb + r];
+    sr[r][c] = v.x;
+    si[r][c] = v.y;
+  }
+  __syncthreads();
+  // acc (+)= A(16 rows from arow, 32 k from acol of the block's L or Linv) x_i(16 cols)
+  auto tile = [&](const cplx<T>* A, int64_t lda, int arow, int acol, int xrow, int ct, int rows_valid, acc_t& cr,
+                  acc_t& ci) {
+    cplx<T> a[TB / 4];
+#pragma unroll
+    for (int s = 0; s < TB / 4; ++s) {
+      const int row = arow + l16;
+      a[s] = row < rows_valid ? A[(int64_t)(acol + 4 * s + lk) * lda + row] : czero<T>();
+    }
+#pragma unroll
+    for (int s = 0; s < TB / 4; ++s) {
+      const int k = xrow + 4 * s + lk, col = 16 * ct + l16;
+      const T br = sr[k][col], bi = si[k][col];
+      cr = MF<T>::mma(a[s].x, br, cr);
+      cr = MF<T>::mma(-a[s].y, bi, cr);
+      ci = MF<T>::mma(a[s].x, bi, ci);
+      ci = MF<T>::mma(a[s].y, br, ci);
+    }
+  };
+  const int nblk = (nb + TB - 1) / TB;
+  for (int i = 0; i < nblk; ++i) {
+    // x_i = Linv_i b_i: 2 x NCT tiles, one per wave
+    acc_t xr = acc_t{0, 0, 0, 0}, xi = acc_t{0, 0, 0, 0};
+    const bool dw = wid < 2 * NCT;
+    const int drt = wid % 2, dct = wid / 2;
+    if (dw) tile(Linv + (int64_t)i * TB2, TB, 16 * drt, 0, TB * i, dct, TB, xr, xi);
+    __syncthreads();  // b_i is read by every diagonal tile before it is replaced
+    if (dw)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = TB * i + 16 * drt + MF<T>::row(lane, r);
+        sr[row][16 * dct + l16] = xr[r];
+        si[row][16 * dct + l16] = xi[r];
+      }
+    __syncthreads();
+    // b_j -= L_ji x_i for the rows below the block (tiles round-robin over the waves)
+    const int rbeg = TB * (i + 1);
+    if (rbeg < nb) {
+      const int nrt = (nb - rbeg + 15) / 16;
+      for (int t = wid; t < nrt * NCT; t += 4) {
+        const int rt = t % nrt, ct = t / nrt;
+        acc_t ur = acc_t{0, 0, 0, 0}, ui = acc_t{0, 0, 0, 0};
+        tile(L, ldl, rbeg + 16 * rt, TB * i, TB * i, ct, nb, ur, ui);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rbeg + 16 * rt + MF<T>::row(lane, r);
+          sr[row][16 * ct + l16] -= ur[r];
+          si[row][16 * ct + l16] -= ui[r];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int idx = tid; idx < TBN * CB; idx += 256) {
+    const int r = idx % TBN, c = idx / TBN;
+    if (r < nb && c0 + c < nrhs) B[(c0 + c) * ldb + r] = mk<T>(sr[r][c], si[r][c]);
+  }
+}
+
+template <class T>
+static void trsm_llu_blk(int64_t nb, int64_t nrhs, MatView<T> LU, const cplx<T>* Linv, MatView<T> B) {
+  constexpr int CB = sizeof(T) == 4 ? 32 : 16;
+  hipLaunchKernelGGL((k_trsm_llu_blk<T, CB>), dim3((unsigned)cdiv_up(nrhs, CB)), dim3(256), 0, stream(), (int)nb,
+                     nrhs, LU.p, LU.ld, Linv, B.p, B.ld);
+  MNP_HIP(hipGetLastError());
+}
+
 // Recursive triangular solves.  Every split point is a multiple of 32 from
 // the factor's origin, so each leaf is one diagonal block and is applied as
 // an in-place product with its inverse (the GEMM tile covers the whole
@@ -1138,7 +1233,12 @@ void trsm_llu(int64_t n, int64_t nrhs, MatView<T> LU, const cplx<T>* Linv, MatVi
     gemm<T>(n, nrhs, n, ONE, MatView<T>{const_cast<cplx<T>*>(Linv), TB}, B, ZERO, B);
     return;
   }
-  const int64_t n1 = split(n, TB), n2 = n - n1;
+  if (n <= TBN) {  // one diagonal block of up to 256 rows: one fused launch
+    trsm_llu_blk<T>(n, nrhs, LU, Linv, B);
+    return;
+  }
+  // above one block, split at multiples of 256 so that the leaves are whole blocks
+  const int64_t n1 = std::max<int64_t>(TBN, (n / 2) / TBN * TBN), n2 = n - n1;
   trsm_llu<T>(n1, nrhs, LU, Linv, B);
   gemm<T>(n2, nrhs, n1, MONE, MatView<T>{LU.at(n1, 0), LU.ld}, B, ONE, MatView<T>{B.at(n1, 0), B.ld});
   trsm_llu<T>(n2, nrhs, MatView<T>{LU.at(n1, n1), LU.ld}, Linv + (n1 / TB) * TB2,
