@@ -1153,15 +1153,18 @@ __global__ void __launch_bounds__(256) k_trsm_llu_blk(int nb, int64_t nrhs, cons
     si[r][c] = v.y;
   }
   __syncthreads();
-  // acc (+)= A(16 rows from arow, 32 k from acol of the block's L or Linv) x_i(16 cols)
-  auto tile = [&](const cplx<T>* A, int64_t lda, int arow, int acol, int xrow, int ct, int rows_valid, acc_t& cr,
-                  acc_t& ci) {
-    cplx<T> a[TB / 4];
+  // A operand of one 16-row tile (rows from arow, the 32 columns from acol of
+  // L or Linv); every stage issues all of its loads before its first MFMA
+  auto load_a = [&](const cplx<T>* A, int64_t lda, int arow, int acol, int rows_valid, cplx<T> (&a)[TB / 4]) {
+    const int row = min(arow + l16, rows_valid - 1);  // clamped: valid address, zeroed below
 #pragma unroll
-    for (int s = 0; s < TB / 4; ++s) {
-      const int row = arow + l16;
-      a[s] = row < rows_valid ? A[(int64_t)(acol + 4 * s + lk) * lda + row] : czero<T>();
-    }
+    for (int s = 0; s < TB / 4; ++s) a[s] = A[(int64_t)(acol + 4 * s + lk) * lda + row];
+    if (arow + l16 >= rows_valid)
+#pragma unroll
+      for (int s = 0; s < TB / 4; ++s) a[s] = czero<T>();
+  };
+  // (cr, ci) += a x_i(16 columns from 16 ct), x_i at LDS rows xrow ..
+  auto mma_tile = [&](const cplx<T> (&a)[TB / 4], int xrow, int ct, acc_t& cr, acc_t& ci) {
 #pragma unroll
     for (int s = 0; s < TB / 4; ++s) {
       const int k = xrow + 4 * s + lk, col = 16 * ct + l16;
@@ -1172,13 +1175,21 @@ __global__ void __launch_bounds__(256) k_trsm_llu_blk(int nb, int64_t nrhs, cons
       ci = MF<T>::mma(a[s].y, br, ci);
     }
   };
+  constexpr int MAXR = (TBN - TB) / 16 / 4;  // row tiles below a block per wave (at most 14 over 4 waves)
   const int nblk = (nb + TB - 1) / TB;
   for (int i = 0; i < nblk; ++i) {
-    // x_i = Linv_i b_i: 2 x NCT tiles, one per wave
-    acc_t xr = acc_t{0, 0, 0, 0}, xi = acc_t{0, 0, 0, 0};
+    const int rbeg = TB * (i + 1), nrt = rbeg < nb ? (nb - rbeg + 15) / 16 : 0;
+    // loads: this wave's diagonal tile (x_i = Linv_i b_i, 2 x NCT tiles, one
+    // per wave) and its row tiles below (rt = wid + 4 q)
     const bool dw = wid < 2 * NCT;
     const int drt = wid % 2, dct = wid / 2;
-    if (dw) tile(Linv + (int64_t)i * TB2, TB, 16 * drt, 0, TB * i, dct, TB, xr, xi);
+    cplx<T> ad[TB / 4], au[MAXR + 1][TB / 4];
+    if (dw) load_a(Linv + (int64_t)i * TB2, TB, 16 * drt, 0, TB, ad);
+#pragma unroll
+    for (int q = 0; q <= MAXR; ++q)
+      if (wid + 4 * q < nrt) load_a(L, ldl, rbeg + 16 * (wid + 4 * q), TB * i, nb, au[q]);
+    acc_t xr = acc_t{0, 0, 0, 0}, xi = acc_t{0, 0, 0, 0};
+    if (dw) mma_tile(ad, TB * i, dct, xr, xi);
     __syncthreads();  // b_i is read by every diagonal tile before it is replaced
     if (dw)
 #pragma unroll
@@ -1188,20 +1199,23 @@ __global__ void __launch_bounds__(256) k_trsm_llu_blk(int nb, int64_t nrhs, cons
         si[row][16 * dct + l16] = xi[r];
       }
     __syncthreads();
-    // b_j -= L_ji x_i for the rows below the block (tiles round-robin over the waves)
-    const int rbeg = TB * (i + 1);
-    if (rbeg < nb) {
-      const int nrt = (nb - rbeg + 15) / 16;
-      for (int t = wid; t < nrt * NCT; t += 4) {
-        const int rt = t % nrt, ct = t / nrt;
-        acc_t ur = acc_t{0, 0, 0, 0}, ui = acc_t{0, 0, 0, 0};
-        tile(L, ldl, rbeg + 16 * rt, TB * i, TB * i, ct, nb, ur, ui);
+    // b_j -= L_ji x_i for the rows below the block
+    if (nrt) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = rbeg + 16 * rt + MF<T>::row(lane, r);
-          sr[row][16 * ct + l16] -= ur[r];
-          si[row][16 * ct + l16] -= ui[r];
-        }
+      for (int q = 0; q <= MAXR; ++q) {
+        const int rt = wid + 4 * q;
+        if (rt < nrt)
+#pragma unroll
+          for (int ct = 0; ct < NCT; ++ct) {
+            acc_t ur = acc_t{0, 0, 0, 0}, ui = acc_t{0, 0, 0, 0};
+            mma_tile(au[q], TB * i, ct, ur, ui);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int row = rbeg + 16 * rt + MF<T>::row(lane, r);
+              sr[row][16 * ct + l16] -= ur[r];
+              si[row][16 * ct + l16] -= ui[r];
+            }
+          }
       }
       __syncthreads();
     }
