@@ -149,11 +149,36 @@ def run_quasistatic(level=5, steps=3):
     err_bf = max(float(np.max(np.abs(a / b - 1))) for a, b in zip(res["mixed_bf16"][1], res["fp64"][1]))
     log(f"[bench] quasistatic: mixed {res['mixed'][0]:.2f}/s, mixed_bf16 {res['mixed_bf16'][0]:.2f}/s, "
         f"fp64 {res['fp64'][0]:.2f}/s")
+    # iterative twin (bemstatiter, bemiter defaults): F filled once per mesh
+    it = B.bemstatiter(p, dict(op, iter=dict(cleaf=200, htol=1e-6, tol=1e-6)))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    it(lams[0])
+    torch.cuda.synchronize()
+    t_fill = time.perf_counter() - t
+    exc.ext(it.solve(exc(p, lams[0])))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ext_it, iters = [], []
+    for lam in lams[1:1 + steps]:
+        ext_it.append(exc.ext(it.solve(exc(p, lam))))
+        iters.append(it.stats()["iter"])
+    torch.cuda.synchronize()
+    v_it = steps / (time.perf_counter() - t)
+    err_it = max(float(np.max(np.abs(a / b - 1))) for a, b in zip(ext_it, res["fp64"][1]))
+    compression = it.stats()["compression"]
+    del it
+    log(f"[bench] quasistatic iterative: fill {t_fill:.3f} s, {v_it:.1f}/s, iterations {iters}, "
+        f"rel err vs dense FP64 {err_it:.2e}")
     return {"workload": f"quasistatic bemstat, same ellipsoid ({len(f)} faces) and wavelengths as the headline",
             "wavelengths_per_s_mixed": res["mixed"][0], "wavelengths_per_s_mixed_bf16": res["mixed_bf16"][0],
             "wavelengths_per_s_fp64": res["fp64"][0],
             "speedup": res["mixed"][0] / res["fp64"][0], "speedup_mixed_bf16": res["mixed_bf16"][0] / res["fp64"][0],
-            "max_rel_err_vs_fp64": err, "max_rel_err_mixed_bf16_vs_fp64": err_bf}
+            "max_rel_err_vs_fp64": err, "max_rel_err_mixed_bf16_vs_fp64": err_bf,
+            "iterative": {"solver": "bemstatiter, FP64 H-matrix of F (cleaf 200, htol 1e-6), GMRES tol 1e-6, "
+                                    "block preconditioner",
+                          "fill_s": t_fill, "wavelengths_per_s": v_it, "gmres_iterations": iters,
+                          "compression": compression, "max_rel_err_ext_vs_dense_fp64": err_it}}
 
 
 def run_multi_particle(per_particle=8):

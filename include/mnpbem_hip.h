@@ -261,6 +261,31 @@ int mnp_bemretiter_solve(mnp_bemretiter_t it, int64_t npol, const void* phi1, co
 int mnp_bemretiter_stats(mnp_bemretiter_t it, int* flag, double* relres, int* iter,
                          double* true_relres, double* compression);
 
+/* ----------------------------- iterative quasistatic solver (bemstatiter)
+ * Replaces bemsolver(p, op) with op.sim = 'stat' and op.iter set ->
+ * bemstatiter (BEM/@bemstatiter): F of the static Green function as an
+ * H-matrix (private/init.m:11-13, filled once: it does not depend on the
+ * wavelength), Lambda = 2 pi (eps1 + eps2) / (eps1 - eps2) per wavelength
+ * (private/initmat.m:12), and GMRES on afun(x) = -(F x + Lambda x)
+ * (private/afun.m:7, solve.m:17-20) with right-hand side phip, so that the
+ * solution is sig.  Flexible right preconditioning by the exact inverses of
+ * -(Lambda + F) on diagonal blocks of at most 32 faces, in place of the
+ * reference's H-matrix LU or dense inverse (private/initmat.m:22-30).
+ * Precision and block arguments as for mnp_hmat_create. */
+typedef struct mnp_bemstatiter_s* mnp_bemstatiter_t;
+int mnp_bemstatiter_create(mnp_greenret_t g, int precision, const int64_t* perm, int64_t nfull,
+                           const int64_t* full, int64_t nlow, const int64_t* low, double htol,
+                           int kmax, mnp_bemstatiter_t* out);
+int mnp_bemstatiter_destroy(mnp_bemstatiter_t it);
+/* GMRES tolerance and iteration limit (bemiter.options: 1e-6, 100); precond 0 disables it. */
+int mnp_bemstatiter_set_options(mnp_bemstatiter_t it, double tol, int maxit, int precond);
+int mnp_bemstatiter_initmat(mnp_bemstatiter_t it, double enei, mnp_complex eps1, mnp_complex eps2);
+/* sig (n x npol, device, complex128) from phip (npol <= 8). */
+int mnp_bemstatiter_solve(mnp_bemstatiter_t it, int64_t npol, const void* phip, void* sig);
+/* As mnp_bemretiter_stats; compression of F. */
+int mnp_bemstatiter_stats(mnp_bemstatiter_t it, int* flag, double* relres, int* iter,
+                          double* true_relres, double* compression);
+
 /* --------------------------------------------- plane wave and observables
  * Simulation/retarded/@planewaveret/potential.m:47-88 for a particle whose
  * outside is the embedding medium: a2 and a2p (device out, complex128,

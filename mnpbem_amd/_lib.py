@@ -80,6 +80,12 @@ PROTOTYPES = {
     "mnp_bemretiter_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
     "mnp_bemretiter_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mnp_bemretiter_stats": (INT, [P, C.POINTER(INT), PD, C.POINTER(INT), PD, PD]),
+    "mnp_bemstatiter_create": (INT, [P, INT, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
+    "mnp_bemstatiter_destroy": (INT, [P]),
+    "mnp_bemstatiter_set_options": (INT, [P, D, INT, INT]),
+    "mnp_bemstatiter_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
+    "mnp_bemstatiter_solve": (INT, [P, I64, P, P]),
+    "mnp_bemstatiter_stats": (INT, [P, C.POINTER(INT), PD, C.POINTER(INT), PD, PD]),
     "mnp_bemstat_create": (INT, [P, INT, C.POINTER(P)]),
     "mnp_bemstat_destroy": (INT, [P]),
     "mnp_bemstat_initmat": (INT, [P, D, MnpComplex, MnpComplex]),

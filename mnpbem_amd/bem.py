@@ -222,6 +222,8 @@ class bemretiter:
     include/mnpbem_hip.h.
     """
 
+    _PREFIX = "mnp_bemretiter"
+
     def __init__(self, p, op):
         from .hmatrix import admissibility, block_ranges, clustertree
         it = bemiter_options(**op.get("iter", {}))
@@ -238,24 +240,24 @@ class bemretiter:
         htol = float(np.min(np.atleast_1d(it["htol"])))
         h = C.c_void_p()
         prec = {"fp64": _lib.MNP_FP64, "fp32": _lib.MNP_FP32}[it["hprecision"]]
-        _lib.call("mnp_bemretiter_create", p.green.handle, prec, perm.ctypes.data_as(C.c_void_p), len(fr),
+        _lib.call(self._PREFIX + "_create", p.green.handle, prec, perm.ctypes.data_as(C.c_void_p), len(fr),
                   fr.ctypes.data_as(C.c_void_p), len(lr), lr.ctypes.data_as(C.c_void_p), htol, kmax,
                   C.byref(h))
         self.handle = h
-        _lib.call("mnp_bemretiter_set_options", h, float(it["tol"]), int(it["maxit"]),
+        _lib.call(self._PREFIX + "_set_options", h, float(it["tol"]), int(it["maxit"]),
                   int(it["precond"] not in (None, "none", False)))
         self.enei = None
 
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value and _lib is not None and _lib._lib is not None:
-            _lib.lib().mnp_bemretiter_destroy(h)
+            getattr(_lib.lib(), self._PREFIX + "_destroy")(h)
             self.handle = None
 
     def __call__(self, enei):
         if self.enei != enei:
             e1, e2 = self.p.eps1(enei), self.p.eps2(enei)
-            _lib.call("mnp_bemretiter_initmat", self.handle, float(enei), _lib.MnpComplex.of(e1),
+            _lib.call(self._PREFIX + "_initmat", self.handle, float(enei), _lib.MnpComplex.of(e1),
                       _lib.MnpComplex.of(e2))
             self.enei = enei
         return self
@@ -278,10 +280,28 @@ class bemretiter:
     def stats(self):
         """dict(flag, relres, iter, true_relres, compression) of the last solve."""
         f, rr, it, tr, cp = C.c_int(), C.c_double(), C.c_int(), C.c_double(), C.c_double()
-        _lib.call("mnp_bemretiter_stats", self.handle, C.byref(f), C.byref(rr), C.byref(it),
+        _lib.call(self._PREFIX + "_stats", self.handle, C.byref(f), C.byref(rr), C.byref(it),
                   C.byref(tr), C.byref(cp))
         return dict(flag=f.value, relres=rr.value, iter=it.value, true_relres=tr.value,
                     compression=cp.value)
+
+
+class bemstatiter(bemretiter):
+    """Iterative quasistatic BEM solver with an H-matrix of F (BEM/@bemstatiter):
+    GMRES on -(F + Lambda) sig = phi' with a block preconditioner; options as
+    for bemretiter (op['iter'])."""
+
+    _PREFIX = "mnp_bemstatiter"
+
+    def solve(self, exc):
+        """[sig, bem] = solve(bem, exc)  (BEM/@bemstatiter/solve.m), exc with field phip."""
+        self(exc.enei)
+        n, npol = self.p.n, exc.phip.shape[-1]
+        sig = dev.colmajor((n, npol))
+        _lib.call("mnp_bemstatiter_solve", self.handle, npol, dev.ptr(dev.base(exc.phip)), dev.ptr(dev.base(sig)))
+        return compstruct(self.p, exc.enei, sig=sig)
+
+    mldivide = solve
 
 
 class bemstat:
@@ -334,13 +354,13 @@ class bemstat:
 
 
 def bemsolver(p, op=None, **kw):
-    """Base/bemsolver.m: bemret for op.sim == 'ret' (bemretiter when op.iter is
-    set), bemstat for op.sim == 'stat'."""
+    """Base/bemsolver.m: bemret for op.sim == 'ret', bemstat for op.sim ==
+    'stat'; bemretiter / bemstatiter when op.iter is set."""
     op = dict(p.op if op is None else op)
     op.update(kw)
     sim = op.get("sim", "ret")
     if sim == "stat":
-        return bemstat(p, op)
+        return bemstatiter(p, op) if op.get("iter") else bemstat(p, op)
     if sim != "ret":
         raise NotImplementedError(f"unknown simulation type {sim!r}")
     if op.get("iter"):

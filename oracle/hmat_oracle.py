@@ -141,3 +141,35 @@ def hmat_dense(green, tree, blocks_low, k, key, htol=1e-6, kmax=100):
         L, R = aca(lambda r: M[r], lambda c: M[:, c], len(rows), len(cols), tol=htol, kmax=kmax)
         dense[np.ix_(rows, cols)] = L @ R.T
     return dense
+
+
+def gmres_relres(A, b, mfun=None, maxit=100):
+    """Residual history of GMRES without restarts, right-preconditioned:
+    res[k] = min over y of |b - A M y| / |b| with y in the k-dimensional
+    Krylov space of A M and b (Arnoldi with twice-iterated classical
+    Gram-Schmidt, least squares on the Hessenberg matrix).  ``mfun`` applies
+    M (identity when None); b is one column.  The iterative solvers of
+    BEM/@bemiter/solve.m:33-36 stop at the first k with res[k] <= tol; with
+    a fixed M the flexible form used on the GPU spans the same space."""
+    b = np.asarray(b, dtype=complex).ravel()
+    m = (lambda v: v) if mfun is None else mfun
+    nb = np.linalg.norm(b)
+    V = [b / nb]
+    H = np.zeros((maxit + 1, maxit), dtype=complex)
+    e1 = np.zeros(maxit + 1, dtype=complex)
+    e1[0] = nb
+    res = [1.0]
+    for k in range(maxit):
+        w = A @ m(V[k])
+        for _ in range(2):
+            Q = np.array(V).T
+            h = Q.conj().T @ w
+            w = w - Q @ h
+            H[:k + 1, k] += h
+        H[k + 1, k] = np.linalg.norm(w)
+        y = np.linalg.lstsq(H[:k + 2, :k + 1], e1[:k + 2], rcond=None)[0]
+        res.append(np.linalg.norm(e1[:k + 2] - H[:k + 2, :k + 1] @ y) / nb)
+        if H[k + 1, k].real < 1e-300:
+            break
+        V.append(w / H[k + 1, k].real)
+    return np.array(res)

@@ -112,3 +112,25 @@ def test_hmat_dense_blocks():
         r, c = perm[cb[i1]:ce[i1]], perm[cb[i2]:ce[i2]]
         M = H.kernel_block(g.pos, g.nvec, g.area, r, c, k, "F")
         assert np.linalg.norm(A[np.ix_(r, c)] - M) <= 1e-6 * np.linalg.norm(M)
+
+
+def test_gmres_relres_matches_brute_force_krylov_least_squares():
+    """The GMRES residual history equals min |b - A M y| over the Krylov space
+    spanned explicitly by [b, (AM) b, ..., (AM)^(k-1) b] (QR of that basis,
+    small k), is non-increasing, and reaches ~0 at k = n."""
+    rng = np.random.default_rng(5)
+    n = 12
+    A = np.eye(n) * 3 + rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    b = rng.normal(size=n) + 1j * rng.normal(size=n)
+    Mi = np.diag(1 + rng.random(n)) + 0.1 * rng.normal(size=(n, n))
+    for mfun, AM in ((None, A), (lambda v: Mi @ v, A @ Mi)):
+        res = H.gmres_relres(A, b, mfun, maxit=n)
+        K = np.empty((n, 0), dtype=complex)
+        v = b.astype(complex)
+        for k in range(1, 7):
+            K = np.column_stack([K, v])
+            v = AM @ v
+            Q, _ = np.linalg.qr(AM @ K)
+            r = b - Q @ (Q.conj().T @ b)
+            assert np.isclose(res[k], np.linalg.norm(r) / np.linalg.norm(b), rtol=1e-8)
+        assert np.all(np.diff(res) <= 1e-12) and res[n] < 1e-10
