@@ -181,6 +181,58 @@ def run_quasistatic(level=5, steps=3):
                           "compression": compression, "max_rel_err_ext_vs_dense_fp64": err_it}}
 
 
+def run_mirror(steps=3):
+    """Mirror symmetry 'xy' (bemretmirror): the quarter x, y >= 0 of the
+    prolate 10 x 10 x 40 nm ellipsoid meshed by octants (2 x 50^2 = 5,000
+    faces), completed to 20,000 faces; Drude gold in eps = 1.77, pol x and
+    pol y along +z, the headline's wavelengths.  FP64, mixed and split-bf16
+    mirror solvers, and the mixed solver without symmetry on the same
+    20,000-face particle; cross sections compared with the FP64 mirror
+    solver."""
+    import torch
+
+    from mnpbem_amd import bem as B, shapes
+
+    part = shapes.ellipsoid_octants(50, (10.0, 10.0, 40.0), "xy")
+    op = B.bemoptions(sim="ret", sym="xy")
+    p = B.comparticlemirror([B.epsconst(EPS_OUT), B.epsdrude(*DRUDE)], [part], [2, 1], 1, op)
+    pol, dirs = [[1.0, 0, 0], [0, 1.0, 0]], [[0, 0, 1.0], [0, 0, 1.0]]
+    exc = B.planewave(pol, dirs, op)
+    lams = np.linspace(500.0, 1000.0, 64)
+
+    def timed(bem, ex, part_, nst):
+        ex.ext(bem.solve(ex(part_, lams[0])))  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        cs = [np.array(ex.cross_sections(bem.solve(ex(part_, lam)))) for lam in lams[1:1 + nst]]
+        torch.cuda.synchronize()
+        return nst / (time.perf_counter() - t), cs
+
+    res, sweeps = {}, {}
+    for prec, nst in (("mixed", steps), ("mixed_bf16", steps), ("fp64", 1)):
+        bem = B.bemsolver(p, op, precision=prec)
+        res[prec] = timed(bem, exc, p, nst)
+        sweeps[prec] = bem.val_stats
+        del bem
+    fop = B.bemoptions(sim="ret")
+    pf = B.comparticle(p.eps, [(p.full().verts, p.full().faces)], [2, 1], 1, fop)
+    bem = B.bemsolver(pf, fop, precision="mixed")
+    res["full_mixed"] = timed(bem, B.planewave(pol, dirs, fop), pf, 1)
+    del bem
+    err = {k: max(float(np.max(np.abs(a / b - 1))) for a, b in zip(res[k][1], res["fp64"][1]))
+           for k in ("mixed", "mixed_bf16", "full_mixed")}
+    log(f"[bench] mirror xy: mixed {res['mixed'][0]:.2f}/s, mixed_bf16 {res['mixed_bf16'][0]:.2f}/s, "
+        f"fp64 {res['fp64'][0]:.2f}/s, full particle mixed {res['full_mixed'][0]:.3f}/s; rel err vs FP64 {err}")
+    return {"workload": f"mirror symmetry xy: quarter of the ellipsoid 10x10x40 nm meshed by octants, {p.n} faces "
+                        f"of {p.full().n}, pol x and y along +z, wavelengths from linspace(500,1000,64)",
+            "wavelengths_per_s_mixed": res["mixed"][0], "wavelengths_per_s_mixed_bf16": res["mixed_bf16"][0],
+            "wavelengths_per_s_fp64": res["fp64"][0], "wavelengths_per_s_full_particle_mixed": res["full_mixed"][0],
+            "speedup_mixed_vs_full_particle_mixed": res["mixed"][0] / res["full_mixed"][0],
+            "refinement_sweeps_mixed": sweeps["mixed"], "refinement_sweeps_mixed_bf16": sweeps["mixed_bf16"],
+            "max_rel_err_vs_fp64": err["mixed"], "max_rel_err_mixed_bf16_vs_fp64": err["mixed_bf16"],
+            "max_rel_err_full_particle_vs_mirror_fp64": err["full_mixed"]}
+
+
 def run_multi_particle(per_particle=8):
     """The seeded many-particle sweep (12 Drude particles, 320-5,120 faces, in
     vacuum): per_particle of its 128 wavelengths (every 16th) per particle
@@ -380,6 +432,7 @@ def main():
     ap.add_argument("--no-split-bf16", action="store_true", help="skip the split-bf16 chain measurement")
     ap.add_argument("--no-quasistatic", action="store_true", help="skip the quasistatic measurement")
     ap.add_argument("--no-multi-particle", action="store_true", help="skip the many-particle sweep measurement")
+    ap.add_argument("--no-mirror", action="store_true", help="skip the mirror-symmetry measurement")
     ap.add_argument("--config4-only", action="store_true",
                     help="only the H-matrix dipole measurement (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -477,6 +530,8 @@ def main():
         }
     if not a.no_quasistatic:
         out["quasistatic"] = run_quasistatic(a.level)
+    if not a.no_mirror:
+        out["mirror"] = run_mirror()
     if not a.no_multi_particle:
         out["multi_particle"] = run_multi_particle()
     if not a.no_config4:

@@ -308,6 +308,48 @@ int mnp_planewave_cross_sections(mnp_greenret_t g, double enei, mnp_complex eps_
                                  const void* pinfty_area, const void* sig2, const void* h2,
                                  double* ext, double* sca, double* abs_);
 
+/* ------------------------------------------------- mirror symmetry solver
+ * Replaces bemsolver(p, op) for a comparticlemirror p (op.sym = 'x', 'y' or
+ * 'xy') -> bemretmirror (BEM/@bemretmirror).  g is the Green function of the
+ * full particle (Particles/@comparticlemirror/full, init.m:42-54): nsym = 2
+ * or 4 copies of the n-face symmetry part, copy j in faces j n .. (j+1) n - 1,
+ * copy 0 the part itself.  symtable is nsym x nsym row-major, +-1, row s the
+ * values of symmetry s on the copies (comparticlemirror/private/init.m:29-37).
+ * initmat assembles, per symmetry value s, G_s = sum_j symtable[s][j]
+ * G(part, copy j) (Greenfun/@compgreenretmirror/eval.m:23-42) for G1, G2,
+ * H1, H2 and factorises the chain of bemretmirror/private/initmat.m:25-66
+ * per s.  solve takes one excitation of the symmetry part (fields as for
+ * mnp_bemret_solve, n rows) with the symmetry indices symx, symy, symz
+ * (0-based rows of symtable) of its x, y and z components (scalars go with
+ * z; mldivide.m:30-33), factorises Sigma of that combination once per
+ * wavelength (initsigmai.m:19-30) and returns sig1, sig2, h1, h2 on the
+ * symmetry part (mldivide.m:25-87).  solver as for mnp_bemret_create. */
+typedef struct mnp_bemretmirror_s* mnp_bemretmirror_t;
+int mnp_bemretmirror_create(mnp_greenret_t g, int solver, int nsym, const int* symtable,
+                            mnp_bemretmirror_t* out);
+int mnp_bemretmirror_destroy(mnp_bemretmirror_t bem);
+int mnp_bemretmirror_initmat(mnp_bemretmirror_t bem, double enei, mnp_complex eps1,
+                             mnp_complex eps2);
+int mnp_bemretmirror_solve(mnp_bemretmirror_t bem, int symx, int symy, int symz, int64_t npol,
+                           const void* phi1, const void* phi1p, const void* a1, const void* a1p,
+                           const void* phi2, const void* phi2p, const void* a2, const void* a2p,
+                           void* sig1, void* sig2, void* h1, void* h2);
+/* Refinement sweeps and final relative residual of the last mixed solve. */
+int mnp_bemretmirror_stats(mnp_bemretmirror_t bem, int* iterations, double* relres);
+int mnp_bemretmirror_set_refinement(mnp_bemretmirror_t bem, double tol, int maxit);
+/* Values on the full particle from two solutions on the symmetry part
+ * (Simulation/retarded/@planewaveretmirror/full.m:22-64 over
+ * Particles/@compstructmirror/expand.m:22-46):
+ *   out[j n + i, c, o] = w[2o] sign1[j][c] in1[i, c, src[o]]
+ *                      + w[2o+1] sign2[j][c] in2[i, c, src[o]],
+ * in1/in2 device n x ncomp x nsrc (in2 may be NULL), out device
+ * (nsym n) x ncomp x nout, ncomp 1 (scalars) or 3 (vectors), sign1/sign2
+ * host nsym x ncomp (+-1: the symmetry value of component c on copy j), w
+ * host nout x 2, src host nout (nout <= 64). */
+int mnp_mirror_full(int64_t n, int nsym, int ncomp, int64_t nsrc, int64_t nout, const int* sign1,
+                    const int* sign2, const double* w, const int* src, const void* in1,
+                    const void* in2, void* out);
+
 /* ---------------------------------------------------- quasistatic solver
  * Replaces bemsolver(p, op) with op.sim = 'stat' -> bemstat (BEM/@bemstat):
  * F is the static surface derivative with refinement and the closed-surface

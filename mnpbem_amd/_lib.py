@@ -80,6 +80,13 @@ PROTOTYPES = {
     "mnp_bemretiter_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
     "mnp_bemretiter_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mnp_bemretiter_stats": (INT, [P, C.POINTER(INT), PD, C.POINTER(INT), PD, PD]),
+    "mnp_bemretmirror_create": (INT, [P, INT, INT, C.POINTER(INT), C.POINTER(P)]),
+    "mnp_bemretmirror_destroy": (INT, [P]),
+    "mnp_bemretmirror_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
+    "mnp_bemretmirror_solve": (INT, [P, INT, INT, INT, I64] + [P] * 12),
+    "mnp_bemretmirror_stats": (INT, [P, C.POINTER(INT), PD]),
+    "mnp_bemretmirror_set_refinement": (INT, [P, D, INT]),
+    "mnp_mirror_full": (INT, [I64, INT, INT, I64, I64, C.POINTER(INT), C.POINTER(INT), PD, C.POINTER(INT), P, P, P]),
     "mnp_bemstatiter_create": (INT, [P, INT, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
     "mnp_bemstatiter_destroy": (INT, [P]),
     "mnp_bemstatiter_set_options": (INT, [P, D, INT, INT]),

@@ -353,12 +353,160 @@ class bemstat:
         return it.value, rr.value
 
 
+# ------------------------------------------------------- mirror symmetry
+# Particles/@comparticlemirror/private/init.m:29-37 and symvalue.m:14-31
+_SYMTABLE = {
+    "x": np.array([[1, 1], [1, -1]]),
+    "y": np.array([[1, 1], [1, -1]]),
+    "xy": np.array([[1, 1, 1, 1], [1, 1, -1, -1], [1, -1, 1, -1], [1, -1, -1, 1]]),
+}
+_SYMVALUE = {"+": (1, 1), "-": (1, -1), "++": (1, 1, 1, 1), "+-": (1, 1, -1, -1),
+             "-+": (1, -1, 1, -1), "--": (1, -1, -1, 1)}
+
+
+def _mirror_full_mesh(verts, faces, sym):
+    """The full particle of comparticlemirror/private/init.m:42-54: copies
+    [p, flip x] ('x'), [p, flip y] ('y') or [p, flip x, flip y, flip y of
+    flip x] ('xy'); a flip negates one coordinate and reverses the vertex
+    order of each face (Particles/@particle/flip.m, flipfaces.m)."""
+    parts = [(np.asarray(verts, np.float64), np.asarray(faces, np.int64))]
+    for k, keys in ((0, ("x", "xy")), (1, ("y", "xy"))):
+        if sym in keys:
+            new = []
+            for v, f in parts:
+                v = v.copy()
+                v[:, k] = -v[:, k]
+                new.append((v, f[:, ::-1].copy()))
+            parts += new
+    off = np.cumsum([0] + [len(v) for v, _ in parts])
+    return (np.vstack([v for v, _ in parts]), np.vstack([f + off[i] for i, (_, f) in enumerate(parts)]))
+
+
+class comparticlemirror(comparticle):
+    """comparticlemirror(eps, {p}, inout, closed, op) with op['sym'] = 'x',
+    'y' or 'xy' (Particles/@comparticlemirror): ``parts`` is the symmetry
+    part, which the mirror copies complete to the closed particle
+    ``full()``.  ``n`` counts the faces of the part; the Green function is
+    the full particle's."""
+
+    def __init__(self, eps, parts, inout, closed=1, op=None):
+        op = bemoptions() if op is None else op
+        sym = op.get("sym")
+        if sym not in _SYMTABLE:
+            raise ValueError("comparticlemirror: op['sym'] must be 'x', 'y' or 'xy'")
+        super().__init__(eps, parts, inout, closed, op)
+        self.sym = sym
+        self.symtable = _SYMTABLE[sym]
+        V, F = _mirror_full_mesh(self.verts, self.faces, sym)
+        self.pfull = comparticle(eps, [(V, F)], inout, closed, {k: v for k, v in op.items() if k != "sym"})
+
+    def full(self):
+        return self.pfull
+
+    @property
+    def green(self):
+        return self.pfull.green
+
+    def geometry(self):
+        g = self.pfull.geometry()
+        return {k: v[:self.n] for k, v in g.items()}
+
+    def symvalue(self, key):
+        return np.array([_SYMVALUE[k] for k in key]) if isinstance(key, (list, tuple)) else np.array(_SYMVALUE[key])
+
+    def symindex(self, tab):
+        return int(np.flatnonzero(np.all(self.symtable == np.asarray(tab), axis=1))[0])
+
+
+class compstructmirror:
+    """Particles/@compstructmirror: ``val`` is a list of compstructs on the
+    symmetry part, each with ``symval`` (3 x J: symmetry values of the x, y
+    and z components; scalars take the z row); ``fun`` maps it to the full
+    particle."""
+
+    def __init__(self, p, enei, fun=None):
+        self.p, self.enei, self.fun, self.val = p, enei, fun, []
+
+    def full(self):
+        return self.fun(self)
+
+
+class bemretmirror:
+    """Dense retarded BEM solver with mirror symmetry (BEM/@bemretmirror),
+    precision 'fp64', 'mixed' or 'mixed_bf16' as for bemret."""
+
+    def __init__(self, p, op=None, precision=None):
+        op = p.op if op is None else op
+        if not isinstance(p, comparticlemirror):
+            raise TypeError("bemretmirror needs a comparticlemirror")
+        self.p = p
+        self.precision = precision or op.get("precision", "fp64")
+        if self.precision not in bemret._SOLVERS:
+            raise ValueError(f"bemretmirror: unknown precision {self.precision!r}")
+        tab = np.ascontiguousarray(p.symtable, dtype=np.int32)
+        h = C.c_void_p()
+        _lib.call("mnp_bemretmirror_create", p.green.handle, bemret._SOLVERS[self.precision], len(tab),
+                  tab.ctypes.data_as(C.POINTER(C.c_int)), C.byref(h))
+        self.handle = h
+        if "refine_tol" in op or "refine_maxit" in op:
+            _lib.call("mnp_bemretmirror_set_refinement", h, float(op.get("refine_tol", 1e-12)),
+                      int(op.get("refine_maxit", 10)))
+        self.enei = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib.lib().mnp_bemretmirror_destroy(h)
+            self.handle = None
+
+    def __call__(self, enei):
+        if self.enei != enei:
+            e1, e2 = self.p.eps1(enei), self.p.eps2(enei)
+            _lib.call("mnp_bemretmirror_initmat", self.handle, float(enei), _lib.MnpComplex.of(e1),
+                      _lib.MnpComplex.of(e2))
+            self.enei = enei
+        return self
+
+    def solve(self, exc):
+        """sig = bem \\ exc  (BEM/@bemretmirror/mldivide.m) for a compstructmirror."""
+        self(exc.enei)
+        n = self.p.n
+        sig = compstructmirror(self.p, exc.enei, exc.fun)
+        self.val_stats = []
+        for val in exc.val:
+            npol = next(v.shape[-1] for k, v in val.items() if v is not None and k != "symval")
+            sx = [self.p.symindex(row) for row in val.symval]
+            args = [None if val.get(k) is None else dev.ptr(dev.base(val[k]))
+                    for k in ("phi1", "phi1p", "a1", "a1p", "phi2", "phi2p", "a2", "a2p")]
+            out = dict(sig1=dev.colmajor((n, npol)), sig2=dev.colmajor((n, npol)),
+                       h1=dev.colmajor((n, 3, npol)), h2=dev.colmajor((n, 3, npol)))
+            _lib.call("mnp_bemretmirror_solve", self.handle, *sx, npol, *args,
+                      *[dev.ptr(dev.base(out[k])) for k in ("sig1", "sig2", "h1", "h2")])
+            sig.val.append(compstruct(self.p, exc.enei, symval=val.symval, **out))
+            self.val_stats.append(self.stats())
+        return sig
+
+    mldivide = solve
+
+    def stats(self):
+        """(refinement sweeps, final relative residual) of the last excitation
+        solved; ``val_stats`` holds them for each excitation of the last solve."""
+        it, rr = C.c_int(), C.c_double()
+        _lib.call("mnp_bemretmirror_stats", self.handle, C.byref(it), C.byref(rr))
+        return it.value, rr.value
+
+
 def bemsolver(p, op=None, **kw):
     """Base/bemsolver.m: bemret for op.sim == 'ret', bemstat for op.sim ==
-    'stat'; bemretiter / bemstatiter when op.iter is set."""
+    'stat'; bemretiter / bemstatiter when op.iter is set; bemretmirror for a
+    comparticlemirror."""
     op = dict(p.op if op is None else op)
     op.update(kw)
     sim = op.get("sim", "ret")
+    if isinstance(p, comparticlemirror) or op.get("sym"):
+        if sim != "ret" or op.get("iter"):
+            raise NotImplementedError("mirror symmetry: only the dense retarded solver is built")
+        return bemretmirror(p, op)
     if sim == "stat":
         return bemstatiter(p, op) if op.get("iter") else bemstat(p, op)
     if sim != "ret":
@@ -480,10 +628,94 @@ class planewavestat:
     extinction, scattering, absorption = ext, sca, abs
 
 
+class planewaveretmirror:
+    """Plane wave for a comparticlemirror (Simulation/retarded/
+    @planewaveretmirror): pol in the xy plane, dir along +z or -z.  The
+    potential holds two excitations of the symmetry part, pol x and pol y,
+    each for dir +z and -z, with their symmetry values (potential.m:16-41);
+    ``full(sig)`` combines their solutions on the full particle for the
+    requested pol/dir (full.m:22-64), where the cross sections are taken."""
+
+    # potential.m:27-40: symmetry keys of the x, y, z components
+    _KEYS = {"x": (("+", "-", "-"), ("-", "+", "+")), "y": (("+", "-", "+"), ("-", "+", "-")),
+             "xy": (("++", "--", "-+"), ("--", "++", "+-"))}
+
+    def __init__(self, pol, dir, op=None, medium=1, pinfty=None):
+        pol = np.atleast_2d(np.asarray(pol, dtype=np.float64))
+        dr = np.atleast_2d(np.asarray(dir, dtype=np.float64))
+        if np.any(pol[:, 2] != 0) or np.any(dr[:, :2] != 0) or np.any(np.abs(dr[:, 2]) != 1):
+            raise ValueError("planewaveretmirror: pol must lie in the xy plane and dir be +z or -z")
+        self.pol, self.dir = pol, dr
+        self.exc = planewaveret(pol, dr, op, medium, pinfty)
+        self.medium = medium
+
+    def __call__(self, p, enei):
+        pf = p.full()
+        pot = compstructmirror(p, enei, self.full)
+        dr = np.asfortranarray([[0, 0, 1.0], [0, 0, -1.0]])
+        for i, keys in enumerate(self._KEYS[p.sym]):
+            pol = np.zeros((2, 3))
+            pol[:, i] = 1.0
+            pol = np.asfortranarray(pol)
+            a2 = dev.colmajor((pf.n, 3, 2))
+            a2p = dev.colmajor((pf.n, 3, 2))
+            _lib.call("mnp_planewave_potential", pf.green.handle, float(enei),
+                      _lib.MnpComplex.of(pf.eps[self.medium - 1](enei)[0]), 2, pol.ctypes.data_as(C.c_void_p),
+                      dr.ctypes.data_as(C.c_void_p), dev.ptr(dev.base(a2)), dev.ptr(dev.base(a2p)))
+            # the symmetry part is the first n faces of the full particle
+            part = {k: dev.base(v)[..., :p.n].contiguous().permute(2, 1, 0) for k, v in (("a2", a2), ("a2p", a2p))}
+            pot.val.append(compstruct(p, enei, symval=p.symvalue(list(keys)), **part))
+        return pot
+
+    potential = __call__
+
+    def full(self, sig):
+        """Fields on the full particle for this plane wave's pol and dir."""
+        p = sig.p
+        pf = p.full()
+        J, n, npol = len(p.symtable), p.n, len(self.pol)
+        w = np.ascontiguousarray(self.pol[:, :2], dtype=np.float64)
+        src = np.ascontiguousarray((self.dir[:, 2] < 0).astype(np.int32))
+        out = {}
+        v1, v2 = sig.val
+        for key in ("sig1", "sig2", "h1", "h2"):
+            if v1.get(key) is None:
+                continue
+            ncomp = 1 if key.startswith("sig") else 3
+            rows = [v.symval[-1:] if ncomp == 1 else v.symval for v in (v1, v2)]
+            s1, s2 = (np.ascontiguousarray(r.T, dtype=np.int32) for r in rows)  # J x ncomp
+            o = dev.colmajor((J * n, npol)) if ncomp == 1 else dev.colmajor((J * n, 3, npol))
+            ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+            _lib.call("mnp_mirror_full", n, J, ncomp, 2, npol, ip(s1), ip(s2),
+                      w.ctypes.data_as(C.POINTER(C.c_double)), ip(src), dev.ptr(dev.base(v1[key])),
+                      dev.ptr(dev.base(v2[key])), dev.ptr(dev.base(o)))
+            out[key] = o
+        return compstruct(pf, sig.enei, **out)
+
+    def cross_sections(self, sig):
+        return self.exc.cross_sections(self.full(sig))
+
+    def ext(self, sig):
+        return self.cross_sections(sig)[0]
+
+    def sca(self, sig):
+        return self.cross_sections(sig)[1]
+
+    def abs(self, sig):
+        return self.cross_sections(sig)[2]
+
+    extinction, scattering, absorption = ext, sca, abs
+
+
 def planewave(pol, dir=None, op=None, **kw):
-    """Base/planewave.m: planewaveret for op.sim == 'ret', planewavestat for 'stat'."""
+    """Base/planewave.m: planewaveret for op.sim == 'ret', planewavestat for
+    'stat', planewaveretmirror when op.sym is set."""
     if op is not None and op.get("sim", "ret") == "stat":
+        if op.get("sym"):
+            raise NotImplementedError("mirror symmetry: only the retarded plane wave is built")
         return planewavestat(pol, op, **kw)
+    if op is not None and op.get("sym"):
+        return planewaveretmirror(pol, dir, op, **kw)
     return planewaveret(pol, dir, op, **kw)
 
 

@@ -476,16 +476,16 @@ __global__ void k_closed_apply(int64_t n, Geo geo, Pairs pr, int64_t nnz, const 
 }
 
 // Dense G(k), F(k) +- 2 pi delta (eval1.m:24-40, :110-124, :140-143) for
-// all (i, j), column-major with leading dimension ld.  Each thread writes
+// rows i < nrows and all columns j, column-major with leading dimension ld.  Each thread writes
 // one element of up to two double-precision matrices and up to two
 // single-precision mirrors (the mixed chain's inputs); NULL outputs are
 // skipped.  Refined elements are overwritten by k_assemble_refined.
 template <class TG>
-__global__ void __launch_bounds__(256) k_assemble(int64_t n, Geo geo, double2 k, double sgn2pi,
+__global__ void __launch_bounds__(256) k_assemble(int64_t n, int64_t nrows, Geo geo, double2 k, double sgn2pi,
                                                   TG outG, TG outH, int64_t ld) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t j = blockIdx.y;
-  if (i >= n) return;
+  if (i >= nrows) return;
   const double dx = geo.pos[i] - geo.pos[j];
   const double dy = geo.pos[n + i] - geo.pos[n + j];
   const double dz = geo.pos[2 * n + i] - geo.pos[2 * n + j];
@@ -510,16 +510,17 @@ __global__ void __launch_bounds__(256) k_assemble(int64_t n, Geo geo, double2 k,
   }
 }
 
-// Overwrites refined elements: G = (sum_n g_n (ik)^n) e^{ikd},
+// Overwrites refined elements in rows i < nrows: G = (sum_n g_n (ik)^n) e^{ikd},
 // F = (n_i . sum_n f_n (ik)^n) e^{ikd}, H = F +- 2 pi on coincident pairs.
 template <class TG>
-__global__ void __launch_bounds__(256) k_assemble_refined(int64_t nnz, int64_t n, Geo geo, Pairs pr,
-                                                          int order, const double* __restrict__ gt,
+__global__ void __launch_bounds__(256) k_assemble_refined(int64_t nnz, int64_t n, int64_t nrows, Geo geo,
+                                                          Pairs pr, int order, const double* __restrict__ gt,
                                                           const double* __restrict__ ft, double2 k,
                                                           double sgn2pi, TG outG, TG outH, int64_t ld) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= nnz) return;
   const int64_t i = pr.row[p], j = pr.col[p];
+  if (i >= nrows) return;
   const double dx = geo.pos[i] - geo.pos[j];
   const double dy = geo.pos[n + i] - geo.pos[n + j];
   const double dz = geo.pos[2 * n + i] - geo.pos[2 * n + j];
@@ -547,16 +548,22 @@ __global__ void __launch_bounds__(256) k_assemble_refined(int64_t nnz, int64_t n
 
 // ------------------------------------------------------------- host side
 void GreenRet::assemble(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t ld) const {
+  assemble_rows(k, sgn2pi, g, h, n, ld);
+}
+
+void GreenRet::assemble_rows(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t nrows, int64_t ld) const {
   if (!g.any() && !h.any()) return;
-  dim3 grid((unsigned)cdiv_up(n, 256), (unsigned)n);
-  const double bytes = (double)n * n *
+  MNP_CHECK(nrows >= 1 && nrows <= n && ld >= nrows, MNP_ERR_ARG, "greenret: bad row range");
+  MNP_CHECK(n <= 65535, MNP_ERR_ARG, "greenret: dense assembly supports at most 65535 faces");
+  dim3 grid((unsigned)cdiv_up(nrows, 256), (unsigned)n);
+  const double bytes = (double)nrows * n *
                        ((g.d ? 16 : 0) + (g.f ? 8 : 0) + (h.d ? 16 : 0) + (h.f ? 8 : 0));
   TimerScope ts(MNP_TIMER_ASSEMBLE, bytes, bytes);
-  hipLaunchKernelGGL(k_assemble<OutPair>, grid, dim3(256), 0, stream(), n, geo, k, sgn2pi, g, h, ld);
+  hipLaunchKernelGGL(k_assemble<OutPair>, grid, dim3(256), 0, stream(), n, nrows, geo, k, sgn2pi, g, h, ld);
   MNP_HIP(hipGetLastError());
   if (nnz) {
     hipLaunchKernelGGL(k_assemble_refined<OutPair>, dim3((unsigned)cdiv_up(nnz, 256)), dim3(256), 0,
-                       stream(), nnz, n, geo, pairs, order, gtab, ftab, k, sgn2pi, g, h, ld);
+                       stream(), nnz, n, nrows, geo, pairs, order, gtab, ftab, k, sgn2pi, g, h, ld);
     MNP_HIP(hipGetLastError());
   }
 }

@@ -1,0 +1,273 @@
+// The dense retarded BEM chain shared by the dense solvers (BEM/@bemret and,
+// per symmetry value, BEM/@bemretmirror): chain state, its factorisation and
+// mldivide on the factors.  See bemret.hip for the algebra.
+#pragma once
+
+#include <cmath>
+#include <string>
+
+#include "mnp_greenret.hpp"
+#include "mnp_linalg.hpp"
+
+namespace mnp {
+
+// k = k0 sqrt(eps), principal root (Material/@epsdrude/subsref.m:24)
+inline double2 wavenumber(double k0, double2 e) {
+  const double r = std::hypot(e.x, e.y);
+  double sr = std::sqrt(0.5 * (r + e.x)), si = std::sqrt(0.5 * (r - e.x));
+  if (e.y < 0) si = -si;
+  return mk<double>(k0 * sr, k0 * si);
+}
+
+// Host-side helpers of the dense solvers (bemret.hip).
+double dev_norm2(int64_t len, const double2* v, Workspace& ws, int slot);  // |v|, synchronises
+void launch_excitation(int64_t n, int64_t p, const double2* const in[8], double2* b, const double* nvec,
+                       double k, double2 e1, double2 e2);
+void launch_residual(int64_t n, int64_t p, const double2* b, const double2* GX1, const double2* GX2,
+                     const double2* HX1, const double2* HX2, double2* r, const double* nvec, double k,
+                     double2 e1, double2 e2);
+void launch_unpack(int64_t n, int64_t p, const double2* x, double2* sig1, double2* h1, double2* sig2,
+                   double2* h2);
+
+// ---------------------------------------------------------- chain state
+template <class T>
+struct Chain {
+  int64_t n = 0;
+  bool exact_pivoting = true;  // FP64: factors must equal partial pivoting's
+  cplx<T>*G1 = nullptr, *G2 = nullptr, *S1 = nullptr, *S2 = nullptr, *D = nullptr, *Di = nullptr;
+  cplx<T>* backup = nullptr;   // copy of a matrix while it is factorised without pivoting
+  cplx<T>* inv = nullptr;      // 32-block inverses: 4 factors x (L, U)
+  cplx<T>* invnb = nullptr;    // 256-block inverses: 4 factors x (L, U)
+  int32_t *piv = nullptr, *sig[4] = {nullptr, nullptr, nullptr, nullptr};
+  LU<T> f[4];                  // G1, G2, Sigma1 - Sigma2, Sigma
+  int fallbacks = 0;           // factorisations redone with partial pivoting
+  double2 eps1{}, eps2{};
+  double k0 = 0;
+  Workspace ws;
+
+  // sigma_only: only the buffers of the Sigma factorisation (S2 holds Sigma,
+  // factor slot 3), for Sigma matrices assembled from other chains' parts.
+  explicit Chain(int64_t n_, bool exact, bool sigma_only = false) : n(n_), exact_pivoting(exact) {
+    const int64_t nn = n * n;
+    if (!sigma_only) {
+      G1 = dev_alloc<cplx<T>>(nn);
+      G2 = dev_alloc<cplx<T>>(nn);
+      S1 = dev_alloc<cplx<T>>(nn);
+      D = dev_alloc<cplx<T>>(nn);
+      Di = dev_alloc<cplx<T>>(nn);
+    }
+    S2 = dev_alloc<cplx<T>>(nn);
+    backup = dev_alloc<cplx<T>>(nn);
+    inv = dev_alloc<cplx<T>>(8 * inv_blocks(n));
+    invnb = dev_alloc<cplx<T>>(8 * inv_blocks_nb(n));
+    piv = dev_alloc<int32_t>(n);
+    for (auto& q : sig) q = dev_alloc<int32_t>(n);
+  }
+  ~Chain() {
+    void* ps[] = {G1, G2, S1, S2, D, Di, backup, inv, invnb, piv, sig[0], sig[1], sig[2], sig[3]};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
+  MatView<T> v(cplx<T>* p) const { return MatView<T>{p, n}; }
+
+  // Factorises A in place as factor slot `s`.  First without row exchanges
+  // (no grid-wide pivot search); when partial pivoting would have exchanged
+  // rows and exact pivoting is required, or on a zero pivot, the matrix is
+  // restored and factorised again with partial pivoting.
+  const LU<T>& factor(int s, cplx<T>* A, const char* what) {
+    cplx<T>* Linv = inv + (2 * s) * inv_blocks(n);
+    cplx<T>* Uinv = Linv + inv_blocks(n);
+    copy_mat<T>(n, n, v(A), v(backup));
+    int swap = 0;
+    int info = getrf<T>(n, n, v(A), piv, Linv, Uinv, PIVOT_NONE, &swap, ws);
+    const int32_t* sigma = nullptr;
+    if (info != 0 || (swap && exact_pivoting)) {
+      ++fallbacks;
+      copy_mat<T>(n, n, v(backup), v(A));
+      info = getrf<T>(n, n, v(A), piv, Linv, Uinv, PIVOT_PARTIAL, nullptr, ws);
+      perm_from_pivots(n, piv, sig[s], ws);
+      sigma = sig[s];
+    }
+    MNP_CHECK(info == 0, MNP_ERR_SINGULAR,
+              std::string("bemret: singular ") + what + " (zero pivot " + std::to_string(info) + ")");
+    cplx<T>* LinvNB = invnb + (2 * s) * inv_blocks_nb(n);
+    cplx<T>* UinvNB = LinvNB + inv_blocks_nb(n);
+    block_inverses_nb<T>(n, v(A), Linv, Uinv, LinvNB, UinvNB, ws);
+    f[s] = LU<T>{v(A), Linv, Uinv, sigma, LinvNB, UinvNB};
+    return f[s];
+  }
+};
+
+// Sigma = e1 S1 - e2 S2 + c (Di o n n^T), written over S2.
+template <class T>
+__global__ void k_sigma(int64_t n, const cplx<T>* S1, cplx<T>* S2, const cplx<T>* Di,
+                        const double* nvec, double2 e1, double2 e2, double2 c) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (i >= n || j >= n) return;
+  const int64_t idx = j * n + i;
+  const double nn = nvec[i] * nvec[j] + nvec[n + i] * nvec[n + j] + nvec[2 * n + i] * nvec[2 * n + j];
+  const cplx<T> s1 = S1[idx], s2 = S2[idx], di = Di[idx];
+  const double2 a = cmul<double>(e1, mk<double>(s1.x, s1.y));
+  const double2 b = cmul<double>(e2, mk<double>(s2.x, s2.y));
+  const double2 d = cmul<double>(c, mk<double>(di.x * nn, di.y * nn));
+  S2[idx] = mk<T>((T)(a.x - b.x + d.x), (T)(a.y - b.y + d.y));
+}
+template <class T>
+__global__ void k_sub(int64_t n, const cplx<T>* A, const cplx<T>* B, cplx<T>* C) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y + (int64_t)blockIdx.z * 65535;
+  if (i >= n || j >= n) return;
+  const int64_t idx = j * n + i;
+  C[idx] = csub<T>(A[idx], B[idx]);
+}
+inline dim3 gridnn(int64_t m, int64_t n) {
+  return dim3((unsigned)cdiv_up(m, 256), (unsigned)std::min<int64_t>(n, 65535),
+              (unsigned)cdiv_up(n, 65535));
+}
+
+// Factorises the parts of the chain that precede Sigma: LU(G1), Sigma1 =
+// H1 G1^{-1} (over H1 in S1), LU(G2), Sigma2 = H2 G2^{-1} (over H2 in S2),
+// LU(Sigma1 - Sigma2) and Deltai.  On entry G1, G2, S1 (= H1), S2 (= H2)
+// hold the assembled matrices.
+template <class T>
+void chain_sector(Chain<T>& c) {
+  const int64_t n = c.n;
+  c.fallbacks = 0;
+  getrs_right<T>(n, n, c.factor(0, c.G1, "G1"), c.v(c.S1), c.ws);  // Sigma1 = H1 G1^{-1}
+  getrs_right<T>(n, n, c.factor(1, c.G2, "G2"), c.v(c.S2), c.ws);  // Sigma2 = H2 G2^{-1}
+  hipLaunchKernelGGL(k_sub<T>, gridnn(n, n), dim3(256), 0, stream(), n, c.S1, c.S2, c.D);
+  MNP_HIP(hipGetLastError());
+  getri<T>(n, c.factor(2, c.D, "Sigma1 - Sigma2"), c.v(c.Di), c.ws);  // Deltai
+}
+
+// Factorises the chain (chain_sector, then Sigma over S2 and its LU).
+template <class T>
+void chain_build(Chain<T>& c, const GreenRet& g) {
+  const int64_t n = c.n;
+  chain_sector(c);
+  const double2 L = csub<double>(c.eps1, c.eps2);
+  const double2 cc = cscale<double>(cmul<double>(L, L), c.k0 * c.k0);
+  hipLaunchKernelGGL(k_sigma<T>, gridnn(n, n), dim3(256), 0, stream(), n, c.S1, c.S2, c.Di,
+                     g.geo.nvec, c.eps1, c.eps2, cc);
+  MNP_HIP(hipGetLastError());
+  c.factor(3, c.S2, "Sigma");
+}
+
+// ------------------------------------------------------------- mldivide
+// Vectors are n x (8 p) blocks.  b = [phi (p) | a (3p) | De (p) | alpha (3p)],
+// x = [sig1 (p) | h1 (3p) | sig2 (p) | h2 (3p)]; vector columns are ordered
+// component-fastest (c + 3 q), as Matlab's n x 3 x npol arrays.
+
+// alpha <- alpha - Y_a + i k L1 n (x) phi ;  De <- De - L1 Y_phi + i k L1 n . a
+template <class T>
+__global__ void k_mld1(int64_t n, int64_t p, const cplx<T>* b, const cplx<T>* Y, cplx<T>* alpha,
+                       cplx<T>* De, const double* nvec, double k, double2 L1) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  const cplx<T>* phi = b;
+  const cplx<T>* a = b + n * p;
+  const cplx<T>* De0 = b + 4 * n * p;
+  const cplx<T>* al0 = b + 5 * n * p;
+  const double2 ikL1 = cmul<double>(mk<double>(0, k), L1);
+  const cplx<T> ph = phi[q * n + i];
+  const double2 ikL1phi = cmul<double>(ikL1, mk<double>(ph.x, ph.y));
+  double2 na = mk<double>(0, 0);
+  for (int c = 0; c < 3; ++c) {
+    const int64_t col = c + 3 * q;
+    const double nc = nvec[c * n + i];
+    const cplx<T> ya = Y[col * n + i], a0 = al0[col * n + i], ac = a[col * n + i];
+    alpha[col * n + i] = mk<T>((T)(a0.x - ya.x + nc * ikL1phi.x), (T)(a0.y - ya.y + nc * ikL1phi.y));
+    na.x += nc * ac.x;
+    na.y += nc * ac.y;
+  }
+  const cplx<T> yp = Y[(3 * p + q) * n + i], d0 = De0[q * n + i];
+  const double2 l1y = cmul<double>(L1, mk<double>(yp.x, yp.y));
+  const double2 t = cmul<double>(ikL1, na);
+  De[q * n + i] = mk<T>((T)(d0.x - l1y.x + t.x), (T)(d0.y - l1y.y + t.y));
+}
+// rhs = De + i k (L1 - L2) n . t
+template <class T>
+__global__ void k_mld2(int64_t n, int64_t p, const cplx<T>* De, const cplx<T>* t, cplx<T>* rhs,
+                       const double* nvec, double2 ikdL) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  double2 s = mk<double>(0, 0);
+  for (int c = 0; c < 3; ++c) {
+    const cplx<T> v = t[(c + 3 * q) * n + i];
+    s.x += nvec[c * n + i] * v.x;
+    s.y += nvec[c * n + i] * v.y;
+  }
+  const double2 u = cmul<double>(ikdL, s);
+  const cplx<T> d = De[q * n + i];
+  rhs[q * n + i] = mk<T>((T)(d.x + u.x), (T)(d.y + u.y));
+}
+// u = i k (L1 - L2) n (x) sig2 + alpha
+template <class T>
+__global__ void k_mld3(int64_t n, int64_t p, const cplx<T>* sig2, const cplx<T>* alpha, cplx<T>* u,
+                       const double* nvec, double2 ikdL) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  const cplx<T> s = sig2[q * n + i];
+  const double2 w = cmul<double>(ikdL, mk<double>(s.x, s.y));
+  for (int c = 0; c < 3; ++c) {
+    const int64_t col = c + 3 * q;
+    const cplx<T> a = alpha[col * n + i];
+    const double nc = nvec[c * n + i];
+    u[col * n + i] = mk<T>((T)(a.x + nc * w.x), (T)(a.y + nc * w.y));
+  }
+}
+// V1 = [sig2 + phi | h2 + a] ; V2 = [sig2 | h2]   (n x 4p each)
+template <class T>
+__global__ void k_mld4(int64_t n, int64_t p, const cplx<T>* b, const cplx<T>* sig2,
+                       const cplx<T>* h2, cplx<T>* V1, cplx<T>* V2) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t col = blockIdx.y;  // 0 .. 4p
+  if (i >= n) return;
+  const cplx<T> src = col < p ? sig2[col * n + i] : h2[(col - p) * n + i];
+  const cplx<T> add = b[col * n + i];  // phi then a
+  V1[col * n + i] = cadd<T>(src, add);
+  V2[col * n + i] = src;
+}
+
+// x = M^{-1} b for the chain c (precision T), b and x n x 8p in T.
+template <class T>
+void chain_apply(Chain<T>& c, const GreenRet& g, int64_t p, const cplx<T>* b, cplx<T>* x,
+                        cplx<T>* tmp) {
+  const int64_t n = c.n;
+  const double k = c.k0;
+  const double2 L1 = c.eps1, L2 = c.eps2;
+  const double2 ikdL = cmul<double>(mk<double>(0, k), csub<double>(L1, L2));
+  // tmp layout: Y (4p) | alpha (3p) | De (p) | t (3p) | rhs/sig2 (p) | u (3p) | h2 (3p)
+  cplx<T>* Y = tmp;
+  cplx<T>* alpha = Y + 4 * p * n;
+  cplx<T>* De = alpha + 3 * p * n;
+  cplx<T>* t = De + p * n;
+  cplx<T>* s2 = t + 3 * p * n;
+  cplx<T>* u = s2 + p * n;
+  cplx<T>* h2 = u + 3 * p * n;
+  const double2 one = {1, 0}, zero = {0, 0};
+  // Y = Sigma1 [a | phi]: a occupies b columns p..4p, phi columns 0..p
+  gemm<T>(n, 3 * p, n, one, c.v(c.S1), MatView<T>{const_cast<cplx<T>*>(b) + n * p, n}, zero,
+          MatView<T>{Y, n});
+  gemm<T>(n, p, n, one, c.v(c.S1), MatView<T>{const_cast<cplx<T>*>(b), n}, zero,
+          MatView<T>{Y + 3 * p * n, n});
+  const dim3 gp((unsigned)cdiv_up(n, 256), (unsigned)p);
+  hipLaunchKernelGGL(k_mld1<T>, gp, dim3(256), 0, stream(), n, p, b, Y, alpha, De, g.geo.nvec, k, L1);
+  gemm<T>(n, 3 * p, n, one, c.v(c.Di), MatView<T>{alpha, n}, zero, MatView<T>{t, n});
+  hipLaunchKernelGGL(k_mld2<T>, gp, dim3(256), 0, stream(), n, p, De, t, s2, g.geo.nvec, ikdL);
+  getrs<T>(n, p, c.f[3], MatView<T>{s2, n}, c.ws);  // sig2 = Sigma^{-1} rhs
+  hipLaunchKernelGGL(k_mld3<T>, gp, dim3(256), 0, stream(), n, p, s2, alpha, u, g.geo.nvec, ikdL);
+  gemm<T>(n, 3 * p, n, one, c.v(c.Di), MatView<T>{u, n}, zero, MatView<T>{h2, n});
+  // V1 -> x[0:4p] (sig1 | h1), V2 -> x[4p:8p] (sig2 | h2)
+  hipLaunchKernelGGL(k_mld4<T>, dim3((unsigned)cdiv_up(n, 256), (unsigned)(4 * p)), dim3(256), 0,
+                     stream(), n, p, b, s2, h2, x, x + 4 * p * n);
+  MNP_HIP(hipGetLastError());
+  getrs<T>(n, 4 * p, c.f[0], MatView<T>{x, n}, c.ws);
+  getrs<T>(n, 4 * p, c.f[1], MatView<T>{x + 4 * p * n, n}, c.ws);
+}
+
+}  // namespace mnp

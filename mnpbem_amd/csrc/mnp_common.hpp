@@ -105,6 +105,12 @@ struct TimerScope {
 };
 
 // Converts a user-facing complex scalar into the device type.
+// "1.234e-05": residuals and tolerances in error messages
+inline std::string sci(double v) {
+  char b[32];
+  std::snprintf(b, sizeof b, "%.3e", v);
+  return b;
+}
 inline double2 to_d2(mnp_complex z) { return mk<double>(z.re, z.im); }
 
 }  // namespace mnp

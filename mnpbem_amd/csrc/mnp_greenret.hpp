@@ -69,6 +69,8 @@ struct GreenRet {
 
   // G(k) into g and F(k) + sgn2pi * I into h (either may be empty).
   void assemble(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t ld) const;
+  // Rows 0 .. nrows-1 of the same matrices (nrows x n, leading dimension ld).
+  void assemble_rows(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t nrows, int64_t ld) const;
 };
 
 }  // namespace mnp

@@ -73,6 +73,73 @@ def ellipsoid(level, semi_axes):
     return verts, orient_outward(verts, faces, center=np.zeros(3))
 
 
+def ellipsoid_segment(nphi, ntheta, semi_axes, phi_min=0.0, phi_max=np.pi / 2):
+    """Flat triangulation of the ellipsoid surface segment phi_min <= phi <=
+    phi_max (all theta): (a sin t cos p, b sin t sin p, c cos t) on a grid of
+    nphi azimuthal and ntheta polar intervals, quads split along one diagonal
+    and fans at the poles, 2 nphi (ntheta - 1) faces, oriented outward.
+    Grid lines with cos(phi) = 0 or sin(phi) = 0 get exact zeros, so that the
+    segments phi in [0, pi/2] ('xy'), [0, pi] ('y') and [-pi/2, pi/2] ('x')
+    are the parts that the mirror symmetries complete to the closed surface
+    (cf. Particles/particleshapes/trispheresegment.m, which grids phi and
+    theta the same way)."""
+    a, b, c = (float(x) for x in semi_axes)
+    ph = np.linspace(float(phi_min), float(phi_max), int(nphi) + 1)
+    th = np.linspace(0.0, np.pi, int(ntheta) + 1)
+    cp, sp = np.cos(ph), np.sin(ph)
+    cp[np.abs(cp) < 1e-12] = 0.0
+    sp[np.abs(sp) < 1e-12] = 0.0
+    ring = np.concatenate([np.column_stack([a * np.sin(t) * cp, b * np.sin(t) * sp, np.full_like(ph, c * np.cos(t))])
+                           for t in th[1:-1]])
+    verts = np.vstack([[0.0, 0.0, c], ring, [0.0, 0.0, -c]])
+    m = len(ph)
+    top, bot = 0, len(verts) - 1
+    idx = lambda r, j: 1 + r * m + j
+    faces = []
+    for j in range(int(nphi)):
+        faces.append([top, idx(0, j), idx(0, j + 1)])
+        for r in range(int(ntheta) - 2):
+            faces.append([idx(r, j), idx(r + 1, j), idx(r + 1, j + 1)])
+            faces.append([idx(r, j), idx(r + 1, j + 1), idx(r, j + 1)])
+        faces.append([idx(int(ntheta) - 2, j), bot, idx(int(ntheta) - 2, j + 1)])
+    return verts, orient_outward(verts, np.array(faces, dtype=np.int64), center=np.zeros(3))
+
+
+_OCTANTS = [(sx, sy, sz) for sx in (1, -1) for sy in (1, -1) for sz in (1, -1)]
+
+
+def ellipsoid_octants(m, semi_axes, sym=None):
+    """Ellipsoid meshed octant by octant: each octant's spherical triangle,
+    corners (+-1, 0, 0), (0, +-1, 0), (0, 0, +-1), is cut into m^2 flat
+    triangles on the barycentric grid of step 1/m, projected radially onto
+    the unit sphere and scaled to the semi-axes (nm).  The octant edges lie
+    in the coordinate planes, so the part x, y >= 0 (sym 'xy', 2 m^2
+    faces), x >= 0 ('x') or y >= 0 ('y', 4 m^2 faces each) is what the
+    mirror copies complete to the closed surface; sym None gives the whole
+    surface (8 m^2 faces).  Faces oriented outward."""
+    keep = {None: lambda o: True, "xy": lambda o: o[0] > 0 and o[1] > 0, "x": lambda o: o[0] > 0,
+            "y": lambda o: o[1] > 0}[sym]
+    m = int(m)
+    verts, faces = [], []
+    for o in _OCTANTS:
+        if not keep(o):
+            continue
+        A, Bv, Cv = np.diag(np.asarray(o, dtype=np.float64))
+        index = {}
+        for i in range(m + 1):
+            for j in range(m + 1 - i):
+                pnt = (i * A + j * Bv + (m - i - j) * Cv) / m
+                index[(i, j)] = len(verts)
+                verts.append(pnt / np.linalg.norm(pnt))
+        for i in range(m):
+            for j in range(m - i):
+                faces.append([index[(i, j)], index[(i + 1, j)], index[(i, j + 1)]])
+                if i + j < m - 1:
+                    faces.append([index[(i + 1, j)], index[(i + 1, j + 1)], index[(i, j + 1)]])
+    verts = np.asarray(verts) * np.asarray(semi_axes, dtype=np.float64)[None, :]
+    return verts, orient_outward(verts, np.array(faces, dtype=np.int64), center=np.zeros(3))
+
+
 def sphere256_points():
     """The 256-point unit-sphere point set behind MNPBEM's ``trisphere(256)`` (float32)."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "sphere256.npy")
