@@ -233,6 +233,45 @@ def run_mirror(steps=3):
             "max_rel_err_full_particle_vs_mirror_fp64": err["full_mixed"]}
 
 
+def run_substrate(level=5, steps=3):
+    """Quasistatic solver above a glass substrate (bemstatlayer): the headline
+    ellipsoid (20,480 faces) with its lowest point 5 nm above the interface,
+    air above and eps = 2.25 below, pol x at normal incidence and a TM wave at
+    0.6 rad; mixed, mixed_bf16 and FP64."""
+    import torch
+
+    from mnpbem_amd import bem as B, shapes
+    from mnpbem_amd.layer import layerstructure
+
+    v, f = shapes.ellipsoid(level, (10.0, 10.0, 40.0))
+    v = v + np.array([0, 0, 45.0])
+    epstab = [B.epsconst(1.0), B.epsdrude(*DRUDE), B.epsconst(2.25)]
+    op = B.bemoptions(sim="stat", layer=layerstructure(epstab, [1, 3], 0.0))
+    p = B.comparticle(epstab, [(v, f)], [2, 1], 1, op)
+    th = 0.6
+    exc = B.planewave([[1.0, 0, 0], [np.cos(th), 0, np.sin(th)]], [[0, 0, -1.0], [np.sin(th), 0, -np.cos(th)]], op)
+    lams = np.linspace(500.0, 1000.0, 64)
+    res = {}
+    for prec in ("mixed", "mixed_bf16", "fp64"):
+        bem = B.bemsolver(p, op, precision=prec)
+        exc.ext(bem.solve(exc(p, lams[0])))  # warm-up (also fills the image Green function)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        cs = [np.array(exc.cross_sections(bem.solve(exc(p, lam)))) for lam in lams[1:1 + steps]]
+        torch.cuda.synchronize()
+        res[prec] = (steps / (time.perf_counter() - t), cs)
+        del bem
+    err = {k: max(float(np.max(np.abs(a / b - 1))) for a, b in zip(res[k][1], res["fp64"][1]))
+           for k in ("mixed", "mixed_bf16")}
+    log(f"[bench] substrate: mixed {res['mixed'][0]:.2f}/s, mixed_bf16 {res['mixed_bf16'][0]:.2f}/s, "
+        f"fp64 {res['fp64'][0]:.2f}/s; rel err vs FP64 {err}")
+    return {"workload": f"quasistatic bemstatlayer: ellipsoid 10x10x40 nm ({len(f)} faces) 5 nm above "
+                        "eps 2.25, air above, pol x normal incidence and TM at 0.6 rad",
+            "wavelengths_per_s_mixed": res["mixed"][0], "wavelengths_per_s_mixed_bf16": res["mixed_bf16"][0],
+            "wavelengths_per_s_fp64": res["fp64"][0],
+            "max_rel_err_vs_fp64": err["mixed"], "max_rel_err_mixed_bf16_vs_fp64": err["mixed_bf16"]}
+
+
 def run_multi_particle(per_particle=8):
     """The seeded many-particle sweep (12 Drude particles, 320-5,120 faces, in
     vacuum): per_particle of its 128 wavelengths (every 16th) per particle
@@ -433,6 +472,7 @@ def main():
     ap.add_argument("--no-quasistatic", action="store_true", help="skip the quasistatic measurement")
     ap.add_argument("--no-multi-particle", action="store_true", help="skip the many-particle sweep measurement")
     ap.add_argument("--no-mirror", action="store_true", help="skip the mirror-symmetry measurement")
+    ap.add_argument("--no-substrate", action="store_true", help="skip the substrate measurement")
     ap.add_argument("--config4-only", action="store_true",
                     help="only the H-matrix dipole measurement (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -532,6 +572,8 @@ def main():
         out["quasistatic"] = run_quasistatic(a.level)
     if not a.no_mirror:
         out["mirror"] = run_mirror()
+    if not a.no_substrate:
+        out["substrate"] = run_substrate(a.level)
     if not a.no_multi_particle:
         out["multi_particle"] = run_multi_particle()
     if not a.no_config4:

@@ -380,6 +380,32 @@ int mnp_planewavestat_cross_sections(mnp_greenret_t g, double enei, mnp_complex 
                                      int64_t npol, const double* pol, const void* sig, double* ext,
                                      double* sca, double* abs_);
 
+/* ---------------------------------------- quasistatic solver, substrate
+ * Replaces bemsolver(p, op) with op.sim = 'stat' and op.layer (one
+ * interface) -> bemstatlayer (BEM/@bemstatlayer) for a particle above the
+ * interface (no face in it).  g is the particle's Green function (with the
+ * closed-surface correction); gimage the Green function of the particle
+ * followed by its image, the particle flipped at the interface plane
+ * (compgreenstatlayer/init.m:21-24: shift, flip(.., 3), shift back), built
+ * with closed = 0.  The reflected F is gimage's rows of the particle against
+ * the image's columns (compgreenstat(p1, p2r), init.m:26-29), assembled once;
+ * initmat forms F = F_direct + f2 F_reflected with f2 = -(eps_down - eps_up)
+ * / (eps_down + eps_up) (compgreenstatlayer/eval.m:16-38) and factorises
+ * eps1 H1 - eps2 H2 = (eps1 - eps2)(Lambda I + F) (bemstatlayer/subsref.m:
+ * 28-32).  The handle is a mnp_bemstat_t: solve, stats and destroy are
+ * mnp_bemstat_solve (sig = mat phip, mldivide.m:13), _stats, _destroy. */
+int mnp_bemstatlayer_create(mnp_greenret_t g, mnp_greenret_t gimage, int solver, mnp_bemstat_t* out);
+int mnp_bemstatlayer_initmat(mnp_bemstat_t bem, double enei, mnp_complex eps1, mnp_complex eps2,
+                             mnp_complex eps_up, mnp_complex eps_down);
+/* phi' = -nvec . e for the field e (host npol x 3, npol <= 64) at the faces
+ * (@planewavestatlayer/potential.m:13-14 with the field of field.m for faces
+ * above the interface). */
+int mnp_planewavestatlayer_potential(mnp_greenret_t g, int64_t npol, const mnp_complex* e, void* phip);
+/* Dipole moment sum_i area_i pos_i sig_i per column (host npol x 3), FP64
+ * (@planewavestatlayer/extinction.m:13, scattering.m:12). */
+int mnp_stat_dipole_moment(mnp_greenret_t g, int64_t npol, const void* sig, mnp_complex* dip);
+
+
 /* ------------------------------------------------ dipoles and point fields
  * Dipoles at points pts (host npt x 3, column-major) with moments dip (host
  * npt x 3 x ndip); column q = ipt + npt * idip of the outputs, as the

@@ -80,6 +80,10 @@ PROTOTYPES = {
     "mnp_bemretiter_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
     "mnp_bemretiter_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mnp_bemretiter_stats": (INT, [P, C.POINTER(INT), PD, C.POINTER(INT), PD, PD]),
+    "mnp_bemstatlayer_create": (INT, [P, P, INT, C.POINTER(P)]),
+    "mnp_bemstatlayer_initmat": (INT, [P, D, MnpComplex, MnpComplex, MnpComplex, MnpComplex]),
+    "mnp_planewavestatlayer_potential": (INT, [P, I64, P, P]),
+    "mnp_stat_dipole_moment": (INT, [P, I64, P, P]),
     "mnp_bemretmirror_create": (INT, [P, INT, INT, C.POINTER(INT), C.POINTER(P)]),
     "mnp_bemretmirror_destroy": (INT, [P]),
     "mnp_bemretmirror_initmat": (INT, [P, D, MnpComplex, MnpComplex]),

@@ -508,7 +508,14 @@ def bemsolver(p, op=None, **kw):
             raise NotImplementedError("mirror symmetry: only the dense retarded solver is built")
         return bemretmirror(p, op)
     if sim == "stat":
+        if op.get("layer") is not None:
+            if op.get("iter"):
+                raise NotImplementedError("substrate: only the dense quasistatic solver is built")
+            from .layer import bemstatlayer
+            return bemstatlayer(p, op)
         return bemstatiter(p, op) if op.get("iter") else bemstat(p, op)
+    if op.get("layer") is not None:
+        raise NotImplementedError("substrate: only the quasistatic solver is built")
     if sim != "ret":
         raise NotImplementedError(f"unknown simulation type {sim!r}")
     if op.get("iter"):
@@ -713,7 +720,12 @@ def planewave(pol, dir=None, op=None, **kw):
     if op is not None and op.get("sim", "ret") == "stat":
         if op.get("sym"):
             raise NotImplementedError("mirror symmetry: only the retarded plane wave is built")
+        if op.get("layer") is not None:
+            from .layer import planewavestatlayer
+            return planewavestatlayer(pol, dir, op, **kw)
         return planewavestat(pol, op, **kw)
+    if op is not None and op.get("layer") is not None:
+        raise NotImplementedError("substrate: only the quasistatic plane wave is built")
     if op is not None and op.get("sym"):
         return planewaveretmirror(pol, dir, op, **kw)
     return planewaveret(pol, dir, op, **kw)

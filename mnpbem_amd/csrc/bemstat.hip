@@ -143,11 +143,32 @@ struct StatFactor {
   }
 };
 
+// F = F0 + f2 Fr  (compgreenstatlayer/eval.m:33-38 for a particle above the interface)
+__global__ void k_axpby2(int64_t len, const double2* F0, const double2* Fr, double2 f2, double2* F) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < len) F[i] = cadd<double>(F0[i], cmul<double>(f2, Fr[i]));
+}
+// phi' = -nvec . e, e complex (npol x 3 in constant-size argument)
+constexpr int MNP_STAT_MAXPOL = 64;
+struct FieldArg {
+  double2 e[MNP_STAT_MAXPOL][3];
+};
+__global__ void k_pwstat_field(int64_t n, const double* nvec, FieldArg a, double2* phip) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  double2 v = mk<double>(0, 0);
+  for (int c = 0; c < 3; ++c) v = cadd<double>(v, cscale<double>(a.e[q][c], nvec[c * n + i]));
+  phip[q * n + i] = mk<double>(-v.x, -v.y);
+}
+
 struct BemStat {
   const GreenRet* g;
   int64_t n;
   int solver;
   double2* F = nullptr;  // static surface derivative, FP64, n x n
+  // substrate: F = F0 + f2 Fr per wavelength (F0 direct, Fr the reflected part)
+  double2 *F0 = nullptr, *Fr = nullptr;
   std::unique_ptr<StatFactor<double>> f64;
   std::unique_ptr<StatFactor<float>> f32;
   double2 lam{};
@@ -166,8 +187,38 @@ struct BemStat {
     else
       f32.reset(new StatFactor<float>(n, false));
   }
+  // Quasistatic solver above a substrate (BEM/@bemstatlayer): gimg is the
+  // Green function of [particle; its image in the interface] without the
+  // closed-surface correction; the reflected part is its rows of the
+  // particle against the image's columns.
+  BemStat(const GreenRet* g_, const GreenRet* gimg, int solver_) : BemStat(g_, solver_) {
+    F0 = F;
+    F = dev_alloc<double2>(n * n);
+    Fr = dev_alloc<double2>(n * n);
+    double2* B = dev_alloc<double2>(n * gimg->n);
+    try {
+      gimg->assemble_rows(mk<double>(0, 0), 0.0, OutPair{}, OutPair{B, nullptr}, n, n);
+      MNP_HIP(hipMemcpyAsync(Fr, B + n * n, sizeof(double2) * n * n, hipMemcpyDeviceToDevice, stream()));
+      MNP_HIP(hipStreamSynchronize(stream()));
+    } catch (...) {
+      (void)hipFree(B);
+      throw;
+    }
+    MNP_HIP(hipFree(B));
+  }
   ~BemStat() {
     if (F) (void)hipFree(F);
+    if (F0) (void)hipFree(F0);
+    if (Fr) (void)hipFree(Fr);
+  }
+
+  // F = F0 + f2 Fr, then initmat
+  void initmat_layer(double2 e1, double2 e2, double2 f2) {
+    MNP_CHECK(Fr, MNP_ERR_STATE, "bemstat: not created for a layer structure");
+    const int64_t len = n * n;
+    hipLaunchKernelGGL(k_axpby2, dim3((unsigned)cdiv_up(len, 256)), dim3(256), 0, stream(), len, F0, Fr, f2, F);
+    MNP_HIP(hipGetLastError());
+    initmat(e1, e2);
   }
 
   void initmat(double2 e1, double2 e2) {
@@ -260,6 +311,79 @@ int mnp_bemstat_create(mnp_greenret_t g, int solver, mnp_bemstat_t* out) {
       throw;
     }
     *out = h;
+  });
+}
+
+int mnp_bemstatlayer_create(mnp_greenret_t g, mnp_greenret_t gimage, int solver, mnp_bemstat_t* out) {
+  return guarded([&] {
+    MNP_CHECK(g && gimage && out, MNP_ERR_ARG, "bemstatlayer_create: NULL argument");
+    MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED || solver == MNP_SOLVER_MIXED_BF16, MNP_ERR_ARG,
+              "bemstatlayer_create: unknown solver");
+    MNP_CHECK(gimage->g->n == 2 * g->g->n, MNP_ERR_ARG,
+              "bemstatlayer_create: the image Green function must hold the particle and its image");
+    MNP_CHECK(!gimage->g->op.closed, MNP_ERR_ARG,
+              "bemstatlayer_create: the image Green function must be built without the closed-surface correction");
+    auto* h = new mnp_bemstat_s{nullptr};
+    try {
+      h->b = new BemStat(g->g, gimage->g, solver);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int mnp_bemstatlayer_initmat(mnp_bemstat_t b, double enei, mnp_complex eps1, mnp_complex eps2, mnp_complex eps_up,
+                             mnp_complex eps_down) {
+  return guarded([&] {
+    MNP_CHECK(b && enei > 0, MNP_ERR_ARG, "bemstatlayer_initmat: bad argument");
+    MNP_CHECK(eps1.re != eps2.re || eps1.im != eps2.im, MNP_ERR_ARG, "bemstatlayer_initmat: eps1 == eps2");
+    const double2 eu = to_d2(eps_up), ed = to_d2(eps_down);
+    const double2 den = cadd<double>(ed, eu);
+    MNP_CHECK(den.x != 0 || den.y != 0, MNP_ERR_ARG, "bemstatlayer_initmat: eps_up + eps_down == 0");
+    // f2 = -(eps2 - eps1) / (eps2 + eps1) of the layer (compgreenstatlayer/eval.m:18)
+    const double2 f2 = cscale<double>(cdiv<double>(csub<double>(ed, eu), den), -1.0);
+    b->b->initmat_layer(to_d2(eps1), to_d2(eps2), f2);
+  });
+}
+
+int mnp_planewavestatlayer_potential(mnp_greenret_t g, int64_t npol, const mnp_complex* e, void* phip) {
+  return guarded([&] {
+    MNP_CHECK(g && e && phip && npol >= 1 && npol <= MNP_STAT_MAXPOL, MNP_ERR_ARG,
+              "planewavestatlayer_potential: bad argument");
+    const GreenRet& G = *g->g;
+    FieldArg a{};
+    for (int64_t q = 0; q < npol; ++q)
+      for (int c = 0; c < 3; ++c) a.e[q][c] = to_d2(e[q * 3 + c]);
+    hipLaunchKernelGGL(k_pwstat_field, dim3((unsigned)cdiv_up(G.n, 256), (unsigned)npol), dim3(256), 0, stream(),
+                       G.n, G.geo.nvec, a, (double2*)phip);
+    MNP_HIP(hipGetLastError());
+  });
+}
+
+int mnp_stat_dipole_moment(mnp_greenret_t g, int64_t npol, const void* sig, mnp_complex* dip) {
+  return guarded([&] {
+    MNP_CHECK(g && sig && dip && npol >= 1 && npol < 65536, MNP_ERR_ARG, "stat_dipole_moment: bad argument");
+    const GreenRet& G = *g->g;
+    const int nchunk = 64;
+    double2* part = dev_alloc<double2>(npol * nchunk * 3);
+    hipLaunchKernelGGL(k_dipmoment, dim3(nchunk, (unsigned)npol), dim3(256), 0, stream(), G.n, G.geo.pos,
+                       G.geo.area, (const double2*)sig, part);
+    MNP_HIP(hipGetLastError());
+    std::vector<double2> h(npol * nchunk * 3);
+    MNP_HIP(hipMemcpyAsync(h.data(), part, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipStreamSynchronize(stream()));
+    MNP_HIP(hipFree(part));
+    for (int64_t q = 0; q < npol; ++q)
+      for (int d = 0; d < 3; ++d) {
+        double re = 0, im = 0;
+        for (int c = 0; c < nchunk; ++c) {
+          re += h[(q * nchunk + c) * 3 + d].x;
+          im += h[(q * nchunk + c) * 3 + d].y;
+        }
+        dip[q * 3 + d] = mnp_complex{re, im};
+      }
   });
 }
 

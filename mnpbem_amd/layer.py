@@ -1,0 +1,272 @@
+"""Quasistatic particle above a substrate: one planar interface between an
+upper and a lower medium (MNPBEM Particles/@layerstructure, BEM/@bemstatlayer,
+Simulation/static/@planewavestatlayer, @spectrumstatlayer).
+
+The solver runs on the GPU (mnp_bemstatlayer_* in include/mnpbem_hip.h): the
+reflected Green function is that of the particle's image in the interface,
+weighted per wavelength by f2 = -(eps_down - eps_up)/(eps_down + eps_up).
+The plane wave's field at the particle (Fresnel reflection of the incoming
+wave) and the cross sections from the induced dipole moment are host-side
+formulas over a handful of vectors, as in the reference.
+
+Supported: one interface; particles whose faces all lie above it (the
+reference also treats faces lying in the interface, which are not built).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from . import device as dev
+from .bem import bemret, bemstat, compstruct, _flat_normals
+from .greenfun import CompGreenRet
+
+_MATLAB_EPS = float(np.finfo(float).eps)
+
+
+class layerstructure:
+    """layerstructure(epstab, ind, z): media epstab[ind[0]-1] above and
+    epstab[ind[1]-1] below the interface z (Particles/@layerstructure/
+    layerstructure.m:27-31); ztol as layerstructure.ztol."""
+
+    def __init__(self, epstab, ind, z, ztol=2e-2):
+        z = np.atleast_1d(np.asarray(z, dtype=np.float64))
+        if len(ind) != 2 or len(z) != 1:
+            raise NotImplementedError("layerstructure: one interface (two media) is supported")
+        self.ind = list(ind)
+        self.eps = [epstab[i - 1] for i in self.ind]
+        self.z = z
+        self.ztol = float(ztol)
+
+    @property
+    def n(self):
+        return len(self.z)
+
+    def epsilon(self, enei):
+        """(eps above, eps below) at the wavelength."""
+        return complex(self.eps[0](enei)[0]), complex(self.eps[1](enei)[0])
+
+    def indlayer(self, z):
+        """(medium index 1 = above / 2 = below, in-interface mask) (indlayer.m)."""
+        z = np.asarray(z, dtype=np.float64)
+        return np.where(z > self.z[0], 1, 2), np.abs(z - self.z[0]) < self.ztol
+
+
+def _image_mesh(verts, faces, z0):
+    """[particle; image]: vertices mirrored at z = z0, faces reversed
+    (compgreenstatlayer/init.m:21-24 with @particle/flip.m)."""
+    v = np.asarray(verts, dtype=np.float64)
+    vi = v.copy()
+    vi[:, 2] = 2.0 * z0 - vi[:, 2]
+    f = np.asarray(faces, dtype=np.int64)
+    return np.vstack([v, vi]), np.vstack([f, f[:, ::-1] + len(v)])
+
+
+def _check_above(p, layer):
+    pos = p.green.geometry()["pos"]
+    med, inside = layer.indlayer(pos[:, 2])
+    if np.any(inside):
+        raise NotImplementedError("substrate: faces lying in the interface are not supported")
+    if np.any(med != 1):
+        raise ValueError("substrate: the particle must lie above the interface")
+
+
+class bemstatlayer(bemstat):
+    """Quasistatic BEM solver above a substrate (BEM/@bemstatlayer), op['layer']
+    a layerstructure; precision as for bemstat."""
+
+    def __init__(self, p, op=None, precision=None):
+        op = p.op if op is None else op
+        self.layer = op["layer"]
+        _check_above(p, self.layer)
+        self.p = p
+        self.precision = precision or op.get("precision", "fp64")
+        if self.precision not in bemret._SOLVERS:
+            raise ValueError(f"bemstatlayer: unknown precision {self.precision!r}")
+        V, F = _image_mesh(p.verts, p.faces, self.layer.z[0])
+        self.gimage = CompGreenRet(V, F, order=op.get("order", 5), rel_cutoff=op.get("RelCutoff", 0.0),
+                                   abs_cutoff=op.get("AbsCutoff", 0.0),
+                                   npol=tuple(np.broadcast_to(op.get("npol", (7, 5)), (2,))), closed=False)
+        h = C.c_void_p()
+        _lib.call("mnp_bemstatlayer_create", p.green.handle, self.gimage.handle, bemret._SOLVERS[self.precision],
+                  C.byref(h))
+        self.handle = h
+        self.enei = None
+
+    def __call__(self, enei):
+        if self.enei != enei:
+            eu, ed = self.layer.epsilon(enei)
+            _lib.call("mnp_bemstatlayer_initmat", self.handle, float(enei), _lib.MnpComplex.of(self.p.eps1(enei)),
+                      _lib.MnpComplex.of(self.p.eps2(enei)), _lib.MnpComplex.of(eu), _lib.MnpComplex.of(ed))
+            self.enei = enei
+        return self
+
+
+def _decompose(pol, dirs):
+    """TE part (along z x k) and TM rest of each polarisation (decompose.m)."""
+    kx, ky = dirs[:, 0], dirs[:, 1]
+    kt = np.maximum(np.hypot(kx, ky), _MATLAB_EPS)
+    s = (ky * pol[:, 0] - kx * pol[:, 1]) / kt
+    te = np.column_stack([ky / kt * s, -kx / kt * s, np.zeros_like(s)])
+    return te, pol - te
+
+
+def _fresnel(dirs, k0, eu, ed):
+    """Fresnel coefficients and wave vectors (fresnel.m) per direction."""
+    k1, k2 = k0 * np.sqrt(eu), k0 * np.sqrt(ed)
+    res = []
+    for d in dirs:
+        if d[2] < 0:  # incoming from above
+            kiz = -k1 * d[2]
+            kz = np.sqrt(k2 * k2 - k1 * k1 + kiz * kiz)
+            kz = np.conj(kz) if kz.imag < 0 else kz
+            ki = k1 * d
+            coef = ((kiz - kz) / (kiz + kz), 2 * kiz / (kiz + kz),
+                    (ed * kiz - eu * kz) / (ed * kiz + eu * kz), 2 * ed * kiz / (ed * kiz + eu * kz))
+            kr, kt = np.array([ki[0], ki[1], kiz]), np.array([ki[0], ki[1], -kz])
+        else:  # incoming from below
+            kiz = k2 * d[2]
+            kz = np.sqrt(k1 * k1 - k2 * k2 + kiz * kiz)
+            kz = np.conj(kz) if kz.imag < 0 else kz
+            ki = k2 * d
+            coef = ((kiz - kz) / (kiz + kz), 2 * kiz / (kiz + kz),
+                    (eu * kiz - ed * kz) / (eu * kiz + ed * kz), 2 * eu * kiz / (eu * kiz + ed * kz))
+            kr, kt = np.array([ki[0], ki[1], -kiz]), np.array([ki[0], ki[1], kz])
+        res.append((coef, ki.astype(complex), kr.astype(complex), kt.astype(complex)))
+    return res
+
+
+def _dipole_far(dip, dirs, k0, eu, ed):
+    """Far field of the dipoles dip (npol x 3) above the interface in the
+    directions dirs (ndir x 3), and the wavenumber per direction
+    (spectrumstatlayer/efarfield.m)."""
+    k1, k2 = k0 * np.sqrt(eu), k0 * np.sqrt(ed)
+    dirs = np.asarray(dirs)
+    real = ~np.any(np.abs(np.imag(dirs)) > 1e-10, axis=1)  # complex directions carry no far field
+    dirs = np.real(dirs)
+    dz = dirs[:, 2]
+    phi = np.arctan2(dirs[:, 1], dirs[:, 0])
+    theta = np.pi / 2 - np.arctan2(dz, np.hypot(dirs[:, 0], dirs[:, 1]))
+    sp, cp, st, ct = np.sin(phi), np.cos(phi), np.sin(theta), np.cos(theta)
+    f1 = np.zeros(len(dirs), complex)
+    f2 = np.zeros(len(dirs), complex)
+    f3 = np.zeros(len(dirs), complex)
+    up, lo = (dz >= 0) & real, (dz < 0) & real
+    if abs(k1.imag) < 1e-10 and up.any():
+        k1z = k1 * dz[up]
+        k2z = np.sqrt(k2 * k2 - k1 * k1 + k1z * k1z + 0j)
+        rte = (k1z - k2z) / (k1z + k2z)
+        rtm = (ed * k1z - eu * k2z) / (ed * k1z + eu * k2z)
+        f1[up], f2[up], f3[up] = 1 + rtm, 1 - rtm, 1 + rte
+    if abs(k2.imag) < 1e-10 and lo.any():
+        k2z = -k2 * dz[lo]
+        k1z = np.sqrt(k1 * k1 - k2 * k2 + k2z * k2z + 0j) + 1e-10j
+        k1z = k1z * np.sign(k1z.imag)
+        s = k1z / k2
+        tte = 2 * k1z / (k1z + k2z)
+        ttm = 2 * ed * k1z / (ed * k1z + eu * k2z) * np.sqrt(eu / ed)
+        r = np.sqrt(ed / eu)
+        f1[lo], f2[lo], f3[lo] = -r * ct[lo] / s * ttm, r * ttm, -ct[lo] / s * tte
+    # e_theta, e_phi per (direction, dipole)
+    eth = (np.outer(cp, dip[:, 0]) + np.outer(sp, dip[:, 1])) * (ct * f2)[:, None] - np.outer(st * f1, dip[:, 2])
+    eph = -(np.outer(sp, dip[:, 0]) - np.outer(cp, dip[:, 1])) * f3[:, None]
+    e = np.stack([eth * (ct * cp)[:, None] - eph * sp[:, None], eth * (ct * sp)[:, None] + eph * cp[:, None],
+                  -eth * st[:, None]], axis=1)
+    k = np.where(dz > 0, k1, np.where(dz < 0, k2, 0))
+    return k1 * k1 * e, k
+
+
+class planewavestatlayer:
+    """Quasistatic plane wave above a substrate (Simulation/static/
+    @planewavestatlayer): pol and dir npol x 3, op['layer'] a layerstructure;
+    scattering is integrated over ``pinfty`` = (nvec, area) of the unit
+    sphere (default: the 256-point sphere of trisphere(256, 2))."""
+
+    def __init__(self, pol, dir, op=None, pinfty=None):
+        self.pol = np.atleast_2d(np.asarray(pol, dtype=np.float64))
+        self.dir = np.atleast_2d(np.asarray(dir, dtype=np.float64))
+        if np.any(np.sum(self.pol * self.dir, axis=1) != 0):
+            raise ValueError("polarisation must be perpendicular to the propagation direction")
+        if len(self.pol) > 64:
+            raise ValueError("planewavestatlayer: at most 64 polarisations")
+        self.layer = op["layer"]
+        if pinfty is None:
+            from . import shapes
+            pinfty = _flat_normals(*shapes.trisphere256(2.0))
+        self.pinf_nvec, self.pinf_area = (np.asarray(a, dtype=np.float64) for a in pinfty)
+
+    def fields(self, enei):
+        """(e above, e below), npol x 3 each (field.m:25-47)."""
+        k0 = 2 * np.pi / enei
+        eu, ed = self.layer.epsilon(enei)
+        te, tm = _decompose(self.pol, self.dir)
+        above, below = [], []
+        for i, (coef, ki, kr, kt) in enumerate(_fresnel(self.dir, k0, eu, ed)):
+            re, tte, rm, ttm = coef
+            m = np.cross(ki, tm[i]) / k0
+            er = np.cross(m, kr) * k0 / np.sum(kr * kr)
+            et = np.cross(m, kt) * k0 / np.sum(kt * kt)
+            refl = self.pol[i] + re * te[i] + rm * er
+            trans = tte * te[i] + ttm * et
+            above.append(refl if self.dir[i, 2] < 0 else trans)
+            below.append(trans if self.dir[i, 2] < 0 else refl)
+        return np.array(above), np.array(below)
+
+    def __call__(self, p, enei):
+        """exc(p, enei): phi' = -nvec . e at the faces (potential.m:13-14)."""
+        _check_above(p, self.layer)
+        e = np.ascontiguousarray(self.fields(enei)[0], dtype=np.complex128)
+        phip = dev.colmajor((p.n, len(self.pol)))
+        _lib.call("mnp_planewavestatlayer_potential", p.green.handle, len(self.pol),
+                  e.ctypes.data_as(C.c_void_p), dev.ptr(dev.base(phip)))
+        return compstruct(p, enei, phip=phip)
+
+    potential = __call__
+
+    def dipole(self, sig):
+        """Induced dipole moments, npol x 3 (FP64 device reduction)."""
+        npol = sig.sig.shape[-1]
+        dip = np.zeros((npol, 3), dtype=np.complex128)
+        _lib.call("mnp_stat_dipole_moment", sig.p.green.handle, npol, dev.ptr(dev.base(sig.sig)),
+                  dip.ctypes.data_as(C.c_void_p))
+        return dip
+
+    def cross_sections(self, sig):
+        """(ext, sca, abs) per polarisation (extinction.m 'full', scattering.m,
+        absorption.m = ext - sca)."""
+        enei = sig.enei
+        k0 = 2 * np.pi / enei
+        eu, ed = self.layer.epsilon(enei)
+        dip = self.dipole(sig)
+        te, tm = _decompose(self.pol, self.dir)
+        ext = np.zeros(len(self.pol))
+        for i, (coef, ki, kr, kt) in enumerate(_fresnel(self.dir, k0, eu, ed)):
+            re, tte, rm, ttm = coef
+            m = np.cross(ki, tm[i]) / k0
+            er = re * te[i] + rm * np.cross(m, kr) * k0 / np.vdot(kr, kr).real
+            et = tte * te[i] + ttm * np.cross(m, kt) * k0 / np.vdot(kt, kt).real
+            nr, nt = np.linalg.norm(kr), np.linalg.norm(kt)
+            far, _ = _dipole_far(dip[i:i + 1], np.vstack([kr / nr, kt / nt]), k0, eu, ed)
+            esr, est = far[0, :, 0], far[1, :, 0]
+            if abs(kt[2].imag) > 1e-10:
+                est = 0 * est
+            ext[i] = 4 * np.pi / nr * (np.vdot(er, esr).imag + np.vdot(et, est).imag)
+        far, k = _dipole_far(dip, self.pinf_nvec, k0, eu, ed)
+        dsca = np.sum(np.abs(far) ** 2, axis=1) * (0.5 * k / k0)[:, None]
+        dsca = np.where(np.imag(dsca) != 0, 0, dsca).real
+        sca = self.pinf_area @ dsca
+        nb = np.sqrt(np.array([eu, ed]))
+        sca = np.where(self.dir[:, 2] < 0, sca / (0.5 * nb[0]), np.where(self.dir[:, 2] > 0, sca / (0.5 * nb[-1]), sca))
+        sca = np.real(sca)
+        return ext, sca, ext - sca
+
+    def ext(self, sig):
+        return self.cross_sections(sig)[0]
+
+    def sca(self, sig):
+        return self.cross_sections(sig)[1]
+
+    def abs(self, sig):
+        return self.cross_sections(sig)[2]
+
+    extinction, scattering, absorption = ext, sca, abs
