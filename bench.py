@@ -475,6 +475,9 @@ def main():
     ap.add_argument("--no-substrate", action="store_true", help="skip the substrate measurement")
     ap.add_argument("--config4-only", action="store_true",
                     help="only the H-matrix dipole measurement (for profiling)")
+    ap.add_argument("--only", default=None,
+                    help="comma list of side measurements to run alone (for profiling): "
+                         "quasistatic, mirror, substrate, multi_particle, config4")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy three times (counter calibration)")
     a = ap.parse_args()
@@ -493,6 +496,16 @@ def main():
         del src, dst
     if a.config4_only:
         print(json.dumps({"hmatrix_dipole": run_config4()}), flush=True)
+        return
+    if a.only:
+        runs = {"quasistatic": lambda: run_quasistatic(a.level), "mirror": run_mirror,
+                "substrate": lambda: run_substrate(a.level), "multi_particle": run_multi_particle,
+                "config4": run_config4}
+        names = [x.strip() for x in a.only.split(",") if x.strip()]
+        bad = [x for x in names if x not in runs]
+        if bad:
+            ap.error(f"--only: unknown measurement(s) {bad}")
+        print(json.dumps({x: runs[x]() for x in names}), flush=True)
         return
     r = run_gpu(a.level, a.steps, a.warmup, a.fp64_steps, not a.no_split_bf16)
     tm = r["timers"]
