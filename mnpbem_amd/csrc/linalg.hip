@@ -216,6 +216,166 @@ __global__ void __launch_bounds__(256) k_cgemm(int64_t M, int64_t N, int64_t K, 
     }
 }
 
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+template <int S> using stage_t = std::integral_constant<int, S>;
+
+// ------------------------------------------ FP32 3M GEMM, direct-to-LDS loads
+// The 3M product of k_cgemm for complex64 operands, with the slices streamed
+// from HBM straight into LDS by global_load_lds_dword (lane l of a wave
+// writes bytes 4 l .. 4 l + 3 of the wave's 256-byte LDS destination), two
+// stages deep, in the operands' own interleaved (re, im) order: a fragment
+// read is one ds_read_b64, and no registers or VALU carry the slices.
+//
+// A stage: the 16 k-rows of the tile's BM rows, row kr at sA[kr][0 .. BM), one
+// complex of padding per row.  Wave w loads rows 32 w .. 32 w + 31 of every
+// k-row: lane l fetches component l & 1 of row m0 + 32 w + l / 2 (clamped to
+// M - 1) in column k0 + kr (clamped to K - 1).
+// B stage: column c of the tile at sB[c][0 .. 16); one wave load covers
+// columns 2 j, 2 j + 1, lane l fetching component l & 1 of k = (l % 32 / 2 + c)
+// mod 16 into position l % 32 / 2.  Element (k, c) thus sits at position
+// (k - c) mod 16, which spreads the 16 columns of a fragment read over the
+// banks.  Rows, columns and k beyond the operands read clamped (valid)
+// addresses; the k-rows of A beyond K are zeroed in LDS before the last
+// slice's MFMAs, which cancels whatever B holds there.
+constexpr int FBK = 16;
+template <int WM, int WN>
+__global__ void __launch_bounds__(256) k_cgemm_f32lds(int64_t M, int64_t N, int64_t K, float2 alpha,
+                                                      const float* __restrict__ A, int64_t lda,
+                                                      const float* __restrict__ B, int64_t ldb, float2 beta,
+                                                      int beta_zero, float2* C, int64_t ldc) {
+  using acc_t = MF<float>::acc_t;
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  constexpr int NB = BN / 2 / 4;  // B wave loads per wave and slice
+  static_assert(BM == 128 && BN % 8 == 0, "four 32-row A wave loads per k-row, B columns in pairs per wave");
+  __shared__ float2 sA0[FBK][BM + 1], sA1[FBK][BM + 1];
+  __shared__ float2 sB0[BN][FBK], sB1[BN][FBK];
+  auto stA = [&](auto st) -> float2(&)[FBK][BM + 1] {
+    if constexpr (decltype(st)::value == 0) return sA0; else return sA1;
+  };
+  auto stB = [&](auto st) -> float2(&)[BN][FBK] {
+    if constexpr (decltype(st)::value == 0) return sB0; else return sB1;
+  };
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int wm = wid & 1, wn = wid >> 1;
+  int64_t tx, ty;
+  {  // XCD-aware tile order, as in k_cgemm
+    const int64_t gx = gridDim.x, gy = gridDim.y, tot = gx * gy;
+    const int64_t L = blockIdx.x + gx * (int64_t)blockIdx.y;
+    const int64_t per = tot / 8, rem = tot % 8, xcd = L % 8, idx = L / 8;
+    const int64_t nl = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    constexpr int64_t G = 8;
+    const int64_t group = nl / (G * gy), first = group * G;
+    const int64_t gsz = min(G, gx - first), r = nl % (G * gy);
+    tx = first + r % gsz;
+    ty = r / gsz;
+  }
+  const int64_t m0 = tx * BM, n0 = ty * BN;
+  // per-lane element offsets (floats) of the wave loads; B's relative to its
+  // tile's first column (below 2^31: checked by the launcher)
+  const int64_t aoff = min(m0 + 32 * wid + (lane >> 1), M - 1) * 2 + (lane & 1);
+  const float* Bt = B + n0 * ldb * 2;
+  const int kb = ((lane & 31) >> 1) + 2 * wid + (lane >> 5);  // k of load j: (kb + 8 j) mod 16
+  int boff[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int c = 2 * (wid + 4 * j) + (lane >> 5);
+    boff[j] = (int)((min(n0 + c, N - 1) - n0) * ldb * 2) + (lane & 1);
+  }
+  acc_t cr[WM][WN], ci[WM][WN], c3[WM][WN];
+#pragma unroll
+  for (int a = 0; a < WM; ++a)
+#pragma unroll
+    for (int b = 0; b < WN; ++b) {
+      cr[a][b] = acc_t{0, 0, 0, 0};
+      ci[a][b] = acc_t{0, 0, 0, 0};
+      c3[a][b] = acc_t{0, 0, 0, 0};
+    }
+  auto issue = [&](auto st, int64_t k0) {
+    auto& dA = stA(st);
+    auto& dB = stB(st);
+    const bool full = k0 + FBK <= K;
+#pragma unroll
+    for (int kr = 0; kr < FBK; ++kr) {
+      const int64_t gk = min(k0 + kr, K - 1);
+      __builtin_amdgcn_global_load_lds((const void*)(A + gk * lda * 2 + aoff), (lds_ptr_t)&dA[kr][32 * wid], 4, 0,
+                                       0);
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int kj = (kb + 8 * j) & 15;
+      const int64_t gk = full ? k0 + kj : min(k0 + kj, K - 1);
+      __builtin_amdgcn_global_load_lds((const void*)(Bt + boff[j] + gk * 2), (lds_ptr_t)&dB[2 * (wid + 4 * j)][0], 4,
+                                       0, 0);
+    }
+  };
+  const int r16 = lane & 15, kq = lane >> 4;
+  auto compute = [&](auto st) {
+    auto& xA = stA(st);
+    auto& xB = stB(st);
+#pragma unroll
+    for (int kk = 0; kk < FBK / 4; ++kk) {
+      const int kr = kk * 4 + kq;
+      float ar[WM], ai[WM], br[WN], bi[WN];
+#pragma unroll
+      for (int a = 0; a < WM; ++a) {
+        const float2 v = xA[kr][wm * 16 * WM + a * 16 + r16];
+        ar[a] = v.x;
+        ai[a] = v.y;
+      }
+#pragma unroll
+      for (int b = 0; b < WN; ++b) {
+        const int col = wn * 16 * WN + b * 16 + r16;
+        const float2 v = xB[col][(kr - col) & 15];
+        br[b] = v.x;
+        bi[b] = v.y;
+      }
+#pragma unroll
+      for (int a = 0; a < WM; ++a)
+#pragma unroll
+        for (int b = 0; b < WN; ++b) {
+          cr[a][b] = MF<float>::mma(ar[a], br[b], cr[a][b]);
+          ci[a][b] = MF<float>::mma(ai[a], bi[b], ci[a][b]);
+          c3[a][b] = MF<float>::mma(ar[a] + ai[a], br[b] + bi[b], c3[a][b]);
+        }
+    }
+  };
+  // slice starting at k0 lives in stage (k0 / FBK) % 2
+  auto step = [&](auto st, int64_t k0) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's loads of the slice have landed
+    __syncthreads();                // all waves' have; nobody reads the other stage any more
+    if (k0 + FBK < K) issue(stage_t<1 - decltype(st)::value>{}, k0 + FBK);
+    if (k0 + FBK > K) {             // last, partial slice: zero A's k-rows beyond K
+      auto& xA = stA(st);
+      const int kv = (int)(K - k0);
+      for (int e = tid; e < (FBK - kv) * BM; e += 256) xA[kv + e / BM][e % BM] = make_float2(0.f, 0.f);
+      __syncthreads();
+    }
+    compute(st);
+  };
+  if (K > 0) issue(stage_t<0>{}, 0);
+  for (int64_t k0 = 0; k0 < K; k0 += 2 * FBK) {
+    step(stage_t<0>{}, k0);
+    if (k0 + FBK < K) step(stage_t<1>{}, k0 + FBK);
+  }
+#pragma unroll
+  for (int a = 0; a < WM; ++a)
+#pragma unroll
+    for (int b = 0; b < WN; ++b) {
+      const int64_t col = n0 + wn * 16 * WN + b * 16 + r16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m0 + wm * 16 * WM + a * 16 + MF<float>::row(lane, r);
+        if (row < M && col < N) {
+          const float2 v = make_float2(cr[a][b][r] - ci[a][b][r], c3[a][b][r] - cr[a][b][r] - ci[a][b][r]);
+          float2 o = cmul<float>(alpha, v);
+          float2* pc = C + col * ldc + row;
+          if (!beta_zero) o = cadd<float>(o, cmul<float>(beta, *pc));
+          *pc = o;
+        }
+      }
+    }
+}
+
 // ------------------------------------------------ split-bf16 complex GEMM
 // C = alpha A B + beta C for complex64 operands on bf16 MFMA.  Each FP32
 // value x is held as a pair of bf16, hi = rn(x) and lo = rn(x - hi), so
@@ -309,7 +469,6 @@ __global__ void __launch_bounds__(256) k_split(int64_t R, int64_t K, const float
   for (int p = 0; p < 4; ++p) P[((p * KS + sl) * Rp + r) * 4 + g] = c[p];
 }
 
-typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
 // LDS slot of k-group g of row r: g ^ lds_swz(r % 16).  A fragment read
 // (ds_read_b128: lane l takes row l % 16, k-group l / 16) then hits distinct
@@ -317,7 +476,6 @@ typedef __attribute__((address_space(3))) void* lds_ptr_t;
 // r + 4, r + 6 get distinct slots) and when 16 lanes are served from 64 banks
 // (rows r, r + 4, r + 8, r + 12 do).
 __device__ __forceinline__ int lds_swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
-template <int S> using stage_t = std::integral_constant<int, S>;
 
 // Each stage is its own __shared__ array and the main loop is unrolled over
 // the stages, so every LDS access names its stage at compile time: the
@@ -651,6 +809,24 @@ static void gemm_launch(int64_t m, int64_t n, int64_t k, cplx<T> al, MatView<T> 
   MNP_HIP(hipGetLastError());
 }
 
+template <int WM, int WN>
+static void gemm_f32lds(int64_t m, int64_t n, int64_t k, float2 al, MatView<float> A, MatView<float> B,
+                        float2 be, bool bz, MatView<float> C) {
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  const int64_t gx = cdiv_up(m, BM), ny = cdiv_up(n, BN);
+  for (int64_t y0 = 0; y0 < ny; y0 += 65535) {
+    const int64_t gy = std::min<int64_t>(65535, ny - y0);
+    const int64_t noff = y0 * BN;
+    const int64_t ncols = std::min<int64_t>(n - noff, gy * BN);
+    TimerScope ts(MNP_TIMER_GEMM_FP32, 8.0 * (double)m * (double)ncols * (double)k,
+                  8.0 * ((double)m * k + (double)k * ncols + (double)m * ncols * (bz ? 1.0 : 2.0)));
+    hipLaunchKernelGGL((k_cgemm_f32lds<WM, WN>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream(), m,
+                       n - noff, k, al, (const float*)A.p, A.ld, (const float*)(B.p + noff * B.ld), B.ld, be, (int)bz,
+                       C.p + noff * C.ld, C.ld);
+  }
+  MNP_HIP(hipGetLastError());
+}
+
 template <class T>
 void gemm(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, MatView<T> B, double2 beta,
           MatView<T> C) {
@@ -673,8 +849,10 @@ void gemm(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, MatView<
       gemm_launch<T, 4, 2>(m, n, k, al, A, B, be, bz, C);  // 128 x 64: accumulators fit without spills
     else if (g_split_bf16)
       gemm_sbf16<4, 4, 2>(m, n, k, al, A, B, be, bz, C);
+    else if (B.ld < ((int64_t)1 << 31) / (2 * 96))
+      gemm_f32lds<4, 3>(m, n, k, al, A, B, be, bz, C);  // 3M on the FP32 chain
     else
-      gemm_launch<T, 4, 3, true>(m, n, k, al, A, B, be, bz, C);  // 3M on the FP32 chain
+      gemm_launch<T, 4, 3, true>(m, n, k, al, A, B, be, bz, C);
   } else
     gemm_launch<T, 2, 2>(m, n, k, al, A, B, be, bz, C);
 }

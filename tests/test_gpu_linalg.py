@@ -25,7 +25,7 @@ def _prec(dtype):
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 @pytest.mark.parametrize("mnk", [(64, 64, 16), (130, 77, 45), (1, 300, 257), (257, 3, 1000),
-                                 (2100, 2300, 700), (2049, 2177, 129),
+                                 (2100, 2300, 700), (2049, 2177, 129), (2051, 2103, 64), (2311, 2067, 75),
                                  # few right-hand sides: the tall-skinny kernel, direct and split over k
                                  (64, 16, 256), (3000, 8, 300), (12000, 16, 5000), (777, 13, 9999)])
 def test_gemm(dtype, mnk):
