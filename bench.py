@@ -536,7 +536,7 @@ def main():
             "faces": n, "npol": 2, "precision": "complex FP32 chain + FP64 refinement",
         },
         "roofline": {
-            "bound": "mfma", "kernel": "k_cgemm<float,4,3,true> (3M complex GEMM on FP32 MFMA)",
+            "bound": "mfma", "kernel": "k_cgemm_f32lds<4,3> (3M complex GEMM on FP32 MFMA, direct-to-LDS operands)",
             "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
             "frac": achieved / PEAK["fp32"], "traffic": None,
             "launches_per_step": tm.launches[gi] / a.steps,

@@ -201,18 +201,23 @@ struct BemRetMirror {
       contract(BG, og);
       contract(BH, oh);
     }
+    // the J symmetry chains are independent: one task (and stream) each;
+    // their large products are small enough (n / J) that overlapping the
+    // chains' latency-bound kernels pays
     SplitBf16Scope split(solver == MNP_SOLVER_MIXED_BF16);
+    std::vector<std::function<void()>> tasks;
     for (int s = 0; s < J; ++s) {
       if (solver == MNP_SOLVER_FP64) {
         Chain<double>& c = *s64[s];
         c.eps1 = e1, c.eps2 = e2, c.k0 = k0;
-        chain_sector<double>(c);
+        tasks.push_back([&c] { chain_sector<double>(c); });
       } else {
         Chain<float>& c = *s32[s];
         c.eps1 = e1, c.eps2 = e2, c.k0 = k0;
-        chain_sector<float>(c);
+        tasks.push_back([&c] { chain_sector<float>(c); });
       }
     }
+    run_concurrent(tasks);
     ready = true;
   }
 

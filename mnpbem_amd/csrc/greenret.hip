@@ -12,7 +12,13 @@
 #include <cmath>
 #include <vector>
 
+#include <condition_variable>
+#include <exception>
+#include <mutex>
+#include <thread>
+
 #include "mnp_greenret.hpp"
+#include "mnp_linalg.hpp"
 #include "mnp_nearfield.hpp"
 
 namespace mnp {
@@ -21,7 +27,9 @@ namespace mnp {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 static hipStream_t g_stream = nullptr;
-hipStream_t stream() { return g_stream; }
+static thread_local hipStream_t t_stream = nullptr;  // a worker's own stream
+static thread_local bool t_in_worker = false;
+hipStream_t stream() { return t_stream ? t_stream : g_stream; }
 
 // ------------------------------------------------------------------ timers
 namespace {
@@ -31,10 +39,12 @@ struct TimerRec {
   hipEvent_t a, b;
 };
 bool g_timers_on = false;
+std::mutex g_timer_mu;  // timed launches may come from worker threads
 std::vector<TimerRec> g_timer_recs;
 std::vector<hipEvent_t> g_event_pool;  // events are reused: creating one per launch costs host time
 mnp_timers g_timer_acc{};
 hipEvent_t event_get() {
+  std::lock_guard<std::mutex> lk(g_timer_mu);
   if (!g_event_pool.empty()) {
     hipEvent_t e = g_event_pool.back();
     g_event_pool.pop_back();
@@ -45,10 +55,11 @@ hipEvent_t event_get() {
   return e;
 }
 void timers_flush() {
+  std::lock_guard<std::mutex> lk(g_timer_mu);
   if (g_timer_recs.empty()) return;
-  MNP_HIP(hipStreamSynchronize(g_stream));
   for (auto& r : g_timer_recs) {
     float ms = 0;
+    MNP_HIP(hipEventSynchronize(r.b));
     MNP_HIP(hipEventElapsedTime(&ms, r.a, r.b));
     g_timer_acc.launches[r.kind] += 1;
     g_timer_acc.ms[r.kind] += ms;
@@ -64,7 +75,7 @@ void timers_flush() {
 TimerScope::TimerScope(int kind_, double work_, double bytes_) : kind(kind_), work(work_), bytes(bytes_) {
   if (!g_timers_on || kind < 0) return;
   hipEvent_t a = event_get();
-  MNP_HIP(hipEventRecord(a, g_stream));
+  MNP_HIP(hipEventRecord(a, stream()));
   ev0 = a;
 }
 TimerScope::~TimerScope() {
@@ -75,14 +86,137 @@ TimerScope::~TimerScope() {
   } catch (...) {
     return;
   }
-  (void)hipEventRecord(b, g_stream);
-  g_timer_recs.push_back(TimerRec{kind, work, bytes, (hipEvent_t)ev0, b});
-  if (g_timer_recs.size() > 16384) {
+  (void)hipEventRecord(b, stream());
+  size_t nrec;
+  {
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    g_timer_recs.push_back(TimerRec{kind, work, bytes, (hipEvent_t)ev0, b});
+    nrec = g_timer_recs.size();
+  }
+  if (nrec > 16384) {
     try {
       timers_flush();
     } catch (...) {
     }
   }
+}
+
+// ----------------------------------------------------- concurrent tasks
+namespace {
+// Three persistent worker threads (detached: they idle until the process
+// ends, and their thread-local workspaces live as long).
+struct WorkerPool {
+  struct W {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::function<void()> job;
+    bool has = false, done = true;
+  };
+  W w[3];
+  WorkerPool() {
+    for (auto& x : w) std::thread([&x] { loop(x); }).detach();
+  }
+  static void loop(W& x) {
+    t_in_worker = true;
+    for (;;) {
+      std::function<void()> j;
+      {
+        std::unique_lock<std::mutex> lk(x.mu);
+        x.cv.wait(lk, [&] { return x.has; });
+        j = std::move(x.job);
+        x.has = false;
+      }
+      j();  // never throws: run_concurrent's jobs catch everything
+      {
+        std::lock_guard<std::mutex> lk(x.mu);
+        x.done = true;
+      }
+      x.cv.notify_all();
+    }
+  }
+  void start(int i, std::function<void()> j) {
+    {
+      std::lock_guard<std::mutex> lk(w[i].mu);
+      w[i].job = std::move(j);
+      w[i].has = true;
+      w[i].done = false;
+    }
+    w[i].cv.notify_all();
+  }
+  void wait(int i) {
+    std::unique_lock<std::mutex> lk(w[i].mu);
+    w[i].cv.wait(lk, [&] { return w[i].done; });
+  }
+};
+WorkerPool& pool() {
+  static WorkerPool* p = new WorkerPool;  // never destroyed (detached threads)
+  return *p;
+}
+// one stream per worker thread and device, default priority
+hipStream_t worker_stream(int dev) {
+  static thread_local std::vector<hipStream_t> s;
+  if ((int)s.size() <= dev) s.resize(dev + 1, nullptr);
+  if (!s[dev]) MNP_HIP(hipStreamCreateWithFlags(&s[dev], hipStreamNonBlocking));
+  return s[dev];
+}
+}  // namespace
+
+static thread_local bool t_in_run = false;  // this thread is inside run_concurrent
+
+void run_concurrent(std::vector<std::function<void()>>& tasks) {
+  if (tasks.size() <= 1 || tasks.size() > 4 || t_in_worker || t_in_run) {
+    for (auto& t : tasks) t();
+    return;
+  }
+  // one caller at a time owns the pool
+  static std::mutex run_mu;
+  std::lock_guard<std::mutex> run_lk(run_mu);
+  struct InRun {
+    InRun() { t_in_run = true; }
+    ~InRun() { t_in_run = false; }
+  } in_run;
+  int dev = 0;
+  MNP_HIP(hipGetDevice(&dev));
+  const hipStream_t caller = stream();
+  const int nt = (int)tasks.size();
+  std::vector<hipEvent_t> ev(nt, nullptr);  // ev[0]: caller's start; ev[i]: task i's end
+  for (auto& e : ev) MNP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  MNP_HIP(hipEventRecord(ev[0], caller));
+  const int split = split_bf16();
+  std::vector<std::exception_ptr> err(nt);
+  std::vector<char> ended(nt, 0);
+  WorkerPool& P = pool();
+  for (int i = 1; i < nt; ++i)
+    P.start(i - 1, [&, i] {
+      hipStream_t s = nullptr;
+      try {
+        MNP_HIP(hipSetDevice(dev));
+        s = worker_stream(dev);
+        t_stream = s;
+        set_split_bf16(split);
+        MNP_HIP(hipStreamWaitEvent(s, ev[0], 0));
+        tasks[i]();
+        MNP_HIP(hipEventRecord(ev[i], s));
+        ended[i] = 1;
+      } catch (...) {
+        err[i] = std::current_exception();
+        if (s) (void)hipStreamSynchronize(s);  // nothing of this task stays in flight
+      }
+      t_stream = nullptr;
+      set_split_bf16(0);
+    });
+  try {
+    tasks[0]();
+  } catch (...) {
+    err[0] = std::current_exception();
+  }
+  for (int i = 1; i < nt; ++i) P.wait(i - 1);
+  for (int i = 1; i < nt; ++i)
+    if (ended[i]) (void)hipStreamWaitEvent(caller, ev[i], 0);
+  if (err[0]) (void)hipStreamSynchronize(caller);
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
 }
 
 // ------------------------------------------------------------- quadrature

@@ -10,6 +10,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -635,7 +636,7 @@ __global__ void __launch_bounds__(256) k_cgemm_sbf16(int64_t M, int64_t N, int64
 // Split planes of the operands of the current product (reused by the next;
 // launches on one stream are ordered).
 static Workspace& split_ws() {
-  static Workspace ws;
+  static thread_local Workspace ws;  // per thread: concurrent tasks run their products on their own streams
   return ws;
 }
 
@@ -749,7 +750,7 @@ __global__ void k_skinny_sum(int64_t m, int n, int64_t ks, const cplx<T>* W, cpl
 
 template <class T>
 static Workspace& skinny_ws() {
-  static Workspace ws;
+  static thread_local Workspace ws;
   return ws;
 }
 
@@ -779,7 +780,7 @@ static void skinny(int64_t m, int64_t n, int64_t k, cplx<T> al, MatView<T> A, Ma
 
 // Products of complex64 matrices on the split-bf16 kernel when set (the
 // MNP_SOLVER_MIXED_BF16 chain sets it around its factorisations).
-static int g_split_bf16 = 0;
+static thread_local int g_split_bf16 = 0;
 void set_split_bf16(int on) { g_split_bf16 = on; }
 int split_bf16() { return g_split_bf16; }
 
@@ -1090,12 +1091,10 @@ __global__ void k_rows_scatter(int64_t maxcnt, const int32_t* count, const int32
 static void launch_perm(int64_t k0, int64_t k1, int64_t rlo, int64_t rhi, const int32_t* ipiv,
                         int32_t* dst, int32_t* src, int32_t* count, int32_t* sigma) {
   const size_t lds = sizeof(int32_t) * (size_t)(rhi - rlo);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)k_perm, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
-    attr = true;
-  }
+  static std::once_flag attr;
+  std::call_once(attr, [] {
+    (void)hipFuncSetAttribute((const void*)k_perm, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  });
   MNP_CHECK(lds <= 160 * 1024, MNP_ERR_ARG, "permutation longer than 40960 rows");
   hipLaunchKernelGGL(k_perm, dim3(1), dim3(256), lds, stream(), k0, k1, rlo, rhi, ipiv, dst, src,
                      count, sigma);

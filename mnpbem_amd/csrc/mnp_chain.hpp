@@ -4,7 +4,9 @@
 #pragma once
 
 #include <cmath>
+#include <functional>
 #include <string>
+#include <vector>
 
 #include "mnp_greenret.hpp"
 #include "mnp_linalg.hpp"
@@ -36,14 +38,15 @@ struct Chain {
   bool exact_pivoting = true;  // FP64: factors must equal partial pivoting's
   cplx<T>*G1 = nullptr, *G2 = nullptr, *S1 = nullptr, *S2 = nullptr, *D = nullptr, *Di = nullptr;
   cplx<T>* backup = nullptr;   // copy of a matrix while it is factorised without pivoting
+  cplx<T>* backup2 = nullptr;  // the same for the G2 branch, factorised concurrently with G1
   cplx<T>* inv = nullptr;      // 32-block inverses: 4 factors x (L, U)
   cplx<T>* invnb = nullptr;    // 256-block inverses: 4 factors x (L, U)
-  int32_t *piv = nullptr, *sig[4] = {nullptr, nullptr, nullptr, nullptr};
+  int32_t *piv = nullptr, *piv2 = nullptr, *sig[4] = {nullptr, nullptr, nullptr, nullptr};
   LU<T> f[4];                  // G1, G2, Sigma1 - Sigma2, Sigma
   int fallbacks = 0;           // factorisations redone with partial pivoting
   double2 eps1{}, eps2{};
   double k0 = 0;
-  Workspace ws;
+  Workspace ws, ws2;  // ws2: the G2 branch
 
   // sigma_only: only the buffers of the Sigma factorisation (S2 holds Sigma,
   // factor slot 3), for Sigma matrices assembled from other chains' parts.
@@ -55,6 +58,8 @@ struct Chain {
       S1 = dev_alloc<cplx<T>>(nn);
       D = dev_alloc<cplx<T>>(nn);
       Di = dev_alloc<cplx<T>>(nn);
+      backup2 = dev_alloc<cplx<T>>(nn);
+      piv2 = dev_alloc<int32_t>(n);
     }
     S2 = dev_alloc<cplx<T>>(nn);
     backup = dev_alloc<cplx<T>>(nn);
@@ -64,37 +69,52 @@ struct Chain {
     for (auto& q : sig) q = dev_alloc<int32_t>(n);
   }
   ~Chain() {
-    void* ps[] = {G1, G2, S1, S2, D, Di, backup, inv, invnb, piv, sig[0], sig[1], sig[2], sig[3]};
+    void* ps[] = {G1, G2, S1, S2, D, Di, backup, backup2, inv, invnb, piv, piv2, sig[0], sig[1], sig[2], sig[3]};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
   MatView<T> v(cplx<T>* p) const { return MatView<T>{p, n}; }
 
-  // Factorises A in place as factor slot `s`.  First without row exchanges
-  // (no grid-wide pivot search); when partial pivoting would have exchanged
-  // rows and exact pivoting is required, or on a zero pivot, the matrix is
-  // restored and factorised again with partial pivoting.
-  const LU<T>& factor(int s, cplx<T>* A, const char* what) {
+  // Factorises A in place as factor slot `s` without row exchanges (scratch
+  // bk, pv, w).  Returns false, with A restored, when partial pivoting would
+  // have exchanged rows and exact pivoting is required, or on a zero pivot.
+  bool factor_try(int s, cplx<T>* A, cplx<T>* bk, int32_t* pv, Workspace& w) {
     cplx<T>* Linv = inv + (2 * s) * inv_blocks(n);
     cplx<T>* Uinv = Linv + inv_blocks(n);
-    copy_mat<T>(n, n, v(A), v(backup));
+    copy_mat<T>(n, n, v(A), v(bk));
     int swap = 0;
-    int info = getrf<T>(n, n, v(A), piv, Linv, Uinv, PIVOT_NONE, &swap, ws);
-    const int32_t* sigma = nullptr;
+    const int info = getrf<T>(n, n, v(A), pv, Linv, Uinv, PIVOT_NONE, &swap, w);
     if (info != 0 || (swap && exact_pivoting)) {
-      ++fallbacks;
-      copy_mat<T>(n, n, v(backup), v(A));
-      info = getrf<T>(n, n, v(A), piv, Linv, Uinv, PIVOT_PARTIAL, nullptr, ws);
-      perm_from_pivots(n, piv, sig[s], ws);
-      sigma = sig[s];
+      copy_mat<T>(n, n, v(bk), v(A));
+      return false;
     }
+    cplx<T>* LinvNB = invnb + (2 * s) * inv_blocks_nb(n);
+    cplx<T>* UinvNB = LinvNB + inv_blocks_nb(n);
+    block_inverses_nb<T>(n, v(A), Linv, Uinv, LinvNB, UinvNB, w);
+    f[s] = LU<T>{v(A), Linv, Uinv, nullptr, LinvNB, UinvNB};
+    return true;
+  }
+  // Partial pivoting after factor_try declined.
+  const LU<T>& factor_pivoted(int s, cplx<T>* A, const char* what) {
+    ++fallbacks;
+    cplx<T>* Linv = inv + (2 * s) * inv_blocks(n);
+    cplx<T>* Uinv = Linv + inv_blocks(n);
+    const int info = getrf<T>(n, n, v(A), piv, Linv, Uinv, PIVOT_PARTIAL, nullptr, ws);
     MNP_CHECK(info == 0, MNP_ERR_SINGULAR,
               std::string("bemret: singular ") + what + " (zero pivot " + std::to_string(info) + ")");
+    perm_from_pivots(n, piv, sig[s], ws);
     cplx<T>* LinvNB = invnb + (2 * s) * inv_blocks_nb(n);
     cplx<T>* UinvNB = LinvNB + inv_blocks_nb(n);
     block_inverses_nb<T>(n, v(A), Linv, Uinv, LinvNB, UinvNB, ws);
-    f[s] = LU<T>{v(A), Linv, Uinv, sigma, LinvNB, UinvNB};
+    f[s] = LU<T>{v(A), Linv, Uinv, sig[s], LinvNB, UinvNB};
     return f[s];
+  }
+  // Factorises A in place as factor slot `s`: without row exchanges first
+  // (no grid-wide pivot search), with partial pivoting when factor_try
+  // declines.
+  const LU<T>& factor(int s, cplx<T>* A, const char* what) {
+    if (factor_try(s, A, backup, piv, ws)) return f[s];
+    return factor_pivoted(s, A, what);
   }
 };
 
@@ -130,12 +150,33 @@ inline dim3 gridnn(int64_t m, int64_t n) {
 // H1 G1^{-1} (over H1 in S1), LU(G2), Sigma2 = H2 G2^{-1} (over H2 in S2),
 // LU(Sigma1 - Sigma2) and Deltai.  On entry G1, G2, S1 (= H1), S2 (= H2)
 // hold the assembled matrices.
+// The G1 and G2 branches are independent.  With `concurrent` they run at the
+// same time, each on its own stream, so that the latency-bound kernels of one
+// (diagonal-block LUs, narrow products) overlap the large products of the
+// other; the large products then share the GPU, and their per-launch times
+// no longer measure the kernel alone, so the dense solvers keep them in
+// order.  A branch whose factorisation needs partial pivoting is redone
+// afterwards, alone.
 template <class T>
-void chain_sector(Chain<T>& c) {
+void chain_sector(Chain<T>& c, bool concurrent = false) {
   const int64_t n = c.n;
   c.fallbacks = 0;
-  getrs_right<T>(n, n, c.factor(0, c.G1, "G1"), c.v(c.S1), c.ws);  // Sigma1 = H1 G1^{-1}
-  getrs_right<T>(n, n, c.factor(1, c.G2, "G2"), c.v(c.S2), c.ws);  // Sigma2 = H2 G2^{-1}
+  bool ok[2] = {false, false};
+  std::vector<std::function<void()>> br = {
+      [&] {
+        ok[0] = c.factor_try(0, c.G1, c.backup, c.piv, c.ws);
+        if (ok[0]) getrs_right<T>(n, n, c.f[0], c.v(c.S1), c.ws);  // Sigma1 = H1 G1^{-1}
+      },
+      [&] {
+        ok[1] = c.factor_try(1, c.G2, c.backup2, c.piv2, c.ws2);
+        if (ok[1]) getrs_right<T>(n, n, c.f[1], c.v(c.S2), c.ws2);  // Sigma2 = H2 G2^{-1}
+      }};
+  if (concurrent)
+    run_concurrent(br);
+  else
+    for (auto& t : br) t();
+  if (!ok[0]) getrs_right<T>(n, n, c.factor_pivoted(0, c.G1, "G1"), c.v(c.S1), c.ws);
+  if (!ok[1]) getrs_right<T>(n, n, c.factor_pivoted(1, c.G2, "G2"), c.v(c.S2), c.ws);
   hipLaunchKernelGGL(k_sub<T>, gridnn(n, n), dim3(256), 0, stream(), n, c.S1, c.S2, c.D);
   MNP_HIP(hipGetLastError());
   getri<T>(n, c.factor(2, c.D, "Sigma1 - Sigma2"), c.v(c.Di), c.ws);  // Deltai

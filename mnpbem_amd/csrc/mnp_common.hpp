@@ -5,7 +5,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
+#include <vector>
 
 #include "../../include/mnpbem_hip.h"
 
@@ -13,6 +15,14 @@ namespace mnp {
 
 // ----------------------------------------------------------------- errors
 void set_error(const std::string& msg);
+
+// Runs independent host tasks that enqueue GPU work: tasks[0] on the calling
+// thread and stream, each other task on a worker thread with a stream of its
+// own that starts after the work the caller's stream holds so far; the
+// caller's stream then waits for all of them.  At most 4 tasks; called from a
+// worker (nested) the tasks run one after another.  The first exception is
+// rethrown after every task has ended and its stream has drained.
+void run_concurrent(std::vector<std::function<void()>>& tasks);
 
 struct Error {
   int code;
