@@ -205,19 +205,27 @@ struct BemRetMirror {
     // their large products are small enough (n / J) that overlapping the
     // chains' latency-bound kernels pays
     SplitBf16Scope split(solver == MNP_SOLVER_MIXED_BF16);
+    // (factorisations that need partial pivoting are finished afterwards, one
+    // chain at a time: the pivoted panel is a cooperative launch)
     std::vector<std::function<void()>> tasks;
     for (int s = 0; s < J; ++s) {
       if (solver == MNP_SOLVER_FP64) {
         Chain<double>& c = *s64[s];
         c.eps1 = e1, c.eps2 = e2, c.k0 = k0;
-        tasks.push_back([&c] { chain_sector<double>(c); });
+        tasks.push_back([&c] { chain_sector_try<double>(c); });
       } else {
         Chain<float>& c = *s32[s];
         c.eps1 = e1, c.eps2 = e2, c.k0 = k0;
-        tasks.push_back([&c] { chain_sector<float>(c); });
+        tasks.push_back([&c] { chain_sector_try<float>(c); });
       }
     }
     run_concurrent(tasks);
+    for (int s = 0; s < J; ++s) {
+      if (solver == MNP_SOLVER_FP64)
+        chain_sector_finish<double>(*s64[s]);
+      else
+        chain_sector_finish<float>(*s32[s]);
+    }
     ready = true;
   }
 

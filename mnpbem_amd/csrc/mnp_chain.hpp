@@ -44,6 +44,7 @@ struct Chain {
   int32_t *piv = nullptr, *piv2 = nullptr, *sig[4] = {nullptr, nullptr, nullptr, nullptr};
   LU<T> f[4];                  // G1, G2, Sigma1 - Sigma2, Sigma
   int fallbacks = 0;           // factorisations redone with partial pivoting
+  bool ok_g1 = false, ok_g2 = false, ok_d = false;  // chain_sector_try's progress
   double2 eps1{}, eps2{};
   double k0 = 0;
   Workspace ws, ws2;  // ws2: the G2 branch
@@ -150,36 +151,50 @@ inline dim3 gridnn(int64_t m, int64_t n) {
 // H1 G1^{-1} (over H1 in S1), LU(G2), Sigma2 = H2 G2^{-1} (over H2 in S2),
 // LU(Sigma1 - Sigma2) and Deltai.  On entry G1, G2, S1 (= H1), S2 (= H2)
 // hold the assembled matrices.
-// The G1 and G2 branches are independent.  With `concurrent` they run at the
-// same time, each on its own stream, so that the latency-bound kernels of one
-// (diagonal-block LUs, narrow products) overlap the large products of the
-// other; the large products then share the GPU, and their per-launch times
-// no longer measure the kernel alone, so the dense solvers keep them in
-// order.  A branch whose factorisation needs partial pivoting is redone
-// afterwards, alone.
+// The parts of the chain that precede Sigma: LU(G1), Sigma1 = H1 G1^{-1}
+// (over H1 in S1), LU(G2), Sigma2 = H2 G2^{-1} (over H2 in S2), LU(Sigma1 -
+// Sigma2) and Deltai.  On entry G1, G2, S1 (= H1), S2 (= H2) hold the
+// assembled matrices.
+//
+// chain_sector_try does what needs no partial pivoting (no grid-wide pivot
+// search, no cooperative launch), so that several chains may run it at once
+// on their own streams; it returns false when a factorisation must be redone
+// with partial pivoting, which chain_sector_finish then does, alone.
 template <class T>
-void chain_sector(Chain<T>& c, bool concurrent = false) {
+bool chain_sector_try(Chain<T>& c) {
   const int64_t n = c.n;
   c.fallbacks = 0;
-  bool ok[2] = {false, false};
-  std::vector<std::function<void()>> br = {
-      [&] {
-        ok[0] = c.factor_try(0, c.G1, c.backup, c.piv, c.ws);
-        if (ok[0]) getrs_right<T>(n, n, c.f[0], c.v(c.S1), c.ws);  // Sigma1 = H1 G1^{-1}
-      },
-      [&] {
-        ok[1] = c.factor_try(1, c.G2, c.backup2, c.piv2, c.ws2);
-        if (ok[1]) getrs_right<T>(n, n, c.f[1], c.v(c.S2), c.ws2);  // Sigma2 = H2 G2^{-1}
-      }};
-  if (concurrent)
-    run_concurrent(br);
-  else
-    for (auto& t : br) t();
-  if (!ok[0]) getrs_right<T>(n, n, c.factor_pivoted(0, c.G1, "G1"), c.v(c.S1), c.ws);
-  if (!ok[1]) getrs_right<T>(n, n, c.factor_pivoted(1, c.G2, "G2"), c.v(c.S2), c.ws);
+  c.ok_g1 = c.ok_g2 = c.ok_d = false;
+  c.ok_g1 = c.factor_try(0, c.G1, c.backup, c.piv, c.ws);
+  if (c.ok_g1) getrs_right<T>(n, n, c.f[0], c.v(c.S1), c.ws);  // Sigma1 = H1 G1^{-1}
+  c.ok_g2 = c.factor_try(1, c.G2, c.backup2, c.piv2, c.ws2);
+  if (c.ok_g2) getrs_right<T>(n, n, c.f[1], c.v(c.S2), c.ws2);  // Sigma2 = H2 G2^{-1}
+  if (!c.ok_g1 || !c.ok_g2) return false;
   hipLaunchKernelGGL(k_sub<T>, gridnn(n, n), dim3(256), 0, stream(), n, c.S1, c.S2, c.D);
   MNP_HIP(hipGetLastError());
-  getri<T>(n, c.factor(2, c.D, "Sigma1 - Sigma2"), c.v(c.Di), c.ws);  // Deltai
+  c.ok_d = c.factor_try(2, c.D, c.backup, c.piv, c.ws);
+  if (c.ok_d) getri<T>(n, c.f[2], c.v(c.Di), c.ws);  // Deltai
+  return c.ok_d;
+}
+template <class T>
+void chain_sector_finish(Chain<T>& c) {
+  const int64_t n = c.n;
+  if (c.ok_d) return;
+  const bool d_formed = c.ok_g1 && c.ok_g2;
+  if (!c.ok_g1) getrs_right<T>(n, n, c.factor_pivoted(0, c.G1, "G1"), c.v(c.S1), c.ws);
+  if (!c.ok_g2) getrs_right<T>(n, n, c.factor_pivoted(1, c.G2, "G2"), c.v(c.S2), c.ws);
+  if (!d_formed) {
+    hipLaunchKernelGGL(k_sub<T>, gridnn(n, n), dim3(256), 0, stream(), n, c.S1, c.S2, c.D);
+    MNP_HIP(hipGetLastError());
+    getri<T>(n, c.factor(2, c.D, "Sigma1 - Sigma2"), c.v(c.Di), c.ws);
+  } else {
+    getri<T>(n, c.factor_pivoted(2, c.D, "Sigma1 - Sigma2"), c.v(c.Di), c.ws);
+  }
+  c.ok_d = true;
+}
+template <class T>
+void chain_sector(Chain<T>& c) {
+  if (!chain_sector_try(c)) chain_sector_finish(c);
 }
 
 // Factorises the chain (chain_sector, then Sigma over S2 and its LU).
