@@ -272,6 +272,59 @@ def run_substrate(level=5, steps=3):
             "max_rel_err_vs_fp64": err["mixed"], "max_rel_err_mixed_bf16_vs_fp64": err["mixed_bf16"]}
 
 
+def run_sensitivity(nsweep=32):
+    """Extinction and its Drude-parameter gradient (one tangent solve with the
+    forward factors per wavelength) on the 5,120-face ellipsoid 10 x 10 x 40 nm
+    in eps = 1.77, pol x (dir z) and z (dir x): 650 nm, checked against central
+    differences of the FP64 solver, and a sweep of linspace(500, 1000, 32)
+    timed with the mixed and FP64 solvers."""
+    import torch
+
+    from mnpbem_amd import bem as B, shapes
+    from mnpbem_amd.sensitivity import drude_extinction_gradient
+
+    v, f = shapes.ellipsoid(4, (10.0, 10.0, 40.0))
+
+    def make(prec):
+        op = B.bemoptions(sim="ret")
+        p = B.comparticle([B.epsconst(EPS_OUT), B.epsdrude(*DRUDE)], [(v, f)], [2, 1], 1, op)
+        return p, B.bemsolver(p, op, precision=prec), B.planewave(POL, DIR, op)
+
+    p, b64, exc = make("fp64")
+    ext, grad = drude_extinction_gradient(exc, b64, 650.0)
+    fd = np.zeros_like(grad)
+    for j in range(3):
+        h = 1e-6 * DRUDE[j]
+        vals = []
+        for sgn in (1, -1):
+            par = list(DRUDE)
+            par[j] += sgn * h
+            p.eps[1] = B.epsdrude(*par)
+            b64.enei = None
+            vals.append(np.asarray(exc.cross_sections(b64.solve(exc(p, 650.0)))[0]))
+        fd[:, j] = (vals[0] - vals[1]) / (2 * h)
+    p.eps[1] = B.epsdrude(*DRUDE)
+    err_fd = float(np.max(np.abs(grad - fd) / np.abs(fd).max(axis=0, keepdims=True)))
+    lams = np.linspace(500.0, 1000.0, nsweep)
+    res = {}
+    for prec, nl in (("mixed", nsweep), ("fp64", 4)):
+        p, bem, exc = make(prec)
+        drude_extinction_gradient(exc, bem, lams[0])  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for lam in lams[:nl]:
+            drude_extinction_gradient(exc, bem, lam)
+        torch.cuda.synchronize()
+        res[prec] = nl / (time.perf_counter() - t)
+        del bem
+    log(f"[bench] sensitivity: gradient vs FD rel err {err_fd:.2e}; mixed {res['mixed']:.2f} λ/s, "
+        f"fp64 {res['fp64']:.2f} λ/s (extinction + Drude gradient)")
+    return {"workload": f"d(ext)/d(eps_inf, wp, gamma), ellipsoid 10x10x40 nm ({len(f)} faces) in eps=1.77, "
+                        "650 nm FD check and linspace(500,1000,32) sweep",
+            "wavelengths_per_s_mixed": res["mixed"], "wavelengths_per_s_fp64": res["fp64"],
+            "max_rel_err_vs_central_fd_fp64": err_fd, "gradient_650nm": grad.tolist(), "ext_650nm": ext.tolist()}
+
+
 def run_multi_particle(per_particle=8):
     """The seeded many-particle sweep (12 Drude particles, 320-5,120 faces, in
     vacuum): per_particle of its 128 wavelengths (every 16th) per particle
@@ -473,11 +526,12 @@ def main():
     ap.add_argument("--no-multi-particle", action="store_true", help="skip the many-particle sweep measurement")
     ap.add_argument("--no-mirror", action="store_true", help="skip the mirror-symmetry measurement")
     ap.add_argument("--no-substrate", action="store_true", help="skip the substrate measurement")
+    ap.add_argument("--no-sensitivity", action="store_true", help="skip the Drude-gradient measurement")
     ap.add_argument("--config4-only", action="store_true",
                     help="only the H-matrix dipole measurement (for profiling)")
     ap.add_argument("--only", default=None,
                     help="comma list of side measurements to run alone (for profiling): "
-                         "quasistatic, mirror, substrate, multi_particle, config4")
+                         "quasistatic, mirror, substrate, multi_particle, config4, sensitivity")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy three times (counter calibration)")
     a = ap.parse_args()
@@ -500,6 +554,7 @@ def main():
     if a.only:
         runs = {"quasistatic": lambda: run_quasistatic(a.level), "mirror": run_mirror,
                 "substrate": lambda: run_substrate(a.level), "multi_particle": run_multi_particle,
+                "sensitivity": run_sensitivity,
                 "config4": run_config4}
         names = [x.strip() for x in a.only.split(",") if x.strip()]
         bad = [x for x in names if x not in runs]
@@ -587,6 +642,8 @@ def main():
         out["mirror"] = run_mirror()
     if not a.no_substrate:
         out["substrate"] = run_substrate(a.level)
+    if not a.no_sensitivity:
+        out["sensitivity"] = run_sensitivity()
     if not a.no_multi_particle:
         out["multi_particle"] = run_multi_particle()
     if not a.no_config4:

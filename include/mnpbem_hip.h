@@ -168,6 +168,19 @@ int mnp_bemret_solve(mnp_bemret_t bem, int64_t npol, const void* phi1, const voi
 int mnp_bemret_stats(mnp_bemret_t bem, int* iterations, double* relres);
 /* Refinement settings for MNP_SOLVER_MIXED (defaults: 1e-12, 10). */
 int mnp_bemret_set_refinement(mnp_bemret_t bem, double tol, int maxit);
+/* Sensitivity of a solution to the inside dielectric function (no counterpart
+ * in the reference; it differentiates the operator that bemretiter/private/
+ * afun.m:6-35 applies and initmat.m:5-72 inverts).  Given the solution
+ * sig1, sig2, h1, h2 of mnp_bemret_solve at the current wavelength for an
+ * excitation from outside (phi1, a1 and their derivatives zero), returns its
+ * complex derivative with respect to eps1: A dx = -(dA/deps1) x, with eps1
+ * entering A through k1 = k0 sqrt(eps1) in G1, H1 (their k-derivatives are
+ * assembled, refined terms included) and explicitly in the alpha and De
+ * rows.  The solve reuses the forward factors (and, for MIXED, the FP64
+ * refinement).  Layouts as mnp_bemret_solve's outputs. */
+int mnp_bemret_tangent_eps1(mnp_bemret_t bem, int64_t npol, const void* sig1, const void* sig2,
+                            const void* h1, const void* h2, void* dsig1, void* dsig2, void* dh1,
+                            void* dh2);
 
 /* ------------------------------------------------ dense building blocks
  * Replace the LAPACK/BLAS calls behind Matlab's inv and mtimes in

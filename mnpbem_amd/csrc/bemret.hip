@@ -124,6 +124,53 @@ __global__ void k_unpack(int64_t n, int64_t p, const double2* x, double2* sig1, 
   }
 }
 
+__global__ void k_pack(int64_t n, int64_t p, const double2* sig1, const double2* h1, const double2* sig2,
+                       const double2* h2, double2* x) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t col = blockIdx.y;  // 0 .. 8p
+  if (i >= n) return;
+  double2 v;
+  if (col < p)
+    v = sig1[col * n + i];
+  else if (col < 4 * p)
+    v = h1[(col - p) * n + i];
+  else if (col < 5 * p)
+    v = sig2[(col - 4 * p) * n + i];
+  else
+    v = h2[(col - 5 * p) * n + i];
+  x[col * n + i] = v;
+}
+
+// r = -(dA/deps1) x in the layout of b (see k_residual for A): with
+// GX = G1 [sig1 | h1], HX = H1 [sig1 | h1] and dGX, dHX their derivatives
+// along eps1 (through k1),
+//   phi:   dGs1                a:  dGh1
+//   alpha: dHh1 - i k0 n (Gs1 + eps1 dGs1)
+//   De:    Hs1 + eps1 dHs1 - i k0 n . (Gh1 + eps1 dGh1)
+__global__ void k_deps1(int64_t n, int64_t p, const double2* GX, const double2* HX, const double2* dGX,
+                        const double2* dHX, double2* r, const double* nvec, double k0, double2 e1) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  const double2 ik = mk<double>(0, k0);
+  const double2 Gs = GX[q * n + i], dGs = dGX[q * n + i];
+  const double2 Hs = HX[q * n + i], dHs = dHX[q * n + i];
+  r[q * n + i] = mk<double>(-dGs.x, -dGs.y);
+  const double2 ts = cadd<double>(Gs, cmul<double>(e1, dGs));  // Gs1 + eps1 dGs1
+  double2 nGh = mk<double>(0, 0);
+  for (int c = 0; c < 3; ++c) {
+    const int64_t col = c + 3 * q;
+    const double nc = nvec[c * n + i];
+    const double2 Gh = GX[(p + col) * n + i], dGh = dGX[(p + col) * n + i], dHh = dHX[(p + col) * n + i];
+    r[(p + col) * n + i] = mk<double>(-dGh.x, -dGh.y);
+    const double2 al = csub<double>(dHh, cmul<double>(ik, cscale<double>(ts, nc)));
+    r[(5 * p + col) * n + i] = mk<double>(-al.x, -al.y);
+    nGh = cadd<double>(nGh, cscale<double>(cadd<double>(Gh, cmul<double>(e1, dGh)), nc));
+  }
+  const double2 De = csub<double>(cadd<double>(Hs, cmul<double>(e1, dHs)), cmul<double>(ik, nGh));
+  r[(4 * p + q) * n + i] = mk<double>(-De.x, -De.y);
+}
+
 double dev_norm2(int64_t len, const double2* v, Workspace& ws, int slot) {
   const int nb = 512;
   double* part = (double*)ws.get(slot, sizeof(double) * nb);
@@ -231,6 +278,16 @@ struct BemRet {
                        stream(), n, p, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], b,
                        g->geo.nvec, k0, eps1, eps2);
     MNP_HIP(hipGetLastError());
+    solve_b(p, b, x);
+    hipLaunchKernelGGL(k_unpack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0,
+                       stream(), n, p, x, sig1, h1, sig2, h2);
+    MNP_HIP(hipGetLastError());
+  }
+
+  // x = A^{-1} b for b, x n x 8p (layouts of chain_apply); b is not modified
+  void solve_b(int64_t p, const double2* b, double2* x) {
+    const int64_t len = 8 * p * n;
+    const double k0 = 2 * M_PI / enei;
     const size_t tmp_elems = (size_t)(18 * p * n);
     last_iter = 0;
     last_relres = 0;
@@ -279,10 +336,58 @@ struct BemRet {
       MNP_CHECK(!(last_relres > tol), MNP_ERR_NOCONV,
                 "bemret_solve: refinement stalled at relative residual " + sci(last_relres));
     }
-    hipLaunchKernelGGL(k_unpack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0,
-                       stream(), n, p, x, sig1, h1, sig2, h2);
+  }
+
+  // Derivative of the solution x = [sig1 | h1 | sig2 | h2] with respect to
+  // the inside dielectric function eps1 at the current wavelength (complex
+  // derivative; A(eps1) x = b with b from the outside excitation):
+  //   A dx = -(dA/deps1) x,
+  // solved with the factors of the forward solve.  eps1 enters A through
+  // k1 = k0 sqrt(eps1) in G1, H1 (dk1/deps1 = k1 / (2 eps1)) and explicitly
+  // in the alpha and De rows.
+  void tangent_eps1(int64_t p, const double2* sig1, const double2* sig2, const double2* h1, const double2* h2,
+                    double2* dsig1, double2* dsig2, double2* dh1, double2* dh2) {
+    MNP_CHECK(ready, MNP_ERR_STATE, "bemret_tangent: initmat not called");
+    MNP_CHECK(p > 0 && p <= 64, MNP_ERR_ARG, "bemret_tangent: npol must be in 1..64");
+    const int64_t len = 8 * p * n, nn = n * n;
+    const double k0 = 2 * M_PI / enei;
+    const double2 k1 = wavenumber(k0, eps1);
+    const double2 dk = cdiv<double>(k1, cscale<double>(eps1, 2.0));  // dk1 / deps1
+    double2* x = (double2*)tws.get(0, sizeof(double2) * len);
+    double2* rhs = (double2*)tws.get(1, sizeof(double2) * len);
+    double2* dx = (double2*)tws.get(2, sizeof(double2) * len);
+    double2* prod = (double2*)tws.get(3, sizeof(double2) * 16 * p * n);  // G1 x1, H1 x1, dG1 x1, dH1 x1
+    double2* dG = (double2*)tws.get(4, sizeof(double2) * nn);
+    double2* dH = (double2*)tws.get(5, sizeof(double2) * nn);
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n, p,
+                       sig1, h1, sig2, h2, x);
+    MNP_HIP(hipGetLastError());
+    // G1, H1 themselves: kept by the mixed solver, re-assembled for the FP64 one
+    const double2 *G1m = G1, *H1m = H1;
+    if (!G1m) {
+      double2* tg = (double2*)tws.get(6, sizeof(double2) * nn);
+      double2* th = (double2*)tws.get(7, sizeof(double2) * nn);
+      g->assemble(k1, 2 * M_PI, OutPair{tg, nullptr}, OutPair{th, nullptr}, n);
+      G1m = tg;
+      H1m = th;
+    }
+    g->assemble_dk(k1, OutPair{dG, nullptr}, OutPair{dH, nullptr}, n);
+    const double2 one = {1, 0}, zero = {0, 0};
+    const MatView<double> X1{x, n};
+    double2 *GX = prod, *HX = prod + 4 * p * n, *dGX = prod + 8 * p * n, *dHX = prod + 12 * p * n;
+    gemm<double>(n, 4 * p, n, one, MatView<double>{const_cast<double2*>(G1m), n}, X1, zero, MatView<double>{GX, n});
+    gemm<double>(n, 4 * p, n, one, MatView<double>{const_cast<double2*>(H1m), n}, X1, zero, MatView<double>{HX, n});
+    gemm<double>(n, 4 * p, n, dk, MatView<double>{dG, n}, X1, zero, MatView<double>{dGX, n});
+    gemm<double>(n, 4 * p, n, dk, MatView<double>{dH, n}, X1, zero, MatView<double>{dHX, n});
+    hipLaunchKernelGGL(k_deps1, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0, stream(), n, p, GX, HX,
+                       dGX, dHX, rhs, g->geo.nvec, k0, eps1);
+    MNP_HIP(hipGetLastError());
+    solve_b(p, rhs, dx);
+    hipLaunchKernelGGL(k_unpack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n, p,
+                       dx, dsig1, dh1, dsig2, dh2);
     MNP_HIP(hipGetLastError());
   }
+  Workspace tws;
   Workspace res_ws;
 };
 
@@ -443,6 +548,16 @@ int mnp_bemret_stats(mnp_bemret_t bem, int* iterations, double* relres) {
     MNP_CHECK(bem, MNP_ERR_ARG, "bemret_stats: NULL handle");
     if (iterations) *iterations = bem->b->last_iter;
     if (relres) *relres = bem->b->last_relres;
+  });
+}
+
+int mnp_bemret_tangent_eps1(mnp_bemret_t bem, int64_t npol, const void* sig1, const void* sig2, const void* h1,
+                            const void* h2, void* dsig1, void* dsig2, void* dh1, void* dh2) {
+  return guarded([&] {
+    MNP_CHECK(bem && sig1 && sig2 && h1 && h2 && dsig1 && dsig2 && dh1 && dh2, MNP_ERR_ARG,
+              "bemret_tangent_eps1: NULL argument");
+    bem->b->tangent_eps1(npol, (const double2*)sig1, (const double2*)sig2, (const double2*)h1, (const double2*)h2,
+                         (double2*)dsig1, (double2*)dsig2, (double2*)dh1, (double2*)dh2);
   });
 }
 

@@ -614,7 +614,10 @@ __global__ void k_closed_apply(int64_t n, Geo geo, Pairs pr, int64_t nnz, const 
 // one element of up to two double-precision matrices and up to two
 // single-precision mirrors (the mixed chain's inputs); NULL outputs are
 // skipped.  Refined elements are overwritten by k_assemble_refined.
-template <class TG>
+// With DK the kernel writes the derivatives with respect to k instead:
+// dG/dk = i area_j e^{ikd}, dF/dk = -k (n_i . (pos_i - pos_j)) area_j e^{ikd} / d
+// (the +-2 pi term does not depend on k).
+template <class TG, bool DK = false>
 __global__ void __launch_bounds__(256) k_assemble(int64_t n, int64_t nrows, Geo geo, double2 k, double sgn2pi,
                                                   TG outG, TG outH, int64_t ld) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -627,6 +630,16 @@ __global__ void __launch_bounds__(256) k_assemble(int64_t n, int64_t nrows, Geo 
   const double aj = geo.area[j];
   const double2 ph = cexp_ikd(k, d);
   const int64_t idx = j * ld + i;
+  if constexpr (DK) {
+    if (outG.any()) outG.store(idx, mk<double>(-aj * ph.y, aj * ph.x));
+    if (outH.any()) {
+      const double in = dx * geo.nvec[i] + dy * geo.nvec[n + i] + dz * geo.nvec[2 * n + i];
+      const double2 kp = cmul<double>(k, ph);
+      const double s = -in * aj / d;
+      outH.store(idx, mk<double>(s * kp.x, s * kp.y));
+    }
+    return;
+  }
   if (outG.any()) {
     const double gr = (1.0 / d) * aj;
     outG.store(idx, mk<double>(gr * ph.x, gr * ph.y));
@@ -646,7 +659,9 @@ __global__ void __launch_bounds__(256) k_assemble(int64_t n, int64_t nrows, Geo 
 
 // Overwrites refined elements in rows i < nrows: G = (sum_n g_n (ik)^n) e^{ikd},
 // F = (n_i . sum_n f_n (ik)^n) e^{ikd}, H = F +- 2 pi on coincident pairs.
-template <class TG>
+// With DK, the derivatives with respect to k: d/dk [P(ik) e^{ikd}] =
+// (i P'(ik) + i d P(ik)) e^{ikd} for P = sum_n c_n z^n.
+template <class TG, bool DK = false>
 __global__ void __launch_bounds__(256) k_assemble_refined(int64_t nnz, int64_t n, int64_t nrows, Geo geo,
                                                           Pairs pr, int order, const double* __restrict__ gt,
                                                           const double* __restrict__ ft, double2 k,
@@ -662,6 +677,7 @@ __global__ void __launch_bounds__(256) k_assemble_refined(int64_t nnz, int64_t n
   const double2 ph = cexp_ikd(k, d);
   const double2 ik = mk<double>(-k.y, k.x);
   double2 pw = mk<double>(1, 0), G = mk<double>(0, 0), F = mk<double>(0, 0);
+  double2 dG = mk<double>(0, 0), dF = mk<double>(0, 0), pwm = mk<double>(0, 0);  // pwm = (ik)^(o-1)
   const double n0 = geo.nvec[i], n1 = geo.nvec[n + i], n2 = geo.nvec[2 * n + i];
   for (int o = 0; o <= order; ++o) {
     const double go = gt[o * nnz + p];
@@ -669,9 +685,23 @@ __global__ void __launch_bounds__(256) k_assemble_refined(int64_t nnz, int64_t n
                       n2 * ft[(o * 3 + 2) * nnz + p];
     G = cadd<double>(G, cscale<double>(pw, go));
     F = cadd<double>(F, cscale<double>(pw, fo));
+    if constexpr (DK) {
+      dG = cadd<double>(dG, cscale<double>(pwm, o * go));
+      dF = cadd<double>(dF, cscale<double>(pwm, o * fo));
+      pwm = pw;
+    }
     pw = cmul<double>(pw, ik);
   }
   const int64_t idx = j * ld + i;
+  if constexpr (DK) {
+    // i P'(ik) + i d P(ik)
+    const double2 i1 = mk<double>(0, 1), id = mk<double>(0, d);
+    const double2 gk = cadd<double>(cmul<double>(i1, dG), cmul<double>(id, G));
+    const double2 fk = cadd<double>(cmul<double>(i1, dF), cmul<double>(id, F));
+    if (outG.any()) outG.store(idx, cmul<double>(gk, ph));
+    if (outH.any()) outH.store(idx, cmul<double>(fk, ph));
+    return;
+  }
   if (outG.any()) outG.store(idx, cmul<double>(G, ph));
   if (outH.any()) {
     double2 H = cmul<double>(F, ph);
@@ -683,6 +713,19 @@ __global__ void __launch_bounds__(256) k_assemble_refined(int64_t nnz, int64_t n
 // ------------------------------------------------------------- host side
 void GreenRet::assemble(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t ld) const {
   assemble_rows(k, sgn2pi, g, h, n, ld);
+}
+
+void GreenRet::assemble_dk(double2 k, OutPair g, OutPair h, int64_t ld) const {
+  if (!g.any() && !h.any()) return;
+  MNP_CHECK(n <= 65535, MNP_ERR_ARG, "greenret: dense assembly supports at most 65535 faces");
+  hipLaunchKernelGGL((k_assemble<OutPair, true>), dim3((unsigned)cdiv_up(n, 256), (unsigned)n), dim3(256), 0,
+                     stream(), n, n, geo, k, 0.0, g, h, ld);
+  MNP_HIP(hipGetLastError());
+  if (nnz) {
+    hipLaunchKernelGGL((k_assemble_refined<OutPair, true>), dim3((unsigned)cdiv_up(nnz, 256)), dim3(256), 0,
+                       stream(), nnz, n, n, geo, pairs, order, gtab, ftab, k, 0.0, g, h, ld);
+    MNP_HIP(hipGetLastError());
+  }
 }
 
 void GreenRet::assemble_rows(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t nrows, int64_t ld) const {

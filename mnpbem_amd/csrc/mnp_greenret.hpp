@@ -71,6 +71,8 @@ struct GreenRet {
   void assemble(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t ld) const;
   // Rows 0 .. nrows-1 of the same matrices (nrows x n, leading dimension ld).
   void assemble_rows(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t nrows, int64_t ld) const;
+  // dG/dk into g and dF/dk (= dH/dk) into h, n x n.
+  void assemble_dk(double2 k, OutPair g, OutPair h, int64_t ld) const;
 };
 
 }  // namespace mnp
