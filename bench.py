@@ -360,7 +360,28 @@ def run_multi_particle(per_particle=8):
             del bem, p
         res[prec] = (len(parts) * len(lams) / tot, np.array(ext))
         out[f"particle_wavelengths_per_s_{prec}"] = res[prec][0]
-        log(f"[bench] multi-particle {prec}: {res[prec][0]:.1f} particle-wavelengths/s")
+        # the same sweep with the particles' initialisations batched four at a time on concurrent streams
+        ps, bems = [], []
+        for part in parts:
+            op = B.bemoptions(sim="ret")
+            ps.append(B.comparticle([B.epsconst(1.0), B.epsdrude(*part["drude"])],
+                                    [(part["verts"], part["faces"])], [2, 1], 1, op))
+            ps[-1].green
+            bems.append(B.bemsolver(ps[-1], op, precision=prec))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        extb = np.zeros((len(parts), len(lams)))
+        for j, lam in enumerate(lams):
+            B.initmat_many(bems, lam)
+            for i, (p, bem) in enumerate(zip(ps, bems)):
+                extb[i, j] = exc.ext(bem.solve(exc(p, lam)))[0]
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t
+        del bems, ps
+        out[f"particle_wavelengths_per_s_{prec}_batched"] = len(parts) * len(lams) / tb
+        out[f"max_rel_diff_batched_vs_sequential_{prec}"] = float(np.max(np.abs(extb / res[prec][1] - 1)))
+        log(f"[bench] multi-particle {prec}: {res[prec][0]:.1f} particle-wavelengths/s, "
+            f"{len(parts) * len(lams) / tb:.1f} with batched initialisation")
     for prec in ("mixed", "mixed_bf16"):
         out[f"max_rel_err_{prec}_vs_fp64"] = float(np.max(np.abs(res[prec][1] / res["fp64"][1] - 1)))
     out["setup_s_per_particle"] = setup / (3 * len(parts))

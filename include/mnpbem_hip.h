@@ -168,6 +168,14 @@ int mnp_bemret_solve(mnp_bemret_t bem, int64_t npol, const void* phi1, const voi
 int mnp_bemret_stats(mnp_bemret_t bem, int* iterations, double* relres);
 /* Refinement settings for MNP_SOLVER_MIXED (defaults: 1e-12, 10). */
 int mnp_bemret_set_refinement(mnp_bemret_t bem, double tol, int maxit);
+/* mnp_bemret_initmat for `count` solvers (distinct handles, e.g. one per
+ * particle of a sweep), each at its own wavelength and dielectric functions:
+ * the assemblies and factorisations run four at a time on concurrent worker
+ * streams (after the work the caller's stream holds), the caller's stream
+ * then waits for them.  No counterpart in the reference; it batches the
+ * per-particle subsref '()' of BEM/@bemret/subsref.m:23-26. */
+int mnp_bemret_initmat_many(int count, const mnp_bemret_t* bems, const double* enei,
+                            const mnp_complex* eps1, const mnp_complex* eps2);
 /* Sensitivity of a solution to the inside dielectric function (no counterpart
  * in the reference; it differentiates the operator that bemretiter/private/
  * afun.m:6-35 applies and initmat.m:5-72 inverts).  Given the solution

@@ -61,6 +61,7 @@ PROTOTYPES = {
     "mnp_bemret_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
     "mnp_bemret_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mnp_bemret_tangent_eps1": (INT, [P, I64, P, P, P, P, P, P, P, P]),
+    "mnp_bemret_initmat_many": (INT, [INT, C.POINTER(P), PD, C.POINTER(MnpComplex), C.POINTER(MnpComplex)]),
     "mnp_bemret_stats": (INT, [P, C.POINTER(INT), PD]),
     "mnp_bemret_set_refinement": (INT, [P, D, INT]),
     "mnp_gemm": (INT, [INT, I64, I64, I64, MnpComplex, P, I64, P, I64, MnpComplex, P, I64]),

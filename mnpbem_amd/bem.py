@@ -204,6 +204,27 @@ class bemret:
         return it.value, rr.value
 
 
+def initmat_many(bems, enei):
+    """bem(enei) for several dense retarded solvers at once (one per particle of
+    a sweep; ``enei`` a scalar or one wavelength per solver): their assemblies
+    and factorisations run four at a time on concurrent streams
+    (mnp_bemret_initmat_many)."""
+    bems = list(bems)
+    lam = np.broadcast_to(np.asarray(enei, dtype=np.float64), (len(bems),)).copy()
+    todo = [i for i, b in enumerate(bems) if b.enei != lam[i]]
+    if not todo:
+        return
+    if len({id(bems[i]) for i in todo}) != len(todo):
+        raise ValueError("initmat_many: a solver appears twice")
+    hs = (C.c_void_p * len(todo))(*[bems[i].handle.value for i in todo])
+    l = np.ascontiguousarray(lam[todo])
+    e1 = (_lib.MnpComplex * len(todo))(*[_lib.MnpComplex.of(bems[i].p.eps1(lam[i])) for i in todo])
+    e2 = (_lib.MnpComplex * len(todo))(*[_lib.MnpComplex.of(bems[i].p.eps2(lam[i])) for i in todo])
+    _lib.call("mnp_bemret_initmat_many", len(todo), hs, l.ctypes.data_as(C.POINTER(C.c_double)), e1, e2)
+    for i in todo:
+        bems[i].enei = lam[i]
+
+
 def bemiter_options(**kw):
     """bemiter.options defaults (BEM/@bemiter/options.m:18-44)."""
     op = dict(solver="gmres", tol=1e-6, maxit=100, restart=None, precond="hmat", output=0,

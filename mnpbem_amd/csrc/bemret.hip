@@ -17,7 +17,9 @@
 // MNP_SOLVER_MIXED_BF16 is MNP_SOLVER_MIXED with the chain's large complex64
 // products on the split-bf16 kernel (a less exact preconditioner; the
 // refinement against the FP64 operator sets the final accuracy).
+#include <algorithm>
 #include <cmath>
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -238,11 +240,18 @@ struct BemRet {
   }
 
   void initmat(double enei_, double2 e1, double2 e2) {
+    if (!initmat_try(enei_, e1, e2)) initmat_finish();
+  }
+  // Assembly and the factorisations that need no partial pivoting (safe on a
+  // worker stream); false when initmat_finish must complete the chain.
+  bool initmat_try(double enei_, double2 e1, double2 e2) {
+    ready = false;
     enei = enei_;
     eps1 = e1;
     eps2 = e2;
     const double k0 = 2 * M_PI / enei;
     const double2 k1 = wavenumber(k0, e1), k2 = wavenumber(k0, e2);
+    bool ok;
     if (solver == MNP_SOLVER_FP64) {
       Chain<double>& c = *c64;
       c.eps1 = e1;
@@ -250,7 +259,7 @@ struct BemRet {
       c.k0 = k0;
       g->assemble(k1, 2 * M_PI, OutPair{c.G1, nullptr}, OutPair{c.S1, nullptr}, n);
       g->assemble(k2, -2 * M_PI, OutPair{c.G2, nullptr}, OutPair{c.S2, nullptr}, n);
-      chain_build<double>(c, *g);
+      ok = chain_build_try<double>(c, *g);
     } else {
       Chain<float>& c = *c32;
       c.eps1 = e1;
@@ -259,7 +268,17 @@ struct BemRet {
       g->assemble(k1, 2 * M_PI, OutPair{G1, c.G1}, OutPair{H1, c.S1}, n);
       g->assemble(k2, -2 * M_PI, OutPair{G2, c.G2}, OutPair{H2, c.S2}, n);
       SplitBf16Scope split(solver == MNP_SOLVER_MIXED_BF16);
-      chain_build<float>(c, *g);
+      ok = chain_build_try<float>(c, *g);
+    }
+    ready = ok;
+    return ok;
+  }
+  void initmat_finish() {
+    if (solver == MNP_SOLVER_FP64) {
+      chain_build_finish<double>(*c64, *g);
+    } else {
+      SplitBf16Scope split(solver == MNP_SOLVER_MIXED_BF16);
+      chain_build_finish<float>(*c32, *g);
     }
     ready = true;
   }
@@ -528,6 +547,31 @@ int mnp_bemret_initmat(mnp_bemret_t bem, double enei, mnp_complex eps1, mnp_comp
     MNP_CHECK(bem, MNP_ERR_ARG, "bemret_initmat: NULL handle");
     MNP_CHECK(enei > 0, MNP_ERR_ARG, "bemret_initmat: wavelength must be positive");
     bem->b->initmat(enei, to_d2(eps1), to_d2(eps2));
+  });
+}
+
+int mnp_bemret_initmat_many(int count, const mnp_bemret_t* bems, const double* enei, const mnp_complex* eps1,
+                            const mnp_complex* eps2) {
+  return guarded([&] {
+    MNP_CHECK(count >= 0 && (count == 0 || (bems && enei && eps1 && eps2)), MNP_ERR_ARG,
+              "bemret_initmat_many: bad argument");
+    for (int i = 0; i < count; ++i) {
+      MNP_CHECK(bems[i] && enei[i] > 0, MNP_ERR_ARG, "bemret_initmat_many: NULL handle or bad wavelength");
+      for (int j = 0; j < i; ++j)
+        MNP_CHECK(bems[j] != bems[i], MNP_ERR_ARG, "bemret_initmat_many: a handle appears twice");
+    }
+    // groups of four on concurrent streams; pivoted refactorisations (a
+    // cooperative launch) afterwards, one solver at a time
+    for (int i0 = 0; i0 < count; i0 += 4) {
+      const int i1 = std::min(count, i0 + 4);
+      std::vector<char> ok(i1 - i0, 0);
+      std::vector<std::function<void()>> tasks;
+      for (int i = i0; i < i1; ++i)
+        tasks.push_back([&, i] { ok[i - i0] = bems[i]->b->initmat_try(enei[i], to_d2(eps1[i]), to_d2(eps2[i])); });
+      run_concurrent(tasks);
+      for (int i = i0; i < i1; ++i)
+        if (!ok[i - i0]) bems[i]->b->initmat_finish();
+    }
   });
 }
 

@@ -45,6 +45,7 @@ struct Chain {
   LU<T> f[4];                  // G1, G2, Sigma1 - Sigma2, Sigma
   int fallbacks = 0;           // factorisations redone with partial pivoting
   bool ok_g1 = false, ok_g2 = false, ok_d = false;  // chain_sector_try's progress
+  bool sigma_formed = false, ok_s = false;          // chain_build_try's
   double2 eps1{}, eps2{};
   double k0 = 0;
   Workspace ws, ws2;  // ws2: the G2 branch
@@ -197,17 +198,42 @@ void chain_sector(Chain<T>& c) {
   if (!chain_sector_try(c)) chain_sector_finish(c);
 }
 
-// Factorises the chain (chain_sector, then Sigma over S2 and its LU).
+// Sigma = eps1 Sigma1 - eps2 Sigma2 + k^2 (eps1 - eps2)^2 (Deltai o n n^T), over S2
 template <class T>
-void chain_build(Chain<T>& c, const GreenRet& g) {
+void chain_sigma(Chain<T>& c, const GreenRet& g) {
   const int64_t n = c.n;
-  chain_sector(c);
   const double2 L = csub<double>(c.eps1, c.eps2);
   const double2 cc = cscale<double>(cmul<double>(L, L), c.k0 * c.k0);
   hipLaunchKernelGGL(k_sigma<T>, gridnn(n, n), dim3(256), 0, stream(), n, c.S1, c.S2, c.Di,
                      g.geo.nvec, c.eps1, c.eps2, cc);
   MNP_HIP(hipGetLastError());
-  c.factor(3, c.S2, "Sigma");
+  c.sigma_formed = true;
+}
+// Factorises the chain (chain_sector, then Sigma over S2 and its LU), split
+// like chain_sector into a part without partial pivoting and a finish.
+template <class T>
+bool chain_build_try(Chain<T>& c, const GreenRet& g) {
+  c.sigma_formed = c.ok_s = false;
+  if (!chain_sector_try(c)) return false;
+  chain_sigma(c, g);
+  c.ok_s = c.factor_try(3, c.S2, c.backup, c.piv, c.ws);
+  return c.ok_s;
+}
+template <class T>
+void chain_build_finish(Chain<T>& c, const GreenRet& g) {
+  if (c.ok_s) return;
+  if (!c.sigma_formed) {
+    chain_sector_finish(c);
+    chain_sigma(c, g);
+    c.factor(3, c.S2, "Sigma");
+  } else {
+    c.factor_pivoted(3, c.S2, "Sigma");
+  }
+  c.ok_s = true;
+}
+template <class T>
+void chain_build(Chain<T>& c, const GreenRet& g) {
+  if (!chain_build_try(c, g)) chain_build_finish(c, g);
 }
 
 // ------------------------------------------------------------- mldivide

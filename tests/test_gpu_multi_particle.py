@@ -49,3 +49,25 @@ def test_multi_particle_sweep_parity(pinfty):
                 ext = O.extinction(g.pos, g.area, sig, POL, DIR, lam, 1.0)
                 sca = O.scattering(g.pos, g.area, sig, lam, 1.0, *pinfty)
                 assert np.allclose(c64[0], ext, rtol=1e-10) and np.allclose(c64[1], sca, rtol=1e-10)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_initmat_equals_sequential(precision):
+    """initmat_many (four solvers at a time on concurrent streams) gives the
+    cross sections of one-by-one initialisation, for six particles of the
+    sweep including two of 5,120 faces."""
+    from mnpbem_amd import bem as B
+    from mnpbem_amd.workloads import multi_particle_sweep
+    parts, enei = multi_particle_sweep()
+    sel = [0, 1, 5, 9, 10, 3]
+    exc = B.planewave(POL, DIR)
+    ps, bems = zip(*[_solver(B, parts[i], precision) for i in sel])
+    ps, bems = list(ps), list(bems)
+    for lam in (enei[10], enei[90]):
+        print(f"[batched initmat {precision}] enei {lam:.1f}", flush=True)
+        B.initmat_many(bems, lam)
+        got = [exc.cross_sections(b.solve(exc(p, lam)))[0] for p, b in zip(ps, bems)]
+        for p, b, g in zip(ps, bems, got):
+            b.enei = None  # initialise again, alone
+            want = exc.cross_sections(b.solve(exc(p, lam)))[0]
+            assert np.allclose(g, want, rtol=1e-13, atol=0)
