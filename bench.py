@@ -388,7 +388,7 @@ def run_multi_particle(per_particle=8):
     return out
 
 
-def run_config4(nlam=16, seed=2024):
+def run_hmatrix_dipole(nlam=16, seed=2024):
     """H-matrix iterative path with dipole excitation: sphere of diameter 150 nm
     (icosphere level 6, 81,920 faces), Drude gold in eps=1, dipole 5 nm above
     the surface along a seeded random direction with a seeded random moment,
@@ -433,7 +433,7 @@ def run_config4(nlam=16, seed=2024):
     mv_ms = tm.ms[hv] / max(tm.launches[hv], 1)
     mv_bytes = tm.work[hv] / max(tm.launches[hv], 1)
     gbs = mv_bytes / (mv_ms * 1e-3) / 1e9 if mv_ms > 0 else 0.0
-    log(f"[bench] config 4: {nlam} wavelengths in {el:.2f}s (setup {setup:.2f}s), iterations {iters}")
+    log(f"[bench] H-matrix dipole: {nlam} wavelengths in {el:.2f}s (setup {setup:.2f}s), iterations {iters}")
     return {
         "workload": "sphere d=150 nm, icosphere level 6 (81920 faces), Drude gold in eps=1, dipole 5 nm "
                     "from the surface (seeded), bemretiter cleaf 32, htol 1e-6, GMRES tol 1e-6, FP32 "
@@ -448,7 +448,7 @@ def run_config4(nlam=16, seed=2024):
     }
 
 
-def cpu_baseline_config4(iterations, compression, nsample=60, seed=7):
+def cpu_baseline_hmatrix(iterations, compression, nsample=60, seed=7):
     """Oracle port of the H-matrix path on the host for a bounded sample,
     scaled to one wavelength of the 81,920-face workload: partially pivoted
     ACA (NumPy restatement) of `nsample` seeded admissible blocks of F and
@@ -541,18 +541,18 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-config4", action="store_true", help="skip the H-matrix dipole measurement")
+    ap.add_argument("--no-hmatrix", action="store_true", help="skip the H-matrix dipole measurement")
     ap.add_argument("--no-split-bf16", action="store_true", help="skip the split-bf16 chain measurement")
     ap.add_argument("--no-quasistatic", action="store_true", help="skip the quasistatic measurement")
     ap.add_argument("--no-multi-particle", action="store_true", help="skip the many-particle sweep measurement")
     ap.add_argument("--no-mirror", action="store_true", help="skip the mirror-symmetry measurement")
     ap.add_argument("--no-substrate", action="store_true", help="skip the substrate measurement")
     ap.add_argument("--no-sensitivity", action="store_true", help="skip the Drude-gradient measurement")
-    ap.add_argument("--config4-only", action="store_true",
+    ap.add_argument("--hmatrix-only", action="store_true",
                     help="only the H-matrix dipole measurement (for profiling)")
     ap.add_argument("--only", default=None,
                     help="comma list of side measurements to run alone (for profiling): "
-                         "quasistatic, mirror, substrate, multi_particle, config4, sensitivity")
+                         "quasistatic, mirror, substrate, multi_particle, hmatrix_dipole, sensitivity")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy three times (counter calibration)")
     a = ap.parse_args()
@@ -569,14 +569,14 @@ def main():
         torch.cuda.synchronize()
         log("[bench] copy calibration: 3 copies of 1 GiB")
         del src, dst
-    if a.config4_only:
-        print(json.dumps({"hmatrix_dipole": run_config4()}), flush=True)
+    if a.hmatrix_only:
+        print(json.dumps({"hmatrix_dipole": run_hmatrix_dipole()}), flush=True)
         return
     if a.only:
         runs = {"quasistatic": lambda: run_quasistatic(a.level), "mirror": run_mirror,
                 "substrate": lambda: run_substrate(a.level), "multi_particle": run_multi_particle,
                 "sensitivity": run_sensitivity,
-                "config4": run_config4}
+                "hmatrix_dipole": run_hmatrix_dipole}
         names = [x.strip() for x in a.only.split(",") if x.strip()]
         bad = [x for x in names if x not in runs]
         if bad:
@@ -667,11 +667,11 @@ def main():
         out["sensitivity"] = run_sensitivity()
     if not a.no_multi_particle:
         out["multi_particle"] = run_multi_particle()
-    if not a.no_config4:
-        out["hmatrix_dipole"] = run_config4()
+    if not a.no_hmatrix:
+        out["hmatrix_dipole"] = run_hmatrix_dipole()
         if not a.no_cpu_baseline:
             it = out["hmatrix_dipole"]["gmres_iterations"]
-            out["hmatrix_dipole"]["cpu_baseline"] = cpu_baseline_config4(int(round(sum(it) / len(it))),
+            out["hmatrix_dipole"]["cpu_baseline"] = cpu_baseline_hmatrix(int(round(sum(it) / len(it))),
                                                                          out["hmatrix_dipole"]["compression"])
     if not a.no_cpu_baseline:
         log("[bench] cpu baseline ...")

@@ -339,6 +339,7 @@ struct BemRetIter {
   }
 
   void initmat(double enei_, double2 e1, double2 e2) {
+    ready = false;  // until the preconditioner is built
     enei = enei_;
     eps1 = e1;
     eps2 = e2;

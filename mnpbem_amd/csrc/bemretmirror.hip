@@ -177,6 +177,7 @@ struct BemRetMirror {
   }
 
   void initmat(double enei_, double2 e1, double2 e2) {
+    ready = false;  // until every chain is complete
     enei = enei_;
     eps1 = e1;
     eps2 = e2;

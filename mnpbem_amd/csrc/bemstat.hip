@@ -222,6 +222,7 @@ struct BemStat {
   }
 
   void initmat(double2 e1, double2 e2) {
+    ready = false;  // until the factorisation is complete
     // Lambda = 2 pi (eps1 + eps2) / (eps1 - eps2)
     lam = cscale<double>(cdiv<double>(cadd<double>(e1, e2), csub<double>(e1, e2)), 2 * M_PI);
     if (f64) {

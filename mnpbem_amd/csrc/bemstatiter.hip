@@ -168,6 +168,7 @@ struct BemStatIter {
 
   // F does not depend on the wavelength: filled once; Lambda per wavelength
   void initmat(double2 e1, double2 e2) {
+    ready = false;  // until the preconditioner is built
     if (!filled) {
       F->fill(mk<double>(0, 0), MNP_KEY_F);
       filled = true;

@@ -70,7 +70,7 @@ def test_solver_matches_oracle(case, precision, pinfty):
     assert np.allclose(ab, ext_ref - sca_ref, rtol=1e3 * tol)
 
 
-def test_config1_sphere_against_mie():
+def test_sphere_1280_faces_against_mie():
     """Config-1 sphere (1,280 faces, 20 nm): BEM spectrum vs Mie within the
     discretisation error, FP64 and mixed agree to 1e-8."""
     from mnpbem_amd import bem as B
@@ -81,7 +81,7 @@ def test_config1_sphere_against_mie():
     enei = np.linspace(400, 800, 5)
     drude = B.epsdrude(*DRUDE)
     for lam in enei:
-        print(f"[config1] enei {lam}", flush=True)
+        print(f"[sphere vs Mie] enei {lam}", flush=True)
         e64 = exc.cross_sections(bem64.solve(exc(p64, lam)))
         emx = exc.cross_sections(bemmx.solve(exc(p64, lam)))
         me, ms, _ = mie_cross_sections(lam, drude(lam)[0], 1.0, 20.0)
