@@ -223,6 +223,24 @@ def run_mirror(steps=3):
            for k in ("mixed", "mixed_bf16", "full_mixed")}
     log(f"[bench] mirror xy: mixed {res['mixed'][0]:.2f}/s, mixed_bf16 {res['mixed_bf16'][0]:.2f}/s, "
         f"fp64 {res['fp64'][0]:.2f}/s, full particle mixed {res['full_mixed'][0]:.3f}/s; rel err vs FP64 {err}")
+    # quasistatic twin (bemstatmirror) on the same part, and bemstat on the full particle
+    sop = B.bemoptions(sim="stat", sym="xy")
+    ps = B.comparticlemirror(p.eps, [part], [2, 1], 1, sop)
+    sexc = B.planewave(pol, None, sop)
+    sres = {}
+    for prec in ("mixed", "fp64"):
+        bem = B.bemsolver(ps, sop, precision=prec)
+        sres[prec] = timed(bem, sexc, ps, 8)
+        del bem
+    fsop = B.bemoptions(sim="stat")
+    psf = B.comparticle(p.eps, [(p.full().verts, p.full().faces)], [2, 1], 1, fsop)
+    bem = B.bemsolver(psf, fsop, precision="mixed")
+    sres["full_mixed"] = timed(bem, B.planewave(pol, None, fsop), psf, 8)
+    del bem
+    serr = {k: max(float(np.max(np.abs(a / b - 1))) for a, b in zip(sres[k][1], sres["fp64"][1]))
+            for k in ("mixed", "full_mixed")}
+    log(f"[bench] quasistatic mirror xy: mixed {sres['mixed'][0]:.1f}/s, fp64 {sres['fp64'][0]:.1f}/s, "
+        f"full particle mixed {sres['full_mixed'][0]:.2f}/s; rel err vs FP64 {serr}")
     return {"workload": f"mirror symmetry xy: quarter of the ellipsoid 10x10x40 nm meshed by octants, {p.n} faces "
                         f"of {p.full().n}, pol x and y along +z, wavelengths from linspace(500,1000,64)",
             "wavelengths_per_s_mixed": res["mixed"][0], "wavelengths_per_s_mixed_bf16": res["mixed_bf16"][0],
@@ -230,7 +248,12 @@ def run_mirror(steps=3):
             "speedup_mixed_vs_full_particle_mixed": res["mixed"][0] / res["full_mixed"][0],
             "refinement_sweeps_mixed": sweeps["mixed"], "refinement_sweeps_mixed_bf16": sweeps["mixed_bf16"],
             "max_rel_err_vs_fp64": err["mixed"], "max_rel_err_mixed_bf16_vs_fp64": err["mixed_bf16"],
-            "max_rel_err_full_particle_vs_mirror_fp64": err["full_mixed"]}
+            "max_rel_err_full_particle_vs_mirror_fp64": err["full_mixed"],
+            "quasistatic": {"solver": "bemstatmirror, same part and pols, 8 wavelengths",
+                            "wavelengths_per_s_mixed": sres["mixed"][0], "wavelengths_per_s_fp64": sres["fp64"][0],
+                            "wavelengths_per_s_full_particle_mixed": sres["full_mixed"][0],
+                            "max_rel_err_vs_fp64": serr["mixed"],
+                            "max_rel_err_full_particle_vs_mirror_fp64": serr["full_mixed"]}}
 
 
 def run_substrate(level=5, steps=3):

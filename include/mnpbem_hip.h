@@ -426,6 +426,27 @@ int mnp_planewavestatlayer_potential(mnp_greenret_t g, int64_t npol, const mnp_c
  * (@planewavestatlayer/extinction.m:13, scattering.m:12). */
 int mnp_stat_dipole_moment(mnp_greenret_t g, int64_t npol, const void* sig, mnp_complex* dip);
 
+/* ------------------------------------ quasistatic solver, mirror symmetry
+ * Replaces bemsolver(p, op) with op.sim = 'stat' and p a comparticlemirror
+ * -> bemstatmirror (BEM/@bemstatmirror).  g is the Green function of the
+ * full particle (nsym n faces, copy j in faces j n ..; closed-surface
+ * correction as for the full particle), symtable the nsym x nsym table of
+ * +-1 (row-major, Particles/@comparticlemirror/private/init.m:29-37).  F_s =
+ * sum_j symtable[s][j] F(p, copy j) is assembled once
+ * (compgreenstatmirror/eval.m:18-40); initmat factorises Lambda I + F_s for
+ * every s (bemstatmirror/subsref.m:29-37); solve returns sig = -(Lambda I +
+ * F_sym)^{-1} phip for the symmetry index sym of the potential's
+ * z-component value (mldivide.m:14-22).  phip, sig: device complex128
+ * n x npol.  Solvers and refinement as mnp_bemstat_*. */
+typedef struct mnp_bemstatmirror_s* mnp_bemstatmirror_t;
+int mnp_bemstatmirror_create(mnp_greenret_t g, int solver, int nsym, const int* symtable,
+                             mnp_bemstatmirror_t* out);
+int mnp_bemstatmirror_destroy(mnp_bemstatmirror_t bem);
+int mnp_bemstatmirror_initmat(mnp_bemstatmirror_t bem, double enei, mnp_complex eps1, mnp_complex eps2);
+int mnp_bemstatmirror_solve(mnp_bemstatmirror_t bem, int sym, int64_t npol, const void* phip, void* sig);
+int mnp_bemstatmirror_stats(mnp_bemstatmirror_t bem, int sym, int* sweeps, double* relres);
+int mnp_bemstatmirror_set_refinement(mnp_bemstatmirror_t bem, double tol, int maxit);
+
 
 /* ------------------------------------------------ dipoles and point fields
  * Dipoles at points pts (host npt x 3, column-major) with moments dip (host

@@ -92,6 +92,12 @@ PROTOTYPES = {
     "mnp_bemretmirror_solve": (INT, [P, INT, INT, INT, I64] + [P] * 12),
     "mnp_bemretmirror_stats": (INT, [P, C.POINTER(INT), PD]),
     "mnp_bemretmirror_set_refinement": (INT, [P, D, INT]),
+    "mnp_bemstatmirror_create": (INT, [P, INT, INT, C.POINTER(INT), C.POINTER(P)]),
+    "mnp_bemstatmirror_destroy": (INT, [P]),
+    "mnp_bemstatmirror_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
+    "mnp_bemstatmirror_solve": (INT, [P, INT, I64, P, P]),
+    "mnp_bemstatmirror_stats": (INT, [P, INT, C.POINTER(INT), PD]),
+    "mnp_bemstatmirror_set_refinement": (INT, [P, D, INT]),
     "mnp_mirror_full": (INT, [I64, INT, INT, I64, I64, C.POINTER(INT), C.POINTER(INT), PD, C.POINTER(INT), P, P, P]),
     "mnp_bemstatiter_create": (INT, [P, INT, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
     "mnp_bemstatiter_destroy": (INT, [P]),

@@ -517,6 +517,68 @@ class bemretmirror:
         return it.value, rr.value
 
 
+class bemstatmirror:
+    """Dense quasistatic BEM solver with mirror symmetry (BEM/@bemstatmirror),
+    precision 'fp64', 'mixed' or 'mixed_bf16' as for bemstat.  One solver
+    per symmetry value; an excitation goes through the one of its z-key
+    (mldivide.m:14-22)."""
+
+    def __init__(self, p, op=None, precision=None):
+        op = p.op if op is None else op
+        if not isinstance(p, comparticlemirror):
+            raise TypeError("bemstatmirror needs a comparticlemirror")
+        self.p = p
+        self.precision = precision or op.get("precision", "fp64")
+        if self.precision not in bemret._SOLVERS:
+            raise ValueError(f"bemstatmirror: unknown precision {self.precision!r}")
+        tab = np.ascontiguousarray(p.symtable, dtype=np.int32)
+        h = C.c_void_p()
+        _lib.call("mnp_bemstatmirror_create", p.green.handle, bemret._SOLVERS[self.precision], len(tab),
+                  tab.ctypes.data_as(C.POINTER(C.c_int)), C.byref(h))
+        self.handle = h
+        if "refine_tol" in op or "refine_maxit" in op:
+            _lib.call("mnp_bemstatmirror_set_refinement", h, float(op.get("refine_tol", 1e-12)),
+                      int(op.get("refine_maxit", 10)))
+        self.enei = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib.lib().mnp_bemstatmirror_destroy(h)
+            self.handle = None
+
+    def __call__(self, enei):
+        if self.enei != enei:
+            e1, e2 = self.p.eps1(enei), self.p.eps2(enei)
+            _lib.call("mnp_bemstatmirror_initmat", self.handle, float(enei), _lib.MnpComplex.of(e1),
+                      _lib.MnpComplex.of(e2))
+            self.enei = enei
+        return self
+
+    def solve(self, exc):
+        """sig = bem \\ exc  (BEM/@bemstatmirror/mldivide.m) for a compstructmirror."""
+        self(exc.enei)
+        sig = compstructmirror(self.p, exc.enei, exc.fun)
+        self.val_stats = []
+        for val in exc.val:
+            s = self.p.symindex(val.symval[-1])
+            npol = val.phip.shape[-1]
+            out = dev.colmajor((self.p.n, npol))
+            _lib.call("mnp_bemstatmirror_solve", self.handle, s, npol, dev.ptr(dev.base(val.phip)),
+                      dev.ptr(dev.base(out)))
+            sig.val.append(compstruct(self.p, exc.enei, symval=val.symval, sig=out))
+            it, rr = C.c_int(), C.c_double()
+            _lib.call("mnp_bemstatmirror_stats", self.handle, s, C.byref(it), C.byref(rr))
+            self.val_stats.append((it.value, rr.value))
+        return sig
+
+    mldivide = solve
+
+    def stats(self):
+        """(refinement sweeps, final relative residual) of the last excitation solved."""
+        return self.val_stats[-1] if getattr(self, "val_stats", None) else (0, 0.0)
+
+
 def bemsolver(p, op=None, **kw):
     """Base/bemsolver.m: bemret for op.sim == 'ret', bemstat for op.sim ==
     'stat'; bemretiter / bemstatiter when op.iter is set; bemretmirror for a
@@ -525,8 +587,12 @@ def bemsolver(p, op=None, **kw):
     op.update(kw)
     sim = op.get("sim", "ret")
     if isinstance(p, comparticlemirror) or op.get("sym"):
-        if sim != "ret" or op.get("iter"):
-            raise NotImplementedError("mirror symmetry: only the dense retarded solver is built")
+        if op.get("iter") or op.get("layer") is not None:
+            raise NotImplementedError("mirror symmetry: only the dense solvers without layers are built")
+        if sim == "stat":
+            return bemstatmirror(p, op)
+        if sim != "ret":
+            raise NotImplementedError(f"unknown simulation type {sim!r}")
         return bemretmirror(p, op)
     if sim == "stat":
         if op.get("layer") is not None:
@@ -735,12 +801,84 @@ class planewaveretmirror:
     extinction, scattering, absorption = ext, sca, abs
 
 
+class planewavestatmirror:
+    """Quasistatic plane wave for a comparticlemirror (Simulation/static/
+    @planewavestatmirror): pol in the xy plane.  The potential holds two
+    excitations of the symmetry part, pol x and pol y, with their symmetry
+    values (potential.m:16-28); ``full(sig)`` combines their surface charges
+    on the full particle, pol_x sig_x + pol_y sig_y (full.m:23-39), where the
+    cross sections of planewavestat are taken.
+
+    The symmetry values are those of planewaveretmirror.  The reference's
+    table for sym 'x' gives the pol-y excitation the z-key '-'
+    (potential.m:22); its potential -y is even under x -> -x, so the key is
+    '+' here, which makes the expanded charge equal the full particle's."""
+
+    _KEYS = planewaveretmirror._KEYS
+
+    def __init__(self, pol, op=None, medium=1):
+        pol = np.atleast_2d(np.asarray(pol, dtype=np.float64))
+        if np.any(pol[:, 2] != 0):
+            raise ValueError("planewavestatmirror: pol must lie in the xy plane")
+        self.pol = pol
+        self.exc = planewavestat(pol, op, medium)
+        self.medium = medium
+
+    def __call__(self, p, enei):
+        pf = p.full()
+        pot = compstructmirror(p, enei, self.full)
+        for i, keys in enumerate(self._KEYS[p.sym]):
+            pol = np.zeros((1, 3))
+            pol[0, i] = 1.0
+            phip = dev.colmajor((pf.n, 1))
+            _lib.call("mnp_planewavestat_potential", pf.green.handle, 1,
+                      np.asfortranarray(pol).ctypes.data_as(C.c_void_p), dev.ptr(dev.base(phip)))
+            # the symmetry part is the first n faces of the full particle
+            part = dev.colmajor((p.n, 1))
+            part.copy_(phip[:p.n])
+            pot.val.append(compstruct(p, enei, symval=p.symvalue(list(keys)), phip=part))
+        return pot
+
+    potential = __call__
+
+    def full(self, sig):
+        """Surface charges on the full particle for this plane wave's pols."""
+        p = sig.p
+        pf = p.full()
+        J, n, npol = len(p.symtable), p.n, len(self.pol)
+        w = np.ascontiguousarray(self.pol[:, :2], dtype=np.float64)
+        src = np.zeros(npol, dtype=np.int32)
+        v1, v2 = sig.val
+        s1, s2 = (np.ascontiguousarray(v.symval[-1:].T, dtype=np.int32) for v in (v1, v2))  # J x 1
+        o = dev.colmajor((J * n, npol))
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        _lib.call("mnp_mirror_full", n, J, 1, 1, npol, ip(s1), ip(s2), w.ctypes.data_as(C.POINTER(C.c_double)),
+                  ip(src), dev.ptr(dev.base(v1.sig)), dev.ptr(dev.base(v2.sig)), dev.ptr(dev.base(o)))
+        return compstruct(pf, sig.enei, sig=o)
+
+    def cross_sections(self, sig):
+        return self.exc.cross_sections(self.full(sig))
+
+    def ext(self, sig):
+        return self.cross_sections(sig)[0]
+
+    def sca(self, sig):
+        return self.cross_sections(sig)[1]
+
+    def abs(self, sig):
+        return self.cross_sections(sig)[2]
+
+    extinction, scattering, absorption = ext, sca, abs
+
+
 def planewave(pol, dir=None, op=None, **kw):
     """Base/planewave.m: planewaveret for op.sim == 'ret', planewavestat for
     'stat', planewaveretmirror when op.sym is set."""
     if op is not None and op.get("sim", "ret") == "stat":
         if op.get("sym"):
-            raise NotImplementedError("mirror symmetry: only the retarded plane wave is built")
+            if op.get("layer") is not None:
+                raise NotImplementedError("mirror symmetry with layers is not built")
+            return planewavestatmirror(pol, op, **kw)
         if op.get("layer") is not None:
             from .layer import planewavestatlayer
             return planewavestatlayer(pol, dir, op, **kw)

@@ -25,38 +25,9 @@
 #include <vector>
 
 #include "mnp_chain.hpp"
+#include "mnp_mirror.hpp"
 
 namespace mnp {
-
-struct SymTab {
-  int8_t t[4][4];
-};
-struct SymOut {
-  OutPair o[4];
-};
-
-// out_s[i, k] = sum_j tab[s][j] B[i, j n + k]  (B n x J n, leading dimension n)
-template <int J>
-__global__ void __launch_bounds__(256) k_sym_contract(int64_t n, SymTab tab, const double2* __restrict__ B,
-                                                      SymOut out) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t k = blockIdx.y;
-  if (i >= n) return;
-  double2 v[J];
-#pragma unroll
-  for (int j = 0; j < J; ++j) v[j] = B[(j * n + k) * n + i];
-#pragma unroll
-  for (int s = 0; s < J; ++s) {
-    double2 acc = mk<double>(0, 0);
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const double t = tab.t[s][j];
-      acc.x += t * v[j].x;
-      acc.y += t * v[j].y;
-    }
-    out.o[s].store(k * n + i, acc);
-  }
-}
 
 // Sigma = e1 S1_z - e2 S2_z + c sum_comp (Di_comp o n_comp n_comp^T)
 // (initsigmai.m:19-30 with scalar L1 = eps1, L2 = eps2).
@@ -141,8 +112,7 @@ struct BemRetMirror {
 
   BemRetMirror(const GreenRet* g_, int solver_, int J_, const int* table)
       : g(g_), solver(solver_), J(J_), n(g_->n / J_) {
-    for (int s = 0; s < J; ++s)
-      for (int j = 0; j < J; ++j) tab.t[s][j] = (int8_t)table[s * J + j];
+    check_symtable("bemretmirror_create", J, table, tab);
     nvec = dev_alloc<double>(3 * n);
     for (int c = 0; c < 3; ++c)
       MNP_HIP(hipMemcpy(nvec + c * n, g->geo.nvec + c * g->n, sizeof(double) * n, hipMemcpyDeviceToDevice));
@@ -165,16 +135,7 @@ struct BemRetMirror {
   }
   double2* opm(int s, int w) const { return op[4 * s + w]; }  // w: 0 G1, 1 G2, 2 H1, 3 H2
 
-  void contract(const double2* B, const OutPair* outs) {
-    SymOut o{};
-    for (int s = 0; s < J; ++s) o.o[s] = outs[s];
-    const dim3 grid((unsigned)cdiv_up(n, 256), (unsigned)n);
-    if (J == 2)
-      hipLaunchKernelGGL(k_sym_contract<2>, grid, dim3(256), 0, stream(), n, tab, B, o);
-    else
-      hipLaunchKernelGGL(k_sym_contract<4>, grid, dim3(256), 0, stream(), n, tab, B, o);
-    MNP_HIP(hipGetLastError());
-  }
+  void contract(const double2* B, const OutPair* outs) { sym_contract(J, n, tab, B, outs); }
 
   void initmat(double enei_, double2 e1, double2 e2) {
     ready = false;  // until every chain is complete
@@ -379,19 +340,6 @@ int mnp_bemretmirror_create(mnp_greenret_t g, int solver, int nsym, const int* s
     MNP_CHECK(nsym == 2 || nsym == 4, MNP_ERR_ARG, "bemretmirror_create: nsym must be 2 or 4");
     MNP_CHECK(g->g->n % nsym == 0, MNP_ERR_ARG, "bemretmirror_create: faces not a multiple of nsym");
     MNP_CHECK(g->g->n <= 65535, MNP_ERR_ARG, "bemretmirror_create: at most 65535 faces in the full particle");
-    // rows: +-1, first column +1 (copy 0 is the part itself), mutually orthogonal
-    for (int s = 0; s < nsym; ++s) {
-      MNP_CHECK(symtable[s * nsym] == 1, MNP_ERR_ARG, "bemretmirror_create: symmetry table column 0 must be +1");
-      for (int t = 0; t < nsym; ++t) {
-        int dot = 0;
-        for (int j = 0; j < nsym; ++j) {
-          const int a = symtable[s * nsym + j], b = symtable[t * nsym + j];
-          MNP_CHECK(a == 1 || a == -1, MNP_ERR_ARG, "bemretmirror_create: symmetry table entries must be +-1");
-          dot += a * b;
-        }
-        MNP_CHECK(dot == (s == t ? nsym : 0), MNP_ERR_ARG, "bemretmirror_create: symmetry table rows not orthogonal");
-      }
-    }
     auto* h = new mnp_bemretmirror_s{nullptr};
     try {
       h->b = new BemRetMirror(g->g, solver, nsym, symtable);

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "mnp_linalg.hpp"
+#include "mnp_mirror.hpp"
 
 namespace mnp {
 
@@ -187,6 +188,14 @@ struct BemStat {
     else
       f32.reset(new StatFactor<float>(n, false));
   }
+  // Solver for a given FP64 surface derivative F (n x n, taken over: freed
+  // with the solver); the quasistatic mirror solver makes one per symmetry value.
+  BemStat(int64_t n_, double2* F_owned, int solver_) : g(nullptr), n(n_), solver(solver_), F(F_owned) {
+    if (solver == MNP_SOLVER_FP64)
+      f64.reset(new StatFactor<double>(n, true));
+    else
+      f32.reset(new StatFactor<float>(n, false));
+  }
   // Quasistatic solver above a substrate (BEM/@bemstatlayer): gimg is the
   // Green function of [particle; its image in the interface] without the
   // closed-surface correction; the reflected part is its rows of the
@@ -288,6 +297,51 @@ struct BemStat {
   }
 };
 
+// Quasistatic solver with mirror symmetry (BEM/@bemstatmirror): one solver
+// per symmetry value s with F_s = sum_j tab[s][j] F(p, copy j)
+// (compgreenstatmirror/eval.m:18-40) and mat_s = -(Lambda + F_s)^{-1}
+// (bemstatmirror/subsref.m:29-37).  g is the full particle (J n faces, copy
+// j in faces j n ..); the n x J n rows of the part are assembled once per
+// mesh in FP64 and contracted into the J matrices.  A surface charge goes
+// through the solver of the symmetry value of its potential
+// (bemstatmirror/mldivide.m:14-22).
+struct BemStatMirror {
+  int J;
+  int64_t n;
+  std::vector<std::unique_ptr<BemStat>> s;
+
+  BemStatMirror(const GreenRet* g, int solver, int J_, const int* table) : J(J_), n(g->n / J_) {
+    SymTab tab{};
+    check_symtable("bemstatmirror_create", J, table, tab);
+    double2* B = dev_alloc<double2>(n * g->n);
+    std::vector<double2*> Fs(J, nullptr);
+    try {
+      g->assemble_rows(mk<double>(0, 0), 0.0, OutPair{}, OutPair{B, nullptr}, n, n);
+      OutPair outs[4];
+      for (int k = 0; k < J; ++k) {
+        Fs[k] = dev_alloc<double2>(n * n);
+        outs[k] = OutPair{Fs[k], nullptr};
+      }
+      sym_contract(J, n, tab, B, outs);
+      MNP_HIP(hipStreamSynchronize(stream()));
+      for (int k = 0; k < J; ++k) {
+        s.emplace_back(new BemStat(n, Fs[k], solver));
+        Fs[k] = nullptr;  // owned by s[k] now
+      }
+    } catch (...) {
+      (void)hipFree(B);
+      for (double2* p : Fs)
+        if (p) (void)hipFree(p);
+      throw;
+    }
+    MNP_HIP(hipFree(B));
+  }
+  void initmat(double2 e1, double2 e2) {
+    for (auto& b : s) b->ready = false;
+    for (auto& b : s) b->initmat(e1, e2);
+  }
+};
+
 }  // namespace mnp
 
 // ============================================================== C ABI
@@ -295,6 +349,9 @@ using namespace mnp;
 
 struct mnp_bemstat_s {
   BemStat* b;
+};
+struct mnp_bemstatmirror_s {
+  BemStatMirror* b;
 };
 
 extern "C" {
@@ -385,6 +442,68 @@ int mnp_stat_dipole_moment(mnp_greenret_t g, int64_t npol, const void* sig, mnp_
         }
         dip[q * 3 + d] = mnp_complex{re, im};
       }
+  });
+}
+
+int mnp_bemstatmirror_create(mnp_greenret_t g, int solver, int nsym, const int* symtable, mnp_bemstatmirror_t* out) {
+  return guarded([&] {
+    MNP_CHECK(g && symtable && out, MNP_ERR_ARG, "bemstatmirror_create: NULL argument");
+    MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED || solver == MNP_SOLVER_MIXED_BF16,
+              MNP_ERR_ARG, "bemstatmirror_create: unknown solver");
+    MNP_CHECK(nsym == 2 || nsym == 4, MNP_ERR_ARG, "bemstatmirror_create: nsym must be 2 or 4");
+    MNP_CHECK(g->g->n % nsym == 0, MNP_ERR_ARG, "bemstatmirror_create: faces not a multiple of nsym");
+    MNP_CHECK(g->g->n <= 65535, MNP_ERR_ARG, "bemstatmirror_create: at most 65535 faces in the full particle");
+    auto* h = new mnp_bemstatmirror_s{nullptr};
+    try {
+      h->b = new BemStatMirror(g->g, solver, nsym, symtable);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int mnp_bemstatmirror_destroy(mnp_bemstatmirror_t b) {
+  return guarded([&] {
+    if (!b) return;
+    delete b->b;
+    delete b;
+  });
+}
+
+int mnp_bemstatmirror_initmat(mnp_bemstatmirror_t b, double enei, mnp_complex eps1, mnp_complex eps2) {
+  return guarded([&] {
+    MNP_CHECK(b && enei > 0, MNP_ERR_ARG, "bemstatmirror_initmat: bad argument");
+    MNP_CHECK(eps1.re != eps2.re || eps1.im != eps2.im, MNP_ERR_ARG, "bemstatmirror_initmat: eps1 == eps2");
+    b->b->initmat(to_d2(eps1), to_d2(eps2));
+  });
+}
+
+int mnp_bemstatmirror_solve(mnp_bemstatmirror_t b, int sym, int64_t npol, const void* phip, void* sig) {
+  return guarded([&] {
+    MNP_CHECK(b && phip && sig, MNP_ERR_ARG, "bemstatmirror_solve: NULL argument");
+    MNP_CHECK(sym >= 0 && sym < b->b->J, MNP_ERR_ARG, "bemstatmirror_solve: symmetry index out of range");
+    b->b->s[sym]->solve(npol, (const double2*)phip, (double2*)sig);
+  });
+}
+
+int mnp_bemstatmirror_stats(mnp_bemstatmirror_t b, int sym, int* sweeps, double* relres) {
+  return guarded([&] {
+    MNP_CHECK(b, MNP_ERR_ARG, "bemstatmirror_stats: NULL handle");
+    MNP_CHECK(sym >= 0 && sym < b->b->J, MNP_ERR_ARG, "bemstatmirror_stats: symmetry index out of range");
+    if (sweeps) *sweeps = b->b->s[sym]->last_sweeps;
+    if (relres) *relres = b->b->s[sym]->last_relres;
+  });
+}
+
+int mnp_bemstatmirror_set_refinement(mnp_bemstatmirror_t b, double tol, int maxit) {
+  return guarded([&] {
+    MNP_CHECK(b && tol > 0 && maxit >= 0, MNP_ERR_ARG, "bemstatmirror_set_refinement: bad argument");
+    for (auto& s : b->b->s) {
+      s->refine_tol = tol;
+      s->refine_maxit = maxit;
+    }
   });
 }
 

@@ -10,12 +10,15 @@
   (G_s = sum_j tab[s, j] G(p, copy j), the rows of p in compgreenret(p, full(p)));
 - solver: BEM/@bemretmirror/private/initmat.m:25-66 (explicit inverses, as
   written), initsigmai.m:19-30, mldivide.m:25-87, excitation.m:12-27;
+- quasistatic solver: BEM/@bemstatmirror/subsref.m:29-37, mldivide.m:14-22,
+  Greenfun/@compgreenstatmirror/eval.m:18-40; plane wave
+  Simulation/static/@planewavestatmirror/potential.m:16-28, full.m:23-39;
 - plane wave: Simulation/retarded/@planewaveretmirror/potential.m:16-41
   (pol x and pol y, each for dir +z and -z, with their symmetry values),
   full.m:22-64 and Particles/@compstructmirror/expand.m:22-46.
 
 Pinned by tests/test_mirror_oracle.py: the expanded mirror solution equals
-the oracle's dense solution (mnpbem_oracle.BemRet) on the full particle, and
+the oracle's dense solution (mnpbem_oracle.BemRet, BemStat) on the full particle, and
 the full particle's Green matrices are equivariant under the mirror group.
 """
 import numpy as np
@@ -119,6 +122,46 @@ class BemRetMirror:
         return {"sig1": G1i[z] @ (sig2 + phi), "sig2": G2i[z] @ sig2,
                 "h1": np.stack([G1i[s] @ (h2[c] + a[:, c]) for c, s in enumerate(sx)], axis=1),
                 "h2": np.stack([G2i[s] @ h2[c] for c, s in enumerate(sx)], axis=1)}
+
+
+class BemStatMirror:
+    """Quasistatic solver with mirror symmetry (BEM/@bemstatmirror):
+    F_s = sum_j tab[s, j] F(p, copy j) (Greenfun/@compgreenstatmirror/
+    eval.m:18-40, the rows of p in compgreenstat(p, full(p))), mat_s =
+    -inv(diag(Lambda) + F_s) (bemstatmirror/subsref.m:29-37), sig = mat_s
+    phi' with s the index of the excitation's z-key (mldivide.m:14-22).
+    ``green`` is mnpbem_oracle.GreenRet of the full particle; the first n
+    faces are the symmetry part."""
+
+    def __init__(self, green, n, sym):
+        self.n, self.sym, self.tab = n, sym, SYMTABLE[sym]
+        F = green.eval(0.0, ("F",))["F"]
+        J = len(self.tab)
+        self.F = [sum(self.tab[s, j] * F[:n, j * n:(j + 1) * n] for j in range(J)) for s in range(J)]
+
+    def initmat(self, enei, eps1, eps2):
+        lam = 2 * np.pi * (eps1 + eps2) / (eps1 - eps2)
+        self.mat = [-np.linalg.inv(lam * np.eye(self.n) + F) for F in self.F]
+        return self
+
+    def solve(self, phip, s):
+        return self.mat[s] @ phip
+
+
+def planewavestat_symvals(sym):
+    """Symmetry values (3 x J) of the quasistatic pol-x and pol-y
+    excitations (Simulation/static/@planewavestatmirror/potential.m:16-28),
+    with the pol-y z-key of sym 'x' taken as '+' (the reference's table has
+    '-' there; the potential -y is even under x -> -x)."""
+    return planewave_symvals(sym)
+
+
+def full_stat(sols, symvals, pol):
+    """planewavestatmirror/full.m:23-39: sig = pol_x sig_x + pol_y sig_y on
+    the full particle, each expanded with its symmetry values."""
+    fx, fy = (expand(s, v) for s, v in zip(sols, symvals))
+    pol = np.atleast_2d(pol)
+    return fx @ pol[:, 0][None, :] + fy @ pol[:, 1][None, :]
 
 
 def expand(val, symval):
