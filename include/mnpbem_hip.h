@@ -426,6 +426,27 @@ int mnp_planewavestatlayer_potential(mnp_greenret_t g, int64_t npol, const mnp_c
  * (@planewavestatlayer/extinction.m:13, scattering.m:12). */
 int mnp_stat_dipole_moment(mnp_greenret_t g, int64_t npol, const void* sig, mnp_complex* dip);
 
+/* ---------------------------------------------- quasistatic dipoles
+ * phi' = -nvec . E at the faces (device out n x npt ndip, column ipt + npt
+ * idip) of dipoles dip (host npt x 3 x ndip) at pts (host npt x 3), E = w
+ * (3 u (u . p) - p) / r^3 with w[ipt] = 1 / eps of the point's medium
+ * (Simulation/static/@dipolestat/field.m:50-78, potential.m:17).  With
+ * wimage set (host npt), the image dipole at the point mirrored at z = z0
+ * with p_z negated adds wimage[ipt] = q1 / eps_image times its field
+ * (@dipolestatlayer/init.m:35-39, field.m:23-28). */
+int mnp_dipolestat_potential(mnp_greenret_t g, int64_t npt, const double* pts, int64_t ndip,
+                             const double* dip, const mnp_complex* w, double z0,
+                             const mnp_complex* wimage, void* phip);
+/* Quasistatic field E = -Gp sig at point ipt = q % npt from column q of sig
+ * (device n x ncol), host out ncol x 3: @compgreenstat/field.m:14-25 for
+ * points off the surface, with the refinement of greenstat/private/init.m
+ * for point-face pairs within the cutoff (the retarded point field at
+ * k = 0).  With g the Green function of [particle; image] and sig padded
+ * by n zero rows first, this is the reflected part of @compgreenstatlayer's
+ * field (before the factor f2). */
+int mnp_stat_field_own(mnp_greenret_t g, int64_t npt, const double* pts, int64_t ncol, const void* sig,
+                       mnp_complex* e);
+
 /* ------------------------------------ quasistatic solver, mirror symmetry
  * Replaces bemsolver(p, op) with op.sim = 'stat' and p a comparticlemirror
  * -> bemstatmirror (BEM/@bemstatmirror).  g is the Green function of the

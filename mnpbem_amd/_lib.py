@@ -92,6 +92,8 @@ PROTOTYPES = {
     "mnp_bemretmirror_solve": (INT, [P, INT, INT, INT, I64] + [P] * 12),
     "mnp_bemretmirror_stats": (INT, [P, C.POINTER(INT), PD]),
     "mnp_bemretmirror_set_refinement": (INT, [P, D, INT]),
+    "mnp_dipolestat_potential": (INT, [P, I64, P, I64, P, P, D, P, P]),
+    "mnp_stat_field_own": (INT, [P, I64, P, I64, P, P]),
     "mnp_bemstatmirror_create": (INT, [P, INT, INT, C.POINTER(INT), C.POINTER(P)]),
     "mnp_bemstatmirror_destroy": (INT, [P]),
     "mnp_bemstatmirror_initmat": (INT, [P, D, MnpComplex, MnpComplex]),

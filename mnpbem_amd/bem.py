@@ -987,6 +987,91 @@ class dipoleret:
         return tuple(out)
 
 
+class dipolestat:
+    """Quasistatic dipole excitation (Simulation/static/@dipolestat) for
+    dipoles outside the particle; ``pt``, ``dip``, ``full`` and ``medium`` as
+    for dipoleret.  Columns of the potentials and solutions are ipt + npt *
+    idip."""
+
+    def __init__(self, pt, dip=None, op=None, medium=None, full=False):
+        self.pt = np.atleast_2d(np.asarray(pt, dtype=np.float64))
+        npt = len(self.pt)
+        dip = np.eye(3) if dip is None else np.asarray(dip, dtype=np.float64)
+        self.dip = dip.reshape(npt, 3, -1) if full else np.repeat(np.atleast_2d(dip).T[None, :, :], npt, axis=0)
+        self.medium = medium
+
+    @property
+    def npol(self):
+        return self.dip.shape[0] * self.dip.shape[2]
+
+    _medium = dipoleret._medium
+    _eps_med = dipoleret._eps_med
+    _host = dipoleret._host
+
+    def _potential(self, p, w, z0=0.0, wimage=None):
+        pts, dip = self._host()
+        phip = dev.colmajor((p.n, self.npol))
+        w = np.ascontiguousarray(w, dtype=np.complex128)
+        wi = None if wimage is None else np.ascontiguousarray(wimage, dtype=np.complex128)
+        _lib.call("mnp_dipolestat_potential", p.green.handle, len(pts), pts.ctypes.data_as(C.c_void_p),
+                  dip.shape[2], dip.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p), float(z0),
+                  None if wi is None else wi.ctypes.data_as(C.c_void_p), dev.ptr(dev.base(phip)))
+        return phip
+
+    def __call__(self, p, enei):
+        """exc(p, enei): phi' = -nvec . E of the dipoles (potential.m:17)."""
+        w = np.full(len(self.pt), 1.0 / complex(self._eps_med(p, enei)))
+        return compstruct(p, enei, phip=self._potential(p, w))
+
+    potential = __call__
+
+    def field_own(self, green, sig):
+        """E = -Gp sig (k = 0) at each dipole from its own column, npol x 3."""
+        pts, _ = self._host()
+        ncol = sig.shape[1]
+        e = np.zeros((ncol, 3), dtype=np.complex128)
+        _lib.call("mnp_stat_field_own", green.handle, len(pts), pts.ctypes.data_as(C.c_void_p), ncol,
+                  dev.ptr(dev.base(sig)), e.ctypes.data_as(C.c_void_p))
+        return e
+
+    @staticmethod
+    def induced_dipole(sig):
+        """sum area pos sig per column, npol x 3 (FP64 device reduction)."""
+        npol = sig.sig.shape[-1]
+        dip = np.zeros((npol, 3), dtype=np.complex128)
+        _lib.call("mnp_stat_dipole_moment", sig.p.green.handle, npol, dev.ptr(dev.base(sig.sig)),
+                  dip.ctypes.data_as(C.c_void_p))
+        return dip
+
+    def decayrate(self, sig):
+        """[tot, rad, rad0], each npt x ndip (decayrate.m:25-56): tot = 1 +
+        Im(e . dip) / (0.5 nb gamma), rad = |nb^2 p_ind + dip|^2."""
+        p, enei = sig.p, sig.enei
+        npt, ndip = self.dip.shape[0], self.dip.shape[2]
+        nb = np.sqrt(complex(self._eps_med(p, enei)))
+        gamma = 4 / 3 * (2 * np.pi / enei) ** 3
+        e = self.field_own(p.green, sig.sig)
+        indip = self.induced_dipole(sig)
+        tot, rad = np.zeros((npt, ndip)), np.zeros((npt, ndip))
+        for ip in range(npt):
+            for i in range(ndip):
+                q = ip + npt * i
+                d = self.dip[ip, :, i]
+                rad[ip, i] = np.linalg.norm(nb ** 2 * indip[q] + d) ** 2
+                tot[ip, i] = np.real(1 + np.imag(e[q] @ d) / (0.5 * nb * gamma))
+        return tot, rad, np.full((npt, ndip), np.real(nb * gamma))
+
+
 def dipole(pt, dip=None, op=None, **kw):
-    """Base/dipole.m for op.sim == 'ret'."""
+    """Base/dipole.m: dipoleret for op.sim == 'ret', dipolestat for 'stat',
+    dipolestatlayer for 'stat' with op.layer."""
+    if op is not None and op.get("sim", "ret") == "stat":
+        if op.get("sym"):
+            raise NotImplementedError("mirror symmetry: the dipole excitations are not built")
+        if op.get("layer") is not None:
+            from .layer import dipolestatlayer
+            return dipolestatlayer(pt, dip, op, **kw)
+        return dipolestat(pt, dip, op, **kw)
+    if op is not None and (op.get("layer") is not None or op.get("sym")):
+        raise NotImplementedError("retarded dipoles with layers or mirror symmetry are not built")
     return dipoleret(pt, dip, op, **kw)

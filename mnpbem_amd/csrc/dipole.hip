@@ -158,6 +158,49 @@ __global__ void __launch_bounds__(256) k_pt_field(int64_t n, Geo geo, int64_t np
   }
 }
 
+// ------------------------------------------------- quasistatic dipoles
+// phi' = -nvec . E with E = w (3 u (u . p) - p) / r^3 (u = r / |r|, r = face -
+// point) of dipole p at each point (Simulation/static/@dipolestat/field.m:
+// 50-78, potential.m:17) plus, when wimg is set, the image dipole at the
+// point mirrored at z = z0 with p_z negated (@dipolestatlayer/init.m:35-39,
+// field.m:23-28).  w, wimg: per point (1 / eps of the point's medium, q1 /
+// eps of the image point's medium).  pts npt x 3, dip npt x 3 x ndip.
+__device__ __forceinline__ double2 dipstat_nE(const double r[3], const double nv[3], const double pz_sign,
+                                              const double* dp, double2 w) {
+  const double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+  const double d = sqrt(d2), inv3 = 1.0 / (d2 * d);
+  const double u[3] = {r[0] / d, r[1] / d, r[2] / d};
+  const double p[3] = {dp[0], dp[1], pz_sign * dp[2]};
+  const double up = u[0] * p[0] + u[1] * p[1] + u[2] * p[2];
+  double ne = 0;
+  for (int c = 0; c < 3; ++c) ne += nv[c] * (3 * u[c] * up - p[c]);
+  ne *= inv3;
+  return mk<double>(w.x * ne, w.y * ne);
+}
+
+__global__ void __launch_bounds__(256) k_dipstat_phip(int64_t n, const double* __restrict__ pos,
+                                                      const double* __restrict__ nvec, int64_t npt,
+                                                      const double* __restrict__ pts, const double* __restrict__ dip,
+                                                      const double2* __restrict__ w, double z0,
+                                                      const double2* __restrict__ wimg, double2* phip) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  const int64_t ipt = q % npt, idip = q / npt;
+  double nv[3], r[3], dp[3];
+  for (int c = 0; c < 3; ++c) {
+    nv[c] = nvec[c * n + i];
+    r[c] = pos[c * n + i] - pts[c * npt + ipt];
+    dp[c] = dip[ipt + npt * (c + 3 * idip)];
+  }
+  double2 v = dipstat_nE(r, nv, 1.0, dp, w[ipt]);
+  if (wimg) {
+    r[2] = pos[2 * n + i] - (2 * z0 - pts[2 * npt + ipt]);
+    v = cadd<double>(v, dipstat_nE(r, nv, -1.0, dp, wimg[ipt]));
+  }
+  phip[q * n + i] = mk<double>(-v.x, -v.y);
+}
+
 static void pt_field(const GreenRet& G, double enei, mnp_complex eps, int64_t npt, const double* dpts, int64_t ncol,
                      const double2* sig, const double2* h, int diag, double2* e, double2* hf) {
   upload_rule18();
@@ -215,6 +258,47 @@ int mnp_field_points(mnp_greenret_t g, double enei, mnp_complex eps_medium, int6
              (double2*)hfield);
     MNP_HIP(hipStreamSynchronize(stream()));
     MNP_HIP(hipFree(dpts));
+  });
+}
+
+int mnp_dipolestat_potential(mnp_greenret_t g, int64_t npt, const double* pts, int64_t ndip, const double* dip,
+                             const mnp_complex* w, double z0, const mnp_complex* wimage, void* phip) {
+  return guarded([&] {
+    MNP_CHECK(g && pts && dip && w && phip, MNP_ERR_ARG, "dipolestat_potential: NULL argument");
+    MNP_CHECK(npt >= 1 && ndip >= 1 && npt * ndip < 65536, MNP_ERR_ARG, "dipolestat_potential: bad sizes");
+    const GreenRet& G = *g->g;
+    double* dpts = upload(pts, 3 * npt);
+    double* ddip = upload(dip, 3 * npt * ndip);
+    double2* dw = dev_alloc<double2>(2 * npt);
+    MNP_HIP(hipMemcpyAsync(dw, w, sizeof(double2) * npt, hipMemcpyHostToDevice, stream()));
+    if (wimage) MNP_HIP(hipMemcpyAsync(dw + npt, wimage, sizeof(double2) * npt, hipMemcpyHostToDevice, stream()));
+    const dim3 grid((unsigned)cdiv_up(G.n, 256), (unsigned)(npt * ndip));
+    hipLaunchKernelGGL(k_dipstat_phip, grid, dim3(256), 0, stream(), G.n, G.geo.pos, G.geo.nvec, npt, dpts, ddip, dw,
+                       z0, wimage ? dw + npt : nullptr, (double2*)phip);
+    MNP_HIP(hipGetLastError());
+    MNP_HIP(hipStreamSynchronize(stream()));
+    MNP_HIP(hipFree(dpts));
+    MNP_HIP(hipFree(ddip));
+    MNP_HIP(hipFree(dw));
+  });
+}
+
+int mnp_stat_field_own(mnp_greenret_t g, int64_t npt, const double* pts, int64_t ncol, const void* sig,
+                       mnp_complex* e) {
+  return guarded([&] {
+    MNP_CHECK(g && pts && sig && e, MNP_ERR_ARG, "stat_field_own: NULL argument");
+    MNP_CHECK(npt >= 1 && ncol >= 1 && ncol < 65536, MNP_ERR_ARG, "stat_field_own: bad sizes");
+    const GreenRet& G = *g->g;
+    double* dpts = upload(pts, 3 * npt);
+    double2* h0 = dev_alloc<double2>(3 * G.n * ncol);  // no currents: k = 0 leaves E = -Gp sig
+    double2* ed = dev_alloc<double2>(3 * ncol);
+    double2* hd = dev_alloc<double2>(3 * ncol);
+    MNP_HIP(hipMemsetAsync(h0, 0, sizeof(double2) * 3 * G.n * ncol, stream()));
+    pt_field(G, 1.0, mnp_complex{0, 0}, npt, dpts, ncol, (const double2*)sig, h0, 1, ed, hd);
+    MNP_HIP(hipMemcpyAsync(e, ed, sizeof(double2) * 3 * ncol, hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipStreamSynchronize(stream()));
+    for (double* p : {dpts}) MNP_HIP(hipFree(p));
+    for (double2* p : {h0, ed, hd}) MNP_HIP(hipFree(p));
   });
 }
 

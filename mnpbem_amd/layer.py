@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _lib
 from . import device as dev
-from .bem import bemret, bemstat, compstruct, _flat_normals
+from .bem import bemret, bemstat, compstruct, dipolestat, _flat_normals
 from .greenfun import CompGreenRet
 
 _MATLAB_EPS = float(np.finfo(float).eps)
@@ -270,3 +270,121 @@ class planewavestatlayer:
         return self.cross_sections(sig)[2]
 
     extinction, scattering, absorption = ext, sca, abs
+
+
+class dipolestatlayer(dipolestat):
+    """Quasistatic dipoles above a substrate (Simulation/static/
+    @dipolestatlayer); ``pt``, ``dip``, ``full`` as for dipolestat, op['layer']
+    a layerstructure, every point above the interface (init.m:28).  The
+    potential adds q1 = -(eps_down - eps_up)/(eps_down + eps_up) times the
+    field of the image dipoles (point mirrored at the interface, p_z negated;
+    init.m:35-39, field.m:23-28).
+
+    image_eps: 'upper' (the default) divides the image dipole's field by the
+    upper medium's permittivity, the image of a dipole in the upper medium
+    and the convention of the reflected Green function of bemstatlayer.
+    'reference' divides it by the lower medium's, as the reference does (its
+    image point lies in the lower medium, field.m:28 with init.m:38-39); over
+    a lossy metal substrate that makes the absorbed part of the decay rate
+    negative.  Decay rates integrate the radiation over ``pinfty`` = (nvec,
+    area) of the unit sphere (default: trisphere(256, 2)).
+    """
+
+    def __init__(self, pt, dip=None, op=None, medium=None, full=False, pinfty=None, image_eps="upper"):
+        super().__init__(pt, dip, op, medium, full)
+        self.layer = op["layer"]
+        if np.any(self.pt[:, 2] <= self.layer.z[0]):
+            raise ValueError("dipolestatlayer: every dipole must lie above the interface")
+        if image_eps not in ("reference", "upper"):
+            raise ValueError("image_eps must be 'reference' or 'upper'")
+        self.image_eps = image_eps
+        if pinfty is None:
+            from . import shapes
+            pinfty = _flat_normals(*shapes.trisphere256(2.0))
+        self.pinf_nvec, self.pinf_area = (np.asarray(a, dtype=np.float64) for a in pinfty)
+        self._gimg = None
+
+    def _eps(self, enei):
+        eu, ed = self.layer.epsilon(enei)
+        return eu, ed, -(ed - eu) / (ed + eu)
+
+    def __call__(self, p, enei):
+        """exc(p, enei): phi' = -nvec . (E_dip + q1 E_image) (potential.m, field.m:23-28)."""
+        _check_above(p, self.layer)
+        eu, ed, q1 = self._eps(enei)
+        npt = len(self.pt)
+        w = np.full(npt, 1.0 / eu)
+        wimg = np.full(npt, q1 / (ed if self.image_eps == "reference" else eu))
+        return compstruct(p, enei, phip=self._potential(p, w, self.layer.z[0], wimg))
+
+    potential = __call__
+
+    def _image_green(self, p):
+        """Green function of [particle; image] (compgreenstatlayer's p2r), kept per particle."""
+        if self._gimg is None or self._gimg[0] is not p:
+            V, F = _image_mesh(p.verts, p.faces, self.layer.z[0])
+            op = p.op
+            g = CompGreenRet(V, F, order=op.get("order", 5), rel_cutoff=op.get("RelCutoff", 0.0),
+                             abs_cutoff=op.get("AbsCutoff", 0.0),
+                             npol=tuple(np.broadcast_to(op.get("npol", (7, 5)), (2,))), closed=False)
+            self._gimg = (p, g)
+        return self._gimg[1]
+
+    def _rad(self, dips, enei):
+        """(sca / (2 pi k0)) / (0.5 nb gamma) of the dipoles dips (m x 3) above
+        the interface (spectrumstatlayer/scattering.m, decayrate.m:54-57)."""
+        k0 = 2 * np.pi / enei
+        eu, ed, _ = self._eps(enei)
+        nb = np.sqrt(eu)
+        far, k = _dipole_far(dips, self.pinf_nvec, k0, eu, ed)
+        dsca = np.sum(np.abs(far) ** 2, axis=1) * (0.5 * k / k0)[:, None]
+        dsca = np.where(np.imag(dsca) != 0, 0, dsca).real
+        return np.real((self.pinf_area @ dsca) / (2 * np.pi * k0) / (0.5 * nb * 4 / 3 * k0 ** 3))
+
+    def decayrate(self, sig):
+        """[tot, rad, rad0], each npt x ndip (decayrate.m:25-61): the charges'
+        field at each dipole -(Gp + f2 Gp_image) sig, rad of the induced
+        dipole plus dip / nb^2, tot = 1 + rad + Im(e . dip) / (0.5 nb gamma)."""
+        import torch
+        p, enei = sig.p, sig.enei
+        npt, ndip = self.dip.shape[0], self.dip.shape[2]
+        eu, ed, f2 = self._eps(enei)
+        nb = np.sqrt(eu)
+        gamma = 4 / 3 * (2 * np.pi / enei) ** 3
+        ncol = sig.sig.shape[1]
+        e = self.field_own(p.green, sig.sig)
+        padded = dev.colmajor((2 * p.n, ncol))
+        padded[:p.n].zero_()
+        padded[p.n:].copy_(sig.sig)
+        e = e + f2 * self.field_own(self._image_green(p), padded)
+        torch.cuda.synchronize()
+        indip = self.induced_dipole(sig)
+        d = np.stack([self.dip[ip, :, i] for i in range(ndip) for ip in range(npt)])  # column order
+        rad = self._rad(indip + d / nb ** 2, enei)
+        nonrad = np.imag(np.sum(e * d, axis=1)) / (0.5 * nb * gamma)
+        tot = np.real(1 + rad + nonrad)
+        shape = lambda v: np.asarray(v).reshape(ndip, npt).T
+        return shape(tot), shape(rad), np.full((npt, ndip), np.real(nb * gamma))
+
+    def decayrate0(self, enei):
+        """[tot, rad, rad0] without a particle (decayrate0.m:44-55): tot = rad
+        for real media, else rad + Im(q1 E_image . dip) / (0.5 nb gamma)."""
+        npt, ndip = self.dip.shape[0], self.dip.shape[2]
+        eu, ed, q1 = self._eps(enei)
+        nb = np.sqrt(eu)
+        gamma = 4 / 3 * (2 * np.pi / enei) ** 3
+        z0 = self.layer.z[0]
+        d = np.stack([self.dip[ip, :, i] for i in range(ndip) for ip in range(npt)])
+        rad = self._rad(d / nb ** 2, enei)
+        pt = np.tile(self.pt, (ndip, 1))
+        r = np.zeros_like(pt)
+        r[:, 2] = 2 * (pt[:, 2] - z0)  # from the image point to the dipole
+        rn = np.linalg.norm(r, axis=1)
+        u = r / rn[:, None]
+        di = d * np.array([1.0, 1.0, -1.0])
+        e = q1 * (3 * u * np.sum(u * di, axis=1)[:, None] - di) / (rn ** 3 * (ed if self.image_eps == "reference"
+                                                                                 else eu))[:, None]
+        real = np.imag(eu) == 0 and np.imag(ed) == 0
+        tot = rad if real else np.real(rad + np.imag(np.sum(e * d, axis=1)) / (0.5 * nb * gamma))
+        shape = lambda v: np.asarray(v).reshape(ndip, npt).T
+        return shape(tot), shape(rad), np.full((npt, ndip), np.real(nb * gamma))

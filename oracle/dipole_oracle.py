@@ -9,7 +9,12 @@
 - induced fields at points: Greenfun/@compgreenret/field.m:14-40;
 - dipole far fields and decay rates: @dipoleret/farfield.m:22-58,
   scattering.m:11-13, decayrate.m:13-62, Simulation/retarded/scattering.m:14-31;
-- Mie decay rates: Mie/@mieret/decayrate.m:16-58 (Kim et al., Surf. Sci. 195, 1 (1988)).
+- Mie decay rates: Mie/@mieret/decayrate.m:16-58 (Kim et al., Surf. Sci. 195, 1 (1988));
+- quasistatic dipoles: Simulation/static/@dipolestat/field.m:50-78,
+  potential.m:17, decayrate.m:25-56 (the field of the charges is the point
+  field above at k = 0, Greenfun/@compgreenstat/field.m:14-25 with the
+  refinement of @greenstat/private/init.m), pinned against the quasistatic
+  image series of a sphere (sphere_stat_decayrate).
 
 Dipole columns are ordered (point fastest, then orientation): column
 q = ipt + npt * idip, as the reference's (n, npt, ndip) arrays flattened.
@@ -148,6 +153,74 @@ def decayrate(green, pts, dip, enei, eps_out, sig2, h2, e_sig_far, h_sig_far,
             tot[p, i] = np.real(1 + np.imag(e[p, :, q] @ dip[p, :, i]) / (0.5 * nb * gamma))
             rad[p, i] = np.real(sca[q] / (2 * np.pi * k0) / (0.5 * nb * gamma))
     return tot, rad, np.real(nb * gamma) * np.ones((npt, ndip))
+
+
+def dipolestat_field(pos, pts, dip, eps):
+    """e (n, 3, npt, ndip) at pos of the quasistatic dipoles dip (npt, 3, ndip)
+    at pts, each divided by eps[ipt] of its medium (@dipolestat/field.m:50-78)."""
+    pts = np.atleast_2d(pts)
+    dip = np.asarray(dip).reshape(len(pts), 3, -1)
+    eps = np.broadcast_to(np.asarray(eps, dtype=complex), (len(pts),))
+    x = pos[:, None, :] - pts[None, :, :]                  # (n, npt, 3)
+    r = np.sqrt(np.sum(x * x, axis=2))
+    u = x / r[:, :, None]
+    e = np.zeros((len(pos), 3, len(pts), dip.shape[2]), complex)
+    for i in range(dip.shape[2]):
+        p = dip[:, :, i][None, :, :]                       # (1, npt, 3)
+        inner = np.sum(u * p, axis=2)
+        e[:, :, :, i] = np.transpose((3 * u * inner[:, :, None] - p) / (r ** 3 * eps[None, :])[:, :, None],
+                                     (0, 2, 1))
+    return e
+
+
+def dipolestat_potential(nvec, e):
+    """phi' (n, npt ndip) = -nvec . e (@dipolestat/potential.m:17), columns
+    ipt + npt idip."""
+    return -np.einsum("ic,icq->iq", nvec, e.reshape(len(nvec), 3, -1, order="F"))
+
+
+def dipolestat_decayrate(green, pts, dip, enei, eps_out, sig, relcutoff=3.0, abscutoff=0.0):
+    """(tot, rad, rad0) (npt, ndip) of @dipolestat/decayrate.m:25-56: e is the
+    field -Gp sig of the surface charges at each dipole (k = 0), indip the
+    induced dipole sum area pos sig."""
+    pts = np.atleast_2d(pts)
+    dip = np.asarray(dip, dtype=float).reshape(len(pts), 3, -1)
+    npt, _, ndip = dip.shape
+    nb = np.sqrt(complex(eps_out))
+    gamma = 4 / 3 * (2 * np.pi / enei) ** 3
+    e, _ = field_points(green, pts, sig, np.zeros((green.n, 3, sig.shape[1]), complex), 0.0, enei,
+                        relcutoff, abscutoff)
+    indip = (green.area[:, None] * green.pos).T @ sig    # (3, ncol)
+    tot, rad = np.zeros((npt, ndip)), np.zeros((npt, ndip))
+    for p in range(npt):
+        for i in range(ndip):
+            q = p + npt * i
+            rad[p, i] = np.linalg.norm(nb ** 2 * indip[:, q] + dip[p, :, i]) ** 2
+            tot[p, i] = np.real(1 + np.imag(e[p, :, q] @ dip[p, :, i]) / (0.5 * nb * gamma))
+    return tot, rad, np.real(nb * gamma) * np.ones((npt, ndip))
+
+
+def sphere_stat_decayrate(enei, eps_in, eps_out, radius, dist, lmax=200):
+    """Quasistatic decay rates (tot, rad) of dipoles at distance ``dist`` from
+    the centre of a sphere, orientations (radial, tangential), in the
+    convention of @dipolestat/decayrate.m (tot = 1 + Im(E_ind . p) / (0.5 nb
+    gamma), rad = |p + p_ind|^2 for |p| = 1).  Image series of the
+    quasistatic sphere: alpha_l = a^(2l+1) l (e - 1) / (l e + l + 1), e =
+    eps_in / eps_out; E_ind . p = sum_l c_l alpha_l / dist^(2l+4) / eps_out
+    with c_l = (l+1)^2 (radial) or l (l+1) / 2 (tangential); the induced
+    dipole (l = 1) is 2 alpha_1 / dist^3 (radial) or -alpha_1 / dist^3."""
+    e = complex(eps_in) / complex(eps_out)
+    nb = np.sqrt(complex(eps_out))
+    gamma = 4 / 3 * (2 * np.pi / enei) ** 3
+    l = np.arange(1, lmax + 1)
+    t = radius / dist
+    s = t ** (2 * l + 1) * l * (e - 1) / (l * e + l + 1) / dist ** 3   # alpha_l / dist^(2l+4)
+    alpha1 = radius ** 3 * (e - 1) / (e + 2)
+    eind = np.array([np.sum((l + 1) ** 2 * s), np.sum(l * (l + 1) / 2 * s)]) / complex(eps_out)
+    tot = np.real(1 + np.imag(eind) / (0.5 * nb * gamma))
+    a1 = alpha1 / dist ** 3
+    rad = np.abs(np.array([1 + 2 * a1, 1 - a1])) ** 2
+    return tot, rad
 
 
 def mie_decayrate(enei, eps_in, eps_out, diameter, z, lmax=20):

@@ -12,7 +12,9 @@ never by the product).
   (eps1 - eps2)), mldivide.m:13;
 - plane wave: Simulation/static/@planewavestatlayer/decompose.m, fresnel.m,
   field.m, potential.m, extinction.m, scattering.m, absorption.m;
-  Simulation/static/@spectrumstatlayer/efarfield.m, scattering.m.
+  Simulation/static/@spectrumstatlayer/efarfield.m, scattering.m;
+- dipoles above the substrate: Simulation/static/@dipolestatlayer/init.m,
+  field.m, field2.m, decayrate.m, decayrate0.m.
 
 Pinned by tests/test_layer_oracle.py (see there): equal media reduce every
 function to the free-space one, Fresnel coefficients conserve energy and
@@ -221,3 +223,98 @@ def scattering(dip, dirs, enei, eps1, eps2, pinf_nvec, pinf_area):
     sca[d < 0] /= 0.5 * nb[0]
     sca[d > 0] /= 0.5 * nb[-1]
     return np.real(sca)
+
+
+# ------------------------------------------------ dipoles above the substrate
+def image_factor(eps_up, eps_down):
+    """q1 = -(eps_down - eps_up) / (eps_down + eps_up) (@dipolestatlayer/field.m:23)."""
+    return -(eps_down - eps_up) / (eps_down + eps_up)
+
+
+def dipolestatlayer_field(pos, pts, dip, eps_up, eps_down, z0, image_eps="reference"):
+    """e (n, 3, npt, ndip) at pos above the interface: the dipoles' field plus
+    q1 times the field of the image dipoles (point mirrored at z0, p_z
+    negated; @dipolestatlayer/init.m:35-39, field.m:23-28).  The image point
+    lies in the lower medium, so @dipolestat/field.m divides its field by
+    eps_down (image_eps 'reference'); 'upper' divides by eps_up, the
+    textbook image of a dipole in the upper medium."""
+    from .dipole_oracle import dipolestat_field
+    pts = np.atleast_2d(pts)
+    dip = np.asarray(dip, dtype=float).reshape(len(pts), 3, -1)
+    ipts = pts.copy()
+    ipts[:, 2] = 2 * z0 - ipts[:, 2]
+    idip = dip.copy()
+    idip[:, 2, :] = -idip[:, 2, :]
+    e_img = eps_down if image_eps == "reference" else eps_up
+    return (dipolestat_field(pos, pts, dip, eps_up)
+            + image_factor(eps_up, eps_down) * dipolestat_field(pos, ipts, idip, e_img))
+
+
+def _layer_rad(p, enei, eps_up, eps_down, pinf_nvec, pinf_area):
+    """(sca / (2 pi k0)) / (0.5 nb gamma) of the dipole p (3,) above the
+    interface over all of pinfty (@spectrumstatlayer/scattering.m:13-38 with
+    no medium selected; @dipolestatlayer/decayrate.m:54-57)."""
+    k0 = 2 * np.pi / enei
+    nb = np.sqrt(complex(eps_up))
+    gamma = 4 / 3 * k0 ** 3
+    f, k = efarfield(p[None, :], enei, pinf_nvec, eps_up, eps_down)
+    dsca = np.sum(np.abs(f) ** 2, axis=1)[:, 0] * (0.5 * k / k0)
+    dsca = np.where(np.abs(np.imag(dsca)) > 0, 0, dsca).real
+    return (pinf_area @ dsca) / (2 * np.pi * k0) / (0.5 * nb * gamma)
+
+
+def dipolestatlayer_decayrate(green, gimage, pts, dip, enei, eps_up, eps_down, sig, pinf_nvec, pinf_area):
+    """(tot, rad, rad0) (npt, ndip) of @dipolestatlayer/decayrate.m:25-61:
+    e is the field of the surface charges at each dipole, -(Gp + f2 Gp_r) sig
+    (@compgreenstatlayer/field.m with its eval), gimage the Green function of
+    [particle; image]; rad from the induced dipole plus dip / nb^2 radiating
+    above the interface; tot = 1 + rad + Im(e . dip) / (0.5 nb gamma)."""
+    from .dipole_oracle import field_points
+    pts = np.atleast_2d(pts)
+    dip = np.asarray(dip, dtype=float).reshape(len(pts), 3, -1)
+    npt, _, ndip = dip.shape
+    n, ncol = green.n, sig.shape[1]
+    k0 = 2 * np.pi / enei
+    nb = np.sqrt(complex(eps_up))
+    gamma = 4 / 3 * k0 ** 3
+    f2 = image_factor(eps_up, eps_down)
+    e, _ = field_points(green, pts, sig, np.zeros((n, 3, ncol), complex), 0.0, enei)
+    sig_r = np.vstack([np.zeros_like(sig), sig])
+    er, _ = field_points(gimage, pts, sig_r, np.zeros((2 * n, 3, ncol), complex), 0.0, enei)
+    e = e + f2 * er
+    indip = (green.area[:, None] * green.pos).T @ sig
+    tot, rad = np.zeros((npt, ndip)), np.zeros((npt, ndip))
+    for p in range(npt):
+        for i in range(ndip):
+            q = p + npt * i
+            rad[p, i] = np.real(_layer_rad(indip[:, q] + dip[p, :, i] / nb ** 2, enei, eps_up, eps_down,
+                                           pinf_nvec, pinf_area))
+            tot[p, i] = np.real(1 + rad[p, i] + np.imag(e[p, :, q] @ dip[p, :, i]) / (0.5 * nb * gamma))
+    return tot, rad, np.real(nb * gamma) * np.ones((npt, ndip))
+
+
+def dipolestatlayer_decayrate0(pts, dip, enei, eps_up, eps_down, z0, pinf_nvec, pinf_area, image_eps="reference"):
+    """(tot, rad, rad0) without a particle (@dipolestatlayer/decayrate0.m:
+    18-56): the reflected field q1 E_image at each dipole; tot = rad for real
+    media, else rad + Im(e . dip) / (0.5 nb gamma)."""
+    from .dipole_oracle import dipolestat_field
+    pts = np.atleast_2d(pts)
+    dip = np.asarray(dip, dtype=float).reshape(len(pts), 3, -1)
+    npt, _, ndip = dip.shape
+    k0 = 2 * np.pi / enei
+    nb = np.sqrt(complex(eps_up))
+    gamma = 4 / 3 * k0 ** 3
+    ipts = pts.copy()
+    ipts[:, 2] = 2 * z0 - ipts[:, 2]
+    idip = dip.copy()
+    idip[:, 2, :] = -idip[:, 2, :]
+    e = image_factor(eps_up, eps_down) * dipolestat_field(
+        pts, ipts, idip, eps_down if image_eps == "reference" else eps_up)
+    real = np.imag(eps_up) == 0 and np.imag(eps_down) == 0
+    tot, rad = np.zeros((npt, ndip)), np.zeros((npt, ndip))
+    for p in range(npt):
+        for i in range(ndip):
+            rad[p, i] = np.real(_layer_rad(dip[p, :, i] / nb ** 2, enei, eps_up, eps_down, pinf_nvec, pinf_area))
+            tot[p, i] = rad[p, i] if real else np.real(
+                rad[p, i] + np.imag(e[p, :, p, i] @ dip[p, :, i]) / (0.5 * nb * gamma))
+    return tot, rad, np.real(nb * gamma) * np.ones((npt, ndip))

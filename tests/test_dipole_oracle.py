@@ -171,3 +171,29 @@ def test_bem_decayrate_converges_to_mie(pinfty):
         errs.append((np.abs(tot[0] / tm[0] - 1).max(), np.abs(rad[0] / rm[0] - 1).max()))
     assert errs[1][0] < 0.05 and errs[1][1] < 0.05, errs
     assert errs[1][0] < errs[0][0]
+
+
+def test_stat_decayrate_converges_to_sphere_series():
+    """Quasistatic dipole decay rates (dipolestat) of the oracle's BEM on a
+    20 nm sphere approach the image series of the quasistatic sphere as the
+    mesh is refined (about 5% at 1,280 faces, 1% at 5,120)."""
+    from mnpbem_amd import shapes
+    from oracle import mnpbem_oracle as O
+    enei, eps_b = 520.0, 1.5
+    pts = np.array([[0, 0, 13.0], [0, 0, 16.0]])
+    dip = np.zeros((2, 3, 2))
+    dip[:, 2, 0] = 1
+    dip[:, 0, 1] = 1
+    errs = []
+    for level in (3, 4):
+        v, f = shapes.ellipsoid(level, (10.0, 10.0, 10.0))
+        g = O.GreenRet(v, f)
+        ein = O.eps_drude(enei, 10.0, 9.065, 0.0708)
+        phip = D.dipolestat_potential(g.nvec, D.dipolestat_field(g.pos, pts, dip, eps_b))
+        sig = O.BemStat(g).initmat(enei, ein, eps_b).solve(phip)
+        tot, rad, rad0 = D.dipolestat_decayrate(g, pts, dip, enei, eps_b, sig)
+        want = [D.sphere_stat_decayrate(enei, ein, eps_b, 10.0, z) for z in (13.0, 16.0)]
+        wt, wr = np.array([w[0] for w in want]), np.array([w[1] for w in want])
+        errs.append(max(np.max(np.abs(tot / wt - 1)), np.max(np.abs(rad / wr - 1))))
+        assert np.allclose(rad0, np.sqrt(eps_b) * 4 / 3 * (2 * np.pi / enei) ** 3)
+    assert errs[0] < 0.08 and errs[1] < 0.025 and errs[1] < errs[0] / 3, errs

@@ -99,3 +99,66 @@ def test_small_sphere_above_substrate_follows_image_dipole_model():
         assert abs(bem - model) < 0.05 * abs(model), (c, bem, model)
     # a sign slip in f2 would move the shift the other way
     assert abs(bem - (1 / (1 - 2 * alpha0 * f2 / D3) - 1)) > 0.5 * abs(model)
+
+
+def _fine_sphere():
+    """5,120 directions and weights of the unit sphere, tilted by 0.1 rad so
+    that no direction is horizontal (where equal media give 0 / 0)."""
+    from mnpbem_amd import shapes
+    v, f = shapes.ellipsoid(4, (1.0, 1.0, 1.0))
+    c, s = np.cos(0.1), np.sin(0.1)
+    v = v @ np.array([[1, 0, 0], [0, c, s], [0, -s, c]])
+    g = O.GreenRet(v, f)
+    return g.nvec, g.area / np.sum(g.area) * 4 * np.pi
+
+
+def test_dipolestatlayer_free_limit():
+    """With eps_down = eps_up the substrate disappears: the potential is the
+    free dipole's, tot - rad equals dipolestat's tot and rad its |p_tot|^2
+    (the radiated power of a free dipole, to the quadrature of the sphere)."""
+    from mnpbem_amd import shapes
+    from oracle import dipole_oracle as D
+    v, f = shapes.ellipsoid(2, (8.0, 8.0, 12.0))
+    v = v + np.array([0, 0, 20.0])
+    g = O.GreenRet(v, f)
+    V, Fc = L.image_mesh(v, f, 0.0)
+    gi = O.GreenRet(V, Fc, closed=False)
+    enei, eb = 560.0, 1.7
+    ein = O.eps_drude(enei, 10.0, 9.065, 0.0708)
+    pts = np.array([[0, 0, 36.0], [3.0, 0, 37.0]])
+    dip = np.zeros((2, 3, 2))
+    dip[:, 2, 0] = 1
+    dip[:, 0, 1] = 1
+    e_free = D.dipolestat_field(g.pos, pts, dip, eb)
+    assert np.allclose(L.dipolestatlayer_field(g.pos, pts, dip, eb, eb, 0.0), e_free, rtol=1e-14)
+    sig = O.BemStat(g).initmat(enei, ein, eb).solve(D.dipolestat_potential(g.nvec, e_free))
+    nv, ar = _fine_sphere()
+    tot, rad, _ = L.dipolestatlayer_decayrate(g, gi, pts, dip, enei, eb, eb, sig, nv, ar)
+    tf, rf, _ = D.dipolestat_decayrate(g, pts, dip, enei, eb, sig)
+    assert np.allclose(tot - rad, tf, rtol=1e-12)
+    assert np.allclose(rad, rf, rtol=2e-3), (rad, rf)
+
+
+def test_dipolestatlayer_decayrate0():
+    """Without a particle: a lossless substrate gives tot = rad; the total
+    rate of a lossy one exceeds the radiative rate by the image term, which
+    decays as the cube of the height; equal media radiate as a free dipole."""
+    nv, ar = _fine_sphere()
+    pts = np.array([[0, 0, 5.0], [0, 0, 10.0]])
+    dip = np.zeros((2, 3, 2))
+    dip[:, 2, 0] = 1
+    dip[:, 0, 1] = 1
+    tot, rad, _ = L.dipolestatlayer_decayrate0(pts, dip, 600.0, 1.0, 2.25, 0.0, nv, ar)
+    assert np.array_equal(tot, rad) and np.all(rad > 0)
+    # lossy metal substrate: with the image field in the upper medium the
+    # absorbed part is positive and falls as height^-3; the reference's
+    # permittivity of the image point's (lower) medium makes it negative
+    tot, rad, _ = L.dipolestatlayer_decayrate0(pts, dip, 600.0, 1.0, -10 + 1j, 0.0, nv, ar, image_eps="upper")
+    nr = tot - rad
+    assert np.all(nr > 0)
+    assert np.allclose(nr[0] / nr[1], 8.0, rtol=1e-12)
+    assert np.allclose(nr[:, 0] / nr[:, 1], 2.0, rtol=1e-12)  # perpendicular vs parallel image field
+    tot, rad, _ = L.dipolestatlayer_decayrate0(pts, dip, 600.0, 1.0, -10 + 1j, 0.0, nv, ar)
+    assert np.all(tot - rad < 0)
+    tot, rad, _ = L.dipolestatlayer_decayrate0(pts, dip, 600.0, 1.3, 1.3, 0.0, nv, ar)
+    assert np.allclose(rad, 1.0, rtol=2e-3) and np.array_equal(tot, rad)
