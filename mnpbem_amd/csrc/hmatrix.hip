@@ -353,6 +353,64 @@ __device__ __forceinline__ void col_product(double2 (&acc)[NR], double2* bs, con
   }
 }
 
+// acc0[q] += a0 * bs[q], acc1[q] += a1 * bs[q]: one LDS read of bs serves two rows
+template <class TS, int NR>
+__device__ __forceinline__ void lds_fma2(double2 (&acc0)[NR], double2 (&acc1)[NR], const cplx<TS> a0,
+                                         const cplx<TS> a1, const double2* bs) {
+  const double ar = a0.x, ai = a0.y, br = a1.x, bi = a1.y;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const double2 b = bs[q];
+    acc0[q].x += ar * b.x - ai * b.y;
+    acc0[q].y += ar * b.y + ai * b.x;
+    acc1[q].x += br * b.x - bi * b.y;
+    acc1[q].y += br * b.y + bi * b.x;
+  }
+}
+
+// col_product for two rows per lane (A0 and A1 already offset to them):
+// each staged right-hand-side value read from LDS feeds both rows, which
+// halves the LDS traffic per matrix element.  Lanes take the columns
+// j = j0 + js t.
+template <class TS, class TB, int NR>
+__device__ __forceinline__ void col_product2(double2 (&acc0)[NR], double2 (&acc1)[NR], double2* bs,
+                                             const cplx<TS>* A0, const cplx<TS>* A1, int64_t lda, int K,
+                                             const TB* B, int nrhs, int lane, int j0, int js) {
+  constexpr int U = HMV_UNROLL / 2;
+  const cplx<TS> z = mk<TS>(TS(0), TS(0));
+  for (int k0 = 0; k0 < K; k0 += HMV_KC) {
+    const int kc = min(HMV_KC, K - k0);
+    const cplx<TS>* Ak0 = A0 + (int64_t)k0 * lda;
+    const cplx<TS>* Ak1 = A1 + (int64_t)k0 * lda;
+    cplx<TS> a0[U], a1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t o = (int64_t)min(j0 + u * js, kc - 1) * lda;
+      a0[u] = Ak0[o];
+      a1[u] = Ak1[o];
+    }
+    stage_rows<NR>(bs, B, k0, kc, nrhs, lane);
+    const int step = U * js, nit = (kc + step - 1) / step;  // the same for every lane
+    for (int i = 0; i < nit; ++i) {
+      const int jb = j0 + i * step;
+      if (i > 0)
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int64_t o = (int64_t)min(jb + u * js, kc - 1) * lda;
+          a0[u] = Ak0[o];
+          a1[u] = Ak1[o];
+        }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = jb + u * js;
+        const bool in = j < kc;
+        lds_fma2<TS, NR>(acc0, acc1, in ? a0[u] : z, in ? a1[u] : z, bs + min(j, kc - 1) * NR);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // First phase, t_b = R_b^T x(cols) (r x nrhs, FP64, t[l * nrhs + q]): one
 // wave per item = (block, <= HMV_CHUNK consecutive columns).  Lane =
 // (l, jj) = (lane % RP, lane / RP), RP = the rank rounded up to 16, 32 or
@@ -420,8 +478,10 @@ __global__ void k_hmv_low_sum(int64_t nmulti, const int32_t* multi, const int64_
 // tile's rows of V times x(cols)) or low-rank block (the tile's contiguous
 // L segment times t_b): A at aoff (rows at stride lda, K columns); boff >= 0
 // is the first column of x, boff < 0 encodes toff as -boff - 1.  The four
-// waves take descriptors round-robin; lane = (row, kk), kk splitting the
-// columns into even/odd, so one load instruction reads two 32-row columns.
+// waves take descriptors round-robin; lane = (rp, kk): rows rp and rp + 16,
+// kk splitting the columns four ways, so one load instruction reads four
+// 16-row column segments and each right-hand-side value read from LDS
+// serves two rows.
 template <class TS, int NR>
 __global__ void __launch_bounds__(256) k_hmv_tile(const int32_t* tilebeg, const int32_t* tdbeg, const int64_t* taoff,
                                                   const int64_t* tboff, const int32_t* tlda, const int32_t* tK,
@@ -431,29 +491,38 @@ __global__ void __launch_bounds__(256) k_hmv_tile(const int32_t* tilebeg, const 
   __shared__ double2 bsh[4][HMV_KC * NR];
   const int tile = blockIdx.x;
   const int tb = tilebeg[tile], mt = tilebeg[tile + 1] - tb;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, row = lane & 31, kk = lane >> 5;
-  const int rowc = min(row, mt - 1);  // rows past the tile read valid memory and are not stored
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, rp = lane & 15, kk = lane >> 4;
+  // rows past the tile read valid memory and are not stored
+  const int r0 = min(rp, mt - 1), r1 = min(rp + 16, mt - 1);
   double2* bs = bsh[wv];
-  double2 acc[NR];
+  double2 acc0[NR], acc1[NR];
 #pragma unroll
-  for (int q = 0; q < NR; ++q) acc[q] = mk<double>(0, 0);
+  for (int q = 0; q < NR; ++q) acc0[q] = acc1[q] = mk<double>(0, 0);
   const int s1 = tdbeg[tile + 1];
   for (int s = tdbeg[tile] + wv; s < s1; s += 4) {
     const int64_t aoff = taoff[s], boff = tboff[s];
     const int lda = tlda[s], K = tK[s];
     if (boff >= 0)
-      col_product<TS, cplx<TS>, NR>(acc, bs, fvals + aoff + rowc, lda, K, xc + boff * nrhs, nrhs, lane, kk, 2);
+      col_product2<TS, cplx<TS>, NR>(acc0, acc1, bs, fvals + aoff + r0, fvals + aoff + r1, lda, K,
+                                     xc + boff * nrhs, nrhs, lane, kk, 4);
     else
-      col_product<TS, double2, NR>(acc, bs, pool + aoff + rowc, lda, K, t + (-boff - 1), nrhs, lane, kk, 2);
+      col_product2<TS, double2, NR>(acc0, acc1, bs, pool + aoff + r0, pool + aoff + r1, lda, K, t + (-boff - 1),
+                                    nrhs, lane, kk, 4);
   }
 #pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    acc[q].x += __shfl_xor(acc[q].x, 32);
-    acc[q].y += __shfl_xor(acc[q].y, 32);
-  }
+  for (int q = 0; q < NR; ++q)
+    for (int sh = 16; sh < 64; sh <<= 1) {
+      acc0[q].x += __shfl_xor(acc0[q].x, sh);
+      acc0[q].y += __shfl_xor(acc0[q].y, sh);
+      acc1[q].x += __shfl_xor(acc1[q].x, sh);
+      acc1[q].y += __shfl_xor(acc1[q].y, sh);
+    }
   if (kk == 0)
 #pragma unroll
-    for (int q = 0; q < NR; ++q) red[wv][row][q] = acc[q];
+    for (int q = 0; q < NR; ++q) {
+      red[wv][rp][q] = acc0[q];
+      red[wv][rp + 16][q] = acc1[q];
+    }
   __syncthreads();
   for (int e = threadIdx.x; e < mt * nrhs; e += blockDim.x) {
     const int i = e / nrhs, q = e % nrhs;
