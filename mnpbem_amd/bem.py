@@ -1062,12 +1062,82 @@ class dipolestat:
         return tot, rad, np.full((npt, ndip), np.real(nb * gamma))
 
 
+class dipolestatmirror:
+    """Quasistatic dipoles for a comparticlemirror (Simulation/static/
+    @dipolestatmirror); ``pt``, ``dip``, ``full`` as for dipolestat, medium
+    the particle's outside.  The potential holds J excitations of the
+    symmetry part: unit dipoles along x, y, z at the points and at their
+    mirror images (moments mirrored), combined with the signs of each
+    symmetry value (init.m:11-25, potential.m:19-41); ``full(sig)`` averages
+    their expansions to the full particle, which leaves the solution of the
+    unit dipoles at the points alone, and contracts it with the moments
+    (full.m:14-43).  Decay rates are dipolestat's on the full particle."""
+
+    # potential.m:28-40: (sign of each mirror set, symmetry keys x, y, z)
+    _COMB = {"x": [((1, 1), ("-", "+", "+")), ((1, -1), ("+", "-", "-"))],
+             "y": [((1, 1), ("+", "-", "+")), ((1, -1), ("-", "+", "-"))],
+             "xy": [((1, 1, 1, 1), ("-+", "+-", "++")), ((1, -1, 1, -1), ("++", "--", "-+")),
+                    ((1, 1, -1, -1), ("--", "++", "+-")), ((1, -1, -1, 1), ("+-", "-+", "--"))]}
+    # mirror sets: (flipped axes, moment signs) (init.m:14-24)
+    _SETS = {"x": [((), (1, 1, 1)), ((0,), (-1, 1, 1))], "y": [((), (1, 1, 1)), ((1,), (1, -1, 1))],
+             "xy": [((), (1, 1, 1)), ((0,), (-1, 1, 1)), ((1,), (1, -1, 1)), ((0, 1), (-1, -1, 1))]}
+
+    def __init__(self, pt, dip=None, op=None, full=False):
+        self.dip = dipolestat(pt, dip, op, full=full)
+        self.pt = self.dip.pt
+
+    def __call__(self, p, enei):
+        pf = p.full()
+        npt = len(self.pt)
+        w = np.full(npt, 1.0 / complex(p.eps2(enei)))
+        phis = []
+        for axes, sgn in self._SETS[p.sym]:
+            pts = self.pt.copy()
+            for a in axes:
+                pts[:, a] = -pts[:, a]
+            m = dipolestat(pts, np.diag(np.asarray(sgn, dtype=np.float64)))
+            phis.append(m._potential(pf, w)[:p.n])   # the symmetry part: first n faces
+        exc = compstructmirror(p, enei, self.full)
+        for signs, keys in self._COMB[p.sym]:
+            v = dev.colmajor((p.n, 3 * npt))
+            v.copy_(sum(s * ph for s, ph in zip(signs, phis)))
+            exc.val.append(compstruct(p, enei, symval=p.symvalue(list(keys)), phip=v))
+        return exc
+
+    potential = __call__
+
+    def full(self, sig):
+        """Surface charge of the full particle for the dipoles' moments (full.m)."""
+        import torch
+        p = sig.p
+        pf = p.full()
+        J, npt = len(p.symtable), len(self.pt)
+        vmean = None
+        for val in sig.val:
+            v = torch.cat([float(s) * val.sig for s in val.symval[-1]], dim=0) / J
+            vmean = v if vmean is None else vmean + v
+        dip = self.dip.dip                                   # npt x 3 x ndip
+        out = dev.colmajor((pf.n, npt * dip.shape[2]))
+        out.zero_()
+        for j in range(dip.shape[2]):
+            for c in range(3):
+                wgt = torch.as_tensor(dip[:, c, j], dtype=torch.float64, device=out.device)
+                out[:, npt * j:npt * (j + 1)] += vmean[:, npt * c:npt * (c + 1)] * wgt[None, :]
+        return compstruct(pf, sig.enei, sig=out)
+
+    def decayrate(self, sig):
+        """[tot, rad, rad0] of the dipoles on the full particle (decayrate.m)."""
+        return self.dip.decayrate(self.full(sig))
+
+
 def dipole(pt, dip=None, op=None, **kw):
     """Base/dipole.m: dipoleret for op.sim == 'ret', dipolestat for 'stat',
     dipolestatlayer for 'stat' with op.layer."""
     if op is not None and op.get("sim", "ret") == "stat":
         if op.get("sym"):
-            raise NotImplementedError("mirror symmetry: the dipole excitations are not built")
+            if op.get("layer") is not None:
+                raise NotImplementedError("mirror symmetry with layers is not built")
+            return dipolestatmirror(pt, dip, op, **kw)
         if op.get("layer") is not None:
             from .layer import dipolestatlayer
             return dipolestatlayer(pt, dip, op, **kw)

@@ -106,3 +106,38 @@ def test_dipolestatlayer_mixed_and_errors():
         dipolestatlayer(np.array([[0, 0, -1.0]]), None, op)
     with pytest.raises(ValueError):
         dipolestatlayer(pts, None, op, image_eps="lower")
+
+
+@pytest.mark.parametrize("sym", ["x", "y", "xy"])
+def test_dipolestatmirror_equals_full_particle(sym):
+    """Dipoles near a mirror-symmetric particle: the expanded mirror solution
+    and its decay rates equal the oracle's dipolestat on the full particle
+    and the GPU dipolestat on the full particle."""
+    from mnpbem_amd import bem as B
+    from mnpbem_amd import device as dev
+    from oracle import mirror_oracle as M
+    seg = {"xy": (0.0, np.pi / 2), "y": (0.0, np.pi), "x": (-np.pi / 2, np.pi / 2)}[sym]
+    mesh = shapes.ellipsoid_segment(4 if sym == "xy" else 8, 10, (10.0, 12.0, 20.0), *seg)
+    op = B.bemoptions(sim="stat", sym=sym)
+    eps_b = 1.2
+    p = B.comparticlemirror([B.epsconst(eps_b), B.epsdrude(*DRUDE)], [mesh], [2, 1], 1, op)
+    pts = np.array([[3.0, 4.0, 24.0], [14.0, 2.0, 3.0]])
+    dip = np.array([[1.0, 0, 0], [0, 0.6, 0.8]])
+    exc = B.dipole(pts, dip, op)
+    assert isinstance(exc, B.dipolestatmirror)
+    bem = B.bemsolver(p, op, precision="fp64")
+    enei = 580.0
+    sig = bem.solve(exc(p, enei))
+    full = exc.full(sig)
+    tot, rad, _ = exc.decayrate(sig)
+    V, F = M.full_mesh(*mesh, sym)
+    g = O.GreenRet(V, F)
+    d3 = np.repeat(dip.T[None, :, :], len(pts), axis=0)
+    phip = D.dipolestat_potential(g.nvec, D.dipolestat_field(g.pos, pts, d3, eps_b))
+    want = O.BemStat(g).initmat(enei, O.eps_drude(enei, *DRUDE), eps_b).solve(phip)
+    err = np.linalg.norm(dev.to_host(full.sig) - want) / np.linalg.norm(want)
+    tw, rw, _ = D.dipolestat_decayrate(g, pts, d3, enei, eps_b, want)
+    print(f"[dipolestatmirror {sym}] part {p.n}: sig rel err {err:.2e}; tot {tot.ravel()} vs {tw.ravel()}",
+          flush=True)
+    assert err < 1e-9
+    assert np.allclose(tot, tw, rtol=1e-9) and np.allclose(rad, rw, rtol=1e-9)
