@@ -1130,6 +1130,78 @@ class dipolestatmirror:
         return self.dip.decayrate(self.full(sig))
 
 
+class dipoleretmirror:
+    """Retarded dipoles for a comparticlemirror (Simulation/retarded/
+    @dipoleretmirror): the construction of dipolestatmirror with dipoleret's
+    potentials phi2, phi2p, a2, a2p (init.m, potential.m); ``full(sig)``
+    expands sig1, sig2 (scalar key) and h1, h2 (per-component keys),
+    averages and contracts with the moments (full.m); decay rates are
+    dipoleret's on the full particle (decayrate.m)."""
+
+    _COMB = dipolestatmirror._COMB
+    _SETS = dipolestatmirror._SETS
+
+    def __init__(self, pt, dip=None, op=None, full=False, pinfty=None):
+        self.dip = dipoleret(pt, dip, op, pinfty=pinfty, full=full)
+        self.pt = self.dip.pt
+
+    def __call__(self, p, enei):
+        pf = p.full()
+        npt, n = len(self.pt), p.n
+        sets = []
+        for axes, sgn in self._SETS[p.sym]:
+            pts = self.pt.copy()
+            for a in axes:
+                pts[:, a] = -pts[:, a]
+            m = dipoleret(pts, np.diag(np.asarray(sgn, dtype=np.float64)), pinfty=(np.zeros((1, 3)), np.zeros(1)))
+            pot = m(pf, enei)
+            sets.append({k: pot[k][:n] for k in ("phi2", "phi2p", "a2", "a2p")})
+        exc = compstructmirror(p, enei, self.full)
+        for signs, keys in self._COMB[p.sym]:
+            fields = {}
+            for k in ("phi2", "phi2p", "a2", "a2p"):
+                shape = (n, 3 * npt) if k.startswith("phi") else (n, 3, 3 * npt)
+                v = dev.colmajor(shape)
+                v.copy_(sum(sg * st[k] for sg, st in zip(signs, sets)))
+                fields[k] = v
+            exc.val.append(compstruct(p, enei, symval=p.symvalue(list(keys)), **fields))
+        return exc
+
+    potential = __call__
+
+    def full(self, sig):
+        """Charges and currents of the full particle for the dipoles' moments (full.m)."""
+        import torch
+        p = sig.p
+        pf = p.full()
+        J, npt = len(p.symtable), len(self.pt)
+        dip = self.dip.dip
+        out = {}
+        for key in ("sig1", "sig2", "h1", "h2"):
+            vmean = None
+            for val in sig.val:
+                sv = val.symval
+                if key.startswith("sig"):
+                    v = torch.cat([float(sv[-1, j]) * val[key] for j in range(J)], dim=0)
+                else:
+                    sc = torch.as_tensor(sv, dtype=torch.float64, device=val[key].device)  # 3 x J
+                    v = torch.cat([val[key] * sc[:, j][None, :, None] for j in range(J)], dim=0)
+                vmean = v / J if vmean is None else vmean + v / J
+            shape = (pf.n, npt * dip.shape[2]) if key.startswith("sig") else (pf.n, 3, npt * dip.shape[2])
+            o = dev.colmajor(shape)
+            o.zero_()
+            for j in range(dip.shape[2]):
+                for c in range(3):
+                    wgt = torch.as_tensor(dip[:, c, j], dtype=torch.float64, device=o.device)
+                    o[..., npt * j:npt * (j + 1)] += vmean[..., npt * c:npt * (c + 1)] * wgt
+            out[key] = o
+        return compstruct(pf, sig.enei, **out)
+
+    def decayrate(self, sig):
+        """[tot, rad, rad0] of the dipoles on the full particle (decayrate.m)."""
+        return self.dip.decayrate(self.full(sig))
+
+
 def dipole(pt, dip=None, op=None, **kw):
     """Base/dipole.m: dipoleret for op.sim == 'ret', dipolestat for 'stat',
     dipolestatlayer for 'stat' with op.layer."""
@@ -1142,6 +1214,8 @@ def dipole(pt, dip=None, op=None, **kw):
             from .layer import dipolestatlayer
             return dipolestatlayer(pt, dip, op, **kw)
         return dipolestat(pt, dip, op, **kw)
-    if op is not None and (op.get("layer") is not None or op.get("sym")):
-        raise NotImplementedError("retarded dipoles with layers or mirror symmetry are not built")
+    if op is not None and op.get("layer") is not None:
+        raise NotImplementedError("retarded dipoles with layers are not built")
+    if op is not None and op.get("sym"):
+        return dipoleretmirror(pt, dip, op, **kw)
     return dipoleret(pt, dip, op, **kw)

@@ -141,3 +141,43 @@ def test_dipolestatmirror_equals_full_particle(sym):
           flush=True)
     assert err < 1e-9
     assert np.allclose(tot, tw, rtol=1e-9) and np.allclose(rad, rw, rtol=1e-9)
+
+
+@pytest.mark.parametrize("sym", ["x", "xy"])
+def test_dipoleretmirror_equals_full_particle(sym):
+    """Retarded dipoles near a mirror-symmetric particle: the expanded
+    charges equal the oracle's bemret solution on the full particle and the
+    decay rates equal the GPU dipoleret's on the full particle."""
+    from mnpbem_amd import bem as B
+    from mnpbem_amd import device as dev
+    from oracle import mirror_oracle as M
+    seg = {"xy": (0.0, np.pi / 2), "x": (-np.pi / 2, np.pi / 2)}[sym]
+    mesh = shapes.ellipsoid_segment(4 if sym == "xy" else 8, 10, (10.0, 12.0, 20.0), *seg)
+    op = B.bemoptions(sim="ret", sym=sym)
+    eps_b = 1.2
+    p = B.comparticlemirror([B.epsconst(eps_b), B.epsdrude(*DRUDE)], [mesh], [2, 1], 1, op)
+    pts = np.array([[3.0, 4.0, 24.0], [14.0, 2.0, 3.0]])
+    dip = np.array([[1.0, 0, 0], [0, 0.6, 0.8]])
+    exc = B.dipole(pts, dip, op)
+    assert isinstance(exc, B.dipoleretmirror)
+    bem = B.bemsolver(p, op, precision="fp64")
+    enei = 580.0
+    sig = bem.solve(exc(p, enei))
+    full = exc.full(sig)
+    tot, rad, _ = exc.decayrate(sig)
+    V, F = M.full_mesh(*mesh, sym)
+    g = O.GreenRet(V, F)
+    d3 = np.repeat(dip.T[None, :, :], len(pts), axis=0)
+    phi, phip, a, ap = D.dipole_potential(g.pos, g.nvec, pts, d3, enei, eps_b)
+    want = O.BemRet(g).initmat(enei, O.eps_drude(enei, *DRUDE), eps_b).solve(
+        {"phi2": phi, "phi2p": phip, "a2": a, "a2p": ap})
+    for key in ("sig2", "h2"):
+        err = np.linalg.norm(dev.to_host(full[key]) - want[key]) / np.linalg.norm(want[key])
+        print(f"[dipoleretmirror {sym}] {key} rel err {err:.2e}", flush=True)
+        assert err < 1e-9
+    fop = B.bemoptions(sim="ret")
+    pf = B.comparticle(p.eps, [(p.full().verts, p.full().faces)], [2, 1], 1, fop)
+    excf = B.dipole(pts, dip, fop)
+    tf, rf, _ = excf.decayrate(B.bemsolver(pf, fop, precision="fp64").solve(excf(pf, enei)))
+    print(f"  tot {tot.ravel()} vs full {tf.ravel()}", flush=True)
+    assert np.allclose(tot, tf, rtol=1e-8) and np.allclose(rad, rf, rtol=1e-8)
