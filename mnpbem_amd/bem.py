@@ -61,6 +61,62 @@ class epsfun:
         return eps, 2 * np.pi / enei * np.sqrt(eps)
 
 
+def _notaknot(x, y):
+    """Cubic spline through (x, y), x increasing, with not-a-knot ends (the
+    end conditions of Matlab's spline for four or more points): returns the
+    second derivatives at the knots."""
+    n = len(x)
+    if n < 4:
+        raise ValueError("epstable: at least four tabulated points are needed")
+    h = np.diff(x)
+    d = np.diff(y) / h
+    A = np.zeros((n, n))
+    r = np.zeros(n, dtype=y.dtype)
+    for i in range(1, n - 1):
+        A[i, i - 1:i + 2] = h[i - 1], 2 * (h[i - 1] + h[i]), h[i]
+        r[i] = 6 * (d[i] - d[i - 1])
+    # third derivative continuous at the second and the second-last knot
+    A[0, :3] = h[1], -(h[0] + h[1]), h[0]
+    A[-1, -3:] = h[-1], -(h[-2] + h[-1]), h[-2]
+    return np.linalg.solve(A, r)
+
+
+class epstable:
+    """Tabulated dielectric function (Material/@epstable): a file of lines
+    'energy (eV)  n  k' ('%' starts a comment); n and k are interpolated by
+    not-a-knot cubic splines in the wavelength eV2nm / energy (epstable.m:
+    19-26) and eps = (n + i k)^2 (subsref.m:22-33).  Wavelengths outside the
+    table raise, as the reference asserts."""
+
+    def __init__(self, path):
+        rows = []
+        with open(path) as fh:
+            for line in fh:
+                line = line.split("%", 1)[0].split()
+                if line:
+                    rows.append([float(v) for v in line[:3]])
+        ene, n, k = np.asarray(rows).T
+        order = np.argsort(EV2NM / ene)
+        self.enei = (EV2NM / ene)[order]
+        self.ni, self.ki = n[order], k[order]
+        self._m = _notaknot(self.enei, self.ni + 1j * self.ki)
+
+    def nk(self, enei):
+        """n + i k at the wavelength (the splines of n and k)."""
+        x, m = self.enei, self._m
+        if not (x[0] <= enei <= x[-1]):
+            raise ValueError(f"epstable: wavelength {enei} nm outside the table [{x[0]}, {x[-1]}]")
+        i = min(max(int(np.searchsorted(x, enei)) - 1, 0), len(x) - 2)
+        h = x[i + 1] - x[i]
+        a, b = (x[i + 1] - enei) / h, (enei - x[i]) / h
+        y = self.ni + 1j * self.ki
+        return a * y[i] + b * y[i + 1] + ((a ** 3 - a) * m[i] + (b ** 3 - b) * m[i + 1]) * h * h / 6
+
+    def __call__(self, enei):
+        eps = complex(self.nk(enei)) ** 2
+        return eps, 2 * np.pi / enei * np.sqrt(eps)
+
+
 # ------------------------------------------------------------------ options
 def bemoptions(**kw):
     """bemoptions('sim','ret', ...) defaults (Misc/bemoptions.m:18-24)."""
