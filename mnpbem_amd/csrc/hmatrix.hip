@@ -310,49 +310,6 @@ __device__ __forceinline__ void stage_rows(double2* bs, const TB* B, int c0, int
   __builtin_amdgcn_wave_barrier();
 }
 
-// acc[q] += a * bs[q], q < NR, bs in LDS (broadcast reads)
-template <class TS, int NR>
-__device__ __forceinline__ void lds_fma(double2 (&acc)[NR], const cplx<TS> a, const double2* bs) {
-  const double ar = a.x, ai = a.y;
-#pragma unroll
-  for (int q = 0; q < NR; ++q) {
-    const double2 b = bs[q];
-    acc[q].x += ar * b.x - ai * b.y;
-    acc[q].y += ar * b.y + ai * b.x;
-  }
-}
-
-// acc += A(:, 0:K) * B(0:K, :) for this lane's row: A's columns at stride lda
-// (A already offset to the lane's row), B's rows staged HMV_KC at a time.
-// Lanes take the columns j = j0 + js * t; the first batch of A loads is
-// issued before the staging, so the two round trips overlap.
-template <class TS, class TB, int NR>
-__device__ __forceinline__ void col_product(double2 (&acc)[NR], double2* bs, const cplx<TS>* A, int64_t lda, int K,
-                                            const TB* B, int nrhs, int lane, int j0, int js) {
-  const cplx<TS> z = mk<TS>(TS(0), TS(0));
-  for (int k0 = 0; k0 < K; k0 += HMV_KC) {
-    const int kc = min(HMV_KC, K - k0);
-    const cplx<TS>* Ak = A + (int64_t)k0 * lda;
-    cplx<TS> a[HMV_UNROLL];
-#pragma unroll
-    for (int u = 0; u < HMV_UNROLL; ++u) a[u] = Ak[(int64_t)min(j0 + u * js, kc - 1) * lda];
-    stage_rows<NR>(bs, B, k0, kc, nrhs, lane);
-    const int step = HMV_UNROLL * js, nit = (kc + step - 1) / step;  // the same for every lane
-    for (int i = 0; i < nit; ++i) {
-      const int jb = j0 + i * step;
-      if (i > 0)
-#pragma unroll
-        for (int u = 0; u < HMV_UNROLL; ++u) a[u] = Ak[(int64_t)min(jb + u * js, kc - 1) * lda];
-#pragma unroll
-      for (int u = 0; u < HMV_UNROLL; ++u) {
-        const int j = jb + u * js;
-        lds_fma<TS, NR>(acc, j < kc ? a[u] : z, bs + min(j, kc - 1) * NR);
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-}
-
 // acc0[q] += a0 * bs[q], acc1[q] += a1 * bs[q]: one LDS read of bs serves two rows
 template <class TS, int NR>
 __device__ __forceinline__ void lds_fma2(double2 (&acc0)[NR], double2 (&acc1)[NR], const cplx<TS> a0,
@@ -368,10 +325,11 @@ __device__ __forceinline__ void lds_fma2(double2 (&acc0)[NR], double2 (&acc1)[NR
   }
 }
 
-// col_product for two rows per lane (A0 and A1 already offset to them):
-// each staged right-hand-side value read from LDS feeds both rows, which
-// halves the LDS traffic per matrix element.  Lanes take the columns
-// j = j0 + js t.
+// acc0 += A0(:, 0:K) * B(0:K, :), acc1 += A1(:, 0:K) * B(0:K, :) for this
+// lane's two rows: A's columns at stride lda (A0, A1 already offset to the
+// rows), B's rows staged HMV_KC at a time; each staged value read from LDS
+// feeds both rows.  Lanes take the columns j = j0 + js t; the first batch of
+// A loads is issued before the staging, so the two round trips overlap.
 template <class TS, class TB, int NR>
 __device__ __forceinline__ void col_product2(double2 (&acc0)[NR], double2 (&acc1)[NR], double2* bs,
                                              const cplx<TS>* A0, const cplx<TS>* A1, int64_t lda, int K,
@@ -412,13 +370,15 @@ __device__ __forceinline__ void col_product2(double2 (&acc0)[NR], double2 (&acc1
 }
 
 // First phase, t_b = R_b^T x(cols) (r x nrhs, FP64, t[l * nrhs + q]): one
-// wave per item = (block, <= HMV_CHUNK consecutive columns).  Lane =
-// (l, jj) = (lane % RP, lane / RP), RP = the rank rounded up to 16, 32 or
-// 64 (ranks above 64 take several passes), so each load instruction reads
-// 64 / RP consecutive columns of R^T, contiguous in memory; x is staged in
-// LDS HMV_KC columns at a time and the partial sums over jj are combined by
-// shuffles.  Items of blocks with one item write t_b, the others write
-// partial sums that k_hmv_low_sum adds in item order.
+// wave per item = (block, <= HMV_CHUNK consecutive columns).  RP = the rank
+// rounded up to 16, 32 or 64 (ranks above 64 take several passes of 64) and
+// H = RP / 2; lane = (l, jj) = (lane % H, lane / H) holds the factors l and
+// l + H, so each x value read from LDS serves two factors, and each load
+// instruction reads 64 / H consecutive columns of R^T (H factors each,
+// contiguous in memory); x is staged in LDS HMV_KC columns at a time and the
+// partial sums over jj are combined by shuffles.  Items of blocks with one
+// item write t_b, the others write partial sums that k_hmv_low_sum adds in
+// item order.
 // item[4 i ..] = {block, first column, end column, partial-sum slot or -1}.
 template <class TS, int NR>
 __global__ void __launch_bounds__(256) k_hmv_low(int64_t nitems, const int32_t* item, const int64_t* tpoff,
@@ -434,27 +394,33 @@ __global__ void __launch_bounds__(256) k_hmv_low(int64_t nitems, const int32_t* 
   const int r0 = blk[4 * b], m = blk[4 * b + 1] - r0, c0 = blk[4 * b + 2];
   const int r = rank[b];
   const int RP = r <= 16 ? 16 : r <= 32 ? 32 : 64;
-  const int JS = 64 / RP;
+  const int H = RP / 2, JS = 64 / H;
   const cplx<TS>* RT = pool + loff[b] + (int64_t)m * r;
   double2* dst = slot < 0 ? t + toff[b] : tpart + tpoff[it];
-  for (int lb = 0; lb < r; lb += 64) {
-    const int l = lb + lane % RP, jj = lane / RP;
-    double2 acc[NR];
+  for (int lb = 0; lb < r; lb += RP) {
+    const int la = lb + lane % H, lh = la + H, jj = lane / H;
+    double2 acc0[NR], acc1[NR];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) acc[q] = mk<double>(0, 0);
-    // column j of the item is R^T row (j0 + j): entries at stride r, this lane's factor l
-    col_product<TS, cplx<TS>, NR>(acc, bs, RT + (int64_t)j0 * r + min(l, r - 1), r, j1 - j0,
-                                  xc + (int64_t)(c0 + j0) * nrhs, nrhs, lane, jj, JS);
-    for (int sh = RP; sh < 64; sh <<= 1)
+    for (int q = 0; q < NR; ++q) acc0[q] = acc1[q] = mk<double>(0, 0);
+    // column j of the item is R^T row (j0 + j): entries at stride r, this lane's factors la, lh
+    const cplx<TS>* RTj = RT + (int64_t)j0 * r;
+    col_product2<TS, cplx<TS>, NR>(acc0, acc1, bs, RTj + min(la, r - 1), RTj + min(lh, r - 1), r, j1 - j0,
+                                   xc + (int64_t)(c0 + j0) * nrhs, nrhs, lane, jj, JS);
+    for (int sh = H; sh < 64; sh <<= 1)
 #pragma unroll
       for (int q = 0; q < NR; ++q) {
-        acc[q].x += __shfl_xor(acc[q].x, sh);
-        acc[q].y += __shfl_xor(acc[q].y, sh);
+        acc0[q].x += __shfl_xor(acc0[q].x, sh);
+        acc0[q].y += __shfl_xor(acc0[q].y, sh);
+        acc1[q].x += __shfl_xor(acc1[q].x, sh);
+        acc1[q].y += __shfl_xor(acc1[q].y, sh);
       }
-    if (jj == 0 && l < r)
+    if (jj == 0)
 #pragma unroll
       for (int q = 0; q < NR; ++q)
-        if (q < nrhs) dst[(int64_t)l * nrhs + q] = acc[q];
+        if (q < nrhs) {
+          if (la < r) dst[(int64_t)la * nrhs + q] = acc0[q];
+          if (lh < r) dst[(int64_t)lh * nrhs + q] = acc1[q];
+        }
   }
 }
 
