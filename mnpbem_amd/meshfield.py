@@ -10,7 +10,8 @@ The side of a point follows Misc/distmin3.m's sign: the point's offset from
 the centroid of its nearest face, projected on that face's normal (distmin3
 takes the face of the nearest quadrature point; the two agree for points
 farther from the surface than a face's size).  Points closer to the surface
-than ``mindist`` get NaN fields, as compoint leaves them out.
+than ``mindist`` get NaN fields, as compoint leaves them out.  Quasistatic
+solutions take e = -Gp sig at k = 0 (the same kernel, no currents).
 """
 import ctypes as C
 
@@ -81,14 +82,22 @@ class meshfield:
         return dev.to_host(e), dev.to_host(hf)
 
     def field(self, sig):
-        """(e, h): mesh shape x 3 x npol, complex128 host arrays (field1.m)."""
+        """(e, h): mesh shape x 3 x npol, complex128 host arrays (field1.m).
+        A quasistatic solution (surface charge ``sig`` only) gives e = -Gp sig
+        at k = 0 on both sides (@compgreenstat/field.m) and h = None."""
         p, enei = self.p, sig.enei
-        ncol = sig.sig2.shape[-1]
+        static = "sig" in sig and "sig2" not in sig
+        ncol = (sig.sig if static else sig.sig2).shape[-1]
         e = np.full((len(self.pts), 3, ncol), np.nan, dtype=np.complex128)
         h = np.full_like(e, np.nan)
-        for sel, eps, s, hh in ((self.outside, p.eps2(enei), sig.sig2, sig.h2),
-                                (self.inside, p.eps1(enei), sig.sig1, sig.h1)):
+        if static:
+            zero = dev.colmajor((p.n, 3, ncol))
+            zero.zero_()
+            sides = ((self.outside | self.inside, 0.0, sig.sig, zero),)
+        else:
+            sides = ((self.outside, p.eps2(enei), sig.sig2, sig.h2), (self.inside, p.eps1(enei), sig.sig1, sig.h1))
+        for sel, eps, s, hh in sides:
             if sel.any():
                 e[sel], h[sel] = self._side(sel, enei, eps, s, hh)
         shape = self.x.shape + (3, ncol)
-        return e.reshape(shape), h.reshape(shape)
+        return e.reshape(shape), (None if static else h.reshape(shape))

@@ -45,3 +45,25 @@ def test_meshfield_inside_and_outside_match_oracle():
     assert np.all(np.isnan(eo[gone]))
     # a point on the z axis far above the particle is outside, the centre inside
     assert mf.outside.reshape(x.shape)[-1, 4] and mf.inside.reshape(x.shape)[5, 4]
+
+
+def test_meshfield_quasistatic_matches_oracle():
+    from mnpbem_amd import bem as B
+    from mnpbem_amd.meshfield import meshfield
+    v, f = shapes.ellipsoid(3, (10.0, 10.0, 15.0))
+    op = B.bemoptions(sim="stat")
+    p = B.comparticle([B.epsconst(1.33), B.epsdrude(*DRUDE)], [(v, f)], [2, 1], 1, op)
+    exc = B.planewave([[1.0, 0, 0], [0, 0, 1.0]], op=op)
+    sig = B.bemsolver(p, op, precision="fp64").solve(exc(p, 600.0))
+    x, z = np.meshgrid(np.linspace(-20, 20, 7), np.linspace(-25, 25, 9))
+    mf = meshfield(p, x, 0.0, z, mindist=0.5)
+    e, h = mf.field(sig)
+    assert h is None and e.shape == x.shape + (3, 2)
+    from mnpbem_amd import device as dev
+    g = O.GreenRet(v, f)
+    s = dev.to_host(sig.sig)
+    sel = mf.inside | mf.outside
+    want, _ = D.field_points(g, mf.pts[sel], s, np.zeros((p.n, 3, 2), complex), 0.0, 600.0)
+    err = np.abs(e.reshape(-1, 3, 2)[sel] - want).max() / np.abs(want).max()
+    print(f"[meshfield stat] {int(sel.sum())} points: rel err {err:.2e}", flush=True)
+    assert err < 1e-10
