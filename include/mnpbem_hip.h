@@ -488,7 +488,9 @@ int mnp_dipole_potential(mnp_greenret_t g, double enei, mnp_complex eps_medium, 
  * within the refinement cutoff of the greenret options (refinematrix.m,
  * the point taking the face radius) use the rule-18 Taylor form
  * (greenret/private/init.m:96-177).  Points must not coincide with
- * quadrature points of a face. */
+ * quadrature points of a face.  h may be NULL (no currents); with
+ * eps_medium = 0 (k = 0) e is the quasistatic field -Gp sig
+ * (@compgreenstat/field.m). */
 int mnp_field_points(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t npt,
                      const double* pts, int64_t ncol, const void* sig, const void* h, void* e,
                      void* hfield);

@@ -120,8 +120,9 @@ __global__ void __launch_bounds__(256) k_pt_field(int64_t n, Geo geo, int64_t np
       for (int c = 0; c < 3; ++c) Gp[c] = cscale<double>(fk, xv[c] * ar);
     }
     const double2 s = sig[q * n + j];
-    double2 hj[3];
-    for (int c = 0; c < 3; ++c) hj[c] = h[(c + 3 * q) * n + j];
+    double2 hj[3] = {{0, 0}, {0, 0}, {0, 0}};  // h == nullptr: no currents
+    if (h)
+      for (int c = 0; c < 3; ++c) hj[c] = h[(c + 3 * q) * n + j];
     const double2 ik0G = cmul<double>(mk<double>(0, k0), G);
     for (int c = 0; c < 3; ++c) {
       const double2 v = csub<double>(cmul<double>(ik0G, hj[c]), cmul<double>(Gp[c], s));
@@ -249,7 +250,7 @@ int mnp_dipole_potential(mnp_greenret_t g, double enei, mnp_complex eps_medium, 
 int mnp_field_points(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t npt, const double* pts,
                      int64_t ncol, const void* sig, const void* h, void* e, void* hfield) {
   return guarded([&] {
-    MNP_CHECK(g && pts && sig && h && e && hfield, MNP_ERR_ARG, "field_points: NULL argument");
+    MNP_CHECK(g && pts && sig && e && hfield, MNP_ERR_ARG, "field_points: NULL argument");
     MNP_CHECK(npt >= 1 && ncol >= 1 && enei > 0, MNP_ERR_ARG, "field_points: bad sizes");
     MNP_CHECK(npt < (1LL << 31) && ncol < 65536, MNP_ERR_ARG, "field_points: too many points or columns");
     const GreenRet& G = *g->g;
@@ -290,15 +291,14 @@ int mnp_stat_field_own(mnp_greenret_t g, int64_t npt, const double* pts, int64_t
     MNP_CHECK(npt >= 1 && ncol >= 1 && ncol < 65536, MNP_ERR_ARG, "stat_field_own: bad sizes");
     const GreenRet& G = *g->g;
     double* dpts = upload(pts, 3 * npt);
-    double2* h0 = dev_alloc<double2>(3 * G.n * ncol);  // no currents: k = 0 leaves E = -Gp sig
     double2* ed = dev_alloc<double2>(3 * ncol);
     double2* hd = dev_alloc<double2>(3 * ncol);
-    MNP_HIP(hipMemsetAsync(h0, 0, sizeof(double2) * 3 * G.n * ncol, stream()));
-    pt_field(G, 1.0, mnp_complex{0, 0}, npt, dpts, ncol, (const double2*)sig, h0, 1, ed, hd);
+    // no currents (h = nullptr) and k = 0 leave E = -Gp sig
+    pt_field(G, 1.0, mnp_complex{0, 0}, npt, dpts, ncol, (const double2*)sig, nullptr, 1, ed, hd);
     MNP_HIP(hipMemcpyAsync(e, ed, sizeof(double2) * 3 * ncol, hipMemcpyDeviceToHost, stream()));
     MNP_HIP(hipStreamSynchronize(stream()));
-    for (double* p : {dpts}) MNP_HIP(hipFree(p));
-    for (double2* p : {h0, ed, hd}) MNP_HIP(hipFree(p));
+    MNP_HIP(hipFree(dpts));
+    for (double2* p : {ed, hd}) MNP_HIP(hipFree(p));
   });
 }
 

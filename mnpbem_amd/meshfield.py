@@ -77,7 +77,8 @@ class meshfield:
         hf = dev.colmajor((npt, 3, ncol))
         pts = np.asfortranarray(self.pts[sel])
         _lib.call("mnp_field_points", self.p.green.handle, float(enei), _lib.MnpComplex.of(eps), npt,
-                  pts.ctypes.data_as(C.c_void_p), ncol, dev.ptr(dev.base(sig)), dev.ptr(dev.base(h)),
+                  pts.ctypes.data_as(C.c_void_p), ncol, dev.ptr(dev.base(sig)),
+                  None if h is None else dev.ptr(dev.base(h)),
                   dev.ptr(dev.base(e)), dev.ptr(dev.base(hf)))
         return dev.to_host(e), dev.to_host(hf)
 
@@ -91,9 +92,7 @@ class meshfield:
         e = np.full((len(self.pts), 3, ncol), np.nan, dtype=np.complex128)
         h = np.full_like(e, np.nan)
         if static:
-            zero = dev.colmajor((p.n, 3, ncol))
-            zero.zero_()
-            sides = ((self.outside | self.inside, 0.0, sig.sig, zero),)
+            sides = ((self.outside | self.inside, 0.0, sig.sig, None),)
         else:
             sides = ((self.outside, p.eps2(enei), sig.sig2, sig.h2), (self.inside, p.eps1(enei), sig.sig1, sig.h1))
         for sel, eps, s, hh in sides:
