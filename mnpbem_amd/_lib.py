@@ -5,6 +5,7 @@ fallback for any entry point.
 """
 import ctypes as C
 import os
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmnpbem_hip.so")
@@ -147,6 +148,14 @@ def lib():
             fn.argtypes = args
         _lib = h
     return _lib
+
+
+def can_release():
+    """True when a handle may be destroyed now: the library is loaded and the
+    interpreter is not shutting down.  At shutdown the handles are left to the
+    process exit (device memory goes with the process), so that no hipFree
+    runs while the HIP runtime or an attached profiler is being torn down."""
+    return _lib is not None and not sys.is_finalizing()
 
 
 def check(status, what=""):

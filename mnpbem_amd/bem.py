@@ -220,13 +220,14 @@ class bemret:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+        if h is not None and h.value and _lib.can_release():
             _lib.lib().mnp_bemret_destroy(h)
             self.handle = None
 
     def __call__(self, enei):
         """bem(enei): initialise the matrices for one wavelength (initmat.m)."""
         if self.enei != enei:
+            self.enei = None  # stays unset if initmat fails part-way
             e1, e2 = self.p.eps1(enei), self.p.eps2(enei)
             _lib.call("mnp_bemret_initmat", self.handle, float(enei), _lib.MnpComplex.of(e1),
                       _lib.MnpComplex.of(e2))
@@ -272,6 +273,11 @@ def initmat_many(bems, enei):
         return
     if len({id(bems[i]) for i in todo}) != len(todo):
         raise ValueError("initmat_many: a solver appears twice")
+    # The C call initialises the solvers group by group; should a later group
+    # fail, earlier ones already hold the new wavelength.  Forget every cached
+    # wavelength first, so that no solver is later taken to be at its old one.
+    for i in todo:
+        bems[i].enei = None
     hs = (C.c_void_p * len(todo))(*[bems[i].handle.value for i in todo])
     l = np.ascontiguousarray(lam[todo])
     e1 = (_lib.MnpComplex * len(todo))(*[_lib.MnpComplex.of(bems[i].p.eps1(lam[i])) for i in todo])
@@ -327,12 +333,13 @@ class bemretiter:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+        if h is not None and h.value and _lib.can_release():
             getattr(_lib.lib(), self._PREFIX + "_destroy")(h)
             self.handle = None
 
     def __call__(self, enei):
         if self.enei != enei:
+            self.enei = None  # stays unset if initmat fails part-way
             e1, e2 = self.p.eps1(enei), self.p.eps2(enei)
             _lib.call(self._PREFIX + "_initmat", self.handle, float(enei), _lib.MnpComplex.of(e1),
                       _lib.MnpComplex.of(e2))
@@ -403,12 +410,13 @@ class bemstat:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+        if h is not None and h.value and _lib.can_release():
             _lib.lib().mnp_bemstat_destroy(h)
             self.handle = None
 
     def __call__(self, enei):
         if self.enei != enei:
+            self.enei = None  # stays unset if initmat fails part-way
             _lib.call("mnp_bemstat_initmat", self.handle, float(enei), _lib.MnpComplex.of(self.p.eps1(enei)),
                       _lib.MnpComplex.of(self.p.eps2(enei)))
             self.enei = enei
@@ -532,12 +540,13 @@ class bemretmirror:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+        if h is not None and h.value and _lib.can_release():
             _lib.lib().mnp_bemretmirror_destroy(h)
             self.handle = None
 
     def __call__(self, enei):
         if self.enei != enei:
+            self.enei = None  # stays unset if initmat fails part-way
             e1, e2 = self.p.eps1(enei), self.p.eps2(enei)
             _lib.call("mnp_bemretmirror_initmat", self.handle, float(enei), _lib.MnpComplex.of(e1),
                       _lib.MnpComplex.of(e2))
@@ -599,12 +608,13 @@ class bemstatmirror:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+        if h is not None and h.value and _lib.can_release():
             _lib.lib().mnp_bemstatmirror_destroy(h)
             self.handle = None
 
     def __call__(self, enei):
         if self.enei != enei:
+            self.enei = None  # stays unset if initmat fails part-way
             e1, e2 = self.p.eps1(enei), self.p.eps2(enei)
             _lib.call("mnp_bemstatmirror_initmat", self.handle, float(enei), _lib.MnpComplex.of(e1),
                       _lib.MnpComplex.of(e2))

@@ -639,9 +639,11 @@ void HMat::fill(double2 k, int key_) {
     fill_impl<float>(k, key_);
 }
 
+// ACA scratch shared by all H-matrices; never destroyed (no hipFree from a
+// static destructor while the process exits).
 static Workspace& aca_ws() {
-  static Workspace ws;
-  return ws;
+  static Workspace* ws = new Workspace;
+  return *ws;
 }
 
 template <class TS>

@@ -635,9 +635,13 @@ __global__ void __launch_bounds__(256) k_cgemm_sbf16(int64_t M, int64_t N, int64
 
 // Split planes of the operands of the current product (reused by the next;
 // launches on one stream are ordered).
+// Per thread (concurrent tasks run their products on their own streams).
+// Never destroyed: a static or thread-local destructor would call hipFree
+// while the process exits, after the HIP runtime or a profiler attached to it
+// may already be torn down; the device memory goes with the process.
 static Workspace& split_ws() {
-  static thread_local Workspace ws;  // per thread: concurrent tasks run their products on their own streams
-  return ws;
+  static thread_local Workspace* ws = new Workspace;
+  return *ws;
 }
 
 template <int WM, int WN, int NS>
@@ -750,8 +754,8 @@ __global__ void k_skinny_sum(int64_t m, int n, int64_t ks, const cplx<T>* W, cpl
 
 template <class T>
 static Workspace& skinny_ws() {
-  static thread_local Workspace ws;
-  return ws;
+  static thread_local Workspace* ws = new Workspace;  // never destroyed, as split_ws()
+  return *ws;
 }
 
 template <class T>
@@ -2009,13 +2013,15 @@ template void convert_mat<float, double>(int64_t, int64_t, MatView<float>, MatVi
 using namespace mnp;
 
 namespace {
+// Process-lifetime scratch of the ABI's building blocks; never destroyed (no
+// hipFree from a static destructor while the process exits).
 Workspace& abi_ws() {
-  static Workspace ws;
-  return ws;
+  static Workspace* ws = new Workspace;
+  return *ws;
 }
 Workspace& abi_inv() {
-  static Workspace ws;
-  return ws;
+  static Workspace* ws = new Workspace;
+  return *ws;
 }
 // Factors handed over through the ABI: permutation and block inverses are
 // rebuilt from (LU, ipiv).

@@ -42,7 +42,7 @@ class CompGreenRet:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+        if h is not None and h.value and _lib.can_release():
             _lib.lib().mnp_greenret_destroy(h)
             self.handle = None
 

@@ -137,7 +137,7 @@ class hmatrix:
         except ImportError:  # interpreter shutdown
             return
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib._lib is not None:
+        if h is not None and h.value and _lib.can_release():
             _lib.lib().mnp_hmat_destroy(h)
             self.handle = None
 

@@ -95,6 +95,7 @@ class bemstatlayer(bemstat):
 
     def __call__(self, enei):
         if self.enei != enei:
+            self.enei = None  # stays unset if initmat fails part-way
             eu, ed = self.layer.epsilon(enei)
             _lib.call("mnp_bemstatlayer_initmat", self.handle, float(enei), _lib.MnpComplex.of(self.p.eps1(enei)),
                       _lib.MnpComplex.of(self.p.eps2(enei)), _lib.MnpComplex.of(eu), _lib.MnpComplex.of(ed))

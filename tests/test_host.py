@@ -60,3 +60,60 @@ def test_meshes_outward_and_sizes():
     pv, pf = shapes.trisphere256()
     assert pv.shape == (256, 3) and len(pf) == 2 * 256 - 4
     assert np.allclose(np.linalg.norm(pv, axis=1), 1.0)
+
+
+def test_initmat_many_forgets_wavelengths_when_the_call_fails(monkeypatch):
+    """mnp_bemret_initmat_many initialises the solvers group by group; when it
+    fails part-way, no solver may keep a cached wavelength (it could then be
+    taken to hold matrices it no longer holds).  The C call is replaced by one
+    that fails, so this runs without a GPU."""
+    import ctypes as C
+    from mnpbem_amd import _lib
+
+    class FakeP:
+        def eps1(self, enei):
+            return -2.0 + 0.3j
+
+        def eps2(self, enei):
+            return 1.0
+
+    class FakeBem:
+        def __init__(self, enei):
+            self.p, self.enei, self.handle = FakeP(), enei, C.c_void_p(0x1000)
+
+    bems = [FakeBem(500.0) for _ in range(6)]
+
+    def fail(name, *args):
+        assert name == "mnp_bemret_initmat_many"
+        raise _lib.MnpError("initmat_many failed in its second group")
+
+    monkeypatch.setattr(_lib, "call", fail)
+    with pytest.raises(_lib.MnpError):
+        B.initmat_many(bems, 650.0)
+    assert all(b.enei is None for b in bems)
+
+
+def test_solver_forgets_wavelength_when_initmat_fails(monkeypatch):
+    """bem(enei) of the dense solver: a failed initmat leaves no cached
+    wavelength behind (bemret.__call__)."""
+    import ctypes as C
+    from mnpbem_amd import _lib
+
+    bem = B.bemret.__new__(B.bemret)
+    bem.handle, bem.enei = C.c_void_p(0), 500.0
+
+    class FakeP:
+        def eps1(self, enei):
+            return -2.0 + 0.3j
+
+        def eps2(self, enei):
+            return 1.0
+    bem.p = FakeP()
+
+    def fail(name, *args):
+        raise _lib.MnpError("initmat failed")
+
+    monkeypatch.setattr(_lib, "call", fail)
+    with pytest.raises(_lib.MnpError):
+        bem(650.0)
+    assert bem.enei is None
