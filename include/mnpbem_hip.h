@@ -328,6 +328,21 @@ int mnp_planewave_cross_sections(mnp_greenret_t g, double enei, mnp_complex eps_
                                  int64_t npinf, const void* pinfty_nvec,
                                  const void* pinfty_area, const void* sig2, const void* h2,
                                  double* ext, double* sca, double* abs_);
+/* mnp_planewave_cross_sections without any device allocation: same
+ * arguments, results and FP64 arithmetic (the scattering sum reduced on the
+ * GPU in a fixed order instead of on the host).  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its directions, far fields and sums live in a file-scope device array of
+ * 3.2 MB per GPU, held until the process ends, so npol + npinf must be at
+ * most 4096 (MNP_ERR_ARG otherwise).  Calls that overlap on one GPU (several
+ * host threads or streams) race on that array; the call synchronises the
+ * library stream before it returns. */
+int mnp_planewave_cross_sections_noalloc(mnp_greenret_t g, double enei, mnp_complex eps_medium,
+                                         int64_t npol, const double* pol, const double* dir,
+                                         int64_t npinf, const void* pinfty_nvec,
+                                         const void* pinfty_area, const void* sig2, const void* h2,
+                                         double* ext, double* sca, double* abs_);
 
 /* ------------------------------------------------- mirror symmetry solver
  * Replaces bemsolver(p, op) for a comparticlemirror p (op.sym = 'x', 'y' or

@@ -122,6 +122,8 @@ PROTOTYPES = {
     "mnp_farfield": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P]),
     "mnp_planewave_cross_sections": (INT, [P, D, MnpComplex, I64, P, P, I64, P, P, P, P,
                                            P, P, P]),
+    "mnp_planewave_cross_sections_noalloc": (INT, [P, D, MnpComplex, I64, P, P, I64, P, P, P, P,
+                                                   P, P, P]),
 }
 
 

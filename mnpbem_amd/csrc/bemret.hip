@@ -21,6 +21,7 @@
 #include <cmath>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "mnp_chain.hpp"
@@ -507,6 +508,83 @@ void farfield(const GreenRet& g, double enei, mnp_complex em, int64_t ndir, cons
   MNP_HIP(hipGetLastError());
 }
 
+// ------------------------------------- cross sections without allocation
+// Scratch of mnp_planewave_cross_sections_noalloc, one copy per GPU for the
+// life of the process: the ndir = npol + npinf directions (3 x ndir
+// doubles), the far fields e and h (ndir x 3 x npol complex each) and the
+// per-polarisation results (ext, sca).  The sizes below are the limits the
+// entry point accepts.
+constexpr int64_t NA_MAX_DIRS = 4096;
+constexpr int64_t NA_DIR_BYTES = 3 * NA_MAX_DIRS * sizeof(double);
+constexpr int64_t NA_FIELD_BYTES = NA_MAX_DIRS * 3 * FF_MAXP * sizeof(double2);
+constexpr int64_t NA_RES_BYTES = 2 * FF_MAXP * sizeof(double);
+constexpr int64_t NA_SCRATCH_BYTES = NA_DIR_BYTES + 2 * NA_FIELD_BYTES + NA_RES_BYTES;
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
+
+// Device address of this GPU's copy of `scratch`: looked up with
+// hipGetSymbolAddress once per device, on the first call with that device
+// current (the array's host-side name is not a device address).
+static void* noalloc_scratch() {
+  static std::mutex mu;
+  static void* ptr[256] = {};
+  int dev = 0;
+  MNP_HIP(hipGetDevice(&dev));
+  MNP_CHECK(dev >= 0 && dev < 256, MNP_ERR_HIP, "cross_sections_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    MNP_CHECK(e == hipSuccess && p, MNP_ERR_HIP,
+              std::string("cross_sections_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    ptr[dev] = p;
+  }
+  return ptr[dev];
+}
+
+struct PolDirs {
+  double pol[FF_MAXP][3];
+};
+
+// One workgroup per polarisation q, from the far fields of k_farfield at the
+// directions [dir_0 .. dir_{npol-1} | pinfty normals]:
+//   ext_q = 4 pi / Re k Im(pol_q . e(dir_q))              (extinction.m:12-16)
+//   sca_q = sum_d area_d 0.5 Re(n_d . (e x conj h)) / (0.5 nb)  (scattering.m)
+// res[2q] = ext_q, res[2q + 1] = sca_q.
+__global__ void __launch_bounds__(256) k_xsec_reduce(int64_t p, int64_t npinf, const double2* __restrict__ ef,
+                                                     const double2* __restrict__ hf, const double* __restrict__ pn,
+                                                     const double* __restrict__ pa, PolDirs pd, double kre, double nb,
+                                                     double* res) {
+  const int64_t q = blockIdx.x;
+  const int64_t nd = p + npinf;
+  double s = 0;
+  for (int64_t d = threadIdx.x; d < npinf; d += blockDim.x) {
+    const int64_t dd = p + d;
+    double2 E[3], H[3];
+    for (int c = 0; c < 3; ++c) {
+      E[c] = ef[dd + nd * (c + 3 * q)];
+      H[c] = hf[dd + nd * (c + 3 * q)];
+      H[c].y = -H[c].y;
+    }
+    const double2 x = csub<double>(cmul<double>(E[1], H[2]), cmul<double>(E[2], H[1]));
+    const double2 y = csub<double>(cmul<double>(E[2], H[0]), cmul<double>(E[0], H[2]));
+    const double2 z = csub<double>(cmul<double>(E[0], H[1]), cmul<double>(E[1], H[0]));
+    s += pa[d] * (0.5 * (pn[d] * x.x + pn[npinf + d] * y.x + pn[2 * npinf + d] * z.x));
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double pe = 0;
+    for (int c = 0; c < 3; ++c) pe += ef[q + nd * (c + 3 * q)].y * pd.pol[q][c];
+    res[2 * q] = 4 * M_PI / kre * pe;
+    res[2 * q + 1] = red[0] / (0.5 * nb);
+  }
+}
+
 }  // namespace mnp
 
 // ============================================================== C ABI
@@ -703,6 +781,51 @@ int mnp_planewave_cross_sections(mnp_greenret_t g, double enei, mnp_complex eps_
       if (ext) ext[q] = e_val;
       if (sca) sca[q] = s_val;
       if (abs_) abs_[q] = e_val - s_val;
+    }
+  });
+}
+
+int mnp_planewave_cross_sections_noalloc(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t npol,
+                                         const double* pol, const double* dir, int64_t npinf,
+                                         const void* pinfty_nvec, const void* pinfty_area, const void* sig2,
+                                         const void* h2, double* ext, double* sca, double* abs_) {
+  return guarded([&] {
+    MNP_CHECK(g && pol && dir && pinfty_nvec && pinfty_area && sig2 && h2, MNP_ERR_ARG,
+              "cross_sections_noalloc: NULL argument");
+    MNP_CHECK(npol >= 1 && npol <= FF_MAXP, MNP_ERR_ARG, "cross_sections_noalloc: npol must be in 1..8");
+    MNP_CHECK(npinf >= 1 && npol + npinf <= NA_MAX_DIRS, MNP_ERR_ARG,
+              "cross_sections_noalloc: npol + npinf must be at most " + std::to_string(NA_MAX_DIRS) +
+                  " (the size of the library's scratch)");
+    const GreenRet& G = *g->g;
+    unsigned char* base = (unsigned char*)noalloc_scratch();
+    // every region is written in full before it is read: nothing to clear
+    double* ddir = (double*)base;
+    double2* ef = (double2*)(base + NA_DIR_BYTES);
+    double2* hf = (double2*)(base + NA_DIR_BYTES + NA_FIELD_BYTES);
+    double* res = (double*)(base + NA_DIR_BYTES + 2 * NA_FIELD_BYTES);
+    const int64_t nd = npol + npinf;
+    for (int c = 0; c < 3; ++c) {
+      MNP_HIP(hipMemcpyAsync(ddir + c * nd, dir + c * npol, sizeof(double) * npol, hipMemcpyHostToDevice,
+                             stream()));
+      MNP_HIP(hipMemcpyAsync(ddir + c * nd + npol, (const double*)pinfty_nvec + c * npinf,
+                             sizeof(double) * npinf, hipMemcpyDeviceToDevice, stream()));
+    }
+    farfield(G, enei, eps_medium, nd, ddir, npol, (const double2*)sig2, (const double2*)h2, ef, hf);
+    PolDirs pd{};
+    for (int64_t q = 0; q < npol; ++q)
+      for (int c = 0; c < 3; ++c) pd.pol[q][c] = pol[c * npol + q];
+    const double2 k = medium_k(enei, eps_medium);
+    const double nb = std::sqrt(std::hypot(eps_medium.re, eps_medium.im));  // |sqrt(eps)| (lossless medium)
+    hipLaunchKernelGGL(k_xsec_reduce, dim3((unsigned)npol), dim3(256), 0, stream(), npol, npinf, ef, hf,
+                       (const double*)pinfty_nvec, (const double*)pinfty_area, pd, k.x, nb, res);
+    MNP_HIP(hipGetLastError());
+    double h[2 * FF_MAXP];
+    MNP_HIP(hipMemcpyAsync(h, res, sizeof(double) * 2 * npol, hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipStreamSynchronize(stream()));
+    for (int64_t q = 0; q < npol; ++q) {
+      if (ext) ext[q] = h[2 * q];
+      if (sca) sca[q] = h[2 * q + 1];
+      if (abs_) abs_[q] = h[2 * q] - h[2 * q + 1];
     }
   });
 }
