@@ -17,7 +17,8 @@ preconditioner of the exact FP64 operator, refined to a relative residual of
 1e-12).  The FP64 drop-in is timed on `--fp64-steps` wavelengths for the
 speed-up and the cross-section error of the mixed path is reported against
 it.  `cpu_baseline` times the NumPy/OpenBLAS FP64 oracle on the host for one
-wavelength of the same shape at 5,120 faces and scales it by N^3 to 20,480.
+wavelength of the same shape at 5,120 faces, its assembly, chain and solve
+apart, and scales the chain by N^3 and the rest by N^2 to 20,480 faces.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -60,8 +61,12 @@ def run_gpu(level, steps, warmup, fp64_steps, split_bf16=True):
     p.green  # geometry, refinement tables, sum rule (once per mesh)
     torch.cuda.synchronize()
     log(f"[bench] n={n} refined pairs={p.green.nnz} setup {time.time() - t0:.2f}s")
-    nlam = warmup + steps
-    enei = np.linspace(500.0, 1000.0, 64)[:max(nlam, fp64_steps)]
+    # the timed wavelengths spread over the 64 of the spectrum; the FP64
+    # drop-in runs on fp64_steps of them, spread likewise
+    spec = np.linspace(500.0, 1000.0, 64)
+    lam_t = spec[np.unique(np.round(np.linspace(0, 63, steps)).astype(int))]
+    lam_w = spec[1:1 + warmup]
+    lam64 = lam_t[np.unique(np.round(np.linspace(0, len(lam_t) - 1, min(fp64_steps, len(lam_t)))).astype(int))]
 
     def spectrum(bem, lams, timed):
         out = []
@@ -83,9 +88,10 @@ def run_gpu(level, steps, warmup, fp64_steps, split_bf16=True):
 
     mixed = B.bemsolver(p, op, precision="mixed")
     if warmup:
-        spectrum(mixed, enei[:warmup], False)
+        spectrum(mixed, lam_w, False)
         log(f"[bench] mixed warmup done ({warmup})")
-    res_mx, t_mx, tm = spectrum(mixed, enei[warmup:warmup + steps], True)
+    mixed.enei = None  # every timed step initialises its wavelength
+    res_mx, t_mx, tm = spectrum(mixed, lam_t, True)
     sweeps = mixed.stats()
     log(f"[bench] mixed {steps} wavelengths in {t_mx:.2f}s, refinement {sweeps}")
     del mixed
@@ -95,30 +101,31 @@ def run_gpu(level, steps, warmup, fp64_steps, split_bf16=True):
     if split_bf16:
         sb = B.bemsolver(p, op, precision="mixed_bf16")
         if warmup:
-            spectrum(sb, enei[:warmup], False)
-        res_bf, t_bf, tm_bf = spectrum(sb, enei[warmup:warmup + steps], True)
+            spectrum(sb, lam_w, False)
+        sb.enei = None
+        res_bf, t_bf, tm_bf = spectrum(sb, lam_t, True)
         sweeps_bf = sb.stats()
         log(f"[bench] mixed_bf16 {steps} wavelengths in {t_bf:.2f}s, refinement {sweeps_bf}")
         del sb
         torch.cuda.empty_cache()
 
-    lam64 = enei[warmup:warmup + fp64_steps]
     res_64, t_64, tm64 = [], 0.0, None
     if fp64_steps:
         fp64 = B.bemsolver(p, op, precision="fp64")
         res_64, t_64, tm64 = spectrum(fp64, lam64, True)
         log(f"[bench] fp64 {len(lam64)} wavelengths in {t_64:.2f}s")
         del fp64
-    errs = []
-    for a, b in zip(res_mx, res_64):
-        for q in range(2):
-            errs.append(abs(a[q] - b[q]) / np.abs(b[q]))
-    max_err = float(np.max(errs)) if errs else None
-    errs_bf = [abs(a[q] - b[q]) / np.abs(b[q]) for a, b in zip(res_bf, res_64) for q in range(2)]
+    where = {float(l): i for i, l in enumerate(lam_t)}
+
+    def max_err(res):
+        # extinction and scattering, both polarisations, at the FP64 wavelengths
+        e = [np.abs(res[where[float(l)]][c] - b[c]) / np.abs(b[c]) for l, b in zip(lam64, res_64) for c in (0, 1)]
+        return float(np.max(e)) if e else None
     return dict(n=n, t_mx=t_mx, t_64=t_64, n64=len(lam64), timers=tm, timers64=tm64,
-                max_err=max_err, sweeps=sweeps, ext=[list(map(float, r[0])) for r in res_mx],
+                max_err=max_err(res_mx), sweeps=sweeps, ext=[list(map(float, r[0])) for r in res_mx],
                 t_bf=t_bf, timers_bf=tm_bf, sweeps_bf=sweeps_bf,
-                max_err_bf=float(np.max(errs_bf)) if errs_bf else None)
+                max_err_bf=max_err(res_bf) if res_bf else None,
+                lam_t=[float(x) for x in lam_t], lam64=[float(x) for x in lam64])
 
 
 def run_quasistatic(level=5, steps=3):
@@ -296,15 +303,20 @@ def run_substrate(level=5, steps=3):
 
 
 def run_sensitivity(nsweep=32):
-    """Extinction and its Drude-parameter gradient (one tangent solve with the
-    forward factors per wavelength) on the 5,120-face ellipsoid 10 x 10 x 40 nm
-    in eps = 1.77, pol x (dir z) and z (dir x): 650 nm, checked against central
-    differences of the FP64 solver, and a sweep of linspace(500, 1000, 32)
-    timed with the mixed and FP64 solvers."""
+    """Config 5: extinction and its gradient with respect to all vertex
+    coordinates and the Drude parameters (eps_inf, wp, gamma) by the adjoint
+    (one transposed solve with the forward factors per wavelength), on the
+    5,120-face ellipsoid 10 x 10 x 40 nm in eps = 1.77, pol x (dir z) and z
+    (dir x), of the summed extinction of both polarisations.  At 650 nm the
+    vertex gradient is checked against central differences of the FP64
+    solver (step 1e-4 nm per vertex, refined pairs held fixed) along seeded
+    directions and the Drude gradient against central differences (relative
+    step 1e-6); the sweep linspace(500, 1000, 32) is timed with the mixed and
+    FP64 solvers."""
     import torch
 
     from mnpbem_amd import bem as B, shapes
-    from mnpbem_amd.sensitivity import drude_extinction_gradient
+    from mnpbem_amd.sensitivity import drude_derivatives, extinction_gradient
 
     v, f = shapes.ellipsoid(4, (10.0, 10.0, 40.0))
 
@@ -313,9 +325,17 @@ def run_sensitivity(nsweep=32):
         p = B.comparticle([B.epsconst(EPS_OUT), B.epsdrude(*DRUDE)], [(v, f)], [2, 1], 1, op)
         return p, B.bemsolver(p, op, precision=prec), B.planewave(POL, DIR, op)
 
+    def gradient(p, bem, exc, lam):
+        sig = bem.solve(exc(p, lam))
+        ext = float(np.sum(exc.ext(sig)))
+        dv, de = extinction_gradient(exc, bem, sig)
+        dd = drude_derivatives(p.eps[1], lam)
+        return ext, dv, de[0] * dd.real + de[1] * dd.imag
+
     p, b64, exc = make("fp64")
-    ext, grad = drude_extinction_gradient(exc, b64, 650.0)
-    fd = np.zeros_like(grad)
+    ext, dv, dpar = gradient(p, b64, exc, 650.0)
+    # Drude parameters: central differences, relative step 1e-6
+    fdp = np.zeros(3)
     for j in range(3):
         h = 1e-6 * DRUDE[j]
         vals = []
@@ -324,28 +344,49 @@ def run_sensitivity(nsweep=32):
             par[j] += sgn * h
             p.eps[1] = B.epsdrude(*par)
             b64.enei = None
-            vals.append(np.asarray(exc.cross_sections(b64.solve(exc(p, 650.0)))[0]))
-        fd[:, j] = (vals[0] - vals[1]) / (2 * h)
+            vals.append(float(np.sum(exc.ext(b64.solve(exc(p, 650.0))))))
+        fdp[j] = (vals[0] - vals[1]) / (2 * h)
     p.eps[1] = B.epsdrude(*DRUDE)
-    err_fd = float(np.max(np.abs(grad - fd) / np.abs(fd).max(axis=0, keepdims=True)))
+    b64.enei = None
+    err_p = float(np.max(np.abs(dpar - fdp) / np.abs(fdp)))
+    # vertices: central differences along seeded directions, step 1e-4 nm per vertex
+    rng = np.random.default_rng(2025)
+    an, fd = [], []
+    for _ in range(4):
+        u = rng.normal(size=v.shape)
+        u *= np.sqrt(len(v)) / np.linalg.norm(u)
+        vals = []
+        for sgn in (1, -1):
+            q = p.moved(v + sgn * 1e-4 * u)
+            vals.append(float(np.sum(exc.ext(B.bemsolver(q, q.op, precision="fp64").solve(exc(q, 650.0))))))
+        fd.append((vals[0] - vals[1]) / 2e-4)
+        an.append(float(np.sum(dv * u)))
+    an, fd = np.array(an), np.array(fd)
+    err_v = float(np.max(np.abs(an - fd)) / np.max(np.abs(an)))
     lams = np.linspace(500.0, 1000.0, nsweep)
-    res = {}
+    res, grads = {}, {}
     for prec, nl in (("mixed", nsweep), ("fp64", 4)):
         p, bem, exc = make(prec)
-        drude_extinction_gradient(exc, bem, lams[0])  # warm-up
+        gradient(p, bem, exc, lams[-1])  # warm-up
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for lam in lams[:nl]:
-            drude_extinction_gradient(exc, bem, lam)
+        grads[prec] = [gradient(p, bem, exc, lam) for lam in lams[:nl]]
         torch.cuda.synchronize()
         res[prec] = nl / (time.perf_counter() - t)
         del bem
-    log(f"[bench] sensitivity: gradient vs FD rel err {err_fd:.2e}; mixed {res['mixed']:.2f} λ/s, "
-        f"fp64 {res['fp64']:.2f} λ/s (extinction + Drude gradient)")
-    return {"workload": f"d(ext)/d(eps_inf, wp, gamma), ellipsoid 10x10x40 nm ({len(f)} faces) in eps=1.77, "
-                        "650 nm FD check and linspace(500,1000,32) sweep",
+    err_mx = max(float(np.max(np.abs(a[1] - b[1])) / np.max(np.abs(b[1])))
+                 for a, b in zip(grads["mixed"], grads["fp64"]))
+    log(f"[bench] adjoint: vertex gradient vs FD rel err {err_v:.2e}, Drude vs FD {err_p:.2e}, mixed vs FP64 "
+        f"{err_mx:.2e}; mixed {res['mixed']:.2f} λ/s, fp64 {res['fp64']:.2f} λ/s (extinction + full gradient)")
+    return {"workload": f"d(sum of ext)/d(vertex coordinates, eps_inf, wp, gamma) by the adjoint, ellipsoid "
+                        f"10x10x40 nm ({len(f)} faces, {len(v)} vertices) in eps=1.77, pols x(dir z) and z(dir x), "
+                        "650 nm FD checks and linspace(500,1000,32) sweep",
             "wavelengths_per_s_mixed": res["mixed"], "wavelengths_per_s_fp64": res["fp64"],
-            "max_rel_err_vs_central_fd_fp64": err_fd, "gradient_650nm": grad.tolist(), "ext_650nm": ext.tolist()}
+            "vertex_gradient_max_rel_err_vs_central_fd_fp64": err_v,
+            "drude_gradient_max_rel_err_vs_central_fd_fp64": err_p,
+            "mixed_vs_fp64_vertex_gradient_max_rel_diff": err_mx,
+            "ext_650nm": ext, "drude_gradient_650nm": dpar.tolist(),
+            "vertex_gradient_norm_650nm": float(np.linalg.norm(dv))}
 
 
 def run_multi_particle(per_particle=8):
@@ -440,13 +481,15 @@ def run_hmatrix_dipole(nlam=16, seed=2024):
     _lib.call("mnp_timers_enable", 1)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    tot, rad, iters = [], [], []
+    tot, rad, iters, flags = [], [], [], []
     for lam in lams:
         sig = it.solve(exc(p, lam))
         a, b, _ = exc.decayrate(sig)
         tot.append(float(a[0, 0]))
         rad.append(float(b[0, 0]))
-        iters.append(it.stats()["iter"])
+        st = it.stats()
+        iters.append(st["iter"])
+        flags.append(st["flag"])
     torch.cuda.synchronize()
     el = time.perf_counter() - t
     tm = _lib.Timers()
@@ -457,12 +500,15 @@ def run_hmatrix_dipole(nlam=16, seed=2024):
     mv_bytes = tm.work[hv] / max(tm.launches[hv], 1)
     gbs = mv_bytes / (mv_ms * 1e-3) / 1e9 if mv_ms > 0 else 0.0
     log(f"[bench] H-matrix dipole: {nlam} wavelengths in {el:.2f}s (setup {setup:.2f}s), iterations {iters}")
+    if any(flags):
+        raise RuntimeError(f"H-matrix dipole: GMRES did not converge at wavelengths "
+                           f"{[float(l) for l, fl in zip(lams, flags) if fl]}")
     return {
         "workload": "sphere d=150 nm, icosphere level 6 (81920 faces), Drude gold in eps=1, dipole 5 nm "
                     "from the surface (seeded), bemretiter cleaf 32, htol 1e-6, GMRES tol 1e-6, FP32 "
                     "H-matrix storage, 16 wavelengths 500-700 nm, total and radiative decay rates",
         "wavelengths_per_s": nlam / el, "ms_per_wavelength": 1e3 * el / nlam, "setup_s": setup,
-        "gmres_iterations": iters, "compression": it.stats()["compression"],
+        "gmres_iterations": iters, "gmres_flags": flags, "compression": it.stats()["compression"],
         "tot": tot, "rad": rad,
         "hmatvec": {"bound": "hbm", "achieved": gbs, "peak": 8000.0, "unit": "GB/s", "frac": gbs / 8000.0,
                     "avg_launch_ms": mv_ms, "bytes_per_launch": mv_bytes,
@@ -542,18 +588,31 @@ def cpu_baseline(level=4):
     _, _, _, pn, pa = O.norm_flat(pv, pf)
     pol, dirs = np.array(POL), np.array(DIR)
     lam = 650.0
+    # the oracle's initmat is assembly + chain; time the assembly alone first
+    k0 = 2 * np.pi / lam
+    t = time.perf_counter()
+    g.eval(k0 * np.sqrt(complex(O.eps_drude(lam, *DRUDE))), ("G", "H1"))
+    g.eval(k0 * np.sqrt(EPS_OUT), ("G", "H2"))
+    t_asm = time.perf_counter() - t
     t = time.perf_counter()
     bem = O.BemRet(g).initmat(lam, O.eps_drude(lam, *DRUDE), EPS_OUT)
+    t_init = time.perf_counter() - t
+    t = time.perf_counter()
     a2, a2p = O.planewave_potential(g.pos, g.nvec, pol, dirs, lam, EPS_OUT)
     sig = bem.solve({"a2": a2, "a2p": a2p})
     O.extinction(g.pos, g.area, sig, pol, dirs, lam, EPS_OUT)
     O.scattering(g.pos, g.area, sig, lam, EPS_OUT, pn, pa)
-    el = time.perf_counter() - t
-    scale = (20480 / len(f)) ** 3
-    return {"value": 1.0 / (el * scale), "unit": "wavelengths/s", "cores": int(cores),
+    t_sol = time.perf_counter() - t
+    t_chain = max(t_init - t_asm, 0.0)
+    r = 20480 / len(f)
+    # O(N^2) parts (assembly, solve, cross sections) scale by r^2, the chain by r^3
+    est = (t_asm + t_sol) * r ** 2 + t_chain * r ** 3
+    return {"value": 1.0 / est, "unit": "wavelengths/s", "cores": int(cores),
             "kind": "port",
-            "sample": f"one wavelength (650 nm, 2 pols) of the same ellipsoid at {len(f)} faces "
-                      f"took {el:.2f} s; scaled by (20480/{len(f)})^3 = {scale:.0f}"}
+            "sample": f"one wavelength (650 nm, 2 pols) of the same ellipsoid at {len(f)} faces: assembly "
+                      f"{t_asm:.2f} s, chain (4 inverses + 2 products) {t_chain:.2f} s, solve + cross "
+                      f"sections {t_sol:.2f} s; scaled to 20480 faces by {r:.0f}^2 (assembly, solve) and "
+                      f"{r:.0f}^3 (chain): {est:.0f} s per wavelength"}
 
 
 def main():
@@ -561,7 +620,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=4)
     ap.add_argument("--level", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-hmatrix", action="store_true", help="skip the H-matrix dipole measurement")
@@ -645,6 +704,8 @@ def main():
         "fp64_wavelengths_per_s": r["n64"] / r["t_64"] if r["n64"] else None,
         "speedup_vs_fp64": (a.steps / r["t_mx"]) / (r["n64"] / r["t_64"]) if r["n64"] else None,
         "max_rel_err_vs_fp64": r["max_err"],
+        "fp64_wavelengths": r["lam64"],
+        "timed_wavelengths": r["lam_t"],
         "refinement": {"sweeps": r["sweeps"][0], "relres": r["sweeps"][1]},
         "chain_effective_tflops": chain_flops / step_s / 1e12,
         "gemm_share_of_step": tm.ms[gi] * 1e-3 / r["t_mx"],
@@ -691,8 +752,11 @@ def main():
     if not a.no_multi_particle:
         out["multi_particle"] = run_multi_particle()
     if not a.no_hmatrix:
-        out["hmatrix_dipole"] = run_hmatrix_dipole()
-        if not a.no_cpu_baseline:
+        try:
+            out["hmatrix_dipole"] = run_hmatrix_dipole()
+        except RuntimeError as e:  # a wavelength that did not converge voids the measurement
+            out["hmatrix_dipole"] = {"error": str(e)}
+        if not a.no_cpu_baseline and "error" not in out["hmatrix_dipole"]:
             it = out["hmatrix_dipole"]["gmres_iterations"]
             out["hmatrix_dipole"]["cpu_baseline"] = cpu_baseline_hmatrix(int(round(sum(it) / len(it))),
                                                                          out["hmatrix_dipole"]["compression"])

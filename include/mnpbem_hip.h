@@ -126,6 +126,16 @@ void mnp_green_options_default(mnp_green_options* op);
  * normal (v1-v2)x(v3-v2) pointing out of the particle (norm_flat.m:36-38). */
 int mnp_greenret_create(const double* verts, int64_t nverts, const int64_t* faces,
                         int64_t nfaces, const mnp_green_options* op, mnp_greenret_t* out);
+/* As mnp_greenret_create, with the refined pairs (refinematrix.m's
+ * selection) of `like`, a Green function of the same faces, instead of a
+ * selection for this mesh.  No counterpart in the reference; for moving
+ * meshes whose elements should stay continuous in the vertices (finite
+ * differences, shape optimisation): the selection is piecewise constant in
+ * the vertices and otherwise makes G and H jump where a pair crosses the
+ * cutoff. */
+int mnp_greenret_create_like(const double* verts, int64_t nverts, const int64_t* faces,
+                             int64_t nfaces, const mnp_green_options* op, mnp_greenret_t like,
+                             mnp_greenret_t* out);
 int mnp_greenret_destroy(mnp_greenret_t g);
 /* Number of faces and of refined (self + near-field) matrix elements. */
 int mnp_greenret_size(mnp_greenret_t g, int64_t* nfaces, int64_t* nrefined);
@@ -189,6 +199,35 @@ int mnp_bemret_initmat_many(int count, const mnp_bemret_t* bems, const double* e
 int mnp_bemret_tangent_eps1(mnp_bemret_t bem, int64_t npol, const void* sig1, const void* sig2,
                             const void* h1, const void* h2, void* dsig1, void* dsig2, void* dh1,
                             void* dh2);
+
+/* Transposed solve with the forward factors (no counterpart in the
+ * reference; the adjoint of mldivide.m:11-49 for the operator of
+ * bemretiter/private/afun.m:6-35): lam = A^{-T} c with the plain transpose.
+ * c (device, complex128, n x 8 npol) is laid out as the solution
+ * [sig1 | h1 | sig2 | h2] (columns q, then c + 3 q for vectors, as
+ * mnp_bemret_solve's outputs stacked); lam (device, n x 8 npol) as the
+ * excitation [phi | a | De | alpha] of excitation.m:21-29.  FP64 solvers
+ * apply the transposed chain (factors of A^T formed from the forward LU
+ * factors, the inverses of Sigma1 and Deltai transposed); MIXED refines the
+ * FP32 transposed chain with the exact FP64 A^T to the refinement tolerance
+ * (mnp_bemret_stats reports it).  Valid until the next initmat. */
+int mnp_bemret_solve_transposed(mnp_bemret_t bem, int64_t npol, const void* c, void* lam);
+/* Gradient of the weighted plane-wave extinction sum_q w_q sigma_ext,q
+ * (@planewaveret/extinction.m:12-16 with the far field of
+ * @spectrumret/farfield.m:59-84) with respect to the vertex positions and the
+ * inside dielectric function, by one transposed solve (no counterpart in the
+ * reference).  sig1..h2: the solution of mnp_bemret_solve at the current
+ * wavelength for the plane wave pol, dir (host npol x 3, npol <= 8, pol
+ * perpendicular to dir) from outside, a2/a2p of mnp_planewave_potential;
+ * weights: host npol (NULL: all 1).  dverts: host nverts x 3 (the mesh of
+ * mnp_greenret_create), d/d vertex coordinates, including the derivatives of
+ * the refined near-field and self integrals (greenret/private/init.m:42-177)
+ * and of the closed-surface sum rule (compgreenret/private/initclosed.m:
+ * 19-89); the refinement selection is held fixed.  deps1: host 2,
+ * (d/d Re eps1, d/d Im eps1).  Either output may be NULL. */
+int mnp_bemret_extinction_gradient(mnp_bemret_t bem, int64_t npol, const double* pol, const double* dir,
+                                   const double* weights, const void* sig1, const void* sig2,
+                                   const void* h1, const void* h2, double* dverts, double* deps1);
 
 /* ------------------------------------------------ dense building blocks
  * Replace the LAPACK/BLAS calls behind Matlab's inv and mtimes in

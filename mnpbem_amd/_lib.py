@@ -52,6 +52,7 @@ PROTOTYPES = {
     "mnp_timers_read": (INT, [C.POINTER(Timers)]),
     "mnp_green_options_default": (None, [C.POINTER(GreenOptions)]),
     "mnp_greenret_create": (INT, [P, I64, P, I64, C.POINTER(GreenOptions), C.POINTER(P)]),
+    "mnp_greenret_create_like": (INT, [P, I64, P, I64, C.POINTER(GreenOptions), P, C.POINTER(P)]),
     "mnp_greenret_destroy": (INT, [P]),
     "mnp_greenret_size": (INT, [P, PI64, PI64]),
     "mnp_greenret_geometry": (INT, [P, P, P, P, P, P]),
@@ -62,6 +63,8 @@ PROTOTYPES = {
     "mnp_bemret_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
     "mnp_bemret_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mnp_bemret_tangent_eps1": (INT, [P, I64, P, P, P, P, P, P, P, P]),
+    "mnp_bemret_solve_transposed": (INT, [P, I64, P, P]),
+    "mnp_bemret_extinction_gradient": (INT, [P, I64, P, P, P, P, P, P, P, P, P]),
     "mnp_bemret_initmat_many": (INT, [INT, C.POINTER(P), PD, C.POINTER(MnpComplex), C.POINTER(MnpComplex)]),
     "mnp_bemret_stats": (INT, [P, C.POINTER(INT), PD]),
     "mnp_bemret_set_refinement": (INT, [P, D, INT]),

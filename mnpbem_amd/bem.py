@@ -160,12 +160,23 @@ class comparticle:
     def green(self):
         """The particle's compgreenret(p, p, op), built once per mesh."""
         if self._green is None:
-            op = self.op
-            self._green = CompGreenRet(
-                self.verts, self.faces, order=op.get("order", 5),
-                rel_cutoff=op.get("RelCutoff", 0.0), abs_cutoff=op.get("AbsCutoff", 0.0),
-                npol=tuple(np.broadcast_to(op.get("npol", (7, 5)), (2,))), closed=self.closed)
+            self._green = self._make_green()
         return self._green
+
+    def _make_green(self, like=None):
+        op = self.op
+        return CompGreenRet(
+            self.verts, self.faces, order=op.get("order", 5),
+            rel_cutoff=op.get("RelCutoff", 0.0), abs_cutoff=op.get("AbsCutoff", 0.0),
+            npol=tuple(np.broadcast_to(op.get("npol", (7, 5)), (2,))), closed=self.closed, like=like)
+
+    def moved(self, verts):
+        """The same particle with vertices ``verts`` and the refined pairs of
+        this one (no new refinement selection: the Green function's elements
+        stay continuous in the vertices)."""
+        q = comparticle(self.eps, [(verts, self.faces)], self.inout, self.closed, self.op)
+        q._green = q._make_green(like=self.green)
+        return q
 
     def geometry(self):
         return self.green.geometry()

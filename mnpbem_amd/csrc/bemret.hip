@@ -24,6 +24,7 @@
 #include <mutex>
 #include <vector>
 
+#include "mnp_bemret.hpp"
 #include "mnp_chain.hpp"
 #include "mnp_linalg.hpp"
 
@@ -207,209 +208,204 @@ void launch_unpack(int64_t n, int64_t p, const double2* x, double2* sig1, double
   MNP_HIP(hipGetLastError());
 }
 
-struct BemRet {
-  const GreenRet* g;
-  int solver;
-  int64_t n;
-  bool ready = false;
-  double enei = 0;
-  double2 eps1{}, eps2{};
-  std::unique_ptr<Chain<double>> c64;
-  std::unique_ptr<Chain<float>> c32;
-  double2 *G1 = nullptr, *G2 = nullptr, *H1 = nullptr, *H2 = nullptr;  // FP64 operator (mixed)
-  Workspace vws;  // vectors
-  double tol = 1e-12;
-  int maxit = 10;
-  int last_iter = 0;
-  double last_relres = 0;
+void launch_pack(int64_t n, int64_t p, const double2* sig1, const double2* h1, const double2* sig2,
+                 const double2* h2, double2* x) {
+  hipLaunchKernelGGL(k_pack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n, p,
+                     sig1, h1, sig2, h2, x);
+  MNP_HIP(hipGetLastError());
+}
+void launch_deps1(int64_t n, int64_t p, const double2* GX, const double2* HX, const double2* dGX,
+                  const double2* dHX, double2* r, const double* nvec, double k0, double2 e1) {
+  hipLaunchKernelGGL(k_deps1, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0, stream(), n, p, GX, HX,
+                     dGX, dHX, r, nvec, k0, e1);
+  MNP_HIP(hipGetLastError());
+}
 
-  BemRet(const GreenRet* g_, int solver_) : g(g_), solver(solver_), n(g_->n) {
-    if (solver == MNP_SOLVER_FP64) {
-      c64.reset(new Chain<double>(n, true));
-    } else {
-      c32.reset(new Chain<float>(n, false));
-      G1 = dev_alloc<double2>(n * n);
-      G2 = dev_alloc<double2>(n * n);
-      H1 = dev_alloc<double2>(n * n);
-      H2 = dev_alloc<double2>(n * n);
-    }
+BemRet::BemRet(const GreenRet* g_, int solver_) : g(g_), solver(solver_), n(g_->n) {
+  if (solver == MNP_SOLVER_FP64) {
+    c64.reset(new Chain<double>(n, true));
+  } else {
+    c32.reset(new Chain<float>(n, false));
+    G1 = dev_alloc<double2>(n * n);
+    G2 = dev_alloc<double2>(n * n);
+    H1 = dev_alloc<double2>(n * n);
+    H2 = dev_alloc<double2>(n * n);
   }
-  ~BemRet() {
-    void* ps[] = {G1, G2, H1, H2};
-    for (void* p : ps)
-      if (p) (void)hipFree(p);
-  }
+}
+BemRet::~BemRet() {
+  adjoint_free(adj);
+  void* ps[] = {G1, G2, H1, H2};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+}
 
-  void initmat(double enei_, double2 e1, double2 e2) {
-    if (!initmat_try(enei_, e1, e2)) initmat_finish();
-  }
-  // Assembly and the factorisations that need no partial pivoting (safe on a
-  // worker stream); false when initmat_finish must complete the chain.
-  bool initmat_try(double enei_, double2 e1, double2 e2) {
-    ready = false;
-    enei = enei_;
-    eps1 = e1;
-    eps2 = e2;
-    const double k0 = 2 * M_PI / enei;
-    const double2 k1 = wavenumber(k0, e1), k2 = wavenumber(k0, e2);
-    bool ok;
-    if (solver == MNP_SOLVER_FP64) {
-      Chain<double>& c = *c64;
-      c.eps1 = e1;
-      c.eps2 = e2;
-      c.k0 = k0;
-      g->assemble(k1, 2 * M_PI, OutPair{c.G1, nullptr}, OutPair{c.S1, nullptr}, n);
-      g->assemble(k2, -2 * M_PI, OutPair{c.G2, nullptr}, OutPair{c.S2, nullptr}, n);
-      ok = chain_build_try<double>(c, *g);
-    } else {
-      Chain<float>& c = *c32;
-      c.eps1 = e1;
-      c.eps2 = e2;
-      c.k0 = k0;
-      g->assemble(k1, 2 * M_PI, OutPair{G1, c.G1}, OutPair{H1, c.S1}, n);
-      g->assemble(k2, -2 * M_PI, OutPair{G2, c.G2}, OutPair{H2, c.S2}, n);
-      SplitBf16Scope split(solver == MNP_SOLVER_MIXED_BF16);
-      ok = chain_build_try<float>(c, *g);
-    }
-    ready = ok;
-    return ok;
-  }
-  void initmat_finish() {
-    if (solver == MNP_SOLVER_FP64) {
-      chain_build_finish<double>(*c64, *g);
-    } else {
-      SplitBf16Scope split(solver == MNP_SOLVER_MIXED_BF16);
-      chain_build_finish<float>(*c32, *g);
-    }
-    ready = true;
-  }
+void BemRet::initmat(double enei_, double2 e1, double2 e2) {
+  if (!initmat_try(enei_, e1, e2)) initmat_finish();
+}
 
-  double norm2(const double2* v, int64_t len) { return dev_norm2(len, v, vws, 5); }
-
-  void solve(int64_t p, const double2* const in[8], double2* sig1, double2* sig2, double2* h1,
-             double2* h2) {
-    MNP_CHECK(ready, MNP_ERR_STATE, "bemret_solve: initmat not called");
-    MNP_CHECK(p > 0 && p <= 64, MNP_ERR_ARG, "bemret_solve: npol must be in 1..64");
-    const int64_t len = 8 * p * n;
-    const double k0 = 2 * M_PI / enei;
-    double2* b = (double2*)vws.get(0, sizeof(double2) * len);
-    double2* x = (double2*)vws.get(1, sizeof(double2) * len);
-    hipLaunchKernelGGL(k_excitation, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0,
-                       stream(), n, p, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], b,
-                       g->geo.nvec, k0, eps1, eps2);
-    MNP_HIP(hipGetLastError());
-    solve_b(p, b, x);
-    hipLaunchKernelGGL(k_unpack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0,
-                       stream(), n, p, x, sig1, h1, sig2, h2);
-    MNP_HIP(hipGetLastError());
+bool BemRet::initmat_try(double enei_, double2 e1, double2 e2) {
+  ready = false;
+  ++epoch;
+  enei = enei_;
+  eps1 = e1;
+  eps2 = e2;
+  const double k0 = 2 * M_PI / enei;
+  const double2 k1 = wavenumber(k0, e1), k2 = wavenumber(k0, e2);
+  bool ok;
+  if (solver == MNP_SOLVER_FP64) {
+    Chain<double>& c = *c64;
+    c.eps1 = e1;
+    c.eps2 = e2;
+    c.k0 = k0;
+    g->assemble(k1, 2 * M_PI, OutPair{c.G1, nullptr}, OutPair{c.S1, nullptr}, n);
+    g->assemble(k2, -2 * M_PI, OutPair{c.G2, nullptr}, OutPair{c.S2, nullptr}, n);
+    ok = chain_build_try<double>(c, *g);
+  } else {
+    Chain<float>& c = *c32;
+    c.eps1 = e1;
+    c.eps2 = e2;
+    c.k0 = k0;
+    g->assemble(k1, 2 * M_PI, OutPair{G1, c.G1}, OutPair{H1, c.S1}, n);
+    g->assemble(k2, -2 * M_PI, OutPair{G2, c.G2}, OutPair{H2, c.S2}, n);
+    SplitBf16Scope split(solver == MNP_SOLVER_MIXED_BF16);
+    ok = chain_build_try<float>(c, *g);
   }
-
-  // x = A^{-1} b for b, x n x 8p (layouts of chain_apply); b is not modified
-  void solve_b(int64_t p, const double2* b, double2* x) {
-    const int64_t len = 8 * p * n;
-    const double k0 = 2 * M_PI / enei;
-    const size_t tmp_elems = (size_t)(18 * p * n);
-    last_iter = 0;
-    last_relres = 0;
-    if (solver == MNP_SOLVER_FP64) {
-      double2* tmp = (double2*)vws.get(2, sizeof(double2) * tmp_elems);
-      chain_apply<double>(*c64, *g, p, b, x, tmp);
-    } else {
-      Chain<float>& c = *c32;
-      float2* b32 = (float2*)vws.get(2, sizeof(float2) * len);
-      float2* x32 = (float2*)vws.get(3, sizeof(float2) * len);
-      float2* tmp = (float2*)vws.get(4, sizeof(float2) * tmp_elems);
-      // residual buffers: 4 products (n x 4p) + r (n x 8p), FP64
-      double2* work = (double2*)res_ws.get(0, sizeof(double2) * (size_t)(24 * p * n));
-      double2 *GX1 = work, *GX2 = work + 4 * p * n, *HX1 = work + 8 * p * n, *HX2 = work + 12 * p * n,
-              *rr = work + 16 * p * n;
-      auto precond = [&](const double2* src, double2* dst) {
-        convert_mat<double, float>(n, 8 * p, MatView<double>{const_cast<double2*>(src), n},
-                                   MatView<float>{b32, n});
-        chain_apply<float>(c, *g, p, b32, x32, tmp);
-        convert_mat<float, double>(n, 8 * p, MatView<float>{x32, n}, MatView<double>{dst, n});
-      };
-      precond(b, x);
-      const double bn = norm2(b, len);
-      const double2 one = {1, 0}, zero = {0, 0};
-      for (int it = 0;; ++it) {
-        gemm<double>(n, 4 * p, n, one, MatView<double>{G1, n}, MatView<double>{x, n}, zero,
-                     MatView<double>{GX1, n});
-        gemm<double>(n, 4 * p, n, one, MatView<double>{G2, n}, MatView<double>{x + 4 * p * n, n}, zero,
-                     MatView<double>{GX2, n});
-        gemm<double>(n, 4 * p, n, one, MatView<double>{H1, n}, MatView<double>{x, n}, zero,
-                     MatView<double>{HX1, n});
-        gemm<double>(n, 4 * p, n, one, MatView<double>{H2, n}, MatView<double>{x + 4 * p * n, n}, zero,
-                     MatView<double>{HX2, n});
-        hipLaunchKernelGGL(k_residual, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0,
-                           stream(), n, p, b, GX1, GX2, HX1, HX2, rr, g->geo.nvec, k0, eps1, eps2);
-        MNP_HIP(hipGetLastError());
-        const double rn = norm2(rr, len);
-        last_relres = bn > 0 ? rn / bn : 0;
-        last_iter = it;
-        if (!(last_relres > tol) || it >= maxit) break;
-        precond(rr, GX1);  // correction into GX1 (n x 8p fits in GX1..GX2)
-        hipLaunchKernelGGL(k_axpy, dim3((unsigned)cdiv_up(len, 256)), dim3(256), 0, stream(), len, x,
-                           GX1);
-        MNP_HIP(hipGetLastError());
-      }
-      MNP_CHECK(!(last_relres > tol), MNP_ERR_NOCONV,
-                "bemret_solve: refinement stalled at relative residual " + sci(last_relres));
-    }
+  ready = ok;
+  return ok;
+}
+void BemRet::initmat_finish() {
+  if (solver == MNP_SOLVER_FP64) {
+    chain_build_finish<double>(*c64, *g);
+  } else {
+    SplitBf16Scope split(solver == MNP_SOLVER_MIXED_BF16);
+    chain_build_finish<float>(*c32, *g);
   }
+  ready = true;
+}
 
-  // Derivative of the solution x = [sig1 | h1 | sig2 | h2] with respect to
-  // the inside dielectric function eps1 at the current wavelength (complex
-  // derivative; A(eps1) x = b with b from the outside excitation):
-  //   A dx = -(dA/deps1) x,
-  // solved with the factors of the forward solve.  eps1 enters A through
-  // k1 = k0 sqrt(eps1) in G1, H1 (dk1/deps1 = k1 / (2 eps1)) and explicitly
-  // in the alpha and De rows.
-  void tangent_eps1(int64_t p, const double2* sig1, const double2* sig2, const double2* h1, const double2* h2,
-                    double2* dsig1, double2* dsig2, double2* dh1, double2* dh2) {
-    MNP_CHECK(ready, MNP_ERR_STATE, "bemret_tangent: initmat not called");
-    MNP_CHECK(p > 0 && p <= 64, MNP_ERR_ARG, "bemret_tangent: npol must be in 1..64");
-    const int64_t len = 8 * p * n, nn = n * n;
-    const double k0 = 2 * M_PI / enei;
-    const double2 k1 = wavenumber(k0, eps1);
-    const double2 dk = cdiv<double>(k1, cscale<double>(eps1, 2.0));  // dk1 / deps1
-    double2* x = (double2*)tws.get(0, sizeof(double2) * len);
-    double2* rhs = (double2*)tws.get(1, sizeof(double2) * len);
-    double2* dx = (double2*)tws.get(2, sizeof(double2) * len);
-    double2* prod = (double2*)tws.get(3, sizeof(double2) * 16 * p * n);  // G1 x1, H1 x1, dG1 x1, dH1 x1
-    double2* dG = (double2*)tws.get(4, sizeof(double2) * nn);
-    double2* dH = (double2*)tws.get(5, sizeof(double2) * nn);
-    hipLaunchKernelGGL(k_pack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n, p,
-                       sig1, h1, sig2, h2, x);
-    MNP_HIP(hipGetLastError());
-    // G1, H1 themselves: kept by the mixed solver, re-assembled for the FP64 one
-    const double2 *G1m = G1, *H1m = H1;
-    if (!G1m) {
-      double2* tg = (double2*)tws.get(6, sizeof(double2) * nn);
-      double2* th = (double2*)tws.get(7, sizeof(double2) * nn);
-      g->assemble(k1, 2 * M_PI, OutPair{tg, nullptr}, OutPair{th, nullptr}, n);
-      G1m = tg;
-      H1m = th;
-    }
-    g->assemble_dk(k1, OutPair{dG, nullptr}, OutPair{dH, nullptr}, n);
+double BemRet::norm2(const double2* v, int64_t len) { return dev_norm2(len, v, vws, 5); }
+
+void BemRet::solve(int64_t p, const double2* const in[8], double2* sig1, double2* sig2, double2* h1,
+                   double2* h2) {
+  MNP_CHECK(ready, MNP_ERR_STATE, "bemret_solve: initmat not called");
+  MNP_CHECK(p > 0 && p <= 64, MNP_ERR_ARG, "bemret_solve: npol must be in 1..64");
+  const int64_t len = 8 * p * n;
+  const double k0 = 2 * M_PI / enei;
+  double2* b = (double2*)vws.get(0, sizeof(double2) * len);
+  double2* x = (double2*)vws.get(1, sizeof(double2) * len);
+  hipLaunchKernelGGL(k_excitation, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0,
+                     stream(), n, p, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], b,
+                     g->geo.nvec, k0, eps1, eps2);
+  MNP_HIP(hipGetLastError());
+  solve_b(p, b, x);
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0,
+                     stream(), n, p, x, sig1, h1, sig2, h2);
+  MNP_HIP(hipGetLastError());
+}
+
+// x = A^{-1} b for b, x n x 8p (layouts of chain_apply); b is not modified
+void BemRet::solve_b(int64_t p, const double2* b, double2* x) {
+  const int64_t len = 8 * p * n;
+  const double k0 = 2 * M_PI / enei;
+  const size_t tmp_elems = (size_t)(18 * p * n);
+  last_iter = 0;
+  last_relres = 0;
+  if (solver == MNP_SOLVER_FP64) {
+    double2* tmp = (double2*)vws.get(2, sizeof(double2) * tmp_elems);
+    chain_apply<double>(*c64, *g, p, b, x, tmp);
+  } else {
+    Chain<float>& c = *c32;
+    float2* b32 = (float2*)vws.get(2, sizeof(float2) * len);
+    float2* x32 = (float2*)vws.get(3, sizeof(float2) * len);
+    float2* tmp = (float2*)vws.get(4, sizeof(float2) * tmp_elems);
+    // residual buffers: 4 products (n x 4p) + r (n x 8p), FP64
+    double2* work = (double2*)res_ws.get(0, sizeof(double2) * (size_t)(24 * p * n));
+    double2 *GX1 = work, *GX2 = work + 4 * p * n, *HX1 = work + 8 * p * n, *HX2 = work + 12 * p * n,
+            *rr = work + 16 * p * n;
+    auto precond = [&](const double2* src, double2* dst) {
+      convert_mat<double, float>(n, 8 * p, MatView<double>{const_cast<double2*>(src), n},
+                                 MatView<float>{b32, n});
+      chain_apply<float>(c, *g, p, b32, x32, tmp);
+      convert_mat<float, double>(n, 8 * p, MatView<float>{x32, n}, MatView<double>{dst, n});
+    };
+    precond(b, x);
+    const double bn = norm2(b, len);
     const double2 one = {1, 0}, zero = {0, 0};
-    const MatView<double> X1{x, n};
-    double2 *GX = prod, *HX = prod + 4 * p * n, *dGX = prod + 8 * p * n, *dHX = prod + 12 * p * n;
-    gemm<double>(n, 4 * p, n, one, MatView<double>{const_cast<double2*>(G1m), n}, X1, zero, MatView<double>{GX, n});
-    gemm<double>(n, 4 * p, n, one, MatView<double>{const_cast<double2*>(H1m), n}, X1, zero, MatView<double>{HX, n});
-    gemm<double>(n, 4 * p, n, dk, MatView<double>{dG, n}, X1, zero, MatView<double>{dGX, n});
-    gemm<double>(n, 4 * p, n, dk, MatView<double>{dH, n}, X1, zero, MatView<double>{dHX, n});
-    hipLaunchKernelGGL(k_deps1, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0, stream(), n, p, GX, HX,
-                       dGX, dHX, rhs, g->geo.nvec, k0, eps1);
-    MNP_HIP(hipGetLastError());
-    solve_b(p, rhs, dx);
-    hipLaunchKernelGGL(k_unpack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n, p,
-                       dx, dsig1, dh1, dsig2, dh2);
-    MNP_HIP(hipGetLastError());
+    for (int it = 0;; ++it) {
+      gemm<double>(n, 4 * p, n, one, MatView<double>{G1, n}, MatView<double>{x, n}, zero,
+                   MatView<double>{GX1, n});
+      gemm<double>(n, 4 * p, n, one, MatView<double>{G2, n}, MatView<double>{x + 4 * p * n, n}, zero,
+                   MatView<double>{GX2, n});
+      gemm<double>(n, 4 * p, n, one, MatView<double>{H1, n}, MatView<double>{x, n}, zero,
+                   MatView<double>{HX1, n});
+      gemm<double>(n, 4 * p, n, one, MatView<double>{H2, n}, MatView<double>{x + 4 * p * n, n}, zero,
+                   MatView<double>{HX2, n});
+      hipLaunchKernelGGL(k_residual, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0,
+                         stream(), n, p, b, GX1, GX2, HX1, HX2, rr, g->geo.nvec, k0, eps1, eps2);
+      MNP_HIP(hipGetLastError());
+      const double rn = norm2(rr, len);
+      last_relres = bn > 0 ? rn / bn : 0;
+      last_iter = it;
+      if (!(last_relres > tol) || it >= maxit) break;
+      precond(rr, GX1);  // correction into GX1 (n x 8p fits in GX1..GX2)
+      hipLaunchKernelGGL(k_axpy, dim3((unsigned)cdiv_up(len, 256)), dim3(256), 0, stream(), len, x,
+                         GX1);
+      MNP_HIP(hipGetLastError());
+    }
+    MNP_CHECK(!(last_relres > tol), MNP_ERR_NOCONV,
+              "bemret_solve: refinement stalled at relative residual " + sci(last_relres));
   }
-  Workspace tws;
-  Workspace res_ws;
-};
+}
+
+// Derivative of the solution x = [sig1 | h1 | sig2 | h2] with respect to
+// the inside dielectric function eps1 at the current wavelength (complex
+// derivative; A(eps1) x = b with b from the outside excitation):
+//   A dx = -(dA/deps1) x,
+// solved with the factors of the forward solve.  eps1 enters A through
+// k1 = k0 sqrt(eps1) in G1, H1 (dk1/deps1 = k1 / (2 eps1)) and explicitly
+// in the alpha and De rows.
+void BemRet::tangent_eps1(int64_t p, const double2* sig1, const double2* sig2, const double2* h1,
+                          const double2* h2, double2* dsig1, double2* dsig2, double2* dh1, double2* dh2) {
+  MNP_CHECK(ready, MNP_ERR_STATE, "bemret_tangent: initmat not called");
+  MNP_CHECK(p > 0 && p <= 64, MNP_ERR_ARG, "bemret_tangent: npol must be in 1..64");
+  const int64_t len = 8 * p * n, nn = n * n;
+  const double k0 = 2 * M_PI / enei;
+  const double2 k1 = wavenumber(k0, eps1);
+  const double2 dk = cdiv<double>(k1, cscale<double>(eps1, 2.0));  // dk1 / deps1
+  double2* x = (double2*)tws.get(0, sizeof(double2) * len);
+  double2* rhs = (double2*)tws.get(1, sizeof(double2) * len);
+  double2* dx = (double2*)tws.get(2, sizeof(double2) * len);
+  double2* prod = (double2*)tws.get(3, sizeof(double2) * 16 * p * n);  // G1 x1, H1 x1, dG1 x1, dH1 x1
+  double2* dG = (double2*)tws.get(4, sizeof(double2) * nn);
+  double2* dH = (double2*)tws.get(5, sizeof(double2) * nn);
+  hipLaunchKernelGGL(k_pack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n, p,
+                     sig1, h1, sig2, h2, x);
+  MNP_HIP(hipGetLastError());
+  // G1, H1 themselves: kept by the mixed solver, re-assembled for the FP64 one
+  const double2 *G1m = G1, *H1m = H1;
+  if (!G1m) {
+    double2* tg = (double2*)tws.get(6, sizeof(double2) * nn);
+    double2* th = (double2*)tws.get(7, sizeof(double2) * nn);
+    g->assemble(k1, 2 * M_PI, OutPair{tg, nullptr}, OutPair{th, nullptr}, n);
+    G1m = tg;
+    H1m = th;
+  }
+  g->assemble_dk(k1, OutPair{dG, nullptr}, OutPair{dH, nullptr}, n);
+  const double2 one = {1, 0}, zero = {0, 0};
+  const MatView<double> X1{x, n};
+  double2 *GX = prod, *HX = prod + 4 * p * n, *dGX = prod + 8 * p * n, *dHX = prod + 12 * p * n;
+  gemm<double>(n, 4 * p, n, one, MatView<double>{const_cast<double2*>(G1m), n}, X1, zero, MatView<double>{GX, n});
+  gemm<double>(n, 4 * p, n, one, MatView<double>{const_cast<double2*>(H1m), n}, X1, zero, MatView<double>{HX, n});
+  gemm<double>(n, 4 * p, n, dk, MatView<double>{dG, n}, X1, zero, MatView<double>{dGX, n});
+  gemm<double>(n, 4 * p, n, dk, MatView<double>{dH, n}, X1, zero, MatView<double>{dHX, n});
+  hipLaunchKernelGGL(k_deps1, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0, stream(), n, p, GX, HX,
+                     dGX, dHX, rhs, g->geo.nvec, k0, eps1);
+  MNP_HIP(hipGetLastError());
+  solve_b(p, rhs, dx);
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n, p,
+                     dx, dsig1, dh1, dsig2, dh2);
+  MNP_HIP(hipGetLastError());
+}
 
 // ---------------------------------------------------- excitation / output
 // a2 = e^{i k pos.dir}/(i k0) pol ; a2p = (i k n.dir) a2  (planewaveret/potential.m:72-76)
@@ -590,9 +586,6 @@ __global__ void __launch_bounds__(256) k_xsec_reduce(int64_t p, int64_t npinf, c
 // ============================================================== C ABI
 using namespace mnp;
 
-struct mnp_bemret_s {
-  BemRet* b;
-};
 
 extern "C" {
 

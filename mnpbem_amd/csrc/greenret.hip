@@ -361,23 +361,6 @@ __global__ void k_geometry(int64_t n, const double* __restrict__ verts, int64_t 
   geo.rad[i] = rad;
 }
 
-// Refinement predicate of Greenfun/+green/refinematrix.m:51-63 with the
-// distance of Misc/+misc/pdist2.m (|a|^2+|b|^2-2a.b, d^2 < 1e-10 -> 0):
-// returns 2 for coincident centroids, 1 for a near-field pair, else 0.
-__device__ __forceinline__ int refine_kind(const Geo& geo, int64_t n, int64_t i, int64_t j,
-                                           double rel, double absc) {
-#pragma clang fp contract(off)
-  const double xi = geo.pos[i], yi = geo.pos[n + i], zi = geo.pos[2 * n + i];
-  const double xj = geo.pos[j], yj = geo.pos[n + j], zj = geo.pos[2 * n + j];
-  double d2 = (xi * xi + yi * yi + zi * zi) + (xj * xj + yj * yj + zj * zj) -
-              2 * (xi * xj + yi * yj + zi * zj);
-  if (d2 < 1e-10) d2 = 0;
-  const double d = sqrt(d2);
-  if (d == 0) return 2;
-  const double dd = d - geo.rad[j];
-  return (dd < absc || dd / geo.rad[i] < rel) ? 1 : 0;
-}
-
 // One workgroup per source column j: number of refined rows.
 __global__ void __launch_bounds__(256) k_refine_count(int64_t n, Geo geo, double rel,
                                                       double absc, int64_t* count) {
@@ -746,7 +729,7 @@ void GreenRet::assemble_rows(double2 k, double sgn2pi, OutPair g, OutPair h, int
 }
 
 GreenRet::GreenRet(const double* verts, int64_t nv, const int64_t* faces, int64_t nf,
-                   const mnp_green_options& op_)
+                   const mnp_green_options& op_, const GreenRet* like)
     : n(nf), op(op_) {
   MNP_CHECK(nf > 0 && nv >= 3, MNP_ERR_ARG, "greenret: empty mesh");
   MNP_CHECK(op.order >= 0 && op.order <= MNP_MAX_ORDER, MNP_ERR_ARG, "greenret: order out of range");
@@ -763,7 +746,13 @@ GreenRet::GreenRet(const double* verts, int64_t nv, const int64_t* faces, int64_
     MNP_CHECK(npolq > 0, MNP_ERR_ARG, "greenret: polar rule too large");
     MNP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_q18), q18, sizeof(q18)));
     MNP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_qpol), qp.data(), sizeof(double) * 3 * MNP_MAX_POLAR));
+    this->q18 = dev_alloc<double>(3 * 28);
+    this->qpol = dev_alloc<double>(3 * MNP_MAX_POLAR);
+    MNP_HIP(hipMemcpy(this->q18, q18, sizeof(q18), hipMemcpyHostToDevice));
+    MNP_HIP(hipMemcpy(this->qpol, qp.data(), sizeof(double) * 3 * MNP_MAX_POLAR, hipMemcpyHostToDevice));
   }
+  nverts = nv;
+  faces_h.assign(faces, faces + 3 * nf);
   // device copies of the mesh (column-major n x 3, as passed)
   double* dverts = dev_alloc<double>(nv * 3);
   int64_t* dfaces = dev_alloc<int64_t>(nf * 3);
@@ -781,23 +770,38 @@ GreenRet::GreenRet(const double* verts, int64_t nv, const int64_t* faces, int64_
   hipLaunchKernelGGL(k_geometry, dim3((unsigned)cdiv_up(n, 256)), dim3(256), 0, s, n, dverts, nv,
                      dfaces, geo);
   MNP_HIP(hipGetLastError());
-  // refinement selection
+  // refinement selection (or the pairs of `like`)
   int64_t* cnt = dev_alloc<int64_t>(n);
   off = dev_alloc<int64_t>(n + 1);
-  hipLaunchKernelGGL(k_refine_count, dim3((unsigned)n), dim3(256), 0, s, n, geo, op.rel_cutoff,
-                     op.abs_cutoff, cnt);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, n, cnt, off);
-  MNP_HIP(hipGetLastError());
-  MNP_HIP(hipMemcpyAsync(&nnz, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  MNP_HIP(hipStreamSynchronize(s));
-  pairs.row = dev_alloc<int32_t>(nnz);
-  pairs.col = dev_alloc<int32_t>(nnz);
-  pairs.kind = dev_alloc<int8_t>(nnz);
-  pairs.diag = dev_alloc<int64_t>(n);
-  MNP_HIP(hipMemsetAsync(pairs.diag, 0xff, sizeof(int64_t) * n, s));
-  hipLaunchKernelGGL(k_refine_fill, dim3((unsigned)n), dim3(256), 0, s, n, geo, op.rel_cutoff,
-                     op.abs_cutoff, off, pairs);
-  MNP_HIP(hipGetLastError());
+  if (like) {
+    MNP_CHECK(like->n == n && like->faces_h == faces_h, MNP_ERR_ARG,
+              "greenret: the template Green function has other faces");
+    nnz = like->nnz;
+    pairs.row = dev_alloc<int32_t>(nnz);
+    pairs.col = dev_alloc<int32_t>(nnz);
+    pairs.kind = dev_alloc<int8_t>(nnz);
+    pairs.diag = dev_alloc<int64_t>(n);
+    MNP_HIP(hipMemcpyAsync(off, like->off, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToDevice, s));
+    MNP_HIP(hipMemcpyAsync(pairs.row, like->pairs.row, sizeof(int32_t) * nnz, hipMemcpyDeviceToDevice, s));
+    MNP_HIP(hipMemcpyAsync(pairs.col, like->pairs.col, sizeof(int32_t) * nnz, hipMemcpyDeviceToDevice, s));
+    MNP_HIP(hipMemcpyAsync(pairs.kind, like->pairs.kind, sizeof(int8_t) * nnz, hipMemcpyDeviceToDevice, s));
+    MNP_HIP(hipMemcpyAsync(pairs.diag, like->pairs.diag, sizeof(int64_t) * n, hipMemcpyDeviceToDevice, s));
+  } else {
+    hipLaunchKernelGGL(k_refine_count, dim3((unsigned)n), dim3(256), 0, s, n, geo, op.rel_cutoff,
+                       op.abs_cutoff, cnt);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, n, cnt, off);
+    MNP_HIP(hipGetLastError());
+    MNP_HIP(hipMemcpyAsync(&nnz, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    MNP_HIP(hipStreamSynchronize(s));
+    pairs.row = dev_alloc<int32_t>(nnz);
+    pairs.col = dev_alloc<int32_t>(nnz);
+    pairs.kind = dev_alloc<int8_t>(nnz);
+    pairs.diag = dev_alloc<int64_t>(n);
+    MNP_HIP(hipMemsetAsync(pairs.diag, 0xff, sizeof(int64_t) * n, s));
+    hipLaunchKernelGGL(k_refine_fill, dim3((unsigned)n), dim3(256), 0, s, n, geo, op.rel_cutoff,
+                       op.abs_cutoff, off, pairs);
+    MNP_HIP(hipGetLastError());
+  }
   // refinement tables
   gtab = dev_alloc<double>(nnz * (order + 1));
   ftab = dev_alloc<double>(nnz * 3 * (order + 1));
@@ -831,7 +835,7 @@ GreenRet::GreenRet(const double* verts, int64_t nv, const int64_t* faces, int64_
 
 GreenRet::~GreenRet() {
   void* ptrs[] = {geo.pos, geo.nvec, geo.t1, geo.t2, geo.vtx, geo.area, geo.qarea, geo.rad,
-                  off, pairs.row, pairs.col, pairs.kind, pairs.diag, gtab, ftab};
+                  off, pairs.row, pairs.col, pairs.kind, pairs.diag, gtab, ftab, q18, qpol};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -893,6 +897,26 @@ int mnp_greenret_create(const double* verts, int64_t nverts, const int64_t* face
     auto* h = new mnp_greenret_s{nullptr};
     try {
       h->g = new GreenRet(verts, nverts, faces, nfaces, o);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int mnp_greenret_create_like(const double* verts, int64_t nverts, const int64_t* faces, int64_t nfaces,
+                             const mnp_green_options* op, mnp_greenret_t like, mnp_greenret_t* out) {
+  return guarded([&] {
+    MNP_CHECK(verts && faces && out && like, MNP_ERR_ARG, "greenret_create_like: NULL argument");
+    mnp_green_options o;
+    if (op)
+      o = *op;
+    else
+      mnp_green_options_default(&o);
+    auto* h = new mnp_greenret_s{nullptr};
+    try {
+      h->g = new GreenRet(verts, nverts, faces, nfaces, o, like->g);
     } catch (...) {
       delete h;
       throw;

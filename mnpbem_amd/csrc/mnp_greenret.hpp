@@ -49,6 +49,23 @@ struct OutPair {
   }
 };
 
+// Refinement predicate of Greenfun/+green/refinematrix.m:51-63 with the
+// distance of Misc/+misc/pdist2.m (|a|^2+|b|^2-2a.b, d^2 < 1e-10 -> 0):
+// returns 2 for coincident centroids, 1 for a near-field pair, else 0.
+__device__ __forceinline__ int refine_kind(const Geo& geo, int64_t n, int64_t i, int64_t j,
+                                           double rel, double absc) {
+#pragma clang fp contract(off)
+  const double xi = geo.pos[i], yi = geo.pos[n + i], zi = geo.pos[2 * n + i];
+  const double xj = geo.pos[j], yj = geo.pos[n + j], zj = geo.pos[2 * n + j];
+  double d2 = (xi * xi + yi * yi + zi * zi) + (xj * xj + yj * yj + zj * zj) -
+              2 * (xi * xj + yi * yj + zi * zj);
+  if (d2 < 1e-10) d2 = 0;
+  const double d = sqrt(d2);
+  if (d == 0) return 2;
+  const double dd = d - geo.rad[j];
+  return (dd < absc || dd / geo.rad[i] < rel) ? 1 : 0;
+}
+
 struct GreenRet {
   int64_t n = 0;
   int64_t nnz = 0;
@@ -60,9 +77,17 @@ struct GreenRet {
   int64_t* off = nullptr;  // column offsets into the pair list (n + 1)
   double* gtab = nullptr;  // nnz x (order + 1)
   double* ftab = nullptr;  // nnz x 3 x (order + 1)
+  // mesh connectivity and the quadrature rules in global memory (the
+  // __constant__ copies are private to greenret.hip's code object)
+  int64_t nverts = 0;
+  std::vector<int64_t> faces_h;  // host n x 3, column-major, 0-based
+  double* q18 = nullptr;         // device [x(28) | y(28) | w(28)]
+  double* qpol = nullptr;        // device [x | y | w], MNP_MAX_POLAR each
 
+  // like: take the refined pairs of that Green function (same faces) instead
+  // of selecting them for this mesh
   GreenRet(const double* verts, int64_t nv, const int64_t* faces, int64_t nf,
-           const mnp_green_options& op);
+           const mnp_green_options& op, const GreenRet* like = nullptr);
   ~GreenRet();
   GreenRet(const GreenRet&) = delete;
   GreenRet& operator=(const GreenRet&) = delete;

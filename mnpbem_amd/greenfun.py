@@ -22,7 +22,9 @@ class CompGreenRet:
     """
 
     def __init__(self, verts, faces, order=5, rel_cutoff=3.0, abs_cutoff=0.0, npol=(7, 5),
-                 closed=True):
+                 closed=True, like=None):
+        """``like``: a CompGreenRet of the same faces whose refined pairs are
+        taken instead of selecting them for this mesh (mnp_greenret_create_like)."""
         dev.ensure()
         verts = np.asfortranarray(np.asarray(verts, dtype=np.float64))
         faces = np.asfortranarray(np.asarray(faces, dtype=np.int64))
@@ -31,8 +33,12 @@ class CompGreenRet:
         op = _lib.GreenOptions(int(order), float(rel_cutoff), float(abs_cutoff), int(npol[0]),
                                int(npol[1]), int(bool(closed)))
         h = C.c_void_p()
-        _lib.call("mnp_greenret_create", verts.ctypes.data_as(C.c_void_p), verts.shape[0],
-                  faces.ctypes.data_as(C.c_void_p), faces.shape[0], C.byref(op), C.byref(h))
+        if like is None:
+            _lib.call("mnp_greenret_create", verts.ctypes.data_as(C.c_void_p), verts.shape[0],
+                      faces.ctypes.data_as(C.c_void_p), faces.shape[0], C.byref(op), C.byref(h))
+        else:
+            _lib.call("mnp_greenret_create_like", verts.ctypes.data_as(C.c_void_p), verts.shape[0],
+                      faces.ctypes.data_as(C.c_void_p), faces.shape[0], C.byref(op), like.handle, C.byref(h))
         self.handle = h
         self.order = int(order)
         n, nnz = C.c_int64(), C.c_int64()

@@ -225,23 +225,37 @@ def sphere_stat_decayrate(enei, eps_in, eps_out, radius, dist, lmax=200):
 
 def mie_decayrate(enei, eps_in, eps_out, diameter, z, lmax=20):
     """Total and radiative decay rates (len(z), 2) for dipoles on the z axis
-    at distances z from the sphere centre, orientations x and z (mieret/decayrate.m)."""
+    at distances z from the sphere centre, orientations x and z (mieret/decayrate.m).
+
+    The series of decayrate.m is summed in ratio form: with
+    A = (j1/h1) a_l and B = (j1/h1) b_l (a_l, b_l built from logarithmic
+    derivatives), the products A h(y)^2, B h(y)^2 and A h(y) are formed as
+    (j1 h(y)/h1) h(y) and so on, which stay in floating-point range where j1
+    underflows and h1, h(y) grow large.  The values are those of the
+    reference's expressions; the form only lets lmax go beyond the
+    reference's 20 (mieret/private/init.m:23-24), which a dipole a few nm
+    from a large sphere needs (the terms fall off like (R/z)^(2l))."""
     k = 2 * np.pi / enei * np.sqrt(complex(eps_out))
     kin = 2 * np.pi / enei * np.sqrt(complex(eps_in))
     epsb, epsin = complex(eps_out), complex(eps_in)
     l = np.arange(1, lmax + 1)
-    j1, h1, zjp1, zhp1 = _riccati(0.5 * k * diameter, lmax)
-    j2, _, zjp2, _ = _riccati(0.5 * kin * diameter, lmax)
-    A = (j1 * zjp2 - j2 * zjp1) / (j2 * zhp1 - h1 * zjp2)
-    B = (epsb * j1 * zjp2 - epsin * j2 * zjp1) / (epsin * j2 * zhp1 - epsb * h1 * zjp2)
+    x, xin = 0.5 * k * diameter, 0.5 * kin * diameter
+    j1, h1, zjp1, zhp1 = _riccati(x, lmax)
+    j2, _, zjp2, _ = _riccati(xin, lmax)
+    rho2, lj1, lh1 = zjp2 / j2, zjp1 / j1, zhp1 / h1
+    a = (rho2 - lj1) / (lh1 - rho2)                            # A = (j1 / h1) a
+    b = (epsb * rho2 - epsin * lj1) / (epsin * lh1 - epsb * rho2)  # B = (j1 / h1) b
     z = np.atleast_1d(z)
     tot = np.zeros((len(z), 2))
     rad = np.zeros((len(z), 2))
     for i, zz in enumerate(z):
         y = k * zz
         j, h, zjp, zhp = _riccati(y, lmax)
-        tot[i, 0] = 1 + 1.5 * np.real(np.sum((l + 0.5) * (B * (zhp / y) ** 2 + A * h ** 2)))
-        tot[i, 1] = 1 + 1.5 * np.real(np.sum((2 * l + 1) * l * (l + 1) * B * (h / y) ** 2))
-        rad[i, 0] = 0.75 * np.sum((2 * l + 1) * (np.abs(j + A * h) ** 2 + np.abs((zjp + B * zhp) / y) ** 2))
-        rad[i, 1] = 1.5 * np.sum((2 * l + 1) * l * (l + 1) * np.abs((j + B * h) / y) ** 2)
+        s, sp = j1 * (h / h1), j1 * (zhp / h1)                 # (j1/h1) h, (j1/h1) zhp
+        tot[i, 0] = 1 + 1.5 * np.real(np.sum((l + 0.5) * (b * sp * zhp / y ** 2 + a * s * h)))
+        tot[i, 1] = 1 + 1.5 * np.real(np.sum((2 * l + 1) * l * (l + 1) * b * s * h / y ** 2))
+        rad[i, 0] = 0.75 * np.sum((2 * l + 1) * (np.abs(j + a * s) ** 2 + np.abs((zjp + b * sp) / y) ** 2))
+        rad[i, 1] = 1.5 * np.sum((2 * l + 1) * l * (l + 1) * np.abs((j + b * s) / y) ** 2)
+    if not (np.all(np.isfinite(tot)) and np.all(np.isfinite(rad))):
+        raise OverflowError(f"mie_decayrate: lmax {lmax} beyond the range of the Bessel functions")
     return tot, rad

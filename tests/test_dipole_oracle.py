@@ -197,3 +197,35 @@ def test_stat_decayrate_converges_to_sphere_series():
         errs.append(max(np.max(np.abs(tot / wt - 1)), np.max(np.abs(rad / wr - 1))))
         assert np.allclose(rad0, np.sqrt(eps_b) * 4 / 3 * (2 * np.pi / enei) ** 3)
     assert errs[0] < 0.08 and errs[1] < 0.025 and errs[1] < errs[0] / 3, errs
+
+
+def test_mie_decayrate_ratio_form_and_convergence():
+    """mie_decayrate sums decayrate.m's series in ratio form.  Where the
+    reference's direct products are finite (lmax 20) the two forms agree to
+    rounding; for a dipole 5 nm from a 150 nm sphere the series needs about
+    140 terms, and there the ratio form stays finite and converges."""
+    from oracle.mie import _riccati
+    for lam, d, z in ((500.0, 150.0, 80.0), (620.0, 20.0, 13.0), (700.0, 60.0, 45.0)):
+        e_in, e_out = O.eps_drude(lam, 10.0, 9.065, 0.0708), 1.0
+        k = 2 * np.pi / lam
+        kin = k * np.sqrt(complex(e_in))
+        l = np.arange(1, 21)
+        j1, h1, zjp1, zhp1 = _riccati(0.5 * k * d, 20)
+        j2, _, zjp2, _ = _riccati(0.5 * kin * d, 20)
+        A = (j1 * zjp2 - j2 * zjp1) / (j2 * zhp1 - h1 * zjp2)
+        B = (e_out * j1 * zjp2 - e_in * j2 * zjp1) / (e_in * j2 * zhp1 - e_out * h1 * zjp2)
+        y = k * z
+        j, h, zjp, zhp = _riccati(y, 20)
+        tx = 1 + 1.5 * np.real(np.sum((l + 0.5) * (B * (zhp / y) ** 2 + A * h ** 2)))
+        tz = 1 + 1.5 * np.real(np.sum((2 * l + 1) * l * (l + 1) * B * (h / y) ** 2))
+        rx = 0.75 * np.sum((2 * l + 1) * (np.abs(j + A * h) ** 2 + np.abs((zjp + B * zhp) / y) ** 2))
+        rz = 1.5 * np.sum((2 * l + 1) * l * (l + 1) * np.abs((j + B * h) / y) ** 2)
+        tot, rad = D.mie_decayrate(lam, e_in, e_out, d, [z])
+        assert np.allclose(tot[0], [tx, tz], rtol=1e-12) and np.allclose(rad[0], [rx, rz], rtol=1e-12)
+    lam = 500.0
+    e_in = O.eps_drude(lam, 10.0, 9.065, 0.0708)
+    t120, r120 = D.mie_decayrate(lam, e_in, 1.0, 150.0, [80.0], lmax=120)
+    t140, r140 = D.mie_decayrate(lam, e_in, 1.0, 150.0, [80.0], lmax=140)
+    t20, _ = D.mie_decayrate(lam, e_in, 1.0, 150.0, [80.0])
+    assert np.all(np.abs(t140 / t120 - 1) < 1e-4) and np.allclose(r140, r120, rtol=1e-12)
+    assert np.all(t140 > 1.6 * t20)  # twenty terms miss most of the near-field part
