@@ -290,6 +290,15 @@ int mnp_hmat_stats(mnp_hmat_t h, int64_t* full_entries, int64_t* lowrank_entries
                    double* compression);
 /* Ranks of the low-rank blocks (host, nlow entries) of the last fill. */
 int mnp_hmat_ranks(mnp_hmat_t h, int32_t* ranks);
+/* Coarse operator P^T H P of the last fill over the row tiles of the
+ * product plan (each leaf cluster cut into near-equal runs of at most 32
+ * consecutive cluster indices): *ntile (host) their number; tilebeg (host,
+ * ntile + 1, may be NULL) their first cluster indices; C (device, may be
+ * NULL) ntile x ntile complex128, column-major, entry (s, t) the sum of H
+ * over the rows of tile s and the columns of tile t, summed from the stored
+ * full blocks and low-rank factors.  No counterpart in the reference: the
+ * coarse level of the iterative solver's preconditioner. */
+int mnp_hmat_coarsen(mnp_hmat_t h, int64_t* ntile, int64_t* tilebeg, void* C);
 
 /* ------------------------------------------ iterative solver (bemretiter)
  * Replaces bemsolver(p, op) with op.iter set -> bemretiter (BEM/@bemretiter):
@@ -297,17 +306,23 @@ int mnp_hmat_ranks(mnp_hmat_t h, int32_t* ranks);
  * the preconditioner; solve runs GMRES without restart on the exact
  * operator afun (private/afun.m:6-35), preconditioned by mfun
  * (bemiter/solve.m:33-36, private/mfun.m) from the right in the flexible
- * form, and stops when |b - A x| <= tol |b|.  The preconditioner is the
- * bemret chain applied exactly on diagonal blocks of at most 32 faces
- * (consecutive cluster indices inside each leaf cluster), in place of the
- * reference's rank-4 H-matrix chain (private/initprecond.m).  Precision and
- * block arguments as for mnp_hmat_create. */
+ * form, and stops when |b - A x| <= tol |b|.  In place of the reference's
+ * rank-4 H-matrix chain (private/initprecond.m), the preconditioner couples
+ * the surface on two levels: the bemret chain on the H-matrices aggregated
+ * over tiles of at most 32 faces, then the bemret chain applied exactly on
+ * diagonal blocks of at most 32 faces (see mnp_bemretiter_set_options).
+ * Precision and block arguments as for mnp_hmat_create. */
 typedef struct mnp_bemretiter_s* mnp_bemretiter_t;
 int mnp_bemretiter_create(mnp_greenret_t g, int precision, const int64_t* perm, int64_t nfull,
                           const int64_t* full, int64_t nlow, const int64_t* low, double htol,
                           int kmax, mnp_bemretiter_t* out);
 int mnp_bemretiter_destroy(mnp_bemretiter_t it);
-/* GMRES tolerance and iteration limit (bemiter.options: 1e-6, 100); precond 0 disables it. */
+/* GMRES tolerance and iteration limit (bemiter.options: 1e-6, 100) and the
+ * preconditioner: 0 none, 1 the block chain alone, 2 (default) two levels:
+ * the bemret chain on the H-matrices aggregated over their row tiles (P^T G1
+ * P, P^T G2 P, P^T H1 P, P^T H2 P, tiles' mean normals; FP32 MFMA chain)
+ * followed by the block chain on what remains, z = P y + B^{-1}(v - A P y),
+ * y = C^{-1} P^T v. */
 int mnp_bemretiter_set_options(mnp_bemretiter_t it, double tol, int maxit, int precond);
 int mnp_bemretiter_initmat(mnp_bemretiter_t it, double enei, mnp_complex eps1, mnp_complex eps2);
 /* Same arguments as mnp_bemret_solve (npol <= 8). */

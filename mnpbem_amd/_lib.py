@@ -80,6 +80,7 @@ PROTOTYPES = {
     "mnp_hmat_matvec": (INT, [P, I64, P, P]),
     "mnp_hmat_stats": (INT, [P, PI64, PI64, C.POINTER(INT), PD]),
     "mnp_hmat_ranks": (INT, [P, P]),
+    "mnp_hmat_coarsen": (INT, [P, PI64, P, P]),
     "mnp_bemretiter_create": (INT, [P, INT, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
     "mnp_bemretiter_destroy": (INT, [P]),
     "mnp_bemretiter_set_options": (INT, [P, D, INT, INT]),

@@ -312,8 +312,9 @@ class bemretiter:
     ``op['iter']`` holds bemiter options (cleaf, htol, kmax, tol, maxit,
     precond) and ``hprecision`` ('fp64', the reference's H-matrices, or
     'fp32' storage).  GMRES runs on the GPU with the exact operator (four
-    H-matvecs) and a block preconditioner; see mnp_bemretiter_* in
-    include/mnpbem_hip.h.
+    H-matvecs) and the preconditioner ``precond`` ('hmat': two levels, a
+    coarse bemret chain on aggregated H-matrices and block Jacobi; 'block':
+    block Jacobi; 'none'); see mnp_bemretiter_* in include/mnpbem_hip.h.
     """
 
     _PREFIX = "mnp_bemretiter"
@@ -339,8 +340,21 @@ class bemretiter:
                   C.byref(h))
         self.handle = h
         _lib.call(self._PREFIX + "_set_options", h, float(it["tol"]), int(it["maxit"]),
-                  int(it["precond"] not in (None, "none", False)))
+                  self._precond_code(it["precond"]))
         self.enei = None
+
+    # precond: 'hmat' (the bemiter default) couples the whole surface: the
+    # reference's rank-4 H-matrix chain, here the two-level preconditioner
+    # (coarse bemret chain on the H-matrices' row tiles + block Jacobi);
+    # 'block' the block-Jacobi smoother alone; None / 'none' none.
+    _PRECOND = {"hmat": 2, "block": 1, "none": 0}
+
+    def _precond_code(self, name):
+        if name in (None, False):
+            return 0
+        if name not in self._PRECOND:
+            raise ValueError(f"unknown precond {name!r}: 'hmat', 'block' or 'none'")
+        return self._PRECOND[name]
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -387,6 +401,8 @@ class bemstatiter(bemretiter):
     for bemretiter (op['iter'])."""
 
     _PREFIX = "mnp_bemstatiter"
+    # the quasistatic solver has the block preconditioner only ('hmat' maps to it)
+    _PRECOND = {"hmat": 1, "block": 1, "none": 0}
 
     def solve(self, exc):
         """[sig, bem] = solve(bem, exc)  (BEM/@bemstatiter/solve.m), exc with field phip."""

@@ -9,16 +9,25 @@
 // flexible form, so that the minimised residual is |b - A x|.
 //
 // Preconditioner: the reference compresses the whole bemret chain into
-// H-matrices of rank 4 (private/initprecond.m, H-LU and H-inverse).  Here
-// the same chain is applied exactly on diagonal blocks of at most 32
-// consecutive cluster indices inside each leaf cluster (block Jacobi): G1,
-// G2, Sigma1, Deltai, Sigma of every block are formed and factorised in LDS
-// by one workgroup, in complex FP32.  The preconditioner changes the
+// H-matrices of rank 4 (private/initprecond.m, H-LU and H-inverse), so that
+// it couples the whole surface.  Here, two levels:
+//  - smoother: the same chain applied exactly on diagonal blocks of at most
+//    32 consecutive cluster indices inside each leaf cluster (block Jacobi):
+//    G1, G2, Sigma1, Deltai, Sigma of every block are formed and factorised
+//    in LDS by one workgroup, in complex FP32;
+//  - coarse level: the bemret chain on the aggregated operators P^T G1 P,
+//    P^T G2 P, P^T H1 P, P^T H2 P (P the indicator of the H-matrix row
+//    tiles, <= 32 faces each; summed from the stored blocks and factors) with
+//    the tiles' mean normals, on the FP32 MFMA chain of the dense solver.
+// mfun applies the coarse correction and then the smoother to what remains:
+//   z = P y + B^{-1} (v - A P y),  y = C^{-1} P^T v.
+// precond 1 keeps the smoother alone.  The preconditioner changes the
 // iteration count only; the converged solution solves the same equations.
 #include <complex>
 #include <memory>
 #include <vector>
 
+#include "mnp_chain.hpp"
 #include "mnp_gmres.hpp"
 #include "mnp_hmatrix.hpp"
 
@@ -286,6 +295,40 @@ __global__ void __launch_bounds__(256) k_bjprec_apply(const int64_t* pb, const i
   }
 }
 
+// ----------------------------------------------------- coarse level
+// y[s, q] = sum over the cluster indices c of tile s of v[perm[c], q]
+__global__ void k_restrict(int64_t n, int64_t nc, int64_t ncol, const int32_t* perm, const int32_t* tilebeg,
+                           const double2* v, float2* y) {
+  const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (s >= nc) return;
+  double2 acc = mk<double>(0, 0);
+  for (int c = tilebeg[s]; c < tilebeg[s + 1]; ++c) acc = cadd<double>(acc, v[q * n + perm[c]]);
+  y[q * nc + s] = make_float2((float)acc.x, (float)acc.y);
+}
+// z[perm[c], q] = y[tile of c, q]
+__global__ void k_prolong(int64_t n, int64_t nc, const int32_t* perm, const int32_t* rowtile, const float2* y,
+                          double2* z) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (c >= n) return;
+  const float2 v = y[q * nc + rowtile[c]];
+  z[q * n + perm[c]] = mk<double>(v.x, v.y);
+}
+// r = v - w ;  z += d
+__global__ void k_vsub(int64_t len, const double2* v, const double2* w, double2* r) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < len) r[i] = csub<double>(v[i], w[i]);
+}
+__global__ void k_vadd(int64_t len, double2* z, const double2* d) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < len) z[i] = cadd<double>(z[i], d[i]);
+}
+__global__ void k_c64_to_c32(int64_t len, const double2* a, float2* b) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < len) b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
 // ------------------------------------------------------------ solver
 struct BemRetIter {
   const GreenRet* g;
@@ -295,7 +338,11 @@ struct BemRetIter {
   int64_t* d_pb = nullptr;  // npb x 5 block descriptors (k_bjprec_build)
   int64_t npb = 0;
   int precision;
-  bool use_prec = true;
+  int prec_mode = 2;  // 0 none, 1 block Jacobi, 2 two levels (coarse chain + block Jacobi)
+  std::unique_ptr<Chain<float>> coarse;  // the chain on the row tiles of the H-matrices
+  double* cnvec = nullptr;               // mean normals of the tiles, nc x 3
+  int64_t nc = 0;
+  Workspace cws;
   double tol = 1e-6;
   int maxit = 100;
   double enei = 0, k0 = 0;
@@ -332,10 +379,42 @@ struct BemRetIter {
     d_pb = dev_alloc<int64_t>((int64_t)pb.size());
     MNP_HIP(hipMemcpy(d_pb, pb.data(), sizeof(int64_t) * pb.size(), hipMemcpyHostToDevice));
     prec = dev_alloc<float2>(npb * 5 * PB * PB);
+    // coarse level: the row tiles and their mean normals
+    nc = G1->ntile;
+    std::vector<double> nv(3 * n), cn(3 * nc, 0.0);
+    MNP_HIP(hipMemcpy(nv.data(), g->geo.nvec, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    for (int64_t t = 0; t < nc; ++t) {
+      const int64_t a = G1->tilebeg_h[t], b = G1->tilebeg_h[t + 1];
+      for (int64_t c = a; c < b; ++c)
+        for (int d = 0; d < 3; ++d) cn[d * nc + t] += nv[d * n + perm[c]];
+      for (int d = 0; d < 3; ++d) cn[d * nc + t] /= (double)(b - a);
+    }
+    cnvec = dev_alloc<double>(3 * nc);
+    MNP_HIP(hipMemcpy(cnvec, cn.data(), sizeof(double) * 3 * nc, hipMemcpyHostToDevice));
   }
   ~BemRetIter() {
     if (prec) (void)hipFree(prec);
     if (d_pb) (void)hipFree(d_pb);
+    if (cnvec) (void)hipFree(cnvec);
+  }
+
+  // The coarse chain of this wavelength from the four filled H-matrices.
+  void build_coarse(double2 e1, double2 e2) {
+    if (!coarse) coarse.reset(new Chain<float>(nc, false));
+    Chain<float>& c = *coarse;
+    c.eps1 = e1;
+    c.eps2 = e2;
+    c.k0 = k0;
+    const int64_t nn = nc * nc;
+    double2* C = (double2*)cws.get(2, sizeof(double2) * (size_t)nn);
+    const HMat* M[4] = {G1.get(), G2.get(), H1.get(), H2.get()};
+    float2* dst[4] = {c.G1, c.G2, c.S1, c.S2};
+    for (int i = 0; i < 4; ++i) {
+      M[i]->coarsen(C, cws);
+      hipLaunchKernelGGL(k_c64_to_c32, dim3((unsigned)cdiv_up(nn, 256)), dim3(256), 0, stream(), nn, C, dst[i]);
+      MNP_HIP(hipGetLastError());
+    }
+    chain_build<float>(c, cnvec);
   }
 
   void initmat(double enei_, double2 e1, double2 e2) {
@@ -366,6 +445,7 @@ struct BemRetIter {
                          (const float2*)G1->fvals, (const float2*)G2->fvals, (const float2*)H1->fvals,
                          (const float2*)H2->fvals, G1->perm, g->geo.nvec, n, e1, e2, cc, prec);
     MNP_HIP(hipGetLastError());
+    if (prec_mode == 2) build_coarse(e1, e2);
     ready = true;
   }
 
@@ -380,13 +460,39 @@ struct BemRetIter {
     const int64_t len = 8 * p * n;
     scale_vec(len, y, -1.0);
   }
-  void mfun(int64_t p, const double2* b, double2* x) {
-    if (!use_prec) {
-      MNP_HIP(hipMemcpyAsync(x, b, sizeof(double2) * 8 * p * n, hipMemcpyDeviceToDevice, stream()));
-      return;
-    }
+  void bj_apply(int64_t p, const double2* b, double2* x) {
     hipLaunchKernelGGL(k_bjprec_apply, dim3((unsigned)npb), dim3(256), 0, stream(), d_pb, G1->perm, prec,
                        g->geo.nvec, n, (int)p, k0, eps1, eps2, b, x);
+    MNP_HIP(hipGetLastError());
+  }
+  void mfun(int64_t p, const double2* b, double2* x, double2* prod) {
+    const int64_t len = 8 * p * n;
+    if (prec_mode == 0) {
+      MNP_HIP(hipMemcpyAsync(x, b, sizeof(double2) * len, hipMemcpyDeviceToDevice, stream()));
+      return;
+    }
+    if (prec_mode == 1) {
+      bj_apply(p, b, x);
+      return;
+    }
+    // x = P y + B^{-1} (b - A P y), y = C^{-1} P^T b
+    float2* yb = (float2*)cws.get(3, sizeof(float2) * (size_t)(8 * p * nc));
+    float2* yx = (float2*)cws.get(4, sizeof(float2) * (size_t)(8 * p * nc));
+    float2* ytmp = (float2*)cws.get(5, sizeof(float2) * (size_t)(18 * p * nc));
+    double2* w = (double2*)cws.get(6, sizeof(double2) * (size_t)len);
+    double2* d = (double2*)cws.get(7, sizeof(double2) * (size_t)len);
+    hipLaunchKernelGGL(k_restrict, dim3((unsigned)cdiv_up(nc, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n,
+                       nc, (int64_t)(8 * p), G1->perm, G1->d_tilebeg, b, yb);
+    MNP_HIP(hipGetLastError());
+    chain_apply<float>(*coarse, cnvec, p, yb, yx, ytmp);
+    hipLaunchKernelGGL(k_prolong, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n, nc,
+                       G1->perm, G1->d_rowtile, yx, x);
+    MNP_HIP(hipGetLastError());
+    afun(p, x, w, prod);
+    const unsigned gl = (unsigned)cdiv_up(len, 256);
+    hipLaunchKernelGGL(k_vsub, dim3(gl), dim3(256), 0, stream(), len, b, w, w);
+    bj_apply(p, w, d);
+    hipLaunchKernelGGL(k_vadd, dim3(gl), dim3(256), 0, stream(), len, x, d);
     MNP_HIP(hipGetLastError());
   }
 
@@ -400,7 +506,7 @@ struct BemRetIter {
     launch_excitation(n, p, in, b, g->geo.nvec, k0, eps1, eps2);
     const GmresOut o = fgmres(
         len, b, x, tol, maxit, [&](const double2* v, double2* w) { afun(p, v, w, prod); },
-        [&](const double2* v, double2* z) { mfun(p, v, z); }, ws);
+        [&](const double2* v, double2* z) { mfun(p, v, z, prod); }, ws);
     last_iter = o.iter;
     last_flag = o.flag;
     last_relres = o.relres;
@@ -449,7 +555,8 @@ int mnp_bemretiter_set_options(mnp_bemretiter_t it, double tol, int maxit, int p
     MNP_CHECK(it && tol > 0 && maxit >= 1, MNP_ERR_ARG, "bemretiter_set_options: bad argument");
     it->b->tol = tol;
     it->b->maxit = maxit;
-    it->b->use_prec = precond != 0;
+    MNP_CHECK(precond >= 0 && precond <= 2, MNP_ERR_ARG, "bemretiter_set_options: precond must be 0, 1 or 2");
+    it->b->prec_mode = precond;
   });
 }
 

@@ -582,6 +582,7 @@ HMat::HMat(const GreenRet* g_, int prec_, const int64_t* perm_h, int64_t nfull_,
     }
   }
   ntile = (int64_t)tileleaf.size();
+  rowtile_h = rowtile;
   tilebeg_h = tilebeg;
   tileleaf_h = tileleaf;
   lentry_h = lentry;
@@ -726,6 +727,7 @@ void HMat::fill_impl(double2 k, int key_) {
   maxrank = 0;
   for (int32_t r : rank_h) maxrank = std::max(maxrank, (int)r);
   rank_host = rank_h;
+  loff_host = loff_h;
   tlen = toff_h[nlow];
   build_plan(loff_h, toff_h);
   filled = true;
@@ -842,6 +844,122 @@ void HMat::matvec_impl(int64_t nrhs, const double2* x, double2* y, Workspace& ws
   }
 }
 
+// ------------------------------------------------------ coarse operator
+// Full blocks: one workgroup per block, the sum of its entries over each
+// (row tile, column tile) pair (with leaves of at most 32 rows, one pair).
+template <class TS>
+__global__ void __launch_bounds__(256) k_coarse_full(const int32_t* fblk, const int64_t* voff, const int32_t* rowtile,
+                                                     const int32_t* tilebeg, const cplx<TS>* fvals, int64_t nc,
+                                                     double2* C) {
+  const int64_t b = blockIdx.x;
+  const int r0 = fblk[4 * b], r1 = fblk[4 * b + 1], c0 = fblk[4 * b + 2], c1 = fblk[4 * b + 3];
+  const int m = r1 - r0;
+  const cplx<TS>* V = fvals + voff[b];
+  __shared__ double2 red[256];
+  for (int s = rowtile[r0]; s <= rowtile[r1 - 1]; ++s)
+    for (int t = rowtile[c0]; t <= rowtile[c1 - 1]; ++t) {
+      const int i0 = tilebeg[s] - r0, i1 = tilebeg[s + 1] - r0, j0 = tilebeg[t] - c0, j1 = tilebeg[t + 1] - c0;
+      const int mi = i1 - i0, cnt = mi * (j1 - j0);
+      double2 acc = mk<double>(0, 0);
+      for (int e = threadIdx.x; e < cnt; e += 256) {
+        const cplx<TS> v = V[(int64_t)(j0 + e / mi) * m + i0 + e % mi];
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      red[threadIdx.x] = acc;
+      __syncthreads();
+      for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] = cadd<double>(red[threadIdx.x], red[threadIdx.x + st]);
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) C[(int64_t)t * nc + s] = red[0];
+      __syncthreads();
+    }
+}
+
+// Low-rank blocks L R^T: one workgroup per block.  Lagg (ns x r) sums L's
+// rows per row tile (each tile's rows are one contiguous segment of L),
+// Ragg (nt x r) sums R's rows per column tile; then C(s, t) = Lagg_s . Ragg_t.
+// agg[aoff[b] ..] is this block's scratch, (ns + nt) r entries.
+template <class TS>
+__global__ void __launch_bounds__(256) k_coarse_low(const int32_t* lblk, const int32_t* rank, const int64_t* loff,
+                                                    const int32_t* rowtile, const int32_t* tilebeg,
+                                                    const cplx<TS>* pool, const int64_t* aoff, double2* agg,
+                                                    int64_t nc, double2* C) {
+  const int64_t b = blockIdx.x;
+  const int r0 = lblk[4 * b], r1 = lblk[4 * b + 1], c0 = lblk[4 * b + 2], c1 = lblk[4 * b + 3];
+  const int m = r1 - r0, r = rank[b];
+  const int s0 = rowtile[r0], ns = rowtile[r1 - 1] - s0 + 1, t0 = rowtile[c0], nt = rowtile[c1 - 1] - t0 + 1;
+  if (r == 0) {
+    for (int e = threadIdx.x; e < ns * nt; e += 256) C[(int64_t)(t0 + e / ns) * nc + s0 + e % ns] = mk<double>(0, 0);
+    return;
+  }
+  const cplx<TS>* L = pool + loff[b];
+  const cplx<TS>* RT = L + (int64_t)m * r;
+  double2* La = agg + aoff[b];
+  double2* Ra = La + (int64_t)ns * r;
+  for (int e = threadIdx.x; e < ns * r; e += 256) {
+    const int si = e / r, l = e % r;
+    const int a = tilebeg[s0 + si] - r0, mt = tilebeg[s0 + si + 1] - tilebeg[s0 + si];
+    const cplx<TS>* seg = L + (int64_t)a * r;  // the tile's segment, mt x r column-major
+    double2 acc = mk<double>(0, 0);
+    for (int i = 0; i < mt; ++i) {
+      const cplx<TS> v = seg[(int64_t)l * mt + i];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    La[(int64_t)si * r + l] = acc;
+  }
+  for (int e = threadIdx.x; e < nt * r; e += 256) {
+    const int ti = e / r, l = e % r;
+    double2 acc = mk<double>(0, 0);
+    for (int j = tilebeg[t0 + ti]; j < tilebeg[t0 + ti + 1]; ++j) {
+      const cplx<TS> v = RT[(int64_t)(j - c0) * r + l];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    Ra[(int64_t)ti * r + l] = acc;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ns * nt; e += 256) {
+    const int si = e % ns, ti = e / ns;
+    double2 acc = mk<double>(0, 0);
+    for (int l = 0; l < r; ++l) acc = cadd<double>(acc, cmul<double>(La[(int64_t)si * r + l], Ra[(int64_t)ti * r + l]));
+    C[(int64_t)(t0 + ti) * nc + s0 + si] = acc;
+  }
+}
+
+void HMat::coarsen(double2* C, Workspace& ws) const {
+  if (prec == MNP_FP64)
+    coarsen_impl<double>(C, ws);
+  else
+    coarsen_impl<float>(C, ws);
+}
+
+template <class TS>
+void HMat::coarsen_impl(double2* C, Workspace& ws) const {
+  MNP_CHECK(filled, MNP_ERR_STATE, "hmat: coarsen before fill");
+  hipStream_t s = stream();
+  if (nfull)
+    hipLaunchKernelGGL(k_coarse_full<TS>, dim3((unsigned)nfull), dim3(256), 0, s, fblk, d_voff, d_rowtile, d_tilebeg,
+                       (const cplx<TS>*)fvals, ntile, C);
+  if (nlow) {
+    std::vector<int64_t> aoff(nlow + 1, 0);
+    for (int64_t b = 0; b < nlow; ++b) {
+      const int64_t ns = rowtile_h[lblk_h[4 * b + 1] - 1] - rowtile_h[lblk_h[4 * b]] + 1;
+      const int64_t nt = rowtile_h[lblk_h[4 * b + 3] - 1] - rowtile_h[lblk_h[4 * b + 2]] + 1;
+      aoff[b + 1] = aoff[b] + (ns + nt) * rank_host[b];
+    }
+    int64_t* d_aoff = (int64_t*)ws.get(0, sizeof(int64_t) * (nlow + 1));
+    double2* agg = (double2*)ws.get(1, sizeof(double2) * (size_t)std::max<int64_t>(aoff[nlow], 1));
+    MNP_HIP(hipMemcpyAsync(d_aoff, aoff.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_coarse_low<TS>, dim3((unsigned)nlow), dim3(256), 0, s, lblk, rank, d_loff, d_rowtile,
+                       d_tilebeg, (const cplx<TS>*)pool, d_aoff, agg, ntile, C);
+    MNP_HIP(hipStreamSynchronize(s));  // aoff is host memory of this call
+  }
+  MNP_HIP(hipGetLastError());
+}
+
 }  // namespace mnp
 
 // ============================================================== C ABI
@@ -902,6 +1020,21 @@ int mnp_hmat_stats(mnp_hmat_t h, int64_t* full_entries, int64_t* lowrank_entries
     if (lowrank_entries) *lowrank_entries = H.nlowvals;
     if (max_rank) *max_rank = H.maxrank;
     if (compression) *compression = (double)(H.nfullvals + H.nlowvals) / ((double)H.n * (double)H.n);
+  });
+}
+
+int mnp_hmat_coarsen(mnp_hmat_t h, int64_t* ntile, int64_t* tilebeg, void* C) {
+  return guarded([&] {
+    MNP_CHECK(h && ntile, MNP_ERR_ARG, "hmat_coarsen: NULL argument");
+    const HMat& H = *h->h;
+    MNP_CHECK(H.filled, MNP_ERR_STATE, "hmat_coarsen: not filled");
+    *ntile = H.ntile;
+    if (tilebeg)
+      for (int64_t t = 0; t <= H.ntile; ++t) tilebeg[t] = H.tilebeg_h[t];
+    if (C) {
+      H.coarsen((double2*)C, h->ws);
+      MNP_HIP(hipStreamSynchronize(stream()));
+    }
   });
 }
 

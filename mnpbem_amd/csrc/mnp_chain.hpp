@@ -199,32 +199,33 @@ void chain_sector(Chain<T>& c) {
 }
 
 // Sigma = eps1 Sigma1 - eps2 Sigma2 + k^2 (eps1 - eps2)^2 (Deltai o n n^T), over S2
+// nvec: n x 3 normals of the chain's unknowns (the faces, or aggregates of faces)
 template <class T>
-void chain_sigma(Chain<T>& c, const GreenRet& g) {
+void chain_sigma(Chain<T>& c, const double* nvec) {
   const int64_t n = c.n;
   const double2 L = csub<double>(c.eps1, c.eps2);
   const double2 cc = cscale<double>(cmul<double>(L, L), c.k0 * c.k0);
   hipLaunchKernelGGL(k_sigma<T>, gridnn(n, n), dim3(256), 0, stream(), n, c.S1, c.S2, c.Di,
-                     g.geo.nvec, c.eps1, c.eps2, cc);
+                     nvec, c.eps1, c.eps2, cc);
   MNP_HIP(hipGetLastError());
   c.sigma_formed = true;
 }
 // Factorises the chain (chain_sector, then Sigma over S2 and its LU), split
 // like chain_sector into a part without partial pivoting and a finish.
 template <class T>
-bool chain_build_try(Chain<T>& c, const GreenRet& g) {
+bool chain_build_try(Chain<T>& c, const double* nvec) {
   c.sigma_formed = c.ok_s = false;
   if (!chain_sector_try(c)) return false;
-  chain_sigma(c, g);
+  chain_sigma(c, nvec);
   c.ok_s = c.factor_try(3, c.S2, c.backup, c.piv, c.ws);
   return c.ok_s;
 }
 template <class T>
-void chain_build_finish(Chain<T>& c, const GreenRet& g) {
+void chain_build_finish(Chain<T>& c, const double* nvec) {
   if (c.ok_s) return;
   if (!c.sigma_formed) {
     chain_sector_finish(c);
-    chain_sigma(c, g);
+    chain_sigma(c, nvec);
     c.factor(3, c.S2, "Sigma");
   } else {
     c.factor_pivoted(3, c.S2, "Sigma");
@@ -232,9 +233,15 @@ void chain_build_finish(Chain<T>& c, const GreenRet& g) {
   c.ok_s = true;
 }
 template <class T>
-void chain_build(Chain<T>& c, const GreenRet& g) {
-  if (!chain_build_try(c, g)) chain_build_finish(c, g);
+void chain_build(Chain<T>& c, const double* nvec) {
+  if (!chain_build_try(c, nvec)) chain_build_finish(c, nvec);
 }
+template <class T>
+bool chain_build_try(Chain<T>& c, const GreenRet& g) { return chain_build_try(c, g.geo.nvec); }
+template <class T>
+void chain_build_finish(Chain<T>& c, const GreenRet& g) { chain_build_finish(c, g.geo.nvec); }
+template <class T>
+void chain_build(Chain<T>& c, const GreenRet& g) { chain_build(c, g.geo.nvec); }
 
 // ------------------------------------------------------------- mldivide
 // Vectors are n x (8 p) blocks.  b = [phi (p) | a (3p) | De (p) | alpha (3p)],
@@ -317,8 +324,7 @@ __global__ void k_mld4(int64_t n, int64_t p, const cplx<T>* b, const cplx<T>* si
 
 // x = M^{-1} b for the chain c (precision T), b and x n x 8p in T.
 template <class T>
-void chain_apply(Chain<T>& c, const GreenRet& g, int64_t p, const cplx<T>* b, cplx<T>* x,
-                        cplx<T>* tmp) {
+void chain_apply(Chain<T>& c, const double* nvec, int64_t p, const cplx<T>* b, cplx<T>* x, cplx<T>* tmp) {
   const int64_t n = c.n;
   const double k = c.k0;
   const double2 L1 = c.eps1, L2 = c.eps2;
@@ -338,11 +344,11 @@ void chain_apply(Chain<T>& c, const GreenRet& g, int64_t p, const cplx<T>* b, cp
   gemm<T>(n, p, n, one, c.v(c.S1), MatView<T>{const_cast<cplx<T>*>(b), n}, zero,
           MatView<T>{Y + 3 * p * n, n});
   const dim3 gp((unsigned)cdiv_up(n, 256), (unsigned)p);
-  hipLaunchKernelGGL(k_mld1<T>, gp, dim3(256), 0, stream(), n, p, b, Y, alpha, De, g.geo.nvec, k, L1);
+  hipLaunchKernelGGL(k_mld1<T>, gp, dim3(256), 0, stream(), n, p, b, Y, alpha, De, nvec, k, L1);
   gemm<T>(n, 3 * p, n, one, c.v(c.Di), MatView<T>{alpha, n}, zero, MatView<T>{t, n});
-  hipLaunchKernelGGL(k_mld2<T>, gp, dim3(256), 0, stream(), n, p, De, t, s2, g.geo.nvec, ikdL);
+  hipLaunchKernelGGL(k_mld2<T>, gp, dim3(256), 0, stream(), n, p, De, t, s2, nvec, ikdL);
   getrs<T>(n, p, c.f[3], MatView<T>{s2, n}, c.ws);  // sig2 = Sigma^{-1} rhs
-  hipLaunchKernelGGL(k_mld3<T>, gp, dim3(256), 0, stream(), n, p, s2, alpha, u, g.geo.nvec, ikdL);
+  hipLaunchKernelGGL(k_mld3<T>, gp, dim3(256), 0, stream(), n, p, s2, alpha, u, nvec, ikdL);
   gemm<T>(n, 3 * p, n, one, c.v(c.Di), MatView<T>{u, n}, zero, MatView<T>{h2, n});
   // V1 -> x[0:4p] (sig1 | h1), V2 -> x[4p:8p] (sig2 | h2)
   hipLaunchKernelGGL(k_mld4<T>, dim3((unsigned)cdiv_up(n, 256), (unsigned)(4 * p)), dim3(256), 0,
@@ -350,6 +356,10 @@ void chain_apply(Chain<T>& c, const GreenRet& g, int64_t p, const cplx<T>* b, cp
   MNP_HIP(hipGetLastError());
   getrs<T>(n, 4 * p, c.f[0], MatView<T>{x, n}, c.ws);
   getrs<T>(n, 4 * p, c.f[1], MatView<T>{x + 4 * p * n, n}, c.ws);
+}
+template <class T>
+void chain_apply(Chain<T>& c, const GreenRet& g, int64_t p, const cplx<T>* b, cplx<T>* x, cplx<T>* tmp) {
+  chain_apply(c, g.geo.nvec, p, b, x, tmp);
 }
 
 }  // namespace mnp

@@ -28,7 +28,7 @@ struct HMat {
   int64_t *d_voff = nullptr, *d_lrow = nullptr, *d_loffs = nullptr, *d_loff = nullptr, *d_toff = nullptr;
   double *cpos = nullptr, *cnvec = nullptr, *carea = nullptr;
   // product plan (rebuilt by each fill): first-phase items, row-tile descriptors
-  std::vector<int32_t> lentry_h, tilebeg_h, tileleaf_h;
+  std::vector<int32_t> lentry_h, tilebeg_h, tileleaf_h, rowtile_h;
   std::vector<int64_t> loffs_h, voff_h;
   int64_t nitems = 0, nmulti = 0, tpartlen = 0;
   int32_t *d_items = nullptr, *d_multi = nullptr, *d_tdbeg = nullptr, *d_tlda = nullptr, *d_tK = nullptr;
@@ -45,6 +45,11 @@ struct HMat {
   void fill(double2 k, int key);
   // y = H x, x and y complex FP64 in face order (n x nrhs).
   void matvec(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const;
+  // C = P^T H P (ntile x ntile, column-major, complex FP64) for P the
+  // indicator of the row tiles (<= 32 consecutive cluster indices each):
+  // entry (s, t) sums the operator over the rows of tile s and the columns
+  // of tile t, from the stored full blocks and low-rank factors.
+  void coarsen(double2* C, Workspace& ws) const;
 
  private:
   void build_plan(const std::vector<int64_t>& loff_h, const std::vector<int64_t>& toff_h);
@@ -52,6 +57,9 @@ struct HMat {
   void fill_impl(double2 k, int key);
   template <class TS>
   void matvec_impl(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const;
+  template <class TS>
+  void coarsen_impl(double2* C, Workspace& ws) const;
+  std::vector<int64_t> loff_host;  // low-rank factor offsets of the last fill
 };
 
 }  // namespace mnp

@@ -165,6 +165,20 @@ class hmatrix:
         _lib.call("mnp_hmat_ranks", self.handle, r.ctypes.data_as(C.c_void_p))
         return r
 
+    def coarsen(self):
+        """(tilebeg, C): the row tiles of the product plan (first cluster
+        indices, ntile + 1) and the coarse operator P^T H P summed over them
+        (device ntile x ntile complex128, column-major; mnp_hmat_coarsen)."""
+        import ctypes as C
+        from . import _lib, device as dev
+        nt = C.c_int64()
+        _lib.call("mnp_hmat_coarsen", self.handle, C.byref(nt), None, None)
+        tb = np.zeros(nt.value + 1, dtype=np.int64)
+        Cm = dev.colmajor((nt.value, nt.value))
+        _lib.call("mnp_hmat_coarsen", self.handle, C.byref(nt), tb.ctypes.data_as(C.c_void_p),
+                  dev.ptr(dev.base(Cm)))
+        return tb, Cm
+
     def stats(self):
         import ctypes as C
         from . import _lib

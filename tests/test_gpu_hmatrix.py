@@ -114,6 +114,32 @@ def test_fp64_matvec_equals_oracle_hmatrix(setup, key):
     assert err < 1e-11
 
 
+@pytest.mark.parametrize("key", ["G", "H1"])
+def test_coarse_operator_equals_aggregated_oracle_hmatrix(setup, key):
+    """FP64 storage: the coarse operator P^T H P summed from the stored blocks
+    and factors equals the oracle's H-matrix (same ACA) summed over the same
+    tiles, to 1e-11; the tiles partition the cluster indices in runs of at
+    most 32."""
+    from mnpbem_amd import device as dev
+    from mnpbem_amd.hmatrix import admissibility, hmatrix
+    ref, gpu, tree = setup
+    k = 0.07 + 0.015j
+    hm = hmatrix(gpu, tree, htol=1e-6, kmax=100, precision="fp64").fill(k, key)
+    tb, Cm = hm.coarsen()
+    Cg = dev.to_host(Cm)
+    nt = len(tb) - 1
+    assert tb[0] == 0 and tb[-1] == gpu.n and np.all(np.diff(tb) >= 1) and np.all(np.diff(tb) <= 32)
+    _, low = admissibility(tree)
+    A = HO.hmat_dense(ref, tree, low, k, key, htol=1e-6, kmax=100)
+    P = np.zeros((gpu.n, nt))
+    for t in range(nt):
+        P[tree.perm[tb[t]:tb[t + 1]], t] = 1.0
+    want = P.T @ A @ P
+    err = np.abs(Cg - want).max() / np.abs(want).max()
+    print(f"  {key}: {nt} tiles, coarse operator rel err {err:.2e}", flush=True)
+    assert err < 1e-11
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
 def test_full_blocks_only_equal_dense(setup, precision, tol):
     """With one cluster (cleaf >= n) the H-matrix is the refined dense matrix."""
@@ -250,18 +276,23 @@ def test_bemretiter_level4_fp32_storage_and_dense():
     assert np.all(np.abs(cs["fp64"] - cs_d) <= 2e-3 * scale)
 
 
-def test_bemretiter_default_options_and_preconditioner(mesh):
-    """bemiter.options defaults (cleaf 200, htol 1e-6, tol 1e-6); the block
-    preconditioner reduces the iteration count."""
+@pytest.mark.parametrize("cleaf", [200, 32])
+def test_bemretiter_preconditioners(mesh, cleaf):
+    """bemiter.options defaults (cleaf 200, htol 1e-6, tol 1e-6) and cleaf 32:
+    the block chain reduces the iteration count, the two-level preconditioner
+    (precond 'hmat') more; all three reach the same solution."""
+    from mnpbem_amd import device as dev
     v, f = mesh
     B, p, op = _particle(v, f, 1.77)
     exc = B.planewave([[1.0, 0, 0]], [[0, 0, 1.0]])
     pot = exc(p, 600.0)
-    its = {}
-    for pre in ("hmat", None):
-        it = B.bemretiter(p, dict(op, iter=dict(maxit=300, precond=pre)))
-        it.solve(pot)
+    its, sol = {}, {}
+    for pre in ("hmat", "block", None):
+        it = B.bemretiter(p, dict(op, iter=dict(maxit=300, precond=pre, cleaf=cleaf)))
+        sol[pre] = dev.to_host(it.solve(pot)["sig2"])
         its[pre] = it.stats()
-    print(f"  iterations {its}", flush=True)
-    assert its["hmat"]["flag"] == 0 and its["hmat"]["true_relres"] < 1e-5
-    assert its["hmat"]["iter"] < its[None]["iter"]
+    print(f"  cleaf {cleaf}: iterations { {k: v['iter'] for k, v in its.items()} }", flush=True)
+    for pre in its:
+        assert its[pre]["flag"] == 0 and its[pre]["true_relres"] < 1e-5
+        assert np.linalg.norm(sol[pre] - sol["hmat"]) <= 1e-4 * np.linalg.norm(sol["hmat"])
+    assert its["hmat"]["iter"] < its["block"]["iter"] < its[None]["iter"]
