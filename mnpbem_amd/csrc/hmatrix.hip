@@ -6,8 +6,9 @@
 // hands it to the MEX kernels.  On the GPU:
 //  - k_hfull: full blocks element-wise, with the refined near-field/self
 //    entries (greenret eval2) and +-2 pi on coincident centroids (H1/H2);
-//  - k_aca:   one workgroup per admissible block runs the partially pivoted
-//    ACA of mex/hlib/aca.cpp:158-218 on the collocation kernel of
+//  - k_aca:   one wave (blocks with m + n <= 320, four per workgroup) or one
+//    workgroup per admissible block runs the partially pivoted ACA of
+//    mex/hlib/aca.cpp:158-218 on the collocation kernel of
 //    mex/acagreen/acagreen.cpp:79-127, in FP64, rank <= min(kmax, m, n);
 //  - k_hmv_low / k_hmv_tile: y = H x with blocks and factors stored in FP64
 //    or FP32 and FP64 accumulation: t_b = R_b^T x per low-rank block (R
@@ -100,148 +101,201 @@ __global__ void __launch_bounds__(256) k_hfull(int64_t nb, const int32_t* blk, c
   }
 }
 
-// Block reductions over 256 threads (4 waves).
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s);
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[w] = v;
-  __syncthreads();
-  return red[0] + red[1] + red[2] + red[3];
+// ACA pivots: the first largest |Re| + |Im| (lowest index on ties, as
+// izamax); idx < 0 means none yet.  z is the entry itself.
+struct Piv {
+  double v;
+  int idx;
+  double2 z;
+};
+__device__ __forceinline__ void piv_merge(Piv& a, double v, int idx, double2 z) {
+  if (idx >= 0 && (a.idx < 0 || v > a.v || (v == a.v && idx < a.idx))) {
+    a.v = v;
+    a.idx = idx;
+    a.z = z;
+  }
 }
-// argmax of val with the first (lowest) index on ties; idx < 0 means none.
-__device__ __forceinline__ void block_argmax(double& val, int64_t& idx, double* rv, int64_t* ri) {
-  for (int s = 32; s > 0; s >>= 1) {
-    const double v2 = __shfl_xor(val, s);
-    const int64_t i2 = __shfl_xor(idx, s);
-    if (i2 >= 0 && (v2 > val || (v2 == val && (idx < 0 || i2 < idx)))) {
-      val = v2;
-      idx = i2;
-    }
+__device__ __forceinline__ void piv_take(Piv& a, double v, int idx, double2 z) {
+  if (v > a.v) {  // ascending indices per thread: strict keeps the first
+    a.v = v;
+    a.idx = idx;
+    a.z = z;
   }
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) {
-    rv[w] = val;
-    ri[w] = idx;
-  }
-  __syncthreads();
-  val = rv[0];
-  idx = ri[0];
-  for (int q = 1; q < 4; ++q)
-    if (ri[q] >= 0 && (rv[q] > val || (rv[q] == val && (idx < 0 || ri[q] < idx)))) {
-      val = rv[q];
-      idx = ri[q];
-    }
 }
 
-// ACA of admissible block b (rows [r0, r1), cols [c0, c1) in cluster order).
-// Scratch: A (m x kb) and B (n x kb) at aoff[b]; rank[b] on exit.
-__global__ void __launch_bounds__(256) k_aca(int64_t first, const int32_t* blk, const int64_t* soff,
-                                             const double* cpos, const double* cnvec, const double* carea,
-                                             int64_t n, double2 wav, int isF, double tol, int kmax,
-                                             double2* scratch, int32_t* rank, uint32_t* usedbits) {
-  const int64_t b = first + blockIdx.x;
+// Orders the memory operations of a group of G threads: G = 64 is one wave
+// (program order already holds between its lanes; this only stops the
+// compiler from moving loads and stores across), G = 256 the workgroup.
+template <int G>
+__device__ __forceinline__ void gsync() {
+  if constexpr (G == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// Sums s[0..NS) and merges the pivot p over the group; every thread gets
+// the same result (xor butterflies are symmetric, the four wave partials
+// are read in one order).  For G = 256 the caller guarantees that no thread
+// still reads red/pred from an earlier call (a gsync lies between calls).
+template <int G, int NS>
+__device__ __forceinline__ void greduce(double (&s)[NS], Piv& p, double* red, Piv* pred) {
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int q = 0; q < NS; ++q) s[q] += __shfl_xor(s[q], o);
+    const double v = __shfl_xor(p.v, o);
+    const int idx = __shfl_xor(p.idx, o);
+    const double zx = __shfl_xor(p.z.x, o), zy = __shfl_xor(p.z.y, o);
+    piv_merge(p, v, idx, mk<double>(zx, zy));
+  }
+  if constexpr (G == 256) {
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+      for (int q = 0; q < NS; ++q) red[4 * q + wv] = s[q];
+      pred[wv] = p;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NS; ++q) s[q] = red[4 * q] + red[4 * q + 1] + red[4 * q + 2] + red[4 * q + 3];
+    p = pred[0];
+    for (int u = 1; u < 4; ++u) piv_merge(p, pred[u].v, pred[u].idx, pred[u].z);
+  }
+}
+
+// v - sum_{l < k} P[l ld + c] coef[l], subtracted in order of l, with the
+// loads issued eight at a time (the columns of P are this thread's own
+// earlier results, coef is shared).
+__device__ __forceinline__ double2 aca_residual(double2 v, const double2* P, int64_t ld, int c, int k,
+                                                const double2* coef) {
+  int l = 0;
+  for (; l + 8 <= k; l += 8) {
+    double2 t[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t[j] = P[(int64_t)(l + j) * ld + c];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v = csub<double>(v, cmul<double>(t[j], coef[l + j]));
+  }
+  for (; l < k; ++l) v = csub<double>(v, cmul<double>(P[(int64_t)l * ld + c], coef[l]));
+  return v;
+}
+
+// Partially pivoted ACA of admissible block b (rows [r0, r1), cols [c0, c1)
+// in cluster order) by a group of G threads, thread tid of the group.
+// Scratch: A (m x kb) then B (n x kb) at soff[b]; rank[b] on exit.  Rows
+// tid + G t are this thread's; the pivot rows already taken are kept as
+// bits t of used0/used1 (m <= 128 G, checked on the host).
+template <int G>
+__device__ __forceinline__ void aca_block(int64_t b, int tid, const int32_t* blk, const int64_t* soff,
+                                          const double* cpos, const double* cnvec, const double* carea,
+                                          int64_t n, double2 wav, int isF, double tol, int kmax, double2* scratch,
+                                          int32_t* rank, double2* coef, double* red, Piv* pred) {
   const int r0 = blk[4 * b], r1 = blk[4 * b + 1], c0 = blk[4 * b + 2], c1 = blk[4 * b + 3];
   const int m = r1 - r0, w = c1 - c0;
   const int kb = min(kmax, min(m, w));
   double2* A = scratch + soff[b];
   double2* B = A + (int64_t)m * kb;
-  uint32_t* used = usedbits + (soff[b] >> 5);  // (m + n) kb >= m bits, disjoint per block
-  __shared__ double red[4];
-  __shared__ double rv[4];
-  __shared__ int64_t ri[4];
-  __shared__ double2 coef[128];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < (m + 31) / 32; i += 256) used[i] = 0;
-  __syncthreads();
+  uint64_t used0 = 0, used1 = 0;  // bits t < 64, 64 <= t < 128
   int r = 0, k = 0;
   double nsum = 0;
   for (k = 0; k < kb; ++k) {
-    // row r -> B(:, k)
-    double s2 = 0;
-    for (int c = tid; c < w; c += 256) {
-      const double2 v = colloc(cpos, cnvec, carea, n, r0 + r, c0 + c, wav, isF);
+    // row r of the residual -> B(:, k), with A(r, l) as coefficients; the
+    // cutoff looks at the row of the matrix itself
+    for (int l = tid; l < k; l += G) coef[l] = A[(int64_t)l * m + r];
+    gsync<G>();
+    double s1[1] = {0};
+    Piv pc{-1.0, -1, mk<double>(0, 0)};
+    for (int c = tid; c < w; c += G) {
+      double2 v = colloc(cpos, cnvec, carea, n, r0 + r, c0 + c, wav, isF);
+      s1[0] += v.x * v.x + v.y * v.y;
+      v = aca_residual(v, B, w, c, k, coef);
       B[(int64_t)k * w + c] = v;
-      s2 += v.x * v.x + v.y * v.y;
+      piv_take(pc, cabs1<double>(v), c, v);
     }
-    for (int i = tid; i < m; i += 256) A[(int64_t)k * m + i] = mk<double>(0, 0);
-    const double rown = sqrt(block_sum(s2, red));
-    if (rown < 1e-10) break;
-    if (k) {
-      for (int l = tid; l < k; l += 256) coef[l] = A[(int64_t)l * m + r];
-      __syncthreads();
-      for (int c = tid; c < w; c += 256) {
-        double2 v = B[(int64_t)k * w + c];
-        for (int l = 0; l < k; ++l) v = csub<double>(v, cmul<double>(B[(int64_t)l * w + c], coef[l]));
-        B[(int64_t)k * w + c] = v;
-      }
+    greduce<G, 1>(s1, pc, red, pred);
+    if (sqrt(s1[0]) < 1e-10) {  // zero row: the rank counts it with A(:, k) = 0
+      for (int i = tid; i < m; i += G) A[(int64_t)k * m + i] = mk<double>(0, 0);
+      break;
     }
-    // pivot column: first largest |Re| + |Im|
-    double bv = -1;
-    int64_t bi = -1;
-    for (int c = tid; c < w; c += 256) {
-      const double a = cabs1<double>(B[(int64_t)k * w + c]);
-      if (a > bv) {
-        bv = a;
-        bi = c;
-      }
-    }
-    __syncthreads();
-    block_argmax(bv, bi, rv, ri);
-    const int cp = (int)bi;
-    const double2 scale = cdiv<double>(mk<double>(1, 0), B[(int64_t)k * w + cp]);
-    __syncthreads();
-    double nb2 = 0;
-    for (int c = tid; c < w; c += 256) {
+    const int cp = pc.idx;
+    const double2 scale = cdiv<double>(mk<double>(1, 0), pc.z);
+    double s2[2] = {0, 0};  // |A(:, k)|^2, |B(:, k)|^2
+    for (int c = tid; c < w; c += G) {
       const double2 v = cmul<double>(B[(int64_t)k * w + c], scale);
       B[(int64_t)k * w + c] = v;
-      nb2 += v.x * v.x + v.y * v.y;
+      s2[1] += v.x * v.x + v.y * v.y;
     }
-    __syncthreads();
-    for (int l = tid; l < k; l += 256) coef[l] = B[(int64_t)l * w + cp];
-    __syncthreads();
-    // column cp -> A(:, k)
-    double na2 = 0;
-    for (int i = tid; i < m; i += 256) {
+    // column cp of the residual -> A(:, k), with the scaled B(cp, l)
+    for (int l = tid; l < k; l += G) coef[l] = B[(int64_t)l * w + cp];
+    gsync<G>();
+    if (r % G == tid) {
+      const int tr = r / G;
+      if (tr < 64)
+        used0 |= 1ull << tr;
+      else
+        used1 |= 1ull << (tr - 64);
+    }
+    Piv pr{-1.0, -1, mk<double>(0, 0)};
+    int t = 0;
+    for (int i = tid; i < m; i += G, ++t) {
       double2 v = colloc(cpos, cnvec, carea, n, r0 + i, c0 + cp, wav, isF);
-      for (int l = 0; l < k; ++l) v = csub<double>(v, cmul<double>(A[(int64_t)l * m + i], coef[l]));
+      v = aca_residual(v, A, m, i, k, coef);
       A[(int64_t)k * m + i] = v;
-      na2 += v.x * v.x + v.y * v.y;
+      s2[0] += v.x * v.x + v.y * v.y;
+      // next pivot row: izamax of the column with the used rows set to zero
+      const bool u = ((t < 64 ? used0 >> t : used1 >> (t - 64)) & 1ull) != 0;
+      piv_take(pr, u ? 0.0 : cabs1<double>(v), i, v);
     }
-    if (tid == 0) used[r >> 5] |= 1u << (r & 31);
-    __syncthreads();
-    // next pivot row: first largest |A(i,k)| over unused rows (izamax of the
-    // work vector, whose used rows are zero)
-    double av = -1;
-    int64_t ai = -1;
-    for (int i = tid; i < m; i += 256) {
-      const bool u = used[i >> 5] & (1u << (i & 31));
-      const double a = u ? 0.0 : cabs1<double>(A[(int64_t)k * m + i]);
-      if (a > av) {
-        av = a;
-        ai = i;
-      }
-    }
-    block_argmax(av, ai, rv, ri);
-    r = (int)ai;
-    const double nk = sqrt(block_sum(na2, red)) * sqrt(block_sum(nb2, red));
+    greduce<G, 2>(s2, pr, red, pred);
+    r = pr.idx;
+    const double nk = sqrt(s2[0]) * sqrt(s2[1]);
     if (nk < tol * nsum || k + 1 >= m) break;
     nsum = sqrt(nsum * nsum + nk * nk);
   }
   if (tid == 0) rank[b] = min(k + 1, kb);
 }
 
-// Compacts the ACA scratch of block b into its factors (FP32 or FP64):
-// L (m x r) as consecutive row-tile segments, each (rows of the tile) x r
-// column-major, so that the product kernel reads one contiguous segment per
-// tile; then R^T (r x n, column-major: the r entries of each column index
-// contiguous).  rowtile[c] = tile of cluster row c, tilebeg[t] = first row.
+// One ACA work item per workgroup of 256 threads: item[1] == -2 marks one
+// large block for the whole workgroup (item[0]), otherwise item[0..3] are up
+// to four small blocks (-1: none), one per wave.
+__global__ void __launch_bounds__(256) k_aca(const int32_t* items, const int32_t* blk, const int64_t* soff,
+                                             const double* cpos, const double* cnvec, const double* carea,
+                                             int64_t n, double2 wav, int isF, double tol, int kmax,
+                                             double2* scratch, int32_t* rank) {
+  __shared__ double2 coef[4][128];
+  __shared__ double red[8];
+  __shared__ Piv pred[4];
+  const int32_t* it = items + 4 * (int64_t)blockIdx.x;
+  if (it[1] == -2) {
+    aca_block<256>(it[0], threadIdx.x, blk, soff, cpos, cnvec, carea, n, wav, isF, tol, kmax, scratch, rank,
+                   coef[0], red, pred);
+    return;
+  }
+  const int wv = threadIdx.x >> 6;
+  const int32_t b = it[wv];
+  if (b >= 0)
+    aca_block<64>(b, threadIdx.x & 63, blk, soff, cpos, cnvec, carea, n, wav, isF, tol, kmax, scratch, rank,
+                  coef[wv], nullptr, nullptr);
+}
+
+// Compacts the ACA scratch of block b = order[blockIdx.x] into its factors
+// (FP32 or FP64): L (m x r) as consecutive row-tile segments, each (rows of
+// the tile) x r column-major, so that the product kernel reads one
+// contiguous segment per tile; then R^T (r x n, column-major: the r entries
+// of each column index contiguous), transposed through LDS in chunks of JC
+// columns so that both the reads of B and the writes of R^T are contiguous.
+// rowtile[c] = tile of cluster row c, tilebeg[t] = first row.
 template <class TS>
-__global__ void k_aca_store(int64_t first, const int32_t* blk, const int64_t* soff, const double2* scratch,
-                            const int32_t* rank, int kmax, const int64_t* loff, const int32_t* rowtile,
-                            const int32_t* tilebeg, cplx<TS>* pool) {
-  const int64_t b = first + blockIdx.x;
+__global__ void __launch_bounds__(256) k_aca_store(const int32_t* order, const int32_t* blk, const int64_t* soff,
+                                                   const double2* scratch, const int32_t* rank, int kmax,
+                                                   const int64_t* loff, const int32_t* rowtile,
+                                                   const int32_t* tilebeg, cplx<TS>* pool) {
+  constexpr int JC = 32768 / (128 * (int)sizeof(cplx<TS>));  // columns per chunk (r <= 128)
+  __shared__ cplx<TS> tile[JC * 128];
+  const int64_t b = order[blockIdx.x];
   const int r0 = blk[4 * b];
   const int m = blk[4 * b + 1] - r0, w = blk[4 * b + 3] - blk[4 * b + 2];
   const int kb = min(kmax, min(m, w));
@@ -255,9 +309,18 @@ __global__ void k_aca_store(int64_t first, const int32_t* blk, const int64_t* so
     const int row = r0 + i, t = rowtile[row], tb = tilebeg[t], ms = tilebeg[t + 1] - tb;
     L[(int64_t)(tb - r0) * r + (int64_t)l * ms + (row - tb)] = mk<TS>((TS)A[e].x, (TS)A[e].y);
   }
-  for (int64_t e = threadIdx.x; e < (int64_t)w * r; e += blockDim.x) {
-    const int j = (int)(e % w), l = (int)(e / w);
-    RT[(int64_t)j * r + l] = mk<TS>((TS)B[e].x, (TS)B[e].y);
+  for (int j0 = 0; j0 < w; j0 += JC) {
+    const int jc = min(JC, w - j0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < r * JC; e += blockDim.x) {
+      const int l = e / JC, jj = e % JC;
+      if (jj < jc) {
+        const double2 v = B[(int64_t)l * w + j0 + jj];
+        tile[jj * r + l] = mk<TS>((TS)v.x, (TS)v.y);
+      }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < jc * r; e += blockDim.x) RT[(int64_t)j0 * r + e] = tile[e];
   }
 }
 
@@ -628,7 +691,7 @@ HMat::~HMat() {
   void* ps[] = {d_items, d_itpoff, d_multi, d_tdbeg, d_taoff, d_tboff, d_tlda, d_tK,
                 perm, fblk, lblk, d_voff, d_lrow, d_loffs, d_lentry, d_tilebeg, d_tileleaf, d_rowtile,
                 cpos, cnvec, carea, rank,
-                d_loff, d_toff, fvals, pool};
+                d_loff, d_toff, fvals, pool, d_aca_items, d_aca_order, d_aca_soff};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -661,66 +724,50 @@ void HMat::fill_impl(double2 k, int key_) {
                        G.pairs, G.off, G.nnz, G.order, G.gtab, G.ftab, k, key, (cplx<TS>*)fvals);
     MNP_HIP(hipGetLastError());
   }
-  // ACA in batches whose FP64 scratch stays below a budget
+  // ACA over the work plan, in batches whose FP64 scratch fits the budget
   const int isF = key == MNP_KEY_G ? 0 : 1;
-  std::vector<int64_t> soff_h(nlow + 1, 0);
-  for (int64_t b = 0; b < nlow; ++b) {
-    const int64_t m = lblk_h[4 * b + 1] - lblk_h[4 * b], w = lblk_h[4 * b + 3] - lblk_h[4 * b + 2];
-    const int64_t kb = std::min<int64_t>(kmax, std::min(m, w));
-    soff_h[b + 1] = soff_h[b] + ((m + w) * kb + 31) / 32 * 32;
-  }
   std::vector<int32_t> rank_h(nlow, 0);
   std::vector<int64_t> loff_h(nlow + 1, 0), toff_h(nlow + 1, 0);
   if (nlow) {
-    const int64_t budget = (int64_t)(1) << 29;  // double2 entries (8 GiB)
-    int64_t* d_soff = dev_alloc<int64_t>(nlow + 1);
-    // batch boundaries
-    std::vector<int64_t> cuts{0};
-    for (int64_t b = 0; b < nlow; ++b)
-      if (soff_h[b + 1] - soff_h[cuts.back()] > budget && b > cuts.back()) cuts.push_back(b);
-    cuts.push_back(nlow);
-    int64_t maxbatch = 0;
-    for (size_t c = 0; c + 1 < cuts.size(); ++c)
-      maxbatch = std::max(maxbatch, soff_h[cuts[c + 1]] - soff_h[cuts[c]]);
+    if (aca_icut.empty()) aca_plan();
     // scratch shared by every H-matrix fill (grown on demand, kept until
-    // the library unloads): a fill per operator and wavelength would
-    // otherwise pay an allocation and a free of several GiB each
-    double2* scratch = (double2*)aca_ws().get(0, sizeof(double2) * (size_t)maxbatch);
-    uint32_t* used = (uint32_t*)aca_ws().get(1, sizeof(uint32_t) * (size_t)(maxbatch / 32 + 1));
-    std::vector<int64_t> rel(nlow + 1);
-    for (size_t c = 0; c + 1 < cuts.size(); ++c) {
-      const int64_t b0 = cuts[c], b1 = cuts[c + 1];
-      for (int64_t b = b0; b <= b1; ++b) rel[b] = soff_h[b] - soff_h[b0];
-      MNP_HIP(hipMemcpy(d_soff + b0, rel.data() + b0, sizeof(int64_t) * (b1 - b0 + 1), hipMemcpyHostToDevice));
+    // the process ends): a fill per operator and wavelength would otherwise
+    // pay an allocation and a free of several GiB each
+    double2* scratch = (double2*)aca_ws().get(0, sizeof(double2) * (size_t)aca_maxbatch);
+    int64_t pos = 0;  // pool entries assigned so far (blocks in work order)
+    for (size_t c = 0; c + 1 < aca_icut.size(); ++c) {
+      const int64_t i0 = aca_icut[c], i1 = aca_icut[c + 1], o0 = aca_bcut[c], o1 = aca_bcut[c + 1];
       {
         TimerScope ts(MNP_TIMER_ACA, 0.0);
-        hipLaunchKernelGGL(k_aca, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, b0, lblk, d_soff, cpos, cnvec, carea,
-                           n, k, isF, htol, kmax, scratch, rank, used);
+        hipLaunchKernelGGL(k_aca, dim3((unsigned)(i1 - i0)), dim3(256), 0, s, d_aca_items + 4 * i0, lblk,
+                           d_aca_soff, cpos, cnvec, carea, n, k, isF, htol, kmax, scratch, rank);
         MNP_HIP(hipGetLastError());
       }
-      MNP_HIP(hipMemcpyAsync(rank_h.data() + b0, rank + b0, sizeof(int32_t) * (b1 - b0), hipMemcpyDeviceToHost, s));
+      MNP_HIP(hipMemcpyAsync(rank_h.data(), rank, sizeof(int32_t) * nlow, hipMemcpyDeviceToHost, s));
       MNP_HIP(hipStreamSynchronize(s));
-      for (int64_t b = b0; b < b1; ++b) {
+      const int64_t pos0 = pos;
+      for (int64_t o = o0; o < o1; ++o) {
+        const int32_t b = aca_order_h[o];
         const int64_t m = lblk_h[4 * b + 1] - lblk_h[4 * b], w = lblk_h[4 * b + 3] - lblk_h[4 * b + 2];
-        loff_h[b + 1] = loff_h[b] + (m + w) * rank_h[b];
-        toff_h[b + 1] = toff_h[b] + rank_h[b] * (int64_t)MNP_HMV_MAXRHS;
+        loff_h[b] = pos;
+        pos += (m + w) * rank_h[b];
       }
-      if (loff_h[b1] > poolcap) {  // grow the pool, keeping earlier batches
-        const int64_t cap = std::max<int64_t>(loff_h[b1] + loff_h[b1] / 2, 1 << 20);
+      if (pos > poolcap) {  // grow the pool, keeping earlier batches
+        const int64_t cap = std::max<int64_t>(pos + pos / 2, 1 << 20);
         void* np = dev_alloc<cplx<TS>>(cap);
-        if (pool && loff_h[b0]) MNP_HIP(hipMemcpyAsync(np, pool, es * loff_h[b0], hipMemcpyDeviceToDevice, s));
+        if (pool && pos0) MNP_HIP(hipMemcpyAsync(np, pool, es * pos0, hipMemcpyDeviceToDevice, s));
         MNP_HIP(hipStreamSynchronize(s));
         if (pool) MNP_HIP(hipFree(pool));
         pool = np;
         poolcap = cap;
       }
-      MNP_HIP(hipMemcpy(d_loff + b0, loff_h.data() + b0, sizeof(int64_t) * (b1 - b0 + 1), hipMemcpyHostToDevice));
-      hipLaunchKernelGGL(k_aca_store<TS>, dim3((unsigned)(b1 - b0)), dim3(256), 0, s, b0, lblk, d_soff, scratch,
-                         rank, kmax, d_loff, d_rowtile, d_tilebeg, (cplx<TS>*)pool);
-      MNP_HIP(hipGetLastError());
-      MNP_HIP(hipStreamSynchronize(s));
+      loff_h[nlow] = pos;
+      MNP_HIP(hipMemcpy(d_loff, loff_h.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_aca_store<TS>, dim3((unsigned)(o1 - o0)), dim3(256), 0, s, d_aca_order + o0, lblk,
+                         d_aca_soff, scratch, rank, kmax, d_loff, d_rowtile, d_tilebeg, (cplx<TS>*)pool);
+      MNP_HIP(hipGetLastError());  // a next batch reuses the scratch after this store, in stream order
     }
-    MNP_HIP(hipFree(d_soff));
+    for (int64_t b = 0; b < nlow; ++b) toff_h[b + 1] = toff_h[b] + rank_h[b] * (int64_t)MNP_HMV_MAXRHS;
   }
   MNP_HIP(hipMemcpy(d_toff, toff_h.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice));
   nlowvals = loff_h[nlow];
@@ -731,6 +778,89 @@ void HMat::fill_impl(double2 k, int key_) {
   tlen = toff_h[nlow];
   build_plan(loff_h, toff_h);
   filled = true;
+}
+
+// ACA work plan: admissible blocks with m + w <= 320 run one per wave (four
+// per workgroup), larger ones one per workgroup of 256 threads; work items
+// in decreasing estimated time (the largest start first, so that the launch
+// does not end on a few long blocks), grouped into batches whose FP64 scratch
+// (m + w) min(kmax, m, w) per block fits a budget of a quarter of the free
+// device memory (at least 8 GiB).
+void HMat::aca_plan() {
+  struct Item {
+    double cost;
+    int32_t b[4];
+  };
+  std::vector<Item> big, small;
+  std::vector<std::pair<double, int32_t>> sm;
+  std::vector<int64_t> sz(nlow);
+  for (int64_t b = 0; b < nlow; ++b) {
+    const int64_t m = lblk_h[4 * b + 1] - lblk_h[4 * b], w = lblk_h[4 * b + 3] - lblk_h[4 * b + 2];
+    const int64_t kb = std::min<int64_t>(kmax, std::min(m, w));
+    sz[b] = ((m + w) * kb + 31) / 32 * 32;
+    const bool wave = m + w <= 320;
+    const int G = wave ? 64 : 256;
+    MNP_CHECK(m <= 128 * (int64_t)G, MNP_ERR_ARG, "hmat: admissible block with more than 32768 rows");
+    // steps x (elements per thread x (kernel + residual loads) + reductions)
+    const double steps = (double)std::min<int64_t>(kb, 24);
+    const double cost = steps * ((double)((m + G - 1) / G + (w + G - 1) / G) * (1.0 + steps / 16.0) + 4.0);
+    if (wave)
+      sm.emplace_back(cost, (int32_t)b);
+    else
+      big.push_back({cost, {(int32_t)b, -2, -1, -1}});
+  }
+  std::stable_sort(sm.begin(), sm.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+  for (size_t i = 0; i < sm.size(); i += 4) {
+    Item it{sm[i].first, {-1, -1, -1, -1}};
+    for (size_t j = 0; j < 4 && i + j < sm.size(); ++j) it.b[j] = sm[i + j].second;
+    small.push_back(it);
+  }
+  std::vector<Item> items(big);
+  items.insert(items.end(), small.begin(), small.end());
+  std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
+  size_t fr = 0, tot = 0;
+  MNP_HIP(hipMemGetInfo(&fr, &tot));
+  const int64_t budget = std::max<int64_t>((int64_t)1 << 29, (int64_t)(fr / 4 / sizeof(double2)));
+  aca_items_h.clear();
+  aca_order_h.clear();
+  aca_icut.assign(1, 0);
+  aca_bcut.assign(1, 0);
+  aca_soff_h.assign(nlow, 0);
+  aca_maxbatch = 0;
+  int64_t acc = 0;
+  for (size_t i = 0; i < items.size(); ++i) {
+    int64_t isz = 0;
+    for (int j = 0; j < 4; ++j)
+      if (items[i].b[j] >= 0) isz += sz[items[i].b[j]];
+    if (acc + isz > budget && (int64_t)i > aca_icut.back()) {
+      aca_icut.push_back((int64_t)i);
+      aca_bcut.push_back((int64_t)aca_order_h.size());
+      acc = 0;
+    }
+    for (int j = 0; j < 4; ++j) {
+      const int32_t b = items[i].b[j];
+      aca_items_h.push_back(b);
+      if (b >= 0) {
+        aca_order_h.push_back(b);
+        aca_soff_h[b] = acc;
+        acc += sz[b];
+      }
+    }
+    aca_maxbatch = std::max(aca_maxbatch, acc);
+  }
+  aca_icut.push_back((int64_t)items.size());
+  aca_bcut.push_back((int64_t)aca_order_h.size());
+  for (void** p : {(void**)&d_aca_items, (void**)&d_aca_order, (void**)&d_aca_soff})
+    if (*p) {
+      MNP_HIP(hipFree(*p));
+      *p = nullptr;
+    }
+  d_aca_items = dev_alloc<int32_t>((int64_t)aca_items_h.size());
+  d_aca_order = dev_alloc<int32_t>((int64_t)aca_order_h.size());
+  d_aca_soff = dev_alloc<int64_t>(nlow);
+  MNP_HIP(hipMemcpy(d_aca_items, aca_items_h.data(), sizeof(int32_t) * aca_items_h.size(), hipMemcpyHostToDevice));
+  MNP_HIP(hipMemcpy(d_aca_order, aca_order_h.data(), sizeof(int32_t) * aca_order_h.size(), hipMemcpyHostToDevice));
+  MNP_HIP(hipMemcpy(d_aca_soff, aca_soff_h.data(), sizeof(int64_t) * nlow, hipMemcpyHostToDevice));
 }
 
 // Product plan for the ranks of the last fill: first-phase items and the

@@ -53,6 +53,7 @@ struct HMat {
 
  private:
   void build_plan(const std::vector<int64_t>& loff_h, const std::vector<int64_t>& toff_h);
+  void aca_plan();
   template <class TS>
   void fill_impl(double2 k, int key);
   template <class TS>
@@ -60,6 +61,14 @@ struct HMat {
   template <class TS>
   void coarsen_impl(double2* C, Workspace& ws) const;
   std::vector<int64_t> loff_host;  // low-rank factor offsets of the last fill
+  // ACA work plan (aca_plan, at the first fill): 4 block ids per workgroup
+  // item, blocks in work order, per-block scratch offsets relative to their
+  // batch, batch boundaries over items and over blocks
+  std::vector<int32_t> aca_items_h, aca_order_h;
+  std::vector<int64_t> aca_soff_h, aca_icut, aca_bcut;
+  int64_t aca_maxbatch = 0;
+  int32_t *d_aca_items = nullptr, *d_aca_order = nullptr;
+  int64_t* d_aca_soff = nullptr;
 };
 
 }  // namespace mnp
