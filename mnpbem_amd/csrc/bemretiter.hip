@@ -415,6 +415,7 @@ struct BemRetIter {
       MNP_HIP(hipGetLastError());
     }
     chain_build<float>(c, cnvec);
+    chain_explicit<float>(c);  // applied once per GMRES iteration: products, not solves
   }
 
   void initmat(double enei_, double2 e1, double2 e2) {

@@ -691,7 +691,7 @@ HMat::~HMat() {
   void* ps[] = {d_items, d_itpoff, d_multi, d_tdbeg, d_taoff, d_tboff, d_tlda, d_tK,
                 perm, fblk, lblk, d_voff, d_lrow, d_loffs, d_lentry, d_tilebeg, d_tileleaf, d_rowtile,
                 cpos, cnvec, carea, rank,
-                d_loff, d_toff, fvals, pool, d_aca_items, d_aca_order, d_aca_soff};
+                d_loff, d_toff, fvals, pool, d_aca_items, d_aca_order, d_aca_soff, d_dsel, d_dlb, d_droff};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -769,14 +769,14 @@ void HMat::fill_impl(double2 k, int key_) {
     }
     for (int64_t b = 0; b < nlow; ++b) toff_h[b + 1] = toff_h[b] + rank_h[b] * (int64_t)MNP_HMV_MAXRHS;
   }
-  MNP_HIP(hipMemcpy(d_toff, toff_h.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice));
+  MNP_HIP(hipMemcpyAsync(d_toff, toff_h.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice, s));
   nlowvals = loff_h[nlow];
   maxrank = 0;
   for (int32_t r : rank_h) maxrank = std::max(maxrank, (int)r);
   rank_host = rank_h;
   loff_host = loff_h;
   tlen = toff_h[nlow];
-  build_plan(loff_h, toff_h);
+  build_plan(toff_h);
   filled = true;
 }
 
@@ -863,18 +863,26 @@ void HMat::aca_plan() {
   MNP_HIP(hipMemcpy(d_aca_soff, aca_soff_h.data(), sizeof(int64_t) * nlow, hipMemcpyHostToDevice));
 }
 
-// Product plan for the ranks of the last fill: first-phase items and the
-// row tiles' descriptor lists (see k_hmv_low and k_hmv_tile).
-void HMat::build_plan(const std::vector<int64_t>& loff_h, const std::vector<int64_t>& toff_h) {
-  for (void** p : {(void**)&d_items, (void**)&d_itpoff, (void**)&d_multi, (void**)&d_tdbeg, (void**)&d_taoff,
-                   (void**)&d_tboff, (void**)&d_tlda, (void**)&d_tK})
-    if (*p) {
-      MNP_HIP(hipFree(*p));
-      *p = nullptr;
-    }
+// Product plan.  The structure (first-phase items, the row tiles'
+// descriptor lists, the full-block descriptors) is fixed by the block tree
+// and built once; the low-rank descriptors (factor offset, t offset, rank)
+// and the first-phase partial-sum offsets follow the ranks of each fill.
+// Low-rank descriptor q of the list: entry dsel[q] of the descriptor arrays,
+// block dlb[q], first row of its tile relative to the block droff[q].
+__global__ void k_plan_low(int64_t nq, const int32_t* dsel, const int32_t* dlb, const int32_t* droff,
+                           const int32_t* rank, const int64_t* loff, const int64_t* toff, int64_t* taoff,
+                           int64_t* tboff, int32_t* tK) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const int64_t d = dsel[q];
+  const int32_t lb = dlb[q], r = rank[lb];
+  taoff[d] = loff[lb] + (int64_t)droff[q] * r;
+  tboff[d] = -toff[lb] - 1;
+  tK[d] = r;
+}
+
+void HMat::build_plan_static() {
   std::vector<int32_t> items, multi;
-  std::vector<int64_t> itpoff;
-  int64_t tp = 0;
   for (int64_t b = 0; b < nlow; ++b) {
     const int32_t c0 = lblk_h[4 * b + 2], w = lblk_h[4 * b + 3] - c0;
     const int32_t ni = (w + HMV_CHUNK - 1) / HMV_CHUNK;
@@ -882,14 +890,12 @@ void HMat::build_plan(const std::vector<int64_t>& loff_h, const std::vector<int6
     for (int32_t c = 0; c < ni; ++c) {
       const int32_t j0 = c * w / ni, j1 = (c + 1) * w / ni;
       items.insert(items.end(), {(int32_t)b, j0, j1, ni > 1 ? (int32_t)c : -1});
-      itpoff.push_back(ni > 1 ? tp : -1);
-      if (ni > 1) tp += (int64_t)rank_host[b] * MNP_HMV_MAXRHS;
     }
   }
   nitems = (int64_t)items.size() / 4;
   nmulti = (int64_t)multi.size() / 3;
-  tpartlen = tp;
-  std::vector<int32_t> tdbeg{0}, tlda, tK;
+  items_h = items;
+  std::vector<int32_t> tdbeg{0}, tlda, tK, dsel, dlb, droff;
   std::vector<int64_t> taoff, tboff;
   for (int64_t t = 0; t < ntile; ++t) {
     const int32_t tb = tilebeg_h[t], mt = tilebeg_h[t + 1] - tb, leaf = tileleaf_h[t];
@@ -902,29 +908,58 @@ void HMat::build_plan(const std::vector<int64_t>& loff_h, const std::vector<int6
         tlda.push_back(bk[1] - bk[0]);
         tK.push_back(bk[3] - bk[2]);
       } else {
-        const int64_t lb = -(int64_t)ent - 1;
-        const int32_t r = rank_host[lb];
-        taoff.push_back(loff_h[lb] + (int64_t)(tb - lblk_h[4 * lb]) * r);
-        tboff.push_back(-toff_h[lb] - 1);
+        const int32_t lb = -ent - 1;
+        dsel.push_back((int32_t)taoff.size());
+        dlb.push_back(lb);
+        droff.push_back(tb - lblk_h[4 * lb]);
+        taoff.push_back(0);  // set per fill by k_plan_low
+        tboff.push_back(0);
         tlda.push_back(mt);
-        tK.push_back(r);
+        tK.push_back(0);
       }
     }
     tdbeg.push_back((int32_t)taoff.size());
   }
+  nlowdesc = (int64_t)dsel.size();
   auto up = [&](auto& vec, auto*& dptr) {
     using E = typename std::remove_reference<decltype(vec)>::type::value_type;
     dptr = dev_alloc<E>((int64_t)vec.size());
     if (!vec.empty()) MNP_HIP(hipMemcpy(dptr, vec.data(), sizeof(E) * vec.size(), hipMemcpyHostToDevice));
   };
   up(items, d_items);
-  up(itpoff, d_itpoff);
   up(multi, d_multi);
   up(tdbeg, d_tdbeg);
   up(taoff, d_taoff);
   up(tboff, d_tboff);
   up(tlda, d_tlda);
   up(tK, d_tK);
+  up(dsel, d_dsel);
+  up(dlb, d_dlb);
+  up(droff, d_droff);
+  d_itpoff = dev_alloc<int64_t>(std::max<int64_t>(nitems, 1));
+  itpoff_h.assign(nitems, -1);
+}
+
+// Rank-dependent part of the plan, on the stream after the fill's uploads
+// of loff and toff.
+void HMat::build_plan(const std::vector<int64_t>& toff_h) {
+  (void)toff_h;
+  if (!d_tdbeg) build_plan_static();
+  int64_t tp = 0;
+  for (int64_t i = 0; i < nitems; ++i) {
+    if (items_h[4 * i + 3] < 0) continue;
+    itpoff_h[i] = tp;
+    tp += (int64_t)rank_host[items_h[4 * i]] * MNP_HMV_MAXRHS;
+  }
+  tpartlen = tp;
+  hipStream_t s = stream();
+  if (nitems)
+    MNP_HIP(hipMemcpyAsync(d_itpoff, itpoff_h.data(), sizeof(int64_t) * nitems, hipMemcpyHostToDevice, s));
+  if (nlowdesc)
+    hipLaunchKernelGGL(k_plan_low, dim3((unsigned)cdiv_up(nlowdesc, 256)), dim3(256), 0, s, nlowdesc, d_dsel, d_dlb,
+                       d_droff, rank, d_loff, d_toff, d_taoff, d_tboff, d_tK);
+  MNP_HIP(hipGetLastError());
+  MNP_HIP(hipStreamSynchronize(s));  // itpoff_h stays valid until the copy is done
 }
 
 void HMat::matvec(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const {

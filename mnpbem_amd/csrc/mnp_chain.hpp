@@ -46,6 +46,10 @@ struct Chain {
   int fallbacks = 0;           // factorisations redone with partial pivoting
   bool ok_g1 = false, ok_g2 = false, ok_d = false;  // chain_sector_try's progress
   bool sigma_formed = false, ok_s = false;          // chain_build_try's
+  // chain_explicit: G1^{-1}, G2^{-1} and Sigma^{-1} as matrices (over
+  // backup, backup2 and D), so that chain_apply multiplies instead of
+  // solving; cleared by every rebuild
+  bool explicit_ok = false;
   double2 eps1{}, eps2{};
   double k0 = 0;
   Workspace ws, ws2;  // ws2: the G2 branch
@@ -165,6 +169,7 @@ template <class T>
 bool chain_sector_try(Chain<T>& c) {
   const int64_t n = c.n;
   c.fallbacks = 0;
+  c.explicit_ok = false;
   c.ok_g1 = c.ok_g2 = c.ok_d = false;
   c.ok_g1 = c.factor_try(0, c.G1, c.backup, c.piv, c.ws);
   if (c.ok_g1) getrs_right<T>(n, n, c.f[0], c.v(c.S1), c.ws);  // Sigma1 = H1 G1^{-1}
@@ -235,6 +240,19 @@ void chain_build_finish(Chain<T>& c, const double* nvec) {
 template <class T>
 void chain_build(Chain<T>& c, const double* nvec) {
   if (!chain_build_try(c, nvec)) chain_build_finish(c, nvec);
+}
+// After chain_build: the inverses of G1, G2 and Sigma as matrices, for a
+// chain applied many times to few vectors (a preconditioner), where the
+// triangular solves of chain_apply would be dozens of small dependent
+// launches each.  Overwrites the factorisation of Sigma1 - Sigma2 (D, no
+// longer needed once Deltai is formed) and the backups.
+template <class T>
+void chain_explicit(Chain<T>& c) {
+  MNP_CHECK(c.ok_s && c.G1 && c.D, MNP_ERR_STATE, "chain_explicit: chain not built");
+  getri<T>(c.n, c.f[0], c.v(c.backup), c.ws);
+  getri<T>(c.n, c.f[1], c.v(c.backup2), c.ws);
+  getri<T>(c.n, c.f[3], c.v(c.D), c.ws);
+  c.explicit_ok = true;
 }
 template <class T>
 bool chain_build_try(Chain<T>& c, const GreenRet& g) { return chain_build_try(c, g.geo.nvec); }
@@ -346,6 +364,19 @@ void chain_apply(Chain<T>& c, const double* nvec, int64_t p, const cplx<T>* b, c
   const dim3 gp((unsigned)cdiv_up(n, 256), (unsigned)p);
   hipLaunchKernelGGL(k_mld1<T>, gp, dim3(256), 0, stream(), n, p, b, Y, alpha, De, nvec, k, L1);
   gemm<T>(n, 3 * p, n, one, c.v(c.Di), MatView<T>{alpha, n}, zero, MatView<T>{t, n});
+  if (c.explicit_ok) {
+    // Y is free after k_mld1, alpha | De (4p columns) after k_mld3
+    hipLaunchKernelGGL(k_mld2<T>, gp, dim3(256), 0, stream(), n, p, De, t, Y, nvec, ikdL);
+    gemm<T>(n, p, n, one, c.v(c.D), MatView<T>{Y, n}, zero, MatView<T>{s2, n});  // Sigma^{-1} rhs
+    hipLaunchKernelGGL(k_mld3<T>, gp, dim3(256), 0, stream(), n, p, s2, alpha, u, nvec, ikdL);
+    gemm<T>(n, 3 * p, n, one, c.v(c.Di), MatView<T>{u, n}, zero, MatView<T>{h2, n});
+    hipLaunchKernelGGL(k_mld4<T>, dim3((unsigned)cdiv_up(n, 256), (unsigned)(4 * p)), dim3(256), 0,
+                       stream(), n, p, b, s2, h2, Y, alpha);
+    MNP_HIP(hipGetLastError());
+    gemm<T>(n, 4 * p, n, one, c.v(c.backup), MatView<T>{Y, n}, zero, MatView<T>{x, n});
+    gemm<T>(n, 4 * p, n, one, c.v(c.backup2), MatView<T>{alpha, n}, zero, MatView<T>{x + 4 * p * n, n});
+    return;
+  }
   hipLaunchKernelGGL(k_mld2<T>, gp, dim3(256), 0, stream(), n, p, De, t, s2, nvec, ikdL);
   getrs<T>(n, p, c.f[3], MatView<T>{s2, n}, c.ws);  // sig2 = Sigma^{-1} rhs
   hipLaunchKernelGGL(k_mld3<T>, gp, dim3(256), 0, stream(), n, p, s2, alpha, u, nvec, ikdL);

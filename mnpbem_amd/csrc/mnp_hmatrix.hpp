@@ -33,6 +33,10 @@ struct HMat {
   int64_t nitems = 0, nmulti = 0, tpartlen = 0;
   int32_t *d_items = nullptr, *d_multi = nullptr, *d_tdbeg = nullptr, *d_tlda = nullptr, *d_tK = nullptr;
   int64_t *d_itpoff = nullptr, *d_taoff = nullptr, *d_tboff = nullptr;
+  int32_t *d_dsel = nullptr, *d_dlb = nullptr, *d_droff = nullptr;  // low-rank descriptors (k_plan_low)
+  int64_t nlowdesc = 0;
+  std::vector<int32_t> items_h;
+  std::vector<int64_t> itpoff_h;
   void* fvals = nullptr;  // full blocks, column-major per block, complex (prec)
   void* pool = nullptr;   // per low-rank block: L (m x r, row-tile segments) then R^T (r x n), complex (prec)
 
@@ -52,7 +56,8 @@ struct HMat {
   void coarsen(double2* C, Workspace& ws) const;
 
  private:
-  void build_plan(const std::vector<int64_t>& loff_h, const std::vector<int64_t>& toff_h);
+  void build_plan_static();
+  void build_plan(const std::vector<int64_t>& toff_h);
   void aca_plan();
   template <class TS>
   void fill_impl(double2 k, int key);
