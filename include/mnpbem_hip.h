@@ -563,6 +563,16 @@ int mnp_dipole_potential(mnp_greenret_t g, double enei, mnp_complex eps_medium, 
 int mnp_field_points(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t npt,
                      const double* pts, int64_t ncol, const void* sig, const void* h, void* e,
                      void* hfield);
+/* Potentials at points from the surface charges sig (device n x ncol) and
+ * currents h (device n x 3 x ncol, or NULL) of the particle side facing the
+ * points: phi = G sig, device out npt x ncol, and a = G h, device out
+ * npt x 3 x ncol (zero when h is NULL), with the wavenumber of eps_medium.
+ * Replaces @compgreenret/potential.m:30-40 with compgreenret(pt, p) (its
+ * phi and a; points carry no normals, so no surface derivatives); the same
+ * near-field refinement as mnp_field_points. */
+int mnp_potential_points(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t npt,
+                         const double* pts, int64_t ncol, const void* sig, const void* h, void* phi,
+                         void* a);
 /* Total and radiative decay rates in units of the free-space rate and the
  * free-space rate nb 4/3 k0^3 (host out npt x ndip) for dipoles outside the
  * particle, from sig2/h2 of the solve (device n x npol, n x 3 x npol):

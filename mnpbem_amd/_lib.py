@@ -121,6 +121,7 @@ PROTOTYPES = {
     "mnp_planewavestat_cross_sections": (INT, [P, D, MnpComplex, I64, P, P, P, P, P]),
     "mnp_dipole_potential": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P, P]),
     "mnp_field_points": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P]),
+    "mnp_potential_points": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P]),
     "mnp_dipole_decayrate": (INT, [P, D, MnpComplex, I64, P, I64, P, I64, P, P, P, P, P, P, P]),
     "mnp_planewave_potential": (INT, [P, D, MnpComplex, I64, P, P, P, P]),
     "mnp_farfield": (INT, [P, D, MnpComplex, I64, P, I64, P, P, P, P]),

@@ -81,6 +81,9 @@ __global__ void __launch_bounds__(256) k_dip_potential(int64_t n, const double* 
 // ------------------------------------------------------- fields at points
 // One workgroup per (point, column); diag != 0 evaluates column q at point
 // q % npt only (grid.x = 1) and writes e/hf as 3 x ncol.
+// POT: the potentials phi = G sig (into e, npt x ncol) and A = G h (into hf,
+// npt x 3 x ncol) instead of the fields (compgreenret/potential.m:30-40).
+template <bool POT>
 __global__ void __launch_bounds__(256) k_pt_field(int64_t n, Geo geo, int64_t npt, const double* __restrict__ pts,
                                                   int64_t ncol, const double2* __restrict__ sig,
                                                   const double2* __restrict__ h, double2 k, double k0, int order,
@@ -123,6 +126,17 @@ __global__ void __launch_bounds__(256) k_pt_field(int64_t n, Geo geo, int64_t np
     double2 hj[3] = {{0, 0}, {0, 0}, {0, 0}};  // h == nullptr: no currents
     if (h)
       for (int c = 0; c < 3; ++c) hj[c] = h[(c + 3 * q) * n + j];
+    if constexpr (POT) {
+      const double2 v = cmul<double>(G, s);
+      acc[0] += v.x;
+      acc[1] += v.y;
+      for (int c = 0; c < 3; ++c) {
+        const double2 a = cmul<double>(G, hj[c]);
+        acc[2 + 2 * c] += a.x;
+        acc[3 + 2 * c] += a.y;
+      }
+      continue;
+    }
     const double2 ik0G = cmul<double>(mk<double>(0, k0), G);
     for (int c = 0; c < 3; ++c) {
       const double2 v = csub<double>(cmul<double>(ik0G, hj[c]), cmul<double>(Gp[c], s));
@@ -146,7 +160,18 @@ __global__ void __launch_bounds__(256) k_pt_field(int64_t n, Geo geo, int64_t np
   if ((threadIdx.x & 63) == 0)
     for (int t = 0; t < 12; ++t) red[w][t] = acc[t];
   __syncthreads();
-  if (threadIdx.x < 12) {
+  if (POT && threadIdx.x < 8) {
+    const int t = threadIdx.x;
+    const double v = red[0][t] + red[1][t] + red[2][t] + red[3][t];
+    // t 0, 1: phi; t 2..7: A component (t - 2) / 2
+    double2* out = t < 2 ? e : hf;
+    const int64_t idx = t < 2 ? ipt + npt * q : ipt + npt * ((t - 2) / 2 + 3 * q);
+    if ((t & 1) == 0)
+      out[idx].x = v;
+    else
+      out[idx].y = v;
+  }
+  if (!POT && threadIdx.x < 12) {
     const int t = threadIdx.x;
     const double v = red[0][t] + red[1][t] + red[2][t] + red[3][t];
     const int c = (t % 6) / 2, re = t & 1;
@@ -206,7 +231,7 @@ static void pt_field(const GreenRet& G, double enei, mnp_complex eps, int64_t np
                      const double2* sig, const double2* h, int diag, double2* e, double2* hf) {
   upload_rule18();
   const dim3 grid(diag ? 1u : (unsigned)npt, (unsigned)ncol);
-  hipLaunchKernelGGL(k_pt_field, grid, dim3(256), 0, stream(), G.n, G.geo, npt, dpts, ncol, sig, h,
+  hipLaunchKernelGGL(k_pt_field<false>, grid, dim3(256), 0, stream(), G.n, G.geo, npt, dpts, ncol, sig, h,
                      wavenumber(enei, eps), 2 * M_PI / enei, G.order, G.op.rel_cutoff, G.op.abs_cutoff, diag, e,
                      hf);
   MNP_HIP(hipGetLastError());
@@ -257,6 +282,24 @@ int mnp_field_points(mnp_greenret_t g, double enei, mnp_complex eps_medium, int6
     double* dpts = upload(pts, 3 * npt);
     pt_field(G, enei, eps_medium, npt, dpts, ncol, (const double2*)sig, (const double2*)h, 0, (double2*)e,
              (double2*)hfield);
+    MNP_HIP(hipStreamSynchronize(stream()));
+    MNP_HIP(hipFree(dpts));
+  });
+}
+
+int mnp_potential_points(mnp_greenret_t g, double enei, mnp_complex eps_medium, int64_t npt, const double* pts,
+                         int64_t ncol, const void* sig, const void* h, void* phi, void* a) {
+  return guarded([&] {
+    MNP_CHECK(g && pts && sig && phi && a, MNP_ERR_ARG, "potential_points: NULL argument");
+    MNP_CHECK(npt >= 1 && ncol >= 1 && enei > 0, MNP_ERR_ARG, "potential_points: bad sizes");
+    MNP_CHECK(npt < (1LL << 31) && ncol < 65536, MNP_ERR_ARG, "potential_points: too many points or columns");
+    const GreenRet& G = *g->g;
+    double* dpts = upload(pts, 3 * npt);
+    upload_rule18();
+    hipLaunchKernelGGL(k_pt_field<true>, dim3((unsigned)npt, (unsigned)ncol), dim3(256), 0, stream(), G.n, G.geo,
+                       npt, dpts, ncol, (const double2*)sig, (const double2*)h, wavenumber(enei, eps_medium),
+                       2 * M_PI / enei, G.order, G.op.rel_cutoff, G.op.abs_cutoff, 0, (double2*)phi, (double2*)a);
+    MNP_HIP(hipGetLastError());
     MNP_HIP(hipStreamSynchronize(stream()));
     MNP_HIP(hipFree(dpts));
   });
