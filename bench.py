@@ -389,66 +389,91 @@ def run_sensitivity(nsweep=32):
             "vertex_gradient_norm_650nm": float(np.linalg.norm(dv))}
 
 
-def run_multi_particle(per_particle=8):
+def run_multi_particle(per_particle=8, fp64_every=4):
     """The seeded many-particle sweep (12 Drude particles, 320-5,120 faces, in
-    vacuum): per_particle of its 128 wavelengths (every 16th) per particle
-    through each solver; mesh set-up (geometry, refinement tables) is timed
-    apart from the solves."""
+    vacuum, 128 wavelengths 400-900 nm each): all 1,536 particle-wavelength
+    solves through the batched path (mnpbem_amd.batch.spectra: per face count
+    the batched pipeline, captured as a hipGraph, with the variant tuned
+    beforehand on one wavelength per particle at the same batch sizes; the
+    tuning time is reported apart) in mixed precision; every fp64_every-th
+    wavelength in FP64; and per_particle
+    wavelengths per particle through the dense solver, one solve at a time,
+    as the reference's loop runs them (the comparison and the parity check)."""
     import torch
 
     from mnpbem_amd import bem as B
+    from mnpbem_amd.batch import plan_capacity, spectra
     from mnpbem_amd.workloads import multi_particle_sweep
 
     parts, enei = multi_particle_sweep()
-    lams = enei[:: len(enei) // per_particle][:per_particle]
     exc = B.planewave([[1.0, 0, 0]], [[0, 0, 1.0]])
-    out = {"workload": f"{len(parts)} seeded Drude ellipsoids (faces {sorted(len(p['faces']) for p in parts)}), "
-                       f"eps 1, pol x dir z, {per_particle} of 128 wavelengths 400-900 nm each",
-           "faces_total": int(sum(len(p["faces"]) for p in parts))}
-    res, setup = {}, 0.0
-    for prec in ("mixed", "mixed_bf16", "fp64"):
-        tot, ext = 0.0, []
-        for part in parts:
-            op = B.bemoptions(sim="ret")
-            t0 = time.perf_counter()
-            p = B.comparticle([B.epsconst(1.0), B.epsdrude(*part["drude"])], [(part["verts"], part["faces"])],
-                              [2, 1], 1, op)
-            p.green
-            bem = B.bemsolver(p, op, precision=prec)
-            torch.cuda.synchronize()
-            setup += time.perf_counter() - t0
-            t = time.perf_counter()
-            ext.append([exc.ext(bem.solve(exc(p, lam)))[0] for lam in lams])
-            torch.cuda.synchronize()
-            tot += time.perf_counter() - t
-            del bem, p
-        res[prec] = (len(parts) * len(lams) / tot, np.array(ext))
-        out[f"particle_wavelengths_per_s_{prec}"] = res[prec][0]
-        # the same sweep with the particles' initialisations batched four at a time on concurrent streams
-        ps, bems = [], []
-        for part in parts:
-            op = B.bemoptions(sim="ret")
-            ps.append(B.comparticle([B.epsconst(1.0), B.epsdrude(*part["drude"])],
-                                    [(part["verts"], part["faces"])], [2, 1], 1, op))
-            ps[-1].green
-            bems.append(B.bemsolver(ps[-1], op, precision=prec))
+    ps = [B.comparticle([B.epsconst(1.0), B.epsdrude(*q["drude"])], [(q["verts"], q["faces"])], [2, 1], 1)
+          for q in parts]
+    t0 = time.perf_counter()
+    for p in ps:
+        p.green
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    out = {"workload": f"{len(parts)} seeded Drude ellipsoids (faces {sorted(p.n for p in ps)}), eps 1, pol x dir z, "
+                       f"128 wavelengths 400-900 nm each",
+           "faces_total": int(sum(p.n for p in ps)), "setup_s_per_particle": setup / len(ps)}
+    full = {}
+    for prec, lams in (("mixed", enei), ("fp64", enei[::fp64_every])):
+        # one wavelength per particle at the timed run's batch sizes: tunes the
+        # variant of each face count (the tuner keeps it for the process)
+        caps = plan_capacity(ps, len(lams), prec)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        extb = np.zeros((len(parts), len(lams)))
-        for j, lam in enumerate(lams):
-            B.initmat_many(bems, lam)
-            for i, (p, bem) in enumerate(zip(ps, bems)):
-                extb[i, j] = exc.ext(bem.solve(exc(p, lam)))[0]
+        tune = {}
+        spectra(ps, enei[:1], exc, precision=prec, capacity=caps, stats=tune)
         torch.cuda.synchronize()
-        tb = time.perf_counter() - t
-        del bems, ps
-        out[f"particle_wavelengths_per_s_{prec}_batched"] = len(parts) * len(lams) / tb
-        out[f"max_rel_diff_batched_vs_sequential_{prec}"] = float(np.max(np.abs(extb / res[prec][1] - 1)))
-        log(f"[bench] multi-particle {prec}: {res[prec][0]:.1f} particle-wavelengths/s, "
-            f"{len(parts) * len(lams) / tb:.1f} with batched initialisation")
-    for prec in ("mixed", "mixed_bf16"):
-        out[f"max_rel_err_{prec}_vs_fp64"] = float(np.max(np.abs(res[prec][1] / res["fp64"][1] - 1)))
-    out["setup_s_per_particle"] = setup / (3 * len(parts))
+        t_tune = time.perf_counter() - t
+        stats = {}
+        t = time.perf_counter()
+        ext, sca, _ = spectra(ps, lams, exc, precision=prec, capacity=caps, stats=stats)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        full[prec] = (lams, ext[..., 0])
+        out[f"batched_{prec}"] = {
+            "particle_wavelengths": len(ps) * len(lams), "seconds": dt, "tuning_seconds": t_tune,
+            "particle_wavelengths_per_s": len(ps) * len(lams) / dt,
+            "per_face_count": {str(n): st for n, st in stats.items()},
+            "tuner_ms_per_variant": {str(n): st.get("variant_ms") for n, st in tune.items()}}
+        for st in stats.values():
+            st.pop("variant_ms", None)
+        log(f"[bench] multi-particle batched {prec}: {len(ps) * len(lams)} solves in {dt:.2f} s "
+            f"({len(ps) * len(lams) / dt:.1f} particle-wavelengths/s); "
+            + "; ".join(f"{n} faces: {'dense' if st.get('dense') else 'variant %s x%s' % (st.get('variant'), st.get('capacity'))}"
+                        for n, st in stats.items()))
+    # the dense solver one solve at a time, per_particle wavelengths per particle
+    sub = np.arange(0, len(enei), len(enei) // per_particle)[:per_particle]
+    seq = {}
+    for prec in ("mixed", "fp64"):
+        tot, ext, per_n = 0.0, [], {}
+        for p in ps:
+            bem = B.bemsolver(p, B.bemoptions(sim="ret"), precision=prec)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            ext.append([exc.ext(bem.solve(exc(p, lam)))[0] for lam in enei[sub]])
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            tot += dt
+            per_n.setdefault(str(p.n), []).append(dt / len(sub))
+            del bem
+        out[f"s_per_solve_dense_{prec}"] = {n: float(np.mean(v)) for n, v in per_n.items()}
+        seq[prec] = np.array(ext)
+        out[f"particle_wavelengths_per_s_{prec}_dense_sequential"] = len(ps) * len(sub) / tot
+        log(f"[bench] multi-particle dense one at a time {prec}: {len(ps) * len(sub) / tot:.1f} particle-wavelengths/s")
+    out["speedup_batched_vs_dense_sequential_mixed"] = (out["batched_mixed"]["particle_wavelengths_per_s"] /
+                                                        out["particle_wavelengths_per_s_mixed_dense_sequential"])
+    out["speedup_batched_vs_dense_sequential_fp64"] = (out["batched_fp64"]["particle_wavelengths_per_s"] /
+                                                       out["particle_wavelengths_per_s_fp64_dense_sequential"])
+    # parity on the wavelengths both runs share
+    ref = seq["fp64"]
+    out["max_rel_err_batched_mixed_vs_dense_fp64"] = float(np.max(np.abs(full["mixed"][1][:, sub] / ref - 1)))
+    assert np.all(sub % fp64_every == 0)  # the FP64 sweep holds every compared wavelength
+    out["max_rel_err_batched_fp64_vs_dense_fp64"] = float(np.max(np.abs(full["fp64"][1][:, sub // fp64_every] / ref - 1)))
+    out["max_rel_err_dense_mixed_vs_dense_fp64"] = float(np.max(np.abs(seq["mixed"] / ref - 1)))
     return out
 
 

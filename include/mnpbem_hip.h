@@ -186,6 +186,52 @@ int mnp_bemret_set_refinement(mnp_bemret_t bem, double tol, int maxit);
  * per-particle subsref '()' of BEM/@bemret/subsref.m:23-26. */
 int mnp_bemret_initmat_many(int count, const mnp_bemret_t* bems, const double* enei,
                             const mnp_complex* eps1, const mnp_complex* eps2);
+/* ------------------------------------------------- batched dense solves
+ * Many (particle, wavelength) members of one face count solved together:
+ * plane-wave cross sections of each member (the spectra loops of
+ * Demo/planewave/ret/demospecret1.m:34-42 over bemsolver(p, op) \
+ * planewave(...)(p, enei) and ext/sca/abs, run for many particles).  Every
+ * stage handles all members in one launch per step (assembly of
+ * Greenfun/@greenret/private/eval1.m, the chain of BEM/@bemret/private/
+ * initmat.m:5-72 by blocked LU and triangular solves without row exchanges,
+ * mldivide.m:11-49, and refinement sweeps with the exact FP64 operator of
+ * BEM/@bemretiter/private/afun.m:6-35), captured once as a hipGraph and
+ * replayed per batch.  capacity: members per launch; npol <= 8 plane waves
+ * in the outside medium; npinf: size of the pinfty surface for the
+ * scattering cross section (0: sca not computed).  solver: MNP_SOLVER_FP64
+ * (FP64 chain, 1 sweep) or MNP_SOLVER_MIXED (complex FP32 chain, 3 sweeps).
+ * No counterpart in the reference. */
+typedef struct mnp_bemret_batch_s* mnp_bemret_batch_t;
+int mnp_bemret_batch_create(int64_t nfaces, int64_t capacity, int solver, int64_t npol,
+                            int64_t npinf, mnp_bemret_batch_t* out);
+int mnp_bemret_batch_destroy(mnp_bemret_batch_t batch);
+/* use_graph: capture and replay (1, default) or launch directly (0);
+ * variant: panel width and product tile, 0: 32 / 64 x 64, 1: 128 / 64 x 64,
+ * 2: 128 / 128 x 64, 3: 128 / 128 x 96 (3-multiplication form, MIXED only),
+ * 4: 256 / 128 x 64, 5: 256 / 128 x 96 (MIXED only); -1 (default): the
+ * fastest, each candidate timed once per (faces, capacity, solver); sweeps:
+ * refinement sweeps (-1: the solver's default). */
+int mnp_bemret_batch_set_options(mnp_bemret_batch_t batch, int use_graph, int variant, int sweeps);
+/* Cross sections of count <= capacity members: member m has the Green
+ * function greens[m] (nfaces faces), wavelength enei[m], eps_in[m] inside and
+ * eps_out[m] outside (the medium of the plane waves); pol, dir host 3 x npol
+ * (column-major, pol perpendicular to dir) shared by all members; pinfty
+ * normals and areas on the device (npinf x 3, npinf).  Out (host): ext, sca,
+ * abs count x npol (member fastest in npol), status[m] (0 solved; 1 zero
+ * pivot; 3 relative residual above 1e-12 after the sweeps); relres[m] (may
+ * be NULL).  A member with status != 0 has no valid cross sections: solve it
+ * with the dense solver. */
+int mnp_bemret_batch_cross_sections(mnp_bemret_batch_t batch, int64_t count,
+                                    const mnp_greenret_t* greens, const double* enei,
+                                    const mnp_complex* eps_in, const mnp_complex* eps_out,
+                                    const double* pol, const double* dir, const void* pinfty_nvec,
+                                    const void* pinfty_area, double* ext, double* sca, double* abs,
+                                    int* status, double* relres);
+/* The variant in use, the tuner's timing of the pipeline per variant (ms, 6
+ * entries; 0 when not timed) and the captured graph's node count. */
+int mnp_bemret_batch_info(mnp_bemret_batch_t batch, int* variant, double* variant_ms,
+                          int64_t* graph_nodes);
+
 /* Sensitivity of a solution to the inside dielectric function (no counterpart
  * in the reference; it differentiates the operator that bemretiter/private/
  * afun.m:6-35 applies and initmat.m:5-72 inverts).  Given the solution

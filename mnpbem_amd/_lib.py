@@ -65,6 +65,11 @@ PROTOTYPES = {
     "mnp_bemret_tangent_eps1": (INT, [P, I64, P, P, P, P, P, P, P, P]),
     "mnp_bemret_solve_transposed": (INT, [P, I64, P, P]),
     "mnp_bemret_extinction_gradient": (INT, [P, I64, P, P, P, P, P, P, P, P, P]),
+    "mnp_bemret_batch_create": (INT, [I64, I64, INT, I64, I64, C.POINTER(P)]),
+    "mnp_bemret_batch_destroy": (INT, [P]),
+    "mnp_bemret_batch_set_options": (INT, [P, INT, INT, INT]),
+    "mnp_bemret_batch_cross_sections": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mnp_bemret_batch_info": (INT, [P, C.POINTER(INT), P, PI64]),
     "mnp_bemret_initmat_many": (INT, [INT, C.POINTER(P), PD, C.POINTER(MnpComplex), C.POINTER(MnpComplex)]),
     "mnp_bemret_stats": (INT, [P, C.POINTER(INT), PD]),
     "mnp_bemret_set_refinement": (INT, [P, D, INT]),

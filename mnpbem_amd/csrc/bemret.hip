@@ -487,6 +487,41 @@ __global__ void __launch_bounds__(256) k_farfield(int64_t n, int64_t p, int64_t 
   }
 }
 
+// Cross sections from the far fields of the npol propagation directions
+// (first npol of the nd directions) and of the npinf pinfty normals pn with
+// areas pa (he, hh: nd x 3 x npol), in FP64:
+//   ext = 4 pi / Re k Im(pol . e(dir_q))            (planewaveret/extinction.m:12-16)
+//   sca = sum area 0.5 Re(n . (e x conj h)) / (0.5 nb)  (scattering.m:20-29)
+// and abs = ext - sca; any output may be NULL.
+void farfield_host_sums(int64_t npol, const double* pol, int64_t nd, int64_t npinf, const double2* he,
+                        const double2* hh, const double* pn, const double* pa, double2 k, double nb, double* ext,
+                        double* sca, double* abs_) {
+  for (int64_t q = 0; q < npol; ++q) {
+    double2 pe = mk<double>(0, 0);
+    for (int c = 0; c < 3; ++c) pe = cadd<double>(pe, cscale<double>(he[q + nd * (c + 3 * q)], pol[c * npol + q]));
+    const double e_val = 4 * M_PI / k.x * pe.y;
+    double s = 0;
+    for (int64_t d = 0; d < npinf; ++d) {
+      const int64_t dd = npol + d;
+      double2 E[3], H[3];
+      for (int c = 0; c < 3; ++c) {
+        E[c] = he[dd + nd * (c + 3 * q)];
+        H[c] = hh[dd + nd * (c + 3 * q)];
+        H[c].y = -H[c].y;
+      }
+      const double2 x = csub<double>(cmul<double>(E[1], H[2]), cmul<double>(E[2], H[1]));
+      const double2 y = csub<double>(cmul<double>(E[2], H[0]), cmul<double>(E[0], H[2]));
+      const double2 z = csub<double>(cmul<double>(E[0], H[1]), cmul<double>(E[1], H[0]));
+      const double ds = 0.5 * (pn[d] * x.x + pn[npinf + d] * y.x + pn[2 * npinf + d] * z.x);
+      s += pa[d] * ds;
+    }
+    const double s_val = s / (0.5 * nb);
+    if (ext) ext[q] = e_val;
+    if (sca) sca[q] = s_val;
+    if (abs_) abs_[q] = e_val - s_val;
+  }
+}
+
 static double2 medium_k(double enei, mnp_complex em) {
   const double k0 = 2 * M_PI / enei;
   const double r = std::hypot(em.re, em.im);
@@ -749,32 +784,7 @@ int mnp_planewave_cross_sections(mnp_greenret_t g, double enei, mnp_complex eps_
     MNP_HIP(hipFree(hf));
     const double2 k = medium_k(enei, eps_medium);
     const double nb = std::sqrt(std::hypot(eps_medium.re, eps_medium.im));  // |sqrt(eps)| (lossless medium)
-    for (int64_t q = 0; q < npol; ++q) {
-      // extinction: 4 pi / k Im(pol . e(dir_q))  (planewaveret/extinction.m:12-16)
-      double2 pe = mk<double>(0, 0);
-      for (int c = 0; c < 3; ++c) pe = cadd<double>(pe, cscale<double>(he[q + nd * (c + 3 * q)], pol[c * npol + q]));
-      const double e_val = 4 * M_PI / k.x * pe.y;
-      // scattering: sum area 0.5 Re(n . (e x conj h)) / (0.5 nb)  (scattering.m:20-29)
-      double s = 0;
-      for (int64_t d = 0; d < npinf; ++d) {
-        const int64_t dd = npol + d;
-        double2 E[3], H[3];
-        for (int c = 0; c < 3; ++c) {
-          E[c] = he[dd + nd * (c + 3 * q)];
-          H[c] = hh[dd + nd * (c + 3 * q)];
-          H[c].y = -H[c].y;
-        }
-        const double2 x = csub<double>(cmul<double>(E[1], H[2]), cmul<double>(E[2], H[1]));
-        const double2 y = csub<double>(cmul<double>(E[2], H[0]), cmul<double>(E[0], H[2]));
-        const double2 z = csub<double>(cmul<double>(E[0], H[1]), cmul<double>(E[1], H[0]));
-        const double ds = 0.5 * (pn[d] * x.x + pn[npinf + d] * y.x + pn[2 * npinf + d] * z.x);
-        s += pa[d] * ds;
-      }
-      const double s_val = s / (0.5 * nb);
-      if (ext) ext[q] = e_val;
-      if (sca) sca[q] = s_val;
-      if (abs_) abs_[q] = e_val - s_val;
-    }
+    farfield_host_sums(npol, pol, nd, npinf, he.data(), hh.data(), pn.data(), pa.data(), k, nb, ext, sca, abs_);
   });
 }
 

@@ -30,6 +30,8 @@ static hipStream_t g_stream = nullptr;
 static thread_local hipStream_t t_stream = nullptr;  // a worker's own stream
 static thread_local bool t_in_worker = false;
 hipStream_t stream() { return t_stream ? t_stream : g_stream; }
+hipStream_t thread_stream() { return t_stream; }
+void set_thread_stream(hipStream_t s) { t_stream = s; }
 
 // ------------------------------------------------------------------ timers
 namespace {
@@ -72,8 +74,10 @@ void timers_flush() {
 }
 }  // namespace
 
+static thread_local bool t_timers_paused = false;  // inside a stream capture
+void pause_timers(bool on) { t_timers_paused = on; }
 TimerScope::TimerScope(int kind_, double work_, double bytes_) : kind(kind_), work(work_), bytes(bytes_) {
-  if (!g_timers_on || kind < 0) return;
+  if (!g_timers_on || kind < 0 || t_timers_paused) return;
   hipEvent_t a = event_get();
   MNP_HIP(hipEventRecord(a, stream()));
   ev0 = a;

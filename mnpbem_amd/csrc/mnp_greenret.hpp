@@ -9,6 +9,12 @@ constexpr int MNP_MAX_ORDER = 8;
 constexpr int MNP_MAX_POLAR = 3 * 16 * 16;
 
 hipStream_t stream();
+// This thread's stream override (NULL: the library's stream), e.g. a stream
+// under capture; and whether this thread's TimerScopes record nothing (event
+// records have no place in a captured graph).
+hipStream_t thread_stream();
+void set_thread_stream(hipStream_t s);
+void pause_timers(bool on);
 
 template <class T>
 T* dev_alloc(int64_t count) {

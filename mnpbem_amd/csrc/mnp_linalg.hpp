@@ -39,6 +39,13 @@ template <class T>
 void gemm(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, MatView<T> B,
           double2 beta, MatView<T> C);
 
+// Batched products of equal shapes at element strides (operand z at X + z sx)
+// with a chosen output tile; see linalg.hip for when C may alias A or B.
+enum { GEMM_TILE_64 = 0, GEMM_TILE_128x64 = 1, GEMM_TILE_128x96_3M = 2, GEMM_NTILES = 3 };
+template <class T>
+void gemm_strided(int64_t m, int64_t n, int64_t k, double2 alpha, MatView<T> A, int64_t sa, MatView<T> B,
+                  int64_t sb, double2 beta, MatView<T> C, int64_t sc, int64_t batch, int tile);
+
 // While set, large complex64 products (the 128 x 96 tile class) run on the
 // split-bf16 kernel: each FP32 operand as hi + lo bf16, three bf16 products
 // per real product, FP32 accumulation (about 2^-16 relative per product).
