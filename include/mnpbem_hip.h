@@ -136,6 +136,19 @@ int mnp_greenret_create(const double* verts, int64_t nverts, const int64_t* face
 int mnp_greenret_create_like(const double* verts, int64_t nverts, const int64_t* faces,
                              int64_t nfaces, const mnp_green_options* op, mnp_greenret_t like,
                              mnp_greenret_t* out);
+/* compgreenret(p, p, op) for a compound of nparts parts (replaces
+ * Greenfun/@compgreenret/private/init.m:4-31 with Particles/@compound/
+ * connect.m and initclosed.m:19-89): faces of part q are
+ * [part_offsets[q], part_offsets[q + 1]) of the concatenated mesh; inout
+ * (nparts x 2, column-major, 1-based media <= 16) the inside and outside
+ * medium of each part; closed_mask[q] has bit r set when part r belongs to
+ * the closed surface of part q (0: no closed-surface correction for part
+ * q's faces), as comparticle's closed arguments give it.  At most 32 parts.
+ * Solvers on it take mnp_bemret_initmat_media. */
+int mnp_greenret_create_compound(const double* verts, int64_t nverts, const int64_t* faces,
+                                 int64_t nfaces, int32_t nparts, const int64_t* part_offsets,
+                                 const int32_t* inout, const uint32_t* closed_mask,
+                                 const mnp_green_options* op, mnp_greenret_t* out);
 int mnp_greenret_destroy(mnp_greenret_t g);
 /* Number of faces and of refined (self + near-field) matrix elements. */
 int mnp_greenret_size(mnp_greenret_t g, int64_t* nfaces, int64_t* nrefined);
@@ -174,6 +187,14 @@ int mnp_bemret_solve(mnp_bemret_t bem, int64_t npol, const void* phi1, const voi
                      const void* a1, const void* a1p, const void* phi2, const void* phi2p,
                      const void* a2, const void* a2p, void* sig1, void* sig2, void* h1,
                      void* h2);
+/* bem(enei) for a solver on a compound Green function
+ * (mnp_greenret_create_compound): eps[m - 1] is the dielectric function of
+ * medium m (1 <= m <= nmed), each face taking those of its part's inside and
+ * outside media.  Replaces BEM/@bemret/private/initmat.m:5-72 with eps1,
+ * eps2 per face (lines 19-24) and, when some part's inside medium is some
+ * part's outside medium (con{1,2} ~= 0), the full L1 = G1 eps1 G1^{-1},
+ * L2 = G2 eps2 G2^{-1} (lines 34-42). */
+int mnp_bemret_initmat_media(mnp_bemret_t bem, double enei, int32_t nmed, const mnp_complex* eps);
 /* Refinement statistics of the last solve (MIXED): sweeps and final relative residual. */
 int mnp_bemret_stats(mnp_bemret_t bem, int* iterations, double* relres);
 /* Refinement settings for MNP_SOLVER_MIXED (defaults: 1e-12, 10). */

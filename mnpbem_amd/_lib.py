@@ -53,6 +53,8 @@ PROTOTYPES = {
     "mnp_green_options_default": (None, [C.POINTER(GreenOptions)]),
     "mnp_greenret_create": (INT, [P, I64, P, I64, C.POINTER(GreenOptions), C.POINTER(P)]),
     "mnp_greenret_create_like": (INT, [P, I64, P, I64, C.POINTER(GreenOptions), P, C.POINTER(P)]),
+    "mnp_greenret_create_compound": (INT, [P, I64, P, I64, C.c_int32, P, P, P, C.POINTER(GreenOptions),
+                                           C.POINTER(P)]),
     "mnp_greenret_destroy": (INT, [P]),
     "mnp_greenret_size": (INT, [P, PI64, PI64]),
     "mnp_greenret_geometry": (INT, [P, P, P, P, P, P]),
@@ -70,6 +72,7 @@ PROTOTYPES = {
     "mnp_bemret_batch_set_options": (INT, [P, INT, INT, INT]),
     "mnp_bemret_batch_cross_sections": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mnp_bemret_batch_info": (INT, [P, C.POINTER(INT), P, PI64]),
+    "mnp_bemret_initmat_media": (INT, [P, D, C.c_int32, P]),
     "mnp_bemret_initmat_many": (INT, [INT, C.POINTER(P), PD, C.POINTER(MnpComplex), C.POINTER(MnpComplex)]),
     "mnp_bemret_stats": (INT, [P, C.POINTER(INT), PD]),
     "mnp_bemret_set_refinement": (INT, [P, D, INT]),

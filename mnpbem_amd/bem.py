@@ -11,8 +11,11 @@ The names and argument meanings follow MNPBEM:
 
 Everything numerical runs in libmnpbem_hip.so on the GPU; this module only
 keeps handles, allocates device buffers through torch, and converts the
-Matlab-style options.  Supported here: one closed particle of flat triangles
-embedded in one medium (inout = [inside, outside]).
+Matlab-style options.  Supported here: particles of flat triangles, one
+closed particle in one medium (inout = [inside, outside]) for every solver,
+and compounds of several parts with their own media (dimers, core-shell
+particles: comparticle(eps, [p1, p2], [[3, 2], [2, 1]], 1, 2, op)) for the
+dense retarded solver with plane waves.
 """
 import ctypes as C
 
@@ -127,34 +130,95 @@ def bemoptions(**kw):
 
 # --------------------------------------------------------------- particles
 class comparticle:
-    """comparticle(eps, {p}, inout, closed, op) for one particle.
+    """comparticle(eps, {p1, p2, ...}, inout, closed..., op).
 
-    ``eps`` is the table of dielectric functions, ``inout`` = [inside, outside]
-    1-based indices into it (Particles/@comparticle/comparticle.m:11-28).
+    ``eps`` is the table of dielectric functions, ``inout`` (one row per part)
+    the 1-based [inside, outside] media of each part
+    (Particles/@comparticle/comparticle.m:11-28).  The arguments after inout
+    are closed arguments (Particles/@comparticle/closed.m: an index or a list
+    of part indices; every listed part not set before gets the closed surface
+    made of the listed parts) followed by the option dict; with one part the
+    legacy form ``comparticle(eps, [p], inout, closed=1, op)`` is the same.
+    Several parts make a compound (dimers, core-shell particles, ...), solved
+    by the dense solver.
     """
 
-    def __init__(self, eps, parts, inout, closed=1, op=None):
-        if len(parts) != 1:
-            raise NotImplementedError("one particle per comparticle is supported")
+    def __init__(self, eps, parts, inout, *args, closed=None, op=None):
+        closed_args = []
+        for a in args:
+            if isinstance(a, dict):
+                op = a
+            else:
+                closed_args.append(a)
+        if closed is not None:
+            closed_args.append(closed)
         op = bemoptions() if op is None else op
         if op.get("interp", "flat") != "flat":
             raise NotImplementedError("only flat boundary elements are supported")
         self.eps = list(eps)
-        self.verts, self.faces = (np.asarray(a) for a in parts[0])
+        self.parts = [(np.asarray(v), np.asarray(f)) for v, f in parts]
+        if not self.parts:
+            raise ValueError("comparticle: no particle")
         self.inout = np.atleast_2d(np.asarray(inout, dtype=int))
-        self.closed = bool(closed)
+        if self.inout.shape != (len(self.parts), 2):
+            raise ValueError("comparticle: inout needs one [inside, outside] row per particle")
         self.op = op
         self._green = None
+        if len(self.parts) == 1:
+            self.verts, self.faces = self.parts[0]
+            # a closed argument 1 (or True) closes the particle; 0, False or none leaves it open
+            self.closed = any(c is not False and np.all(np.asarray(c) != 0) for c in closed_args)
+            self.closed_mask = np.array([1 if self.closed else 0], dtype=np.uint32)
+        else:
+            verts, faces, off = [], [], 0
+            self.part_off = [0]
+            for v, f in self.parts:
+                verts.append(v)
+                faces.append(f + off)
+                off += len(v)
+                self.part_off.append(self.part_off[-1] + len(f))
+            self.verts, self.faces = np.vstack(verts), np.vstack(faces)
+            mask = [0] * len(self.parts)
+            done = [False] * len(self.parts)
+            for c in closed_args:
+                idx = [int(i) for i in np.atleast_1d(c)]
+                if any(i <= 0 or i > len(self.parts) for i in idx):
+                    raise NotImplementedError("closed arguments: part indices 1..nparts (no flipped parts)")
+                bits = sum(1 << (i - 1) for i in idx)
+                for i in idx:
+                    if not done[i - 1]:
+                        mask[i - 1], done[i - 1] = bits, True
+            self.closed_mask = np.array(mask, dtype=np.uint32)
+            self.closed = bool(np.any(self.closed_mask))
+
+    @property
+    def compound(self):
+        return len(self.parts) > 1
 
     @property
     def n(self):
         return len(self.faces)
 
+    @property
+    def face_inout(self):
+        """n x 2 media (1-based) of every face: its part's inout row."""
+        if not self.compound:
+            return np.broadcast_to(self.inout[0], (self.n, 2))
+        return np.repeat(self.inout, np.diff(self.part_off), axis=0)
+
     def eps1(self, enei):
+        if self.compound:
+            raise NotImplementedError("eps1 of a compound: use eps_media")
         return self.eps[self.inout[0, 0] - 1](enei)[0]
 
     def eps2(self, enei):
+        if self.compound:
+            raise NotImplementedError("eps2 of a compound: use eps_media")
         return self.eps[self.inout[0, 1] - 1](enei)[0]
+
+    def eps_media(self, enei):
+        """Dielectric function of every medium of the table at enei."""
+        return [e(enei)[0] for e in self.eps]
 
     @property
     def green(self):
@@ -165,16 +229,21 @@ class comparticle:
 
     def _make_green(self, like=None):
         op = self.op
-        return CompGreenRet(
-            self.verts, self.faces, order=op.get("order", 5),
-            rel_cutoff=op.get("RelCutoff", 0.0), abs_cutoff=op.get("AbsCutoff", 0.0),
-            npol=tuple(np.broadcast_to(op.get("npol", (7, 5)), (2,))), closed=self.closed, like=like)
+        kw = dict(order=op.get("order", 5), rel_cutoff=op.get("RelCutoff", 0.0), abs_cutoff=op.get("AbsCutoff", 0.0),
+                  npol=tuple(np.broadcast_to(op.get("npol", (7, 5)), (2,))), closed=self.closed, like=like)
+        if self.compound:
+            if like is not None:
+                raise NotImplementedError("moved compound particles")
+            kw["compound"] = (np.asarray(self.part_off, np.int64), self.inout, self.closed_mask)
+        return CompGreenRet(self.verts, self.faces, **kw)
 
     def moved(self, verts):
         """The same particle with vertices ``verts`` and the refined pairs of
         this one (no new refinement selection: the Green function's elements
         stay continuous in the vertices)."""
-        q = comparticle(self.eps, [(verts, self.faces)], self.inout, self.closed, self.op)
+        if self.compound:
+            raise NotImplementedError("moved compound particles")
+        q = comparticle(self.eps, [(verts, self.faces)], self.inout, closed=1 if self.closed else 0, op=self.op)
         q._green = q._make_green(like=self.green)
         return q
 
@@ -239,9 +308,14 @@ class bemret:
         """bem(enei): initialise the matrices for one wavelength (initmat.m)."""
         if self.enei != enei:
             self.enei = None  # stays unset if initmat fails part-way
-            e1, e2 = self.p.eps1(enei), self.p.eps2(enei)
-            _lib.call("mnp_bemret_initmat", self.handle, float(enei), _lib.MnpComplex.of(e1),
-                      _lib.MnpComplex.of(e2))
+            if getattr(self.p, "compound", False):
+                em = self.p.eps_media(enei)
+                arr = (_lib.MnpComplex * len(em))(*[_lib.MnpComplex.of(e) for e in em])
+                _lib.call("mnp_bemret_initmat_media", self.handle, float(enei), len(em), arr)
+            else:
+                e1, e2 = self.p.eps1(enei), self.p.eps2(enei)
+                _lib.call("mnp_bemret_initmat", self.handle, float(enei), _lib.MnpComplex.of(e1),
+                          _lib.MnpComplex.of(e2))
             self.enei = enei
         return self
 
@@ -679,6 +753,9 @@ def bemsolver(p, op=None, **kw):
     op = dict(p.op if op is None else op)
     op.update(kw)
     sim = op.get("sim", "ret")
+    if getattr(p, "compound", False) and (sim != "ret" or op.get("iter") or op.get("layer") is not None
+                                          or op.get("sym")):
+        raise NotImplementedError("compound particles: only the dense retarded solver (bemret) is built")
     if isinstance(p, comparticlemirror) or op.get("sym"):
         if op.get("iter") or op.get("layer") is not None:
             raise NotImplementedError("mirror symmetry: only the dense solvers without layers are built")
@@ -704,6 +781,25 @@ def bemsolver(p, op=None, **kw):
 
 
 # ------------------------------------------------------------ excitations
+def _side_masks(p, medium):
+    """Device 0/1 masks of the faces whose inside (m1) or outside (m2) is medium."""
+    fio = p.face_inout
+    m1 = torch.as_tensor((fio[:, 0] == medium).astype(np.float64), device="cuda")
+    m2 = torch.as_tensor((fio[:, 1] == medium).astype(np.float64), device="cuda")
+    return m1, m2
+
+
+def _radiating(sig, p, medium):
+    """compstruct whose sig2, h2 are the charges and currents facing medium:
+    sig1, h1 on faces whose inside is it plus sig2, h2 on faces whose outside is it."""
+    m1, m2 = _side_masks(p, medium)
+    s2 = dev.colmajor(tuple(sig.sig2.shape))
+    h2 = dev.colmajor(tuple(sig.h2.shape))
+    s2.copy_(sig.sig1 * m1[:, None] + sig.sig2 * m2[:, None])
+    h2.copy_(sig.h1 * m1[:, None, None] + sig.h2 * m2[:, None, None])
+    return compstruct(p, sig.enei, sig1=sig.sig1, h1=sig.h1, sig2=s2, h2=h2)
+
+
 class planewaveret:
     """Plane-wave excitation (Simulation/retarded/@planewaveret).
 
@@ -731,8 +827,12 @@ class planewaveret:
         return p.eps[self.medium - 1](enei)[0]
 
     def __call__(self, p, enei):
-        """exc(p, enei): potentials a2, a2p on the faces (potential.m)."""
-        if p.inout[0, 1] != self.medium:
+        """exc(p, enei): potentials a2, a2p on the faces (potential.m); for a
+        compound a1, a1p on the faces whose inside is the medium and a2, a2p
+        on those whose outside is it (potential.m:47-88, the two inout
+        columns)."""
+        compound = getattr(p, "compound", False)
+        if not compound and p.inout[0, 1] != self.medium:
             raise NotImplementedError("the particle's outside must be the embedding medium")
         n, npol = p.n, len(self.pol)
         a2 = dev.colmajor((n, 3, npol))
@@ -743,14 +843,26 @@ class planewaveret:
                   _lib.MnpComplex.of(self._eps_med(p, enei)), npol,
                   pol.ctypes.data_as(C.c_void_p), dr.ctypes.data_as(C.c_void_p),
                   dev.ptr(dev.base(a2)), dev.ptr(dev.base(a2p)))
-        return compstruct(p, enei, a2=a2, a2p=a2p)
+        if not compound:
+            return compstruct(p, enei, a2=a2, a2p=a2p)
+        m1, m2 = _side_masks(p, self.medium)
+        out = {}
+        for key, v in (("a1", a2), ("a1p", a2p), ("a2", a2), ("a2p", a2p)):
+            t = dev.colmajor((n, 3, npol))
+            t.copy_(v * (m1 if key.startswith("a1") else m2)[:, None, None])
+            out[key] = t
+        return compstruct(p, enei, **out)
 
     potential = __call__
 
     def cross_sections(self, sig):
-        """(ext, sca, abs) per polarisation, FP64."""
+        """(ext, sca, abs) per polarisation, FP64.  For a compound the far field
+        sums both sides: sig1, h1 of faces whose inside is the medium and sig2,
+        h2 of faces whose outside is it (@spectrumret/farfield.m:30-60)."""
         p, enei = sig.p, sig.enei
         npol = len(self.pol)
+        if getattr(p, "compound", False):
+            sig = _radiating(sig, p, self.medium)
         out = [np.zeros(npol) for _ in range(3)]
         pol = np.asfortranarray(self.pol)
         dr = np.asfortranarray(self.dir)

@@ -336,6 +336,7 @@ void adjoint_free(Adjoint* a) { delete a; }
 
 void BemRet::solve_transposed(int64_t p, const double2* c, double2* lam) {
   MNP_CHECK(ready, MNP_ERR_STATE, "bemret_solve_transposed: initmat not called");
+  MNP_CHECK(!g->compound(), MNP_ERR_STATE, "bemret_solve_transposed: not for compound particles");
   MNP_CHECK(p > 0 && p <= 64, MNP_ERR_ARG, "bemret_solve_transposed: npol must be in 1..64");
   if (!adj) adj = new Adjoint(n);
   Adjoint& a = *adj;

@@ -937,6 +937,7 @@ int mnp_bemret_batch_cross_sections(mnp_bemret_batch_t bh, int64_t count, const 
                 "bemret_batch_cross_sections: every member needs a mesh of the batch's face count");
       MNP_CHECK(enei[m] > 0, MNP_ERR_ARG, "bemret_batch_cross_sections: wavelengths must be positive");
       const GreenRet& G = *greens[m]->g;
+      MNP_CHECK(!G.compound(), MNP_ERR_ARG, "bemret_batch_cross_sections: compound particles take the dense solver");
       hg[i] = BGeo{G.geo.pos, G.geo.nvec, G.geo.area, G.pairs.row, G.pairs.col, G.pairs.kind, G.gtab, G.ftab,
                    G.nnz, G.order};
       const double k0 = 2 * M_PI / enei[m];

@@ -34,10 +34,12 @@ namespace mnp {
 __global__ void k_excitation(int64_t n, int64_t p, const double2* phi1, const double2* phi1p,
                              const double2* a1, const double2* a1p, const double2* phi2,
                              const double2* phi2p, const double2* a2, const double2* a2p,
-                             double2* b, const double* nvec, double k, double2 e1, double2 e2) {
+                             double2* b, const double* nvec, double k, double2 e1_, double2 e2_,
+                             const double2* e1v, const double2* e2v) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t q = blockIdx.y;
   if (i >= n) return;
+  const double2 e1 = e1v ? e1v[i] : e1_, e2 = e2v ? e2v[i] : e2_;  // per face for a compound
   const double2 z = mk<double>(0, 0);
   auto ld = [&](const double2* v, int64_t col) { return v ? v[col * n + i] : z; };
   const double2 p1 = ld(phi1, q), p2 = ld(phi2, q);
@@ -65,10 +67,12 @@ __global__ void k_excitation(int64_t n, int64_t p, const double2* phi1, const do
 // With b == NULL the kernel writes -A x instead.
 __global__ void k_residual(int64_t n, int64_t p, const double2* b_, const double2* GX1,
                            const double2* GX2, const double2* HX1, const double2* HX2, double2* r,
-                           const double* nvec, double k, double2 e1, double2 e2) {
+                           const double* nvec, double k, double2 e1_, double2 e2_, const double2* e1v,
+                           const double2* e2v) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t q = blockIdx.y;
   if (i >= n) return;
+  const double2 e1 = e1v ? e1v[i] : e1_, e2 = e2v ? e2v[i] : e2_;  // per face for a compound
   const double2 zero = mk<double>(0, 0);
   auto b = [&](int64_t idx) { return b_ ? b_[idx] : zero; };
   const double2 ik = mk<double>(0, k);
@@ -91,6 +95,48 @@ __global__ void k_residual(int64_t n, int64_t p, const double2* b_, const double
   const double2 De = csub<double>(csub<double>(cmul<double>(e1, Hs1), cmul<double>(e2, Hs2)),
                                   cmul<double>(ik, nGh));
   r[(4 * p + q) * n + i] = csub<double>(b((4 * p + q) * n + i), De);
+}
+
+// Nested media (full L1, L2): the operator the chain inverts puts eps on the
+// charges, r = b - A x with
+//   phi = G1 s1 - G2 s2,  a = G1 h1 - G2 h2,
+//   alpha = H1 h1 - H2 h2 - i k n (G1 (e1 s1) - G2 (e2 s2)),
+//   De = H1 (e1 s1) - H2 (e2 s2) - i k n . (G1 (e1 h1) - G2 (e2 h2))
+// from GX = G [s | h], GE = G [e s | e h], HM = H [e s | h] (each n x 4p).
+__global__ void k_residual_nested(int64_t n, int64_t p, const double2* b, const double2* GX1, const double2* GX2,
+                                  const double2* GE1, const double2* GE2, const double2* HM1, const double2* HM2,
+                                  double2* r, const double* nvec, double k) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  const double2 ik = mk<double>(0, k);
+  r[q * n + i] = csub<double>(b[q * n + i], csub<double>(GX1[q * n + i], GX2[q * n + i]));
+  const double2 ges = csub<double>(GE1[q * n + i], GE2[q * n + i]);
+  double2 nge = mk<double>(0, 0);
+  for (int c = 0; c < 3; ++c) {
+    const int64_t col = c + 3 * q;
+    const double nc = nvec[c * n + i];
+    r[(p + col) * n + i] =
+        csub<double>(b[(p + col) * n + i], csub<double>(GX1[(p + col) * n + i], GX2[(p + col) * n + i]));
+    const double2 al = csub<double>(csub<double>(HM1[(p + col) * n + i], HM2[(p + col) * n + i]),
+                                    cmul<double>(ik, cscale<double>(ges, nc)));
+    r[(5 * p + col) * n + i] = csub<double>(b[(5 * p + col) * n + i], al);
+    nge = cadd<double>(nge, cscale<double>(csub<double>(GE1[(p + col) * n + i], GE2[(p + col) * n + i]), nc));
+  }
+  const double2 De = csub<double>(csub<double>(HM1[q * n + i], HM2[q * n + i]), cmul<double>(ik, nge));
+  r[(4 * p + q) * n + i] = csub<double>(b[(4 * p + q) * n + i], De);
+}
+// xe = [e1 s1 | e1 h1 | e2 s2 | e2 h2], xm = [e1 s1 | h1 | e2 s2 | h2] from x (n x 8p)
+__global__ void k_scale_x(int64_t n, int64_t p, const double2* x, const double2* e1, const double2* e2, double2* xe,
+                          double2* xm) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t col = blockIdx.y;  // 0 .. 8p
+  if (i >= n) return;
+  const int64_t side = col / (4 * p), within = col % (4 * p);
+  const double2 e = side ? e2[i] : e1[i];
+  const double2 v = x[col * n + i], ev = cmul<double>(e, v);
+  xe[col * n + i] = ev;
+  xm[col * n + i] = within < p ? ev : v;
 }
 
 __global__ void k_norm2(int64_t len, const double2* v, double* out) {
@@ -191,14 +237,15 @@ double dev_norm2(int64_t len, const double2* v, Workspace& ws, int slot) {
 void launch_excitation(int64_t n, int64_t p, const double2* const in[8], double2* b, const double* nvec,
                        double k, double2 e1, double2 e2) {
   hipLaunchKernelGGL(k_excitation, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0, stream(), n, p,
-                     in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], b, nvec, k, e1, e2);
+                     in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], b, nvec, k, e1, e2,
+                     (const double2*)nullptr, (const double2*)nullptr);
   MNP_HIP(hipGetLastError());
 }
 void launch_residual(int64_t n, int64_t p, const double2* b, const double2* GX1, const double2* GX2,
                      const double2* HX1, const double2* HX2, double2* r, const double* nvec, double k,
                      double2 e1, double2 e2) {
   hipLaunchKernelGGL(k_residual, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0, stream(), n, p, b,
-                     GX1, GX2, HX1, HX2, r, nvec, k, e1, e2);
+                     GX1, GX2, HX1, HX2, r, nvec, k, e1, e2, (const double2*)nullptr, (const double2*)nullptr);
   MNP_HIP(hipGetLastError());
 }
 void launch_unpack(int64_t n, int64_t p, const double2* x, double2* sig1, double2* h1, double2* sig2,
@@ -234,7 +281,7 @@ BemRet::BemRet(const GreenRet* g_, int solver_) : g(g_), solver(solver_), n(g_->
 }
 BemRet::~BemRet() {
   adjoint_free(adj);
-  void* ps[] = {G1, G2, H1, H2};
+  void* ps[] = {G1, G2, H1, H2, e1v, e2v};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -243,7 +290,57 @@ void BemRet::initmat(double enei_, double2 e1, double2 e2) {
   if (!initmat_try(enei_, e1, e2)) initmat_finish();
 }
 
+// Compound particle (initmat.m:5-72 with eps per face and the connectivity of
+// compgreenret/private/eval1.m): epsm[m - 1] is medium m's dielectric function.
+void BemRet::initmat_media(double enei_, const std::vector<double2>& epsm) {
+  MNP_CHECK(g->compound(), MNP_ERR_STATE, "bemret_initmat_media: the Green function is not a compound");
+  MNP_CHECK((int)epsm.size() >= g->nmed, MNP_ERR_ARG, "bemret_initmat_media: a dielectric function per medium");
+  ready = false;
+  ++epoch;
+  enei = enei_;
+  eps1 = eps2 = mk<double>(0, 0);
+  const double k0 = 2 * M_PI / enei;
+  KTab kt{};
+  for (int m = 0; m < g->nmed; ++m) kt.k[m] = wavenumber(k0, epsm[m]);
+  std::vector<double2> h1(n), h2(n);
+  for (int64_t i = 0; i < n; ++i) {
+    h1[i] = epsm[g->fin_h[i] - 1];
+    h2[i] = epsm[g->fout_h[i] - 1];
+  }
+  if (!e1v) {
+    e1v = dev_alloc<double2>(n);
+    e2v = dev_alloc<double2>(n);
+  }
+  MNP_HIP(hipMemcpyAsync(e1v, h1.data(), sizeof(double2) * n, hipMemcpyHostToDevice, stream()));
+  MNP_HIP(hipMemcpyAsync(e2v, h2.data(), sizeof(double2) * n, hipMemcpyHostToDevice, stream()));
+  MNP_HIP(hipStreamSynchronize(stream()));
+  bool ok;
+  if (solver == MNP_SOLVER_FP64) {
+    Chain<double>& c = *c64;
+    c.e1v = e1v;
+    c.e2v = e2v;
+    c.k0 = k0;
+    if (g->con12) c.enable_full_L();
+    g->assemble_side(1, kt, OutPair{c.G1, nullptr}, OutPair{c.S1, nullptr}, n);
+    g->assemble_side(2, kt, OutPair{c.G2, nullptr}, OutPair{c.S2, nullptr}, n);
+    ok = chain_build_try<double>(c, *g);
+  } else {
+    Chain<float>& c = *c32;
+    c.e1v = e1v;
+    c.e2v = e2v;
+    c.k0 = k0;
+    if (g->con12) c.enable_full_L();
+    g->assemble_side(1, kt, OutPair{G1, c.G1}, OutPair{H1, c.S1}, n);
+    g->assemble_side(2, kt, OutPair{G2, c.G2}, OutPair{H2, c.S2}, n);
+    SplitBf16Scope split(solver == MNP_SOLVER_MIXED_BF16);
+    ok = chain_build_try<float>(c, *g);
+  }
+  ready = ok;
+  if (!ok) initmat_finish();
+}
+
 bool BemRet::initmat_try(double enei_, double2 e1, double2 e2) {
+  MNP_CHECK(!g->compound(), MNP_ERR_STATE, "bemret_initmat: a compound takes mnp_bemret_initmat_media");
   ready = false;
   ++epoch;
   enei = enei_;
@@ -295,7 +392,7 @@ void BemRet::solve(int64_t p, const double2* const in[8], double2* sig1, double2
   double2* x = (double2*)vws.get(1, sizeof(double2) * len);
   hipLaunchKernelGGL(k_excitation, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0,
                      stream(), n, p, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], b,
-                     g->geo.nvec, k0, eps1, eps2);
+                     g->geo.nvec, k0, eps1, eps2, (const double2*)e1v, (const double2*)e2v);
   MNP_HIP(hipGetLastError());
   solve_b(p, b, x);
   hipLaunchKernelGGL(k_unpack, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0,
@@ -307,7 +404,7 @@ void BemRet::solve(int64_t p, const double2* const in[8], double2* sig1, double2
 void BemRet::solve_b(int64_t p, const double2* b, double2* x) {
   const int64_t len = 8 * p * n;
   const double k0 = 2 * M_PI / enei;
-  const size_t tmp_elems = (size_t)(18 * p * n);
+  const size_t tmp_elems = (size_t)((g->compound() ? 26 : 18) * p * n);
   last_iter = 0;
   last_relres = 0;
   if (solver == MNP_SOLVER_FP64) {
@@ -318,10 +415,13 @@ void BemRet::solve_b(int64_t p, const double2* b, double2* x) {
     float2* b32 = (float2*)vws.get(2, sizeof(float2) * len);
     float2* x32 = (float2*)vws.get(3, sizeof(float2) * len);
     float2* tmp = (float2*)vws.get(4, sizeof(float2) * tmp_elems);
-    // residual buffers: 4 products (n x 4p) + r (n x 8p), FP64
-    double2* work = (double2*)res_ws.get(0, sizeof(double2) * (size_t)(24 * p * n));
+    // residual buffers: 4 products (n x 4p) + r (n x 8p), FP64; nested media
+    // also G [eps s | eps h] (2 x n x 4p) and the scaled vectors (2 x n x 8p)
+    const bool nested = g->compound() && g->con12;
+    double2* work = (double2*)res_ws.get(0, sizeof(double2) * (size_t)((nested ? 48 : 24) * p * n));
     double2 *GX1 = work, *GX2 = work + 4 * p * n, *HX1 = work + 8 * p * n, *HX2 = work + 12 * p * n,
             *rr = work + 16 * p * n;
+    double2 *GE1 = work + 24 * p * n, *GE2 = work + 28 * p * n, *xe = work + 32 * p * n, *xm = work + 40 * p * n;
     auto precond = [&](const double2* src, double2* dst) {
       convert_mat<double, float>(n, 8 * p, MatView<double>{const_cast<double2*>(src), n},
                                  MatView<float>{b32, n});
@@ -336,12 +436,29 @@ void BemRet::solve_b(int64_t p, const double2* b, double2* x) {
                    MatView<double>{GX1, n});
       gemm<double>(n, 4 * p, n, one, MatView<double>{G2, n}, MatView<double>{x + 4 * p * n, n}, zero,
                    MatView<double>{GX2, n});
-      gemm<double>(n, 4 * p, n, one, MatView<double>{H1, n}, MatView<double>{x, n}, zero,
-                   MatView<double>{HX1, n});
-      gemm<double>(n, 4 * p, n, one, MatView<double>{H2, n}, MatView<double>{x + 4 * p * n, n}, zero,
-                   MatView<double>{HX2, n});
-      hipLaunchKernelGGL(k_residual, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0,
-                         stream(), n, p, b, GX1, GX2, HX1, HX2, rr, g->geo.nvec, k0, eps1, eps2);
+      if (nested) {
+        // the operator the chain inverts with full L1, L2 (eps on the charges)
+        hipLaunchKernelGGL(k_scale_x, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n,
+                           p, x, (const double2*)e1v, (const double2*)e2v, xe, xm);
+        gemm<double>(n, 4 * p, n, one, MatView<double>{G1, n}, MatView<double>{xe, n}, zero,
+                     MatView<double>{GE1, n});
+        gemm<double>(n, 4 * p, n, one, MatView<double>{G2, n}, MatView<double>{xe + 4 * p * n, n}, zero,
+                     MatView<double>{GE2, n});
+        gemm<double>(n, 4 * p, n, one, MatView<double>{H1, n}, MatView<double>{xm, n}, zero,
+                     MatView<double>{HX1, n});
+        gemm<double>(n, 4 * p, n, one, MatView<double>{H2, n}, MatView<double>{xm + 4 * p * n, n}, zero,
+                     MatView<double>{HX2, n});
+        hipLaunchKernelGGL(k_residual_nested, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0, stream(),
+                           n, p, b, GX1, GX2, GE1, GE2, HX1, HX2, rr, g->geo.nvec, k0);
+      } else {
+        gemm<double>(n, 4 * p, n, one, MatView<double>{H1, n}, MatView<double>{x, n}, zero,
+                     MatView<double>{HX1, n});
+        gemm<double>(n, 4 * p, n, one, MatView<double>{H2, n}, MatView<double>{x + 4 * p * n, n}, zero,
+                     MatView<double>{HX2, n});
+        hipLaunchKernelGGL(k_residual, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0,
+                           stream(), n, p, b, GX1, GX2, HX1, HX2, rr, g->geo.nvec, k0, eps1, eps2,
+                           (const double2*)e1v, (const double2*)e2v);
+      }
       MNP_HIP(hipGetLastError());
       const double rn = norm2(rr, len);
       last_relres = bn > 0 ? rn / bn : 0;
@@ -367,6 +484,7 @@ void BemRet::solve_b(int64_t p, const double2* b, double2* x) {
 void BemRet::tangent_eps1(int64_t p, const double2* sig1, const double2* sig2, const double2* h1,
                           const double2* h2, double2* dsig1, double2* dsig2, double2* dh1, double2* dh2) {
   MNP_CHECK(ready, MNP_ERR_STATE, "bemret_tangent: initmat not called");
+  MNP_CHECK(!g->compound(), MNP_ERR_STATE, "bemret_tangent: not for compound particles");
   MNP_CHECK(p > 0 && p <= 64, MNP_ERR_ARG, "bemret_tangent: npol must be in 1..64");
   const int64_t len = 8 * p * n, nn = n * n;
   const double k0 = 2 * M_PI / enei;
@@ -653,6 +771,17 @@ int mnp_bemret_initmat(mnp_bemret_t bem, double enei, mnp_complex eps1, mnp_comp
     MNP_CHECK(bem, MNP_ERR_ARG, "bemret_initmat: NULL handle");
     MNP_CHECK(enei > 0, MNP_ERR_ARG, "bemret_initmat: wavelength must be positive");
     bem->b->initmat(enei, to_d2(eps1), to_d2(eps2));
+  });
+}
+
+int mnp_bemret_initmat_media(mnp_bemret_t bem, double enei, int32_t nmed, const mnp_complex* eps) {
+  return guarded([&] {
+    MNP_CHECK(bem && eps, MNP_ERR_ARG, "bemret_initmat_media: NULL argument");
+    MNP_CHECK(enei > 0, MNP_ERR_ARG, "bemret_initmat_media: wavelength must be positive");
+    MNP_CHECK(nmed >= 1 && nmed <= MNP_MAX_MED, MNP_ERR_ARG, "bemret_initmat_media: 1..16 media");
+    std::vector<double2> e(nmed);
+    for (int m = 0; m < nmed; ++m) e[m] = to_d2(eps[m]);
+    bem->b->initmat_media(enei, e);
   });
 }
 

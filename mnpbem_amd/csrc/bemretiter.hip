@@ -531,6 +531,7 @@ int mnp_bemretiter_create(mnp_greenret_t g, int precision, const int64_t* perm, 
                           const int64_t* full, int64_t nlow, const int64_t* low, double htol, int kmax,
                           mnp_bemretiter_t* out) {
   return guarded([&] {
+    MNP_CHECK(!g || !g->g->compound(), MNP_ERR_ARG, "bemretiter_create: compound particles take the dense solver (mnp_bemret_*)");
     MNP_CHECK(g && perm && out, MNP_ERR_ARG, "bemretiter_create: NULL argument");
     auto* h = new mnp_bemretiter_s{nullptr};
     try {

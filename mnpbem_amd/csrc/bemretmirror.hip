@@ -334,6 +334,7 @@ extern "C" {
 
 int mnp_bemretmirror_create(mnp_greenret_t g, int solver, int nsym, const int* symtable, mnp_bemretmirror_t* out) {
   return guarded([&] {
+    MNP_CHECK(!g || !g->g->compound(), MNP_ERR_ARG, "bemretmirror_create: compound particles take the dense solver (mnp_bemret_*)");
     MNP_CHECK(g && symtable && out, MNP_ERR_ARG, "bemretmirror_create: NULL argument");
     MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED || solver == MNP_SOLVER_MIXED_BF16,
               MNP_ERR_ARG, "bemretmirror_create: unknown solver");

@@ -358,6 +358,7 @@ extern "C" {
 
 int mnp_bemstat_create(mnp_greenret_t g, int solver, mnp_bemstat_t* out) {
   return guarded([&] {
+    MNP_CHECK(!g || !g->g->compound(), MNP_ERR_ARG, "bemstat_create: compound particles take the dense solver (mnp_bemret_*)");
     MNP_CHECK(g && out, MNP_ERR_ARG, "bemstat_create: NULL argument");
     MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED || solver == MNP_SOLVER_MIXED_BF16, MNP_ERR_ARG,
               "bemstat_create: unknown solver");
@@ -374,6 +375,7 @@ int mnp_bemstat_create(mnp_greenret_t g, int solver, mnp_bemstat_t* out) {
 
 int mnp_bemstatlayer_create(mnp_greenret_t g, mnp_greenret_t gimage, int solver, mnp_bemstat_t* out) {
   return guarded([&] {
+    MNP_CHECK(!g || !g->g->compound(), MNP_ERR_ARG, "bemstatlayer_create: compound particles take the dense solver (mnp_bemret_*)");
     MNP_CHECK(g && gimage && out, MNP_ERR_ARG, "bemstatlayer_create: NULL argument");
     MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED || solver == MNP_SOLVER_MIXED_BF16, MNP_ERR_ARG,
               "bemstatlayer_create: unknown solver");
@@ -447,6 +449,7 @@ int mnp_stat_dipole_moment(mnp_greenret_t g, int64_t npol, const void* sig, mnp_
 
 int mnp_bemstatmirror_create(mnp_greenret_t g, int solver, int nsym, const int* symtable, mnp_bemstatmirror_t* out) {
   return guarded([&] {
+    MNP_CHECK(!g || !g->g->compound(), MNP_ERR_ARG, "bemstatmirror_create: compound particles take the dense solver (mnp_bemret_*)");
     MNP_CHECK(g && symtable && out, MNP_ERR_ARG, "bemstatmirror_create: NULL argument");
     MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED || solver == MNP_SOLVER_MIXED_BF16,
               MNP_ERR_ARG, "bemstatmirror_create: unknown solver");

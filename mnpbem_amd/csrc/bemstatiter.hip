@@ -222,6 +222,7 @@ int mnp_bemstatiter_create(mnp_greenret_t g, int precision, const int64_t* perm,
                            const int64_t* full, int64_t nlow, const int64_t* low, double htol, int kmax,
                            mnp_bemstatiter_t* out) {
   return guarded([&] {
+    MNP_CHECK(!g || !g->g->compound(), MNP_ERR_ARG, "bemstatiter_create: compound particles take the dense solver (mnp_bemret_*)");
     MNP_CHECK(g && perm && out, MNP_ERR_ARG, "bemstatiter_create: NULL argument");
     auto* h = new mnp_bemstatiter_s{nullptr};
     try {

@@ -544,11 +544,16 @@ __global__ void __launch_bounds__(256) k_tables(int64_t nnz, Geo geo, int64_t n,
 //   fsum_i = sum_row area_row F_static(row, i) / area_i
 // with the refined static F = n_row . f_0 where (row, i) is refined
 // (compgreenret/private/initclosed.m:85-119, k = 0).
+// For a compound (fpart set) only the rows of the parts of column i's closed
+// surface are summed (initclosed.m:34-47 with closedparticle.m).
 __global__ void __launch_bounds__(256) k_closed_sum(int64_t n, Geo geo, Pairs pr, const int64_t* off,
                                                     int64_t nnz, const double* __restrict__ ft,
-                                                    double* fsum) {
+                                                    const int32_t* __restrict__ fpart,
+                                                    const uint32_t* __restrict__ cmask, double* fsum) {
   extern __shared__ unsigned int bits[];
   const int64_t i = blockIdx.x;
+  const uint32_t rows = fpart ? cmask[fpart[i]] : 0xffffffffu;
+  auto summed = [&](int64_t r) { return !fpart || ((rows >> fpart[r]) & 1u); };
   const int64_t nw = (n + 31) / 32;
   for (int64_t w = threadIdx.x; w < nw; w += blockDim.x) bits[w] = 0;
   __syncthreads();
@@ -562,7 +567,7 @@ __global__ void __launch_bounds__(256) k_closed_sum(int64_t n, Geo geo, Pairs pr
   const double ai = geo.area[i], inva = 1.0 / ai;
   double s = 0;
   for (int64_t r = threadIdx.x; r < n; r += blockDim.x) {
-    if (bits[r >> 5] & (1u << (r & 31))) continue;
+    if ((bits[r >> 5] & (1u << (r & 31))) || !summed(r)) continue;
     const double dx = geo.pos[r] - xi, dy = geo.pos[n + r] - yi, dz = geo.pos[2 * n + r] - zi;
     const double d = fmax(sqrt(dx * dx + dy * dy + dz * dz), 2.220446049250313e-16);
     const double in = geo.nvec[r] * dx + geo.nvec[n + r] * dy + geo.nvec[2 * n + r] * dz;
@@ -571,6 +576,7 @@ __global__ void __launch_bounds__(256) k_closed_sum(int64_t n, Geo geo, Pairs pr
   }
   for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     const int64_t r = pr.row[p];
+    if (!summed(r)) continue;
     const double F = geo.nvec[r] * ft[0 * nnz + p] + geo.nvec[n + r] * ft[1 * nnz + p] +
                      geo.nvec[2 * n + r] * ft[2 * nnz + p];
     s += geo.area[r] * (F * inva);
@@ -585,13 +591,14 @@ __global__ void __launch_bounds__(256) k_closed_sum(int64_t n, Geo geo, Pairs pr
   if (threadIdx.x == 0) fsum[i] = red[0];
 }
 
-// f_ii,0 += n_i (-2 pi - fsum_i)  (initclosed.m:78, greenret/diag.m:146-147).
+// f_ii,0 += n_i (-2 pi - fsum_i)  (initclosed.m:78, greenret/diag.m:146-147);
+// faces of a compound part without a closed surface keep theirs.
 __global__ void k_closed_apply(int64_t n, Geo geo, Pairs pr, int64_t nnz, const double* fsum,
-                               double* ft) {
+                               const int32_t* __restrict__ fpart, const uint32_t* __restrict__ cmask, double* ft) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t p = pr.diag[i];
-  if (p < 0) return;
+  if (p < 0 || (fpart && cmask[fpart[i]] == 0)) return;
   const double v = -2 * M_PI - fsum[i];
   for (int c = 0; c < 3; ++c) ft[c * nnz + p] += geo.nvec[c * n + i] * v;
 }
@@ -697,7 +704,111 @@ __global__ void __launch_bounds__(256) k_assemble_refined(int64_t nnz, int64_t n
   }
 }
 
+// Compound assembly of one side (eval1.m:16-51 through initmat.m:27-31):
+// column j takes the wavenumber of its side's medium m = colmed[j]; row i
+// enters with + when its same side faces m (g{s,s}), with - when its other
+// side does (g{3-s,s}), else the element is zero.  The +-2 pi of H sits on
+// the diagonal, which is always a same-side element.
+__device__ __forceinline__ double conn_sign(int32_t m, int32_t same, int32_t other) {
+  return same == m ? 1.0 : (other == m ? -1.0 : 0.0);
+}
+template <class TG>
+__global__ void __launch_bounds__(256) k_assemble_conn(int64_t n, Geo geo, const int32_t* __restrict__ colmed,
+                                                       const int32_t* __restrict__ same,
+                                                       const int32_t* __restrict__ other, KTab kt, double sgn2pi,
+                                                       TG outG, TG outH, int64_t ld) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y;
+  if (i >= n) return;
+  const int32_t m = colmed[j];
+  const double sg = conn_sign(m, same[i], other[i]);
+  const int64_t idx = j * ld + i;
+  if (sg == 0) {
+    if (outG.any()) outG.store(idx, mk<double>(0, 0));
+    if (outH.any()) outH.store(idx, mk<double>(0, 0));
+    return;
+  }
+  const double2 k = kt.k[m - 1];
+  const double dx = geo.pos[i] - geo.pos[j];
+  const double dy = geo.pos[n + i] - geo.pos[n + j];
+  const double dz = geo.pos[2 * n + i] - geo.pos[2 * n + j];
+  const double d = fmax(sqrt(dx * dx + dy * dy + dz * dz), 2.220446049250313e-16);
+  const double aj = geo.area[j];
+  const double2 ph = cexp_ikd(k, d);
+  if (outG.any()) {
+    const double gr = sg * (1.0 / d) * aj;
+    outG.store(idx, mk<double>(gr * ph.x, gr * ph.y));
+  }
+  if (outH.any()) {
+    const double id2 = 1.0 / (d * d);
+    const double2 fk = mk<double>((-k.y - 1.0 / d) * id2, k.x * id2);
+    const double in = sg * (dx * geo.nvec[i] + dy * geo.nvec[n + i] + dz * geo.nvec[2 * n + i]);
+    double2 F = cmul<double>(mk<double>(fk.x * in * aj, fk.y * in * aj), ph);
+    if (i == j) F.x += sgn2pi;
+    outH.store(idx, F);
+  }
+}
+template <class TG>
+__global__ void __launch_bounds__(256) k_assemble_refined_conn(int64_t nnz, int64_t n, Geo geo, Pairs pr, int order,
+                                                               const double* __restrict__ gt,
+                                                               const double* __restrict__ ft,
+                                                               const int32_t* __restrict__ colmed,
+                                                               const int32_t* __restrict__ same,
+                                                               const int32_t* __restrict__ other, KTab kt,
+                                                               double sgn2pi, TG outG, TG outH, int64_t ld) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= nnz) return;
+  const int64_t i = pr.row[p], j = pr.col[p];
+  const int32_t m = colmed[j];
+  const double sg = conn_sign(m, same[i], other[i]);
+  if (sg == 0) return;  // k_assemble_conn wrote the zero
+  const double2 k = kt.k[m - 1];
+  const double dx = geo.pos[i] - geo.pos[j];
+  const double dy = geo.pos[n + i] - geo.pos[n + j];
+  const double dz = geo.pos[2 * n + i] - geo.pos[2 * n + j];
+  const double d = fmax(sqrt(dx * dx + dy * dy + dz * dz), 2.220446049250313e-16);
+  const double2 ph = cexp_ikd(k, d);
+  const double2 ik = mk<double>(-k.y, k.x);
+  double2 pw = mk<double>(1, 0), G = mk<double>(0, 0), F = mk<double>(0, 0);
+  const double n0 = geo.nvec[i], n1 = geo.nvec[n + i], n2 = geo.nvec[2 * n + i];
+  for (int o = 0; o <= order; ++o) {
+    const double go = gt[o * nnz + p];
+    const double fo = n0 * ft[(o * 3 + 0) * nnz + p] + n1 * ft[(o * 3 + 1) * nnz + p] + n2 * ft[(o * 3 + 2) * nnz + p];
+    G = cadd<double>(G, cscale<double>(pw, go));
+    F = cadd<double>(F, cscale<double>(pw, fo));
+    pw = cmul<double>(pw, ik);
+  }
+  const int64_t idx = j * ld + i;
+  if (outG.any()) outG.store(idx, cscale<double>(cmul<double>(G, ph), sg));
+  if (outH.any()) {
+    double2 H = cscale<double>(cmul<double>(F, ph), sg);
+    if (pr.kind[p] == 2) H.x += sgn2pi;
+    outH.store(idx, H);
+  }
+}
+
 // ------------------------------------------------------------- host side
+void GreenRet::assemble_side(int side, const KTab& kt, OutPair g, OutPair h, int64_t ld) const {
+  MNP_CHECK(compound(), MNP_ERR_STATE, "greenret: assemble_side needs a compound");
+  MNP_CHECK(side == 1 || side == 2, MNP_ERR_ARG, "greenret: side must be 1 or 2");
+  MNP_CHECK(n <= 65535, MNP_ERR_ARG, "greenret: dense assembly supports at most 65535 faces");
+  if (!g.any() && !h.any()) return;
+  const int32_t* colmed = side == 1 ? fin : fout;
+  const int32_t* same = side == 1 ? fin : fout;
+  const int32_t* other = side == 1 ? fout : fin;
+  const double sgn2pi = side == 1 ? 2 * M_PI : -2 * M_PI;
+  const double bytes = (double)n * n * ((g.d ? 16 : 0) + (g.f ? 8 : 0) + (h.d ? 16 : 0) + (h.f ? 8 : 0));
+  TimerScope ts(MNP_TIMER_ASSEMBLE, bytes, bytes);
+  hipLaunchKernelGGL(k_assemble_conn<OutPair>, dim3((unsigned)cdiv_up(n, 256), (unsigned)n), dim3(256), 0, stream(),
+                     n, geo, colmed, same, other, kt, sgn2pi, g, h, ld);
+  MNP_HIP(hipGetLastError());
+  if (nnz) {
+    hipLaunchKernelGGL(k_assemble_refined_conn<OutPair>, dim3((unsigned)cdiv_up(nnz, 256)), dim3(256), 0, stream(),
+                       nnz, n, geo, pairs, order, gtab, ftab, colmed, same, other, kt, sgn2pi, g, h, ld);
+    MNP_HIP(hipGetLastError());
+  }
+}
+
 void GreenRet::assemble(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t ld) const {
   assemble_rows(k, sgn2pi, g, h, n, ld);
 }
@@ -733,7 +844,7 @@ void GreenRet::assemble_rows(double2 k, double sgn2pi, OutPair g, OutPair h, int
 }
 
 GreenRet::GreenRet(const double* verts, int64_t nv, const int64_t* faces, int64_t nf,
-                   const mnp_green_options& op_, const GreenRet* like)
+                   const mnp_green_options& op_, const GreenRet* like, const CompoundSpec* cs)
     : n(nf), op(op_) {
   MNP_CHECK(nf > 0 && nv >= 3, MNP_ERR_ARG, "greenret: empty mesh");
   MNP_CHECK(op.order >= 0 && op.order <= MNP_MAX_ORDER, MNP_ERR_ARG, "greenret: order out of range");
@@ -806,6 +917,37 @@ GreenRet::GreenRet(const double* verts, int64_t nv, const int64_t* faces, int64_
                        op.abs_cutoff, off, pairs);
     MNP_HIP(hipGetLastError());
   }
+  if (cs) {
+    const int np = cs->nparts;
+    MNP_CHECK(np >= 1 && np <= MNP_MAX_PARTS && cs->part_off && cs->inout && cs->cmask, MNP_ERR_ARG,
+              "greenret: compound needs 1..32 parts");
+    MNP_CHECK(cs->part_off[0] == 0 && cs->part_off[np] == nf, MNP_ERR_ARG, "greenret: part offsets must cover the faces");
+    nparts = np;
+    fin_h.resize(nf);
+    fout_h.resize(nf);
+    std::vector<int32_t> fp(nf);
+    for (int q = 0; q < np; ++q) {
+      MNP_CHECK(cs->part_off[q] <= cs->part_off[q + 1], MNP_ERR_ARG, "greenret: part offsets must ascend");
+      const int32_t a = cs->inout[2 * q], b = cs->inout[2 * q + 1];
+      MNP_CHECK(a >= 1 && b >= 1 && a <= MNP_MAX_MED && b <= MNP_MAX_MED && a != b, MNP_ERR_ARG,
+                "greenret: media must be distinct indices in 1..16");
+      nmed = std::max(nmed, (int)std::max(a, b));
+      for (int64_t i = cs->part_off[q]; i < cs->part_off[q + 1]; ++i) {
+        fin_h[i] = a;
+        fout_h[i] = b;
+        fp[i] = q;
+      }
+      for (int r = 0; r < np; ++r) con12 = con12 || a == cs->inout[2 * r + 1];
+    }
+    fin = dev_alloc<int32_t>(nf);
+    fout = dev_alloc<int32_t>(nf);
+    fpart = dev_alloc<int32_t>(nf);
+    cmask = dev_alloc<uint32_t>(np);
+    MNP_HIP(hipMemcpy(fin, fin_h.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice));
+    MNP_HIP(hipMemcpy(fout, fout_h.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice));
+    MNP_HIP(hipMemcpy(fpart, fp.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice));
+    MNP_HIP(hipMemcpy(cmask, cs->cmask, sizeof(uint32_t) * np, hipMemcpyHostToDevice));
+  }
   // refinement tables
   gtab = dev_alloc<double>(nnz * (order + 1));
   ftab = dev_alloc<double>(nnz * 3 * (order + 1));
@@ -823,9 +965,9 @@ GreenRet::GreenRet(const double* verts, int64_t nv, const int64_t* faces, int64_
     const size_t lds = sizeof(unsigned int) * (size_t)((n + 31) / 32);
     MNP_CHECK(lds <= 64 * 1024, MNP_ERR_ARG, "greenret: mesh too large for the sum-rule kernel");
     hipLaunchKernelGGL(k_closed_sum, dim3((unsigned)n), dim3(256), lds, s, n, geo, pairs, off, nnz,
-                       ftab, fsum);
+                       ftab, (const int32_t*)fpart, (const uint32_t*)cmask, fsum);
     hipLaunchKernelGGL(k_closed_apply, dim3((unsigned)cdiv_up(n, 256)), dim3(256), 0, s, n, geo,
-                       pairs, nnz, fsum, ftab);
+                       pairs, nnz, fsum, (const int32_t*)fpart, (const uint32_t*)cmask, ftab);
     MNP_HIP(hipGetLastError());
     MNP_HIP(hipStreamSynchronize(s));
     MNP_HIP(hipFree(fsum));
@@ -838,8 +980,8 @@ GreenRet::GreenRet(const double* verts, int64_t nv, const int64_t* faces, int64_
 }
 
 GreenRet::~GreenRet() {
-  void* ptrs[] = {geo.pos, geo.nvec, geo.t1, geo.t2, geo.vtx, geo.area, geo.qarea, geo.rad,
-                  off, pairs.row, pairs.col, pairs.kind, pairs.diag, gtab, ftab, q18, qpol};
+  void* ptrs[] = {geo.pos, geo.nvec, geo.t1, geo.t2, geo.vtx,   geo.area, geo.qarea, geo.rad, off, pairs.row,
+                  pairs.col, pairs.kind, pairs.diag, gtab, ftab, q18, qpol, fin, fout, fpart, cmask};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -901,6 +1043,38 @@ int mnp_greenret_create(const double* verts, int64_t nverts, const int64_t* face
     auto* h = new mnp_greenret_s{nullptr};
     try {
       h->g = new GreenRet(verts, nverts, faces, nfaces, o);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int mnp_greenret_create_compound(const double* verts, int64_t nverts, const int64_t* faces, int64_t nfaces,
+                                 int32_t nparts, const int64_t* part_offsets, const int32_t* inout,
+                                 const uint32_t* closed_mask, const mnp_green_options* op, mnp_greenret_t* out) {
+  return guarded([&] {
+    MNP_CHECK(verts && faces && out && part_offsets && inout && closed_mask, MNP_ERR_ARG,
+              "greenret_create_compound: NULL argument");
+    mnp_green_options o;
+    if (op)
+      o = *op;
+    else
+      mnp_green_options_default(&o);
+    // inout arrives column-major (nparts x 2); the spec takes (in, out) pairs
+    std::vector<int32_t> io(2 * (size_t)std::max(nparts, 0));
+    for (int32_t q = 0; q < nparts; ++q) {
+      io[2 * q] = inout[q];
+      io[2 * q + 1] = inout[nparts + q];
+    }
+    bool any_closed = false;
+    for (int32_t q = 0; q < nparts; ++q) any_closed = any_closed || closed_mask[q] != 0;
+    o.closed = any_closed ? 1 : 0;
+    const CompoundSpec cs{nparts, part_offsets, io.data(), closed_mask};
+    auto* h = new mnp_greenret_s{nullptr};
+    try {
+      h->g = new GreenRet(verts, nverts, faces, nfaces, o, nullptr, &cs);
     } catch (...) {
       delete h;
       throw;

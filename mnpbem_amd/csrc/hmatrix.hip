@@ -1140,6 +1140,7 @@ extern "C" {
 int mnp_hmat_create(mnp_greenret_t g, int precision, const int64_t* perm, int64_t nfull, const int64_t* full,
                     int64_t nlow, const int64_t* low, double htol, int kmax, mnp_hmat_t* out) {
   return guarded([&] {
+    MNP_CHECK(!g || !g->g->compound(), MNP_ERR_ARG, "hmat_create: compound particles take the dense solver (mnp_bemret_*)");
     MNP_CHECK(g && perm && out && (full || !nfull) && (low || !nlow), MNP_ERR_ARG, "hmat_create: NULL argument");
     MNP_CHECK(htol > 0, MNP_ERR_ARG, "hmat_create: htol must be positive");
     auto* h = new mnp_hmat_s{nullptr, {}};

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <memory>
+#include <vector>
 
 #include "mnp_chain.hpp"
 #include "mnp_linalg.hpp"
@@ -31,6 +32,7 @@ struct BemRet {
   std::unique_ptr<Chain<double>> c64;
   std::unique_ptr<Chain<float>> c32;
   double2 *G1 = nullptr, *G2 = nullptr, *H1 = nullptr, *H2 = nullptr;  // FP64 operator (mixed)
+  double2 *e1v = nullptr, *e2v = nullptr;  // eps inside / outside per face (compound)
   Workspace vws;  // vectors
   Workspace tws;  // tangent solve
   Workspace res_ws;
@@ -51,6 +53,8 @@ struct BemRet {
   // worker stream); false when initmat_finish must complete the chain.
   bool initmat_try(double enei_, double2 e1, double2 e2);
   void initmat_finish();
+  // compound particle: epsm[m - 1] the dielectric function of medium m
+  void initmat_media(double enei_, const std::vector<double2>& epsm);
   double norm2(const double2* v, int64_t len);
   void solve(int64_t p, const double2* const in[8], double2* sig1, double2* sig2, double2* h1, double2* h2);
   // x = A^{-1} b for b, x n x 8p (layouts of chain_apply); b is not modified

@@ -72,6 +72,24 @@ __device__ __forceinline__ int refine_kind(const Geo& geo, int64_t n, int64_t i,
   return (dd < absc || dd / geo.rad[i] < rel) ? 1 : 0;
 }
 
+// Several parts with their own media (Particles/@comparticle): faces of part
+// q are [part_off[q], part_off[q + 1]); inout[2 q], inout[2 q + 1] its inside
+// and outside media (1-based); bit r of cmask[q] set when part r belongs to
+// the closed surface of part q (Particles/@comparticle/closed.m; 0: part q
+// has no closed surface).
+struct CompoundSpec {
+  int nparts;
+  const int64_t* part_off;
+  const int32_t* inout;
+  const uint32_t* cmask;
+};
+constexpr int MNP_MAX_MED = 16;    // media of a compound
+constexpr int MNP_MAX_PARTS = 32;  // parts of a compound (bits of cmask)
+// Wavenumber of each medium, by value into the assembly kernels.
+struct KTab {
+  double2 k[MNP_MAX_MED];
+};
+
 struct GreenRet {
   int64_t n = 0;
   int64_t nnz = 0;
@@ -90,10 +108,22 @@ struct GreenRet {
   double* q18 = nullptr;         // device [x(28) | y(28) | w(28)]
   double* qpol = nullptr;        // device [x | y | w], MNP_MAX_POLAR each
 
+  // compound particle: device per-face media (1-based) and part, the closed
+  // surface of each part; all null for one particle
+  int nparts = 1;
+  int nmed = 0;                    // largest medium index of a compound
+  bool con12 = false;              // some part's inside medium is some part's outside medium
+  int32_t* fin = nullptr;
+  int32_t* fout = nullptr;
+  int32_t* fpart = nullptr;
+  uint32_t* cmask = nullptr;
+  std::vector<int32_t> fin_h, fout_h;
+  bool compound() const { return fin != nullptr; }
+
   // like: take the refined pairs of that Green function (same faces) instead
-  // of selecting them for this mesh
+  // of selecting them for this mesh; cs: the parts of a compound
   GreenRet(const double* verts, int64_t nv, const int64_t* faces, int64_t nf,
-           const mnp_green_options& op, const GreenRet* like = nullptr);
+           const mnp_green_options& op, const GreenRet* like = nullptr, const CompoundSpec* cs = nullptr);
   ~GreenRet();
   GreenRet(const GreenRet&) = delete;
   GreenRet& operator=(const GreenRet&) = delete;
@@ -104,6 +134,10 @@ struct GreenRet {
   void assemble_rows(double2 k, double sgn2pi, OutPair g, OutPair h, int64_t nrows, int64_t ld) const;
   // dG/dk into g and dF/dk (= dH/dk) into h, n x n.
   void assemble_dk(double2 k, OutPair g, OutPair h, int64_t ld) const;
+  // Compound: side 1 writes G1 = g{1,1} - g{2,1} and H1 = F1 + 2 pi, side 2
+  // G2 = g{2,2} - g{1,2} and H2 = F2 - 2 pi (BEM/@bemret/private/initmat.m:
+  // 27-31), kt.k[m - 1] the wavenumber of medium m.
+  void assemble_side(int side, const KTab& kt, OutPair g, OutPair h, int64_t ld) const;
 };
 
 }  // namespace mnp

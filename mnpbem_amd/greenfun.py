@@ -1,7 +1,8 @@
-"""Retarded Green function of one closed particle (compgreenret / greenret).
+"""Retarded Green function of a particle (compgreenret / greenret).
 
 Mirrors ``compgreenret(p, p, op)`` of MNPBEM (Greenfun/@compgreenret) for a
-single closed particle of flat triangles.  The object owns a device-resident
+closed particle of flat triangles, or a compound of several parts with their
+own media.  The object owns a device-resident
 ``mnp_greenret`` handle (geometry, refinement tables, sum rule) and evaluates
 ``G``, ``F``, ``H1``, ``H2`` on the GPU.
 """
@@ -22,9 +23,11 @@ class CompGreenRet:
     """
 
     def __init__(self, verts, faces, order=5, rel_cutoff=3.0, abs_cutoff=0.0, npol=(7, 5),
-                 closed=True, like=None):
+                 closed=True, like=None, compound=None):
         """``like``: a CompGreenRet of the same faces whose refined pairs are
-        taken instead of selecting them for this mesh (mnp_greenret_create_like)."""
+        taken instead of selecting them for this mesh (mnp_greenret_create_like).
+        ``compound``: (part offsets (nparts + 1), inout (nparts x 2), closed
+        masks (nparts)) of several parts (mnp_greenret_create_compound)."""
         dev.ensure()
         verts = np.asfortranarray(np.asarray(verts, dtype=np.float64))
         faces = np.asfortranarray(np.asarray(faces, dtype=np.int64))
@@ -33,7 +36,15 @@ class CompGreenRet:
         op = _lib.GreenOptions(int(order), float(rel_cutoff), float(abs_cutoff), int(npol[0]),
                                int(npol[1]), int(bool(closed)))
         h = C.c_void_p()
-        if like is None:
+        if compound is not None:
+            off, inout, cmask = compound
+            off = np.ascontiguousarray(off, dtype=np.int64)
+            io = np.asfortranarray(np.asarray(inout, dtype=np.int32))
+            cm = np.ascontiguousarray(cmask, dtype=np.uint32)
+            _lib.call("mnp_greenret_create_compound", verts.ctypes.data_as(C.c_void_p), verts.shape[0],
+                      faces.ctypes.data_as(C.c_void_p), faces.shape[0], len(cm), off.ctypes.data_as(C.c_void_p),
+                      io.ctypes.data_as(C.c_void_p), cm.ctypes.data_as(C.c_void_p), C.byref(op), C.byref(h))
+        elif like is None:
             _lib.call("mnp_greenret_create", verts.ctypes.data_as(C.c_void_p), verts.shape[0],
                       faces.ctypes.data_as(C.c_void_p), faces.shape[0], C.byref(op), C.byref(h))
         else:

@@ -36,10 +36,31 @@ def test_comparticle_media():
     assert p.n == 80
     assert p.eps2(500.0) == 1.0
     assert p.eps1(500.0) == B.epsdrude(10.0, 9.065, 0.0708)(500.0)[0]
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(ValueError):  # one inout row per part
         B.comparticle([B.epsconst(1.0)], [(v, f), (v, f)], [2, 1], 1)
     with pytest.raises(NotImplementedError):
         B.comparticle([B.epsconst(1.0)], [(v, f)], [2, 1], 1, B.bemoptions(interp="curv"))
+
+
+def test_compound_comparticle_host():
+    """Two parts (core-shell): concatenated mesh, face media, closed masks as
+    comparticle's closed arguments give them (Particles/@comparticle/closed.m)."""
+    v1, f1 = shapes.icosphere(1, 3.0)
+    v2, f2 = shapes.icosphere(1, 5.0)
+    eps = [B.epsconst(1.0), B.epsdrude(10.0, 9.065, 0.0708), B.epsconst(2.0)]
+    p = B.comparticle(eps, [(v1, f1), (v2, f2)], [[3, 2], [2, 1]], 1, 2, B.bemoptions())
+    assert p.compound and p.n == 160 and p.part_off == [0, 80, 160]
+    assert np.array_equal(p.verts[len(v1):], v2) and np.array_equal(p.faces[80:], f2 + len(v1))
+    fio = p.face_inout
+    assert np.all(fio[:80] == [3, 2]) and np.all(fio[80:] == [2, 1])
+    assert p.closed_mask.tolist() == [1, 2]
+    q = B.comparticle(eps, [(v1, f1), (v2, f2)], [[3, 2], [2, 1]], [1, 2])
+    assert q.closed_mask.tolist() == [3, 3]
+    assert [e for e in p.eps_media(500.0)][2] == 2.0
+    with pytest.raises(NotImplementedError):
+        p.eps1(500.0)
+    with pytest.raises(NotImplementedError):
+        B.bemsolver(p, B.bemoptions(sim="stat"))
 
 
 def test_planewave_rejects_parallel_polarisation():
