@@ -477,7 +477,7 @@ def run_multi_particle(per_particle=8, fp64_every=4):
     return out
 
 
-def run_hmatrix_dipole(nlam=16, seed=2024):
+def run_hmatrix_dipole(nlam=16, seed=2024, coarse=2):
     """H-matrix iterative path with dipole excitation: sphere of diameter 150 nm
     (icosphere level 6, 81,920 faces), Drude gold in eps=1, dipole 5 nm above
     the surface along a seeded random direction with a seeded random moment,
@@ -494,7 +494,8 @@ def run_hmatrix_dipole(nlam=16, seed=2024):
     u /= np.linalg.norm(u)
     dip = rng.normal(size=3)
     dip /= np.linalg.norm(dip)
-    op = B.bemoptions(sim="ret", iter=dict(cleaf=32, htol=1e-6, kmax=[4, 100], tol=1e-6, hprecision="fp32"))
+    op = B.bemoptions(sim="ret", iter=dict(cleaf=32, htol=1e-6, kmax=[4, 100], tol=1e-6, hprecision="fp32",
+                                           coarse=coarse))
     p = B.comparticle([B.epsconst(1.0), B.epsdrude(*DRUDE)], [(v, f)], [2, 1], 1, op)
     t0 = time.perf_counter()
     p.green
@@ -657,6 +658,8 @@ def main():
     ap.add_argument("--no-sensitivity", action="store_true", help="skip the Drude-gradient measurement")
     ap.add_argument("--hmatrix-only", action="store_true",
                     help="only the H-matrix dipole measurement (for profiling)")
+    ap.add_argument("--hmatrix-coarse", type=int, default=2,
+                    help="row tiles per coarse aggregate of the H-matrix preconditioner")
     ap.add_argument("--only", default=None,
                     help="comma list of side measurements to run alone (for profiling): "
                          "quasistatic, mirror, substrate, multi_particle, hmatrix_dipole, sensitivity")
@@ -677,7 +680,7 @@ def main():
         log("[bench] copy calibration: 3 copies of 1 GiB")
         del src, dst
     if a.hmatrix_only:
-        print(json.dumps({"hmatrix_dipole": run_hmatrix_dipole()}), flush=True)
+        print(json.dumps({"hmatrix_dipole": run_hmatrix_dipole(coarse=a.hmatrix_coarse)}), flush=True)
         return
     if a.only:
         runs = {"quasistatic": lambda: run_quasistatic(a.level), "mirror": run_mirror,

@@ -391,6 +391,12 @@ int mnp_bemretiter_destroy(mnp_bemretiter_t it);
  * followed by the block chain on what remains, z = P y + B^{-1}(v - A P y),
  * y = C^{-1} P^T v. */
 int mnp_bemretiter_set_options(mnp_bemretiter_t it, double tol, int maxit, int precond);
+/* Coarse level of precond 2: aggregates of tiles_per_aggregate consecutive
+ * row tiles (1..64; default 2).  Fewer, larger
+ * aggregates make the coarse chain cheaper and GMRES take more iterations.
+ * No counterpart in the reference (its preconditioner is the H-matrix chain
+ * of private/initprecond.m). */
+int mnp_bemretiter_set_coarse(mnp_bemretiter_t it, int tiles_per_aggregate);
 int mnp_bemretiter_initmat(mnp_bemretiter_t it, double enei, mnp_complex eps1, mnp_complex eps2);
 /* Same arguments as mnp_bemret_solve (npol <= 8). */
 int mnp_bemretiter_solve(mnp_bemretiter_t it, int64_t npol, const void* phi1, const void* phi1p,

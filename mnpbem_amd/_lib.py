@@ -91,6 +91,7 @@ PROTOTYPES = {
     "mnp_hmat_coarsen": (INT, [P, PI64, P, P]),
     "mnp_bemretiter_create": (INT, [P, INT, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
     "mnp_bemretiter_destroy": (INT, [P]),
+    "mnp_bemretiter_set_coarse": (INT, [P, INT]),
     "mnp_bemretiter_set_options": (INT, [P, D, INT, INT]),
     "mnp_bemretiter_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
     "mnp_bemretiter_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),

@@ -415,6 +415,9 @@ class bemretiter:
         self.handle = h
         _lib.call(self._PREFIX + "_set_options", h, float(it["tol"]), int(it["maxit"]),
                   self._precond_code(it["precond"]))
+        if "coarse" in it and self._PREFIX == "mnp_bemretiter":
+            # row tiles per aggregate of the two-level preconditioner's coarse level
+            _lib.call("mnp_bemretiter_set_coarse", h, int(it["coarse"]))
         self.enei = None
 
     # precond: 'hmat' (the bemiter default) couples the whole surface: the

@@ -315,6 +315,18 @@ __global__ void k_prolong(int64_t n, int64_t nc, const int32_t* perm, const int3
   const float2 v = y[q * nc + rowtile[c]];
   z[q * n + perm[c]] = mk<double>(v.x, v.y);
 }
+// Coarse operator of aggregates of `merge` consecutive tiles from the tile
+// one: Ca[a, b] = sum over the tiles s of a and t of b of C[s, t], in FP32
+// for the chain (nc tiles, na aggregates).
+__global__ void k_coarse_merge(int64_t nc, int64_t na, int merge, const double2* C, float2* Ca) {
+  const int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t b = blockIdx.y;
+  if (a >= na) return;
+  double2 acc = mk<double>(0, 0);
+  for (int64_t t = b * merge; t < min(nc, (b + 1) * merge); ++t)
+    for (int64_t s = a * merge; s < min(nc, (a + 1) * merge); ++s) acc = cadd<double>(acc, C[t * nc + s]);
+  Ca[b * na + a] = make_float2((float)acc.x, (float)acc.y);
+}
 // r = v - w ;  z += d
 __global__ void k_vsub(int64_t len, const double2* v, const double2* w, double2* r) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -339,9 +351,13 @@ struct BemRetIter {
   int64_t npb = 0;
   int precision;
   int prec_mode = 2;  // 0 none, 1 block Jacobi, 2 two levels (coarse chain + block Jacobi)
-  std::unique_ptr<Chain<float>> coarse;  // the chain on the row tiles of the H-matrices
-  double* cnvec = nullptr;               // mean normals of the tiles, nc x 3
-  int64_t nc = 0;
+  std::unique_ptr<Chain<float>> coarse;  // the chain on aggregates of the H-matrices' row tiles
+  double* cnvec = nullptr;               // mean normals of the aggregates, na x 3
+  int64_t nc = 0;                        // row tiles
+  int merge = 1;                         // row tiles per aggregate
+  int64_t na = 0;                        // aggregates
+  int32_t* d_aggbeg = nullptr;           // first cluster index of each aggregate (na + 1)
+  int32_t* d_rowagg = nullptr;           // aggregate of each cluster index (n)
   Workspace cws;
   double tol = 1e-6;
   int maxit = 100;
@@ -379,28 +395,52 @@ struct BemRetIter {
     d_pb = dev_alloc<int64_t>((int64_t)pb.size());
     MNP_HIP(hipMemcpy(d_pb, pb.data(), sizeof(int64_t) * pb.size(), hipMemcpyHostToDevice));
     prec = dev_alloc<float2>(npb * 5 * PB * PB);
-    // coarse level: the row tiles and their mean normals
     nc = G1->ntile;
-    std::vector<double> nv(3 * n), cn(3 * nc, 0.0);
-    MNP_HIP(hipMemcpy(nv.data(), g->geo.nvec, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
-    for (int64_t t = 0; t < nc; ++t) {
-      const int64_t a = G1->tilebeg_h[t], b = G1->tilebeg_h[t + 1];
-      for (int64_t c = a; c < b; ++c)
-        for (int d = 0; d < 3; ++d) cn[d * nc + t] += nv[d * n + perm[c]];
-      for (int d = 0; d < 3; ++d) cn[d * nc + t] /= (double)(b - a);
+    nv_h.resize(3 * n);
+    perm_h.assign(perm, perm + n);
+    MNP_HIP(hipMemcpy(nv_h.data(), g->geo.nvec, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    set_merge(2);  // measured on the 81,920-face sphere: 14 iterations instead of 10, 10% less time
+  }
+  std::vector<double> nv_h;
+  std::vector<int64_t> perm_h;
+  // Coarse level: aggregates of `m` consecutive row tiles (consecutive
+  // cluster indices, so spatially compact) and their mean normals.
+  void set_merge(int m) {
+    MNP_CHECK(m >= 1 && m <= 64, MNP_ERR_ARG, "bemretiter: 1..64 tiles per coarse aggregate");
+    if (m == merge && cnvec) return;
+    merge = m;
+    na = (nc + m - 1) / m;
+    std::vector<double> cn(3 * na, 0.0);
+    std::vector<int32_t> beg(na + 1), rowagg(n);
+    for (int64_t a = 0; a < na; ++a) {
+      const int64_t c0 = G1->tilebeg_h[a * m], c1 = G1->tilebeg_h[std::min<int64_t>(nc, (a + 1) * m)];
+      beg[a] = (int32_t)c0;
+      beg[a + 1] = (int32_t)c1;
+      for (int64_t c = c0; c < c1; ++c) {
+        rowagg[c] = (int32_t)a;
+        for (int d = 0; d < 3; ++d) cn[d * na + a] += nv_h[d * n + perm_h[c]];
+      }
+      for (int d = 0; d < 3; ++d) cn[d * na + a] /= (double)(c1 - c0);
     }
-    cnvec = dev_alloc<double>(3 * nc);
-    MNP_HIP(hipMemcpy(cnvec, cn.data(), sizeof(double) * 3 * nc, hipMemcpyHostToDevice));
+    for (void* q : {(void*)cnvec, (void*)d_aggbeg, (void*)d_rowagg})
+      if (q) MNP_HIP(hipFree(q));
+    cnvec = dev_alloc<double>(3 * na);
+    d_aggbeg = dev_alloc<int32_t>(na + 1);
+    d_rowagg = dev_alloc<int32_t>(n);
+    MNP_HIP(hipMemcpy(cnvec, cn.data(), sizeof(double) * 3 * na, hipMemcpyHostToDevice));
+    MNP_HIP(hipMemcpy(d_aggbeg, beg.data(), sizeof(int32_t) * (na + 1), hipMemcpyHostToDevice));
+    MNP_HIP(hipMemcpy(d_rowagg, rowagg.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    coarse.reset();
+    ready = false;
   }
   ~BemRetIter() {
-    if (prec) (void)hipFree(prec);
-    if (d_pb) (void)hipFree(d_pb);
-    if (cnvec) (void)hipFree(cnvec);
+    for (void* q : {(void*)prec, (void*)d_pb, (void*)cnvec, (void*)d_aggbeg, (void*)d_rowagg})
+      if (q) (void)hipFree(q);
   }
 
   // The coarse chain of this wavelength from the four filled H-matrices.
   void build_coarse(double2 e1, double2 e2) {
-    if (!coarse) coarse.reset(new Chain<float>(nc, false));
+    if (!coarse) coarse.reset(new Chain<float>(na, false));
     Chain<float>& c = *coarse;
     c.eps1 = e1;
     c.eps2 = e2;
@@ -411,7 +451,11 @@ struct BemRetIter {
     float2* dst[4] = {c.G1, c.G2, c.S1, c.S2};
     for (int i = 0; i < 4; ++i) {
       M[i]->coarsen(C, cws);
-      hipLaunchKernelGGL(k_c64_to_c32, dim3((unsigned)cdiv_up(nn, 256)), dim3(256), 0, stream(), nn, C, dst[i]);
+      if (merge == 1)
+        hipLaunchKernelGGL(k_c64_to_c32, dim3((unsigned)cdiv_up(nn, 256)), dim3(256), 0, stream(), nn, C, dst[i]);
+      else
+        hipLaunchKernelGGL(k_coarse_merge, dim3((unsigned)cdiv_up(na, 256), (unsigned)na), dim3(256), 0, stream(), nc,
+                           na, merge, C, dst[i]);
       MNP_HIP(hipGetLastError());
     }
     chain_build<float>(c, cnvec);
@@ -477,17 +521,17 @@ struct BemRetIter {
       return;
     }
     // x = P y + B^{-1} (b - A P y), y = C^{-1} P^T b
-    float2* yb = (float2*)cws.get(3, sizeof(float2) * (size_t)(8 * p * nc));
-    float2* yx = (float2*)cws.get(4, sizeof(float2) * (size_t)(8 * p * nc));
-    float2* ytmp = (float2*)cws.get(5, sizeof(float2) * (size_t)(18 * p * nc));
+    float2* yb = (float2*)cws.get(3, sizeof(float2) * (size_t)(8 * p * na));
+    float2* yx = (float2*)cws.get(4, sizeof(float2) * (size_t)(8 * p * na));
+    float2* ytmp = (float2*)cws.get(5, sizeof(float2) * (size_t)(18 * p * na));
     double2* w = (double2*)cws.get(6, sizeof(double2) * (size_t)len);
     double2* d = (double2*)cws.get(7, sizeof(double2) * (size_t)len);
-    hipLaunchKernelGGL(k_restrict, dim3((unsigned)cdiv_up(nc, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n,
-                       nc, (int64_t)(8 * p), G1->perm, G1->d_tilebeg, b, yb);
+    hipLaunchKernelGGL(k_restrict, dim3((unsigned)cdiv_up(na, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n,
+                       na, (int64_t)(8 * p), G1->perm, d_aggbeg, b, yb);
     MNP_HIP(hipGetLastError());
     chain_apply<float>(*coarse, cnvec, p, yb, yx, ytmp);
-    hipLaunchKernelGGL(k_prolong, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n, nc,
-                       G1->perm, G1->d_rowtile, yx, x);
+    hipLaunchKernelGGL(k_prolong, dim3((unsigned)cdiv_up(n, 256), (unsigned)(8 * p)), dim3(256), 0, stream(), n, na,
+                       G1->perm, d_rowagg, yx, x);
     MNP_HIP(hipGetLastError());
     afun(p, x, w, prod);
     const unsigned gl = (unsigned)cdiv_up(len, 256);
@@ -559,6 +603,13 @@ int mnp_bemretiter_set_options(mnp_bemretiter_t it, double tol, int maxit, int p
     it->b->maxit = maxit;
     MNP_CHECK(precond >= 0 && precond <= 2, MNP_ERR_ARG, "bemretiter_set_options: precond must be 0, 1 or 2");
     it->b->prec_mode = precond;
+  });
+}
+
+int mnp_bemretiter_set_coarse(mnp_bemretiter_t it, int tiles_per_aggregate) {
+  return guarded([&] {
+    MNP_CHECK(it, MNP_ERR_ARG, "bemretiter_set_coarse: NULL handle");
+    it->b->set_merge(tiles_per_aggregate);
   });
 }
 
