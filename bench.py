@@ -477,6 +477,34 @@ def run_multi_particle(per_particle=8, fp64_every=4):
     return out
 
 
+def run_batch_sweep(precision="mixed"):
+    """Only the batched part of run_multi_particle (tuning pass, then the
+    1,536 solves of the seeded sweep) in one precision, for profiling."""
+    import torch
+
+    from mnpbem_amd import bem as B
+    from mnpbem_amd.batch import plan_capacity, spectra
+    from mnpbem_amd.workloads import multi_particle_sweep
+
+    parts, enei = multi_particle_sweep()
+    exc = B.planewave([[1.0, 0, 0]], [[0, 0, 1.0]])
+    ps = [B.comparticle([B.epsconst(1.0), B.epsdrude(*q["drude"])], [(q["verts"], q["faces"])], [2, 1], 1)
+          for q in parts]
+    for p in ps:
+        p.green
+    caps = plan_capacity(ps, len(enei), precision)
+    spectra(ps, enei[:1], exc, precision=precision, capacity=caps)
+    torch.cuda.synchronize()
+    stats = {}
+    t = time.perf_counter()
+    spectra(ps, enei, exc, precision=precision, capacity=caps, stats=stats)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    return {"precision": precision, "particle_wavelengths": len(ps) * len(enei), "seconds": dt,
+            "particle_wavelengths_per_s": len(ps) * len(enei) / dt,
+            "per_face_count": {str(n): {k: v for k, v in st.items() if k != "variant_ms"} for n, st in stats.items()}}
+
+
 def run_hmatrix_dipole(nlam=16, seed=2024, coarse=2):
     """H-matrix iterative path with dipole excitation: sphere of diameter 150 nm
     (icosphere level 6, 81,920 faces), Drude gold in eps=1, dipole 5 nm above
@@ -662,7 +690,7 @@ def main():
                     help="row tiles per coarse aggregate of the H-matrix preconditioner")
     ap.add_argument("--only", default=None,
                     help="comma list of side measurements to run alone (for profiling): "
-                         "quasistatic, mirror, substrate, multi_particle, hmatrix_dipole, sensitivity")
+                         "quasistatic, mirror, substrate, multi_particle, hmatrix_dipole, sensitivity, batch_sweep")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy three times (counter calibration)")
     a = ap.parse_args()
@@ -685,7 +713,7 @@ def main():
     if a.only:
         runs = {"quasistatic": lambda: run_quasistatic(a.level), "mirror": run_mirror,
                 "substrate": lambda: run_substrate(a.level), "multi_particle": run_multi_particle,
-                "sensitivity": run_sensitivity,
+                "sensitivity": run_sensitivity, "batch_sweep": run_batch_sweep,
                 "hmatrix_dipole": run_hmatrix_dipole}
         names = [x.strip() for x in a.only.split(",") if x.strip()]
         bad = [x for x in names if x not in runs]
