@@ -568,6 +568,82 @@ int mnp_planewavestatlayer_potential(mnp_greenret_t g, int64_t npol, const mnp_c
  * (@planewavestatlayer/extinction.m:13, scattering.m:12). */
 int mnp_stat_dipole_moment(mnp_greenret_t g, int64_t npol, const void* sig, mnp_complex* dip);
 
+/* ---------------------------------------------- retarded substrate
+ * One planar interface at z0 between an upper medium (eps_up) and a lower
+ * one (eps_down), particles above it: Particles/@layerstructure with one z
+ * (options of layerstructure.options: ztol, rmin, zmin, semi, ratio),
+ * Greenfun/@greentablayer, @greenretlayer, @compgreenretlayer,
+ * BEM/@bemretlayer, Simulation/retarded/@planewaveretlayer and
+ * @spectrumretlayer.  Faces lying in the interface are not supported. */
+typedef struct {
+  double z0, ztol, rmin, zmin, semi, ratio;
+} mnp_layer_options;
+/* layerstructure.options defaults (Particles/@layerstructure/options.m:22-29), z0 = 0. */
+void mnp_layer_options_default(mnp_layer_options* op);
+
+/* Table of the reflected Green function for sources at height z2 (host r:
+ * nr increasing radii, z1: nz increasing observer heights, all above z0)
+ * (Greenfun/@greentablayer/greentablayer.m, private/init.m:8-17). */
+typedef struct mnp_layertab_s* mnp_layertab_t;
+int mnp_layertab_create(const mnp_layer_options* op, int64_t nr, const double* r, int64_t nz,
+                        const double* z1, double z2, mnp_layertab_t* out);
+int mnp_layertab_destroy(mnp_layertab_t t);
+/* Tables at the wavelengths enei (host, increasing; eps_up/eps_down there):
+ * greentablayer/set.m.  The Sommerfeld integrals of Particles/@layerstructure/
+ * green.m:22-127 run on the GPU, one wavefront per grid point. */
+int mnp_layertab_set(mnp_layertab_t t, int64_t nenei, const double* enei, const mnp_complex* eps_up,
+                     const mnp_complex* eps_down);
+/* The table at enei (greentablayer/eval.m:17-50: new when nothing is set,
+ * else linear in wavelength between the set tables): out (device, may be
+ * NULL) nr x nz x 15 complex128 at (i nz + j) 15 + q, q = 0..4 d G,
+ * 5..9 d^3 Fr / r, 10..14 d^3 Fz / zmin of the components p, ss, hs, sh, hh
+ * (greentablayer/norm.m:12-27). */
+int mnp_layertab_eval(mnp_layertab_t t, double enei, mnp_complex eps_up, mnp_complex eps_down,
+                      void* out);
+
+/* Reflected Green function of a particle with itself on a table
+ * (Greenfun/@greenretlayer/greenretlayer.m, private/init.m, init2.m): the
+ * near pairs of green.refinematrixlayer (RelCutoff/AbsCutoff of g's options)
+ * get their shape-function weights once here. */
+typedef struct mnp_greenretlayer_s* mnp_greenretlayer_t;
+int mnp_greenretlayer_create(mnp_greenret_t g, mnp_layertab_t tab, mnp_greenretlayer_t* out);
+int mnp_greenretlayer_destroy(mnp_greenretlayer_t gl);
+/* Refined off-diagonal pairs and diagonal elements (host out). */
+int mnp_greenretlayer_refined(mnp_greenretlayer_t gl, int64_t* npairs, int64_t* ndiag);
+/* Reflected G and F = nvec . Gp at enei (greenretlayer/private/initrefl2.m):
+ * G, F device out 5 x n x n complex128 (component c at c n^2), components
+ * p, ss, hs, sh, hh.  Fails when a pair lies outside the table. */
+int mnp_greenretlayer_eval(mnp_greenretlayer_t gl, double enei, mnp_complex eps_up,
+                           mnp_complex eps_down, void* G, void* F);
+
+/* Dense retarded solver above the substrate, complex FP64 (BEM/@bemretlayer:
+ * private/initmat.m:17-95 with unique eps inside and outside, mldivide.m);
+ * the outside medium is the upper one.  Arguments of solve as
+ * mnp_bemret_solve. */
+typedef struct mnp_bemretlayer_s* mnp_bemretlayer_t;
+int mnp_bemretlayer_create(mnp_greenret_t g, mnp_greenretlayer_t gl, mnp_bemretlayer_t* out);
+int mnp_bemretlayer_destroy(mnp_bemretlayer_t bem);
+int mnp_bemretlayer_initmat(mnp_bemretlayer_t bem, double enei, mnp_complex eps_in, mnp_complex eps_up,
+                            mnp_complex eps_down);
+int mnp_bemretlayer_solve(mnp_bemretlayer_t bem, int64_t npol, const void* phi1, const void* phi1p,
+                          const void* a1, const void* a1p, const void* phi2, const void* phi2p,
+                          const void* a2, const void* a2p, void* sig1, void* sig2, void* h1, void* h2);
+/* Outside potentials of plane waves above the substrate at the faces
+ * (Simulation/retarded/@planewaveretlayer/potential.m:20-105 with
+ * Particles/@layerstructure/fresnel.m; the normal derivative of the phase
+ * factors is exact where the reference differences with a 1e-8 step):
+ * pol, dir host npol x 3; phi2, phi2p device n x npol, a2, a2p n x 3 x npol. */
+int mnp_planewaveretlayer_potential(mnp_greenret_t g, const mnp_layer_options* op, double enei,
+                                    mnp_complex eps_up, mnp_complex eps_down, int64_t npol,
+                                    const double* pol, const double* dir, void* phi2, void* phi2p,
+                                    void* a2, void* a2p);
+/* Far fields with the interface (Simulation/retarded/@spectrumretlayer/
+ * farfield.m:20-104): directions dir device ndir x 3; e, h device out
+ * ndir x 3 x npol (npol <= 8). */
+int mnp_farfield_layer(mnp_greenret_t g, const mnp_layer_options* op, double enei, mnp_complex eps_up,
+                       mnp_complex eps_down, int64_t ndir, const void* dir, int64_t npol,
+                       const void* sig2, const void* h2, void* e, void* h);
+
 /* ---------------------------------------------- quasistatic dipoles
  * phi' = -nvec . E at the faces (device out n x npt ndip, column ipt + npt
  * idip) of dipoles dip (host npt x 3 x ndip) at pts (host npt x 3), E = w

@@ -31,6 +31,11 @@ class GreenOptions(C.Structure):
                 ("npol_rho", C.c_int), ("npol_phi", C.c_int), ("closed", C.c_int)]
 
 
+class LayerOptions(C.Structure):
+    _fields_ = [("z0", C.c_double), ("ztol", C.c_double), ("rmin", C.c_double), ("zmin", C.c_double),
+                ("semi", C.c_double), ("ratio", C.c_double)]
+
+
 class Timers(C.Structure):
     _fields_ = [("launches", C.c_int64 * 7), ("ms", C.c_double * 7), ("work", C.c_double * 7),
                 ("bytes", C.c_double * 7)]
@@ -99,6 +104,23 @@ PROTOTYPES = {
     "mnp_bemstatlayer_create": (INT, [P, P, INT, C.POINTER(P)]),
     "mnp_bemstatlayer_initmat": (INT, [P, D, MnpComplex, MnpComplex, MnpComplex, MnpComplex]),
     "mnp_planewavestatlayer_potential": (INT, [P, I64, P, P]),
+    "mnp_layer_options_default": (None, [C.POINTER(LayerOptions)]),
+    "mnp_layertab_create": (INT, [C.POINTER(LayerOptions), I64, P, I64, P, D, C.POINTER(P)]),
+    "mnp_layertab_destroy": (INT, [P]),
+    "mnp_layertab_set": (INT, [P, I64, P, P, P]),
+    "mnp_layertab_eval": (INT, [P, D, MnpComplex, MnpComplex, P]),
+    "mnp_greenretlayer_create": (INT, [P, P, C.POINTER(P)]),
+    "mnp_greenretlayer_destroy": (INT, [P]),
+    "mnp_greenretlayer_refined": (INT, [P, PI64, PI64]),
+    "mnp_greenretlayer_eval": (INT, [P, D, MnpComplex, MnpComplex, P, P]),
+    "mnp_bemretlayer_create": (INT, [P, P, C.POINTER(P)]),
+    "mnp_bemretlayer_destroy": (INT, [P]),
+    "mnp_bemretlayer_initmat": (INT, [P, D, MnpComplex, MnpComplex, MnpComplex]),
+    "mnp_bemretlayer_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mnp_planewaveretlayer_potential": (INT, [P, C.POINTER(LayerOptions), D, MnpComplex, MnpComplex, I64, P, P,
+                                              P, P, P, P]),
+    "mnp_farfield_layer": (INT, [P, C.POINTER(LayerOptions), D, MnpComplex, MnpComplex, I64, P, I64, P, P, P,
+                                 P]),
     "mnp_stat_dipole_moment": (INT, [P, I64, P, P]),
     "mnp_bemretmirror_create": (INT, [P, INT, INT, C.POINTER(INT), C.POINTER(P)]),
     "mnp_bemretmirror_destroy": (INT, [P]),

@@ -752,7 +752,7 @@ class bemstatmirror:
 def bemsolver(p, op=None, **kw):
     """Base/bemsolver.m: bemret for op.sim == 'ret', bemstat for op.sim ==
     'stat'; bemretiter / bemstatiter when op.iter is set; bemretmirror for a
-    comparticlemirror."""
+    comparticlemirror; bemstatlayer / bemretlayer with op.layer."""
     op = dict(p.op if op is None else op)
     op.update(kw)
     sim = op.get("sim", "ret")
@@ -775,7 +775,10 @@ def bemsolver(p, op=None, **kw):
             return bemstatlayer(p, op)
         return bemstatiter(p, op) if op.get("iter") else bemstat(p, op)
     if op.get("layer") is not None:
-        raise NotImplementedError("substrate: only the quasistatic solver is built")
+        if op.get("iter") or sim != "ret":
+            raise NotImplementedError("substrate: the dense quasistatic and retarded solvers are built")
+        from .layer import bemretlayer
+        return bemretlayer(p, op)
     if sim != "ret":
         raise NotImplementedError(f"unknown simulation type {sim!r}")
     if op.get("iter"):
@@ -1092,7 +1095,10 @@ def planewave(pol, dir=None, op=None, **kw):
             return planewavestatlayer(pol, dir, op, **kw)
         return planewavestat(pol, op, **kw)
     if op is not None and op.get("layer") is not None:
-        raise NotImplementedError("substrate: only the quasistatic plane wave is built")
+        if op.get("sym"):
+            raise NotImplementedError("mirror symmetry with layers is not built")
+        from .layer import planewaveretlayer
+        return planewaveretlayer(pol, dir, op, **kw)
     if op is not None and op.get("sym"):
         return planewaveretmirror(pol, dir, op, **kw)
     return planewaveret(pol, dir, op, **kw)

@@ -1,0 +1,1378 @@
+// Retarded particle above a substrate (one planar interface at z0, the
+// particle in the upper medium), MNPBEM's bemretlayer path:
+//
+//   * the reflected Green function as Sommerfeld integrals over k_parallel
+//     on the path of Particles/@layerstructure/green.m:39-127 (semi-ellipse
+//     to 2 k1max, then the real axis when Z >= r / ratio, else a Hankel
+//     split up and down from 2 k1max; intbessel.m, inthankel.m), with the
+//     coefficients of reflectionsubs.m:27-60.  One wavefront per table point
+//     integrates all 15 functions (G, dG/dr, dG/dz of the components p, ss,
+//     hs, sh, hh) by composite Gauss-Legendre panels in FP64; Bessel and
+//     Hankel functions of complex argument by their power series (|z| < 2)
+//     and Watson's integral with generalized Gauss-Laguerre nodes;
+//   * the normalised tables d G, d^3 Fr / r, d^3 Fz / zmin on the (r, z1)
+//     grid of Greenfun/@greentablayer (norm.m), saved at several wavelengths
+//     and interpolated linearly in between (eval.m:24-50);
+//   * the reflected matrices of the particle with itself
+//     (Greenfun/@greenretlayer/private/initrefl2.m): bilinear interpolation
+//     (private/interp2.m, Misc/@igrid2/finterp.m) at the centroids, the
+//     shape-function refinement of near pairs (init2.m:23-99) and the polar
+//     quadrature of diagonal elements (initrefl2.m:39-99);
+//   * the chain of BEM/@bemretlayer/private/initmat.m:17-95 and mldivide.m
+//     for unique eps inside and outside, in complex FP64 on the library's
+//     GEMM and LU kernels;
+//   * the plane wave's potentials at the faces (Simulation/retarded/
+//     @planewaveretlayer/potential.m with Particles/@layerstructure/
+//     fresnel.m) and the far field with the interface (@spectrumretlayer/
+//     farfield.m).
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "mnp_chain.hpp"
+#include "mnp_common.hpp"
+#include "mnp_greenret.hpp"
+#include "mnp_linalg.hpp"
+
+namespace mnp {
+namespace {
+
+constexpr int NGL = 16;             // Gauss-Legendre nodes per panel
+constexpr int NLAG = 32;            // generalized Gauss-Laguerre nodes of Watson's integral
+constexpr int PANELS_ELLIPSE = 16;  // panels of the semi-ellipse leg
+constexpr double TAIL_DECAY = 40.0; // tail legs end at exp(-40)
+constexpr int NQ = 15;              // G, Fr, Fz x (p, ss, hs, sh, hh)
+
+__constant__ double c_gl[2][NGL];        // nodes on [0, 1], weights
+__constant__ double c_lag[2][2][NLAG];   // alpha = -1/2, +1/2: nodes, weights / Gamma(alpha + 1)
+
+// Gauss rule of a Jacobi matrix (diagonal a, off-diagonal b[k] coupling k-1
+// and k, total mass mu0): eigenvalues by Sturm-sequence bisection, weights
+// mu0 / sum_k q_k(x)^2 from the three-term recurrence (Christoffel).
+void jacobi_rule(int n, const std::vector<double>& a, const std::vector<double>& b, double mu0, double* x,
+                 double* w) {
+  double lo = 1e300, hi = -1e300;
+  for (int k = 0; k < n; ++k) {
+    const double rad = (k > 0 ? std::fabs(b[k]) : 0) + (k + 1 < n ? std::fabs(b[k + 1]) : 0);
+    lo = std::min(lo, a[k] - rad);
+    hi = std::max(hi, a[k] + rad);
+  }
+  auto count_below = [&](double t) {
+    int c = 0;
+    double d = a[0] - t;
+    if (d < 0) ++c;
+    for (int k = 1; k < n; ++k) {
+      if (d == 0) d = 1e-300;
+      d = (a[k] - t) - b[k] * b[k] / d;
+      if (d < 0) ++c;
+    }
+    return c;
+  };
+  for (int k = 0; k < n; ++k) {
+    double l = lo, h = hi;
+    for (int it = 0; it < 200; ++it) {
+      const double m = 0.5 * (l + h);
+      if (m <= l || m >= h) break;
+      if (count_below(m) > k)
+        h = m;
+      else
+        l = m;
+    }
+    const double t = 0.5 * (l + h);
+    double qm = 0, q = 1, s = 1;
+    for (int j = 0; j + 1 < n; ++j) {
+      const double qn = ((t - a[j]) * q - (j > 0 ? b[j] * qm : 0.0)) / b[j + 1];
+      qm = q;
+      q = qn;
+      s += q * q;
+    }
+    x[k] = t;
+    w[k] = mu0 / s;
+  }
+}
+
+// The rules into constant memory, once per device.
+void upload_rules() {
+  static std::mutex mu;
+  static std::vector<int> devs;
+  std::lock_guard<std::mutex> lock(mu);
+  int dev = 0;
+  MNP_HIP(hipGetDevice(&dev));
+  if (std::find(devs.begin(), devs.end(), dev) != devs.end()) return;
+  std::vector<double> a(NLAG), b(NLAG + 1, 0.0);
+  double gl[2][NGL];
+  {
+    std::vector<double> al(NGL, 0.0), bl(NGL + 1, 0.0);
+    for (int k = 1; k < NGL; ++k) bl[k] = k / std::sqrt(4.0 * k * k - 1.0);
+    double x[NGL], w[NGL];
+    jacobi_rule(NGL, al, bl, 2.0, x, w);
+    for (int k = 0; k < NGL; ++k) {
+      gl[0][k] = 0.5 * (x[k] + 1);
+      gl[1][k] = 0.5 * w[k];
+    }
+  }
+  double lag[2][2][NLAG];
+  for (int s = 0; s < 2; ++s) {
+    const double alpha = s == 0 ? -0.5 : 0.5;
+    for (int k = 0; k < NLAG; ++k) a[k] = 2.0 * k + alpha + 1;
+    for (int k = 1; k < NLAG; ++k) b[k] = std::sqrt(k * (k + alpha));
+    jacobi_rule(NLAG, a, b, 1.0, lag[s][0], lag[s][1]);
+  }
+  MNP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gl), gl, sizeof gl));
+  MNP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lag), lag, sizeof lag));
+  devs.push_back(dev);
+}
+
+// ------------------------------------------------------ complex helpers
+__host__ __device__ inline double2 csqrt_p(double2 z) {  // principal branch, signed zeros respected
+  const double r = hypot(z.x, z.y);
+  const double t = sqrt(0.5 * (r + fabs(z.x)));
+  if (t == 0) return mk<double>(0, z.y);
+  if (z.x >= 0) return mk<double>(t, z.y / (2 * t));
+  return mk<double>(fabs(z.y) / (2 * t), copysign(t, z.y));
+}
+__device__ inline double2 cexp_(double2 z) {
+  double s, c;
+  sincos(z.y, &s, &c);
+  const double m = exp(z.x);
+  return mk<double>(m * c, m * s);
+}
+__device__ inline double2 clog_(double2 z) { return mk<double>(log(hypot(z.x, z.y)), atan2(z.y, z.x)); }
+__device__ inline double2 conj_(double2 z) { return mk<double>(z.x, -z.y); }
+__device__ inline double2 mul_i(double2 z) { return mk<double>(-z.y, z.x); }
+__device__ inline double2 cfma(double2 acc, double2 a, double2 b) { return cadd<double>(acc, cmul<double>(a, b)); }
+
+// kz = sqrt(k^2 - kpar^2) times sign(Im kz + 1e-10) (reflectionsubs.m:24-25)
+__device__ inline double2 kz_of(double2 k, double2 kp) {
+  const double2 kz = csqrt_p(csub<double>(cmul<double>(k, k), cmul<double>(kp, kp)));
+  const double s = kz.y + 1e-10;
+  return s > 0 ? kz : (s < 0 ? mk<double>(-kz.x, -kz.y) : mk<double>(0, 0));
+}
+
+// ------------------------------------------------------ Bessel functions
+// J0, J1, Y0, Y1 by their power series, for |z| < 2.
+__device__ void bessel_series(double2 z, double2& J0, double2& J1, double2& Y0, double2& Y1) {
+  const double EG = 0.5772156649015329;
+  const double2 q = cscale<double>(cmul<double>(z, z), 0.25);
+  const double2 mq = mk<double>(-q.x, -q.y);
+  double2 t0 = mk<double>(1, 0), t1 = mk<double>(1, 0);
+  double2 s0 = mk<double>(0, 0), s1 = mk<double>(0, 0), y0 = mk<double>(0, 0), y1 = mk<double>(0, 0);
+  double H = 0;
+  for (int m = 0; m < 22; ++m) {
+    if (m > 0) {
+      t0 = cscale<double>(cmul<double>(t0, mq), 1.0 / ((double)m * m));
+      t1 = cscale<double>(cmul<double>(t1, mq), 1.0 / ((double)m * (m + 1)));
+      H += 1.0 / m;
+    }
+    s0 = cadd<double>(s0, t0);
+    s1 = cadd<double>(s1, t1);
+    y0 = csub<double>(y0, cscale<double>(t0, H));
+    y1 = cadd<double>(y1, cscale<double>(t1, 2 * (-EG + H) + 1.0 / (m + 1)));
+  }
+  const double2 zh = cscale<double>(z, 0.5);
+  J0 = s0;
+  J1 = cmul<double>(s1, zh);
+  const double2 L = clog_(zh);
+  Y0 = cscale<double>(cadd<double>(cmul<double>(cadd<double>(L, mk<double>(EG, 0)), J0), y0), 2 / M_PI);
+  Y1 = csub<double>(csub<double>(cscale<double>(cmul<double>(L, J1), 2 / M_PI),
+                                 cdiv<double>(mk<double>(2 / M_PI, 0), z)),
+                    cscale<double>(cmul<double>(zh, y1), 1 / M_PI));
+}
+
+// Hankel functions of the first kind H0, H1 for |z| >= 2, -pi/2 < arg z < 3 pi/2,
+// by Watson's integral H_n(z) = sqrt(2 / (pi z)) exp(i (z - n pi/2 - pi/4)) /
+// Gamma(n + 1/2) int_0^inf exp(-u) u^(n - 1/2) (1 + i u / (2 z))^(n - 1/2) du.
+__device__ void hankel1_large(double2 z, double2& H0, double2& H1) {
+  const double2 pre = cmul<double>(csqrt_p(cdiv<double>(mk<double>(2 / M_PI, 0), z)),
+                                   cexp_(mul_i(mk<double>(z.x - 0.25 * M_PI, z.y))));
+  const double2 iz2 = cdiv<double>(mk<double>(0, 0.5), z);  // i / (2 z)
+  double2 a0 = mk<double>(0, 0), a1 = mk<double>(0, 0);
+  for (int k = 0; k < NLAG; ++k) {
+    const double2 w0 = cadd<double>(mk<double>(1, 0), cscale<double>(iz2, c_lag[0][0][k]));
+    const double2 w1 = cadd<double>(mk<double>(1, 0), cscale<double>(iz2, c_lag[1][0][k]));
+    a0 = cadd<double>(a0, cscale<double>(cdiv<double>(mk<double>(1, 0), csqrt_p(w0)), c_lag[0][1][k]));
+    a1 = cadd<double>(a1, cscale<double>(csqrt_p(w1), c_lag[1][1][k]));
+  }
+  H0 = cmul<double>(pre, a0);
+  H1 = mk<double>(0, 0);
+  const double2 t = cmul<double>(pre, a1);
+  H1 = mk<double>(t.y, -t.x);  // times exp(-i pi / 2)
+}
+
+__device__ void hankel1(double2 z, double2& H0, double2& H1) {
+  if (hypot(z.x, z.y) < 2) {
+    double2 J0, J1, Y0, Y1;
+    bessel_series(z, J0, J1, Y0, Y1);
+    H0 = cadd<double>(J0, mul_i(Y0));
+    H1 = cadd<double>(J1, mul_i(Y1));
+  } else {
+    hankel1_large(z, H0, H1);
+  }
+}
+
+// J0, J1 for complex z near the real axis: series below |z| = 2, else
+// (H1(z) + conj(H1(conj z))) / 2.
+__device__ void bessel_j(double2 z, double2& J0, double2& J1) {
+  if (hypot(z.x, z.y) < 2) {
+    double2 Y0, Y1;
+    bessel_series(z, J0, J1, Y0, Y1);
+    return;
+  }
+  double2 a0, a1, b0, b1;
+  hankel1_large(z, a0, a1);
+  hankel1_large(conj_(z), b0, b1);
+  J0 = cscale<double>(cadd<double>(a0, conj_(b0)), 0.5);
+  J1 = cscale<double>(cadd<double>(a1, conj_(b1)), 0.5);
+}
+
+// ------------------------------------------------------ table kernel
+struct LayPrm {
+  double k0;
+  double2 e1, e2, k1, k2;  // upper and lower media
+  double k1max, semi, ratio;
+};
+
+// Coefficient [0, 0] (both points above) of every component at kpar
+// (reflectionsubs.m:27-60), order p, ss, hs, sh, hh; returns k1z.
+__device__ double2 coeff_upper(const LayPrm& P, double2 kp, double2 c[5]) {
+  const double2 k1z = kz_of(P.k1, kp), k2z = kz_of(P.k2, kp);
+  const double2 sum = cadd<double>(k1z, k2z), dif = csub<double>(k1z, k2z);
+  const double2 D = cmul<double>(cadd<double>(k2z, k1z), cadd<double>(cmul<double>(P.e1, k2z), cmul<double>(P.e2, k1z)));
+  const double2 e1k1 = cmul<double>(P.e1, k1z), e2k1 = cmul<double>(P.e2, k1z), e1k2 = cmul<double>(P.e1, k2z);
+  c[0] = cdiv<double>(dif, cadd<double>(k2z, k1z));
+  c[1] = cdiv<double>(cmul<double>(sum, csub<double>(csub<double>(cscale<double>(e1k1, 2), e2k1), e1k2)), D);
+  const double2 de = csub<double>(P.e2, P.e1);
+  c[2] = cdiv<double>(cscale<double>(cmul<double>(cmul<double>(de, P.e1), k1z), -2 * P.k0), D);
+  c[3] = cdiv<double>(cscale<double>(cmul<double>(de, k1z), -2 * P.k0), D);
+  c[4] = cdiv<double>(cmul<double>(dif, cadd<double>(csub<double>(cscale<double>(e1k1, 2), e2k1), e1k2)), D);
+  return k1z;
+}
+
+// Adds w times the integrands of G, Fr, Fz at kpar (intbessel.m:45-58).
+__device__ void add_integrand(const LayPrm& P, double Z, double2 kp, double2 w, double2 b0, double2 b1,
+                              double2 acc[NQ]) {
+  double2 c[5];
+  const double2 k1z = coeff_upper(P, kp, c);
+  const double2 ph = cexp_(mul_i(cscale<double>(k1z, Z)));
+  const double2 base = cdiv<double>(kp, k1z);                        // kpar / k1z
+  const double2 wg = cmul<double>(w, mul_i(cmul<double>(b0, base)));  // i J0 kpar / k1z
+  const double2 wr = cmul<double>(w, mul_i(cmul<double>(b1, cmul<double>(mk<double>(-kp.x, -kp.y), base))));
+  const double2 wz = cmul<double>(w, cmul<double>(mk<double>(-b0.x, -b0.y), kp));
+  for (int q = 0; q < 5; ++q) {
+    const double2 R = cmul<double>(c[q], ph);
+    acc[q] = cfma(acc[q], wg, R);
+    acc[5 + q] = cfma(acc[5 + q], wr, R);
+    acc[10 + q] = cfma(acc[10 + q], wz, R);
+  }
+}
+
+// One wavefront per table point (r_i, Z_j): the integrals of the three legs
+// and the normalised values d G, d^3 Fr / r, d^3 Fz / Z at out[(i nz + j) 15 + q].
+__global__ void __launch_bounds__(64) k_layer_table(int64_t nr, int64_t nz, const double* __restrict__ rg,
+                                                    const double* __restrict__ Zt, LayPrm P,
+                                                    double2* __restrict__ out) {
+  const int64_t pt = blockIdx.x;
+  const int64_t i = pt / nz, j = pt % nz;
+  const double r = rg[i], Z = Zt[j];
+  const int lane = threadIdx.x;
+  double2 acc[NQ];
+  for (int q = 0; q < NQ; ++q) acc[q] = mk<double>(0, 0);
+  // semi-ellipse kpar = k1max (1 - cos t - i semi sin t), t in [0, pi]
+  for (int g = lane; g < PANELS_ELLIPSE * NGL; g += 64) {
+    const int p = g / NGL, q = g % NGL;
+    const double t = (p + c_gl[0][q]) * M_PI / PANELS_ELLIPSE;
+    const double wt = c_gl[1][q] * M_PI / PANELS_ELLIPSE;
+    const double st = sin(t), ct = cos(t);
+    const double2 kp = mk<double>(P.k1max * (1 - ct), -P.k1max * P.semi * st);
+    const double2 w = mk<double>(P.k1max * st * wt, -P.k1max * P.semi * ct * wt);
+    double2 J0, J1;
+    bessel_j(cscale<double>(kp, r), J0, J1);
+    add_integrand(P, Z, kp, w, J0, J1, acc);
+  }
+  // tail: the real axis (Z >= r / ratio) or the Hankel leg, panels of width
+  // k1max / 4 doubling up to the decay length
+  const bool real_leg = Z >= r / P.ratio;
+  const double scale = 1.0 / (real_leg ? Z : r);
+  const double L = TAIL_DECAY * scale;
+  double e = 0, h = 0.25 * P.k1max;
+  const int sub = lane / NGL, q = lane % NGL;
+  for (int p = 0; e < L; ++p) {
+    const double e1 = fmin(e + fmin(h, scale), L);
+    if ((p & 3) == sub) {
+      const double hp = e1 - e;
+      const double s = e + hp * c_gl[0][q], ws = hp * c_gl[1][q];
+      if (real_leg) {
+        const double2 kp = mk<double>(2 * P.k1max + s, 0);
+        double2 J0, J1;
+        bessel_j(mk<double>(kp.x * r, 0), J0, J1);
+        add_integrand(P, Z, kp, mk<double>(ws, 0), J0, J1, acc);
+      } else {
+        // 0.5 H1(kpar r) f(kpar) dkpar + 0.5 H2(kpar* r) f(kpar*) dkpar*
+        const double2 kp = mk<double>(2 * P.k1max, s);
+        double2 H0, H1;
+        hankel1(cscale<double>(kp, r), H0, H1);
+        add_integrand(P, Z, kp, mk<double>(0, 0.5 * ws), H0, H1, acc);
+        add_integrand(P, Z, conj_(kp), mk<double>(0, -0.5 * ws), conj_(H0), conj_(H1), acc);
+      }
+    }
+    e = e1;
+    h *= 2;
+  }
+  for (int qq = 0; qq < NQ; ++qq)
+    for (int off = 32; off > 0; off >>= 1) {
+      acc[qq].x += __shfl_xor(acc[qq].x, off);
+      acc[qq].y += __shfl_xor(acc[qq].y, off);
+    }
+  if (lane == 0) {
+    const double d = sqrt(r * r + Z * Z);
+    const double fr = d * d * d / r, fz = d * d * d / Z;
+    double2* o = out + pt * NQ;
+    for (int c = 0; c < 5; ++c) {
+      o[c] = cscale<double>(acc[c], d);
+      o[5 + c] = cscale<double>(acc[5 + c], fr);
+      o[10 + c] = cscale<double>(acc[10 + c], fz);
+    }
+  }
+}
+
+// out = (1 - t) a + t b
+__global__ void k_lerp(int64_t len, const double2* a, const double2* b, double t, double2* out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < len) out[i] = mk<double>((1 - t) * a[i].x + t * b[i].x, (1 - t) * a[i].y + t * b[i].y);
+}
+
+// ------------------------------------------------------ interpolation
+struct TabView {
+  int64_t nr, nz;
+  const double* r;
+  const double* z1;
+  const double2* v;  // (i nz + j) 15 + q
+  double z0, rmin, zmin;
+  int* err;          // set when a point lies outside the table
+};
+
+__device__ inline double round_z(double z, double z0, double zmin) {  // round.m
+  const double d = fabs(z - z0);
+  if (d <= zmin) {
+    const double s = z > z0 ? 1.0 : (z < z0 ? -1.0 : 0.0);
+    return z0 + s * zmin;
+  }
+  return z;
+}
+
+__device__ inline int64_t upper_index(const double* x, int64_t n, double v) {  // #{x <= v}
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t m = (lo + hi) / 2;
+    if (x[m] <= v)
+      lo = m + 1;
+    else
+      hi = m;
+  }
+  return lo;
+}
+
+// interp2.m:17-44: G, Fr, Fz at (r, z1, z2), and the clamped r.
+__device__ double tab_interp(const TabView& T, double r, double z1, double z2, double2 o[NQ]) {
+#pragma clang fp contract(off)
+  r = fmax(T.rmin, r);
+  z1 = round_z(z1, T.z0, T.zmin);
+  z2 = round_z(z2, T.z0, T.zmin);
+  const double z = z1 + fabs(round_z(z2, T.z0, T.zmin) - T.z0);
+  const double zm = fabs(z - T.z0);
+  const double d = sqrt(r * r + zm * zm);
+  int64_t ix = upper_index(T.r, T.nr, r), iy = upper_index(T.z1, T.nz, z);
+  if (ix == 0 || iy == 0 || r > T.r[T.nr - 1] || z > T.z1[T.nz - 1]) {
+    atomicOr(T.err, 1);
+    ix = ix == 0 ? 1 : ix;
+    iy = iy == 0 ? 1 : iy;
+  }
+  ix = (ix < T.nr - 1 ? ix : T.nr - 1) - 1;
+  iy = (iy < T.nz - 1 ? iy : T.nz - 1) - 1;
+  const double xx = (r - T.r[ix]) / (T.r[ix + 1] - T.r[ix]);
+  const double yy = (z - T.z1[iy]) / (T.z1[iy + 1] - T.z1[iy]);
+  const double w0 = (1 - xx) * (1 - yy), w1 = xx * (1 - yy), w2 = (1 - xx) * yy, w3 = xx * yy;
+  const double2* v00 = T.v + (ix * T.nz + iy) * NQ;
+  const double2* v10 = v00 + T.nz * NQ;
+  const double2* v01 = v00 + NQ;
+  const double2* v11 = v10 + NQ;
+  const double d3 = d * d * d;
+  for (int q = 0; q < NQ; ++q) {
+    const double fx = w0 * v00[q].x + w1 * v10[q].x + w2 * v01[q].x + w3 * v11[q].x;
+    const double fy = w0 * v00[q].y + w1 * v10[q].y + w2 * v01[q].y + w3 * v11[q].y;
+    if (q < 5)
+      o[q] = mk<double>(fx / d, fy / d);
+    else if (q < 10)
+      o[q] = mk<double>(fx * r / d3, fy * r / d3);
+    else
+      o[q] = mk<double>(fx * zm / d3, fy * zm / d3);
+  }
+  return r;
+}
+
+// Reflected G and F = nvec . Gp at the centroids, all pairs
+// (initrefl2.m:21-37): G[c n^2 + j n + i], F likewise.
+__global__ void __launch_bounds__(256) k_layer_assemble(int64_t n, Geo geo, TabView T, double2* __restrict__ G,
+                                                        double2* __restrict__ F) {
+#pragma clang fp contract(off)
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int64_t i = idx % n, j = idx / n;
+  const double x = geo.pos[i] - geo.pos[j], y = geo.pos[n + i] - geo.pos[n + j];
+  const double r = sqrt(x * x + y * y);
+  double2 o[NQ];
+  tab_interp(T, r, geo.pos[2 * n + i], geo.pos[2 * n + j], o);
+  const double a = geo.area[j], rr = fmax(r, 1e-10);
+  const double nx = geo.nvec[i], ny = geo.nvec[n + i], nz = geo.nvec[2 * n + i];
+  const int64_t nn = n * n;
+  for (int c = 0; c < 5; ++c) {
+    G[c * nn + idx] = cscale<double>(o[c], a);
+    const double2 gx = cscale<double>(cscale<double>(cscale<double>(o[5 + c], x), 1.0 / rr), a);
+    const double2 gy = cscale<double>(cscale<double>(cscale<double>(o[5 + c], y), 1.0 / rr), a);
+    const double2 gz = cscale<double>(o[10 + c], a);
+    F[c * nn + idx] = mk<double>(nx * gx.x + ny * gy.x + nz * gz.x, nx * gx.y + ny * gy.y + nz * gz.y);
+  }
+}
+
+// Refined near pairs, weights once per geometry (init2.m:23-99): per pair p
+// (row i, face j) and vertex a, W[(p 3 + a) 4 + {g, f1, f2, fz}], the
+// vertex's clamped radius R0[p 3 + a] and height Z2[p 3 + a], and the row's
+// height Z1[p].
+struct LayGeo {
+  double z0, rmin, zmin;
+};
+__device__ inline void lay_dist(const LayGeo& L, const double* p1, const double* p2, double& r, double& z,
+                                double& d, double& x, double& y) {
+#pragma clang fp contract(off)
+  x = p1[0] - p2[0];
+  y = p1[1] - p2[1];
+  z = round_z(fabs(p1[2] - L.z0), L.z0, L.zmin) + round_z(fabs(p2[2] - L.z0), L.z0, L.zmin);
+  r = fmax(L.rmin, sqrt(x * x + y * y));
+  d = sqrt(r * r + z * z);
+}
+
+__global__ void __launch_bounds__(256) k_layer_refine_init(int64_t np, int64_t n, Geo geo, LayGeo L,
+                                                           const int32_t* __restrict__ rows,
+                                                           const int32_t* __restrict__ cols,
+                                                           const double* __restrict__ q18, double* W, double* R0,
+                                                           double* Z2, double* Z1) {
+#pragma clang fp contract(off)
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int64_t i = rows[p], j = cols[p];
+  const double pi3[3] = {geo.pos[i], geo.pos[n + i], geo.pos[2 * n + i]};
+  double v[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) v[a][c] = geo.vtx[(a * 3 + c) * n + j];
+  double r0[3], z0[3], d0[3];
+  for (int a = 0; a < 3; ++a) {
+    double x, y;
+    lay_dist(L, pi3, v[a], r0[a], z0[a], d0[a], x, y);
+  }
+  double acc[3][4] = {};
+  const double qa = geo.qarea[j];
+  for (int q = 0; q < 28; ++q) {
+    const double xq = q18[q], yq = q18[28 + q], wq = q18[56 + q] * qa, zq = 1 - xq - yq;
+    double pt[3];
+    for (int c = 0; c < 3; ++c) pt[c] = xq * v[0][c] + yq * v[1][c] + zq * v[2][c];
+    double r, z, d, x, y;
+    lay_dist(L, pi3, pt, r, z, d, x, y);
+    const double d3 = d * d * d;
+    const double s[3] = {xq, yq, zq};
+    for (int a = 0; a < 3; ++a) {
+      const double fr0 = d0[a] * d0[a] * d0[a] / r0[a] * s[a];
+      const double fz0 = d0[a] * d0[a] * d0[a] / z0[a] * s[a];
+      acc[a][0] += d0[a] * s[a] / d * wq;
+      acc[a][1] += fr0 * x / d3 * wq;
+      acc[a][2] += fr0 * y / d3 * wq;
+      acc[a][3] += fz0 * z / d3 * wq;
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    for (int k = 0; k < 4; ++k) W[(p * 3 + a) * 4 + k] = acc[a][k];
+    R0[p * 3 + a] = r0[a];
+    Z2[p * 3 + a] = round_z(v[a][2], L.z0, L.zmin);
+  }
+  Z1[p] = round_z(pi3[2], L.z0, L.zmin);
+}
+
+__global__ void __launch_bounds__(256) k_layer_refine_apply(int64_t np, int64_t n, Geo geo, TabView T,
+                                                            const int32_t* __restrict__ rows,
+                                                            const int32_t* __restrict__ cols,
+                                                            const double* __restrict__ W,
+                                                            const double* __restrict__ R0,
+                                                            const double* __restrict__ Z2,
+                                                            const double* __restrict__ Z1, double2* G, double2* F) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int64_t i = rows[p], j = cols[p];
+  double2 g[5], gx[5], gy[5], gz[5];
+  for (int c = 0; c < 5; ++c) g[c] = gx[c] = gy[c] = gz[c] = mk<double>(0, 0);
+  for (int a = 0; a < 3; ++a) {
+    double2 o[NQ];
+    tab_interp(T, R0[p * 3 + a], Z1[p], Z2[p * 3 + a], o);
+    const double* w = W + (p * 3 + a) * 4;
+    for (int c = 0; c < 5; ++c) {
+      g[c] = cadd<double>(g[c], cscale<double>(o[c], w[0]));
+      gx[c] = cadd<double>(gx[c], cscale<double>(o[5 + c], w[1]));
+      gy[c] = cadd<double>(gy[c], cscale<double>(o[5 + c], w[2]));
+      gz[c] = cadd<double>(gz[c], cscale<double>(o[10 + c], w[3]));
+    }
+  }
+  const double nx = geo.nvec[i], ny = geo.nvec[n + i], nz = geo.nvec[2 * n + i];
+  const int64_t nn = n * n, idx = j * n + i;
+  for (int c = 0; c < 5; ++c) {
+    G[c * nn + idx] = g[c];
+    F[c * nn + idx] = mk<double>(nx * gx[c].x + ny * gy[c].x + nz * gz[c].x, nx * gx[c].y + ny * gy[c].y + nz * gz[c].y);
+  }
+}
+
+// Diagonal elements with the polar rule (initrefl2.m:39-99), faces id[0..nd).
+__global__ void __launch_bounds__(64) k_layer_diag(int64_t nd, int64_t n, Geo geo, TabView T,
+                                                   const int32_t* __restrict__ id, const double* __restrict__ qpol,
+                                                   int npol, double2* G, double2* F) {
+#pragma clang fp contract(off)
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nd) return;
+  const int64_t f = id[t];
+  double v[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) v[a][c] = geo.vtx[(a * 3 + c) * n + f];
+  const double px = geo.pos[f], py = geo.pos[n + f], pz = geo.pos[2 * n + f], area = geo.area[f];
+  double2 g[5], gx[5], gy[5], gz[5];
+  for (int c = 0; c < 5; ++c) g[c] = gx[c] = gy[c] = gz[c] = mk<double>(0, 0);
+  for (int q = 0; q < npol; ++q) {
+    const double xq = qpol[q], yq = qpol[MNP_MAX_POLAR + q], zq = 1 - xq - yq;
+    const double w = qpol[2 * MNP_MAX_POLAR + q] * area;
+    double pt[3];
+    for (int c = 0; c < 3; ++c) pt[c] = xq * v[0][c] + yq * v[1][c] + zq * v[2][c];
+    const double x = pt[0] - px, y = pt[1] - py;
+    const double r = sqrt(x * x + y * y);
+    double2 o[NQ];
+    const double rc = fmax(tab_interp(T, r, pt[2], pz, o), 1e-10);
+    for (int c = 0; c < 5; ++c) {
+      g[c] = cadd<double>(g[c], cscale<double>(o[c], w));
+      gx[c] = cadd<double>(gx[c], cscale<double>(cscale<double>(cscale<double>(o[5 + c], x), 1.0 / rc), w));
+      gy[c] = cadd<double>(gy[c], cscale<double>(cscale<double>(cscale<double>(o[5 + c], y), 1.0 / rc), w));
+      gz[c] = cadd<double>(gz[c], cscale<double>(o[10 + c], w));
+    }
+  }
+  const double nx = geo.nvec[f], ny = geo.nvec[n + f], nz = geo.nvec[2 * n + f];
+  const int64_t nn = n * n, idx = f * n + f;
+  for (int c = 0; c < 5; ++c) {
+    G[c * nn + idx] = g[c];
+    F[c * nn + idx] = mk<double>(nx * gx[c].x + ny * gy[c].x + nz * gz[c].x, nx * gx[c].y + ny * gy[c].y + nz * gz[c].y);
+  }
+}
+
+
+// ------------------------------------------------------ plane wave and far field
+// Coefficient of component set (p, hh, sh, hs) for the observer in medium
+// o and the source in medium s (1 = above), reflectionsubs.m:27-60,
+// without phases; returns kz of media 1 and 2 in kz[0..1].
+struct Coef4 {
+  double2 p, hh, sh, hs;
+};
+__device__ Coef4 coeff_entry(double k0, double2 e1, double2 e2, double2 k1z, double2 k2z, int o, int s) {
+  const double2 one = mk<double>(1, 0);
+  const double2 rr = cdiv<double>(csub<double>(k1z, k2z), cadd<double>(k2z, k1z));
+  const double2 D = cmul<double>(cadd<double>(k2z, k1z), cadd<double>(cmul<double>(e1, k2z), cmul<double>(e2, k1z)));
+  const double2 e1k1 = cmul<double>(e1, k1z), e2k1 = cmul<double>(e2, k1z), e1k2 = cmul<double>(e1, k2z),
+                e2k2 = cmul<double>(e2, k2z);
+  const double2 hh1 = cdiv<double>(cmul<double>(csub<double>(k1z, k2z), cadd<double>(csub<double>(cscale<double>(e1k1, 2), e2k1), e1k2)), D);
+  const double2 hh2 = cdiv<double>(cmul<double>(csub<double>(k2z, k1z), cadd<double>(csub<double>(cscale<double>(e2k2, 2), e1k2), e2k1)), D);
+  const double2 sh1 = cdiv<double>(cscale<double>(cmul<double>(csub<double>(e2, e1), k1z), -2 * k0), D);
+  const double2 sh2 = cdiv<double>(cscale<double>(cmul<double>(csub<double>(e1, e2), k2z), -2 * k0), D);
+  const double2 hs1 = cdiv<double>(cscale<double>(cmul<double>(cmul<double>(csub<double>(e2, e1), e1), k1z), -2 * k0), D);
+  const double2 hs2 = cdiv<double>(cscale<double>(cmul<double>(cmul<double>(csub<double>(e1, e2), e2), k2z), -2 * k0), D);
+  const double2 q12 = cdiv<double>(k1z, k2z), q21 = cdiv<double>(k2z, k1z);
+  Coef4 c;
+  if (o == 1 && s == 1) {
+    c.p = rr; c.hh = hh1; c.sh = sh1; c.hs = hs1;
+  } else if (o == 1 && s == 2) {  // mat1 / mat2 entry (1, 2)
+    c.p = cadd<double>(one, rr);
+    c.hh = cmul<double>(q12, cadd<double>(hh2, one));
+    c.sh = cmul<double>(mk<double>(-q12.x, -q12.y), sh2);
+    c.hs = cmul<double>(mk<double>(-q12.x, -q12.y), hs2);
+  } else if (o == 2 && s == 1) {  // entry (2, 1)
+    c.p = csub<double>(one, rr);
+    c.hh = cmul<double>(q21, cadd<double>(hh1, one));
+    c.sh = cmul<double>(q21, sh1);
+    c.hs = cmul<double>(q21, hs1);
+  } else {
+    c.p = mk<double>(-rr.x, -rr.y);
+    c.hh = hh2;
+    c.sh = mk<double>(-sh2.x, -sh2.y);
+    c.hs = mk<double>(-hs2.x, -hs2.y);
+  }
+  return c;
+}
+
+// kz with fresnel.m's 1e-10 i offset before the sign
+__device__ inline double2 kz_off(double2 k, double2 kp) {
+  double2 kz = csqrt_p(csub<double>(cmul<double>(k, k), cmul<double>(kp, kp)));
+  kz.y += 1e-10;
+  return kz.y > 0 ? kz : (kz.y < 0 ? mk<double>(-kz.x, -kz.y) : mk<double>(0, 0));
+}
+
+// Outside potentials of the plane waves at the faces (potential.m:20-105,
+// every face above the interface): one thread per (face, wave); the normal
+// derivative differentiates the phase factors exactly.
+__global__ void k_pw_layer(int64_t n, int64_t npol, const double* __restrict__ pos, const double* __restrict__ nvec,
+                           const double* __restrict__ pd, double k0, double2 e1, double2 e2, double z0,
+                           double2* phi, double2* phip, double2* a, double2* ap) {
+#pragma clang fp contract(off)
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * npol) return;
+  const int64_t i = t % n, q = t / n;
+  const double pol[3] = {pd[q], pd[npol + q], pd[2 * npol + q]};
+  const double dir[3] = {pd[3 * npol + q], pd[4 * npol + q], pd[5 * npol + q]};
+  const int medium = dir[2] < 0 ? 1 : 2;
+  const double z2 = medium == 1 ? z0 + 1e-10 : z0 - 1e-10;
+  const double2 nb = csqrt_p(medium == 1 ? e1 : e2);
+  const double2 kp = cscale<double>(nb, k0 * sqrt(dir[0] * dir[0] + dir[1] * dir[1]));
+  const double2 k1 = cscale<double>(csqrt_p(e1), k0), k2 = cscale<double>(csqrt_p(e2), k0);
+  const double2 k1z = kz_of(k1, kp), k2z = kz_of(k2, kp);
+  const Coef4 c = coeff_entry(k0, e1, e2, k1z, k2z, 1, medium);
+  const double x = pos[i], y = pos[n + i], z = pos[2 * n + i];
+  const double2 g1 = cexp_(mul_i(cscale<double>(k1z, fabs(z - z0))));
+  const double2 g2 = cexp_(mul_i(cscale<double>(medium == 1 ? k1z : k2z, fabs(z2 - z0))));
+  const double2 ratio = cdiv<double>(medium == 1 ? kz_off(k1, kp) : kz_off(k2, kp), kz_off(k1, kp));
+  const double2 ph = cmul<double>(cmul<double>(g1, g2), ratio);
+  const double inn = x * dir[0] + y * dir[1] + z2 * dir[2];
+  const double2 fac = cdiv<double>(cexp_(mul_i(cscale<double>(nb, k0 * inn))), mk<double>(0, k0));
+  const double nx = nvec[i], ny = nvec[n + i], nz = nvec[2 * n + i];
+  const double2 dph = mul_i(cadd<double>(cscale<double>(nb, k0 * (nx * dir[0] + ny * dir[1])), cscale<double>(k1z, nz)));
+  double2 av[3];
+  const double2 rp = cmul<double>(c.p, ph);
+  av[0] = cmul<double>(cscale<double>(fac, pol[0]), rp);
+  av[1] = cmul<double>(cscale<double>(fac, pol[1]), rp);
+  av[2] = cmul<double>(cscale<double>(fac, pol[2]), cmul<double>(c.hh, ph));
+  const double2 pv = cmul<double>(cscale<double>(fac, pol[2]), cmul<double>(c.sh, ph));
+  double2 apv[3];
+  for (int cc = 0; cc < 3; ++cc) apv[cc] = cmul<double>(av[cc], dph);
+  if (medium == 1) {
+    const double2 f0 = cdiv<double>(cexp_(mul_i(cscale<double>(nb, k0 * (x * dir[0] + y * dir[1] + z * dir[2])))),
+                                    mk<double>(0, k0));
+    const double2 d0 = mul_i(cscale<double>(nb, k0 * (nx * dir[0] + ny * dir[1] + nz * dir[2])));
+    for (int cc = 0; cc < 3; ++cc) {
+      const double2 a0 = cscale<double>(f0, pol[cc]);
+      av[cc] = cadd<double>(av[cc], a0);
+      apv[cc] = cadd<double>(apv[cc], cmul<double>(a0, d0));
+    }
+  }
+  phi[q * n + i] = pv;
+  phip[q * n + i] = cmul<double>(pv, dph);
+  for (int cc = 0; cc < 3; ++cc) {
+    a[(cc + 3 * q) * n + i] = av[cc];
+    ap[(cc + 3 * q) * n + i] = apv[cc];
+  }
+}
+
+constexpr int LFF_MAXP = 8;
+
+// Far field with the interface (@spectrumretlayer/farfield.m:20-104), one
+// workgroup per direction: reflected (upper half space) or transmitted
+// (lower) part of the outside charges and currents, through a lossless
+// medium only, plus the direct far field above; e transverse, h = k / k0
+// dir x e.  e, h at d + ndir (c + 3 q).
+__global__ void __launch_bounds__(256) k_farfield_layer(int64_t n, int64_t p, int64_t ndir, const double* dir,
+                                                        const double* pos, const double* area, const double2* sig2,
+                                                        const double2* h2, double k0, double2 e1, double2 e2,
+                                                        double z0, double2* e, double2* h) {
+  const int64_t d = blockIdx.x;
+  const double dx = dir[d], dy = dir[ndir + d], dz = dir[2 * ndir + d];
+  const bool up = dz >= 0;
+  const double2 k1 = cscale<double>(csqrt_p(e1), k0), k2 = cscale<double>(csqrt_p(e2), k0);
+  const double2 km = up ? k1 : k2;
+  const bool refl = km.y == 0;
+  const double2 kp = mk<double>(km.x * sqrt(1 - dz * dz), 0);
+  const double2 k1z = kz_of(k1, kp), k2z = kz_of(k2, kp);
+  const Coef4 c = coeff_entry(k0, e1, e2, k1z, k2z, up ? 1 : 2, 1);
+  const double2 g1 = cexp_(mul_i(cscale<double>(up ? k1z : k2z, 0.0)));
+  double2 acc[LFF_MAXP][7];
+  for (int q = 0; q < LFF_MAXP; ++q)
+    for (int cc = 0; cc < 7; ++cc) acc[q][cc] = mk<double>(0, 0);
+  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+    const double x = pos[j], y = pos[n + j], z = pos[2 * n + j];
+    if (refl) {
+      const double dist = x * dx + y * dy + z0 * dz;
+      const double2 phase = cscale<double>(cexp_(mul_i(cscale<double>(km, -dist))), area[j]);
+      const double2 g = cmul<double>(g1, cexp_(mul_i(cscale<double>(k1z, fabs(z - z0)))));
+      const double2 wp = cmul<double>(cmul<double>(c.p, g), phase), wh = cmul<double>(cmul<double>(c.hh, g), phase),
+                    ws = cmul<double>(cmul<double>(c.hs, g), phase);
+      for (int q = 0; q < p; ++q) {
+        acc[q][0] = cfma(acc[q][0], wp, h2[(0 + 3 * q) * n + j]);
+        acc[q][1] = cfma(acc[q][1], wp, h2[(1 + 3 * q) * n + j]);
+        acc[q][2] = cfma(cfma(acc[q][2], wh, h2[(2 + 3 * q) * n + j]), ws, sig2[q * n + j]);
+      }
+    }
+    if (up) {
+      const double pd = dx * x + dy * y + dz * z;
+      const double2 ph = cscale<double>(cexp_ikd(mk<double>(-k1.x, -k1.y), pd), area[j]);
+      for (int q = 0; q < p; ++q) {
+        for (int cc = 0; cc < 3; ++cc) acc[q][3 + cc] = cfma(acc[q][3 + cc], ph, h2[(cc + 3 * q) * n + j]);
+        acc[q][6] = cfma(acc[q][6], ph, sig2[q * n + j]);
+      }
+    }
+  }
+  __shared__ double2 red[256];
+  for (int q = 0; q < p; ++q)
+    for (int cc = 0; cc < 7; ++cc) {
+      red[threadIdx.x] = acc[q][cc];
+      __syncthreads();
+      for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = cadd<double>(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) acc[q][cc] = red[0];
+      __syncthreads();
+    }
+  if (threadIdx.x == 0) {
+    const double2 ik0 = mk<double>(0, k0), ik1 = mul_i(k1);
+    const double dv[3] = {dx, dy, dz};
+    const double kr = km.x / k0, ki = km.y / k0;
+    for (int q = 0; q < p; ++q) {
+      double2 ev[3];
+      for (int cc = 0; cc < 3; ++cc) {
+        ev[cc] = cmul<double>(ik0, acc[q][cc]);
+        if (up)
+          ev[cc] = cadd<double>(ev[cc], csub<double>(cmul<double>(ik0, acc[q][3 + cc]),
+                                                     cscale<double>(cmul<double>(ik1, acc[q][6]), dv[cc])));
+      }
+      const double2 de = cadd<double>(cadd<double>(cscale<double>(ev[0], dx), cscale<double>(ev[1], dy)),
+                                      cscale<double>(ev[2], dz));
+      for (int cc = 0; cc < 3; ++cc) ev[cc] = csub<double>(ev[cc], cscale<double>(de, dv[cc]));
+      const double2 cx = csub<double>(cscale<double>(ev[2], dy), cscale<double>(ev[1], dz));
+      const double2 cy = csub<double>(cscale<double>(ev[0], dz), cscale<double>(ev[2], dx));
+      const double2 cz = csub<double>(cscale<double>(ev[1], dx), cscale<double>(ev[0], dy));
+      const double2 kk = mk<double>(kr, ki);
+      for (int cc = 0; cc < 3; ++cc) e[d + ndir * (cc + 3 * q)] = ev[cc];
+      h[d + ndir * (0 + 3 * q)] = cmul<double>(kk, cx);
+      h[d + ndir * (1 + 3 * q)] = cmul<double>(kk, cy);
+      h[d + ndir * (2 + 3 * q)] = cmul<double>(kk, cz);
+    }
+  }
+}
+
+// ------------------------------------------------------ chain helpers
+// out = alpha x + beta y
+__global__ void k_zaxpby(int64_t len, double2 al, const double2* x, double2 be, const double2* y, double2* out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < len) out[i] = cadd<double>(cmul<double>(al, x[i]), cmul<double>(be, y[i]));
+}
+// W = e1 S1 + c (Gamma .* npar npar')
+__global__ void k_lay_w(int64_t n, const double* nvec, double2 e1, const double2* S1, double2 c, const double2* Gam,
+                        double2* W) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int64_t i = idx % n, j = idx / n;
+  const double nn = nvec[i] * nvec[j] + nvec[n + i] * nvec[n + j];
+  W[idx] = cadd<double>(cmul<double>(e1, S1[idx]), cscale<double>(cmul<double>(c, Gam[idx]), nn));
+}
+// block of M (ld 2n): Mb(i, j) += -al H(i, j) - be nz_i Gc(i, j)
+__global__ void k_lay_mblock(int64_t n, int64_t ld, double2* Mb, double2 al, const double2* H, double2 be,
+                             const double* nz, const double2* Gc) {
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int64_t i = idx % n, j = idx / n;
+  double2& m = Mb[j * ld + i];
+  m = csub<double>(csub<double>(m, cmul<double>(al, H[idx])), cscale<double>(cmul<double>(be, Gc[idx]), nz[i]));
+}
+// excitation.m:17-29 with scalar eps1 (inside) and eps2 (outside)
+__global__ void k_lay_exc(int64_t n, int64_t p, const double2* phi1, const double2* phi1p, const double2* a1,
+                          const double2* a1p, const double2* phi2, const double2* phi2p, const double2* a2,
+                          const double2* a2p, const double* nvec, double k, double2 e1, double2 e2, double2* phi,
+                          double2* a, double2* alpha, double2* De) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * p) return;
+  const int64_t i = t % n, q = t / n;
+  auto ld = [&](const double2* v, int64_t idx) { return v ? v[idx] : mk<double>(0, 0); };
+  const double2 f1 = ld(phi1, t), f2 = ld(phi2, t);
+  phi[t] = csub<double>(f2, f1);
+  const double2 ik = mk<double>(0, k);
+  double2 na1 = mk<double>(0, 0), na2 = mk<double>(0, 0);
+  for (int c = 0; c < 3; ++c) {
+    const int64_t id = (c + 3 * q) * n + i;
+    const double nc = nvec[c * n + i];
+    const double2 A1 = ld(a1, id), A2 = ld(a2, id);
+    a[id] = csub<double>(A2, A1);
+    na1 = cadd<double>(na1, cscale<double>(A1, nc));
+    na2 = cadd<double>(na2, cscale<double>(A2, nc));
+    const double2 s = csub<double>(cmul<double>(f2, e2), cmul<double>(f1, e1));
+    alpha[id] = csub<double>(csub<double>(ld(a2p, id), ld(a1p, id)), cscale<double>(cmul<double>(ik, s), nc));
+  }
+  De[t] = csub<double>(csub<double>(cmul<double>(e2, ld(phi2p, t)), cmul<double>(e1, ld(phi1p, t))),
+                       cmul<double>(ik, csub<double>(cmul<double>(e2, na2), cmul<double>(e1, na1))));
+}
+// alpha <- alpha - T + c n phi  (T = Sigma1 a)
+__global__ void k_lay_alpha(int64_t n, int64_t p, double2* alpha, const double2* T, const double2* phi,
+                            const double* nvec, double2 c) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * p) return;
+  const int64_t i = t % n, q = t / n;
+  const double2 cp = cmul<double>(c, phi[t]);
+  for (int cc = 0; cc < 3; ++cc) {
+    const int64_t id = (cc + 3 * q) * n + i;
+    alpha[id] = cadd<double>(csub<double>(alpha[id], T[id]), cscale<double>(cp, nvec[cc * n + i]));
+  }
+}
+// rhs = [De - e1 T2 + c1 n.a + c2 npar.T3 ; alpha_z]  (2n x p, ld 2n)
+__global__ void k_lay_rhs(int64_t n, int64_t p, const double2* De, const double2* T2, const double2* a,
+                          const double2* T3, const double2* alpha, const double* nvec, double2 e1, double2 c1,
+                          double2 c2, double2* rhs) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * p) return;
+  const int64_t i = t % n, q = t / n;
+  double2 na = mk<double>(0, 0), nt = mk<double>(0, 0);
+  for (int cc = 0; cc < 3; ++cc) {
+    const int64_t id = (cc + 3 * q) * n + i;
+    na = cadd<double>(na, cscale<double>(a[id], nvec[cc * n + i]));
+    if (cc < 2) nt = cadd<double>(nt, cscale<double>(T3[id], nvec[cc * n + i]));
+  }
+  rhs[q * 2 * n + i] = cadd<double>(cadd<double>(csub<double>(De[t], cmul<double>(e1, T2[t])), cmul<double>(c1, na)),
+                                    cmul<double>(c2, nt));
+  rhs[q * 2 * n + n + i] = alpha[(2 + 3 * q) * n + i];
+}
+// V = alpha_par + c npar U, V_z = 0
+__global__ void k_lay_v(int64_t n, int64_t p, const double2* alpha, const double2* U, const double* nvec, double2 c,
+                        double2* V) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * p) return;
+  const int64_t i = t % n, q = t / n;
+  const double2 cu = cmul<double>(c, U[t]);
+  for (int cc = 0; cc < 2; ++cc) {
+    const int64_t id = (cc + 3 * q) * n + i;
+    V[id] = cadd<double>(alpha[id], cscale<double>(cu, nvec[cc * n + i]));
+  }
+  V[(2 + 3 * q) * n + i] = mk<double>(0, 0);
+}
+// h (n x 3 x p): z-component from src (column q at src + q ld)
+__global__ void k_lay_setz(int64_t n, int64_t p, const double2* src, int64_t ld, double2* h) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * p) return;
+  const int64_t i = t % n, q = t / n;
+  h[(2 + 3 * q) * n + i] = src[q * ld + i];
+}
+// sig2 (n x p) from the first n rows of xi (ld 2n)
+__global__ void k_lay_copy_rows(int64_t n, int64_t p, const double2* src, int64_t ld, double2* dst) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * p) return;
+  dst[t] = src[(t / n) * ld + t % n];
+}
+
+inline unsigned nblk(int64_t len, int bs = 256) { return (unsigned)cdiv_up(len, bs); }
+
+}  // namespace
+
+// ------------------------------------------------------ host objects
+struct LayerTab {
+  mnp_layer_options op;
+  int64_t nr = 0, nz = 0;
+  double z2 = 0;
+  double *r = nullptr, *z1 = nullptr, *Zt = nullptr;
+  double2* cur = nullptr;                 // table at the current wavelength
+  double2* saved = nullptr;               // tables of set(), nsaved x size
+  std::vector<double> enei;               // wavelengths of the saved tables
+  int* err = nullptr;
+  double cur_enei = -1;
+  double2 cur_e1{}, cur_e2{};
+
+  int64_t size() const { return nr * nz * NQ; }
+  LayerTab(const mnp_layer_options& o, int64_t nr_, const double* r_, int64_t nz_, const double* z1_, double z2_)
+      : op(o), nr(nr_), nz(nz_), z2(z2_) {
+    MNP_CHECK(nr >= 2 && nz >= 2, MNP_ERR_ARG, "layertab: the grid needs at least 2 x 2 points");
+    for (int64_t i = 0; i + 1 < nr; ++i)
+      MNP_CHECK(r_[i] < r_[i + 1] && r_[0] > 0, MNP_ERR_ARG, "layertab: radii must be positive and increasing");
+    for (int64_t j = 0; j + 1 < nz; ++j)
+      MNP_CHECK(z1_[j] < z1_[j + 1], MNP_ERR_ARG, "layertab: heights must be increasing");
+    MNP_CHECK(z1_[0] > op.z0 && z2 > op.z0, MNP_ERR_ARG, "layertab: the table must lie above the interface");
+    upload_rules();
+    std::vector<double> zt(nz);
+    for (int64_t j = 0; j < nz; ++j) zt[j] = std::fabs(z1_[j] - op.z0) + std::fabs(z2 - op.z0);
+    r = dev_alloc<double>(nr);
+    z1 = dev_alloc<double>(nz);
+    Zt = dev_alloc<double>(nz);
+    cur = dev_alloc<double2>(size());
+    err = dev_alloc<int>(1);
+    MNP_HIP(hipMemcpy(r, r_, sizeof(double) * nr, hipMemcpyHostToDevice));
+    MNP_HIP(hipMemcpy(z1, z1_, sizeof(double) * nz, hipMemcpyHostToDevice));
+    MNP_HIP(hipMemcpy(Zt, zt.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+  }
+  ~LayerTab() {
+    void* ps[] = {r, z1, Zt, cur, saved, err};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
+  // green.m for the whole grid at one wavelength into out
+  void compute(double enei_, double2 e1, double2 e2, double2* out) const {
+    LayPrm P;
+    P.k0 = 2 * M_PI / enei_;
+    P.e1 = e1;
+    P.e2 = e2;
+    P.k1 = wavenumber(P.k0, e1);
+    P.k2 = wavenumber(P.k0, e2);
+    P.k1max = std::max(P.k1.x, P.k2.x) + P.k0;
+    P.semi = op.semi;
+    P.ratio = op.ratio;
+    hipLaunchKernelGGL(k_layer_table, dim3((unsigned)(nr * nz)), dim3(64), 0, stream(), nr, nz, r, Zt, P, out);
+    MNP_HIP(hipGetLastError());
+  }
+  void set(int64_t ne, const double* en, const mnp_complex* e1, const mnp_complex* e2) {
+    MNP_CHECK(ne >= 1, MNP_ERR_ARG, "layertab set: no wavelengths");
+    for (int64_t i = 0; i + 1 < ne; ++i)
+      MNP_CHECK(en[i] < en[i + 1], MNP_ERR_ARG, "layertab set: wavelengths must be increasing");
+    if (saved) MNP_HIP(hipFree(saved));
+    saved = dev_alloc<double2>(ne * size());
+    for (int64_t i = 0; i < ne; ++i) compute(en[i], to_d2(e1[i]), to_d2(e2[i]), saved + i * size());
+    enei.assign(en, en + ne);
+    cur_enei = -1;
+  }
+  // eval.m:17-50: the table at enei
+  const double2* at(double enei_, double2 e1, double2 e2) {
+    if (enei_ == cur_enei && e1.x == cur_e1.x && e1.y == cur_e1.y && e2.x == cur_e2.x && e2.y == cur_e2.y) return cur;
+    if (enei.empty()) {
+      compute(enei_, e1, e2, cur);
+    } else {
+      MNP_CHECK(enei_ >= enei.front() && enei_ <= enei.back(), MNP_ERR_ARG,
+                "greentablayer: wavelength outside the tabulated range");
+      if (enei.size() == 1) {
+        MNP_HIP(hipMemcpyAsync(cur, saved, sizeof(double2) * size(), hipMemcpyDeviceToDevice, stream()));
+      } else {
+        size_t i = std::upper_bound(enei.begin(), enei.end(), enei_) - enei.begin();
+        i = std::min(i, enei.size() - 1) - 1;
+        const double t = (enei_ - enei[i]) / (enei[i + 1] - enei[i]);
+        hipLaunchKernelGGL(k_lerp, dim3(nblk(size())), dim3(256), 0, stream(), size(), saved + i * size(),
+                           saved + (i + 1) * size(), t, cur);
+        MNP_HIP(hipGetLastError());
+      }
+    }
+    cur_enei = enei_;
+    cur_e1 = e1;
+    cur_e2 = e2;
+    return cur;
+  }
+  TabView view(const double2* v) const {
+    TabView T;
+    T.nr = nr;
+    T.nz = nz;
+    T.r = r;
+    T.z1 = z1;
+    T.v = v;
+    T.z0 = op.z0;
+    T.rmin = op.rmin;
+    T.zmin = op.zmin;
+    T.err = err;
+    return T;
+  }
+};
+
+struct GreenRetLayer {
+  const GreenRet* g;
+  LayerTab* tab;
+  int64_t np = 0, nd = 0;
+  int32_t *rows = nullptr, *cols = nullptr, *id = nullptr;
+  double *W = nullptr, *R0 = nullptr, *Z2 = nullptr, *Z1 = nullptr;
+
+  GreenRetLayer(const GreenRet* g_, LayerTab* t_) : g(g_), tab(t_) {
+    MNP_CHECK(!g->compound(), MNP_ERR_ARG, "greenretlayer: compound particles are not supported");
+    const int64_t n = g->n;
+    std::vector<double> pos(3 * n), rad(n);
+    MNP_HIP(hipMemcpy(pos.data(), g->geo.pos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    MNP_HIP(hipMemcpy(rad.data(), g->geo.rad, sizeof(double) * n, hipMemcpyDeviceToHost));
+    const double z0 = tab->op.z0;
+    for (int64_t i = 0; i < n; ++i)
+      MNP_CHECK(pos[2 * n + i] - z0 >= tab->op.ztol, MNP_ERR_ARG,
+                "greenretlayer: every face must lie above the interface (faces in it are not supported)");
+    // refinematrixlayer.m:24-56 (column-major order of find)
+    std::vector<int32_t> hr, hc, hd;
+    std::vector<double> a2(n);
+    for (int64_t i = 0; i < n; ++i) a2[i] = pos[i] * pos[i] + pos[n + i] * pos[n + i];
+    const double rel = g->op.rel_cutoff, absc = g->op.abs_cutoff;
+    for (int64_t j = 0; j < n; ++j)
+      for (int64_t i = 0; i < n; ++i) {
+        double r2 = a2[i] + a2[j] - 2 * (pos[i] * pos[j] + pos[n + i] * pos[n + j]);
+        if (r2 < 1e-10) r2 = 0;
+        const double z = std::fabs(pos[2 * n + i] - z0) + std::fabs(pos[2 * n + j] - z0);
+        const double d = std::sqrt(r2 + z * z);
+        const double d2 = d - rad[j];
+        if (!(d2 < absc || d2 / rad[i] < rel)) continue;
+        if (i == j) {
+          hd.push_back((int32_t)i);
+        } else {
+          hr.push_back((int32_t)i);
+          hc.push_back((int32_t)j);
+        }
+      }
+    np = (int64_t)hr.size();
+    nd = (int64_t)hd.size();
+    rows = dev_alloc<int32_t>(np);
+    cols = dev_alloc<int32_t>(np);
+    id = dev_alloc<int32_t>(nd);
+    W = dev_alloc<double>(12 * np);
+    R0 = dev_alloc<double>(3 * np);
+    Z2 = dev_alloc<double>(3 * np);
+    Z1 = dev_alloc<double>(np);
+    if (np) {
+      MNP_HIP(hipMemcpy(rows, hr.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
+      MNP_HIP(hipMemcpy(cols, hc.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
+      LayGeo L{z0, tab->op.rmin, tab->op.zmin};
+      hipLaunchKernelGGL(k_layer_refine_init, dim3(nblk(np)), dim3(256), 0, stream(), np, n, g->geo, L, rows, cols,
+                         g->q18, W, R0, Z2, Z1);
+      MNP_HIP(hipGetLastError());
+    }
+    if (nd) MNP_HIP(hipMemcpy(id, hd.data(), sizeof(int32_t) * nd, hipMemcpyHostToDevice));
+    MNP_HIP(hipStreamSynchronize(stream()));
+  }
+  ~GreenRetLayer() {
+    void* ps[] = {rows, cols, id, W, R0, Z2, Z1};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
+  // Reflected G and F (5 x n x n each) at enei.
+  void eval(double enei, double2 e1, double2 e2, double2* G, double2* F) {
+    const int64_t n = g->n;
+    const double2* tv = tab->at(enei, e1, e2);
+    const TabView T = tab->view(tv);
+    MNP_HIP(hipMemsetAsync(tab->err, 0, sizeof(int), stream()));
+    hipLaunchKernelGGL(k_layer_assemble, dim3(nblk(n * n)), dim3(256), 0, stream(), n, g->geo, T, G, F);
+    MNP_HIP(hipGetLastError());
+    if (np) {
+      hipLaunchKernelGGL(k_layer_refine_apply, dim3(nblk(np)), dim3(256), 0, stream(), np, n, g->geo, T, rows, cols,
+                         W, R0, Z2, Z1, G, F);
+      MNP_HIP(hipGetLastError());
+    }
+    if (nd) {
+      hipLaunchKernelGGL(k_layer_diag, dim3(nblk(nd, 64)), dim3(64), 0, stream(), nd, n, g->geo, T, id, g->qpol,
+                         g->npolq, G, F);
+      MNP_HIP(hipGetLastError());
+    }
+    int herr = 0;
+    MNP_HIP(hipMemcpyAsync(&herr, tab->err, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipStreamSynchronize(stream()));
+    MNP_CHECK(herr == 0, MNP_ERR_ARG, "greentablayer: a face pair lies outside the table (tabspace of this particle?)");
+  }
+};
+
+// The bemretlayer chain in complex FP64 (initmat.m:17-95, mldivide.m:31-76)
+// for unique eps1 (inside) and eps2 (outside = upper medium): L1 = eps1,
+// L2p = eps2, so Gammapar = i k (eps1 - eps2) Gamma .* (npar npar').
+struct BemRetLayer {
+  const GreenRet* g;
+  GreenRetLayer* gl;
+  int64_t n;
+  double2 *G1, *H1, *G2c, *H2c, *G1i, *G2pi, *S1, *Gam, *T, *M;
+  int32_t *piv = nullptr, *sigM = nullptr, *sigA = nullptr;
+  double2 *inv = nullptr, *invM = nullptr;
+  LU<double> fM;
+  Workspace ws, wsv;  // factorisations and solves; solve's vectors
+  double k = 0, enei = -1;
+  double2 e1{}, e2{};
+  bool ready = false;
+  double* nz = nullptr;  // nvec z column (device view)
+
+  BemRetLayer(const GreenRet* g_, GreenRetLayer* gl_) : g(g_), gl(gl_), n(g_->n) {
+    MNP_CHECK(gl->g == g, MNP_ERR_ARG, "bemretlayer: the layer Green function belongs to another particle");
+    const int64_t nn = n * n;
+    G1 = dev_alloc<double2>(nn);
+    H1 = dev_alloc<double2>(nn);
+    G2c = dev_alloc<double2>(5 * nn);
+    H2c = dev_alloc<double2>(5 * nn);
+    G1i = dev_alloc<double2>(nn);
+    G2pi = dev_alloc<double2>(nn);
+    S1 = dev_alloc<double2>(nn);
+    Gam = dev_alloc<double2>(nn);
+    T = dev_alloc<double2>(2 * nn);
+    M = dev_alloc<double2>(4 * nn);
+    piv = dev_alloc<int32_t>(2 * n);
+    sigM = dev_alloc<int32_t>(2 * n);
+    sigA = dev_alloc<int32_t>(n);
+    inv = dev_alloc<double2>(2 * inv_blocks(n));
+    invM = dev_alloc<double2>(2 * inv_blocks(2 * n));
+    nz = g->geo.nvec + 2 * n;
+  }
+  ~BemRetLayer() {
+    void* ps[] = {G1, H1, G2c, H2c, G1i, G2pi, S1, Gam, T, M, piv, sigM, sigA, inv, invM};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
+  double2* G2(int c) const { return G2c + c * n * n; }
+  double2* H2(int c) const { return H2c + c * n * n; }
+  MatView<double> v(double2* p, int64_t ld = -1) const { return MatView<double>{p, ld < 0 ? n : ld}; }
+
+  // X = A^{-1} (A destroyed)
+  void invert(double2* A, double2* X, const char* what) {
+    const int info = getrf<double>(n, n, v(A), piv, inv, inv + inv_blocks(n), PIVOT_PARTIAL, nullptr, ws);
+    MNP_CHECK(info == 0, MNP_ERR_SINGULAR, std::string("bemretlayer: singular ") + what);
+    perm_from_pivots(n, piv, sigA, ws);
+    LU<double> f{v(A), inv, inv + inv_blocks(n), sigA};
+    getri<double>(n, f, v(X), ws);
+  }
+
+  void initmat(double enei_, double2 eps_in, double2 eps_up, double2 eps_down) {
+    const int64_t nn = n * n;
+    enei = enei_;
+    e1 = eps_in;
+    e2 = eps_up;
+    const double k0 = 2 * M_PI / enei;
+    k = k0;
+    ready = false;
+    g->assemble(wavenumber(k0, eps_in), 2 * M_PI, OutPair{G1, nullptr}, OutPair{H1, nullptr}, n);
+    // reflected parts into G2c, H2c, the direct outside G, H into T
+    gl->eval(enei, eps_up, eps_down, G2c, H2c);
+    g->assemble(wavenumber(k0, eps_up), -2 * M_PI, OutPair{T, nullptr}, OutPair{T + nn, nullptr}, n);
+    const double2 one = mk<double>(1, 0);
+    for (int c : {0, 1, 4}) {  // p, ss, hh carry the direct part (compgreenretlayer/private/eval1.m:37-45)
+      hipLaunchKernelGGL(k_zaxpby, dim3(nblk(nn)), dim3(256), 0, stream(), nn, one, G2(c), one, T, G2(c));
+      hipLaunchKernelGGL(k_zaxpby, dim3(nblk(nn)), dim3(256), 0, stream(), nn, one, H2(c), one, T + nn, H2(c));
+    }
+    MNP_HIP(hipGetLastError());
+    // G1^{-1}, G2p^{-1}
+    copy_mat<double>(n, n, v(G1), v(T));
+    invert(T, G1i, "G1");
+    copy_mat<double>(n, n, v(G2(0)), v(T));
+    invert(T, G2pi, "G2.p");
+    // Sigma1 = H1 G1^{-1}; Gamma = (Sigma1 - H2p G2p^{-1})^{-1}
+    const double2 zero = mk<double>(0, 0), mone = mk<double>(-1, 0);
+    gemm<double>(n, n, n, one, v(H1), v(G1i), zero, v(S1));
+    copy_mat<double>(n, n, v(S1), v(T));
+    gemm<double>(n, n, n, mone, v(H2(0)), v(G2pi), one, v(T));
+    invert(T, Gam, "Sigma1 - Sigma2p");
+    // W = eps1 Sigma1 + k^2 (eps1 - eps2)^2 Gamma .* (npar npar')
+    const double2 de = csub<double>(e1, e2);
+    const double2 de2k = cscale<double>(cmul<double>(de, de), k * k);
+    hipLaunchKernelGGL(k_lay_w, dim3(nblk(nn)), dim3(256), 0, stream(), n, g->geo.nvec, e1, S1, de2k, Gam, T);
+    MNP_HIP(hipGetLastError());
+    // M = [W G2ss - eps2 H2ss - ik de nz G2sh,  W G2sh - eps2 H2sh - ik de nz G2hh;
+    //      S1 G2hs - H2hs - ik de nz G2ss,      S1 G2hh - H2hh - ik de nz G2sh]
+    const int64_t ld = 2 * n;
+    const double2 ikde = cmul<double>(mk<double>(0, k), de);
+    struct Blk {
+      double2* A;
+      int gc, hc, zc;
+      double2 al;
+      int64_t off;
+    };
+    const Blk blks[4] = {{T, 1, 1, 3, e2, 0},
+                         {T, 3, 3, 4, e2, n * ld},
+                         {S1, 2, 2, 1, one, n},
+                         {S1, 4, 4, 3, one, n * ld + n}};
+    for (const Blk& b : blks) {
+      gemm<double>(n, n, n, one, v(b.A), v(G2(b.gc)), zero, v(M + b.off, ld));
+      hipLaunchKernelGGL(k_lay_mblock, dim3(nblk(nn)), dim3(256), 0, stream(), n, ld, M + b.off, b.al, H2(b.hc), ikde,
+                         nz, G2(b.zc));
+      MNP_HIP(hipGetLastError());
+    }
+    const int info = getrf<double>(ld, ld, v(M, ld), piv, invM, invM + inv_blocks(ld), PIVOT_PARTIAL, nullptr, ws);
+    MNP_CHECK(info == 0, MNP_ERR_SINGULAR, "bemretlayer: singular response matrix");
+    perm_from_pivots(ld, piv, sigM, ws);
+    fM = LU<double>{v(M, ld), invM, invM + inv_blocks(ld), sigM};
+    ready = true;
+  }
+
+  void solve(int64_t p, const double2* const in[8], double2* sig1, double2* sig2, double2* h1, double2* h2) {
+    MNP_CHECK(ready, MNP_ERR_STATE, "bemretlayer: solve before initmat");
+    const int64_t np1 = n * p, np3 = 3 * n * p;
+    double2* buf = (double2*)wsv.get(0, sizeof(double2) * (size_t)(4 * np1 + 6 * np3 + 2 * np1));
+    double2 *phi = buf, *De = phi + np1, *T2 = De + np1, *U = T2 + np1;
+    double2 *a = U + np1, *alpha = a + np3, *T1 = alpha + np3, *T3 = T1 + np3, *V = T3 + np3, *Y = V + np3;
+    double2* xi = Y + np3;
+    const double2 one = mk<double>(1, 0), zero = mk<double>(0, 0);
+    const unsigned b1 = nblk(np1);
+    hipLaunchKernelGGL(k_lay_exc, dim3(b1), dim3(256), 0, stream(), n, p, in[0], in[1], in[2], in[3], in[4], in[5],
+                       in[6], in[7], g->geo.nvec, k, e1, e2, phi, a, alpha, De);
+    // alpha = alpha - Sigma1 a + ik eps1 n phi
+    gemm<double>(n, 3 * p, n, one, v(S1), v(a), zero, v(T1));
+    hipLaunchKernelGGL(k_lay_alpha, dim3(b1), dim3(256), 0, stream(), n, p, alpha, T1, phi, g->geo.nvec,
+                       cmul<double>(mk<double>(0, k), e1));
+    // De = De - eps1 Sigma1 phi + ik eps1 n.a + ik (eps1 - eps2) npar.(Gamma alpha); rhs = [De; alpha_z]
+    gemm<double>(n, p, n, one, v(S1), v(phi), zero, v(T2));
+    gemm<double>(n, 3 * p, n, one, v(Gam), v(alpha), zero, v(T3));
+    const double2 de = csub<double>(e1, e2);
+    hipLaunchKernelGGL(k_lay_rhs, dim3(b1), dim3(256), 0, stream(), n, p, De, T2, a, T3, alpha, g->geo.nvec, e1,
+                       cmul<double>(mk<double>(0, k), e1), cmul<double>(mk<double>(0, k), de), xi);
+    MNP_HIP(hipGetLastError());
+    getrs<double>(2 * n, p, fM, v(xi, 2 * n), ws);
+    // sig2, h2z = xi; U = G2ss sig2 + G2sh h2z
+    gemm<double>(n, p, n, one, v(G2(1)), v(xi, 2 * n), zero, v(U));
+    gemm<double>(n, p, n, one, v(G2(3)), v(xi + n, 2 * n), one, v(U));
+    // h2par = G2p^{-1} Gamma (alpha_par + ik (eps1 - eps2) npar U)
+    hipLaunchKernelGGL(k_lay_v, dim3(b1), dim3(256), 0, stream(), n, p, alpha, U, g->geo.nvec,
+                       cmul<double>(mk<double>(0, k), de), V);
+    gemm<double>(n, 3 * p, n, one, v(Gam), v(V), zero, v(T1));
+    gemm<double>(n, 3 * p, n, one, v(G2pi), v(T1), zero, v(h2));  // z-components zero
+    // h1 = G1^{-1} ([G2p h2par; G2hs sig2 + G2hh h2z] + a)
+    gemm<double>(n, 3 * p, n, one, v(G2(0)), v(h2), zero, v(Y));
+    hipLaunchKernelGGL(k_lay_setz, dim3(b1), dim3(256), 0, stream(), n, p, xi + n, 2 * n, h2);
+    hipLaunchKernelGGL(k_lay_copy_rows, dim3(b1), dim3(256), 0, stream(), n, p, xi, 2 * n, sig2);
+    // sig1 = G1^{-1} (U + phi)
+    hipLaunchKernelGGL(k_zaxpby, dim3(b1), dim3(256), 0, stream(), np1, one, U, one, phi, U);
+    gemm<double>(n, p, n, one, v(G1i), v(U), zero, v(sig1));
+    gemm<double>(n, p, n, one, v(G2(2)), v(xi, 2 * n), zero, v(Y + 2 * n, 3 * n));
+    gemm<double>(n, p, n, one, v(G2(4)), v(xi + n, 2 * n), one, v(Y + 2 * n, 3 * n));
+    hipLaunchKernelGGL(k_zaxpby, dim3(nblk(np3)), dim3(256), 0, stream(), np3, one, Y, one, a, Y);
+    gemm<double>(n, 3 * p, n, one, v(G1i), v(Y), zero, v(h1));
+    MNP_HIP(hipGetLastError());
+  }
+};
+
+}  // namespace mnp
+
+struct mnp_layertab_s {
+  mnp::LayerTab* t;
+};
+struct mnp_greenretlayer_s {
+  mnp::GreenRetLayer* gl;
+};
+struct mnp_bemretlayer_s {
+  mnp::BemRetLayer* b;
+};
+
+using namespace mnp;
+
+extern "C" {
+
+void mnp_layer_options_default(mnp_layer_options* op) {
+  if (!op) return;
+  op->z0 = 0;
+  op->ztol = 2e-2;
+  op->rmin = 1e-2;
+  op->zmin = 1e-2;
+  op->semi = 0.1;
+  op->ratio = 2;
+}
+
+int mnp_layertab_create(const mnp_layer_options* op, int64_t nr, const double* r, int64_t nz, const double* z1,
+                        double z2, mnp_layertab_t* out) {
+  return guarded([&] {
+    MNP_CHECK(op && r && z1 && out, MNP_ERR_ARG, "layertab_create: NULL argument");
+    auto* t = new LayerTab(*op, nr, r, nz, z1, z2);
+    *out = new mnp_layertab_s{t};
+  });
+}
+int mnp_layertab_destroy(mnp_layertab_t t) {
+  return guarded([&] {
+    if (!t) return;
+    delete t->t;
+    delete t;
+  });
+}
+int mnp_layertab_set(mnp_layertab_t t, int64_t nenei, const double* enei, const mnp_complex* eps_up,
+                     const mnp_complex* eps_down) {
+  return guarded([&] {
+    MNP_CHECK(t && enei && eps_up && eps_down, MNP_ERR_ARG, "layertab_set: NULL argument");
+    t->t->set(nenei, enei, eps_up, eps_down);
+    MNP_HIP(hipStreamSynchronize(stream()));
+  });
+}
+int mnp_layertab_eval(mnp_layertab_t t, double enei, mnp_complex eps_up, mnp_complex eps_down, void* out) {
+  return guarded([&] {
+    MNP_CHECK(t, MNP_ERR_ARG, "layertab_eval: NULL argument");
+    const double2* v = t->t->at(enei, to_d2(eps_up), to_d2(eps_down));
+    if (out)
+      MNP_HIP(hipMemcpyAsync(out, v, sizeof(double2) * t->t->size(), hipMemcpyDeviceToDevice, stream()));
+    MNP_HIP(hipStreamSynchronize(stream()));
+  });
+}
+int mnp_greenretlayer_create(mnp_greenret_t g, mnp_layertab_t tab, mnp_greenretlayer_t* out) {
+  return guarded([&] {
+    MNP_CHECK(g && tab && out, MNP_ERR_ARG, "greenretlayer_create: NULL argument");
+    auto* gl = new GreenRetLayer(g->g, tab->t);
+    *out = new mnp_greenretlayer_s{gl};
+  });
+}
+int mnp_greenretlayer_destroy(mnp_greenretlayer_t gl) {
+  return guarded([&] {
+    if (!gl) return;
+    delete gl->gl;
+    delete gl;
+  });
+}
+int mnp_greenretlayer_refined(mnp_greenretlayer_t gl, int64_t* npairs, int64_t* ndiag) {
+  return guarded([&] {
+    MNP_CHECK(gl && npairs && ndiag, MNP_ERR_ARG, "greenretlayer_refined: NULL argument");
+    *npairs = gl->gl->np;
+    *ndiag = gl->gl->nd;
+  });
+}
+int mnp_greenretlayer_eval(mnp_greenretlayer_t gl, double enei, mnp_complex eps_up, mnp_complex eps_down, void* G,
+                           void* F) {
+  return guarded([&] {
+    MNP_CHECK(gl && G && F, MNP_ERR_ARG, "greenretlayer_eval: NULL argument");
+    gl->gl->eval(enei, to_d2(eps_up), to_d2(eps_down), (double2*)G, (double2*)F);
+  });
+}
+int mnp_bemretlayer_create(mnp_greenret_t g, mnp_greenretlayer_t gl, mnp_bemretlayer_t* out) {
+  return guarded([&] {
+    MNP_CHECK(g && gl && out, MNP_ERR_ARG, "bemretlayer_create: NULL argument");
+    auto* b = new BemRetLayer(g->g, gl->gl);
+    *out = new mnp_bemretlayer_s{b};
+  });
+}
+int mnp_bemretlayer_destroy(mnp_bemretlayer_t b) {
+  return guarded([&] {
+    if (!b) return;
+    delete b->b;
+    delete b;
+  });
+}
+int mnp_bemretlayer_initmat(mnp_bemretlayer_t b, double enei, mnp_complex eps_in, mnp_complex eps_up,
+                            mnp_complex eps_down) {
+  return guarded([&] {
+    MNP_CHECK(b, MNP_ERR_ARG, "bemretlayer_initmat: NULL argument");
+    b->b->initmat(enei, to_d2(eps_in), to_d2(eps_up), to_d2(eps_down));
+  });
+}
+int mnp_bemretlayer_solve(mnp_bemretlayer_t b, int64_t npol, const void* phi1, const void* phi1p, const void* a1,
+                          const void* a1p, const void* phi2, const void* phi2p, const void* a2, const void* a2p,
+                          void* sig1, void* sig2, void* h1, void* h2) {
+  return guarded([&] {
+    MNP_CHECK(b && sig1 && sig2 && h1 && h2, MNP_ERR_ARG, "bemretlayer_solve: NULL argument");
+    MNP_CHECK(npol >= 1, MNP_ERR_ARG, "bemretlayer_solve: npol must be positive");
+    const double2* in[8] = {(const double2*)phi1, (const double2*)phi1p, (const double2*)a1, (const double2*)a1p,
+                            (const double2*)phi2, (const double2*)phi2p, (const double2*)a2, (const double2*)a2p};
+    b->b->solve(npol, in, (double2*)sig1, (double2*)sig2, (double2*)h1, (double2*)h2);
+  });
+}
+int mnp_planewaveretlayer_potential(mnp_greenret_t g, const mnp_layer_options* op, double enei, mnp_complex eps_up,
+                                    mnp_complex eps_down, int64_t npol, const double* pol, const double* dir,
+                                    void* phi2, void* phi2p, void* a2, void* a2p) {
+  return guarded([&] {
+    MNP_CHECK(g && op && pol && dir && phi2 && phi2p && a2 && a2p, MNP_ERR_ARG,
+              "planewaveretlayer_potential: NULL argument");
+    MNP_CHECK(npol >= 1, MNP_ERR_ARG, "planewaveretlayer_potential: npol must be positive");
+    const int64_t n = g->g->n;
+    std::vector<double> pd(6 * npol);
+    for (int64_t i = 0; i < 3 * npol; ++i) {
+      pd[i] = pol[i];
+      pd[3 * npol + i] = dir[i];
+    }
+    double* d = dev_alloc<double>(6 * npol);
+    MNP_HIP(hipMemcpy(d, pd.data(), sizeof(double) * 6 * npol, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_pw_layer, dim3(nblk(n * npol)), dim3(256), 0, stream(), n, npol, g->g->geo.pos,
+                       g->g->geo.nvec, d, 2 * M_PI / enei, to_d2(eps_up), to_d2(eps_down), op->z0, (double2*)phi2,
+                       (double2*)phi2p, (double2*)a2, (double2*)a2p);
+    MNP_HIP(hipGetLastError());
+    MNP_HIP(hipStreamSynchronize(stream()));
+    MNP_HIP(hipFree(d));
+  });
+}
+int mnp_farfield_layer(mnp_greenret_t g, const mnp_layer_options* op, double enei, mnp_complex eps_up,
+                       mnp_complex eps_down, int64_t ndir, const void* dir, int64_t npol, const void* sig2,
+                       const void* h2, void* e, void* h) {
+  return guarded([&] {
+    MNP_CHECK(g && op && dir && sig2 && h2 && e && h, MNP_ERR_ARG, "farfield_layer: NULL argument");
+    MNP_CHECK(npol >= 1 && npol <= LFF_MAXP, MNP_ERR_ARG, "farfield_layer: npol must be in 1..8");
+    const GreenRet& G = *g->g;
+    hipLaunchKernelGGL(k_farfield_layer, dim3((unsigned)ndir), dim3(256), 0, stream(), G.n, npol, ndir,
+                       (const double*)dir, G.geo.pos, G.geo.area, (const double2*)sig2, (const double2*)h2,
+                       2 * M_PI / enei, to_d2(eps_up), to_d2(eps_down), op->z0, (double2*)e, (double2*)h);
+    MNP_HIP(hipGetLastError());
+    MNP_HIP(hipStreamSynchronize(stream()));
+  });
+}
+
+}  // extern "C"

@@ -1,13 +1,24 @@
-"""Quasistatic particle above a substrate: one planar interface between an
-upper and a lower medium (MNPBEM Particles/@layerstructure, BEM/@bemstatlayer,
-Simulation/static/@planewavestatlayer, @spectrumstatlayer).
+"""Particle above a substrate: one planar interface between an upper and a
+lower medium (MNPBEM Particles/@layerstructure), quasistatic
+(BEM/@bemstatlayer, Simulation/static/@planewavestatlayer, @spectrumstatlayer)
+and retarded (Greenfun/@greentablayer, @compgreentablayer, @greenretlayer,
+@compgreenretlayer, BEM/@bemretlayer, Simulation/retarded/@planewaveretlayer,
+@spectrumretlayer).
 
-The solver runs on the GPU (mnp_bemstatlayer_* in include/mnpbem_hip.h): the
-reflected Green function is that of the particle's image in the interface,
-weighted per wavelength by f2 = -(eps_down - eps_up)/(eps_down + eps_up).
-The plane wave's field at the particle (Fresnel reflection of the incoming
-wave) and the cross sections from the induced dipole moment are host-side
-formulas over a handful of vectors, as in the reference.
+Quasistatic: the solver runs on the GPU (mnp_bemstatlayer_* in
+include/mnpbem_hip.h); the reflected Green function is that of the particle's
+image in the interface, weighted per wavelength by f2 = -(eps_down -
+eps_up)/(eps_down + eps_up).  The plane wave's field at the particle (Fresnel
+reflection of the incoming wave) and the cross sections from the induced
+dipole moment are host-side formulas over a handful of vectors, as in the
+reference.
+
+Retarded: the Sommerfeld integrals of the reflected Green function are
+tabulated on the GPU (mnp_layertab_*), interpolated into the reflected
+matrices with the near-pair refinement (mnp_greenretlayer_*) and solved with
+the chain of bemretlayer in complex FP64 (mnp_bemretlayer_*); the plane
+wave's potentials and the far fields with the interface are GPU kernels, the
+cross sections FP64 host sums over them.
 
 Supported: one interface; particles whose faces all lie above it (the
 reference also treats faces lying in the interface, which are not built).
@@ -25,18 +36,74 @@ _MATLAB_EPS = float(np.finfo(float).eps)
 
 
 class layerstructure:
-    """layerstructure(epstab, ind, z): media epstab[ind[0]-1] above and
-    epstab[ind[1]-1] below the interface z (Particles/@layerstructure/
-    layerstructure.m:27-31); ztol as layerstructure.ztol."""
+    """layerstructure(epstab, ind, z, **options): media epstab[ind[0]-1] above
+    and epstab[ind[1]-1] below the interface z (Particles/@layerstructure/
+    layerstructure.m:27-31); options ztol, rmin, zmin, semi, ratio as
+    layerstructure.options (options.m:22-29)."""
 
-    def __init__(self, epstab, ind, z, ztol=2e-2):
+    def __init__(self, epstab, ind, z, ztol=2e-2, rmin=1e-2, zmin=1e-2, semi=0.1, ratio=2.0, **_ignored):
         z = np.atleast_1d(np.asarray(z, dtype=np.float64))
         if len(ind) != 2 or len(z) != 1:
             raise NotImplementedError("layerstructure: one interface (two media) is supported")
         self.ind = list(ind)
         self.eps = [epstab[i - 1] for i in self.ind]
         self.z = z
-        self.ztol = float(ztol)
+        self.ztol, self.rmin, self.zmin = float(ztol), float(rmin), float(zmin)
+        self.semi, self.ratio = float(semi), float(ratio)
+
+    @staticmethod
+    def options(**kw):
+        """layerstructure.options: the defaults, overridden by kw."""
+        op = dict(ztol=2e-2, rmin=1e-2, zmin=1e-2, semi=0.1, ratio=2.0)
+        op.update(kw)
+        return op
+
+    def c_options(self):
+        return _lib.LayerOptions(float(self.z[0]), self.ztol, self.rmin, self.zmin, self.semi, self.ratio)
+
+    def mindist(self, z):
+        return np.abs(np.asarray(z, dtype=np.float64) - self.z[0])
+
+    def round(self, z, zmin=None):
+        """round.m:13-24: points closer than zmin to the interface move to zmin."""
+        zmin = self.zmin if zmin is None else zmin
+        z = np.array(z, dtype=np.float64, copy=True)
+        d = np.abs(z - self.z[0])
+        sel = d <= zmin
+        z[sel] = self.z[0] + np.sign(z[sel] - self.z[0]) * zmin
+        return z
+
+    def tabspace(self, p, nr=30, nz=30, scale=1.05):
+        """Table grid for the particle p above the interface (tabspace2.m:
+        44-75 with limits and adjust.m:18-33, tabspace1.m:24-48 with 'log'
+        spacing, greentablayer/private/init.m:8-14): dict r, z1, z2."""
+        v = np.asarray(p.verts, dtype=np.float64)
+        z0 = self.z[0]
+        if np.any(v[:, 2] <= z0):
+            raise NotImplementedError("tabspace: vertices at or below the interface are not supported")
+        xy = v[:, :2]
+        rmax = 0.0
+        for i in range(0, len(xy), 2048):  # largest vertex distance, in chunks
+            d2 = np.sum((xy[i:i + 2048, None, :] - xy[None, :, :]) ** 2, axis=-1)
+            rmax = max(rmax, float(np.sqrt(d2.max())))
+        m = 0.5 * rmax
+        r = [max(m - 0.5 * scale * rmax, 0.0), m + 0.5 * scale * rmax]
+        zl, zh = float(v[:, 2].min()), float(v[:, 2].max())
+        if abs(zl - zh) < 1e-10:
+            zr = [zl - 1e-2, zl + 1e-2]
+        else:
+            mz = 0.5 * (zl + zh)
+            zr = [mz + 0.5 * scale * (zl - zh), mz - 0.5 * scale * (zl - zh)]
+        if zr[0] <= z0:  # adjust.m:45 has <, which leaves a bound exactly at z0 (log10(0) later)
+            zr[0] = z0 + 1e-10
+        z1 = [2 * zr[0] - z0, 2 * zr[1] - z0]
+        rg = np.logspace(np.log10(max(r[0], self.rmin)), np.log10(r[1]), nr)
+        zz = np.sort(self.round(np.array(z1), zmin=0.999 * self.zmin))
+        if abs(zz[0] - zz[1]) < 1e-3:
+            raise NotImplementedError("tabspace: a flat particle at the interface is not supported")
+        zg = z0 + np.logspace(np.log10(zz[0] - z0), np.log10(zz[1] - z0), nz)
+        return {"r": np.unique(np.maximum(self.rmin, rg)), "z1": np.unique(self.round(zg)),
+                "z2": float(self.round(np.array([z0 + 1e-10]))[0])}
 
     @property
     def n(self):
@@ -389,3 +456,301 @@ class dipolestatlayer(dipolestat):
         tot = rad if real else np.real(rad + np.imag(np.sum(e * d, axis=1)) / (0.5 * nb * gamma))
         shape = lambda v: np.asarray(v).reshape(ndip, npt).T
         return shape(tot), shape(rad), np.full((npt, ndip), np.real(nb * gamma))
+
+
+# ------------------------------------------------------------ retarded
+_COMPONENTS = ("p", "ss", "hs", "sh", "hh")  # order of reflectionsubs.m's fields
+
+
+class compgreentablayer:
+    """Tabulated reflected Green function of the substrate (Greenfun/
+    @compgreentablayer, @greentablayer) on the grid tab = layer.tabspace(p);
+    set(enei) precomputes the tables at those wavelengths (set.m), which
+    later wavelengths interpolate linearly (eval.m:24-50); without set the
+    table is computed at each new wavelength."""
+
+    def __init__(self, layer, tab):
+        self.layer = layer
+        self.r = np.ascontiguousarray(tab["r"], dtype=np.float64)
+        self.z1 = np.ascontiguousarray(tab["z1"], dtype=np.float64)
+        self.z2 = float(tab["z2"])
+        h = C.c_void_p()
+        op = layer.c_options()
+        _lib.call("mnp_layertab_create", C.byref(op), len(self.r), self.r.ctypes.data_as(C.c_void_p), len(self.z1),
+                  self.z1.ctypes.data_as(C.c_void_p), self.z2, C.byref(h))
+        self.handle = h
+        self.enei = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib.can_release():
+            _lib.lib().mnp_layertab_destroy(h)
+            self.handle = None
+
+    def set(self, enei, op=None):
+        enei = np.ascontiguousarray(np.atleast_1d(enei), dtype=np.float64)
+        eu = (_lib.MnpComplex * len(enei))(*[_lib.MnpComplex.of(self.layer.epsilon(e)[0]) for e in enei])
+        ed = (_lib.MnpComplex * len(enei))(*[_lib.MnpComplex.of(self.layer.epsilon(e)[1]) for e in enei])
+        _lib.call("mnp_layertab_set", self.handle, len(enei), enei.ctypes.data_as(C.c_void_p), eu, ed)
+        self.enei = enei
+        return self
+
+    def ismember(self, layer, enei=None, p=None):
+        """greentablayer/ismember.m: same layer, wavelengths inside the set range."""
+        if self.enei is None or layer is not self.layer:
+            return False
+        if enei is not None and (np.min(enei) < self.enei.min() or np.max(enei) > self.enei.max()):
+            return False
+        return True
+
+    def values(self, enei):
+        """The normalised table at enei as host (nr, nz, 15) complex: d G,
+        d^3 Fr / r, d^3 Fz / zmin of p, ss, hs, sh, hh (norm.m)."""
+        eu, ed = self.layer.epsilon(enei)
+        out = dev.colmajor((15 * len(self.r) * len(self.z1),))
+        _lib.call("mnp_layertab_eval", self.handle, float(enei), _lib.MnpComplex.of(eu), _lib.MnpComplex.of(ed),
+                  dev.ptr(dev.base(out)))
+        return dev.to_host(out).reshape(len(self.r), len(self.z1), 15)
+
+
+class greenretlayer:
+    """Reflected Green function of the particle with itself on a table
+    (Greenfun/@greenretlayer); eval(enei) returns the device G and F = nvec .
+    Gp of the components p, ss, hs, sh, hh, each 5 x n x n."""
+
+    def __init__(self, p, tab):
+        h = C.c_void_p()
+        _lib.call("mnp_greenretlayer_create", p.green.handle, tab.handle, C.byref(h))
+        self.handle, self.p, self.tab = h, p, tab
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib.can_release():
+            _lib.lib().mnp_greenretlayer_destroy(h)
+            self.handle = None
+
+    def refined(self):
+        npairs, ndiag = C.c_int64(), C.c_int64()
+        _lib.call("mnp_greenretlayer_refined", self.handle, C.byref(npairs), C.byref(ndiag))
+        return npairs.value, ndiag.value
+
+    def eval(self, enei):
+        n = self.p.n
+        eu, ed = self.tab.layer.epsilon(enei)
+        G = dev.colmajor((n, n, 5))
+        F = dev.colmajor((n, n, 5))
+        _lib.call("mnp_greenretlayer_eval", self.handle, float(enei), _lib.MnpComplex.of(eu), _lib.MnpComplex.of(ed),
+                  dev.ptr(dev.base(G)), dev.ptr(dev.base(F)))
+        return G, F
+
+
+def _greentab(p, op):
+    tab = op.get("greentab")
+    layer = op["layer"]
+    if tab is None:
+        tab = compgreentablayer(layer, layer.tabspace(p))
+    elif tab.layer is not layer:
+        raise ValueError("greentab: tabulated for another layer structure")
+    return tab
+
+
+class bemretlayer:
+    """Dense retarded BEM solver above a substrate (BEM/@bemretlayer), complex
+    FP64: op['layer'] a layerstructure whose upper medium is the particle's
+    outside; op['greentab'] a compgreentablayer (default: the tabspace of p,
+    tabulated at each wavelength)."""
+
+    def __init__(self, p, op=None, precision=None):
+        op = p.op if op is None else op
+        self.layer = op["layer"]
+        _check_above(p, self.layer)
+        if (precision or op.get("precision", "fp64")) != "fp64":
+            raise NotImplementedError("bemretlayer: only the complex FP64 chain is built")
+        self.p, self.tab = p, _greentab(p, op)
+        self.green = greenretlayer(p, self.tab)
+        h = C.c_void_p()
+        _lib.call("mnp_bemretlayer_create", p.green.handle, self.green.handle, C.byref(h))
+        self.handle = h
+        self.enei = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib.can_release():
+            _lib.lib().mnp_bemretlayer_destroy(h)
+            self.handle = None
+
+    def __call__(self, enei):
+        if self.enei != enei:
+            self.enei = None
+            eu, ed = self.layer.epsilon(enei)
+            if abs(complex(self.p.eps2(enei)) - eu) > 1e-12 * max(1.0, abs(eu)):
+                raise ValueError("bemretlayer: the particle's outside must be the upper medium of the layer")
+            _lib.call("mnp_bemretlayer_initmat", self.handle, float(enei), _lib.MnpComplex.of(self.p.eps1(enei)),
+                      _lib.MnpComplex.of(eu), _lib.MnpComplex.of(ed))
+            self.enei = enei
+        return self
+
+    def solve(self, exc):
+        """sig = bem \\ exc (mldivide.m:31-76)."""
+        self(exc.enei)
+        n = self.p.n
+        npol = next(v.shape[-1] for v in exc.values() if v is not None)
+        args = []
+        for key in ("phi1", "phi1p", "a1", "a1p", "phi2", "phi2p", "a2", "a2p"):
+            v = exc.get(key)
+            args.append(None if v is None else dev.ptr(dev.base(v)))
+        out = [dev.colmajor((n, npol)), dev.colmajor((n, npol)), dev.colmajor((n, 3, npol)),
+               dev.colmajor((n, 3, npol))]
+        _lib.call("mnp_bemretlayer_solve", self.handle, npol, *args, *[dev.ptr(dev.base(t)) for t in out])
+        return compstruct(self.p, exc.enei, sig1=out[0], sig2=out[1], h1=out[2], h2=out[3])
+
+    mldivide = solve
+
+
+def _kz(k, kpar, offset=False):
+    kz = np.sqrt(k * k - kpar * kpar + 0j)
+    if offset:
+        kz = kz + 1e-10j
+        return kz * np.sign(kz.imag)
+    return kz * np.sign((kz + 1e-10j).imag)
+
+
+def _coef(k0, eu, ed, kpar, o, s, offset_ratio=True):
+    """p, hh, sh coefficients for the observer in medium o and the source in
+    medium s (1 above) at kpar times fresnel.m's k_s,z / k_o,z
+    (reflectionsubs.m:27-60, fresnel.m:17-33), and kz of both media."""
+    k1, k2 = k0 * np.sqrt(eu + 0j), k0 * np.sqrt(ed + 0j)
+    k1z, k2z = _kz(k1, kpar), _kz(k2, kpar)
+    rr = (k1z - k2z) / (k2z + k1z)
+    D = (k2z + k1z) * (eu * k2z + ed * k1z)
+    hh = ((k1z - k2z) * (2 * eu * k1z - ed * k1z + eu * k2z) / D,
+          (k2z - k1z) * (2 * ed * k2z - eu * k2z + ed * k1z) / D)
+    sh = (-2 * k0 * (ed - eu) * k1z / D, -2 * k0 * (eu - ed) * k2z / D)
+    q12, q21 = k1z / k2z, k2z / k1z
+    table = {(1, 1): (rr, hh[0], sh[0]), (1, 2): (1 + rr, q12 * (hh[1] + 1), -q12 * sh[1]),
+             (2, 1): (1 - rr, q21 * (hh[0] + 1), q21 * sh[0]), (2, 2): (-rr, hh[1], -sh[1])}
+    c = table[(o, s)]
+    kzo = {1: _kz(k1, kpar, True), 2: _kz(k2, kpar, True)}
+    ratio = kzo[s] / kzo[o] if offset_ratio else 1.0
+    return tuple(v * ratio for v in c), (k1z, k2z)
+
+
+class planewaveretlayer:
+    """Plane wave above a substrate, retarded (Simulation/retarded/
+    @planewaveretlayer): pol and dir npol x 3, op['layer'] a layerstructure;
+    scattering over pinfty = (nvec, area) of the unit sphere (default: the
+    256-point sphere of trisphere(256, 2))."""
+
+    def __init__(self, pol, dir, op=None, pinfty=None):
+        self.pol = np.atleast_2d(np.asarray(pol, dtype=np.float64))
+        self.dir = np.atleast_2d(np.asarray(dir, dtype=np.float64))
+        if np.any(np.sum(self.pol * self.dir, axis=1) != 0):
+            raise ValueError("polarisation must be perpendicular to the propagation direction")
+        if len(self.pol) > 8:
+            raise ValueError("planewaveretlayer: at most 8 polarisations")
+        self.layer = op["layer"]
+        if pinfty is None:
+            from . import shapes
+            pinfty = _flat_normals(*shapes.trisphere256(2.0))
+        self.pinf_nvec, self.pinf_area = (np.asarray(a, dtype=np.float64) for a in pinfty)
+
+    def __call__(self, p, enei):
+        """exc(p, enei): phi2, phi2p, a2, a2p at the faces (potential.m:20-105)."""
+        _check_above(p, self.layer)
+        n, npol = p.n, len(self.pol)
+        eu, ed = self.layer.epsilon(enei)
+        out = [dev.colmajor((n, npol)), dev.colmajor((n, npol)), dev.colmajor((n, 3, npol)),
+               dev.colmajor((n, 3, npol))]
+        op = self.layer.c_options()
+        pol, dr = np.asfortranarray(self.pol), np.asfortranarray(self.dir)
+        _lib.call("mnp_planewaveretlayer_potential", p.green.handle, C.byref(op), float(enei),
+                  _lib.MnpComplex.of(eu), _lib.MnpComplex.of(ed), npol, pol.ctypes.data_as(C.c_void_p),
+                  dr.ctypes.data_as(C.c_void_p), *[dev.ptr(dev.base(t)) for t in out])
+        return compstruct(p, enei, phi2=out[0], phi2p=out[1], a2=out[2], a2p=out[3])
+
+    potential = __call__
+
+    def farfield(self, sig, dirs):
+        """e, h (host ndir x 3 x npol) in the directions dirs
+        (@spectrumretlayer/farfield.m:20-104)."""
+        dirs = np.asfortranarray(np.atleast_2d(dirs), dtype=np.float64)
+        npol = sig.sig2.shape[-1]
+        eu, ed = self.layer.epsilon(sig.enei)
+        ddir = dev.to_device_colmajor(dirs, dev.torch.float64)
+        e = dev.colmajor((len(dirs), 3, npol))
+        h = dev.colmajor((len(dirs), 3, npol))
+        op = self.layer.c_options()
+        _lib.call("mnp_farfield_layer", sig.p.green.handle, C.byref(op), float(sig.enei), _lib.MnpComplex.of(eu),
+                  _lib.MnpComplex.of(ed), len(dirs), dev.ptr(dev.base(ddir)), npol, dev.ptr(dev.base(sig.sig2)),
+                  dev.ptr(dev.base(sig.h2)), dev.ptr(dev.base(e)), dev.ptr(dev.base(h)))
+        return dev.to_host(e), dev.to_host(h)
+
+    def efresnel(self, enei):
+        """Reflected and transmitted fields and wavevectors per polarisation
+        (Particles/@layerstructure/efresnel.m:19-66)."""
+        k0 = 2 * np.pi / enei
+        eu, ed = self.layer.epsilon(enei)
+        ks = (k0 * np.sqrt(eu + 0j), k0 * np.sqrt(ed + 0j))
+        z0 = self.layer.z[0]
+        za, zb = z0 + 1e-10, z0 - 1e-10
+        out = []
+        for pl, d in zip(self.pol, self.dir):
+            if d[2] < 0:
+                posr, post = (za, 1, za, 1), (zb, 2, za, 1)
+            else:
+                posr, post = (zb, 2, zb, 2), (za, 1, zb, 2)
+            kpv = ks[post[3] - 1] * d[:2]
+            kp2 = np.dot(kpv, kpv)
+            kzr = np.sqrt(ks[posr[1] - 1] ** 2 - kp2 + 0j)
+            kzr = kzr * np.sign((kzr + 1e-10j).imag)
+            kzt = np.sqrt(ks[post[1] - 1] ** 2 - kp2 + 0j)
+            kzt = kzt * np.sign((kzt + 1e-10j).imag)
+            kr = np.array([kpv[0], kpv[1], -np.sign(d[2]) * kzr])
+            kt = np.array([kpv[0], kpv[1], np.sign(d[2]) * kzt])
+            kpn = np.sqrt(kp2 + 0j)
+            fields = []
+            for (z1, i1, z2, i2), kv in ((posr, kr), (post, kt)):
+                (cp, chh, csh), kz = _coef(k0, eu, ed, kpn, i1, i2)
+                g = np.exp(1j * kz[i1 - 1] * abs(z1 - z0)) * np.exp(1j * kz[i2 - 1] * abs(z2 - z0))
+                ev = (np.array([cp * pl[0], cp * pl[1], chh * pl[2]]) - kv / k0 * csh * pl[2]) * g
+                fields.append(ev)
+            er = np.exp(-1j * kr[2] * posr[2] - 1j * kr[2] * posr[0]) * fields[0]
+            et = np.exp(-1j * kr[2] * post[2] - 1j * kt[2] * post[0]) * fields[1]
+            out.append((er, et, kr, kt))
+        return out
+
+    def cross_sections(self, sig):
+        """(ext, sca, abs) per polarisation (extinction.m 'full', scattering.m,
+        absorption.m = ext - sca), FP64 sums on the host over the GPU far fields."""
+        npol = len(self.pol)
+        ext = np.zeros(npol)
+        for i, (er, et, kr, kt) in enumerate(self.efresnel(sig.enei)):
+            knr = np.linalg.norm(kr)
+            dirs = [(kr / knr).real]
+            trans = abs(kt[2].imag) <= 1e-10
+            if trans:
+                dirs.append((kt / np.linalg.norm(kt)).real)
+            e, _ = self.farfield(sig, np.array(dirs))
+            esr = e[0, :, i]
+            est = e[1, :, i] if trans else np.zeros(3, complex)
+            ext[i] = 4 * np.pi / knr * (np.vdot(er, esr).imag + np.vdot(et, est).imag)
+        e, h = self.farfield(sig, self.pinf_nvec)
+        s = np.cross(e, np.conj(h), axisa=1, axisb=1, axisc=1)
+        dsca = 0.5 * np.real(np.einsum("dc,dcp->dp", self.pinf_nvec, s))
+        sca = self.pinf_area @ dsca
+        eu, ed = self.layer.epsilon(sig.enei)
+        dz = self.dir[:, 2]
+        fac = np.where(dz < 0, 0.5 * np.sqrt(eu + 0j), np.where(dz > 0, 0.5 * np.sqrt(ed + 0j), 1.0))
+        sca = np.real(sca / fac)
+        return ext, sca, ext - sca
+
+    def ext(self, sig):
+        return self.cross_sections(sig)[0]
+
+    def sca(self, sig):
+        return self.cross_sections(sig)[1]
+
+    def abs(self, sig):
+        return self.cross_sections(sig)[2]
+
+    extinction, scattering, absorption = ext, sca, abs

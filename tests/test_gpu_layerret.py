@@ -1,0 +1,138 @@
+"""GPU parity of the retarded substrate path through the C ABI against the
+oracle's restatement (oracle/layerret_oracle.py): the Sommerfeld tables,
+their interpolation in wavelength, the reflected Green matrices with the
+near-pair and diagonal refinement, the bemretlayer solution, the plane wave's
+potentials and the cross sections; with equal media the solution of the free
+particle."""
+import numpy as np
+import pytest
+
+from mnpbem_amd import shapes
+from oracle import layerret_oracle as L
+from oracle import mnpbem_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DRUDE = (10.0, 9.065, 0.0708)
+POL = np.array([[1.0, 0, 0], [0.6, 0, 0.8], [0, 1.0, 0], [0.8, 0, -0.6]])
+DIR = np.array([[0, 0, -1.0], [0.8, 0, -0.6], [0.8, 0, -0.6], [0.6, 0, 0.8]])
+NTAB = 10
+
+
+def _setup(eps_down, level=2, radius=5.0, gap=1.0, ntab=NTAB, eps_up=1.0):
+    from mnpbem_amd import bem as B
+    from mnpbem_amd import layer as LY
+    v, f = shapes.icosphere(level, radius)
+    v = v + [0, 0, radius + gap]
+    epstab = [B.epsconst(eps_up), B.epsdrude(*DRUDE), B.epsconst(eps_down)]
+    layer = LY.layerstructure(epstab, [1, 3], 0.0)
+    p = B.comparticle(epstab, [(v, f)], [2, 1], 1, B.bemoptions(sim="ret"))
+    tab = LY.compgreentablayer(layer, layer.tabspace(p, nr=ntab, nz=ntab))
+    op = B.bemoptions(sim="ret", layer=layer, greentab=tab)
+    sub = L.Substrate(eps_up, eps_down, 0.0)
+    otab = L.GreenTable(sub, tab.r, tab.z1, tab.z2)
+    return B, LY, p, op, tab, sub, otab, (v, f)
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("eps_down", [2.25, -10 + 1.2j])
+def test_tables_match_oracle(eps_down):
+    """The Sommerfeld tables (both legs: the grid has r / Z from below 1e-3
+    to above 2) equal the oracle's quadrature with scipy's Bessel functions."""
+    B, LY, p, op, tab, sub, otab, _ = _setup(eps_down)
+    rg = np.asarray(tab.r)
+    Zt = sub.mindist(tab.z1) + sub.mindist(tab.z2)
+    assert (rg[:, None] > 2 * Zt[None, :]).any() and (rg[:, None] <= 2 * Zt[None, :]).any()
+    for lam in (450.0, 700.0):
+        got = tab.values(lam)
+        want = otab.compute(lam)[1]  # (3, 5, nr, nz)
+        for w in range(3):
+            for c in range(5):
+                e = _rel(got[:, :, 5 * w + c], want[w, c])
+                assert e < 1e-10, (lam, w, c, e)
+    print(f"[layerret] tables eps_down={eps_down}: {got.shape} match", flush=True)
+
+
+def test_tables_interpolate_in_wavelength():
+    """set() tables at 500 and 600 nm; at 530 nm the table is the linear
+    interpolation (greentablayer/eval.m)."""
+    from mnpbem_amd import layer as LY
+    B, _, p, op, tab, sub, otab, _ = _setup(2.25, ntab=6)
+    t2 = LY.compgreentablayer(tab.layer, {"r": tab.r, "z1": tab.z1, "z2": tab.z2}).set([500.0, 600.0])
+    otab.set([500.0, 600.0])
+    got = t2.values(530.0)
+    want = otab.normalised(530.0)
+    assert _rel(np.moveaxis(got.reshape(len(tab.r), len(tab.z1), 3, 5), (2, 3), (0, 1)), want) < 1e-10
+    with pytest.raises(Exception):
+        t2.values(700.0)
+
+
+def test_reflected_matrices_match_oracle():
+    """Reflected G and F of the five components, with the shape-function
+    refinement of near pairs and the polar rule on diagonal elements."""
+    from mnpbem_amd import device as dev
+    B, LY, p, op, tab, sub, otab, (v, f) = _setup(-10 + 1.2j)
+    gl = LY.greenretlayer(p, tab)
+    og = L.GreenRetLayer(sub, v, f, otab)
+    npairs, ndiag = gl.refined()
+    assert npairs == int((og.ir == 1).sum()) and ndiag == int((np.diag(og.ir) == 2).sum()) and ndiag > 0
+    lam = 560.0
+    G, F = gl.eval(lam)
+    G, F = dev.to_host(G), dev.to_host(F)
+    ref = og.eval(lam)
+    for c in range(5):
+        eg = _rel(G[:, :, c], ref["G"][c])
+        ef = _rel(F[:, :, c], ref["F"][c])
+        print(f"[layerret] component {L.COMPONENTS[c]}: G {eg:.1e} F {ef:.1e}", flush=True)
+        assert eg < 1e-10 and ef < 1e-10
+
+
+@pytest.mark.parametrize("eps_down", [2.25, -10 + 1.2j])
+def test_solver_matches_oracle(eps_down, pinfty):
+    """sig1, sig2, h1, h2 and the cross sections for light from above (normal
+    and oblique TM and TE) and from below, at two wavelengths."""
+    from mnpbem_amd import device as dev
+    from mnpbem_amd.layer import bemretlayer, planewaveretlayer
+    B, LY, p, op, tab, sub, otab, (v, f) = _setup(eps_down)
+    bem = B.bemsolver(p, op)
+    exc = B.planewave(POL, DIR, op)
+    assert isinstance(bem, bemretlayer) and isinstance(exc, planewaveretlayer)
+    g = O.GreenRet(v, f)
+    ob = L.BemRetLayer(g, L.GreenRetLayer(sub, v, f, otab))
+    for lam in (520.0, 640.0):
+        ex = exc(p, lam)
+        phi, phip, a, ap = L.planewave_potential(sub, g.pos, g.nvec, POL, DIR, lam)
+        for key, want in (("phi2", phi), ("phi2p", phip), ("a2", a), ("a2p", ap)):
+            assert _rel(dev.to_host(ex[key]), want) < 1e-12, key
+        sig = bem.solve(ex)
+        ref = ob.initmat(lam, O.eps_drude(lam, *DRUDE)).solve({"phi2": phi, "phi2p": phip, "a2": a, "a2p": ap})
+        e_sig = max(_rel(dev.to_host(sig[k]), ref[k]) for k in ref)
+        ext, sca, _ = exc.cross_sections(sig)
+        oext = L.extinction(sub, g.pos, g.area, ref, POL, DIR, lam)
+        osca = L.scattering(sub, g.pos, g.area, ref, DIR, lam, *pinfty)
+        # light from below through the lossy metal has no far field: ext 0
+        e_cs = max(np.max(np.abs(ext - oext)) / np.max(np.abs(oext)), np.max(np.abs(sca / osca - 1)))
+        print(f"[layerret] eps_down={eps_down} {lam:.0f} nm: sig {e_sig:.1e}, cross sections {e_cs:.1e}, "
+              f"ext {np.round(ext, 4)}", flush=True)
+        assert e_sig < 1e-9 and e_cs < 1e-9
+
+
+def test_equal_media_is_the_free_particle():
+    """eps_down = eps_up: the layer solver and plane wave give the free
+    particle's solution and cross sections (bemret, planewaveret)."""
+    from mnpbem_amd import device as dev
+    B, LY, p, op, tab, sub, otab, _ = _setup(1.77, eps_up=1.77, ntab=6)
+    lam = 560.0
+    sig = B.bemsolver(p, op).solve(B.planewave(POL[:3], DIR[:3], op)(p, lam))
+    pf = B.comparticle([B.epsconst(1.77), B.epsdrude(*DRUDE)], [(p.verts, p.faces)], [2, 1], 1,
+                       B.bemoptions(sim="ret"))
+    excf = B.planewave(POL[:3], DIR[:3], B.bemoptions(sim="ret"))
+    ref = B.bemret(pf, precision="fp64").solve(excf(pf, lam))
+    for k in ("sig1", "sig2", "h1", "h2"):
+        assert _rel(dev.to_host(sig[k]), dev.to_host(ref[k])) < 1e-10
+    a = B.planewave(POL[:3], DIR[:3], op).cross_sections(sig)
+    b = excf.cross_sections(ref)
+    assert np.allclose(a[0], b[0], rtol=1e-10) and np.allclose(a[1], b[1], rtol=1e-10)
