@@ -302,6 +302,55 @@ def run_substrate(level=5, steps=3):
             "max_rel_err_vs_fp64": err["mixed"], "max_rel_err_mixed_bf16_vs_fp64": err["mixed_bf16"]}
 
 
+def run_substrate_ret(level=4, steps=4):
+    """Retarded solver above a glass substrate (bemretlayer, complex FP64):
+    the 10 x 10 x 40 nm ellipsoid at 5,120 faces 5 nm above eps = 2.25, air
+    above, five TM plane waves from above at 0..80 degrees (the excitation of
+    the reference's substrate spectra demo), the reflected Green function
+    tabulated on the 30 x 30 tabspace grid at linspace(350, 800, 5) and
+    interpolated in wavelength; per wavelength the reflected and direct
+    matrices, the chain, the solve and the cross sections."""
+    import torch
+
+    from mnpbem_amd import bem as B, shapes
+    from mnpbem_amd.layer import compgreentablayer, layerstructure
+
+    v, f = shapes.ellipsoid(level, (10.0, 10.0, 40.0))
+    v = v + np.array([0, 0, 45.0])
+    epstab = [B.epsconst(1.0), B.epsdrude(*DRUDE), B.epsconst(2.25)]
+    layer = layerstructure(epstab, [1, 3], 0.0)
+    p = B.comparticle(epstab, [(v, f)], [2, 1], 1, B.bemoptions(sim="ret"))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    tab = compgreentablayer(layer, layer.tabspace(p)).set(np.linspace(350.0, 800.0, 5))
+    torch.cuda.synchronize()
+    t_tab = time.perf_counter() - t
+    op = B.bemoptions(sim="ret", layer=layer, greentab=tab)
+    theta = np.linspace(0, 80, 5) * np.pi / 180
+    dirs = np.column_stack([np.sin(theta), 0 * theta, -np.cos(theta)])
+    pols = np.column_stack([np.cos(theta), 0 * theta, np.sin(theta)])
+    exc = B.planewave(pols, dirs, op)
+    t = time.perf_counter()
+    bem = B.bemsolver(p, op)
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t
+    lams = np.linspace(450.0, 750.0, 20)
+    exc.cross_sections(bem.solve(exc(p, lams[0])))  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    cs = [exc.cross_sections(bem.solve(exc(p, lam))) for lam in lams[1:1 + steps]]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    log(f"[bench] substrate (retarded): {steps / dt:.3f} wavelengths/s, tables {t_tab:.2f} s, "
+        f"refinement weights {t_init:.2f} s; ext {np.round(cs[0][0], 2)}")
+    return {"workload": f"retarded bemretlayer FP64: ellipsoid 10x10x40 nm ({len(f)} faces) 5 nm above eps 2.25, "
+                        "air above, 5 TM plane waves from above at 0-80 deg, 30x30 table set at 5 wavelengths",
+            "wavelengths_per_s_fp64": steps / dt, "s_per_wavelength": dt / steps, "table_set_s": t_tab,
+            "refinement_setup_s": t_init, "pivoting_fallbacks": bem.stats(),
+            "ext_first": [float(x) for x in cs[0][0]],
+            "sca_first": [float(x) for x in cs[0][1]]}
+
+
 def run_sensitivity(nsweep=32):
     """Config 5: extinction and its gradient with respect to all vertex
     coordinates and the Drude parameters (eps_inf, wp, gamma) by the adjoint
@@ -690,7 +739,7 @@ def main():
                     help="row tiles per coarse aggregate of the H-matrix preconditioner")
     ap.add_argument("--only", default=None,
                     help="comma list of side measurements to run alone (for profiling): "
-                         "quasistatic, mirror, substrate, multi_particle, hmatrix_dipole, sensitivity, batch_sweep")
+                         "quasistatic, mirror, substrate, substrate_ret, multi_particle, hmatrix_dipole, sensitivity, batch_sweep")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy three times (counter calibration)")
     a = ap.parse_args()
@@ -712,7 +761,8 @@ def main():
         return
     if a.only:
         runs = {"quasistatic": lambda: run_quasistatic(a.level), "mirror": run_mirror,
-                "substrate": lambda: run_substrate(a.level), "multi_particle": run_multi_particle,
+                "substrate": lambda: run_substrate(a.level), "substrate_ret": run_substrate_ret,
+                "multi_particle": run_multi_particle,
                 "sensitivity": run_sensitivity, "batch_sweep": run_batch_sweep,
                 "hmatrix_dipole": run_hmatrix_dipole}
         names = [x.strip() for x in a.only.split(",") if x.strip()]
@@ -803,6 +853,7 @@ def main():
         out["mirror"] = run_mirror()
     if not a.no_substrate:
         out["substrate"] = run_substrate(a.level)
+        out["substrate_ret"] = run_substrate_ret()
     if not a.no_sensitivity:
         out["sensitivity"] = run_sensitivity()
     if not a.no_multi_particle:

@@ -628,6 +628,9 @@ int mnp_bemretlayer_initmat(mnp_bemretlayer_t bem, double enei, mnp_complex eps_
 int mnp_bemretlayer_solve(mnp_bemretlayer_t bem, int64_t npol, const void* phi1, const void* phi1p,
                           const void* a1, const void* a1p, const void* phi2, const void* phi2p,
                           const void* a2, const void* a2p, void* sig1, void* sig2, void* h1, void* h2);
+/* LU factorisations so far that partial pivoting had to redo (each is tried
+ * without row exchanges first; the factors are partial pivoting's either way). */
+int mnp_bemretlayer_stats(mnp_bemretlayer_t bem, int* fallbacks);
 /* Outside potentials of plane waves above the substrate at the faces
  * (Simulation/retarded/@planewaveretlayer/potential.m:20-105 with
  * Particles/@layerstructure/fresnel.m; the normal derivative of the phase

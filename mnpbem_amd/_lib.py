@@ -117,6 +117,7 @@ PROTOTYPES = {
     "mnp_bemretlayer_destroy": (INT, [P]),
     "mnp_bemretlayer_initmat": (INT, [P, D, MnpComplex, MnpComplex, MnpComplex]),
     "mnp_bemretlayer_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mnp_bemretlayer_stats": (INT, [P, C.POINTER(INT)]),
     "mnp_planewaveretlayer_potential": (INT, [P, C.POINTER(LayerOptions), D, MnpComplex, MnpComplex, I64, P, P,
                                               P, P, P, P]),
     "mnp_farfield_layer": (INT, [P, C.POINTER(LayerOptions), D, MnpComplex, MnpComplex, I64, P, I64, P, P, P,

@@ -1059,19 +1059,28 @@ struct GreenRetLayer {
 
 // The bemretlayer chain in complex FP64 (initmat.m:17-95, mldivide.m:31-76)
 // for unique eps1 (inside) and eps2 (outside = upper medium): L1 = eps1,
-// L2p = eps2, so Gammapar = i k (eps1 - eps2) Gamma .* (npar npar').
+// L2p = eps2, so Gammapar = i k (eps1 - eps2) Gamma .* (npar npar').  G1 and
+// G2.p stay factorised (their inverses are applied by triangular solves),
+// Gamma is formed explicitly for its elementwise product; each LU first runs
+// without row exchanges and is redone with partial pivoting when that would
+// have exchanged a row, so the factors are partial pivoting's.
 struct BemRetLayer {
+  struct Fac {
+    int32_t* sig = nullptr;
+    double2* inv = nullptr;
+    LU<double> f{};
+  };
   const GreenRet* g;
   GreenRetLayer* gl;
   int64_t n;
-  double2 *G1, *H1, *G2c, *H2c, *G1i, *G2pi, *S1, *Gam, *T, *M;
-  int32_t *piv = nullptr, *sigM = nullptr, *sigA = nullptr;
-  double2 *inv = nullptr, *invM = nullptr;
-  LU<double> fM;
+  double2 *G1, *H1, *G2c, *H2c, *F2, *S1, *Gam, *T, *M, *Mb;
+  int32_t* piv = nullptr;
+  Fac f1, f2, fg, fm;
   Workspace ws, wsv;  // factorisations and solves; solve's vectors
   double k = 0, enei = -1;
   double2 e1{}, e2{};
   bool ready = false;
+  int fallbacks = 0;     // factorisations redone with partial pivoting
   double* nz = nullptr;  // nvec z column (device view)
 
   BemRetLayer(const GreenRet* g_, GreenRetLayer* gl_) : g(g_), gl(gl_), n(g_->n) {
@@ -1081,21 +1090,24 @@ struct BemRetLayer {
     H1 = dev_alloc<double2>(nn);
     G2c = dev_alloc<double2>(5 * nn);
     H2c = dev_alloc<double2>(5 * nn);
-    G1i = dev_alloc<double2>(nn);
-    G2pi = dev_alloc<double2>(nn);
+    F2 = dev_alloc<double2>(nn);
     S1 = dev_alloc<double2>(nn);
     Gam = dev_alloc<double2>(nn);
     T = dev_alloc<double2>(2 * nn);
     M = dev_alloc<double2>(4 * nn);
+    Mb = dev_alloc<double2>(4 * nn);
     piv = dev_alloc<int32_t>(2 * n);
-    sigM = dev_alloc<int32_t>(2 * n);
-    sigA = dev_alloc<int32_t>(n);
-    inv = dev_alloc<double2>(2 * inv_blocks(n));
-    invM = dev_alloc<double2>(2 * inv_blocks(2 * n));
+    for (Fac* f : {&f1, &f2, &fg}) {
+      f->sig = dev_alloc<int32_t>(n);
+      f->inv = dev_alloc<double2>(2 * inv_blocks(n));
+    }
+    fm.sig = dev_alloc<int32_t>(2 * n);
+    fm.inv = dev_alloc<double2>(2 * inv_blocks(2 * n));
     nz = g->geo.nvec + 2 * n;
   }
   ~BemRetLayer() {
-    void* ps[] = {G1, H1, G2c, H2c, G1i, G2pi, S1, Gam, T, M, piv, sigM, sigA, inv, invM};
+    void* ps[] = {G1, H1, G2c, H2c, F2, S1, Gam, T, M, Mb, piv, f1.sig, f1.inv, f2.sig, f2.inv, fg.sig, fg.inv,
+                  fm.sig, fm.inv};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -1103,13 +1115,23 @@ struct BemRetLayer {
   double2* H2(int c) const { return H2c + c * n * n; }
   MatView<double> v(double2* p, int64_t ld = -1) const { return MatView<double>{p, ld < 0 ? n : ld}; }
 
-  // X = A^{-1} (A destroyed)
-  void invert(double2* A, double2* X, const char* what) {
-    const int info = getrf<double>(n, n, v(A), piv, inv, inv + inv_blocks(n), PIVOT_PARTIAL, nullptr, ws);
-    MNP_CHECK(info == 0, MNP_ERR_SINGULAR, std::string("bemretlayer: singular ") + what);
-    perm_from_pivots(n, piv, sigA, ws);
-    LU<double> f{v(A), inv, inv + inv_blocks(n), sigA};
-    getri<double>(n, f, v(X), ws);
+  // LU of the m x m matrix A (ld m) in place, backup: m x m scratch.
+  void factor(int64_t m, double2* A, double2* backup, Fac& F, const char* what) {
+    MatView<double> a{A, m};
+    double2 *Li = F.inv, *Ui = F.inv + inv_blocks(m);
+    copy_mat<double>(m, m, a, MatView<double>{backup, m});
+    int swap = 0;
+    int info = getrf<double>(m, m, a, piv, Li, Ui, PIVOT_NONE, &swap, ws);
+    const int32_t* sig = nullptr;
+    if (info != 0 || swap) {
+      ++fallbacks;
+      copy_mat<double>(m, m, MatView<double>{backup, m}, a);
+      info = getrf<double>(m, m, a, piv, Li, Ui, PIVOT_PARTIAL, nullptr, ws);
+      MNP_CHECK(info == 0, MNP_ERR_SINGULAR, std::string("bemretlayer: singular ") + what);
+      perm_from_pivots(m, piv, F.sig, ws);
+      sig = F.sig;
+    }
+    F.f = LU<double>{a, Li, Ui, sig};
   }
 
   void initmat(double enei_, double2 eps_in, double2 eps_up, double2 eps_down) {
@@ -1124,23 +1146,25 @@ struct BemRetLayer {
     // reflected parts into G2c, H2c, the direct outside G, H into T
     gl->eval(enei, eps_up, eps_down, G2c, H2c);
     g->assemble(wavenumber(k0, eps_up), -2 * M_PI, OutPair{T, nullptr}, OutPair{T + nn, nullptr}, n);
-    const double2 one = mk<double>(1, 0);
+    const double2 one = mk<double>(1, 0), zero = mk<double>(0, 0);
     for (int c : {0, 1, 4}) {  // p, ss, hh carry the direct part (compgreenretlayer/private/eval1.m:37-45)
       hipLaunchKernelGGL(k_zaxpby, dim3(nblk(nn)), dim3(256), 0, stream(), nn, one, G2(c), one, T, G2(c));
       hipLaunchKernelGGL(k_zaxpby, dim3(nblk(nn)), dim3(256), 0, stream(), nn, one, H2(c), one, T + nn, H2(c));
     }
     MNP_HIP(hipGetLastError());
-    // G1^{-1}, G2p^{-1}
-    copy_mat<double>(n, n, v(G1), v(T));
-    invert(T, G1i, "G1");
-    copy_mat<double>(n, n, v(G2(0)), v(T));
-    invert(T, G2pi, "G2.p");
-    // Sigma1 = H1 G1^{-1}; Gamma = (Sigma1 - H2p G2p^{-1})^{-1}
-    const double2 zero = mk<double>(0, 0), mone = mk<double>(-1, 0);
-    gemm<double>(n, n, n, one, v(H1), v(G1i), zero, v(S1));
-    copy_mat<double>(n, n, v(S1), v(T));
-    gemm<double>(n, n, n, mone, v(H2(0)), v(G2pi), one, v(T));
-    invert(T, Gam, "Sigma1 - Sigma2p");
+    // G1 and G2.p factorised; Sigma1 = H1 G1^{-1}, Sigma1 - Sigma2p = Sigma1 - H2p G2p^{-1}
+    factor(n, G1, Mb, f1, "G1");
+    copy_mat<double>(n, n, v(G2(0)), v(F2));
+    factor(n, F2, Mb, f2, "G2.p");
+    copy_mat<double>(n, n, v(H1), v(S1));
+    getrs_right<double>(n, n, f1.f, v(S1), ws);
+    copy_mat<double>(n, n, v(H2(0)), v(T));
+    getrs_right<double>(n, n, f2.f, v(T), ws);
+    hipLaunchKernelGGL(k_zaxpby, dim3(nblk(nn)), dim3(256), 0, stream(), nn, one, S1, mk<double>(-1, 0), T, T);
+    MNP_HIP(hipGetLastError());
+    // Gamma = (Sigma1 - Sigma2p)^{-1}
+    factor(n, T, Mb, fg, "Sigma1 - Sigma2p");
+    getri<double>(n, fg.f, v(Gam), ws);
     // W = eps1 Sigma1 + k^2 (eps1 - eps2)^2 Gamma .* (npar npar')
     const double2 de = csub<double>(e1, e2);
     const double2 de2k = cscale<double>(cmul<double>(de, de), k * k);
@@ -1166,20 +1190,17 @@ struct BemRetLayer {
                          nz, G2(b.zc));
       MNP_HIP(hipGetLastError());
     }
-    const int info = getrf<double>(ld, ld, v(M, ld), piv, invM, invM + inv_blocks(ld), PIVOT_PARTIAL, nullptr, ws);
-    MNP_CHECK(info == 0, MNP_ERR_SINGULAR, "bemretlayer: singular response matrix");
-    perm_from_pivots(ld, piv, sigM, ws);
-    fM = LU<double>{v(M, ld), invM, invM + inv_blocks(ld), sigM};
+    factor(ld, M, Mb, fm, "response matrix");
     ready = true;
   }
 
   void solve(int64_t p, const double2* const in[8], double2* sig1, double2* sig2, double2* h1, double2* h2) {
     MNP_CHECK(ready, MNP_ERR_STATE, "bemretlayer: solve before initmat");
     const int64_t np1 = n * p, np3 = 3 * n * p;
-    double2* buf = (double2*)wsv.get(0, sizeof(double2) * (size_t)(4 * np1 + 6 * np3 + 2 * np1));
+    double2* buf = (double2*)wsv.get(0, sizeof(double2) * (size_t)(4 * np1 + 5 * np3 + 2 * np1));
     double2 *phi = buf, *De = phi + np1, *T2 = De + np1, *U = T2 + np1;
-    double2 *a = U + np1, *alpha = a + np3, *T1 = alpha + np3, *T3 = T1 + np3, *V = T3 + np3, *Y = V + np3;
-    double2* xi = Y + np3;
+    double2 *a = U + np1, *alpha = a + np3, *T1 = alpha + np3, *T3 = T1 + np3, *V = T3 + np3;
+    double2* xi = V + np3;
     const double2 one = mk<double>(1, 0), zero = mk<double>(0, 0);
     const unsigned b1 = nblk(np1);
     hipLaunchKernelGGL(k_lay_exc, dim3(b1), dim3(256), 0, stream(), n, p, in[0], in[1], in[2], in[3], in[4], in[5],
@@ -1195,26 +1216,26 @@ struct BemRetLayer {
     hipLaunchKernelGGL(k_lay_rhs, dim3(b1), dim3(256), 0, stream(), n, p, De, T2, a, T3, alpha, g->geo.nvec, e1,
                        cmul<double>(mk<double>(0, k), e1), cmul<double>(mk<double>(0, k), de), xi);
     MNP_HIP(hipGetLastError());
-    getrs<double>(2 * n, p, fM, v(xi, 2 * n), ws);
+    getrs<double>(2 * n, p, fm.f, v(xi, 2 * n), ws);
     // sig2, h2z = xi; U = G2ss sig2 + G2sh h2z
     gemm<double>(n, p, n, one, v(G2(1)), v(xi, 2 * n), zero, v(U));
     gemm<double>(n, p, n, one, v(G2(3)), v(xi + n, 2 * n), one, v(U));
     // h2par = G2p^{-1} Gamma (alpha_par + ik (eps1 - eps2) npar U)
     hipLaunchKernelGGL(k_lay_v, dim3(b1), dim3(256), 0, stream(), n, p, alpha, U, g->geo.nvec,
                        cmul<double>(mk<double>(0, k), de), V);
-    gemm<double>(n, 3 * p, n, one, v(Gam), v(V), zero, v(T1));
-    gemm<double>(n, 3 * p, n, one, v(G2pi), v(T1), zero, v(h2));  // z-components zero
+    gemm<double>(n, 3 * p, n, one, v(Gam), v(V), zero, v(h2));
+    getrs<double>(n, 3 * p, f2.f, v(h2), ws);  // z-components stay zero
     // h1 = G1^{-1} ([G2p h2par; G2hs sig2 + G2hh h2z] + a)
-    gemm<double>(n, 3 * p, n, one, v(G2(0)), v(h2), zero, v(Y));
+    gemm<double>(n, 3 * p, n, one, v(G2(0)), v(h2), zero, v(h1));
+    gemm<double>(n, p, n, one, v(G2(2)), v(xi, 2 * n), zero, v(h1 + 2 * n, 3 * n));
+    gemm<double>(n, p, n, one, v(G2(4)), v(xi + n, 2 * n), one, v(h1 + 2 * n, 3 * n));
+    hipLaunchKernelGGL(k_zaxpby, dim3(nblk(np3)), dim3(256), 0, stream(), np3, one, h1, one, a, h1);
+    getrs<double>(n, 3 * p, f1.f, v(h1), ws);
     hipLaunchKernelGGL(k_lay_setz, dim3(b1), dim3(256), 0, stream(), n, p, xi + n, 2 * n, h2);
     hipLaunchKernelGGL(k_lay_copy_rows, dim3(b1), dim3(256), 0, stream(), n, p, xi, 2 * n, sig2);
     // sig1 = G1^{-1} (U + phi)
-    hipLaunchKernelGGL(k_zaxpby, dim3(b1), dim3(256), 0, stream(), np1, one, U, one, phi, U);
-    gemm<double>(n, p, n, one, v(G1i), v(U), zero, v(sig1));
-    gemm<double>(n, p, n, one, v(G2(2)), v(xi, 2 * n), zero, v(Y + 2 * n, 3 * n));
-    gemm<double>(n, p, n, one, v(G2(4)), v(xi + n, 2 * n), one, v(Y + 2 * n, 3 * n));
-    hipLaunchKernelGGL(k_zaxpby, dim3(nblk(np3)), dim3(256), 0, stream(), np3, one, Y, one, a, Y);
-    gemm<double>(n, 3 * p, n, one, v(G1i), v(Y), zero, v(h1));
+    hipLaunchKernelGGL(k_zaxpby, dim3(b1), dim3(256), 0, stream(), np1, one, U, one, phi, sig1);
+    getrs<double>(n, p, f1.f, v(sig1), ws);
     MNP_HIP(hipGetLastError());
   }
 };
@@ -1335,6 +1356,12 @@ int mnp_bemretlayer_solve(mnp_bemretlayer_t b, int64_t npol, const void* phi1, c
     const double2* in[8] = {(const double2*)phi1, (const double2*)phi1p, (const double2*)a1, (const double2*)a1p,
                             (const double2*)phi2, (const double2*)phi2p, (const double2*)a2, (const double2*)a2p};
     b->b->solve(npol, in, (double2*)sig1, (double2*)sig2, (double2*)h1, (double2*)h2);
+  });
+}
+int mnp_bemretlayer_stats(mnp_bemretlayer_t b, int* fallbacks) {
+  return guarded([&] {
+    MNP_CHECK(b && fallbacks, MNP_ERR_ARG, "bemretlayer_stats: NULL argument");
+    *fallbacks = b->b->fallbacks;
   });
 }
 int mnp_planewaveretlayer_potential(mnp_greenret_t g, const mnp_layer_options* op, double enei, mnp_complex eps_up,

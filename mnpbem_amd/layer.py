@@ -606,6 +606,12 @@ class bemretlayer:
 
     mldivide = solve
 
+    def stats(self):
+        """LU factorisations so far redone with partial pivoting."""
+        fb = C.c_int()
+        _lib.call("mnp_bemretlayer_stats", self.handle, C.byref(fb))
+        return fb.value
+
 
 def _kz(k, kpar, offset=False):
     kz = np.sqrt(k * k - kpar * kpar + 0j)
