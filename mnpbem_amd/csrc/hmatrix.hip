@@ -691,7 +691,8 @@ HMat::~HMat() {
   void* ps[] = {d_items, d_itpoff, d_multi, d_tdbeg, d_taoff, d_tboff, d_tlda, d_tK,
                 perm, fblk, lblk, d_voff, d_lrow, d_loffs, d_lentry, d_tilebeg, d_tileleaf, d_rowtile,
                 cpos, cnvec, carea, rank,
-                d_loff, d_toff, fvals, pool, d_aca_items, d_aca_order, d_aca_soff, d_dsel, d_dlb, d_droff};
+                d_loff, d_toff, fvals, pool, d_aca_items, d_aca_order, d_aca_soff, d_dsel, d_dlb, d_droff,
+                d_citem, d_ccoff};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -1042,56 +1043,81 @@ __global__ void __launch_bounds__(256) k_coarse_full(const int32_t* fblk, const 
     }
 }
 
-// Low-rank blocks L R^T: one workgroup per block.  Lagg (ns x r) sums L's
-// rows per row tile (each tile's rows are one contiguous segment of L),
-// Ragg (nt x r) sums R's rows per column tile; then C(s, t) = Lagg_s . Ragg_t.
-// agg[aoff[b] ..] is this block's scratch, (ns + nt) r entries.
+// Low-rank blocks L R^T in two flattened passes, so that large blocks spread
+// over many workgroups: (1) one wave per (block, tile) item sums the tile's
+// rows of L (or of R) into Lagg_s (or Ragg_t), r entries, lane l < r taking
+// factor l; (2) one thread per coarse entry (s, t) of every block forms
+// C(s, t) = Lagg_s . Ragg_t.  item[2 i ..] = {block, tile index within the
+// block's rows (>= 0) or -1 - index within its columns}; entries of block b are [coff[b], coff[b + 1]) in column-major order of
+// its ns x nt coarse block, with its aggregates at aggoff[b] (ns r of L
+// then nt r of R).
 template <class TS>
-__global__ void __launch_bounds__(256) k_coarse_low(const int32_t* lblk, const int32_t* rank, const int64_t* loff,
+__global__ void __launch_bounds__(256) k_coarse_agg(int64_t nitems, const int32_t* item, const int64_t* aggoff,
+                                                    const int32_t* lblk, const int32_t* rank, const int64_t* loff,
                                                     const int32_t* rowtile, const int32_t* tilebeg,
-                                                    const cplx<TS>* pool, const int64_t* aoff, double2* agg,
-                                                    int64_t nc, double2* C) {
-  const int64_t b = blockIdx.x;
-  const int r0 = lblk[4 * b], r1 = lblk[4 * b + 1], c0 = lblk[4 * b + 2], c1 = lblk[4 * b + 3];
-  const int m = r1 - r0, r = rank[b];
-  const int s0 = rowtile[r0], ns = rowtile[r1 - 1] - s0 + 1, t0 = rowtile[c0], nt = rowtile[c1 - 1] - t0 + 1;
-  if (r == 0) {
-    for (int e = threadIdx.x; e < ns * nt; e += 256) C[(int64_t)(t0 + e / ns) * nc + s0 + e % ns] = mk<double>(0, 0);
-    return;
-  }
+                                                    const cplx<TS>* pool, double2* agg) {
+  const int64_t it = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (it >= nitems) return;
+  const int lane = threadIdx.x & 63;
+  const int b = item[2 * it], sel = item[2 * it + 1];
+  const int r0 = lblk[4 * b], m = lblk[4 * b + 1] - r0, c0 = lblk[4 * b + 2];
+  const int ns = rowtile[lblk[4 * b + 1] - 1] - rowtile[r0] + 1;
+  const int r = rank[b];
   const cplx<TS>* L = pool + loff[b];
-  const cplx<TS>* RT = L + (int64_t)m * r;
-  double2* La = agg + aoff[b];
-  double2* Ra = La + (int64_t)ns * r;
-  for (int e = threadIdx.x; e < ns * r; e += 256) {
-    const int si = e / r, l = e % r;
-    const int a = tilebeg[s0 + si] - r0, mt = tilebeg[s0 + si + 1] - tilebeg[s0 + si];
+  double2* out = agg + aggoff[b] + (int64_t)(sel >= 0 ? sel : ns + (-1 - sel)) * r;
+  if (sel >= 0) {
+    const int s = rowtile[r0] + sel;
+    const int a = tilebeg[s] - r0, mt = tilebeg[s + 1] - tilebeg[s];
     const cplx<TS>* seg = L + (int64_t)a * r;  // the tile's segment, mt x r column-major
-    double2 acc = mk<double>(0, 0);
-    for (int i = 0; i < mt; ++i) {
-      const cplx<TS> v = seg[(int64_t)l * mt + i];
-      acc.x += v.x;
-      acc.y += v.y;
+    for (int l = lane; l < r; l += 64) {
+      double2 acc = mk<double>(0, 0);
+      for (int i = 0; i < mt; ++i) {
+        const cplx<TS> v = seg[(int64_t)l * mt + i];
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      out[l] = acc;
     }
-    La[(int64_t)si * r + l] = acc;
-  }
-  for (int e = threadIdx.x; e < nt * r; e += 256) {
-    const int ti = e / r, l = e % r;
-    double2 acc = mk<double>(0, 0);
-    for (int j = tilebeg[t0 + ti]; j < tilebeg[t0 + ti + 1]; ++j) {
-      const cplx<TS> v = RT[(int64_t)(j - c0) * r + l];
-      acc.x += v.x;
-      acc.y += v.y;
+  } else {
+    const int t = rowtile[c0] + (-1 - sel);
+    const cplx<TS>* RT = L + (int64_t)m * r;
+    for (int l = lane; l < r; l += 64) {
+      double2 acc = mk<double>(0, 0);
+      for (int j = tilebeg[t]; j < tilebeg[t + 1]; ++j) {
+        const cplx<TS> v = RT[(int64_t)(j - c0) * r + l];
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+      out[l] = acc;
     }
-    Ra[(int64_t)ti * r + l] = acc;
   }
-  __syncthreads();
-  for (int e = threadIdx.x; e < ns * nt; e += 256) {
-    const int si = e % ns, ti = e / ns;
-    double2 acc = mk<double>(0, 0);
-    for (int l = 0; l < r; ++l) acc = cadd<double>(acc, cmul<double>(La[(int64_t)si * r + l], Ra[(int64_t)ti * r + l]));
-    C[(int64_t)(t0 + ti) * nc + s0 + si] = acc;
+}
+
+__global__ void __launch_bounds__(256) k_coarse_dot(int64_t nlow, int64_t ntot, const int64_t* coff,
+                                                    const int64_t* aggoff, const int32_t* lblk, const int32_t* rank,
+                                                    const int32_t* rowtile, const double2* agg, int64_t nc,
+                                                    double2* C) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= ntot) return;
+  int64_t lo = 0, hi = nlow;  // block b with coff[b] <= e < coff[b + 1]
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) / 2;
+    if (coff[mid] <= e)
+      lo = mid;
+    else
+      hi = mid;
   }
+  const int64_t b = lo;
+  const int r0 = lblk[4 * b], r1 = lblk[4 * b + 1], c0 = lblk[4 * b + 2];
+  const int s0 = rowtile[r0], ns = rowtile[r1 - 1] - s0 + 1, t0 = rowtile[c0];
+  const int64_t q = e - coff[b];
+  const int si = (int)(q % ns), ti = (int)(q / ns);
+  const int r = rank[b];
+  const double2* La = agg + aggoff[b] + (int64_t)si * r;
+  const double2* Ra = agg + aggoff[b] + (int64_t)ns * r + (int64_t)ti * r;
+  double2 acc = mk<double>(0, 0);
+  for (int l = 0; l < r; ++l) acc = cadd<double>(acc, cmul<double>(La[l], Ra[l]));
+  C[(int64_t)(t0 + ti) * nc + s0 + si] = acc;
 }
 
 void HMat::coarsen(double2* C, Workspace& ws) const {
@@ -1109,18 +1135,40 @@ void HMat::coarsen_impl(double2* C, Workspace& ws) const {
     hipLaunchKernelGGL(k_coarse_full<TS>, dim3((unsigned)nfull), dim3(256), 0, s, fblk, d_voff, d_rowtile, d_tilebeg,
                        (const cplx<TS>*)fvals, ntile, C);
   if (nlow) {
-    std::vector<int64_t> aoff(nlow + 1, 0);
+    if (!d_citem) {  // items and entry offsets depend on the block structure only
+      std::vector<int64_t> coff(nlow + 1, 0);
+      std::vector<int32_t> item;
+      for (int64_t b = 0; b < nlow; ++b) {
+        const int64_t ns = rowtile_h[lblk_h[4 * b + 1] - 1] - rowtile_h[lblk_h[4 * b]] + 1;
+        const int64_t nt = rowtile_h[lblk_h[4 * b + 3] - 1] - rowtile_h[lblk_h[4 * b + 2]] + 1;
+        coff[b + 1] = coff[b] + ns * nt;
+        for (int64_t i = 0; i < ns + nt; ++i) {
+          item.push_back((int32_t)b);
+          item.push_back(i < ns ? (int32_t)i : (int32_t)(-1 - (i - ns)));
+        }
+      }
+      ncitem = (int64_t)item.size() / 2;
+      ncentry = coff[nlow];
+      d_citem = dev_alloc<int32_t>(2 * ncitem);
+      d_ccoff = dev_alloc<int64_t>(nlow + 1);
+      MNP_HIP(hipMemcpy(d_citem, item.data(), sizeof(int32_t) * 2 * ncitem, hipMemcpyHostToDevice));
+      MNP_HIP(hipMemcpy(d_ccoff, coff.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice));
+    }
+    // aggregate offsets from this fill's ranks
+    std::vector<int64_t> aggoff(nlow + 1, 0);
     for (int64_t b = 0; b < nlow; ++b) {
       const int64_t ns = rowtile_h[lblk_h[4 * b + 1] - 1] - rowtile_h[lblk_h[4 * b]] + 1;
       const int64_t nt = rowtile_h[lblk_h[4 * b + 3] - 1] - rowtile_h[lblk_h[4 * b + 2]] + 1;
-      aoff[b + 1] = aoff[b] + (ns + nt) * rank_host[b];
+      aggoff[b + 1] = aggoff[b] + (ns + nt) * rank_host[b];
     }
-    int64_t* d_aoff = (int64_t*)ws.get(0, sizeof(int64_t) * (nlow + 1));
-    double2* agg = (double2*)ws.get(1, sizeof(double2) * (size_t)std::max<int64_t>(aoff[nlow], 1));
-    MNP_HIP(hipMemcpyAsync(d_aoff, aoff.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_coarse_low<TS>, dim3((unsigned)nlow), dim3(256), 0, s, lblk, rank, d_loff, d_rowtile,
-                       d_tilebeg, (const cplx<TS>*)pool, d_aoff, agg, ntile, C);
-    MNP_HIP(hipStreamSynchronize(s));  // aoff is host memory of this call
+    int64_t* d_aggoff = (int64_t*)ws.get(0, sizeof(int64_t) * (nlow + 1));
+    double2* agg = (double2*)ws.get(1, sizeof(double2) * (size_t)std::max<int64_t>(aggoff[nlow], 1));
+    MNP_HIP(hipMemcpyAsync(d_aggoff, aggoff.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_coarse_agg<TS>, dim3((unsigned)cdiv_up(ncitem, 4)), dim3(256), 0, s, ncitem, d_citem,
+                       d_aggoff, lblk, rank, d_loff, d_rowtile, d_tilebeg, (const cplx<TS>*)pool, agg);
+    hipLaunchKernelGGL(k_coarse_dot, dim3((unsigned)cdiv_up(ncentry, 256)), dim3(256), 0, s, nlow, ncentry, d_ccoff,
+                       d_aggoff, lblk, rank, d_rowtile, agg, ntile, C);
+    MNP_HIP(hipStreamSynchronize(s));  // aggoff is host memory of this call
   }
   MNP_HIP(hipGetLastError());
 }

@@ -35,6 +35,11 @@ struct HMat {
   int64_t *d_itpoff = nullptr, *d_taoff = nullptr, *d_tboff = nullptr;
   int32_t *d_dsel = nullptr, *d_dlb = nullptr, *d_droff = nullptr;  // low-rank descriptors (k_plan_low)
   int64_t nlowdesc = 0;
+  // coarsening of low-rank blocks (coarsen): (block, tile) items and the
+  // blocks' first coarse entries, built at the first call
+  mutable int32_t* d_citem = nullptr;
+  mutable int64_t* d_ccoff = nullptr;
+  mutable int64_t ncitem = 0, ncentry = 0;
   std::vector<int32_t> items_h;
   std::vector<int64_t> itpoff_h;
   void* fvals = nullptr;  // full blocks, column-major per block, complex (prec)
