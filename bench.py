@@ -330,25 +330,36 @@ def run_substrate_ret(level=4, steps=4):
     dirs = np.column_stack([np.sin(theta), 0 * theta, -np.cos(theta)])
     pols = np.column_stack([np.cos(theta), 0 * theta, np.sin(theta)])
     exc = B.planewave(pols, dirs, op)
-    t = time.perf_counter()
-    bem = B.bemsolver(p, op)
-    torch.cuda.synchronize()
-    t_init = time.perf_counter() - t
     lams = np.linspace(450.0, 750.0, 20)
-    exc.cross_sections(bem.solve(exc(p, lams[0])))  # warm-up
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    cs = [exc.cross_sections(bem.solve(exc(p, lam))) for lam in lams[1:1 + steps]]
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t
-    log(f"[bench] substrate (retarded): {steps / dt:.3f} wavelengths/s, tables {t_tab:.2f} s, "
-        f"refinement weights {t_init:.2f} s; ext {np.round(cs[0][0], 2)}")
-    return {"workload": f"retarded bemretlayer FP64: ellipsoid 10x10x40 nm ({len(f)} faces) 5 nm above eps 2.25, "
+    res = {}
+    for prec in ("mixed", "fp64"):
+        t = time.perf_counter()
+        bem = B.bemsolver(p, op, precision=prec)
+        torch.cuda.synchronize()
+        t_init = time.perf_counter() - t
+        exc.cross_sections(bem.solve(exc(p, lams[0])))  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        cs, sweeps = [], []
+        for lam in lams[1:1 + steps]:
+            cs.append(exc.cross_sections(bem.solve(exc(p, lam))))
+            sweeps.append(bem.stats()["iterations"])
+        torch.cuda.synchronize()
+        res[prec] = (steps / (time.perf_counter() - t), cs, sweeps, bem.stats()["fallbacks"], t_init)
+        del bem
+    err = max(float(np.max(np.abs(a[j] - b[j]) / np.max(np.abs(b[j]))))
+              for a, b in zip(res["mixed"][1], res["fp64"][1]) for j in (0, 1))
+    log(f"[bench] substrate (retarded): mixed {res['mixed'][0]:.3f}/s (sweeps {res['mixed'][2]}), "
+        f"fp64 {res['fp64'][0]:.3f}/s, rel err {err:.1e}; tables {t_tab:.2f} s")
+    return {"workload": f"retarded bemretlayer: ellipsoid 10x10x40 nm ({len(f)} faces) 5 nm above eps 2.25, "
                         "air above, 5 TM plane waves from above at 0-80 deg, 30x30 table set at 5 wavelengths",
-            "wavelengths_per_s_fp64": steps / dt, "s_per_wavelength": dt / steps, "table_set_s": t_tab,
-            "refinement_setup_s": t_init, "pivoting_fallbacks": bem.stats(),
-            "ext_first": [float(x) for x in cs[0][0]],
-            "sca_first": [float(x) for x in cs[0][1]]}
+            "wavelengths_per_s_mixed": res["mixed"][0], "wavelengths_per_s_fp64": res["fp64"][0],
+            "speedup_mixed_vs_fp64": res["mixed"][0] / res["fp64"][0],
+            "max_rel_err_mixed_vs_fp64": err, "refinement_sweeps": res["mixed"][2],
+            "table_set_s": t_tab, "refinement_setup_s": res["fp64"][4],
+            "pivoting_fallbacks": {"mixed": res["mixed"][3], "fp64": res["fp64"][3]},
+            "ext_first": [float(x) for x in res["fp64"][1][0][0]],
+            "sca_first": [float(x) for x in res["fp64"][1][0][1]]}
 
 
 def run_sensitivity(nsweep=32):

@@ -616,12 +616,15 @@ int mnp_greenretlayer_refined(mnp_greenretlayer_t gl, int64_t* npairs, int64_t* 
 int mnp_greenretlayer_eval(mnp_greenretlayer_t gl, double enei, mnp_complex eps_up,
                            mnp_complex eps_down, void* G, void* F);
 
-/* Dense retarded solver above the substrate, complex FP64 (BEM/@bemretlayer:
+/* Dense retarded solver above the substrate (BEM/@bemretlayer:
  * private/initmat.m:17-95 with unique eps inside and outside, mldivide.m);
- * the outside medium is the upper one.  Arguments of solve as
+ * the outside medium is the upper one.  solver: MNP_SOLVER_FP64 (the
+ * complex FP64 chain) or MNP_SOLVER_MIXED (the chain in complex FP32,
+ * refined with the FP64 operator the FP64 chain inverts until the relative
+ * residual is below 1e-12, at most 10 sweeps).  Arguments of solve as
  * mnp_bemret_solve. */
 typedef struct mnp_bemretlayer_s* mnp_bemretlayer_t;
-int mnp_bemretlayer_create(mnp_greenret_t g, mnp_greenretlayer_t gl, mnp_bemretlayer_t* out);
+int mnp_bemretlayer_create(mnp_greenret_t g, mnp_greenretlayer_t gl, int solver, mnp_bemretlayer_t* out);
 int mnp_bemretlayer_destroy(mnp_bemretlayer_t bem);
 int mnp_bemretlayer_initmat(mnp_bemretlayer_t bem, double enei, mnp_complex eps_in, mnp_complex eps_up,
                             mnp_complex eps_down);
@@ -629,8 +632,10 @@ int mnp_bemretlayer_solve(mnp_bemretlayer_t bem, int64_t npol, const void* phi1,
                           const void* a1, const void* a1p, const void* phi2, const void* phi2p,
                           const void* a2, const void* a2p, void* sig1, void* sig2, void* h1, void* h2);
 /* LU factorisations so far that partial pivoting had to redo (each is tried
- * without row exchanges first; the factors are partial pivoting's either way). */
-int mnp_bemretlayer_stats(mnp_bemretlayer_t bem, int* fallbacks);
+ * without row exchanges first; the factors are partial pivoting's either
+ * way); refinement sweeps and final relative residual of the last solve
+ * (MIXED; 0 and 0 for FP64). */
+int mnp_bemretlayer_stats(mnp_bemretlayer_t bem, int* fallbacks, int* iterations, double* relres);
 /* Outside potentials of plane waves above the substrate at the faces
  * (Simulation/retarded/@planewaveretlayer/potential.m:20-105 with
  * Particles/@layerstructure/fresnel.m; the normal derivative of the phase

@@ -113,11 +113,11 @@ PROTOTYPES = {
     "mnp_greenretlayer_destroy": (INT, [P]),
     "mnp_greenretlayer_refined": (INT, [P, PI64, PI64]),
     "mnp_greenretlayer_eval": (INT, [P, D, MnpComplex, MnpComplex, P, P]),
-    "mnp_bemretlayer_create": (INT, [P, P, C.POINTER(P)]),
+    "mnp_bemretlayer_create": (INT, [P, P, INT, C.POINTER(P)]),
     "mnp_bemretlayer_destroy": (INT, [P]),
     "mnp_bemretlayer_initmat": (INT, [P, D, MnpComplex, MnpComplex, MnpComplex]),
     "mnp_bemretlayer_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
-    "mnp_bemretlayer_stats": (INT, [P, C.POINTER(INT)]),
+    "mnp_bemretlayer_stats": (INT, [P, C.POINTER(INT), C.POINTER(INT), C.POINTER(D)]),
     "mnp_planewaveretlayer_potential": (INT, [P, C.POINTER(LayerOptions), D, MnpComplex, MnpComplex, I64, P, P,
                                               P, P, P, P]),
     "mnp_farfield_layer": (INT, [P, C.POINTER(LayerOptions), D, MnpComplex, MnpComplex, I64, P, I64, P, P, P,

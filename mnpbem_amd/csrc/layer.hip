@@ -30,6 +30,7 @@
 #include <cmath>
 #include <memory>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "mnp_chain.hpp"
@@ -759,30 +760,39 @@ __global__ void __launch_bounds__(256) k_farfield_layer(int64_t n, int64_t p, in
 }
 
 // ------------------------------------------------------ chain helpers
+// (templated on the chain's precision T; scalars arrive as double2)
+template <class T>
+__host__ __device__ inline cplx<T> tc(double2 z) {
+  return mk<T>((T)z.x, (T)z.y);
+}
 // out = alpha x + beta y
-__global__ void k_zaxpby(int64_t len, double2 al, const double2* x, double2 be, const double2* y, double2* out) {
+template <class T>
+__global__ void k_zaxpby(int64_t len, double2 al_, const cplx<T>* x, double2 be_, const cplx<T>* y, cplx<T>* out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < len) out[i] = cadd<double>(cmul<double>(al, x[i]), cmul<double>(be, y[i]));
+  const cplx<T> al = tc<T>(al_), be = tc<T>(be_);
+  if (i < len) out[i] = cadd<T>(cmul<T>(al, x[i]), cmul<T>(be, y[i]));
 }
 // W = e1 S1 + c (Gamma .* npar npar')
-__global__ void k_lay_w(int64_t n, const double* nvec, double2 e1, const double2* S1, double2 c, const double2* Gam,
-                        double2* W) {
+template <class T>
+__global__ void k_lay_w(int64_t n, const double* nvec, double2 e1_, const cplx<T>* S1, double2 c_, const cplx<T>* Gam,
+                        cplx<T>* W) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= n * n) return;
   const int64_t i = idx % n, j = idx / n;
-  const double nn = nvec[i] * nvec[j] + nvec[n + i] * nvec[n + j];
-  W[idx] = cadd<double>(cmul<double>(e1, S1[idx]), cscale<double>(cmul<double>(c, Gam[idx]), nn));
+  const T nn = (T)(nvec[i] * nvec[j] + nvec[n + i] * nvec[n + j]);
+  W[idx] = cadd<T>(cmul<T>(tc<T>(e1_), S1[idx]), cscale<T>(cmul<T>(tc<T>(c_), Gam[idx]), nn));
 }
 // block of M (ld 2n): Mb(i, j) += -al H(i, j) - be nz_i Gc(i, j)
-__global__ void k_lay_mblock(int64_t n, int64_t ld, double2* Mb, double2 al, const double2* H, double2 be,
-                             const double* nz, const double2* Gc) {
+template <class T>
+__global__ void k_lay_mblock(int64_t n, int64_t ld, cplx<T>* Mb, double2 al_, const cplx<T>* H, double2 be_,
+                             const double* nz, const cplx<T>* Gc) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= n * n) return;
   const int64_t i = idx % n, j = idx / n;
-  double2& m = Mb[j * ld + i];
-  m = csub<double>(csub<double>(m, cmul<double>(al, H[idx])), cscale<double>(cmul<double>(be, Gc[idx]), nz[i]));
+  cplx<T>& m = Mb[j * ld + i];
+  m = csub<T>(csub<T>(m, cmul<T>(tc<T>(al_), H[idx])), cscale<T>(cmul<T>(tc<T>(be_), Gc[idx]), (T)nz[i]));
 }
-// excitation.m:17-29 with scalar eps1 (inside) and eps2 (outside)
+// excitation.m:17-29 with scalar eps1 (inside) and eps2 (outside), FP64
 __global__ void k_lay_exc(int64_t n, int64_t p, const double2* phi1, const double2* phi1p, const double2* a1,
                           const double2* a1p, const double2* phi2, const double2* phi2p, const double2* a2,
                           const double2* a2p, const double* nvec, double k, double2 e1, double2 e2, double2* phi,
@@ -809,59 +819,101 @@ __global__ void k_lay_exc(int64_t n, int64_t p, const double2* phi1, const doubl
                        cmul<double>(ik, csub<double>(cmul<double>(e2, na2), cmul<double>(e1, na1))));
 }
 // alpha <- alpha - T + c n phi  (T = Sigma1 a)
-__global__ void k_lay_alpha(int64_t n, int64_t p, double2* alpha, const double2* T, const double2* phi,
-                            const double* nvec, double2 c) {
+template <class T>
+__global__ void k_lay_alpha(int64_t n, int64_t p, cplx<T>* alpha, const cplx<T>* Tm, const cplx<T>* phi,
+                            const double* nvec, double2 c_) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= n * p) return;
   const int64_t i = t % n, q = t / n;
-  const double2 cp = cmul<double>(c, phi[t]);
+  const cplx<T> cp = cmul<T>(tc<T>(c_), phi[t]);
   for (int cc = 0; cc < 3; ++cc) {
     const int64_t id = (cc + 3 * q) * n + i;
-    alpha[id] = cadd<double>(csub<double>(alpha[id], T[id]), cscale<double>(cp, nvec[cc * n + i]));
+    alpha[id] = cadd<T>(csub<T>(alpha[id], Tm[id]), cscale<T>(cp, (T)nvec[cc * n + i]));
   }
 }
 // rhs = [De - e1 T2 + c1 n.a + c2 npar.T3 ; alpha_z]  (2n x p, ld 2n)
-__global__ void k_lay_rhs(int64_t n, int64_t p, const double2* De, const double2* T2, const double2* a,
-                          const double2* T3, const double2* alpha, const double* nvec, double2 e1, double2 c1,
-                          double2 c2, double2* rhs) {
+template <class T>
+__global__ void k_lay_rhs(int64_t n, int64_t p, const cplx<T>* De, const cplx<T>* T2, const cplx<T>* a,
+                          const cplx<T>* T3, const cplx<T>* alpha, const double* nvec, double2 e1_, double2 c1_,
+                          double2 c2_, cplx<T>* rhs) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= n * p) return;
   const int64_t i = t % n, q = t / n;
-  double2 na = mk<double>(0, 0), nt = mk<double>(0, 0);
+  cplx<T> na = mk<T>(0, 0), nt = mk<T>(0, 0);
   for (int cc = 0; cc < 3; ++cc) {
     const int64_t id = (cc + 3 * q) * n + i;
-    na = cadd<double>(na, cscale<double>(a[id], nvec[cc * n + i]));
-    if (cc < 2) nt = cadd<double>(nt, cscale<double>(T3[id], nvec[cc * n + i]));
+    na = cadd<T>(na, cscale<T>(a[id], (T)nvec[cc * n + i]));
+    if (cc < 2) nt = cadd<T>(nt, cscale<T>(T3[id], (T)nvec[cc * n + i]));
   }
-  rhs[q * 2 * n + i] = cadd<double>(cadd<double>(csub<double>(De[t], cmul<double>(e1, T2[t])), cmul<double>(c1, na)),
-                                    cmul<double>(c2, nt));
+  rhs[q * 2 * n + i] = cadd<T>(cadd<T>(csub<T>(De[t], cmul<T>(tc<T>(e1_), T2[t])), cmul<T>(tc<T>(c1_), na)),
+                               cmul<T>(tc<T>(c2_), nt));
   rhs[q * 2 * n + n + i] = alpha[(2 + 3 * q) * n + i];
 }
 // V = alpha_par + c npar U, V_z = 0
-__global__ void k_lay_v(int64_t n, int64_t p, const double2* alpha, const double2* U, const double* nvec, double2 c,
-                        double2* V) {
+template <class T>
+__global__ void k_lay_v(int64_t n, int64_t p, const cplx<T>* alpha, const cplx<T>* U, const double* nvec, double2 c_,
+                        cplx<T>* V) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= n * p) return;
   const int64_t i = t % n, q = t / n;
-  const double2 cu = cmul<double>(c, U[t]);
+  const cplx<T> cu = cmul<T>(tc<T>(c_), U[t]);
   for (int cc = 0; cc < 2; ++cc) {
     const int64_t id = (cc + 3 * q) * n + i;
-    V[id] = cadd<double>(alpha[id], cscale<double>(cu, nvec[cc * n + i]));
+    V[id] = cadd<T>(alpha[id], cscale<T>(cu, (T)nvec[cc * n + i]));
   }
-  V[(2 + 3 * q) * n + i] = mk<double>(0, 0);
+  V[(2 + 3 * q) * n + i] = mk<T>(0, 0);
 }
 // h (n x 3 x p): z-component from src (column q at src + q ld)
-__global__ void k_lay_setz(int64_t n, int64_t p, const double2* src, int64_t ld, double2* h) {
+template <class T>
+__global__ void k_lay_setz(int64_t n, int64_t p, const cplx<T>* src, int64_t ld, cplx<T>* h) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= n * p) return;
   const int64_t i = t % n, q = t / n;
   h[(2 + 3 * q) * n + i] = src[q * ld + i];
 }
 // sig2 (n x p) from the first n rows of xi (ld 2n)
-__global__ void k_lay_copy_rows(int64_t n, int64_t p, const double2* src, int64_t ld, double2* dst) {
+template <class T>
+__global__ void k_lay_copy_rows(int64_t n, int64_t p, const cplx<T>* src, int64_t ld, cplx<T>* dst) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= n * p) return;
   dst[t] = src[(t / n) * ld + t % n];
+}
+// Residual r = b - A x of the operator the chain inverts, FP64, from the
+// products: Gs1 = G1 s1, Gh1 = G1 h1, Hs1 = H1 s1, Hh1 = H1 h1, A2 = [G2p h2x,
+// G2p h2y, G2hh h2z] with X = G2hs s2 still to add to its z, Y = G2sh s2,
+// phi2 = G2ss s2 + G2sh h2z, dA2 = [H2p h2x, H2p h2y, H2hs s2 + H2hh h2z],
+// dphi2 = H2ss s2 + H2sh h2z.  b and r hold (phi, a, alpha, De) blocks.
+struct LayProd {
+  const double2 *Gs1, *Gh1, *Hs1, *Hh1, *A2, *X, *Y, *phi2, *dA2, *dphi2;
+};
+__global__ void k_lay_resid(int64_t n, int64_t p, LayProd P, const double2* b, double2* r, const double* nvec,
+                            double k, double2 e1, double2 e2) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * p) return;
+  const int64_t i = t % n, q = t / n;
+  const int64_t np1 = n * p, np3 = 3 * n * p;
+  const double2 ik = mk<double>(0, k);
+  const double2 *bphi = b, *ba = b + np1, *balpha = ba + np3, *bDe = balpha + np3;
+  double2 *rphi = r, *ra = r + np1, *ralpha = ra + np3, *rDe = ralpha + np3;
+  const double2 gs1 = P.Gs1[t], ph2 = P.phi2[t];
+  rphi[t] = csub<double>(bphi[t], csub<double>(gs1, ph2));
+  const double2 s = csub<double>(cmul<double>(e1, gs1), cmul<double>(e2, ph2));
+  double2 nin = mk<double>(0, 0);
+  for (int c = 0; c < 3; ++c) {
+    const int64_t id = (c + 3 * q) * n + i;
+    const double nc = nvec[c * n + i];
+    const double2 A = c < 2 ? P.A2[id] : cadd<double>(P.A2[id], P.X[t]);
+    ra[id] = csub<double>(ba[id], csub<double>(P.Gh1[id], A));
+    const double2 al = csub<double>(csub<double>(P.Hh1[id], P.dA2[id]), cscale<double>(cmul<double>(ik, s), nc));
+    ralpha[id] = csub<double>(balpha[id], al);
+    nin = cadd<double>(nin, cscale<double>(csub<double>(cmul<double>(e1, P.Gh1[id]), cmul<double>(e2, A)), nc));
+  }
+  const double2 corr = cscale<double>(cmul<double>(cmul<double>(ik, csub<double>(e1, e2)), csub<double>(P.X[t], P.Y[t])),
+                                      nvec[2 * n + i]);
+  const double2 De = cadd<double>(csub<double>(csub<double>(cmul<double>(e1, P.Hs1[t]), cmul<double>(e2, P.dphi2[t])),
+                                               cmul<double>(ik, nin)),
+                                  corr);
+  rDe[t] = csub<double>(bDe[t], De);
 }
 
 inline unsigned nblk(int64_t len, int bs = 256) { return (unsigned)cdiv_up(len, bs); }
@@ -1057,82 +1109,231 @@ struct GreenRetLayer {
   }
 };
 
-// The bemretlayer chain in complex FP64 (initmat.m:17-95, mldivide.m:31-76)
+// The bemretlayer chain (initmat.m:17-95, mldivide.m:31-76) in precision T
 // for unique eps1 (inside) and eps2 (outside = upper medium): L1 = eps1,
 // L2p = eps2, so Gammapar = i k (eps1 - eps2) Gamma .* (npar npar').  G1 and
 // G2.p stay factorised (their inverses are applied by triangular solves),
 // Gamma is formed explicitly for its elementwise product; each LU first runs
 // without row exchanges and is redone with partial pivoting when that would
 // have exchanged a row, so the factors are partial pivoting's.
-struct BemRetLayer {
+template <class T>
+struct LayerChain {
+  using C = cplx<T>;
   struct Fac {
     int32_t* sig = nullptr;
-    double2* inv = nullptr;
-    LU<double> f{};
+    C* inv = nullptr;
+    LU<T> f{};
   };
+  int64_t n;
+  const double* nvec;
+  C *G1, *F2, *S1, *Gam, *Tm, *M, *Mb;
+  C* G2c;            // the five G2 components in T (the owner's FP64 ones when T is double)
+  bool own_g2 = false;
+  int32_t* piv = nullptr;
+  Fac f1, f2, fg, fm;
+  Workspace ws, wsv;  // factorisations; the vectors of apply
+  double k = 0;
+  double2 e1{}, e2{};
+  int fallbacks = 0;  // factorisations redone with partial pivoting
+
+  LayerChain(int64_t n_, const double* nvec_) : n(n_), nvec(nvec_) {
+    const int64_t nn = n * n;
+    G1 = dev_alloc<C>(nn);
+    F2 = dev_alloc<C>(nn);
+    S1 = dev_alloc<C>(nn);
+    Gam = dev_alloc<C>(nn);
+    Tm = dev_alloc<C>(2 * nn);
+    M = dev_alloc<C>(4 * nn);
+    Mb = dev_alloc<C>(4 * nn);
+    piv = dev_alloc<int32_t>(2 * n);
+    for (Fac* f : {&f1, &f2, &fg}) {
+      f->sig = dev_alloc<int32_t>(n);
+      f->inv = dev_alloc<C>(2 * inv_blocks(n));
+    }
+    fm.sig = dev_alloc<int32_t>(2 * n);
+    fm.inv = dev_alloc<C>(2 * inv_blocks(2 * n));
+    if (!std::is_same<T, double>::value) {
+      G2c = dev_alloc<C>(5 * nn);
+      own_g2 = true;
+    }
+  }
+  ~LayerChain() {
+    void* ps[] = {G1, F2, S1, Gam, Tm, M, Mb, piv, f1.sig, f1.inv, f2.sig, f2.inv, fg.sig, fg.inv, fm.sig, fm.inv,
+                  own_g2 ? (void*)G2c : nullptr};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
+  C* G2(int c) const { return G2c + c * n * n; }
+  MatView<T> v(C* p, int64_t ld = -1) const { return MatView<T>{p, ld < 0 ? n : ld}; }
+  // dst (T) = src (FP64), m x m
+  void load(const double2* src, C* dst, int64_t m) {
+    if constexpr (std::is_same<T, double>::value)
+      copy_mat<double>(m, m, MatView<double>{(double2*)src, m}, MatView<double>{dst, m});
+    else
+      convert_mat<double, float>(m, m, MatView<double>{(double2*)src, m}, MatView<float>{dst, m});
+  }
+
+  // LU of the m x m matrix A (ld m) in place, backup: m x m scratch.
+  void factor(int64_t m, C* A, C* backup, Fac& F, const char* what) {
+    MatView<T> a{A, m};
+    C *Li = F.inv, *Ui = F.inv + inv_blocks(m);
+    copy_mat<T>(m, m, a, MatView<T>{backup, m});
+    int swap = 0;
+    int info = getrf<T>(m, m, a, piv, Li, Ui, PIVOT_NONE, &swap, ws);
+    const int32_t* sig = nullptr;
+    if (info != 0 || swap) {
+      ++fallbacks;
+      copy_mat<T>(m, m, MatView<T>{backup, m}, a);
+      info = getrf<T>(m, m, a, piv, Li, Ui, PIVOT_PARTIAL, nullptr, ws);
+      MNP_CHECK(info == 0, MNP_ERR_SINGULAR, std::string("bemretlayer: singular ") + what);
+      perm_from_pivots(m, piv, F.sig, ws);
+      sig = F.sig;
+    }
+    F.f = LU<T>{a, Li, Ui, sig};
+  }
+
+  // From the FP64 G1, H1 and the structured G2c, H2c (each 5 x n x n).
+  void build(double k0, double2 eps1, double2 eps2, const double2* G1d, const double2* H1d, double2* G2d,
+             const double2* H2d) {
+    const int64_t nn = n * n;
+    k = k0;
+    e1 = eps1;
+    e2 = eps2;
+    if constexpr (std::is_same<T, double>::value)
+      G2c = G2d;
+    else
+      for (int c = 0; c < 5; ++c) load(G2d + c * nn, G2(c), n);
+    const double2 one = mk<double>(1, 0), zero = mk<double>(0, 0);
+    // G1 and G2.p factorised; Sigma1 = H1 G1^{-1}, Sigma1 - Sigma2p = Sigma1 - H2p G2p^{-1}
+    load(G1d, G1, n);
+    factor(n, G1, Mb, f1, "G1");
+    copy_mat<T>(n, n, v(G2(0)), v(F2));
+    factor(n, F2, Mb, f2, "G2.p");
+    load(H1d, S1, n);
+    getrs_right<T>(n, n, f1.f, v(S1), ws);
+    load(H2d, Tm, n);  // H2.p
+    getrs_right<T>(n, n, f2.f, v(Tm), ws);
+    hipLaunchKernelGGL(k_zaxpby<T>, dim3(nblk(nn)), dim3(256), 0, stream(), nn, one, S1, mk<double>(-1, 0), Tm, Tm);
+    MNP_HIP(hipGetLastError());
+    // Gamma = (Sigma1 - Sigma2p)^{-1}
+    factor(n, Tm, Mb, fg, "Sigma1 - Sigma2p");
+    getri<T>(n, fg.f, v(Gam), ws);
+    // W = eps1 Sigma1 + k^2 (eps1 - eps2)^2 Gamma .* (npar npar')
+    const double2 de = csub<double>(e1, e2);
+    const double2 de2k = cscale<double>(cmul<double>(de, de), k * k);
+    hipLaunchKernelGGL(k_lay_w<T>, dim3(nblk(nn)), dim3(256), 0, stream(), n, nvec, e1, S1, de2k, Gam, Tm);
+    MNP_HIP(hipGetLastError());
+    // M = [W G2ss - eps2 H2ss - ik de nz G2sh,  W G2sh - eps2 H2sh - ik de nz G2hh;
+    //      S1 G2hs - H2hs - ik de nz G2ss,      S1 G2hh - H2hh - ik de nz G2sh]
+    const int64_t ld = 2 * n;
+    const double2 ikde = cmul<double>(mk<double>(0, k), de);
+    struct Blk {
+      C* A;
+      int gc, hc, zc;
+      double2 al;
+      int64_t off;
+    };
+    const Blk blks[4] = {{Tm, 1, 1, 3, e2, 0},
+                         {Tm, 3, 3, 4, e2, n * ld},
+                         {S1, 2, 2, 1, one, n},
+                         {S1, 4, 4, 3, one, n * ld + n}};
+    C* Hc = Tm + nn;  // H2 component in T
+    for (const Blk& b : blks) {
+      gemm<T>(n, n, n, one, v(b.A), v(G2(b.gc)), zero, v(M + b.off, ld));
+      load(H2d + b.hc * nn, Hc, n);
+      hipLaunchKernelGGL(k_lay_mblock<T>, dim3(nblk(nn)), dim3(256), 0, stream(), n, ld, M + b.off, b.al, Hc, ikde,
+                         nvec + 2 * n, G2(b.zc));
+      MNP_HIP(hipGetLastError());
+    }
+    factor(ld, M, Mb, fm, "response matrix");
+  }
+
+  // mldivide.m:31-76 from phi, a, alpha, De (T, consumed) into sig1, sig2, h1, h2 (T).
+  void apply(int64_t p, C* phi, C* a, C* alpha, C* De, C* sig1, C* sig2, C* h1, C* h2) {
+    const int64_t np1 = n * p, np3 = 3 * n * p;
+    C* buf = (C*)wsv.get(0, sizeof(C) * (size_t)(2 * np1 + 3 * np3 + 2 * np1));
+    C *T2 = buf, *U = T2 + np1, *T1 = U + np1, *T3 = T1 + np3, *V = T3 + np3, *xi = V + np3;
+    const double2 one = mk<double>(1, 0), zero = mk<double>(0, 0);
+    const unsigned b1 = nblk(np1);
+    // alpha = alpha - Sigma1 a + ik eps1 n phi
+    gemm<T>(n, 3 * p, n, one, v(S1), v(a), zero, v(T1));
+    hipLaunchKernelGGL(k_lay_alpha<T>, dim3(b1), dim3(256), 0, stream(), n, p, alpha, T1, phi, nvec,
+                       cmul<double>(mk<double>(0, k), e1));
+    // De = De - eps1 Sigma1 phi + ik eps1 n.a + ik (eps1 - eps2) npar.(Gamma alpha); rhs = [De; alpha_z]
+    gemm<T>(n, p, n, one, v(S1), v(phi), zero, v(T2));
+    gemm<T>(n, 3 * p, n, one, v(Gam), v(alpha), zero, v(T3));
+    const double2 de = csub<double>(e1, e2);
+    hipLaunchKernelGGL(k_lay_rhs<T>, dim3(b1), dim3(256), 0, stream(), n, p, De, T2, a, T3, alpha, nvec, e1,
+                       cmul<double>(mk<double>(0, k), e1), cmul<double>(mk<double>(0, k), de), xi);
+    MNP_HIP(hipGetLastError());
+    getrs<T>(2 * n, p, fm.f, v(xi, 2 * n), ws);
+    // sig2, h2z = xi; U = G2ss sig2 + G2sh h2z
+    gemm<T>(n, p, n, one, v(G2(1)), v(xi, 2 * n), zero, v(U));
+    gemm<T>(n, p, n, one, v(G2(3)), v(xi + n, 2 * n), one, v(U));
+    // h2par = G2p^{-1} Gamma (alpha_par + ik (eps1 - eps2) npar U)
+    hipLaunchKernelGGL(k_lay_v<T>, dim3(b1), dim3(256), 0, stream(), n, p, alpha, U, nvec,
+                       cmul<double>(mk<double>(0, k), de), V);
+    gemm<T>(n, 3 * p, n, one, v(Gam), v(V), zero, v(h2));
+    getrs<T>(n, 3 * p, f2.f, v(h2), ws);  // z-components stay zero
+    // h1 = G1^{-1} ([G2p h2par; G2hs sig2 + G2hh h2z] + a)
+    gemm<T>(n, 3 * p, n, one, v(G2(0)), v(h2), zero, v(h1));
+    gemm<T>(n, p, n, one, v(G2(2)), v(xi, 2 * n), zero, v(h1 + 2 * n, 3 * n));
+    gemm<T>(n, p, n, one, v(G2(4)), v(xi + n, 2 * n), one, v(h1 + 2 * n, 3 * n));
+    hipLaunchKernelGGL(k_zaxpby<T>, dim3(nblk(np3)), dim3(256), 0, stream(), np3, one, h1, one, a, h1);
+    getrs<T>(n, 3 * p, f1.f, v(h1), ws);
+    hipLaunchKernelGGL(k_lay_setz<T>, dim3(b1), dim3(256), 0, stream(), n, p, xi + n, 2 * n, h2);
+    hipLaunchKernelGGL(k_lay_copy_rows<T>, dim3(b1), dim3(256), 0, stream(), n, p, xi, 2 * n, sig2);
+    // sig1 = G1^{-1} (U + phi)
+    hipLaunchKernelGGL(k_zaxpby<T>, dim3(b1), dim3(256), 0, stream(), np1, one, U, one, phi, sig1);
+    getrs<T>(n, p, f1.f, v(sig1), ws);
+    MNP_HIP(hipGetLastError());
+  }
+};
+
+// Dense solver above the substrate: the FP64 operator (G1, H1 and the
+// structured G2, H2 with the reflected parts), and either the FP64 chain
+// (MNP_SOLVER_FP64, the plain drop-in) or the complex FP32 chain refined
+// against the FP64 operator (MNP_SOLVER_MIXED): x += chain(b - A x) until
+// |b - A x| <= tol |b|, A the operator the FP64 chain inverts.
+struct BemRetLayer {
   const GreenRet* g;
   GreenRetLayer* gl;
   int64_t n;
-  double2 *G1, *H1, *G2c, *H2c, *F2, *S1, *Gam, *T, *M, *Mb;
-  int32_t* piv = nullptr;
-  Fac f1, f2, fg, fm;
-  Workspace ws, wsv;  // factorisations and solves; solve's vectors
+  int solver;
+  double2 *G1, *H1, *G2c, *H2c;  // FP64 operator
+  std::unique_ptr<LayerChain<double>> c64;
+  std::unique_ptr<LayerChain<float>> c32;
+  Workspace wsr;  // refinement vectors
   double k = 0, enei = -1;
   double2 e1{}, e2{};
   bool ready = false;
-  int fallbacks = 0;     // factorisations redone with partial pivoting
-  double* nz = nullptr;  // nvec z column (device view)
+  double tol = 1e-12;
+  int maxit = 10, last_it = 0;
+  double last_relres = 0;
 
-  BemRetLayer(const GreenRet* g_, GreenRetLayer* gl_) : g(g_), gl(gl_), n(g_->n) {
+  BemRetLayer(const GreenRet* g_, GreenRetLayer* gl_, int solver_) : g(g_), gl(gl_), n(g_->n), solver(solver_) {
     MNP_CHECK(gl->g == g, MNP_ERR_ARG, "bemretlayer: the layer Green function belongs to another particle");
+    MNP_CHECK(solver == MNP_SOLVER_FP64 || solver == MNP_SOLVER_MIXED, MNP_ERR_ARG,
+              "bemretlayer: solver must be MNP_SOLVER_FP64 or MNP_SOLVER_MIXED");
     const int64_t nn = n * n;
     G1 = dev_alloc<double2>(nn);
     H1 = dev_alloc<double2>(nn);
     G2c = dev_alloc<double2>(5 * nn);
     H2c = dev_alloc<double2>(5 * nn);
-    F2 = dev_alloc<double2>(nn);
-    S1 = dev_alloc<double2>(nn);
-    Gam = dev_alloc<double2>(nn);
-    T = dev_alloc<double2>(2 * nn);
-    M = dev_alloc<double2>(4 * nn);
-    Mb = dev_alloc<double2>(4 * nn);
-    piv = dev_alloc<int32_t>(2 * n);
-    for (Fac* f : {&f1, &f2, &fg}) {
-      f->sig = dev_alloc<int32_t>(n);
-      f->inv = dev_alloc<double2>(2 * inv_blocks(n));
-    }
-    fm.sig = dev_alloc<int32_t>(2 * n);
-    fm.inv = dev_alloc<double2>(2 * inv_blocks(2 * n));
-    nz = g->geo.nvec + 2 * n;
+    if (solver == MNP_SOLVER_FP64)
+      c64.reset(new LayerChain<double>(n, g->geo.nvec));
+    else
+      c32.reset(new LayerChain<float>(n, g->geo.nvec));
   }
   ~BemRetLayer() {
-    void* ps[] = {G1, H1, G2c, H2c, F2, S1, Gam, T, M, Mb, piv, f1.sig, f1.inv, f2.sig, f2.inv, fg.sig, fg.inv,
-                  fm.sig, fm.inv};
+    void* ps[] = {G1, H1, G2c, H2c};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
+  int fallbacks() const { return c64 ? c64->fallbacks : c32->fallbacks; }
   double2* G2(int c) const { return G2c + c * n * n; }
   double2* H2(int c) const { return H2c + c * n * n; }
   MatView<double> v(double2* p, int64_t ld = -1) const { return MatView<double>{p, ld < 0 ? n : ld}; }
-
-  // LU of the m x m matrix A (ld m) in place, backup: m x m scratch.
-  void factor(int64_t m, double2* A, double2* backup, Fac& F, const char* what) {
-    MatView<double> a{A, m};
-    double2 *Li = F.inv, *Ui = F.inv + inv_blocks(m);
-    copy_mat<double>(m, m, a, MatView<double>{backup, m});
-    int swap = 0;
-    int info = getrf<double>(m, m, a, piv, Li, Ui, PIVOT_NONE, &swap, ws);
-    const int32_t* sig = nullptr;
-    if (info != 0 || swap) {
-      ++fallbacks;
-      copy_mat<double>(m, m, MatView<double>{backup, m}, a);
-      info = getrf<double>(m, m, a, piv, Li, Ui, PIVOT_PARTIAL, nullptr, ws);
-      MNP_CHECK(info == 0, MNP_ERR_SINGULAR, std::string("bemretlayer: singular ") + what);
-      perm_from_pivots(m, piv, F.sig, ws);
-      sig = F.sig;
-    }
-    F.f = LU<double>{a, Li, Ui, sig};
-  }
 
   void initmat(double enei_, double2 eps_in, double2 eps_up, double2 eps_down) {
     const int64_t nn = n * n;
@@ -1143,100 +1344,108 @@ struct BemRetLayer {
     k = k0;
     ready = false;
     g->assemble(wavenumber(k0, eps_in), 2 * M_PI, OutPair{G1, nullptr}, OutPair{H1, nullptr}, n);
-    // reflected parts into G2c, H2c, the direct outside G, H into T
+    // reflected parts into G2c, H2c; the direct outside G, H added to p, ss, hh
+    // (compgreenretlayer/private/eval1.m:37-45)
     gl->eval(enei, eps_up, eps_down, G2c, H2c);
-    g->assemble(wavenumber(k0, eps_up), -2 * M_PI, OutPair{T, nullptr}, OutPair{T + nn, nullptr}, n);
-    const double2 one = mk<double>(1, 0), zero = mk<double>(0, 0);
-    for (int c : {0, 1, 4}) {  // p, ss, hh carry the direct part (compgreenretlayer/private/eval1.m:37-45)
-      hipLaunchKernelGGL(k_zaxpby, dim3(nblk(nn)), dim3(256), 0, stream(), nn, one, G2(c), one, T, G2(c));
-      hipLaunchKernelGGL(k_zaxpby, dim3(nblk(nn)), dim3(256), 0, stream(), nn, one, H2(c), one, T + nn, H2(c));
+    double2* Tg = (double2*)wsr.get(0, sizeof(double2) * (size_t)(2 * nn));
+    g->assemble(wavenumber(k0, eps_up), -2 * M_PI, OutPair{Tg, nullptr}, OutPair{Tg + nn, nullptr}, n);
+    const double2 one = mk<double>(1, 0);
+    for (int c : {0, 1, 4}) {
+      hipLaunchKernelGGL(k_zaxpby<double>, dim3(nblk(nn)), dim3(256), 0, stream(), nn, one, G2(c), one, Tg, G2(c));
+      hipLaunchKernelGGL(k_zaxpby<double>, dim3(nblk(nn)), dim3(256), 0, stream(), nn, one, H2(c), one, Tg + nn,
+                         H2(c));
     }
     MNP_HIP(hipGetLastError());
-    // G1 and G2.p factorised; Sigma1 = H1 G1^{-1}, Sigma1 - Sigma2p = Sigma1 - H2p G2p^{-1}
-    factor(n, G1, Mb, f1, "G1");
-    copy_mat<double>(n, n, v(G2(0)), v(F2));
-    factor(n, F2, Mb, f2, "G2.p");
-    copy_mat<double>(n, n, v(H1), v(S1));
-    getrs_right<double>(n, n, f1.f, v(S1), ws);
-    copy_mat<double>(n, n, v(H2(0)), v(T));
-    getrs_right<double>(n, n, f2.f, v(T), ws);
-    hipLaunchKernelGGL(k_zaxpby, dim3(nblk(nn)), dim3(256), 0, stream(), nn, one, S1, mk<double>(-1, 0), T, T);
-    MNP_HIP(hipGetLastError());
-    // Gamma = (Sigma1 - Sigma2p)^{-1}
-    factor(n, T, Mb, fg, "Sigma1 - Sigma2p");
-    getri<double>(n, fg.f, v(Gam), ws);
-    // W = eps1 Sigma1 + k^2 (eps1 - eps2)^2 Gamma .* (npar npar')
-    const double2 de = csub<double>(e1, e2);
-    const double2 de2k = cscale<double>(cmul<double>(de, de), k * k);
-    hipLaunchKernelGGL(k_lay_w, dim3(nblk(nn)), dim3(256), 0, stream(), n, g->geo.nvec, e1, S1, de2k, Gam, T);
-    MNP_HIP(hipGetLastError());
-    // M = [W G2ss - eps2 H2ss - ik de nz G2sh,  W G2sh - eps2 H2sh - ik de nz G2hh;
-    //      S1 G2hs - H2hs - ik de nz G2ss,      S1 G2hh - H2hh - ik de nz G2sh]
-    const int64_t ld = 2 * n;
-    const double2 ikde = cmul<double>(mk<double>(0, k), de);
-    struct Blk {
-      double2* A;
-      int gc, hc, zc;
-      double2 al;
-      int64_t off;
-    };
-    const Blk blks[4] = {{T, 1, 1, 3, e2, 0},
-                         {T, 3, 3, 4, e2, n * ld},
-                         {S1, 2, 2, 1, one, n},
-                         {S1, 4, 4, 3, one, n * ld + n}};
-    for (const Blk& b : blks) {
-      gemm<double>(n, n, n, one, v(b.A), v(G2(b.gc)), zero, v(M + b.off, ld));
-      hipLaunchKernelGGL(k_lay_mblock, dim3(nblk(nn)), dim3(256), 0, stream(), n, ld, M + b.off, b.al, H2(b.hc), ikde,
-                         nz, G2(b.zc));
-      MNP_HIP(hipGetLastError());
-    }
-    factor(ld, M, Mb, fm, "response matrix");
+    if (c64)
+      c64->build(k0, e1, e2, G1, H1, G2c, H2c);
+    else
+      c32->build(k0, e1, e2, G1, H1, G2c, H2c);
     ready = true;
+  }
+
+  // r = b - A x (all FP64; x = sig1 | sig2 | h1 | h2 buffers, b and r packed phi | a | alpha | De)
+  void residual(int64_t p, const double2* b, const double2* s1, const double2* s2, const double2* h1,
+                const double2* h2, double2* r) {
+    const int64_t np1 = n * p, np3 = 3 * np1;
+    double2* buf = (double2*)wsr.get(1, sizeof(double2) * (size_t)(7 * np1 + 4 * np3));
+    double2 *Gs1 = buf, *Hs1 = Gs1 + np1, *X = Hs1 + np1, *Y = X + np1, *phi2 = Y + np1, *dphi2 = phi2 + np1,
+            *Gh1 = dphi2 + np1 + np1, *Hh1 = Gh1 + np3, *A2 = Hh1 + np3, *dA2 = A2 + np3;
+    const double2 one = mk<double>(1, 0), zero = mk<double>(0, 0);
+    const MatView<double> S1v{(double2*)s1, n}, S2v{(double2*)s2, n}, H1v{(double2*)h1, n};
+    const MatView<double> hx{(double2*)h2, 3 * n}, hy{(double2*)h2 + n, 3 * n}, hz{(double2*)h2 + 2 * n, 3 * n};
+    gemm<double>(n, p, n, one, v(G1), S1v, zero, v(Gs1));
+    gemm<double>(n, 3 * p, n, one, v(G1), H1v, zero, v(Gh1));
+    gemm<double>(n, p, n, one, v(H1), S1v, zero, v(Hs1));
+    gemm<double>(n, 3 * p, n, one, v(H1), H1v, zero, v(Hh1));
+    gemm<double>(n, p, n, one, v(G2(0)), hx, zero, v(A2, 3 * n));
+    gemm<double>(n, p, n, one, v(G2(0)), hy, zero, v(A2 + n, 3 * n));
+    gemm<double>(n, p, n, one, v(G2(4)), hz, zero, v(A2 + 2 * n, 3 * n));
+    gemm<double>(n, p, n, one, v(G2(2)), S2v, zero, v(X));
+    gemm<double>(n, p, n, one, v(G2(3)), S2v, zero, v(Y));
+    gemm<double>(n, p, n, one, v(G2(1)), S2v, zero, v(phi2));
+    gemm<double>(n, p, n, one, v(G2(3)), hz, one, v(phi2));
+    gemm<double>(n, p, n, one, v(H2(0)), hx, zero, v(dA2, 3 * n));
+    gemm<double>(n, p, n, one, v(H2(0)), hy, zero, v(dA2 + n, 3 * n));
+    gemm<double>(n, p, n, one, v(H2(2)), S2v, zero, v(dA2 + 2 * n, 3 * n));
+    gemm<double>(n, p, n, one, v(H2(4)), hz, one, v(dA2 + 2 * n, 3 * n));
+    gemm<double>(n, p, n, one, v(H2(1)), S2v, zero, v(dphi2));
+    gemm<double>(n, p, n, one, v(H2(3)), hz, one, v(dphi2));
+    LayProd P{Gs1, Gh1, Hs1, Hh1, A2, X, Y, phi2, dA2, dphi2};
+    hipLaunchKernelGGL(k_lay_resid, dim3(nblk(np1)), dim3(256), 0, stream(), n, p, P, b, r, g->geo.nvec, k, e1, e2);
+    MNP_HIP(hipGetLastError());
   }
 
   void solve(int64_t p, const double2* const in[8], double2* sig1, double2* sig2, double2* h1, double2* h2) {
     MNP_CHECK(ready, MNP_ERR_STATE, "bemretlayer: solve before initmat");
-    const int64_t np1 = n * p, np3 = 3 * n * p;
-    double2* buf = (double2*)wsv.get(0, sizeof(double2) * (size_t)(4 * np1 + 5 * np3 + 2 * np1));
-    double2 *phi = buf, *De = phi + np1, *T2 = De + np1, *U = T2 + np1;
-    double2 *a = U + np1, *alpha = a + np3, *T1 = alpha + np3, *T3 = T1 + np3, *V = T3 + np3;
-    double2* xi = V + np3;
-    const double2 one = mk<double>(1, 0), zero = mk<double>(0, 0);
-    const unsigned b1 = nblk(np1);
-    hipLaunchKernelGGL(k_lay_exc, dim3(b1), dim3(256), 0, stream(), n, p, in[0], in[1], in[2], in[3], in[4], in[5],
-                       in[6], in[7], g->geo.nvec, k, e1, e2, phi, a, alpha, De);
-    // alpha = alpha - Sigma1 a + ik eps1 n phi
-    gemm<double>(n, 3 * p, n, one, v(S1), v(a), zero, v(T1));
-    hipLaunchKernelGGL(k_lay_alpha, dim3(b1), dim3(256), 0, stream(), n, p, alpha, T1, phi, g->geo.nvec,
-                       cmul<double>(mk<double>(0, k), e1));
-    // De = De - eps1 Sigma1 phi + ik eps1 n.a + ik (eps1 - eps2) npar.(Gamma alpha); rhs = [De; alpha_z]
-    gemm<double>(n, p, n, one, v(S1), v(phi), zero, v(T2));
-    gemm<double>(n, 3 * p, n, one, v(Gam), v(alpha), zero, v(T3));
-    const double2 de = csub<double>(e1, e2);
-    hipLaunchKernelGGL(k_lay_rhs, dim3(b1), dim3(256), 0, stream(), n, p, De, T2, a, T3, alpha, g->geo.nvec, e1,
-                       cmul<double>(mk<double>(0, k), e1), cmul<double>(mk<double>(0, k), de), xi);
+    const int64_t np1 = n * p, np3 = 3 * np1, nb = 2 * np1 + 2 * np3;
+    double2* b = (double2*)wsr.get(2, sizeof(double2) * (size_t)nb);
+    double2 *phi = b, *a = b + np1, *alpha = a + np3, *De = alpha + np3;
+    hipLaunchKernelGGL(k_lay_exc, dim3(nblk(np1)), dim3(256), 0, stream(), n, p, in[0], in[1], in[2], in[3], in[4],
+                       in[5], in[6], in[7], g->geo.nvec, k, e1, e2, phi, a, alpha, De);
     MNP_HIP(hipGetLastError());
-    getrs<double>(2 * n, p, fm.f, v(xi, 2 * n), ws);
-    // sig2, h2z = xi; U = G2ss sig2 + G2sh h2z
-    gemm<double>(n, p, n, one, v(G2(1)), v(xi, 2 * n), zero, v(U));
-    gemm<double>(n, p, n, one, v(G2(3)), v(xi + n, 2 * n), one, v(U));
-    // h2par = G2p^{-1} Gamma (alpha_par + ik (eps1 - eps2) npar U)
-    hipLaunchKernelGGL(k_lay_v, dim3(b1), dim3(256), 0, stream(), n, p, alpha, U, g->geo.nvec,
-                       cmul<double>(mk<double>(0, k), de), V);
-    gemm<double>(n, 3 * p, n, one, v(Gam), v(V), zero, v(h2));
-    getrs<double>(n, 3 * p, f2.f, v(h2), ws);  // z-components stay zero
-    // h1 = G1^{-1} ([G2p h2par; G2hs sig2 + G2hh h2z] + a)
-    gemm<double>(n, 3 * p, n, one, v(G2(0)), v(h2), zero, v(h1));
-    gemm<double>(n, p, n, one, v(G2(2)), v(xi, 2 * n), zero, v(h1 + 2 * n, 3 * n));
-    gemm<double>(n, p, n, one, v(G2(4)), v(xi + n, 2 * n), one, v(h1 + 2 * n, 3 * n));
-    hipLaunchKernelGGL(k_zaxpby, dim3(nblk(np3)), dim3(256), 0, stream(), np3, one, h1, one, a, h1);
-    getrs<double>(n, 3 * p, f1.f, v(h1), ws);
-    hipLaunchKernelGGL(k_lay_setz, dim3(b1), dim3(256), 0, stream(), n, p, xi + n, 2 * n, h2);
-    hipLaunchKernelGGL(k_lay_copy_rows, dim3(b1), dim3(256), 0, stream(), n, p, xi, 2 * n, sig2);
-    // sig1 = G1^{-1} (U + phi)
-    hipLaunchKernelGGL(k_zaxpby, dim3(b1), dim3(256), 0, stream(), np1, one, U, one, phi, sig1);
-    getrs<double>(n, p, f1.f, v(sig1), ws);
-    MNP_HIP(hipGetLastError());
+    last_it = 0;
+    last_relres = 0;
+    if (c64) {
+      c64->apply(p, phi, a, alpha, De, sig1, sig2, h1, h2);  // consumes b
+      return;
+    }
+    // mixed: FP32 chain applications refined with the FP64 operator
+    float2* f = (float2*)wsr.get(3, sizeof(float2) * (size_t)(2 * nb));
+    float2* fb = f;            // packed right-hand side in FP32
+    float2* fx = f + nb;       // sig1 | sig2 | h1 | h2 in FP32
+    double2* r = (double2*)wsr.get(4, sizeof(double2) * (size_t)nb);
+    double2* dx = (double2*)wsr.get(5, sizeof(double2) * (size_t)nb);
+    auto to32 = [&](const double2* src, float2* dst) {
+      convert_mat<double, float>(nb, 1, MatView<double>{(double2*)src, nb}, MatView<float>{dst, nb});
+    };
+    auto chain = [&](const double2* rhs, double2* out) {  // out = sig1 | sig2 | h1 | h2 (FP64)
+      to32(rhs, fb);
+      c32->apply(p, fb, fb + np1, fb + np1 + np3, fb + np1 + 2 * np3, fx, fx + np1, fx + 2 * np1, fx + 2 * np1 + np3);
+      convert_mat<float, double>(nb, 1, MatView<float>{fx, nb}, MatView<double>{out, nb});
+    };
+    double2* x = dx;  // first solution directly into the outputs
+    chain(b, r);
+    MNP_HIP(hipMemcpyAsync(sig1, r, sizeof(double2) * np1, hipMemcpyDeviceToDevice, stream()));
+    MNP_HIP(hipMemcpyAsync(sig2, r + np1, sizeof(double2) * np1, hipMemcpyDeviceToDevice, stream()));
+    MNP_HIP(hipMemcpyAsync(h1, r + 2 * np1, sizeof(double2) * np3, hipMemcpyDeviceToDevice, stream()));
+    MNP_HIP(hipMemcpyAsync(h2, r + 2 * np1 + np3, sizeof(double2) * np3, hipMemcpyDeviceToDevice, stream()));
+    const double nrm_b = dev_norm2(nb, b, wsr, 6);
+    for (int it = 0;; ++it) {
+      residual(p, b, sig1, sig2, h1, h2, r);
+      last_relres = nrm_b > 0 ? dev_norm2(nb, r, wsr, 6) / nrm_b : 0.0;
+      last_it = it;
+      if (last_relres <= tol) break;
+      MNP_CHECK(it < maxit, MNP_ERR_NOCONV,
+                "bemretlayer: refinement stopped at relative residual " + sci(last_relres));
+      chain(r, x);
+      const double2 one = mk<double>(1, 0);
+      double2* outs[4] = {sig1, sig2, h1, h2};
+      const int64_t off[4] = {0, np1, 2 * np1, 2 * np1 + np3}, len[4] = {np1, np1, np3, np3};
+      for (int i = 0; i < 4; ++i)
+        hipLaunchKernelGGL(k_zaxpby<double>, dim3(nblk(len[i])), dim3(256), 0, stream(), len[i], one, outs[i], one,
+                           x + off[i], outs[i]);
+      MNP_HIP(hipGetLastError());
+    }
   }
 };
 
@@ -1326,10 +1535,10 @@ int mnp_greenretlayer_eval(mnp_greenretlayer_t gl, double enei, mnp_complex eps_
     gl->gl->eval(enei, to_d2(eps_up), to_d2(eps_down), (double2*)G, (double2*)F);
   });
 }
-int mnp_bemretlayer_create(mnp_greenret_t g, mnp_greenretlayer_t gl, mnp_bemretlayer_t* out) {
+int mnp_bemretlayer_create(mnp_greenret_t g, mnp_greenretlayer_t gl, int solver, mnp_bemretlayer_t* out) {
   return guarded([&] {
     MNP_CHECK(g && gl && out, MNP_ERR_ARG, "bemretlayer_create: NULL argument");
-    auto* b = new BemRetLayer(g->g, gl->gl);
+    auto* b = new BemRetLayer(g->g, gl->gl, solver);
     *out = new mnp_bemretlayer_s{b};
   });
 }
@@ -1358,10 +1567,12 @@ int mnp_bemretlayer_solve(mnp_bemretlayer_t b, int64_t npol, const void* phi1, c
     b->b->solve(npol, in, (double2*)sig1, (double2*)sig2, (double2*)h1, (double2*)h2);
   });
 }
-int mnp_bemretlayer_stats(mnp_bemretlayer_t b, int* fallbacks) {
+int mnp_bemretlayer_stats(mnp_bemretlayer_t b, int* fallbacks, int* iterations, double* relres) {
   return guarded([&] {
-    MNP_CHECK(b && fallbacks, MNP_ERR_ARG, "bemretlayer_stats: NULL argument");
-    *fallbacks = b->b->fallbacks;
+    MNP_CHECK(b && fallbacks && iterations && relres, MNP_ERR_ARG, "bemretlayer_stats: NULL argument");
+    *fallbacks = b->b->fallbacks();
+    *iterations = b->b->last_it;
+    *relres = b->b->last_relres;
   });
 }
 int mnp_planewaveretlayer_potential(mnp_greenret_t g, const mnp_layer_options* op, double enei, mnp_complex eps_up,

@@ -555,21 +555,27 @@ def _greentab(p, op):
 
 
 class bemretlayer:
-    """Dense retarded BEM solver above a substrate (BEM/@bemretlayer), complex
-    FP64: op['layer'] a layerstructure whose upper medium is the particle's
+    """Dense retarded BEM solver above a substrate (BEM/@bemretlayer):
+    op['layer'] a layerstructure whose upper medium is the particle's
     outside; op['greentab'] a compgreentablayer (default: the tabspace of p,
-    tabulated at each wavelength)."""
+    tabulated at each wavelength).  precision 'fp64' (the complex FP64 chain,
+    the drop-in) or 'mixed' (the chain in complex FP32, refined with the FP64
+    operator)."""
+
+    _SOLVERS = {"fp64": _lib.MNP_SOLVER_FP64, "mixed": _lib.MNP_SOLVER_MIXED}
 
     def __init__(self, p, op=None, precision=None):
         op = p.op if op is None else op
         self.layer = op["layer"]
         _check_above(p, self.layer)
-        if (precision or op.get("precision", "fp64")) != "fp64":
-            raise NotImplementedError("bemretlayer: only the complex FP64 chain is built")
+        self.precision = precision or op.get("precision", "fp64")
+        if self.precision not in self._SOLVERS:
+            raise ValueError(f"bemretlayer: precision must be 'fp64' or 'mixed', not {self.precision!r}")
         self.p, self.tab = p, _greentab(p, op)
         self.green = greenretlayer(p, self.tab)
         h = C.c_void_p()
-        _lib.call("mnp_bemretlayer_create", p.green.handle, self.green.handle, C.byref(h))
+        _lib.call("mnp_bemretlayer_create", p.green.handle, self.green.handle, self._SOLVERS[self.precision],
+                  C.byref(h))
         self.handle = h
         self.enei = None
 
@@ -607,10 +613,12 @@ class bemretlayer:
     mldivide = solve
 
     def stats(self):
-        """LU factorisations so far redone with partial pivoting."""
-        fb = C.c_int()
-        _lib.call("mnp_bemretlayer_stats", self.handle, C.byref(fb))
-        return fb.value
+        """{'fallbacks': LU factorisations so far redone with partial pivoting,
+        'iterations', 'relres': refinement sweeps and final relative residual
+        of the last solve (mixed)}."""
+        fb, it, rr = C.c_int(), C.c_int(), C.c_double()
+        _lib.call("mnp_bemretlayer_stats", self.handle, C.byref(fb), C.byref(it), C.byref(rr))
+        return {"fallbacks": fb.value, "iterations": it.value, "relres": rr.value}
 
 
 def _kz(k, kpar, offset=False):

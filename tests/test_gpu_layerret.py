@@ -136,3 +136,24 @@ def test_equal_media_is_the_free_particle():
     a = B.planewave(POL[:3], DIR[:3], op).cross_sections(sig)
     b = excf.cross_sections(ref)
     assert np.allclose(a[0], b[0], rtol=1e-10) and np.allclose(a[1], b[1], rtol=1e-10)
+
+
+@pytest.mark.parametrize("eps_down", [2.25, -10 + 1.2j])
+def test_mixed_matches_fp64(eps_down):
+    """The complex FP32 chain refined with the FP64 operator gives the FP64
+    solution: sig, h to 1e-10 and cross sections to 1e-10 in a few sweeps."""
+    from mnpbem_amd import device as dev
+    B, LY, p, op, tab, sub, otab, _ = _setup(eps_down, level=3)
+    exc = B.planewave(POL, DIR, op)
+    b64 = B.bemsolver(p, op, precision="fp64")
+    bmx = B.bemsolver(p, op, precision="mixed")
+    for lam in (520.0, 640.0):
+        ex = exc(p, lam)
+        s64, smx = b64.solve(ex), bmx.solve(ex)
+        e_sig = max(_rel(dev.to_host(smx[k]), dev.to_host(s64[k])) for k in ("sig1", "sig2", "h1", "h2"))
+        c64, cmx = exc.cross_sections(s64), exc.cross_sections(smx)
+        e_cs = max(np.max(np.abs(a - b)) / np.max(np.abs(b)) for a, b in zip(cmx[:2], c64[:2]))
+        st = bmx.stats()
+        print(f"[layerret] mixed eps_down={eps_down} {lam:.0f} nm: sig {e_sig:.1e}, cross sections {e_cs:.1e}, "
+              f"sweeps {st['iterations']}, relres {st['relres']:.1e}", flush=True)
+        assert e_sig < 1e-10 and e_cs < 1e-10 and st["relres"] <= 1e-12 and st["iterations"] <= 6
