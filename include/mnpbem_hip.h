@@ -636,6 +636,20 @@ int mnp_bemretlayer_solve(mnp_bemretlayer_t bem, int64_t npol, const void* phi1,
  * way); refinement sweeps and final relative residual of the last solve
  * (MIXED; 0 and 0 for FP64). */
 int mnp_bemretlayer_stats(mnp_bemretlayer_t bem, int* fallbacks, int* iterations, double* relres);
+/* Reflected potentials and fields at points above the interface
+ * (compgreenretlayer(pt, p) for a compoint in the upper medium:
+ * Greenfun/@compgreenretlayer/potential.m, field.m with private/matmul2.m,
+ * matmul3.m; greenretlayer with p1 the points, near pairs by
+ * refinematrixlayer.m).  pts host npt x 3; fields: sig2 (n x ncol) and h2
+ * (n x 3 x ncol) the outside charges and currents (device); phi (npt x
+ * ncol), a, e, h (npt x 3 x ncol) device out, the reflected parts only (the
+ * direct ones are mnp_potential_points / mnp_field_points). */
+typedef struct mnp_pointlayer_s* mnp_pointlayer_t;
+int mnp_pointlayer_create(mnp_greenret_t g, mnp_layertab_t tab, int64_t npt, const double* pts,
+                          mnp_pointlayer_t* out);
+int mnp_pointlayer_destroy(mnp_pointlayer_t pl);
+int mnp_pointlayer_fields(mnp_pointlayer_t pl, double enei, mnp_complex eps_up, mnp_complex eps_down,
+                          int64_t ncol, const void* sig2, const void* h2, void* phi, void* a, void* e, void* h);
 /* Outside potentials of plane waves above the substrate at the faces
  * (Simulation/retarded/@planewaveretlayer/potential.m:20-105 with
  * Particles/@layerstructure/fresnel.m; the normal derivative of the phase

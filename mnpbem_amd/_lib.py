@@ -118,6 +118,9 @@ PROTOTYPES = {
     "mnp_bemretlayer_initmat": (INT, [P, D, MnpComplex, MnpComplex, MnpComplex]),
     "mnp_bemretlayer_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mnp_bemretlayer_stats": (INT, [P, C.POINTER(INT), C.POINTER(INT), C.POINTER(D)]),
+    "mnp_pointlayer_create": (INT, [P, P, I64, P, C.POINTER(P)]),
+    "mnp_pointlayer_destroy": (INT, [P]),
+    "mnp_pointlayer_fields": (INT, [P, D, MnpComplex, MnpComplex, I64, P, P, P, P, P, P]),
     "mnp_planewaveretlayer_potential": (INT, [P, C.POINTER(LayerOptions), D, MnpComplex, MnpComplex, I64, P, P,
                                               P, P, P, P]),
     "mnp_farfield_layer": (INT, [P, C.POINTER(LayerOptions), D, MnpComplex, MnpComplex, I64, P, I64, P, P, P,

@@ -455,7 +455,10 @@ __device__ inline void lay_dist(const LayGeo& L, const double* p1, const double*
   d = sqrt(r * r + z * z);
 }
 
+// rpos: the rows' positions (nr x 3, column-major): the faces' centroids,
+// or points for compgreenretlayer(pt, p).
 __global__ void __launch_bounds__(256) k_layer_refine_init(int64_t np, int64_t n, Geo geo, LayGeo L,
+                                                           const double* __restrict__ rpos, int64_t nr,
                                                            const int32_t* __restrict__ rows,
                                                            const int32_t* __restrict__ cols,
                                                            const double* __restrict__ q18, double* W, double* R0,
@@ -464,7 +467,7 @@ __global__ void __launch_bounds__(256) k_layer_refine_init(int64_t np, int64_t n
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= np) return;
   const int64_t i = rows[p], j = cols[p];
-  const double pi3[3] = {geo.pos[i], geo.pos[n + i], geo.pos[2 * n + i]};
+  const double pi3[3] = {rpos[i], rpos[nr + i], rpos[2 * nr + i]};
   double v[3][3];
   for (int a = 0; a < 3; ++a)
     for (int c = 0; c < 3; ++c) v[a][c] = geo.vtx[(a * 3 + c) * n + j];
@@ -569,6 +572,86 @@ __global__ void __launch_bounds__(64) k_layer_diag(int64_t nd, int64_t n, Geo ge
   }
 }
 
+
+// Reflected potentials and fields at points above the interface from the
+// outside charges and currents (compgreenretlayer(pt, p): potential.m,
+// field.m with private/matmul2.m, matmul3.m), one thread per (point, column
+// q): every face by the table at the centroids, the refined pairs of point
+// i (rowptr[i] .. rowptr[i + 1], faces increasing) from their vertex
+// weights.  phi (npt x ncol), a, e, h (npt x 3 x ncol): phi = Gss s + Gsh hz,
+// a = [Gp hx, Gp hy, Ghh hz + Ghs s], e = i k0 a - (Gp_ss s + Gp_sh hz),
+// h = curl of the structured product.
+__global__ void __launch_bounds__(256) k_pt_layer(int64_t npt, int64_t ncol, const double* __restrict__ pts,
+                                                  int64_t n, Geo geo, TabView T, const int64_t* __restrict__ rowptr,
+                                                  const int32_t* __restrict__ pcol, const double* __restrict__ W,
+                                                  const double* __restrict__ R0, const double* __restrict__ Z2,
+                                                  const double* __restrict__ Z1, double k0,
+                                                  const double2* __restrict__ sig2, const double2* __restrict__ h2,
+                                                  double2* phi, double2* a, double2* e, double2* h) {
+#pragma clang fp contract(off)
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= npt * ncol) return;
+  const int64_t i = t % npt, q = t / npt;
+  const double px = pts[i], py = pts[npt + i], pz = pts[2 * npt + i];
+  double2 ph = mk<double>(0, 0), av[3], ev[3], hv[3];
+  for (int c = 0; c < 3; ++c) av[c] = ev[c] = hv[c] = mk<double>(0, 0);
+  int64_t pr = rowptr[i];
+  const int64_t pe = rowptr[i + 1];
+  for (int64_t j = 0; j < n; ++j) {
+    double2 g[5], gp[5][3];
+    if (pr < pe && pcol[pr] == j) {  // refined pair: sum over the face's vertices
+      for (int c = 0; c < 5; ++c) g[c] = gp[c][0] = gp[c][1] = gp[c][2] = mk<double>(0, 0);
+      for (int v = 0; v < 3; ++v) {
+        double2 o[NQ];
+        tab_interp(T, R0[pr * 3 + v], Z1[pr], Z2[pr * 3 + v], o);
+        const double* w = W + (pr * 3 + v) * 4;
+        for (int c = 0; c < 5; ++c) {
+          g[c] = cadd<double>(g[c], cscale<double>(o[c], w[0]));
+          gp[c][0] = cadd<double>(gp[c][0], cscale<double>(o[5 + c], w[1]));
+          gp[c][1] = cadd<double>(gp[c][1], cscale<double>(o[5 + c], w[2]));
+          gp[c][2] = cadd<double>(gp[c][2], cscale<double>(o[10 + c], w[3]));
+        }
+      }
+      ++pr;
+    } else {
+      const double x = px - geo.pos[j], y = py - geo.pos[n + j];
+      const double r = sqrt(x * x + y * y);
+      double2 o[NQ];
+      tab_interp(T, r, pz, geo.pos[2 * n + j], o);
+      const double ar = geo.area[j], rr = fmax(r, 1e-10);
+      for (int c = 0; c < 5; ++c) {
+        g[c] = cscale<double>(o[c], ar);
+        gp[c][0] = cscale<double>(cscale<double>(cscale<double>(o[5 + c], x), 1.0 / rr), ar);
+        gp[c][1] = cscale<double>(cscale<double>(cscale<double>(o[5 + c], y), 1.0 / rr), ar);
+        gp[c][2] = cscale<double>(o[10 + c], ar);
+      }
+    }
+    // components p 0, ss 1, hs 2, sh 3, hh 4
+    const double2 s = sig2[q * n + j];
+    const double2 hx = h2[(0 + 3 * q) * n + j], hy = h2[(1 + 3 * q) * n + j], hz = h2[(2 + 3 * q) * n + j];
+    ph = cfma(cfma(ph, g[1], s), g[3], hz);
+    av[0] = cfma(av[0], g[0], hx);
+    av[1] = cfma(av[1], g[0], hy);
+    av[2] = cfma(cfma(av[2], g[4], hz), g[2], s);
+    double2 M[3][3];
+    for (int c = 0; c < 3; ++c) {
+      ev[c] = csub<double>(ev[c], cadd<double>(cmul<double>(gp[1][c], s), cmul<double>(gp[3][c], hz)));
+      M[c][0] = cmul<double>(gp[0][c], hx);
+      M[c][1] = cmul<double>(gp[0][c], hy);
+      M[c][2] = cadd<double>(cmul<double>(gp[4][c], hz), cmul<double>(gp[2][c], s));
+    }
+    hv[0] = cadd<double>(hv[0], csub<double>(M[1][2], M[2][1]));
+    hv[1] = cadd<double>(hv[1], csub<double>(M[2][0], M[0][2]));
+    hv[2] = cadd<double>(hv[2], csub<double>(M[0][1], M[1][0]));
+  }
+  phi[q * npt + i] = ph;
+  for (int c = 0; c < 3; ++c) {
+    const int64_t id = (c + 3 * q) * npt + i;
+    a[id] = av[c];
+    e[id] = cadd<double>(mul_i(cscale<double>(av[c], k0)), ev[c]);
+    h[id] = hv[c];
+  }
+}
 
 // ------------------------------------------------------ plane wave and far field
 // Coefficient of component set (p, hh, sh, hs) for the observer in medium
@@ -1072,8 +1155,8 @@ struct GreenRetLayer {
       MNP_HIP(hipMemcpy(rows, hr.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
       MNP_HIP(hipMemcpy(cols, hc.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
       LayGeo L{z0, tab->op.rmin, tab->op.zmin};
-      hipLaunchKernelGGL(k_layer_refine_init, dim3(nblk(np)), dim3(256), 0, stream(), np, n, g->geo, L, rows, cols,
-                         g->q18, W, R0, Z2, Z1);
+      hipLaunchKernelGGL(k_layer_refine_init, dim3(nblk(np)), dim3(256), 0, stream(), np, n, g->geo, L, g->geo.pos, n,
+                         rows, cols, g->q18, W, R0, Z2, Z1);
       MNP_HIP(hipGetLastError());
     }
     if (nd) MNP_HIP(hipMemcpy(id, hd.data(), sizeof(int32_t) * nd, hipMemcpyHostToDevice));
@@ -1106,6 +1189,90 @@ struct GreenRetLayer {
     MNP_HIP(hipMemcpyAsync(&herr, tab->err, sizeof(int), hipMemcpyDeviceToHost, stream()));
     MNP_HIP(hipStreamSynchronize(stream()));
     MNP_CHECK(herr == 0, MNP_ERR_ARG, "greentablayer: a face pair lies outside the table (tabspace of this particle?)");
+  }
+};
+
+// Reflected Green function between points above the interface and a
+// particle (greenretlayer with p1 a compoint): the near pairs of
+// refinematrixlayer.m (divided by the face radius, as a compoint has none)
+// with their shape-function weights, per point in face order.
+struct PointLayer {
+  const GreenRet* g;
+  LayerTab* tab;
+  int64_t npt = 0, np = 0;
+  double* pts = nullptr;  // npt x 3 column-major
+  int64_t* rowptr = nullptr;
+  int32_t *rows = nullptr, *cols = nullptr;
+  double *W = nullptr, *R0 = nullptr, *Z2 = nullptr, *Z1 = nullptr;
+
+  PointLayer(const GreenRet* g_, LayerTab* t_, int64_t npt_, const double* pts_h) : g(g_), tab(t_), npt(npt_) {
+    MNP_CHECK(!g->compound(), MNP_ERR_ARG, "compgreenretlayer: compound particles are not supported");
+    MNP_CHECK(npt >= 1, MNP_ERR_ARG, "compgreenretlayer: no points");
+    const int64_t n = g->n;
+    const double z0 = tab->op.z0;
+    for (int64_t i = 0; i < npt; ++i)
+      MNP_CHECK(pts_h[2 * npt + i] > z0, MNP_ERR_ARG, "compgreenretlayer: every point must lie above the interface");
+    std::vector<double> pos(3 * n), rad(n);
+    MNP_HIP(hipMemcpy(pos.data(), g->geo.pos, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    MNP_HIP(hipMemcpy(rad.data(), g->geo.rad, sizeof(double) * n, hipMemcpyDeviceToHost));
+    std::vector<double> b2(n);
+    for (int64_t j = 0; j < n; ++j) b2[j] = pos[j] * pos[j] + pos[n + j] * pos[n + j];
+    const double rel = g->op.rel_cutoff, absc = g->op.abs_cutoff;
+    std::vector<int64_t> rp(npt + 1, 0);
+    std::vector<int32_t> hr, hc;
+    for (int64_t i = 0; i < npt; ++i) {
+      const double xi = pts_h[i], yi = pts_h[npt + i], zi = std::fabs(pts_h[2 * npt + i] - z0);
+      const double a2 = xi * xi + yi * yi;
+      for (int64_t j = 0; j < n; ++j) {
+        double r2 = a2 + b2[j] - 2 * (xi * pos[j] + yi * pos[n + j]);
+        if (r2 < 1e-10) r2 = 0;
+        const double z = zi + std::fabs(pos[2 * n + j] - z0);
+        const double d2 = std::sqrt(r2 + z * z) - rad[j];
+        if (d2 < absc || d2 / rad[j] < rel) {
+          hr.push_back((int32_t)i);
+          hc.push_back((int32_t)j);
+        }
+      }
+      rp[i + 1] = (int64_t)hr.size();
+    }
+    np = (int64_t)hr.size();
+    pts = dev_alloc<double>(3 * npt);
+    rowptr = dev_alloc<int64_t>(npt + 1);
+    rows = dev_alloc<int32_t>(np);
+    cols = dev_alloc<int32_t>(np);
+    W = dev_alloc<double>(12 * np);
+    R0 = dev_alloc<double>(3 * np);
+    Z2 = dev_alloc<double>(3 * np);
+    Z1 = dev_alloc<double>(np);
+    MNP_HIP(hipMemcpy(pts, pts_h, sizeof(double) * 3 * npt, hipMemcpyHostToDevice));
+    MNP_HIP(hipMemcpy(rowptr, rp.data(), sizeof(int64_t) * (npt + 1), hipMemcpyHostToDevice));
+    if (np) {
+      MNP_HIP(hipMemcpy(rows, hr.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
+      MNP_HIP(hipMemcpy(cols, hc.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice));
+      LayGeo L{z0, tab->op.rmin, tab->op.zmin};
+      hipLaunchKernelGGL(k_layer_refine_init, dim3(nblk(np)), dim3(256), 0, stream(), np, n, g->geo, L, pts, npt, rows,
+                         cols, g->q18, W, R0, Z2, Z1);
+      MNP_HIP(hipGetLastError());
+    }
+    MNP_HIP(hipStreamSynchronize(stream()));
+  }
+  ~PointLayer() {
+    void* ps[] = {pts, rowptr, rows, cols, W, R0, Z2, Z1};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
+  void fields(double enei, double2 e1, double2 e2, int64_t ncol, const double2* sig2, const double2* h2, double2* phi,
+              double2* a, double2* e, double2* h) {
+    const double2* tv = tab->at(enei, e1, e2);
+    const TabView T = tab->view(tv);
+    MNP_HIP(hipMemsetAsync(tab->err, 0, sizeof(int), stream()));
+    hipLaunchKernelGGL(k_pt_layer, dim3(nblk(npt * ncol)), dim3(256), 0, stream(), npt, ncol, pts, g->n, g->geo, T,
+                       rowptr, cols, W, R0, Z2, Z1, 2 * M_PI / enei, sig2, h2, phi, a, e, h);
+    MNP_HIP(hipGetLastError());
+    int herr = 0;
+    MNP_HIP(hipMemcpyAsync(&herr, tab->err, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipStreamSynchronize(stream()));
+    MNP_CHECK(herr == 0, MNP_ERR_ARG, "greentablayer: a point-face pair lies outside the table (tabspace with the points?)");
   }
 };
 
@@ -1460,6 +1627,9 @@ struct mnp_greenretlayer_s {
 struct mnp_bemretlayer_s {
   mnp::BemRetLayer* b;
 };
+struct mnp_pointlayer_s {
+  mnp::PointLayer* pl;
+};
 
 using namespace mnp;
 
@@ -1573,6 +1743,30 @@ int mnp_bemretlayer_stats(mnp_bemretlayer_t b, int* fallbacks, int* iterations, 
     *fallbacks = b->b->fallbacks();
     *iterations = b->b->last_it;
     *relres = b->b->last_relres;
+  });
+}
+int mnp_pointlayer_create(mnp_greenret_t g, mnp_layertab_t tab, int64_t npt, const double* pts,
+                          mnp_pointlayer_t* out) {
+  return guarded([&] {
+    MNP_CHECK(g && tab && pts && out, MNP_ERR_ARG, "pointlayer_create: NULL argument");
+    auto* pl = new PointLayer(g->g, tab->t, npt, pts);
+    *out = new mnp_pointlayer_s{pl};
+  });
+}
+int mnp_pointlayer_destroy(mnp_pointlayer_t pl) {
+  return guarded([&] {
+    if (!pl) return;
+    delete pl->pl;
+    delete pl;
+  });
+}
+int mnp_pointlayer_fields(mnp_pointlayer_t pl, double enei, mnp_complex eps_up, mnp_complex eps_down, int64_t ncol,
+                          const void* sig2, const void* h2, void* phi, void* a, void* e, void* h) {
+  return guarded([&] {
+    MNP_CHECK(pl && sig2 && h2 && phi && a && e && h, MNP_ERR_ARG, "pointlayer_fields: NULL argument");
+    MNP_CHECK(ncol >= 1, MNP_ERR_ARG, "pointlayer_fields: ncol must be positive");
+    pl->pl->fields(enei, to_d2(eps_up), to_d2(eps_down), ncol, (const double2*)sig2, (const double2*)h2,
+                   (double2*)phi, (double2*)a, (double2*)e, (double2*)h);
   });
 }
 int mnp_planewaveretlayer_potential(mnp_greenret_t g, const mnp_layer_options* op, double enei, mnp_complex eps_up,

@@ -73,30 +73,43 @@ class layerstructure:
         z[sel] = self.z[0] + np.sign(z[sel] - self.z[0]) * zmin
         return z
 
-    def tabspace(self, p, nr=30, nz=30, scale=1.05):
-        """Table grid for the particle p above the interface (tabspace2.m:
-        44-75 with limits and adjust.m:18-33, tabspace1.m:24-48 with 'log'
-        spacing, greentablayer/private/init.m:8-14): dict r, z1, z2."""
+    def tabspace(self, p, pt=None, nr=30, nz=30, scale=1.05):
+        """Table grid for the particle p above the interface, and for the
+        points pt (a compoint or an npt x 3 array above the interface) when
+        given (tabspace2.m:44-75 with limits and adjust.m:18-33, tabspace1.m:
+        24-48 with 'log' spacing, greentablayer/private/init.m:8-14): dict r,
+        z1, z2."""
         v = np.asarray(p.verts, dtype=np.float64)
         z0 = self.z[0]
         if np.any(v[:, 2] <= z0):
             raise NotImplementedError("tabspace: vertices at or below the interface are not supported")
-        xy = v[:, :2]
+        obs = v  # observers (rows) of the table: the vertices and the points
+        if pt is not None:
+            q = np.atleast_2d(np.asarray(getattr(pt, "pos", pt), dtype=np.float64))
+            if np.any(q[:, 2] <= z0):
+                raise NotImplementedError("tabspace: points at or below the interface are not supported")
+            obs = np.vstack([v, q])
+        xy, oxy = v[:, :2], obs[:, :2]
         rmax = 0.0
-        for i in range(0, len(xy), 2048):  # largest vertex distance, in chunks
-            d2 = np.sum((xy[i:i + 2048, None, :] - xy[None, :, :]) ** 2, axis=-1)
+        for i in range(0, len(oxy), 2048):  # largest observer-vertex distance, in chunks
+            d2 = np.sum((oxy[i:i + 2048, None, :] - xy[None, :, :]) ** 2, axis=-1)
             rmax = max(rmax, float(np.sqrt(d2.max())))
         m = 0.5 * rmax
         r = [max(m - 0.5 * scale * rmax, 0.0), m + 0.5 * scale * rmax]
-        zl, zh = float(v[:, 2].min()), float(v[:, 2].max())
-        if abs(zl - zh) < 1e-10:
-            zr = [zl - 1e-2, zl + 1e-2]
-        else:
-            mz = 0.5 * (zl + zh)
-            zr = [mz + 0.5 * scale * (zl - zh), mz - 0.5 * scale * (zl - zh)]
-        if zr[0] <= z0:  # adjust.m:45 has <, which leaves a bound exactly at z0 (log10(0) later)
-            zr[0] = z0 + 1e-10
-        z1 = [2 * zr[0] - z0, 2 * zr[1] - z0]
+
+        def enlarge(z):
+            zl, zh = float(z.min()), float(z.max())
+            if abs(zl - zh) < 1e-10:
+                zr = [zl - 1e-2, zl + 1e-2]
+            else:
+                mz = 0.5 * (zl + zh)
+                zr = [mz + 0.5 * scale * (zl - zh), mz - 0.5 * scale * (zl - zh)]
+            if zr[0] <= z0:  # adjust.m:45 has <, which leaves a bound exactly at z0 (log10(0) later)
+                zr[0] = z0 + 1e-10
+            return zr
+
+        za, zb = enlarge(obs[:, 2]), enlarge(v[:, 2])
+        z1 = [za[0] + zb[0] - z0, za[1] + zb[1] - z0]
         rg = np.logspace(np.log10(max(r[0], self.rmin)), np.log10(r[1]), nr)
         zz = np.sort(self.round(np.array(z1), zmin=0.999 * self.zmin))
         if abs(zz[0] - zz[1]) < 1e-3:
@@ -542,6 +555,88 @@ class greenretlayer:
         _lib.call("mnp_greenretlayer_eval", self.handle, float(enei), _lib.MnpComplex.of(eu), _lib.MnpComplex.of(ed),
                   dev.ptr(dev.base(G)), dev.ptr(dev.base(F)))
         return G, F
+
+
+class compgreenretlayer_points:
+    """compgreenretlayer(pt, p, op) for a compoint pt above the interface and
+    a particle p (Greenfun/@compgreenretlayer/potential.m, field.m): the
+    direct part of compgreenret(pt, p) plus, for the points in the upper
+    medium, the reflected part from the charges and currents facing it
+    (private/matmul2.m, matmul3.m), on the GPU (mnp_pointlayer_*).
+    op['greentab'] must cover the point-face pairs (layer.tabspace(p, pt));
+    without it that table is made here."""
+
+    def __init__(self, pt, p, op):
+        from .points import compgreenret_points
+        self.layer = op["layer"]
+        _check_above(p, self.layer)
+        up = self.layer.ind[0]
+        self.pt, self.p = pt, p
+        self.direct = compgreenret_points(pt, p)
+        self.sel = np.nonzero(np.asarray(pt.inout) == up)[0]
+        pos = np.asarray(pt.pos, dtype=np.float64)[self.sel]
+        if np.any(pos[:, 2] <= self.layer.z[0]):
+            raise NotImplementedError("compgreenretlayer: points below the interface (transmitted fields) "
+                                      "are not supported")
+        tab = op.get("greentab")
+        if tab is None:
+            tab = compgreentablayer(self.layer, self.layer.tabspace(p, pos))
+        elif tab.layer is not self.layer:
+            raise ValueError("greentab: tabulated for another layer structure")
+        self.tab, self.handle = tab, None
+        if len(self.sel):
+            h = C.c_void_p()
+            pts = np.asfortranarray(pos)
+            _lib.call("mnp_pointlayer_create", p.green.handle, tab.handle, len(pos), pts.ctypes.data_as(C.c_void_p),
+                      C.byref(h))
+            self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib.can_release():
+            _lib.lib().mnp_pointlayer_destroy(h)
+            self.handle = None
+
+    def reflected(self, sig):
+        """The reflected phi (nsel x ncol), a, e, h (nsel x 3 x ncol) at the
+        points of the upper medium (host arrays)."""
+        enei, ncol, m = sig.enei, sig.sig2.shape[-1], len(self.sel)
+        s, h = self.direct._charges(sig, self.layer.ind[0])
+        eu, ed = self.layer.epsilon(enei)
+        out = [dev.colmajor((m, ncol))] + [dev.colmajor((m, 3, ncol)) for _ in range(3)]
+        _lib.call("mnp_pointlayer_fields", self.handle, float(enei), _lib.MnpComplex.of(eu), _lib.MnpComplex.of(ed),
+                  ncol, dev.ptr(dev.base(s)), dev.ptr(dev.base(h)), *[dev.ptr(dev.base(t)) for t in out])
+        return [dev.to_host(t) for t in out]
+
+    def potential(self, sig, inout=1):
+        """phi1 (npt x ncol), a1 (npt x 3 x ncol) at the points, direct plus
+        reflected (potential.m; points carry no normals, so phi1p, a1p are
+        not formed)."""
+        pot = self.direct.potential(sig, inout)
+        if len(self.sel):
+            phi, a, _, _ = self.reflected(sig)
+            pot["phi1"][self.sel] += phi
+            pot["a1"][self.sel] += a
+        return pot
+
+    def field(self, sig, inout=1):
+        """e, h (npt x 3 x ncol) at the points, direct plus reflected
+        (field.m)."""
+        f = self.direct.field(sig, inout)
+        if len(self.sel):
+            _, _, e, h = self.reflected(sig)
+            f["e"][self.sel] += e
+            f["h"][self.sel] += h
+        return f
+
+
+def compgreenretlayer(p1, p2, op):
+    """compgreenretlayer(pt, p, op) for a compoint pt (the particle's own
+    Green function with the interface is bemretlayer's greenretlayer)."""
+    from .points import compoint
+    if isinstance(p1, compoint):
+        return compgreenretlayer_points(p1, p2, op)
+    raise NotImplementedError("compgreenretlayer: points (compoint) and a particle")
 
 
 def _greentab(p, op):

@@ -27,6 +27,9 @@ substrate path, one interface at z0 with the particle in the upper medium.
     mldivide.m:13-76, private/excitation.m) and the boundary operator the
     chain inverts (afun below: continuity of phi and A with the structured
     outside Green function, and the two normal-derivative conditions);
+  * potentials and fields at points above the interface from the reflected
+    part (compgreenretlayer(pt, p): potential.m, field.m, private/matmul2.m,
+    matmul3.m; greenretlayer with p1 a compoint);
   * the plane wave above the substrate (Simulation/retarded/
     @planewaveretlayer/potential.m:20-105, with Particles/@layerstructure/
     fresnel.m) and its cross sections through the far field with the
@@ -229,18 +232,18 @@ def reflected_green(sub, enei, r, Z, coeff=None):
 # --------------------------------------------------------------------------
 # table grid and interpolation (tabspace2.m, tabspace1.m, greentablayer)
 # --------------------------------------------------------------------------
-def tab_grid(sub, verts, nr=30, nz=30, scale=1.05):
+def tab_grid(sub, verts, nr=30, nz=30, scale=1.05, pts=None):
     """Radii and observer heights of the 2D table for one particle above the
-    interface (tabspace2.m:44-75 with limits, adjust.m:18-33, tabspace1.m:
-    24-48 'log' spacing, greentablayer/private/init.m:8-14).  Returns r (nr,),
-    z1 (nz,) and the source height z2 of the table."""
+    interface, and for points above it when given (tabspace2.m:16-75 with
+    limits, adjust.m:18-33, tabspace1.m:24-48 'log' spacing,
+    greentablayer/private/init.m:8-14).  Returns r (nr,), z1 (nz,) and the
+    source height z2 of the table."""
     v = np.asarray(verts, dtype=float)
-    xy = v[:, :2]
-    d = np.sqrt(np.sum((xy[:, None, :] - xy[None, :, :]) ** 2, axis=-1))
+    obs = v if pts is None else np.vstack([v, np.atleast_2d(np.asarray(pts, dtype=float))])
+    d = np.sqrt(np.sum((obs[:, None, :2] - v[None, :, :2]) ** 2, axis=-1))
     rlo, rhi = d.min(), d.max()
     m = 0.5 * (rlo + rhi)
     r = [max(m + 0.5 * scale * (rlo - rhi), 0.0), max(m - 0.5 * scale * (rlo - rhi), 0.0)]
-    zl, zh = v[:, 2].min(), v[:, 2].max()
 
     def enlarge(lo, hi):
         if abs(lo - hi) < 1e-10:
@@ -252,8 +255,8 @@ def tab_grid(sub, verts, nr=30, nz=30, scale=1.05):
             z[0] = sub.z0 + 1e-10
         return z
 
-    z1 = enlarge(zl, zh)
-    z2 = enlarge(zl, zh)
+    z1 = enlarge(obs[:, 2].min(), obs[:, 2].max())
+    z2 = enlarge(v[:, 2].min(), v[:, 2].max())
     z1 = [z1[0] + z2[0] - sub.z0, z1[1] + z2[1] - sub.z0]
     z2s = sub.z0 + 1e-10
     rg = np.logspace(np.log10(max(r[0], sub.rmin)), np.log10(r[1]), nr)
@@ -773,3 +776,82 @@ def scattering(sub, pos, area, sig, dirs, enei, pinf_nvec, pinf_area):
     dz = np.atleast_2d(dirs)[:, 2]
     fac = np.where(dz < 0, 0.5 * np.sqrt(e1 + 0j), np.where(dz > 0, 0.5 * np.sqrt(e2 + 0j), 1.0))
     return np.real(sca / fac)
+
+
+# --------------------------------------------------------------------------
+# points above the interface: compgreenretlayer(pt, p) (potential.m, field.m)
+# --------------------------------------------------------------------------
+class PointGreenLayer(GreenRetLayer):
+    """Reflected G (5, npt, n) and Gp (5, npt, 3, n) from the faces of the
+    particle to points above the interface (greenretlayer with p1 a
+    compoint: refinematrixlayer.m divides by the face radius, as compoint has
+    none; init2.m with p1 the points; no diagonal elements)."""
+
+    def __init__(self, sub, pts, verts, faces, table, relcutoff=3.0, abscutoff=0.0):
+        self.pts = np.atleast_2d(np.asarray(pts, dtype=float))
+        self.sub, self.tab = sub, table
+        self.verts = np.asarray(verts, dtype=float)
+        self.faces = np.asarray(faces, dtype=np.int64)
+        _, _, _, self.nvec, self.area = O.norm_flat(self.verts, self.faces)
+        self.fpos = O.norm_flat(self.verts, self.faces)[0]
+        self.n = len(self.faces)
+        rad = O.bradius(self.verts, self.faces, self.fpos)
+        r = O.pdist2(self.pts[:, :2], self.fpos[:, :2])
+        z = sub.mindist(self.pts[:, 2])[:, None] + sub.mindist(self.fpos[:, 2])[None, :]
+        d2 = np.sqrt(r ** 2 + z ** 2) - rad[None, :]
+        self.ir = ((d2 < abscutoff) | (d2 / rad[None, :] < relcutoff)).astype(np.int8)
+        self.pos = self.pts  # rows of the refinement weights
+        self._init_refine()
+
+    def eval(self, enei):
+        tab = self.tab
+        tabn = tab.normalised(enei)
+        p1, p2 = self.pts, self.fpos
+        x = p1[:, None, 0] - p2[None, :, 0]
+        y = p1[:, None, 1] - p2[None, :, 1]
+        r = np.sqrt(x * x + y * y)
+        z1 = np.repeat(p1[:, 2:3], self.n, axis=1)
+        z2 = np.repeat(p2[None, :, 2], len(p1), axis=0)
+        G, Fr, Fz, _, _ = tab.interp(tabn, r, z1, z2)
+        G = G * self.area[None, None, :]
+        rr = np.maximum(r, 1e-10)
+        Gp = np.stack([Fr * x / rr, Fr * y / rr, Fz], axis=2) * self.area[None, None, None, :]
+        acc = {}
+        for nb, face, r0, z1v, z2v, g, f1, f2, fz in self.ref:
+            gv, frv, fzv, _, _ = tab.interp(tabn, r0, z1v, z2v)
+            if face not in acc:
+                acc[face] = [nb, np.zeros((5, len(nb)), complex), np.zeros((5, 3, len(nb)), complex)]
+            acc[face][1] += gv * g
+            acc[face][2][:, 0] += frv * f1
+            acc[face][2][:, 1] += frv * f2
+            acc[face][2][:, 2] += fzv * fz
+        for face, (nb, gs, gps) in acc.items():
+            G[:, nb, face] = gs
+            Gp[:, nb, :, face] = np.transpose(gps, (2, 0, 1))
+        return {"G": G, "Gp": Gp}
+
+
+def points_reflected(pg, enei, sig):
+    """Reflected potentials and fields at the points from the outside charges
+    and currents (compgreenretlayer/potential.m, field.m with matmul2.m,
+    matmul3.m): phi = Gss s2 + Gsh h2z, a = [Gp h2x, Gp h2y, Ghh h2z + Ghs s2],
+    e = i k0 a - (Gp_ss s2 + Gp_sh h2z), h = curl of the structured product."""
+    k0 = 2 * np.pi / enei
+    m = pg.eval(enei)
+    G, Gp = m["G"], m["Gp"]
+    ic = {nm: i for i, nm in enumerate(COMPONENTS)}
+    s2, h2 = sig["sig2"], sig["h2"]
+    mm = lambda A, X: np.einsum("ij,jq->iq", A, X)
+    phi = mm(G[ic["ss"]], s2) + mm(G[ic["sh"]], h2[:, 2])
+    a = np.stack([mm(G[ic["p"]], h2[:, 0]), mm(G[ic["p"]], h2[:, 1]),
+                  mm(G[ic["hh"]], h2[:, 2]) + mm(G[ic["hs"]], s2)], axis=1)
+    e = 1j * k0 * a - np.stack([mm(Gp[ic["ss"]][:, c], s2) + mm(Gp[ic["sh"]][:, c], h2[:, 2]) for c in range(3)],
+                               axis=1)
+
+    def M(i1, i2):
+        if i2 < 2:
+            return mm(Gp[ic["p"]][:, i1], h2[:, i2])
+        return mm(Gp[ic["hh"]][:, i1], h2[:, 2]) + mm(Gp[ic["hs"]][:, i1], s2)
+
+    h = np.stack([M(1, 2) - M(2, 1), M(2, 0) - M(0, 2), M(0, 1) - M(1, 0)], axis=1)
+    return {"phi": phi, "a": a, "e": e, "h": h}

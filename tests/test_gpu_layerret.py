@@ -157,3 +157,48 @@ def test_mixed_matches_fp64(eps_down):
         print(f"[layerret] mixed eps_down={eps_down} {lam:.0f} nm: sig {e_sig:.1e}, cross sections {e_cs:.1e}, "
               f"sweeps {st['iterations']}, relres {st['relres']:.1e}", flush=True)
         assert e_sig < 1e-10 and e_cs < 1e-10 and st["relres"] <= 1e-12 and st["iterations"] <= 6
+
+
+def test_point_fields_match_oracle():
+    """compgreenretlayer(pt, p): potentials and fields at points above the
+    substrate (far, near the surface, near the interface below the particle
+    with refined pairs, and inside the particle, which see no reflection)
+    equal the oracle's direct plus reflected parts."""
+    from mnpbem_amd import device as dev
+    from mnpbem_amd.layer import compgreenretlayer
+    from mnpbem_amd.points import compoint
+    from oracle import dipole_oracle as D
+    B, LY, p, op, tab, sub, otab, (v, f) = _setup(-10 + 1.2j)
+    pts = np.array([[0.0, 0.0, 14.0], [9.0, 2.0, 7.0], [-3.0, 12.0, 20.0], [0.0, 0.0, 11.8], [5.6, 0.0, 6.0],
+                    [3.0, 3.0, 1.5], [1.5, 0.0, 0.6], [0.0, 0.0, 0.3], [0.0, 0.0, 6.0], [1.0, -1.0, 4.0]])
+    pt = compoint(p, pts, mindist=0.2)
+    assert pt.n == len(pts) and set(np.unique(pt.inout)) == {1, 2}
+    up = pt.inout == 1
+    tabp = LY.compgreentablayer(tab.layer, tab.layer.tabspace(p, pt.pos[up], nr=NTAB, nz=NTAB))
+    gf = compgreenretlayer(pt, p, dict(op, greentab=tabp))
+    otp = L.GreenTable(sub, tabp.r, tabp.z1, tabp.z2)
+    pg = L.PointGreenLayer(sub, pt.pos[up], v, f, otp)
+    assert pg.ir.any()  # some pairs are refined
+    g = O.GreenRet(v, f)
+    lam = 600.0
+    exc = B.planewave(POL[:2], DIR[:2], op)
+    sig = B.bemsolver(p, op).solve(exc(p, lam))
+    hs = {k: dev.to_host(sig[k]) for k in ("sig1", "sig2", "h1", "h2")}
+    refl = L.points_reflected(pg, lam, hs)
+    fld, pot = gf.field(sig), gf.potential(sig)
+    k0 = 2 * np.pi / lam
+    for m, eps, s, hh in ((1, 1.0, hs["sig2"], hs["h2"]), (2, O.eps_drude(lam, *DRUDE), hs["sig1"], hs["h1"])):
+        sel = pt.inout == m
+        k = k0 * np.sqrt(complex(eps))
+        e, h = D.field_points(g, pt.pos[sel], s, hh, k, lam)
+        G, _ = D.point_green(g, pt.pos[sel], k)
+        phi, a = G @ s, np.einsum("pj,jcq->pcq", G, hh)
+        if m == 1:
+            e, h = e + refl["e"], h + refl["h"]
+            phi, a = phi + refl["phi"], a + refl["a"]
+        errs = [_rel(fld["e"][sel], e), _rel(fld["h"][sel], h), _rel(pot["phi1"][sel], phi), _rel(pot["a1"][sel], a)]
+        print(f"[layerret] points in medium {m}: e {errs[0]:.1e} h {errs[1]:.1e} phi {errs[2]:.1e} a {errs[3]:.1e}",
+              flush=True)
+        assert max(errs) < 1e-10
+    # the reflected part alone is a sizeable share at the points above
+    assert np.abs(refl["e"]).max() > 1e-3 * np.abs(fld["e"][up]).max()

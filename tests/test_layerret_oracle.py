@@ -216,3 +216,104 @@ def test_small_sphere_follows_the_quasistatic_substrate(sphere):
         s0 = O.stat_cross_sections(g.pos, g.area, O.BemStat(g).initmat(lam, eg, eu).solve(
             O.planewavestat_potential(g.nvec, POL)), POL, lam, eu)[0]
         assert np.all(np.abs((ext / e0) / (es / s0) - 1) < 0.02)
+
+
+class _ExactTable:
+    """The reflected Green function by direct quadrature at every pair, in
+    place of the interpolated table (same clamping and rounding)."""
+
+    def __init__(self, sub):
+        self.sub = sub
+
+    def normalised(self, enei):
+        return enei
+
+    def interp(self, enei, r, z1, z2):
+        sub = self.sub
+        r = np.maximum(sub.rmin, np.asarray(r, dtype=float))
+        Z = sub.mindist(sub.round(z1)) + sub.mindist(sub.round(z2))
+        out = [np.empty((5,) + r.shape, complex) for _ in range(3)]
+        for idx in np.ndindex(r.shape):
+            for o, v in zip(out, L.reflected_green(sub, enei, r[idx], Z[idx])):
+                o[(slice(None),) + idx] = v
+        return out[0], out[1], out[2], r, Z
+
+
+def _charges(n, ncol, seed=3):
+    rng = np.random.default_rng(seed)
+    c = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    return {"sig2": c(n, ncol), "h2": c(n, 3, ncol)}
+
+
+def test_point_fields_are_the_potentials_derivatives():
+    """Reflected fields at points above the interface: e = i k0 a - grad phi
+    and h = curl a, the derivatives by central differences (step 1e-3 nm) of
+    the reflected potentials, with the Green function integrated exactly at
+    every point-face pair.  Pins the direction of Gp, the structured products
+    (ss, sh into phi; p, hh, hs into a) and the curl's index order."""
+    v, f = shapes.icosphere(1, 4.0)
+    v = v + [0, 0, 5.0]
+    sub = L.Substrate(1.0, -8 + 1.5j, 0.0)
+    pts = np.array([[12.0, 3.0, 9.0], [-4.0, 10.0, 14.0], [2.0, -1.0, 16.0]])
+    lam, hstep = 600.0, 1e-3
+    sig = _charges(len(f), 2)
+    tab = _ExactTable(sub)
+    pg = L.PointGreenLayer(sub, pts, v, f, tab)
+    assert not pg.ir.any()  # far points: centroid values throughout
+    ref = L.points_reflected(pg, lam, sig)
+    dphi = np.zeros_like(ref["a"])
+    da = np.zeros(ref["a"].shape[:2] + (3,) + ref["a"].shape[2:], complex)  # (npt, d/dx_c, comp, ncol)
+    for c in range(3):
+        dx = np.zeros(3)
+        dx[c] = hstep
+        up = L.points_reflected(L.PointGreenLayer(sub, pts + dx, v, f, tab), lam, sig)
+        dn = L.points_reflected(L.PointGreenLayer(sub, pts - dx, v, f, tab), lam, sig)
+        dphi[:, c] = (up["phi"] - dn["phi"]) / (2 * hstep)
+        da[:, c] = (up["a"] - dn["a"]) / (2 * hstep)
+    k0 = 2 * np.pi / lam
+    e = 1j * k0 * ref["a"] - dphi
+    h = np.stack([da[:, 1, 2] - da[:, 2, 1], da[:, 2, 0] - da[:, 0, 2], da[:, 0, 1] - da[:, 1, 0]], axis=1)
+    ee = np.abs(ref["e"] - e).max() / np.abs(e).max()
+    eh = np.abs(ref["h"] - h).max() / np.abs(h).max()
+    assert ee < 1e-6 and eh < 1e-6, (ee, eh)
+
+
+def test_point_green_matches_the_particle_green():
+    """Points at the particle's own centroids: the point Green function equals
+    the particle's reflected G and Gp off the diagonal wherever both refine
+    (or both do not), and the refined entries differ from the centroid ones
+    (the near-pair weights are in use)."""
+    v, f = shapes.icosphere(1, 4.0)
+    v = v + [0, 0, 5.0]
+    sub = L.Substrate(1.0, 2.25, 0.0)
+    pos = O.norm_flat(v, f)[0]
+    rg, zg, z2 = L.tab_grid(sub, v, nr=12, nz=12, pts=pos)
+    tab = L.GreenTable(sub, rg, zg, z2)
+    gp = L.GreenRetLayer(sub, v, f, tab)
+    pp = L.PointGreenLayer(sub, pos, v, f, tab)
+    a, b = gp.eval(550.0), pp.eval(550.0)
+    same = (gp.ir == 1) == (pp.ir == 1)
+    np.fill_diagonal(same, False)
+    assert same.sum() > 0.9 * (len(f) ** 2 - len(f)) and (pp.ir == 1).sum() > len(f)
+    for c in range(5):
+        assert np.abs(a["G"][c][same] - b["G"][c][same]).max() < 1e-13 * np.abs(a["G"][c]).max()
+        for d in range(3):
+            x, y = a["Gp"][c][:, d][same], b["Gp"][c][:, d][same]
+            assert np.abs(x - y).max() < 1e-13 * np.abs(a["Gp"][c][:, d]).max()
+    # the refined pairs are not the centroid values
+    cen = L.PointGreenLayer(sub, pos, v, f, tab, relcutoff=0.0).eval(550.0)
+    ir = pp.ir == 1
+    np.fill_diagonal(ir, False)
+    assert np.abs(cen["G"][1][ir] - b["G"][1][ir]).max() > 1e-3 * np.abs(b["G"][1][ir]).max()
+
+
+def test_point_fields_vanish_for_equal_media():
+    v, f = shapes.icosphere(1, 4.0)
+    v = v + [0, 0, 5.0]
+    sub = L.Substrate(1.77, 1.77, 0.0)
+    pts = np.array([[0.0, 0.0, 10.0], [6.0, 0.0, 5.0]])
+    rg, zg, z2 = L.tab_grid(sub, v, nr=8, nz=8, pts=pts)
+    pg = L.PointGreenLayer(sub, pts, v, f, L.GreenTable(sub, rg, zg, z2))
+    out = L.points_reflected(pg, 600.0, _charges(len(f), 1))
+    for k in out:
+        assert np.abs(out[k]).max() < 1e-12, k
