@@ -10,12 +10,15 @@
 //    workgroup per admissible block runs the partially pivoted ACA of
 //    mex/hlib/aca.cpp:158-218 on the collocation kernel of
 //    mex/acagreen/acagreen.cpp:79-127, in FP64, rank <= min(kmax, m, n);
-//  - k_hmv_low / k_hmv_tile: y = H x with blocks and factors stored in FP64
-//    or FP32 and FP64 accumulation: t_b = R_b^T x per low-rank block (R
-//    stored transposed, read contiguously), then one workgroup per row tile
-//    (<= 32 rows of a leaf) sums its slices of the full blocks times x and
-//    the tile's contiguous L_b segment times t_b over the low-rank blocks
-//    covering its rows (deterministic, no atomics).
+//  - k_hmv_cols / k_hmv_rows: y = H x with blocks and factors stored in
+//    FP64 or FP32 and FP64 accumulation: t = R^T x over the column groups
+//    (the R_b^T of all low-rank blocks of one column cluster side by side,
+//    copied once per fill into contiguous slices, k_hmv_pack_cols), then
+//    one workgroup per row tile
+//    (<= 32 rows of a leaf) streams the tile's packed row panel (its slices
+//    of the full blocks and its L_b segments, copied once per fill into one
+//    contiguous column-major panel, k_hmv_pack) against the rows of x and t
+//    its columns name (deterministic, no atomics).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -324,15 +327,14 @@ __global__ void __launch_bounds__(256) k_aca_store(const int32_t* order, const i
   }
 }
 
-// x (face order, FP64, n x nrhs) -> xc (cluster order, storage precision
-// TS, right-hand sides interleaved: xc[c * nrhs + q])
-template <class TS>
-__global__ void k_to_cluster(int64_t n, int nrhs, const int32_t* perm, const double2* x, cplx<TS>* xc) {
+// x (face order, FP64, n x nrhs) -> xc (cluster order, right-hand sides
+// interleaved: xc[c * nrhs + q]); the rows of t follow in the same buffer
+__global__ void k_to_cluster(int64_t n, int nrhs, const int32_t* perm, const double2* x, double2* xc) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t q = blockIdx.y;
   if (i < n) {
     const double2 v = x[q * n + perm[i]];
-    xc[i * nrhs + q] = mk<TS>((TS)v.x, (TS)v.y);
+    xc[i * nrhs + q] = v;
   }
 }
 __global__ void k_from_cluster(int64_t n, int nrhs, const int32_t* perm, const double2* yc, double2* y) {
@@ -342,7 +344,6 @@ __global__ void k_from_cluster(int64_t n, int nrhs, const int32_t* perm, const d
 }
 
 constexpr int HMV_UNROLL = 8;  // independent loads in flight per lane
-constexpr int HMV_CHUNK = 256;  // columns of R^T per wave in the first phase
 constexpr int HMV_KC = 64;      // right-hand-side rows staged in LDS per wave
 
 // The product kernels are instantiated for NR = 4 or 8 right-hand sides (the
@@ -381,10 +382,10 @@ __device__ __forceinline__ void lds_fma2(double2 (&acc0)[NR], double2 (&acc1)[NR
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
     const double2 b = bs[q];
-    acc0[q].x += ar * b.x - ai * b.y;
-    acc0[q].y += ar * b.y + ai * b.x;
-    acc1[q].x += br * b.x - bi * b.y;
-    acc1[q].y += br * b.y + bi * b.x;
+    acc0[q].x = fma(-ai, b.y, fma(ar, b.x, acc0[q].x));
+    acc0[q].y = fma(ai, b.x, fma(ar, b.y, acc0[q].y));
+    acc1[q].x = fma(-bi, b.y, fma(br, b.x, acc1[q].x));
+    acc1[q].y = fma(bi, b.x, fma(br, b.y, acc1[q].y));
   }
 }
 
@@ -432,111 +433,327 @@ __device__ __forceinline__ void col_product2(double2 (&acc0)[NR], double2 (&acc1
   }
 }
 
-// First phase, t_b = R_b^T x(cols) (r x nrhs, FP64, t[l * nrhs + q]): one
-// wave per item = (block, <= HMV_CHUNK consecutive columns).  RP = the rank
-// rounded up to 16, 32 or 64 (ranks above 64 take several passes of 64) and
-// H = RP / 2; lane = (l, jj) = (lane % H, lane / H) holds the factors l and
-// l + H, so each x value read from LDS serves two factors, and each load
-// instruction reads 64 / H consecutive columns of R^T (H factors each,
-// contiguous in memory); x is staged in LDS HMV_KC columns at a time and the
-// partial sums over jj are combined by shuffles.  Items of blocks with one
-// item write t_b, the others write partial sums that k_hmv_low_sum adds in
-// item order.
-// item[4 i ..] = {block, first column, end column, partial-sum slot or -1}.
+// Buffer loads of one complex value at byte offset voff + soff of a buffer
+// resource: offsets past its size read zero (the hardware bounds check).
+template <class TS>
+__device__ __forceinline__ cplx<TS> bload(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ cplx<float> bload<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  return mk<float>(__uint_as_float(v[0]), __uint_as_float(v[1]));
+}
+template <>
+__device__ __forceinline__ cplx<double> bload<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  return mk<double>(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
+}
+constexpr int BUF_WORD3 = 0x00020000;  // raw (untyped) buffer access on gfx9
+
+// First phase, t = R^T x over the column groups (the low-rank blocks of
+// one column cluster; their factors R_b^T side by side, so that the x rows
+// of the cluster are read once for all of them): the group's factors are
+// cut into slices of <= 64, each stored (columns of the cluster) x nf
+// row-major and contiguous (k_hmv_pack_cols), and t rows follow the groups'
+// factors in order.  One wave per item = (slice, consecutive columns, about
+// 16K entries): meta[4 i ..] = {nf, number of columns nj, first row of x, t
+// row of the slice's first factor or -(1 + partial-sum offset)}, off[i] =
+// the first panel entry.  lane = (l, jj): factors l and l + 32, columns jj,
+// jj + 2, ...; the item is summed in chunks of CCH columns with the
+// software pipeline of k_hmv_rows (the next chunk's panel values and x rows
+// in flight while the current one is summed; the item as a buffer, so
+// reads past its end return zero).  Factors l >= nf read other columns of
+// the slice and are not stored.  The partial sums over jj are combined by
+// a shuffle.  Items of a slice with several items write partial sums (row
+// stride nrhs) that k_hmv_cols_sum adds in item order.
+constexpr int CCH = 16;  // columns per chunk of the first phase
 template <class TS, int NR>
-__global__ void __launch_bounds__(256) k_hmv_low(int64_t nitems, const int32_t* item, const int64_t* tpoff,
-                                                 const int32_t* blk, const int32_t* rank, const int64_t* loff,
-                                                 const int64_t* toff, const cplx<TS>* pool, int nrhs,
-                                                 const cplx<TS>* xc, double2* t, double2* tpart) {
-  __shared__ double2 bsh[4][HMV_KC * NR];
-  const int64_t it = blockIdx.x * 4 + (threadIdx.x >> 6);
+__global__ void __launch_bounds__(256) k_hmv_cols(int64_t nitems, const int64_t* off, const int32_t* meta,
+                                                  const cplx<TS>* rpanel, int nrhs, const double2* xc, double2* t,
+                                                  double2* tpart) {
+  constexpr int U = CCH / 2;          // columns per lane per chunk
+  constexpr int PER = CCH * NR / 64;  // x values per lane per chunk
+  __shared__ double2 bsh[4][CCH * NR];
+  const int64_t it = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   if (it >= nitems) return;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, la = lane & 31, jj = lane >> 5;
   double2* bs = bsh[threadIdx.x >> 6];
-  const int b = item[4 * it], j0 = item[4 * it + 1], j1 = item[4 * it + 2], slot = item[4 * it + 3];
-  const int r0 = blk[4 * b], m = blk[4 * b + 1] - r0, c0 = blk[4 * b + 2];
-  const int r = rank[b];
-  const int RP = r <= 16 ? 16 : r <= 32 ? 32 : 64;
-  const int H = RP / 2, JS = 64 / H;
-  const cplx<TS>* RT = pool + loff[b] + (int64_t)m * r;
-  double2* dst = slot < 0 ? t + toff[b] : tpart + tpoff[it];
-  for (int lb = 0; lb < r; lb += RP) {
-    const int la = lb + lane % H, lh = la + H, jj = lane / H;
-    double2 acc0[NR], acc1[NR];
+  const int nf = __builtin_amdgcn_readfirstlane(meta[4 * it]), nj = __builtin_amdgcn_readfirstlane(meta[4 * it + 1]);
+  const int xr = meta[4 * it + 2], dst = meta[4 * it + 3];
+  const int64_t o64 = off[it];
+  const cplx<TS>* RT = rpanel + (((int64_t)__builtin_amdgcn_readfirstlane((int)(o64 >> 32)) << 32) |
+                                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)o64));
+  constexpr int ES = (int)sizeof(cplx<TS>);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)RT, (short)0, nj * nf * ES, BUF_WORD3);
+  int ou[U];
 #pragma unroll
-    for (int q = 0; q < NR; ++q) acc0[q] = acc1[q] = mk<double>(0, 0);
-    // column j of the item is R^T row (j0 + j): entries at stride r, this lane's factors la, lh
-    const cplx<TS>* RTj = RT + (int64_t)j0 * r;
-    col_product2<TS, cplx<TS>, NR>(acc0, acc1, bs, RTj + min(la, r - 1), RTj + min(lh, r - 1), r, j1 - j0,
-                                   xc + (int64_t)(c0 + j0) * nrhs, nrhs, lane, jj, JS);
-    for (int sh = H; sh < 64; sh <<= 1)
+  for (int u = 0; u < U; ++u) ou[u] = ((jj + 2 * u) * nf + la) * ES;
+  const double2* X = xc + (int64_t)xr * nrhs;
+  double2 acc0[NR], acc1[NR];
 #pragma unroll
-      for (int q = 0; q < NR; ++q) {
-        acc0[q].x += __shfl_xor(acc0[q].x, sh);
-        acc0[q].y += __shfl_xor(acc0[q].y, sh);
-        acc1[q].x += __shfl_xor(acc1[q].x, sh);
-        acc1[q].y += __shfl_xor(acc1[q].y, sh);
-      }
-    if (jj == 0)
+  for (int q = 0; q < NR; ++q) acc0[q] = acc1[q] = mk<double>(0, 0);
+  auto load_panel = [&](int c0, cplx<TS> (&a0)[U], cplx<TS> (&a1)[U]) {
+    const int so = __builtin_amdgcn_readfirstlane(c0 * nf * ES);
 #pragma unroll
-      for (int q = 0; q < NR; ++q)
-        if (q < nrhs) {
-          if (la < r) dst[(int64_t)la * nrhs + q] = acc0[q];
-          if (lh < r) dst[(int64_t)lh * nrhs + q] = acc1[q];
-        }
+    for (int u = 0; u < U; ++u) {
+      a0[u] = bload<TS>(rs, ou[u], so);
+      a1[u] = bload<TS>(rs, ou[u] + 32 * ES, so);
+    }
+  };
+  auto load_x = [&](int c0, double2 (&v)[PER]) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = lane + 64 * i, row = e / NR, q = e % NR;
+      const double2 x = X[(int64_t)min(c0 + row, nj - 1) * nrhs + min(q, nrhs - 1)];
+      v[i] = (c0 + row < nj && q < nrhs) ? x : mk<double>(0, 0);
+    }
+  };
+  auto stage = [&](int c0, const cplx<TS> (&a0)[U], const cplx<TS> (&a1)[U], const double2 (&v)[PER],
+                   cplx<TS> (&b0)[U], cplx<TS> (&b1)[U], double2 (&w)[PER]) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) bs[lane + 64 * i] = v[i];
+    load_x(c0 + CCH, w);
+    load_panel(c0 + CCH, b0, b1);
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < U; ++u) lds_fma2<TS, NR>(acc0, acc1, a0[u], a1[u], bs + (jj + 2 * u) * NR);
+    __builtin_amdgcn_wave_barrier();
+  };
+  {
+    cplx<TS> a0[U], a1[U], b0[U], b1[U];
+    double2 v[PER], w[PER];
+    int c0 = 0;
+    load_x(c0, v);
+    load_panel(c0, a0, a1);
+#pragma unroll 1
+    for (;;) {
+      stage(c0, a0, a1, v, b0, b1, w);
+      c0 += CCH;
+      if (c0 >= nj) break;
+      stage(c0, b0, b1, w, a0, a1, v);
+      c0 += CCH;
+      if (c0 >= nj) break;
+    }
   }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    acc0[q].x += __shfl_xor(acc0[q].x, 32);
+    acc0[q].y += __shfl_xor(acc0[q].y, 32);
+    acc1[q].x += __shfl_xor(acc1[q].x, 32);
+    acc1[q].y += __shfl_xor(acc1[q].y, 32);
+  }
+  double2* d = dst >= 0 ? t + (int64_t)dst * nrhs : tpart + (-(int64_t)dst - 1);
+  if (jj == 0)
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+      if (q < nrhs) {
+        if (la < nf) d[(int64_t)la * nrhs + q] = acc0[q];
+        if (la + 32 < nf) d[(int64_t)(la + 32) * nrhs + q] = acc1[q];
+      }
 }
 
-// t_b = sum of the partial sums of its items, in item order.
-// multi[3 i ..] = {block, first item, number of items}.
-__global__ void k_hmv_low_sum(int64_t nmulti, const int32_t* multi, const int64_t* tpoff, const int32_t* rank,
-                              const int64_t* toff, int nrhs, const double2* tpart, double2* t) {
+// t rows of a slice with several items = the sum of their partial sums, in
+// item order.  multi[4 i ..] = {t row, nf, first partial-sum offset, number
+// of items}; the items' partial sums lie nf * MNP_HMV_MAXRHS apart.
+__global__ void k_hmv_cols_sum(int64_t nmulti, const int32_t* multi, int nrhs, const double2* tpart, double2* t) {
   const int64_t i = blockIdx.x;
   if (i >= nmulti) return;
-  const int b = multi[3 * i], i0 = multi[3 * i + 1], ni = multi[3 * i + 2];
-  const int len = rank[b] * nrhs;
+  const int tr = multi[4 * i], nf = multi[4 * i + 1], p0 = multi[4 * i + 2], ni = multi[4 * i + 3];
+  const int len = nf * nrhs;
   for (int e = threadIdx.x; e < len; e += blockDim.x) {
-    double2 v = tpart[tpoff[i0] + e];
-    for (int k = 1; k < ni; ++k) v = cadd<double>(v, tpart[tpoff[i0 + k] + e]);
-    t[toff[b] + e] = v;
+    double2 v = mk<double>(0, 0);
+    for (int k0 = 0; k0 < ni; k0 += 8) {  // eight loads in flight, summed in item order
+      double2 p[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        p[u] = tpart[p0 + (int64_t)min(k0 + u, ni - 1) * nf * MNP_HMV_MAXRHS + e];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (k0 + u < ni) v = cadd<double>(v, p[u]);
+    }
+    t[(int64_t)tr * nrhs + e] = v;
   }
 }
 
-// Second phase: one workgroup per row tile (<= 32 rows of one leaf) sums
-// the products of its descriptor list, one descriptor per full block (the
-// tile's rows of V times x(cols)) or low-rank block (the tile's contiguous
-// L segment times t_b): A at aoff (rows at stride lda, K columns); boff >= 0
-// is the first column of x, boff < 0 encodes toff as -boff - 1.  The four
-// waves take descriptors round-robin; lane = (rp, kk): rows rp and rp + 16,
-// kk splitting the columns four ways, so one load instruction reads four
-// 16-row column segments and each right-hand-side value read from LDS
-// serves two rows.
-template <class TS, int NR>
-__global__ void __launch_bounds__(256) k_hmv_tile(const int32_t* tilebeg, const int32_t* tdbeg, const int64_t* taoff,
+// rowblk[trow[b] + l] = b for the factors l < r_b of every low-rank block.
+__global__ void k_hmv_rowblk(int64_t nlow, const int64_t* trow, const int32_t* rank, int32_t* rowblk) {
+  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b >= nlow) return;
+  for (int l = 0; l < rank[b]; ++l) rowblk[trow[b] + l] = (int32_t)b;
+}
+
+// Column-group slices, built once per fill: slice i (sl[4 i ..] = {t row of
+// its first factor, nf, first column of the cluster, number of columns},
+// panel entries from soff[i]) holds entry (j, f) = R^T of the block owning t
+// row (first + f), at column j, factor (first + f - trow[b]).  One
+// workgroup per slice; the panel is written contiguously.
+template <class TS>
+__global__ void __launch_bounds__(256) k_hmv_pack_cols(const int32_t* sl, const int64_t* soff, const int32_t* rowblk,
+                                                       const int64_t* trow, const int32_t* blk, const int32_t* rank,
+                                                       const int64_t* loff, const cplx<TS>* pool, cplx<TS>* rpanel) {
+  __shared__ int64_t src[64];
+  __shared__ int stride[64];
+  const int64_t i = blockIdx.x;
+  const int t0 = sl[4 * i], nf = sl[4 * i + 1], w = sl[4 * i + 3];
+  cplx<TS>* P = rpanel + soff[i];
+  if (threadIdx.x < nf) {  // factor f: R^T column of its block, stride r
+    const int f = threadIdx.x, b = rowblk[t0 + f], r = rank[b];
+    const int m = blk[4 * b + 1] - blk[4 * b];
+    src[f] = loff[b] + (int64_t)m * r + (t0 + f - trow[b]);
+    stride[f] = r;
+  }
+  __syncthreads();
+  for (int64_t e = threadIdx.x; e < (int64_t)nf * w; e += blockDim.x) {
+    const int j = (int)(e / nf), f = (int)(e % nf);
+    P[e] = pool[src[f] + (int64_t)j * stride[f]];
+  }
+}
+
+// Row panels, built once per fill: tile T's panel is (rows of T) x K_T,
+// column-major with leading dimension mt (contiguous), at poff[T]; its
+// columns are, in descriptor order, the tile's rows of each full block
+// (column j of the block) and the tile's L_b segment of each low-rank block
+// (factor l).  cidx[tcol[T] + k] is the row of the right-hand-side buffer
+// that column k multiplies: the column of x (rows 0 .. n) or n + the row of
+// t.  Descriptors: A at aoff (rows at stride lda, K columns), boff >= 0 the
+// first column of x, boff < 0 the first row of t as -boff - 1.  One
+// workgroup per tile; source and panel reads and writes are contiguous per
+// column.
+template <class TS>
+__global__ void __launch_bounds__(256) k_hmv_pack(const int32_t* tilebeg, const int32_t* tdbeg, const int64_t* taoff,
                                                   const int64_t* tboff, const int32_t* tlda, const int32_t* tK,
-                                                  const cplx<TS>* fvals, const cplx<TS>* pool, const double2* t,
-                                                  int nrhs, const cplx<TS>* xc, double2* yc) {
+                                                  const int64_t* tcol, const int64_t* poff, const cplx<TS>* fvals,
+                                                  const cplx<TS>* pool, int64_t n, cplx<TS>* panel,
+                                                  int32_t* cidx) {
+  const int tile = blockIdx.x;
+  const int mt = tilebeg[tile + 1] - tilebeg[tile];
+  cplx<TS>* P = panel + poff[tile];
+  int32_t* C = cidx + tcol[tile];
+  int64_t col = 0;
+  const int s1 = tdbeg[tile + 1];
+  for (int s = tdbeg[tile]; s < s1; ++s) {
+    const int64_t aoff = taoff[s], boff = tboff[s];
+    const int lda = tlda[s], K = tK[s];
+    const cplx<TS>* src = (boff >= 0 ? fvals : pool) + aoff;
+    for (int e = threadIdx.x; e < mt * K; e += blockDim.x) {
+      const int i = e % mt, j = e / mt;
+      P[(col + j) * mt + i] = src[(int64_t)j * lda + i];
+    }
+    const int64_t b0 = boff >= 0 ? boff : n + (-boff - 1);
+    for (int j = threadIdx.x; j < K; j += blockDim.x) C[col + j] = (int32_t)(b0 + j);
+    col += K;
+  }
+}
+
+// Second phase: one workgroup per row tile (<= 32 rows of one leaf) streams
+// its row panel against the rows of xt = [x; t] its columns name, and
+// writes the tile's rows of y in face order (y[q n + perm[row]]).  The four
+// waves take chunks of RKC columns round-robin.  Software pipeline per
+// wave: while chunk c is summed, the panel values of chunk c + 1 and its
+// right-hand-side rows (gathered by the indices loaded one chunk earlier)
+// are in flight, with the indices of chunk c + 2.  lane = (rp, kk): rows rp
+// and rp + 16, kk splitting the columns four ways, so one load instruction
+// reads four 16-row column segments and each LDS value read serves two
+// rows.
+constexpr int RKC = 32;  // columns per chunk of the second phase
+
+template <class TS, int NR>
+__global__ void __launch_bounds__(256) k_hmv_rows(const int32_t* tilebeg, const int64_t* tcol, const int64_t* poff,
+                                                  const int32_t* cidx, const cplx<TS>* panel, const double2* xt,
+                                                  int nrhs, const int32_t* perm, int64_t n, double2* y) {
+  constexpr int PER = RKC * NR / 64;  // right-hand-side values per lane per chunk
+  constexpr int CPL = RKC / 4;        // columns per lane per chunk
   __shared__ double2 red[4][32][NR];
-  __shared__ double2 bsh[4][HMV_KC * NR];
+  __shared__ double2 bsh[4][RKC * NR];
   const int tile = blockIdx.x;
   const int tb = tilebeg[tile], mt = tilebeg[tile + 1] - tb;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, rp = lane & 15, kk = lane >> 4;
-  // rows past the tile read valid memory and are not stored
-  const int r0 = min(rp, mt - 1), r1 = min(rp + 16, mt - 1);
+  const int64_t cb = tcol[tile];
+  // wave-uniform by construction; read through readfirstlane so that the
+  // buffer resource below lives in scalar registers
+  const int K = __builtin_amdgcn_readfirstlane((int)(tcol[tile + 1] - cb));
+  const int64_t po = poff[tile];
+  const cplx<TS>* A = panel + (((int64_t)__builtin_amdgcn_readfirstlane((int)(po >> 32)) << 32) |
+                               (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)po));
+  const int32_t* idx = cidx + cb;
+  const int lane = threadIdx.x & 63, rp = lane & 15, kk = lane >> 4;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int mtu = __builtin_amdgcn_readfirstlane(mt);
   double2* bs = bsh[wv];
   double2 acc0[NR], acc1[NR];
 #pragma unroll
   for (int q = 0; q < NR; ++q) acc0[q] = acc1[q] = mk<double>(0, 0);
-  const int s1 = tdbeg[tile + 1];
-  for (int s = tdbeg[tile] + wv; s < s1; s += 4) {
-    const int64_t aoff = taoff[s], boff = tboff[s];
-    const int lda = tlda[s], K = tK[s];
-    if (boff >= 0)
-      col_product2<TS, cplx<TS>, NR>(acc0, acc1, bs, fvals + aoff + r0, fvals + aoff + r1, lda, K,
-                                     xc + boff * nrhs, nrhs, lane, kk, 4);
-    else
-      col_product2<TS, double2, NR>(acc0, acc1, bs, pool + aoff + r0, pool + aoff + r1, lda, K, t + (-boff - 1),
-                                    nrhs, lane, kk, 4);
+  // the tile's panel as a buffer: columns past K read zero; rows past the
+  // tile read the next column (or zero) and are not stored
+  constexpr int ES = (int)sizeof(cplx<TS>);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)A, (short)0, K * mtu * ES, BUF_WORD3);
+  int ou[CPL];
+#pragma unroll
+  for (int u = 0; u < CPL; ++u) ou[u] = ((kk + 4 * u) * mtu + rp) * ES;
+  // the tile's column sources as a buffer too: indices past K read zero
+  const __amdgpu_buffer_rsrc_t ri = __builtin_amdgcn_make_buffer_rsrc((void*)idx, (short)0, K * 4, BUF_WORD3);
+  int oi[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) oi[i] = ((lane + 64 * i) / NR) * 4;
+  // every load below is unconditional (out-of-range columns read zero and
+  // gather row 0), so the loop body is straight-line code
+  auto load_ids = [&](int k0, int (&id)[PER]) {
+    const int so = __builtin_amdgcn_readfirstlane(k0 * 4);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) id[i] = (int)__builtin_amdgcn_raw_buffer_load_b32(ri, oi[i], so, 0);
+  };
+  auto load_panel = [&](int k0, cplx<TS> (&a0)[CPL], cplx<TS> (&a1)[CPL]) {
+    const int so = __builtin_amdgcn_readfirstlane(k0 * mtu * ES);
+#pragma unroll
+    for (int u = 0; u < CPL; ++u) {
+      a0[u] = bload<TS>(rs, ou[u], so);
+      a1[u] = bload<TS>(rs, ou[u] + 16 * ES, so);
+    }
+  };
+  auto load_rhs = [&](int k0, const int (&id)[PER], double2 (&v)[PER]) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = lane + 64 * i, row = e / NR, q = e % NR;
+      const double2 x = xt[(int64_t)id[i] * nrhs + min(q, nrhs - 1)];
+      v[i] = (k0 + row < K && q < nrhs) ? x : mk<double>(0, 0);
+    }
+  };
+  const int step = 4 * RKC;
+  // one pipeline stage: stage chunk k0's right-hand sides in LDS; put in
+  // flight the indices of chunk k0 + 2 step, the right-hand sides of chunk
+  // k0 + step (by the indices loaded one stage earlier) and its panel
+  // values, in the order in which the next stage waits for them; sum chunk
+  // k0.  The register
+  // sets alternate between stages, so no copy waits on a load in flight.
+  auto stage = [&](int k0, const cplx<TS> (&a0)[CPL], const cplx<TS> (&a1)[CPL], const double2 (&v)[PER],
+                   cplx<TS> (&b0)[CPL], cplx<TS> (&b1)[CPL], double2 (&w)[PER], const int (&idc)[PER],
+                   int (&idf)[PER]) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) bs[lane + 64 * i] = v[i];
+    load_ids(k0 + 2 * step, idf);
+    load_rhs(k0 + step, idc, w);
+    load_panel(k0 + step, b0, b1);
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < CPL; ++u) lds_fma2<TS, NR>(acc0, acc1, a0[u], a1[u], bs + (kk + 4 * u) * NR);
+    __builtin_amdgcn_wave_barrier();
+  };
+  int k0 = wv * RKC;
+  if (k0 < K) {
+    int id0[PER], id1[PER];
+    cplx<TS> a0[CPL], a1[CPL], b0[CPL], b1[CPL];
+    double2 v[PER], w[PER];
+    load_ids(k0, id0);
+    load_ids(k0 + step, id1);
+    load_rhs(k0, id0, v);
+    load_panel(k0, a0, a1);
+#pragma unroll 1
+    for (;;) {
+      stage(k0, a0, a1, v, b0, b1, w, id1, id0);
+      k0 += step;
+      if (k0 >= K) break;
+      stage(k0, b0, b1, w, a0, a1, v, id0, id1);
+      k0 += step;
+      if (k0 >= K) break;
+    }
   }
 #pragma unroll
   for (int q = 0; q < NR; ++q)
@@ -557,7 +774,7 @@ __global__ void __launch_bounds__(256) k_hmv_tile(const int32_t* tilebeg, const 
     const int i = e / nrhs, q = e % nrhs;
     double2 v = red[0][i][q];
     for (int w2 = 1; w2 < 4; ++w2) v = cadd<double>(v, red[w2][i][q]);
-    yc[(int64_t)(tb + i) * nrhs + q] = v;
+    y[(int64_t)q * n + perm[tb + i]] = v;
   }
 }
 
@@ -688,11 +905,11 @@ HMat::HMat(const GreenRet* g_, int prec_, const int64_t* perm_h, int64_t nfull_,
 }
 
 HMat::~HMat() {
-  void* ps[] = {d_items, d_itpoff, d_multi, d_tdbeg, d_taoff, d_tboff, d_tlda, d_tK,
+  void* ps[] = {d_ioff, d_imeta, d_multi, d_slice, d_soff, d_rowblk, rpanel, d_tdbeg, d_taoff, d_tboff, d_tlda, d_tK,
                 perm, fblk, lblk, d_voff, d_lrow, d_loffs, d_lentry, d_tilebeg, d_tileleaf, d_rowtile,
                 cpos, cnvec, carea, rank,
                 d_loff, d_toff, fvals, pool, d_aca_items, d_aca_order, d_aca_soff, d_dsel, d_dlb, d_droff,
-                d_citem, d_ccoff};
+                d_citem, d_ccoff, d_tcol, d_poff, d_cidx, panel};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -728,7 +945,7 @@ void HMat::fill_impl(double2 k, int key_) {
   // ACA over the work plan, in batches whose FP64 scratch fits the budget
   const int isF = key == MNP_KEY_G ? 0 : 1;
   std::vector<int32_t> rank_h(nlow, 0);
-  std::vector<int64_t> loff_h(nlow + 1, 0), toff_h(nlow + 1, 0);
+  std::vector<int64_t> loff_h(nlow + 1, 0);
   if (nlow) {
     if (aca_icut.empty()) aca_plan();
     // scratch shared by every H-matrix fill (grown on demand, kept until
@@ -768,16 +985,24 @@ void HMat::fill_impl(double2 k, int key_) {
                          d_aca_soff, scratch, rank, kmax, d_loff, d_rowtile, d_tilebeg, (cplx<TS>*)pool);
       MNP_HIP(hipGetLastError());  // a next batch reuses the scratch after this store, in stream order
     }
-    for (int64_t b = 0; b < nlow; ++b) toff_h[b + 1] = toff_h[b] + rank_h[b] * (int64_t)MNP_HMV_MAXRHS;
   }
-  MNP_HIP(hipMemcpyAsync(d_toff, toff_h.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice, s));
   nlowvals = loff_h[nlow];
   maxrank = 0;
   for (int32_t r : rank_h) maxrank = std::max(maxrank, (int)r);
   rank_host = rank_h;
   loff_host = loff_h;
-  tlen = toff_h[nlow];
-  build_plan(toff_h);
+  build_plan();
+  if (nslice) {
+    hipLaunchKernelGGL(k_hmv_rowblk, dim3((unsigned)cdiv_up(nlow, 256)), dim3(256), 0, s, nlow, d_toff, rank,
+                       d_rowblk);
+    hipLaunchKernelGGL(k_hmv_pack_cols<TS>, dim3((unsigned)nslice), dim3(256), 0, s, d_slice, d_soff, d_rowblk,
+                       d_toff, lblk, rank, d_loff, (const cplx<TS>*)pool, (cplx<TS>*)rpanel);
+  }
+  if (ntile)
+    hipLaunchKernelGGL(k_hmv_pack<TS>, dim3((unsigned)ntile), dim3(256), 0, s, d_tilebeg, d_tdbeg, d_taoff, d_tboff,
+                       d_tlda, d_tK, d_tcol, d_poff, (const cplx<TS>*)fvals, (const cplx<TS>*)pool, n,
+                       (cplx<TS>*)panel, d_cidx);
+  MNP_HIP(hipGetLastError());
   filled = true;
 }
 
@@ -883,21 +1108,25 @@ __global__ void k_plan_low(int64_t nq, const int32_t* dsel, const int32_t* dlb, 
 }
 
 void HMat::build_plan_static() {
-  std::vector<int32_t> items, multi;
-  for (int64_t b = 0; b < nlow; ++b) {
-    const int32_t c0 = lblk_h[4 * b + 2], w = lblk_h[4 * b + 3] - c0;
-    const int32_t ni = (w + HMV_CHUNK - 1) / HMV_CHUNK;
-    if (ni > 1) multi.insert(multi.end(), {(int32_t)b, (int32_t)(items.size() / 4), ni});
-    for (int32_t c = 0; c < ni; ++c) {
-      const int32_t j0 = c * w / ni, j1 = (c + 1) * w / ni;
-      items.insert(items.end(), {(int32_t)b, j0, j1, ni > 1 ? (int32_t)c : -1});
-    }
+  // column groups: the low-rank blocks of each column cluster, in block order
+  {
+    std::vector<int32_t> order(nlow);
+    for (int64_t b = 0; b < nlow; ++b) order[b] = (int32_t)b;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+      return std::make_pair(lblk_h[4 * a + 2], lblk_h[4 * a + 3]) < std::make_pair(lblk_h[4 * b + 2], lblk_h[4 * b + 3]);
+    });
+    grp_beg_h.assign(1, 0);
+    grp_blk_h = order;
+    for (int64_t i = 1; i <= nlow; ++i)
+      if (i == nlow || lblk_h[4 * order[i] + 2] != lblk_h[4 * order[i - 1] + 2] ||
+          lblk_h[4 * order[i] + 3] != lblk_h[4 * order[i - 1] + 3])
+        grp_beg_h.push_back((int32_t)i);
+    if (nlow == 0) grp_beg_h.assign(1, 0);
   }
-  nitems = (int64_t)items.size() / 4;
-  nmulti = (int64_t)multi.size() / 3;
-  items_h = items;
   std::vector<int32_t> tdbeg{0}, tlda, tK, dsel, dlb, droff;
   std::vector<int64_t> taoff, tboff;
+  tkfull_h.assign(ntile, 0);
+  dtile_h.clear();
   for (int64_t t = 0; t < ntile; ++t) {
     const int32_t tb = tilebeg_h[t], mt = tilebeg_h[t + 1] - tb, leaf = tileleaf_h[t];
     for (int64_t s = loffs_h[leaf]; s < loffs_h[leaf + 1]; ++s) {
@@ -908,8 +1137,10 @@ void HMat::build_plan_static() {
         tboff.push_back(bk[2]);
         tlda.push_back(bk[1] - bk[0]);
         tK.push_back(bk[3] - bk[2]);
+        tkfull_h[t] += bk[3] - bk[2];
       } else {
         const int32_t lb = -ent - 1;
+        dtile_h.push_back((int32_t)t);
         dsel.push_back((int32_t)taoff.size());
         dlb.push_back(lb);
         droff.push_back(tb - lblk_h[4 * lb]);
@@ -927,8 +1158,6 @@ void HMat::build_plan_static() {
     dptr = dev_alloc<E>((int64_t)vec.size());
     if (!vec.empty()) MNP_HIP(hipMemcpy(dptr, vec.data(), sizeof(E) * vec.size(), hipMemcpyHostToDevice));
   };
-  up(items, d_items);
-  up(multi, d_multi);
   up(tdbeg, d_tdbeg);
   up(taoff, d_taoff);
   up(tboff, d_tboff);
@@ -936,31 +1165,122 @@ void HMat::build_plan_static() {
   up(tK, d_tK);
   up(dsel, d_dsel);
   up(dlb, d_dlb);
+  lowdesc_block_h = dlb;
   up(droff, d_droff);
-  d_itpoff = dev_alloc<int64_t>(std::max<int64_t>(nitems, 1));
-  itpoff_h.assign(nitems, -1);
+  d_tcol = dev_alloc<int64_t>(ntile + 1);
+  d_poff = dev_alloc<int64_t>(ntile + 1);
 }
 
 // Rank-dependent part of the plan, on the stream after the fill's uploads
-// of loff and toff.
-void HMat::build_plan(const std::vector<int64_t>& toff_h) {
-  (void)toff_h;
+// of loff and toff: the low-rank descriptors, the partial-sum offsets of the
+// first phase, the row panels' columns and offsets (panels and column
+// sources grown on demand).
+void HMat::build_plan() {
   if (!d_tdbeg) build_plan_static();
-  int64_t tp = 0;
-  for (int64_t i = 0; i < nitems; ++i) {
-    if (items_h[4 * i + 3] < 0) continue;
-    itpoff_h[i] = tp;
-    tp += (int64_t)rank_host[items_h[4 * i]] * MNP_HMV_MAXRHS;
-  }
-  tpartlen = tp;
   hipStream_t s = stream();
-  if (nitems)
-    MNP_HIP(hipMemcpyAsync(d_itpoff, itpoff_h.data(), sizeof(int64_t) * nitems, hipMemcpyHostToDevice, s));
+  // t rows in column-group order; slices of <= 64 factors per group; items
+  // of <= jch columns per slice, with partial sums where a slice has several
+  toff_h.assign(nlow + 1, 0);
+  slice_h.clear();
+  soff_h.assign(1, 0);
+  ioff_h.clear();
+  imeta_h.clear();
+  multi_h.clear();
+  {
+    int64_t tr = 0, tp = 0;
+    const int64_t ngrp = (int64_t)grp_beg_h.size() - 1;
+    for (int64_t g = 0; g < ngrp; ++g) {
+      const int64_t t0 = tr;
+      for (int32_t i = grp_beg_h[g]; i < grp_beg_h[g + 1]; ++i) {
+        const int32_t b = grp_blk_h[i];
+        toff_h[b] = tr;
+        tr += rank_host[b];
+      }
+      if (tr == t0) continue;
+      const int32_t b0 = grp_blk_h[grp_beg_h[g]];
+      const int32_t c0 = lblk_h[4 * b0 + 2], w = lblk_h[4 * b0 + 3] - c0;
+      for (int64_t f0 = t0; f0 < tr; f0 += 64) {
+        const int32_t nf = (int32_t)std::min<int64_t>(64, tr - f0);
+        const int64_t base = soff_h.back();
+        slice_h.insert(slice_h.end(), {(int32_t)f0, nf, c0, w});
+        soff_h.push_back(base + (int64_t)nf * w);
+        const int32_t jch = std::max(64, std::min(512, (16384 / nf) / 64 * 64));
+        const int32_t nj = (w + jch - 1) / jch;
+        if (nj > 1) multi_h.insert(multi_h.end(), {(int32_t)f0, nf, (int32_t)tp, nj});
+        for (int32_t c = 0; c < nj; ++c) {
+          const int32_t j0 = (int32_t)((int64_t)c * w / nj), j1 = (int32_t)((int64_t)(c + 1) * w / nj);
+          ioff_h.push_back(base + (int64_t)j0 * nf);
+          imeta_h.insert(imeta_h.end(), {nf, j1 - j0, c0 + j0, nj > 1 ? (int32_t)(-1 - tp) : (int32_t)f0});
+          if (nj > 1) tp += (int64_t)nf * MNP_HMV_MAXRHS;
+        }
+      }
+    }
+    toff_h[nlow] = tr;
+    trows = tr;
+    tpartlen = tp;
+    MNP_CHECK(tp < ((int64_t)1 << 31), MNP_ERR_ARG, "hmat: too many partial sums");
+  }
+  nslice = (int64_t)slice_h.size() / 4;
+  nitems = (int64_t)ioff_h.size();
+  nmulti = (int64_t)multi_h.size() / 4;
+  auto grow = [&](auto*& ptr, int64_t& cap, int64_t need) {
+    using E = typename std::remove_reference<decltype(*ptr)>::type;
+    if (need <= cap) return;
+    if (ptr) MNP_HIP(hipFree(ptr));
+    cap = need + need / 8;
+    ptr = dev_alloc<E>(cap);
+  };
+  grow(d_slice, slicecap, 4 * std::max<int64_t>(nslice, 1));
+  grow(d_soff, soffcap, nslice + 1);
+  grow(d_ioff, ioffcap, std::max<int64_t>(nitems, 1));
+  grow(d_imeta, imetacap, 4 * std::max<int64_t>(nitems, 1));
+  grow(d_multi, multicap, 4 * std::max<int64_t>(nmulti, 1));
+  grow(d_rowblk, rowblkcap, std::max<int64_t>(trows, 1));
+  if (soff_h.back() > rpanelcap) {
+    if (rpanel) MNP_HIP(hipFree(rpanel));
+    rpanelcap = soff_h.back() + soff_h.back() / 8;
+    rpanel = prec == MNP_FP64 ? (void*)dev_alloc<double2>(rpanelcap) : (void*)dev_alloc<float2>(rpanelcap);
+  }
+  MNP_HIP(hipMemcpyAsync(d_toff, toff_h.data(), sizeof(int64_t) * (nlow + 1), hipMemcpyHostToDevice, s));
+  if (nslice) {
+    MNP_HIP(hipMemcpyAsync(d_slice, slice_h.data(), sizeof(int32_t) * slice_h.size(), hipMemcpyHostToDevice, s));
+    MNP_HIP(hipMemcpyAsync(d_soff, soff_h.data(), sizeof(int64_t) * soff_h.size(), hipMemcpyHostToDevice, s));
+    MNP_HIP(hipMemcpyAsync(d_ioff, ioff_h.data(), sizeof(int64_t) * ioff_h.size(), hipMemcpyHostToDevice, s));
+    MNP_HIP(hipMemcpyAsync(d_imeta, imeta_h.data(), sizeof(int32_t) * imeta_h.size(), hipMemcpyHostToDevice, s));
+  }
+  if (nmulti)
+    MNP_HIP(hipMemcpyAsync(d_multi, multi_h.data(), sizeof(int32_t) * multi_h.size(), hipMemcpyHostToDevice, s));
+  tcol_h.assign(ntile + 1, 0);
+  poff_h.assign(ntile + 1, 0);
+  {
+    std::vector<int64_t> kt(tkfull_h);
+    int64_t q = 0;
+    for (int64_t t = 0; t < ntile; ++t) {
+      // low-rank descriptors of tile t are consecutive in dtile_h
+      for (; q < (int64_t)dtile_h.size() && dtile_h[q] == t; ++q) kt[t] += rank_host[lowdesc_block_h[q]];
+      const int64_t mt = tilebeg_h[t + 1] - tilebeg_h[t];
+      tcol_h[t + 1] = tcol_h[t] + kt[t];
+      poff_h[t + 1] = poff_h[t] + mt * kt[t];
+    }
+  }
+  if (poff_h[ntile] > panelcap) {
+    if (panel) MNP_HIP(hipFree(panel));
+    panelcap = poff_h[ntile] + poff_h[ntile] / 8;
+    panel = prec == MNP_FP64 ? (void*)dev_alloc<double2>(panelcap) : (void*)dev_alloc<float2>(panelcap);
+  }
+  if (tcol_h[ntile] > cidxcap) {
+    if (d_cidx) MNP_HIP(hipFree(d_cidx));
+    cidxcap = tcol_h[ntile] + tcol_h[ntile] / 8;
+    d_cidx = dev_alloc<int32_t>(cidxcap);
+  }
+  MNP_CHECK(n + trows < ((int64_t)1 << 31), MNP_ERR_ARG, "hmat: too many rows for the column sources");
   if (nlowdesc)
     hipLaunchKernelGGL(k_plan_low, dim3((unsigned)cdiv_up(nlowdesc, 256)), dim3(256), 0, s, nlowdesc, d_dsel, d_dlb,
                        d_droff, rank, d_loff, d_toff, d_taoff, d_tboff, d_tK);
   MNP_HIP(hipGetLastError());
-  MNP_HIP(hipStreamSynchronize(s));  // itpoff_h stays valid until the copy is done
+  MNP_HIP(hipMemcpyAsync(d_tcol, tcol_h.data(), sizeof(int64_t) * (ntile + 1), hipMemcpyHostToDevice, s));
+  MNP_HIP(hipMemcpyAsync(d_poff, poff_h.data(), sizeof(int64_t) * (ntile + 1), hipMemcpyHostToDevice, s));
+  MNP_HIP(hipStreamSynchronize(s));  // the host vectors stay valid until the copies are done
 }
 
 void HMat::matvec(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const {
@@ -980,32 +1300,30 @@ void HMat::matvec_impl(int64_t nrhs, const double2* x, double2* y, Workspace& ws
   TimerScope ts(MNP_TIMER_HMATVEC, mvbytes, mvbytes);
   for (int64_t q0 = 0; q0 < nrhs; q0 += MNP_HMV_MAXRHS) {
     const int nq = (int)std::min<int64_t>(MNP_HMV_MAXRHS, nrhs - q0);
-    cplx<TS>* xc = (cplx<TS>*)ws.get(0, sizeof(cplx<TS>) * n * nq);
+    // xt = [x in cluster order; t], row stride nq
+    double2* xc = (double2*)ws.get(0, sizeof(double2) * (n + std::max<int64_t>(trows, 1)) * nq);
+    double2* t = xc + n * nq;
     double2* yc = (double2*)ws.get(1, sizeof(double2) * n * nq);
-    double2* t = (double2*)ws.get(2, sizeof(double2) * std::max<int64_t>(tlen, 1));
     double2* tpart = (double2*)ws.get(3, sizeof(double2) * std::max<int64_t>(tpartlen, 1));
     const dim3 gv((unsigned)cdiv_up(n, 256), (unsigned)nq);
-    hipLaunchKernelGGL(k_to_cluster<TS>, gv, dim3(256), 0, s, n, nq, perm, x + q0 * n, xc);
+    hipLaunchKernelGGL(k_to_cluster, gv, dim3(256), 0, s, n, nq, perm, x + q0 * n, xc);
     if (nitems) {
       const dim3 g((unsigned)cdiv_up(nitems, 4));
       if (nq <= 4)
-        hipLaunchKernelGGL((k_hmv_low<TS, 4>), g, dim3(256), 0, s, nitems, d_items, d_itpoff, lblk, rank, d_loff,
-                           d_toff, (const cplx<TS>*)pool, nq, xc, t, tpart);
+        hipLaunchKernelGGL((k_hmv_cols<TS, 4>), g, dim3(256), 0, s, nitems, d_ioff, d_imeta,
+                           (const cplx<TS>*)rpanel, nq, xc, t, tpart);
       else
-        hipLaunchKernelGGL((k_hmv_low<TS, MNP_HMV_MAXRHS>), g, dim3(256), 0, s, nitems, d_items, d_itpoff, lblk,
-                           rank, d_loff, d_toff, (const cplx<TS>*)pool, nq, xc, t, tpart);
+        hipLaunchKernelGGL((k_hmv_cols<TS, MNP_HMV_MAXRHS>), g, dim3(256), 0, s, nitems, d_ioff, d_imeta,
+                           (const cplx<TS>*)rpanel, nq, xc, t, tpart);
     }
     if (nmulti)
-      hipLaunchKernelGGL(k_hmv_low_sum, dim3((unsigned)nmulti), dim3(128), 0, s, nmulti, d_multi, d_itpoff, rank,
-                         d_toff, nq, tpart, t);
+      hipLaunchKernelGGL(k_hmv_cols_sum, dim3((unsigned)nmulti), dim3(128), 0, s, nmulti, d_multi, nq, tpart, t);
     if (nq <= 4)
-      hipLaunchKernelGGL((k_hmv_tile<TS, 4>), dim3((unsigned)ntile), dim3(256), 0, s, d_tilebeg, d_tdbeg, d_taoff,
-                         d_tboff, d_tlda, d_tK, (const cplx<TS>*)fvals, (const cplx<TS>*)pool, t, nq, xc, yc);
+      hipLaunchKernelGGL((k_hmv_rows<TS, 4>), dim3((unsigned)ntile), dim3(256), 0, s, d_tilebeg, d_tcol, d_poff,
+                         d_cidx, (const cplx<TS>*)panel, xc, nq, perm, n, y + q0 * n);
     else
-      hipLaunchKernelGGL((k_hmv_tile<TS, MNP_HMV_MAXRHS>), dim3((unsigned)ntile), dim3(256), 0, s, d_tilebeg,
-                         d_tdbeg, d_taoff, d_tboff, d_tlda, d_tK, (const cplx<TS>*)fvals, (const cplx<TS>*)pool, t,
-                         nq, xc, yc);
-    hipLaunchKernelGGL(k_from_cluster, gv, dim3(256), 0, s, n, nq, perm, yc, y + q0 * n);
+      hipLaunchKernelGGL((k_hmv_rows<TS, MNP_HMV_MAXRHS>), dim3((unsigned)ntile), dim3(256), 0, s, d_tilebeg, d_tcol,
+                         d_poff, d_cidx, (const cplx<TS>*)panel, xc, nq, perm, n, y + q0 * n);
     MNP_HIP(hipGetLastError());
   }
 }

@@ -21,27 +21,43 @@ struct HMat {
   std::vector<int32_t> fblk_h, lblk_h, diag_h;
   std::vector<int64_t> lrow_h;
   std::vector<int32_t> rank_host;
-  int64_t nleaf = 0, ntile = 0, nfullvals = 0, nlowvals = 0, poolcap = 0, tlen = 0;
+  int64_t nleaf = 0, ntile = 0, nfullvals = 0, nlowvals = 0, poolcap = 0, trows = 0;
   int maxrank = 0;
   int32_t *perm = nullptr, *fblk = nullptr, *lblk = nullptr, *rank = nullptr, *d_lentry = nullptr;
   int32_t *d_tilebeg = nullptr, *d_tileleaf = nullptr, *d_rowtile = nullptr;  // row tiles of <= 32 rows
+  // d_toff: first row of each low-rank block's t_b in the product's t
   int64_t *d_voff = nullptr, *d_lrow = nullptr, *d_loffs = nullptr, *d_loff = nullptr, *d_toff = nullptr;
   double *cpos = nullptr, *cnvec = nullptr, *carea = nullptr;
-  // product plan (rebuilt by each fill): first-phase items, row-tile descriptors
+  // product plan: row-tile descriptors (static); column groups (static) and
+  // their slices and first-phase items (per fill)
   std::vector<int32_t> lentry_h, tilebeg_h, tileleaf_h, rowtile_h;
   std::vector<int64_t> loffs_h, voff_h;
-  int64_t nitems = 0, nmulti = 0, tpartlen = 0;
-  int32_t *d_items = nullptr, *d_multi = nullptr, *d_tdbeg = nullptr, *d_tlda = nullptr, *d_tK = nullptr;
-  int64_t *d_itpoff = nullptr, *d_taoff = nullptr, *d_tboff = nullptr;
+  int32_t *d_tdbeg = nullptr, *d_tlda = nullptr, *d_tK = nullptr;
+  int64_t *d_taoff = nullptr, *d_tboff = nullptr;
+  std::vector<int32_t> grp_beg_h, grp_blk_h;  // blocks of each column group (CSR)
+  std::vector<int32_t> slice_h, imeta_h, multi_h;
+  std::vector<int64_t> toff_h, soff_h, ioff_h;
+  int64_t nslice = 0, nitems = 0, nmulti = 0, tpartlen = 0;
+  int64_t slicecap = 0, soffcap = 0, ioffcap = 0, imetacap = 0, multicap = 0, rowblkcap = 0, rpanelcap = 0;
+  int32_t *d_slice = nullptr, *d_imeta = nullptr, *d_multi = nullptr, *d_rowblk = nullptr;
+  int64_t *d_soff = nullptr, *d_ioff = nullptr;
+  void* rpanel = nullptr;  // column-group slices of R^T (k_hmv_pack_cols)
   int32_t *d_dsel = nullptr, *d_dlb = nullptr, *d_droff = nullptr;  // low-rank descriptors (k_plan_low)
   int64_t nlowdesc = 0;
+  // row panels (k_hmv_pack, per fill): per tile the columns of its full
+  // blocks (static) and the tile of each low-rank descriptor; per fill the
+  // tiles' first columns and panel offsets, the column sources, the panels
+  std::vector<int64_t> tkfull_h, tcol_h, poff_h;
+  std::vector<int32_t> dtile_h, lowdesc_block_h;
+  int64_t *d_tcol = nullptr, *d_poff = nullptr;
+  int32_t* d_cidx = nullptr;
+  void* panel = nullptr;
+  int64_t panelcap = 0, cidxcap = 0;
   // coarsening of low-rank blocks (coarsen): (block, tile) items and the
   // blocks' first coarse entries, built at the first call
   mutable int32_t* d_citem = nullptr;
   mutable int64_t* d_ccoff = nullptr;
   mutable int64_t ncitem = 0, ncentry = 0;
-  std::vector<int32_t> items_h;
-  std::vector<int64_t> itpoff_h;
   void* fvals = nullptr;  // full blocks, column-major per block, complex (prec)
   void* pool = nullptr;   // per low-rank block: L (m x r, row-tile segments) then R^T (r x n), complex (prec)
 
@@ -62,7 +78,7 @@ struct HMat {
 
  private:
   void build_plan_static();
-  void build_plan(const std::vector<int64_t>& toff_h);
+  void build_plan();
   void aca_plan();
   template <class TS>
   void fill_impl(double2 k, int key);
