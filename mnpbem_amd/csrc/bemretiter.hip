@@ -366,7 +366,7 @@ struct BemRetIter {
   bool ready = false;
   int last_iter = 0, last_flag = 0;
   double last_relres = 0, last_true_relres = 0;
-  Workspace ws, hws;
+  Workspace ws, hws, mws[4];  // mws: one per product of the batched H-matrix products
 
   BemRetIter(const GreenRet* g_, int prec_, const int64_t* perm, int64_t nfull, const int64_t* full,
              int64_t nlow, const int64_t* low, double htol, int kmax)
@@ -497,10 +497,11 @@ struct BemRetIter {
   // y = A x (afun.m), x, y n x 8p
   void afun(int64_t p, const double2* x, double2* y, double2* prod) {
     double2 *GX1 = prod, *GX2 = prod + 4 * p * n, *HX1 = prod + 8 * p * n, *HX2 = prod + 12 * p * n;
-    G1->matvec(4 * p, x, GX1, hws);
-    G2->matvec(4 * p, x + 4 * p * n, GX2, hws);
-    H1->matvec(4 * p, x, HX1, hws);
-    H2->matvec(4 * p, x + 4 * p * n, HX2, hws);
+    const HMat* A[4] = {G1.get(), G2.get(), H1.get(), H2.get()};
+    const double2* xs[4] = {x, x + 4 * p * n, x, x + 4 * p * n};
+    double2* ys[4] = {GX1, GX2, HX1, HX2};
+    Workspace* w[4] = {&mws[0], &mws[1], &mws[2], &mws[3]};
+    HMat::matvec_batch(4, A, 4 * p, xs, ys, w);
     launch_residual(n, p, nullptr, GX1, GX2, HX1, HX2, y, g->geo.nvec, k0, eps1, eps2);  // y = -A x
     const int64_t len = 8 * p * n;
     scale_vec(len, y, -1.0);

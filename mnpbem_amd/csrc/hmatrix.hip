@@ -327,20 +327,47 @@ __global__ void __launch_bounds__(256) k_aca_store(const int32_t* order, const i
   }
 }
 
-// x (face order, FP64, n x nrhs) -> xc (cluster order, right-hand sides
-// interleaved: xc[c * nrhs + q]); the rows of t follow in the same buffer
-__global__ void k_to_cluster(int64_t n, int nrhs, const int32_t* perm, const double2* x, double2* xc) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t q = blockIdx.y;
-  if (i < n) {
-    const double2 v = x[q * n + perm[i]];
-    xc[i * nrhs + q] = v;
-  }
+// Products of one batched launch (up to HMV_MAXB H-matrices of one size and
+// storage precision, e.g. G1, G2, H1, H2 of the iterative solver): per
+// product m its plan and buffers; work units (first-phase items, partial
+// sums, row tiles) of product m are [beg[m], beg[m + 1]) of the launch.
+constexpr int HMV_MAXB = 4;
+struct HmvBatch {
+  int nm;
+  int64_t n;
+  int64_t ibeg[HMV_MAXB + 1], mbeg[HMV_MAXB + 1], tbeg[HMV_MAXB + 1];
+  const int32_t* perm[HMV_MAXB];
+  const double2* x[HMV_MAXB];  // face order, n x nrhs
+  double2* y[HMV_MAXB];        // face order, n x nrhs
+  double2* xt[HMV_MAXB];       // [x in cluster order; t], row stride nrhs
+  double2* tpart[HMV_MAXB];
+  const int64_t* ioff[HMV_MAXB];
+  const int32_t* imeta[HMV_MAXB];
+  const int32_t* multi[HMV_MAXB];
+  const void* rpanel[HMV_MAXB];
+  const int32_t* tilebeg[HMV_MAXB];
+  const int64_t* tcol[HMV_MAXB];
+  const int64_t* poff[HMV_MAXB];
+  const int32_t* cidx[HMV_MAXB];
+  const void* panel[HMV_MAXB];
+  int64_t trow0[HMV_MAXB];  // first row of t in xt (= n)
+};
+
+// Product of work unit u (wave-uniform): the last m with beg[m] <= u.
+__device__ __forceinline__ int batch_of(const int64_t* beg, int nm, int64_t u) {
+  int m = 0;
+  while (m + 1 < nm && u >= beg[m + 1]) ++m;
+  return m;
 }
-__global__ void k_from_cluster(int64_t n, int nrhs, const int32_t* perm, const double2* yc, double2* y) {
+
+// x (face order, FP64, n x nrhs) -> xt (cluster order, right-hand sides
+// interleaved: xt[c * nrhs + q]); the rows of t follow in the same buffer.
+// blockIdx.z = product.
+__global__ void k_to_cluster(HmvBatch B, int nrhs) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t q = blockIdx.y;
-  if (i < n) y[q * n + perm[i]] = yc[i * nrhs + q];
+  const int m = blockIdx.z;
+  if (i < B.n) B.xt[m][i * nrhs + q] = B.x[m][q * B.n + B.perm[m][i]];
 }
 
 constexpr int HMV_UNROLL = 8;  // independent loads in flight per lane
@@ -447,6 +474,22 @@ __device__ __forceinline__ cplx<double> bload<double>(__amdgpu_buffer_rsrc_t r, 
   const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
   return mk<double>(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
 }
+// Two consecutive complex values (one 16-byte load for FP32 storage).
+template <class TS>
+__device__ __forceinline__ void bload2(__amdgpu_buffer_rsrc_t r, int voff, int soff, cplx<TS>& c0, cplx<TS>& c1);
+template <>
+__device__ __forceinline__ void bload2<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff, cplx<float>& c0,
+                                              cplx<float>& c1) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  c0 = mk<float>(__uint_as_float(v[0]), __uint_as_float(v[1]));
+  c1 = mk<float>(__uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+template <>
+__device__ __forceinline__ void bload2<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff, cplx<double>& c0,
+                                               cplx<double>& c1) {
+  c0 = bload<double>(r, voff, soff);
+  c1 = bload<double>(r, voff + 16, soff);
+}
 constexpr int BUF_WORD3 = 0x00020000;  // raw (untyped) buffer access on gfx9
 
 // First phase, t = R^T x over the column groups (the low-rank blocks of
@@ -457,26 +500,40 @@ constexpr int BUF_WORD3 = 0x00020000;  // raw (untyped) buffer access on gfx9
 // factors in order.  One wave per item = (slice, consecutive columns, about
 // 16K entries): meta[4 i ..] = {nf, number of columns nj, first row of x, t
 // row of the slice's first factor or -(1 + partial-sum offset)}, off[i] =
-// the first panel entry.  lane = (l, jj): factors l and l + 32, columns jj,
-// jj + 2, ...; the item is summed in chunks of CCH columns with the
+// the first panel entry.  lane = (l, jj): factors 2 l and 2 l + 1 (one
+// 16-byte load for FP32 storage), columns jj, jj + 2, ...; the item is summed in chunks of CCH columns with the
 // software pipeline of k_hmv_rows (the next chunk's panel values and x rows
 // in flight while the current one is summed; the item as a buffer, so
-// reads past its end return zero).  Factors l >= nf read other columns of
+// reads past its end return zero).  Factors >= nf read other columns of
 // the slice and are not stored.  The partial sums over jj are combined by
 // a shuffle.  Items of a slice with several items write partial sums (row
 // stride nrhs) that k_hmv_cols_sum adds in item order.
+// Workgroup b -> work unit, each of the eight XCDs (to which workgroups are
+// dispatched round-robin) taking a contiguous range of units, so that
+// neighbouring units, which read the same right-hand-side rows, share an
+// L2.  The grid has 8 ceil(nunits / 8) workgroups; units >= nunits exit.
+__device__ __forceinline__ int64_t xcd_unit(int64_t b, int64_t nunits) {
+  const int64_t per = (nunits + 7) / 8;
+  return (b % 8) * per + b / 8;
+}
+
 constexpr int CCH = 16;  // columns per chunk of the first phase
 template <class TS, int NR>
-__global__ void __launch_bounds__(256) k_hmv_cols(int64_t nitems, const int64_t* off, const int32_t* meta,
-                                                  const cplx<TS>* rpanel, int nrhs, const double2* xc, double2* t,
-                                                  double2* tpart) {
+__global__ void __launch_bounds__(64) k_hmv_cols(HmvBatch B, int nrhs) {
   constexpr int U = CCH / 2;          // columns per lane per chunk
   constexpr int PER = CCH * NR / 64;  // x values per lane per chunk
-  __shared__ double2 bsh[4][CCH * NR];
-  const int64_t it = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  if (it >= nitems) return;
+  __shared__ double2 bs[CCH * NR];
+  const int64_t u0 = xcd_unit(blockIdx.x, B.ibeg[B.nm]);  // one wave per workgroup: items end independently
+  if (u0 >= B.ibeg[B.nm]) return;
+  const int mb = batch_of(B.ibeg, B.nm, u0);
+  const int64_t it = u0 - B.ibeg[mb];
+  const int64_t* off = B.ioff[mb];
+  const int32_t* meta = B.imeta[mb];
+  const cplx<TS>* rpanel = (const cplx<TS>*)B.rpanel[mb];
+  const double2* xc = B.xt[mb];
+  double2* t = B.xt[mb] + B.trow0[mb] * nrhs;
+  double2* tpart = B.tpart[mb];
   const int lane = threadIdx.x & 63, la = lane & 31, jj = lane >> 5;
-  double2* bs = bsh[threadIdx.x >> 6];
   const int nf = __builtin_amdgcn_readfirstlane(meta[4 * it]), nj = __builtin_amdgcn_readfirstlane(meta[4 * it + 1]);
   const int xr = meta[4 * it + 2], dst = meta[4 * it + 3];
   const int64_t o64 = off[it];
@@ -486,7 +543,7 @@ __global__ void __launch_bounds__(256) k_hmv_cols(int64_t nitems, const int64_t*
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)RT, (short)0, nj * nf * ES, BUF_WORD3);
   int ou[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) ou[u] = ((jj + 2 * u) * nf + la) * ES;
+  for (int u = 0; u < U; ++u) ou[u] = ((jj + 2 * u) * nf + 2 * la) * ES;
   const double2* X = xc + (int64_t)xr * nrhs;
   double2 acc0[NR], acc1[NR];
 #pragma unroll
@@ -494,10 +551,7 @@ __global__ void __launch_bounds__(256) k_hmv_cols(int64_t nitems, const int64_t*
   auto load_panel = [&](int c0, cplx<TS> (&a0)[U], cplx<TS> (&a1)[U]) {
     const int so = __builtin_amdgcn_readfirstlane(c0 * nf * ES);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      a0[u] = bload<TS>(rs, ou[u], so);
-      a1[u] = bload<TS>(rs, ou[u] + 32 * ES, so);
-    }
+    for (int u = 0; u < U; ++u) bload2<TS>(rs, ou[u], so, a0[u], a1[u]);
   };
   auto load_x = [&](int c0, double2 (&v)[PER]) {
 #pragma unroll
@@ -546,17 +600,22 @@ __global__ void __launch_bounds__(256) k_hmv_cols(int64_t nitems, const int64_t*
 #pragma unroll
     for (int q = 0; q < NR; ++q)
       if (q < nrhs) {
-        if (la < nf) d[(int64_t)la * nrhs + q] = acc0[q];
-        if (la + 32 < nf) d[(int64_t)(la + 32) * nrhs + q] = acc1[q];
+        if (2 * la < nf) d[(int64_t)(2 * la) * nrhs + q] = acc0[q];
+        if (2 * la + 1 < nf) d[(int64_t)(2 * la + 1) * nrhs + q] = acc1[q];
       }
 }
 
 // t rows of a slice with several items = the sum of their partial sums, in
 // item order.  multi[4 i ..] = {t row, nf, first partial-sum offset, number
 // of items}; the items' partial sums lie nf * MNP_HMV_MAXRHS apart.
-__global__ void k_hmv_cols_sum(int64_t nmulti, const int32_t* multi, int nrhs, const double2* tpart, double2* t) {
-  const int64_t i = blockIdx.x;
-  if (i >= nmulti) return;
+__global__ void k_hmv_cols_sum(HmvBatch B, int nrhs) {
+  const int64_t u0 = blockIdx.x;
+  if (u0 >= B.mbeg[B.nm]) return;
+  const int mb = batch_of(B.mbeg, B.nm, u0);
+  const int64_t i = u0 - B.mbeg[mb];
+  const int32_t* multi = B.multi[mb];
+  const double2* tpart = B.tpart[mb];
+  double2* t = B.xt[mb] + B.trow0[mb] * nrhs;
   const int tr = multi[4 * i], nf = multi[4 * i + 1], p0 = multi[4 * i + 2], ni = multi[4 * i + 3];
   const int len = nf * nrhs;
   for (int e = threadIdx.x; e < len; e += blockDim.x) {
@@ -650,21 +709,31 @@ __global__ void __launch_bounds__(256) k_hmv_pack(const int32_t* tilebeg, const 
 // waves take chunks of RKC columns round-robin.  Software pipeline per
 // wave: while chunk c is summed, the panel values of chunk c + 1 and its
 // right-hand-side rows (gathered by the indices loaded one chunk earlier)
-// are in flight, with the indices of chunk c + 2.  lane = (rp, kk): rows rp
-// and rp + 16, kk splitting the columns four ways, so one load instruction
-// reads four 16-row column segments and each LDS value read serves two
-// rows.
+// are in flight, with the indices of chunk c + 2.  lane = (rp, kk): rows
+// 2 rp and 2 rp + 1 (one 16-byte load for FP32 storage), kk splitting the
+// columns four ways, so one load instruction reads four 32-row column
+// segments and each LDS value read serves two rows.
 constexpr int RKC = 32;  // columns per chunk of the second phase
 
 template <class TS, int NR>
-__global__ void __launch_bounds__(256) k_hmv_rows(const int32_t* tilebeg, const int64_t* tcol, const int64_t* poff,
-                                                  const int32_t* cidx, const cplx<TS>* panel, const double2* xt,
-                                                  int nrhs, const int32_t* perm, int64_t n, double2* y) {
+__global__ void __launch_bounds__(256) k_hmv_rows(HmvBatch B, int nrhs) {
   constexpr int PER = RKC * NR / 64;  // right-hand-side values per lane per chunk
   constexpr int CPL = RKC / 4;        // columns per lane per chunk
   __shared__ double2 red[4][32][NR];
   __shared__ double2 bsh[4][RKC * NR];
-  const int tile = blockIdx.x;
+  const int64_t u0 = xcd_unit(blockIdx.x, B.tbeg[B.nm]);
+  if (u0 >= B.tbeg[B.nm]) return;
+  const int mb = batch_of(B.tbeg, B.nm, u0);
+  const int64_t tile = u0 - B.tbeg[mb];
+  const int32_t* tilebeg = B.tilebeg[mb];
+  const int64_t* tcol = B.tcol[mb];
+  const int64_t* poff = B.poff[mb];
+  const int32_t* cidx = B.cidx[mb];
+  const cplx<TS>* panel = (const cplx<TS>*)B.panel[mb];
+  const double2* xt = B.xt[mb];
+  const int32_t* perm = B.perm[mb];
+  const int64_t n = B.n;
+  double2* y = B.y[mb];
   const int tb = tilebeg[tile], mt = tilebeg[tile + 1] - tb;
   const int64_t cb = tcol[tile];
   // wave-uniform by construction; read through readfirstlane so that the
@@ -687,7 +756,7 @@ __global__ void __launch_bounds__(256) k_hmv_rows(const int32_t* tilebeg, const 
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)A, (short)0, K * mtu * ES, BUF_WORD3);
   int ou[CPL];
 #pragma unroll
-  for (int u = 0; u < CPL; ++u) ou[u] = ((kk + 4 * u) * mtu + rp) * ES;
+  for (int u = 0; u < CPL; ++u) ou[u] = ((kk + 4 * u) * mtu + 2 * rp) * ES;
   // the tile's column sources as a buffer too: indices past K read zero
   const __amdgpu_buffer_rsrc_t ri = __builtin_amdgcn_make_buffer_rsrc((void*)idx, (short)0, K * 4, BUF_WORD3);
   int oi[PER];
@@ -703,10 +772,7 @@ __global__ void __launch_bounds__(256) k_hmv_rows(const int32_t* tilebeg, const 
   auto load_panel = [&](int k0, cplx<TS> (&a0)[CPL], cplx<TS> (&a1)[CPL]) {
     const int so = __builtin_amdgcn_readfirstlane(k0 * mtu * ES);
 #pragma unroll
-    for (int u = 0; u < CPL; ++u) {
-      a0[u] = bload<TS>(rs, ou[u], so);
-      a1[u] = bload<TS>(rs, ou[u] + 16 * ES, so);
-    }
+    for (int u = 0; u < CPL; ++u) bload2<TS>(rs, ou[u], so, a0[u], a1[u]);
   };
   auto load_rhs = [&](int k0, const int (&id)[PER], double2 (&v)[PER]) {
 #pragma unroll
@@ -766,8 +832,8 @@ __global__ void __launch_bounds__(256) k_hmv_rows(const int32_t* tilebeg, const 
   if (kk == 0)
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
-      red[wv][rp][q] = acc0[q];
-      red[wv][rp + 16][q] = acc1[q];
+      red[wv][2 * rp][q] = acc0[q];
+      red[wv][2 * rp + 1][q] = acc1[q];
     }
   __syncthreads();
   for (int e = threadIdx.x; e < mt * nrhs; e += blockDim.x) {
@@ -1284,46 +1350,77 @@ void HMat::build_plan() {
 }
 
 void HMat::matvec(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const {
-  if (prec == MNP_FP64)
-    matvec_impl<double>(nrhs, x, y, ws);
+  const HMat* self = this;
+  Workspace* w = &ws;
+  matvec_batch(1, &self, nrhs, &x, &y, &w);
+}
+
+void HMat::matvec_batch(int nm, const HMat* const* A, int64_t nrhs, const double2* const* x, double2* const* y,
+                        Workspace* const* ws) {
+  MNP_CHECK(nm >= 1 && nm <= HMV_MAXB, MNP_ERR_ARG, "hmat: 1 to 4 products per batch");
+  for (int m = 0; m < nm; ++m) {
+    MNP_CHECK(A[m]->filled, MNP_ERR_STATE, "hmat: matvec before fill");
+    MNP_CHECK(A[m]->n == A[0]->n && A[m]->prec == A[0]->prec, MNP_ERR_ARG,
+              "hmat: batched products need one size and one storage precision");
+  }
+  MNP_CHECK(nrhs >= 1, MNP_ERR_ARG, "hmat: nrhs < 1");
+  if (A[0]->prec == MNP_FP64)
+    matvec_batch_impl<double>(nm, A, nrhs, x, y, ws);
   else
-    matvec_impl<float>(nrhs, x, y, ws);
+    matvec_batch_impl<float>(nm, A, nrhs, x, y, ws);
 }
 
 template <class TS>
-void HMat::matvec_impl(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const {
-  MNP_CHECK(filled, MNP_ERR_STATE, "hmat: matvec before fill");
-  MNP_CHECK(nrhs >= 1, MNP_ERR_ARG, "hmat: nrhs < 1");
+void HMat::matvec_batch_impl(int nm, const HMat* const* A, int64_t nrhs, const double2* const* x,
+                             double2* const* y, Workspace* const* ws) {
   hipStream_t s = stream();
-  const double mvbytes = (double)sizeof(cplx<TS>) * (double)(nfullvals + nlowvals) *
-                         (double)((nrhs + MNP_HMV_MAXRHS - 1) / MNP_HMV_MAXRHS);
+  const int64_t n = A[0]->n;
+  double vals = 0;
+  for (int m = 0; m < nm; ++m) vals += (double)(A[m]->nfullvals + A[m]->nlowvals);
+  const double mvbytes = (double)sizeof(cplx<TS>) * vals * (double)((nrhs + MNP_HMV_MAXRHS - 1) / MNP_HMV_MAXRHS);
   TimerScope ts(MNP_TIMER_HMATVEC, mvbytes, mvbytes);
   for (int64_t q0 = 0; q0 < nrhs; q0 += MNP_HMV_MAXRHS) {
     const int nq = (int)std::min<int64_t>(MNP_HMV_MAXRHS, nrhs - q0);
-    // xt = [x in cluster order; t], row stride nq
-    double2* xc = (double2*)ws.get(0, sizeof(double2) * (n + std::max<int64_t>(trows, 1)) * nq);
-    double2* t = xc + n * nq;
-    double2* yc = (double2*)ws.get(1, sizeof(double2) * n * nq);
-    double2* tpart = (double2*)ws.get(3, sizeof(double2) * std::max<int64_t>(tpartlen, 1));
-    const dim3 gv((unsigned)cdiv_up(n, 256), (unsigned)nq);
-    hipLaunchKernelGGL(k_to_cluster, gv, dim3(256), 0, s, n, nq, perm, x + q0 * n, xc);
-    if (nitems) {
-      const dim3 g((unsigned)cdiv_up(nitems, 4));
-      if (nq <= 4)
-        hipLaunchKernelGGL((k_hmv_cols<TS, 4>), g, dim3(256), 0, s, nitems, d_ioff, d_imeta,
-                           (const cplx<TS>*)rpanel, nq, xc, t, tpart);
-      else
-        hipLaunchKernelGGL((k_hmv_cols<TS, MNP_HMV_MAXRHS>), g, dim3(256), 0, s, nitems, d_ioff, d_imeta,
-                           (const cplx<TS>*)rpanel, nq, xc, t, tpart);
+    HmvBatch B{};
+    B.nm = nm;
+    B.n = n;
+    for (int m = 0; m < nm; ++m) {
+      const HMat& H = *A[m];
+      // xt = [x in cluster order; t], row stride nq
+      B.xt[m] = (double2*)ws[m]->get(0, sizeof(double2) * (n + std::max<int64_t>(H.trows, 1)) * nq);
+      B.tpart[m] = (double2*)ws[m]->get(3, sizeof(double2) * std::max<int64_t>(H.tpartlen, 1));
+      B.trow0[m] = n;
+      B.perm[m] = H.perm;
+      B.x[m] = x[m] + q0 * n;
+      B.y[m] = y[m] + q0 * n;
+      B.ioff[m] = H.d_ioff;
+      B.imeta[m] = H.d_imeta;
+      B.multi[m] = H.d_multi;
+      B.rpanel[m] = H.rpanel;
+      B.tilebeg[m] = H.d_tilebeg;
+      B.tcol[m] = H.d_tcol;
+      B.poff[m] = H.d_poff;
+      B.cidx[m] = H.d_cidx;
+      B.panel[m] = H.panel;
+      B.ibeg[m + 1] = B.ibeg[m] + H.nitems;
+      B.mbeg[m + 1] = B.mbeg[m] + H.nmulti;
+      B.tbeg[m + 1] = B.tbeg[m] + H.ntile;
     }
-    if (nmulti)
-      hipLaunchKernelGGL(k_hmv_cols_sum, dim3((unsigned)nmulti), dim3(128), 0, s, nmulti, d_multi, nq, tpart, t);
+    hipLaunchKernelGGL(k_to_cluster, dim3((unsigned)cdiv_up(n, 256), (unsigned)nq, (unsigned)nm), dim3(256), 0, s, B,
+                       nq);
+    if (const int64_t ni = B.ibeg[nm]) {
+      const dim3 g((unsigned)(8 * cdiv_up(ni, 8)));
+      if (nq <= 4)
+        hipLaunchKernelGGL((k_hmv_cols<TS, 4>), g, dim3(64), 0, s, B, nq);
+      else
+        hipLaunchKernelGGL((k_hmv_cols<TS, MNP_HMV_MAXRHS>), g, dim3(64), 0, s, B, nq);
+    }
+    if (const int64_t nmu = B.mbeg[nm]) hipLaunchKernelGGL(k_hmv_cols_sum, dim3((unsigned)nmu), dim3(128), 0, s, B, nq);
+    const dim3 gt((unsigned)(8 * cdiv_up(B.tbeg[nm], 8)));
     if (nq <= 4)
-      hipLaunchKernelGGL((k_hmv_rows<TS, 4>), dim3((unsigned)ntile), dim3(256), 0, s, d_tilebeg, d_tcol, d_poff,
-                         d_cidx, (const cplx<TS>*)panel, xc, nq, perm, n, y + q0 * n);
+      hipLaunchKernelGGL((k_hmv_rows<TS, 4>), gt, dim3(256), 0, s, B, nq);
     else
-      hipLaunchKernelGGL((k_hmv_rows<TS, MNP_HMV_MAXRHS>), dim3((unsigned)ntile), dim3(256), 0, s, d_tilebeg, d_tcol,
-                         d_poff, d_cidx, (const cplx<TS>*)panel, xc, nq, perm, n, y + q0 * n);
+      hipLaunchKernelGGL((k_hmv_rows<TS, MNP_HMV_MAXRHS>), gt, dim3(256), 0, s, B, nq);
     MNP_HIP(hipGetLastError());
   }
 }

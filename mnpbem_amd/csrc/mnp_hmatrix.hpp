@@ -70,6 +70,10 @@ struct HMat {
   void fill(double2 k, int key);
   // y = H x, x and y complex FP64 in face order (n x nrhs).
   void matvec(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const;
+  // y[m] = A[m] x[m] for m < nm <= 4 H-matrices of one size and storage
+  // precision, each phase one launch for all of them; ws[m] per product.
+  static void matvec_batch(int nm, const HMat* const* A, int64_t nrhs, const double2* const* x, double2* const* y,
+                           Workspace* const* ws);
   // C = P^T H P (ntile x ntile, column-major, complex FP64) for P the
   // indicator of the row tiles (<= 32 consecutive cluster indices each):
   // entry (s, t) sums the operator over the rows of tile s and the columns
@@ -83,7 +87,8 @@ struct HMat {
   template <class TS>
   void fill_impl(double2 k, int key);
   template <class TS>
-  void matvec_impl(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const;
+  static void matvec_batch_impl(int nm, const HMat* const* A, int64_t nrhs, const double2* const* x,
+                                double2* const* y, Workspace* const* ws);
   template <class TS>
   void coarsen_impl(double2* C, Workspace& ws) const;
   std::vector<int64_t> loff_host;  // low-rank factor offsets of the last fill

@@ -59,9 +59,12 @@ class CompGreenRet:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib.can_release():
-            _lib.lib().mnp_greenret_destroy(h)
-            self.handle = None
+        try:
+            if h is not None and h.value and _lib.can_release():
+                _lib.lib().mnp_greenret_destroy(h)
+                self.handle = None
+        except Exception:  # interpreter shutdown: module globals may be gone
+            pass
 
     # ------------------------------------------------------------ geometry
     def geometry(self):

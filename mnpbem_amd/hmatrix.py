@@ -134,7 +134,7 @@ class hmatrix:
     def __del__(self):
         try:
             from . import _lib
-        except ImportError:  # interpreter shutdown
+        except Exception:  # interpreter shutdown: the module may be gone
             return
         h = getattr(self, "handle", None)
         if h is not None and h.value and _lib.can_release():
