@@ -631,6 +631,53 @@ def run_hmatrix_dipole(nlam=16, seed=2024, coarse=2):
     }
 
 
+def run_hmatrix_product(nprod=50):
+    """The H-matrix product alone, for timing and hardware counters: G of the
+    81,920-face sphere (icosphere level 6, d = 150 nm) at 600 nm, cleaf 32,
+    htol 1e-6, FP32 storage, 4 right-hand sides, `nprod` products; beside it
+    the device's plain streaming rates (torch sum over 2 GiB: the read
+    calibration of FETCH_SIZE; fill of 2 GiB: the write calibration of
+    WRITE_SIZE; copy of 2 GiB)."""
+    import torch
+
+    from mnpbem_amd import bem as B, shapes
+    from mnpbem_amd import hmatrix as H
+
+    def timed(fn, reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    x = torch.ones(1 << 28, dtype=torch.float64, device="cuda")  # 2 GiB
+    y = torch.empty_like(x)
+    gib2 = float(1 << 31)
+    read_ms = timed(lambda: x.sum(), 10)
+    fill_ms = timed(lambda: y.fill_(2.0), 3)
+    copy_ms = timed(lambda: y.copy_(x), 10)
+    del x, y
+    v, f = shapes.icosphere(6, 75.0)
+    p = B.comparticle([B.epsconst(1.0), B.epsdrude(*DRUDE)], [(v, f)], [2, 1], 1, B.bemoptions(sim="ret"))
+    g = p.green
+    tree = H.clustertree(g.geometry()["pos"], cleaf=32)
+    hm = H.hmatrix(g, tree, htol=1e-6, kmax=100, precision="fp32").fill(2 * np.pi / 600.0, "G")
+    st = hm.stats()
+    nbytes = 8.0 * (st["full_entries"] + st["lowrank_entries"])
+    xr = torch.randn(g.n, 4, dtype=torch.complex128, device="cuda")
+    mv_ms = timed(lambda: hm.matvec(xr), nprod)
+    log(f"[bench] H-matrix product: {mv_ms:.3f} ms for {nbytes / 1e9:.3f} GB")
+    return {"workload": "G of the 81920-face sphere at 600 nm, cleaf 32, htol 1e-6, FP32 storage, 4 right-hand sides",
+            "ms_per_product": mv_ms, "bytes_per_product": nbytes, "gb_per_s": nbytes / (mv_ms * 1e-3) / 1e9,
+            "stream_read_gb_per_s": gib2 / (read_ms * 1e-3) / 1e9,
+            "stream_fill_gb_per_s": gib2 / (fill_ms * 1e-3) / 1e9,
+            "stream_copy_gb_per_s": 2 * gib2 / (copy_ms * 1e-3) / 1e9}
+
+
 def cpu_baseline_hmatrix(iterations, compression, nsample=60, seed=7):
     """Oracle port of the H-matrix path on the host for a bounded sample,
     scaled to one wavelength of the 81,920-face workload: partially pivoted
@@ -775,7 +822,7 @@ def main():
                 "substrate": lambda: run_substrate(a.level), "substrate_ret": run_substrate_ret,
                 "multi_particle": run_multi_particle,
                 "sensitivity": run_sensitivity, "batch_sweep": run_batch_sweep,
-                "hmatrix_dipole": run_hmatrix_dipole}
+                "hmatrix_dipole": run_hmatrix_dipole, "hmatrix_product": run_hmatrix_product}
         names = [x.strip() for x in a.only.split(",") if x.strip()]
         bad = [x for x in names if x not in runs]
         if bad:

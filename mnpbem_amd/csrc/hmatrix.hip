@@ -498,7 +498,7 @@ constexpr int BUF_WORD3 = 0x00020000;  // raw (untyped) buffer access on gfx9
 // cut into slices of <= 64, each stored (columns of the cluster) x nf
 // row-major and contiguous (k_hmv_pack_cols), and t rows follow the groups'
 // factors in order.  One wave per item = (slice, consecutive columns, about
-// 32K entries): meta[4 i ..] = {nf, number of columns nj, first row of x, t
+// 16K entries): meta[4 i ..] = {nf, number of columns nj, first row of x, t
 // row of the slice's first factor or -(1 + partial-sum offset)}, off[i] =
 // the first panel entry.  lane = (l, jj): factors 2 l and 2 l + 1 (one
 // 16-byte load for FP32 storage), columns jj, jj + 2, ...; the item is summed in chunks of CCH columns with the
@@ -1272,7 +1272,7 @@ void HMat::build_plan() {
         const int64_t base = soff_h.back();
         slice_h.insert(slice_h.end(), {(int32_t)f0, nf, c0, w});
         soff_h.push_back(align_up(base + (int64_t)nf * w, 32));  // slices start on 256-byte boundaries
-        const int32_t jch = std::max(64, std::min(1024, (32768 / nf) / 64 * 64));
+        const int32_t jch = std::max(64, std::min(512, (16384 / nf) / 64 * 64));
         const int32_t nj = (w + jch - 1) / jch;
         if (nj > 1) multi_h.insert(multi_h.end(), {(int32_t)f0, nf, (int32_t)tp, nj});
         for (int32_t c = 0; c < nj; ++c) {
