@@ -349,6 +349,14 @@ int mnp_hmat_create(mnp_greenret_t g, int precision, const int64_t* perm, int64_
 int mnp_hmat_destroy(mnp_hmat_t h);
 /* Fills G, F, H1 (F + 2 pi) or H2 (F - 2 pi) at wavenumber k (key MNP_KEY_*). */
 int mnp_hmat_fill(mnp_hmat_t h, mnp_complex k, int key);
+/* Component comp (p, ss, hs, sh, hh = 0..4) of the outside Green function
+ * above a substrate as an H-matrix (Greenfun/+aca/@compgreenretlayer/
+ * private/eval2.m for g{2,2}): the tabulated reflected G (key MNP_KEY_G) or
+ * F (MNP_KEY_H2) of gl at enei, with the refined near pairs in the full
+ * blocks, plus, if direct != 0, the direct part at wavenumber k (with the
+ * -2 pi of H2); ACA on the sum of the two in the admissible blocks. */
+int mnp_hmat_fill_layer(mnp_hmat_t h, mnp_complex k, int key, int comp, int direct, struct mnp_greenretlayer_s* gl,
+                        double enei, mnp_complex eps_up, mnp_complex eps_down);
 /* y = H x (device, complex128, face order, n x nrhs). */
 int mnp_hmat_matvec(mnp_hmat_t h, int64_t nrhs, const void* x, void* y);
 /* Stored entries of full and low-rank blocks, largest rank and compression
@@ -398,6 +406,25 @@ int mnp_bemretiter_set_options(mnp_bemretiter_t it, double tol, int maxit, int p
  * of private/initprecond.m). */
 int mnp_bemretiter_set_coarse(mnp_bemretiter_t it, int tiles_per_aggregate);
 int mnp_bemretiter_initmat(mnp_bemretiter_t it, double enei, mnp_complex eps1, mnp_complex eps2);
+/* Iterative solver above a substrate (bemsolver with op.layer and iterative
+ * options -> BEM/@bemretlayeriter: private/initmat.m:1-42, afun.m:1-47,
+ * excitation.m, solve.m; the H-matrices of the components p, ss, hs, sh, hh
+ * of Greenfun/+aca/@compgreenretlayer/private/eval2.m, lowrank1.m,
+ * lowrank2.m with mex/hmatgreentab1.cpp, hmatgreentab2.cpp,
+ * acagreen/greentab.cpp): gl the particle's reflected Green function
+ * (mnp_greenretlayer_create, its table covering the particle); the other
+ * arguments as mnp_bemretiter_create.  ACA runs on the sum of the direct and
+ * the tabulated reflected kernel where the reference compresses the two
+ * separately and truncates their concatenation; the preconditioner is that
+ * of mnp_bemretiter_set_options with the p components in place of G2, H2.
+ * Options, solve, stats and destroy as for mnp_bemretiter_*. */
+int mnp_bemretiter_create_layer(mnp_greenret_t g, struct mnp_greenretlayer_s* gl, int precision, const int64_t* perm,
+                                int64_t nfull, const int64_t* full, int64_t nlow, const int64_t* low,
+                                double htol, int kmax, mnp_bemretiter_t* out);
+/* eps_in inside the particle, eps_up (the particle's outside) above and
+ * eps_down below the interface (bemretlayeriter/private/initmat.m). */
+int mnp_bemretiter_initmat_layer(mnp_bemretiter_t it, double enei, mnp_complex eps_in, mnp_complex eps_up,
+                                 mnp_complex eps_down);
 /* Same arguments as mnp_bemret_solve (npol <= 8). */
 int mnp_bemretiter_solve(mnp_bemretiter_t it, int64_t npol, const void* phi1, const void* phi1p,
                          const void* a1, const void* a1p, const void* phi2, const void* phi2p,

@@ -90,6 +90,7 @@ PROTOTYPES = {
     "mnp_hmat_create": (INT, [P, INT, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
     "mnp_hmat_destroy": (INT, [P]),
     "mnp_hmat_fill": (INT, [P, MnpComplex, INT]),
+    "mnp_hmat_fill_layer": (INT, [P, MnpComplex, INT, INT, INT, P, D, MnpComplex, MnpComplex]),
     "mnp_hmat_matvec": (INT, [P, I64, P, P]),
     "mnp_hmat_stats": (INT, [P, PI64, PI64, C.POINTER(INT), PD]),
     "mnp_hmat_ranks": (INT, [P, P]),
@@ -99,6 +100,8 @@ PROTOTYPES = {
     "mnp_bemretiter_set_coarse": (INT, [P, INT]),
     "mnp_bemretiter_set_options": (INT, [P, D, INT, INT]),
     "mnp_bemretiter_initmat": (INT, [P, D, MnpComplex, MnpComplex]),
+    "mnp_bemretiter_create_layer": (INT, [P, P, INT, P, I64, P, I64, P, D, INT, C.POINTER(P)]),
+    "mnp_bemretiter_initmat_layer": (INT, [P, D, MnpComplex, MnpComplex, MnpComplex]),
     "mnp_bemretiter_solve": (INT, [P, I64, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mnp_bemretiter_stats": (INT, [P, C.POINTER(INT), PD, C.POINTER(INT), PD, PD]),
     "mnp_bemstatlayer_create": (INT, [P, P, INT, C.POINTER(P)]),

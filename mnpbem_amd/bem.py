@@ -409,9 +409,7 @@ class bemretiter:
         htol = float(np.min(np.atleast_1d(it["htol"])))
         h = C.c_void_p()
         prec = {"fp64": _lib.MNP_FP64, "fp32": _lib.MNP_FP32}[it["hprecision"]]
-        _lib.call(self._PREFIX + "_create", p.green.handle, prec, perm.ctypes.data_as(C.c_void_p), len(fr),
-                  fr.ctypes.data_as(C.c_void_p), len(lr), lr.ctypes.data_as(C.c_void_p), htol, kmax,
-                  C.byref(h))
+        self._create(prec, perm, fr, lr, htol, kmax, h)
         self.handle = h
         _lib.call(self._PREFIX + "_set_options", h, float(it["tol"]), int(it["maxit"]),
                   self._precond_code(it["precond"]))
@@ -419,6 +417,11 @@ class bemretiter:
             # row tiles per aggregate of the two-level preconditioner's coarse level
             _lib.call("mnp_bemretiter_set_coarse", h, int(it["coarse"]))
         self.enei = None
+
+    def _create(self, prec, perm, fr, lr, htol, kmax, h):
+        _lib.call(self._PREFIX + "_create", self.p.green.handle, prec, perm.ctypes.data_as(C.c_void_p), len(fr),
+                  fr.ctypes.data_as(C.c_void_p), len(lr), lr.ctypes.data_as(C.c_void_p), htol, kmax,
+                  C.byref(h))
 
     # precond: 'hmat' (the bemiter default) couples the whole surface: the
     # reference's rank-4 H-matrix chain, here the two-level preconditioner
@@ -775,6 +778,9 @@ def bemsolver(p, op=None, **kw):
             return bemstatlayer(p, op)
         return bemstatiter(p, op) if op.get("iter") else bemstat(p, op)
     if op.get("layer") is not None:
+        if op.get("iter") and sim == "ret":
+            from .layer import bemretlayeriter
+            return bemretlayeriter(p, op)
         if op.get("iter") or sim != "ret":
             raise NotImplementedError("substrate: the dense quasistatic and retarded solvers are built")
         from .layer import bemretlayer

@@ -23,12 +23,21 @@
 //   z = P y + B^{-1} (v - A P y),  y = C^{-1} P^T v.
 // precond 1 keeps the smoother alone.  The preconditioner changes the
 // iteration count only; the converged solution solves the same equations.
+//
+// Substrate (bemretlayeriter, BEM/@bemretlayeriter/private/initmat.m,
+// afun.m:1-47): G2 and H2 become the five components p, ss, hs, sh, hh of
+// the outside Green function with the interface (Greenfun/+aca/
+// @compgreenretlayer/private/eval2.m: direct + reflected for p, ss, hh,
+// reflected only for hs, sh), each an H-matrix of its own; afun takes the
+// structured products.  The preconditioner uses the p components in place
+// of G2 and H2 (the free-space chain with the parallel reflection).
 #include <complex>
 #include <memory>
 #include <vector>
 
 #include "mnp_chain.hpp"
 #include "mnp_gmres.hpp"
+#include "mnp_layertab.hpp"
 #include "mnp_hmatrix.hpp"
 
 namespace mnp {
@@ -342,6 +351,51 @@ __global__ void k_c64_to_c32(int64_t len, const double2* a, float2* b) {
 }
 
 // ------------------------------------------------------------ solver
+// X2 = [sig2 (p) | h2perp (p) | h2par (2p, c + 2q)] from x = [sig1 | h1 | sig2
+// | h2] (n x 8p, h columns c + 3q), the inputs of the substrate components.
+__global__ void k_layit_gather(int64_t n, int64_t p, const double2* x, double2* X2) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  X2[q * n + i] = x[(4 * p + q) * n + i];
+  X2[(p + q) * n + i] = x[(5 * p + 2 + 3 * q) * n + i];
+  for (int c = 0; c < 2; ++c) X2[(2 * p + c + 2 * q) * n + i] = x[(5 * p + c + 3 * q) * n + i];
+}
+
+// y = A x with the substrate (bemretlayeriter/private/afun.m:3-46) from the
+// products GX1 = G1 [s1 | h1], HX1 = H1 [s1 | h1] (n x 4p) and, for G2 and H2,
+// Css = C.ss s2, Chs = C.hs s2, Csh = C.sh h2perp, Chh = C.hh h2perp (p
+// columns each) and Cp = C.p h2par (2p); layout of y as b: phi | a | De |
+// alpha.
+__global__ void k_layit_apply(int64_t n, int64_t p, const double2* GX1, const double2* HX1, const double2* Gss,
+                              const double2* Ghs, const double2* Gsh, const double2* Ghh, const double2* Gp,
+                              const double2* Hss, const double2* Hhs, const double2* Hsh, const double2* Hhh,
+                              const double2* Hp, const double* nvec, double k, double2 e1, double2 e2, double2* y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t q = blockIdx.y;
+  if (i >= n) return;
+  const double2 ik = mk<double>(0, k);
+  const int64_t s = q * n + i;
+  const double2 Gs1 = GX1[s], Hs1 = HX1[s];
+  const double2 Gs2 = cadd<double>(Gss[s], Gsh[s]), Hs2 = cadd<double>(Hss[s], Hsh[s]);
+  const double2 Gperp2 = cadd<double>(Ghh[s], Ghs[s]), Hperp2 = cadd<double>(Hhh[s], Hhs[s]);
+  y[s] = csub<double>(Gs1, Gs2);
+  const double2 w = csub<double>(cmul<double>(e1, Gs1), cmul<double>(e2, Gs2));  // for outer(n, G sig, eps)
+  double2 nGh = mk<double>(0, 0);
+  for (int c = 0; c < 3; ++c) {
+    const int64_t col = c + 3 * q;
+    const double nc = nvec[c * n + i];
+    const double2 Gh1 = GX1[(p + col) * n + i], Hh1 = HX1[(p + col) * n + i];
+    const double2 Gh2 = c < 2 ? Gp[(c + 2 * q) * n + i] : Gperp2;
+    const double2 Hh2 = c < 2 ? Hp[(c + 2 * q) * n + i] : Hperp2;
+    y[(p + col) * n + i] = csub<double>(Gh1, Gh2);
+    y[(5 * p + col) * n + i] = csub<double>(csub<double>(Hh1, Hh2), cmul<double>(ik, cscale<double>(w, nc)));
+    nGh = cadd<double>(nGh, cscale<double>(csub<double>(cmul<double>(e1, Gh1), cmul<double>(e2, Gh2)), nc));
+  }
+  y[(4 * p + q) * n + i] =
+      csub<double>(csub<double>(cmul<double>(e1, Hs1), cmul<double>(e2, Hs2)), cmul<double>(ik, nGh));
+}
+
 struct BemRetIter {
   const GreenRet* g;
   int64_t n;
@@ -367,10 +421,19 @@ struct BemRetIter {
   int last_iter = 0, last_flag = 0;
   double last_relres = 0, last_true_relres = 0;
   Workspace ws, hws, mws[4];  // mws: one per product of the batched H-matrix products
+  // substrate: G2, H2 hold the p components, G2x, H2x the ss, hs, sh, hh ones
+  GreenRetLayer* gl = nullptr;
+  std::unique_ptr<HMat> G2x[4], H2x[4];
+  double2 eps_down{};
 
   BemRetIter(const GreenRet* g_, int prec_, const int64_t* perm, int64_t nfull, const int64_t* full,
-             int64_t nlow, const int64_t* low, double htol, int kmax)
-      : g(g_), n(g_->n), precision(prec_) {
+             int64_t nlow, const int64_t* low, double htol, int kmax, GreenRetLayer* gl_ = nullptr)
+      : g(g_), n(g_->n), precision(prec_), gl(gl_) {
+    if (gl)
+      for (int c = 0; c < 4; ++c) {
+        G2x[c].reset(new HMat(g, prec_, perm, nfull, full, nlow, low, htol, kmax));
+        H2x[c].reset(new HMat(g, prec_, perm, nfull, full, nlow, low, htol, kmax));
+      }
     G1.reset(new HMat(g, prec_, perm, nfull, full, nlow, low, htol, kmax));
     G2.reset(new HMat(g, prec_, perm, nfull, full, nlow, low, htol, kmax));
     H1.reset(new HMat(g, prec_, perm, nfull, full, nlow, low, htol, kmax));
@@ -477,8 +540,20 @@ struct BemRetIter {
     const double2 k1 = ksq(e1), k2 = ksq(e2);
     G1->fill(k1, MNP_KEY_G);
     H1->fill(k1, MNP_KEY_H1);
-    G2->fill(k2, MNP_KEY_G);
-    H2->fill(k2, MNP_KEY_H2);
+    if (gl) {  // components p, ss, hs, sh, hh; the direct part in p, ss, hh
+      const LayerPairs P = greenretlayer_pairs(gl, enei, e2, eps_down);
+      G2->fill_layer(k2, MNP_KEY_G, 0, true, P);
+      H2->fill_layer(k2, MNP_KEY_H2, 0, true, P);
+      for (int c = 1; c < 5; ++c) {
+        const bool direct = c == 1 || c == 4;
+        G2x[c - 1]->fill_layer(k2, MNP_KEY_G, c, direct, P);
+        H2x[c - 1]->fill_layer(k2, MNP_KEY_H2, c, direct, P);
+      }
+      greenretlayer_check(gl);
+    } else {
+      G2->fill(k2, MNP_KEY_G);
+      H2->fill(k2, MNP_KEY_H2);
+    }
     const double2 L = csub<double>(e1, e2);
     const double2 cc = cscale<double>(cmul<double>(L, L), k0 * k0);
     if (precision == MNP_FP64)
@@ -494,8 +569,45 @@ struct BemRetIter {
     ready = true;
   }
 
+  // y = A x with the substrate: 12 H-matrix products in three batches
+  void afun_layer(int64_t p, const double2* x, double2* y, double2* prod) {
+    double2* X2 = prod + 20 * p * n;  // n x 4p, after GX1, HX1 (4p each) and the components (6p each)
+    double2 *GX1 = prod, *HX1 = prod + 4 * p * n;
+    double2* Gc = prod + 8 * p * n;  // ss, hs, sh, hh (p each), p (2p)
+    double2* Hc = Gc + 6 * p * n;
+    hipLaunchKernelGGL(k_layit_gather, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0, stream(), n, p, x,
+                       X2);
+    MNP_HIP(hipGetLastError());
+    const double2 *s2 = X2, *hperp = X2 + p * n, *hpar = X2 + 2 * p * n;
+    Workspace* w[4] = {&mws[0], &mws[1], &mws[2], &mws[3]};
+    {
+      const HMat* A[4] = {G1.get(), H1.get(), G2.get(), H2.get()};
+      const double2* xs[4] = {x, x, hpar, hpar};
+      double2* ys[4] = {GX1, HX1, Gc + 4 * p * n, Hc + 4 * p * n};
+      // G1, H1 take 4p columns, the p components 2p: two batches
+      HMat::matvec_batch(2, A, 4 * p, xs, ys, w);
+      HMat::matvec_batch(2, A + 2, 2 * p, xs + 2, ys + 2, w);
+    }
+    for (int side = 0; side < 2; ++side) {
+      std::unique_ptr<HMat>* C = side ? H2x : G2x;
+      double2* out = side ? Hc : Gc;
+      const HMat* A[4] = {C[0].get(), C[1].get(), C[2].get(), C[3].get()};  // ss, hs, sh, hh
+      const double2* xs[4] = {s2, s2, hperp, hperp};
+      double2* ys[4] = {out, out + p * n, out + 2 * p * n, out + 3 * p * n};
+      HMat::matvec_batch(4, A, p, xs, ys, w);
+    }
+    hipLaunchKernelGGL(k_layit_apply, dim3((unsigned)cdiv_up(n, 256), (unsigned)p), dim3(256), 0, stream(), n, p, GX1,
+                       HX1, Gc, Gc + p * n, Gc + 2 * p * n, Gc + 3 * p * n, Gc + 4 * p * n, Hc, Hc + p * n,
+                       Hc + 2 * p * n, Hc + 3 * p * n, Hc + 4 * p * n, g->geo.nvec, k0, eps1, eps2, y);
+    MNP_HIP(hipGetLastError());
+  }
+
   // y = A x (afun.m), x, y n x 8p
   void afun(int64_t p, const double2* x, double2* y, double2* prod) {
+    if (gl) {
+      afun_layer(p, x, y, prod);
+      return;
+    }
     double2 *GX1 = prod, *GX2 = prod + 4 * p * n, *HX1 = prod + 8 * p * n, *HX2 = prod + 12 * p * n;
     const HMat* A[4] = {G1.get(), G2.get(), H1.get(), H2.get()};
     const double2* xs[4] = {x, x + 4 * p * n, x, x + 4 * p * n};
@@ -548,7 +660,7 @@ struct BemRetIter {
     const int64_t len = 8 * p * n;
     double2* b = (double2*)ws.get(0, sizeof(double2) * len);
     double2* x = (double2*)ws.get(6, sizeof(double2) * len);
-    double2* prod = (double2*)hws.get(7, sizeof(double2) * 16 * p * n);
+    double2* prod = (double2*)hws.get(7, sizeof(double2) * (gl ? 24 : 16) * p * n);
     launch_excitation(n, p, in, b, g->geo.nvec, k0, eps1, eps2);
     const GmresOut o = fgmres(
         len, b, x, tol, maxit, [&](const double2* v, double2* w) { afun(p, v, w, prod); },
@@ -589,6 +701,33 @@ int mnp_bemretiter_create(mnp_greenret_t g, int precision, const int64_t* perm, 
   });
 }
 
+int mnp_bemretiter_create_layer(mnp_greenret_t g, struct mnp_greenretlayer_s* gl, int precision, const int64_t* perm,
+                                int64_t nfull, const int64_t* full, int64_t nlow, const int64_t* low, double htol,
+                                int kmax, mnp_bemretiter_t* out) {
+  return guarded([&] {
+    MNP_CHECK(g && gl && perm && out, MNP_ERR_ARG, "bemretiter_create_layer: NULL argument");
+    MNP_CHECK(!g->g->compound(), MNP_ERR_ARG, "bemretiter_create_layer: compound particles are not supported");
+    auto* h = new mnp_bemretiter_s{nullptr};
+    try {
+      h->b = new BemRetIter(g->g, precision, perm, nfull, full, nlow, low, htol, kmax, greenretlayer_of(gl));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int mnp_bemretiter_initmat_layer(mnp_bemretiter_t it, double enei, mnp_complex eps_in, mnp_complex eps_up,
+                                 mnp_complex eps_down) {
+  return guarded([&] {
+    MNP_CHECK(it && enei > 0, MNP_ERR_ARG, "bemretiter_initmat_layer: bad argument");
+    MNP_CHECK(it->b->gl, MNP_ERR_STATE, "bemretiter_initmat_layer: the solver was not created with a substrate");
+    it->b->eps_down = to_d2(eps_down);
+    it->b->initmat(enei, to_d2(eps_in), to_d2(eps_up));
+  });
+}
+
 int mnp_bemretiter_destroy(mnp_bemretiter_t it) {
   return guarded([&] {
     if (!it) return;
@@ -617,6 +756,7 @@ int mnp_bemretiter_set_coarse(mnp_bemretiter_t it, int tiles_per_aggregate) {
 int mnp_bemretiter_initmat(mnp_bemretiter_t it, double enei, mnp_complex eps1, mnp_complex eps2) {
   return guarded([&] {
     MNP_CHECK(it && enei > 0, MNP_ERR_ARG, "bemretiter_initmat: bad argument");
+    MNP_CHECK(!it->b->gl, MNP_ERR_STATE, "bemretiter_initmat: a substrate solver takes mnp_bemretiter_initmat_layer");
     it->b->initmat(enei, to_d2(eps1), to_d2(eps2));
   });
 }

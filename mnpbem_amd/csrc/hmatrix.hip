@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "mnp_hmatrix.hpp"
+#include "mnp_layertab.hpp"
 
 namespace mnp {
 
@@ -43,63 +44,115 @@ __device__ __forceinline__ double2 colloc(const double* cpos, const double* cnve
   return mk<double>(v.x * s, v.y * s);
 }
 
-// Full blocks: entry (i, j) of block b -> faces fi = perm[r0+i], fj = perm[c0+j].
+// The reflected part of a substrate H-matrix (Greenfun/+aca/
+// @compgreenretlayer/private/eval2.m): component comp of the tabulated
+// reflected G or F, the refined values of its near pairs (pv, column
+// offsets coff over rows) and polar diagonal elements (dv by dmap), and
+// whether the direct part is kept (p, ss, hh) or not (hs, sh).
+struct LayerFill {
+  int on, direct, comp;
+  TabView T;
+  const int64_t* coff;
+  const int32_t* rows;
+  const int32_t* dmap;
+  const double2* pv;  // refined pair values of this component, np
+  const double2* dv;  // refined diagonal values of this component, nd
+};
+
+// Entries of admissible blocks for ACA, (i, j) in cluster order: the direct
+// collocation kernel, and for a substrate the reflected component at the
+// centroids (admissible blocks hold no refined pairs).
+struct EntDirect {
+  const double *cpos, *cnvec, *carea;
+  int64_t n;
+  double2 wav;
+  int isF;
+  __device__ double2 operator()(int64_t i, int64_t j) const { return colloc(cpos, cnvec, carea, n, i, j, wav, isF); }
+};
+struct EntLayer {
+  EntDirect d;
+  int direct, comp;
+  TabView T;
+  __device__ double2 operator()(int64_t i, int64_t j) const {
+    const double2 r = refl_entry(T, d.cpos, d.cnvec, d.carea, d.n, i, j, comp, d.isF);
+    return direct ? cadd<double>(d(i, j), r) : r;
+  }
+};
+
+// Full blocks: entry (i, j) of block b -> faces fi = perm[r0+i], fj = perm[c0+j];
+// with a substrate (L.on) the reflected part of component L.comp is added:
+// the refined value of a near pair or polar diagonal element, else the
+// centroid value (initrefl2.m), and the direct part is kept only where it
+// belongs (L.direct).
 template <class TS>
 __global__ void __launch_bounds__(256) k_hfull(int64_t nb, const int32_t* blk, const int64_t* voff,
                                                const int32_t* perm, int64_t n, Geo geo, Pairs pr,
                                                const int64_t* off, int64_t nnz, int order,
                                                const double* gt, const double* ft, double2 k,
-                                               int key, cplx<TS>* vals) {
+                                               int key, LayerFill L, cplx<TS>* vals) {
   const int64_t b = blockIdx.x;
   const int r0 = blk[4 * b], r1 = blk[4 * b + 1], c0 = blk[4 * b + 2], c1 = blk[4 * b + 3];
   const int m = r1 - r0, w = c1 - c0;
   cplx<TS>* out = vals + voff[b];
   const double2 ik = mk<double>(-k.y, k.x);
-  for (int64_t e = threadIdx.x; e < (int64_t)m * w; e += blockDim.x) {
-    const int i = (int)(e % m), j = (int)(e / m);
-    const int64_t fi = perm[r0 + i], fj = perm[c0 + j];
-    const double dx = geo.pos[fi] - geo.pos[fj], dy = geo.pos[n + fi] - geo.pos[n + fj],
-                 dz = geo.pos[2 * n + fi] - geo.pos[2 * n + fj];
-    const double d = fmax(sqrt(dx * dx + dy * dy + dz * dz), 2.220446049250313e-16);
-    const double2 ph = cexp_ikd(k, d);
-    // refined pair?  rows of column fj are sorted in [off[fj], off[fj+1])
-    int64_t lo = off[fj], hi = off[fj + 1] - 1, p = -1;
+  // refined pair of (fi, fj) in a column's sorted row list, or -1
+  auto find = [](const int32_t* rows, int64_t lo, int64_t hi, int64_t fi) -> int64_t {
     while (lo <= hi) {
       const int64_t mid = (lo + hi) >> 1;
-      const int32_t rr = pr.row[mid];
-      if (rr == fi) {
-        p = mid;
-        break;
-      }
+      const int32_t rr = rows[mid];
+      if (rr == fi) return mid;
       if (rr < fi)
         lo = mid + 1;
       else
         hi = mid - 1;
     }
-    double2 v;
-    if (p >= 0) {
-      double2 pw = mk<double>(1, 0), s = mk<double>(0, 0);
-      for (int o = 0; o <= order; ++o) {
-        double c;
-        if (key == MNP_KEY_G)
-          c = gt[o * nnz + p];
-        else
-          c = geo.nvec[fi] * ft[(o * 3 + 0) * nnz + p] + geo.nvec[n + fi] * ft[(o * 3 + 1) * nnz + p] +
-              geo.nvec[2 * n + fi] * ft[(o * 3 + 2) * nnz + p];
-        s = cadd<double>(s, cscale<double>(pw, c));
-        pw = cmul<double>(pw, ik);
+    return -1;
+  };
+  for (int64_t e = threadIdx.x; e < (int64_t)m * w; e += blockDim.x) {
+    const int i = (int)(e % m), j = (int)(e / m);
+    const int64_t fi = perm[r0 + i], fj = perm[c0 + j];
+    double2 v = mk<double>(0, 0);
+    if (!L.on || L.direct) {
+      const double dx = geo.pos[fi] - geo.pos[fj], dy = geo.pos[n + fi] - geo.pos[n + fj],
+                   dz = geo.pos[2 * n + fi] - geo.pos[2 * n + fj];
+      const double d = fmax(sqrt(dx * dx + dy * dy + dz * dz), 2.220446049250313e-16);
+      const double2 ph = cexp_ikd(k, d);
+      const int64_t p = find(pr.row, off[fj], off[fj + 1] - 1, fi);  // rows of column fj sorted
+      if (p >= 0) {
+        double2 pw = mk<double>(1, 0), s = mk<double>(0, 0);
+        for (int o = 0; o <= order; ++o) {
+          double c;
+          if (key == MNP_KEY_G)
+            c = gt[o * nnz + p];
+          else
+            c = geo.nvec[fi] * ft[(o * 3 + 0) * nnz + p] + geo.nvec[n + fi] * ft[(o * 3 + 1) * nnz + p] +
+                geo.nvec[2 * n + fi] * ft[(o * 3 + 2) * nnz + p];
+          s = cadd<double>(s, cscale<double>(pw, c));
+          pw = cmul<double>(pw, ik);
+        }
+        v = cmul<double>(s, ph);
+      } else if (key == MNP_KEY_G) {
+        const double g = (1.0 / d) * geo.area[fj];
+        v = mk<double>(g * ph.x, g * ph.y);
+      } else {
+        const double id2 = 1.0 / (d * d);
+        const double2 fk = mk<double>((-k.y - 1.0 / d) * id2, k.x * id2);
+        const double in = dx * geo.nvec[fi] + dy * geo.nvec[n + fi] + dz * geo.nvec[2 * n + fi];
+        v = cmul<double>(mk<double>(fk.x * in * geo.area[fj], fk.y * in * geo.area[fj]), ph);
       }
-      v = cmul<double>(s, ph);
-    } else if (key == MNP_KEY_G) {
-      const double g = (1.0 / d) * geo.area[fj];
-      v = mk<double>(g * ph.x, g * ph.y);
-    } else {
-      const double id2 = 1.0 / (d * d);
-      const double2 fk = mk<double>((-k.y - 1.0 / d) * id2, k.x * id2);
-      const double in = dx * geo.nvec[fi] + dy * geo.nvec[n + fi] + dz * geo.nvec[2 * n + fi];
-      v = cmul<double>(mk<double>(fk.x * in * geo.area[fj], fk.y * in * geo.area[fj]), ph);
+      if (fi == fj) v.x += key == MNP_KEY_H1 ? 2 * M_PI : key == MNP_KEY_H2 ? -2 * M_PI : 0.0;
     }
-    if (fi == fj) v.x += key == MNP_KEY_H1 ? 2 * M_PI : key == MNP_KEY_H2 ? -2 * M_PI : 0.0;
+    if (L.on) {
+      const int64_t q = find(L.rows, L.coff[fj], L.coff[fj + 1] - 1, fi);
+      double2 r;
+      if (q >= 0)
+        r = L.pv[q];
+      else if (fi == fj && L.dmap[fi] >= 0)
+        r = L.dv[L.dmap[fi]];
+      else
+        r = refl_entry(L.T, geo.pos, geo.nvec, geo.area, n, fi, fj, L.comp, key != MNP_KEY_G);
+      v = cadd<double>(v, r);
+    }
     out[(int64_t)j * m + i] = mk<TS>((TS)v.x, (TS)v.y);
   }
 }
@@ -191,11 +244,10 @@ __device__ __forceinline__ double2 aca_residual(double2 v, const double2* P, int
 // Scratch: A (m x kb) then B (n x kb) at soff[b]; rank[b] on exit.  Rows
 // tid + G t are this thread's; the pivot rows already taken are kept as
 // bits t of used0/used1 (m <= 128 G, checked on the host).
-template <int G>
-__device__ __forceinline__ void aca_block(int64_t b, int tid, const int32_t* blk, const int64_t* soff,
-                                          const double* cpos, const double* cnvec, const double* carea,
-                                          int64_t n, double2 wav, int isF, double tol, int kmax, double2* scratch,
-                                          int32_t* rank, double2* coef, double* red, Piv* pred) {
+template <int G, class Ent>
+__device__ __forceinline__ void aca_block(int64_t b, int tid, const int32_t* blk, const int64_t* soff, const Ent& E,
+                                          double tol, int kmax, double2* scratch, int32_t* rank, double2* coef,
+                                          double* red, Piv* pred) {
   const int r0 = blk[4 * b], r1 = blk[4 * b + 1], c0 = blk[4 * b + 2], c1 = blk[4 * b + 3];
   const int m = r1 - r0, w = c1 - c0;
   const int kb = min(kmax, min(m, w));
@@ -212,7 +264,7 @@ __device__ __forceinline__ void aca_block(int64_t b, int tid, const int32_t* blk
     double s1[1] = {0};
     Piv pc{-1.0, -1, mk<double>(0, 0)};
     for (int c = tid; c < w; c += G) {
-      double2 v = colloc(cpos, cnvec, carea, n, r0 + r, c0 + c, wav, isF);
+      double2 v = E(r0 + r, c0 + c);
       s1[0] += v.x * v.x + v.y * v.y;
       v = aca_residual(v, B, w, c, k, coef);
       B[(int64_t)k * w + c] = v;
@@ -244,7 +296,7 @@ __device__ __forceinline__ void aca_block(int64_t b, int tid, const int32_t* blk
     Piv pr{-1.0, -1, mk<double>(0, 0)};
     int t = 0;
     for (int i = tid; i < m; i += G, ++t) {
-      double2 v = colloc(cpos, cnvec, carea, n, r0 + i, c0 + cp, wav, isF);
+      double2 v = E(r0 + i, c0 + cp);
       v = aca_residual(v, A, m, i, k, coef);
       A[(int64_t)k * m + i] = v;
       s2[0] += v.x * v.x + v.y * v.y;
@@ -264,24 +316,21 @@ __device__ __forceinline__ void aca_block(int64_t b, int tid, const int32_t* blk
 // One ACA work item per workgroup of 256 threads: item[1] == -2 marks one
 // large block for the whole workgroup (item[0]), otherwise item[0..3] are up
 // to four small blocks (-1: none), one per wave.
-__global__ void __launch_bounds__(256) k_aca(const int32_t* items, const int32_t* blk, const int64_t* soff,
-                                             const double* cpos, const double* cnvec, const double* carea,
-                                             int64_t n, double2 wav, int isF, double tol, int kmax,
-                                             double2* scratch, int32_t* rank) {
+template <class Ent>
+__global__ void __launch_bounds__(256) k_aca(const int32_t* items, const int32_t* blk, const int64_t* soff, Ent E,
+                                             double tol, int kmax, double2* scratch, int32_t* rank) {
   __shared__ double2 coef[4][128];
   __shared__ double red[8];
   __shared__ Piv pred[4];
   const int32_t* it = items + 4 * (int64_t)blockIdx.x;
   if (it[1] == -2) {
-    aca_block<256>(it[0], threadIdx.x, blk, soff, cpos, cnvec, carea, n, wav, isF, tol, kmax, scratch, rank,
-                   coef[0], red, pred);
+    aca_block<256>(it[0], threadIdx.x, blk, soff, E, tol, kmax, scratch, rank, coef[0], red, pred);
     return;
   }
   const int wv = threadIdx.x >> 6;
   const int32_t b = it[wv];
   if (b >= 0)
-    aca_block<64>(b, threadIdx.x & 63, blk, soff, cpos, cnvec, carea, n, wav, isF, tol, kmax, scratch, rank,
-                  coef[wv], nullptr, nullptr);
+    aca_block<64>(b, threadIdx.x & 63, blk, soff, E, tol, kmax, scratch, rank, coef[wv], nullptr, nullptr);
 }
 
 // Compacts the ACA scratch of block b = order[blockIdx.x] into its factors
@@ -982,9 +1031,20 @@ HMat::~HMat() {
 
 void HMat::fill(double2 k, int key_) {
   if (prec == MNP_FP64)
-    fill_impl<double>(k, key_);
+    fill_impl<double>(k, key_, nullptr);
   else
-    fill_impl<float>(k, key_);
+    fill_impl<float>(k, key_, nullptr);
+}
+
+void HMat::fill_layer(double2 k, int key_, int comp, bool direct, const LayerPairs& P) {
+  MNP_CHECK(comp >= 0 && comp < 5, MNP_ERR_ARG, "hmat: substrate component must be 0..4");
+  const bool isF = key_ != MNP_KEY_G;
+  LayerFill L{1, direct ? 1 : 0, comp, P.T, P.coff, P.rows, P.dmap,
+              (isF ? P.pF : P.pG) + (int64_t)comp * P.np, (isF ? P.dF : P.dG) + (int64_t)comp * P.nd};
+  if (prec == MNP_FP64)
+    fill_impl<double>(k, key_, &L);
+  else
+    fill_impl<float>(k, key_, &L);
 }
 
 // ACA scratch shared by all H-matrices; never destroyed (no hipFree from a
@@ -995,7 +1055,7 @@ static Workspace& aca_ws() {
 }
 
 template <class TS>
-void HMat::fill_impl(double2 k, int key_) {
+void HMat::fill_impl(double2 k, int key_, const LayerFill* lay) {
   key = key_;
   hipStream_t s = stream();
   const size_t es = sizeof(cplx<TS>);
@@ -1004,8 +1064,9 @@ void HMat::fill_impl(double2 k, int key_) {
   if (nfull) {
     const GreenRet& G = *g;
     TimerScope ts(MNP_TIMER_ASSEMBLE, (double)es * (double)nfullvals, (double)es * (double)nfullvals);
+    const LayerFill off{};
     hipLaunchKernelGGL(k_hfull<TS>, dim3((unsigned)nfull), dim3(256), 0, s, nfull, fblk, d_voff, perm, n, G.geo,
-                       G.pairs, G.off, G.nnz, G.order, G.gtab, G.ftab, k, key, (cplx<TS>*)fvals);
+                       G.pairs, G.off, G.nnz, G.order, G.gtab, G.ftab, k, key, lay ? *lay : off, (cplx<TS>*)fvals);
     MNP_HIP(hipGetLastError());
   }
   // ACA over the work plan, in batches whose FP64 scratch fits the budget
@@ -1023,8 +1084,13 @@ void HMat::fill_impl(double2 k, int key_) {
       const int64_t i0 = aca_icut[c], i1 = aca_icut[c + 1], o0 = aca_bcut[c], o1 = aca_bcut[c + 1];
       {
         TimerScope ts(MNP_TIMER_ACA, 0.0);
-        hipLaunchKernelGGL(k_aca, dim3((unsigned)(i1 - i0)), dim3(256), 0, s, d_aca_items + 4 * i0, lblk,
-                           d_aca_soff, cpos, cnvec, carea, n, k, isF, htol, kmax, scratch, rank);
+        const EntDirect ed{cpos, cnvec, carea, n, k, isF};
+        if (lay)
+          hipLaunchKernelGGL(k_aca<EntLayer>, dim3((unsigned)(i1 - i0)), dim3(256), 0, s, d_aca_items + 4 * i0, lblk,
+                             d_aca_soff, EntLayer{ed, lay->direct, lay->comp, lay->T}, htol, kmax, scratch, rank);
+        else
+          hipLaunchKernelGGL(k_aca<EntDirect>, dim3((unsigned)(i1 - i0)), dim3(256), 0, s, d_aca_items + 4 * i0, lblk,
+                             d_aca_soff, ed, htol, kmax, scratch, rank);
         MNP_HIP(hipGetLastError());
       }
       MNP_HIP(hipMemcpyAsync(rank_h.data(), rank, sizeof(int32_t) * nlow, hipMemcpyDeviceToHost, s));
@@ -1632,6 +1698,18 @@ int mnp_hmat_fill(mnp_hmat_t h, mnp_complex k, int key) {
     MNP_CHECK(h, MNP_ERR_ARG, "hmat_fill: NULL handle");
     MNP_CHECK(key >= MNP_KEY_G && key <= MNP_KEY_H2, MNP_ERR_ARG, "hmat_fill: bad key");
     h->h->fill(to_d2(k), key);
+  });
+}
+
+int mnp_hmat_fill_layer(mnp_hmat_t h, mnp_complex k, int key, int comp, int direct, struct mnp_greenretlayer_s* gl,
+                        double enei, mnp_complex eps_up, mnp_complex eps_down) {
+  return guarded([&] {
+    MNP_CHECK(h && gl, MNP_ERR_ARG, "hmat_fill_layer: NULL handle");
+    MNP_CHECK(key == MNP_KEY_G || key == MNP_KEY_H2, MNP_ERR_ARG, "hmat_fill_layer: key must be MNP_KEY_G or MNP_KEY_H2");
+    GreenRetLayer* L = greenretlayer_of(gl);
+    const LayerPairs P = greenretlayer_pairs(L, enei, to_d2(eps_up), to_d2(eps_down));
+    h->h->fill_layer(to_d2(k), key, comp, direct != 0, P);
+    greenretlayer_check(L);
   });
 }
 

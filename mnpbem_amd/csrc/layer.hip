@@ -36,6 +36,7 @@
 #include "mnp_chain.hpp"
 #include "mnp_common.hpp"
 #include "mnp_greenret.hpp"
+#include "mnp_layertab.hpp"
 #include "mnp_linalg.hpp"
 
 namespace mnp {
@@ -45,7 +46,6 @@ constexpr int NGL = 16;             // Gauss-Legendre nodes per panel
 constexpr int NLAG = 32;            // generalized Gauss-Laguerre nodes of Watson's integral
 constexpr int PANELS_ELLIPSE = 16;  // panels of the semi-ellipse leg
 constexpr double TAIL_DECAY = 40.0; // tail legs end at exp(-40)
-constexpr int NQ = 15;              // G, Fr, Fz x (p, ss, hs, sh, hh)
 
 __constant__ double c_gl[2][NGL];        // nodes on [0, 1], weights
 __constant__ double c_lag[2][2][NLAG];   // alpha = -1/2, +1/2: nodes, weights / Gamma(alpha + 1)
@@ -346,74 +346,6 @@ __global__ void k_lerp(int64_t len, const double2* a, const double2* b, double t
 }
 
 // ------------------------------------------------------ interpolation
-struct TabView {
-  int64_t nr, nz;
-  const double* r;
-  const double* z1;
-  const double2* v;  // (i nz + j) 15 + q
-  double z0, rmin, zmin;
-  int* err;          // set when a point lies outside the table
-};
-
-__device__ inline double round_z(double z, double z0, double zmin) {  // round.m
-  const double d = fabs(z - z0);
-  if (d <= zmin) {
-    const double s = z > z0 ? 1.0 : (z < z0 ? -1.0 : 0.0);
-    return z0 + s * zmin;
-  }
-  return z;
-}
-
-__device__ inline int64_t upper_index(const double* x, int64_t n, double v) {  // #{x <= v}
-  int64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const int64_t m = (lo + hi) / 2;
-    if (x[m] <= v)
-      lo = m + 1;
-    else
-      hi = m;
-  }
-  return lo;
-}
-
-// interp2.m:17-44: G, Fr, Fz at (r, z1, z2), and the clamped r.
-__device__ double tab_interp(const TabView& T, double r, double z1, double z2, double2 o[NQ]) {
-#pragma clang fp contract(off)
-  r = fmax(T.rmin, r);
-  z1 = round_z(z1, T.z0, T.zmin);
-  z2 = round_z(z2, T.z0, T.zmin);
-  const double z = z1 + fabs(round_z(z2, T.z0, T.zmin) - T.z0);
-  const double zm = fabs(z - T.z0);
-  const double d = sqrt(r * r + zm * zm);
-  int64_t ix = upper_index(T.r, T.nr, r), iy = upper_index(T.z1, T.nz, z);
-  if (ix == 0 || iy == 0 || r > T.r[T.nr - 1] || z > T.z1[T.nz - 1]) {
-    atomicOr(T.err, 1);
-    ix = ix == 0 ? 1 : ix;
-    iy = iy == 0 ? 1 : iy;
-  }
-  ix = (ix < T.nr - 1 ? ix : T.nr - 1) - 1;
-  iy = (iy < T.nz - 1 ? iy : T.nz - 1) - 1;
-  const double xx = (r - T.r[ix]) / (T.r[ix + 1] - T.r[ix]);
-  const double yy = (z - T.z1[iy]) / (T.z1[iy + 1] - T.z1[iy]);
-  const double w0 = (1 - xx) * (1 - yy), w1 = xx * (1 - yy), w2 = (1 - xx) * yy, w3 = xx * yy;
-  const double2* v00 = T.v + (ix * T.nz + iy) * NQ;
-  const double2* v10 = v00 + T.nz * NQ;
-  const double2* v01 = v00 + NQ;
-  const double2* v11 = v10 + NQ;
-  const double d3 = d * d * d;
-  for (int q = 0; q < NQ; ++q) {
-    const double fx = w0 * v00[q].x + w1 * v10[q].x + w2 * v01[q].x + w3 * v11[q].x;
-    const double fy = w0 * v00[q].y + w1 * v10[q].y + w2 * v01[q].y + w3 * v11[q].y;
-    if (q < 5)
-      o[q] = mk<double>(fx / d, fy / d);
-    else if (q < 10)
-      o[q] = mk<double>(fx * r / d3, fy * r / d3);
-    else
-      o[q] = mk<double>(fx * zm / d3, fy * zm / d3);
-  }
-  return r;
-}
-
 // Reflected G and F = nvec . Gp at the centroids, all pairs
 // (initrefl2.m:21-37): G[c n^2 + j n + i], F likewise.
 __global__ void __launch_bounds__(256) k_layer_assemble(int64_t n, Geo geo, TabView T, double2* __restrict__ G,
@@ -442,9 +374,6 @@ __global__ void __launch_bounds__(256) k_layer_assemble(int64_t n, Geo geo, TabV
 // (row i, face j) and vertex a, W[(p 3 + a) 4 + {g, f1, f2, fz}], the
 // vertex's clamped radius R0[p 3 + a] and height Z2[p 3 + a], and the row's
 // height Z1[p].
-struct LayGeo {
-  double z0, rmin, zmin;
-};
 __device__ inline void lay_dist(const LayGeo& L, const double* p1, const double* p2, double& r, double& z,
                                 double& d, double& x, double& y) {
 #pragma clang fp contract(off)
@@ -509,7 +438,8 @@ __global__ void __launch_bounds__(256) k_layer_refine_apply(int64_t np, int64_t 
                                                             const double* __restrict__ W,
                                                             const double* __restrict__ R0,
                                                             const double* __restrict__ Z2,
-                                                            const double* __restrict__ Z1, double2* G, double2* F) {
+                                                            const double* __restrict__ Z1, double2* G, double2* F,
+                                                            int compact) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= np) return;
   const int64_t i = rows[p], j = cols[p];
@@ -527,7 +457,8 @@ __global__ void __launch_bounds__(256) k_layer_refine_apply(int64_t np, int64_t 
     }
   }
   const double nx = geo.nvec[i], ny = geo.nvec[n + i], nz = geo.nvec[2 * n + i];
-  const int64_t nn = n * n, idx = j * n + i;
+  // dense 5 x n x n, or compact 5 x np (compact != 0)
+  const int64_t nn = compact ? np : n * n, idx = compact ? p : j * n + i;
   for (int c = 0; c < 5; ++c) {
     G[c * nn + idx] = g[c];
     F[c * nn + idx] = mk<double>(nx * gx[c].x + ny * gy[c].x + nz * gz[c].x, nx * gx[c].y + ny * gy[c].y + nz * gz[c].y);
@@ -537,7 +468,7 @@ __global__ void __launch_bounds__(256) k_layer_refine_apply(int64_t np, int64_t 
 // Diagonal elements with the polar rule (initrefl2.m:39-99), faces id[0..nd).
 __global__ void __launch_bounds__(64) k_layer_diag(int64_t nd, int64_t n, Geo geo, TabView T,
                                                    const int32_t* __restrict__ id, const double* __restrict__ qpol,
-                                                   int npol, double2* G, double2* F) {
+                                                   int npol, double2* G, double2* F, int compact) {
 #pragma clang fp contract(off)
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nd) return;
@@ -565,7 +496,7 @@ __global__ void __launch_bounds__(64) k_layer_diag(int64_t nd, int64_t n, Geo ge
     }
   }
   const double nx = geo.nvec[f], ny = geo.nvec[n + f], nz = geo.nvec[2 * n + f];
-  const int64_t nn = n * n, idx = f * n + f;
+  const int64_t nn = compact ? nd : n * n, idx = compact ? t : f * n + f;  // dense or compact 5 x nd
   for (int c = 0; c < 5; ++c) {
     G[c * nn + idx] = g[c];
     F[c * nn + idx] = mk<double>(nx * gx[c].x + ny * gy[c].x + nz * gz[c].x, nx * gx[c].y + ny * gy[c].y + nz * gz[c].y);
@@ -1160,10 +1091,56 @@ struct GreenRetLayer {
       MNP_HIP(hipGetLastError());
     }
     if (nd) MNP_HIP(hipMemcpy(id, hd.data(), sizeof(int32_t) * nd, hipMemcpyHostToDevice));
+    // column offsets of the pairs (found column by column, rows increasing)
+    // and the diagonal element of each face
+    std::vector<int64_t> co(n + 1, 0);
+    for (int64_t p = 0; p < np; ++p) ++co[hc[p] + 1];
+    for (int64_t j = 0; j < n; ++j) co[j + 1] += co[j];
+    std::vector<int32_t> dm(n, -1);
+    for (int64_t t = 0; t < nd; ++t) dm[hd[t]] = (int32_t)t;
+    coff = dev_alloc<int64_t>(n + 1);
+    dmap = dev_alloc<int32_t>(n);
+    MNP_HIP(hipMemcpy(coff, co.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
+    MNP_HIP(hipMemcpy(dmap, dm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     MNP_HIP(hipStreamSynchronize(stream()));
   }
+  int64_t* coff = nullptr;
+  int32_t* dmap = nullptr;
+  // Refined values of the pairs and diagonal elements alone (component-major,
+  // 5 x np and 5 x nd), with the table at enei: the near-field part of the
+  // reflected matrices for H-matrix storage.
+  double2 *pG = nullptr, *pF = nullptr, *dG = nullptr, *dF = nullptr;
+  LayerPairs pairs(double enei, double2 e1, double2 e2) {
+    const int64_t n = g->n;
+    const double2* tv = tab->at(enei, e1, e2);
+    const TabView T = tab->view(tv);
+    if (!pG) {
+      pG = dev_alloc<double2>(5 * std::max<int64_t>(np, 1));
+      pF = dev_alloc<double2>(5 * std::max<int64_t>(np, 1));
+      dG = dev_alloc<double2>(5 * std::max<int64_t>(nd, 1));
+      dF = dev_alloc<double2>(5 * std::max<int64_t>(nd, 1));
+    }
+    MNP_HIP(hipMemsetAsync(tab->err, 0, sizeof(int), stream()));
+    if (np) {
+      hipLaunchKernelGGL(k_layer_refine_apply, dim3(nblk(np)), dim3(256), 0, stream(), np, n, g->geo, T, rows, cols,
+                         W, R0, Z2, Z1, pG, pF, 1);
+      MNP_HIP(hipGetLastError());
+    }
+    if (nd) {
+      hipLaunchKernelGGL(k_layer_diag, dim3(nblk(nd, 64)), dim3(64), 0, stream(), nd, n, g->geo, T, id, g->qpol,
+                         g->npolq, dG, dF, 1);
+      MNP_HIP(hipGetLastError());
+    }
+    return LayerPairs{T, np, nd, rows, cols, id, pG, pF, dG, dF, coff, dmap};
+  }
+  void check() {
+    int herr = 0;
+    MNP_HIP(hipMemcpyAsync(&herr, tab->err, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    MNP_HIP(hipStreamSynchronize(stream()));
+    MNP_CHECK(herr == 0, MNP_ERR_ARG, "greentablayer: a face pair lies outside the table (tabspace of this particle?)");
+  }
   ~GreenRetLayer() {
-    void* ps[] = {rows, cols, id, W, R0, Z2, Z1};
+    void* ps[] = {rows, cols, id, W, R0, Z2, Z1, pG, pF, dG, dF, coff, dmap};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -1177,12 +1154,12 @@ struct GreenRetLayer {
     MNP_HIP(hipGetLastError());
     if (np) {
       hipLaunchKernelGGL(k_layer_refine_apply, dim3(nblk(np)), dim3(256), 0, stream(), np, n, g->geo, T, rows, cols,
-                         W, R0, Z2, Z1, G, F);
+                         W, R0, Z2, Z1, G, F, 0);
       MNP_HIP(hipGetLastError());
     }
     if (nd) {
       hipLaunchKernelGGL(k_layer_diag, dim3(nblk(nd, 64)), dim3(64), 0, stream(), nd, n, g->geo, T, id, g->qpol,
-                         g->npolq, G, F);
+                         g->npolq, G, F, 0);
       MNP_HIP(hipGetLastError());
     }
     int herr = 0;
@@ -1191,6 +1168,11 @@ struct GreenRetLayer {
     MNP_CHECK(herr == 0, MNP_ERR_ARG, "greentablayer: a face pair lies outside the table (tabspace of this particle?)");
   }
 };
+
+LayerPairs greenretlayer_pairs(GreenRetLayer* gl, double enei, double2 e1, double2 e2) {
+  return gl->pairs(enei, e1, e2);
+}
+void greenretlayer_check(GreenRetLayer* gl) { gl->check(); }
 
 // Reflected Green function between points above the interface and a
 // particle (greenretlayer with p1 a compoint): the near pairs of
@@ -1624,6 +1606,9 @@ struct mnp_layertab_s {
 struct mnp_greenretlayer_s {
   mnp::GreenRetLayer* gl;
 };
+namespace mnp {
+GreenRetLayer* greenretlayer_of(mnp_greenretlayer_t h) { return h ? h->gl : nullptr; }
+}  // namespace mnp
 struct mnp_bemretlayer_s {
   mnp::BemRetLayer* b;
 };

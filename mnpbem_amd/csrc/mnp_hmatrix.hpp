@@ -7,6 +7,9 @@
 
 namespace mnp {
 
+struct LayerPairs;  // mnp_layertab.hpp
+struct LayerFill;   // hmatrix.hip
+
 constexpr int MNP_HMV_MAXRHS = 8;  // right-hand sides per H-matvec pass
 
 struct HMat {
@@ -68,6 +71,11 @@ struct HMat {
   HMat& operator=(const HMat&) = delete;
   // Fills the operator `key` (MNP_KEY_*) at wavenumber k.
   void fill(double2 k, int key);
+  // Fills component comp (p, ss, hs, sh, hh = 0..4) of the substrate
+  // operator `key` (MNP_KEY_G or MNP_KEY_H2) at the outside wavenumber k:
+  // the reflected part from P (tables and refined near-field values of this
+  // wavelength) plus, if `direct`, the direct part.
+  void fill_layer(double2 k, int key, int comp, bool direct, const LayerPairs& P);
   // y = H x, x and y complex FP64 in face order (n x nrhs).
   void matvec(int64_t nrhs, const double2* x, double2* y, Workspace& ws) const;
   // y[m] = A[m] x[m] for m < nm <= 4 H-matrices of one size and storage
@@ -85,7 +93,7 @@ struct HMat {
   void build_plan();
   void aca_plan();
   template <class TS>
-  void fill_impl(double2 k, int key);
+  void fill_impl(double2 k, int key, const LayerFill* lay);
   template <class TS>
   static void matvec_batch_impl(int nm, const HMat* const* A, int64_t nrhs, const double2* const* x,
                                 double2* const* y, Workspace* const* ws);

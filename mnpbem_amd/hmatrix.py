@@ -146,6 +146,18 @@ class hmatrix:
         _lib.call("mnp_hmat_fill", self.handle, _lib.MnpComplex.of(k), _lib.MNP_KEY[key])
         return self
 
+    def fill_layer(self, k, key, comp, direct, gl, enei):
+        """Component comp ('p', 'ss', 'hs', 'sh', 'hh') of the outside Green
+        function above the substrate of gl (a layer.greenretlayer): key 'G' or
+        'H2'; direct adds the direct part at wavenumber k
+        (mnp_hmat_fill_layer)."""
+        from . import _lib
+        c = ("p", "ss", "hs", "sh", "hh").index(comp) if isinstance(comp, str) else int(comp)
+        eu, ed = gl.tab.layer.epsilon(enei)
+        _lib.call("mnp_hmat_fill_layer", self.handle, _lib.MnpComplex.of(k), _lib.MNP_KEY[key], c, int(bool(direct)),
+                  gl.handle, float(enei), _lib.MnpComplex.of(eu), _lib.MnpComplex.of(ed))
+        return self
+
     def matvec(self, x):
         """y = H x for a device (n,) or (n, nrhs) complex128 column-major tensor."""
         from . import _lib, device as dev

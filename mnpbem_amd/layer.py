@@ -29,7 +29,7 @@ import numpy as np
 
 from . import _lib
 from . import device as dev
-from .bem import bemret, bemstat, compstruct, dipolestat, _flat_normals
+from .bem import bemret, bemretiter, bemstat, compstruct, dipolestat, _flat_normals
 from .greenfun import CompGreenRet
 
 _MATLAB_EPS = float(np.finfo(float).eps)
@@ -714,6 +714,43 @@ class bemretlayer:
         fb, it, rr = C.c_int(), C.c_int(), C.c_double()
         _lib.call("mnp_bemretlayer_stats", self.handle, C.byref(fb), C.byref(it), C.byref(rr))
         return {"fallbacks": fb.value, "iterations": it.value, "relres": rr.value}
+
+
+class bemretlayeriter(bemretiter):
+    """Iterative retarded BEM solver above a substrate (BEM/@bemretlayeriter):
+    op['layer'] the layerstructure, op['greentab'] a compgreentablayer
+    covering the particle (default: its tabspace, tabulated per wavelength),
+    op['iter'] the bemiter options as for bemretiter.  The outside Green
+    function is five H-matrices per operator (components p, ss, hs, sh, hh:
+    direct + reflected for p, ss, hh, reflected for hs, sh; Greenfun/+aca/
+    @compgreenretlayer), built on the GPU by ACA of the direct plus the
+    tabulated reflected kernel, with the refined near-field values in the full
+    blocks; GMRES on the operator of bemretlayeriter/private/afun.m with the
+    two-level preconditioner of bemretiter on the p components
+    (mnp_bemretiter_create_layer)."""
+
+    def __init__(self, p, op):
+        self.layer = op["layer"]
+        _check_above(p, self.layer)
+        self.tab = _greentab(p, op)
+        self.green = greenretlayer(p, self.tab)
+        super().__init__(p, op)
+
+    def _create(self, prec, perm, fr, lr, htol, kmax, h):
+        _lib.call("mnp_bemretiter_create_layer", self.p.green.handle, self.green.handle, prec,
+                  perm.ctypes.data_as(C.c_void_p), len(fr), fr.ctypes.data_as(C.c_void_p), len(lr),
+                  lr.ctypes.data_as(C.c_void_p), htol, kmax, C.byref(h))
+
+    def __call__(self, enei):
+        if self.enei != enei:
+            self.enei = None
+            eu, ed = self.layer.epsilon(enei)
+            if abs(complex(self.p.eps2(enei)) - eu) > 1e-12 * max(1.0, abs(eu)):
+                raise ValueError("bemretlayeriter: the particle's outside must be the upper medium of the layer")
+            _lib.call("mnp_bemretiter_initmat_layer", self.handle, float(enei),
+                      _lib.MnpComplex.of(self.p.eps1(enei)), _lib.MnpComplex.of(eu), _lib.MnpComplex.of(ed))
+            self.enei = enei
+        return self
 
 
 def _kz(k, kpar, offset=False):

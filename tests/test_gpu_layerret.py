@@ -202,3 +202,77 @@ def test_point_fields_match_oracle():
         assert max(errs) < 1e-10
     # the reflected part alone is a sizeable share at the points above
     assert np.abs(refl["e"]).max() > 1e-3 * np.abs(fld["e"][up]).max()
+
+
+def _strict(r1, r2, d):
+    """Admissibility that keeps every near pair of the 1,280-face sphere in a
+    full block (the default 2.5 min(rad) < dist puts some refined pairs into
+    admissible blocks, where ACA sees the unrefined kernel, as in the
+    reference's H-matrices)."""
+    return 6.0 * min(r1, r2) < d
+
+
+@pytest.mark.parametrize("hprec,tol", [("fp64", 1e-7), ("fp32", 1e-5)])
+def test_iterative_matches_dense(hprec, tol):
+    """bemretlayeriter (five H-matrices per operator, GMRES) against the dense
+    FP64 bemretlayer and the oracle, air above (where the dense chain's
+    operator and the iterative one coincide: hs = sh) and a lossy metal
+    below, 1,280 faces: sig, h and the cross sections."""
+    from mnpbem_amd import device as dev
+    B, LY, p, op, tab, sub, otab, (v, f) = _setup(-10 + 1.2j, level=3)
+    dense = B.bemsolver(p, op, precision="fp64")
+    it = B.bemsolver(p, dict(op, iter=dict(cleaf=32, htol=1e-10, tol=1e-10, maxit=300, hprecision=hprec,
+                                           fadmiss=_strict)))
+    assert isinstance(it, LY.bemretlayeriter)
+    exc = B.planewave(POL, DIR, op)
+    g = O.GreenRet(v, f)
+    ob = L.BemRetLayer(g, L.GreenRetLayer(sub, v, f, otab))
+    for lam in (520.0, 640.0):
+        ex = exc(p, lam)
+        a, b = dense.solve(ex), it.solve(ex)
+        st = it.stats()
+        e = max(_rel(dev.to_host(b[k]), dev.to_host(a[k])) for k in ("sig1", "sig2", "h1", "h2"))
+        ca, cb = exc.cross_sections(a), exc.cross_sections(b)
+        ecs = max(np.max(np.abs(x - y)) / np.max(np.abs(y)) for x, y in zip(cb[:2], ca[:2]))
+        print(f"[layerret] iterative {hprec} {lam:.0f} nm: {st['iter']} iterations, flag {st['flag']}, "
+              f"sig {e:.1e}, cross sections {ecs:.1e}", flush=True)
+        assert st["flag"] == 0 and e < tol and ecs < tol
+        if lam == 520.0 and hprec == "fp64":
+            phi, phip, aa, ap = L.planewave_potential(sub, g.pos, g.nvec, POL, DIR, lam)
+            ref = ob.initmat(lam, O.eps_drude(lam, *DRUDE)).solve({"phi2": phi, "phi2p": phip, "a2": aa, "a2p": ap})
+            eo = max(_rel(dev.to_host(b[k]), ref[k]) for k in ref)
+            print(f"[layerret] iterative vs oracle: {eo:.1e}", flush=True)
+            assert eo < tol
+
+
+@pytest.mark.parametrize("key", ["G", "H2"])
+def test_layer_hmatrix_components(key):
+    """The substrate H-matrices with admissible blocks (default admissibility,
+    1,280 faces): each reflected component equals the dense reflected matrix
+    of greenretlayer to the ACA tolerance, and with the direct part the
+    H-matrix minus the direct H-matrix equals it too (ACA of the sum
+    kernel)."""
+    import torch
+    from mnpbem_amd import device as dev
+    from mnpbem_amd import hmatrix as H
+    B, LY, p, op, tab, sub, otab, _ = _setup(-10 + 1.2j, level=3)
+    gl = LY.greenretlayer(p, tab)
+    lam = 560.0
+    G, F = gl.eval(lam)
+    D = dev.to_host(G if key == "G" else F)
+    g = p.green
+    tree = H.clustertree(g.geometry()["pos"], cleaf=32)
+    rng = np.random.default_rng(5)
+    xh = rng.normal(size=(g.n, 2)) + 1j * rng.normal(size=(g.n, 2))
+    x = torch.as_tensor(xh, device="cuda")
+    k = 2 * np.pi / lam
+    mk = lambda: H.hmatrix(g, tree, htol=1e-10, kmax=100, precision="fp64")
+    yd = dev.to_host(mk().fill(k, key).matvec(x))
+    for c, nm in enumerate(L.COMPONENTS):
+        want = D[:, :, c] @ xh
+        hr = mk().fill_layer(k, key, nm, False, gl, lam)
+        assert hr.stats()["lowrank_entries"] > 0  # admissible blocks present
+        er = _rel(dev.to_host(hr.matvec(x)), want)
+        ed = _rel(dev.to_host(mk().fill_layer(k, key, nm, True, gl, lam).matvec(x)) - yd, want)
+        print(f"[layerret] H-matrix {key}.{nm}: reflected {er:.1e}, with direct {ed:.1e}", flush=True)
+        assert er < 1e-8 and ed < 1e-7
