@@ -362,6 +362,62 @@ def run_substrate_ret(level=4, steps=4):
             "sca_first": [float(x) for x in res["fp64"][1][0][1]]}
 
 
+def run_substrate_iter(steps=4):
+    """Iterative retarded solver above a glass substrate (bemretlayeriter):
+    the workload of run_substrate_ret (10 x 10 x 40 nm ellipsoid 5 nm above
+    eps = 2.25, five TM plane waves, 30 x 30 tables set at 5 wavelengths) at
+    5,120 faces against the dense FP64 solver, and at 20,480 faces alone;
+    cleaf 32, htol 1e-6, FP32 H-matrix storage, GMRES tol 1e-6."""
+    import torch
+
+    from mnpbem_amd import bem as B, shapes
+    from mnpbem_amd.layer import compgreentablayer, layerstructure
+
+    theta = np.linspace(0, 80, 5) * np.pi / 180
+    dirs = np.column_stack([np.sin(theta), 0 * theta, -np.cos(theta)])
+    pols = np.column_stack([np.cos(theta), 0 * theta, np.sin(theta)])
+    lams = np.linspace(450.0, 750.0, 20)
+    out = {}
+    for level in (4, 5):
+        v, f = shapes.ellipsoid(level, (10.0, 10.0, 40.0))
+        v = v + np.array([0, 0, 45.0])
+        epstab = [B.epsconst(1.0), B.epsdrude(*DRUDE), B.epsconst(2.25)]
+        layer = layerstructure(epstab, [1, 3], 0.0)
+        p = B.comparticle(epstab, [(v, f)], [2, 1], 1, B.bemoptions(sim="ret"))
+        tab = compgreentablayer(layer, layer.tabspace(p)).set(np.linspace(350.0, 800.0, 5))
+        op = B.bemoptions(sim="ret", layer=layer, greentab=tab)
+        exc = B.planewave(pols, dirs, op)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        it = B.bemsolver(p, dict(op, iter=dict(cleaf=32, htol=1e-6, tol=1e-6, hprecision="fp32")))
+        exc.cross_sections(it.solve(exc(p, lams[0])))  # warm-up
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t
+        t = time.perf_counter()
+        cs, iters = [], []
+        for lam in lams[1:1 + steps]:
+            cs.append(exc.cross_sections(it.solve(exc(p, lam))))
+            iters.append(it.stats()["iter"])
+        torch.cuda.synchronize()
+        rate = steps / (time.perf_counter() - t)
+        r = {"faces": len(f), "wavelengths_per_s": rate, "gmres_iterations": iters, "setup_s": t_setup,
+             "ext_first": [float(x) for x in cs[0][0]]}
+        if level == 4:
+            dense = B.bemsolver(p, op, precision="fp64")
+            err = max(float(np.max(np.abs(a[j] - b[j]) / np.max(np.abs(b[j]))))
+                      for lam, a in zip(lams[1:1 + steps], cs)
+                      for b in [exc.cross_sections(dense.solve(exc(p, lam)))] for j in (0, 1))
+            r["max_rel_err_vs_dense_fp64"] = err
+            del dense
+        del it
+        log(f"[bench] substrate (iterative) {len(f)} faces: {rate:.3f}/s, iterations {iters}"
+            + (f", rel err vs dense {r['max_rel_err_vs_dense_fp64']:.1e}" if level == 4 else ""))
+        out[f"faces_{len(f)}"] = r
+    out["workload"] = ("retarded bemretlayeriter: ellipsoid 10x10x40 nm 5 nm above eps 2.25, air above, 5 TM "
+                       "plane waves from above at 0-80 deg, cleaf 32, htol 1e-6, FP32 H-matrices, GMRES tol 1e-6")
+    return out
+
+
 def run_sensitivity(nsweep=32):
     """Config 5: extinction and its gradient with respect to all vertex
     coordinates and the Drude parameters (eps_inf, wp, gamma) by the adjoint
@@ -820,6 +876,7 @@ def main():
     if a.only:
         runs = {"quasistatic": lambda: run_quasistatic(a.level), "mirror": run_mirror,
                 "substrate": lambda: run_substrate(a.level), "substrate_ret": run_substrate_ret,
+                "substrate_iter": run_substrate_iter,
                 "multi_particle": run_multi_particle,
                 "sensitivity": run_sensitivity, "batch_sweep": run_batch_sweep,
                 "hmatrix_dipole": run_hmatrix_dipole, "hmatrix_product": run_hmatrix_product}
