@@ -222,20 +222,54 @@ __device__ __forceinline__ void greduce(double (&s)[NS], Piv& p, double* red, Pi
   }
 }
 
-// v - sum_{l < k} P[l ld + c] coef[l], subtracted in order of l, with the
-// loads issued eight at a time (the columns of P are this thread's own
-// earlier results, coef is shared).
-__device__ __forceinline__ double2 aca_residual(double2 v, const double2* P, int64_t ld, int c, int k,
+// Buffer loads of one complex value at byte offset voff + soff of a buffer
+// resource: offsets past its size read zero (the hardware bounds check).
+template <class TS>
+__device__ __forceinline__ cplx<TS> bload(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ cplx<float> bload<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  return mk<float>(__uint_as_float(v[0]), __uint_as_float(v[1]));
+}
+template <>
+__device__ __forceinline__ cplx<double> bload<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  return mk<double>(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
+}
+// Two consecutive complex values (one 16-byte load for FP32 storage).
+template <class TS>
+__device__ __forceinline__ void bload2(__amdgpu_buffer_rsrc_t r, int voff, int soff, cplx<TS>& c0, cplx<TS>& c1);
+template <>
+__device__ __forceinline__ void bload2<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff, cplx<float>& c0,
+                                              cplx<float>& c1) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  c0 = mk<float>(__uint_as_float(v[0]), __uint_as_float(v[1]));
+  c1 = mk<float>(__uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+template <>
+__device__ __forceinline__ void bload2<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff, cplx<double>& c0,
+                                               cplx<double>& c1) {
+  c0 = bload<double>(r, voff, soff);
+  c1 = bload<double>(r, voff + 16, soff);
+}
+constexpr int BUF_WORD3 = 0x00020000;  // raw (untyped) buffer access on gfx9
+
+// v - sum_{l < k} P(c, l) coef[l], subtracted in order of l: column l of P
+// at byte offset l ldb of the buffer resource rs, entry c at byte offset
+// voff (the columns are this thread's own earlier results, coef is shared).
+// The loads go out eight at a time with the column offsets in scalar
+// registers; the last group of eight reads column k - 1 in place of the
+// columns past it and skips their terms, so no load waits alone.
+__device__ __forceinline__ double2 aca_residual(double2 v, __amdgpu_buffer_rsrc_t rs, int voff, int ldb, int k,
                                                 const double2* coef) {
-  int l = 0;
-  for (; l + 8 <= k; l += 8) {
+  for (int l = 0; l < k; l += 8) {
     double2 t[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) t[j] = P[(int64_t)(l + j) * ld + c];
+    for (int j = 0; j < 8; ++j) t[j] = bload<double>(rs, voff, min(l + j, k - 1) * ldb);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v = csub<double>(v, cmul<double>(t[j], coef[l + j]));
+    for (int j = 0; j < 8; ++j)
+      if (l + j < k) v = csub<double>(v, cmul<double>(t[j], coef[l + j]));
   }
-  for (; l < k; ++l) v = csub<double>(v, cmul<double>(P[(int64_t)l * ld + c], coef[l]));
   return v;
 }
 
@@ -245,14 +279,17 @@ __device__ __forceinline__ double2 aca_residual(double2 v, const double2* P, int
 // tid + G t are this thread's; the pivot rows already taken are kept as
 // bits t of used0/used1 (m <= 128 G, checked on the host).
 template <int G, class Ent>
-__device__ __forceinline__ void aca_block(int64_t b, int tid, const int32_t* blk, const int64_t* soff, const Ent& E,
+__device__ __forceinline__ void aca_block(int b, int tid, const int32_t* blk, const int64_t* soff, const Ent& E,
                                           double tol, int kmax, double2* scratch, int32_t* rank, double2* coef,
                                           double* red, Piv* pred) {
+  b = __builtin_amdgcn_readfirstlane(b);
   const int r0 = blk[4 * b], r1 = blk[4 * b + 1], c0 = blk[4 * b + 2], c1 = blk[4 * b + 3];
   const int m = r1 - r0, w = c1 - c0;
   const int kb = min(kmax, min(m, w));
   double2* A = scratch + soff[b];
   double2* B = A + (int64_t)m * kb;
+  const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc((void*)A, (short)0, m * kb * 16, BUF_WORD3);
+  const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc((void*)B, (short)0, w * kb * 16, BUF_WORD3);
   uint64_t used0 = 0, used1 = 0;  // bits t < 64, 64 <= t < 128
   int r = 0, k = 0;
   double nsum = 0;
@@ -266,7 +303,7 @@ __device__ __forceinline__ void aca_block(int64_t b, int tid, const int32_t* blk
     for (int c = tid; c < w; c += G) {
       double2 v = E(r0 + r, c0 + c);
       s1[0] += v.x * v.x + v.y * v.y;
-      v = aca_residual(v, B, w, c, k, coef);
+      v = aca_residual(v, rb, 16 * c, 16 * w, k, coef);
       B[(int64_t)k * w + c] = v;
       piv_take(pc, cabs1<double>(v), c, v);
     }
@@ -277,14 +314,19 @@ __device__ __forceinline__ void aca_block(int64_t b, int tid, const int32_t* blk
     }
     const int cp = pc.idx;
     const double2 scale = cdiv<double>(mk<double>(1, 0), pc.z);
+    // column cp of the residual -> A(:, k), with the scaled B(cp, l) as
+    // coefficients; their loads go out with those of the scaling pass
+    double2 cf0 = mk<double>(0, 0), cf1 = mk<double>(0, 0);
+    if (tid < k) cf0 = B[(int64_t)tid * w + cp];
+    if (tid + G < k) cf1 = B[(int64_t)(tid + G) * w + cp];
     double s2[2] = {0, 0};  // |A(:, k)|^2, |B(:, k)|^2
     for (int c = tid; c < w; c += G) {
       const double2 v = cmul<double>(B[(int64_t)k * w + c], scale);
       B[(int64_t)k * w + c] = v;
       s2[1] += v.x * v.x + v.y * v.y;
     }
-    // column cp of the residual -> A(:, k), with the scaled B(cp, l)
-    for (int l = tid; l < k; l += G) coef[l] = B[(int64_t)l * w + cp];
+    if (tid < k) coef[tid] = cf0;
+    if (tid + G < k) coef[tid + G] = cf1;
     gsync<G>();
     if (r % G == tid) {
       const int tr = r / G;
@@ -297,7 +339,7 @@ __device__ __forceinline__ void aca_block(int64_t b, int tid, const int32_t* blk
     int t = 0;
     for (int i = tid; i < m; i += G, ++t) {
       double2 v = E(r0 + i, c0 + cp);
-      v = aca_residual(v, A, m, i, k, coef);
+      v = aca_residual(v, ra, 16 * i, 16 * m, k, coef);
       A[(int64_t)k * m + i] = v;
       s2[0] += v.x * v.x + v.y * v.y;
       // next pivot row: izamax of the column with the used rows set to zero
@@ -508,38 +550,6 @@ __device__ __forceinline__ void col_product2(double2 (&acc0)[NR], double2 (&acc1
     __builtin_amdgcn_wave_barrier();
   }
 }
-
-// Buffer loads of one complex value at byte offset voff + soff of a buffer
-// resource: offsets past its size read zero (the hardware bounds check).
-template <class TS>
-__device__ __forceinline__ cplx<TS> bload(__amdgpu_buffer_rsrc_t r, int voff, int soff);
-template <>
-__device__ __forceinline__ cplx<float> bload<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-  return mk<float>(__uint_as_float(v[0]), __uint_as_float(v[1]));
-}
-template <>
-__device__ __forceinline__ cplx<double> bload<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
-  return mk<double>(__hiloint2double((int)v[1], (int)v[0]), __hiloint2double((int)v[3], (int)v[2]));
-}
-// Two consecutive complex values (one 16-byte load for FP32 storage).
-template <class TS>
-__device__ __forceinline__ void bload2(__amdgpu_buffer_rsrc_t r, int voff, int soff, cplx<TS>& c0, cplx<TS>& c1);
-template <>
-__device__ __forceinline__ void bload2<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff, cplx<float>& c0,
-                                              cplx<float>& c1) {
-  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
-  c0 = mk<float>(__uint_as_float(v[0]), __uint_as_float(v[1]));
-  c1 = mk<float>(__uint_as_float(v[2]), __uint_as_float(v[3]));
-}
-template <>
-__device__ __forceinline__ void bload2<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff, cplx<double>& c0,
-                                               cplx<double>& c1) {
-  c0 = bload<double>(r, voff, soff);
-  c1 = bload<double>(r, voff + 16, soff);
-}
-constexpr int BUF_WORD3 = 0x00020000;  // raw (untyped) buffer access on gfx9
 
 // First phase, t = R^T x over the column groups (the low-rank blocks of
 // one column cluster; their factors R_b^T side by side, so that the x rows
