@@ -855,3 +855,188 @@ def points_reflected(pg, enei, sig):
 
     h = np.stack([M(1, 2) - M(2, 1), M(2, 0) - M(0, 2), M(0, 1) - M(1, 0)], axis=1)
     return {"phi": phi, "a": a, "e": e, "h": h}
+
+
+# --------------------------------------------------------------------------
+# dipoles above the interface (Simulation/retarded/@dipoleretlayer)
+# --------------------------------------------------------------------------
+def greenderiv(table, tabn, pos1, pos2):
+    """@dipoleretlayer/private/greenderiv.m:15-99: reflected Green functions
+    G (5, n1, n2) between observers pos1 and dipoles pos2 and their
+    derivatives F[(a, b)] (5, n1, n2): a = 2, 3, 4 differentiates in x, y, z
+    at the observer, b = 2, 3, 4 in x', y', z' at the dipole (1: none).
+    Second radial derivatives and the z' derivatives are finite differences
+    with eta = 1e-6 of the interpolated table; equal point sets move the
+    dipoles by rmin along x (the self-interaction)."""
+    sub = table.sub
+    pos1, pos2 = np.atleast_2d(pos1), np.atleast_2d(pos2)
+    if pos1.shape == pos2.shape and np.all(pos1 == pos2):
+        pos2 = pos2 + np.array([sub.rmin, 0.0, 0.0])
+    x = pos1[:, 0, None] - pos2[None, :, 0]
+    y = pos1[:, 1, None] - pos2[None, :, 1]
+    r = np.sqrt(x ** 2 + y ** 2)
+    z1 = np.broadcast_to(pos1[:, 2, None], r.shape)
+    z2 = np.broadcast_to(pos2[None, :, 2], r.shape)
+    eta = 1e-6
+    r = np.where(r < sub.rmin, sub.rmin, r)
+    ux, uy = x / r, y / r
+    G, Fr, F1 = table.interp(tabn, r, z1, z2)[:3]
+    _, fr, fz = table.interp(tabn, r + eta, z1, z2)[:3]
+    Frr, Fr1 = (fr - Fr) / eta, (fz - F1) / eta
+    g, fr, fz = table.interp(tabn, r, z1, z2 + eta)[:3]
+    F2, Fr2, F12 = (g - G) / eta, (fr - Fr) / eta, (fz - F1) / eta
+    F = {(1, 2): -Fr * ux, (1, 3): -Fr * uy, (2, 1): Fr * ux, (3, 1): Fr * uy, (4, 1): F1, (1, 4): F2,
+         (2, 2): -Fr * uy ** 2 / r - Frr * ux ** 2,
+         (2, 3): -Fr * ux * uy / r - Frr * ux * uy,
+         (3, 3): -Fr * ux ** 2 / r - Frr * uy ** 2,
+         (2, 4): Fr2 * ux, (3, 4): Fr2 * uy, (4, 2): -Fr1 * ux, (4, 3): -Fr1 * uy, (4, 4): F12}
+    F[(3, 2)] = F[(2, 3)]
+    return G, F
+
+
+def _dip_moments(pts, dip, eps_pt):
+    """dip (npt, 3, ndip) and dip / eps of the point's medium."""
+    pts = np.atleast_2d(pts)
+    dip = np.asarray(dip, dtype=float).reshape(len(pts), 3, -1)
+    return dip, dip / np.asarray(eps_pt, dtype=complex).reshape(-1, 1, 1)
+
+
+def _cols(v):
+    """(..., npt, ndip) -> (..., npt ndip), column ipt + npt idip."""
+    return v.reshape(v.shape[:-2] + (-1,), order="F")
+
+
+def dipole_layer_potential(table, tabn, enei, pos, nvec, pts, dip, eps_pt):
+    """Reflected part of the dipole potentials at the faces pos (normals
+    nvec), @dipoleretlayer/potential.m:29-96: phi2, phi2p (n, npt ndip),
+    a2, a2p (n, 3, npt ndip); the caller adds dipoleret's direct part.  The
+    hh term of a2p(:, 3) takes the y component of the moment, as
+    potential.m:82 does."""
+    k0 = 2 * np.pi / enei
+    ik = -1j * k0
+    dp, d2 = _dip_moments(pts, dip, eps_pt)
+    G, F = greenderiv(table, tabn, pos, pts)
+    c = {nm: i for i, nm in enumerate(COMPONENTS)}
+    f = lambda g, d: g[:, :, None] * d[None, :, :]
+    dd = lambda nm, b: (F[(2, b)][c[nm]] * nvec[:, 0, None] + F[(3, b)][c[nm]] * nvec[:, 1, None]
+                        + F[(4, b)][c[nm]] * nvec[:, 2, None])
+    hs = lambda M, b: M[(1, b)][c["hs"]]
+    a = np.stack([ik * f(G[c["p"]], dp[:, 0]), ik * f(G[c["p"]], dp[:, 1]),
+                  ik * f(G[c["hh"]], dp[:, 2]) + f(hs(F, 2), d2[:, 0]) + f(hs(F, 3), d2[:, 1])
+                  + f(hs(F, 4), d2[:, 2])], axis=1)
+    phi = (f(F[(1, 2)][c["ss"]], d2[:, 0]) + f(F[(1, 3)][c["ss"]], d2[:, 1]) + f(F[(1, 4)][c["ss"]], d2[:, 2])
+           + ik * f(G[c["sh"]], dp[:, 2]))
+    ap = np.stack([ik * f(dd("p", 1), dp[:, 0]), ik * f(dd("p", 1), dp[:, 1]),
+                   ik * f(dd("hh", 1), dp[:, 1]) + f(dd("hs", 2), d2[:, 0]) + f(dd("hs", 3), d2[:, 1])
+                   + f(dd("hs", 4), d2[:, 2])], axis=1)
+    phip = (f(dd("ss", 2), d2[:, 0]) + f(dd("ss", 3), d2[:, 1]) + f(dd("ss", 4), d2[:, 2])
+            + ik * f(dd("sh", 1), dp[:, 2]))
+    return _cols(phi), _cols(phip), _cols(a), _cols(ap)
+
+
+def dipole_layer_field(table, tabn, enei, obs, pts, dip, eps_pt):
+    """Reflected field of the dipoles at the points obs,
+    @dipoleretlayer/field.m:45-128 with key 'refl': e, h (nobs, 3, npt ndip),
+    e = i k0 a - grad phi and h = -(the curl of a) as field.m:122-128 writes
+    them (phi3 takes the z' derivative of sh, as field.m:104 does)."""
+    k0 = 2 * np.pi / enei
+    ik = -1j * k0
+    dp, d2 = _dip_moments(pts, dip, eps_pt)
+    G, F = greenderiv(table, tabn, obs, pts)
+    c = {nm: i for i, nm in enumerate(COMPONENTS)}
+    f = lambda g, d: g[:, :, None] * d[None, :, :]
+    Fc = lambda a, b, nm: F[(a, b)][c[nm]]
+
+    def hsum(a, nm):  # F{a,2}.nm d2x + F{a,3}.nm d2y + F{a,4}.nm d2z
+        return f(Fc(a, 2, nm), d2[:, 0]) + f(Fc(a, 3, nm), d2[:, 1]) + f(Fc(a, 4, nm), d2[:, 2])
+
+    a1 = ik * f(G[c["p"]], dp[:, 0])
+    a2 = ik * f(G[c["p"]], dp[:, 1])
+    a3 = ik * f(G[c["hh"]], dp[:, 2]) + hsum(1, "hs")
+    a23 = ik * f(Fc(3, 1, "hh"), dp[:, 2]) + hsum(3, "hs")
+    a32 = ik * f(Fc(4, 1, "p"), dp[:, 1])
+    a31 = ik * f(Fc(4, 1, "p"), dp[:, 0])
+    a13 = ik * f(Fc(4, 1, "hh"), dp[:, 2]) + hsum(4, "hs")
+    a12 = ik * f(Fc(2, 1, "p"), dp[:, 1])
+    a21 = ik * f(Fc(3, 1, "p"), dp[:, 0])
+    phi1 = hsum(2, "ss") + ik * f(Fc(2, 1, "sh"), dp[:, 2])
+    phi2 = hsum(3, "ss") + ik * f(Fc(3, 1, "sh"), dp[:, 2])
+    phi3 = hsum(4, "ss") + ik * f(Fc(1, 4, "sh"), dp[:, 2])
+    e = 1j * k0 * np.stack([a1, a2, a3], axis=1) - np.stack([phi1, phi2, phi3], axis=1)
+    h = -np.stack([a23 - a32, a31 - a13, a12 - a21], axis=1)
+    return _cols(e), _cols(h)
+
+
+def dipole_layer_farfield(sub, enei, pts, dip, eps_pt, dirs):
+    """Far field of dipoles above the interface, @dipoleretlayer/
+    farfield.m:21-177: e, h (ndir, 3, npt ndip).  Upward directions: direct
+    plus reflected; downward: transmitted (each only through a lossless
+    medium); the reflected and transmitted parts take the gradient at the
+    dipole by forward differences with eta = 1e-6 (farfield.m:41-50)."""
+    k0, k1, k2 = sub.k(enei)
+    dirs = np.atleast_2d(np.asarray(dirs, float))
+    dp, d2 = _dip_moments(pts, dip, eps_pt)
+    pts = np.atleast_2d(pts)
+    npt, _, ndip = dp.shape
+    eta = 1e-6
+    p4 = np.concatenate([pts[:, None, :], pts[:, None, :] + eta * np.eye(3)[None]], axis=1)  # (npt, 4, 3)
+    a = np.zeros((len(dirs), 3, npt, ndip), complex)
+    for i, d in enumerate(dirs):
+        up = d[2] >= 0
+        km = k1 if up else k2
+        if np.imag(km) != 0:
+            continue
+        kpar = km * np.sqrt(1 - d[2] ** 2)
+        z2 = p4[:, :, 2].ravel()
+        r, _, _ = _phase_coeff(sub, enei, kpar, np.full(z2.shape, sub.z0), 1 if up else 2, z2, 1)
+        dist = p4[:, :, :2].reshape(-1, 2) @ d[:2] + sub.z0 * d[2]
+        phase = np.exp(-1j * km * dist)
+        rr = {}
+        for nm in ("p", "hh", "hs"):
+            v = (r[nm] * phase).reshape(npt, 4)
+            v[:, 1:] = (v[:, 1:] - v[:, :1]) / eta
+            rr[nm] = v
+        a[i, 0] = -1j * k0 * rr["p"][:, 0, None] * dp[:, 0]
+        a[i, 1] = -1j * k0 * rr["p"][:, 0, None] * dp[:, 1]
+        a[i, 2] = (-1j * k0 * rr["hh"][:, 0, None] * dp[:, 2] + rr["hs"][:, 1, None] * d2[:, 0]
+                   + rr["hs"][:, 2, None] * d2[:, 1] + rr["hs"][:, 3, None] * d2[:, 2])
+    e = 1j * k0 * a
+    upm = dirs[:, 2] >= 0
+    if np.any(upm):
+        from . import dipole_oracle as DO
+        ed, _ = DO.dipole_farfield(dirs[upm], pts, dp, enei, sub.eps(enei)[0])
+        e[upm] += ed.reshape(int(upm.sum()), 3, npt, ndip, order="F")
+    e = e - dirs[:, :, None, None] * np.einsum("dc,dcpq->dpq", dirs, e)[:, None]
+    kk = np.where(upm, k1, k2)
+    h = (kk / k0)[:, None, None, None] * np.cross(dirs[:, :, None, None], e, axisa=1, axisb=1, axisc=1)
+    return _cols(e), _cols(h)
+
+
+def dipole_layer_decayrate(table, tabn, sub, enei, pts, dip, e_sig, e_sig_far, h_sig_far, pinfty_nvec,
+                           pinfty_area):
+    """@dipoleretlayer/decayrate.m:14-60: (tot, rad, rad0) (npt, ndip) for
+    dipoles above the interface in the upper medium.  e_sig (npt, 3,
+    npt ndip) is the field of the surface charges at the dipoles (direct and
+    reflected, compgreenretlayer field.m), e_sig_far/h_sig_far their far
+    fields on the pinfty normals (spectrumretlayer farfield.m); zero for no
+    particle."""
+    pts = np.atleast_2d(pts)
+    eu = sub.eps(enei)[0]
+    dp, _ = _dip_moments(pts, dip, np.full(len(pts), eu))
+    npt, _, ndip = dp.shape
+    k0 = 2 * np.pi / enei
+    nb = np.sqrt(eu)
+    gamma = 4 / 3 * k0 ** 3
+    er, _ = dipole_layer_field(table, tabn, enei, pts, pts, dp, np.full(npt, eu))
+    e = e_sig + er
+    ed, hd = dipole_layer_farfield(sub, enei, pts, dp, np.full(npt, eu), pinfty_nvec)
+    from . import dipole_oracle as DO
+    sca = DO.radiated_power(e_sig_far + ed, h_sig_far + hd, pinfty_nvec, pinfty_area)
+    tot = np.zeros((npt, ndip))
+    rad = np.zeros((npt, ndip))
+    for p in range(npt):
+        for i in range(ndip):
+            q = p + npt * i
+            tot[p, i] = np.real(1 + np.imag(e[p, :, q] @ dp[p, :, i]) / (0.5 * nb * gamma))
+            rad[p, i] = np.real(sca[q] / (2 * np.pi * k0) / (0.5 * nb * gamma))
+    return tot, rad, np.real(nb * gamma) * np.ones((npt, ndip))
