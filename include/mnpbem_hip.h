@@ -693,6 +693,32 @@ int mnp_farfield_layer(mnp_greenret_t g, const mnp_layer_options* op, double ene
                        mnp_complex eps_down, int64_t ndir, const void* dir, int64_t npol,
                        const void* sig2, const void* h2, void* e, void* h);
 
+/* Dipoles above the interface (Simulation/retarded/@dipoleretlayer) at pts
+ * (host npt x 3, above z0, in the upper medium) with moments dip (host
+ * npt x 3 x ndip), columns ipt + npt idip; tab must cover the face-dipole
+ * and dipole-dipole pairs (tabspace with the dipoles as sources).
+ * Reflected part of the potentials at the faces, ADDED to phi, phip
+ * (device n x npol) and a, ap (device n x 3 x npol), e.g. after
+ * mnp_dipole_potential with eps_up: replaces @dipoleretlayer/potential.m:
+ * 29-96 with private/greenderiv.m:15-99 (forward differences eta = 1e-6 of
+ * the interpolated table; the hh term of ap(:, 3) takes the moment's y
+ * component, as potential.m:82 does). */
+int mnp_dipolelayer_potential(mnp_greenret_t g, mnp_layertab_t tab, double enei, mnp_complex eps_up,
+                              mnp_complex eps_down, int64_t npt, const double* pts, int64_t ndip, const double* dip,
+                              void* phi, void* phip, void* a, void* ap);
+/* Reflected field of the dipoles at the points obs (host nobs x 3): e, h
+ * device out nobs x 3 x npol (@dipoleretlayer/field.m:45-128 with 'refl';
+ * obs equal to pts moves the dipoles by rmin along x, greenderiv.m:19-21). */
+int mnp_dipolelayer_field(mnp_layertab_t tab, double enei, mnp_complex eps_up, mnp_complex eps_down, int64_t nobs,
+                          const double* obs, int64_t npt, const double* pts, int64_t ndip, const double* dip, void* e,
+                          void* h);
+/* Far field of the dipoles in the directions dir (device ndir x 3): e, h
+ * device out ndir x 3 x npol, upward direct plus reflected, downward
+ * transmitted (@dipoleretlayer/farfield.m:21-177). */
+int mnp_dipolelayer_farfield(const mnp_layer_options* op, double enei, mnp_complex eps_up, mnp_complex eps_down,
+                             int64_t ndir, const void* dir, int64_t npt, const double* pts, int64_t ndip,
+                             const double* dip, void* e, void* h);
+
 /* ---------------------------------------------- quasistatic dipoles
  * phi' = -nvec . E at the faces (device out n x npt ndip, column ipt + npt
  * idip) of dipoles dip (host npt x 3 x ndip) at pts (host npt x 3), E = w

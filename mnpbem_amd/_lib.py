@@ -126,6 +126,9 @@ PROTOTYPES = {
     "mnp_pointlayer_fields": (INT, [P, D, MnpComplex, MnpComplex, I64, P, P, P, P, P, P]),
     "mnp_planewaveretlayer_potential": (INT, [P, C.POINTER(LayerOptions), D, MnpComplex, MnpComplex, I64, P, P,
                                               P, P, P, P]),
+    "mnp_dipolelayer_potential": (INT, [P, P, D, MnpComplex, MnpComplex, I64, P, I64, P, P, P, P, P]),
+    "mnp_dipolelayer_field": (INT, [P, D, MnpComplex, MnpComplex, I64, P, I64, P, I64, P, P, P]),
+    "mnp_dipolelayer_farfield": (INT, [C.POINTER(LayerOptions), D, MnpComplex, MnpComplex, I64, P, I64, P, I64, P, P, P]),
     "mnp_farfield_layer": (INT, [P, C.POINTER(LayerOptions), D, MnpComplex, MnpComplex, I64, P, I64, P, P, P,
                                  P]),
     "mnp_stat_dipole_moment": (INT, [P, I64, P, P]),

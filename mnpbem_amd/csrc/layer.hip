@@ -773,6 +773,231 @@ __global__ void __launch_bounds__(256) k_farfield_layer(int64_t n, int64_t p, in
   }
 }
 
+// ------------------------------------------------------ dipoles above the interface
+// greenderiv.m:32-57's base values of component c for one observer-dipole
+// pair: the interpolated G, Fr, F1 (d/dz at the observer) and forward
+// differences with eta = 1e-6 of the interpolated table in r (Frr, Fr1)
+// and in the dipole height (F2, Fr2, F12).
+struct GdBase {
+  double2 G, Fr, F1, Frr, Fr1, F2, Fr2, F12;
+};
+__device__ inline double2 fdiff(double2 a, double2 b, double eta) {
+  return mk<double>((a.x - b.x) / eta, (a.y - b.y) / eta);
+}
+__device__ GdBase gd_base(const TabView& T, double r, double z1, double z2, int c) {
+  const double eta = 1e-6;
+  GdBase b;
+  double2 t, fr, fz, g;
+  tab_interp_c(T, r, z1, z2, c, 0, b.G, t, t);
+  tab_interp_c(T, r, z1, z2, c, 1, t, b.Fr, b.F1);
+  tab_interp_c(T, r + eta, z1, z2, c, 1, t, fr, fz);
+  b.Frr = fdiff(fr, b.Fr, eta);
+  b.Fr1 = fdiff(fz, b.F1, eta);
+  tab_interp_c(T, r, z1, z2 + eta, c, 0, g, t, t);
+  tab_interp_c(T, r, z1, z2 + eta, c, 1, t, fr, fz);
+  b.F2 = fdiff(g, b.G, eta);
+  b.Fr2 = fdiff(fr, b.Fr, eta);
+  b.F12 = fdiff(fz, b.F1, eta);
+  return b;
+}
+// F{a, b} of greenderiv.m:58-96: a = 2, 3, 4 differentiates in x, y, z at
+// the observer, b = 2, 3, 4 in x', y', z' at the dipole, 1 neither; the
+// mixed F{2, 3} = F{3, 2} as greenderiv.m:73-78 writes it.
+__device__ inline double2 gd_F(const GdBase& B, int a, int b, double ux, double uy, double r) {
+  auto sc = [](double2 z, double s) { return mk<double>(z.x * s, z.y * s); };
+  auto sub2 = [](double2 p, double2 q) { return mk<double>(p.x - q.x, p.y - q.y); };
+  switch (10 * a + b) {
+    case 11: return B.G;
+    case 12: return sc(B.Fr, -ux);
+    case 13: return sc(B.Fr, -uy);
+    case 21: return sc(B.Fr, ux);
+    case 31: return sc(B.Fr, uy);
+    case 41: return B.F1;
+    case 14: return B.F2;
+    case 22: return sub2(sc(sc(B.Fr, -uy * uy), 1.0 / r), sc(B.Frr, ux * ux));
+    case 23:
+    case 32: return sub2(sc(sc(B.Fr, -ux * uy), 1.0 / r), sc(B.Frr, ux * uy));
+    case 33: return sub2(sc(sc(B.Fr, -ux * ux), 1.0 / r), sc(B.Frr, uy * uy));
+    case 24: return sc(B.Fr2, ux);
+    case 34: return sc(B.Fr2, uy);
+    case 42: return sc(B.Fr1, -ux);
+    case 43: return sc(B.Fr1, -uy);
+    default: return B.F12;  // 44
+  }
+}
+
+// Reflected part of the dipoles' potentials at the faces (MODE 0,
+// @dipoleretlayer/potential.m:29-96, added to o0 = phi, o1 = phip, o2 = a,
+// o3 = ap) or their reflected field at points (MODE 1, field.m:45-128 with
+// 'refl', written to o2 = e, o3 = h): one thread per (observer, dipole
+// position), the moments dip (npt x 3 x ndip) and dip / eps of the dipoles'
+// medium (ieps = 1 / eps); column ipt + npt idip.  shift moves the dipoles
+// along x (the self-interaction of greenderiv.m:19-21).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_dip_layer(int64_t no, const double* __restrict__ obs,
+                                                   const double* __restrict__ onv, int64_t npt,
+                                                   const double* __restrict__ pts, double shift, int64_t ndip,
+                                                   const double* __restrict__ dip, double2 ieps, double k0, TabView T,
+                                                   double2* o0, double2* o1, double2* o2, double2* o3) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= no * npt) return;
+  const int64_t i = t % no, j = t / no;
+  const double x = obs[i] - (pts[j] + shift), y = obs[no + i] - pts[npt + j];
+  const double z1 = obs[2 * no + i], z2 = pts[2 * npt + j];
+  double r = sqrt(x * x + y * y);
+  r = r < T.rmin ? T.rmin : r;
+  const double ux = x / r, uy = y / r;
+  const double2 mik = mk<double>(0, -k0);
+  auto F = [&](const GdBase& B, int a, int b) { return gd_F(B, a, b, ux, uy, r); };
+  auto dot3 = [](double2 a, double2 b, double2 c, double2 d2x, double2 d2y, double2 d2z) {
+    return cadd<double>(cadd<double>(cmul<double>(a, d2x), cmul<double>(b, d2y)), cmul<double>(c, d2z));
+  };
+  auto rs = [](double2 z, double s) { return mk<double>(z.x * s, z.y * s); };
+  if constexpr (MODE == 0) {
+    const double n1 = onv[i], n2 = onv[no + i], n3 = onv[2 * no + i];
+    auto D = [&](const GdBase& B, int b) {
+      return cadd<double>(cadd<double>(rs(F(B, 2, b), n1), rs(F(B, 3, b), n2)), rs(F(B, 4, b), n3));
+    };
+    GdBase B = gd_base(T, r, z1, z2, 0);  // p
+    const double2 Gp = B.G, Dp = D(B, 1);
+    B = gd_base(T, r, z1, z2, 1);  // ss
+    const double2 S2 = F(B, 1, 2), S3 = F(B, 1, 3), S4 = F(B, 1, 4), DS2 = D(B, 2), DS3 = D(B, 3), DS4 = D(B, 4);
+    B = gd_base(T, r, z1, z2, 2);  // hs
+    const double2 H2 = F(B, 1, 2), H3 = F(B, 1, 3), H4 = F(B, 1, 4), DH2 = D(B, 2), DH3 = D(B, 3), DH4 = D(B, 4);
+    B = gd_base(T, r, z1, z2, 3);  // sh
+    const double2 Gsh = B.G, Dsh = D(B, 1);
+    B = gd_base(T, r, z1, z2, 4);  // hh
+    const double2 Ghh = B.G, Dhh = D(B, 1);
+    for (int64_t q = 0; q < ndip; ++q) {
+      const double dx = dip[j + npt * (3 * q)], dy = dip[j + npt * (1 + 3 * q)], dz = dip[j + npt * (2 + 3 * q)];
+      const double2 ex = rs(ieps, dx), ey = rs(ieps, dy), ez = rs(ieps, dz);
+      const int64_t col = j + npt * q;
+      const double2 phi = cadd<double>(dot3(S2, S3, S4, ex, ey, ez), cmul<double>(mik, rs(Gsh, dz)));
+      const double2 phip = cadd<double>(dot3(DS2, DS3, DS4, ex, ey, ez), cmul<double>(mik, rs(Dsh, dz)));
+      const double2 a[3] = {cmul<double>(mik, rs(Gp, dx)), cmul<double>(mik, rs(Gp, dy)),
+                            cadd<double>(cmul<double>(mik, rs(Ghh, dz)), dot3(H2, H3, H4, ex, ey, ez))};
+      // the hh term of a2p(:, 3) with the y component, potential.m:82
+      const double2 ap[3] = {cmul<double>(mik, rs(Dp, dx)), cmul<double>(mik, rs(Dp, dy)),
+                             cadd<double>(cmul<double>(mik, rs(Dhh, dy)), dot3(DH2, DH3, DH4, ex, ey, ez))};
+      o0[i + no * col] = cadd<double>(o0[i + no * col], phi);
+      o1[i + no * col] = cadd<double>(o1[i + no * col], phip);
+      for (int c = 0; c < 3; ++c) {
+        o2[i + no * (c + 3 * col)] = cadd<double>(o2[i + no * (c + 3 * col)], a[c]);
+        o3[i + no * (c + 3 * col)] = cadd<double>(o3[i + no * (c + 3 * col)], ap[c]);
+      }
+    }
+  } else {
+    GdBase B = gd_base(T, r, z1, z2, 0);  // p
+    const double2 Gp = B.G, P41 = F(B, 4, 1), P21 = F(B, 2, 1), P31 = F(B, 3, 1);
+    B = gd_base(T, r, z1, z2, 1);  // ss
+    const double2 S22 = F(B, 2, 2), S23 = F(B, 2, 3), S24 = F(B, 2, 4), S32 = F(B, 3, 2), S33 = F(B, 3, 3),
+                  S34 = F(B, 3, 4), S42 = F(B, 4, 2), S43 = F(B, 4, 3), S44 = F(B, 4, 4);
+    B = gd_base(T, r, z1, z2, 2);  // hs
+    const double2 H12 = F(B, 1, 2), H13 = F(B, 1, 3), H14 = F(B, 1, 4), H32 = F(B, 3, 2), H33 = F(B, 3, 3),
+                  H34 = F(B, 3, 4), H42 = F(B, 4, 2), H43 = F(B, 4, 3), H44 = F(B, 4, 4);
+    B = gd_base(T, r, z1, z2, 3);  // sh
+    const double2 Sh21 = F(B, 2, 1), Sh31 = F(B, 3, 1), Sh14 = F(B, 1, 4);
+    B = gd_base(T, r, z1, z2, 4);  // hh
+    const double2 Ghh = B.G, Hh31 = F(B, 3, 1), Hh41 = F(B, 4, 1);
+    const double2 ik0 = mk<double>(0, k0);
+    for (int64_t q = 0; q < ndip; ++q) {
+      const double dx = dip[j + npt * (3 * q)], dy = dip[j + npt * (1 + 3 * q)], dz = dip[j + npt * (2 + 3 * q)];
+      const double2 ex = rs(ieps, dx), ey = rs(ieps, dy), ez = rs(ieps, dz);
+      const int64_t col = j + npt * q;
+      const double2 a1 = cmul<double>(mik, rs(Gp, dx)), a2 = cmul<double>(mik, rs(Gp, dy));
+      const double2 a3 = cadd<double>(cmul<double>(mik, rs(Ghh, dz)), dot3(H12, H13, H14, ex, ey, ez));
+      const double2 a23 = cadd<double>(cmul<double>(mik, rs(Hh31, dz)), dot3(H32, H33, H34, ex, ey, ez));
+      const double2 a32 = cmul<double>(mik, rs(P41, dy)), a31 = cmul<double>(mik, rs(P41, dx));
+      const double2 a13 = cadd<double>(cmul<double>(mik, rs(Hh41, dz)), dot3(H42, H43, H44, ex, ey, ez));
+      const double2 a12 = cmul<double>(mik, rs(P21, dy)), a21 = cmul<double>(mik, rs(P31, dx));
+      const double2 phi1 = cadd<double>(dot3(S22, S23, S24, ex, ey, ez), cmul<double>(mik, rs(Sh21, dz)));
+      const double2 phi2 = cadd<double>(dot3(S32, S33, S34, ex, ey, ez), cmul<double>(mik, rs(Sh31, dz)));
+      const double2 phi3 = cadd<double>(dot3(S42, S43, S44, ex, ey, ez), cmul<double>(mik, rs(Sh14, dz)));
+      const double2 e[3] = {csub<double>(cmul<double>(ik0, a1), phi1), csub<double>(cmul<double>(ik0, a2), phi2),
+                            csub<double>(cmul<double>(ik0, a3), phi3)};
+      const double2 h[3] = {csub<double>(a32, a23), csub<double>(a13, a31), csub<double>(a21, a12)};
+      for (int c = 0; c < 3; ++c) {
+        o2[i + no * (c + 3 * col)] = e[c];
+        o3[i + no * (c + 3 * col)] = h[c];
+      }
+    }
+  }
+}
+
+// Far field of dipoles above the interface in the directions dir (device
+// ndir x 3), @dipoleretlayer/farfield.m:21-177: one thread per (direction,
+// dipole position).  Reflected (upward) or transmitted (downward) part,
+// only through a lossless medium, with the gradient at the dipole by
+// forward differences (eta 1e-6); the dipole's own far field in the upper
+// medium upward (dipoleret/farfield.m:40-55); transverse part of e and
+// h = k / k0 dir x e.  e, h device out ndir x 3 x (npt ndip).
+__global__ void __launch_bounds__(256) k_dip_layer_ff(int64_t ndir, const double* __restrict__ dir, int64_t npt,
+                                                      const double* __restrict__ pts, int64_t ndip,
+                                                      const double* __restrict__ dip, double k0, double2 e1,
+                                                      double2 e2, double z0, double2* e, double2* h) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ndir * npt) return;
+  const int64_t d = t % ndir, j = t / ndir;
+  const double dv[3] = {dir[d], dir[ndir + d], dir[2 * ndir + d]};
+  const bool up = dv[2] >= 0;
+  const double2 k1 = cscale<double>(csqrt_p(e1), k0), k2 = cscale<double>(csqrt_p(e2), k0);
+  const double2 km = up ? k1 : k2;
+  const double eta = 1e-6;
+  const double px = pts[j], py = pts[npt + j], pz = pts[2 * npt + j];
+  double2 rp[4], rh[4], rs_[4];
+  const bool refl = km.y == 0;
+  if (refl) {
+    const double2 kp = mk<double>(km.x * sqrt(1 - dv[2] * dv[2]), 0);
+    const double2 k1z = kz_of(k1, kp), k2z = kz_of(k2, kp);
+    const Coef4 c = coeff_entry(k0, e1, e2, k1z, k2z, up ? 1 : 2, 1);
+    for (int m = 0; m < 4; ++m) {
+      const double x = px + (m == 1 ? eta : 0), y = py + (m == 2 ? eta : 0), z = pz + (m == 3 ? eta : 0);
+      const double2 g2 = cexp_(mul_i(cscale<double>(k1z, fabs(z - z0))));
+      const double dist = x * dv[0] + y * dv[1] + z0 * dv[2];
+      const double2 ph = cmul<double>(g2, cexp_(mul_i(cscale<double>(km, -dist))));
+      rp[m] = cmul<double>(c.p, ph);
+      rh[m] = cmul<double>(c.hh, ph);
+      rs_[m] = cmul<double>(c.hs, ph);
+    }
+    for (int m = 1; m < 4; ++m) rs_[m] = fdiff(rs_[m], rs_[0], eta);
+  }
+  const double2 ie1 = cdiv<double>(mk<double>(1, 0), e1);
+  const double2 mik = mk<double>(0, -k0), ik0 = mk<double>(0, k0);
+  for (int64_t q = 0; q < ndip; ++q) {
+    const double dm[3] = {dip[j + npt * (3 * q)], dip[j + npt * (1 + 3 * q)], dip[j + npt * (2 + 3 * q)]};
+    double2 ev[3] = {mk<double>(0, 0), mk<double>(0, 0), mk<double>(0, 0)};
+    if (refl) {
+      const double2 a0 = cmul<double>(mik, cscale<double>(rp[0], dm[0]));
+      const double2 a1 = cmul<double>(mik, cscale<double>(rp[0], dm[1]));
+      double2 a2 = cmul<double>(mik, cscale<double>(rh[0], dm[2]));
+      for (int m = 1; m < 4; ++m) a2 = cadd<double>(a2, cmul<double>(rs_[m], cscale<double>(ie1, dm[m - 1])));
+      ev[0] = cmul<double>(ik0, a0);
+      ev[1] = cmul<double>(ik0, a1);
+      ev[2] = cmul<double>(ik0, a2);
+    }
+    if (up) {  // k^2 ((dir x p) x dir) exp(-i k dir . pos) / eps
+      const double2 g = cexp_ikd(mk<double>(-k1.x, -k1.y), dv[0] * px + dv[1] * py + dv[2] * pz);
+      const double hx = dv[1] * dm[2] - dv[2] * dm[1], hy = dv[2] * dm[0] - dv[0] * dm[2],
+                   hz = dv[0] * dm[1] - dv[1] * dm[0];
+      const double ee[3] = {hy * dv[2] - hz * dv[1], hz * dv[0] - hx * dv[2], hx * dv[1] - hy * dv[0]};
+      const double2 f = cmul<double>(cmul<double>(cmul<double>(k1, k1), g), ie1);
+      for (int c = 0; c < 3; ++c) ev[c] = cadd<double>(ev[c], cscale<double>(f, ee[c]));
+    }
+    const double2 de =
+        cadd<double>(cadd<double>(cscale<double>(ev[0], dv[0]), cscale<double>(ev[1], dv[1])), cscale<double>(ev[2], dv[2]));
+    for (int c = 0; c < 3; ++c) ev[c] = csub<double>(ev[c], cscale<double>(de, dv[c]));
+    const double2 kk = cdiv<double>(km, mk<double>(k0, 0));
+    const double2 hv[3] = {csub<double>(cscale<double>(ev[2], dv[1]), cscale<double>(ev[1], dv[2])),
+                           csub<double>(cscale<double>(ev[0], dv[2]), cscale<double>(ev[2], dv[0])),
+                           csub<double>(cscale<double>(ev[1], dv[0]), cscale<double>(ev[0], dv[1]))};
+    const int64_t col = j + npt * q;
+    for (int c = 0; c < 3; ++c) {
+      e[d + ndir * (c + 3 * col)] = ev[c];
+      h[d + ndir * (c + 3 * col)] = cmul<double>(kk, hv[c]);
+    }
+  }
+}
+
 // ------------------------------------------------------ chain helpers
 // (templated on the chain's precision T; scalars arrive as double2)
 template <class T>
@@ -1600,6 +1825,33 @@ struct BemRetLayer {
 
 }  // namespace mnp
 
+namespace mnp {
+// Uploads the dipoles (host pts npt x 3, dip npt x 3 x ndip) and runs
+// k_dip_layer<MODE> against the table at enei; raises when a pair fell
+// outside the table.
+template <int MODE>
+void run_dip_layer(LayerTab& tab, double enei, double2 e1, double2 e2, int64_t no, const double* obs,
+                   const double* onv, int64_t npt, const double* pts, double shift, int64_t ndip, const double* dip,
+                   double2* o0, double2* o1, double2* o2, double2* o3) {
+  for (int64_t i = 0; i < npt; ++i)
+    MNP_CHECK(pts[2 * npt + i] > tab.op.z0, MNP_ERR_ARG, "dipolelayer: the dipoles must lie above the interface");
+  double* dp = dev_alloc<double>(3 * npt + 3 * npt * ndip);
+  MNP_HIP(hipMemcpy(dp, pts, sizeof(double) * 3 * npt, hipMemcpyHostToDevice));
+  MNP_HIP(hipMemcpy(dp + 3 * npt, dip, sizeof(double) * 3 * npt * ndip, hipMemcpyHostToDevice));
+  const TabView T = tab.view(tab.at(enei, e1, e2));
+  MNP_HIP(hipMemsetAsync(tab.err, 0, sizeof(int), stream()));
+  const double2 ieps = cdiv<double>(mk<double>(1, 0), e1);
+  hipLaunchKernelGGL(k_dip_layer<MODE>, dim3(nblk(no * npt)), dim3(256), 0, stream(), no, obs, onv, npt, dp, shift,
+                     ndip, dp + 3 * npt, ieps, 2 * M_PI / enei, T, o0, o1, o2, o3);
+  MNP_HIP(hipGetLastError());
+  int herr = 0;
+  MNP_HIP(hipMemcpyAsync(&herr, tab.err, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  MNP_HIP(hipStreamSynchronize(stream()));
+  MNP_HIP(hipFree(dp));
+  MNP_CHECK(herr == 0, MNP_ERR_ARG, "greentablayer: a dipole pair lies outside the table (tabspace with the dipoles?)");
+}
+}  // namespace mnp
+
 struct mnp_layertab_s {
   mnp::LayerTab* t;
 };
@@ -1775,6 +2027,51 @@ int mnp_planewaveretlayer_potential(mnp_greenret_t g, const mnp_layer_options* o
     MNP_HIP(hipGetLastError());
     MNP_HIP(hipStreamSynchronize(stream()));
     MNP_HIP(hipFree(d));
+  });
+}
+int mnp_dipolelayer_potential(mnp_greenret_t g, mnp_layertab_t tab, double enei, mnp_complex eps_up,
+                              mnp_complex eps_down, int64_t npt, const double* pts, int64_t ndip, const double* dip,
+                              void* phi, void* phip, void* a, void* ap) {
+  return guarded([&] {
+    MNP_CHECK(g && tab && pts && dip && phi && phip && a && ap, MNP_ERR_ARG, "dipolelayer_potential: NULL argument");
+    MNP_CHECK(npt >= 1 && ndip >= 1, MNP_ERR_ARG, "dipolelayer_potential: npt and ndip must be positive");
+    const GreenRet& G = *g->g;
+    run_dip_layer<0>(*tab->t, enei, to_d2(eps_up), to_d2(eps_down), G.n, G.geo.pos, G.geo.nvec, npt, pts, 0.0, ndip,
+                     dip, (double2*)phi, (double2*)phip, (double2*)a, (double2*)ap);
+  });
+}
+int mnp_dipolelayer_field(mnp_layertab_t tab, double enei, mnp_complex eps_up, mnp_complex eps_down, int64_t nobs,
+                          const double* obs, int64_t npt, const double* pts, int64_t ndip, const double* dip, void* e,
+                          void* h) {
+  return guarded([&] {
+    MNP_CHECK(tab && obs && pts && dip && e && h, MNP_ERR_ARG, "dipolelayer_field: NULL argument");
+    MNP_CHECK(nobs >= 1 && npt >= 1 && ndip >= 1, MNP_ERR_ARG, "dipolelayer_field: sizes must be positive");
+    bool same = nobs == npt;
+    for (int64_t i = 0; same && i < 3 * npt; ++i) same = obs[i] == pts[i];
+    double* d = dev_alloc<double>(3 * nobs);
+    MNP_HIP(hipMemcpy(d, obs, sizeof(double) * 3 * nobs, hipMemcpyHostToDevice));
+    run_dip_layer<1>(*tab->t, enei, to_d2(eps_up), to_d2(eps_down), nobs, d, nullptr, npt, pts,
+                     same ? tab->t->op.rmin : 0.0, ndip, dip, nullptr, nullptr, (double2*)e, (double2*)h);
+    MNP_HIP(hipFree(d));
+  });
+}
+int mnp_dipolelayer_farfield(const mnp_layer_options* op, double enei, mnp_complex eps_up, mnp_complex eps_down,
+                             int64_t ndir, const void* dir, int64_t npt, const double* pts, int64_t ndip,
+                             const double* dip, void* e, void* h) {
+  return guarded([&] {
+    MNP_CHECK(op && dir && pts && dip && e && h, MNP_ERR_ARG, "dipolelayer_farfield: NULL argument");
+    MNP_CHECK(ndir >= 1 && npt >= 1 && ndip >= 1, MNP_ERR_ARG, "dipolelayer_farfield: sizes must be positive");
+    for (int64_t i = 0; i < npt; ++i)
+      MNP_CHECK(pts[2 * npt + i] > op->z0, MNP_ERR_ARG, "dipolelayer_farfield: the dipoles must lie above the interface");
+    double* dp = dev_alloc<double>(3 * npt + 3 * npt * ndip);
+    MNP_HIP(hipMemcpy(dp, pts, sizeof(double) * 3 * npt, hipMemcpyHostToDevice));
+    MNP_HIP(hipMemcpy(dp + 3 * npt, dip, sizeof(double) * 3 * npt * ndip, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_dip_layer_ff, dim3(nblk(ndir * npt)), dim3(256), 0, stream(), ndir, (const double*)dir, npt,
+                       dp, ndip, dp + 3 * npt, 2 * M_PI / enei, to_d2(eps_up), to_d2(eps_down), op->z0, (double2*)e,
+                       (double2*)h);
+    MNP_HIP(hipGetLastError());
+    MNP_HIP(hipStreamSynchronize(stream()));
+    MNP_HIP(hipFree(dp));
   });
 }
 int mnp_farfield_layer(mnp_greenret_t g, const mnp_layer_options* op, double enei, mnp_complex eps_up,

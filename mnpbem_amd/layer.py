@@ -29,7 +29,7 @@ import numpy as np
 
 from . import _lib
 from . import device as dev
-from .bem import bemret, bemretiter, bemstat, compstruct, dipolestat, _flat_normals
+from .bem import bemret, bemretiter, bemstat, compstruct, dipoleret, dipolestat, _flat_normals
 from .greenfun import CompGreenRet
 
 _MATLAB_EPS = float(np.finfo(float).eps)
@@ -73,22 +73,27 @@ class layerstructure:
         z[sel] = self.z[0] + np.sign(z[sel] - self.z[0]) * zmin
         return z
 
-    def tabspace(self, p, pt=None, nr=30, nz=30, scale=1.05):
+    def tabspace(self, p, pt=None, nr=30, nz=30, scale=1.05, dip=None):
         """Table grid for the particle p above the interface, and for the
         points pt (a compoint or an npt x 3 array above the interface) when
         given (tabspace2.m:44-75 with limits and adjust.m:18-33, tabspace1.m:
         24-48 with 'log' spacing, greentablayer/private/init.m:8-14): dict r,
-        z1, z2."""
+        z1, z2.  Dipole positions dip (npt x 3) are sources as well as
+        observers (tabspace(layer, {p, pt}) of the dipole demos)."""
         v = np.asarray(p.verts, dtype=np.float64)
         z0 = self.z[0]
         if np.any(v[:, 2] <= z0):
             raise NotImplementedError("tabspace: vertices at or below the interface are not supported")
         obs = v  # observers (rows) of the table: the vertices and the points
-        if pt is not None:
-            q = np.atleast_2d(np.asarray(getattr(pt, "pos", pt), dtype=np.float64))
+        for q in (pt, dip):
+            if q is None:
+                continue
+            q = np.atleast_2d(np.asarray(getattr(q, "pos", q), dtype=np.float64))
             if np.any(q[:, 2] <= z0):
                 raise NotImplementedError("tabspace: points at or below the interface are not supported")
-            obs = np.vstack([v, q])
+            obs = np.vstack([obs, q])
+        if dip is not None:  # sources (columns): the vertices and the dipoles
+            v = np.vstack([v, np.atleast_2d(np.asarray(dip, dtype=np.float64))])
         xy, oxy = v[:, :2], obs[:, :2]
         rmax = 0.0
         for i in range(0, len(oxy), 2048):  # largest observer-vertex distance, in chunks
@@ -628,6 +633,121 @@ class compgreenretlayer_points:
             f["e"][self.sel] += e
             f["h"][self.sel] += h
         return f
+
+
+class dipoleretlayer(dipoleret):
+    """Dipoles above the substrate (Simulation/retarded/@dipoleretlayer):
+    pt, dip, full and pinfty as dipoleret, op['layer'] the layerstructure,
+    op['greentab'] a compgreentablayer whose grid has the dipoles as sources
+    and observers (layer.tabspace(p, dip=pt)); without it that table is made
+    at the first call.  The dipoles lie in the upper medium, outside the
+    particle; potentials, fields and decay rates on the GPU
+    (mnp_dipolelayer_*)."""
+
+    def __init__(self, pt, dip=None, op=None, pinfty=None, full=False):
+        super().__init__(pt, dip, op, None, pinfty, full)
+        self.layer = op["layer"]
+        if np.any(self.pt[:, 2] <= self.layer.z[0]):
+            raise NotImplementedError("dipoleretlayer: dipoles at or below the interface are not supported")
+        self.tab = op.get("greentab")
+        if self.tab is not None and self.tab.layer is not self.layer:
+            raise ValueError("greentab: tabulated for another layer structure")
+        self._points = None
+
+    def _table(self, p):
+        if self.tab is None:
+            self.tab = compgreentablayer(self.layer, self.layer.tabspace(p, dip=self.pt))
+        return self.tab
+
+    def _eps(self, enei):
+        return (_lib.MnpComplex.of(e) for e in self.layer.epsilon(enei))
+
+    def __call__(self, p, enei):
+        """exc(p, enei): phi2, phi2p, a2, a2p on the faces, direct (dipoleret)
+        plus reflected (potential.m:15-96)."""
+        _check_above(p, self.layer)
+        exc = super().__call__(p, enei)
+        pts, dip = self._host()
+        _lib.call("mnp_dipolelayer_potential", p.green.handle, self._table(p).handle, float(enei), *self._eps(enei),
+                  len(pts), pts.ctypes.data_as(C.c_void_p), dip.shape[2], dip.ctypes.data_as(C.c_void_p),
+                  *[dev.ptr(dev.base(getattr(exc, k))) for k in ("phi2", "phi2p", "a2", "a2p")])
+        return exc
+
+    potential = __call__
+
+    def field_refl(self, obs, enei):
+        """Reflected e, h (host nobs x 3 x npol) of the dipoles at the points
+        obs (field.m with 'refl'; obs = the dipole positions gives their
+        self-interaction)."""
+        obs = np.asfortranarray(np.atleast_2d(obs), dtype=np.float64)
+        pts, dip = self._host()
+        npol = self.npol
+        e = dev.colmajor((len(obs), 3, npol))
+        h = dev.colmajor((len(obs), 3, npol))
+        if self.tab is None:
+            raise ValueError("dipoleretlayer: no table yet (pass op['greentab'] or call the excitation first)")
+        _lib.call("mnp_dipolelayer_field", self.tab.handle, float(enei), *self._eps(enei), len(obs),
+                  obs.ctypes.data_as(C.c_void_p), len(pts), pts.ctypes.data_as(C.c_void_p), dip.shape[2],
+                  dip.ctypes.data_as(C.c_void_p), dev.ptr(dev.base(e)), dev.ptr(dev.base(h)))
+        return dev.to_host(e), dev.to_host(h)
+
+    def farfield(self, dirs, enei):
+        """e, h (host ndir x 3 x npol) of the dipoles in the directions dirs
+        (farfield.m:21-177)."""
+        ddir = dev.to_device_colmajor(np.atleast_2d(dirs), dev.torch.float64)
+        pts, dip = self._host()
+        nd = ddir.shape[0]
+        e = dev.colmajor((nd, 3, self.npol))
+        h = dev.colmajor((nd, 3, self.npol))
+        op = self.layer.c_options()
+        _lib.call("mnp_dipolelayer_farfield", C.byref(op), float(enei), *self._eps(enei), nd, dev.ptr(dev.base(ddir)),
+                  len(pts), pts.ctypes.data_as(C.c_void_p), dip.shape[2], dip.ctypes.data_as(C.c_void_p),
+                  dev.ptr(dev.base(e)), dev.ptr(dev.base(h)))
+        return dev.to_host(e), dev.to_host(h)
+
+    def field(self, sig):
+        """Induced e (host npt x 3 x npol) at the dipoles: compgreenretlayer's
+        field of the surface charges, direct plus reflected (decayrate.m:22-23,
+        first term)."""
+        from .points import compoint
+        p = sig.p
+        e = dev.to_host(super().field(sig).e)
+        if self._points is None or self._points.p is not p:
+            self._points = compgreenretlayer_points(compoint(p, self.pt), p,
+                                                    {"layer": self.layer, "greentab": self._table(p)})
+        pl = self._points
+        if len(pl.pt.ind) != len(self.pt):
+            raise NotImplementedError("dipoleretlayer: a dipole lies on the particle surface")
+        if len(pl.sel):
+            e[pl.pt.ind[pl.sel]] += pl.reflected(sig)[2]
+        return e
+
+    def decayrate(self, sig):
+        """[tot, rad, rad0] (npt x ndip) in units of the free-space rate
+        (decayrate.m:14-60): tot from the induced field at the dipoles plus
+        their reflected self-field, rad from the power of the far fields of
+        the particle (spectrumretlayer) and of the dipoles over pinfty."""
+        enei = sig.enei
+        npt, ndip = self.dip.shape[0], self.dip.shape[2]
+        e = self.field(sig) + self.field_refl(self.pt, enei)[0]
+        nvec = dev.to_host(self.pinf_nvec)
+        area = dev.to_host(self.pinf_area)
+        ed, hd = self.farfield(nvec, enei)
+        es, hs = planewaveretlayer.farfield(self, sig, nvec)
+        s = np.cross(es + ed, np.conj(hs + hd), axisa=1, axisb=1, axisc=1)
+        sca = area @ (0.5 * np.real(np.einsum("dc,dcq->dq", nvec, s)))
+        k0 = 2 * np.pi / enei
+        eu = complex(self.layer.epsilon(enei)[0])
+        nb = np.sqrt(eu)
+        gamma = 4 / 3 * k0 ** 3
+        tot = np.zeros((npt, ndip))
+        rad = np.zeros((npt, ndip))
+        for ip in range(npt):
+            for i in range(ndip):
+                q = ip + npt * i
+                tot[ip, i] = np.real(1 + np.imag(e[ip, :, q] @ self.dip[ip, :, i]) / (0.5 * nb * gamma))
+                rad[ip, i] = np.real(sca[q] / (2 * np.pi * k0) / (0.5 * nb * gamma))
+        return tot, rad, np.real(nb * gamma) * np.ones((npt, ndip))
 
 
 def compgreenretlayer(p1, p2, op):
