@@ -418,6 +418,45 @@ def run_substrate_iter(steps=4):
     return out
 
 
+def run_substrate_dipole(steps=4):
+    """Dipoles above a glass substrate (dipoleretlayer): three orientations
+    at one point 5 nm above the tip of the 10 x 10 x 40 nm ellipsoid (5,120
+    faces) 5 nm above eps = 2.25, air above; per wavelength the excitation
+    (direct plus reflected potentials), the dense mixed bemretlayer solve
+    and the total and radiative decay rates (induced and reflected fields at
+    the dipole, far fields over the 256-point sphere); 30 x 30 tables set at
+    5 wavelengths."""
+    import torch
+
+    from mnpbem_amd import bem as B, shapes
+    from mnpbem_amd.layer import compgreentablayer, layerstructure
+
+    v, f = shapes.ellipsoid(4, (10.0, 10.0, 40.0))
+    v = v + np.array([0, 0, 45.0])
+    pts = np.array([[0.0, 0.0, 90.0]])  # the tip is at z = 85
+    epstab = [B.epsconst(1.0), B.epsdrude(*DRUDE), B.epsconst(2.25)]
+    layer = layerstructure(epstab, [1, 3], 0.0)
+    p = B.comparticle(epstab, [(v, f)], [2, 1], 1, B.bemoptions(sim="ret"))
+    tab = compgreentablayer(layer, layer.tabspace(p, dip=pts)).set(np.linspace(350.0, 800.0, 5))
+    op = B.bemoptions(sim="ret", layer=layer, greentab=tab)
+    exc = B.dipole(pts, np.eye(3), op)
+    lams = np.linspace(450.0, 750.0, 20)
+    bem = B.bemsolver(p, op)
+    exc.decayrate(bem.solve(exc(p, lams[0])))  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    rates = []
+    for lam in lams[1:1 + steps]:
+        tot, rad, _ = exc.decayrate(bem.solve(exc(p, lam)))
+        rates.append((tot.ravel().tolist(), rad.ravel().tolist()))
+    torch.cuda.synchronize()
+    rate = steps / (time.perf_counter() - t)
+    log(f"[bench] substrate dipole {len(f)} faces: {rate:.3f} wavelengths/s, tot {rates[0][0]}")
+    return {"workload": "retarded dipoleretlayer + bemretlayer (mixed): 3 dipole orientations 5 nm above the tip of "
+                        "the 10x10x40 nm ellipsoid (5120 faces) 5 nm above eps 2.25, air above",
+            "faces": len(f), "wavelengths_per_s": rate, "tot_rad_first": rates[0]}
+
+
 def run_sensitivity(nsweep=32):
     """Config 5: extinction and its gradient with respect to all vertex
     coordinates and the Drude parameters (eps_inf, wp, gamma) by the adjoint
@@ -877,6 +916,7 @@ def main():
         runs = {"quasistatic": lambda: run_quasistatic(a.level), "mirror": run_mirror,
                 "substrate": lambda: run_substrate(a.level), "substrate_ret": run_substrate_ret,
                 "substrate_iter": run_substrate_iter,
+                "substrate_dipole": run_substrate_dipole,
                 "multi_particle": run_multi_particle,
                 "sensitivity": run_sensitivity, "batch_sweep": run_batch_sweep,
                 "hmatrix_dipole": run_hmatrix_dipole, "hmatrix_product": run_hmatrix_product}

@@ -1423,8 +1423,9 @@ class dipoleretmirror:
 
 
 def dipole(pt, dip=None, op=None, **kw):
-    """Base/dipole.m: dipoleret for op.sim == 'ret', dipolestat for 'stat',
-    dipolestatlayer for 'stat' with op.layer."""
+    """Base/dipole.m: dipoleret for op.sim == 'ret' (dipoleretlayer with
+    op.layer), dipolestat for 'stat', dipolestatlayer for 'stat' with
+    op.layer."""
     if op is not None and op.get("sim", "ret") == "stat":
         if op.get("sym"):
             if op.get("layer") is not None:
@@ -1435,7 +1436,11 @@ def dipole(pt, dip=None, op=None, **kw):
             return dipolestatlayer(pt, dip, op, **kw)
         return dipolestat(pt, dip, op, **kw)
     if op is not None and op.get("layer") is not None:
-        raise NotImplementedError("retarded dipoles with layers are not built")
+        if op.get("sym"):
+            raise NotImplementedError("mirror symmetry with layers is not built")
+        from .layer import dipoleretlayer
+        kw.pop("medium", None)  # the dipoles lie in the upper medium
+        return dipoleretlayer(pt, dip, op, **kw)
     if op is not None and op.get("sym"):
         return dipoleretmirror(pt, dip, op, **kw)
     return dipoleret(pt, dip, op, **kw)

@@ -662,10 +662,19 @@ class dipoleretlayer(dipoleret):
     def _eps(self, enei):
         return (_lib.MnpComplex.of(e) for e in self.layer.epsilon(enei))
 
+    def _check_outside(self, p):
+        from .points import compoint
+        cp = compoint(p, self.pt)
+        if len(cp.ind) != len(self.pt) or np.any(np.asarray(cp.inout) != self.layer.ind[0]):
+            raise NotImplementedError("dipoleretlayer: the dipoles must lie outside the particle, in the upper medium")
+
     def __call__(self, p, enei):
         """exc(p, enei): phi2, phi2p, a2, a2p on the faces, direct (dipoleret)
         plus reflected (potential.m:15-96)."""
         _check_above(p, self.layer)
+        if getattr(self, "_checked", None) is not p:
+            self._check_outside(p)
+            self._checked = p
         exc = super().__call__(p, enei)
         pts, dip = self._host()
         _lib.call("mnp_dipolelayer_potential", p.green.handle, self._table(p).handle, float(enei), *self._eps(enei),
