@@ -34,7 +34,11 @@ substrate path, one interface at z0 with the particle in the upper medium.
     @planewaveretlayer/potential.m:20-105, with Particles/@layerstructure/
     fresnel.m) and its cross sections through the far field with the
     interface (Simulation/retarded/@spectrumretlayer/farfield.m:20-104,
-    @planewaveretlayer/extinction.m, scattering.m; efresnel.m).
+    @planewaveretlayer/extinction.m, scattering.m; efresnel.m);
+  * dipoles above the interface (Simulation/retarded/@dipoleretlayer:
+    private/greenderiv.m, potential.m, field.m with 'refl', farfield.m,
+    decayrate.m), with the reference's forms of the mixed derivative
+    F{2,3}, of field.m's h_y and of potential.m's hh term of a2p(:, 3).
 
 Pins (tests/test_layerret_oracle.py): the Sommerfeld identity (reflection
 coefficient 1 gives exp(i k D)/D and its derivatives) on both path kinds;
@@ -43,7 +47,9 @@ and quasistatic limits of the p and ss coefficients; equal media reduce the
 solver, the excitation and the cross sections to the free-space ones; the
 chain inverts afun for random structured matrices; a small sphere follows the
 quasistatic substrate solver; the table interpolation reproduces direct
-integration between grid points.
+integration between grid points; a dipole alone above glass has the
+Novotny-Hecht rates, and its fields and surface derivatives are finite
+differences of its potentials (tests/test_dipolelayer_oracle.py).
 """
 import numpy as np
 from scipy import special as sp
