@@ -422,7 +422,7 @@ def run_substrate_dipole(steps=4):
     """Dipoles above a glass substrate (dipoleretlayer): three orientations
     at one point 5 nm above the tip of the 10 x 10 x 40 nm ellipsoid (5,120
     faces) 5 nm above eps = 2.25, air above; per wavelength the excitation
-    (direct plus reflected potentials), the dense mixed bemretlayer solve
+    (direct plus reflected potentials), the dense FP64 bemretlayer solve
     and the total and radiative decay rates (induced and reflected fields at
     the dipole, far fields over the 256-point sphere); 30 x 30 tables set at
     5 wavelengths."""
@@ -452,7 +452,7 @@ def run_substrate_dipole(steps=4):
     torch.cuda.synchronize()
     rate = steps / (time.perf_counter() - t)
     log(f"[bench] substrate dipole {len(f)} faces: {rate:.3f} wavelengths/s, tot {rates[0][0]}")
-    return {"workload": "retarded dipoleretlayer + bemretlayer (mixed): 3 dipole orientations 5 nm above the tip of "
+    return {"workload": "retarded dipoleretlayer + bemretlayer (FP64): 3 dipole orientations 5 nm above the tip of "
                         "the 10x10x40 nm ellipsoid (5120 faces) 5 nm above eps 2.25, air above",
             "faces": len(f), "wavelengths_per_s": rate, "tot_rad_first": rates[0]}
 
