@@ -506,8 +506,9 @@ __global__ void __launch_bounds__(64) k_layer_diag(int64_t nd, int64_t n, Geo ge
 
 // Reflected potentials and fields at points above the interface from the
 // outside charges and currents (compgreenretlayer(pt, p): potential.m,
-// field.m with private/matmul2.m, matmul3.m), one thread per (point, column
-// q): every face by the table at the centroids, the refined pairs of point
+// field.m with private/matmul2.m, matmul3.m), one workgroup of 256 per
+// (point, column q), its threads over the faces, summed in the workgroup:
+// every face by the table at the centroids, the refined pairs of point
 // i (rowptr[i] .. rowptr[i + 1], faces increasing) from their vertex
 // weights.  phi (npt x ncol), a, e, h (npt x 3 x ncol): phi = Gss s + Gsh hz,
 // a = [Gp hx, Gp hy, Ghh hz + Ghs s], e = i k0 a - (Gp_ss s + Gp_sh hz),
@@ -520,17 +521,26 @@ __global__ void __launch_bounds__(256) k_pt_layer(int64_t npt, int64_t ncol, con
                                                   const double2* __restrict__ sig2, const double2* __restrict__ h2,
                                                   double2* phi, double2* a, double2* e, double2* h) {
 #pragma clang fp contract(off)
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= npt * ncol) return;
-  const int64_t i = t % npt, q = t / npt;
+  const int64_t i = blockIdx.x % npt, q = blockIdx.x / npt;
   const double px = pts[i], py = pts[npt + i], pz = pts[2 * npt + i];
   double2 ph = mk<double>(0, 0), av[3], ev[3], hv[3];
   for (int c = 0; c < 3; ++c) av[c] = ev[c] = hv[c] = mk<double>(0, 0);
-  int64_t pr = rowptr[i];
-  const int64_t pe = rowptr[i + 1];
-  for (int64_t j = 0; j < n; ++j) {
+  const int64_t p0 = rowptr[i], pe = rowptr[i + 1];
+  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
     double2 g[5], gp[5][3];
-    if (pr < pe && pcol[pr] == j) {  // refined pair: sum over the face's vertices
+    int64_t pr = -1;  // refined pair (i, j): binary search of the point's sorted face list
+    for (int64_t lo = p0, hi = pe - 1; lo <= hi;) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (pcol[mid] == j) {
+        pr = mid;
+        break;
+      }
+      if (pcol[mid] < j)
+        lo = mid + 1;
+      else
+        hi = mid - 1;
+    }
+    if (pr >= 0) {  // refined pair: sum over the face's vertices
       for (int c = 0; c < 5; ++c) g[c] = gp[c][0] = gp[c][1] = gp[c][2] = mk<double>(0, 0);
       for (int v = 0; v < 3; ++v) {
         double2 o[NQ];
@@ -543,7 +553,6 @@ __global__ void __launch_bounds__(256) k_pt_layer(int64_t npt, int64_t ncol, con
           gp[c][2] = cadd<double>(gp[c][2], cscale<double>(o[10 + c], w[3]));
         }
       }
-      ++pr;
     } else {
       const double x = px - geo.pos[j], y = py - geo.pos[n + j];
       const double r = sqrt(x * x + y * y);
@@ -575,12 +584,33 @@ __global__ void __launch_bounds__(256) k_pt_layer(int64_t npt, int64_t ncol, con
     hv[1] = cadd<double>(hv[1], csub<double>(M[2][0], M[0][2]));
     hv[2] = cadd<double>(hv[2], csub<double>(M[0][1], M[1][0]));
   }
-  phi[q * npt + i] = ph;
+  // sums over the workgroup: xor butterflies in each wave, then the four
+  // wave partials in order
+  __shared__ double2 part[4][10];
+  auto wred = [&](double2 v, int u) {
+    for (int o = 32; o > 0; o >>= 1) {
+      v.x += __shfl_xor(v.x, o);
+      v.y += __shfl_xor(v.y, o);
+    }
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][u] = v;
+  };
+  wred(ph, 0);
+  for (int c = 0; c < 3; ++c) {
+    wred(av[c], 1 + c);
+    wred(ev[c], 4 + c);
+    wred(hv[c], 7 + c);
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double2 r[10];
+  for (int u = 0; u < 10; ++u)
+    r[u] = cadd<double>(cadd<double>(part[0][u], part[1][u]), cadd<double>(part[2][u], part[3][u]));
+  phi[q * npt + i] = r[0];
   for (int c = 0; c < 3; ++c) {
     const int64_t id = (c + 3 * q) * npt + i;
-    a[id] = av[c];
-    e[id] = cadd<double>(mul_i(cscale<double>(av[c], k0)), ev[c]);
-    h[id] = hv[c];
+    a[id] = r[1 + c];
+    e[id] = cadd<double>(mul_i(cscale<double>(r[1 + c], k0)), r[4 + c]);
+    h[id] = r[7 + c];
   }
 }
 
@@ -1473,7 +1503,7 @@ struct PointLayer {
     const double2* tv = tab->at(enei, e1, e2);
     const TabView T = tab->view(tv);
     MNP_HIP(hipMemsetAsync(tab->err, 0, sizeof(int), stream()));
-    hipLaunchKernelGGL(k_pt_layer, dim3(nblk(npt * ncol)), dim3(256), 0, stream(), npt, ncol, pts, g->n, g->geo, T,
+    hipLaunchKernelGGL(k_pt_layer, dim3((unsigned)(npt * ncol)), dim3(256), 0, stream(), npt, ncol, pts, g->n, g->geo, T,
                        rowptr, cols, W, R0, Z2, Z1, 2 * M_PI / enei, sig2, h2, phi, a, e, h);
     MNP_HIP(hipGetLastError());
     int herr = 0;
